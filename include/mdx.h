@@ -1,0 +1,132 @@
+/*
+ * mdx.h -- C ABI of libmdx, the MI355X-native per-step force path of a
+ * DASH-style (dreid1991/md_engine) molecular-dynamics engine.
+ *
+ * Plain C types only: pointers to host arrays, sizes and scalars.  All arrays
+ * that describe atoms are indexed by atom id 0..n-1 (the library keeps its own
+ * cell-ordered copy on the device).  Every entry point returns 0 on success or
+ * one of the MDX_ERR_* codes; mdx_last_error() gives the message.  No C++
+ * exception crosses this boundary.  Work is ordered on the context's own HIP
+ * stream; results reach the host only through the *_get calls.
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the md_engine source tree).
+ */
+#ifndef MDX_H
+#define MDX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mdx_ctx mdx_ctx;
+
+#define MDX_OK 0
+#define MDX_ERR_ARG 1      /* invalid argument (size, style, cutoff, ...) */
+#define MDX_ERR_HIP 2      /* HIP / hipFFT runtime failure */
+#define MDX_ERR_STATE 3    /* call out of order (e.g. no system set) */
+#define MDX_ERR_INTERNAL 4
+
+/* force terms for mdx_forces_compute / mdx_run */
+#define MDX_PAIR 1      /* LJ + Ewald real space (FixLJCut with offloaded charge pairs) */
+#define MDX_PPPM 2      /* P3M mesh part (FixChargeEwald) */
+#define MDX_E3B 4       /* E3B two- and three-body (FixE3B) */
+#define MDX_ENERGY 8    /* also accumulate energies */
+#define MDX_VIRIAL 16   /* also accumulate the virial tensor */
+
+int mdx_version(void);
+
+/* Context on HIP device `device`; replaces State construction plus
+ * DeviceManager::setDevice (src/State.cpp:40-80, src/DeviceManager.cpp). */
+int mdx_create(int device, mdx_ctx **out);
+int mdx_destroy(mdx_ctx *ctx);
+const char *mdx_last_error(const mdx_ctx *ctx);
+
+/* Atoms and box.  Replaces State::prepareForRun's packing of the device
+ * arrays (src/State.cpp:562-647, GPUData.h:15-71): x, v, inv_mass (vs.w),
+ * q, type (xs.w).  x/v are n x 3 FP64, by id.  periodic[3] per axis. */
+int mdx_system_set(mdx_ctx *ctx, int n, const double *x, const double *v, const double *inv_mass,
+                   const double *q, const int32_t *type, const double lo[3], const double len[3],
+                   const int32_t periodic[3]);
+/* New positions (by id), velocities kept; the list is revalidated on use. */
+int mdx_positions_set(mdx_ctx *ctx, const double *x);
+/* Positions / velocities by id (either pointer may be NULL). */
+int mdx_system_get(mdx_ctx *ctx, double *x, double *v);
+/* Orthorhombic box (BoundsGPU, src/BoundsGPU.h:91-154). */
+int mdx_box_set(mdx_ctx *ctx, const double lo[3], const double len[3]);
+
+/* Neighbor list.  Replaces GridGPU (src/GridGPU.cu:84-113, 824-1103):
+ * r_list = rCut + padding, rebuild when any atom moved more than skin/2
+ * (setBuildFlag, GridGPU.cu:755-774). */
+int mdx_nlist_config(mdx_ctx *ctx, double r_list, double skin);
+/* Exclusion CSR by id: offsets[n_ids+1], entries other_id | tag << 30 with
+ * tag 1/2/3 = 1-2/1-3/1-4 (GridGPU::handleExclusionsDistance, GridGPU.cu:1377-1434). */
+int mdx_exclusions_set(mdx_ctx *ctx, int n_ids, const int32_t *offsets, const uint32_t *tagged_ids);
+/* Same, from bonds (pairs of ids) by breadth-first search to max_depth
+ * (GridGPU::generateExclusionList, GridGPU.cu:1458-1518). */
+int mdx_exclusions_from_bonds(mdx_ctx *ctx, int nbonds, const int32_t *bonds, int max_depth);
+/* Rebuild now (force != 0) or when the skin criterion says so; *rebuilt set
+ * to 1 when a build happened (periodicBoundaryConditions, GridGPU.cu:824-1103). */
+int mdx_nlist_update(mdx_ctx *ctx, int force, int *rebuilt);
+/* Per-id neighbor lists, ids ascending, each entry neighbor_id | tag << 30:
+ * row r of tagged_ids (row_cap entries) belongs to atom id r.  *max_count
+ * receives the longest row; rows longer than row_cap are truncated.
+ * Replaces GridGPU::getNeighborList (GridGPU.cu:1520-1648). */
+int mdx_nlist_get(mdx_ctx *ctx, int32_t *counts, uint32_t *tagged_ids, int row_cap, int *max_count);
+/* Rebuild count since system_set, row capacity, cells per axis. */
+int mdx_nlist_stats(mdx_ctx *ctx, int64_t *rebuilds, int *row_cap, int32_t cells[3]);
+
+/* LJ type-pair tables, ntypes x ntypes row-major: rc^2, 24 eps, sigma^6
+ * (FixLJCut::prepareForRun, src/Fixes/FixLJCut.cu:83-122). */
+int mdx_lj_set(mdx_ctx *ctx, int ntypes, const double *rc2, const double *eps24, const double *sig6);
+/* 1-2 / 1-3 / 1-4 multipliers (State::specialNeighborCoefs, State.cpp:55-57). */
+int mdx_special_set(mdx_ctx *ctx, double c12, double c13, double c14);
+/* Ewald real-space term fused into the pair kernel (ChargeEvaluatorEwald,
+ * src/Evaluators/ChargeEvaluatorEwald.h:11-63).  alpha <= 0 takes the P3M
+ * alpha once mdx_pppm_set is active. */
+int mdx_ewald_real_set(mdx_ctx *ctx, int enable, double alpha, double qqr, double r_cut);
+
+/* P3M mesh (FixChargeEwald::setParameters / setError, FixChargeEwald.cu:969-1085).
+ * Mesh sizes must be powers of two in [32, 1024] as in the reference; order
+ * 1, 3, 5 or 7; alpha <= 0 solves for it (find_optimal_parameters, 935-967). */
+int mdx_pppm_set(mdx_ctx *ctx, int nx, int ny, int nz, int order, double r_cut, double qqr,
+                 double alpha);
+/* Current alpha and the RMS error estimate DeltaF_k + DeltaF_real. */
+int mdx_pppm_info(mdx_ctx *ctx, double *alpha, double *rms_error);
+
+/* E3B water (FixE3B::addMolecule / createEvaluator, src/Fixes/FixE3B.cu:515-593,
+ * 753-792).  mol_ids: 4 ids per molecule ordered O, H1, H2, M.  params:
+ * E2, Ea, Eb, Ec (energy units), k2, k3, rs, rf (length units); r_two_body is
+ * the O-O cutoff of the two-body term (rf + padding = 6.2 A in the reference),
+ * r_max the three-body pair test (6.1572 A). */
+int mdx_e3b_set(mdx_ctx *ctx, int nmol, const int32_t *mol_ids, const double params[8],
+                double r_two_body, double r_max);
+
+/* Zero and accumulate the requested force terms at the current positions
+ * (IntegratorUtil::force, src/Integrators/IntegratorUtil.cpp:15-30). */
+int mdx_forces_compute(mdx_ctx *ctx, int flags);
+/* Forces by id, n x 3 FP64 (converted from the 64-bit fixed-point buffer). */
+int mdx_forces_get(mdx_ctx *ctx, double *f);
+/* Energies of the last compute with MDX_ENERGY: e[0] LJ + real space,
+ * e[1] P3M reciprocal incl. self term, e[2] E3B two-body, e[3] E3B three-body;
+ * virial[6] xx yy zz xy xz yz (sum F (x) dr). */
+int mdx_energy_get(mdx_ctx *ctx, double e[4], double virial[6]);
+
+/* Velocity Verlet (IntegratorVerlet::run, src/Integrators/IntegratorVerlet.cu:649-770):
+ * dt and the force-to-acceleration unit factor (Units::ftm_to_v). */
+int mdx_integrator_set(mdx_ctx *ctx, double dt, double ftm_to_v);
+/* Advance nsteps with the force terms in `flags`. */
+int mdx_run(mdx_ctx *ctx, int nsteps, int flags);
+/* Launch geometry knobs (Tunable nThreadPerAtom, src/Tunable.h:3-24):
+ * lanes per atom in the list build and the pair kernel, lanes per molecule in
+ * E3B.  0 keeps the current value.  Powers of two only. */
+int mdx_teams_set(mdx_ctx *ctx, int build_team, int pair_team, int e3b_team);
+int mdx_sync(mdx_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MDX_H */

@@ -1,0 +1,247 @@
+"""ctypes binding of libmdx.so (the C ABI in include/mdx.h).
+
+The library is loaded from this package directory only.  There is no CPU
+fallback: if the HIP library is missing or cannot be loaded, every call raises
+MdxError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmdx.so")
+
+PAIR, PPPM, E3B, ENERGY, VIRIAL = 1, 2, 4, 8, 16
+
+# every symbol declared in include/mdx.h
+SYMBOLS = [
+    "mdx_version", "mdx_create", "mdx_destroy", "mdx_last_error", "mdx_system_set",
+    "mdx_positions_set", "mdx_system_get", "mdx_box_set", "mdx_nlist_config",
+    "mdx_exclusions_set", "mdx_exclusions_from_bonds", "mdx_nlist_update", "mdx_nlist_get",
+    "mdx_nlist_stats", "mdx_lj_set", "mdx_special_set", "mdx_ewald_real_set", "mdx_pppm_set",
+    "mdx_pppm_info", "mdx_e3b_set", "mdx_forces_compute", "mdx_forces_get", "mdx_energy_get",
+    "mdx_integrator_set", "mdx_run", "mdx_teams_set", "mdx_sync",
+]
+
+
+class MdxError(RuntimeError):
+    pass
+
+
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+
+def load():
+    """Load libmdx.so (raises MdxError when it is absent or broken)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MdxError(f"{LIB_PATH} not built: run md_engine_amd/build.py (hipcc, gfx950)")
+    try:
+        L = C.CDLL(LIB_PATH)
+    except OSError as e:
+        raise MdxError(f"cannot load {LIB_PATH}: {e}") from e
+    c = C.c_int
+    d = C.c_double
+    L.mdx_version.restype = c
+    L.mdx_create.argtypes = [c, C.POINTER(_vp)]
+    L.mdx_destroy.argtypes = [_vp]
+    L.mdx_last_error.argtypes = [_vp]
+    L.mdx_last_error.restype = C.c_char_p
+    L.mdx_system_set.argtypes = [_vp, c, _dp, _vp, _vp, _vp, _vp, _dp, _dp, _ip]
+    L.mdx_positions_set.argtypes = [_vp, _dp]
+    L.mdx_system_get.argtypes = [_vp, _vp, _vp]
+    L.mdx_box_set.argtypes = [_vp, _dp, _dp]
+    L.mdx_nlist_config.argtypes = [_vp, d, d]
+    L.mdx_exclusions_set.argtypes = [_vp, c, _ip, _up]
+    L.mdx_exclusions_from_bonds.argtypes = [_vp, c, _ip, c]
+    L.mdx_nlist_update.argtypes = [_vp, c, C.POINTER(c)]
+    L.mdx_nlist_get.argtypes = [_vp, _vp, _vp, c, C.POINTER(c)]
+    L.mdx_nlist_stats.argtypes = [_vp, C.POINTER(C.c_int64), C.POINTER(c), _ip]
+    L.mdx_lj_set.argtypes = [_vp, c, _dp, _dp, _dp]
+    L.mdx_special_set.argtypes = [_vp, d, d, d]
+    L.mdx_ewald_real_set.argtypes = [_vp, c, d, d, d]
+    L.mdx_pppm_set.argtypes = [_vp, c, c, c, c, d, d, d]
+    L.mdx_pppm_info.argtypes = [_vp, C.POINTER(d), C.POINTER(d)]
+    L.mdx_e3b_set.argtypes = [_vp, c, _ip, _dp, d, d]
+    L.mdx_forces_compute.argtypes = [_vp, c]
+    L.mdx_forces_get.argtypes = [_vp, _dp]
+    L.mdx_energy_get.argtypes = [_vp, _dp, _dp]
+    L.mdx_integrator_set.argtypes = [_vp, d, d]
+    L.mdx_run.argtypes = [_vp, c, c]
+    L.mdx_teams_set.argtypes = [_vp, c, c, c]
+    L.mdx_sync.argtypes = [_vp]
+    for name in SYMBOLS:
+        fn = getattr(L, name)
+        if name not in ("mdx_last_error", "mdx_version"):
+            fn.restype = c
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Engine:
+    """One libmdx context (one system on one GPU)."""
+
+    def __init__(self, device: int = 0):
+        L = load()
+        h = C.c_void_p()
+        rc = L.mdx_create(int(device), C.byref(h))
+        if rc != 0:
+            raise MdxError(f"mdx_create(device={device}) failed with code {rc}")
+        self._h = h
+        self._L = L
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.mdx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self._L.mdx_last_error(self._h).decode()
+            raise MdxError(f"{what}: {msg} (code {rc})")
+
+    # system -----------------------------------------------------------------
+    def set_system(self, x, lo, L, v=None, inv_mass=None, q=None, types=None, periodic=(1, 1, 1)):
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        n = len(x)
+        v = None if v is None else np.ascontiguousarray(v, np.float64).reshape(-1, 3)
+        im = None if inv_mass is None else np.ascontiguousarray(inv_mass, np.float64)
+        q = None if q is None else np.ascontiguousarray(q, np.float64)
+        t = None if types is None else np.ascontiguousarray(types, np.int32)
+        self._keep = (v, im, q, t)
+        self._chk(self._L.mdx_system_set(self._h, n, x, _ptr(v), _ptr(im), _ptr(q), _ptr(t),
+                                         np.asarray(lo, np.float64), np.asarray(L, np.float64),
+                                         np.asarray(periodic, np.int32)), "mdx_system_set")
+        self.n = n
+
+    def set_positions(self, x):
+        self._chk(self._L.mdx_positions_set(self._h, np.ascontiguousarray(x, np.float64).reshape(-1)),
+                  "mdx_positions_set")
+
+    def get_system(self):
+        x = np.zeros((self.n, 3))
+        v = np.zeros((self.n, 3))
+        self._chk(self._L.mdx_system_get(self._h, _ptr(x), _ptr(v)), "mdx_system_get")
+        return x, v
+
+    def set_box(self, lo, L):
+        self._chk(self._L.mdx_box_set(self._h, np.asarray(lo, np.float64), np.asarray(L, np.float64)),
+                  "mdx_box_set")
+
+    # neighbor list ------------------------------------------------------------
+    def nlist_config(self, r_list, skin):
+        self._chk(self._L.mdx_nlist_config(self._h, float(r_list), float(skin)), "mdx_nlist_config")
+
+    def exclusions_from_bonds(self, bonds, max_depth=3):
+        b = np.ascontiguousarray(np.asarray(bonds, np.int32).reshape(-1))
+        self._chk(self._L.mdx_exclusions_from_bonds(self._h, len(b) // 2, b, int(max_depth)),
+                  "mdx_exclusions_from_bonds")
+
+    def exclusions_set(self, offsets, tagged):
+        off = np.ascontiguousarray(offsets, np.int32)
+        tg = np.ascontiguousarray(tagged, np.uint32)
+        if tg.size == 0:
+            tg = np.zeros(1, np.uint32)
+        self._chk(self._L.mdx_exclusions_set(self._h, len(off) - 1, off, tg), "mdx_exclusions_set")
+
+    def nlist_update(self, force=False):
+        r = C.c_int(0)
+        self._chk(self._L.mdx_nlist_update(self._h, int(bool(force)), C.byref(r)), "mdx_nlist_update")
+        return bool(r.value)
+
+    def nlist_get(self):
+        """Per-id neighbor rows: (counts, list of uint32 arrays id | tag << 30, ids ascending)."""
+        counts = np.zeros(self.n, np.int32)
+        mx = C.c_int(0)
+        self._chk(self._L.mdx_nlist_get(self._h, _ptr(counts), None, 0, C.byref(mx)), "mdx_nlist_get")
+        cap = max(1, mx.value)
+        tab = np.zeros(self.n * cap, np.uint32)
+        self._chk(self._L.mdx_nlist_get(self._h, _ptr(counts), _ptr(tab), cap, C.byref(mx)),
+                  "mdx_nlist_get")
+        tab = tab.reshape(self.n, cap)
+        return counts, [tab[i, : counts[i]].copy() for i in range(self.n)]
+
+    def nlist_stats(self):
+        nb = C.c_int64(0)
+        cap = C.c_int(0)
+        cells = np.zeros(3, np.int32)
+        self._chk(self._L.mdx_nlist_stats(self._h, C.byref(nb), C.byref(cap), cells), "mdx_nlist_stats")
+        return dict(rebuilds=nb.value, row_cap=cap.value, cells=cells.tolist())
+
+    # force terms ----------------------------------------------------------------
+    def lj_set(self, rc2, eps24, sig6):
+        rc2 = np.ascontiguousarray(rc2, np.float64).ravel()
+        T = int(round(np.sqrt(rc2.size)))
+        self._chk(self._L.mdx_lj_set(self._h, T, rc2, np.ascontiguousarray(eps24, np.float64).ravel(),
+                                     np.ascontiguousarray(sig6, np.float64).ravel()), "mdx_lj_set")
+
+    def special_set(self, c12, c13, c14):
+        self._chk(self._L.mdx_special_set(self._h, c12, c13, c14), "mdx_special_set")
+
+    def ewald_real_set(self, enable, alpha, qqr, r_cut):
+        self._chk(self._L.mdx_ewald_real_set(self._h, int(enable), float(alpha), float(qqr), float(r_cut)),
+                  "mdx_ewald_real_set")
+
+    def pppm_set(self, sz, order, r_cut, qqr, alpha=-1.0):
+        self._chk(self._L.mdx_pppm_set(self._h, int(sz[0]), int(sz[1]), int(sz[2]), int(order),
+                                       float(r_cut), float(qqr), float(alpha)), "mdx_pppm_set")
+
+    def pppm_info(self):
+        a = C.c_double(0)
+        e = C.c_double(0)
+        self._chk(self._L.mdx_pppm_info(self._h, C.byref(a), C.byref(e)), "mdx_pppm_info")
+        return a.value, e.value
+
+    def e3b_set(self, mol_ids, params, r_two_body=6.2, r_max=6.1572):
+        m = np.ascontiguousarray(mol_ids, np.int32).reshape(-1)
+        p = np.ascontiguousarray(params, np.float64)
+        self._chk(self._L.mdx_e3b_set(self._h, len(m) // 4, m, p, float(r_two_body), float(r_max)),
+                  "mdx_e3b_set")
+
+    def compute(self, flags):
+        self._chk(self._L.mdx_forces_compute(self._h, int(flags)), "mdx_forces_compute")
+
+    def forces(self):
+        f = np.zeros(self.n * 3)
+        self._chk(self._L.mdx_forces_get(self._h, f), "mdx_forces_get")
+        return f.reshape(-1, 3)
+
+    def energies(self):
+        e = np.zeros(4)
+        v = np.zeros(6)
+        self._chk(self._L.mdx_energy_get(self._h, e, v), "mdx_energy_get")
+        return e, v
+
+    # integration ------------------------------------------------------------------
+    def integrator_set(self, dt, ftm_to_v):
+        self._chk(self._L.mdx_integrator_set(self._h, float(dt), float(ftm_to_v)), "mdx_integrator_set")
+
+    def run(self, nsteps, flags):
+        self._chk(self._L.mdx_run(self._h, int(nsteps), int(flags)), "mdx_run")
+
+    def teams_set(self, build=0, pair=0, e3b=0):
+        self._chk(self._L.mdx_teams_set(self._h, int(build), int(pair), int(e3b)), "mdx_teams_set")
+
+    def sync(self):
+        self._chk(self._L.mdx_sync(self._h), "mdx_sync")

@@ -1,0 +1,604 @@
+// C ABI of libmdx: argument checking, host<->device marshalling by atom id and
+// the step loop.  See include/mdx.h for the contract of each entry point.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <set>
+
+#include "../../include/mdx.h"
+#include "mdx_internal.h"
+
+struct mdx_ctx {
+    mdx::Ctx c;
+};
+
+namespace mdx {
+
+void dfree(Ctx &c, void *p) {
+    if (!p) return;
+    auto it = std::find(c.allocs.begin(), c.allocs.end(), p);
+    if (it != c.allocs.end()) {
+        hipFree(p);
+        c.allocs.erase(it);
+    }
+}
+
+namespace {
+
+template <class F>
+int guarded(mdx_ctx *ctx, F &&f) {
+    if (!ctx) return MDX_ERR_ARG;
+    try {
+        f(ctx->c);
+        ctx->c.err.clear();
+        return MDX_OK;
+    } catch (const Error &e) {
+        ctx->c.err = e.msg;
+        return e.code;
+    } catch (const std::exception &e) {
+        ctx->c.err = e.what();
+        return MDX_ERR_INTERNAL;
+    }
+}
+
+void set_geometry(Ctx &c, const double lo[3], const double len[3]) {
+    for (int a = 0; a < 3; ++a) {
+        MDX_REQUIRE(len[a] > 0.0, MDX_ERR_ARG, "box lengths must be positive");
+        c.geo.lo[a] = lo[a];
+        c.geo.L[a] = len[a];
+        c.geo.invL[a] = 1.0 / len[a];
+    }
+}
+
+void require_system(const Ctx &c) { MDX_REQUIRE(c.n > 0, MDX_ERR_STATE, "no system set"); }
+
+void ensure_list(Ctx &c) {
+    if (!c.nlistConfigured) nlist_setup(c);
+    if (!c.listValid) {
+        nlist_rebuild(c, true);
+    } else {
+        nlist_check(c);
+        nlist_rebuild(c, false);
+    }
+}
+
+void compute(Ctx &c, int flags) {
+    const bool energy = flags & MDX_ENERGY, virial = flags & MDX_VIRIAL;
+    refresh_xq(c);
+    zero_forces(c, true);
+    if ((flags & MDX_PAIR)) pair_forces(c, energy, virial);
+    if ((flags & MDX_PPPM) && c.pppm.on) pppm_forces(c, energy, virial);
+    if ((flags & MDX_E3B) && c.e3b.on) e3b_forces(c, energy, virial);
+    c.forceFlags = flags;
+}
+
+void prepare_terms(Ctx &c) {
+    if (c.pppm.on) {
+        pppm_prepare(c);
+        if (c.ewaldReal) c.alpha = (float)c.pppm.alpha;
+    }
+}
+
+template <class T>
+void upload(Ctx &c, T *dst, const T *src, size_t count) {
+    MDX_HIP(hipMemcpyAsync(dst, src, sizeof(T) * count, hipMemcpyHostToDevice, c.stream));
+}
+
+}  // namespace
+}  // namespace mdx
+
+using namespace mdx;
+
+extern "C" {
+
+int mdx_version(void) { return 1; }
+
+int mdx_create(int device, mdx_ctx **out) {
+    if (!out) return MDX_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return MDX_ERR_HIP;
+    if (device < 0 || device >= ndev) return MDX_ERR_ARG;
+    mdx_ctx *ctx = new mdx_ctx();
+    ctx->c.device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->c.stream) != hipSuccess) {
+        delete ctx;
+        return MDX_ERR_HIP;
+    }
+    int rc = guarded(ctx, [](Ctx &c) {
+        c.acc = dalloc<unsigned long long>(c, ACC_N);
+        c.dFlags = dalloc<int>(c, 4);
+        MDX_HIP(hipMemsetAsync(c.dFlags, 0, sizeof(int) * 4, c.stream));
+        MDX_HIP(hipMemsetAsync(c.acc, 0, sizeof(unsigned long long) * ACC_N, c.stream));
+    });
+    if (rc != MDX_OK) {
+        mdx_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return MDX_OK;
+}
+
+int mdx_destroy(mdx_ctx *ctx) {
+    if (!ctx) return MDX_OK;
+    Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    if (c.stream) hipStreamSynchronize(c.stream);
+    pppm_release(c);
+    for (void *p : c.allocs) hipFree(p);
+    c.allocs.clear();
+    if (c.stream) hipStreamDestroy(c.stream);
+    delete ctx;
+    return MDX_OK;
+}
+
+const char *mdx_last_error(const mdx_ctx *ctx) { return ctx ? ctx->c.err.c_str() : "null context"; }
+
+int mdx_system_set(mdx_ctx *ctx, int n, const double *x, const double *v, const double *inv_mass,
+                   const double *q, const int32_t *type, const double lo[3], const double len[3],
+                   const int32_t periodic[3]) {
+    return guarded(ctx, [&](Ctx &c) {
+        MDX_REQUIRE(n > 0 && n < (1 << 30), MDX_ERR_ARG, "atom count must be in [1, 2^30)");
+        MDX_REQUIRE(x && lo && len, MDX_ERR_ARG, "positions and box are required");
+        MDX_REQUIRE(c.n == 0 || c.n == n, MDX_ERR_STATE, "atom count cannot change on a context");
+        hipSetDevice(c.device);
+        set_geometry(c, lo, len);
+        for (int a = 0; a < 3; ++a) c.geo.periodic[a] = periodic ? (periodic[a] != 0) : 1;
+        const bool fresh = c.n == 0;
+        c.n = n;
+        if (fresh) {
+            for (int b = 0; b < 2; ++b) {
+                c.pos[b] = dalloc<double4>(c, n);
+                c.vel[b] = dalloc<double4>(c, n);
+                c.q[b] = dalloc<float>(c, n);
+                c.type[b] = dalloc<int>(c, n);
+                c.id[b] = dalloc<int>(c, n);
+            }
+            c.idToIdx = dalloc<int>(c, n);
+            c.xq = dalloc<float4>(c, n);
+            c.force = dalloc<unsigned long long>(c, 3 * (size_t)n);
+            c.posLast = dalloc<double4>(c, n);
+        }
+        std::vector<double4> hp(n), hv(n);
+        std::vector<float> hq(n);
+        std::vector<int> ht(n), hid(n);
+        c.invMassHost.assign(n, 1.0);
+        c.hasCharges = false;
+        c.maxType = 0;
+        for (int i = 0; i < n; ++i) {
+            hp[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], 0.0);
+            double im = inv_mass ? inv_mass[i] : 1.0;
+            c.invMassHost[i] = im;
+            hv[i] = v ? make_double4(v[3 * i], v[3 * i + 1], v[3 * i + 2], im) : make_double4(0, 0, 0, im);
+            hq[i] = q ? (float)q[i] : 0.0f;
+            if (hq[i] != 0.0f) c.hasCharges = true;
+            ht[i] = type ? type[i] : 0;
+            MDX_REQUIRE(ht[i] >= 0 && ht[i] < kMaxTypes, MDX_ERR_ARG, "atom type out of range [0, 16)");
+            c.maxType = std::max(c.maxType, ht[i]);
+            hid[i] = i;
+        }
+        c.cur = 0;
+        MDX_REQUIRE(!c.ljOn || c.ntypes > c.maxType, MDX_ERR_ARG, "LJ table does not cover every atom type");
+        upload(c, c.pos[0], hp.data(), n);
+        upload(c, c.vel[0], hv.data(), n);
+        upload(c, c.q[0], hq.data(), n);
+        upload(c, c.type[0], ht.data(), n);
+        upload(c, c.id[0], hid.data(), n);
+        upload(c, c.idToIdx, hid.data(), n);
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        c.listValid = false;
+        c.nlistConfigured = false;
+        c.nRebuilds = 0;
+        if (c.pppm.on) {
+            double s = 0.0;
+            for (int i = 0; i < n; ++i) s += (double)hq[i] * hq[i];
+            c.pppm.totalQ2 = c.pppm.qqr * s;
+            c.pppm.alpha = 0.0;
+        }
+    });
+}
+
+int mdx_positions_set(mdx_ctx *ctx, const double *x) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        MDX_REQUIRE(x, MDX_ERR_ARG, "positions required");
+        std::vector<int> idx(c.n);
+        MDX_HIP(hipMemcpyAsync(idx.data(), c.idToIdx, sizeof(int) * c.n, hipMemcpyDeviceToHost, c.stream));
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        std::vector<double4> hp(c.n);
+        for (int i = 0; i < c.n; ++i) hp[idx[i]] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], 0.0);
+        upload(c, c.pos[c.cur], hp.data(), c.n);
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        c.listValid = false;
+    });
+}
+
+int mdx_system_get(mdx_ctx *ctx, double *x, double *v) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        std::vector<int> idx(c.n);
+        std::vector<double4> hp(c.n), hv(c.n);
+        MDX_HIP(hipMemcpyAsync(idx.data(), c.idToIdx, sizeof(int) * c.n, hipMemcpyDeviceToHost, c.stream));
+        MDX_HIP(hipMemcpyAsync(hp.data(), c.pos[c.cur], sizeof(double4) * c.n, hipMemcpyDeviceToHost, c.stream));
+        MDX_HIP(hipMemcpyAsync(hv.data(), c.vel[c.cur], sizeof(double4) * c.n, hipMemcpyDeviceToHost, c.stream));
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        for (int i = 0; i < c.n; ++i) {
+            const double4 &p = hp[idx[i]];
+            const double4 &w = hv[idx[i]];
+            if (x) {
+                x[3 * i] = p.x;
+                x[3 * i + 1] = p.y;
+                x[3 * i + 2] = p.z;
+            }
+            if (v) {
+                v[3 * i] = w.x;
+                v[3 * i + 1] = w.y;
+                v[3 * i + 2] = w.z;
+            }
+        }
+    });
+}
+
+int mdx_box_set(mdx_ctx *ctx, const double lo[3], const double len[3]) {
+    return guarded(ctx, [&](Ctx &c) {
+        MDX_REQUIRE(lo && len, MDX_ERR_ARG, "box required");
+        set_geometry(c, lo, len);
+        c.listValid = false;
+        c.nlistConfigured = false;
+    });
+}
+
+int mdx_nlist_config(mdx_ctx *ctx, double r_list, double skin) {
+    return guarded(ctx, [&](Ctx &c) {
+        MDX_REQUIRE(r_list > 0.0 && skin >= 0.0 && skin < r_list, MDX_ERR_ARG,
+                    "need r_list > 0 and 0 <= skin < r_list");
+        if (r_list != c.rlist && c.nl) {
+            dfree(c, c.nl);
+            dfree(c, c.nlCount);
+            c.nl = nullptr;
+            c.nlCount = nullptr;
+            c.cap = 0;
+        }
+        c.rlist = r_list;
+        c.skin = skin;
+        c.nlistConfigured = false;
+        c.listValid = false;
+    });
+}
+
+int mdx_exclusions_set(mdx_ctx *ctx, int n_ids, const int32_t *offsets, const uint32_t *tagged_ids) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        MDX_REQUIRE(n_ids == c.n && offsets, MDX_ERR_ARG, "exclusion offsets must cover every id");
+        int total = offsets[n_ids];
+        int maxe = 0;
+        for (int i = 0; i < n_ids; ++i) {
+            MDX_REQUIRE(offsets[i + 1] >= offsets[i], MDX_ERR_ARG, "offsets must be non-decreasing");
+            maxe = std::max(maxe, offsets[i + 1] - offsets[i]);
+        }
+        if (c.exclOff) {
+            dfree(c, c.exclOff);
+            dfree(c, c.exclIds);
+        }
+        c.exclOff = dalloc<int>(c, n_ids + 1);
+        c.exclIds = dalloc<uint32_t>(c, std::max(total, 1));
+        upload(c, c.exclOff, offsets, n_ids + 1);
+        if (total) upload(c, c.exclIds, tagged_ids, total);
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        c.nExclIds = n_ids;
+        c.maxExcl = maxe;
+        c.listValid = false;
+    });
+}
+
+int mdx_exclusions_from_bonds(mdx_ctx *ctx, int nbonds, const int32_t *bonds, int max_depth) {
+    if (!ctx) return MDX_ERR_ARG;
+    std::vector<int32_t> off;
+    std::vector<uint32_t> ids;
+    int rc = guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        MDX_REQUIRE(max_depth >= 1 && max_depth <= 3, MDX_ERR_ARG, "max_depth must be 1, 2 or 3");
+        const int n = c.n;
+        std::vector<std::set<int>> adj(n);
+        for (int b = 0; b < nbonds; ++b) {
+            int a0 = bonds[2 * b], a1 = bonds[2 * b + 1];
+            MDX_REQUIRE(a0 >= 0 && a0 < n && a1 >= 0 && a1 < n, MDX_ERR_ARG, "bond id out of range");
+            adj[a0].insert(a1);
+            adj[a1].insert(a0);
+        }
+        off.assign(n + 1, 0);
+        for (int i = 0; i < n; ++i) {
+            // levels[d]: ids at bond distance d+1 not closer (generateExclusionList)
+            std::vector<std::set<int>> levels(1, adj[i]);
+            for (int d = 1; d < max_depth; ++d) {
+                std::set<int> nxt;
+                for (int k : levels[d - 1]) nxt.insert(adj[k].begin(), adj[k].end());
+                for (auto it = nxt.begin(); it != nxt.end();) {
+                    bool closer = (*it == i);
+                    for (int e = 0; e < d && !closer; ++e) closer = levels[e].count(*it) > 0;
+                    it = closer ? nxt.erase(it) : std::next(it);
+                }
+                levels.push_back(nxt);
+            }
+            for (int d = 0; d < (int)levels.size(); ++d)
+                for (int j : levels[d]) ids.push_back((uint32_t)j | ((uint32_t)(d + 1) << 30));
+            off[i + 1] = (int32_t)ids.size();
+        }
+    });
+    if (rc != MDX_OK) return rc;
+    return mdx_exclusions_set(ctx, ctx->c.n, off.data(), ids.data());
+}
+
+int mdx_nlist_update(mdx_ctx *ctx, int force, int *rebuilt) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        MDX_REQUIRE(c.rlist > 0, MDX_ERR_STATE, "mdx_nlist_config not called");
+        long long before = c.nRebuilds;
+        if (force || !c.listValid) {
+            if (!c.nlistConfigured) nlist_setup(c);
+            nlist_rebuild(c, true);
+        } else {
+            ensure_list(c);
+        }
+        if (rebuilt) *rebuilt = c.nRebuilds != before;
+    });
+}
+
+int mdx_nlist_get(mdx_ctx *ctx, int32_t *counts, uint32_t *tagged_ids, int row_cap, int *max_count) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        MDX_REQUIRE(c.listValid, MDX_ERR_STATE, "no neighbor list built");
+        const int n = c.n;
+        std::vector<int> cnt(n), ids(n);
+        std::vector<uint32_t> nl((size_t)n * c.cap);
+        MDX_HIP(hipMemcpyAsync(cnt.data(), c.nlCount, sizeof(int) * n, hipMemcpyDeviceToHost, c.stream));
+        MDX_HIP(hipMemcpyAsync(ids.data(), c.id[c.cur], sizeof(int) * n, hipMemcpyDeviceToHost, c.stream));
+        MDX_HIP(hipMemcpyAsync(nl.data(), c.nl, sizeof(uint32_t) * nl.size(), hipMemcpyDeviceToHost, c.stream));
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        int mx = 0;
+        std::vector<uint32_t> row;
+        for (int k = 0; k < n; ++k) {
+            const int myid = ids[k];
+            const int cn = std::min(cnt[k], c.cap);
+            mx = std::max(mx, cnt[k]);
+            row.resize(cn);
+            for (int s = 0; s < cn; ++s) {
+                uint32_t e = nl[(size_t)k * c.cap + s];
+                row[s] = (uint32_t)ids[e & 0x3FFFFFFFu] | (e & 0xC0000000u);
+            }
+            std::sort(row.begin(), row.end(),
+                      [](uint32_t a, uint32_t b) { return (a & 0x3FFFFFFFu) < (b & 0x3FFFFFFFu); });
+            if (counts) counts[myid] = cnt[k];
+            if (tagged_ids)
+                for (int s = 0; s < std::min(cn, row_cap); ++s) tagged_ids[(size_t)myid * row_cap + s] = row[s];
+        }
+        if (max_count) *max_count = mx;
+    });
+}
+
+int mdx_nlist_stats(mdx_ctx *ctx, int64_t *rebuilds, int *row_cap, int32_t cells[3]) {
+    return guarded(ctx, [&](Ctx &c) {
+        if (rebuilds) *rebuilds = c.nRebuilds;
+        if (row_cap) *row_cap = c.cap;
+        if (cells)
+            for (int a = 0; a < 3; ++a) cells[a] = c.nc[a];
+    });
+}
+
+int mdx_lj_set(mdx_ctx *ctx, int ntypes, const double *rc2, const double *eps24, const double *sig6) {
+    return guarded(ctx, [&](Ctx &c) {
+        MDX_REQUIRE(ntypes >= 1 && ntypes <= kMaxTypes, MDX_ERR_ARG, "ntypes must be in [1, 16]");
+        MDX_REQUIRE(c.n == 0 || ntypes > c.maxType, MDX_ERR_ARG, "LJ table does not cover every atom type");
+        const int T2 = ntypes * ntypes;
+        std::vector<float> p(3 * T2);
+        for (int t = 0; t < T2; ++t) {
+            p[t] = (float)rc2[t];
+            p[T2 + t] = (float)eps24[t];
+            p[2 * T2 + t] = (float)sig6[t];
+        }
+        if (!c.ljParams) c.ljParams = dalloc<float>(c, 3 * kMaxTypes * kMaxTypes);
+        upload(c, c.ljParams, p.data(), p.size());
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        c.ntypes = ntypes;
+        c.ljOn = true;
+    });
+}
+
+int mdx_special_set(mdx_ctx *ctx, double c12, double c13, double c14) {
+    return guarded(ctx, [&](Ctx &c) {
+        c.special[0] = (float)c12;
+        c.special[1] = (float)c13;
+        c.special[2] = (float)c14;
+    });
+}
+
+int mdx_ewald_real_set(mdx_ctx *ctx, int enable, double alpha, double qqr, double r_cut) {
+    return guarded(ctx, [&](Ctx &c) {
+        c.ewaldReal = enable != 0;
+        c.alpha = (float)alpha;
+        c.qqr = (float)qqr;
+        c.rq2 = (float)(r_cut * r_cut);
+    });
+}
+
+int mdx_pppm_set(mdx_ctx *ctx, int nx, int ny, int nz, int order, double r_cut, double qqr,
+                 double alpha) {
+    return guarded(ctx, [&](Ctx &c) {
+        auto pow2 = [](int v) { return v >= 32 && v <= 1024 && (v & (v - 1)) == 0; };
+        MDX_REQUIRE(pow2(nx) && pow2(ny) && pow2(nz), MDX_ERR_ARG,
+                    "P3M mesh sizes must be powers of two in [32, 1024] (as FixChargeEwald::setParameters)");
+        MDX_REQUIRE(order == 1 || order == 3 || order == 5 || order == 7, MDX_ERR_ARG,
+                    "interpolation order must be 1, 3, 5 or 7");
+        MDX_REQUIRE(r_cut > 0, MDX_ERR_ARG, "r_cut must be positive");
+        pppm_release(c);
+        c.pppm = PppmState();
+        c.pppm.on = true;
+        c.pppm.sz[0] = nx;
+        c.pppm.sz[1] = ny;
+        c.pppm.sz[2] = nz;
+        c.pppm.order = order;
+        c.pppm.rcut = r_cut;
+        c.pppm.qqr = qqr;
+        c.pppm.alphaUser = alpha;
+        if (c.n > 0) {
+            std::vector<float> hq(c.n);
+            MDX_HIP(hipMemcpyAsync(hq.data(), c.q[c.cur], sizeof(float) * c.n, hipMemcpyDeviceToHost, c.stream));
+            MDX_HIP(hipStreamSynchronize(c.stream));
+            double s = 0.0;
+            for (int i = 0; i < c.n; ++i) s += (double)hq[i] * hq[i];
+            c.pppm.totalQ2 = qqr * s;
+        }
+    });
+}
+
+int mdx_pppm_info(mdx_ctx *ctx, double *alpha, double *rms_error) {
+    return guarded(ctx, [&](Ctx &c) {
+        MDX_REQUIRE(c.pppm.on, MDX_ERR_STATE, "P3M not configured");
+        require_system(c);
+        pppm_prepare(c);
+        if (alpha) *alpha = c.pppm.alpha;
+        if (rms_error) *rms_error = c.pppm.err;
+    });
+}
+
+int mdx_e3b_set(mdx_ctx *ctx, int nmol, const int32_t *mol_ids, const double params[8], double r_two_body,
+                double r_max) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        MDX_REQUIRE(nmol > 0 && mol_ids && params, MDX_ERR_ARG, "molecules and parameters required");
+        E3BState &E = c.e3b;
+        if (E.molIds) {
+            dfree(c, E.molIds);
+            dfree(c, E.molAtoms);
+            dfree(c, E.atomMolO);
+            dfree(c, E.nl);
+            dfree(c, E.nlCount);
+            dfree(c, E.tot);
+        }
+        std::vector<int4> ids(nmol);
+        std::vector<int> seen(c.n, 0);
+        for (int m = 0; m < nmol; ++m) {
+            ids[m] = make_int4(mol_ids[4 * m], mol_ids[4 * m + 1], mol_ids[4 * m + 2], mol_ids[4 * m + 3]);
+            for (int k = 0; k < 4; ++k) {
+                int a = mol_ids[4 * m + k];
+                MDX_REQUIRE(a >= 0 && a < c.n, MDX_ERR_ARG, "E3B molecule id out of range");
+                MDX_REQUIRE(!seen[a]++, MDX_ERR_ARG, "atom listed in two E3B molecules");
+            }
+        }
+        E.on = true;
+        E.nmol = nmol;
+        E.E2 = (float)params[0];
+        E.Ea = (float)params[1];
+        E.Eb = (float)params[2];
+        E.Ec = (float)params[3];
+        E.k2 = (float)params[4];
+        E.k3 = (float)params[5];
+        E.rs = (float)params[6];
+        E.rf = (float)params[7];
+        E.r2b = (float)r_two_body;
+        E.rmax = (float)r_max;
+        E.molIds = dalloc<int4>(c, nmol);
+        E.molAtoms = dalloc<int4>(c, nmol);
+        E.atomMolO = dalloc<int>(c, c.n);
+        E.tot = dalloc<float4>(c, nmol);
+        E.nlCount = dalloc<int>(c, nmol);
+        E.cap = 64;
+        E.nl = dalloc<int>(c, (size_t)nmol * E.cap);
+        upload(c, E.molIds, ids.data(), nmol);
+        fill_int(c, E.atomMolO, c.n, -1);
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        c.listValid = false;
+    });
+}
+
+int mdx_forces_compute(mdx_ctx *ctx, int flags) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        MDX_REQUIRE(c.rlist > 0, MDX_ERR_STATE, "mdx_nlist_config not called");
+        prepare_terms(c);
+        ensure_list(c);
+        compute(c, flags);
+        MDX_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int mdx_forces_get(mdx_ctx *ctx, double *f) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        MDX_REQUIRE(f, MDX_ERR_ARG, "output required");
+        std::vector<unsigned long long> hf(3 * (size_t)c.n);
+        std::vector<int> ids(c.n);
+        MDX_HIP(hipMemcpyAsync(hf.data(), c.force, sizeof(unsigned long long) * hf.size(), hipMemcpyDeviceToHost, c.stream));
+        MDX_HIP(hipMemcpyAsync(ids.data(), c.id[c.cur], sizeof(int) * c.n, hipMemcpyDeviceToHost, c.stream));
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        for (int k = 0; k < c.n; ++k)
+            for (int a = 0; a < 3; ++a) f[3 * ids[k] + a] = (double)(long long)hf[3 * k + a] * kFxInv;
+    });
+}
+
+int mdx_energy_get(mdx_ctx *ctx, double e[4], double virial[6]) {
+    return guarded(ctx, [&](Ctx &c) {
+        unsigned long long h[ACC_N];
+        MDX_HIP(hipMemcpyAsync(h, c.acc, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        auto d = [&](int s) { return (double)(long long)h[s] * kFxInv; };
+        if (e) {
+            e[0] = d(ACC_PAIR);
+            e[1] = d(ACC_KSPACE);
+            if (c.pppm.on) e[1] -= c.pppm.alpha / std::sqrt(M_PI) * c.pppm.totalQ2;
+            e[2] = d(ACC_E3B2);
+            e[3] = d(ACC_E3B3);
+        }
+        if (virial)
+            for (int a = 0; a < 6; ++a) virial[a] = d(ACC_VIR + a);
+    });
+}
+
+int mdx_integrator_set(mdx_ctx *ctx, double dt, double ftm_to_v) {
+    return guarded(ctx, [&](Ctx &c) {
+        MDX_REQUIRE(dt > 0, MDX_ERR_ARG, "dt must be positive");
+        c.dt = dt;
+        c.ftmToV = ftm_to_v;
+    });
+}
+
+int mdx_run(mdx_ctx *ctx, int nsteps, int flags) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        MDX_REQUIRE(c.rlist > 0, MDX_ERR_STATE, "mdx_nlist_config not called");
+        MDX_REQUIRE(nsteps >= 0, MDX_ERR_ARG, "nsteps must be >= 0");
+        const int fflags = flags & (MDX_PAIR | MDX_PPPM | MDX_E3B);
+        prepare_terms(c);
+        if (!c.listValid || c.forceFlags != fflags) {
+            ensure_list(c);
+            compute(c, fflags);
+        }
+        for (int s = 0; s < nsteps; ++s) {
+            integrate_first_half(c);
+            nlist_check(c);
+            nlist_rebuild(c, false);
+            compute(c, fflags);
+            integrate_second_half(c);
+            ++c.turn;
+        }
+        MDX_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int mdx_teams_set(mdx_ctx *ctx, int build_team, int pair_team, int e3b_team) {
+    return guarded(ctx, [&](Ctx &c) {
+        auto ok = [](int v, int lo, int hi) { return v == 0 || (v >= lo && v <= hi && (v & (v - 1)) == 0); };
+        MDX_REQUIRE(ok(build_team, 8, 64) && ok(pair_team, 4, 32) && ok(e3b_team, 4, 16), MDX_ERR_ARG,
+                    "team sizes: build 8..64, pair 4..32, e3b 4..16, powers of two");
+        if (build_team) c.buildTeam = build_team;
+        if (pair_team) c.pairTeam = pair_team;
+        if (e3b_team) c.e3b.team = e3b_team;
+    });
+}
+
+int mdx_sync(mdx_ctx *ctx) {
+    return guarded(ctx, [&](Ctx &c) { MDX_HIP(hipStreamSynchronize(c.stream)); });
+}
+
+}  // extern "C"

@@ -1,0 +1,305 @@
+// E3B three-body water forces.  Replaces FixE3B::compute (src/Fixes/FixE3B.cu:
+// 85-347) and its kernels compute_E3B_force_twobody / _threebody
+// (src/Evaluators/ThreeBodyE3B.h:18-549) with the evaluator functions of
+// EvaluatorE3B.h:67-249.
+//
+// Design (gfx950): owner-computes in two passes, no float atomics and no
+// per-slot scratch arrays (the reference stores four force vectors and four
+// energies per list slot and clears seven arrays every step):
+//   pass 1: a team per molecule walks its molecule list, adds the two-body
+//           O-O force to its own oxygen, and sums its own totals
+//           x = sum f(H_j -> O_i), y = sum f(H1_i -> O_j), z = sum f(H2_i -> O_j);
+//   pass 2: a team per molecule recomputes the four hydrogen-bond weights of
+//           each pair within r_max, reads the partner's totals, and applies
+//           dE/df of every bond to its own O, H1, H2 (the fscale values of
+//           threeBodyPairForce).  Each pair is evaluated by both owners, so
+//           every atom is written by exactly one team.
+// The pair test r_OO <= 6.1572 (RMAX_E3B, ThreeBodyE3B.h:15) is symmetric, so
+// both owners agree on it.
+#include "mdx_internal.h"
+
+namespace mdx {
+
+namespace {
+
+struct E3BArgs {
+    int nmol;
+    const int4 *molAtoms;
+    const float4 *xq;
+    int cap;
+    const int *nl;
+    const int *nlCount;
+    float L[3], invL[3];
+    float E2, Ea, Eb, Ec, k2, k3, rs, rf, r2b2, rmax;
+    float4 *tot;
+    unsigned long long *force;
+    unsigned long long *acc;
+};
+
+__device__ __forceinline__ float3 mi3(float4 a, float4 b, const E3BArgs &p) {
+    float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    dx -= p.L[0] * rintf(dx * p.invL[0]);
+    dy -= p.L[1] * rintf(dy * p.invL[1]);
+    dz -= p.L[2] * rintf(dz * p.invL[2]);
+    return make_float3(dx, dy, dz);
+}
+
+// f(r) = s(r) exp(-k3 r) for dr = r_H - r_O (threeBodyPairEnergy,
+// EvaluatorE3B.h:119-125); g = dr (k3 e s - e s') / r = -grad_H f
+// (threeBodyForce, 127-141).
+__device__ __forceinline__ float hb(const float3 &dr, const E3BArgs &p, float3 &g) {
+    const float r = sqrtf(dr.x * dr.x + dr.y * dr.y + dr.z * dr.z);
+    if (r > p.rf) {
+        g = make_float3(0.f, 0.f, 0.f);
+        return 0.f;
+    }
+    const float es = __expf(-p.k3 * r);
+    const float inv = 1.0f / ((p.rf - p.rs) * (p.rf - p.rs) * (p.rf - p.rs));
+    float s = 1.0f, fs = p.k3 * es / r;
+    if (r > p.rs) {
+        const float u = p.rf - r;
+        s = u * u * (p.rf + 2.0f * r - 3.0f * p.rs) * inv;
+        const float ds = 6.0f * (p.rf - r) * (p.rs - r) * inv;
+        fs = fs * s - es * ds / r;
+    }
+    g = make_float3(dr.x * fs, dr.y * fs, dr.z * fs);
+    return s * es;
+}
+
+__device__ __forceinline__ float hb_only(const float3 &dr, const E3BArgs &p) {
+    const float r = sqrtf(dr.x * dr.x + dr.y * dr.y + dr.z * dr.z);
+    if (r > p.rf) return 0.f;
+    const float es = __expf(-p.k3 * r);
+    if (r <= p.rs) return es;
+    const float u = p.rf - r;
+    return u * u * (p.rf + 2.0f * r - 3.0f * p.rs) / ((p.rf - p.rs) * (p.rf - p.rs) * (p.rf - p.rs)) * es;
+}
+
+template <int P>
+__device__ __forceinline__ void block_acc(double v, int slot, unsigned long long *acc) {
+    // one wave64 reduction, then lane 0 adds (few hundred atomics per launch)
+    double x = team_sum_d<64>(v);
+    if ((threadIdx.x & 63) == 0 && x != 0.0) fx_add(&acc[slot], x);
+}
+
+template <int P, bool EV>
+__global__ void __launch_bounds__(256) k_e3b_pass1(E3BArgs p) {
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = gt / P;
+    const int tl = threadIdx.x & (P - 1);
+    float fx = 0.f, fy = 0.f, fz = 0.f, tx = 0.f, ty = 0.f, tz = 0.f, e2 = 0.f;
+    float w0 = 0.f, w1 = 0.f, w2 = 0.f, w3 = 0.f, w4 = 0.f, w5 = 0.f;
+    int4 ai = make_int4(0, 0, 0, 0);
+    if (i < p.nmol) {
+        ai = p.molAtoms[i];
+        const float4 Oi = p.xq[ai.x], H1i = p.xq[ai.y], H2i = p.xq[ai.z];
+        const int cnt = min(p.nlCount[i], p.cap);
+        const int *row = p.nl + (size_t)i * p.cap;
+        for (int k = tl; k < cnt; k += P) {
+            const int4 aj = p.molAtoms[row[k]];
+            const float4 Oj = p.xq[aj.x];
+            const float3 d = mi3(Oi, Oj, p);
+            const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+            const float r = sqrtf(r2);
+            if (r2 < p.r2b2) {
+                const float ex = __expf(-p.k2 * r);
+                const float s = p.k2 * p.E2 * ex / r;
+                fx += d.x * s;
+                fy += d.y * s;
+                fz += d.z * s;
+                if (EV) {
+                    e2 += 0.5f * p.E2 * ex;
+                    w0 += d.x * s * d.x;
+                    w1 += d.y * s * d.y;
+                    w2 += d.z * s * d.z;
+                    w3 += d.x * s * d.y;
+                    w4 += d.x * s * d.z;
+                    w5 += d.y * s * d.z;
+                }
+            }
+            if (r <= p.rmax) {
+                const float4 H1j = p.xq[aj.y], H2j = p.xq[aj.z];
+                tx += hb_only(mi3(H1j, Oi, p), p) + hb_only(mi3(H2j, Oi, p), p);
+                ty += hb_only(mi3(H1i, Oj, p), p);
+                tz += hb_only(mi3(H2i, Oj, p), p);
+            }
+        }
+    }
+    fx = team_sum<P>(fx);
+    fy = team_sum<P>(fy);
+    fz = team_sum<P>(fz);
+    tx = team_sum<P>(tx);
+    ty = team_sum<P>(ty);
+    tz = team_sum<P>(tz);
+    if (tl == 0 && i < p.nmol) {
+        fx_add(&p.force[3 * ai.x + 0], fx);
+        fx_add(&p.force[3 * ai.x + 1], fy);
+        fx_add(&p.force[3 * ai.x + 2], fz);
+        p.tot[i] = make_float4(tx, ty, tz, 0.f);
+    }
+    if (EV) {
+        block_acc<P>(e2, ACC_E3B2, p.acc);
+        block_acc<P>(0.5 * w0, ACC_VIR + 0, p.acc);
+        block_acc<P>(0.5 * w1, ACC_VIR + 1, p.acc);
+        block_acc<P>(0.5 * w2, ACC_VIR + 2, p.acc);
+        block_acc<P>(0.5 * w3, ACC_VIR + 3, p.acc);
+        block_acc<P>(0.5 * w4, ACC_VIR + 4, p.acc);
+        block_acc<P>(0.5 * w5, ACC_VIR + 5, p.acc);
+    }
+}
+
+template <int P, bool EV>
+__global__ void __launch_bounds__(256) k_e3b_pass2(E3BArgs p) {
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = gt / P;
+    const int tl = threadIdx.x & (P - 1);
+    float3 FO = make_float3(0.f, 0.f, 0.f), FH1 = FO, FH2 = FO;
+    float ea = 0.f, eb = 0.f, ec = 0.f;
+    float w0 = 0.f, w1 = 0.f, w2 = 0.f, w3 = 0.f, w4 = 0.f, w5 = 0.f;
+    int4 ai = make_int4(0, 0, 0, 0);
+    float4 ti = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (i < p.nmol) {
+        ai = p.molAtoms[i];
+        ti = p.tot[i];
+        const float4 Oi = p.xq[ai.x], H1i = p.xq[ai.y], H2i = p.xq[ai.z];
+        const int cnt = min(p.nlCount[i], p.cap);
+        const int *row = p.nl + (size_t)i * p.cap;
+        for (int k = tl; k < cnt; k += P) {
+            const int mj = row[k];
+            const int4 aj = p.molAtoms[mj];
+            const float4 Oj = p.xq[aj.x];
+            const float3 d = mi3(Oi, Oj, p);
+            const float r = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+            if (!(r <= p.rmax)) continue;
+            const float4 H1j = p.xq[aj.y], H2j = p.xq[aj.z];
+            const float4 tj = p.tot[mj];
+            float3 gA1, gA2, gB1, gB2;
+            const float3 dA1 = mi3(H1i, Oj, p), dA2 = mi3(H2i, Oj, p);
+            const float3 dB1 = mi3(H1j, Oi, p), dB2 = mi3(H2j, Oi, p);
+            const float fA1 = hb(dA1, p, gA1), fA2 = hb(dA2, p, gA2);
+            const float fB1 = hb(dB1, p, gB1), fB2 = hb(dB2, p, gB2);
+            const float Dij = fA1 + fA2, Dji = fB1 + fB2;
+            const float bI = p.Eb * (ti.x + tj.y + tj.z - 2.0f * Dji);
+            const float bJ = p.Eb * (tj.x + ti.y + ti.z - 2.0f * Dij);
+            const float cA = p.Ec * (tj.x - Dij);
+            const float cB = p.Ec * (ti.x - Dji);
+            const float cA1 = p.Ea * (ti.z - fA2) + bI + cA;
+            const float cA2 = p.Ea * (ti.y - fA1) + bI + cA;
+            const float cB1 = p.Ea * (tj.z - fB2) + bJ + cB;
+            const float cB2 = p.Ea * (tj.y - fB1) + bJ + cB;
+            FH1.x += cA1 * gA1.x;
+            FH1.y += cA1 * gA1.y;
+            FH1.z += cA1 * gA1.z;
+            FH2.x += cA2 * gA2.x;
+            FH2.y += cA2 * gA2.y;
+            FH2.z += cA2 * gA2.z;
+            FO.x -= cB1 * gB1.x + cB2 * gB2.x;
+            FO.y -= cB1 * gB1.y + cB2 * gB2.y;
+            FO.z -= cB1 * gB1.z + cB2 * gB2.z;
+            if (EV) {
+                ea += fA1 * fA2;
+                eb += Dij * Dji;
+                ec += Dji * Dji;
+                // bond virial c g (x) (r_H - r_O), half per owner
+                w0 += 0.5f * (cA1 * gA1.x * dA1.x + cA2 * gA2.x * dA2.x + cB1 * gB1.x * dB1.x + cB2 * gB2.x * dB2.x);
+                w1 += 0.5f * (cA1 * gA1.y * dA1.y + cA2 * gA2.y * dA2.y + cB1 * gB1.y * dB1.y + cB2 * gB2.y * dB2.y);
+                w2 += 0.5f * (cA1 * gA1.z * dA1.z + cA2 * gA2.z * dA2.z + cB1 * gB1.z * dB1.z + cB2 * gB2.z * dB2.z);
+                w3 += 0.5f * (cA1 * gA1.x * dA1.y + cA2 * gA2.x * dA2.y + cB1 * gB1.x * dB1.y + cB2 * gB2.x * dB2.y);
+                w4 += 0.5f * (cA1 * gA1.x * dA1.z + cA2 * gA2.x * dA2.z + cB1 * gB1.x * dB1.z + cB2 * gB2.x * dB2.z);
+                w5 += 0.5f * (cA1 * gA1.y * dA1.z + cA2 * gA2.y * dA2.z + cB1 * gB1.y * dB1.z + cB2 * gB2.y * dB2.z);
+            }
+        }
+    }
+    FO.x = team_sum<P>(FO.x);
+    FO.y = team_sum<P>(FO.y);
+    FO.z = team_sum<P>(FO.z);
+    FH1.x = team_sum<P>(FH1.x);
+    FH1.y = team_sum<P>(FH1.y);
+    FH1.z = team_sum<P>(FH1.z);
+    FH2.x = team_sum<P>(FH2.x);
+    FH2.y = team_sum<P>(FH2.y);
+    FH2.z = team_sum<P>(FH2.z);
+    if (tl == 0 && i < p.nmol) {
+        fx_add(&p.force[3 * ai.x + 0], FO.x);
+        fx_add(&p.force[3 * ai.x + 1], FO.y);
+        fx_add(&p.force[3 * ai.x + 2], FO.z);
+        fx_add(&p.force[3 * ai.y + 0], FH1.x);
+        fx_add(&p.force[3 * ai.y + 1], FH1.y);
+        fx_add(&p.force[3 * ai.y + 2], FH1.z);
+        fx_add(&p.force[3 * ai.z + 0], FH2.x);
+        fx_add(&p.force[3 * ai.z + 1], FH2.y);
+        fx_add(&p.force[3 * ai.z + 2], FH2.z);
+    }
+    if (EV) {
+        ea = team_sum<P>(ea);
+        eb = team_sum<P>(eb);
+        ec = team_sum<P>(ec);
+        double e3 = 0.0;
+        if (tl == 0 && i < p.nmol) {
+            // E_A + E_B + E_C contributions owned by molecule i (donor for A,
+            // centre for B, acceptor for C)
+            e3 = (double)p.Ea * ((double)ti.y * ti.z - ea) +
+                 (double)p.Eb * ((double)ti.x * (ti.y + ti.z) - eb) +
+                 0.5 * (double)p.Ec * ((double)ti.x * ti.x - ec);
+        }
+        block_acc<P>(e3, ACC_E3B3, p.acc);
+        block_acc<P>(w0, ACC_VIR + 0, p.acc);
+        block_acc<P>(w1, ACC_VIR + 1, p.acc);
+        block_acc<P>(w2, ACC_VIR + 2, p.acc);
+        block_acc<P>(w3, ACC_VIR + 3, p.acc);
+        block_acc<P>(w4, ACC_VIR + 4, p.acc);
+        block_acc<P>(w5, ACC_VIR + 5, p.acc);
+    }
+}
+
+template <int P>
+void launch(const E3BArgs &a, bool ev, hipStream_t s) {
+    const int threads = 256;
+    const int blocks = cdiv((long long)a.nmol * P, threads);
+    if (ev) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass1<P, true>), dim3(blocks), dim3(threads), 0, s, a);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass2<P, true>), dim3(blocks), dim3(threads), 0, s, a);
+    } else {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass1<P, false>), dim3(blocks), dim3(threads), 0, s, a);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass2<P, false>), dim3(blocks), dim3(threads), 0, s, a);
+    }
+}
+
+}  // namespace
+
+void e3b_forces(Ctx &c, bool energy, bool virial) {
+    E3BState &E = c.e3b;
+    if (!E.on || E.nmol == 0) return;
+    E3BArgs a;
+    a.nmol = E.nmol;
+    a.molAtoms = E.molAtoms;
+    a.xq = c.xq;
+    a.cap = E.cap;
+    a.nl = E.nl;
+    a.nlCount = E.nlCount;
+    for (int d = 0; d < 3; ++d) {
+        a.L[d] = (float)c.geo.L[d];
+        a.invL[d] = c.geo.periodic[d] ? (float)(1.0 / c.geo.L[d]) : 0.0f;
+    }
+    a.E2 = E.E2;
+    a.Ea = E.Ea;
+    a.Eb = E.Eb;
+    a.Ec = E.Ec;
+    a.k2 = E.k2;
+    a.k3 = E.k3;
+    a.rs = E.rs;
+    a.rf = E.rf;
+    a.r2b2 = E.r2b * E.r2b;
+    a.rmax = E.rmax;
+    a.tot = E.tot;
+    a.force = c.force;
+    a.acc = c.acc;
+    const bool ev = energy || virial;
+    switch (E.team) {
+        case 4: launch<4>(a, ev, c.stream); break;
+        case 8: launch<8>(a, ev, c.stream); break;
+        default: launch<16>(a, ev, c.stream); break;
+    }
+}
+
+}  // namespace mdx

@@ -1,0 +1,73 @@
+// FP64 velocity-Verlet integration on the cell-ordered arrays.  Replaces the
+// reference's preForce_cu (half kick + drift + force zero,
+// src/Integrators/IntegratorVerlet.cu:307-338) and postForce_cu (half kick,
+// 610-627).  Forces are read from the 64-bit fixed-point buffer; the inverse
+// mass rides in vel.w (GPUData.h:18-28 keeps it in vs.w), massless sites
+// (inverse mass 0) are left to their constraint fix.
+#include "mdx_internal.h"
+
+namespace mdx {
+
+namespace {
+
+__global__ void k_zero_u64(int n, unsigned long long *__restrict__ p) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+__device__ __forceinline__ double fx_to_d(unsigned long long v) { return (double)(long long)v * kFxInv; }
+
+__global__ void k_vv1(int n, double4 *__restrict__ pos, double4 *__restrict__ vel,
+                      const unsigned long long *__restrict__ f, double hdtf, double dt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 v = vel[i];
+    if (v.w == 0.0) return;
+    double4 x = pos[i];
+    const double s = hdtf * v.w;
+    v.x += s * fx_to_d(f[3 * i + 0]);
+    v.y += s * fx_to_d(f[3 * i + 1]);
+    v.z += s * fx_to_d(f[3 * i + 2]);
+    x.x += dt * v.x;
+    x.y += dt * v.y;
+    x.z += dt * v.z;
+    vel[i] = v;
+    pos[i] = x;
+}
+
+__global__ void k_vv2(int n, double4 *__restrict__ vel, const unsigned long long *__restrict__ f,
+                      double hdtf) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 v = vel[i];
+    if (v.w == 0.0) return;
+    const double s = hdtf * v.w;
+    v.x += s * fx_to_d(f[3 * i + 0]);
+    v.y += s * fx_to_d(f[3 * i + 1]);
+    v.z += s * fx_to_d(f[3 * i + 2]);
+    vel[i] = v;
+}
+
+}  // namespace
+
+void zero_forces(Ctx &c, bool clear_acc) {
+    const int threads = 256;
+    hipLaunchKernelGGL(k_zero_u64, dim3(cdiv(3LL * c.n, threads)), dim3(threads), 0, c.stream, 3 * c.n,
+                       c.force);
+    if (clear_acc)
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, (int)ACC_N, c.acc);
+}
+
+void integrate_first_half(Ctx &c) {
+    const int threads = 256;
+    hipLaunchKernelGGL(k_vv1, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[c.cur],
+                       c.vel[c.cur], c.force, 0.5 * c.dt * c.ftmToV, c.dt);
+}
+
+void integrate_second_half(Ctx &c) {
+    const int threads = 256;
+    hipLaunchKernelGGL(k_vv2, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.vel[c.cur],
+                       c.force, 0.5 * c.dt * c.ftmToV);
+}
+
+}  // namespace mdx

@@ -1,0 +1,237 @@
+// Internal state and helpers of libmdx (not part of the C ABI).
+//
+// Data layout in HBM (one context = one system on one GPU):
+//   * per-atom arrays are kept in cell order (columns of cells ranked along a
+//     2-D Morton curve, z contiguous inside a column); the order is rebuilt at
+//     every neighbor-list rebuild and idToIdx maps atom id -> current index;
+//   * positions and velocities are FP64 (double4, w unused / inverse mass);
+//   * xq is an FP32 copy of (x - lo, q) refreshed every step for the pair,
+//     E3B and mesh kernels;
+//   * forces, energies and virials accumulate as 64-bit fixed point
+//     (value * 2^32) through integer atomics, so the result does not depend on
+//     the order in which kernels or lanes add their contributions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace mdx {
+
+constexpr double kFxScale = 4294967296.0;         // 2^32, forces / energies / virials
+constexpr double kFxInv = 1.0 / 4294967296.0;
+constexpr double kMeshScale = 1099511627776.0;    // 2^40, P3M charge mesh
+constexpr double kMeshInv = 1.0 / 1099511627776.0;
+constexpr int kMaxTypes = 16;
+
+// accumulator slots (fixed point, kFxScale)
+enum AccSlot {
+    ACC_PAIR = 0,     // LJ + Ewald real-space energy
+    ACC_KSPACE = 1,   // P3M reciprocal-space energy (without self term)
+    ACC_E3B2 = 2,     // E3B two-body energy
+    ACC_E3B3 = 3,     // E3B three-body energy
+    ACC_VIR = 4,      // 6 slots: xx yy zz xy xz yz
+    ACC_N = 16
+};
+
+struct Geometry {
+    double lo[3];
+    double L[3];
+    double invL[3];
+    int periodic[3];
+};
+
+struct PppmState {
+    bool on = false;
+    int sz[3] = {64, 64, 64};
+    int order = 5;
+    double rcut = 9.0;
+    double qqr = 332.06371;
+    double alpha = 0.0;
+    double alphaUser = -1.0;   // > 0: fixed alpha, else secant solve
+    double totalQ2 = 0.0;
+    double err = 0.0;
+    double boxAtGreen[3] = {0, 0, 0};
+    long long *mesh = nullptr;       // sx*sy*sz fixed point
+    float *real = nullptr;           // sx*sy*sz
+    float2 *rhoHat = nullptr;        // sx*sy*(sz/2+1)
+    float2 *eHat = nullptr;          // 3 * sx*sy*(sz/2+1)
+    float *eReal = nullptr;          // 3 * sx*sy*sz
+    float *green = nullptr;          // sx*sy*(sz/2+1)
+    hipfftHandle planR2C = 0;
+    hipfftHandle planC2R = 0;
+    bool plansMade = false;
+};
+
+struct E3BState {
+    bool on = false;
+    int nmol = 0;
+    float E2 = 0, Ea = 0, Eb = 0, Ec = 0, k2 = 0, k3 = 0, rs = 0, rf = 0;
+    float r2b = 6.2f;       // two-body cutoff (molecule-list radius of the reference)
+    float rmax = 6.1572f;   // three-body pair test
+    int4 *molIds = nullptr;   // by molecule: atom ids (O, H1, H2, M)
+    int4 *molAtoms = nullptr; // by molecule: current atom indices
+    int *atomMolO = nullptr;  // by current atom index: molecule index if the atom is an E3B oxygen, else -1
+    int *molIdsOfO = nullptr; // by atom id: molecule index if oxygen, else -1
+    int cap = 0;
+    int *nl = nullptr;        // nmol * cap molecule indices
+    int *nlCount = nullptr;
+    float4 *tot = nullptr;    // per molecule totals (x accepted, y H1 donated, z H2 donated)
+    int team = 8;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // system
+    int n = 0;
+    Geometry geo{};
+    double4 *pos[2] = {nullptr, nullptr};
+    double4 *vel[2] = {nullptr, nullptr};
+    float *q[2] = {nullptr, nullptr};
+    int *type[2] = {nullptr, nullptr};
+    int *id[2] = {nullptr, nullptr};
+    int cur = 0;
+    int *idToIdx = nullptr;
+    float4 *xq = nullptr;
+    unsigned long long *force = nullptr;   // 3 per atom
+    unsigned long long *acc = nullptr;     // ACC_N
+    double4 *posLast = nullptr;
+    std::vector<double> invMassHost;       // by id
+    bool hasCharges = false;
+    int maxType = 0;
+
+    // neighbor list
+    double rlist = 0.0, skin = 0.0;
+    bool nlistConfigured = false;
+    int nc[3] = {0, 0, 0};
+    int reach = 2;
+    int ncell = 0;
+    int *colRank = nullptr;
+    int *cellCount = nullptr;
+    int *cellStart = nullptr;
+    int *cellFill = nullptr;
+    int *cellOf = nullptr;
+    int *perm = nullptr;
+    int cap = 0;
+    uint32_t *nl = nullptr;
+    int *nlCount = nullptr;
+    int *dFlags = nullptr;        // [0] rebuild flag, [1] max count, [2] e3b max count
+    int buildTeam = 32;
+    long long nRebuilds = 0;
+    bool listValid = false;
+
+    // exclusions (CSR by id, entries id | tag << 30)
+    int *exclOff = nullptr;
+    uint32_t *exclIds = nullptr;
+    int nExclIds = 0;
+    int maxExcl = 0;
+
+    // LJ + Ewald real space
+    bool ljOn = false;
+    int ntypes = 1;
+    float *ljParams = nullptr;    // 3 * T*T: rc2, eps24, sig6
+    float special[3] = {0.0f, 0.0f, 0.5f};
+    bool ewaldReal = false;
+    float alpha = 0.0f, qqr = 332.06371f, rq2 = 0.0f;
+    int pairTeam = 16;
+
+    PppmState pppm;
+    E3BState e3b;
+
+    // integration
+    double dt = 0.005;
+    double ftmToV = 1.0;
+    long long turn = 0;
+    int forceFlags = 0;
+
+    // all device allocations, freed in mdx_destroy
+    std::vector<void *> allocs;
+};
+
+// error helpers -------------------------------------------------------------
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define MDX_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            throw ::mdx::Error{2, std::string(#call) + ": " + hipGetErrorString(e_)};   \
+    } while (0)
+
+#define MDX_REQUIRE(cond, code, msg)                                                     \
+    do {                                                                                 \
+        if (!(cond)) throw ::mdx::Error{(code), (msg)};                                  \
+    } while (0)
+
+template <class T>
+T *dalloc(Ctx &c, size_t count) {
+    void *p = nullptr;
+    if (count == 0) count = 1;
+    MDX_HIP(hipMalloc(&p, count * sizeof(T)));
+    c.allocs.push_back(p);
+    return static_cast<T *>(p);
+}
+
+void dfree(Ctx &c, void *p);
+
+inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// device helpers --------------------------------------------------------------
+__device__ __forceinline__ long long to_fx(double v) { return __double2ll_rn(v * kFxScale); }
+
+__device__ __forceinline__ void fx_add(unsigned long long *p, double v) {
+    atomicAdd(p, (unsigned long long)to_fx(v));
+}
+
+// sub-wave team sum by butterfly shuffles (T divides 64)
+template <int T>
+__device__ __forceinline__ float team_sum(float v) {
+#pragma unroll
+    for (int o = T / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, T);
+    return v;
+}
+
+template <int T>
+__device__ __forceinline__ double team_sum_d(double v) {
+#pragma unroll
+    for (int o = T / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, T);
+    return v;
+}
+
+// per-module entry points ------------------------------------------------------
+// nlist.hip
+void nlist_setup(Ctx &c);
+void nlist_check(Ctx &c);             // sets dFlags[0] when a rebuild is due
+void nlist_rebuild(Ctx &c, bool forced);  // rebuild if dFlags[0] or forced (host reads flag)
+void refresh_xq(Ctx &c);
+void fill_int(Ctx &c, int *p, int count, int v);  // stream-ordered fill (no memset nodes)
+void e3b_build_list(Ctx &c);
+
+// pair.hip
+void pair_forces(Ctx &c, bool energy, bool virial);
+
+// pppm.hip
+void pppm_prepare(Ctx &c);            // alpha + Green function for the current box
+void pppm_forces(Ctx &c, bool energy, bool virial);
+void pppm_release(Ctx &c);
+double pppm_solve_alpha(int natoms, double totalQ2, const double *L, const int *sz, int order,
+                        double rcut, double *err);
+
+// e3b.hip
+void e3b_forces(Ctx &c, bool energy, bool virial);
+
+// integrate.hip
+void integrate_first_half(Ctx &c);
+void integrate_second_half(Ctx &c);
+void zero_forces(Ctx &c, bool clear_acc);
+
+}  // namespace mdx

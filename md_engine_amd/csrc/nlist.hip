@@ -1,0 +1,572 @@
+// Cell binning, deterministic cell-order sort and full neighbor-list build.
+//
+// Replaces GridGPU::periodicBoundaryConditions (src/GridGPU.cu:824-1103) and
+// its kernels setBuildFlag (755-774), periodicWrap (138-161),
+// countNumInGridCells (190-203), sortPerAtomArrays (253-322),
+// countNumNeighbors / assignNeighbors (326-734).
+//
+// Design (gfx950):
+//   * cells have edge >= r_list / reach (reach = 2), so a (2*reach+1)^3 stencil
+//     holds every neighbor; columns of cells are ranked along a 2-D Morton
+//     curve and z runs contiguously inside a column, so each stencil column is
+//     one or two contiguous ranges of the sorted arrays (coalesced loads);
+//   * the sort is a counting sort whose slots inside a cell are re-ordered by
+//     atom id, so the layout depends only on positions, never on atomics order;
+//   * the list is built in one pass: a team of T lanes per atom tests T
+//     candidates per step with the canonical FP64 distance (no FMA contraction:
+//     this file is compiled with -ffp-contract=off) and compacts hits with a
+//     wave64 ballot; rows have a fixed capacity and the true count is kept so
+//     overflow is detected and the host grows the rows and rebuilds;
+//   * entries are index | tag << 30 with the 1-2/1-3/1-4 tag of the atom pair
+//     (GridGPU.cu:471-520, globalDefs.h:40).
+#include <algorithm>
+#include <cstring>
+
+#include "mdx_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace mdx {
+
+namespace {
+
+struct CellGrid {
+    int nc[3];
+    double cellInv[3];  // nc / L
+    int reach;
+};
+
+__device__ __forceinline__ double mi_d(double d, double L, double invL, int periodic) {
+    if (periodic) {
+        double img = rint(d * invL);
+        d = d - img * L;
+    }
+    return d;
+}
+
+// canonical squared distance, evaluated exactly as orc_dist2 in the oracle
+__device__ __forceinline__ double dist2(const double4 &a, const double4 &b, const Geometry &g) {
+    double dx = mi_d(b.x - a.x, g.L[0], g.invL[0], g.periodic[0]);
+    double dy = mi_d(b.y - a.y, g.L[1], g.invL[1], g.periodic[1]);
+    double dz = mi_d(b.z - a.z, g.L[2], g.invL[2], g.periodic[2]);
+    return dx * dx + dy * dy + dz * dz;
+}
+
+__device__ __forceinline__ int cell_coord(double x, double lo, double cellInv, int nc) {
+    int c = (int)floor((x - lo) * cellInv);
+    return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
+}
+
+// Max displacement since the last build (setBuildFlag, GridGPU.cu:755-774):
+// flag when any |minImage(x - x_last)|^2 > (skin/2)^2.
+__global__ void k_check(int n, const double4 *__restrict__ pos, const double4 *__restrict__ last,
+                        Geometry g, double thr2, int *__restrict__ flags) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool move = false;
+    if (i < n) move = dist2(pos[i], last[i], g) > thr2;
+    if (__any(move) && (threadIdx.x & 63) == 0) flags[0] = 1;
+}
+
+// Wrap into the box (periodicWrap) and bin into cells (countNumInGridCells).
+__global__ void k_bin(int n, double4 *__restrict__ pos, Geometry g, CellGrid cg,
+                      const int *__restrict__ colRank, int *__restrict__ cellOf,
+                      int *__restrict__ cellCount) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 p = pos[i];
+    double *pc = &p.x;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (g.periodic[a]) {
+            double img = floor((pc[a] - g.lo[a]) * g.invL[a]);
+            pc[a] = pc[a] - img * g.L[a];
+        }
+    }
+    pos[i] = p;
+    int cx = cell_coord(p.x, g.lo[0], cg.cellInv[0], cg.nc[0]);
+    int cy = cell_coord(p.y, g.lo[1], cg.cellInv[1], cg.nc[1]);
+    int cz = cell_coord(p.z, g.lo[2], cg.cellInv[2], cg.nc[2]);
+    int rank = colRank[cx * cg.nc[1] + cy] * cg.nc[2] + cz;
+    cellOf[i] = rank;
+    atomicAdd(&cellCount[rank], 1);
+}
+
+// Exclusive scan of the cell counts by one workgroup (replaces the host
+// cumulativeSum round trip, helpers.h:15-24).  Also seeds the fill cursors.
+__global__ void __launch_bounds__(1024) k_scan(int ncell, const int *__restrict__ count,
+                                               int *__restrict__ start, int *__restrict__ fill) {
+    __shared__ int wsum[16];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int base = 0; base < ncell; base += 1024) {
+        int idx = base + threadIdx.x;
+        int v = idx < ncell ? count[idx] : 0;
+        int x = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            int y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wid] = x;
+        __syncthreads();
+        if (wid == 0) {
+            int s = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                int y = __shfl_up(s, o, 64);
+                if (lane >= o) s += y;
+            }
+            if (lane < 16) wsum[lane] = s;
+        }
+        __syncthreads();
+        int excl = carry + (wid > 0 ? wsum[wid - 1] : 0) + x - v;
+        if (idx < ncell) {
+            start[idx] = excl;
+            fill[idx] = excl;
+        }
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = excl + v;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) start[ncell] = carry;
+}
+
+__global__ void k_place(int n, const int *__restrict__ cellOf, int *__restrict__ fill,
+                        int *__restrict__ perm) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int slot = atomicAdd(&fill[cellOf[i]], 1);
+    perm[slot] = i;
+}
+
+// Make the order inside each cell independent of atomic slot order: sort the
+// cell's entries by atom id (cells hold tens of atoms; insertion sort).
+__global__ void k_cellsort(int ncell, const int *__restrict__ start, int *__restrict__ perm,
+                           const int *__restrict__ id) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    int a = start[c], b = start[c + 1];
+    for (int s = a + 1; s < b; ++s) {
+        int v = perm[s];
+        int key = id[v];
+        int t = s - 1;
+        while (t >= a && id[perm[t]] > key) {
+            perm[t + 1] = perm[t];
+            --t;
+        }
+        perm[t + 1] = v;
+    }
+}
+
+// Gather the per-atom records into cell order (sortPerAtomArrays,
+// GridGPU.cu:253-293) and refresh idToIdx, the last-build positions and xq.
+__global__ void k_permute(int n, const int *__restrict__ perm, const double4 *__restrict__ pos0,
+                          const double4 *__restrict__ vel0, const float *__restrict__ q0,
+                          const int *__restrict__ t0, const int *__restrict__ id0,
+                          double4 *__restrict__ pos1, double4 *__restrict__ vel1,
+                          float *__restrict__ q1, int *__restrict__ t1, int *__restrict__ id1,
+                          int *__restrict__ idToIdx, double4 *__restrict__ last,
+                          float4 *__restrict__ xq, Geometry g) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int o = perm[k];
+    double4 p = pos0[o];
+    pos1[k] = p;
+    vel1[k] = vel0[o];
+    float qq = q0[o];
+    q1[k] = qq;
+    t1[k] = t0[o];
+    int myid = id0[o];
+    id1[k] = myid;
+    idToIdx[myid] = k;
+    last[k] = p;
+    xq[k] = make_float4((float)(p.x - g.lo[0]), (float)(p.y - g.lo[1]), (float)(p.z - g.lo[2]), qq);
+}
+
+__global__ void k_xq(int n, const double4 *__restrict__ pos, const float *__restrict__ q,
+                     float4 *__restrict__ xq, Geometry g) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double4 p = pos[k];
+    xq[k] = make_float4((float)(p.x - g.lo[0]), (float)(p.y - g.lo[1]), (float)(p.z - g.lo[2]), q[k]);
+}
+
+// Stencil ranges along one axis: either the (2 reach + 1) cells around c with
+// periodic wrap, or every cell once when the axis has fewer cells.
+struct AxisRange {
+    int count;
+    int v[5];
+};
+
+__device__ __forceinline__ AxisRange axis_cells(int c, int nc, int reach, int periodic) {
+    AxisRange r;
+    r.count = 0;
+    if (nc < 2 * reach + 1) {
+        for (int k = 0; k < nc && k < 5; ++k) r.v[r.count++] = k;
+        return r;
+    }
+    for (int d = -reach; d <= reach; ++d) {
+        int cc = c + d;
+        if (cc < 0 || cc >= nc) {
+            if (!periodic) continue;
+            cc = (cc + nc) % nc;
+        }
+        r.v[r.count++] = cc;
+    }
+    return r;
+}
+
+// One team of T lanes per atom.  Candidates come from contiguous z-runs of
+// each stencil column; hits are compacted with a ballot restricted to the
+// team, so row order is (stencil column, z-run, index) - deterministic.
+template <int T>
+__global__ void __launch_bounds__(256) k_build(
+    int n, const double4 *__restrict__ pos, const int *__restrict__ id, Geometry g, CellGrid cg,
+    const int *__restrict__ colRank, const int *__restrict__ cellStart, double rlist2,
+    const int *__restrict__ exclOff, const uint32_t *__restrict__ exclIds, int cap,
+    uint32_t *__restrict__ nl, int *__restrict__ nlCount, int *__restrict__ flags) {
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = gtid / T;
+    const int tl = threadIdx.x & (T - 1);
+    const int lane = threadIdx.x & 63;
+    if (i >= n) return;  // whole team leaves together (T divides 64)
+    const unsigned long long teamMask =
+        (T == 64) ? ~0ull : (((1ull << T) - 1ull) << (lane & ~(T - 1)));
+    const unsigned long long below = (1ull << lane) - 1ull;
+
+    const double4 pi = pos[i];
+    const int myid = id[i];
+    const int e0 = exclOff ? exclOff[myid] : 0;
+    const int e1 = exclOff ? exclOff[myid + 1] : 0;
+    const int cx = cell_coord(pi.x, g.lo[0], cg.cellInv[0], cg.nc[0]);
+    const int cy = cell_coord(pi.y, g.lo[1], cg.cellInv[1], cg.nc[1]);
+    const int cz = cell_coord(pi.z, g.lo[2], cg.cellInv[2], cg.nc[2]);
+    const AxisRange ax = axis_cells(cx, cg.nc[0], cg.reach, g.periodic[0]);
+    const AxisRange ay = axis_cells(cy, cg.nc[1], cg.reach, g.periodic[1]);
+    const int ncz = cg.nc[2];
+
+    // z-runs: at most two contiguous [z0, z1] runs
+    int zr0[2], zr1[2], nzr = 0;
+    if (ncz < 2 * cg.reach + 1) {
+        zr0[0] = 0;
+        zr1[0] = ncz - 1;
+        nzr = 1;
+    } else {
+        int lo = cz - cg.reach, hi = cz + cg.reach;
+        if (lo < 0) {
+            if (g.periodic[2]) {
+                zr0[nzr] = lo + ncz;
+                zr1[nzr] = ncz - 1;
+                ++nzr;
+            }
+            lo = 0;
+        }
+        if (hi >= ncz) {
+            if (g.periodic[2]) {
+                zr0[nzr] = 0;
+                zr1[nzr] = hi - ncz;
+                ++nzr;
+            }
+            hi = ncz - 1;
+        }
+        zr0[nzr] = lo;
+        zr1[nzr] = hi;
+        ++nzr;
+    }
+
+    uint32_t *row = nl + (size_t)i * cap;
+    int cnt = 0;
+    for (int a = 0; a < ax.count; ++a) {
+        for (int b = 0; b < ay.count; ++b) {
+            const int colBase = colRank[ax.v[a] * cg.nc[1] + ay.v[b]] * ncz;
+            for (int r = 0; r < nzr; ++r) {
+                const int s0 = cellStart[colBase + zr0[r]];
+                const int s1 = cellStart[colBase + zr1[r] + 1];
+                for (int base = s0; base < s1; base += T) {
+                    const int j = base + tl;
+                    bool hit = false;
+                    uint32_t entry = 0;
+                    if (j < s1 && j != i) {
+                        const double4 pj = pos[j];
+                        if (dist2(pi, pj, g) < rlist2) {
+                            hit = true;
+                            uint32_t tag = 0;
+                            if (e1 > e0) {
+                                const uint32_t jid = (uint32_t)id[j];
+                                for (int e = e0; e < e1; ++e) {
+                                    const uint32_t ex = exclIds[e];
+                                    if ((ex & 0x3FFFFFFFu) == jid) tag = ex >> 30;
+                                }
+                            }
+                            entry = (uint32_t)j | (tag << 30);
+                        }
+                    }
+                    const unsigned long long m = __ballot(hit) & teamMask;
+                    if (hit) {
+                        const int k = cnt + __popcll(m & below);
+                        if (k < cap) row[k] = entry;
+                    }
+                    cnt += __popcll(m);
+                }
+            }
+        }
+    }
+    if (tl == 0) {
+        nlCount[i] = cnt;
+        atomicMax(&flags[1], cnt);
+    }
+}
+
+// E3B molecule list from the atom list: for each water, the oxygens among its
+// oxygen's neighbors within (r2b + skin) at build time.  Replaces the private
+// molecule GridGPU of FixE3B (FixE3B.cu:656-684) that the reference rebuilds
+// every step; with the skin criterion the per-step r2b set is a subset.
+template <int T>
+__global__ void __launch_bounds__(256) k_e3b_list(
+    int nmol, const int4 *__restrict__ molAtoms, const int *__restrict__ atomMolO,
+    const double4 *__restrict__ pos, Geometry g, double rcut2, int cap, const uint32_t *__restrict__ nl,
+    const int *__restrict__ nlCount, int mcap, int *__restrict__ mnl, int *__restrict__ mcount,
+    int *__restrict__ flags) {
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = gtid / T;
+    const int tl = threadIdx.x & (T - 1);
+    const int lane = threadIdx.x & 63;
+    if (m >= nmol) return;
+    const unsigned long long teamMask =
+        (T == 64) ? ~0ull : (((1ull << T) - 1ull) << (lane & ~(T - 1)));
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const int oi = molAtoms[m].x;
+    const double4 pi = pos[oi];
+    const int cnt_i = min(nlCount[oi], cap);
+    const uint32_t *row = nl + (size_t)oi * cap;
+    int cnt = 0;
+    for (int base = 0; base < cnt_i; base += T) {
+        const int k = base + tl;
+        bool hit = false;
+        int mj = -1;
+        if (k < cnt_i) {
+            const int j = (int)(row[k] & 0x3FFFFFFFu);
+            mj = atomMolO[j];
+            if (mj >= 0 && mj != m) hit = dist2(pi, pos[j], g) < rcut2;
+        }
+        const unsigned long long msk = __ballot(hit) & teamMask;
+        if (hit) {
+            const int s = cnt + __popcll(msk & below);
+            if (s < mcap) mnl[(size_t)m * mcap + s] = mj;
+        }
+        cnt += __popcll(msk);
+    }
+    if (tl == 0) {
+        mcount[m] = cnt;
+        atomicMax(&flags[2], cnt);
+    }
+}
+
+__global__ void k_mol_atoms(int nmol, const int4 *__restrict__ molIds, const int *__restrict__ idToIdx,
+                            int4 *__restrict__ molAtoms, int *__restrict__ atomMolO) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmol) return;
+    int4 ids = molIds[m];
+    int4 a = make_int4(idToIdx[ids.x], idToIdx[ids.y], idToIdx[ids.z], idToIdx[ids.w]);
+    molAtoms[m] = a;
+    atomMolO[a.x] = m;
+}
+
+__global__ void k_fill_int(int n, int *__restrict__ p, int v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+uint32_t morton2(uint32_t x, uint32_t y) {
+    uint64_t r = 0;
+    for (int b = 0; b < 16; ++b) {
+        r |= (uint64_t)((x >> b) & 1u) << (2 * b);
+        r |= (uint64_t)((y >> b) & 1u) << (2 * b + 1);
+    }
+    return (uint32_t)r;
+}
+
+CellGrid make_grid(const Ctx &c) {
+    CellGrid cg;
+    cg.reach = c.reach;
+    for (int a = 0; a < 3; ++a) {
+        cg.nc[a] = c.nc[a];
+        cg.cellInv[a] = (double)c.nc[a] / c.geo.L[a];
+    }
+    return cg;
+}
+
+template <int T>
+void launch_build(Ctx &c, const CellGrid &cg, int cur) {
+    int threads = 256;
+    int blocks = cdiv((long long)c.n * T, threads);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_build<T>), dim3(blocks), dim3(threads), 0, c.stream, c.n,
+                       c.pos[cur], c.id[cur], c.geo, cg, c.colRank, c.cellStart, c.rlist * c.rlist,
+                       c.exclOff, c.exclIds, c.cap, c.nl, c.nlCount, c.dFlags);
+}
+
+template <int T>
+void launch_e3b_list(Ctx &c, int cur) {
+    int threads = 256;
+    int blocks = cdiv((long long)c.e3b.nmol * T, threads);
+    double rc = (double)c.e3b.r2b + c.skin;
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_list<T>), dim3(blocks), dim3(threads), 0, c.stream,
+                       c.e3b.nmol, c.e3b.molAtoms, c.e3b.atomMolO, c.pos[cur], c.geo, rc * rc, c.cap,
+                       c.nl, c.nlCount, c.e3b.cap, c.e3b.nl, c.e3b.nlCount, c.dFlags);
+}
+
+}  // namespace
+
+void nlist_setup(Ctx &c) {
+    MDX_REQUIRE(c.n > 0, 3, "system not set");
+    MDX_REQUIRE(c.rlist > 0, 1, "neighbor list cutoff must be positive");
+    for (int a = 0; a < 3; ++a) {
+        if (c.geo.periodic[a])
+            MDX_REQUIRE(c.rlist < 0.5 * c.geo.L[a], 1,
+                        "neighbor cutoff (rCut + padding) must be below half the box length");
+        double edge = c.rlist / c.reach * (1.0 + 1e-9);
+        int nca = (int)std::floor(c.geo.L[a] / edge);
+        c.nc[a] = std::max(1, nca);
+    }
+    int ncell = c.nc[0] * c.nc[1] * c.nc[2];
+    if (ncell != c.ncell) {
+        if (c.ncell) {
+            dfree(c, c.colRank);
+            dfree(c, c.cellCount);
+            dfree(c, c.cellStart);
+            dfree(c, c.cellFill);
+        }
+        c.ncell = ncell;
+        c.colRank = dalloc<int>(c, (size_t)c.nc[0] * c.nc[1]);
+        c.cellCount = dalloc<int>(c, ncell);
+        c.cellStart = dalloc<int>(c, ncell + 1);
+        c.cellFill = dalloc<int>(c, ncell);
+    }
+    // column ranks along a 2-D Morton curve
+    std::vector<std::pair<uint32_t, int>> keys;
+    for (int x = 0; x < c.nc[0]; ++x)
+        for (int y = 0; y < c.nc[1]; ++y) keys.push_back({morton2(x, y), x * c.nc[1] + y});
+    std::sort(keys.begin(), keys.end());
+    std::vector<int> rank(keys.size());
+    for (size_t r = 0; r < keys.size(); ++r) rank[keys[r].second] = (int)r;
+    MDX_HIP(hipMemcpyAsync(c.colRank, rank.data(), sizeof(int) * rank.size(), hipMemcpyHostToDevice,
+                           c.stream));
+    if (!c.cellOf) {
+        c.cellOf = dalloc<int>(c, c.n);
+        c.perm = dalloc<int>(c, c.n);
+    }
+    // team width: next power of two of the expected z-run population
+    double dens = c.n / (c.geo.L[0] * c.geo.L[1] * c.geo.L[2]);
+    double run = dens * (c.geo.L[0] / c.nc[0]) * (c.geo.L[1] / c.nc[1]) * (c.geo.L[2] / c.nc[2]) *
+                 std::min(c.nc[2], 2 * c.reach + 1);
+    int T = 8;
+    while (T < 64 && T < run) T *= 2;
+    c.buildTeam = T;
+    if (c.cap == 0) {
+        double expect = dens * 4.18879020479 * c.rlist * c.rlist * c.rlist;
+        c.cap = ((int)(expect * 1.3) + 32 + 15) & ~15;
+        c.nl = dalloc<uint32_t>(c, (size_t)c.n * c.cap);
+        c.nlCount = dalloc<int>(c, c.n);
+    }
+    c.nlistConfigured = true;
+}
+
+void refresh_xq(Ctx &c) {
+    int threads = 256;
+    hipLaunchKernelGGL(k_xq, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[c.cur],
+                       c.q[c.cur], c.xq, c.geo);
+}
+
+void nlist_check(Ctx &c) {
+    int threads = 256;
+    double thr = 0.5 * c.skin;
+    hipLaunchKernelGGL(k_check, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n,
+                       c.pos[c.cur], c.posLast, c.geo, thr * thr, c.dFlags);
+}
+
+static void run_build(Ctx &c) {
+    CellGrid cg = make_grid(c);
+    int threads = 256;
+    int cur = c.cur, nxt = 1 - c.cur;
+    hipLaunchKernelGGL(k_fill_int, dim3(cdiv(c.ncell, threads)), dim3(threads), 0, c.stream, c.ncell,
+                       c.cellCount, 0);
+    hipLaunchKernelGGL(k_bin, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur],
+                       c.geo, cg, c.colRank, c.cellOf, c.cellCount);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c.stream, c.ncell, c.cellCount, c.cellStart,
+                       c.cellFill);
+    hipLaunchKernelGGL(k_place, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.cellOf,
+                       c.cellFill, c.perm);
+    hipLaunchKernelGGL(k_cellsort, dim3(cdiv(c.ncell, 64)), dim3(64), 0, c.stream, c.ncell,
+                       c.cellStart, c.perm, c.id[cur]);
+    hipLaunchKernelGGL(k_permute, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.perm,
+                       c.pos[cur], c.vel[cur], c.q[cur], c.type[cur], c.id[cur], c.pos[nxt],
+                       c.vel[nxt], c.q[nxt], c.type[nxt], c.id[nxt], c.idToIdx, c.posLast, c.xq,
+                       c.geo);
+    c.cur = nxt;
+    for (;;) {
+        fill_int(c, c.dFlags + 1, 2, 0);
+        switch (c.buildTeam) {
+            case 8: launch_build<8>(c, cg, c.cur); break;
+            case 16: launch_build<16>(c, cg, c.cur); break;
+            case 32: launch_build<32>(c, cg, c.cur); break;
+            default: launch_build<64>(c, cg, c.cur); break;
+        }
+        int maxc = 0;
+        MDX_HIP(hipMemcpyAsync(&maxc, c.dFlags + 1, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        if (maxc <= c.cap) break;
+        // grow the rows and rebuild (the reference reallocates at 1.5x, GridGPU.cu:1026-1030)
+        dfree(c, c.nl);
+        c.cap = ((int)(maxc * 1.25) + 15) & ~15;
+        c.nl = dalloc<uint32_t>(c, (size_t)c.n * c.cap);
+    }
+    if (c.e3b.on) {
+        fill_int(c, c.e3b.atomMolO, c.n, -1);  // indices were permuted: drop the old map
+        hipLaunchKernelGGL(k_mol_atoms, dim3(cdiv(c.e3b.nmol, threads)), dim3(threads), 0, c.stream,
+                           c.e3b.nmol, c.e3b.molIds, c.idToIdx, c.e3b.molAtoms, c.e3b.atomMolO);
+        e3b_build_list(c);
+    }
+    c.nRebuilds++;
+    c.listValid = true;
+}
+
+void e3b_build_list(Ctx &c) {
+    MDX_REQUIRE(c.rlist >= (double)c.e3b.r2b + c.skin - 1e-9, 1,
+                "E3B needs rCut + padding >= 6.2 A + padding (the molecule list is taken from the "
+                "pair list)");
+    for (;;) {
+        fill_int(c, c.dFlags + 2, 1, 0);
+        switch (c.e3b.team) {
+            case 8: launch_e3b_list<8>(c, c.cur); break;
+            case 16: launch_e3b_list<16>(c, c.cur); break;
+            default: launch_e3b_list<32>(c, c.cur); break;
+        }
+        int maxc = 0;
+        MDX_HIP(hipMemcpyAsync(&maxc, c.dFlags + 2, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        if (maxc <= c.e3b.cap) break;
+        dfree(c, c.e3b.nl);
+        c.e3b.cap = ((int)(maxc * 1.25) + 15) & ~15;
+        c.e3b.nl = dalloc<int>(c, (size_t)c.e3b.nmol * c.e3b.cap);
+    }
+}
+
+void nlist_rebuild(Ctx &c, bool forced) {
+    if (!c.nlistConfigured) nlist_setup(c);
+    if (!forced) {
+        int flag = 0;
+        MDX_HIP(hipMemcpyAsync(&flag, c.dFlags, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        if (!flag && c.listValid) return;
+    }
+    run_build(c);
+    fill_int(c, c.dFlags, 1, 0);
+}
+
+void fill_int(Ctx &c, int *p, int count, int v) {
+    hipLaunchKernelGGL(k_fill_int, dim3(cdiv(count, 256)), dim3(256), 0, c.stream, count, p, v);
+}
+
+}  // namespace mdx

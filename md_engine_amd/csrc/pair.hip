@@ -1,0 +1,197 @@
+// Fused Lennard-Jones + Ewald real-space pair forces over the full neighbor
+// list.  Replaces compute_force_iso / compute_energy_iso
+// (src/Evaluators/PairEvaluateIso.h:8-358) instantiated with EvaluatorLJ
+// (PairEvaluatorLJ.h:17-60) and ChargeEvaluatorEwald
+// (ChargeEvaluatorEwald.h:11-63), the fusion the reference reaches through
+// charge offloading (State.cpp:648-660, FixPair.cpp:89-97,
+// EvaluatorWrapper.h:176-194).
+//
+// Design (gfx950): a team of P lanes (P = 4..32) owns one atom and strides
+// through its row, so each row load is a P-wide coalesced segment and the
+// neighbor positions (float4 x - lo, q) are gathered from cell-ordered memory
+// that stays L2-resident.  Arithmetic is FP32; the team's partial sums are
+// combined with butterfly shuffles and added once per atom into the 64-bit
+// fixed-point force buffer.  Energy and virial (0.5-weighted, full list) are
+// reduced in FP64 per block and added with one fixed-point atomic per block.
+#include "mdx_internal.h"
+
+namespace mdx {
+
+namespace {
+
+struct PairArgs {
+    int n;
+    const float4 *xq;
+    const int *type;
+    int cap;
+    const uint32_t *nl;
+    const int *nlCount;
+    const float *params;
+    int ntypes;
+    float L[3], invL[3];
+    float spec[3];
+    float alpha, qqr, rq2;
+    unsigned long long *force;
+    unsigned long long *acc;
+};
+
+template <int P, bool LJ, bool COUL, bool EV>
+__global__ void __launch_bounds__(256) k_pair(PairArgs a) {
+    __shared__ float sp[3 * kMaxTypes * kMaxTypes];
+    __shared__ double wred[4][7];
+    const int T2 = a.ntypes * a.ntypes;
+    if (LJ) {
+        for (int t = threadIdx.x; t < 3 * T2; t += blockDim.x) sp[t] = a.params[t];
+        __syncthreads();
+    }
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = gt / P;
+    const int tl = threadIdx.x & (P - 1);
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    float e = 0.f, v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
+    if (i < a.n) {
+        const float4 pi = a.xq[i];
+        const int ti = (LJ && a.ntypes > 1) ? a.type[i] : 0;
+        const int cnt = min(a.nlCount[i], a.cap);
+        const uint32_t *row = a.nl + (size_t)i * a.cap;
+        const float twoAlphaRtPi = 2.0f * 0.5641895835477563f * a.alpha;
+        const float alpha2 = a.alpha * a.alpha;
+        for (int k = tl; k < cnt; k += P) {
+            const uint32_t raw = row[k];
+            const int j = (int)(raw & 0x3FFFFFFFu);
+            const uint32_t tag = raw >> 30;
+            const float4 pj = a.xq[j];
+            float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            dx -= a.L[0] * rintf(dx * a.invL[0]);
+            dy -= a.L[1] * rintf(dy * a.invL[1]);
+            dz -= a.L[2] * rintf(dz * a.invL[2]);
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            const float m = tag == 0 ? 1.0f : a.spec[tag - 1];
+            float fs = 0.f;
+            bool any = false;
+            if (LJ) {
+                const int tp = a.ntypes > 1 ? ti * a.ntypes + a.type[j] : 0;
+                if (r2 < sp[tp]) {
+                    any = true;
+                    if (m != 0.0f) {
+                        const float eps24 = sp[T2 + tp], sig6 = sp[2 * T2 + tp];
+                        const float r2inv = 1.0f / r2;
+                        const float r6inv = r2inv * r2inv * r2inv;
+                        const float p1 = eps24 * 2.0f * sig6 * sig6;
+                        const float p2 = eps24 * sig6;
+                        fs += r6inv * r2inv * (p1 * r6inv - p2) * m;
+                        if (EV) {
+                            const float rc2 = sp[tp];
+                            const float s6r6 = sig6 * r6inv;
+                            const float rc6 = rc2 * rc2 * rc2;
+                            const float s6rc6 = sig6 / rc6;
+                            e += 0.5f * (eps24 * (1.0f / 6.0f)) *
+                                 (s6r6 * (s6r6 - 1.0f) - s6rc6 * (s6rc6 - 1.0f)) * m;
+                        }
+                    }
+                }
+            }
+            if (COUL && r2 < a.rq2) {
+                any = true;
+                const float ls = fmaxf(r2, 1e-10f);
+                const float rinv = rsqrtf(ls);
+                const float len = ls * rinv;
+                const float qq = a.qqr * pi.w * pj.w;
+                const float ar = a.alpha * len;
+                // erfc(ar) - (1 - m), written as -erf(ar) for a full exclusion
+                const float ec = (m == 0.0f) ? -erff(ar) : (erfcf(ar) - (1.0f - m));
+                const float s = qq * (ec * rinv + twoAlphaRtPi * __expf(-alpha2 * ls));
+                fs += s * rinv * rinv;
+                if (EV) e += 0.5f * qq * rinv * ec;
+            }
+            if (any) {
+                fx += dx * fs;
+                fy += dy * fs;
+                fz += dz * fs;
+                if (EV) {
+                    v0 += dx * fs * dx;
+                    v1 += dy * fs * dy;
+                    v2 += dz * fs * dz;
+                    v3 += dx * fs * dy;
+                    v4 += dx * fs * dz;
+                    v5 += dy * fs * dz;
+                }
+            }
+        }
+    }
+    fx = team_sum<P>(fx);
+    fy = team_sum<P>(fy);
+    fz = team_sum<P>(fz);
+    if (tl == 0 && i < a.n) {
+        fx_add(&a.force[3 * i + 0], fx);
+        fx_add(&a.force[3 * i + 1], fy);
+        fx_add(&a.force[3 * i + 2], fz);
+    }
+    if (EV) {
+        double vals[7] = {e, 0.5 * v0, 0.5 * v1, 0.5 * v2, 0.5 * v3, 0.5 * v4, 0.5 * v5};
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+        for (int s = 0; s < 7; ++s) {
+            double x = team_sum_d<64>(vals[s]);
+            if (lane == 0) wred[w][s] = x;
+        }
+        __syncthreads();
+        if (threadIdx.x < 7) {
+            double x = 0.0;
+            for (int ww = 0; ww < (int)(blockDim.x >> 6); ++ww) x += wred[ww][threadIdx.x];
+            fx_add(&a.acc[threadIdx.x == 0 ? ACC_PAIR : ACC_VIR + threadIdx.x - 1], x);
+        }
+    }
+}
+
+template <int P, bool LJ, bool COUL>
+void launch3(const PairArgs &a, bool ev, hipStream_t s) {
+    const int threads = 256;
+    const int blocks = cdiv((long long)a.n * P, threads);
+    if (ev)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair<P, LJ, COUL, true>), dim3(blocks), dim3(threads), 0, s, a);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair<P, LJ, COUL, false>), dim3(blocks), dim3(threads), 0, s, a);
+}
+
+template <int P>
+void launch2(const PairArgs &a, bool lj, bool coul, bool ev, hipStream_t s) {
+    if (lj && coul) launch3<P, true, true>(a, ev, s);
+    else if (lj) launch3<P, true, false>(a, ev, s);
+    else launch3<P, false, true>(a, ev, s);
+}
+
+}  // namespace
+
+void pair_forces(Ctx &c, bool energy, bool virial) {
+    const bool coul = c.ewaldReal && c.hasCharges;
+    if (!c.ljOn && !coul) return;
+    PairArgs a;
+    a.n = c.n;
+    a.xq = c.xq;
+    a.type = c.type[c.cur];
+    a.cap = c.cap;
+    a.nl = c.nl;
+    a.nlCount = c.nlCount;
+    a.params = c.ljParams;
+    a.ntypes = c.ntypes;
+    for (int d = 0; d < 3; ++d) {
+        a.L[d] = (float)c.geo.L[d];
+        a.invL[d] = c.geo.periodic[d] ? (float)(1.0 / c.geo.L[d]) : 0.0f;
+        a.spec[d] = c.special[d];
+    }
+    a.alpha = c.alpha;
+    a.qqr = c.qqr;
+    a.rq2 = c.rq2;
+    a.force = c.force;
+    a.acc = c.acc;
+    const bool ev = energy || virial;
+    switch (c.pairTeam) {
+        case 4: launch2<4>(a, c.ljOn, coul, ev, c.stream); break;
+        case 8: launch2<8>(a, c.ljOn, coul, ev, c.stream); break;
+        case 16: launch2<16>(a, c.ljOn, coul, ev, c.stream); break;
+        default: launch2<32>(a, c.ljOn, coul, ev, c.stream); break;
+    }
+}
+
+}  // namespace mdx

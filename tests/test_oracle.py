@@ -1,0 +1,352 @@
+"""Pins the FP64 oracle against closed forms, the reference's own analytic test
+vectors, brute force and finite differences (CPU only)."""
+import numpy as np
+import pytest
+from scipy.special import erf, erfc
+
+from md_engine_amd import systems
+
+
+def lj_tables(sigma, eps, rc):
+    return (np.array([[rc * rc]]), np.array([[24.0 * eps]]), np.array([[sigma ** 6]]))
+
+
+def two_atom_pair(orc, dr, sigma=3.0, eps=6.0, rc=10.0, q=(0.0, 0.0), charge=False, alpha=0.0,
+                  qqr=1.0, rq=0.0, tag=0, special=(0.0, 0.0, 0.5)):
+    L = np.array([100.0, 100.0, 100.0])
+    x = np.array([[10.0, 10.0, 10.0], np.array([10.0, 10.0, 10.0]) - dr])
+    rows = [np.array([1 | (tag << 30)], np.uint32), np.array([0 | (tag << 30)], np.uint32)]
+    rc2, e24, s6 = lj_tables(sigma, eps, rc)
+    f, e, v = orc.pair(x, np.array(q, float), np.zeros(2, np.int32), L, [1, 1], rows, rc2, e24, s6,
+                       special=special, with_lj=not charge, with_charge=charge, alpha=alpha, qqr=qqr,
+                       rq=rq)
+    return f, e, v
+
+
+# --- LJ: the reference's analytic vectors (tests/Evaluators/test_PairEvaluatorLJ.py:72-151) ---
+def test_lj_reference_vectors(orc):
+    sigma, eps, rc = 3.0, 6.0, 10.0
+    A = np.array([10.0, 10.0, 10.0])
+    B = [np.array([9.0, 10.0, 10.0]), np.array([8.0, 10.0, 10.0]), np.array([7.0, 10.0, 10.0]),
+         np.array([10.0 - sigma * 2.0 ** (1.0 / 6.0), 10.0, 10.0]),
+         np.array([9.23472, 9.84752, 8.293478]), np.array([11.0, 9.0, 12.0])]
+    s6 = sigma ** 6
+    s12 = s6 * s6
+    for b in B:
+        dr = A - b
+        r = np.linalg.norm(dr)
+        r2i = 1.0 / r ** 2
+        r6i = r2i ** 3
+        expect = dr * r6i * r2i * 24.0 * eps * (2.0 * s12 * r6i - s6)
+        f, _, _ = two_atom_pair(orc, dr, sigma, eps, rc)
+        diff = np.linalg.norm(f[0] - expect)
+        if diff > 1e-8:
+            assert diff / np.linalg.norm(expect) <= 1e-10
+        else:
+            assert diff <= 1e-10
+        assert np.allclose(f[1], -f[0], rtol=0, atol=1e-12)
+
+
+def test_lj_zero_force_at_minimum(orc):
+    sigma = 3.0
+    dr = np.array([sigma * 2 ** (1 / 6), 0, 0])
+    f, _, _ = two_atom_pair(orc, dr, sigma)
+    assert np.abs(f).max() < 1e-12
+
+
+def test_lj_energy_shift_and_fd(orc):
+    sigma, eps, rc = 1.0, 1.0, 2.5
+    # energy is shifted to zero at the cutoff (PairEvaluatorLJ.h:35-60)
+    _, e, _ = two_atom_pair(orc, np.array([rc * (1 - 1e-12), 0, 0]), sigma, eps, rc)
+    assert abs(e.sum()) < 1e-9
+    # F = -dE/dr by central differences
+    for r in (0.95, 1.1, 1.7, 2.3):
+        h = 1e-6
+        ep = two_atom_pair(orc, np.array([r + h, 0, 0]), sigma, eps, rc)[1].sum()
+        em = two_atom_pair(orc, np.array([r - h, 0, 0]), sigma, eps, rc)[1].sum()
+        f = two_atom_pair(orc, np.array([r, 0, 0]), sigma, eps, rc)[0][0, 0]
+        assert f == pytest.approx(-(ep - em) / (2 * h), rel=1e-6)
+
+
+def test_lj_special_multiplier(orc):
+    dr = np.array([1.2, 0.0, 0.0])
+    f1, e1, _ = two_atom_pair(orc, dr, 1.0, 1.0, 2.5)
+    f3, e3, _ = two_atom_pair(orc, dr, 1.0, 1.0, 2.5, tag=3)   # 1-4 -> 0.5
+    f2, e2, _ = two_atom_pair(orc, dr, 1.0, 1.0, 2.5, tag=1)   # 1-2 -> 0
+    assert np.allclose(f3, 0.5 * f1, rtol=1e-14)
+    assert np.allclose(e3, 0.5 * e1, rtol=1e-14)
+    assert np.abs(f2).max() == 0.0 and np.abs(e2).max() == 0.0
+
+
+# --- Ewald real space (ChargeEvaluatorEwald.h:11-63) ---
+@pytest.mark.parametrize("tag,m", [(0, 1.0), (1, 0.0), (3, 0.5)])
+def test_ewald_real_closed_form(orc, tag, m):
+    alpha, qqr, q = 0.31, 332.06371, (0.5564, -1.1128)
+    for r in (0.3, 0.9572, 2.7, 8.9):
+        dr = np.array([r / np.sqrt(3)] * 3)
+        f, e, _ = two_atom_pair(orc, dr, q=q, charge=True, alpha=alpha, qqr=qqr, rq=9.0, tag=tag)
+        qq = qqr * q[0] * q[1]
+        fs = qq * (erfc(alpha * r) / r ** 2 + 2 * alpha / np.sqrt(np.pi) * np.exp(-(alpha * r) ** 2) / r
+                   - (1 - m) / r ** 2)
+        assert np.allclose(f[0], fs * dr / r, rtol=1e-12, atol=1e-12)
+        assert e.sum() == pytest.approx(qq * (erfc(alpha * r) - (1 - m)) / r, rel=1e-12)
+        if tag == 1:  # full exclusion: -qq erf(alpha r)/r
+            assert e.sum() == pytest.approx(-qq * erf(alpha * r) / r, rel=1e-10)
+
+
+def test_ewald_real_fd(orc):
+    alpha, qqr, q = 0.28, 332.06371, (1.0, -1.0)
+    for r in (1.5, 4.0):
+        h = 1e-6
+        ep = two_atom_pair(orc, np.array([r + h, 0, 0]), q=q, charge=True, alpha=alpha, qqr=qqr, rq=9)[1].sum()
+        em = two_atom_pair(orc, np.array([r - h, 0, 0]), q=q, charge=True, alpha=alpha, qqr=qqr, rq=9)[1].sum()
+        f = two_atom_pair(orc, np.array([r, 0, 0]), q=q, charge=True, alpha=alpha, qqr=qqr, rq=9)[0][0, 0]
+        assert f == pytest.approx(-(ep - em) / (2 * h), rel=1e-6)
+
+
+# --- geometry and neighbor lists ---
+def test_wrap_and_min_image(orc):
+    L = np.array([10.0, 12.0, 7.0])
+    x = np.array([[-0.5, 12.5, 14.2], [9.999, 0.0, -7.0]])
+    w = orc.wrap(x, np.zeros(3), L)
+    assert np.all(w >= 0) and np.all(w < L)
+    assert np.allclose((w - x) / L, np.round((w - x) / L), atol=1e-12)
+
+
+def _perturbed_fcc(n_cells, density, amp, seed):
+    x, L, lo = systems.fcc_lj(n_cells, density)
+    rng = np.random.default_rng(seed)
+    return x + rng.uniform(-amp, amp, x.shape), L, lo
+
+
+@pytest.mark.parametrize("ncell,rlist", [(4, 2.8), (6, 2.8), (5, 1.6)])
+def test_nlist_cell_equals_brute(orc, ncell, rlist):
+    x, L, lo = _perturbed_fcc(ncell, 0.8442, 0.3, 7)
+    x = orc.wrap(x, lo, L)
+    cb, rb = orc.nlist_brute(x, L, rlist)
+    cc, tab = orc.nlist_cell(x, lo, L, rlist)
+    assert np.array_equal(cb, cc)
+    for i in range(len(x)):
+        assert np.array_equal(rb[i], tab[i, : cc[i]])
+    # counts from an independent numpy distance matrix (min image)
+    d = x[:, None, :] - x[None, :, :]
+    d -= L * np.rint(d / L)
+    r2 = (d ** 2).sum(-1)
+    np.fill_diagonal(r2, np.inf)
+    cnt = (r2 < rlist * rlist).sum(1)
+    # float64 rounding of the two formulas can only differ at |r - rlist| ~ 1e-15
+    near = np.abs(np.sqrt(r2) - rlist) < 1e-9
+    assert np.all((cnt == cb) | near.any(1))
+
+
+def test_exclusions_water_closed_form(orc):
+    # O-H1, O-H2, H1-H2, O-M bonds (FixRigid.cu:1664-1677): every intramolecular
+    # pair is 1-2 except H-M, which is 1-3
+    mol = np.array([[0, 1, 2, 3], [4, 5, 6, 7]])
+    ex = orc.exclusions_bfs(8, systems.water_bonds(mol))
+    for o, h1, h2, m in mol:
+        assert ex[o] == {h1: 1, h2: 1, m: 1}
+        assert ex[h1] == {o: 1, h2: 1, m: 2}
+        assert ex[h2] == {o: 1, h1: 1, m: 2}
+        assert ex[m] == {o: 1, h1: 2, h2: 2}
+
+
+def test_exclusions_chain(orc):
+    ex = orc.exclusions_bfs(6, [(0, 1), (1, 2), (2, 3), (3, 4), (4, 5)])
+    assert ex[0] == {1: 1, 2: 2, 3: 3}
+    assert ex[2] == {1: 1, 3: 1, 0: 2, 4: 2, 5: 3}
+    # a ring: the closer depth wins
+    ex = orc.exclusions_bfs(4, [(0, 1), (1, 2), (2, 3), (3, 0)])
+    assert ex[0] == {1: 1, 3: 1, 2: 2}
+
+
+def test_pair_newton_third_law(orc):
+    x, L, lo = _perturbed_fcc(4, 0.8442, 0.2, 3)
+    x = orc.wrap(x, lo, L)
+    counts, rows = orc.nlist_brute(x, L, 2.8)
+    rc2, e24, s6 = lj_tables(1.0, 1.0, 2.5)
+    f, e, v = orc.pair(x, np.zeros(len(x)), np.zeros(len(x), np.int32), L, counts, rows, rc2, e24, s6)
+    assert np.abs(f.sum(0)).max() < 1e-9 * np.abs(f).max() * len(x)
+    # virial tensor = sum over unordered pairs of F_ij (x) dr_ij, from a numpy
+    # distance matrix (independent of the neighbor list)
+    d = x[:, None, :] - x[None, :, :]
+    d -= L * np.rint(d / L)
+    r2 = (d ** 2).sum(-1)
+    iu = np.triu_indices(len(x), 1)
+    dr = d[iu]
+    rr = r2[iu]
+    m = rr < 2.5 ** 2
+    dr, rr = dr[m], rr[m]
+    fs = 24.0 * (2.0 / rr ** 7 - 1.0 / rr ** 4)
+    F = dr * fs[:, None]
+    W = np.array([(F[:, 0] * dr[:, 0]).sum(), (F[:, 1] * dr[:, 1]).sum(), (F[:, 2] * dr[:, 2]).sum(),
+                  (F[:, 0] * dr[:, 1]).sum(), (F[:, 0] * dr[:, 2]).sum(), (F[:, 1] * dr[:, 2]).sum()])
+    assert np.allclose(v, W, rtol=1e-10, atol=1e-9)
+
+
+# --- P3M ---
+@pytest.mark.parametrize("order", [3, 5, 7])
+def test_weights_partition_of_unity(orc, order):
+    h = (order - 1) // 2
+    for d in np.linspace(-0.5, 0.5, 11):
+        s = sum(orc.wp(order, i, d) for i in range(-h, h + 1))
+        assert s == pytest.approx(1.0, abs=1e-14)
+        # first moment: the particle sits at ngp + d, so sum i W(i, d) = d
+        m1 = sum(i * orc.wp(order, i, d) for i in range(-h, h + 1))
+        assert m1 == pytest.approx(d, abs=1e-14)
+
+
+def test_fft_matches_numpy(orc):
+    rng = np.random.default_rng(1)
+    c = rng.normal(size=(8, 16, 4)) + 1j * rng.normal(size=(8, 16, 4))
+    assert np.allclose(orc.fft3(c, -1), np.fft.fftn(c), atol=1e-12)
+    assert np.allclose(orc.fft3(c, +1), np.fft.ifftn(c) * c.size, atol=1e-12)
+
+
+def test_alpha_balances_errors(orc):
+    L = np.array([30.0, 30.0, 30.0])
+    sz = np.array([32, 32, 32])
+    q2 = 332.06371 * 1000 * 0.5
+    a, err = orc.pppm_alpha(2000, q2, L, sz, 5, 9.0)
+    dk = orc.deltaf_k(a, 2000, q2, L, sz, 5)
+    dr = orc.deltaf_real(a, 2000, q2, L, 9.0)
+    assert abs(dk - dr) / dr <= 1e-5
+    assert err == pytest.approx(dk + dr, rel=1e-12)
+
+
+def _random_neutral(n, L, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0, L, size=(n, 3))
+    q = rng.choice([-1.0, 1.0], size=n)
+    q[-1] = -q[:-1].sum() if abs(q[:-1].sum()) <= 1 else q[-1]
+    q -= q.mean()
+    return x, q
+
+
+def test_spread_conserves_charge(orc):
+    x, q = _random_neutral(50, 12.0, 3)
+    q = q + 0.3  # non-neutral on purpose
+    L = np.array([12.0, 12.0, 12.0])
+    sz = np.array([32, 32, 32])
+    G = np.zeros((32, 32, 32))
+    out = orc.pppm(x, q, np.zeros(3), L, sz, 5, 4.0, 0.5, 0.0, G, forces=False, rho=True)
+    assert out["rho"].sum() == pytest.approx(2.0 * q.sum(), rel=1e-12)
+
+
+def test_ewald_direct_madelung(orc):
+    # rock salt, 2x2x2 conventional cells, nearest-neighbour distance 1
+    a = 2.0
+    base = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]]) * a
+    x, q = [], []
+    for i in range(2):
+        for j in range(2):
+            for k in range(2):
+                off = np.array([i, j, k]) * a
+                for b in base:
+                    x.append(off + b)
+                    q.append(1.0)
+                    x.append(off + b + np.array([0.5, 0, 0]) * a)
+                    q.append(-1.0)
+    x, q = np.array(x), np.array(q)
+    L = np.full(3, 2 * a)
+    e, f = orc.ewald_direct(x, q, L, alpha=1.2, qqr=1.0, rcut_real=3.9, kmax=10)
+    assert e / (len(x) / 2) == pytest.approx(-1.7475645946, rel=1e-8)
+    assert np.abs(f).max() < 1e-8
+
+
+def test_pppm_matches_ewald(orc):
+    L = np.array([14.0, 14.0, 14.0])
+    x, q = _random_neutral(60, 14.0, 11)
+    # spread out overlapping charges
+    alpha, rc, qqr = 0.55, 6.5, 1.0
+    sz = np.array([64, 64, 64])
+    order = 7
+    G = orc.pppm_green(L, sz, alpha, order)
+    q2 = qqr * (q * q).sum()
+    kres = orc.pppm(x, q, np.zeros(3), L, sz, order, qqr, alpha, q2, G, energy=True)
+    xw = orc.wrap(x, np.zeros(3), L)
+    counts, rows = orc.nlist_brute(xw, L, rc)
+    z = np.zeros((1, 1))
+    fr, er, _ = orc.pair(xw, q, np.zeros(len(x), np.int32), L, counts, rows, z, z, z, with_lj=False,
+                         with_charge=True, alpha=alpha, qqr=qqr, rq=rc)
+    e_ref, f_ref = orc.ewald_direct(x, q, L, alpha, qqr, rcut_real=rc, kmax=14)
+    f = fr + kres["f"]
+    rel = np.sqrt(((f - f_ref) ** 2).sum() / (f_ref ** 2).sum())
+    assert rel < 2e-4
+    assert er.sum() + kres["energy"] == pytest.approx(e_ref, rel=1e-5)
+
+
+def test_pppm_green_nyquist_symmetric(orc):
+    L = np.array([20.0, 22.0, 24.0])
+    sz = np.array([32, 32, 32])
+    G = orc.pppm_green(L, sz, 0.3, 5)
+    assert G[0, 0, 0] == 0.0
+    # G(k) = G(-k) away from the Nyquist planes
+    Gm = G[(-np.arange(32)) % 32][:, (-np.arange(32)) % 32][:, :, (-np.arange(32)) % 32]
+    inner = np.ix_(range(1, 16), range(1, 16), range(1, 16))
+    assert np.allclose(G[inner], Gm[inner], rtol=1e-12)
+
+
+# --- E3B ---
+def _water_cluster(nmol, L, seed, spacing=2.9):
+    box = systems.tip4p2005_box(nmol, L=L, seed=seed)
+    return box
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_e3b_decomposition_equals_bond_pair_enumeration(orc, seed):
+    box = systems.tip4p2005_box(27, L=9.3, seed=seed, min_sep=1.2)
+    p = orc.e3b_params()
+    f, e2, e3 = orc.e3b(box["mol"], box["x"], box["L"], p)
+    eb = orc.e3b_brute_energy(box["mol"], box["x"], box["L"], p)
+    assert abs(eb) > 1e-6          # the configuration has hydrogen bonds
+    assert e3 == pytest.approx(eb, rel=1e-11, abs=1e-12)
+
+
+def test_e3b_forces_finite_difference(orc):
+    box = systems.tip4p2005_box(27, L=9.3, seed=5, min_sep=1.2)
+    p = orc.e3b_params()
+    x = box["x"]
+    f, _, _ = orc.e3b(box["mol"], x, box["L"], p)
+    rng = np.random.default_rng(0)
+    h = 1e-5
+    for atom in rng.choice(len(x), 12, replace=False):
+        for a in range(3):
+            xp = x.copy()
+            xm = x.copy()
+            xp[atom, a] += h
+            xm[atom, a] -= h
+            _, e2p, e3p = orc.e3b(box["mol"], xp, box["L"], p)
+            _, e2m, e3m = orc.e3b(box["mol"], xm, box["L"], p)
+            fd = -((e2p + e3p) - (e2m + e3m)) / (2 * h)
+            assert f[atom, a] == pytest.approx(fd, rel=1e-5, abs=1e-6)
+    assert np.abs(f.sum(0)).max() < 1e-8
+
+
+def test_e3b_three_body_types_by_hand(orc):
+    """A three-molecule chain where each bond pair type is known: energies of
+    A/B/C are checked by setting the other prefactors to zero."""
+    box = systems.tip4p2005_box(8, L=7.0, seed=9, min_sep=1.1)
+    base = orc.e3b_params()
+    tot = orc.e3b_brute_energy(box["mol"], box["x"], box["L"], base)
+    parts = []
+    for keep in ("Ea", "Eb", "Ec"):
+        p = orc.e3b_params()
+        for k in ("Ea", "Eb", "Ec"):
+            if k != keep:
+                setattr(p, k, 0.0)
+        _, _, e3 = orc.e3b(box["mol"], box["x"], box["L"], p)
+        parts.append(e3)
+        assert e3 == pytest.approx(orc.e3b_brute_energy(box["mol"], box["x"], box["L"], p), rel=1e-11, abs=1e-12)
+    assert sum(parts) == pytest.approx(tot, rel=1e-11, abs=1e-12)
+
+
+# --- integration ---
+def test_lj_verlet_energy_conservation(orc):
+    x, L, lo = _perturbed_fcc(4, 0.8442, 0.05, 1)
+    v = systems.maxwell_boltzmann(np.ones(len(x)), 1.2, seed=4)
+    _, _, nb, et = orc.lj_verlet(x, v, 1.0, lo, L, 1.0, 1.0, 2.5, 0.3, 0.005, 400, record_energy=True)
+    assert nb >= 2
+    # shifted-energy LJ is not force-shifted: small jumps as pairs cross rc
+    drift = (et.max() - et.min()) / len(x)
+    assert drift < 2e-3
