@@ -168,7 +168,8 @@ def test_forces_deterministic(gpu_engine_factory):
     b.compute(PAIR)
     f3 = b.forces()[::-1]
     assert np.array_equal(f1, f2)
-    assert np.abs(f1 - f3).max() < 1e-4
+    # a different id order changes the FP32 summation order only
+    assert rel_rmse(f3, f1) < 1e-6
 
 
 # ---------------------------------------------------------------------------
