@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""Benchmark of the per-step force path on one MI355X.
+
+Default workload: TIP4P/2005 + E3B3 water, 8,192 molecules (32,768 sites
+counting M), cubic box at 0.997 g/cm^3 (L = 62.64 A), LJ O-O + Ewald real
+space at rc = 9 A with a 1 A skin, P3M 64^3 order 5, E3B3 (5.0-5.2 A switch,
+two-body O-O correction).  One step = the force path of one MD step: skin
+check, neighbor-list rebuild every --rebuild-every steps, LJ + Ewald real
+space, P3M spread/FFT/field/FFT/gather, E3B two passes.  Positions are the
+synthetic lattice start (seeded); inputs are resident in HBM before timing.
+
+Prints one JSON line: atom-steps/s (sites x steps / s), ms per step, the
+roofline of the dominant kernel (HIP events on the library stream during the
+timed region) and an FP64 CPU baseline of the same formulas (the oracle,
+OpenMP on the host cores) on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+E3B3_KJ = (453000.0, 150.0, -1005.0, 1880.0)
+KJ_TO_KCAL = 0.23900573614
+
+
+def e3b3_params():
+    E2, Ea, Eb, Ec = (v * KJ_TO_KCAL for v in E3B3_KJ)
+    return np.array([E2, Ea, Eb, Ec, 4.872, 1.907, 5.0, 5.2])
+
+
+def make_workload(name):
+    from md_engine_amd import systems
+
+    if name == "tip4p-e3b":
+        box = systems.tip4p2005_box(8192, seed=20261202)
+        return dict(box=box, rc=9.0, skin=1.0, sz=(64, 64, 64), order=5, e3b=True, dt_fs=2.0,
+                    desc="TIP4P/2005+E3B3 water, 8192 molecules (32768 sites), L=%.3f A" % box["L"][0])
+    if name == "tip4p":
+        box = systems.tip4p2005_box(4096, seed=20261202)
+        return dict(box=box, rc=9.0, skin=1.0, sz=(64, 64, 64), order=5, e3b=False, dt_fs=2.0,
+                    desc="TIP4P/2005 water, 4096 molecules (16384 sites), L=%.3f A" % box["L"][0])
+    if name == "lj":
+        x, L, lo = systems.fcc_lj(20, 0.8442)
+        box = dict(x=x, L=L, lo=lo, q=np.zeros(len(x)), type=np.zeros(len(x), np.int32),
+                   mass=np.ones(len(x)), mol=None)
+        return dict(box=box, rc=2.5, skin=0.3, sz=None, order=None, e3b=False, dt_fs=None,
+                    desc="LJ fluid FCC 32000 atoms, rho*=0.8442")
+    raise SystemExit(f"unknown config {name}")
+
+
+def setup_engine(w):
+    from md_engine_amd import E3B, PAIR, PPPM, Engine, systems
+
+    box = w["box"]
+    e = Engine(0)
+    mass = box["mass"]
+    inv = np.where(mass > 0, 1.0 / np.maximum(mass, 1e-30), 0.0)
+    e.set_system(box["x"], box["lo"], box["L"], q=box["q"], types=box["type"], inv_mass=inv)
+    flags = PAIR
+    if box["mol"] is not None:
+        e.exclusions_from_bonds(systems.water_bonds(box["mol"]))
+        eps24, sig6 = systems.water_lj_tables()
+        e.lj_set(np.full((3, 3), w["rc"] ** 2), eps24, sig6)
+        e.pppm_set(w["sz"], w["order"], w["rc"], systems.REAL_QQR_TO_ENG)
+        e.ewald_real_set(True, -1.0, systems.REAL_QQR_TO_ENG, w["rc"])
+        flags |= PPPM
+        if w["e3b"]:
+            e.e3b_set(box["mol"], e3b3_params())
+            flags |= E3B
+    else:
+        e.lj_set([[w["rc"] ** 2]], [[24.0]], [[1.0]])
+    e.nlist_config(w["rc"] + w["skin"], w["skin"])
+    return e, flags
+
+
+def cpu_baseline(w, rebuild_every, budget_s=12.0):
+    """FP64 oracle of the same step on the host cores (bounded sample)."""
+    import oracle
+    from md_engine_amd import systems
+
+    box = w["box"]
+    n = len(box["x"])
+    L, lo, rc, skin = box["L"], box["lo"], w["rc"], w["skin"]
+    x = oracle.wrap(box["x"], lo, L)
+    t0 = time.perf_counter()
+    counts, tab = oracle.nlist_cell(x, lo, L, rc + skin)
+    t_build = time.perf_counter() - t0
+    rows = [tab[i, : counts[i]] for i in range(n)]
+    if box["mol"] is not None:
+        excl = oracle.exclusions_bfs(n, systems.water_bonds(box["mol"]))
+        rows = oracle.tag_rows(rows, np.arange(n), excl)
+        eps24, sig6 = systems.water_lj_tables()
+        rc2 = np.full((3, 3), rc * rc)
+        qqr = systems.REAL_QQR_TO_ENG
+        q2 = qqr * (box["q"] ** 2).sum()
+        alpha, _ = oracle.pppm_alpha(n, q2, L, np.array(w["sz"]), w["order"], rc)
+        G = oracle.pppm_green(L, np.array(w["sz"]), alpha, w["order"])
+    else:
+        rows = [r.astype(np.uint32) for r in rows]
+        eps24, sig6, rc2 = np.array([[24.0]]), np.array([[1.0]]), np.array([[rc * rc]])
+    rows = oracle.rows_to_table(counts, rows)
+
+    def step():
+        if box["mol"] is not None:
+            oracle.pair(x, box["q"], box["type"], L, counts, rows, rc2, eps24, sig6, with_charge=True,
+                        alpha=alpha, qqr=qqr, rq=rc)
+            oracle.pppm(x, box["q"], lo, L, np.array(w["sz"]), w["order"], qqr, alpha, q2, G)
+            if w["e3b"]:
+                oracle.e3b(box["mol"], x, L)
+        else:
+            oracle.pair(x, box["q"], box["type"], L, counts, rows, rc2, eps24, sig6)
+
+    step()  # warm
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        step()
+        steps += 1
+        if time.perf_counter() - t0 > budget_s or steps >= 50:
+            break
+    t_step = (time.perf_counter() - t0) / steps
+    per_step = t_step + (t_build / rebuild_every if rebuild_every else 0.0)
+    return dict(value=n / per_step, unit="atom-steps/s", cores=oracle.num_threads(), kind="port",
+                sample=f"{steps} force-path steps of the same system + 1 list build amortised over "
+                       f"{rebuild_every} steps; FP64 oracle (C, OpenMP); {t_step * 1e3:.1f} ms/step, "
+                       f"build {t_build * 1e3:.1f} ms")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--config", default="tip4p-e3b", choices=["tip4p-e3b", "tip4p", "lj"])
+    ap.add_argument("--rebuild-every", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pair-team", type=int, default=0)
+    ap.add_argument("--build-team", type=int, default=0)
+    ap.add_argument("--e3b-team", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (asked for --gpus {args.gpus}); this build has no "
+              "multi-GPU path", file=sys.stderr)
+        sys.exit(2)
+
+    w = make_workload(args.config)
+    e, flags = setup_engine(w)
+    e.teams_set(args.build_team, args.pair_team, args.e3b_team)
+    n = len(w["box"]["x"])
+    R = args.rebuild_every
+
+    e.force_steps(args.warmup, flags, R)
+    e.sync()
+    e.profile(True)
+    t0 = time.perf_counter()
+    e.force_steps(args.steps, flags, R)
+    e.sync()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    prof = {k: e.profile_get(k) for k in ("pair", "build", "spread", "gather", "fft", "field", "e3b1", "e3b2")}
+    e.profile(False)
+    pairs, mols = e.nlist_totals()
+    st = e.nlist_stats()
+
+    ms_step = wall / args.steps * 1e3
+    value = n * args.steps / wall
+    # dominant kernel: pair (LJ + Ewald real space).  Algorithmic bytes per
+    # launch: per atom xq 16 B + type 4 B + row count 4 B + force 24 B, plus
+    # 4 B per neighbor-list entry.
+    cnt, tot_ms = prof["pair"]
+    avg_s = tot_ms / max(cnt, 1) * 1e-3
+    bytes_launch = n * (16 + 4 + 4 + 24) + 4 * pairs
+    achieved = bytes_launch / avg_s / 1e9
+    kernel_ms = {k: (v[1] / max(v[0], 1)) for k, v in prof.items() if v[0]}
+
+    out = {
+        "metric": "atom-steps/s",
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair/3-body arithmetic, f64 positions, i64 fixed-point accumulation",
+        "data": "synthetic (seeded lattice, random orientations)",
+        "config": {
+            "workload": w["desc"],
+            "step": "force path: skin check, list rebuild every %d steps, LJ+Ewald real, P3M, E3B" % R,
+            "rc": w["rc"], "skin": w["skin"], "pppm_mesh": w["sz"], "pppm_order": w["order"],
+            "e3b": w["e3b"], "sites": n, "pair_list_entries": pairs, "e3b_list_entries": mols,
+            "list_rebuilds_in_run": st["rebuilds"],
+            "ns_per_day_at_dt": (w["dt_fs"] * 1e-6 * args.steps / wall * 86400.0) if w["dt_fs"] else None,
+            "kernel_avg_ms": kernel_ms,
+        },
+        "roofline": {
+            "kernel": "k_pair (LJ + Ewald real space)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_launch": bytes_launch,
+            "avg_launch_ms": avg_s * 1e3,
+            "launches": cnt,
+        },
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, R)
+    e.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -125,6 +125,22 @@ int mdx_run(mdx_ctx *ctx, int nsteps, int flags);
 int mdx_teams_set(mdx_ctx *ctx, int build_team, int pair_team, int e3b_team);
 int mdx_sync(mdx_ctx *ctx);
 
+/* Force-path steps at fixed positions: nsteps evaluations of the terms in
+ * `flags`, each preceded by the skin check, with a forced neighbor-list
+ * rebuild every rebuild_every steps (0: only when the skin criterion fires).
+ * Measures the force path in isolation, the part of IntegratorVerlet::run
+ * between the half-kicks (IntegratorVerlet.cu:696-729). */
+int mdx_force_steps(mdx_ctx *ctx, int nsteps, int flags, int rebuild_every);
+/* Kernel timing with HIP events on the context stream (no counterpart in the
+ * reference, whose only timer is the wall clock of IntegratorVerlet.cu:687,
+ * 759-767).  Enabling clears earlier records.  Kernel ids: 0 list build,
+ * 1 pair, 2 mesh spread, 3 mesh gather, 4 E3B pass 1, 5 E3B pass 2,
+ * 6 FFTs, 7 field. */
+int mdx_profile_enable(mdx_ctx *ctx, int enable);
+int mdx_profile_get(mdx_ctx *ctx, int kernel, int *count, double *total_ms);
+/* Entries in the current pair list and E3B molecule list (sum of counts). */
+int mdx_nlist_totals(mdx_ctx *ctx, int64_t *pair_entries, int64_t *mol_entries);
+
 #ifdef __cplusplus
 }
 #endif

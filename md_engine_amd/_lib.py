@@ -23,8 +23,13 @@ SYMBOLS = [
     "mdx_exclusions_set", "mdx_exclusions_from_bonds", "mdx_nlist_update", "mdx_nlist_get",
     "mdx_nlist_stats", "mdx_lj_set", "mdx_special_set", "mdx_ewald_real_set", "mdx_pppm_set",
     "mdx_pppm_info", "mdx_e3b_set", "mdx_forces_compute", "mdx_forces_get", "mdx_energy_get",
-    "mdx_integrator_set", "mdx_run", "mdx_teams_set", "mdx_sync",
+    "mdx_integrator_set", "mdx_run", "mdx_teams_set", "mdx_sync", "mdx_force_steps",
+    "mdx_profile_enable", "mdx_profile_get", "mdx_nlist_totals",
 ]
+
+# kernel ids of mdx_profile_get
+PROF_KERNELS = {"build": 0, "pair": 1, "spread": 2, "gather": 3, "e3b1": 4, "e3b2": 5, "fft": 6,
+                "field": 7}
 
 
 class MdxError(RuntimeError):
@@ -80,6 +85,10 @@ def load():
     L.mdx_run.argtypes = [_vp, c, c]
     L.mdx_teams_set.argtypes = [_vp, c, c, c]
     L.mdx_sync.argtypes = [_vp]
+    L.mdx_force_steps.argtypes = [_vp, c, c, c]
+    L.mdx_profile_enable.argtypes = [_vp, c]
+    L.mdx_profile_get.argtypes = [_vp, c, C.POINTER(c), C.POINTER(d)]
+    L.mdx_nlist_totals.argtypes = [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     for name in SYMBOLS:
         fn = getattr(L, name)
         if name not in ("mdx_last_error", "mdx_version"):
@@ -245,3 +254,23 @@ class Engine:
 
     def sync(self):
         self._chk(self._L.mdx_sync(self._h), "mdx_sync")
+
+    def force_steps(self, nsteps, flags, rebuild_every=0):
+        self._chk(self._L.mdx_force_steps(self._h, int(nsteps), int(flags), int(rebuild_every)),
+                  "mdx_force_steps")
+
+    def profile(self, enable=True):
+        self._chk(self._L.mdx_profile_enable(self._h, int(bool(enable))), "mdx_profile_enable")
+
+    def profile_get(self, kernel):
+        n = C.c_int(0)
+        t = C.c_double(0.0)
+        k = PROF_KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
+        self._chk(self._L.mdx_profile_get(self._h, k, C.byref(n), C.byref(t)), "mdx_profile_get")
+        return n.value, t.value
+
+    def nlist_totals(self):
+        a = C.c_int64(0)
+        b = C.c_int64(0)
+        self._chk(self._L.mdx_nlist_totals(self._h, C.byref(a), C.byref(b)), "mdx_nlist_totals")
+        return a.value, b.value

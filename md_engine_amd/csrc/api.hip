@@ -24,6 +24,38 @@ void dfree(Ctx &c, void *p) {
     }
 }
 
+static hipEvent_t pool_event(Ctx &c) {
+    if (!c.eventPool.empty()) {
+        hipEvent_t e = c.eventPool.back();
+        c.eventPool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    MDX_HIP(hipEventCreate(&e));
+    return e;
+}
+
+ProfScope::ProfScope(Ctx &c_, int k_) : c(c_), k(k_) {
+    if (!c.prof) return;
+    a = pool_event(c);
+    b = pool_event(c);
+    hipEventRecord(a, c.stream);
+}
+
+ProfScope::~ProfScope() {
+    if (!a) return;
+    hipEventRecord(b, c.stream);
+    c.profRecs.push_back({k, a, b});
+}
+
+static void prof_clear(Ctx &c) {
+    for (auto &r : c.profRecs) {
+        c.eventPool.push_back(r.a);
+        c.eventPool.push_back(r.b);
+    }
+    c.profRecs.clear();
+}
+
 namespace {
 
 template <class F>
@@ -126,6 +158,9 @@ int mdx_destroy(mdx_ctx *ctx) {
     hipSetDevice(c.device);
     if (c.stream) hipStreamSynchronize(c.stream);
     pppm_release(c);
+    prof_clear(c);
+    for (hipEvent_t e : c.eventPool) hipEventDestroy(e);
+    c.eventPool.clear();
     for (void *p : c.allocs) hipFree(p);
     c.allocs.clear();
     if (c.stream) hipStreamDestroy(c.stream);
@@ -591,9 +626,76 @@ int mdx_teams_set(mdx_ctx *ctx, int build_team, int pair_team, int e3b_team) {
         auto ok = [](int v, int lo, int hi) { return v == 0 || (v >= lo && v <= hi && (v & (v - 1)) == 0); };
         MDX_REQUIRE(ok(build_team, 8, 64) && ok(pair_team, 4, 32) && ok(e3b_team, 4, 16), MDX_ERR_ARG,
                     "team sizes: build 8..64, pair 4..32, e3b 4..16, powers of two");
-        if (build_team) c.buildTeam = build_team;
+        if (build_team) {
+            c.buildTeam = build_team;
+            c.buildTeamUser = true;
+        }
         if (pair_team) c.pairTeam = pair_team;
         if (e3b_team) c.e3b.team = e3b_team;
+    });
+}
+
+int mdx_force_steps(mdx_ctx *ctx, int nsteps, int flags, int rebuild_every) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        MDX_REQUIRE(c.rlist > 0, MDX_ERR_STATE, "mdx_nlist_config not called");
+        MDX_REQUIRE(nsteps >= 0 && rebuild_every >= 0, MDX_ERR_ARG, "counts must be >= 0");
+        prepare_terms(c);
+        for (int s = 0; s < nsteps; ++s) {
+            if (!c.nlistConfigured) nlist_setup(c);
+            if (!c.listValid || (rebuild_every > 0 && s % rebuild_every == 0)) {
+                nlist_rebuild(c, true);
+            } else {
+                nlist_check(c);
+                nlist_rebuild(c, false);
+            }
+            compute(c, flags & (MDX_PAIR | MDX_PPPM | MDX_E3B));
+        }
+        MDX_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int mdx_profile_enable(mdx_ctx *ctx, int enable) {
+    return guarded(ctx, [&](Ctx &c) {
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        prof_clear(c);
+        c.prof = enable != 0;
+    });
+}
+
+int mdx_profile_get(mdx_ctx *ctx, int kernel, int *count, double *total_ms) {
+    return guarded(ctx, [&](Ctx &c) {
+        MDX_REQUIRE(kernel >= 0 && kernel < PK_N, MDX_ERR_ARG, "unknown kernel id");
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        int n = 0;
+        double t = 0.0;
+        for (auto &r : c.profRecs) {
+            if (r.kernel != kernel) continue;
+            float ms = 0.f;
+            MDX_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+            t += ms;
+            ++n;
+        }
+        if (count) *count = n;
+        if (total_ms) *total_ms = t;
+    });
+}
+
+int mdx_nlist_totals(mdx_ctx *ctx, int64_t *pair_entries, int64_t *mol_entries) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        MDX_REQUIRE(c.listValid, MDX_ERR_STATE, "no neighbor list built");
+        std::vector<int> cnt(c.n);
+        MDX_HIP(hipMemcpyAsync(cnt.data(), c.nlCount, sizeof(int) * c.n, hipMemcpyDeviceToHost, c.stream));
+        std::vector<int> mc(c.e3b.on ? c.e3b.nmol : 0);
+        if (c.e3b.on)
+            MDX_HIP(hipMemcpyAsync(mc.data(), c.e3b.nlCount, sizeof(int) * mc.size(), hipMemcpyDeviceToHost, c.stream));
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        int64_t s = 0, m = 0;
+        for (int v : cnt) s += std::min(v, c.cap);
+        for (int v : mc) m += std::min(v, c.e3b.cap);
+        if (pair_entries) *pair_entries = s;
+        if (mol_entries) *mol_entries = m;
     });
 }
 

@@ -253,16 +253,22 @@ __global__ void __launch_bounds__(256) k_e3b_pass2(E3BArgs p) {
 }
 
 template <int P>
-void launch(const E3BArgs &a, bool ev, hipStream_t s) {
+void launch(Ctx &c, const E3BArgs &a, bool ev) {
     const int threads = 256;
     const int blocks = cdiv((long long)a.nmol * P, threads);
-    if (ev) {
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass1<P, true>), dim3(blocks), dim3(threads), 0, s, a);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass2<P, true>), dim3(blocks), dim3(threads), 0, s, a);
-    } else {
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass1<P, false>), dim3(blocks), dim3(threads), 0, s, a);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass2<P, false>), dim3(blocks), dim3(threads), 0, s, a);
+    hipStream_t s = c.stream;
+    {
+        ProfScope ps(c, PK_E3B1);
+        if (ev)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass1<P, true>), dim3(blocks), dim3(threads), 0, s, a);
+        else
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass1<P, false>), dim3(blocks), dim3(threads), 0, s, a);
     }
+    ProfScope ps(c, PK_E3B2);
+    if (ev)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass2<P, true>), dim3(blocks), dim3(threads), 0, s, a);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass2<P, false>), dim3(blocks), dim3(threads), 0, s, a);
 }
 
 }  // namespace
@@ -296,9 +302,9 @@ void e3b_forces(Ctx &c, bool energy, bool virial) {
     a.acc = c.acc;
     const bool ev = energy || virial;
     switch (E.team) {
-        case 4: launch<4>(a, ev, c.stream); break;
-        case 8: launch<8>(a, ev, c.stream); break;
-        default: launch<16>(a, ev, c.stream); break;
+        case 4: launch<4>(c, a, ev); break;
+        case 8: launch<8>(c, a, ev); break;
+        default: launch<16>(c, a, ev); break;
     }
 }
 

@@ -83,6 +83,14 @@ struct E3BState {
     int team = 8;
 };
 
+// kernels timed with HIP events when profiling is on (mdx_profile_enable)
+enum ProfKernel { PK_BUILD = 0, PK_PAIR, PK_SPREAD, PK_GATHER, PK_E3B1, PK_E3B2, PK_FFT, PK_EFIELD, PK_N };
+
+struct ProfRec {
+    int kernel;
+    hipEvent_t a, b;
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -123,6 +131,7 @@ struct Ctx {
     int *nlCount = nullptr;
     int *dFlags = nullptr;        // [0] rebuild flag, [1] max count, [2] e3b max count
     int buildTeam = 32;
+    bool buildTeamUser = false;   // set by mdx_teams_set, else chosen from the cell population
     long long nRebuilds = 0;
     bool listValid = false;
 
@@ -150,8 +159,22 @@ struct Ctx {
     long long turn = 0;
     int forceFlags = 0;
 
+    // profiling: event pairs recorded on the context stream around kernels
+    bool prof = false;
+    std::vector<ProfRec> profRecs;
+    std::vector<hipEvent_t> eventPool;
+
     // all device allocations, freed in mdx_destroy
     std::vector<void *> allocs;
+};
+
+// Records a HIP event pair around the launches in its scope when profiling is on.
+struct ProfScope {
+    Ctx &c;
+    int k;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(Ctx &c_, int k_);
+    ~ProfScope();
 };
 
 // error helpers -------------------------------------------------------------

@@ -463,7 +463,7 @@ void nlist_setup(Ctx &c) {
                  std::min(c.nc[2], 2 * c.reach + 1);
     int T = 8;
     while (T < 64 && T < run) T *= 2;
-    c.buildTeam = T;
+    if (!c.buildTeamUser) c.buildTeam = T;
     if (c.cap == 0) {
         double expect = dens * 4.18879020479 * c.rlist * c.rlist * c.rlist;
         c.cap = ((int)(expect * 1.3) + 32 + 15) & ~15;
@@ -507,6 +507,7 @@ static void run_build(Ctx &c) {
     c.cur = nxt;
     for (;;) {
         fill_int(c, c.dFlags + 1, 2, 0);
+        ProfScope ps(c, PK_BUILD);
         switch (c.buildTeam) {
             case 8: launch_build<8>(c, cg, c.cur); break;
             case 16: launch_build<16>(c, cg, c.cur); break;

@@ -186,6 +186,7 @@ void pair_forces(Ctx &c, bool energy, bool virial) {
     a.force = c.force;
     a.acc = c.acc;
     const bool ev = energy || virial;
+    ProfScope ps(c, PK_PAIR);
     switch (c.pairTeam) {
         case 4: launch2<4>(a, c.ljOn, coul, ev, c.stream); break;
         case 8: launch2<8>(a, c.ljOn, coul, ev, c.stream); break;

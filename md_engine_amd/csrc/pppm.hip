@@ -404,6 +404,8 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
     const float qunit = (float)std::sqrt(P.qqr);
     const int threads = 256;
     const int cur = c.cur;
+    {
+    ProfScope ps(c, PK_SPREAD);
     switch (P.order) {
         case 1: hipLaunchKernelGGL(k_spread<1>, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.q[cur], m, qunit, (unsigned long long *)P.mesh); break;
         case 3: hipLaunchKernelGGL(k_spread<3>, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.q[cur], m, qunit, (unsigned long long *)P.mesh); break;
@@ -411,19 +413,30 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
         default: hipLaunchKernelGGL(k_spread<7>, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.q[cur], m, qunit, (unsigned long long *)P.mesh); break;
     }
     hipLaunchKernelGGL(k_mesh_to_real, dim3(cdiv(ng, threads)), dim3(threads), 0, c.stream, ng, P.mesh, P.real);
+    }
+    {
+    ProfScope ps(c, PK_FFT);
     MDX_REQUIRE(hipfftExecR2C(P.planR2C, P.real, (hipfftComplex *)P.rhoHat) == HIPFFT_SUCCESS, 2,
                 "hipfftExecR2C failed");
+    }
     if (energy || virial) {
         double V = c.geo.L[0] * c.geo.L[1] * c.geo.L[2];
         hipLaunchKernelGGL(k_kspace_ev, dim3(cdiv(nh, threads)), dim3(threads), 0, c.stream, m, szh,
                            P.alpha, 1.0 / V, P.green, P.rhoHat, c.acc);
     }
-    hipLaunchKernelGGL(k_efield, dim3(cdiv(nh, threads)), dim3(threads), 0, c.stream, m, szh, P.green,
-                       P.rhoHat, P.eHat);
-    MDX_REQUIRE(hipfftExecC2R(P.planC2R, (hipfftComplex *)P.eHat, P.eReal) == HIPFFT_SUCCESS, 2,
-                "hipfftExecC2R failed");
+    {
+        ProfScope ps(c, PK_EFIELD);
+        hipLaunchKernelGGL(k_efield, dim3(cdiv(nh, threads)), dim3(threads), 0, c.stream, m, szh, P.green,
+                           P.rhoHat, P.eHat);
+    }
+    {
+        ProfScope ps(c, PK_FFT);
+        MDX_REQUIRE(hipfftExecC2R(P.planC2R, (hipfftComplex *)P.eHat, P.eReal) == HIPFFT_SUCCESS, 2,
+                    "hipfftExecC2R failed");
+    }
     const double V = c.geo.L[0] * c.geo.L[1] * c.geo.L[2];
     const float scale = (float)(-std::sqrt(P.qqr) / V);
+    ProfScope ps(c, PK_GATHER);
     switch (P.order) {
         case 1: hipLaunchKernelGGL(k_gather<1>, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.q[cur], m, scale, P.eReal, c.force); break;
         case 3: hipLaunchKernelGGL(k_gather<3>, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.q[cur], m, scale, P.eReal, c.force); break;

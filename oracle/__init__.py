@@ -194,25 +194,47 @@ def exclusions_bfs(n_ids, bonds, max_depth=3):
 def tag_rows(rows, ids, excl):
     """Apply exclusion tags to neighbor index rows: entry = j | tag << 30
     (assignFromCell + addExclusion, GridGPU.cu:471-520).  ids maps index -> id."""
-    tagged = []
-    for i, row in enumerate(rows):
-        ex = excl.get(int(ids[i]), {})
-        r = row.astype(np.uint32).copy()
-        if ex:
-            for k, j in enumerate(row):
-                t = ex.get(int(ids[j]), 0)
-                r[k] = np.uint32(int(j) | (t << 30))
-        tagged.append(r)
-    return tagged
+    ids = np.asarray(ids, np.int64)
+    n = len(ids)
+    keys, tags = [], []
+    for a, d in excl.items():
+        for b, t in d.items():
+            keys.append(a * (n + 1) + b)
+            tags.append(t)
+    keys = np.asarray(keys, np.int64)
+    tags = np.asarray(tags, np.uint32)
+    order = np.argsort(keys)
+    keys, tags = keys[order], tags[order]
+    lens = np.array([len(r) for r in rows])
+    if lens.sum() == 0:
+        return [r.astype(np.uint32) for r in rows]
+    flat = np.concatenate(rows).astype(np.int64)
+    owner = np.repeat(np.arange(n), lens)
+    k = ids[owner] * (n + 1) + ids[flat]
+    pos = np.clip(np.searchsorted(keys, k), 0, max(len(keys) - 1, 0))
+    hit = (keys[pos] == k) if len(keys) else np.zeros(len(k), bool)
+    t = np.where(hit, tags[pos] if len(keys) else 0, 0).astype(np.uint32)
+    out = (flat.astype(np.uint32) | (t << np.uint32(30))).astype(np.uint32)
+    return np.split(out, np.cumsum(lens)[:-1])
+
+
+class NbrTable(tuple):
+    """Pre-marshalled (counts int32, table uint32 n*cap, cap) for orc_pair."""
 
 
 def rows_to_table(counts, rows, cap=None):
+    if isinstance(rows, NbrTable):
+        return rows
     n = len(rows)
-    cap = cap or max(1, int(max(counts) if n else 1))
+    lens = np.array([len(r) for r in rows], np.int64)
+    cap = cap or max(1, int(lens.max()) if n else 1)
     tab = np.zeros(n * cap, np.uint32)
-    for i, r in enumerate(rows):
-        tab[i * cap: i * cap + len(r)] = r
-    return _i(counts), tab, cap
+    if lens.sum():
+        flat = np.concatenate(rows).astype(np.uint32)
+        start = np.cumsum(lens) - lens
+        within = np.arange(len(flat)) - np.repeat(start, lens)
+        tab[np.repeat(np.arange(n, dtype=np.int64) * cap, lens) + within] = flat
+    return NbrTable((_i(counts), tab, cap))
 
 
 # --------------------------------------------------------------------------

@@ -460,9 +460,9 @@ static void orc_fft1(double *data, int n, size_t stride, int sign, double *tmp) 
     }
     for (int len = 2; len <= n; len <<= 1) {
         double ang = sign * 2.0 * ORC_PI / len;
-        for (int i = 0; i < n; i += len) {
-            for (int k = 0; k < len / 2; ++k) {
-                double wr = cos(ang * k), wi = sin(ang * k);
+        for (int k = 0; k < len / 2; ++k) {
+            double wr = cos(ang * k), wi = sin(ang * k);
+            for (int i = 0; i < n; i += len) {
                 int a = i + k, b = i + k + len / 2;
                 double br = tmp[2 * b] * wr - tmp[2 * b + 1] * wi;
                 double bi = tmp[2 * b] * wi + tmp[2 * b + 1] * wr;
