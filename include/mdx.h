@@ -75,8 +75,12 @@ int mdx_nlist_update(mdx_ctx *ctx, int force, int *rebuilt);
  * receives the longest row; rows longer than row_cap are truncated.
  * Replaces GridGPU::getNeighborList (GridGPU.cu:1520-1648). */
 int mdx_nlist_get(mdx_ctx *ctx, int32_t *counts, uint32_t *tagged_ids, int row_cap, int *max_count);
-/* Rebuild count since system_set, row capacity, cells per axis. */
-int mdx_nlist_stats(mdx_ctx *ctx, int64_t *rebuilds, int *row_cap, int32_t cells[3]);
+/* Rebuild count since system_set, row capacity, cells per axis, and the
+ * number of mdx_run steps whose list was older than the skin allows (a
+ * displacement above skin/2 that the rebuild decision missed; 0 in a valid
+ * run).  Any pointer may be NULL. */
+int mdx_nlist_stats(mdx_ctx *ctx, int64_t *rebuilds, int *row_cap, int32_t cells[3],
+                    int64_t *violations);
 
 /* LJ type-pair tables, ntypes x ntypes row-major: rc^2, 24 eps, sigma^6
  * (FixLJCut::prepareForRun, src/Fixes/FixLJCut.cu:83-122). */
@@ -124,6 +128,10 @@ int mdx_run(mdx_ctx *ctx, int nsteps, int flags);
  * E3B.  0 keeps the current value.  Powers of two only. */
 int mdx_teams_set(mdx_ctx *ctx, int build_team, int pair_team, int e3b_team);
 int mdx_sync(mdx_ctx *ctx);
+/* Implementation switches (no counterpart in the reference): "tiled_spread"
+ * 1 (default) accumulates the charge mesh in LDS tiles, 0 adds every weight
+ * to the global mesh directly; both give bit-identical meshes. */
+int mdx_option_set(mdx_ctx *ctx, const char *name, int value);
 
 /* Force-path steps at fixed positions: nsteps evaluations of the terms in
  * `flags`, each preceded by the skin check, with a forced neighbor-list

@@ -24,7 +24,7 @@ SYMBOLS = [
     "mdx_nlist_stats", "mdx_lj_set", "mdx_special_set", "mdx_ewald_real_set", "mdx_pppm_set",
     "mdx_pppm_info", "mdx_e3b_set", "mdx_forces_compute", "mdx_forces_get", "mdx_energy_get",
     "mdx_integrator_set", "mdx_run", "mdx_teams_set", "mdx_sync", "mdx_force_steps",
-    "mdx_profile_enable", "mdx_profile_get", "mdx_nlist_totals",
+    "mdx_profile_enable", "mdx_profile_get", "mdx_nlist_totals", "mdx_option_set",
 ]
 
 # kernel ids of mdx_profile_get
@@ -71,7 +71,8 @@ def load():
     L.mdx_exclusions_from_bonds.argtypes = [_vp, c, _ip, c]
     L.mdx_nlist_update.argtypes = [_vp, c, C.POINTER(c)]
     L.mdx_nlist_get.argtypes = [_vp, _vp, _vp, c, C.POINTER(c)]
-    L.mdx_nlist_stats.argtypes = [_vp, C.POINTER(C.c_int64), C.POINTER(c), _ip]
+    L.mdx_nlist_stats.argtypes = [_vp, C.POINTER(C.c_int64), C.POINTER(c), _ip, C.POINTER(C.c_int64)]
+    L.mdx_option_set.argtypes = [_vp, C.c_char_p, c]
     L.mdx_lj_set.argtypes = [_vp, c, _dp, _dp, _dp]
     L.mdx_special_set.argtypes = [_vp, d, d, d]
     L.mdx_ewald_real_set.argtypes = [_vp, c, d, d, d]
@@ -194,9 +195,14 @@ class Engine:
     def nlist_stats(self):
         nb = C.c_int64(0)
         cap = C.c_int(0)
+        viol = C.c_int64(0)
         cells = np.zeros(3, np.int32)
-        self._chk(self._L.mdx_nlist_stats(self._h, C.byref(nb), C.byref(cap), cells), "mdx_nlist_stats")
-        return dict(rebuilds=nb.value, row_cap=cap.value, cells=cells.tolist())
+        self._chk(self._L.mdx_nlist_stats(self._h, C.byref(nb), C.byref(cap), cells, C.byref(viol)),
+                  "mdx_nlist_stats")
+        return dict(rebuilds=nb.value, row_cap=cap.value, cells=cells.tolist(), violations=viol.value)
+
+    def option(self, name, value):
+        self._chk(self._L.mdx_option_set(self._h, name.encode(), int(value)), "mdx_option_set")
 
     # force terms ----------------------------------------------------------------
     def lj_set(self, rc2, eps24, sig6):

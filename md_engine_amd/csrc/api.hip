@@ -112,6 +112,63 @@ void prepare_terms(Ctx &c) {
     }
 }
 
+// Device copy of the per-atom state at a chunk boundary of mdx_run, restored
+// when a neighbor-row overflow is found at the end of the chunk.
+void snapshot_save(Ctx &c) {
+    if (!c.snap.pos) {
+        c.snap.pos = dalloc<double4>(c, c.n);
+        c.snap.vel = dalloc<double4>(c, c.n);
+        c.snap.q = dalloc<float>(c, c.n);
+        c.snap.type = dalloc<int>(c, c.n);
+        c.snap.id = dalloc<int>(c, c.n);
+        c.snap.idToIdx = dalloc<int>(c, c.n);
+    }
+    const int k = c.cur;
+    MDX_HIP(hipMemcpyAsync(c.snap.pos, c.pos[k], sizeof(double4) * c.n, hipMemcpyDeviceToDevice, c.stream));
+    MDX_HIP(hipMemcpyAsync(c.snap.vel, c.vel[k], sizeof(double4) * c.n, hipMemcpyDeviceToDevice, c.stream));
+    MDX_HIP(hipMemcpyAsync(c.snap.q, c.q[k], sizeof(float) * c.n, hipMemcpyDeviceToDevice, c.stream));
+    MDX_HIP(hipMemcpyAsync(c.snap.type, c.type[k], sizeof(int) * c.n, hipMemcpyDeviceToDevice, c.stream));
+    MDX_HIP(hipMemcpyAsync(c.snap.id, c.id[k], sizeof(int) * c.n, hipMemcpyDeviceToDevice, c.stream));
+    MDX_HIP(hipMemcpyAsync(c.snap.idToIdx, c.idToIdx, sizeof(int) * c.n, hipMemcpyDeviceToDevice, c.stream));
+    c.snap.turn = c.turn;
+    c.snap.rebuilds = c.nRebuilds;
+}
+
+void snapshot_restore(Ctx &c) {
+    const int k = c.cur;
+    MDX_HIP(hipMemcpyAsync(c.pos[k], c.snap.pos, sizeof(double4) * c.n, hipMemcpyDeviceToDevice, c.stream));
+    MDX_HIP(hipMemcpyAsync(c.vel[k], c.snap.vel, sizeof(double4) * c.n, hipMemcpyDeviceToDevice, c.stream));
+    MDX_HIP(hipMemcpyAsync(c.q[k], c.snap.q, sizeof(float) * c.n, hipMemcpyDeviceToDevice, c.stream));
+    MDX_HIP(hipMemcpyAsync(c.type[k], c.snap.type, sizeof(int) * c.n, hipMemcpyDeviceToDevice, c.stream));
+    MDX_HIP(hipMemcpyAsync(c.id[k], c.snap.id, sizeof(int) * c.n, hipMemcpyDeviceToDevice, c.stream));
+    MDX_HIP(hipMemcpyAsync(c.idToIdx, c.snap.idToIdx, sizeof(int) * c.n, hipMemcpyDeviceToDevice, c.stream));
+    c.turn = c.snap.turn;
+    c.nRebuilds = c.snap.rebuilds;
+    c.listValid = false;
+}
+
+// Reads the dMax slot written one step earlier (waits only for that step's
+// check, while later work keeps the GPU busy).  Returns the rebuild decision
+// for the current step and counts a violation when the list of the step
+// that was checked was already older than the skin allows.
+bool lagged_rebuild_decision(Ctx &c, int parity) {
+    MDX_HIP(hipEventSynchronize(c.stepEvent[parity]));
+    float d2, v2;
+    std::memcpy(&d2, &c.pinned[2 + 2 * parity], sizeof(float));
+    std::memcpy(&v2, &c.pinned[3 + 2 * parity], sizeof(float));
+    const double half = 0.5 * c.skin;
+    const double d = std::sqrt((double)d2), v = std::sqrt((double)v2);
+    if (d > half) c.listViolations++;
+    return d + 2.0 * c.dt * v > half;
+}
+
+void check_and_copy(Ctx &c, int parity) {
+    check_max(c, parity);
+    MDX_HIP(hipMemcpyAsync(c.pinned + 2 + 2 * parity, c.dMax + 2 * parity, sizeof(unsigned) * 2,
+                           hipMemcpyDeviceToHost, c.stream));
+    MDX_HIP(hipEventRecord(c.stepEvent[parity], c.stream));
+}
+
 template <class T>
 void upload(Ctx &c, T *dst, const T *src, size_t count) {
     MDX_HIP(hipMemcpyAsync(dst, src, sizeof(T) * count, hipMemcpyHostToDevice, c.stream));
@@ -141,6 +198,11 @@ int mdx_create(int device, mdx_ctx **out) {
     int rc = guarded(ctx, [](Ctx &c) {
         c.acc = dalloc<unsigned long long>(c, ACC_N);
         c.dFlags = dalloc<int>(c, 4);
+        c.dMax = dalloc<unsigned>(c, 4);
+        MDX_HIP(hipHostMalloc((void **)&c.pinned, sizeof(int) * 8, hipHostMallocDefault));
+        MDX_HIP(hipEventCreateWithFlags(&c.capEvent, hipEventDisableTiming));
+        for (int k = 0; k < 2; ++k) MDX_HIP(hipEventCreateWithFlags(&c.stepEvent[k], hipEventDisableTiming));
+        MDX_HIP(hipMemsetAsync(c.dMax, 0, sizeof(unsigned) * 4, c.stream));
         MDX_HIP(hipMemsetAsync(c.dFlags, 0, sizeof(int) * 4, c.stream));
         MDX_HIP(hipMemsetAsync(c.acc, 0, sizeof(unsigned long long) * ACC_N, c.stream));
     });
@@ -159,6 +221,10 @@ int mdx_destroy(mdx_ctx *ctx) {
     if (c.stream) hipStreamSynchronize(c.stream);
     pppm_release(c);
     prof_clear(c);
+    if (c.pinned) hipHostFree(c.pinned);
+    if (c.capEvent) hipEventDestroy(c.capEvent);
+    for (int k = 0; k < 2; ++k)
+        if (c.stepEvent[k]) hipEventDestroy(c.stepEvent[k]);
     for (hipEvent_t e : c.eventPool) hipEventDestroy(e);
     c.eventPool.clear();
     for (void *p : c.allocs) hipFree(p);
@@ -412,9 +478,11 @@ int mdx_nlist_get(mdx_ctx *ctx, int32_t *counts, uint32_t *tagged_ids, int row_c
     });
 }
 
-int mdx_nlist_stats(mdx_ctx *ctx, int64_t *rebuilds, int *row_cap, int32_t cells[3]) {
+int mdx_nlist_stats(mdx_ctx *ctx, int64_t *rebuilds, int *row_cap, int32_t cells[3],
+                    int64_t *violations) {
     return guarded(ctx, [&](Ctx &c) {
         if (rebuilds) *rebuilds = c.nRebuilds;
+        if (violations) *violations = c.listViolations;
         if (row_cap) *row_cap = c.cap;
         if (cells)
             for (int a = 0; a < 3; ++a) cells[a] = c.nc[a];
@@ -605,19 +673,60 @@ int mdx_run(mdx_ctx *ctx, int nsteps, int flags) {
         MDX_REQUIRE(nsteps >= 0, MDX_ERR_ARG, "nsteps must be >= 0");
         const int fflags = flags & (MDX_PAIR | MDX_PPPM | MDX_E3B);
         prepare_terms(c);
-        if (!c.listValid || c.forceFlags != fflags) {
-            ensure_list(c);
-            compute(c, fflags);
-        }
-        for (int s = 0; s < nsteps; ++s) {
-            integrate_first_half(c);
-            nlist_check(c);
-            nlist_rebuild(c, false);
-            compute(c, fflags);
-            integrate_second_half(c);
-            ++c.turn;
+        constexpr int kChunk = 64;
+        int done = 0;
+        while (done < nsteps) {
+            const int len = std::min(kChunk, nsteps - done);
+            snapshot_save(c);
+            if (!c.listValid || c.forceFlags != fflags) {
+                ensure_list(c);
+                compute(c, fflags);
+            }
+            // the "previous step" check for step 0 of the chunk, from cleared slots
+            fill_int(c, (int *)c.dMax, 4, 0);
+            check_and_copy(c, 1);
+            for (int s = 0; s < len; ++s) {
+                const int par = s & 1;
+                integrate_first_half(c);
+                if (lagged_rebuild_decision(c, 1 - par)) nlist_rebuild_async(c);
+                check_and_copy(c, par);
+                compute(c, fflags);
+                integrate_second_half(c);
+                ++c.turn;
+            }
+            MDX_HIP(hipStreamSynchronize(c.stream));
+            lagged_rebuild_decision(c, (len - 1) & 1);  // count a violation in the last step
+            if (!capacity_ok(c)) {
+                // a build overflowed its rows: redo the chunk with the grown rows
+                snapshot_restore(c);
+                c.forceFlags = -1;
+                continue;
+            }
+            done += len;
         }
         MDX_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int mdx_force_steps(mdx_ctx *ctx, int nsteps, int flags, int rebuild_every) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        MDX_REQUIRE(c.rlist > 0, MDX_ERR_STATE, "mdx_nlist_config not called");
+        MDX_REQUIRE(nsteps >= 0 && rebuild_every >= 0, MDX_ERR_ARG, "counts must be >= 0");
+        prepare_terms(c);
+        for (;;) {
+            for (int s = 0; s < nsteps; ++s) {
+                if (!c.listValid) {
+                    ensure_list(c);
+                } else if (rebuild_every > 0 && s % rebuild_every == 0) {
+                    nlist_rebuild_async(c);
+                }
+                compute(c, flags & (MDX_PAIR | MDX_PPPM | MDX_E3B));
+            }
+            MDX_HIP(hipStreamSynchronize(c.stream));
+            if (capacity_ok(c)) break;
+            c.listValid = false;  // rows grew: evaluate again with complete rows
+        }
     });
 }
 
@@ -632,26 +741,6 @@ int mdx_teams_set(mdx_ctx *ctx, int build_team, int pair_team, int e3b_team) {
         }
         if (pair_team) c.pairTeam = pair_team;
         if (e3b_team) c.e3b.team = e3b_team;
-    });
-}
-
-int mdx_force_steps(mdx_ctx *ctx, int nsteps, int flags, int rebuild_every) {
-    return guarded(ctx, [&](Ctx &c) {
-        require_system(c);
-        MDX_REQUIRE(c.rlist > 0, MDX_ERR_STATE, "mdx_nlist_config not called");
-        MDX_REQUIRE(nsteps >= 0 && rebuild_every >= 0, MDX_ERR_ARG, "counts must be >= 0");
-        prepare_terms(c);
-        for (int s = 0; s < nsteps; ++s) {
-            if (!c.nlistConfigured) nlist_setup(c);
-            if (!c.listValid || (rebuild_every > 0 && s % rebuild_every == 0)) {
-                nlist_rebuild(c, true);
-            } else {
-                nlist_check(c);
-                nlist_rebuild(c, false);
-            }
-            compute(c, flags & (MDX_PAIR | MDX_PPPM | MDX_E3B));
-        }
-        MDX_HIP(hipStreamSynchronize(c.stream));
     });
 }
 
@@ -696,6 +785,15 @@ int mdx_nlist_totals(mdx_ctx *ctx, int64_t *pair_entries, int64_t *mol_entries) 
         for (int v : mc) m += std::min(v, c.e3b.cap);
         if (pair_entries) *pair_entries = s;
         if (mol_entries) *mol_entries = m;
+    });
+}
+
+int mdx_option_set(mdx_ctx *ctx, const char *name, int value) {
+    return guarded(ctx, [&](Ctx &c) {
+        MDX_REQUIRE(name, MDX_ERR_ARG, "option name required");
+        std::string k(name);
+        if (k == "tiled_spread") c.mdxDisableTiledSpread = (value == 0);
+        else throw Error{MDX_ERR_ARG, "unknown option " + k};
     });
 }
 

@@ -126,10 +126,17 @@ struct Ctx {
     int *cellFill = nullptr;
     int *cellOf = nullptr;
     int *perm = nullptr;
+    int *perm2 = nullptr;
+    int2 *colXY = nullptr;        // by column rank: cell (x, y)
     int cap = 0;
     uint32_t *nl = nullptr;
     int *nlCount = nullptr;
     int *dFlags = nullptr;        // [0] rebuild flag, [1] max count, [2] e3b max count
+    unsigned *dMax = nullptr;     // 2 slots x (max |x - x_last|^2, max |v|^2) as float bits
+    int *pinned = nullptr;        // host-pinned: [0..1] last build's max counts, [2..5] dMax copies
+    hipEvent_t capEvent = nullptr;
+    bool capPending = false;
+    long long listViolations = 0; // steps whose list was older than the skin allows
     int buildTeam = 32;
     bool buildTeamUser = false;   // set by mdx_teams_set, else chosen from the cell population
     long long nRebuilds = 0;
@@ -152,12 +159,21 @@ struct Ctx {
 
     PppmState pppm;
     E3BState e3b;
+    bool mdxDisableTiledSpread = false;   // test knob: force the plain global-atomic spread
 
     // integration
     double dt = 0.005;
     double ftmToV = 1.0;
     long long turn = 0;
     int forceFlags = 0;
+
+    hipEvent_t stepEvent[2] = {nullptr, nullptr};
+    struct Snapshot {
+        double4 *pos = nullptr, *vel = nullptr;
+        float *q = nullptr;
+        int *type = nullptr, *id = nullptr, *idToIdx = nullptr;
+        long long turn = 0, rebuilds = 0;
+    } snap;
 
     // profiling: event pairs recorded on the context stream around kernels
     bool prof = false;
@@ -237,7 +253,10 @@ void nlist_check(Ctx &c);             // sets dFlags[0] when a rebuild is due
 void nlist_rebuild(Ctx &c, bool forced);  // rebuild if dFlags[0] or forced (host reads flag)
 void refresh_xq(Ctx &c);
 void fill_int(Ctx &c, int *p, int count, int v);  // stream-ordered fill (no memset nodes)
-void e3b_build_list(Ctx &c);
+void nlist_rebuild_async(Ctx &c);     // rebuild now; capacity checked later (capacity_ok)
+bool capacity_ok(Ctx &c);             // false: rows grew, results since the last check are invalid
+bool grow_if_needed(Ctx &c);
+void check_max(Ctx &c, int parity);   // max displacement / speed into dMax slot `parity`
 
 // pair.hip
 void pair_forces(Ctx &c, bool energy, bool virial);

@@ -141,22 +141,42 @@ __global__ void k_place(int n, const int *__restrict__ cellOf, int *__restrict__
     perm[slot] = i;
 }
 
-// Make the order inside each cell independent of atomic slot order: sort the
-// cell's entries by atom id (cells hold tens of atoms; insertion sort).
-__global__ void k_cellsort(int ncell, const int *__restrict__ start, int *__restrict__ perm,
-                           const int *__restrict__ id) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
-    int a = start[c], b = start[c + 1];
-    for (int s = a + 1; s < b; ++s) {
-        int v = perm[s];
-        int key = id[v];
-        int t = s - 1;
-        while (t >= a && id[perm[t]] > key) {
-            perm[t + 1] = perm[t];
-            --t;
+// Make the order inside each cell independent of atomic slot order: one
+// workgroup per cell ranks its entries by atom id in LDS (ids are unique, so
+// rank = number of smaller ids) and writes them in rank order.
+__global__ void __launch_bounds__(64) k_cellsort(int ncell, const int *__restrict__ start,
+                                                 const int *__restrict__ permIn,
+                                                 int *__restrict__ permOut,
+                                                 const int *__restrict__ id) {
+    constexpr int CAP = 1024;
+    __shared__ int sid[CAP];
+    __shared__ int sperm[CAP];
+    const int c = blockIdx.x;
+    const int a = start[c], b = start[c + 1], m = b - a;
+    if (m <= CAP) {
+        for (int t = threadIdx.x; t < m; t += 64) {
+            const int p = permIn[a + t];
+            sperm[t] = p;
+            sid[t] = id[p];
         }
-        perm[t + 1] = v;
+        __syncthreads();
+        for (int t = threadIdx.x; t < m; t += 64) {
+            const int key = sid[t];
+            int r = 0;
+            for (int j = 0; j < m; ++j) r += sid[j] < key;
+            permOut[a + r] = sperm[t];
+        }
+    } else if (threadIdx.x == 0) {
+        // very crowded cell: serial insertion sort
+        for (int s = a; s < b; ++s) permOut[s] = permIn[s];
+        for (int s = a + 1; s < b; ++s) {
+            int v = permOut[s], key = id[v], t = s - 1;
+            while (t >= a && id[permOut[t]] > key) {
+                permOut[t + 1] = permOut[t];
+                --t;
+            }
+            permOut[t + 1] = v;
+        }
     }
 }
 
@@ -433,6 +453,7 @@ void nlist_setup(Ctx &c) {
     int ncell = c.nc[0] * c.nc[1] * c.nc[2];
     if (ncell != c.ncell) {
         if (c.ncell) {
+            dfree(c, c.colXY);
             dfree(c, c.colRank);
             dfree(c, c.cellCount);
             dfree(c, c.cellStart);
@@ -440,6 +461,7 @@ void nlist_setup(Ctx &c) {
         }
         c.ncell = ncell;
         c.colRank = dalloc<int>(c, (size_t)c.nc[0] * c.nc[1]);
+        c.colXY = dalloc<int2>(c, (size_t)c.nc[0] * c.nc[1]);
         c.cellCount = dalloc<int>(c, ncell);
         c.cellStart = dalloc<int>(c, ncell + 1);
         c.cellFill = dalloc<int>(c, ncell);
@@ -450,12 +472,20 @@ void nlist_setup(Ctx &c) {
         for (int y = 0; y < c.nc[1]; ++y) keys.push_back({morton2(x, y), x * c.nc[1] + y});
     std::sort(keys.begin(), keys.end());
     std::vector<int> rank(keys.size());
-    for (size_t r = 0; r < keys.size(); ++r) rank[keys[r].second] = (int)r;
+    std::vector<int2> xy(keys.size());
+    for (size_t r = 0; r < keys.size(); ++r) {
+        rank[keys[r].second] = (int)r;
+        xy[r] = make_int2(keys[r].second / c.nc[1], keys[r].second % c.nc[1]);
+    }
     MDX_HIP(hipMemcpyAsync(c.colRank, rank.data(), sizeof(int) * rank.size(), hipMemcpyHostToDevice,
                            c.stream));
+    MDX_HIP(hipMemcpyAsync(c.colXY, xy.data(), sizeof(int2) * xy.size(), hipMemcpyHostToDevice,
+                           c.stream));
+    MDX_HIP(hipStreamSynchronize(c.stream));
     if (!c.cellOf) {
         c.cellOf = dalloc<int>(c, c.n);
         c.perm = dalloc<int>(c, c.n);
+        c.perm2 = dalloc<int>(c, c.n);
     }
     // team width: next power of two of the expected z-run population
     double dens = c.n / (c.geo.L[0] * c.geo.L[1] * c.geo.L[2]);
@@ -486,7 +516,7 @@ void nlist_check(Ctx &c) {
                        c.pos[c.cur], c.posLast, c.geo, thr * thr, c.dFlags);
 }
 
-static void run_build(Ctx &c) {
+static void run_build(Ctx &c, bool sync) {
     CellGrid cg = make_grid(c);
     int threads = 256;
     int cur = c.cur, nxt = 1 - c.cur;
@@ -498,60 +528,82 @@ static void run_build(Ctx &c) {
                        c.cellFill);
     hipLaunchKernelGGL(k_place, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.cellOf,
                        c.cellFill, c.perm);
-    hipLaunchKernelGGL(k_cellsort, dim3(cdiv(c.ncell, 64)), dim3(64), 0, c.stream, c.ncell,
-                       c.cellStart, c.perm, c.id[cur]);
-    hipLaunchKernelGGL(k_permute, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.perm,
+    hipLaunchKernelGGL(k_cellsort, dim3(c.ncell), dim3(64), 0, c.stream, c.ncell, c.cellStart, c.perm,
+                       c.perm2, c.id[cur]);
+    hipLaunchKernelGGL(k_permute, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.perm2,
                        c.pos[cur], c.vel[cur], c.q[cur], c.type[cur], c.id[cur], c.pos[nxt],
                        c.vel[nxt], c.q[nxt], c.type[nxt], c.id[nxt], c.idToIdx, c.posLast, c.xq,
                        c.geo);
     c.cur = nxt;
-    for (;;) {
-        fill_int(c, c.dFlags + 1, 2, 0);
-        ProfScope ps(c, PK_BUILD);
-        switch (c.buildTeam) {
-            case 8: launch_build<8>(c, cg, c.cur); break;
-            case 16: launch_build<16>(c, cg, c.cur); break;
-            case 32: launch_build<32>(c, cg, c.cur); break;
-            default: launch_build<64>(c, cg, c.cur); break;
-        }
-        int maxc = 0;
-        MDX_HIP(hipMemcpyAsync(&maxc, c.dFlags + 1, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-        MDX_HIP(hipStreamSynchronize(c.stream));
-        if (maxc <= c.cap) break;
-        // grow the rows and rebuild (the reference reallocates at 1.5x, GridGPU.cu:1026-1030)
-        dfree(c, c.nl);
-        c.cap = ((int)(maxc * 1.25) + 15) & ~15;
-        c.nl = dalloc<uint32_t>(c, (size_t)c.n * c.cap);
-    }
     if (c.e3b.on) {
         fill_int(c, c.e3b.atomMolO, c.n, -1);  // indices were permuted: drop the old map
         hipLaunchKernelGGL(k_mol_atoms, dim3(cdiv(c.e3b.nmol, threads)), dim3(threads), 0, c.stream,
                            c.e3b.nmol, c.e3b.molIds, c.idToIdx, c.e3b.molAtoms, c.e3b.atomMolO);
-        e3b_build_list(c);
+        MDX_REQUIRE(c.rlist >= (double)c.e3b.r2b + c.skin - 1e-9, 1,
+                    "E3B needs rCut + padding >= 6.2 A + padding (the molecule list is taken from the "
+                    "pair list)");
+    }
+    for (;;) {
+        fill_int(c, c.dFlags + 1, 2, 0);
+        {
+            ProfScope ps(c, PK_BUILD);
+            switch (c.buildTeam) {
+                case 8: launch_build<8>(c, cg, c.cur); break;
+                case 16: launch_build<16>(c, cg, c.cur); break;
+                case 32: launch_build<32>(c, cg, c.cur); break;
+                default: launch_build<64>(c, cg, c.cur); break;
+            }
+        }
+        if (c.e3b.on) {
+            switch (c.e3b.team) {
+                case 8: launch_e3b_list<8>(c, c.cur); break;
+                case 16: launch_e3b_list<16>(c, c.cur); break;
+                default: launch_e3b_list<32>(c, c.cur); break;
+            }
+        }
+        // largest row counts -> pinned host memory; checked now (sync) or at
+        // the next synchronisation point (capacity_ok)
+        MDX_HIP(hipMemcpyAsync(c.pinned, c.dFlags + 1, sizeof(int) * 2, hipMemcpyDeviceToHost, c.stream));
+        if (!sync) {
+            MDX_HIP(hipEventRecord(c.capEvent, c.stream));
+            c.capPending = true;
+            break;
+        }
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        if (grow_if_needed(c)) continue;
+        break;
     }
     c.nRebuilds++;
     c.listValid = true;
 }
 
-void e3b_build_list(Ctx &c) {
-    MDX_REQUIRE(c.rlist >= (double)c.e3b.r2b + c.skin - 1e-9, 1,
-                "E3B needs rCut + padding >= 6.2 A + padding (the molecule list is taken from the "
-                "pair list)");
-    for (;;) {
-        fill_int(c, c.dFlags + 2, 1, 0);
-        switch (c.e3b.team) {
-            case 8: launch_e3b_list<8>(c, c.cur); break;
-            case 16: launch_e3b_list<16>(c, c.cur); break;
-            default: launch_e3b_list<32>(c, c.cur); break;
-        }
-        int maxc = 0;
-        MDX_HIP(hipMemcpyAsync(&maxc, c.dFlags + 2, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-        MDX_HIP(hipStreamSynchronize(c.stream));
-        if (maxc <= c.e3b.cap) break;
-        dfree(c, c.e3b.nl);
-        c.e3b.cap = ((int)(maxc * 1.25) + 15) & ~15;
-        c.e3b.nl = dalloc<int>(c, (size_t)c.e3b.nmol * c.e3b.cap);
+// Grows the pair / molecule rows after an overflow seen in pinned[0..1];
+// returns true when something grew (the caller rebuilds).  The reference
+// reallocates its list at 1.5x (GridGPU.cu:1026-1030).
+bool grow_if_needed(Ctx &c) {
+    bool grew = false;
+    if (c.pinned[0] > c.cap) {
+        dfree(c, c.nl);
+        c.cap = ((int)(c.pinned[0] * 1.25) + 15) & ~15;
+        c.nl = dalloc<uint32_t>(c, (size_t)c.n * c.cap);
+        grew = true;
     }
+    if (c.e3b.on && c.pinned[1] > c.e3b.cap) {
+        dfree(c, c.e3b.nl);
+        c.e3b.cap = ((int)(c.pinned[1] * 1.25) + 15) & ~15;
+        c.e3b.nl = dalloc<int>(c, (size_t)c.e3b.nmol * c.e3b.cap);
+        grew = true;
+    }
+    return grew;
+}
+
+// True when every build since the last call fitted its rows.  Waits for the
+// last asynchronous build only.
+bool capacity_ok(Ctx &c) {
+    if (!c.capPending) return true;
+    MDX_HIP(hipEventSynchronize(c.capEvent));
+    c.capPending = false;
+    return !grow_if_needed(c);
 }
 
 void nlist_rebuild(Ctx &c, bool forced) {
@@ -562,12 +614,55 @@ void nlist_rebuild(Ctx &c, bool forced) {
         MDX_HIP(hipStreamSynchronize(c.stream));
         if (!flag && c.listValid) return;
     }
-    run_build(c);
+    run_build(c, true);
     fill_int(c, c.dFlags, 1, 0);
+}
+
+void nlist_rebuild_async(Ctx &c) {
+    if (!c.nlistConfigured) nlist_setup(c);
+    run_build(c, !c.listValid);
 }
 
 void fill_int(Ctx &c, int *p, int count, int v) {
     hipLaunchKernelGGL(k_fill_int, dim3(cdiv(count, 256)), dim3(256), 0, c.stream, count, p, v);
+}
+
+namespace {
+// Lag-1 rebuild decision for the step loop: max |x - x_last|^2 and max |v|^2
+// over all sites into one slot of dMax (as float bits; non-negative floats
+// order like their bit patterns); the other slot is cleared for the next step.
+// The host reads the slot one step later.
+__global__ void k_check_max(int n, const double4 *__restrict__ pos, const double4 *__restrict__ last,
+                            const double4 *__restrict__ vel, Geometry g, unsigned *__restrict__ slot,
+                            unsigned *__restrict__ other) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        other[0] = 0u;
+        other[1] = 0u;
+    }
+    float d2 = 0.f, v2 = 0.f;
+    if (i < n) {
+        d2 = (float)dist2(pos[i], last[i], g);
+        double4 v = vel[i];
+        v2 = (float)(v.x * v.x + v.y * v.y + v.z * v.z);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        d2 = fmaxf(d2, __shfl_xor(d2, o, 64));
+        v2 = fmaxf(v2, __shfl_xor(v2, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&slot[0], __float_as_uint(d2));
+        atomicMax(&slot[1], __float_as_uint(v2));
+    }
+}
+}  // namespace
+
+void check_max(Ctx &c, int parity) {
+    int threads = 256;
+    hipLaunchKernelGGL(k_check_max, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n,
+                       c.pos[c.cur], c.posLast, c.vel[c.cur], c.geo, c.dMax + 2 * parity,
+                       c.dMax + 2 * (1 - parity));
 }
 
 }  // namespace mdx

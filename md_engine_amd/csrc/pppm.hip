@@ -116,6 +116,105 @@ __global__ void __launch_bounds__(256) k_spread(int n, const double4 *__restrict
     }
 }
 
+// LDS-tiled spread.  One workgroup per (cell column, chunk of zs cells in z):
+// its atoms are contiguous in the cell-ordered arrays and, with positions
+// within skin/2 of their cell since the last build, every stencil point lies in
+// a box of mesh points known from the cell geometry.  Weights are added to that
+// box in LDS (64-bit integer atomics, order independent), and the box is then
+// added to the global mesh with one atomic per point whose addresses are
+// consecutive across a wave.  Rows that leave the box (never, unless an atom
+// drifted further than the margin) go straight to global memory.
+struct TileGeo {
+    int ncz, zs, nzc;
+    int ext[3];
+    double cellw[3];
+    double margin;
+};
+
+__device__ __forceinline__ int tile_origin(double lo_pos, double h) {
+    // same rounding as mesh_point on the lowest reachable coordinate
+    return (int)floor((lo_pos + 0.5 * h) / h);
+}
+
+template <int ORDER>
+__global__ void __launch_bounds__(256) k_spread_tiled(const double4 *__restrict__ pos,
+                                                      const float *__restrict__ q, MeshGeo m,
+                                                      float qunit, const int *__restrict__ cellStart,
+                                                      const int2 *__restrict__ colXY, TileGeo tg,
+                                                      unsigned long long *__restrict__ mesh) {
+    extern __shared__ unsigned long long tile[];
+    constexpr int H = (ORDER - 1) / 2;
+    const int col = blockIdx.x / tg.nzc;
+    const int zc = blockIdx.x % tg.nzc;
+    const int z0 = zc * tg.zs;
+    const int z1 = min(z0 + tg.zs, tg.ncz);
+    const int a0 = cellStart[col * tg.ncz + z0];
+    const int a1 = cellStart[col * tg.ncz + z1];
+    const int2 cxy = colXY[col];
+    const int ox = tile_origin(cxy.x * tg.cellw[0] - tg.margin, m.h[0]) - H;
+    const int oy = tile_origin(cxy.y * tg.cellw[1] - tg.margin, m.h[1]) - H;
+    const int oz = tile_origin(z0 * tg.cellw[2] - tg.margin, m.h[2]) - H;
+    const int ex = tg.ext[0], ey = tg.ext[1], ez = tg.ext[2];
+    const int vol = ex * ey * ez;
+    for (int k = threadIdx.x; k < vol; k += blockDim.x) tile[k] = 0ull;
+    __syncthreads();
+    const int items = (a1 - a0) * ORDER;
+    for (int it = threadIdx.x; it < items; it += blockDim.x) {
+        const int i = a0 + it / ORDER;
+        const int ix = it % ORDER;
+        const float qi = q[i];
+        if (qi == 0.0f) continue;
+        const double4 p = pos[i];
+        int g[3];
+        float d[3];
+        mesh_point(p.x, 0, m, g[0], d[0]);
+        mesh_point(p.y, 1, m, g[1], d[1]);
+        mesh_point(p.z, 2, m, g[2], d[2]);
+        float wx[ORDER], wy[ORDER], wz[ORDER];
+        weights<ORDER>(d[0], wx);
+        weights<ORDER>(d[1], wy);
+        weights<ORDER>(d[2], wz);
+        const float a = qunit * qi * wx[ix];
+        const int rx = wrapi(g[0] - H + ix - ox, m.sz[0]);
+        const int ry = wrapi(g[1] - H - oy, m.sz[1]);
+        const int rz = wrapi(g[2] - H - oz, m.sz[2]);
+        if (rx < ex && ry + ORDER <= ey && rz + ORDER <= ez) {
+#pragma unroll
+            for (int iy = 0; iy < ORDER; ++iy) {
+                const float b = a * wy[iy];
+                const int rb = (rx * ey + ry + iy) * ez + rz;
+#pragma unroll
+                for (int iz = 0; iz < ORDER; ++iz) {
+                    const long long v = __double2ll_rn((double)(b * wz[iz]) * kMeshScale);
+                    atomicAdd(&tile[rb + iz], (unsigned long long)v);
+                }
+            }
+        } else {
+            const int px = wrapi(g[0] + ix - H, m.sz[0]);
+#pragma unroll
+            for (int iy = 0; iy < ORDER; ++iy) {
+                const int py = wrapi(g[1] + iy - H, m.sz[1]);
+                const float b = a * wy[iy];
+                const size_t rowb = ((size_t)px * m.sz[1] + py) * m.sz[2];
+#pragma unroll
+                for (int iz = 0; iz < ORDER; ++iz) {
+                    const int pz = wrapi(g[2] + iz - H, m.sz[2]);
+                    const long long v = __double2ll_rn((double)(b * wz[iz]) * kMeshScale);
+                    atomicAdd(&mesh[rowb + pz], (unsigned long long)v);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < vol; k += blockDim.x) {
+        const unsigned long long v = tile[k];
+        if (v == 0ull) continue;
+        const int kz = k % ez, ky = (k / ez) % ey, kx = k / (ez * ey);
+        const int gx = wrapi(ox + kx, m.sz[0]), gy = wrapi(oy + ky, m.sz[1]), gz = wrapi(oz + kz, m.sz[2]);
+        atomicAdd(&mesh[((size_t)gx * m.sz[1] + gy) * m.sz[2] + gz], v);
+    }
+}
+
 __global__ void k_mesh_to_real(int ng, long long *__restrict__ mesh, float *__restrict__ out) {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ng) return;
@@ -318,6 +417,38 @@ double deltaf_real(double a, int natoms, double q2, const double *L, double rcut
     return 2 * q2 / std::sqrt(natoms * rcut * L[0] * L[1] * L[2]) * std::exp(-a * a * rcut * rcut);
 }
 
+// Tile shape for the LDS spread: z-chunks of zs cells so that the tile stays
+// within 48 KB of LDS.  Returns false (plain spread) when no cell list is
+// valid, the box is not periodic, or even one cell does not fit.
+bool tiled_geometry(const Ctx &c, TileGeo &tg) {
+    if (!c.listValid || c.ncell == 0 || !c.colXY || c.mdxDisableTiledSpread) return false;
+    for (int a = 0; a < 3; ++a)
+        if (!c.geo.periodic[a]) return false;
+    const int order = c.pppm.order;
+    tg.ncz = c.nc[2];
+    tg.margin = 0.5 * c.skin + 1e-6 * c.geo.L[0];
+    for (int a = 0; a < 3; ++a) tg.cellw[a] = c.geo.L[a] / c.nc[a];
+    auto extent = [&](int a, int ncells) {
+        double h = c.geo.L[a] / c.pppm.sz[a];
+        return (int)std::floor((ncells * tg.cellw[a] + 2.0 * tg.margin) / h) + order + 2;
+    };
+    tg.ext[0] = extent(0, 1);
+    tg.ext[1] = extent(1, 1);
+    if (tg.ext[0] >= c.pppm.sz[0] || tg.ext[1] >= c.pppm.sz[1]) return false;
+    const int budget = 48 * 1024 / 8;
+    int zs = 0;
+    for (int z = 1; z <= tg.ncz; ++z) {
+        int ez = extent(2, z);
+        if (ez >= c.pppm.sz[2] || tg.ext[0] * tg.ext[1] * ez > budget) break;
+        zs = z;
+    }
+    if (zs == 0) return false;
+    tg.zs = zs;
+    tg.nzc = (tg.ncz + zs - 1) / zs;
+    tg.ext[2] = extent(2, zs);
+    return true;
+}
+
 }  // namespace
 
 // Secant solve of DeltaF_k(alpha) = DeltaF_real(alpha)
@@ -406,6 +537,17 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
     const int cur = c.cur;
     {
     ProfScope ps(c, PK_SPREAD);
+    TileGeo tg;
+    if (tiled_geometry(c, tg)) {
+        const int nblk = c.nc[0] * c.nc[1] * tg.nzc;
+        const size_t shm = sizeof(unsigned long long) * tg.ext[0] * tg.ext[1] * tg.ext[2];
+        switch (P.order) {
+            case 1: hipLaunchKernelGGL(k_spread_tiled<1>, dim3(nblk), dim3(threads), shm, c.stream, c.pos[cur], c.q[cur], m, qunit, c.cellStart, c.colXY, tg, (unsigned long long *)P.mesh); break;
+            case 3: hipLaunchKernelGGL(k_spread_tiled<3>, dim3(nblk), dim3(threads), shm, c.stream, c.pos[cur], c.q[cur], m, qunit, c.cellStart, c.colXY, tg, (unsigned long long *)P.mesh); break;
+            case 5: hipLaunchKernelGGL(k_spread_tiled<5>, dim3(nblk), dim3(threads), shm, c.stream, c.pos[cur], c.q[cur], m, qunit, c.cellStart, c.colXY, tg, (unsigned long long *)P.mesh); break;
+            default: hipLaunchKernelGGL(k_spread_tiled<7>, dim3(nblk), dim3(threads), shm, c.stream, c.pos[cur], c.q[cur], m, qunit, c.cellStart, c.colXY, tg, (unsigned long long *)P.mesh); break;
+        }
+    } else
     switch (P.order) {
         case 1: hipLaunchKernelGGL(k_spread<1>, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.q[cur], m, qunit, (unsigned long long *)P.mesh); break;
         case 3: hipLaunchKernelGGL(k_spread<3>, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.q[cur], m, qunit, (unsigned long long *)P.mesh); break;
