@@ -135,6 +135,21 @@ def cpu_baseline(w, rebuild_every, budget_s=12.0):
                        f"build {t_build * 1e3:.1f} ms")
 
 
+def copy_probe(mib=256, reps=5):
+    """Plain device-to-device copy of a known size, for scaling the FETCH_SIZE /
+    WRITE_SIZE counters to bytes (profiles/ records the factor)."""
+    import torch
+
+    n = mib * 1024 * 1024 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read_per_launch": n * 4, "bytes_written_per_launch": n * 4,
+                      "launches": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,7 +161,12 @@ def main():
     ap.add_argument("--pair-team", type=int, default=0)
     ap.add_argument("--build-team", type=int, default=0)
     ap.add_argument("--e3b-team", type=int, default=0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain device copy of known size (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for --gpus {args.gpus}); this build has no "
               "multi-GPU path", file=sys.stderr)
@@ -181,6 +201,11 @@ def main():
     bytes_launch = n * (16 + 4 + 4 + 24) + 4 * pairs
     achieved = bytes_launch / avg_s / 1e9
     kernel_ms = {k: (v[1] / max(v[0], 1)) for k, v in prof.items() if v[0]}
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "c3_pair_traffic.json")
+    if args.config == "tip4p-e3b" and os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
 
     out = {
         "metric": "atom-steps/s",
@@ -211,7 +236,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/c3_force_path_tip4p_e3b_8192.md"
+            if traffic else None,
             "bytes_per_launch": bytes_launch,
             "avg_launch_ms": avg_s * 1e3,
             "launches": cnt,
