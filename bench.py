@@ -82,6 +82,27 @@ def setup_engine(w):
     return e, flags
 
 
+def setup_md(e, w):
+    """Rigid TIP4P/2005 MD: Maxwell-Boltzmann velocities at 298 K (seeded),
+    SETTLE geometry, dt = 2 fs."""
+    from md_engine_amd import systems
+
+    box = w["box"]
+    if box["mol"] is None:
+        v = systems.maxwell_boltzmann(box["mass"], 1.2, seed=20261202)
+        e.set_system(box["x"], box["lo"], box["L"], v=v, inv_mass=np.ones(len(v)), q=box["q"],
+                     types=box["type"])
+        e.integrator_set(0.005, 1.0)
+        return
+    p = systems.TIP4P2005
+    v = systems.maxwell_boltzmann(box["mass"], 298.0, seed=20261202, boltz=systems.REAL_BOLTZ,
+                                  mvv_to_eng=systems.REAL_MVV_TO_ENG)
+    inv = np.where(box["mass"] > 0, 1.0 / np.maximum(box["mass"], 1e-30), 0.0)
+    e.set_system(box["x"], box["lo"], box["L"], v=v, inv_mass=inv, q=box["q"], types=box["type"])
+    e.rigid_set(box["mol"], p["m_O"], p["m_H"], p["r_OH"], p["r_HH"], p["gamma"])
+    e.integrator_set(w["dt_fs"], systems.REAL_FTM_TO_V)
+
+
 def cpu_baseline(w, rebuild_every, budget_s=12.0):
     """FP64 oracle of the same step on the host cores (bounded sample)."""
     import oracle
@@ -156,7 +177,10 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--config", default="tip4p-e3b", choices=["tip4p-e3b", "tip4p", "lj"])
-    ap.add_argument("--rebuild-every", type=int, default=10)
+    ap.add_argument("--rebuild-every", type=int, default=10,
+                    help="force mode: forced list rebuild interval")
+    ap.add_argument("--mode", default="force", choices=["force", "md"],
+                    help="force: force path at fixed positions; md: full rigid-water MD steps")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pair-team", type=int, default=0)
     ap.add_argument("--build-team", type=int, default=0)
@@ -178,11 +202,18 @@ def main():
     n = len(w["box"]["x"])
     R = args.rebuild_every
 
-    e.force_steps(args.warmup, flags, R)
+    md = args.mode == "md"
+    if md:
+        setup_md(e, w)
+        step = lambda k: e.run(k, flags)  # noqa: E731
+    else:
+        step = lambda k: e.force_steps(k, flags, R)  # noqa: E731
+    step(args.warmup)
     e.sync()
+    rebuilds0 = e.nlist_stats()["rebuilds"]
     e.profile(True)
     t0 = time.perf_counter()
-    e.force_steps(args.steps, flags, R)
+    step(args.steps)
     e.sync()
     t1 = time.perf_counter()
     wall = t1 - t0
@@ -222,7 +253,11 @@ def main():
         "data": "synthetic (seeded lattice, random orientations)",
         "config": {
             "workload": w["desc"],
-            "step": "force path: skin check, list rebuild every %d steps, LJ+Ewald real, P3M, E3B" % R,
+            "step": ("MD step: velocity Verlet + FP64 SETTLE/RATTLE + M site, skin-driven list rebuilds, "
+                     "LJ+Ewald real, P3M, E3B" if md else
+                     "force path: list rebuild every %d steps, LJ+Ewald real, P3M, E3B" % R),
+            "list_rebuilds_timed": e.nlist_stats()["rebuilds"] - rebuilds0,
+            "list_violations": e.nlist_stats()["violations"],
             "rc": w["rc"], "skin": w["skin"], "pppm_mesh": w["sz"], "pppm_order": w["order"],
             "e3b": w["e3b"], "sites": n, "pair_list_entries": pairs, "e3b_list_entries": mols,
             "list_rebuilds_in_run": st["rebuilds"],

@@ -108,6 +108,14 @@ int mdx_pppm_info(mdx_ctx *ctx, double *alpha, double *rms_error);
 int mdx_e3b_set(mdx_ctx *ctx, int nmol, const int32_t *mol_ids, const double params[8],
                 double r_two_body, double r_max);
 
+/* Rigid water (FixRigid::createRigid with a TIP4P/2005-style style,
+ * src/Fixes/FixRigid.cu:1372-1379, 1631-1681): mol_ids 4 per molecule, O, H1,
+ * H2, M (M = -1 for three-site water); masses, the O-H and H-H distances kept
+ * by SETTLE, and gamma with M = gamma O + (1 - gamma)/2 (H1 + H2).  mdx_run
+ * then integrates these sites with SETTLE / RATTLE in FP64 and places M. */
+int mdx_rigid_set(mdx_ctx *ctx, int nmol, const int32_t *mol_ids, double m_O, double m_H, double d_OH,
+                  double d_HH, double gamma);
+
 /* Zero and accumulate the requested force terms at the current positions
  * (IntegratorUtil::force, src/Integrators/IntegratorUtil.cpp:15-30). */
 int mdx_forces_compute(mdx_ctx *ctx, int flags);

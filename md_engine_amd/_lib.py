@@ -24,7 +24,7 @@ SYMBOLS = [
     "mdx_nlist_stats", "mdx_lj_set", "mdx_special_set", "mdx_ewald_real_set", "mdx_pppm_set",
     "mdx_pppm_info", "mdx_e3b_set", "mdx_forces_compute", "mdx_forces_get", "mdx_energy_get",
     "mdx_integrator_set", "mdx_run", "mdx_teams_set", "mdx_sync", "mdx_force_steps",
-    "mdx_profile_enable", "mdx_profile_get", "mdx_nlist_totals", "mdx_option_set",
+    "mdx_profile_enable", "mdx_profile_get", "mdx_nlist_totals", "mdx_option_set", "mdx_rigid_set",
 ]
 
 # kernel ids of mdx_profile_get
@@ -73,6 +73,7 @@ def load():
     L.mdx_nlist_get.argtypes = [_vp, _vp, _vp, c, C.POINTER(c)]
     L.mdx_nlist_stats.argtypes = [_vp, C.POINTER(C.c_int64), C.POINTER(c), _ip, C.POINTER(C.c_int64)]
     L.mdx_option_set.argtypes = [_vp, C.c_char_p, c]
+    L.mdx_rigid_set.argtypes = [_vp, c, _ip, d, d, d, d, d]
     L.mdx_lj_set.argtypes = [_vp, c, _dp, _dp, _dp]
     L.mdx_special_set.argtypes = [_vp, d, d, d]
     L.mdx_ewald_real_set.argtypes = [_vp, c, d, d, d]
@@ -233,6 +234,11 @@ class Engine:
         p = np.ascontiguousarray(params, np.float64)
         self._chk(self._L.mdx_e3b_set(self._h, len(m) // 4, m, p, float(r_two_body), float(r_max)),
                   "mdx_e3b_set")
+
+    def rigid_set(self, mol_ids, m_O, m_H, d_OH, d_HH, gamma):
+        m = np.ascontiguousarray(mol_ids, np.int32).reshape(-1)
+        self._chk(self._L.mdx_rigid_set(self._h, len(m) // 4, m, float(m_O), float(m_H), float(d_OH),
+                                        float(d_HH), float(gamma)), "mdx_rigid_set")
 
     def compute(self, flags):
         self._chk(self._L.mdx_forces_compute(self._h, int(flags)), "mdx_forces_compute")

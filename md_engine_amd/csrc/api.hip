@@ -291,6 +291,10 @@ int mdx_system_set(mdx_ctx *ctx, int n, const double *x, const double *v, const 
         c.listValid = false;
         c.nlistConfigured = false;
         c.nRebuilds = 0;
+        if (c.rigid.on) {
+            rigid_map(c);
+            c.rigid.fresh = true;
+        }
         if (c.pppm.on) {
             double s = 0.0;
             for (int i = 0; i < n; ++i) s += (double)hq[i] * hq[i];
@@ -615,6 +619,55 @@ int mdx_e3b_set(mdx_ctx *ctx, int nmol, const int32_t *mol_ids, const double par
     });
 }
 
+int mdx_rigid_set(mdx_ctx *ctx, int nmol, const int32_t *mol_ids, double m_O, double m_H, double d_OH,
+                  double d_HH, double gamma) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        MDX_REQUIRE(nmol > 0 && mol_ids, MDX_ERR_ARG, "molecules required");
+        MDX_REQUIRE(m_O > 0 && m_H > 0 && d_OH > 0 && d_HH > 0 && d_HH < 2 * d_OH, MDX_ERR_ARG,
+                    "invalid rigid water geometry");
+        RigidState &R = c.rigid;
+        if (R.ids) {
+            dfree(c, R.ids);
+            dfree(c, R.atoms);
+            dfree(c, R.member);
+        }
+        std::vector<int4> ids(nmol);
+        std::vector<int> seen(c.n, 0);
+        for (int m = 0; m < nmol; ++m) {
+            const int *p = mol_ids + 4 * m;
+            for (int k = 0; k < 3; ++k) {
+                MDX_REQUIRE(p[k] >= 0 && p[k] < c.n, MDX_ERR_ARG, "rigid molecule id out of range");
+                MDX_REQUIRE(!seen[p[k]]++, MDX_ERR_ARG, "atom listed in two rigid molecules");
+            }
+            MDX_REQUIRE(p[3] < c.n, MDX_ERR_ARG, "M-site id out of range");
+            if (p[3] >= 0) MDX_REQUIRE(!seen[p[3]]++, MDX_ERR_ARG, "atom listed in two rigid molecules");
+            // masses ordered O > H1 == H2 > M, as FixRigid::createRigid demands (FixRigid.cu:1646-1658)
+            const double iO = c.invMassHost[p[0]], i1 = c.invMassHost[p[1]], i2 = c.invMassHost[p[2]];
+            MDX_REQUIRE(iO > 0 && i1 > 0 && i1 == i2 && iO < i1, MDX_ERR_ARG,
+                        "ids of a rigid water must be ordered O, H1, H2, M");
+            if (p[3] >= 0) MDX_REQUIRE(c.invMassHost[p[3]] == 0.0, MDX_ERR_ARG, "M site must be massless");
+            ids[m] = make_int4(p[0], p[1], p[2], p[3]);
+        }
+        R.on = true;
+        R.fresh = true;
+        R.nmol = nmol;
+        R.nsites = 0;
+        for (int m = 0; m < nmol; ++m) R.nsites += ids[m].w >= 0 ? 4 : 3;
+        R.mO = m_O;
+        R.mH = m_H;
+        R.dOH = d_OH;
+        R.dHH = d_HH;
+        R.gamma = gamma;
+        R.ids = dalloc<int4>(c, nmol);
+        R.atoms = dalloc<int4>(c, nmol);
+        R.member = dalloc<int>(c, c.n);
+        upload(c, R.ids, ids.data(), nmol);
+        rigid_map(c);
+        MDX_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
 int mdx_forces_compute(mdx_ctx *ctx, int flags) {
     return guarded(ctx, [&](Ctx &c) {
         require_system(c);
@@ -673,6 +726,12 @@ int mdx_run(mdx_ctx *ctx, int nsteps, int flags) {
         MDX_REQUIRE(nsteps >= 0, MDX_ERR_ARG, "nsteps must be >= 0");
         const int fflags = flags & (MDX_PAIR | MDX_PPPM | MDX_E3B);
         prepare_terms(c);
+        if (c.rigid.on && c.rigid.fresh) {
+            // constrain the starting velocities and place the M sites
+            rigid_second_half(c, false);
+            c.rigid.fresh = false;
+            c.listValid = false;
+        }
         constexpr int kChunk = 64;
         int done = 0;
         while (done < nsteps) {

@@ -18,9 +18,11 @@ __global__ void k_zero_u64(int n, unsigned long long *__restrict__ p) {
 __device__ __forceinline__ double fx_to_d(unsigned long long v) { return (double)(long long)v * kFxInv; }
 
 __global__ void k_vv1(int n, double4 *__restrict__ pos, double4 *__restrict__ vel,
-                      const unsigned long long *__restrict__ f, double hdtf, double dt) {
+                      const unsigned long long *__restrict__ f, double hdtf, double dt,
+                      const int *__restrict__ member) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (member && member[i]) return;  // moved by its rigid molecule
     double4 v = vel[i];
     if (v.w == 0.0) return;
     double4 x = pos[i];
@@ -36,9 +38,10 @@ __global__ void k_vv1(int n, double4 *__restrict__ pos, double4 *__restrict__ ve
 }
 
 __global__ void k_vv2(int n, double4 *__restrict__ vel, const unsigned long long *__restrict__ f,
-                      double hdtf) {
+                      double hdtf, const int *__restrict__ member) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (member && member[i]) return;
     double4 v = vel[i];
     if (v.w == 0.0) return;
     const double s = hdtf * v.w;
@@ -60,14 +63,20 @@ void zero_forces(Ctx &c, bool clear_acc) {
 
 void integrate_first_half(Ctx &c) {
     const int threads = 256;
-    hipLaunchKernelGGL(k_vv1, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[c.cur],
-                       c.vel[c.cur], c.force, 0.5 * c.dt * c.ftmToV, c.dt);
+    const int *member = c.rigid.on ? c.rigid.member : nullptr;
+    if (!c.rigid.on || c.rigid.nsites < c.n)
+        hipLaunchKernelGGL(k_vv1, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n,
+                           c.pos[c.cur], c.vel[c.cur], c.force, 0.5 * c.dt * c.ftmToV, c.dt, member);
+    if (c.rigid.on) rigid_first_half(c);
 }
 
 void integrate_second_half(Ctx &c) {
     const int threads = 256;
-    hipLaunchKernelGGL(k_vv2, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.vel[c.cur],
-                       c.force, 0.5 * c.dt * c.ftmToV);
+    const int *member = c.rigid.on ? c.rigid.member : nullptr;
+    if (!c.rigid.on || c.rigid.nsites < c.n)
+        hipLaunchKernelGGL(k_vv2, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n,
+                           c.vel[c.cur], c.force, 0.5 * c.dt * c.ftmToV, member);
+    if (c.rigid.on) rigid_second_half(c, true);
 }
 
 }  // namespace mdx

@@ -91,6 +91,17 @@ struct ProfRec {
     hipEvent_t a, b;
 };
 
+struct RigidState {
+    bool on = false;
+    bool fresh = false;       // velocities not yet constrained (start of the next run)
+    int nmol = 0;
+    int nsites = 0;           // sites in rigid molecules (3 or 4 per molecule)
+    int4 *ids = nullptr;      // by molecule: O, H1, H2, M ids (M = -1: three sites)
+    int4 *atoms = nullptr;    // by molecule: current indices
+    int *member = nullptr;    // by current index: 1 for sites of a rigid molecule
+    double mO = 0, mH = 0, dOH = 0, dHH = 0, gamma = 0;
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -159,6 +170,7 @@ struct Ctx {
 
     PppmState pppm;
     E3BState e3b;
+    RigidState rigid;
     bool mdxDisableTiledSpread = false;   // test knob: force the plain global-atomic spread
 
     // integration
@@ -270,6 +282,11 @@ double pppm_solve_alpha(int natoms, double totalQ2, const double *L, const int *
 
 // e3b.hip
 void e3b_forces(Ctx &c, bool energy, bool virial);
+
+// rigid.hip
+void rigid_map(Ctx &c);
+void rigid_first_half(Ctx &c);
+void rigid_second_half(Ctx &c, bool kick);
 
 // integrate.hip
 void integrate_first_half(Ctx &c);

@@ -535,6 +535,7 @@ static void run_build(Ctx &c, bool sync) {
                        c.vel[nxt], c.q[nxt], c.type[nxt], c.id[nxt], c.idToIdx, c.posLast, c.xq,
                        c.geo);
     c.cur = nxt;
+    rigid_map(c);
     if (c.e3b.on) {
         fill_int(c, c.e3b.atomMolO, c.n, -1);  // indices were permuted: drop the old map
         hipLaunchKernelGGL(k_mol_atoms, dim3(cdiv(c.e3b.nmol, threads)), dim3(threads), 0, c.stream,

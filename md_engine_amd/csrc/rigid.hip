@@ -1,0 +1,258 @@
+// Rigid water in FP64: SETTLE positions, RATTLE velocities, M-site placement
+// and force distribution.  Replaces FixRigid's settlePositions
+// (src/Fixes/FixRigid.cu:850-1136), settleVelocities (658-849), setMSite
+// (560-599) and distributeMSite (475-558) as sequenced by postNVE_X and
+// preStepFinal (1234-1279, 1916-1984).
+//
+// Design (gfx950): one thread per molecule does the whole half step of its
+// four sites, so the velocity-Verlet kick, the drift, the constraint solve and
+// the M placement are one pass over the molecule's data (no separate
+// force-distribution or velocity-correction kernels).  The M-site force is
+// distributed inside the kernel (F_O += gamma F_M, F_H += (1 - gamma)/2 F_M)
+// from the raw fixed-point force buffer, which is left untouched.
+// SETTLE follows Miyamoto & Kollman, J. Comput. Chem. 13, 952 (1992); the
+// velocity constraint is the exact 3x3 solve of the three bond conditions.
+#include "mdx_internal.h"
+
+namespace mdx {
+
+namespace {
+
+struct d3 {
+    double x, y, z;
+};
+__device__ __forceinline__ d3 mk(double x, double y, double z) { return {x, y, z}; }
+__device__ __forceinline__ d3 operator+(d3 a, d3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ d3 operator-(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ d3 operator*(double s, d3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ double dot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ d3 cross(d3 a, d3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ d3 unit(d3 a) { return (1.0 / sqrt(dot(a, a))) * a; }
+__device__ __forceinline__ d3 xyz(double4 a) { return {a.x, a.y, a.z}; }
+__device__ __forceinline__ double fxd(unsigned long long v) { return (double)(long long)v * kFxInv; }
+__device__ __forceinline__ d3 force_of(const unsigned long long *f, int i) {
+    return {fxd(f[3 * i]), fxd(f[3 * i + 1]), fxd(f[3 * i + 2])};
+}
+__device__ __forceinline__ d3 mi(d3 d, const Geometry &g) {
+    if (g.periodic[0]) d.x -= g.L[0] * rint(d.x * g.invL[0]);
+    if (g.periodic[1]) d.y -= g.L[1] * rint(d.y * g.invL[1]);
+    if (g.periodic[2]) d.z -= g.L[2] * rint(d.z * g.invL[2]);
+    return d;
+}
+
+struct RigidArgs {
+    int nmol;
+    const int4 *atoms;
+    double mO, mH, dOH, dHH, gamma;
+    double dt, hdtf;
+    Geometry g;
+};
+
+// SETTLE: old positions relative to the old oxygen (A0 = 0, B0, C0) and
+// unconstrained new positions A1, B1, C1 -> constrained positions in place.
+__device__ void settle(d3 B0, d3 C0, d3 &A1, d3 &B1, d3 &C1, const RigidArgs &r) {
+    const double mT = r.mO + 2.0 * r.mH, wO = r.mO / mT, wH = r.mH / mT;
+    const double rc = 0.5 * r.dHH;
+    const double dmid = sqrt(r.dOH * r.dOH - rc * rc);
+    const double ra = 2.0 * wH * dmid, rb = dmid - ra;
+    const d3 com = wO * A1 + wH * (B1 + C1);
+    const d3 a1 = A1 - com, b1 = B1 - com, c1 = C1 - com;
+    const d3 n0 = cross(B0, C0);
+    const d3 xa = cross(a1, n0);
+    const d3 ya = cross(n0, xa);
+    const d3 ez = unit(n0), ex = unit(xa), ey = unit(ya);
+    const double xb0 = dot(B0, ex), yb0 = dot(B0, ey), xc0 = dot(C0, ex), yc0 = dot(C0, ey);
+    const double za1 = dot(a1, ez);
+    const double xb1 = dot(b1, ex), yb1 = dot(b1, ey), zb1 = dot(b1, ez);
+    const double xc1 = dot(c1, ex), yc1 = dot(c1, ey), zc1 = dot(c1, ez);
+    const double sinphi = za1 / ra, cosphi = sqrt(1.0 - sinphi * sinphi);
+    const double sinpsi = (zb1 - zc1) / (2.0 * rc * cosphi), cospsi = sqrt(1.0 - sinpsi * sinpsi);
+    const double ya2 = ra * cosphi, xb2 = -rc * cospsi;
+    const double t1 = -rb * cosphi, t2 = rc * sinpsi * sinphi;
+    const double yb2 = t1 - t2, yc2 = t1 + t2;
+    const double alpha = xb2 * (xb0 - xc0) + yb0 * yb2 + yc0 * yc2;
+    const double beta = xb2 * (yc0 - yb0) + xb0 * yb2 + xc0 * yc2;
+    const double gam = xb0 * yb1 - xb1 * yb0 + xc0 * yc1 - xc1 * yc0;
+    const double ab2 = alpha * alpha + beta * beta;
+    const double sinth = (alpha * gam - beta * sqrt(ab2 - gam * gam)) / ab2;
+    const double costh = sqrt(1.0 - sinth * sinth);
+    A1 = com + (-ya2 * sinth) * ex + (ya2 * costh) * ey + za1 * ez;
+    B1 = com + (xb2 * costh - yb2 * sinth) * ex + (xb2 * sinth + yb2 * costh) * ey + zb1 * ez;
+    C1 = com + (-xb2 * costh - yc2 * sinth) * ex + (-xb2 * sinth + yc2 * costh) * ey + zc1 * ez;
+}
+
+__device__ __forceinline__ double det3(const double M[3][3]) {
+    return M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) -
+           M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+           M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+}
+
+// Velocity constraint: impulses along the three bonds making every
+// (v_i - v_j).(r_i - r_j) zero; rO/rH1/rH2 relative positions.
+__device__ void rattle(d3 rO, d3 rH1, d3 rH2, d3 &vO, d3 &vH1, d3 &vH2, double mO, double mH) {
+    const d3 r[3] = {rO, rH1, rH2};
+    d3 v[3] = {vO, vH1, vH2};
+    const double im[3] = {1.0 / mO, 1.0 / mH, 1.0 / mH};
+    const int bi[3] = {0, 0, 1}, bj[3] = {1, 2, 2};
+    d3 e[3];
+    double M[3][3], rhs[3];
+    for (int k = 0; k < 3; ++k) e[k] = r[bi[k]] - r[bj[k]];
+    for (int k = 0; k < 3; ++k) {
+        rhs[k] = -dot(v[bi[k]] - v[bj[k]], e[k]);
+        for (int l = 0; l < 3; ++l) {
+            const double si = (bi[l] == bi[k]) ? 1.0 : (bj[l] == bi[k] ? -1.0 : 0.0);
+            const double sj = (bi[l] == bj[k]) ? 1.0 : (bj[l] == bj[k] ? -1.0 : 0.0);
+            M[k][l] = dot(e[k], e[l]) * (si * im[bi[k]] - sj * im[bj[k]]);
+        }
+    }
+    const double det = det3(M);
+    double lam[3];
+    for (int c = 0; c < 3; ++c) {
+        double A[3][3];
+        for (int k = 0; k < 3; ++k)
+            for (int l = 0; l < 3; ++l) A[k][l] = (l == c) ? rhs[k] : M[k][l];
+        lam[c] = det3(A) / det;
+    }
+    for (int k = 0; k < 3; ++k) {
+        v[bi[k]] = v[bi[k]] + (lam[k] * im[bi[k]]) * e[k];
+        v[bj[k]] = v[bj[k]] - (lam[k] * im[bj[k]]) * e[k];
+    }
+    vO = v[0];
+    vH1 = v[1];
+    vH2 = v[2];
+}
+
+__device__ __forceinline__ void dist_forces(const unsigned long long *f, int4 a, double gamma, d3 &fO,
+                                            d3 &fH1, d3 &fH2) {
+    fO = force_of(f, a.x);
+    fH1 = force_of(f, a.y);
+    fH2 = force_of(f, a.z);
+    if (a.w >= 0) {
+        const d3 fM = force_of(f, a.w);
+        fO = fO + gamma * fM;
+        const double h = 0.5 * (1.0 - gamma);
+        fH1 = fH1 + h * fM;
+        fH2 = fH2 + h * fM;
+    }
+}
+
+__global__ void k_rigid_vv1(RigidArgs r, double4 *__restrict__ pos, double4 *__restrict__ vel,
+                            const unsigned long long *__restrict__ f) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= r.nmol) return;
+    const int4 a = r.atoms[m];
+    d3 fO, fH1, fH2;
+    dist_forces(f, a, r.gamma, fO, fH1, fH2);
+    const double4 pO = pos[a.x], pH1 = pos[a.y], pH2 = pos[a.z];
+    const double4 wO = vel[a.x], wH1 = vel[a.y], wH2 = vel[a.z];
+    d3 vO = xyz(wO) + (r.hdtf / r.mO) * fO;
+    d3 vH1 = xyz(wH1) + (r.hdtf / r.mH) * fH1;
+    d3 vH2 = xyz(wH2) + (r.hdtf / r.mH) * fH2;
+    const d3 xO = xyz(pO);
+    const d3 B0 = mi(xyz(pH1) - xO, r.g), C0 = mi(xyz(pH2) - xO, r.g);
+    const d3 A1 = r.dt * vO, B1 = B0 + r.dt * vH1, C1 = C0 + r.dt * vH2;
+    d3 A3 = A1, B3 = B1, C3 = C1;
+    settle(B0, C0, A3, B3, C3, r);
+    const double idt = 1.0 / r.dt;
+    vO = vO + idt * (A3 - A1);
+    vH1 = vH1 + idt * (B3 - B1);
+    vH2 = vH2 + idt * (C3 - C1);
+    const d3 nO = xO + A3;
+    const d3 nH1 = nO + (B3 - A3), nH2 = nO + (C3 - A3);
+    pos[a.x] = make_double4(nO.x, nO.y, nO.z, pO.w);
+    pos[a.y] = make_double4(nH1.x, nH1.y, nH1.z, pH1.w);
+    pos[a.z] = make_double4(nH2.x, nH2.y, nH2.z, pH2.w);
+    vel[a.x] = make_double4(vO.x, vO.y, vO.z, wO.w);
+    vel[a.y] = make_double4(vH1.x, vH1.y, vH1.z, wH1.w);
+    vel[a.z] = make_double4(vH2.x, vH2.y, vH2.z, wH2.w);
+    if (a.w >= 0) {
+        const d3 nM = nO + (0.5 * (1.0 - r.gamma)) * ((B3 - A3) + (C3 - A3));
+        const double4 pM = pos[a.w];
+        pos[a.w] = make_double4(nM.x, nM.y, nM.z, pM.w);
+        vel[a.w] = make_double4(0.0, 0.0, 0.0, vel[a.w].w);
+    }
+}
+
+// kick = false: only the velocity constraint and the M placement (start of a run)
+__global__ void k_rigid_vv2(RigidArgs r, double4 *__restrict__ pos, double4 *__restrict__ vel,
+                            const unsigned long long *__restrict__ f, int kick) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= r.nmol) return;
+    const int4 a = r.atoms[m];
+    const double4 wO = vel[a.x], wH1 = vel[a.y], wH2 = vel[a.z];
+    d3 vO = xyz(wO), vH1 = xyz(wH1), vH2 = xyz(wH2);
+    if (kick) {
+        d3 fO, fH1, fH2;
+        dist_forces(f, a, r.gamma, fO, fH1, fH2);
+        vO = vO + (r.hdtf / r.mO) * fO;
+        vH1 = vH1 + (r.hdtf / r.mH) * fH1;
+        vH2 = vH2 + (r.hdtf / r.mH) * fH2;
+    }
+    const d3 xO = xyz(pos[a.x]);
+    const d3 rH1 = mi(xyz(pos[a.y]) - xO, r.g), rH2 = mi(xyz(pos[a.z]) - xO, r.g);
+    rattle(mk(0, 0, 0), rH1, rH2, vO, vH1, vH2, r.mO, r.mH);
+    vel[a.x] = make_double4(vO.x, vO.y, vO.z, wO.w);
+    vel[a.y] = make_double4(vH1.x, vH1.y, vH1.z, wH1.w);
+    vel[a.z] = make_double4(vH2.x, vH2.y, vH2.z, wH2.w);
+    if (!kick && a.w >= 0) {
+        const d3 nM = xO + (0.5 * (1.0 - r.gamma)) * (rH1 + rH2);
+        const double4 pM = pos[a.w];
+        pos[a.w] = make_double4(nM.x, nM.y, nM.z, pM.w);
+        vel[a.w] = make_double4(0.0, 0.0, 0.0, vel[a.w].w);
+    }
+}
+
+// molecule -> current site indices, and the per-site membership flag that
+// keeps the plain integrator off these sites
+__global__ void k_rigid_map(int nmol, const int4 *__restrict__ ids, const int *__restrict__ idToIdx,
+                            int4 *__restrict__ atoms, int *__restrict__ member) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmol) return;
+    const int4 i = ids[m];
+    const int4 a = make_int4(idToIdx[i.x], idToIdx[i.y], idToIdx[i.z], i.w >= 0 ? idToIdx[i.w] : -1);
+    atoms[m] = a;
+    member[a.x] = 1;
+    member[a.y] = 1;
+    member[a.z] = 1;
+    if (a.w >= 0) member[a.w] = 1;
+}
+
+RigidArgs make_args(const Ctx &c) {
+    RigidArgs r;
+    r.nmol = c.rigid.nmol;
+    r.atoms = c.rigid.atoms;
+    r.mO = c.rigid.mO;
+    r.mH = c.rigid.mH;
+    r.dOH = c.rigid.dOH;
+    r.dHH = c.rigid.dHH;
+    r.gamma = c.rigid.gamma;
+    r.dt = c.dt;
+    r.hdtf = 0.5 * c.dt * c.ftmToV;
+    r.g = c.geo;
+    return r;
+}
+
+}  // namespace
+
+void rigid_map(Ctx &c) {
+    if (!c.rigid.on) return;
+    fill_int(c, c.rigid.member, c.n, 0);
+    hipLaunchKernelGGL(k_rigid_map, dim3(cdiv(c.rigid.nmol, 128)), dim3(128), 0, c.stream, c.rigid.nmol,
+                       c.rigid.ids, c.idToIdx, c.rigid.atoms, c.rigid.member);
+}
+
+void rigid_first_half(Ctx &c) {
+    RigidArgs r = make_args(c);
+    hipLaunchKernelGGL(k_rigid_vv1, dim3(cdiv(r.nmol, 128)), dim3(128), 0, c.stream, r, c.pos[c.cur],
+                       c.vel[c.cur], c.force);
+}
+
+void rigid_second_half(Ctx &c, bool kick) {
+    RigidArgs r = make_args(c);
+    hipLaunchKernelGGL(k_rigid_vv2, dim3(cdiv(r.nmol, 128)), dim3(128), 0, c.stream, r, c.pos[c.cur],
+                       c.vel[c.cur], c.force, kick ? 1 : 0);
+}
+
+}  // namespace mdx

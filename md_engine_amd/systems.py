@@ -27,6 +27,9 @@ REAL_FTM_TO_V = 1.0 / (48.88821291 * 48.88821291)
 TIP4P2005 = dict(
     q_H=0.5564, q_M=-2.0 * 0.5564, r_OM=0.1546, r_OH=0.9572, theta=1.824218134,
     eps_OO=0.185207422, sig_OO=3.1589, m_O=15.9994, m_H=1.008, m_M=0.0,
+    # rigid geometry kept by SETTLE and the M-site weight, as FixRigid sets
+    # them for TIP4P/2005 (src/Fixes/FixRigid.cu:1372-1379)
+    r_HH=1.5139, gamma=0.73612446364836,
 )
 
 WATER_DENSITY_G_CM3 = 0.997
@@ -91,12 +94,16 @@ def _random_rotations(rng, n):
 
 
 def tip4p2005_frame():
-    """Body-frame site offsets (O, H1, H2, M) relative to O."""
+    """Body-frame site offsets (O, H1, H2, M) relative to O: |OH| = r_OH,
+    |HH| = r_HH exactly (the constraint SETTLE keeps), M = O + (1 - gamma)/2
+    (r_OH1 + r_OH2), which puts M 0.1546 A along the bisector."""
     p = TIP4P2005
-    h1 = np.array([p["r_OH"], 0.0, 0.0])
-    h2 = np.array([np.cos(p["theta"]) * p["r_OH"], np.sin(p["theta"]) * p["r_OH"], 0.0])
-    bis = (h1 + h2) / np.linalg.norm(h1 + h2)
-    return np.stack([np.zeros(3), h1, h2, bis * p["r_OM"]])
+    half = 0.5 * p["r_HH"]
+    mid = np.sqrt(p["r_OH"] ** 2 - half ** 2)
+    h1 = np.array([half, mid, 0.0])
+    h2 = np.array([-half, mid, 0.0])
+    m = 0.5 * (1.0 - p["gamma"]) * (h1 + h2)
+    return np.stack([np.zeros(3), h1, h2, m])
 
 
 def tip4p2005_box(nmol, L=None, seed=20261202, min_sep=1.5):

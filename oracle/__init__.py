@@ -385,3 +385,114 @@ def lj_verlet(x, v, mass, lo, L, eps, sigma, rc, skin, dt, nsteps, record_energy
     nb = lib().orc_lj_verlet(len(x), x, v, float(mass), _d(lo), _d(L), float(eps), float(sigma),
                              float(rc), float(skin), float(dt), int(nsteps), _ptr(et))
     return x, v, nb, et
+
+
+# --------------------------------------------------------------------------
+# rigid water (SETTLE / RATTLE / M site) and an FP64 water MD driver
+# --------------------------------------------------------------------------
+
+_rigid_declared = False
+
+
+def _declare_rigid():
+    global _rigid_declared
+    if _rigid_declared:
+        return
+    L = lib()
+    L.orc_settle_one.argtypes = [_dp, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double]
+    L.orc_shake_one.argtypes = [_dp, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double,
+                                C.c_double, C.c_int]
+    L.orc_rattle_one.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_double]
+    L.orc_water_step_half.argtypes = [C.c_int, C.c_int, _ip, _dp, _dp, _dp, C.c_double, C.c_double,
+                                      C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _ip]
+    _rigid_declared = True
+
+
+def settle_one(B0, C0, A1, B1, C1, mO, mH, dOH, dHH):
+    """SETTLE for one molecule (FixRigid.cu:850-1136), positions relative to the old O."""
+    _declare_rigid()
+    a, b, c = (_d(v).copy() for v in (A1, B1, C1))
+    lib().orc_settle_one(_d(B0), _d(C0), a, b, c, mO, mH, dOH, dHH)
+    return a, b, c
+
+
+def shake_one(B0, C0, A1, B1, C1, mO, mH, dOH, dHH, tol=1e-15, maxit=1000):
+    """Iterative SHAKE on the same constraints (pins SETTLE)."""
+    _declare_rigid()
+    a, b, c = (_d(v).copy() for v in (A1, B1, C1))
+    lib().orc_shake_one(_d(B0), _d(C0), a, b, c, mO, mH, dOH, dHH, tol, maxit)
+    return a, b, c
+
+
+def rattle_one(rO, rH1, rH2, vO, vH1, vH2, mO, mH):
+    """Velocity constraint of a rigid water (FixRigid.cu:658-849)."""
+    _declare_rigid()
+    a, b, c = (_d(v).copy() for v in (vO, vH1, vH2))
+    lib().orc_rattle_one(_d(rO), _d(rH1), _d(rH2), a, b, c, mO, mH)
+    return a, b, c
+
+
+WATER_GEOM = dict(mO=15.9994, mH=1.008, dOH=0.9572, dHH=1.5139, gamma=0.73612446364836)
+
+
+def water_step_half(phase, mol, x, v, f, dt, ftm, L, geom=WATER_GEOM, periodic=(1, 1, 1)):
+    """In-place half step of rigid-water velocity Verlet (phase 1: kick, drift,
+    SETTLE, M site; phase 2: kick, RATTLE).  f: raw per-site forces."""
+    _declare_rigid()
+    lib().orc_water_step_half(int(phase), len(mol), _i(mol).ravel(), x, v, _d(f).reshape(-1),
+                              geom["mO"], geom["mH"], geom["dOH"], geom["dHH"], geom["gamma"],
+                              float(dt), float(ftm), _d(L), _i(periodic))
+
+
+def water_forces(box, x, rc, skin, sz, order, e3b_on, alpha=None):
+    """Raw per-site forces and energies of rigid water (LJ O-O + Ewald real
+    space + P3M + optional E3B3) at positions x, FP64."""
+    from md_engine_amd import systems  # input tables only
+
+    n = len(x)
+    L, lo = box["L"], box["lo"]
+    qqr = systems.REAL_QQR_TO_ENG
+    q2 = qqr * (box["q"] ** 2).sum()
+    if alpha is None:
+        alpha, _ = pppm_alpha(n, q2, L, np.array(sz), order, rc)
+    xw = wrap(x, lo, L)
+    counts, tab = nlist_cell(xw, lo, L, rc + skin)
+    rows = [tab[i, : counts[i]] for i in range(n)]
+    excl = exclusions_bfs(n, systems.water_bonds(box["mol"]))
+    rows = tag_rows(rows, np.arange(n), excl)
+    eps24, sig6 = systems.water_lj_tables()
+    fp, ep, _ = pair(xw, box["q"], box["type"], L, counts, rows, np.full((3, 3), rc * rc), eps24, sig6,
+                     with_charge=True, alpha=alpha, qqr=qqr, rq=rc)
+    G = pppm_green(L, np.array(sz), alpha, order)
+    k = pppm(x, box["q"], lo, L, np.array(sz), order, qqr, alpha, q2, G, energy=True)
+    f = fp + k["f"]
+    e = ep.sum() + k["energy"]
+    if e3b_on:
+        fe, e2, e3 = e3b(box["mol"], x, L)
+        f = f + fe
+        e += e2 + e3
+    return f, e, alpha
+
+
+def water_kinetic(box, v):
+    from md_engine_amd import systems
+
+    return 0.5 * (box["mass"][:, None] * v * v).sum() * systems.REAL_MVV_TO_ENG
+
+
+def water_md(box, v, nsteps, dt=2.0, rc=6.5, skin=0.5, sz=(32, 32, 32), order=5, e3b_on=False):
+    """FP64 rigid-water NVE (velocity Verlet + SETTLE + RATTLE), forces
+    rebuilt from a fresh list every step.  Returns (x, v, total energies)."""
+    from md_engine_amd import systems
+
+    x = _d(box["x"]).copy()
+    v = _d(v).copy()
+    ftm = systems.REAL_FTM_TO_V
+    f, e, alpha = water_forces(box, x, rc, skin, sz, order, e3b_on)
+    et = [e + water_kinetic(box, v)]
+    for _ in range(nsteps):
+        water_step_half(1, box["mol"], x, v, f, dt, ftm, box["L"])
+        f, e, _ = water_forces(box, x, rc, skin, sz, order, e3b_on, alpha=alpha)
+        water_step_half(2, box["mol"], x, v, f, dt, ftm, box["L"])
+        et.append(e + water_kinetic(box, v))
+    return x, v, np.array(et)

@@ -1022,3 +1022,213 @@ int orc_num_threads(void) {
     return 1;
 #endif
 }
+
+/* ------------------------------------------------------------------------ */
+/* Rigid water: SETTLE, RATTLE velocities, M site                            */
+/* ------------------------------------------------------------------------ */
+
+static inline void v3sub(const double *a, const double *b, double *o) {
+    o[0] = a[0] - b[0];
+    o[1] = a[1] - b[1];
+    o[2] = a[2] - b[2];
+}
+static inline double v3dot(const double *a, const double *b) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+static inline void v3cross(const double *a, const double *b, double *o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+static inline void v3norm(double *a) {
+    double s = 1.0 / sqrt(v3dot(a, a));
+    a[0] *= s;
+    a[1] *= s;
+    a[2] *= s;
+}
+
+/* SETTLE for one water (Miyamoto & Kollman, J. Comput. Chem. 13, 952 (1992)),
+ * the algorithm of settlePositions (src/Fixes/FixRigid.cu:850-1136).  Inputs
+ * relative to the old oxygen: old positions A0 = 0, B0, C0 (constrained) and
+ * unconstrained new positions A1, B1, C1; outputs the constrained new
+ * positions in A1, B1, C1.  mO, mH masses, dOH, dHH distances. */
+void orc_settle_one(const double *B0, const double *C0, double *A1, double *B1, double *C1,
+                    double mO, double mH, double dOH, double dHH) {
+    double mT = mO + 2.0 * mH, wO = mO / mT, wH = mH / mT;
+    double rc = 0.5 * dHH;
+    double dmid = sqrt(dOH * dOH - rc * rc);
+    double ra = 2.0 * wH * dmid, rb = dmid - ra;
+    double com[3], a1[3], b1[3], c1[3];
+    for (int k = 0; k < 3; ++k) com[k] = wO * A1[k] + wH * (B1[k] + C1[k]);
+    v3sub(A1, com, a1);
+    v3sub(B1, com, b1);
+    v3sub(C1, com, c1);
+    double ez[3], ex[3], ey[3];
+    v3cross(B0, C0, ez);
+    v3cross(a1, ez, ex);
+    v3cross(ez, ex, ey);
+    v3norm(ez);
+    v3norm(ex);
+    v3norm(ey);
+    double xb0 = v3dot(B0, ex), yb0 = v3dot(B0, ey), xc0 = v3dot(C0, ex), yc0 = v3dot(C0, ey);
+    double za1 = v3dot(a1, ez);
+    double xb1 = v3dot(b1, ex), yb1 = v3dot(b1, ey), zb1 = v3dot(b1, ez);
+    double xc1 = v3dot(c1, ex), yc1 = v3dot(c1, ey), zc1 = v3dot(c1, ez);
+    double sinphi = za1 / ra, cosphi = sqrt(1.0 - sinphi * sinphi);
+    double sinpsi = (zb1 - zc1) / (2.0 * rc * cosphi), cospsi = sqrt(1.0 - sinpsi * sinpsi);
+    double ya2 = ra * cosphi, xb2 = -rc * cospsi;
+    double t1 = -rb * cosphi, t2 = rc * sinpsi * sinphi;
+    double yb2 = t1 - t2, yc2 = t1 + t2;
+    double alpha = xb2 * (xb0 - xc0) + yb0 * yb2 + yc0 * yc2;
+    double beta = xb2 * (yc0 - yb0) + xb0 * yb2 + xc0 * yc2;
+    double gamma = xb0 * yb1 - xb1 * yb0 + xc0 * yc1 - xc1 * yc0;
+    double ab2 = alpha * alpha + beta * beta;
+    double sinth = (alpha * gamma - beta * sqrt(ab2 - gamma * gamma)) / ab2;
+    double costh = sqrt(1.0 - sinth * sinth);
+    double a3[3] = {-ya2 * sinth, ya2 * costh, za1};
+    double b3[3] = {xb2 * costh - yb2 * sinth, xb2 * sinth + yb2 * costh, zb1};
+    double c3[3] = {-xb2 * costh - yc2 * sinth, -xb2 * sinth + yc2 * costh, zc1};
+    for (int k = 0; k < 3; ++k) {
+        A1[k] = com[k] + a3[0] * ex[k] + a3[1] * ey[k] + a3[2] * ez[k];
+        B1[k] = com[k] + b3[0] * ex[k] + b3[1] * ey[k] + b3[2] * ez[k];
+        C1[k] = com[k] + c3[0] * ex[k] + c3[1] * ey[k] + c3[2] * ez[k];
+    }
+}
+
+/* Iterative SHAKE (Ryckaert, Ciccotti & Berendsen 1977) on the same three
+ * distance constraints, corrections along the old bond vectors; converged to
+ * tol.  Independent of SETTLE; used to pin it. */
+void orc_shake_one(const double *B0, const double *C0, double *A1, double *B1, double *C1,
+                   double mO, double mH, double dOH, double dHH, double tol, int maxit) {
+    double *p[3] = {A1, B1, C1};
+    double o[3][3] = {{0, 0, 0}, {B0[0], B0[1], B0[2]}, {C0[0], C0[1], C0[2]}};
+    double m[3] = {mO, mH, mH};
+    int bi[3] = {0, 0, 1}, bj[3] = {1, 2, 2};
+    double d[3] = {dOH, dOH, dHH};
+    for (int it = 0; it < maxit; ++it) {
+        double worst = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            int i = bi[k], j = bj[k];
+            double r[3], r0[3];
+            v3sub(p[i], p[j], r);
+            v3sub(o[i], o[j], r0);
+            double diff = d[k] * d[k] - v3dot(r, r);
+            if (fabs(diff) / (d[k] * d[k]) > worst) worst = fabs(diff) / (d[k] * d[k]);
+            double g = diff / (2.0 * v3dot(r, r0) * (1.0 / m[i] + 1.0 / m[j]));
+            for (int a = 0; a < 3; ++a) {
+                p[i][a] += g * r0[a] / m[i];
+                p[j][a] -= g * r0[a] / m[j];
+            }
+        }
+        if (worst < tol) break;
+    }
+}
+
+/* Velocity constraint for a rigid water (settleVelocities,
+ * src/Fixes/FixRigid.cu:658-849): the three bond-relative velocities
+ * (v_i - v_j).(r_i - r_j) are made zero by momentum-conserving impulses along
+ * the bonds, solved exactly (3x3, Cramer).  r relative positions O, H1, H2
+ * (any origin), v in/out. */
+void orc_rattle_one(const double *rO, const double *rH1, const double *rH2, double *vO,
+                    double *vH1, double *vH2, double mO, double mH) {
+    const double *r[3] = {rO, rH1, rH2};
+    double *v[3] = {vO, vH1, vH2};
+    double im[3] = {1.0 / mO, 1.0 / mH, 1.0 / mH};
+    int bi[3] = {0, 0, 1}, bj[3] = {1, 2, 2};
+    double e[3][3], M[3][3], rhs[3];
+    for (int k = 0; k < 3; ++k) v3sub(r[bi[k]], r[bj[k]], e[k]);
+    for (int k = 0; k < 3; ++k) {
+        double dv[3];
+        v3sub(v[bi[k]], v[bj[k]], dv);
+        rhs[k] = -v3dot(dv, e[k]);
+        for (int l = 0; l < 3; ++l) {
+            double si = (bi[l] == bi[k]) ? 1.0 : (bj[l] == bi[k] ? -1.0 : 0.0);
+            double sj = (bi[l] == bj[k]) ? 1.0 : (bj[l] == bj[k] ? -1.0 : 0.0);
+            M[k][l] = v3dot(e[k], e[l]) * (si * im[bi[k]] - sj * im[bj[k]]);
+        }
+    }
+    double det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) -
+                 M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+                 M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+    double lam[3];
+    for (int c = 0; c < 3; ++c) {
+        double A[3][3];
+        for (int k = 0; k < 3; ++k)
+            for (int l = 0; l < 3; ++l) A[k][l] = (l == c) ? rhs[k] : M[k][l];
+        lam[c] = (A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) -
+                  A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+                  A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0])) / det;
+    }
+    for (int k = 0; k < 3; ++k)
+        for (int a = 0; a < 3; ++a) {
+            v[bi[k]][a] += lam[k] * e[k][a] * im[bi[k]];
+            v[bj[k]][a] -= lam[k] * e[k][a] * im[bj[k]];
+        }
+}
+
+/* Rigid-water velocity Verlet step parts on whole arrays (FP64), the
+ * sequence of IntegratorVerlet::run with FixRigid (postNVE_X settles positions
+ * and places M, FixRigid.cu:1234-1279; preStepFinal distributes the M force,
+ * settles velocities and places M, FixRigid.cu:1916-1984).  mol: (O, H1, H2,
+ * M) indices, M = -1 for three-site water.  gamma: M = gamma O + (1-gamma)/2
+ * (H1 + H2).  Forces f are the raw per-site forces; the M force is
+ * distributed inside.  phase 1: kick + drift + SETTLE + velocity correction +
+ * M; phase 2: kick + RATTLE. */
+static void orc_fdist(const double *f, const int *m, double gamma, double fO[3], double fH1[3],
+                      double fH2[3]) {
+    for (int a = 0; a < 3; ++a) {
+        double fm = m[3] >= 0 ? f[3 * m[3] + a] : 0.0;
+        fO[a] = f[3 * m[0] + a] + gamma * fm;
+        fH1[a] = f[3 * m[1] + a] + 0.5 * (1.0 - gamma) * fm;
+        fH2[a] = f[3 * m[2] + a] + 0.5 * (1.0 - gamma) * fm;
+    }
+}
+
+void orc_water_step_half(int phase, int nmol, const int *mol, double *x, double *v,
+                         const double *f, double mO, double mH, double dOH, double dHH,
+                         double gamma, double dt, double ftm, const double *L, const int *per) {
+    double invL[3] = {1.0 / L[0], 1.0 / L[1], 1.0 / L[2]};
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nmol; ++i) {
+        const int *m = mol + 4 * i;
+        double fO[3], fH1[3], fH2[3];
+        orc_fdist(f, m, gamma, fO, fH1, fH2);
+        double *F[3] = {fO, fH1, fH2};
+        double ms[3] = {mO, mH, mH};
+        for (int s = 0; s < 3; ++s)
+            for (int a = 0; a < 3; ++a) v[3 * m[s] + a] += 0.5 * dt * ftm * F[s][a] / ms[s];
+        double rO[3], rH1[3], rH2[3];
+        for (int a = 0; a < 3; ++a) {
+            rO[a] = 0.0;
+            rH1[a] = orc_mi(x[3 * m[1] + a] - x[3 * m[0] + a], L[a], invL[a], per[a]);
+            rH2[a] = orc_mi(x[3 * m[2] + a] - x[3 * m[0] + a], L[a], invL[a], per[a]);
+        }
+        if (phase == 1) {
+            double A1[3], B1[3], C1[3];
+            for (int a = 0; a < 3; ++a) {
+                A1[a] = dt * v[3 * m[0] + a];
+                B1[a] = rH1[a] + dt * v[3 * m[1] + a];
+                C1[a] = rH2[a] + dt * v[3 * m[2] + a];
+            }
+            double A3[3] = {A1[0], A1[1], A1[2]}, B3[3] = {B1[0], B1[1], B1[2]}, C3[3] = {C1[0], C1[1], C1[2]};
+            orc_settle_one(rH1, rH2, A3, B3, C3, mO, mH, dOH, dHH);
+            double xO[3];
+            for (int a = 0; a < 3; ++a) {
+                v[3 * m[0] + a] += (A3[a] - A1[a]) / dt;
+                v[3 * m[1] + a] += (B3[a] - B1[a]) / dt;
+                v[3 * m[2] + a] += (C3[a] - C1[a]) / dt;
+                xO[a] = x[3 * m[0] + a] + A3[a];
+            }
+            for (int a = 0; a < 3; ++a) {
+                x[3 * m[0] + a] = xO[a];
+                x[3 * m[1] + a] = xO[a] + (B3[a] - A3[a]);
+                x[3 * m[2] + a] = xO[a] + (C3[a] - A3[a]);
+                if (m[3] >= 0)
+                    x[3 * m[3] + a] = xO[a] + 0.5 * (1.0 - gamma) * ((B3[a] - A3[a]) + (C3[a] - A3[a]));
+            }
+        } else {
+            orc_rattle_one(rO, rH1, rH2, v + 3 * m[0], v + 3 * m[1], v + 3 * m[2], mO, mH);
+        }
+        if (m[3] >= 0) v[3 * m[3]] = v[3 * m[3] + 1] = v[3 * m[3] + 2] = 0.0;
+    }
+}
