@@ -101,6 +101,9 @@ def setup_md(e, w):
     e.set_system(box["x"], box["lo"], box["L"], v=v, inv_mass=inv, q=box["q"], types=box["type"])
     e.rigid_set(box["mol"], p["m_O"], p["m_H"], p["r_OH"], p["r_HH"], p["gamma"])
     e.integrator_set(w["dt_fs"], systems.REAL_FTM_TO_V)
+    # 298 K velocity rescaling every 10 steps (FixNVTRescale) during
+    # equilibration and the timed run
+    e.thermostat(True, 298.0, 10, 0.0, systems.REAL_BOLTZ, systems.REAL_MVV_TO_ENG)
 
 
 def cpu_baseline(w, rebuild_every, budget_s=12.0):
@@ -181,10 +184,13 @@ def main():
                     help="force mode: forced list rebuild interval")
     ap.add_argument("--mode", default="force", choices=["force", "md"],
                     help="force: force path at fixed positions; md: full rigid-water MD steps")
+    ap.add_argument("--equil", type=int, default=3000,
+                    help="md mode: untimed thermostatted steps before warm-up (lattice start)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pair-team", type=int, default=0)
     ap.add_argument("--build-team", type=int, default=0)
     ap.add_argument("--e3b-team", type=int, default=0)
+    ap.add_argument("--build-mode", type=int, default=0, help="0 auto, 1 per-cell LDS, 2 per-atom")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of known size (counter calibration)")
     args = ap.parse_args()
@@ -199,6 +205,7 @@ def main():
     w = make_workload(args.config)
     e, flags = setup_engine(w)
     e.teams_set(args.build_team, args.pair_team, args.e3b_team)
+    e.option("build_mode", args.build_mode)
     n = len(w["box"]["x"])
     R = args.rebuild_every
 
@@ -206,6 +213,8 @@ def main():
     if md:
         setup_md(e, w)
         step = lambda k: e.run(k, flags)  # noqa: E731
+        if args.equil:
+            step(args.equil)
     else:
         step = lambda k: e.force_steps(k, flags, R)  # noqa: E731
     step(args.warmup)

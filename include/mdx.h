@@ -116,6 +116,18 @@ int mdx_e3b_set(mdx_ctx *ctx, int nmol, const int32_t *mol_ids, const double par
 int mdx_rigid_set(mdx_ctx *ctx, int nmol, const int32_t *mol_ids, double m_O, double m_H, double d_OH,
                   double d_HH, double gamma);
 
+/* Thermostat applied every `every` steps of mdx_run: tau <= 0 rescales the
+ * velocities to `temperature` (FixNVTRescale, src/Fixes/FixNVTRescale.cu),
+ * tau > 0 is Berendsen weak coupling with time constant tau.  Degrees of
+ * freedom: 3 per massive site - 3 per rigid molecule - 3.  boltz and
+ * mvv_to_eng are the unit constants (Units.cpp:5-32). */
+int mdx_thermostat_set(mdx_ctx *ctx, int enable, double temperature, int every, double tau,
+                       double boltz, double mvv_to_eng);
+/* Instantaneous temperature and kinetic energy (FP64 reduction on the device;
+ * DataComputerTemperature). */
+int mdx_temperature(mdx_ctx *ctx, double boltz, double mvv_to_eng, double *temperature,
+                    double *kinetic);
+
 /* Zero and accumulate the requested force terms at the current positions
  * (IntegratorUtil::force, src/Integrators/IntegratorUtil.cpp:15-30). */
 int mdx_forces_compute(mdx_ctx *ctx, int flags);
@@ -138,7 +150,8 @@ int mdx_teams_set(mdx_ctx *ctx, int build_team, int pair_team, int e3b_team);
 int mdx_sync(mdx_ctx *ctx);
 /* Implementation switches (no counterpart in the reference): "tiled_spread"
  * 1 (default) accumulates the charge mesh in LDS tiles, 0 adds every weight
- * to the global mesh directly; both give bit-identical meshes. */
+ * to the global mesh directly (bit-identical meshes); "build_mode" 0 (auto),
+ * 1 LDS-staged build per cell, 2 per-atom teams (identical lists). */
 int mdx_option_set(mdx_ctx *ctx, const char *name, int value);
 
 /* Force-path steps at fixed positions: nsteps evaluations of the terms in

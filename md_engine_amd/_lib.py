@@ -25,6 +25,7 @@ SYMBOLS = [
     "mdx_pppm_info", "mdx_e3b_set", "mdx_forces_compute", "mdx_forces_get", "mdx_energy_get",
     "mdx_integrator_set", "mdx_run", "mdx_teams_set", "mdx_sync", "mdx_force_steps",
     "mdx_profile_enable", "mdx_profile_get", "mdx_nlist_totals", "mdx_option_set", "mdx_rigid_set",
+    "mdx_thermostat_set", "mdx_temperature",
 ]
 
 # kernel ids of mdx_profile_get
@@ -74,6 +75,8 @@ def load():
     L.mdx_nlist_stats.argtypes = [_vp, C.POINTER(C.c_int64), C.POINTER(c), _ip, C.POINTER(C.c_int64)]
     L.mdx_option_set.argtypes = [_vp, C.c_char_p, c]
     L.mdx_rigid_set.argtypes = [_vp, c, _ip, d, d, d, d, d]
+    L.mdx_thermostat_set.argtypes = [_vp, c, d, c, d, d, d]
+    L.mdx_temperature.argtypes = [_vp, d, d, C.POINTER(d), C.POINTER(d)]
     L.mdx_lj_set.argtypes = [_vp, c, _dp, _dp, _dp]
     L.mdx_special_set.argtypes = [_vp, d, d, d]
     L.mdx_ewald_real_set.argtypes = [_vp, c, d, d, d]
@@ -239,6 +242,18 @@ class Engine:
         m = np.ascontiguousarray(mol_ids, np.int32).reshape(-1)
         self._chk(self._L.mdx_rigid_set(self._h, len(m) // 4, m, float(m_O), float(m_H), float(d_OH),
                                         float(d_HH), float(gamma)), "mdx_rigid_set")
+
+    def thermostat(self, enable, temperature=0.0, every=1, tau=0.0, boltz=1.0, mvv_to_eng=1.0):
+        self._chk(self._L.mdx_thermostat_set(self._h, int(enable), float(temperature), int(every),
+                                             float(tau), float(boltz), float(mvv_to_eng)),
+                  "mdx_thermostat_set")
+
+    def temperature(self, boltz=1.0, mvv_to_eng=1.0):
+        t = C.c_double(0)
+        k = C.c_double(0)
+        self._chk(self._L.mdx_temperature(self._h, float(boltz), float(mvv_to_eng), C.byref(t), C.byref(k)),
+                  "mdx_temperature")
+        return t.value, k.value
 
     def compute(self, flags):
         self._chk(self._L.mdx_forces_compute(self._h, int(flags)), "mdx_forces_compute")

@@ -153,13 +153,12 @@ void snapshot_restore(Ctx &c) {
 // that was checked was already older than the skin allows.
 bool lagged_rebuild_decision(Ctx &c, int parity) {
     MDX_HIP(hipEventSynchronize(c.stepEvent[parity]));
-    float d2, v2;
+    float d2, p2;
     std::memcpy(&d2, &c.pinned[2 + 2 * parity], sizeof(float));
-    std::memcpy(&v2, &c.pinned[3 + 2 * parity], sizeof(float));
+    std::memcpy(&p2, &c.pinned[3 + 2 * parity], sizeof(float));
     const double half = 0.5 * c.skin;
-    const double d = std::sqrt((double)d2), v = std::sqrt((double)v2);
-    if (d > half) c.listViolations++;
-    return d + 2.0 * c.dt * v > half;
+    if (std::sqrt((double)d2) > half) c.listViolations++;
+    return std::sqrt((double)p2) > half;
 }
 
 void check_and_copy(Ctx &c, int parity) {
@@ -199,6 +198,7 @@ int mdx_create(int device, mdx_ctx **out) {
         c.acc = dalloc<unsigned long long>(c, ACC_N);
         c.dFlags = dalloc<int>(c, 4);
         c.dMax = dalloc<unsigned>(c, 4);
+        c.thermo = dalloc<unsigned long long>(c, 2);
         MDX_HIP(hipHostMalloc((void **)&c.pinned, sizeof(int) * 8, hipHostMallocDefault));
         MDX_HIP(hipEventCreateWithFlags(&c.capEvent, hipEventDisableTiming));
         for (int k = 0; k < 2; ++k) MDX_HIP(hipEventCreateWithFlags(&c.stepEvent[k], hipEventDisableTiming));
@@ -393,6 +393,13 @@ int mdx_exclusions_set(mdx_ctx *ctx, int n_ids, const int32_t *offsets, const ui
         MDX_HIP(hipStreamSynchronize(c.stream));
         c.nExclIds = n_ids;
         c.maxExcl = maxe;
+        c.exclSpan = 0;
+        for (int i = 0; i < n_ids; ++i)
+            for (int k = offsets[i]; k < offsets[i + 1]; ++k) {
+                const int j = (int)(tagged_ids[k] & 0x3FFFFFFFu);
+                MDX_REQUIRE(j >= 0 && j < n_ids, MDX_ERR_ARG, "excluded id out of range");
+                c.exclSpan = std::max(c.exclSpan, std::abs(j - i));
+            }
         c.listValid = false;
     });
 }
@@ -752,6 +759,7 @@ int mdx_run(mdx_ctx *ctx, int nsteps, int flags) {
                 compute(c, fflags);
                 integrate_second_half(c);
                 ++c.turn;
+                if (c.thermoOn && c.turn % c.thermoEvery == 0) thermostat_apply(c);
             }
             MDX_HIP(hipStreamSynchronize(c.stream));
             lagged_rebuild_decision(c, (len - 1) & 1);  // count a violation in the last step
@@ -852,7 +860,47 @@ int mdx_option_set(mdx_ctx *ctx, const char *name, int value) {
         MDX_REQUIRE(name, MDX_ERR_ARG, "option name required");
         std::string k(name);
         if (k == "tiled_spread") c.mdxDisableTiledSpread = (value == 0);
+        else if (k == "build_mode") {
+            MDX_REQUIRE(value >= 0 && value <= 2, MDX_ERR_ARG, "build_mode: 0 auto, 1 per cell, 2 per atom");
+            c.buildMode = value;
+        }
         else throw Error{MDX_ERR_ARG, "unknown option " + k};
+    });
+}
+
+static double degrees_of_freedom(const Ctx &c) {
+    double massive = 0;
+    for (double im : c.invMassHost) massive += im > 0 ? 1 : 0;
+    double ndf = 3.0 * massive - 3.0;
+    if (c.rigid.on) ndf -= 3.0 * c.rigid.nmol;
+    return ndf > 0 ? ndf : 1.0;
+}
+
+int mdx_thermostat_set(mdx_ctx *ctx, int enable, double temperature, int every, double tau,
+                       double boltz, double mvv_to_eng) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        MDX_REQUIRE(!enable || (temperature > 0 && every >= 1 && boltz > 0 && mvv_to_eng > 0), MDX_ERR_ARG,
+                    "thermostat needs T > 0, every >= 1, boltz > 0, mvv_to_eng > 0");
+        c.thermoOn = enable != 0;
+        c.thermoT = temperature;
+        c.thermoEvery = every;
+        c.thermoTau = tau;
+        c.tconv = mvv_to_eng / (degrees_of_freedom(c) * boltz);
+    });
+}
+
+int mdx_temperature(mdx_ctx *ctx, double boltz, double mvv_to_eng, double *temperature,
+                    double *kinetic) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        thermo_measure(c);
+        unsigned long long h = 0;
+        MDX_HIP(hipMemcpyAsync(&h, c.thermo, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        const double mv2 = (double)(long long)h * kFxInv;
+        if (kinetic) *kinetic = 0.5 * mv2 * mvv_to_eng;
+        if (temperature) *temperature = mv2 * mvv_to_eng / (degrees_of_freedom(c) * boltz);
     });
 }
 

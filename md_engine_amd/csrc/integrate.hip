@@ -51,7 +51,59 @@ __global__ void k_vv2(int n, double4 *__restrict__ vel, const unsigned long long
     vel[i] = v;
 }
 
+// Sum of m v^2 over massive sites (FP64 per block, one fixed-point atomic per
+// block) -> thermo[0]; the temperature follows from the degrees of freedom.
+__global__ void __launch_bounds__(256) k_mv2(int n, const double4 *__restrict__ vel,
+                                             unsigned long long *__restrict__ thermo) {
+    __shared__ double wsum[4];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    if (i < n) {
+        double4 v = vel[i];
+        if (v.w > 0.0) s = (v.x * v.x + v.y * v.y + v.z * v.z) / v.w;
+    }
+    s = team_sum_d<64>(s);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += wsum[w];
+        atomicAdd(&thermo[0], (unsigned long long)__double2ll_rn(t * kFxScale));
+    }
+}
+
+// Velocity rescaling toward T0: mode 0 sets T to T0 (FixNVTRescale,
+// src/Fixes/FixNVTRescale.cu), mode 1 is Berendsen weak coupling
+// lambda^2 = 1 + (interval dt / tau)(T0/T - 1).
+__global__ void k_vscale(int n, double4 *__restrict__ vel, const unsigned long long *__restrict__ thermo,
+                         double tconv, double T0, int mode, double coupling) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double T = (double)(long long)thermo[0] * kFxInv * tconv;
+    if (!(T > 0.0)) return;
+    double l2 = mode == 0 ? T0 / T : 1.0 + coupling * (T0 / T - 1.0);
+    if (l2 < 0.0) l2 = 0.0;
+    const double lam = sqrt(l2);
+    double4 v = vel[i];
+    v.x *= lam;
+    v.y *= lam;
+    v.z *= lam;
+    vel[i] = v;
+}
+
 }  // namespace
+
+void thermo_measure(Ctx &c) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, 2, c.thermo);
+    hipLaunchKernelGGL(k_mv2, dim3(cdiv(c.n, 256)), dim3(256), 0, c.stream, c.n, c.vel[c.cur], c.thermo);
+}
+
+void thermostat_apply(Ctx &c) {
+    thermo_measure(c);
+    const double coupling = c.thermoTau > 0 ? c.thermoEvery * c.dt / c.thermoTau : 1.0;
+    hipLaunchKernelGGL(k_vscale, dim3(cdiv(c.n, 256)), dim3(256), 0, c.stream, c.n, c.vel[c.cur], c.thermo,
+                       c.tconv, c.thermoT, c.thermoTau > 0 ? 1 : 0, coupling);
+}
 
 void zero_forces(Ctx &c, bool clear_acc) {
     const int threads = 256;

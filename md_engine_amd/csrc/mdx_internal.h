@@ -143,13 +143,14 @@ struct Ctx {
     uint32_t *nl = nullptr;
     int *nlCount = nullptr;
     int *dFlags = nullptr;        // [0] rebuild flag, [1] max count, [2] e3b max count
-    unsigned *dMax = nullptr;     // 2 slots x (max |x - x_last|^2, max |v|^2) as float bits
+    unsigned *dMax = nullptr;     // 2 slots x (max |x - x_last|^2, max predicted^2) as float bits
     int *pinned = nullptr;        // host-pinned: [0..1] last build's max counts, [2..5] dMax copies
     hipEvent_t capEvent = nullptr;
     bool capPending = false;
     long long listViolations = 0; // steps whose list was older than the skin allows
     int buildTeam = 32;
     bool buildTeamUser = false;   // set by mdx_teams_set, else chosen from the cell population
+    int buildMode = 0;            // 0 auto, 1 LDS-staged per-cell build, 2 per-atom teams
     long long nRebuilds = 0;
     bool listValid = false;
 
@@ -158,6 +159,7 @@ struct Ctx {
     uint32_t *exclIds = nullptr;
     int nExclIds = 0;
     int maxExcl = 0;
+    int exclSpan = 0;             // largest |id_i - id_j| over excluded pairs
 
     // LJ + Ewald real space
     bool ljOn = false;
@@ -172,6 +174,13 @@ struct Ctx {
     E3BState e3b;
     RigidState rigid;
     bool mdxDisableTiledSpread = false;   // test knob: force the plain global-atomic spread
+
+    // thermostat (velocity rescaling every thermoEvery steps)
+    bool thermoOn = false;
+    double thermoT = 0.0, thermoTau = 0.0;
+    int thermoEvery = 1;
+    double tconv = 0.0;                      // temperature per unit of sum m v^2
+    unsigned long long *thermo = nullptr;    // [0] sum m v^2 (fixed point)
 
     // integration
     double dt = 0.005;
@@ -292,5 +301,7 @@ void rigid_second_half(Ctx &c, bool kick);
 void integrate_first_half(Ctx &c);
 void integrate_second_half(Ctx &c);
 void zero_forces(Ctx &c, bool clear_acc);
+void thermo_measure(Ctx &c);          // sum m v^2 into thermo[0]
+void thermostat_apply(Ctx &c);
 
 }  // namespace mdx

@@ -213,40 +213,114 @@ __global__ void k_xq(int n, const double4 *__restrict__ pos, const float *__rest
     xq[k] = make_float4((float)(p.x - g.lo[0]), (float)(p.y - g.lo[1]), (float)(p.z - g.lo[2]), q[k]);
 }
 
-// Stencil ranges along one axis: either the (2 reach + 1) cells around c with
-// periodic wrap, or every cell once when the axis has fewer cells.
-struct AxisRange {
-    int count;
-    int v[5];
+// Stencil walk without local arrays (a dynamically indexed local array would
+// live in scratch memory): along x and y, slot s of axis_slots() is either
+// one of the (2 reach + 1) cells around c with periodic wrap, or every cell
+// once when the axis has fewer cells; -1 marks a slot outside a non-periodic
+// box.  Along z the candidates of a column are one or two contiguous runs.
+__device__ __forceinline__ int axis_slots(int nc, int reach) { return nc < 2 * reach + 1 ? nc : 2 * reach + 1; }
+
+__device__ __forceinline__ int axis_cell(int s, int c, int nc, int reach, int periodic) {
+    if (nc < 2 * reach + 1) return s;
+    int cc = c - reach + s;
+    if (cc < 0 || cc >= nc) {
+        if (!periodic) return -1;
+        cc = cc < 0 ? cc + nc : cc - nc;
+    }
+    return cc;
+}
+
+// Up to kExclReg exclusions of an atom live in registers (fully unrolled, so no
+// scratch); atoms with more fall back to reading them from memory.
+constexpr int kExclReg = 16;
+
+struct ExclCache {
+    uint32_t e[kExclReg];
+    int n;
 };
 
-__device__ __forceinline__ AxisRange axis_cells(int c, int nc, int reach, int periodic) {
-    AxisRange r;
-    r.count = 0;
-    if (nc < 2 * reach + 1) {
-        for (int k = 0; k < nc && k < 5; ++k) r.v[r.count++] = k;
-        return r;
+__device__ __forceinline__ void excl_load(ExclCache &x, const uint32_t *__restrict__ exclIds, int e0, int e1) {
+    x.n = e1 - e0;
+#pragma unroll
+    for (int k = 0; k < kExclReg; ++k) x.e[k] = (k < x.n) ? exclIds[e0 + k] : 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ uint32_t excl_tag(const ExclCache &x, uint32_t jid, const uint32_t *__restrict__ exclIds,
+                                             int e0) {
+    uint32_t tag = 0;
+#pragma unroll
+    for (int k = 0; k < kExclReg; ++k)
+        if ((x.e[k] & 0x3FFFFFFFu) == jid && k < x.n) tag = x.e[k] >> 30;
+    for (int k = kExclReg; k < x.n; ++k) {
+        const uint32_t ex = exclIds[e0 + k];
+        if ((ex & 0x3FFFFFFFu) == jid) tag = ex >> 30;
     }
-    for (int d = -reach; d <= reach; ++d) {
-        int cc = c + d;
-        if (cc < 0 || cc >= nc) {
-            if (!periodic) continue;
-            cc = (cc + nc) % nc;
+    return tag;
+}
+
+struct ZRuns {
+    int n, lo0, hi0, lo1, hi1;
+};
+
+__device__ __forceinline__ ZRuns z_runs(int cz, int ncz, int reach, int periodic) {
+    ZRuns z;
+    if (ncz < 2 * reach + 1) {
+        z.n = 1;
+        z.lo0 = 0;
+        z.hi0 = ncz - 1;
+        z.lo1 = z.hi1 = 0;
+        return z;
+    }
+    int lo = cz - reach, hi = cz + reach;
+    z.n = 1;
+    z.lo1 = z.hi1 = 0;
+    if (lo < 0) {
+        if (periodic) {
+            z.lo1 = lo + ncz;
+            z.hi1 = ncz - 1;
+            z.n = 2;
         }
-        r.v[r.count++] = cc;
+        lo = 0;
+    } else if (hi >= ncz) {
+        if (periodic) {
+            z.lo1 = 0;
+            z.hi1 = hi - ncz;
+            z.n = 2;
+        }
+        hi = ncz - 1;
     }
-    return r;
+    z.lo0 = lo;
+    z.hi0 = hi;
+    return z;
 }
 
 // One team of T lanes per atom.  Candidates come from contiguous z-runs of
 // each stencil column; hits are compacted with a ballot restricted to the
 // team, so row order is (stencil column, z-run, index) - deterministic.
+// FP32 pre-test on xq (x - lo in FP32): candidates with r^2 clearly inside or
+// outside r_list^2 (by more than `band`, a bound on the FP32 rounding of the
+// positions and of the distance) are decided in FP32; only the few inside the
+// band take the canonical FP64 test, so membership is unchanged.
+struct Prefilter {
+    float L[3], invL[3];
+    float lo2, hi2;   // rlist^2 -+ band
+};
+
+__device__ __forceinline__ float dist2f(const float4 &a, const float4 &b, const Prefilter &p) {
+    float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+    dx -= p.L[0] * rintf(dx * p.invL[0]);
+    dy -= p.L[1] * rintf(dy * p.invL[1]);
+    dz -= p.L[2] * rintf(dz * p.invL[2]);
+    return dx * dx + dy * dy + dz * dz;
+}
+
 template <int T>
 __global__ void __launch_bounds__(256) k_build(
-    int n, const double4 *__restrict__ pos, const int *__restrict__ id, Geometry g, CellGrid cg,
-    const int *__restrict__ colRank, const int *__restrict__ cellStart, double rlist2,
-    const int *__restrict__ exclOff, const uint32_t *__restrict__ exclIds, int cap,
-    uint32_t *__restrict__ nl, int *__restrict__ nlCount, int *__restrict__ flags) {
+    int n, const double4 *__restrict__ pos, const float4 *__restrict__ xq, const int *__restrict__ id,
+    Geometry g, CellGrid cg, Prefilter pf, const int *__restrict__ colRank,
+    const int *__restrict__ cellStart, double rlist2, const int *__restrict__ exclOff,
+    const uint32_t *__restrict__ exclIds, int exclSpan, int cap, uint32_t *__restrict__ nl,
+    int *__restrict__ nlCount, int *__restrict__ flags) {
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = gtid / T;
     const int tl = threadIdx.x & (T - 1);
@@ -257,68 +331,44 @@ __global__ void __launch_bounds__(256) k_build(
     const unsigned long long below = (1ull << lane) - 1ull;
 
     const double4 pi = pos[i];
+    const float4 xi = xq[i];
     const int myid = id[i];
     const int e0 = exclOff ? exclOff[myid] : 0;
     const int e1 = exclOff ? exclOff[myid + 1] : 0;
+    ExclCache xc;
+    excl_load(xc, exclIds, e0, e1);
     const int cx = cell_coord(pi.x, g.lo[0], cg.cellInv[0], cg.nc[0]);
     const int cy = cell_coord(pi.y, g.lo[1], cg.cellInv[1], cg.nc[1]);
     const int cz = cell_coord(pi.z, g.lo[2], cg.cellInv[2], cg.nc[2]);
-    const AxisRange ax = axis_cells(cx, cg.nc[0], cg.reach, g.periodic[0]);
-    const AxisRange ay = axis_cells(cy, cg.nc[1], cg.reach, g.periodic[1]);
     const int ncz = cg.nc[2];
-
-    // z-runs: at most two contiguous [z0, z1] runs
-    int zr0[2], zr1[2], nzr = 0;
-    if (ncz < 2 * cg.reach + 1) {
-        zr0[0] = 0;
-        zr1[0] = ncz - 1;
-        nzr = 1;
-    } else {
-        int lo = cz - cg.reach, hi = cz + cg.reach;
-        if (lo < 0) {
-            if (g.periodic[2]) {
-                zr0[nzr] = lo + ncz;
-                zr1[nzr] = ncz - 1;
-                ++nzr;
-            }
-            lo = 0;
-        }
-        if (hi >= ncz) {
-            if (g.periodic[2]) {
-                zr0[nzr] = 0;
-                zr1[nzr] = hi - ncz;
-                ++nzr;
-            }
-            hi = ncz - 1;
-        }
-        zr0[nzr] = lo;
-        zr1[nzr] = hi;
-        ++nzr;
-    }
+    const int sx = axis_slots(cg.nc[0], cg.reach), sy = axis_slots(cg.nc[1], cg.reach);
+    const ZRuns zr = z_runs(cz, ncz, cg.reach, g.periodic[2]);
 
     uint32_t *row = nl + (size_t)i * cap;
     int cnt = 0;
-    for (int a = 0; a < ax.count; ++a) {
-        for (int b = 0; b < ay.count; ++b) {
-            const int colBase = colRank[ax.v[a] * cg.nc[1] + ay.v[b]] * ncz;
-            for (int r = 0; r < nzr; ++r) {
-                const int s0 = cellStart[colBase + zr0[r]];
-                const int s1 = cellStart[colBase + zr1[r] + 1];
+    for (int a = 0; a < sx; ++a) {
+        const int xx = axis_cell(a, cx, cg.nc[0], cg.reach, g.periodic[0]);
+        if (xx < 0) continue;
+        for (int b = 0; b < sy; ++b) {
+            const int yy = axis_cell(b, cy, cg.nc[1], cg.reach, g.periodic[1]);
+            if (yy < 0) continue;
+            const int colBase = colRank[xx * cg.nc[1] + yy] * ncz;
+            for (int r = 0; r < zr.n; ++r) {
+                const int s0 = cellStart[colBase + (r ? zr.lo1 : zr.lo0)];
+                const int s1 = cellStart[colBase + (r ? zr.hi1 : zr.hi0) + 1];
                 for (int base = s0; base < s1; base += T) {
                     const int j = base + tl;
                     bool hit = false;
                     uint32_t entry = 0;
                     if (j < s1 && j != i) {
-                        const double4 pj = pos[j];
-                        if (dist2(pi, pj, g) < rlist2) {
-                            hit = true;
+                        const float r2f = dist2f(xi, xq[j], pf);
+                        hit = r2f < pf.lo2;
+                        if (!hit && r2f <= pf.hi2) hit = dist2(pi, pos[j], g) < rlist2;
+                        if (hit) {
                             uint32_t tag = 0;
                             if (e1 > e0) {
-                                const uint32_t jid = (uint32_t)id[j];
-                                for (int e = e0; e < e1; ++e) {
-                                    const uint32_t ex = exclIds[e];
-                                    if ((ex & 0x3FFFFFFFu) == jid) tag = ex >> 30;
-                                }
+                                const int jid = id[j];
+                                if (abs(jid - myid) <= exclSpan) tag = excl_tag(xc, (uint32_t)jid, exclIds, e0);
                             }
                             entry = (uint32_t)j | (tag << 30);
                         }
@@ -336,6 +386,123 @@ __global__ void __launch_bounds__(256) k_build(
     if (tl == 0) {
         nlCount[i] = cnt;
         atomicMax(&flags[1], cnt);
+    }
+}
+
+// LDS-staged build: one workgroup per cell.  Every atom of a cell has the same
+// stencil, so each stencil column's z-run of candidates is staged in LDS once
+// (FP64 position + id) and each wave tests it for the cell's atoms it owns
+// (atoms a0 + w, a0 + w + 4, ...; all 64 lanes on one atom).  Row order is
+// (stencil column, z-run, index), the same as k_build, and membership uses the
+// same canonical FP64 distance, so lists are identical.
+constexpr int kStage = 512;
+constexpr int kExclLds = 8;   // exclusions per atom cached in LDS (more are read from memory)
+
+__global__ void __launch_bounds__(256) k_build_cell(
+    const double4 *__restrict__ pos, const float4 *__restrict__ xq, const int *__restrict__ id,
+    Geometry g, CellGrid cg, Prefilter pf, const int2 *__restrict__ colXY,
+    const int *__restrict__ colRank, const int *__restrict__ cellStart, double rlist2,
+    const int *__restrict__ exclOff, const uint32_t *__restrict__ exclIds, int exclSpan, int cap,
+    uint32_t *__restrict__ nl, int *__restrict__ nlCount, int *__restrict__ flags) {
+    __shared__ float4 sXq[kStage];
+    __shared__ int sId[kStage];
+    __shared__ float4 sXi[256];
+    __shared__ int sMyId[256], sE0[256], sE1[256], sCnt[256];
+    __shared__ uint32_t sEx[256 * kExclLds];
+    const int c = blockIdx.x;
+    const int ncz = cg.nc[2];
+    const int cellA0 = cellStart[c], cellA1 = cellStart[c + 1];
+    if (cellA1 == cellA0) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const int2 cxy = colXY[c / ncz];
+    const int cz = c % ncz;
+    const int sx = axis_slots(cg.nc[0], cg.reach), sy = axis_slots(cg.nc[1], cg.reach);
+    const ZRuns zr = z_runs(cz, ncz, cg.reach, g.periodic[2]);
+    // crowded cells are handled in batches of 256 atoms, each against the whole stencil
+    for (int a0 = cellA0; a0 < cellA1; a0 += 256) {
+        const int nA = min(cellA1 - a0, 256);
+        for (int k = threadIdx.x; k < nA; k += blockDim.x) {
+            sXi[k] = xq[a0 + k];
+            const int myid = id[a0 + k];
+            const int e0 = exclOff ? exclOff[myid] : 0;
+            const int e1 = exclOff ? exclOff[myid + 1] : 0;
+            sMyId[k] = myid;
+            sE0[k] = e0;
+            sE1[k] = e1;
+            sCnt[k] = 0;
+            for (int q = 0; q < kExclLds; ++q) sEx[k * kExclLds + q] = q < e1 - e0 ? exclIds[e0 + q] : 0xFFFFFFFFu;
+        }
+        __syncthreads();
+        for (int a = 0; a < sx; ++a) {
+            const int xx = axis_cell(a, cxy.x, cg.nc[0], cg.reach, g.periodic[0]);
+            if (xx < 0) continue;
+            for (int b = 0; b < sy; ++b) {
+                const int yy = axis_cell(b, cxy.y, cg.nc[1], cg.reach, g.periodic[1]);
+                if (yy < 0) continue;
+                const int colBase = colRank[xx * cg.nc[1] + yy] * ncz;
+                for (int r = 0; r < zr.n; ++r) {
+                    const int s0 = cellStart[colBase + (r ? zr.lo1 : zr.lo0)];
+                    const int s1 = cellStart[colBase + (r ? zr.hi1 : zr.hi0) + 1];
+                    for (int base = s0; base < s1; base += kStage) {
+                        const int len = min(kStage, s1 - base);
+                        for (int t = threadIdx.x; t < len; t += blockDim.x) {
+                            sXq[t] = xq[base + t];
+                            sId[t] = id[base + t];
+                        }
+                        __syncthreads();
+                        for (int k = w; k < nA; k += 4) {
+                            const int i = a0 + k;
+                            const float4 xi = sXi[k];
+                            const int myid = sMyId[k];
+                            const int e0 = sE0[k], e1 = sE1[k];
+                            uint32_t *row = nl + (size_t)i * cap;
+                            int cnt = sCnt[k];
+                            for (int t0 = 0; t0 < len; t0 += 64) {
+                                const int t = t0 + lane;
+                                bool hit = false;
+                                uint32_t entry = 0;
+                                if (t < len && base + t != i) {
+                                    const float r2f = dist2f(xi, sXq[t], pf);
+                                    hit = r2f < pf.lo2;
+                                    if (!hit && r2f <= pf.hi2)
+                                        hit = dist2(pos[i], pos[base + t], g) < rlist2;
+                                    if (hit) {
+                                        uint32_t tag = 0;
+                                        const int jid = sId[t];
+                                        if (e1 > e0 && abs(jid - myid) <= exclSpan) {
+                                            const int ne = e1 - e0;
+                                            for (int q = 0; q < min(ne, kExclLds); ++q) {
+                                                const uint32_t ex = sEx[k * kExclLds + q];
+                                                if ((ex & 0x3FFFFFFFu) == (uint32_t)jid) tag = ex >> 30;
+                                            }
+                                            for (int q = kExclLds; q < ne; ++q) {
+                                                const uint32_t ex = exclIds[e0 + q];
+                                                if ((ex & 0x3FFFFFFFu) == (uint32_t)jid) tag = ex >> 30;
+                                            }
+                                        }
+                                        entry = (uint32_t)(base + t) | (tag << 30);
+                                    }
+                                }
+                                const unsigned long long m = __ballot(hit);
+                                if (hit) {
+                                    const int s = cnt + __popcll(m & below);
+                                    if (s < cap) row[s] = entry;
+                                }
+                                cnt += __popcll(m);
+                            }
+                            if (lane == 0) sCnt[k] = cnt;
+                        }
+                        __syncthreads();
+                    }
+                }
+            }
+        }
+        for (int k = threadIdx.x; k < nA; k += blockDim.x) {
+            nlCount[a0 + k] = sCnt[k];
+            atomicMax(&flags[1], sCnt[k]);
+        }
+        __syncthreads();
     }
 }
 
@@ -418,13 +585,42 @@ CellGrid make_grid(const Ctx &c) {
     return cg;
 }
 
+// Band that covers the FP32 error of r^2 for wrapped positions (x - lo in
+// [0, L]) and r <= r_list.  Per component the FP32 difference is off by at
+// most ~5 L 2^-24 (two rounded inputs, the subtraction, the rounded L and the
+// image shift); r^2 is then off by at most 2 sqrt(3) r_list of that plus a few
+// ulp of r^2 for the squares and the sum.  The band takes 16 L 2^-24 and
+// 32 r^2 2^-24, about three times both.
+Prefilter make_prefilter(const Ctx &c) {
+    Prefilter p;
+    double lmax = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        p.L[a] = (float)c.geo.L[a];
+        p.invL[a] = c.geo.periodic[a] ? (float)(1.0 / c.geo.L[a]) : 0.0f;
+        lmax = std::max(lmax, c.geo.L[a]);
+    }
+    const double r2 = c.rlist * c.rlist;
+    const double eps = std::ldexp(1.0, -23);
+    const double band = 8.0 * 2.0 * c.rlist * 1.01 * lmax * eps * std::sqrt(3.0) + 16.0 * r2 * eps;
+    p.lo2 = (float)(r2 - band);
+    p.hi2 = (float)(r2 + band);
+    // an open axis leaves positions unwrapped (no bound on |x - lo|): every
+    // candidate then takes the FP64 test
+    if (!c.geo.periodic[0] || !c.geo.periodic[1] || !c.geo.periodic[2]) {
+        p.lo2 = -1.0f;
+        p.hi2 = INFINITY;
+    }
+    return p;
+}
+
 template <int T>
 void launch_build(Ctx &c, const CellGrid &cg, int cur) {
     int threads = 256;
     int blocks = cdiv((long long)c.n * T, threads);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_build<T>), dim3(blocks), dim3(threads), 0, c.stream, c.n,
-                       c.pos[cur], c.id[cur], c.geo, cg, c.colRank, c.cellStart, c.rlist * c.rlist,
-                       c.exclOff, c.exclIds, c.cap, c.nl, c.nlCount, c.dFlags);
+                       c.pos[cur], c.xq, c.id[cur], c.geo, cg, make_prefilter(c), c.colRank,
+                       c.cellStart, c.rlist * c.rlist, c.exclOff, c.exclIds, c.exclSpan, c.cap, c.nl,
+                       c.nlCount, c.dFlags);
 }
 
 template <int T>
@@ -548,11 +744,19 @@ static void run_build(Ctx &c, bool sync) {
         fill_int(c, c.dFlags + 1, 2, 0);
         {
             ProfScope ps(c, PK_BUILD);
-            switch (c.buildTeam) {
-                case 8: launch_build<8>(c, cg, c.cur); break;
-                case 16: launch_build<16>(c, cg, c.cur); break;
-                case 32: launch_build<32>(c, cg, c.cur); break;
-                default: launch_build<64>(c, cg, c.cur); break;
+            const double perCell = (double)c.n / c.ncell;
+            if (c.buildMode == 1 || (c.buildMode == 0 && perCell >= 8.0)) {
+                hipLaunchKernelGGL(k_build_cell, dim3(c.ncell), dim3(256), 0, c.stream, c.pos[c.cur],
+                                   c.xq, c.id[c.cur], c.geo, cg, make_prefilter(c), c.colXY, c.colRank,
+                                   c.cellStart, c.rlist * c.rlist, c.exclOff, c.exclIds, c.exclSpan,
+                                   c.cap, c.nl, c.nlCount, c.dFlags);
+            } else {
+                switch (c.buildTeam) {
+                    case 8: launch_build<8>(c, cg, c.cur); break;
+                    case 16: launch_build<16>(c, cg, c.cur); break;
+                    case 32: launch_build<32>(c, cg, c.cur); break;
+                    default: launch_build<64>(c, cg, c.cur); break;
+                }
             }
         }
         if (c.e3b.on) {
@@ -629,32 +833,36 @@ void fill_int(Ctx &c, int *p, int count, int v) {
 }
 
 namespace {
-// Lag-1 rebuild decision for the step loop: max |x - x_last|^2 and max |v|^2
-// over all sites into one slot of dMax (as float bits; non-negative floats
-// order like their bit patterns); the other slot is cleared for the next step.
-// The host reads the slot one step later.
+// Lag-1 rebuild decision for the step loop: per site d = |x - x_last| and the
+// prediction p = d + 2 dt |v| (an M site, moved by its molecule, never moves
+// further than its fastest heavy site); max d^2 and max p^2 over all sites go
+// into one slot of dMax as float bits (non-negative floats order like their
+// bit patterns); the other slot is cleared for the next step.  The host reads
+// the slot one step later.
 __global__ void k_check_max(int n, const double4 *__restrict__ pos, const double4 *__restrict__ last,
-                            const double4 *__restrict__ vel, Geometry g, unsigned *__restrict__ slot,
-                            unsigned *__restrict__ other) {
+                            const double4 *__restrict__ vel, Geometry g, double dt2,
+                            unsigned *__restrict__ slot, unsigned *__restrict__ other) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
         other[0] = 0u;
         other[1] = 0u;
     }
-    float d2 = 0.f, v2 = 0.f;
+    float d2 = 0.f, p2 = 0.f;
     if (i < n) {
-        d2 = (float)dist2(pos[i], last[i], g);
-        double4 v = vel[i];
-        v2 = (float)(v.x * v.x + v.y * v.y + v.z * v.z);
+        const double dd = dist2(pos[i], last[i], g);
+        const double4 v = vel[i];
+        const double p = sqrt(dd) + dt2 * sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+        d2 = (float)dd;
+        p2 = (float)(p * p);
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         d2 = fmaxf(d2, __shfl_xor(d2, o, 64));
-        v2 = fmaxf(v2, __shfl_xor(v2, o, 64));
+        p2 = fmaxf(p2, __shfl_xor(p2, o, 64));
     }
     if ((threadIdx.x & 63) == 0) {
         atomicMax(&slot[0], __float_as_uint(d2));
-        atomicMax(&slot[1], __float_as_uint(v2));
+        atomicMax(&slot[1], __float_as_uint(p2));
     }
 }
 }  // namespace
@@ -662,7 +870,7 @@ __global__ void k_check_max(int n, const double4 *__restrict__ pos, const double
 void check_max(Ctx &c, int parity) {
     int threads = 256;
     hipLaunchKernelGGL(k_check_max, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n,
-                       c.pos[c.cur], c.posLast, c.vel[c.cur], c.geo, c.dMax + 2 * parity,
+                       c.pos[c.cur], c.posLast, c.vel[c.cur], c.geo, 2.0 * c.dt, c.dMax + 2 * parity,
                        c.dMax + 2 * (1 - parity));
 }
 

@@ -64,11 +64,12 @@ def assert_same_lists(gc, grows, oc, orows):
 # ---------------------------------------------------------------------------
 # neighbor lists
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("ncell", [6, 10])
-def test_nlist_lj_bitexact(orc, gpu_engine_factory, ncell):
-    say(f"nlist LJ fcc ncell={ncell}")
+@pytest.mark.parametrize("ncell,mode", [(6, 0), (10, 0), (10, 1)])
+def test_nlist_lj_bitexact(orc, gpu_engine_factory, ncell, mode):
+    say(f"nlist LJ fcc ncell={ncell} build mode {mode}")
     x, L, lo = perturbed_fcc(ncell)
     e = lj_engine(gpu_engine_factory, x, L, lo)
+    e.option("build_mode", mode)
     assert e.nlist_update(force=True)
     gc, grows = e.nlist_get()
     oc, orows, _ = oracle_rows_by_id(orc, x, lo, L, 2.8)
@@ -101,12 +102,14 @@ def test_nlist_small_noncubic_box(orc, gpu_engine_factory):
     assert_same_lists(gc, grows, oc, orows)
 
 
-def test_nlist_water_tags_bitexact(orc, gpu_engine_factory):
-    say("nlist water 512 molecules with exclusion tags")
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_nlist_water_tags_bitexact(orc, gpu_engine_factory, mode):
+    say(f"nlist water 512 molecules with exclusion tags, build mode {mode}")
     box = systems.tip4p2005_box(512, seed=5)
     e = gpu_engine_factory()
     e.set_system(box["x"], box["lo"], box["L"], q=box["q"], types=box["type"])
     e.exclusions_from_bonds(systems.water_bonds(box["mol"]))
+    e.option("build_mode", mode)
     e.nlist_config(10.0, 1.0)
     e.nlist_update(force=True)
     gc, grows = e.nlist_get()
@@ -117,14 +120,18 @@ def test_nlist_water_tags_bitexact(orc, gpu_engine_factory):
     assert set(np.unique(tags)) == {0, 1, 2}
 
 
-def test_nlist_capacity_growth(orc, gpu_engine_factory):
-    """A dense cluster overflows the initial row capacity: rows grow, no loss."""
-    say("nlist capacity growth")
+@pytest.mark.parametrize("mode", [1, 2])
+def test_nlist_capacity_growth(orc, gpu_engine_factory, mode):
+    """A dense cluster overflows the initial row capacity and crowds one cell
+    past 256 atoms: rows grow, batches cover the cell, no loss (both builds)."""
+    say(f"nlist capacity growth, build mode {mode}")
     rng = np.random.default_rng(9)
     L = np.full(3, 20.0)
     x = rng.uniform(0, 20, size=(3000, 3))
     x[:400] = rng.uniform(8, 10, size=(400, 3))
+    x[400:700] = rng.uniform(12.1, 12.9, size=(300, 3))
     e = lj_engine(gpu_engine_factory, x, L, np.zeros(3))
+    e.option("build_mode", mode)
     e.nlist_update(force=True)
     gc, grows = e.nlist_get()
     oc, orows, _ = oracle_rows_by_id(orc, x, np.zeros(3), L, 2.8)

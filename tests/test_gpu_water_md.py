@@ -108,3 +108,21 @@ def test_water_constraints_and_energy(orc, gpu_engine_factory):
     ke = orc.water_kinetic(box, vv)
     drift = abs(energies[-1] - e0)
     assert drift < 0.01 * ke, (e0, energies, ke)
+
+
+def test_thermostat_rescale(orc, gpu_engine_factory):
+    say("thermostat")
+    box = systems.tip4p2005_box(216, seed=23, min_sep=1.8)
+    v = start_velocities(orc, box, seed=3)
+    e, flags = water_md_engine(gpu_engine_factory, box, v, rc=7.0, skin=1.0, sz=(32, 32, 32), order=5,
+                               e3b=False, dt=1.0)
+    kb, mvv = systems.REAL_BOLTZ, systems.REAL_MVV_TO_ENG
+    e.thermostat(True, 350.0, 5, 0.0, kb, mvv)
+    e.run(100, flags)          # ends right after a rescale (100 % 5 == 0)
+    T, ke = e.temperature(kb, mvv)
+    _, vv = e.get_system()
+    ke_host = orc.water_kinetic(box, vv)
+    ndf = 3 * 3 * len(box["mol"]) - 3 * len(box["mol"]) - 3
+    assert ke == pytest.approx(ke_host, rel=1e-9)
+    assert T == pytest.approx(2 * ke_host / (ndf * kb), rel=1e-9)
+    assert T == pytest.approx(350.0, rel=1e-6)
