@@ -376,11 +376,18 @@ int mdx_exclusions_set(mdx_ctx *ctx, int n_ids, const int32_t *offsets, const ui
     return guarded(ctx, [&](Ctx &c) {
         require_system(c);
         MDX_REQUIRE(n_ids == c.n && offsets, MDX_ERR_ARG, "exclusion offsets must cover every id");
-        int total = offsets[n_ids];
-        int maxe = 0;
+        MDX_REQUIRE(offsets[0] == 0, MDX_ERR_ARG, "offsets[0] must be 0");
+        const int total = offsets[n_ids];
+        MDX_REQUIRE(total == 0 || tagged_ids, MDX_ERR_ARG, "missing excluded ids");
+        int maxe = 0, span = 0;
         for (int i = 0; i < n_ids; ++i) {
             MDX_REQUIRE(offsets[i + 1] >= offsets[i], MDX_ERR_ARG, "offsets must be non-decreasing");
             maxe = std::max(maxe, offsets[i + 1] - offsets[i]);
+            for (int k = offsets[i]; k < offsets[i + 1]; ++k) {
+                const int j = (int)(tagged_ids[k] & 0x3FFFFFFFu);
+                MDX_REQUIRE(j < n_ids && j != i, MDX_ERR_ARG, "excluded id out of range");
+                span = std::max(span, std::abs(j - i));
+            }
         }
         if (c.exclOff) {
             dfree(c, c.exclOff);
@@ -393,13 +400,7 @@ int mdx_exclusions_set(mdx_ctx *ctx, int n_ids, const int32_t *offsets, const ui
         MDX_HIP(hipStreamSynchronize(c.stream));
         c.nExclIds = n_ids;
         c.maxExcl = maxe;
-        c.exclSpan = 0;
-        for (int i = 0; i < n_ids; ++i)
-            for (int k = offsets[i]; k < offsets[i + 1]; ++k) {
-                const int j = (int)(tagged_ids[k] & 0x3FFFFFFFu);
-                MDX_REQUIRE(j >= 0 && j < n_ids, MDX_ERR_ARG, "excluded id out of range");
-                c.exclSpan = std::max(c.exclSpan, std::abs(j - i));
-            }
+        c.exclSpan = span;
         c.listValid = false;
     });
 }

@@ -160,6 +160,8 @@ struct Ctx {
     int nExclIds = 0;
     int maxExcl = 0;
     int exclSpan = 0;             // largest |id_i - id_j| over excluded pairs
+    uint32_t *exclIdx = nullptr;  // per atom: maxExcl partners as current index | tag << 30
+    size_t exclIdxN = 0;
 
     // LJ + Ewald real space
     bool ljOn = false;
@@ -257,6 +259,13 @@ template <int T>
 __device__ __forceinline__ float team_sum(float v) {
 #pragma unroll
     for (int o = T / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, T);
+    return v;
+}
+
+// maximum over the 64 lanes of a wave (every lane gets it)
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
     return v;
 }
 

@@ -230,40 +230,15 @@ __device__ __forceinline__ int axis_cell(int s, int c, int nc, int reach, int pe
     return cc;
 }
 
-// Up to kExclReg exclusions of an atom live in registers (fully unrolled, so no
-// scratch); atoms with more fall back to reading them from memory.
-constexpr int kExclReg = 16;
-
-struct ExclCache {
-    uint32_t e[kExclReg];
-    int n;
-};
-
-__device__ __forceinline__ void excl_load(ExclCache &x, const uint32_t *__restrict__ exclIds, int e0, int e1) {
-    x.n = e1 - e0;
-#pragma unroll
-    for (int k = 0; k < kExclReg; ++k) x.e[k] = (k < x.n) ? exclIds[e0 + k] : 0xFFFFFFFFu;
-}
-
-__device__ __forceinline__ uint32_t excl_tag(const ExclCache &x, uint32_t jid, const uint32_t *__restrict__ exclIds,
-                                             int e0) {
-    uint32_t tag = 0;
-#pragma unroll
-    for (int k = 0; k < kExclReg; ++k)
-        if ((x.e[k] & 0x3FFFFFFFu) == jid && k < x.n) tag = x.e[k] >> 30;
-    for (int k = kExclReg; k < x.n; ++k) {
-        const uint32_t ex = exclIds[e0 + k];
-        if ((ex & 0x3FFFFFFFu) == jid) tag = ex >> 30;
-    }
-    return tag;
-}
-
+// One or two contiguous runs of z-cells of a column; k0 / k1 is the image
+// (-1, 0, +1 box lengths) in which a run's cells sit in the unwrapped stencil.
 struct ZRuns {
-    int n, lo0, hi0, lo1, hi1;
+    int n, lo0, hi0, lo1, hi1, k0, k1;
 };
 
 __device__ __forceinline__ ZRuns z_runs(int cz, int ncz, int reach, int periodic) {
     ZRuns z;
+    z.k0 = z.k1 = 0;
     if (ncz < 2 * reach + 1) {
         z.n = 1;
         z.lo0 = 0;
@@ -278,6 +253,7 @@ __device__ __forceinline__ ZRuns z_runs(int cz, int ncz, int reach, int periodic
         if (periodic) {
             z.lo1 = lo + ncz;
             z.hi1 = ncz - 1;
+            z.k1 = -1;
             z.n = 2;
         }
         lo = 0;
@@ -285,6 +261,7 @@ __device__ __forceinline__ ZRuns z_runs(int cz, int ncz, int reach, int periodic
         if (periodic) {
             z.lo1 = 0;
             z.hi1 = hi - ncz;
+            z.k1 = 1;
             z.n = 2;
         }
         hi = ncz - 1;
@@ -294,9 +271,6 @@ __device__ __forceinline__ ZRuns z_runs(int cz, int ncz, int reach, int periodic
     return z;
 }
 
-// One team of T lanes per atom.  Candidates come from contiguous z-runs of
-// each stencil column; hits are compacted with a ballot restricted to the
-// team, so row order is (stencil column, z-run, index) - deterministic.
 // FP32 pre-test on xq (x - lo in FP32): candidates with r^2 clearly inside or
 // outside r_list^2 (by more than `band`, a bound on the FP32 rounding of the
 // positions and of the distance) are decided in FP32; only the few inside the
@@ -314,101 +288,275 @@ __device__ __forceinline__ float dist2f(const float4 &a, const float4 &b, const 
     return dx * dx + dy * dy + dz * dz;
 }
 
-template <int T>
+// Per-atom culling of the stencil: column (a, b) is skipped when its xy
+// distance from the atom exceeds r_list, and its z-run is cut to the cells
+// within sqrt(r_list^2 - d_xy^2) of the atom.  Distances are measured to the
+// unwrapped cell (cx - reach + a) * w: with nc >= 2 reach + 1 each wrapped
+// cell appears once in the stencil and every neighbour's minimum image lies in
+// its unwrapped cell.  The margin in r2 covers the FP32 rounding of the
+// coordinates and of the cell bounds.  Axes that are open or hold fewer than
+// 2 reach + 1 cells are not culled.
+struct Cull {
+    float w[3];
+    float r2;
+    int on[3];
+};
+
+// z-cells [ulo, uhi] (unwrapped, within one period of [0, nc)) as one or two
+// contiguous runs of a periodic column
+__device__ __forceinline__ ZRuns z_runs_cut(int ulo, int uhi, int ncz) {
+    ZRuns z;
+    z.n = 1;
+    z.lo1 = z.hi1 = 0;
+    z.k0 = z.k1 = 0;
+    if (ulo > uhi) {
+        z.n = 0;
+        z.lo0 = z.hi0 = 0;
+        return z;
+    }
+    if (ulo < 0) {
+        if (uhi < 0) {
+            z.lo0 = ulo + ncz;
+            z.hi0 = uhi + ncz;
+            z.k0 = -1;
+        } else {
+            z.lo0 = 0;
+            z.hi0 = uhi;
+            z.lo1 = ulo + ncz;
+            z.hi1 = ncz - 1;
+            z.k1 = -1;
+            z.n = 2;
+        }
+    } else if (uhi >= ncz) {
+        if (ulo >= ncz) {
+            z.lo0 = ulo - ncz;
+            z.hi0 = uhi - ncz;
+            z.k0 = 1;
+        } else {
+            z.lo0 = ulo;
+            z.hi0 = ncz - 1;
+            z.lo1 = 0;
+            z.hi1 = uhi - ncz;
+            z.k1 = 1;
+            z.n = 2;
+        }
+    } else {
+        z.lo0 = ulo;
+        z.hi0 = uhi;
+    }
+    return z;
+}
+
+__device__ __forceinline__ float axis_gap(int u, float w, float x) {
+    const float a = (float)u * w;
+    return fmaxf(0.0f, fmaxf(a - x, x - (a + w)));
+}
+
+// One team of T lanes per atom walks the culled stencil.  The team first
+// lays out its table of candidate runs in LDS, one stencil column per lane
+// (column bases and cell starts are then fetched in parallel, not as a chain
+// of dependent loads per column), then tests the runs T candidates at a time
+// with the next chunk's positions already in flight.  Hits are compacted with
+// a ballot restricted to the team, so row order is (stencil column, z-run,
+// index) - deterministic.
+//
+// SHIFT: every axis is open or holds >= 2 reach + 1 cells, so each candidate
+// of a run sits in one known image of the unwrapped stencil (the run's image
+// offsets are stored with it) and the FP32 pre-test is a plain difference
+// against the shifted atom; otherwise it takes the minimum image by rint.
+//
+// Exclusion partners arrive as current indices (idx | tag << 30, rows of
+// maxExcl padded with ~0u); the first four sit in registers and are compared
+// only within runs that contain one of them, further ones are read for hit
+// lanes only.
+constexpr int kMaxCols = 25;  // (2 reach + 1)^2 columns for reach 2
+
+__device__ __forceinline__ bool in_run(uint32_t ex, int s0, int s1) {
+    const int j = (int)(ex & 0x3FFFFFFFu);
+    return ex != ~0u && j >= s0 && j < s1;
+}
+
+template <int T, bool SHIFT>
 __global__ void __launch_bounds__(256) k_build(
-    int n, const double4 *__restrict__ pos, const float4 *__restrict__ xq, const int *__restrict__ id,
-    Geometry g, CellGrid cg, Prefilter pf, const int *__restrict__ colRank,
-    const int *__restrict__ cellStart, double rlist2, const int *__restrict__ exclOff,
-    const uint32_t *__restrict__ exclIds, int exclSpan, int cap, uint32_t *__restrict__ nl,
-    int *__restrict__ nlCount, int *__restrict__ flags) {
+    int n, const double4 *__restrict__ pos, const float4 *__restrict__ xq, Geometry g, CellGrid cg,
+    Prefilter pf, Cull cu, const int *__restrict__ colRank, const int *__restrict__ cellStart,
+    double rlist2, const uint32_t *__restrict__ exclIdx, int maxExcl, int cap,
+    uint32_t *__restrict__ nl, int *__restrict__ nlCount, int *__restrict__ flags) {
+    constexpr int U = 64 / T;  // chunks of T candidates per load block
+    __shared__ int4 sRuns[256 / T][kMaxCols];
+    __shared__ int sImg[256 / T][kMaxCols];  // (kx+1) | (ky+1) << 2 | (kz0+1) << 4 | (kz1+1) << 6
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = gtid / T;
+    const int team = threadIdx.x / T;
     const int tl = threadIdx.x & (T - 1);
     const int lane = threadIdx.x & 63;
-    if (i >= n) return;  // whole team leaves together (T divides 64)
-    const unsigned long long teamMask =
-        (T == 64) ? ~0ull : (((1ull << T) - 1ull) << (lane & ~(T - 1)));
-    const unsigned long long below = (1ull << lane) - 1ull;
-
-    const double4 pi = pos[i];
-    const float4 xi = xq[i];
-    const int myid = id[i];
-    const int e0 = exclOff ? exclOff[myid] : 0;
-    const int e1 = exclOff ? exclOff[myid + 1] : 0;
-    ExclCache xc;
-    excl_load(xc, exclIds, e0, e1);
-    const int cx = cell_coord(pi.x, g.lo[0], cg.cellInv[0], cg.nc[0]);
-    const int cy = cell_coord(pi.y, g.lo[1], cg.cellInv[1], cg.nc[1]);
-    const int cz = cell_coord(pi.z, g.lo[2], cg.cellInv[2], cg.nc[2]);
-    const int ncz = cg.nc[2];
+    const bool live = i < n;
     const int sx = axis_slots(cg.nc[0], cg.reach), sy = axis_slots(cg.nc[1], cg.reach);
-    const ZRuns zr = z_runs(cz, ncz, cg.reach, g.periodic[2]);
-
-    uint32_t *row = nl + (size_t)i * cap;
-    int cnt = 0;
-    for (int a = 0; a < sx; ++a) {
-        const int xx = axis_cell(a, cx, cg.nc[0], cg.reach, g.periodic[0]);
-        if (xx < 0) continue;
-        for (int b = 0; b < sy; ++b) {
+    const int ncol = sx * sy;
+    const int ncz = cg.nc[2];
+    if (live) {
+        const double4 pi = pos[i];
+        const float4 xi = xq[i];
+        const int cx = cell_coord(pi.x, g.lo[0], cg.cellInv[0], cg.nc[0]);
+        const int cy = cell_coord(pi.y, g.lo[1], cg.cellInv[1], cg.nc[1]);
+        const int cz = cell_coord(pi.z, g.lo[2], cg.cellInv[2], cg.nc[2]);
+        for (int col = tl; col < ncol; col += T) {
+            const int a = col / sy, b = col - a * sy;
+            int4 run = make_int4(0, 0, 0, 0);
+            int img = 0;
+            const int xx = axis_cell(a, cx, cg.nc[0], cg.reach, g.periodic[0]);
             const int yy = axis_cell(b, cy, cg.nc[1], cg.reach, g.periodic[1]);
-            if (yy < 0) continue;
-            const int colBase = colRank[xx * cg.nc[1] + yy] * ncz;
-            for (int r = 0; r < zr.n; ++r) {
-                const int s0 = cellStart[colBase + (r ? zr.lo1 : zr.lo0)];
-                const int s1 = cellStart[colBase + (r ? zr.hi1 : zr.hi0) + 1];
-                for (int base = s0; base < s1; base += T) {
-                    const int j = base + tl;
-                    bool hit = false;
-                    uint32_t entry = 0;
-                    if (j < s1 && j != i) {
-                        const float r2f = dist2f(xi, xq[j], pf);
-                        hit = r2f < pf.lo2;
-                        if (!hit && r2f <= pf.hi2) hit = dist2(pi, pos[j], g) < rlist2;
-                        if (hit) {
-                            uint32_t tag = 0;
-                            if (e1 > e0) {
-                                const int jid = id[j];
-                                if (abs(jid - myid) <= exclSpan) tag = excl_tag(xc, (uint32_t)jid, exclIds, e0);
-                            }
-                            entry = (uint32_t)j | (tag << 30);
+            const float gx = cu.on[0] ? axis_gap(cx - cg.reach + a, cu.w[0], xi.x) : 0.0f;
+            const float gy = cu.on[1] ? axis_gap(cy - cg.reach + b, cu.w[1], xi.y) : 0.0f;
+            const float gxy2 = gx * gx + gy * gy;
+            if (xx >= 0 && yy >= 0 && gxy2 <= cu.r2) {
+                ZRuns zr;
+                if (cu.on[2]) {
+                    const float h = sqrtf(cu.r2 - gxy2);
+                    const int ulo = max((int)floorf((xi.z - h) / cu.w[2]), cz - cg.reach);
+                    const int uhi = min((int)floorf((xi.z + h) / cu.w[2]), cz + cg.reach);
+                    zr = z_runs_cut(ulo, uhi, ncz);
+                } else {
+                    zr = z_runs(cz, ncz, cg.reach, g.periodic[2]);
+                }
+                const int colBase = colRank[xx * cg.nc[1] + yy] * ncz;
+                if (zr.n > 0) {
+                    run.x = cellStart[colBase + zr.lo0];
+                    run.y = cellStart[colBase + zr.hi0 + 1];
+                }
+                if (zr.n > 1) {
+                    run.z = cellStart[colBase + zr.lo1];
+                    run.w = cellStart[colBase + zr.hi1 + 1];
+                }
+                const int ux = cx - cg.reach + a, uy = cy - cg.reach + b;
+                const int kx = ux < 0 ? -1 : (ux >= cg.nc[0] ? 1 : 0);
+                const int ky = uy < 0 ? -1 : (uy >= cg.nc[1] ? 1 : 0);
+                img = (kx + 1) | ((ky + 1) << 2) | ((zr.k0 + 1) << 4) | ((zr.k1 + 1) << 6);
+            }
+            sRuns[team][col] = run;
+            sImg[team][col] = img;
+        }
+    }
+    __syncthreads();
+    int cnt = 0;
+    if (live) {
+        const unsigned long long teamMask =
+            (T == 64) ? ~0ull : (((1ull << T) - 1ull) << (lane & ~(T - 1)));
+        const unsigned long long pre = ((1ull << lane) - 1ull) & teamMask;
+        const double4 pi = pos[i];
+        const float4 xi = xq[i];
+        const uint32_t *exRow = exclIdx + (size_t)i * maxExcl;
+        const uint32_t ex0 = maxExcl > 0 ? exRow[0] : ~0u, ex1 = maxExcl > 1 ? exRow[1] : ~0u;
+        const uint32_t ex2 = maxExcl > 2 ? exRow[2] : ~0u, ex3 = maxExcl > 3 ? exRow[3] : ~0u;
+        uint32_t *row = nl + (size_t)i * cap;
+        for (int r = 0; r < 2 * ncol; ++r) {
+            const int4 run = sRuns[team][r >> 1];
+            const int s0 = (r & 1) ? run.z : run.x;
+            const int s1 = (r & 1) ? run.w : run.y;
+            if (s0 >= s1) continue;
+            float4 xs = xi;  // the atom moved into the run's image frame
+            if (SHIFT) {
+                const int img = sImg[team][r >> 1];
+                const int kz = (r & 1) ? ((img >> 6) & 3) - 1 : ((img >> 4) & 3) - 1;
+                xs.x = xi.x - (float)((img & 3) - 1) * pf.L[0];
+                xs.y = xi.y - (float)(((img >> 2) & 3) - 1) * pf.L[1];
+                xs.z = xi.z - (float)kz * pf.L[2];
+            }
+            const bool runEx = maxExcl > 4 || in_run(ex0, s0, s1) || in_run(ex1, s0, s1) ||
+                               in_run(ex2, s0, s1) || in_run(ex3, s0, s1);
+            // blocks of U chunks: all U loads are issued (indices clamped into
+            // the run, so unconditionally) before the first use, one wait per block
+            for (int b0 = s0; b0 < s1; b0 += U * T) {
+                float4 blk[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) blk[u] = xq[min(b0 + u * T + tl, s1 - 1)];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                const int j = b0 + u * T + tl;
+                const float4 cur = blk[u];
+                bool hit = false;
+                if (j < s1 && j != i) {
+                    float r2f;
+                    if (SHIFT) {
+                        const float dx = cur.x - xs.x, dy = cur.y - xs.y, dz = cur.z - xs.z;
+                        r2f = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    } else {
+                        r2f = dist2f(xi, cur, pf);
+                    }
+                    hit = r2f < pf.lo2;
+                    if (!hit && r2f <= pf.hi2) hit = dist2(pi, pos[j], g) < rlist2;
+                }
+                uint32_t tag = 0;
+                if (runEx) {
+                    const uint32_t uj = (uint32_t)j;
+                    tag = (ex0 & 0x3FFFFFFFu) == uj ? ex0 >> 30 : tag;
+                    tag = (ex1 & 0x3FFFFFFFu) == uj ? ex1 >> 30 : tag;
+                    tag = (ex2 & 0x3FFFFFFFu) == uj ? ex2 >> 30 : tag;
+                    tag = (ex3 & 0x3FFFFFFFu) == uj ? ex3 >> 30 : tag;
+                    if (maxExcl > 4 && hit)
+                        for (int q = 4; q < maxExcl; ++q) {
+                            const uint32_t e = exRow[q];
+                            if ((e & 0x3FFFFFFFu) == uj) tag = e >> 30;
                         }
-                    }
-                    const unsigned long long m = __ballot(hit) & teamMask;
-                    if (hit) {
-                        const int k = cnt + __popcll(m & below);
-                        if (k < cap) row[k] = entry;
-                    }
-                    cnt += __popcll(m);
+                }
+                const unsigned long long m = __ballot(hit);
+                if (hit) {
+                    const int k = cnt + __popcll(m & pre);
+                    if (k < cap) row[k] = (uint32_t)j | (tag << 30);
+                }
+                cnt += __popcll(m & teamMask);
                 }
             }
         }
+        if (tl == 0) nlCount[i] = cnt;
     }
-    if (tl == 0) {
-        nlCount[i] = cnt;
-        atomicMax(&flags[1], cnt);
+    // one atomic per wave: same-address atomics serialise in L2
+    const int wmax = wave_max(cnt);
+    if (lane == 0) atomicMax(&flags[1], wmax);
+}
+
+// Exclusion partners of each atom as current indices (idx | tag << 30), one
+// row of maxExcl per atom padded with ~0u, refreshed after each permutation.
+__global__ void k_excl_idx(int n, const int *__restrict__ id, const int *__restrict__ exclOff,
+                           const uint32_t *__restrict__ exclIds, const int *__restrict__ idToIdx,
+                           int maxExcl, uint32_t *__restrict__ exclIdx) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int myid = id[i];
+    const int e0 = exclOff[myid], e1 = exclOff[myid + 1];
+    for (int q = 0; q < maxExcl; ++q) {
+        uint32_t v = ~0u;
+        if (q < e1 - e0) {
+            const uint32_t e = exclIds[e0 + q];
+            v = (uint32_t)idToIdx[e & 0x3FFFFFFFu] | (e & 0xC0000000u);
+        }
+        exclIdx[(size_t)i * maxExcl + q] = v;
     }
 }
 
-// LDS-staged build: one workgroup per cell.  Every atom of a cell has the same
-// stencil, so each stencil column's z-run of candidates is staged in LDS once
-// (FP64 position + id) and each wave tests it for the cell's atoms it owns
-// (atoms a0 + w, a0 + w + 4, ...; all 64 lanes on one atom).  Row order is
-// (stencil column, z-run, index), the same as k_build, and membership uses the
-// same canonical FP64 distance, so lists are identical.
-constexpr int kStage = 512;
-constexpr int kExclLds = 8;   // exclusions per atom cached in LDS (more are read from memory)
+// LDS-staged build: one workgroup per cell.  Every atom of a cell shares the
+// stencil, so the (2 reach + 1)^2 columns x (2 reach + 1) z-slots of
+// candidates are staged in LDS once per cell, in (column, unwrapped z-slot)
+// order, each as its FP32 position with the candidate index in .w.  A wave then
+// takes one home atom at a time: lane l culls column l against the atom (as
+// in k_build) and turns the surviving z-slots into one LDS range; the wave
+// walks the ranges 64 candidates at a time, all operands wave-uniform except
+// the candidate.  A neighbourhood larger than the stage is processed in
+// windows of kStage candidates, a crowded home cell in batches of 256 atoms.
+// Membership uses the FP32 pre-test and the canonical FP64 distance in the
+// band, as k_build, so the rows hold the same sets (in another order).
+constexpr int kStage = 3072;
+constexpr int kSlots = 5;  // 2 reach + 1
 
 __global__ void __launch_bounds__(256) k_build_cell(
-    const double4 *__restrict__ pos, const float4 *__restrict__ xq, const int *__restrict__ id,
-    Geometry g, CellGrid cg, Prefilter pf, const int2 *__restrict__ colXY,
-    const int *__restrict__ colRank, const int *__restrict__ cellStart, double rlist2,
-    const int *__restrict__ exclOff, const uint32_t *__restrict__ exclIds, int exclSpan, int cap,
-    uint32_t *__restrict__ nl, int *__restrict__ nlCount, int *__restrict__ flags) {
+    const double4 *__restrict__ pos, const float4 *__restrict__ xq, Geometry g, CellGrid cg,
+    Prefilter pf, Cull cu, const int2 *__restrict__ colXY, const int *__restrict__ colRank,
+    const int *__restrict__ cellStart, double rlist2, const uint32_t *__restrict__ exclIdx, int maxExcl,
+    int cap, uint32_t *__restrict__ nl, int *__restrict__ nlCount, int *__restrict__ flags) {
     __shared__ float4 sXq[kStage];
-    __shared__ int sId[kStage];
-    __shared__ float4 sXi[256];
-    __shared__ int sMyId[256], sE0[256], sE1[256], sCnt[256];
-    __shared__ uint32_t sEx[256 * kExclLds];
+    __shared__ int sZoff[kMaxCols][kSlots + 1];  // virtual offset of each z-slot, and the column end
+    __shared__ int sGz[kMaxCols][kSlots];        // first candidate index of each z-slot
+    __shared__ int sCnt[256];
     const int c = blockIdx.x;
     const int ncz = cg.nc[2];
     const int cellA0 = cellStart[c], cellA1 = cellStart[c + 1];
@@ -418,92 +566,150 @@ __global__ void __launch_bounds__(256) k_build_cell(
     const int2 cxy = colXY[c / ncz];
     const int cz = c % ncz;
     const int sx = axis_slots(cg.nc[0], cg.reach), sy = axis_slots(cg.nc[1], cg.reach);
-    const ZRuns zr = z_runs(cz, ncz, cg.reach, g.periodic[2]);
-    // crowded cells are handled in batches of 256 atoms, each against the whole stencil
-    for (int a0 = cellA0; a0 < cellA1; a0 += 256) {
-        const int nA = min(cellA1 - a0, 256);
-        for (int k = threadIdx.x; k < nA; k += blockDim.x) {
-            sXi[k] = xq[a0 + k];
-            const int myid = id[a0 + k];
-            const int e0 = exclOff ? exclOff[myid] : 0;
-            const int e1 = exclOff ? exclOff[myid + 1] : 0;
-            sMyId[k] = myid;
-            sE0[k] = e0;
-            sE1[k] = e1;
-            sCnt[k] = 0;
-            for (int q = 0; q < kExclLds; ++q) sEx[k * kExclLds + q] = q < e1 - e0 ? exclIds[e0 + q] : 0xFFFFFFFFu;
-        }
-        __syncthreads();
-        for (int a = 0; a < sx; ++a) {
+    const int sz = axis_slots(ncz, cg.reach);
+    const int ncol = sx * sy;
+
+    // slot table: counts and first indices (thread = one (column, z-slot))
+    for (int t = threadIdx.x; t < kMaxCols * kSlots; t += blockDim.x) {
+        const int col = t / kSlots, s = t - col * kSlots;
+        int cnt = 0, first = 0;
+        if (col < ncol && s < sz) {
+            const int a = col / sy, b = col - a * sy;
             const int xx = axis_cell(a, cxy.x, cg.nc[0], cg.reach, g.periodic[0]);
-            if (xx < 0) continue;
-            for (int b = 0; b < sy; ++b) {
-                const int yy = axis_cell(b, cxy.y, cg.nc[1], cg.reach, g.periodic[1]);
-                if (yy < 0) continue;
-                const int colBase = colRank[xx * cg.nc[1] + yy] * ncz;
-                for (int r = 0; r < zr.n; ++r) {
-                    const int s0 = cellStart[colBase + (r ? zr.lo1 : zr.lo0)];
-                    const int s1 = cellStart[colBase + (r ? zr.hi1 : zr.hi0) + 1];
-                    for (int base = s0; base < s1; base += kStage) {
-                        const int len = min(kStage, s1 - base);
-                        for (int t = threadIdx.x; t < len; t += blockDim.x) {
-                            sXq[t] = xq[base + t];
-                            sId[t] = id[base + t];
-                        }
-                        __syncthreads();
-                        for (int k = w; k < nA; k += 4) {
-                            const int i = a0 + k;
-                            const float4 xi = sXi[k];
-                            const int myid = sMyId[k];
-                            const int e0 = sE0[k], e1 = sE1[k];
-                            uint32_t *row = nl + (size_t)i * cap;
-                            int cnt = sCnt[k];
-                            for (int t0 = 0; t0 < len; t0 += 64) {
-                                const int t = t0 + lane;
-                                bool hit = false;
-                                uint32_t entry = 0;
-                                if (t < len && base + t != i) {
-                                    const float r2f = dist2f(xi, sXq[t], pf);
-                                    hit = r2f < pf.lo2;
-                                    if (!hit && r2f <= pf.hi2)
-                                        hit = dist2(pos[i], pos[base + t], g) < rlist2;
-                                    if (hit) {
-                                        uint32_t tag = 0;
-                                        const int jid = sId[t];
-                                        if (e1 > e0 && abs(jid - myid) <= exclSpan) {
-                                            const int ne = e1 - e0;
-                                            for (int q = 0; q < min(ne, kExclLds); ++q) {
-                                                const uint32_t ex = sEx[k * kExclLds + q];
-                                                if ((ex & 0x3FFFFFFFu) == (uint32_t)jid) tag = ex >> 30;
-                                            }
-                                            for (int q = kExclLds; q < ne; ++q) {
-                                                const uint32_t ex = exclIds[e0 + q];
-                                                if ((ex & 0x3FFFFFFFu) == (uint32_t)jid) tag = ex >> 30;
-                                            }
-                                        }
-                                        entry = (uint32_t)(base + t) | (tag << 30);
-                                    }
-                                }
-                                const unsigned long long m = __ballot(hit);
-                                if (hit) {
-                                    const int s = cnt + __popcll(m & below);
-                                    if (s < cap) row[s] = entry;
-                                }
-                                cnt += __popcll(m);
-                            }
-                            if (lane == 0) sCnt[k] = cnt;
-                        }
-                        __syncthreads();
-                    }
-                }
+            const int yy = axis_cell(b, cxy.y, cg.nc[1], cg.reach, g.periodic[1]);
+            const int zz = axis_cell(s, cz, ncz, cg.reach, g.periodic[2]);
+            if (xx >= 0 && yy >= 0 && zz >= 0) {
+                const int cell = colRank[xx * cg.nc[1] + yy] * ncz + zz;
+                first = cellStart[cell];
+                cnt = cellStart[cell + 1] - first;
             }
         }
-        for (int k = threadIdx.x; k < nA; k += blockDim.x) {
-            nlCount[a0 + k] = sCnt[k];
-            atomicMax(&flags[1], sCnt[k]);
+        sGz[col][s] = first;
+        sZoff[col][s + 1] = cnt;  // counts for now, scanned below
+    }
+    __syncthreads();
+    // exclusive scan of the kMaxCols * kSlots counts by wave 0 (two per lane)
+    if (w == 0) {
+        const int t0 = 2 * lane, t1 = 2 * lane + 1;
+        const int n0 = t0 < kMaxCols * kSlots ? sZoff[t0 / kSlots][t0 % kSlots + 1] : 0;
+        const int n1 = t1 < kMaxCols * kSlots ? sZoff[t1 / kSlots][t1 % kSlots + 1] : 0;
+        int incl = n0 + n1;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += v;
+        }
+        const int excl = incl - n0 - n1;
+        // every slot gets its start; a column's end is the next column's start
+        if (t0 < kMaxCols * kSlots) sZoff[t0 / kSlots][t0 % kSlots] = excl;
+        if (t1 < kMaxCols * kSlots) sZoff[t1 / kSlots][t1 % kSlots] = excl + n0;
+        if (lane == 63) sZoff[kMaxCols - 1][kSlots] = incl;
+    }
+    __syncthreads();
+    if (threadIdx.x < kMaxCols - 1) sZoff[threadIdx.x][kSlots] = sZoff[threadIdx.x + 1][0];
+    __syncthreads();
+    const int total = sZoff[kMaxCols - 1][kSlots];
+
+    int wmax = 0;
+    for (int a0 = cellA0; a0 < cellA1; a0 += 256) {
+        const int nA = min(cellA1 - a0, 256);
+        for (int k = threadIdx.x; k < nA; k += blockDim.x) sCnt[k] = 0;
+        for (int w0 = 0; w0 < total; w0 += kStage) {
+            const int w1 = min(total, w0 + kStage);
+            __syncthreads();  // previous window fully consumed
+            // stage [w0, w1): wave w fills columns w, w + 4, ...
+            for (int col = w; col < ncol; col += 4) {
+                const int v0 = max(sZoff[col][0], w0), v1 = min(sZoff[col][kSlots], w1);
+                for (int v = v0 + lane; v < v1; v += 64) {
+                    int s = 0;
+#pragma unroll
+                    for (int q = 1; q < kSlots; ++q) s += v >= sZoff[col][q] ? 1 : 0;
+                    const int j = sGz[col][s] + (v - sZoff[col][s]);
+                    float4 p = xq[j];
+                    p.w = __int_as_float(j);
+                    sXq[v - w0] = p;
+                }
+            }
+            __syncthreads();
+            for (int k = w; k < nA; k += 4) {
+                const int i = a0 + k;
+                const float4 xi = xq[i];
+                const double4 pi = pos[i];
+                const uint32_t *exRow = exclIdx + (size_t)i * maxExcl;
+                const uint32_t ex0 = maxExcl > 0 ? exRow[0] : ~0u, ex1 = maxExcl > 1 ? exRow[1] : ~0u;
+                const uint32_t ex2 = maxExcl > 2 ? exRow[2] : ~0u, ex3 = maxExcl > 3 ? exRow[3] : ~0u;
+                const bool anyEx = ex0 != ~0u;
+                uint32_t *row = nl + (size_t)i * cap;
+                int cnt = sCnt[k];
+                // lane l: the LDS range of column l that survives culling
+                int lo = 0, hi = 0;
+                if (lane < ncol) {
+                    const int a = lane / sy, b = lane - a * sy;
+                    const float gx = cu.on[0] ? axis_gap(cxy.x - cg.reach + a, cu.w[0], xi.x) : 0.0f;
+                    const float gy = cu.on[1] ? axis_gap(cxy.y - cg.reach + b, cu.w[1], xi.y) : 0.0f;
+                    const float gxy2 = gx * gx + gy * gy;
+                    if (gxy2 <= cu.r2) {
+                        int slo = 0, shi = sz - 1;
+                        if (cu.on[2]) {
+                            const float h = sqrtf(cu.r2 - gxy2);
+                            slo = max((int)floorf((xi.z - h) / cu.w[2]) - (cz - cg.reach), 0);
+                            shi = min((int)floorf((xi.z + h) / cu.w[2]) - (cz - cg.reach), sz - 1);
+                        }
+                        if (slo <= shi) {
+                            lo = max(sZoff[lane][slo], w0) - w0;
+                            hi = min(sZoff[lane][shi + 1], w1) - w0;
+                        }
+                    }
+                }
+                for (int col = 0; col < ncol; ++col) {
+                    const int t0 = __builtin_amdgcn_readlane(lo, col);
+                    const int t1 = __builtin_amdgcn_readlane(hi, col);
+                    for (int tb = t0; tb < t1; tb += 64) {
+                        const int t = tb + lane;
+                        bool hit = false;
+                        int j = -1;
+                        if (t < t1) {
+                            const float4 pj = sXq[t];
+                            j = __float_as_int(pj.w);
+                            if (j != i) {
+                                const float r2f = dist2f(xi, pj, pf);
+                                hit = r2f < pf.lo2;
+                                if (!hit && r2f <= pf.hi2) hit = dist2(pi, pos[j], g) < rlist2;
+                            }
+                        }
+                        uint32_t tag = 0;
+                        if (anyEx) {
+                            const uint32_t uj = (uint32_t)j;
+                            tag = (ex0 & 0x3FFFFFFFu) == uj ? ex0 >> 30 : tag;
+                            tag = (ex1 & 0x3FFFFFFFu) == uj ? ex1 >> 30 : tag;
+                            tag = (ex2 & 0x3FFFFFFFu) == uj ? ex2 >> 30 : tag;
+                            tag = (ex3 & 0x3FFFFFFFu) == uj ? ex3 >> 30 : tag;
+                            if (maxExcl > 4 && hit)
+                                for (int q = 4; q < maxExcl; ++q) {
+                                    const uint32_t e = exRow[q];
+                                    if ((e & 0x3FFFFFFFu) == uj) tag = e >> 30;
+                                }
+                        }
+                        const unsigned long long m = __ballot(hit);
+                        if (hit) {
+                            const int s = cnt + __popcll(m & below);
+                            if (s < cap) row[s] = (uint32_t)j | (tag << 30);
+                        }
+                        cnt += __popcll(m);
+                    }
+                }
+                if (lane == 0) sCnt[k] = cnt;
+            }
         }
         __syncthreads();
+        for (int k = threadIdx.x; k < nA; k += blockDim.x) {
+            nlCount[a0 + k] = sCnt[k];
+            wmax = max(wmax, sCnt[k]);
+        }
     }
+    // one atomic per wave: same-address atomics serialise in L2
+    wmax = wave_max(wmax);
+    if (lane == 0) atomicMax(&flags[1], wmax);
 }
 
 // E3B molecule list from the atom list: for each water, the oxygens among its
@@ -520,35 +726,35 @@ __global__ void __launch_bounds__(256) k_e3b_list(
     const int m = gtid / T;
     const int tl = threadIdx.x & (T - 1);
     const int lane = threadIdx.x & 63;
-    if (m >= nmol) return;
-    const unsigned long long teamMask =
-        (T == 64) ? ~0ull : (((1ull << T) - 1ull) << (lane & ~(T - 1)));
-    const unsigned long long below = (1ull << lane) - 1ull;
-    const int oi = molAtoms[m].x;
-    const double4 pi = pos[oi];
-    const int cnt_i = min(nlCount[oi], cap);
-    const uint32_t *row = nl + (size_t)oi * cap;
     int cnt = 0;
-    for (int base = 0; base < cnt_i; base += T) {
-        const int k = base + tl;
-        bool hit = false;
-        int mj = -1;
-        if (k < cnt_i) {
-            const int j = (int)(row[k] & 0x3FFFFFFFu);
-            mj = atomMolO[j];
-            if (mj >= 0 && mj != m) hit = dist2(pi, pos[j], g) < rcut2;
+    if (m < nmol) {
+        const unsigned long long teamMask =
+            (T == 64) ? ~0ull : (((1ull << T) - 1ull) << (lane & ~(T - 1)));
+        const unsigned long long below = (1ull << lane) - 1ull;
+        const int oi = molAtoms[m].x;
+        const double4 pi = pos[oi];
+        const int cnt_i = min(nlCount[oi], cap);
+        const uint32_t *row = nl + (size_t)oi * cap;
+        for (int base = 0; base < cnt_i; base += T) {
+            const int k = base + tl;
+            bool hit = false;
+            int mj = -1;
+            if (k < cnt_i) {
+                const int j = (int)(row[k] & 0x3FFFFFFFu);
+                mj = atomMolO[j];
+                if (mj >= 0 && mj != m) hit = dist2(pi, pos[j], g) < rcut2;
+            }
+            const unsigned long long msk = __ballot(hit) & teamMask;
+            if (hit) {
+                const int s = cnt + __popcll(msk & below);
+                if (s < mcap) mnl[(size_t)m * mcap + s] = mj;
+            }
+            cnt += __popcll(msk);
         }
-        const unsigned long long msk = __ballot(hit) & teamMask;
-        if (hit) {
-            const int s = cnt + __popcll(msk & below);
-            if (s < mcap) mnl[(size_t)m * mcap + s] = mj;
-        }
-        cnt += __popcll(msk);
+        if (tl == 0) mcount[m] = cnt;
     }
-    if (tl == 0) {
-        mcount[m] = cnt;
-        atomicMax(&flags[2], cnt);
-    }
+    const int wmax = wave_max(cnt);
+    if (lane == 0) atomicMax(&flags[2], wmax);
 }
 
 __global__ void k_mol_atoms(int nmol, const int4 *__restrict__ molIds, const int *__restrict__ idToIdx,
@@ -613,14 +819,37 @@ Prefilter make_prefilter(const Ctx &c) {
     return p;
 }
 
+// Culling margin: far above the FP32 rounding of coordinates and cell bounds
+// (~L 2^-24), far below a cell.
+Cull make_cull(const Ctx &c) {
+    Cull u;
+    double lmax = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        u.w[a] = (float)(c.geo.L[a] / c.nc[a]);
+        u.on[a] = c.geo.periodic[a] && c.nc[a] >= 2 * c.reach + 1;
+        lmax = std::max(lmax, c.geo.L[a]);
+    }
+    const double r = c.rlist + 1e-5 * (lmax + c.rlist);
+    u.r2 = (float)(r * r);
+    // open axes leave atoms outside the box in the edge cells: no culling
+    if (!c.geo.periodic[0] || !c.geo.periodic[1] || !c.geo.periodic[2]) u.on[0] = u.on[1] = u.on[2] = 0;
+    return u;
+}
+
 template <int T>
 void launch_build(Ctx &c, const CellGrid &cg, int cur) {
+    MDX_REQUIRE(std::min(c.nc[0], 2 * c.reach + 1) * std::min(c.nc[1], 2 * c.reach + 1) <= kMaxCols, 1,
+                "stencil wider than the build kernel's column table");
     int threads = 256;
     int blocks = cdiv((long long)c.n * T, threads);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_build<T>), dim3(blocks), dim3(threads), 0, c.stream, c.n,
-                       c.pos[cur], c.xq, c.id[cur], c.geo, cg, make_prefilter(c), c.colRank,
-                       c.cellStart, c.rlist * c.rlist, c.exclOff, c.exclIds, c.exclSpan, c.cap, c.nl,
-                       c.nlCount, c.dFlags);
+    // image shifts need every periodic axis to hold the full stencil
+    bool shift = true;
+    for (int a = 0; a < 3; ++a)
+        if (c.geo.periodic[a] && c.nc[a] < 2 * c.reach + 1) shift = false;
+    auto kern = shift ? k_build<T, true> : k_build<T, false>;
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.xq, c.geo, cg,
+                       make_prefilter(c), make_cull(c), c.colRank, c.cellStart, c.rlist * c.rlist,
+                       c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.dFlags);
 }
 
 template <int T>
@@ -731,6 +960,15 @@ static void run_build(Ctx &c, bool sync) {
                        c.vel[nxt], c.q[nxt], c.type[nxt], c.id[nxt], c.idToIdx, c.posLast, c.xq,
                        c.geo);
     c.cur = nxt;
+    if (c.exclOff && c.maxExcl > 0) {
+        if (c.exclIdxN != (size_t)c.n * c.maxExcl) {
+            if (c.exclIdx) dfree(c, c.exclIdx);
+            c.exclIdxN = (size_t)c.n * c.maxExcl;
+            c.exclIdx = dalloc<uint32_t>(c, c.exclIdxN);
+        }
+        hipLaunchKernelGGL(k_excl_idx, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n,
+                           c.id[c.cur], c.exclOff, c.exclIds, c.idToIdx, c.maxExcl, c.exclIdx);
+    }
     rigid_map(c);
     if (c.e3b.on) {
         fill_int(c, c.e3b.atomMolO, c.n, -1);  // indices were permuted: drop the old map
@@ -746,9 +984,10 @@ static void run_build(Ctx &c, bool sync) {
             ProfScope ps(c, PK_BUILD);
             const double perCell = (double)c.n / c.ncell;
             if (c.buildMode == 1 || (c.buildMode == 0 && perCell >= 8.0)) {
+                MDX_REQUIRE(c.reach == 2, 1, "the per-cell build kernel assumes a 5-cell stencil");
                 hipLaunchKernelGGL(k_build_cell, dim3(c.ncell), dim3(256), 0, c.stream, c.pos[c.cur],
-                                   c.xq, c.id[c.cur], c.geo, cg, make_prefilter(c), c.colXY, c.colRank,
-                                   c.cellStart, c.rlist * c.rlist, c.exclOff, c.exclIds, c.exclSpan,
+                                   c.xq, c.geo, cg, make_prefilter(c), make_cull(c), c.colXY, c.colRank,
+                                   c.cellStart, c.rlist * c.rlist, c.exclIdx, c.exclOff ? c.maxExcl : 0,
                                    c.cap, c.nl, c.nlCount, c.dFlags);
             } else {
                 switch (c.buildTeam) {

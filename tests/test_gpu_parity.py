@@ -86,19 +86,39 @@ def test_nlist_lj_full_size_bitexact(orc, gpu_engine_factory):
     assert_same_lists(gc, grows, oc, orows)
 
 
-def test_nlist_small_noncubic_box(orc, gpu_engine_factory):
+@pytest.mark.parametrize("mode", [1, 2])
+def test_nlist_small_noncubic_box(orc, gpu_engine_factory, mode):
     """Axes with fewer than 5 cells scan every cell once (no duplicates)."""
-    say("nlist small non-cubic box")
+    say(f"nlist small non-cubic box, build mode {mode}")
     rng = np.random.default_rng(3)
     L = np.array([6.5, 12.0, 9.0])
     n = int(0.8 * L.prod())
     x = rng.uniform(-3.0, 15.0, size=(n, 3))  # outside the box on purpose: wrapped by the build
     e = lj_engine(gpu_engine_factory, x, L, np.zeros(3))
+    e.option("build_mode", mode)
     e.nlist_update(force=True)
     st = e.nlist_stats()
     assert min(st["cells"]) < 5
     gc, grows = e.nlist_get()
     oc, orows, _ = oracle_rows_by_id(orc, x, np.zeros(3), L, 2.8)
+    assert_same_lists(gc, grows, oc, orows)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_nlist_wide_neighbourhood(orc, gpu_engine_factory, mode):
+    """r_list = 7 sigma at liquid density: a 5x5x5-cell neighbourhood holds
+    ~4500 candidates, more than one LDS stage of the per-cell build."""
+    say(f"nlist wide neighbourhood (r_list 7), build mode {mode}")
+    rng = np.random.default_rng(21)
+    L = np.array([17.6, 17.9, 18.3])
+    n = int(0.85 * L.prod())
+    x = rng.uniform(0.0, 1.0, size=(n, 3)) * L
+    e = lj_engine(gpu_engine_factory, x, L, np.zeros(3), rc=6.7, skin=0.3)
+    e.option("build_mode", mode)
+    e.nlist_update(force=True)
+    gc, grows = e.nlist_get()
+    oc, orows, _ = oracle_rows_by_id(orc, x, np.zeros(3), L, 7.0, brute=False)
+    assert gc.mean() > 1000
     assert_same_lists(gc, grows, oc, orows)
 
 
