@@ -141,12 +141,27 @@ int mdx_energy_get(mdx_ctx *ctx, double e[4], double virial[6]);
 /* Velocity Verlet (IntegratorVerlet::run, src/Integrators/IntegratorVerlet.cu:649-770):
  * dt and the force-to-acceleration unit factor (Units::ftm_to_v). */
 int mdx_integrator_set(mdx_ctx *ctx, double dt, double ftm_to_v);
-/* Advance nsteps with the force terms in `flags`. */
+/* Advance nsteps with the force terms in `flags`.  With option "graph" = 1
+ * the steps are recorded once into a hipGraph (8 steps per graph, rebuild
+ * and thermostat decided on the device each step) and replayed. */
 int mdx_run(mdx_ctx *ctx, int nsteps, int flags);
+/* Batched replicas: advances every context (same device, each with its own
+ * system) by nsteps with one captured graph per 8 steps that holds all of
+ * them, each replica on its own stream.  No counterpart in the reference,
+ * whose integrator runs one system per process; results per replica equal a
+ * single-context captured run bit for bit.  Errors are reported on ctxs[0]. */
+int mdx_run_batch(mdx_ctx *const *ctxs, int n, int nsteps, int flags);
 /* Launch geometry knobs (Tunable nThreadPerAtom, src/Tunable.h:3-24):
  * lanes per atom in the list build and the pair kernel, lanes per molecule in
  * E3B.  0 keeps the current value.  Powers of two only. */
 int mdx_teams_set(mdx_ctx *ctx, int build_team, int pair_team, int e3b_team);
+/* Autotuner (Integrator::tune, src/Integrators/Integrator.cu:272-376, grid
+ * search of Tunable knobs by timing): times `evals` runs of every list-build
+ * kernel/team width, pair team width and E3B team width for the terms in
+ * `flags`, keeps the fastest, and recomputes forces.  chosen[4] (may be NULL)
+ * = build mode (1 per-cell, 2 per-atom), build team, pair team, E3B team;
+ * avg_ms[3] (may be NULL) = the winners' average build, pair, E3B times. */
+int mdx_tune(mdx_ctx *ctx, int flags, int evals, int32_t *chosen, double *avg_ms);
 int mdx_sync(mdx_ctx *ctx);
 /* Implementation switches (no counterpart in the reference): "tiled_spread"
  * 1 (default) accumulates the charge mesh in LDS tiles, 0 adds every weight

@@ -25,7 +25,7 @@ SYMBOLS = [
     "mdx_pppm_info", "mdx_e3b_set", "mdx_forces_compute", "mdx_forces_get", "mdx_energy_get",
     "mdx_integrator_set", "mdx_run", "mdx_teams_set", "mdx_sync", "mdx_force_steps",
     "mdx_profile_enable", "mdx_profile_get", "mdx_nlist_totals", "mdx_option_set", "mdx_rigid_set",
-    "mdx_thermostat_set", "mdx_temperature",
+    "mdx_thermostat_set", "mdx_temperature", "mdx_tune", "mdx_run_batch",
 ]
 
 # kernel ids of mdx_profile_get
@@ -94,6 +94,8 @@ def load():
     L.mdx_profile_enable.argtypes = [_vp, c]
     L.mdx_profile_get.argtypes = [_vp, c, C.POINTER(c), C.POINTER(d)]
     L.mdx_nlist_totals.argtypes = [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.mdx_tune.argtypes = [_vp, c, c, _ip, _dp]
+    L.mdx_run_batch.argtypes = [C.POINTER(_vp), c, c, c]
     for name in SYMBOLS:
         fn = getattr(L, name)
         if name not in ("mdx_last_error", "mdx_version"):
@@ -279,6 +281,17 @@ class Engine:
     def teams_set(self, build=0, pair=0, e3b=0):
         self._chk(self._L.mdx_teams_set(self._h, int(build), int(pair), int(e3b)), "mdx_teams_set")
 
+    def tune(self, flags, evals=10):
+        """Time every launch variant of the list build, pair and E3B kernels and
+        keep the fastest (the reference's Integrator::tune).  Returns the choice
+        and the winners' average kernel times in ms."""
+        ch = np.zeros(4, np.int32)
+        ms = np.zeros(3)
+        self._chk(self._L.mdx_tune(self._h, int(flags), int(evals), ch, ms), "mdx_tune")
+        return dict(build_mode=int(ch[0]), build_team=int(ch[1]), pair_team=int(ch[2]),
+                    e3b_team=int(ch[3]), build_ms=float(ms[0]), pair_ms=float(ms[1]),
+                    e3b_ms=float(ms[2]))
+
     def sync(self):
         self._chk(self._L.mdx_sync(self._h), "mdx_sync")
 
@@ -301,3 +314,17 @@ class Engine:
         b = C.c_int64(0)
         self._chk(self._L.mdx_nlist_totals(self._h, C.byref(a), C.byref(b)), "mdx_nlist_totals")
         return a.value, b.value
+
+
+def run_batch(engines, nsteps, flags):
+    """Advance several engines (one system each, same device) by nsteps with
+    one captured graph holding every replica (mdx_run_batch)."""
+    engines = list(engines)
+    if not engines:
+        return
+    L = load()
+    arr = (_vp * len(engines))(*[e._h for e in engines])
+    rc = L.mdx_run_batch(arr, len(engines), int(nsteps), int(flags))
+    if rc != 0:
+        msg = L.mdx_last_error(engines[0]._h)
+        raise MdxError(f"mdx_run_batch failed ({rc}): {msg.decode() if msg else ''}")

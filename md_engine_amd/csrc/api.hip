@@ -173,6 +173,164 @@ void upload(Ctx &c, T *dst, const T *src, size_t count) {
     MDX_HIP(hipMemcpyAsync(dst, src, sizeof(T) * count, hipMemcpyHostToDevice, c.stream));
 }
 
+// ---------------------------------------------------------------------------
+// Captured steps.  One MD step (kick-drift + constraints, device rebuild
+// decision, gated list rebuild, forces, kick + constraints, gated thermostat)
+// is recorded G times into one hipGraph for every context of a batch: each
+// context's work stays on its own stream, forked from and joined to the first
+// context's stream inside the capture, so one graph launch advances every
+// replica by G steps.  Buffers are fixed while a graph lives; row overflow is
+// found at the end of a chunk and the chunk is replayed with grown rows.
+constexpr int kGraphSteps = 8;
+constexpr int kGraphChunk = 64;
+
+void captured_step(Ctx &c, int fflags) {
+    integrate_first_half(c);
+    decide_step(c);
+    nlist_rebuild_async(c);  // every kernel gated by dFlags[3]
+    compute(c, fflags);
+    integrate_second_half(c);
+    if (c.thermoOn) thermostat_apply(c);  // gated by dFlags[5]
+}
+
+hipGraphExec_t capture_steps(const std::vector<Ctx *> &cs, int steps, int fflags) {
+    Ctx &c0 = *cs[0];
+    std::vector<hipEvent_t> evs(cs.size() * 2, nullptr);
+    for (auto &e : evs) MDX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    std::vector<bool> prof(cs.size());
+    hipGraph_t graph = nullptr;
+    MDX_HIP(hipStreamBeginCapture(c0.stream, hipStreamCaptureModeThreadLocal));
+    try {
+        MDX_HIP(hipEventRecord(evs[0], c0.stream));
+        for (size_t k = 0; k < cs.size(); ++k) {
+            Ctx &c = *cs[k];
+            if (k) MDX_HIP(hipStreamWaitEvent(c.stream, evs[0], 0));
+            prof[k] = c.prof;
+            c.prof = false;
+            c.capture = true;
+            for (int s = 0; s < steps; ++s) captured_step(c, fflags);
+            c.capture = false;
+            c.prof = prof[k];
+            if (k) {
+                MDX_HIP(hipEventRecord(evs[cs.size() + k], c.stream));
+                MDX_HIP(hipStreamWaitEvent(c0.stream, evs[cs.size() + k], 0));
+            }
+        }
+    } catch (...) {
+        for (size_t k = 0; k < cs.size(); ++k) {
+            cs[k]->capture = false;
+            if (k < prof.size()) cs[k]->prof = prof[k];
+        }
+        hipStreamEndCapture(c0.stream, &graph);
+        if (graph) hipGraphDestroy(graph);
+        for (auto e : evs) hipEventDestroy(e);
+        throw;
+    }
+    MDX_HIP(hipStreamEndCapture(c0.stream, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    for (auto e : evs) hipEventDestroy(e);
+    MDX_HIP(ie);
+    return exec;
+}
+
+// Waits on `from` in stream `to` (cross-context ordering outside a graph).
+void stream_after(hipStream_t to, hipStream_t from) {
+    hipEvent_t e;
+    MDX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    MDX_HIP(hipEventRecord(e, from));
+    MDX_HIP(hipStreamWaitEvent(to, e, 0));
+    MDX_HIP(hipEventDestroy(e));
+}
+
+void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags) {
+    const int fflags = flags & (MDX_PAIR | MDX_PPPM | MDX_E3B);
+    for (Ctx *cp : cs) {
+        Ctx &c = *cp;
+        require_system(c);
+        MDX_REQUIRE(c.rlist > 0, MDX_ERR_STATE, "mdx_nlist_config not called");
+        MDX_REQUIRE(c.device == cs[0]->device, MDX_ERR_ARG, "batched contexts must share one device");
+        MDX_HIP(hipSetDevice(c.device));
+        prepare_terms(c);
+        if (c.rigid.on && c.rigid.fresh) {
+            rigid_second_half(c, false);
+            c.rigid.fresh = false;
+            c.listValid = false;
+        }
+    }
+    hipGraphExec_t big = nullptr, one = nullptr;
+    auto drop = [&] {
+        if (big) hipGraphExecDestroy(big);
+        if (one) hipGraphExecDestroy(one);
+        big = one = nullptr;
+    };
+    try {
+        int done = 0;
+        while (done < nsteps) {
+            const int len = std::min(kGraphChunk, nsteps - done);
+            for (Ctx *cp : cs) {
+                Ctx &c = *cp;
+                snapshot_save(c);
+                if (!c.listValid || c.forceFlags != fflags) {
+                    ensure_list(c);
+                    compute(c, fflags);
+                }
+                MDX_HIP(hipStreamSynchronize(c.stream));
+                if (!capacity_ok(c)) {
+                    nlist_rebuild(c, true);
+                    compute(c, fflags);
+                }
+                // chunk state: displacement slots, running row maxima, counters, turn
+                fill_int(c, (int *)c.dMax, 4, 0);
+                fill_int(c, c.dFlags + 1, 5, 0);
+                fill_int(c, c.dFlags + 6, 1, c.thermoOn ? c.turn % c.thermoEvery : 0);
+                if (cp != cs[0]) stream_after(cs[0]->stream, c.stream);
+            }
+            if (!big) big = capture_steps(cs, kGraphSteps, fflags);
+            const int nbig = len / kGraphSteps, nrest = len % kGraphSteps;
+            if (nrest && !one) one = capture_steps(cs, 1, fflags);
+            for (int k = 0; k < nbig; ++k) MDX_HIP(hipGraphLaunch(big, cs[0]->stream));
+            for (int k = 0; k < nrest; ++k) MDX_HIP(hipGraphLaunch(one, cs[0]->stream));
+            MDX_HIP(hipStreamSynchronize(cs[0]->stream));
+            bool regrow = false;
+            for (Ctx *cp : cs) {
+                Ctx &c = *cp;
+                int h[4];
+                MDX_HIP(hipMemcpy(h, c.dFlags + 1, sizeof(int) * 4, hipMemcpyDeviceToHost));
+                c.pinned[0] = h[0];
+                c.pinned[1] = h[1];
+                if (h[0] > c.cap || (c.e3b.on && h[1] > c.e3b.cap)) {
+                    snapshot_restore(c);
+                    grow_if_needed(c);
+                    c.forceFlags = -1;
+                    regrow = true;
+                } else {
+                    c.nRebuilds += h[3];
+                    c.turn += len;
+                }
+            }
+            if (regrow) {
+                // rows were reallocated: every replica replays the chunk
+                for (Ctx *cp : cs) {
+                    if (cp->listValid) {
+                        snapshot_restore(*cp);
+                        cp->forceFlags = -1;
+                    }
+                }
+                drop();
+                continue;
+            }
+            done += len;
+        }
+    } catch (...) {
+        drop();
+        throw;
+    }
+    drop();
+    for (Ctx *cp : cs) MDX_HIP(hipStreamSynchronize(cp->stream));
+}
+
 }  // namespace
 }  // namespace mdx
 
@@ -196,14 +354,14 @@ int mdx_create(int device, mdx_ctx **out) {
     }
     int rc = guarded(ctx, [](Ctx &c) {
         c.acc = dalloc<unsigned long long>(c, ACC_N);
-        c.dFlags = dalloc<int>(c, 4);
+        c.dFlags = dalloc<int>(c, 8);
         c.dMax = dalloc<unsigned>(c, 4);
         c.thermo = dalloc<unsigned long long>(c, 2);
         MDX_HIP(hipHostMalloc((void **)&c.pinned, sizeof(int) * 8, hipHostMallocDefault));
         MDX_HIP(hipEventCreateWithFlags(&c.capEvent, hipEventDisableTiming));
         for (int k = 0; k < 2; ++k) MDX_HIP(hipEventCreateWithFlags(&c.stepEvent[k], hipEventDisableTiming));
         MDX_HIP(hipMemsetAsync(c.dMax, 0, sizeof(unsigned) * 4, c.stream));
-        MDX_HIP(hipMemsetAsync(c.dFlags, 0, sizeof(int) * 4, c.stream));
+        MDX_HIP(hipMemsetAsync(c.dFlags, 0, sizeof(int) * 8, c.stream));
         MDX_HIP(hipMemsetAsync(c.acc, 0, sizeof(unsigned long long) * ACC_N, c.stream));
     });
     if (rc != MDX_OK) {
@@ -732,6 +890,10 @@ int mdx_run(mdx_ctx *ctx, int nsteps, int flags) {
         require_system(c);
         MDX_REQUIRE(c.rlist > 0, MDX_ERR_STATE, "mdx_nlist_config not called");
         MDX_REQUIRE(nsteps >= 0, MDX_ERR_ARG, "nsteps must be >= 0");
+        if (c.graphSteps) {
+            run_captured({&c}, nsteps, flags);
+            return;
+        }
         const int fflags = flags & (MDX_PAIR | MDX_PPPM | MDX_E3B);
         prepare_terms(c);
         if (c.rigid.on && c.rigid.fresh) {
@@ -776,6 +938,21 @@ int mdx_run(mdx_ctx *ctx, int nsteps, int flags) {
     });
 }
 
+int mdx_run_batch(mdx_ctx *const *ctxs, int n, int nsteps, int flags) {
+    if (!ctxs || n <= 0) return MDX_ERR_ARG;
+    for (int k = 0; k < n; ++k)
+        if (!ctxs[k]) return MDX_ERR_ARG;
+    return guarded(ctxs[0], [&](Ctx &) {
+        MDX_REQUIRE(nsteps >= 0, MDX_ERR_ARG, "nsteps must be >= 0");
+        std::vector<Ctx *> cs;
+        for (int k = 0; k < n; ++k) {
+            for (Ctx *o : cs) MDX_REQUIRE(o != &ctxs[k]->c, MDX_ERR_ARG, "a context appears twice in the batch");
+            cs.push_back(&ctxs[k]->c);
+        }
+        run_captured(cs, nsteps, flags);
+    });
+}
+
 int mdx_force_steps(mdx_ctx *ctx, int nsteps, int flags, int rebuild_every) {
     return guarded(ctx, [&](Ctx &c) {
         require_system(c);
@@ -809,6 +986,104 @@ int mdx_teams_set(mdx_ctx *ctx, int build_team, int pair_team, int e3b_team) {
         }
         if (pair_team) c.pairTeam = pair_team;
         if (e3b_team) c.e3b.team = e3b_team;
+    });
+}
+
+// Grid search of the launch knobs by timing, as Integrator::tune does for
+// nThreadPerBlock x nThreadPerAtom (src/Integrators/Integrator.cu:272-376):
+// each candidate runs `evals` times between HIP events on the library stream
+// and the fastest average wins.  The list build is timed as a forced rebuild
+// (both kernels, every team width); pair and E3B kernels at the current list.
+int mdx_tune(mdx_ctx *ctx, int flags, int evals, int32_t *chosen, double *avg_ms) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        MDX_REQUIRE(c.rlist > 0, MDX_ERR_STATE, "mdx_nlist_config not called");
+        MDX_REQUIRE(evals > 0 && evals <= 1000, MDX_ERR_ARG, "evals must be in 1..1000");
+        prepare_terms(c);
+        ensure_list(c);
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        if (!capacity_ok(c)) nlist_rebuild(c, true);
+        hipEvent_t e0, e1;
+        MDX_HIP(hipEventCreate(&e0));
+        MDX_HIP(hipEventCreate(&e1));
+        auto timed = [&](auto &&body) {
+            body();  // warm
+            MDX_HIP(hipEventRecord(e0, c.stream));
+            for (int k = 0; k < evals; ++k) body();
+            MDX_HIP(hipEventRecord(e1, c.stream));
+            MDX_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            MDX_HIP(hipEventElapsedTime(&ms, e0, e1));
+            return (double)ms / evals;
+        };
+        const bool prof = c.prof;
+        c.prof = false;
+        // list build: (mode, team) candidates; rows regrow if a candidate overflows
+        int bestMode = 2, bestTeam = c.buildTeam;
+        double bestBuild = 1e30;
+        const int modes[5][2] = {{2, 8}, {2, 16}, {2, 32}, {2, 64}, {1, 0}};
+        for (auto &mt : modes) {
+            c.buildMode = mt[0];
+            if (mt[1]) c.buildTeam = mt[1];
+            const double t = timed([&] { nlist_rebuild_async(c); });
+            MDX_HIP(hipStreamSynchronize(c.stream));
+            MDX_REQUIRE(capacity_ok(c), MDX_ERR_STATE, "list rows changed size while tuning");
+            if (t < bestBuild) {
+                bestBuild = t;
+                bestMode = mt[0];
+                bestTeam = mt[1] ? mt[1] : bestTeam;
+            }
+        }
+        c.buildMode = bestMode;
+        c.buildTeam = bestTeam;
+        c.buildTeamUser = true;
+        nlist_rebuild(c, true);
+        refresh_xq(c);
+        // pair kernel
+        int bestPair = c.pairTeam;
+        double bestPairT = 1e30;
+        if (flags & MDX_PAIR) {
+            for (int t : {4, 8, 16, 32}) {
+                c.pairTeam = t;
+                const double ms = timed([&] { pair_forces(c, false, false); });
+                if (ms < bestPairT) {
+                    bestPairT = ms;
+                    bestPair = t;
+                }
+            }
+            c.pairTeam = bestPair;
+        }
+        // E3B
+        int bestE3b = c.e3b.team;
+        double bestE3bT = 1e30;
+        if ((flags & MDX_E3B) && c.e3b.on) {
+            for (int t : {4, 8, 16}) {
+                c.e3b.team = t;
+                const double ms = timed([&] { e3b_forces(c, false, false); });
+                if (ms < bestE3bT) {
+                    bestE3bT = ms;
+                    bestE3b = t;
+                }
+            }
+            c.e3b.team = bestE3b;
+        }
+        c.prof = prof;
+        MDX_HIP(hipEventDestroy(e0));
+        MDX_HIP(hipEventDestroy(e1));
+        // leave forces consistent with the current positions and flags
+        compute(c, c.forceFlags ? c.forceFlags : (flags & (MDX_PAIR | MDX_PPPM | MDX_E3B)));
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        if (chosen) {
+            chosen[0] = bestMode;
+            chosen[1] = bestTeam;
+            chosen[2] = bestPair;
+            chosen[3] = bestE3b;
+        }
+        if (avg_ms) {
+            avg_ms[0] = bestBuild;
+            avg_ms[1] = bestPairT < 1e30 ? bestPairT : 0.0;
+            avg_ms[2] = bestE3bT < 1e30 ? bestE3bT : 0.0;
+        }
     });
 }
 
@@ -861,6 +1136,10 @@ int mdx_option_set(mdx_ctx *ctx, const char *name, int value) {
         MDX_REQUIRE(name, MDX_ERR_ARG, "option name required");
         std::string k(name);
         if (k == "tiled_spread") c.mdxDisableTiledSpread = (value == 0);
+        else if (k == "graph") {
+            MDX_REQUIRE(value == 0 || value == 1, MDX_ERR_ARG, "graph: 0 eager steps, 1 captured steps");
+            c.graphSteps = value;
+        }
         else if (k == "build_mode") {
             MDX_REQUIRE(value >= 0 && value <= 2, MDX_ERR_ARG, "build_mode: 0 auto, 1 per cell, 2 per atom");
             c.buildMode = value;

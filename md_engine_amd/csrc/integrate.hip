@@ -10,7 +10,8 @@ namespace mdx {
 
 namespace {
 
-__global__ void k_zero_u64(int n, unsigned long long *__restrict__ p) {
+__global__ void k_zero_u64(const int *__restrict__ gate, int n, unsigned long long *__restrict__ p) {
+    if (gate && *gate == 0) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;
 }
@@ -53,8 +54,9 @@ __global__ void k_vv2(int n, double4 *__restrict__ vel, const unsigned long long
 
 // Sum of m v^2 over massive sites (FP64 per block, one fixed-point atomic per
 // block) -> thermo[0]; the temperature follows from the degrees of freedom.
-__global__ void __launch_bounds__(256) k_mv2(int n, const double4 *__restrict__ vel,
+__global__ void __launch_bounds__(256) k_mv2(const int *__restrict__ gate, int n, const double4 *__restrict__ vel,
                                              unsigned long long *__restrict__ thermo) {
+    if (gate && *gate == 0) return;
     __shared__ double wsum[4];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double s = 0.0;
@@ -75,8 +77,9 @@ __global__ void __launch_bounds__(256) k_mv2(int n, const double4 *__restrict__ 
 // Velocity rescaling toward T0: mode 0 sets T to T0 (FixNVTRescale,
 // src/Fixes/FixNVTRescale.cu), mode 1 is Berendsen weak coupling
 // lambda^2 = 1 + (interval dt / tau)(T0/T - 1).
-__global__ void k_vscale(int n, double4 *__restrict__ vel, const unsigned long long *__restrict__ thermo,
+__global__ void k_vscale(const int *__restrict__ gate, int n, double4 *__restrict__ vel, const unsigned long long *__restrict__ thermo,
                          double tconv, double T0, int mode, double coupling) {
+    if (gate && *gate == 0) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double T = (double)(long long)thermo[0] * kFxInv * tconv;
@@ -94,23 +97,24 @@ __global__ void k_vscale(int n, double4 *__restrict__ vel, const unsigned long l
 }  // namespace
 
 void thermo_measure(Ctx &c) {
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, 2, c.thermo);
-    hipLaunchKernelGGL(k_mv2, dim3(cdiv(c.n, 256)), dim3(256), 0, c.stream, c.n, c.vel[c.cur], c.thermo);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, thermo_gate(c), 2, c.thermo);
+    hipLaunchKernelGGL(k_mv2, dim3(cdiv(c.n, 256)), dim3(256), 0, c.stream, thermo_gate(c), c.n, c.vel[c.cur],
+                       c.thermo);
 }
 
 void thermostat_apply(Ctx &c) {
     thermo_measure(c);
     const double coupling = c.thermoTau > 0 ? c.thermoEvery * c.dt / c.thermoTau : 1.0;
-    hipLaunchKernelGGL(k_vscale, dim3(cdiv(c.n, 256)), dim3(256), 0, c.stream, c.n, c.vel[c.cur], c.thermo,
+    hipLaunchKernelGGL(k_vscale, dim3(cdiv(c.n, 256)), dim3(256), 0, c.stream, thermo_gate(c), c.n, c.vel[c.cur], c.thermo,
                        c.tconv, c.thermoT, c.thermoTau > 0 ? 1 : 0, coupling);
 }
 
 void zero_forces(Ctx &c, bool clear_acc) {
     const int threads = 256;
-    hipLaunchKernelGGL(k_zero_u64, dim3(cdiv(3LL * c.n, threads)), dim3(threads), 0, c.stream, 3 * c.n,
+    hipLaunchKernelGGL(k_zero_u64, dim3(cdiv(3LL * c.n, threads)), dim3(threads), 0, c.stream, nullptr, 3 * c.n,
                        c.force);
     if (clear_acc)
-        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, (int)ACC_N, c.acc);
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, nullptr, (int)ACC_N, c.acc);
 }
 
 void integrate_first_half(Ctx &c) {

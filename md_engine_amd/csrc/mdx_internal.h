@@ -142,7 +142,11 @@ struct Ctx {
     int cap = 0;
     uint32_t *nl = nullptr;
     int *nlCount = nullptr;
-    int *dFlags = nullptr;        // [0] rebuild flag, [1] max count, [2] e3b max count
+    // [0] rebuild flag, [1] max count, [2] e3b max count; captured steps:
+    // [3] rebuild gate, [4] rebuilds, [5] thermostat gate, [6] turn
+    int *dFlags = nullptr;
+    bool capture = false;         // a step is being recorded into a graph
+    int graphSteps = 0;           // mdx_run: 1 = captured steps (option "graph")
     unsigned *dMax = nullptr;     // 2 slots x (max |x - x_last|^2, max predicted^2) as float bits
     int *pinned = nullptr;        // host-pinned: [0..1] last build's max counts, [2..5] dMax copies
     hipEvent_t capEvent = nullptr;
@@ -276,17 +280,22 @@ __device__ __forceinline__ double team_sum_d(double v) {
     return v;
 }
 
+// device words that gate kernels inside a captured step (nullptr: always run)
+inline const int *rebuild_gate(const Ctx &c) { return c.capture ? c.dFlags + 3 : nullptr; }
+inline const int *thermo_gate(const Ctx &c) { return c.capture ? c.dFlags + 5 : nullptr; }
+
 // per-module entry points ------------------------------------------------------
 // nlist.hip
 void nlist_setup(Ctx &c);
 void nlist_check(Ctx &c);             // sets dFlags[0] when a rebuild is due
 void nlist_rebuild(Ctx &c, bool forced);  // rebuild if dFlags[0] or forced (host reads flag)
 void refresh_xq(Ctx &c);
-void fill_int(Ctx &c, int *p, int count, int v);  // stream-ordered fill (no memset nodes)
+void fill_int(Ctx &c, int *p, int count, int v, const int *gate = nullptr);  // kernel fill (no memset nodes)
 void nlist_rebuild_async(Ctx &c);     // rebuild now; capacity checked later (capacity_ok)
 bool capacity_ok(Ctx &c);             // false: rows grew, results since the last check are invalid
 bool grow_if_needed(Ctx &c);
 void check_max(Ctx &c, int parity);   // max displacement / speed into dMax slot `parity`
+void decide_step(Ctx &c);             // captured step: device rebuild / thermostat gates
 
 // pair.hip
 void pair_forces(Ctx &c, bool energy, bool virial);

@@ -68,9 +68,10 @@ __global__ void k_check(int n, const double4 *__restrict__ pos, const double4 *_
 }
 
 // Wrap into the box (periodicWrap) and bin into cells (countNumInGridCells).
-__global__ void k_bin(int n, double4 *__restrict__ pos, Geometry g, CellGrid cg,
+__global__ void k_bin(const int *__restrict__ gate, int n, double4 *__restrict__ pos, Geometry g, CellGrid cg,
                       const int *__restrict__ colRank, int *__restrict__ cellOf,
                       int *__restrict__ cellCount) {
+    if (gate && *gate == 0) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double4 p = pos[i];
@@ -93,8 +94,9 @@ __global__ void k_bin(int n, double4 *__restrict__ pos, Geometry g, CellGrid cg,
 
 // Exclusive scan of the cell counts by one workgroup (replaces the host
 // cumulativeSum round trip, helpers.h:15-24).  Also seeds the fill cursors.
-__global__ void __launch_bounds__(1024) k_scan(int ncell, const int *__restrict__ count,
+__global__ void __launch_bounds__(1024) k_scan(const int *__restrict__ gate, int ncell, const int *__restrict__ count,
                                                int *__restrict__ start, int *__restrict__ fill) {
+    if (gate && *gate == 0) return;
     __shared__ int wsum[16];
     __shared__ int carry;
     if (threadIdx.x == 0) carry = 0;
@@ -133,8 +135,9 @@ __global__ void __launch_bounds__(1024) k_scan(int ncell, const int *__restrict_
     if (threadIdx.x == 0) start[ncell] = carry;
 }
 
-__global__ void k_place(int n, const int *__restrict__ cellOf, int *__restrict__ fill,
+__global__ void k_place(const int *__restrict__ gate, int n, const int *__restrict__ cellOf, int *__restrict__ fill,
                         int *__restrict__ perm) {
+    if (gate && *gate == 0) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int slot = atomicAdd(&fill[cellOf[i]], 1);
@@ -144,10 +147,11 @@ __global__ void k_place(int n, const int *__restrict__ cellOf, int *__restrict__
 // Make the order inside each cell independent of atomic slot order: one
 // workgroup per cell ranks its entries by atom id in LDS (ids are unique, so
 // rank = number of smaller ids) and writes them in rank order.
-__global__ void __launch_bounds__(64) k_cellsort(int ncell, const int *__restrict__ start,
+__global__ void __launch_bounds__(64) k_cellsort(const int *__restrict__ gate, int ncell, const int *__restrict__ start,
                                                  const int *__restrict__ permIn,
                                                  int *__restrict__ permOut,
                                                  const int *__restrict__ id) {
+    if (gate && *gate == 0) return;
     constexpr int CAP = 1024;
     __shared__ int sid[CAP];
     __shared__ int sperm[CAP];
@@ -182,13 +186,14 @@ __global__ void __launch_bounds__(64) k_cellsort(int ncell, const int *__restric
 
 // Gather the per-atom records into cell order (sortPerAtomArrays,
 // GridGPU.cu:253-293) and refresh idToIdx, the last-build positions and xq.
-__global__ void k_permute(int n, const int *__restrict__ perm, const double4 *__restrict__ pos0,
+__global__ void k_permute(const int *__restrict__ gate, int n, const int *__restrict__ perm, const double4 *__restrict__ pos0,
                           const double4 *__restrict__ vel0, const float *__restrict__ q0,
                           const int *__restrict__ t0, const int *__restrict__ id0,
                           double4 *__restrict__ pos1, double4 *__restrict__ vel1,
                           float *__restrict__ q1, int *__restrict__ t1, int *__restrict__ id1,
                           int *__restrict__ idToIdx, double4 *__restrict__ last,
                           float4 *__restrict__ xq, Geometry g) {
+    if (gate && *gate == 0) return;
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     int o = perm[k];
@@ -211,6 +216,38 @@ __global__ void k_xq(int n, const double4 *__restrict__ pos, const float *__rest
     if (k >= n) return;
     double4 p = pos[k];
     xq[k] = make_float4((float)(p.x - g.lo[0]), (float)(p.y - g.lo[1]), (float)(p.z - g.lo[2]), q[k]);
+}
+
+__global__ void k_copy_back(const int *__restrict__ gate, int n, const double4 *__restrict__ pos1,
+                            const double4 *__restrict__ vel1, const float *__restrict__ q1,
+                            const int *__restrict__ t1, const int *__restrict__ id1, double4 *__restrict__ pos0,
+                            double4 *__restrict__ vel0, float *__restrict__ q0, int *__restrict__ t0,
+                            int *__restrict__ id0) {
+    if (gate && *gate == 0) return;
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    pos0[k] = pos1[k];
+    vel0[k] = vel1[k];
+    q0[k] = q1[k];
+    t0[k] = t1[k];
+    id0[k] = id1[k];
+}
+
+// Device-side step decisions of a captured step (one thread): rebuild the
+// list iff the largest displacement since the last build exceeds skin/2 (the
+// reference's setBuildFlag test, GridGPU.cu:755-774), count it, and set the
+// thermostat gate for the steps where the turn reaches a multiple of its
+// interval.  Clears the displacement slot for the next step.
+__global__ void k_decide(unsigned *__restrict__ dmax, int *__restrict__ flags, float half2, int every) {
+    const float d2 = __uint_as_float(dmax[0]);
+    const int build = d2 > half2 ? 1 : 0;
+    flags[3] = build;
+    flags[4] += build;
+    const int turn = flags[6] + 1;
+    flags[6] = turn;
+    flags[5] = (every > 0 && turn % every == 0) ? 1 : 0;
+    dmax[0] = 0u;
+    dmax[1] = 0u;
 }
 
 // Stencil walk without local arrays (a dynamically indexed local array would
@@ -377,11 +414,12 @@ __device__ __forceinline__ bool in_run(uint32_t ex, int s0, int s1) {
 }
 
 template <int T, bool SHIFT>
-__global__ void __launch_bounds__(256) k_build(
+__global__ void __launch_bounds__(256) k_build(const int *__restrict__ gate, 
     int n, const double4 *__restrict__ pos, const float4 *__restrict__ xq, Geometry g, CellGrid cg,
     Prefilter pf, Cull cu, const int *__restrict__ colRank, const int *__restrict__ cellStart,
     double rlist2, const uint32_t *__restrict__ exclIdx, int maxExcl, int cap,
     uint32_t *__restrict__ nl, int *__restrict__ nlCount, int *__restrict__ flags) {
+    if (gate && *gate == 0) return;
     constexpr int U = 64 / T;  // chunks of T candidates per load block
     __shared__ int4 sRuns[256 / T][kMaxCols];
     __shared__ int sImg[256 / T][kMaxCols];  // (kx+1) | (ky+1) << 2 | (kz0+1) << 4 | (kz1+1) << 6
@@ -517,9 +555,10 @@ __global__ void __launch_bounds__(256) k_build(
 
 // Exclusion partners of each atom as current indices (idx | tag << 30), one
 // row of maxExcl per atom padded with ~0u, refreshed after each permutation.
-__global__ void k_excl_idx(int n, const int *__restrict__ id, const int *__restrict__ exclOff,
+__global__ void k_excl_idx(const int *__restrict__ gate, int n, const int *__restrict__ id, const int *__restrict__ exclOff,
                            const uint32_t *__restrict__ exclIds, const int *__restrict__ idToIdx,
                            int maxExcl, uint32_t *__restrict__ exclIdx) {
+    if (gate && *gate == 0) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int myid = id[i];
@@ -548,11 +587,12 @@ __global__ void k_excl_idx(int n, const int *__restrict__ id, const int *__restr
 constexpr int kStage = 3072;
 constexpr int kSlots = 5;  // 2 reach + 1
 
-__global__ void __launch_bounds__(256) k_build_cell(
+__global__ void __launch_bounds__(256) k_build_cell(const int *__restrict__ gate, 
     const double4 *__restrict__ pos, const float4 *__restrict__ xq, Geometry g, CellGrid cg,
     Prefilter pf, Cull cu, const int2 *__restrict__ colXY, const int *__restrict__ colRank,
     const int *__restrict__ cellStart, double rlist2, const uint32_t *__restrict__ exclIdx, int maxExcl,
     int cap, uint32_t *__restrict__ nl, int *__restrict__ nlCount, int *__restrict__ flags) {
+    if (gate && *gate == 0) return;
     __shared__ float4 sXq[kStage];
     __shared__ int sZoff[kMaxCols][kSlots + 1];  // virtual offset of each z-slot, and the column end
     __shared__ int sGz[kMaxCols][kSlots];        // first candidate index of each z-slot
@@ -717,11 +757,12 @@ __global__ void __launch_bounds__(256) k_build_cell(
 // molecule GridGPU of FixE3B (FixE3B.cu:656-684) that the reference rebuilds
 // every step; with the skin criterion the per-step r2b set is a subset.
 template <int T>
-__global__ void __launch_bounds__(256) k_e3b_list(
+__global__ void __launch_bounds__(256) k_e3b_list(const int *__restrict__ gate, 
     int nmol, const int4 *__restrict__ molAtoms, const int *__restrict__ atomMolO,
     const double4 *__restrict__ pos, Geometry g, double rcut2, int cap, const uint32_t *__restrict__ nl,
     const int *__restrict__ nlCount, int mcap, int *__restrict__ mnl, int *__restrict__ mcount,
     int *__restrict__ flags) {
+    if (gate && *gate == 0) return;
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int m = gtid / T;
     const int tl = threadIdx.x & (T - 1);
@@ -757,8 +798,9 @@ __global__ void __launch_bounds__(256) k_e3b_list(
     if (lane == 0) atomicMax(&flags[2], wmax);
 }
 
-__global__ void k_mol_atoms(int nmol, const int4 *__restrict__ molIds, const int *__restrict__ idToIdx,
+__global__ void k_mol_atoms(const int *__restrict__ gate, int nmol, const int4 *__restrict__ molIds, const int *__restrict__ idToIdx,
                             int4 *__restrict__ molAtoms, int *__restrict__ atomMolO) {
+    if (gate && *gate == 0) return;
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nmol) return;
     int4 ids = molIds[m];
@@ -767,7 +809,8 @@ __global__ void k_mol_atoms(int nmol, const int4 *__restrict__ molIds, const int
     atomMolO[a.x] = m;
 }
 
-__global__ void k_fill_int(int n, int *__restrict__ p, int v) {
+__global__ void k_fill_int(const int *__restrict__ gate, int n, int *__restrict__ p, int v) {
+    if (gate && *gate == 0) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
@@ -847,7 +890,7 @@ void launch_build(Ctx &c, const CellGrid &cg, int cur) {
     for (int a = 0; a < 3; ++a)
         if (c.geo.periodic[a] && c.nc[a] < 2 * c.reach + 1) shift = false;
     auto kern = shift ? k_build<T, true> : k_build<T, false>;
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.xq, c.geo, cg,
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), 0, c.stream, rebuild_gate(c), c.n, c.pos[cur], c.xq, c.geo, cg,
                        make_prefilter(c), make_cull(c), c.colRank, c.cellStart, c.rlist * c.rlist,
                        c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.dFlags);
 }
@@ -857,7 +900,7 @@ void launch_e3b_list(Ctx &c, int cur) {
     int threads = 256;
     int blocks = cdiv((long long)c.e3b.nmol * T, threads);
     double rc = (double)c.e3b.r2b + c.skin;
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_list<T>), dim3(blocks), dim3(threads), 0, c.stream,
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_list<T>), dim3(blocks), dim3(threads), 0, c.stream, rebuild_gate(c),
                        c.e3b.nmol, c.e3b.molAtoms, c.e3b.atomMolO, c.pos[cur], c.geo, rc * rc, c.cap,
                        c.nl, c.nlCount, c.e3b.cap, c.e3b.nl, c.e3b.nlCount, c.dFlags);
 }
@@ -945,47 +988,57 @@ static void run_build(Ctx &c, bool sync) {
     CellGrid cg = make_grid(c);
     int threads = 256;
     int cur = c.cur, nxt = 1 - c.cur;
-    hipLaunchKernelGGL(k_fill_int, dim3(cdiv(c.ncell, threads)), dim3(threads), 0, c.stream, c.ncell,
+    hipLaunchKernelGGL(k_fill_int, dim3(cdiv(c.ncell, threads)), dim3(threads), 0, c.stream, rebuild_gate(c), c.ncell,
                        c.cellCount, 0);
-    hipLaunchKernelGGL(k_bin, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur],
+    hipLaunchKernelGGL(k_bin, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, rebuild_gate(c), c.n, c.pos[cur],
                        c.geo, cg, c.colRank, c.cellOf, c.cellCount);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c.stream, c.ncell, c.cellCount, c.cellStart,
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c.stream, rebuild_gate(c), c.ncell, c.cellCount, c.cellStart,
                        c.cellFill);
-    hipLaunchKernelGGL(k_place, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.cellOf,
+    hipLaunchKernelGGL(k_place, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, rebuild_gate(c), c.n, c.cellOf,
                        c.cellFill, c.perm);
-    hipLaunchKernelGGL(k_cellsort, dim3(c.ncell), dim3(64), 0, c.stream, c.ncell, c.cellStart, c.perm,
+    hipLaunchKernelGGL(k_cellsort, dim3(c.ncell), dim3(64), 0, c.stream, rebuild_gate(c), c.ncell, c.cellStart, c.perm,
                        c.perm2, c.id[cur]);
-    hipLaunchKernelGGL(k_permute, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.perm2,
+    hipLaunchKernelGGL(k_permute, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, rebuild_gate(c), c.n, c.perm2,
                        c.pos[cur], c.vel[cur], c.q[cur], c.type[cur], c.id[cur], c.pos[nxt],
                        c.vel[nxt], c.q[nxt], c.type[nxt], c.id[nxt], c.idToIdx, c.posLast, c.xq,
                        c.geo);
-    c.cur = nxt;
+    if (c.capture) {
+        // a captured step keeps fixed buffers: the sorted records come back
+        hipLaunchKernelGGL(k_copy_back, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, rebuild_gate(c),
+                           c.n, c.pos[nxt], c.vel[nxt], c.q[nxt], c.type[nxt], c.id[nxt], c.pos[cur],
+                           c.vel[cur], c.q[cur], c.type[cur], c.id[cur]);
+    } else {
+        c.cur = nxt;
+    }
     if (c.exclOff && c.maxExcl > 0) {
+        MDX_REQUIRE(!c.capture || c.exclIdxN == (size_t)c.n * c.maxExcl, 3,
+                    "exclusion index rows must exist before a step is captured");
         if (c.exclIdxN != (size_t)c.n * c.maxExcl) {
             if (c.exclIdx) dfree(c, c.exclIdx);
             c.exclIdxN = (size_t)c.n * c.maxExcl;
             c.exclIdx = dalloc<uint32_t>(c, c.exclIdxN);
         }
-        hipLaunchKernelGGL(k_excl_idx, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n,
+        hipLaunchKernelGGL(k_excl_idx, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, rebuild_gate(c), c.n,
                            c.id[c.cur], c.exclOff, c.exclIds, c.idToIdx, c.maxExcl, c.exclIdx);
     }
     rigid_map(c);
     if (c.e3b.on) {
-        fill_int(c, c.e3b.atomMolO, c.n, -1);  // indices were permuted: drop the old map
-        hipLaunchKernelGGL(k_mol_atoms, dim3(cdiv(c.e3b.nmol, threads)), dim3(threads), 0, c.stream,
+        fill_int(c, c.e3b.atomMolO, c.n, -1, rebuild_gate(c));  // indices were permuted: drop the old map
+        hipLaunchKernelGGL(k_mol_atoms, dim3(cdiv(c.e3b.nmol, threads)), dim3(threads), 0, c.stream, rebuild_gate(c),
                            c.e3b.nmol, c.e3b.molIds, c.idToIdx, c.e3b.molAtoms, c.e3b.atomMolO);
         MDX_REQUIRE(c.rlist >= (double)c.e3b.r2b + c.skin - 1e-9, 1,
                     "E3B needs rCut + padding >= 6.2 A + padding (the molecule list is taken from the "
                     "pair list)");
     }
     for (;;) {
-        fill_int(c, c.dFlags + 1, 2, 0);
+        // a captured step keeps the running maximum of the chunk in dFlags[1..2]
+        if (!c.capture) fill_int(c, c.dFlags + 1, 2, 0);
         {
             ProfScope ps(c, PK_BUILD);
             const double perCell = (double)c.n / c.ncell;
             if (c.buildMode == 1 || (c.buildMode == 0 && perCell >= 8.0)) {
                 MDX_REQUIRE(c.reach == 2, 1, "the per-cell build kernel assumes a 5-cell stencil");
-                hipLaunchKernelGGL(k_build_cell, dim3(c.ncell), dim3(256), 0, c.stream, c.pos[c.cur],
+                hipLaunchKernelGGL(k_build_cell, dim3(c.ncell), dim3(256), 0, c.stream, rebuild_gate(c), c.pos[c.cur],
                                    c.xq, c.geo, cg, make_prefilter(c), make_cull(c), c.colXY, c.colRank,
                                    c.cellStart, c.rlist * c.rlist, c.exclIdx, c.exclOff ? c.maxExcl : 0,
                                    c.cap, c.nl, c.nlCount, c.dFlags);
@@ -1005,6 +1058,7 @@ static void run_build(Ctx &c, bool sync) {
                 default: launch_e3b_list<32>(c, c.cur); break;
             }
         }
+        if (c.capture) return;  // rows checked by the host after the chunk
         // largest row counts -> pinned host memory; checked now (sync) or at
         // the next synchronisation point (capacity_ok)
         MDX_HIP(hipMemcpyAsync(c.pinned, c.dFlags + 1, sizeof(int) * 2, hipMemcpyDeviceToHost, c.stream));
@@ -1067,8 +1121,8 @@ void nlist_rebuild_async(Ctx &c) {
     run_build(c, !c.listValid);
 }
 
-void fill_int(Ctx &c, int *p, int count, int v) {
-    hipLaunchKernelGGL(k_fill_int, dim3(cdiv(count, 256)), dim3(256), 0, c.stream, count, p, v);
+void fill_int(Ctx &c, int *p, int count, int v, const int *gate) {
+    hipLaunchKernelGGL(k_fill_int, dim3(cdiv(count, 256)), dim3(256), 0, c.stream, gate, count, p, v);
 }
 
 namespace {
@@ -1105,6 +1159,13 @@ __global__ void k_check_max(int n, const double4 *__restrict__ pos, const double
     }
 }
 }  // namespace
+
+void decide_step(Ctx &c) {
+    check_max(c, 0);
+    const float half = (float)(0.5 * c.skin);
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(1), 0, c.stream, c.dMax, c.dFlags, half * half,
+                       c.thermoOn ? c.thermoEvery : 0);
+}
 
 void check_max(Ctx &c, int parity) {
     int threads = 256;

@@ -206,8 +206,9 @@ __global__ void k_rigid_vv2(RigidArgs r, double4 *__restrict__ pos, double4 *__r
 
 // molecule -> current site indices, and the per-site membership flag that
 // keeps the plain integrator off these sites
-__global__ void k_rigid_map(int nmol, const int4 *__restrict__ ids, const int *__restrict__ idToIdx,
+__global__ void k_rigid_map(const int *__restrict__ gate, int nmol, const int4 *__restrict__ ids, const int *__restrict__ idToIdx,
                             int4 *__restrict__ atoms, int *__restrict__ member) {
+    if (gate && *gate == 0) return;
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nmol) return;
     const int4 i = ids[m];
@@ -238,8 +239,8 @@ RigidArgs make_args(const Ctx &c) {
 
 void rigid_map(Ctx &c) {
     if (!c.rigid.on) return;
-    fill_int(c, c.rigid.member, c.n, 0);
-    hipLaunchKernelGGL(k_rigid_map, dim3(cdiv(c.rigid.nmol, 128)), dim3(128), 0, c.stream, c.rigid.nmol,
+    fill_int(c, c.rigid.member, c.n, 0, rebuild_gate(c));
+    hipLaunchKernelGGL(k_rigid_map, dim3(cdiv(c.rigid.nmol, 128)), dim3(128), 0, c.stream, rebuild_gate(c), c.rigid.nmol,
                        c.rigid.ids, c.idToIdx, c.rigid.atoms, c.rigid.member);
 }
 

@@ -1,0 +1,113 @@
+"""Captured MD steps (hipGraph) and batched replicas on the GPU.
+
+A captured step decides list rebuilds and thermostat turns on the device, so
+its trajectory is checked against the FP64 oracle integrator like the eager
+path; replicas advanced together in one graph must equal the same replicas
+advanced alone, bit for bit (fixed-point force sums make the step
+deterministic)."""
+import sys
+
+import numpy as np
+import pytest
+
+from md_engine_amd import systems
+
+from test_gpu_water_md import G, min_image_diff, start_velocities, water_md_engine
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("nsteps", [1, 15])
+def test_graph_trajectory_matches_oracle(orc, gpu_engine_factory, nsteps):
+    say(f"captured water trajectory {nsteps} steps vs oracle")
+    box = systems.tip4p2005_box(216, seed=21, min_sep=1.8)
+    v = start_velocities(orc, box)
+    kw = dict(rc=7.0, skin=1.0, sz=(32, 32, 32), order=5)
+    e, flags = water_md_engine(gpu_engine_factory, box, v, e3b=True, dt=1.0, **kw)
+    e.option("graph", 1)
+    e.run(nsteps, flags)
+    xg, vg = e.get_system()
+    xo, vo, _ = orc.water_md(box, v, nsteps, dt=1.0, e3b_on=True, **kw)
+    dx = np.abs(min_image_diff(xg, xo, box["L"])).max()
+    assert dx < (1e-8 if nsteps == 1 else 1e-6), dx
+    heavy = box["mass"] > 0
+    rel_v = np.abs(vg[heavy] - vo[heavy]).max() / np.abs(vo[heavy]).max()
+    assert rel_v < (1e-6 if nsteps == 1 else 1e-4), rel_v
+
+
+def test_graph_long_run_constraints_rebuilds(orc, gpu_engine_factory):
+    """300 captured steps (37 full graphs + a 4-step tail): the device-side
+    rebuild decision keeps rebuilding and the constraints hold."""
+    say("captured water 300 steps")
+    box = systems.tip4p2005_box(343, seed=31, min_sep=1.8)
+    v = start_velocities(orc, box, seed=4)
+    e, flags = water_md_engine(gpu_engine_factory, box, v, rc=8.0, skin=1.0, sz=(32, 32, 32), order=5,
+                               e3b=True, dt=2.0)
+    e.option("graph", 1)
+    e.run(300, flags)
+    st = e.nlist_stats()
+    assert st["rebuilds"] > 3
+    x, vv = e.get_system()
+    for o, h1, h2, m in box["mol"]:
+        d = min_image_diff(x[[h1, h2]], x[o], box["L"])
+        assert np.allclose(np.linalg.norm(d, axis=1), G["dOH"], atol=1e-9)
+        for i, j in ((o, h1), (o, h2), (h1, h2)):
+            r = min_image_diff(x[i], x[j], box["L"])
+            assert abs(np.dot(vv[i] - vv[j], r)) < 1e-9
+    # the list built by the captured steps is still the oracle's list
+    from test_gpu_parity import assert_same_lists, oracle_rows_by_id
+    excl = orc.exclusions_bfs(len(box["x"]), systems.water_bonds(box["mol"]))
+    e.nlist_update(force=True)
+    gc, grows = e.nlist_get()
+    oc, orows, _ = oracle_rows_by_id(orc, x, box["lo"], box["L"], 9.0, excl=excl, brute=False)
+    assert_same_lists(gc, grows, oc, orows)
+
+
+def test_graph_thermostat_turns(orc, gpu_engine_factory):
+    """The device turn counter applies the rescale on the same turns as the
+    eager loop: after 100 steps with every=5 the temperature is the target."""
+    say("captured thermostat")
+    box = systems.tip4p2005_box(216, seed=23, min_sep=1.8)
+    v = start_velocities(orc, box, seed=3)
+    e, flags = water_md_engine(gpu_engine_factory, box, v, rc=7.0, skin=1.0, sz=(32, 32, 32), order=5,
+                               e3b=False, dt=1.0)
+    kb, mvv = systems.REAL_BOLTZ, systems.REAL_MVV_TO_ENG
+    e.thermostat(True, 350.0, 5, 0.0, kb, mvv)
+    e.option("graph", 1)
+    e.run(37, flags)
+    e.run(63, flags)           # 100 % 5 == 0: ends right after a rescale
+    T, _ = e.temperature(kb, mvv)
+    assert T == pytest.approx(350.0, rel=1e-6)
+
+
+def test_batch_equals_single(orc, gpu_engine_factory):
+    """Three replicas of different sizes in one graph == each alone."""
+    say("batched replicas vs single runs")
+    from md_engine_amd import run_batch
+
+    specs = [(216, 41), (125, 42), (343, 43)]
+    boxes = [systems.tip4p2005_box(m, seed=s, min_sep=1.8) for m, s in specs]
+    vels = [start_velocities(orc, b, seed=s) for b, (_, s) in zip(boxes, specs)]
+
+    def make(b, v):
+        e, flags = water_md_engine(gpu_engine_factory, b, v, rc=6.5, skin=0.9, sz=(32, 32, 32), order=5,
+                                   e3b=True, dt=2.0)
+        e.option("graph", 1)
+        return e, flags
+
+    batch = [make(b, v) for b, v in zip(boxes, vels)]
+    flags = batch[0][1]
+    run_batch([e for e, _ in batch], 70, flags)
+    for (eb, _), b, v in zip(batch, boxes, vels):
+        es, fl = make(b, v)
+        es.run(70, fl)
+        xb, vb = eb.get_system()
+        xs, vs = es.get_system()
+        assert np.array_equal(xb, xs)
+        assert np.array_equal(vb, vs)
+        assert eb.nlist_stats()["rebuilds"] == es.nlist_stats()["rebuilds"]

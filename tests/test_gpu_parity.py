@@ -341,3 +341,29 @@ def test_lj_md_short_trajectory_and_rebuilds(orc, gpu_engine_factory):
     gc, grows = e.nlist_get()
     oc, orows, _ = oracle_rows_by_id(orc, xg, lo, L, 2.8, brute=False)
     assert_same_lists(gc, grows, oc, orows)
+
+
+# ---------------------------------------------------------------------------
+# autotuner
+# ---------------------------------------------------------------------------
+def test_tune_keeps_lists_and_forces(orc, gpu_engine_factory):
+    """The autotuner only changes launch geometry: after it, the list is still
+    the oracle's and the full force matches FP64."""
+    say("autotuner on 512-molecule water")
+    from md_engine_amd._lib import E3B, PAIR, PPPM
+
+    box = systems.tip4p2005_box(512, seed=5)
+    e = water_engine(gpu_engine_factory, box, e3b=True)
+    e.compute(PAIR | PPPM | E3B)
+    choice = e.tune(PAIR | PPPM | E3B, evals=3)
+    assert choice["build_mode"] in (1, 2) and choice["build_team"] in (8, 16, 32, 64)
+    assert choice["pair_team"] in (4, 8, 16, 32) and choice["e3b_team"] in (4, 8, 16)
+    assert choice["build_ms"] > 0 and choice["pair_ms"] > 0 and choice["e3b_ms"] > 0
+    gc, grows = e.nlist_get()
+    excl = orc.exclusions_bfs(len(box["x"]), systems.water_bonds(box["mol"]))
+    oc, orows, _ = oracle_rows_by_id(orc, box["x"], box["lo"], box["L"], 10.0, excl=excl, brute=False)
+    assert_same_lists(gc, grows, oc, orows)
+    ref = oracle_water(orc, box)
+    fo, _, _ = orc.e3b(box["mol"], box["x"], box["L"])
+    e.compute(PAIR | PPPM | E3B)
+    assert rel_rmse(e.forces(), ref["fp"] + ref["fk"] + fo) <= FORCE_RMSE
