@@ -4,10 +4,19 @@
 Default workload: TIP4P/2005 + E3B3 water, 8,192 molecules (32,768 sites
 counting M), cubic box at 0.997 g/cm^3 (L = 62.64 A), LJ O-O + Ewald real
 space at rc = 9 A with a 1 A skin, P3M 64^3 order 5, E3B3 (5.0-5.2 A switch,
-two-body O-O correction).  One step = the force path of one MD step: skin
-check, neighbor-list rebuild every --rebuild-every steps, LJ + Ewald real
-space, P3M spread/FFT/field/FFT/gather, E3B two passes.  Positions are the
-synthetic lattice start (seeded); inputs are resident in HBM before timing.
+two-body O-O correction), rigid molecules (FP64 SETTLE / RATTLE), dt = 2 fs,
+298 K velocity rescaling every 10 steps.  One step = one MD step: kick-drift
+and constraints, skin check and list rebuild when due, LJ + Ewald real space,
+P3M spread/FFT/field/FFT/gather, E3B, kick and constraints.  The lattice start
+is equilibrated (untimed) first; inputs are resident in HBM before timing.
+
+Modes: md (default; --graph 1 replays captured 8-step hipGraphs with device-
+side rebuild decisions), force (force path at fixed positions, list rebuilt
+every --rebuild-every steps), batch (48 uneven TIP4P/2005 boxes of 512-4096
+molecules advanced by one captured graph).  --three-body surrogate swaps the
+E3B three-body term for the learned surrogate: it is trained on the GPU from
+frames of the equilibration (untimed), the timed run uses it, and the line
+reports its three-body force RMSE against E3B in kJ/mol/nm.
 
 Prints one JSON line: atom-steps/s (sites x steps / s), ms per step, the
 roofline of the dominant kernel (HIP events on the library stream during the
@@ -174,6 +183,93 @@ def copy_probe(mib=256, reps=5):
                       "launches": reps}))
 
 
+def train_surrogate(e, w, flags, frames=20, every=100, epochs=60, batch=8192):
+    """Record `frames` frames of the thermostatted run (every `every` steps)
+    and fit the surrogate on the GPU; returns fit statistics."""
+    from md_engine_amd import SURROGATE  # noqa: F401
+    from md_engine_amd._lib import Engine
+
+    nm = len(w["box"]["mol"])
+    t0 = time.perf_counter()
+    for k in range(frames):
+        e.run(every, flags)
+        e.surrogate_record(reset=(k == 0), capacity=frames * nm)
+    rng = np.random.default_rng(7)
+    p = _init_surrogate(rng)
+    e.surrogate_set(p)
+    l0 = e.surrogate_fit(1, batch=batch, lr=1e-3, seed=1, normalize=True)
+    l1 = e.surrogate_fit(epochs, batch=batch, lr=1e-3, seed=2)
+    l2 = e.surrogate_fit(epochs // 2, batch=batch, lr=2e-4, seed=3)
+    return dict(frames=frames, samples=frames * nm, epochs=epochs + epochs // 2 + 1, loss_first=l0,
+                loss_final=l2, train_s=time.perf_counter() - t0, model_params=Engine.SUR_NPARAM)
+
+
+def _init_surrogate(rng):
+    nf, nh = 48, 64
+    parts = [rng.normal(0, 1 / np.sqrt(nf), nf * nh), np.zeros(nh), rng.normal(0, 1 / np.sqrt(nh), nh * nh),
+             np.zeros(nh), rng.normal(0, 1 / np.sqrt(nh), nh), [0.0], np.zeros(nf), np.ones(nf), [1.0, 0.0]]
+    return np.concatenate([np.asarray(a, np.float64) for a in parts]).astype(np.float32)
+
+
+def three_body_rmse(e):
+    """RMSE over atoms and components of surrogate vs E3B forces (the two-body
+    term is identical), in kcal/mol/A and kJ/mol/nm."""
+    from md_engine_amd import E3B, SURROGATE
+
+    e.compute(E3B)
+    fe = e.forces()
+    e.compute(SURROGATE)
+    fs = e.forces()
+    rms = float(np.sqrt(((fs - fe) ** 2).mean()))
+    return rms, rms * 41.84
+
+
+def batch_workload(seed=20261202, nbox=48):
+    from md_engine_amd import systems
+
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(512, 4097, nbox)
+    return [systems.tip4p2005_box(int(m), seed=int(seed + k)) for k, m in enumerate(sizes)]
+
+
+def run_batch_mode(args):
+    from md_engine_amd import E3B, PAIR, PPPM, run_batch, systems
+
+    boxes = batch_workload()
+    engines = []
+    flags = PAIR | PPPM | E3B
+    for b in boxes:
+        w = dict(box=b, rc=9.0, skin=1.0, sz=(32, 32, 32) if b["L"][0] < 40 else (64, 64, 64), order=5,
+                 e3b=True, dt_fs=2.0, desc="")
+        e, fl = setup_engine(w)
+        setup_md(e, w)
+        engines.append(e)
+    n = sum(len(b["x"]) for b in boxes)
+    run_batch(engines, args.equil, flags)
+    run_batch(engines, args.warmup, flags)
+    for e in engines:
+        e.sync()
+    t0 = time.perf_counter()
+    run_batch(engines, args.steps, flags)
+    for e in engines:
+        e.sync()
+    wall = time.perf_counter() - t0
+    out = {
+        "metric": "atom-steps/s", "value": n * args.steps / wall, "unit": "atom-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 pair/3-body arithmetic, f64 positions, i64 fixed-point accumulation",
+        "data": "synthetic (seeded lattices, random orientations)",
+        "config": {"workload": "48 TIP4P/2005+E3B3 boxes, %d-%d molecules (seeded uniform), %d sites in all"
+                               % (min(len(b["mol"]) for b in boxes), max(len(b["mol"]) for b in boxes), n),
+                   "step": "one captured hipGraph per 8 MD steps holding every replica",
+                   "pimd_ring": "not built"},
+    }
+    for e in engines:
+        e.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,8 +278,10 @@ def main():
     ap.add_argument("--config", default="tip4p-e3b", choices=["tip4p-e3b", "tip4p", "lj"])
     ap.add_argument("--rebuild-every", type=int, default=10,
                     help="force mode: forced list rebuild interval")
-    ap.add_argument("--mode", default="force", choices=["force", "md"],
-                    help="force: force path at fixed positions; md: full rigid-water MD steps")
+    ap.add_argument("--mode", default="md", choices=["md", "force", "batch"],
+                    help="md: rigid-water MD steps; force: force path at fixed positions; batch: 48 replicas")
+    ap.add_argument("--graph", type=int, default=1, help="md mode: 1 = captured hipGraph steps")
+    ap.add_argument("--three-body", default="e3b", choices=["e3b", "surrogate"])
     ap.add_argument("--equil", type=int, default=3000,
                     help="md mode: untimed thermostatted steps before warm-up (lattice start)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -191,6 +289,7 @@ def main():
     ap.add_argument("--build-team", type=int, default=0)
     ap.add_argument("--e3b-team", type=int, default=0)
     ap.add_argument("--build-mode", type=int, default=0, help="0 auto, 1 per-cell LDS, 2 per-atom")
+    ap.add_argument("--tune", action="store_true", help="run the autotuner before timing")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of known size (counter calibration)")
     args = ap.parse_args()
@@ -201,6 +300,11 @@ def main():
         print(f"bench.py runs on exactly one GPU (asked for --gpus {args.gpus}); this build has no "
               "multi-GPU path", file=sys.stderr)
         sys.exit(2)
+    if args.mode == "batch":
+        run_batch_mode(args)
+        return
+
+    from md_engine_amd import E3B, SURROGATE
 
     w = make_workload(args.config)
     e, flags = setup_engine(w)
@@ -208,15 +312,25 @@ def main():
     e.option("build_mode", args.build_mode)
     n = len(w["box"]["x"])
     R = args.rebuild_every
+    surrogate = args.three_body == "surrogate"
+    if surrogate and not (flags & E3B):
+        raise SystemExit("--three-body surrogate needs the tip4p-e3b config")
 
     md = args.mode == "md"
+    train = None
     if md:
         setup_md(e, w)
+        if args.graph:
+            e.option("graph", 1)
         step = lambda k: e.run(k, flags)  # noqa: E731
         if args.equil:
             step(args.equil)
+        if surrogate:
+            train = train_surrogate(e, w, flags)
+            flags = (flags & ~E3B) | SURROGATE
     else:
         step = lambda k: e.force_steps(k, flags, R)  # noqa: E731
+    tuned = e.tune(flags, evals=10) if args.tune else None
     step(args.warmup)
     e.sync()
     rebuilds0 = e.nlist_stats()["rebuilds"]
@@ -230,6 +344,7 @@ def main():
     e.profile(False)
     pairs, mols = e.nlist_totals()
     st = e.nlist_stats()
+    rmse = three_body_rmse(e) if surrogate else None
 
     ms_step = wall / args.steps * 1e3
     value = n * args.steps / wall
@@ -263,8 +378,13 @@ def main():
         "config": {
             "workload": w["desc"],
             "step": ("MD step: velocity Verlet + FP64 SETTLE/RATTLE + M site, skin-driven list rebuilds, "
-                     "LJ+Ewald real, P3M, E3B" if md else
-                     "force path: list rebuild every %d steps, LJ+Ewald real, P3M, E3B" % R),
+                     "LJ+Ewald real, P3M, %s%s" % ("E3B two-body + learned three-body" if surrogate else "E3B",
+                                                  ", captured hipGraph (8 steps)" if args.graph else "")
+                     if md else "force path: list rebuild every %d steps, LJ+Ewald real, P3M, E3B" % R),
+            "three_body": args.three_body,
+            "surrogate_training": train,
+            "three_body_force_rmse_vs_e3b": ({"kcal_mol_A": rmse[0], "kJ_mol_nm": rmse[1]} if rmse else None),
+            "tuned": tuned,
             "list_rebuilds_timed": e.nlist_stats()["rebuilds"] - rebuilds0,
             "list_violations": e.nlist_stats()["violations"],
             "rc": w["rc"], "skin": w["skin"], "pppm_mesh": w["sz"], "pppm_order": w["order"],
@@ -289,7 +409,10 @@ def main():
         },
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w, R)
+        # in md mode the CPU sample amortises one build over the GPU run's
+        # measured rebuild interval
+        rb = out["config"]["list_rebuilds_timed"]
+        out["cpu_baseline"] = cpu_baseline(w, max(1, round(args.steps / rb)) if md and rb else R)
     e.close()
     print(json.dumps(out))
 

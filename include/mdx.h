@@ -35,6 +35,7 @@ typedef struct mdx_ctx mdx_ctx;
 #define MDX_E3B 4       /* E3B two- and three-body (FixE3B) */
 #define MDX_ENERGY 8    /* also accumulate energies */
 #define MDX_VIRIAL 16   /* also accumulate the virial tensor */
+#define MDX_SURROGATE 32 /* E3B with the learned three-body term (mdx_surrogate_set) */
 
 int mdx_version(void);
 
@@ -155,6 +156,28 @@ int mdx_run_batch(mdx_ctx *const *ctxs, int n, int nsteps, int flags);
  * lanes per atom in the list build and the pair kernel, lanes per molecule in
  * E3B.  0 keeps the current value.  Powers of two only. */
 int mdx_teams_set(mdx_ctx *ctx, int build_team, int pair_team, int e3b_team);
+/* Learned E3B three-body surrogate.  No counterpart in the reference (its
+ * FixE3B, src/Fixes/FixE3B.cu, only has the analytic three-body term); a
+ * model replaces that term under MDX_SURROGATE while the two-body E3B term
+ * stays analytic.  Model: per-molecule descriptors (16 radial Gaussians on
+ * the H..O distances: accepted, H1-donated, H2-donated channels), MLP
+ * 48 -> 64 -> 64 -> 1 with SiLU on BF16 MFMA tiles, forces from its
+ * hand-written backward pass.  params: kSurNParam = 7459 floats packed as
+ * W1[48][64] b1[64] W2[64][64] b2[64] w3[64] b3 mu[48] sd[48] escale eshift.
+ * record: appends the current frame (features + the E3B three-body energy
+ * owned by each molecule, the training target) to the device sample set;
+ * fit: Adam on the per-molecule energy MSE (normalize: set mu, sd, escale,
+ * eshift from the samples first); eval: per-molecule energies (nmol) and raw
+ * descriptor channels (nmol x 48), either may be NULL. */
+int mdx_surrogate_set(mdx_ctx *ctx, const float *params, int nparams);
+int mdx_surrogate_get(mdx_ctx *ctx, float *params, int nparams);
+int mdx_surrogate_record(mdx_ctx *ctx, int reset, int capacity);
+int mdx_surrogate_fit(mdx_ctx *ctx, int epochs, int batch, double lr, int seed, int normalize, double *loss);
+int mdx_surrogate_eval(mdx_ctx *ctx, float *emol, float *feat);
+/* E3B three-body energy owned by each molecule (donor of type A, centre of
+ * type B, acceptor of type C; the per-molecule shares of
+ * compute_E3B_energy_threebody, src/Evaluators/ThreeBodyE3B.h:553-918). */
+int mdx_e3b_molecule_energies(mdx_ctx *ctx, float *emol);
 /* Autotuner (Integrator::tune, src/Integrators/Integrator.cu:272-376, grid
  * search of Tunable knobs by timing): times `evals` runs of every list-build
  * kernel/team width, pair team width and E3B team width for the terms in

@@ -14,7 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmdx.so")
 
-PAIR, PPPM, E3B, ENERGY, VIRIAL = 1, 2, 4, 8, 16
+PAIR, PPPM, E3B, ENERGY, VIRIAL, SURROGATE = 1, 2, 4, 8, 16, 32
 
 # every symbol declared in include/mdx.h
 SYMBOLS = [
@@ -25,7 +25,9 @@ SYMBOLS = [
     "mdx_pppm_info", "mdx_e3b_set", "mdx_forces_compute", "mdx_forces_get", "mdx_energy_get",
     "mdx_integrator_set", "mdx_run", "mdx_teams_set", "mdx_sync", "mdx_force_steps",
     "mdx_profile_enable", "mdx_profile_get", "mdx_nlist_totals", "mdx_option_set", "mdx_rigid_set",
-    "mdx_thermostat_set", "mdx_temperature", "mdx_tune", "mdx_run_batch",
+    "mdx_thermostat_set", "mdx_temperature", "mdx_tune", "mdx_run_batch", "mdx_surrogate_set",
+    "mdx_surrogate_get", "mdx_surrogate_record", "mdx_surrogate_fit", "mdx_surrogate_eval",
+    "mdx_e3b_molecule_energies",
 ]
 
 # kernel ids of mdx_profile_get
@@ -96,6 +98,13 @@ def load():
     L.mdx_nlist_totals.argtypes = [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.mdx_tune.argtypes = [_vp, c, c, _ip, _dp]
     L.mdx_run_batch.argtypes = [C.POINTER(_vp), c, c, c]
+    _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    L.mdx_surrogate_set.argtypes = [_vp, _fp, c]
+    L.mdx_surrogate_get.argtypes = [_vp, _fp, c]
+    L.mdx_surrogate_record.argtypes = [_vp, c, c]
+    L.mdx_surrogate_fit.argtypes = [_vp, c, c, d, c, c, C.POINTER(d)]
+    L.mdx_surrogate_eval.argtypes = [_vp, _vp, _vp]
+    L.mdx_e3b_molecule_energies.argtypes = [_vp, _fp]
     for name in SYMBOLS:
         fn = getattr(L, name)
         if name not in ("mdx_last_error", "mdx_version"):
@@ -280,6 +289,39 @@ class Engine:
 
     def teams_set(self, build=0, pair=0, e3b=0):
         self._chk(self._L.mdx_teams_set(self._h, int(build), int(pair), int(e3b)), "mdx_teams_set")
+
+    # -- learned E3B three-body surrogate -----------------------------------
+    SUR_NPARAM = 7459
+
+    def surrogate_set(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        self._chk(self._L.mdx_surrogate_set(self._h, p, len(p)), "mdx_surrogate_set")
+
+    def surrogate_get(self):
+        p = np.zeros(self.SUR_NPARAM, np.float32)
+        self._chk(self._L.mdx_surrogate_get(self._h, p, len(p)), "mdx_surrogate_get")
+        return p
+
+    def surrogate_record(self, reset=False, capacity=0):
+        self._chk(self._L.mdx_surrogate_record(self._h, int(bool(reset)), int(capacity)),
+                  "mdx_surrogate_record")
+
+    def surrogate_fit(self, epochs, batch=4096, lr=1e-3, seed=0, normalize=False):
+        loss = C.c_double(0.0)
+        self._chk(self._L.mdx_surrogate_fit(self._h, int(epochs), int(batch), float(lr), int(seed),
+                                            int(bool(normalize)), C.byref(loss)), "mdx_surrogate_fit")
+        return loss.value
+
+    def surrogate_eval(self, nmol):
+        e = np.zeros(nmol, np.float32)
+        f = np.zeros((nmol, 48), np.float32)
+        self._chk(self._L.mdx_surrogate_eval(self._h, e.ctypes.data, f.ctypes.data), "mdx_surrogate_eval")
+        return e, f
+
+    def e3b_molecule_energies(self, nmol):
+        e = np.zeros(nmol, np.float32)
+        self._chk(self._L.mdx_e3b_molecule_energies(self._h, e), "mdx_e3b_molecule_energies")
+        return e
 
     def tune(self, flags, evals=10):
         """Time every launch variant of the list build, pair and E3B kernels and

@@ -19,7 +19,8 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmdx.so")
 
-SOURCES = ["api.hip", "nlist.hip", "pair.hip", "pppm.hip", "e3b.hip", "integrate.hip", "rigid.hip"]
+SOURCES = ["api.hip", "nlist.hip", "pair.hip", "pppm.hip", "e3b.hip", "integrate.hip", "rigid.hip",
+           "surrogate.hip"]
 NO_CONTRACT = {"nlist.hip"}
 
 ARCH = os.environ.get("MDX_OFFLOAD_ARCH", "gfx950")

@@ -95,13 +95,21 @@ void ensure_list(Ctx &c) {
     }
 }
 
+constexpr int kForceTerms = MDX_PAIR | MDX_PPPM | MDX_E3B | MDX_SURROGATE;
+
 void compute(Ctx &c, int flags) {
     const bool energy = flags & MDX_ENERGY, virial = flags & MDX_VIRIAL;
     refresh_xq(c);
     zero_forces(c, true);
     if ((flags & MDX_PAIR)) pair_forces(c, energy, virial);
     if ((flags & MDX_PPPM) && c.pppm.on) pppm_forces(c, energy, virial);
-    if ((flags & MDX_E3B) && c.e3b.on) e3b_forces(c, energy, virial);
+    if ((flags & MDX_SURROGATE) && c.e3b.on) {
+        MDX_REQUIRE(c.sur.on, MDX_ERR_STATE, "MDX_SURROGATE needs a model (mdx_surrogate_set)");
+        e3b_forces(c, energy, virial, false);  // two-body term stays analytic
+        surrogate_forces(c, energy);
+    } else if ((flags & MDX_E3B) && c.e3b.on) {
+        e3b_forces(c, energy, virial);
+    }
     c.forceFlags = flags;
 }
 
@@ -245,7 +253,7 @@ void stream_after(hipStream_t to, hipStream_t from) {
 }
 
 void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags) {
-    const int fflags = flags & (MDX_PAIR | MDX_PPPM | MDX_E3B);
+    const int fflags = flags & kForceTerms;
     for (Ctx *cp : cs) {
         Ctx &c = *cp;
         require_system(c);
@@ -894,7 +902,7 @@ int mdx_run(mdx_ctx *ctx, int nsteps, int flags) {
             run_captured({&c}, nsteps, flags);
             return;
         }
-        const int fflags = flags & (MDX_PAIR | MDX_PPPM | MDX_E3B);
+        const int fflags = flags & kForceTerms;
         prepare_terms(c);
         if (c.rigid.on && c.rigid.fresh) {
             // constrain the starting velocities and place the M sites
@@ -966,7 +974,7 @@ int mdx_force_steps(mdx_ctx *ctx, int nsteps, int flags, int rebuild_every) {
                 } else if (rebuild_every > 0 && s % rebuild_every == 0) {
                     nlist_rebuild_async(c);
                 }
-                compute(c, flags & (MDX_PAIR | MDX_PPPM | MDX_E3B));
+                compute(c, flags & kForceTerms);
             }
             MDX_HIP(hipStreamSynchronize(c.stream));
             if (capacity_ok(c)) break;
@@ -1071,7 +1079,7 @@ int mdx_tune(mdx_ctx *ctx, int flags, int evals, int32_t *chosen, double *avg_ms
         MDX_HIP(hipEventDestroy(e0));
         MDX_HIP(hipEventDestroy(e1));
         // leave forces consistent with the current positions and flags
-        compute(c, c.forceFlags ? c.forceFlags : (flags & (MDX_PAIR | MDX_PPPM | MDX_E3B)));
+        compute(c, c.forceFlags ? c.forceFlags : (flags & kForceTerms));
         MDX_HIP(hipStreamSynchronize(c.stream));
         if (chosen) {
             chosen[0] = bestMode;
@@ -1084,6 +1092,123 @@ int mdx_tune(mdx_ctx *ctx, int flags, int evals, int32_t *chosen, double *avg_ms
             avg_ms[1] = bestPairT < 1e30 ? bestPairT : 0.0;
             avg_ms[2] = bestE3bT < 1e30 ? bestE3bT : 0.0;
         }
+    });
+}
+
+static void surrogate_alloc(Ctx &c) {
+    SurrogateState &S = c.sur;
+    if (!S.params) S.params = dalloc<float>(c, kSurNParam);
+    if (S.nmolAlloc != c.e3b.nmol) {
+        if (S.feat) {
+            dfree(c, S.feat);
+            dfree(c, S.emol);
+            dfree(c, S.wgt);
+        }
+        S.nmolAlloc = c.e3b.nmol;
+        S.feat = dalloc<float>(c, (size_t)S.nmolAlloc * 3 * kSurNB);
+        S.emol = dalloc<float>(c, S.nmolAlloc);
+        S.wgt = dalloc<float>(c, (size_t)S.nmolAlloc * 3 * kSurNB);
+    }
+}
+
+int mdx_surrogate_set(mdx_ctx *ctx, const float *params, int nparams) {
+    return guarded(ctx, [&](Ctx &c) {
+        MDX_REQUIRE(c.e3b.on, MDX_ERR_STATE, "the surrogate replaces E3B: call mdx_e3b_set first");
+        MDX_REQUIRE(params && nparams == kSurNParam, MDX_ERR_ARG, "surrogate parameter count mismatch");
+        surrogate_alloc(c);
+        upload(c, c.sur.params, params, kSurNParam);
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        c.sur.on = true;
+        c.forceFlags = -1;
+    });
+}
+
+int mdx_surrogate_get(mdx_ctx *ctx, float *params, int nparams) {
+    return guarded(ctx, [&](Ctx &c) {
+        MDX_REQUIRE(c.sur.params, MDX_ERR_STATE, "no surrogate model");
+        MDX_REQUIRE(params && nparams == kSurNParam, MDX_ERR_ARG, "surrogate parameter count mismatch");
+        MDX_HIP(hipMemcpyAsync(params, c.sur.params, sizeof(float) * kSurNParam, hipMemcpyDeviceToHost, c.stream));
+        MDX_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int mdx_surrogate_record(mdx_ctx *ctx, int reset, int capacity) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        MDX_REQUIRE(c.e3b.on, MDX_ERR_STATE, "training targets come from E3B: call mdx_e3b_set first");
+        SurrogateState &S = c.sur;
+        surrogate_alloc(c);
+        if (reset) S.nsamp = 0;
+        if (capacity > S.capSamp) {
+            MDX_REQUIRE(S.nsamp == 0, MDX_ERR_STATE, "set the sample capacity on the first (reset) record");
+            if (S.trainX) {
+                dfree(c, S.trainX);
+                dfree(c, S.trainT);
+            }
+            S.capSamp = capacity;
+            S.trainX = dalloc<float>(c, (size_t)capacity * 3 * kSurNB);
+            S.trainT = dalloc<float>(c, capacity);
+        }
+        MDX_REQUIRE(S.nsamp + c.e3b.nmol <= S.capSamp, MDX_ERR_ARG, "sample capacity exhausted");
+        prepare_terms(c);
+        ensure_list(c);
+        refresh_xq(c);
+        surrogate_features(c, S.trainX + (size_t)S.nsamp * 3 * kSurNB);
+        zero_forces(c, true);
+        e3b_forces(c, false, false, true, S.trainT + S.nsamp);
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        MDX_REQUIRE(capacity_ok(c), MDX_ERR_STATE, "list rows grew while recording: record again");
+        S.nsamp += c.e3b.nmol;
+        c.forceFlags = -1;  // the force buffer now holds E3B only
+    });
+}
+
+int mdx_surrogate_fit(mdx_ctx *ctx, int epochs, int batch, double lr, int seed, int normalize, double *loss) {
+    return guarded(ctx, [&](Ctx &c) {
+        SurrogateState &S = c.sur;
+        MDX_REQUIRE(S.params && S.nsamp > 0, MDX_ERR_STATE, "no model or no recorded samples");
+        MDX_REQUIRE(epochs >= 0 && batch >= 16 && lr > 0, MDX_ERR_ARG, "epochs >= 0, batch >= 16, lr > 0");
+        if (normalize) surrogate_normalize(c);
+        const double l = surrogate_fit(c, epochs, batch, lr, (unsigned)seed);
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        if (loss) *loss = l;
+        c.forceFlags = -1;
+    });
+}
+
+int mdx_surrogate_eval(mdx_ctx *ctx, float *emol, float *feat) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        MDX_REQUIRE(c.sur.on, MDX_ERR_STATE, "no surrogate model");
+        prepare_terms(c);
+        ensure_list(c);
+        refresh_xq(c);
+        zero_forces(c, true);
+        surrogate_forces(c, false);
+        const size_t m = c.e3b.nmol;
+        if (emol) MDX_HIP(hipMemcpyAsync(emol, c.sur.emol, sizeof(float) * m, hipMemcpyDeviceToHost, c.stream));
+        if (feat)
+            MDX_HIP(hipMemcpyAsync(feat, c.sur.feat, sizeof(float) * m * 3 * kSurNB, hipMemcpyDeviceToHost,
+                                   c.stream));
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        c.forceFlags = -1;
+    });
+}
+
+int mdx_e3b_molecule_energies(mdx_ctx *ctx, float *emol) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        MDX_REQUIRE(c.e3b.on && emol, MDX_ERR_ARG, "E3B not set or no output");
+        prepare_terms(c);
+        ensure_list(c);
+        refresh_xq(c);
+        zero_forces(c, true);
+        float *d = dalloc<float>(c, c.e3b.nmol);
+        e3b_forces(c, false, false, true, d);
+        MDX_HIP(hipMemcpyAsync(emol, d, sizeof(float) * c.e3b.nmol, hipMemcpyDeviceToHost, c.stream));
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        dfree(c, d);
+        c.forceFlags = -1;
     });
 }
 

@@ -34,6 +34,8 @@ struct E3BArgs {
     float4 *tot;
     unsigned long long *force;
     unsigned long long *acc;
+    int three;    // 0: two-body term only
+    float *emol;  // optional: three-body energy owned by each molecule
 };
 
 __device__ __forceinline__ float3 mi3(float4 a, float4 b, const E3BArgs &p) {
@@ -117,7 +119,7 @@ __global__ void __launch_bounds__(256) k_e3b_pass1(E3BArgs p) {
                     w5 += d.y * s * d.z;
                 }
             }
-            if (r <= p.rmax) {
+            if (p.three && r <= p.rmax) {
                 const float4 H1j = p.xq[aj.y], H2j = p.xq[aj.z];
                 tx += hb_only(mi3(H1j, Oi, p), p) + hb_only(mi3(H2j, Oi, p), p);
                 ty += hb_only(mi3(H1i, Oj, p), p);
@@ -241,6 +243,7 @@ __global__ void __launch_bounds__(256) k_e3b_pass2(E3BArgs p) {
             e3 = (double)p.Ea * ((double)ti.y * ti.z - ea) +
                  (double)p.Eb * ((double)ti.x * (ti.y + ti.z) - eb) +
                  0.5 * (double)p.Ec * ((double)ti.x * ti.x - ec);
+            if (p.emol) p.emol[i] = (float)e3;
         }
         block_acc<P>(e3, ACC_E3B3, p.acc);
         block_acc<P>(w0, ACC_VIR + 0, p.acc);
@@ -264,6 +267,7 @@ void launch(Ctx &c, const E3BArgs &a, bool ev) {
         else
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass1<P, false>), dim3(blocks), dim3(threads), 0, s, a);
     }
+    if (!a.three) return;
     ProfScope ps(c, PK_E3B2);
     if (ev)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass2<P, true>), dim3(blocks), dim3(threads), 0, s, a);
@@ -273,7 +277,7 @@ void launch(Ctx &c, const E3BArgs &a, bool ev) {
 
 }  // namespace
 
-void e3b_forces(Ctx &c, bool energy, bool virial) {
+void e3b_forces(Ctx &c, bool energy, bool virial, bool three_body, float *emol) {
     E3BState &E = c.e3b;
     if (!E.on || E.nmol == 0) return;
     E3BArgs a;
@@ -300,7 +304,9 @@ void e3b_forces(Ctx &c, bool energy, bool virial) {
     a.tot = E.tot;
     a.force = c.force;
     a.acc = c.acc;
-    const bool ev = energy || virial;
+    a.three = three_body ? 1 : 0;
+    a.emol = emol;
+    const bool ev = energy || virial || emol;
     switch (E.team) {
         case 4: launch<4>(c, a, ev); break;
         case 8: launch<8>(c, a, ev); break;

@@ -94,7 +94,16 @@ __global__ void k_vscale(const int *__restrict__ gate, int n, double4 *__restric
     vel[i] = v;
 }
 
+__global__ void k_fill_float(int n, float *__restrict__ p, float v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 }  // namespace
+
+void fill_float(Ctx &c, float *p, int count, float v) {
+    hipLaunchKernelGGL(k_fill_float, dim3(cdiv(count, 256)), dim3(256), 0, c.stream, count, p, v);
+}
 
 void thermo_measure(Ctx &c) {
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, thermo_gate(c), 2, c.thermo);

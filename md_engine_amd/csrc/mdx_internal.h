@@ -17,6 +17,8 @@
 
 #include <cstdint>
 #include <string>
+#include <cmath>
+#include <numeric>
 #include <vector>
 
 namespace mdx {
@@ -81,6 +83,24 @@ struct E3BState {
     int *nlCount = nullptr;
     float4 *tot = nullptr;    // per molecule totals (x accepted, y H1 donated, z H2 donated)
     int team = 8;
+};
+
+// learned E3B three-body surrogate (surrogate.hip)
+constexpr int kSurNB = 16, kSurNH = 64;
+constexpr int kSurNParam = 3 * kSurNB * kSurNH + kSurNH + kSurNH * kSurNH + kSurNH + kSurNH + 1 + 6 * kSurNB + 2;
+
+struct SurrogateState {
+    bool on = false;
+    float *params = nullptr;  // packed model (kSurNParam)
+    float *feat = nullptr;    // per molecule raw channels acc, d1, d2 (48)
+    float *emol = nullptr;    // per molecule energy
+    float *wgt = nullptr;     // per molecule dE / d(raw channel) (48)
+    int nmolAlloc = 0;
+    float *trainX = nullptr;  // recorded samples: raw channels
+    float *trainT = nullptr;  // recorded samples: E3B three-body energy of the molecule
+    int nsamp = 0, capSamp = 0;
+    float *part = nullptr, *m1 = nullptr, *m2 = nullptr, *loss = nullptr;
+    long step = 0;
 };
 
 // kernels timed with HIP events when profiling is on (mdx_profile_enable)
@@ -178,6 +198,7 @@ struct Ctx {
 
     PppmState pppm;
     E3BState e3b;
+    SurrogateState sur;
     RigidState rigid;
     bool mdxDisableTiledSpread = false;   // test knob: force the plain global-atomic spread
 
@@ -308,7 +329,14 @@ double pppm_solve_alpha(int natoms, double totalQ2, const double *L, const int *
                         double rcut, double *err);
 
 // e3b.hip
-void e3b_forces(Ctx &c, bool energy, bool virial);
+void e3b_forces(Ctx &c, bool energy, bool virial, bool three_body = true, float *emol = nullptr);
+
+// surrogate.hip
+void surrogate_features(Ctx &c, float *raw);
+void surrogate_forces(Ctx &c, bool energy);
+void surrogate_normalize(Ctx &c);
+double surrogate_fit(Ctx &c, int epochs, int batch, double lr, unsigned seed);
+void fill_float(Ctx &c, float *p, int count, float v);
 
 // rigid.hip
 void rigid_map(Ctx &c);

@@ -1,0 +1,640 @@
+// Learned E3B three-body surrogate (no counterpart in the reference, which
+// only has the analytic E3B of src/Evaluators/ThreeBodyE3B.h): per-molecule
+// descriptors from the E3B molecule list, a 48 -> 64 -> 64 -> 1 SiLU MLP on
+// BF16 MFMA tiles (FP32 accumulation) with its hand-written backward pass for
+// forces, and on-device training (per-molecule energy regression, Adam).
+//
+// Descriptors of molecule i for 16 radial Gaussians
+// g_k(r) = exp(-ETA (r - MU_k)^2) fc(r)  (fc: switching polynomial RS..RF):
+//   acc_k = sum_{j != i, H of j} g_k(|H - O_i|),
+//   d1_k  = sum_{j != i} g_k(|H1_i - O_j|), d2_k likewise for H2_i;
+// network input F = [acc, d1 + d2, d1 * d2], standardised by (mu, sd).
+// Forces: dE/dF per molecule (MLP backward) -> per-channel pair weights ->
+// every owner recomputes its own pairs (deterministic, no cross-molecule
+// atomics beyond the fixed-point force buffer).
+#include "mdx_internal.h"
+
+namespace mdx {
+
+namespace {
+
+constexpr int NB = kSurNB, NF = 3 * kSurNB, NH = kSurNH;
+constexpr float RS_C = 4.6f, RF_C = 5.2f, ETA = 1.0f / (2.0f * 0.28f * 0.28f);
+
+// parameter offsets in the packed model (oracle/surrogate.py layout)
+constexpr int OW1 = 0, OB1 = OW1 + NF * NH, OW2 = OB1 + NH, OB2 = OW2 + NH * NH, OW3 = OB2 + NH,
+              OB3 = OW3 + NH, OMU = OB3 + 1, OSD = OMU + NF, OES = OSD + NF, OEH = OES + 1;
+constexpr int NTRAIN = OMU;  // W1 b1 W2 b2 w3 b3 are trained; mu sd escale eshift are fixed
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float mu_k(int k) { return 0.9f + 0.28f * (float)k; }
+
+// orders LDS traffic between the lanes of one wave (no cross-wave barrier)
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ short to_bf16(float f) {
+    unsigned u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    return (short)(u >> 16);
+}
+
+__device__ __forceinline__ void fcut(float r, float &s, float &ds) {
+    const float inv = 1.0f / ((RF_C - RS_C) * (RF_C - RS_C) * (RF_C - RS_C));
+    if (r <= RS_C) {
+        s = 1.0f;
+        ds = 0.0f;
+    } else if (r >= RF_C) {
+        s = 0.0f;
+        ds = 0.0f;
+    } else {
+        s = (RF_C - r) * (RF_C - r) * (RF_C + 2.0f * r - 3.0f * RS_C) * inv;
+        ds = 6.0f * (RF_C - r) * (RS_C - r) * inv;
+    }
+}
+
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float dsilu(float z) {
+    const float s = 1.0f / (1.0f + __expf(-z));
+    return s * (1.0f + z * (1.0f - s));
+}
+
+struct Geo3 {
+    float L[3], invL[3];
+};
+
+__device__ __forceinline__ float3 mi(float4 a, float4 b, const Geo3 &g) {
+    float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    dx -= g.L[0] * rintf(dx * g.invL[0]);
+    dy -= g.L[1] * rintf(dy * g.invL[1]);
+    dz -= g.L[2] * rintf(dz * g.invL[2]);
+    return make_float3(dx, dy, dz);
+}
+
+// ---------------------------------------------------------------------------
+// descriptors: 16 lanes per molecule, lane k owns basis k (no reductions)
+__global__ void __launch_bounds__(256) k_sur_feat(int nmol, const int4 *__restrict__ molAtoms,
+                                                  const float4 *__restrict__ xq, Geo3 g, int cap,
+                                                  const int *__restrict__ nl, const int *__restrict__ nlCount,
+                                                  float *__restrict__ feat) {
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = gt >> 4, k = gt & 15;
+    if (i >= nmol) return;
+    const int4 ai = molAtoms[i];
+    const float4 Oi = xq[ai.x], H1i = xq[ai.y], H2i = xq[ai.z];
+    const float mu = mu_k(k);
+    float acc = 0.f, d1 = 0.f, d2 = 0.f;
+    const int cnt = min(nlCount[i], cap);
+    const int *row = nl + (size_t)i * cap;
+    for (int t = 0; t < cnt; ++t) {
+        const int4 aj = molAtoms[row[t]];
+        const float4 Oj = xq[aj.x], H1j = xq[aj.y], H2j = xq[aj.z];
+        const float3 r4[4] = {mi(H1j, Oi, g), mi(H2j, Oi, g), mi(H1i, Oj, g), mi(H2i, Oj, g)};
+        float v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float r = sqrtf(r4[q].x * r4[q].x + r4[q].y * r4[q].y + r4[q].z * r4[q].z);
+            float s, ds;
+            fcut(r, s, ds);
+            v[q] = s > 0.f ? __expf(-ETA * (r - mu) * (r - mu)) * s : 0.f;
+        }
+        acc += v[0] + v[1];
+        d1 += v[2];
+        d2 += v[3];
+    }
+    float *f = feat + (size_t)i * NF;
+    f[k] = acc;
+    f[NB + k] = d1;
+    f[2 * NB + k] = d2;
+}
+
+// ---------------------------------------------------------------------------
+// wave-level 16x16 tiles: C += A (16 x K) B (K x 16), operands FP32 in LDS
+// converted to BF16 on load; A(m,k) = A[m*am + k*ak], B(k,n) = B[k*bk + n*bn].
+// Lane l: A row l%16, k 4*(l/16)+j; B k 4*(l/16)+j, column l%16; C rows
+// 4*(l/16)+j, column l%16 (v_mfma_f32_16x16x16bf16_1k layout).
+__device__ __forceinline__ f32x4 mm16(const float *A, int am, int ak, const float *B, int bk, int bn, int K,
+                                      f32x4 c) {
+    const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        bf16x4 a, b;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int kk = k0 + 4 * q + j;
+            a[j] = to_bf16(A[r * am + kk * ak]);
+            b[j] = to_bf16(B[kk * bk + r * bn]);
+        }
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+    }
+    return c;
+}
+
+__device__ __forceinline__ void st16(float *D, int ld, int n0, f32x4 c) {
+    const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) D[(4 * q + j) * ld + n0 + r] = c[j];
+}
+
+__device__ __forceinline__ f32x4 zero4() {
+    f32x4 z = {0.f, 0.f, 0.f, 0.f};
+    return z;
+}
+
+// per-wave LDS scratch of the fused MLP (16 molecules)
+struct WaveBuf {
+    float X[16 * NF];
+    float Z1[16 * NH];
+    float H1[16 * NH];
+    float Z2[16 * NH];
+    float T[16 * NH];  // dZ2, then dH1 / dZ1
+};
+
+// Standardised input rows of 16 samples from raw channels (acc, d1, d2).
+__device__ __forceinline__ void load_x(WaveBuf &w, const float *__restrict__ raw, const int *rows, int nrow,
+                                       const float *mu, const float *sd) {
+    const int l = threadIdx.x & 63;
+    for (int e = l; e < 16 * NB; e += 64) {
+        const int s = e / NB, k = e % NB;
+        float a = 0.f, d1 = 0.f, d2 = 0.f;
+        if (s < nrow) {
+            const float *f = raw + (size_t)rows[s] * NF;
+            a = f[k];
+            d1 = f[NB + k];
+            d2 = f[2 * NB + k];
+        }
+        const float F[3] = {a, d1 + d2, d1 * d2};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int col = c * NB + k;
+            w.X[s * NF + col] = (F[c] - mu[col]) / sd[col];
+        }
+    }
+}
+
+// forward through both hidden layers: Z1, H1, Z2 (and H2 into T)
+__device__ __forceinline__ void forward(WaveBuf &w, const float *W1, const float *b1, const float *W2,
+                                        const float *b2) {
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int n0 = 0; n0 < NH; n0 += 16) st16(w.Z1, NH, n0, mm16(w.X, NF, 1, W1 + n0, NH, 1, NF, zero4()));
+    wave_sync();
+    for (int e = l; e < 16 * NH; e += 64) {
+        const float z = w.Z1[e] + b1[e % NH];
+        w.Z1[e] = z;
+        w.H1[e] = silu(z);
+    }
+    wave_sync();
+#pragma unroll
+    for (int n0 = 0; n0 < NH; n0 += 16) st16(w.Z2, NH, n0, mm16(w.H1, NH, 1, W2 + n0, NH, 1, NH, zero4()));
+    wave_sync();
+    for (int e = l; e < 16 * NH; e += 64) {
+        const float z = w.Z2[e] + b2[e % NH];
+        w.Z2[e] = z;
+        w.T[e] = silu(z);
+    }
+    wave_sync();
+}
+
+// network output of row s (lane-parallel dot over NH, result in every lane)
+__device__ __forceinline__ float out_row(const WaveBuf &w, int s, const float *w3, float b3) {
+    const int l = threadIdx.x & 63;
+    float v = w.T[s * NH + l] * w3[l];
+    v = team_sum<64>(v);
+    return v + b3;
+}
+
+// given dE/de per row in g[16] (LDS), back to dZ2 (T), dZ1 (T after) ; X grads optional
+__device__ __forceinline__ void backward_hidden(WaveBuf &w, const float *gscale, const float *w3, const float *W2) {
+    const int l = threadIdx.x & 63;
+    for (int e = l; e < 16 * NH; e += 64) w.T[e] = gscale[e / NH] * w3[e % NH] * dsilu(w.Z2[e]);  // dZ2
+    wave_sync();
+    // dH1 = dZ2 W2^T  -> into H1's place is needed later for dW2; use Z1 buffer after dsilu
+    f32x4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = mm16(w.T, NH, 1, W2 + t * 16 * NH, 1, NH, NH, zero4());
+    wave_sync();
+    // dZ1 = dH1 * silu'(Z1), kept in Z1
+    {
+        const int r = l & 15, q = l >> 4;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int e = (4 * q + j) * NH + t * 16 + r;
+                w.Z1[e] = acc[t][j] * dsilu(w.Z1[e]);
+            }
+    }
+    wave_sync();
+}
+
+struct SurModel {
+    const float *p;  // packed parameters (device)
+};
+
+// ---------------------------------------------------------------------------
+// inference: energies and dE/d(raw channel) per molecule, 4 waves x 16 molecules
+__global__ void __launch_bounds__(256) k_sur_mlp(int nmol, const float *__restrict__ P,
+                                                 const float *__restrict__ raw, float *__restrict__ emol,
+                                                 float *__restrict__ wgt, unsigned long long *__restrict__ acc,
+                                                 int energy) {
+    __shared__ float sW1[NF * NH], sW2[NH * NH], sb1[NH], sb2[NH], sw3[NH], smu[NF], ssd[NF];
+    __shared__ WaveBuf wb[4];
+    __shared__ float sg[4][16];
+    __shared__ int srow[4][16];
+    for (int e = threadIdx.x; e < NF * NH; e += blockDim.x) sW1[e] = P[OW1 + e];
+    for (int e = threadIdx.x; e < NH * NH; e += blockDim.x) sW2[e] = P[OW2 + e];
+    for (int e = threadIdx.x; e < NH; e += blockDim.x) {
+        sb1[e] = P[OB1 + e];
+        sb2[e] = P[OB2 + e];
+        sw3[e] = P[OW3 + e];
+    }
+    for (int e = threadIdx.x; e < NF; e += blockDim.x) {
+        smu[e] = P[OMU + e];
+        ssd[e] = P[OSD + e];
+    }
+    __syncthreads();
+    const float b3 = P[OB3], es = P[OES], eh = P[OEH];
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+    WaveBuf &w = wb[wv];
+    const int m0 = (blockIdx.x * 4 + wv) * 16;
+    if (m0 >= nmol) return;  // whole wave; no block barrier follows
+    const int nrow = min(16, nmol - m0);
+    if (l < 16) srow[wv][l] = m0 + min(l, nrow - 1);
+    wave_sync();
+    load_x(w, raw, srow[wv], nrow, smu, ssd);
+    wave_sync();
+    forward(w, sW1, sb1, sW2, sb2);
+    double esum = 0.0;
+    for (int s = 0; s < 16; ++s) {
+        const float e = out_row(w, s, sw3, b3);
+        if (l == 0) sg[wv][s] = s < nrow ? es : 0.f;  // dE_i/de = escale
+        if (l == 0 && s < nrow) {
+            emol[m0 + s] = es * e + eh;
+            esum += (double)(es * e + eh);
+        }
+    }
+    wave_sync();
+    backward_hidden(w, sg[wv], sw3, sW2);
+    // dX = dZ1 W1^T (16 x 48), then to raw channels
+    f32x4 dx[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) dx[t] = mm16(w.Z1, NH, 1, sW1 + t * 16 * NH, 1, NH, NH, zero4());
+    wave_sync();
+#pragma unroll
+    for (int t = 0; t < 3; ++t) st16(w.X, NF, t * 16, dx[t]);  // X now holds dE/dx_hat
+    wave_sync();
+    for (int e = l; e < 16 * NB; e += 64) {
+        const int s = e / NB, k = e % NB;
+        if (s >= nrow) continue;
+        const float *f = raw + (size_t)(m0 + s) * NF;
+        const float d1 = f[NB + k], d2 = f[2 * NB + k];
+        const float ga = w.X[s * NF + k] / ssd[k];
+        const float gs = w.X[s * NF + NB + k] / ssd[NB + k];
+        const float gp = w.X[s * NF + 2 * NB + k] / ssd[2 * NB + k];
+        float *o = wgt + (size_t)(m0 + s) * NF;
+        o[k] = ga;
+        o[NB + k] = gs + gp * d2;
+        o[2 * NB + k] = gs + gp * d1;
+    }
+    if (energy && l == 0 && esum != 0.0) fx_add(&acc[ACC_E3B3], esum);
+}
+
+// ---------------------------------------------------------------------------
+// forces: 16 lanes per molecule (lane k = basis k); owner-computes
+__global__ void __launch_bounds__(256) k_sur_force(int nmol, const int4 *__restrict__ molAtoms,
+                                                   const float4 *__restrict__ xq, Geo3 g, int cap,
+                                                   const int *__restrict__ nl, const int *__restrict__ nlCount,
+                                                   const float *__restrict__ wgt,
+                                                   unsigned long long *__restrict__ force) {
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = gt >> 4, k = gt & 15;
+    float3 FO = make_float3(0.f, 0.f, 0.f), F1 = FO, F2 = FO;
+    int4 ai = make_int4(0, 0, 0, 0);
+    if (i < nmol) {
+        ai = molAtoms[i];
+        const float4 Oi = xq[ai.x], H1i = xq[ai.y], H2i = xq[ai.z];
+        const float mu = mu_k(k);
+        const float wa_i = wgt[(size_t)i * NF + k], w1_i = wgt[(size_t)i * NF + NB + k],
+                    w2_i = wgt[(size_t)i * NF + 2 * NB + k];
+        const int cnt = min(nlCount[i], cap);
+        const int *row = nl + (size_t)i * cap;
+        for (int t = 0; t < cnt; ++t) {
+            const int mj = row[t];
+            const int4 aj = molAtoms[mj];
+            const float4 Oj = xq[aj.x], H1j = xq[aj.y], H2j = xq[aj.z];
+            const float wa_j = wgt[(size_t)mj * NF + k], w1_j = wgt[(size_t)mj * NF + NB + k],
+                        w2_j = wgt[(size_t)mj * NF + 2 * NB + k];
+            // pairs with an H of i: weight (donor channel of i + acceptor channel of j)
+            // pairs with an H of j: weight (donor channel of j + acceptor channel of i)
+            const float3 d[4] = {mi(H1i, Oj, g), mi(H2i, Oj, g), mi(H1j, Oi, g), mi(H2j, Oi, g)};
+            const float wq[4] = {w1_i + wa_j, w2_i + wa_j, w1_j + wa_i, w2_j + wa_i};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float r = sqrtf(d[q].x * d[q].x + d[q].y * d[q].y + d[q].z * d[q].z);
+                float s, ds;
+                fcut(r, s, ds);
+                if (s <= 0.f && ds == 0.f) continue;
+                const float ga = __expf(-ETA * (r - mu) * (r - mu));
+                const float dg = ga * (ds - 2.0f * ETA * (r - mu) * s);
+                const float c = wq[q] * dg / r;  // dE/dr / r
+                // F_H = -c d, F_O = +c d  (d = r_H - r_O)
+                if (q == 0) {
+                    F1.x -= c * d[q].x;
+                    F1.y -= c * d[q].y;
+                    F1.z -= c * d[q].z;
+                } else if (q == 1) {
+                    F2.x -= c * d[q].x;
+                    F2.y -= c * d[q].y;
+                    F2.z -= c * d[q].z;
+                } else {
+                    FO.x += c * d[q].x;
+                    FO.y += c * d[q].y;
+                    FO.z += c * d[q].z;
+                }
+            }
+        }
+    }
+    FO.x = team_sum<16>(FO.x);
+    FO.y = team_sum<16>(FO.y);
+    FO.z = team_sum<16>(FO.z);
+    F1.x = team_sum<16>(F1.x);
+    F1.y = team_sum<16>(F1.y);
+    F1.z = team_sum<16>(F1.z);
+    F2.x = team_sum<16>(F2.x);
+    F2.y = team_sum<16>(F2.y);
+    F2.z = team_sum<16>(F2.z);
+    if (k == 0 && i < nmol) {
+        fx_add(&force[3 * ai.x + 0], FO.x);
+        fx_add(&force[3 * ai.x + 1], FO.y);
+        fx_add(&force[3 * ai.x + 2], FO.z);
+        fx_add(&force[3 * ai.y + 0], F1.x);
+        fx_add(&force[3 * ai.y + 1], F1.y);
+        fx_add(&force[3 * ai.y + 2], F1.z);
+        fx_add(&force[3 * ai.z + 0], F2.x);
+        fx_add(&force[3 * ai.z + 1], F2.y);
+        fx_add(&force[3 * ai.z + 2], F2.z);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// training: per-wave partial gradients of the mean squared error of the
+// standardised per-molecule energy over one mini-batch
+constexpr int kTrainWaves = 256;  // 64 blocks of 4 waves
+constexpr int kGradStride = NTRAIN + 1;  // trained parameters + loss
+
+__global__ void __launch_bounds__(256) k_sur_grad(int nsamp, int bstart, int bsize, unsigned mulA, unsigned addB,
+                                                  const float *__restrict__ P, const float *__restrict__ raw,
+                                                  const float *__restrict__ target, float *__restrict__ part) {
+    __shared__ float sW1[NF * NH], sW2[NH * NH], sb1[NH], sb2[NH], sw3[NH], smu[NF], ssd[NF];
+    __shared__ WaveBuf wb[4];
+    __shared__ float sg[4][16];
+    __shared__ int srow[4][16];
+    for (int e = threadIdx.x; e < NF * NH; e += blockDim.x) sW1[e] = P[OW1 + e];
+    for (int e = threadIdx.x; e < NH * NH; e += blockDim.x) sW2[e] = P[OW2 + e];
+    for (int e = threadIdx.x; e < NH; e += blockDim.x) {
+        sb1[e] = P[OB1 + e];
+        sb2[e] = P[OB2 + e];
+        sw3[e] = P[OW3 + e];
+    }
+    for (int e = threadIdx.x; e < NF; e += blockDim.x) {
+        smu[e] = P[OMU + e];
+        ssd[e] = P[OSD + e];
+    }
+    __syncthreads();
+    const float b3 = P[OB3], es = P[OES], eh = P[OEH];
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+    const int wave = blockIdx.x * 4 + wv;
+    WaveBuf &w = wb[wv];
+    f32x4 gW1[3][4], gW2[4][4];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) gW1[a][b] = zero4();
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) gW2[a][b] = zero4();
+    float gb1 = 0.f, gb2 = 0.f, gw3 = 0.f, gb3 = 0.f, loss = 0.f;
+    const int ntile = (bsize + 15) / 16;
+    const float inv = 1.0f / (float)bsize;
+    for (int t = wave; t < ntile; t += gridDim.x * 4) {
+        const int s0 = t * 16;
+        const int nrow = min(16, bsize - s0);
+        if (l < 16) {
+            const int s = min(l, nrow - 1);
+            // pseudo-shuffled sample order: (bstart + s0 + s) * a + b mod nsamp
+            srow[wv][l] = (int)(((unsigned long long)(unsigned)(bstart + s0 + s) * mulA + addB) % (unsigned)nsamp);
+        }
+        wave_sync();
+        load_x(w, raw, srow[wv], nrow, smu, ssd);
+        wave_sync();
+        forward(w, sW1, sb1, sW2, sb2);
+        for (int s = 0; s < 16; ++s) {
+            const float e = out_row(w, s, sw3, b3);
+            if (l == 0) {
+                float gr = 0.f;
+                if (s < nrow) {
+                    const float tt = (target[srow[wv][s]] - eh) / es;
+                    const float diff = e - tt;
+                    loss += diff * diff * inv;
+                    gr = 2.0f * diff * inv;
+                }
+                sg[wv][s] = gr;
+            }
+        }
+        wave_sync();
+        // dw3 += H2^T g, db3 += sum g (lane l owns hidden unit l)
+        for (int s = 0; s < 16; ++s) gw3 += sg[wv][s] * w.T[s * NH + l];
+        if (l == 0)
+            for (int s = 0; s < 16; ++s) gb3 += sg[wv][s];
+        wave_sync();
+        backward_hidden(w, sg[wv], sw3, sW2);  // T = dZ2, Z1 = dZ1
+        for (int s = 0; s < 16; ++s) {
+            gb2 += w.T[s * NH + l];
+            gb1 += w.Z1[s * NH + l];
+        }
+        // dW2 += H1^T dZ2: A(m=a, k=s) = H1[s][a], B(k=s, n=b) = dZ2[s][b]
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) gW2[a][b] = mm16(w.H1 + a * 16, 1, NH, w.T + b * 16, NH, 1, 16, gW2[a][b]);
+        // dW1 += X^T dZ1
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) gW1[a][b] = mm16(w.X + a * 16, 1, NF, w.Z1 + b * 16, NH, 1, 16, gW1[a][b]);
+        wave_sync();
+    }
+    float *o = part + (size_t)wave * kGradStride;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) o[OW1 + (a * 16 + 4 * q + j) * NH + b * 16 + r] = gW1[a][b][j];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) o[OW2 + (a * 16 + 4 * q + j) * NH + b * 16 + r] = gW2[a][b][j];
+    o[OB1 + l] = gb1;
+    o[OB2 + l] = gb2;
+    o[OW3 + l] = gw3;
+    if (l == 0) {
+        o[OB3] = gb3;
+        o[NTRAIN] = loss;
+    }
+}
+
+// sum of the per-wave partials in wave order (deterministic), then Adam
+__global__ void k_sur_adam(const float *__restrict__ part, int nwaves, float *__restrict__ P,
+                           float *__restrict__ m1, float *__restrict__ m2, float lr, float c1, float c2,
+                           float *__restrict__ lossOut) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e > NTRAIN) return;
+    float gsum = 0.f;
+    for (int w = 0; w < nwaves; ++w) gsum += part[(size_t)w * kGradStride + e];
+    if (e == NTRAIN) {
+        *lossOut = gsum;
+        return;
+    }
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float a = b1 * m1[e] + (1.f - b1) * gsum;
+    const float v = b2 * m2[e] + (1.f - b2) * gsum * gsum;
+    m1[e] = a;
+    m2[e] = v;
+    P[e] -= lr * (a * c1) / (sqrtf(v * c2) + 1e-8f);
+}
+
+// per-feature mean and standard deviation of the network inputs over the
+// recorded samples (one block per input column, fixed reduction order)
+__global__ void __launch_bounds__(256) k_sur_norm(int nsamp, const float *__restrict__ raw,
+                                                  const float *__restrict__ target, float *__restrict__ P) {
+    __shared__ double s1[256], s2[256];
+    const int col = blockIdx.x;  // 0..NF-1 inputs, NF = target
+    double a = 0.0, b = 0.0;
+    for (int s = threadIdx.x; s < nsamp; s += blockDim.x) {
+        float v;
+        if (col == NF) {
+            v = target[s];
+        } else {
+            const float *f = raw + (size_t)s * NF;
+            const int c = col / NB, k = col % NB;
+            const float d1 = f[NB + k], d2 = f[2 * NB + k];
+            v = c == 0 ? f[k] : (c == 1 ? d1 + d2 : d1 * d2);
+        }
+        a += v;
+        b += (double)v * v;
+    }
+    s1[threadIdx.x] = a;
+    s2[threadIdx.x] = b;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            s1[threadIdx.x] += s1[threadIdx.x + o];
+            s2[threadIdx.x] += s2[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double m = s1[0] / nsamp;
+        const double var = fmax(s2[0] / nsamp - m * m, 0.0);
+        const float sd = (float)fmax(sqrt(var), 1e-6);
+        if (col == NF) {
+            P[OEH] = (float)m;
+            P[OES] = sd;
+        } else {
+            P[OMU + col] = (float)m;
+            P[OSD + col] = sd;
+        }
+    }
+}
+
+Geo3 geo3(const Ctx &c) {
+    Geo3 g;
+    for (int d = 0; d < 3; ++d) {
+        g.L[d] = (float)c.geo.L[d];
+        g.invL[d] = c.geo.periodic[d] ? (float)(1.0 / c.geo.L[d]) : 0.0f;
+    }
+    return g;
+}
+
+}  // namespace
+
+static_assert(OEH + 1 == kSurNParam, "packed surrogate layout");
+
+void surrogate_features(Ctx &c, float *raw) {
+    E3BState &E = c.e3b;
+    hipLaunchKernelGGL(k_sur_feat, dim3(cdiv((long long)E.nmol * 16, 256)), dim3(256), 0, c.stream, E.nmol,
+                       E.molAtoms, c.xq, geo3(c), E.cap, E.nl, E.nlCount, raw);
+}
+
+void surrogate_forces(Ctx &c, bool energy) {
+    SurrogateState &S = c.sur;
+    E3BState &E = c.e3b;
+    surrogate_features(c, S.feat);
+    hipLaunchKernelGGL(k_sur_mlp, dim3(cdiv(E.nmol, 64)), dim3(256), 0, c.stream, E.nmol, S.params, S.feat,
+                       S.emol, S.wgt, c.acc, energy ? 1 : 0);
+    hipLaunchKernelGGL(k_sur_force, dim3(cdiv((long long)E.nmol * 16, 256)), dim3(256), 0, c.stream, E.nmol,
+                       E.molAtoms, c.xq, geo3(c), E.cap, E.nl, E.nlCount, S.wgt, c.force);
+}
+
+void surrogate_normalize(Ctx &c) {
+    SurrogateState &S = c.sur;
+    hipLaunchKernelGGL(k_sur_norm, dim3(NF + 1), dim3(256), 0, c.stream, S.nsamp, S.trainX, S.trainT, S.params);
+}
+
+double surrogate_fit(Ctx &c, int epochs, int batch, double lr, unsigned seed) {
+    SurrogateState &S = c.sur;
+    if (!S.part) {
+        S.part = dalloc<float>(c, (size_t)kTrainWaves * kGradStride);
+        S.m1 = dalloc<float>(c, NTRAIN);
+        S.m2 = dalloc<float>(c, NTRAIN);
+        S.loss = dalloc<float>(c, 1);
+        fill_float(c, S.m1, NTRAIN, 0.f);
+        fill_float(c, S.m2, NTRAIN, 0.f);
+    }
+    const int n = S.nsamp;
+    batch = std::min(batch, n);
+    double last = 0.0;
+    float hl = 0.f;
+    unsigned state = seed * 2654435761u + 12345u;
+    for (int ep = 0; ep < epochs; ++ep) {
+        // affine pseudo-permutation of the samples for this epoch (a odd and
+        // coprime with n is enough for a bijection when n is a power of two;
+        // otherwise some samples repeat within an epoch, which is harmless)
+        state = state * 1664525u + 1013904223u;
+        unsigned a = (state | 1u) % (unsigned)n;
+        if (a == 0) a = 1;
+        while (std::gcd(a, (unsigned)n) != 1) ++a;
+        state = state * 1664525u + 1013904223u;
+        const unsigned b = state % (unsigned)n;
+        double esum = 0.0;
+        int nb = 0;
+        for (int b0 = 0; b0 + batch <= n; b0 += batch) {
+            ++S.step;
+            const float c1 = 1.0f / (1.0f - std::pow(0.9f, (float)S.step));
+            const float c2 = 1.0f / (1.0f - std::pow(0.999f, (float)S.step));
+            hipLaunchKernelGGL(k_sur_grad, dim3(kTrainWaves / 4), dim3(256), 0, c.stream, n, b0, batch, a, b,
+                               S.params, S.trainX, S.trainT, S.part);
+            hipLaunchKernelGGL(k_sur_adam, dim3(cdiv(NTRAIN + 1, 256)), dim3(256), 0, c.stream, S.part,
+                               kTrainWaves, S.params, S.m1, S.m2, (float)lr, c1, c2, S.loss);
+            if (ep == epochs - 1) {
+                MDX_HIP(hipMemcpyAsync(&hl, S.loss, sizeof(float), hipMemcpyDeviceToHost, c.stream));
+                MDX_HIP(hipStreamSynchronize(c.stream));
+                esum += hl;
+                ++nb;
+            }
+        }
+        if (nb) last = esum / nb;
+    }
+    return last;
+}
+
+}  // namespace mdx

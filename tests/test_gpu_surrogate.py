@@ -1,0 +1,123 @@
+"""Learned E3B three-body surrogate on the GPU against the FP64 restatement
+(oracle/surrogate.py): descriptors, per-molecule energies and forces of the
+BF16-MFMA network, the per-molecule E3B targets, and on-device training
+(loss falls; the trained surrogate's three-body forces beat dropping the
+three-body term)."""
+import sys
+
+import numpy as np
+import pytest
+
+from md_engine_amd import systems
+from oracle import surrogate as S
+
+from test_gpu_water_md import start_velocities, water_md_engine
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel_rmse(f, ref):
+    return float(np.sqrt(((f - ref) ** 2).sum() / (ref ** 2).sum()))
+
+
+def e2_forces(box, params):
+    """two-body E3B forces (EvaluatorE3B.h:67-87) in FP64 numpy"""
+    from scipy.spatial import cKDTree
+
+    mol, x, L = box["mol"], box["x"], box["L"]
+    O = x[mol[:, 0]]
+    pairs = cKDTree(O % L, boxsize=L).query_pairs(params.r2b, output_type="ndarray")
+    f = np.zeros_like(x)
+    d = O[pairs[:, 0]] - O[pairs[:, 1]]
+    d -= L * np.rint(d / L)
+    r = np.linalg.norm(d, axis=1)
+    s = params.k2 * params.E2 * np.exp(-params.k2 * r) / r
+    np.add.at(f, mol[pairs[:, 0], 0], d * s[:, None])
+    np.add.at(f, mol[pairs[:, 1], 0], -d * s[:, None])
+    return f
+
+
+def random_model(seed, feats):
+    rng = np.random.default_rng(seed)
+    m = S.unpack(S.init_params(seed))
+    m["mu"] = feats.mean(0)
+    m["sd"] = np.maximum(feats.std(0), 1e-3)
+    m["escale"] = 0.05
+    m["eshift"] = -0.01
+    m["b1"] = rng.normal(0, 0.1, S.NH)
+    m["b2"] = rng.normal(0, 0.1, S.NH)
+    return S.pack(m).astype(np.float32)
+
+
+def test_e3b_molecule_energies(orc, gpu_engine_factory):
+    say("per-molecule E3B three-body energies")
+    box = systems.tip4p2005_box(216, seed=8, min_sep=1.8)
+    e, _ = water_md_engine(gpu_engine_factory, box, np.zeros_like(box["x"]), rc=7.0, skin=1.0,
+                           sz=(32, 32, 32), order=5, e3b=True, dt=1.0)
+    g = e.e3b_molecule_energies(len(box["mol"]))
+    o = S.e3b_molecule_energies(box["mol"], box["x"], box["L"])
+    assert np.abs(o).max() > 1e-3
+    assert np.allclose(g, o, rtol=1e-4, atol=1e-5 * np.abs(o).max())
+
+
+def test_surrogate_forward_backward_vs_oracle(orc, gpu_engine_factory):
+    say("surrogate descriptors, energies, forces vs FP64")
+    from md_engine_amd import ENERGY, SURROGATE
+
+    box = systems.tip4p2005_box(216, seed=9, min_sep=1.8)
+    acc, d1, d2 = S.descriptors(box["mol"], box["x"], box["L"])
+    p = random_model(1, S.features(acc, d1, d2))
+    e, _ = water_md_engine(gpu_engine_factory, box, np.zeros_like(box["x"]), rc=7.0, skin=1.0,
+                           sz=(32, 32, 32), order=5, e3b=True, dt=1.0)
+    e.surrogate_set(p)
+    eg, fg = e.surrogate_eval(len(box["mol"]))
+    raw = np.concatenate([acc, d1, d2], axis=1)
+    assert np.allclose(fg, raw, rtol=1e-4, atol=1e-5 * np.abs(raw).max())
+    fo, eo = S.forces(p.astype(np.float64), box["mol"], box["x"], box["L"])
+    # BF16 operands, FP32 accumulation: a few 1e-3 of the energy spread
+    assert np.abs(eg - eo).max() < 2e-2 * (np.abs(eo - eo.mean()).max() + 1e-3)
+    e.compute(SURROGATE | ENERGY)
+    f = e.forces() - e2_forces(box, orc.e3b_params())
+    assert rel_rmse(f, fo) < 3e-2
+    en, _ = e.energies()
+    assert en[3] == pytest.approx(eo.sum(), rel=2e-2)
+
+
+def test_surrogate_training_gain(orc, gpu_engine_factory):
+    """Record MD frames, fit on the GPU, and compare three-body forces on a
+    later frame: surrogate vs exact E3B, against dropping the term."""
+    say("surrogate training on 512-molecule water")
+    from md_engine_amd import E3B, SURROGATE
+
+    box = systems.tip4p2005_box(512, seed=12, min_sep=1.8)
+    v = start_velocities(orc, box, seed=2)
+    e, flags = water_md_engine(gpu_engine_factory, box, v, rc=8.0, skin=1.0, sz=(32, 32, 32), order=5,
+                               e3b=True, dt=2.0)
+    e.run(200, flags)
+    nm = len(box["mol"])
+    for k in range(8):
+        e.surrogate_record(reset=(k == 0), capacity=8 * nm)
+        e.run(50, flags)
+    e.surrogate_set(S.init_params(3).astype(np.float32))
+    l0 = e.surrogate_fit(1, batch=512, lr=1e-3, seed=1, normalize=True)
+    l1 = e.surrogate_fit(400, batch=512, lr=1e-3, seed=2)
+    say(f"loss {l0:.4f} -> {l1:.4f}")
+    assert l1 < 0.5 * l0
+    e.run(100, flags)
+    e.compute(E3B)
+    f_exact = e.forces()
+    e.compute(SURROGATE)
+    f_sur = e.forces()
+    x, _ = e.get_system()
+    b2 = dict(box, x=x)
+    f2 = e2_forces(b2, orc.e3b_params())
+    three = f_exact - f2
+    err_sur = np.sqrt(((f_sur - f_exact) ** 2).sum(1).mean())
+    err_drop = np.sqrt((three ** 2).sum(1).mean())
+    say(f"three-body force rms: dropped {err_drop:.4f}, surrogate error {err_sur:.4f} kcal/mol/A")
+    assert err_sur < 0.5 * err_drop
