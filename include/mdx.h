@@ -178,6 +178,15 @@ int mdx_surrogate_eval(mdx_ctx *ctx, float *emol, float *feat);
  * type B, acceptor of type C; the per-molecule shares of
  * compute_E3B_energy_threebody, src/Evaluators/ThreeBodyE3B.h:553-918). */
 int mdx_e3b_molecule_energies(mdx_ctx *ctx, float *emol);
+/* Nose-Hoover chain thermostat (FixNoseHoover, src/Fixes/FixNoseHoover.cu:
+ * 120-1098; setTemperature(T, timeConstant), chainLength 10 there): half a
+ * chain step before the first kick and after the second, scaling all
+ * velocities; chain state stays on the device.  Excludes mdx_thermostat_set.
+ * chain_energy: sum_k Q_k v_k^2 / 2 + ndf kT xi_0 + sum_{k>0} kT xi_k, the
+ * thermostat part of the conserved extended energy. */
+int mdx_nose_hoover_set(mdx_ctx *ctx, int enable, double temperature, double time_constant, int chain_length,
+                        double boltz, double mvv_to_eng);
+int mdx_nose_hoover_energy(mdx_ctx *ctx, double *chain_energy);
 /* Autotuner (Integrator::tune, src/Integrators/Integrator.cu:272-376, grid
  * search of Tunable knobs by timing): times `evals` runs of every list-build
  * kernel/team width, pair team width and E3B team width for the terms in

@@ -27,7 +27,7 @@ SYMBOLS = [
     "mdx_profile_enable", "mdx_profile_get", "mdx_nlist_totals", "mdx_option_set", "mdx_rigid_set",
     "mdx_thermostat_set", "mdx_temperature", "mdx_tune", "mdx_run_batch", "mdx_surrogate_set",
     "mdx_surrogate_get", "mdx_surrogate_record", "mdx_surrogate_fit", "mdx_surrogate_eval",
-    "mdx_e3b_molecule_energies",
+    "mdx_e3b_molecule_energies", "mdx_nose_hoover_set", "mdx_nose_hoover_energy",
 ]
 
 # kernel ids of mdx_profile_get
@@ -105,6 +105,8 @@ def load():
     L.mdx_surrogate_fit.argtypes = [_vp, c, c, d, c, c, C.POINTER(d)]
     L.mdx_surrogate_eval.argtypes = [_vp, _vp, _vp]
     L.mdx_e3b_molecule_energies.argtypes = [_vp, _fp]
+    L.mdx_nose_hoover_set.argtypes = [_vp, c, d, d, c, d, d]
+    L.mdx_nose_hoover_energy.argtypes = [_vp, C.POINTER(d)]
     for name in SYMBOLS:
         fn = getattr(L, name)
         if name not in ("mdx_last_error", "mdx_version"):
@@ -289,6 +291,16 @@ class Engine:
 
     def teams_set(self, build=0, pair=0, e3b=0):
         self._chk(self._L.mdx_teams_set(self._h, int(build), int(pair), int(e3b)), "mdx_teams_set")
+
+    def nose_hoover(self, enable, temperature, time_constant, chain_length=10, boltz=1.0, mvv_to_eng=1.0):
+        self._chk(self._L.mdx_nose_hoover_set(self._h, int(bool(enable)), float(temperature),
+                                              float(time_constant), int(chain_length), float(boltz),
+                                              float(mvv_to_eng)), "mdx_nose_hoover_set")
+
+    def nose_hoover_energy(self):
+        e = C.c_double(0.0)
+        self._chk(self._L.mdx_nose_hoover_energy(self._h, C.byref(e)), "mdx_nose_hoover_energy")
+        return e.value
 
     # -- learned E3B three-body surrogate -----------------------------------
     SUR_NPARAM = 7459

@@ -95,6 +95,15 @@ void ensure_list(Ctx &c) {
     }
 }
 
+double degrees_of_freedom(const Ctx &c) {
+    // massive sites x 3, minus centre-of-mass motion, minus 3 per rigid molecule
+    double massive = 0;
+    for (double im : c.invMassHost) massive += im > 0 ? 1 : 0;
+    double ndf = 3.0 * massive - 3.0;
+    if (c.rigid.on) ndf -= 3.0 * c.rigid.nmol;
+    return ndf > 0 ? ndf : 1.0;
+}
+
 constexpr int kForceTerms = MDX_PAIR | MDX_PPPM | MDX_E3B | MDX_SURROGATE;
 
 void compute(Ctx &c, int flags) {
@@ -265,6 +274,10 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags) {
             rigid_second_half(c, false);
             c.rigid.fresh = false;
             c.listValid = false;
+        }
+        if (c.nh.on && c.nh.fresh) {
+            c.nh.ndf = degrees_of_freedom(c);
+            nose_hoover_start(c);
         }
     }
     hipGraphExec_t big = nullptr, one = nullptr;
@@ -910,6 +923,10 @@ int mdx_run(mdx_ctx *ctx, int nsteps, int flags) {
             c.rigid.fresh = false;
             c.listValid = false;
         }
+        if (c.nh.on && c.nh.fresh) {
+            c.nh.ndf = degrees_of_freedom(c);
+            nose_hoover_start(c);
+        }
         constexpr int kChunk = 64;
         int done = 0;
         while (done < nsteps) {
@@ -1273,25 +1290,60 @@ int mdx_option_set(mdx_ctx *ctx, const char *name, int value) {
     });
 }
 
-static double degrees_of_freedom(const Ctx &c) {
-    double massive = 0;
-    for (double im : c.invMassHost) massive += im > 0 ? 1 : 0;
-    double ndf = 3.0 * massive - 3.0;
-    if (c.rigid.on) ndf -= 3.0 * c.rigid.nmol;
-    return ndf > 0 ? ndf : 1.0;
-}
-
 int mdx_thermostat_set(mdx_ctx *ctx, int enable, double temperature, int every, double tau,
                        double boltz, double mvv_to_eng) {
     return guarded(ctx, [&](Ctx &c) {
         require_system(c);
         MDX_REQUIRE(!enable || (temperature > 0 && every >= 1 && boltz > 0 && mvv_to_eng > 0), MDX_ERR_ARG,
                     "thermostat needs T > 0, every >= 1, boltz > 0, mvv_to_eng > 0");
+        MDX_REQUIRE(!(enable && c.nh.on), MDX_ERR_STATE, "Nose-Hoover thermostat is on");
         c.thermoOn = enable != 0;
         c.thermoT = temperature;
         c.thermoEvery = every;
         c.thermoTau = tau;
         c.tconv = mvv_to_eng / (degrees_of_freedom(c) * boltz);
+    });
+}
+
+int mdx_nose_hoover_set(mdx_ctx *ctx, int enable, double temperature, double time_constant, int chain_length,
+                        double boltz, double mvv_to_eng) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        MDX_REQUIRE(!enable || (temperature > 0 && time_constant > 0 && chain_length >= 2 &&
+                                chain_length <= 64 && boltz > 0 && mvv_to_eng > 0),
+                    MDX_ERR_ARG, "Nose-Hoover needs T > 0, time constant > 0, 2 <= chain <= 64, units > 0");
+        MDX_REQUIRE(!(enable && c.thermoOn), MDX_ERR_STATE, "velocity-rescale thermostat is on");
+        NoseHooverState &H = c.nh;
+        if (enable && (!H.dev || H.chain != chain_length)) {
+            if (H.dev) dfree(c, H.dev);
+            H.dev = dalloc<double>(c, 4 * chain_length + 2);
+        }
+        H.on = enable != 0;
+        H.fresh = true;
+        H.chain = chain_length;
+        H.T = temperature;
+        H.freq = 1.0 / time_constant;
+        H.boltz = boltz;
+        H.mvv = mvv_to_eng;
+        H.ndf = degrees_of_freedom(c);
+    });
+}
+
+int mdx_nose_hoover_energy(mdx_ctx *ctx, double *chain_energy) {
+    return guarded(ctx, [&](Ctx &c) {
+        NoseHooverState &H = c.nh;
+        MDX_REQUIRE(H.on && H.dev && chain_energy, MDX_ERR_STATE, "Nose-Hoover is off");
+        std::vector<double> st(4 * H.chain + 2);
+        MDX_HIP(hipMemcpyAsync(st.data(), H.dev, sizeof(double) * st.size(), hipMemcpyDeviceToHost, c.stream));
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        const int M = H.chain;
+        const double kT = H.boltz * H.T;
+        double e = 0.0;
+        for (int k = 0; k < M; ++k) {
+            e += 0.5 * st[2 * M + k] * st[k] * st[k];
+            e += (k == 0 ? H.ndf : 1.0) * kT * st[3 * M + k];
+        }
+        *chain_energy = H.fresh ? 0.0 : e;
     });
 }
 

@@ -99,17 +99,106 @@ __global__ void k_fill_float(int n, float *__restrict__ p, float v) {
     if (i < n) p[i] = v;
 }
 
+// Nose-Hoover chain (FixNoseHoover::thermostatIntegrate,
+// src/Fixes/FixNoseHoover.cu:795-866, after Martyna, Tuckerman, Tobias and
+// Klein 1996): one thread advances the chain by half a step and leaves the
+// velocity scale factor in st[4M + 1]; `measure` refreshes twice the kinetic
+// energy (sum m v^2, which equipartition pairs with ndf kT) from thermo[0]
+// first (the stepFinal half).  Chain positions are integrated
+// as well so that the conserved extended energy can be reported.
+__global__ void k_nh_half(const int *__restrict__ gate, const unsigned long long *__restrict__ thermo,
+                          double *__restrict__ st, int M, double ndf, double kT, double dt, double mvv,
+                          int measure) {
+    if (gate && *gate == 0) return;
+    double *vel = st, *frc = st + M, *mass = st + 2 * M, *pos = st + 3 * M;
+    double &ke = st[4 * M];
+    const double nkt = ndf * kT;
+    if (measure) {
+        ke = (double)(long long)thermo[0] * kFxInv * mvv;  // sum m v^2 = 2 KE, paired with ndf kT
+        frc[0] = (ke - nkt) / mass[0];
+    }
+    const double t2 = 0.5 * dt, t4 = 0.25 * dt, t8 = 0.125 * dt;
+    vel[M - 1] += t4 * frc[M - 1];
+    for (int k = M - 2; k > 0; --k) {
+        const double pf = exp(-t8 * vel[k + 1]);
+        vel[k] = (vel[k] * pf + t4 * frc[k]) * pf;
+    }
+    double pf = exp(-t8 * vel[1]);
+    vel[0] = (vel[0] * pf + t4 * frc[0]) * pf;
+    const double s = exp(-t2 * vel[0]);
+    ke *= s * s;
+    for (int k = 0; k < M; ++k) pos[k] += t2 * vel[k];
+    vel[0] *= pf;
+    frc[0] = (ke - nkt) / mass[0];
+    vel[0] = (vel[0] + t4 * frc[0]) * pf;
+    for (int k = 1; k < M - 1; ++k) {
+        pf = exp(-t8 * vel[k + 1]);
+        vel[k] *= pf;
+        frc[k] = (mass[k - 1] * vel[k - 1] * vel[k - 1] - kT) / mass[k];
+        vel[k] = (vel[k] + t4 * frc[k]) * pf;
+    }
+    frc[M - 1] = (mass[M - 2] * vel[M - 2] * vel[M - 2] - kT) / mass[M - 1];
+    vel[M - 1] += t4 * frc[M - 1];
+    st[4 * M + 1] = s;
+}
+
+// chain start (FixNoseHoover::prepareFinal, FixNoseHoover.cu:240-300 and
+// updateThermalMasses, 1034-1042): Q_0 = ndf kT / w^2, Q_k = kT / w^2
+__global__ void k_nh_start(const unsigned long long *__restrict__ thermo, double *__restrict__ st, int M,
+                           double ndf, double kT, double freq, double mvv) {
+    double *vel = st, *frc = st + M, *mass = st + 2 * M, *pos = st + 3 * M;
+    const double ke = (double)(long long)thermo[0] * kFxInv * mvv;  // 2 KE
+    st[4 * M] = ke;
+    st[4 * M + 1] = 1.0;
+    for (int k = 0; k < M; ++k) {
+        vel[k] = 0.0;
+        pos[k] = 0.0;
+        mass[k] = (k == 0 ? ndf : 1.0) * kT / (freq * freq);
+    }
+    frc[0] = (ke - ndf * kT) / mass[0];
+    for (int k = 1; k < M; ++k) frc[k] = (mass[k - 1] * vel[k - 1] * vel[k - 1] - kT) / mass[k];
+}
+
+__global__ void k_vscale_dev(int n, double4 *__restrict__ vel, const double *__restrict__ scale) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double s = *scale;
+    double4 v = vel[i];
+    v.x *= s;
+    v.y *= s;
+    v.z *= s;
+    vel[i] = v;
+}
+
 }  // namespace
+
+static void measure_mv2(Ctx &c, const int *gate) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, gate, 2, c.thermo);
+    hipLaunchKernelGGL(k_mv2, dim3(cdiv(c.n, 256)), dim3(256), 0, c.stream, gate, c.n, c.vel[c.cur], c.thermo);
+}
+
+static void nh_half(Ctx &c, bool final_half) {
+    NoseHooverState &H = c.nh;
+    if (final_half) measure_mv2(c, nullptr);  // every step, also inside a captured step
+    hipLaunchKernelGGL(k_nh_half, dim3(1), dim3(1), 0, c.stream, nullptr, c.thermo, H.dev, H.chain, H.ndf,
+                       H.boltz * H.T, c.dt, H.mvv, final_half ? 1 : 0);
+    hipLaunchKernelGGL(k_vscale_dev, dim3(cdiv(c.n, 256)), dim3(256), 0, c.stream, c.n, c.vel[c.cur],
+                       H.dev + 4 * H.chain + 1);
+}
+
+void nose_hoover_start(Ctx &c) {
+    NoseHooverState &H = c.nh;
+    measure_mv2(c, nullptr);
+    hipLaunchKernelGGL(k_nh_start, dim3(1), dim3(1), 0, c.stream, c.thermo, H.dev, H.chain, H.ndf,
+                       H.boltz * H.T, H.freq, H.mvv);
+    H.fresh = false;
+}
 
 void fill_float(Ctx &c, float *p, int count, float v) {
     hipLaunchKernelGGL(k_fill_float, dim3(cdiv(count, 256)), dim3(256), 0, c.stream, count, p, v);
 }
 
-void thermo_measure(Ctx &c) {
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, thermo_gate(c), 2, c.thermo);
-    hipLaunchKernelGGL(k_mv2, dim3(cdiv(c.n, 256)), dim3(256), 0, c.stream, thermo_gate(c), c.n, c.vel[c.cur],
-                       c.thermo);
-}
+void thermo_measure(Ctx &c) { measure_mv2(c, thermo_gate(c)); }
 
 void thermostat_apply(Ctx &c) {
     thermo_measure(c);
@@ -128,6 +217,7 @@ void zero_forces(Ctx &c, bool clear_acc) {
 
 void integrate_first_half(Ctx &c) {
     const int threads = 256;
+    if (c.nh.on) nh_half(c, false);
     const int *member = c.rigid.on ? c.rigid.member : nullptr;
     if (!c.rigid.on || c.rigid.nsites < c.n)
         hipLaunchKernelGGL(k_vv1, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n,
@@ -142,6 +232,7 @@ void integrate_second_half(Ctx &c) {
         hipLaunchKernelGGL(k_vv2, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n,
                            c.vel[c.cur], c.force, 0.5 * c.dt * c.ftmToV, member);
     if (c.rigid.on) rigid_second_half(c, true);
+    if (c.nh.on) nh_half(c, true);
 }
 
 }  // namespace mdx

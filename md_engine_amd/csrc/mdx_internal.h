@@ -85,6 +85,15 @@ struct E3BState {
     int team = 8;
 };
 
+// Nose-Hoover chain thermostat state, kept on the device so a captured step
+// needs no host round trip: dev = [vel M | force M | mass M | pos M | ke | scale]
+struct NoseHooverState {
+    bool on = false, fresh = true;
+    int chain = 10;
+    double T = 0.0, freq = 0.0, boltz = 0.0, mvv = 0.0, ndf = 0.0;
+    double *dev = nullptr;
+};
+
 // learned E3B three-body surrogate (surrogate.hip)
 constexpr int kSurNB = 16, kSurNH = 64;
 constexpr int kSurNParam = 3 * kSurNB * kSurNH + kSurNH + kSurNH * kSurNH + kSurNH + kSurNH + 1 + 6 * kSurNB + 2;
@@ -199,6 +208,7 @@ struct Ctx {
     PppmState pppm;
     E3BState e3b;
     SurrogateState sur;
+    NoseHooverState nh;
     RigidState rigid;
     bool mdxDisableTiledSpread = false;   // test knob: force the plain global-atomic spread
 
@@ -347,6 +357,7 @@ void rigid_second_half(Ctx &c, bool kick);
 void integrate_first_half(Ctx &c);
 void integrate_second_half(Ctx &c);
 void zero_forces(Ctx &c, bool clear_acc);
+void nose_hoover_start(Ctx &c);       // chain masses and forces from the current kinetic energy
 void thermo_measure(Ctx &c);          // sum m v^2 into thermo[0]
 void thermostat_apply(Ctx &c);
 

@@ -77,40 +77,61 @@ __device__ __forceinline__ float3 mi(float4 a, float4 b, const Geo3 &g) {
 }
 
 // ---------------------------------------------------------------------------
-// descriptors: 16 lanes per molecule, lane k owns basis k (no reductions)
+// descriptors: 4 lanes per molecule, lane t owns basis functions 4t .. 4t+3,
+// so every output channel has one owner (no reductions); a neighbour whose
+// O-O distance exceeds RF_C + r_OH cannot reach the cutoff and is skipped.
+constexpr float kSkipOO2 = (RF_C + 1.0f) * (RF_C + 1.0f);
+
+__device__ __forceinline__ void gauss4(float r, int k0, float s, float g[4]) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const float d = r - mu_k(k0 + u);
+        g[u] = __expf(-ETA * d * d) * s;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_sur_feat(int nmol, const int4 *__restrict__ molAtoms,
                                                   const float4 *__restrict__ xq, Geo3 g, int cap,
                                                   const int *__restrict__ nl, const int *__restrict__ nlCount,
                                                   float *__restrict__ feat) {
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = gt >> 4, k = gt & 15;
+    const int i = gt >> 2, k0 = 4 * (gt & 3);
     if (i >= nmol) return;
     const int4 ai = molAtoms[i];
     const float4 Oi = xq[ai.x], H1i = xq[ai.y], H2i = xq[ai.z];
-    const float mu = mu_k(k);
-    float acc = 0.f, d1 = 0.f, d2 = 0.f;
+    float acc[4] = {0.f, 0.f, 0.f, 0.f}, d1[4] = {0.f, 0.f, 0.f, 0.f}, d2[4] = {0.f, 0.f, 0.f, 0.f};
     const int cnt = min(nlCount[i], cap);
     const int *row = nl + (size_t)i * cap;
     for (int t = 0; t < cnt; ++t) {
         const int4 aj = molAtoms[row[t]];
-        const float4 Oj = xq[aj.x], H1j = xq[aj.y], H2j = xq[aj.z];
+        const float4 Oj = xq[aj.x];
+        const float3 oo = mi(Oj, Oi, g);
+        if (oo.x * oo.x + oo.y * oo.y + oo.z * oo.z > kSkipOO2) continue;
+        const float4 H1j = xq[aj.y], H2j = xq[aj.z];
         const float3 r4[4] = {mi(H1j, Oi, g), mi(H2j, Oi, g), mi(H1i, Oj, g), mi(H2i, Oj, g)};
-        float v[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const float r = sqrtf(r4[q].x * r4[q].x + r4[q].y * r4[q].y + r4[q].z * r4[q].z);
-            float s, ds;
-            fcut(r, s, ds);
-            v[q] = s > 0.f ? __expf(-ETA * (r - mu) * (r - mu)) * s : 0.f;
+            float sw, ds;
+            fcut(r, sw, ds);
+            if (sw <= 0.f) continue;
+            float gv[4];
+            gauss4(r, k0, sw, gv);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (q < 2) acc[u] += gv[u];
+                else if (q == 2) d1[u] += gv[u];
+                else d2[u] += gv[u];
+            }
         }
-        acc += v[0] + v[1];
-        d1 += v[2];
-        d2 += v[3];
     }
     float *f = feat + (size_t)i * NF;
-    f[k] = acc;
-    f[NB + k] = d1;
-    f[2 * NB + k] = d2;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        f[k0 + u] = acc[u];
+        f[NB + k0 + u] = d1[u];
+        f[2 * NB + k0 + u] = d2[u];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -305,43 +326,57 @@ __global__ void __launch_bounds__(256) k_sur_mlp(int nmol, const float *__restri
 }
 
 // ---------------------------------------------------------------------------
-// forces: 16 lanes per molecule (lane k = basis k); owner-computes
+// forces: 4 lanes per molecule (lane t: basis 4t .. 4t+3), owner-computes;
+// the lanes' partial forces are summed by shuffles
 __global__ void __launch_bounds__(256) k_sur_force(int nmol, const int4 *__restrict__ molAtoms,
                                                    const float4 *__restrict__ xq, Geo3 g, int cap,
                                                    const int *__restrict__ nl, const int *__restrict__ nlCount,
                                                    const float *__restrict__ wgt,
                                                    unsigned long long *__restrict__ force) {
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = gt >> 4, k = gt & 15;
+    const int i = gt >> 2, k0 = 4 * (gt & 3);
     float3 FO = make_float3(0.f, 0.f, 0.f), F1 = FO, F2 = FO;
     int4 ai = make_int4(0, 0, 0, 0);
     if (i < nmol) {
         ai = molAtoms[i];
         const float4 Oi = xq[ai.x], H1i = xq[ai.y], H2i = xq[ai.z];
-        const float mu = mu_k(k);
-        const float wa_i = wgt[(size_t)i * NF + k], w1_i = wgt[(size_t)i * NF + NB + k],
-                    w2_i = wgt[(size_t)i * NF + 2 * NB + k];
+        const float4 wa_i = *(const float4 *)(wgt + (size_t)i * NF + k0);
+        const float4 w1_i = *(const float4 *)(wgt + (size_t)i * NF + NB + k0);
+        const float4 w2_i = *(const float4 *)(wgt + (size_t)i * NF + 2 * NB + k0);
         const int cnt = min(nlCount[i], cap);
         const int *row = nl + (size_t)i * cap;
         for (int t = 0; t < cnt; ++t) {
             const int mj = row[t];
             const int4 aj = molAtoms[mj];
-            const float4 Oj = xq[aj.x], H1j = xq[aj.y], H2j = xq[aj.z];
-            const float wa_j = wgt[(size_t)mj * NF + k], w1_j = wgt[(size_t)mj * NF + NB + k],
-                        w2_j = wgt[(size_t)mj * NF + 2 * NB + k];
-            // pairs with an H of i: weight (donor channel of i + acceptor channel of j)
-            // pairs with an H of j: weight (donor channel of j + acceptor channel of i)
+            const float4 Oj = xq[aj.x];
+            const float3 oo = mi(Oj, Oi, g);
+            if (oo.x * oo.x + oo.y * oo.y + oo.z * oo.z > kSkipOO2) continue;
+            const float4 H1j = xq[aj.y], H2j = xq[aj.z];
+            const float4 wa_j = *(const float4 *)(wgt + (size_t)mj * NF + k0);
+            const float4 w1_j = *(const float4 *)(wgt + (size_t)mj * NF + NB + k0);
+            const float4 w2_j = *(const float4 *)(wgt + (size_t)mj * NF + 2 * NB + k0);
+            // an H of i feeds i's donor channel and j's acceptor channel;
+            // an H of j feeds j's donor channel and i's acceptor channel
             const float3 d[4] = {mi(H1i, Oj, g), mi(H2i, Oj, g), mi(H1j, Oi, g), mi(H2j, Oi, g)};
-            const float wq[4] = {w1_i + wa_j, w2_i + wa_j, w1_j + wa_i, w2_j + wa_i};
+            const float4 wq[4] = {make_float4(w1_i.x + wa_j.x, w1_i.y + wa_j.y, w1_i.z + wa_j.z, w1_i.w + wa_j.w),
+                                  make_float4(w2_i.x + wa_j.x, w2_i.y + wa_j.y, w2_i.z + wa_j.z, w2_i.w + wa_j.w),
+                                  make_float4(w1_j.x + wa_i.x, w1_j.y + wa_i.y, w1_j.z + wa_i.z, w1_j.w + wa_i.w),
+                                  make_float4(w2_j.x + wa_i.x, w2_j.y + wa_i.y, w2_j.z + wa_i.z, w2_j.w + wa_i.w)};
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const float r = sqrtf(d[q].x * d[q].x + d[q].y * d[q].y + d[q].z * d[q].z);
                 float s, ds;
                 fcut(r, s, ds);
                 if (s <= 0.f && ds == 0.f) continue;
-                const float ga = __expf(-ETA * (r - mu) * (r - mu));
-                const float dg = ga * (ds - 2.0f * ETA * (r - mu) * s);
-                const float c = wq[q] * dg / r;  // dE/dr / r
+                const float w4[4] = {wq[q].x, wq[q].y, wq[q].z, wq[q].w};
+                float dEdr = 0.f;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const float dm = r - mu_k(k0 + u);
+                    const float ga = __expf(-ETA * dm * dm);
+                    dEdr += w4[u] * ga * (ds - 2.0f * ETA * dm * s);
+                }
+                const float c = dEdr / r;
                 // F_H = -c d, F_O = +c d  (d = r_H - r_O)
                 if (q == 0) {
                     F1.x -= c * d[q].x;
@@ -359,16 +394,16 @@ __global__ void __launch_bounds__(256) k_sur_force(int nmol, const int4 *__restr
             }
         }
     }
-    FO.x = team_sum<16>(FO.x);
-    FO.y = team_sum<16>(FO.y);
-    FO.z = team_sum<16>(FO.z);
-    F1.x = team_sum<16>(F1.x);
-    F1.y = team_sum<16>(F1.y);
-    F1.z = team_sum<16>(F1.z);
-    F2.x = team_sum<16>(F2.x);
-    F2.y = team_sum<16>(F2.y);
-    F2.z = team_sum<16>(F2.z);
-    if (k == 0 && i < nmol) {
+    FO.x = team_sum<4>(FO.x);
+    FO.y = team_sum<4>(FO.y);
+    FO.z = team_sum<4>(FO.z);
+    F1.x = team_sum<4>(F1.x);
+    F1.y = team_sum<4>(F1.y);
+    F1.z = team_sum<4>(F1.z);
+    F2.x = team_sum<4>(F2.x);
+    F2.y = team_sum<4>(F2.y);
+    F2.z = team_sum<4>(F2.z);
+    if ((gt & 3) == 0 && i < nmol) {
         fx_add(&force[3 * ai.x + 0], FO.x);
         fx_add(&force[3 * ai.x + 1], FO.y);
         fx_add(&force[3 * ai.x + 2], FO.z);
@@ -571,7 +606,7 @@ static_assert(OEH + 1 == kSurNParam, "packed surrogate layout");
 
 void surrogate_features(Ctx &c, float *raw) {
     E3BState &E = c.e3b;
-    hipLaunchKernelGGL(k_sur_feat, dim3(cdiv((long long)E.nmol * 16, 256)), dim3(256), 0, c.stream, E.nmol,
+    hipLaunchKernelGGL(k_sur_feat, dim3(cdiv((long long)E.nmol * 4, 256)), dim3(256), 0, c.stream, E.nmol,
                        E.molAtoms, c.xq, geo3(c), E.cap, E.nl, E.nlCount, raw);
 }
 
@@ -581,7 +616,7 @@ void surrogate_forces(Ctx &c, bool energy) {
     surrogate_features(c, S.feat);
     hipLaunchKernelGGL(k_sur_mlp, dim3(cdiv(E.nmol, 64)), dim3(256), 0, c.stream, E.nmol, S.params, S.feat,
                        S.emol, S.wgt, c.acc, energy ? 1 : 0);
-    hipLaunchKernelGGL(k_sur_force, dim3(cdiv((long long)E.nmol * 16, 256)), dim3(256), 0, c.stream, E.nmol,
+    hipLaunchKernelGGL(k_sur_force, dim3(cdiv((long long)E.nmol * 4, 256)), dim3(256), 0, c.stream, E.nmol,
                        E.molAtoms, c.xq, geo3(c), E.cap, E.nl, E.nlCount, S.wgt, c.force);
 }
 

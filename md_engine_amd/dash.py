@@ -546,7 +546,7 @@ class EvaluatorLJ:
 
     def energy(self, sigma, epsilon, rcut, r):
         _, e = self._pair(sigma, epsilon, rcut, Vector(r, 0.0, 0.0))
-        return 2.0 * e  # the kernel splits each pair energy half to each atom
+        return e  # total of the two atoms' halves = the pair energy
 
 
 def create_TIP4P_2005(state, oxygenHandle, hydrogenHandle, mSiteHandle, center=None, orientation=None,

