@@ -1283,7 +1283,7 @@ int mdx_option_set(mdx_ctx *ctx, const char *name, int value) {
             c.graphSteps = value;
         }
         else if (k == "build_mode") {
-            MDX_REQUIRE(value >= 0 && value <= 2, MDX_ERR_ARG, "build_mode: 0 auto, 1 per cell, 2 per atom");
+            MDX_REQUIRE(value >= 0 && value <= 2, MDX_ERR_ARG, "build_mode: 0 default (per atom), 1 per cell, 2 per atom");
             c.buildMode = value;
         }
         else throw Error{MDX_ERR_ARG, "unknown option " + k};

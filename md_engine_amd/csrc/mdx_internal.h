@@ -183,7 +183,7 @@ struct Ctx {
     long long listViolations = 0; // steps whose list was older than the skin allows
     int buildTeam = 32;
     bool buildTeamUser = false;   // set by mdx_teams_set, else chosen from the cell population
-    int buildMode = 0;            // 0 auto, 1 LDS-staged per-cell build, 2 per-atom teams
+    int buildMode = 0;            // 1 LDS-staged per-cell build; 0 (default) and 2 per-atom teams
     long long nRebuilds = 0;
     bool listValid = false;
 

@@ -955,13 +955,10 @@ void nlist_setup(Ctx &c) {
         c.perm = dalloc<int>(c, c.n);
         c.perm2 = dalloc<int>(c, c.n);
     }
-    // team width: next power of two of the expected z-run population
-    double dens = c.n / (c.geo.L[0] * c.geo.L[1] * c.geo.L[2]);
-    double run = dens * (c.geo.L[0] / c.nc[0]) * (c.geo.L[1] / c.nc[1]) * (c.geo.L[2] / c.nc[2]) *
-                 std::min(c.nc[2], 2 * c.reach + 1);
-    int T = 8;
-    while (T < 64 && T < run) T *= 2;
-    if (!c.buildTeamUser) c.buildTeam = T;
+    // team width of the per-atom build: 8 lanes measured fastest for dense
+    // water (runs of ~60 culled candidates) and LJ alike; mdx_tune revisits it
+    const double dens = c.n / (c.geo.L[0] * c.geo.L[1] * c.geo.L[2]);
+    if (!c.buildTeamUser) c.buildTeam = 8;
     if (c.cap == 0) {
         double expect = dens * 4.18879020479 * c.rlist * c.rlist * c.rlist;
         c.cap = ((int)(expect * 1.3) + 32 + 15) & ~15;
@@ -1035,8 +1032,7 @@ static void run_build(Ctx &c, bool sync) {
         if (!c.capture) fill_int(c, c.dFlags + 1, 2, 0);
         {
             ProfScope ps(c, PK_BUILD);
-            const double perCell = (double)c.n / c.ncell;
-            if (c.buildMode == 1 || (c.buildMode == 0 && perCell >= 8.0)) {
+            if (c.buildMode == 1) {
                 MDX_REQUIRE(c.reach == 2, 1, "the per-cell build kernel assumes a 5-cell stencil");
                 hipLaunchKernelGGL(k_build_cell, dim3(c.ncell), dim3(256), 0, c.stream, rebuild_gate(c), c.pos[c.cur],
                                    c.xq, c.geo, cg, make_prefilter(c), make_cull(c), c.colXY, c.colRank,

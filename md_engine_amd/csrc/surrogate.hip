@@ -4,8 +4,8 @@
 // BF16 MFMA tiles (FP32 accumulation) with its hand-written backward pass for
 // forces, and on-device training (per-molecule energy regression, Adam).
 //
-// Descriptors of molecule i for 16 radial Gaussians
-// g_k(r) = exp(-ETA (r - MU_k)^2) fc(r)  (fc: switching polynomial RS..RF):
+// Descriptors of molecule i for 16 radial functions
+// g_k(r) = exp(-kappa_k r) fc(r)  (fc: the E3B switching polynomial RS..RF):
 //   acc_k = sum_{j != i, H of j} g_k(|H - O_i|),
 //   d1_k  = sum_{j != i} g_k(|H1_i - O_j|), d2_k likewise for H2_i;
 // network input F = [acc, d1 + d2, d1 * d2], standardised by (mu, sd).
@@ -19,7 +19,9 @@ namespace mdx {
 namespace {
 
 constexpr int NB = kSurNB, NF = 3 * kSurNB, NH = kSurNH;
-constexpr float RS_C = 4.6f, RF_C = 5.2f, ETA = 1.0f / (2.0f * 0.28f * 0.28f);
+// radial basis g_k(r) = exp(-kappa_k r) fc(r), kappa_k = k3 (0.5 + 0.1 k): basis 5
+// is exactly the E3B hydrogen-bond function (k3 = 1.907 / A, switching 5.0-5.2 A)
+constexpr float RS_C = 5.0f, RF_C = 5.2f, KAPPA0 = 0.5f * 1.907f, DKAPPA = 0.1f * 1.907f;
 
 // parameter offsets in the packed model (oracle/surrogate.py layout)
 constexpr int OW1 = 0, OB1 = OW1 + NF * NH, OW2 = OB1 + NH, OB2 = OW2 + NH * NH, OW3 = OB2 + NH,
@@ -29,7 +31,7 @@ constexpr int NTRAIN = OMU;  // W1 b1 W2 b2 w3 b3 are trained; mu sd escale eshi
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ float mu_k(int k) { return 0.9f + 0.28f * (float)k; }
+__device__ __forceinline__ float kappa_k(int k) { return KAPPA0 + DKAPPA * (float)k; }
 
 // orders LDS traffic between the lanes of one wave (no cross-wave barrier)
 __device__ __forceinline__ void wave_sync() {
@@ -77,60 +79,73 @@ __device__ __forceinline__ float3 mi(float4 a, float4 b, const Geo3 &g) {
 }
 
 // ---------------------------------------------------------------------------
-// descriptors: 4 lanes per molecule, lane t owns basis functions 4t .. 4t+3,
-// so every output channel has one owner (no reductions); a neighbour whose
-// O-O distance exceeds RF_C + r_OH cannot reach the cutoff and is skipped.
+// descriptors: 16 lanes per molecule split its neighbours; each lane sums all
+// 48 channels over its neighbours in registers, the 16 partial rows go
+// through LDS and lane t adds up channel column t (acc, d1, d2).  A
+// neighbour whose O-O distance exceeds RF_C + r_OH cannot reach the cutoff.
 constexpr float kSkipOO2 = (RF_C + 1.0f) * (RF_C + 1.0f);
-
-__device__ __forceinline__ void gauss4(float r, int k0, float s, float g[4]) {
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        const float d = r - mu_k(k0 + u);
-        g[u] = __expf(-ETA * d * d) * s;
-    }
-}
+constexpr int kFeatLanes = 16;
 
 __global__ void __launch_bounds__(256) k_sur_feat(int nmol, const int4 *__restrict__ molAtoms,
                                                   const float4 *__restrict__ xq, Geo3 g, int cap,
                                                   const int *__restrict__ nl, const int *__restrict__ nlCount,
                                                   float *__restrict__ feat) {
+    __shared__ float part[256 / kFeatLanes][kFeatLanes][NF + 1];
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = gt >> 2, k0 = 4 * (gt & 3);
-    if (i >= nmol) return;
-    const int4 ai = molAtoms[i];
-    const float4 Oi = xq[ai.x], H1i = xq[ai.y], H2i = xq[ai.z];
-    float acc[4] = {0.f, 0.f, 0.f, 0.f}, d1[4] = {0.f, 0.f, 0.f, 0.f}, d2[4] = {0.f, 0.f, 0.f, 0.f};
-    const int cnt = min(nlCount[i], cap);
-    const int *row = nl + (size_t)i * cap;
-    for (int t = 0; t < cnt; ++t) {
-        const int4 aj = molAtoms[row[t]];
-        const float4 Oj = xq[aj.x];
-        const float3 oo = mi(Oj, Oi, g);
-        if (oo.x * oo.x + oo.y * oo.y + oo.z * oo.z > kSkipOO2) continue;
-        const float4 H1j = xq[aj.y], H2j = xq[aj.z];
-        const float3 r4[4] = {mi(H1j, Oi, g), mi(H2j, Oi, g), mi(H1i, Oj, g), mi(H2i, Oj, g)};
+    const int i = gt / kFeatLanes, tl = threadIdx.x % kFeatLanes, team = threadIdx.x / kFeatLanes;
+    float acc[NB], d1[NB], d2[NB];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const float r = sqrtf(r4[q].x * r4[q].x + r4[q].y * r4[q].y + r4[q].z * r4[q].z);
-            float sw, ds;
-            fcut(r, sw, ds);
-            if (sw <= 0.f) continue;
-            float gv[4];
-            gauss4(r, k0, sw, gv);
+    for (int k = 0; k < NB; ++k) acc[k] = d1[k] = d2[k] = 0.f;
+    if (i < nmol) {
+        const int4 ai = molAtoms[i];
+        const float4 Oi = xq[ai.x], H1i = xq[ai.y], H2i = xq[ai.z];
+        const int cnt = min(nlCount[i], cap);
+        const int *row = nl + (size_t)i * cap;
+        for (int t = tl; t < cnt; t += kFeatLanes) {
+            const int4 aj = molAtoms[row[t]];
+            const float4 Oj = xq[aj.x];
+            const float3 oo = mi(Oj, Oi, g);
+            if (oo.x * oo.x + oo.y * oo.y + oo.z * oo.z > kSkipOO2) continue;
+            const float4 H1j = xq[aj.y], H2j = xq[aj.z];
+            const float3 r4[4] = {mi(H1j, Oi, g), mi(H2j, Oi, g), mi(H1i, Oj, g), mi(H2i, Oj, g)};
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (q < 2) acc[u] += gv[u];
-                else if (q == 2) d1[u] += gv[u];
-                else d2[u] += gv[u];
+            for (int q = 0; q < 4; ++q) {
+                const float r = sqrtf(r4[q].x * r4[q].x + r4[q].y * r4[q].y + r4[q].z * r4[q].z);
+                float sw, ds;
+                fcut(r, sw, ds);
+                if (sw <= 0.f) continue;
+                // exp(-kappa_k r) by recurrence: two exponentials per distance
+                const float step = __expf(-DKAPPA * r);
+                float v = __expf(-KAPPA0 * r) * sw;
+#pragma unroll
+                for (int k = 0; k < NB; ++k) {
+                    if (q < 2) acc[k] += v;
+                    else if (q == 2) d1[k] += v;
+                    else d2[k] += v;
+                    v *= step;
+                }
             }
         }
     }
-    float *f = feat + (size_t)i * NF;
+    float *pr = part[team][tl];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        f[k0 + u] = acc[u];
-        f[NB + k0 + u] = d1[u];
-        f[2 * NB + k0 + u] = d2[u];
+    for (int k = 0; k < NB; ++k) {
+        pr[k] = acc[k];
+        pr[NB + k] = d1[k];
+        pr[2 * NB + k] = d2[k];
+    }
+    __syncthreads();
+    if (i < nmol) {
+        float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+        for (int l = 0; l < kFeatLanes; ++l) {
+            s0 += part[team][l][tl];
+            s1 += part[team][l][NB + tl];
+            s2 += part[team][l][2 * NB + tl];
+        }
+        float *f = feat + (size_t)i * NF;
+        f[tl] = s0;
+        f[NB + tl] = s1;
+        f[2 * NB + tl] = s2;
     }
 }
 
@@ -221,12 +236,25 @@ __device__ __forceinline__ void forward(WaveBuf &w, const float *W1, const float
     wave_sync();
 }
 
-// network output of row s (lane-parallel dot over NH, result in every lane)
-__device__ __forceinline__ float out_row(const WaveBuf &w, int s, const float *w3, float b3) {
-    const int l = threadIdx.x & 63;
-    float v = w.T[s * NH + l] * w3[l];
-    v = team_sum<64>(v);
-    return v + b3;
+// network outputs of the 16 rows: H2 (16 x 64) . w3 as one MFMA column tile
+// (B column 0 = w3, other columns 0); lanes with column 0 hold rows 4q + j
+__device__ __forceinline__ void out_rows(const WaveBuf &w, const float *w3, float b3, float *e16) {
+    const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+    f32x4 c = zero4();
+    for (int k0 = 0; k0 < NH; k0 += 16) {
+        bf16x4 a, b;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int kk = k0 + 4 * q + j;
+            a[j] = to_bf16(w.T[r * NH + kk]);
+            b[j] = r == 0 ? to_bf16(w3[kk]) : (short)0;
+        }
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+    }
+    if (r == 0) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) e16[4 * q + j] = c[j] + b3;
+    }
 }
 
 // given dE/de per row in g[16] (LDS), back to dZ2 (T), dZ1 (T after) ; X grads optional
@@ -253,20 +281,22 @@ __device__ __forceinline__ void backward_hidden(WaveBuf &w, const float *gscale,
     wave_sync();
 }
 
+constexpr int kWaves = 2;  // waves (16-molecule tiles) per workgroup of the MLP kernels
+
 struct SurModel {
     const float *p;  // packed parameters (device)
 };
 
 // ---------------------------------------------------------------------------
 // inference: energies and dE/d(raw channel) per molecule, 4 waves x 16 molecules
-__global__ void __launch_bounds__(256) k_sur_mlp(int nmol, const float *__restrict__ P,
+__global__ void __launch_bounds__(64 * kWaves) k_sur_mlp(int nmol, const float *__restrict__ P,
                                                  const float *__restrict__ raw, float *__restrict__ emol,
                                                  float *__restrict__ wgt, unsigned long long *__restrict__ acc,
                                                  int energy) {
     __shared__ float sW1[NF * NH], sW2[NH * NH], sb1[NH], sb2[NH], sw3[NH], smu[NF], ssd[NF];
-    __shared__ WaveBuf wb[4];
-    __shared__ float sg[4][16];
-    __shared__ int srow[4][16];
+    __shared__ WaveBuf wb[kWaves];
+    __shared__ float sg[kWaves][16], se[kWaves][16];
+    __shared__ int srow[kWaves][16];
     for (int e = threadIdx.x; e < NF * NH; e += blockDim.x) sW1[e] = P[OW1 + e];
     for (int e = threadIdx.x; e < NH * NH; e += blockDim.x) sW2[e] = P[OW2 + e];
     for (int e = threadIdx.x; e < NH; e += blockDim.x) {
@@ -282,7 +312,7 @@ __global__ void __launch_bounds__(256) k_sur_mlp(int nmol, const float *__restri
     const float b3 = P[OB3], es = P[OES], eh = P[OEH];
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
     WaveBuf &w = wb[wv];
-    const int m0 = (blockIdx.x * 4 + wv) * 16;
+    const int m0 = (blockIdx.x * kWaves + wv) * 16;
     if (m0 >= nmol) return;  // whole wave; no block barrier follows
     const int nrow = min(16, nmol - m0);
     if (l < 16) srow[wv][l] = m0 + min(l, nrow - 1);
@@ -290,15 +320,16 @@ __global__ void __launch_bounds__(256) k_sur_mlp(int nmol, const float *__restri
     load_x(w, raw, srow[wv], nrow, smu, ssd);
     wave_sync();
     forward(w, sW1, sb1, sW2, sb2);
+    out_rows(w, sw3, b3, se[wv]);
+    wave_sync();
     double esum = 0.0;
-    for (int s = 0; s < 16; ++s) {
-        const float e = out_row(w, s, sw3, b3);
-        if (l == 0) sg[wv][s] = s < nrow ? es : 0.f;  // dE_i/de = escale
-        if (l == 0 && s < nrow) {
-            emol[m0 + s] = es * e + eh;
-            esum += (double)(es * e + eh);
+    if (l < 16) sg[wv][l] = l < nrow ? es : 0.f;  // dE_i/de = escale
+    if (l == 0)
+        for (int s = 0; s < nrow; ++s) {
+            const float e = es * se[wv][s] + eh;
+            emol[m0 + s] = e;
+            esum += (double)e;
         }
-    }
     wave_sync();
     backward_hidden(w, sg[wv], sw3, sW2);
     // dX = dZ1 W1^T (16 x 48), then to raw channels
@@ -326,7 +357,8 @@ __global__ void __launch_bounds__(256) k_sur_mlp(int nmol, const float *__restri
 }
 
 // ---------------------------------------------------------------------------
-// forces: 4 lanes per molecule (lane t: basis 4t .. 4t+3), owner-computes;
+// forces: 16 lanes per molecule split its neighbours (owner-computes); each
+// lane holds the molecule's 48 channel weights, gathers the neighbour's, and
 // the lanes' partial forces are summed by shuffles
 __global__ void __launch_bounds__(256) k_sur_force(int nmol, const int4 *__restrict__ molAtoms,
                                                    const float4 *__restrict__ xq, Geo3 g, int cap,
@@ -334,47 +366,51 @@ __global__ void __launch_bounds__(256) k_sur_force(int nmol, const int4 *__restr
                                                    const float *__restrict__ wgt,
                                                    unsigned long long *__restrict__ force) {
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = gt >> 2, k0 = 4 * (gt & 3);
+    const int i = gt / kFeatLanes, tl = threadIdx.x % kFeatLanes;
     float3 FO = make_float3(0.f, 0.f, 0.f), F1 = FO, F2 = FO;
     int4 ai = make_int4(0, 0, 0, 0);
     if (i < nmol) {
         ai = molAtoms[i];
         const float4 Oi = xq[ai.x], H1i = xq[ai.y], H2i = xq[ai.z];
-        const float4 wa_i = *(const float4 *)(wgt + (size_t)i * NF + k0);
-        const float4 w1_i = *(const float4 *)(wgt + (size_t)i * NF + NB + k0);
-        const float4 w2_i = *(const float4 *)(wgt + (size_t)i * NF + 2 * NB + k0);
+        float wa_i[NB], w1_i[NB], w2_i[NB];
+        const float4 *wi = (const float4 *)(wgt + (size_t)i * NF);
+#pragma unroll
+        for (int v = 0; v < NB / 4; ++v) {
+            const float4 a4 = wi[v], b4 = wi[NB / 4 + v], c4 = wi[2 * NB / 4 + v];
+            wa_i[4 * v] = a4.x; wa_i[4 * v + 1] = a4.y; wa_i[4 * v + 2] = a4.z; wa_i[4 * v + 3] = a4.w;
+            w1_i[4 * v] = b4.x; w1_i[4 * v + 1] = b4.y; w1_i[4 * v + 2] = b4.z; w1_i[4 * v + 3] = b4.w;
+            w2_i[4 * v] = c4.x; w2_i[4 * v + 1] = c4.y; w2_i[4 * v + 2] = c4.z; w2_i[4 * v + 3] = c4.w;
+        }
         const int cnt = min(nlCount[i], cap);
         const int *row = nl + (size_t)i * cap;
-        for (int t = 0; t < cnt; ++t) {
+        for (int t = tl; t < cnt; t += kFeatLanes) {
             const int mj = row[t];
             const int4 aj = molAtoms[mj];
             const float4 Oj = xq[aj.x];
             const float3 oo = mi(Oj, Oi, g);
             if (oo.x * oo.x + oo.y * oo.y + oo.z * oo.z > kSkipOO2) continue;
             const float4 H1j = xq[aj.y], H2j = xq[aj.z];
-            const float4 wa_j = *(const float4 *)(wgt + (size_t)mj * NF + k0);
-            const float4 w1_j = *(const float4 *)(wgt + (size_t)mj * NF + NB + k0);
-            const float4 w2_j = *(const float4 *)(wgt + (size_t)mj * NF + 2 * NB + k0);
+            const float *wj = wgt + (size_t)mj * NF;
             // an H of i feeds i's donor channel and j's acceptor channel;
             // an H of j feeds j's donor channel and i's acceptor channel
             const float3 d[4] = {mi(H1i, Oj, g), mi(H2i, Oj, g), mi(H1j, Oi, g), mi(H2j, Oi, g)};
-            const float4 wq[4] = {make_float4(w1_i.x + wa_j.x, w1_i.y + wa_j.y, w1_i.z + wa_j.z, w1_i.w + wa_j.w),
-                                  make_float4(w2_i.x + wa_j.x, w2_i.y + wa_j.y, w2_i.z + wa_j.z, w2_i.w + wa_j.w),
-                                  make_float4(w1_j.x + wa_i.x, w1_j.y + wa_i.y, w1_j.z + wa_i.z, w1_j.w + wa_i.w),
-                                  make_float4(w2_j.x + wa_i.x, w2_j.y + wa_i.y, w2_j.z + wa_i.z, w2_j.w + wa_i.w)};
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const float r = sqrtf(d[q].x * d[q].x + d[q].y * d[q].y + d[q].z * d[q].z);
                 float s, ds;
                 fcut(r, s, ds);
                 if (s <= 0.f && ds == 0.f) continue;
-                const float w4[4] = {wq[q].x, wq[q].y, wq[q].z, wq[q].w};
+                // d/dr [exp(-kappa r) fc] = exp(-kappa r) (fc' - kappa fc)
                 float dEdr = 0.f;
+                const float step = __expf(-DKAPPA * r);
+                float ek = __expf(-KAPPA0 * r);
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const float dm = r - mu_k(k0 + u);
-                    const float ga = __expf(-ETA * dm * dm);
-                    dEdr += w4[u] * ga * (ds - 2.0f * ETA * dm * s);
+                for (int k = 0; k < NB; ++k) {
+                    const float w = q == 0 ? w1_i[k] + wj[k]
+                                  : q == 1 ? w2_i[k] + wj[k]
+                                  : q == 2 ? wj[NB + k] + wa_i[k] : wj[2 * NB + k] + wa_i[k];
+                    dEdr += w * ek * (ds - kappa_k(k) * s);
+                    ek *= step;
                 }
                 const float c = dEdr / r;
                 // F_H = -c d, F_O = +c d  (d = r_H - r_O)
@@ -394,16 +430,16 @@ __global__ void __launch_bounds__(256) k_sur_force(int nmol, const int4 *__restr
             }
         }
     }
-    FO.x = team_sum<4>(FO.x);
-    FO.y = team_sum<4>(FO.y);
-    FO.z = team_sum<4>(FO.z);
-    F1.x = team_sum<4>(F1.x);
-    F1.y = team_sum<4>(F1.y);
-    F1.z = team_sum<4>(F1.z);
-    F2.x = team_sum<4>(F2.x);
-    F2.y = team_sum<4>(F2.y);
-    F2.z = team_sum<4>(F2.z);
-    if ((gt & 3) == 0 && i < nmol) {
+    FO.x = team_sum<kFeatLanes>(FO.x);
+    FO.y = team_sum<kFeatLanes>(FO.y);
+    FO.z = team_sum<kFeatLanes>(FO.z);
+    F1.x = team_sum<kFeatLanes>(F1.x);
+    F1.y = team_sum<kFeatLanes>(F1.y);
+    F1.z = team_sum<kFeatLanes>(F1.z);
+    F2.x = team_sum<kFeatLanes>(F2.x);
+    F2.y = team_sum<kFeatLanes>(F2.y);
+    F2.z = team_sum<kFeatLanes>(F2.z);
+    if (tl == 0 && i < nmol) {
         fx_add(&force[3 * ai.x + 0], FO.x);
         fx_add(&force[3 * ai.x + 1], FO.y);
         fx_add(&force[3 * ai.x + 2], FO.z);
@@ -419,16 +455,16 @@ __global__ void __launch_bounds__(256) k_sur_force(int nmol, const int4 *__restr
 // ---------------------------------------------------------------------------
 // training: per-wave partial gradients of the mean squared error of the
 // standardised per-molecule energy over one mini-batch
-constexpr int kTrainWaves = 256;  // 64 blocks of 4 waves
+constexpr int kTrainWaves = 256;  // 128 blocks of 2 waves
 constexpr int kGradStride = NTRAIN + 1;  // trained parameters + loss
 
-__global__ void __launch_bounds__(256) k_sur_grad(int nsamp, int bstart, int bsize, unsigned mulA, unsigned addB,
+__global__ void __launch_bounds__(64 * kWaves) k_sur_grad(int nsamp, int bstart, int bsize, unsigned mulA, unsigned addB,
                                                   const float *__restrict__ P, const float *__restrict__ raw,
                                                   const float *__restrict__ target, float *__restrict__ part) {
     __shared__ float sW1[NF * NH], sW2[NH * NH], sb1[NH], sb2[NH], sw3[NH], smu[NF], ssd[NF];
-    __shared__ WaveBuf wb[4];
-    __shared__ float sg[4][16];
-    __shared__ int srow[4][16];
+    __shared__ WaveBuf wb[kWaves];
+    __shared__ float sg[kWaves][16], se[kWaves][16];
+    __shared__ int srow[kWaves][16];
     for (int e = threadIdx.x; e < NF * NH; e += blockDim.x) sW1[e] = P[OW1 + e];
     for (int e = threadIdx.x; e < NH * NH; e += blockDim.x) sW2[e] = P[OW2 + e];
     for (int e = threadIdx.x; e < NH; e += blockDim.x) {
@@ -443,7 +479,7 @@ __global__ void __launch_bounds__(256) k_sur_grad(int nsamp, int bstart, int bsi
     __syncthreads();
     const float b3 = P[OB3], es = P[OES], eh = P[OEH];
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, q = l >> 4;
-    const int wave = blockIdx.x * 4 + wv;
+    const int wave = blockIdx.x * kWaves + wv;
     WaveBuf &w = wb[wv];
     f32x4 gW1[3][4], gW2[4][4];
 #pragma unroll
@@ -457,7 +493,7 @@ __global__ void __launch_bounds__(256) k_sur_grad(int nsamp, int bstart, int bsi
     float gb1 = 0.f, gb2 = 0.f, gw3 = 0.f, gb3 = 0.f, loss = 0.f;
     const int ntile = (bsize + 15) / 16;
     const float inv = 1.0f / (float)bsize;
-    for (int t = wave; t < ntile; t += gridDim.x * 4) {
+    for (int t = wave; t < ntile; t += gridDim.x * kWaves) {
         const int s0 = t * 16;
         const int nrow = min(16, bsize - s0);
         if (l < 16) {
@@ -469,20 +505,23 @@ __global__ void __launch_bounds__(256) k_sur_grad(int nsamp, int bstart, int bsi
         load_x(w, raw, srow[wv], nrow, smu, ssd);
         wave_sync();
         forward(w, sW1, sb1, sW2, sb2);
-        for (int s = 0; s < 16; ++s) {
-            const float e = out_row(w, s, sw3, b3);
-            if (l == 0) {
-                float gr = 0.f;
-                if (s < nrow) {
-                    const float tt = (target[srow[wv][s]] - eh) / es;
-                    const float diff = e - tt;
-                    loss += diff * diff * inv;
-                    gr = 2.0f * diff * inv;
-                }
-                sg[wv][s] = gr;
+        out_rows(w, sw3, b3, se[wv]);
+        wave_sync();
+        if (l < 16) {
+            float gr = 0.f;
+            if (l < nrow) {
+                const float tt = (target[srow[wv][l]] - eh) / es;
+                const float diff = se[wv][l] - tt;
+                gr = 2.0f * diff * inv;
+                se[wv][l] = diff * diff * inv;
+            } else {
+                se[wv][l] = 0.f;
             }
+            sg[wv][l] = gr;
         }
         wave_sync();
+        if (l == 0)
+            for (int s = 0; s < 16; ++s) loss += se[wv][s];
         // dw3 += H2^T g, db3 += sum g (lane l owns hidden unit l)
         for (int s = 0; s < 16; ++s) gw3 += sg[wv][s] * w.T[s * NH + l];
         if (l == 0)
@@ -606,7 +645,7 @@ static_assert(OEH + 1 == kSurNParam, "packed surrogate layout");
 
 void surrogate_features(Ctx &c, float *raw) {
     E3BState &E = c.e3b;
-    hipLaunchKernelGGL(k_sur_feat, dim3(cdiv((long long)E.nmol * 4, 256)), dim3(256), 0, c.stream, E.nmol,
+    hipLaunchKernelGGL(k_sur_feat, dim3(cdiv((long long)E.nmol * kFeatLanes, 256)), dim3(256), 0, c.stream, E.nmol,
                        E.molAtoms, c.xq, geo3(c), E.cap, E.nl, E.nlCount, raw);
 }
 
@@ -614,9 +653,9 @@ void surrogate_forces(Ctx &c, bool energy) {
     SurrogateState &S = c.sur;
     E3BState &E = c.e3b;
     surrogate_features(c, S.feat);
-    hipLaunchKernelGGL(k_sur_mlp, dim3(cdiv(E.nmol, 64)), dim3(256), 0, c.stream, E.nmol, S.params, S.feat,
+    hipLaunchKernelGGL(k_sur_mlp, dim3(cdiv(E.nmol, 16 * kWaves)), dim3(64 * kWaves), 0, c.stream, E.nmol, S.params, S.feat,
                        S.emol, S.wgt, c.acc, energy ? 1 : 0);
-    hipLaunchKernelGGL(k_sur_force, dim3(cdiv((long long)E.nmol * 4, 256)), dim3(256), 0, c.stream, E.nmol,
+    hipLaunchKernelGGL(k_sur_force, dim3(cdiv((long long)E.nmol * kFeatLanes, 256)), dim3(256), 0, c.stream, E.nmol,
                        E.molAtoms, c.xq, geo3(c), E.cap, E.nl, E.nlCount, S.wgt, c.force);
 }
 
@@ -656,7 +695,7 @@ double surrogate_fit(Ctx &c, int epochs, int batch, double lr, unsigned seed) {
             ++S.step;
             const float c1 = 1.0f / (1.0f - std::pow(0.9f, (float)S.step));
             const float c2 = 1.0f / (1.0f - std::pow(0.999f, (float)S.step));
-            hipLaunchKernelGGL(k_sur_grad, dim3(kTrainWaves / 4), dim3(256), 0, c.stream, n, b0, batch, a, b,
+            hipLaunchKernelGGL(k_sur_grad, dim3(kTrainWaves / kWaves), dim3(64 * kWaves), 0, c.stream, n, b0, batch, a, b,
                                S.params, S.trainX, S.trainT, S.part);
             hipLaunchKernelGGL(k_sur_adam, dim3(cdiv(NTRAIN + 1, 256)), dim3(256), 0, c.stream, S.part,
                                kTrainWaves, S.params, S.m1, S.m2, (float)lr, c1, c2, S.loss);

@@ -7,9 +7,10 @@ The surrogate has no counterpart in the reference: it is a learned
 replacement for the E3B three-body energy of FixE3B (src/Fixes/FixE3B.cu,
 src/Evaluators/ThreeBodyE3B.h).  Its definition, shared with the HIP kernels:
 
-* descriptors per molecule i (48): for 16 radial Gaussians
-  g_k(r) = exp(-ETA (r - MU_k)^2) fc(r), fc the E3B switching polynomial
-  between RS_C and RF_C,
+* descriptors per molecule i (48): for 16 radial functions
+  g_k(r) = exp(-KAPPA_k r) fc(r), KAPPA_k = k3 (0.5 + 0.1 k) (basis 5 is the
+  E3B hydrogen-bond function itself), fc the E3B switching polynomial between
+  RS_C = 5.0 and RF_C = 5.2,
     acc_k = sum_{j != i} sum_{H of j} g_k(|H - O_i|)   (hydrogens donated to O_i)
     d1_k  = sum_{j != i} g_k(|H1_i - O_j|),  d2_k likewise for H2_i,
   features F = [acc, d1 + d2, d1 * d2] (symmetric in H1 <-> H2);
@@ -29,9 +30,8 @@ import numpy as np
 NB = 16
 NF = 3 * NB
 NH = 64
-MU = 0.9 + 0.28 * np.arange(NB)
-ETA = 1.0 / (2.0 * 0.28 ** 2)
-RS_C, RF_C = 4.6, 5.2
+KAPPA = 1.907 * (0.5 + 0.1 * np.arange(NB))
+RS_C, RF_C = 5.0, 5.2
 NPARAM = NF * NH + NH + NH * NH + NH + NH + 1 + NF + NF + 2
 
 
@@ -82,9 +82,9 @@ def fc(r):
 def basis(r):
     """g (n, NB) and dg/dr (n, NB) at distances r (n,)"""
     s, ds = fc(r)
-    gauss = np.exp(-ETA * (r[:, None] - MU[None, :]) ** 2)
-    g = gauss * s[:, None]
-    dg = gauss * (ds[:, None] - 2 * ETA * (r[:, None] - MU[None, :]) * s[:, None])
+    ex = np.exp(-KAPPA[None, :] * r[:, None])
+    g = ex * s[:, None]
+    dg = ex * (ds[:, None] - KAPPA[None, :] * s[:, None])
     return g, dg
 
 

@@ -62,3 +62,16 @@ def test_molecule_energies_sum_to_total(orc):
     _, _, e3 = orc.e3b(box["mol"], box["x"], box["L"])
     assert abs(e3) > 0.1
     assert em.sum() == pytest.approx(e3, rel=1e-10)
+
+
+def test_basis_five_is_the_e3b_bond_function(orc):
+    P = orc.e3b_params()
+    r = np.linspace(0.8, 5.3, 200)
+    g, dg = S.basis(r)
+    s = np.where(r <= P.rs, 1.0, np.where(r > P.rf, 0.0, (P.rf - r) ** 2 * (P.rf + 2 * r - 3 * P.rs) / (P.rf - P.rs) ** 3))
+    assert np.allclose(g[:, 5], np.exp(-P.k3 * r) * s, rtol=1e-12, atol=0)
+    h = 1e-6
+    gp, _ = S.basis(r + h)
+    gm, _ = S.basis(r - h)
+    inner = (r > 0.9) & (np.abs(r - P.rs) > 1e-3) & (np.abs(r - P.rf) > 1e-3)
+    assert np.allclose(dg[inner], ((gp - gm) / (2 * h))[inner], rtol=1e-6, atol=1e-12)
