@@ -57,6 +57,8 @@ int mdx_positions_set(mdx_ctx *ctx, const double *x);
 int mdx_system_get(mdx_ctx *ctx, double *x, double *v);
 /* Orthorhombic box (BoundsGPU, src/BoundsGPU.h:91-154). */
 int mdx_box_set(mdx_ctx *ctx, const double lo[3], const double len[3]);
+/* Current box (it changes under a barostat). */
+int mdx_box_get(mdx_ctx *ctx, double lo[3], double len[3]);
 
 /* Neighbor list.  Replaces GridGPU (src/GridGPU.cu:84-113, 824-1103):
  * r_list = rCut + padding, rebuild when any atom moved more than skin/2
@@ -187,6 +189,18 @@ int mdx_e3b_molecule_energies(mdx_ctx *ctx, float *emol);
 int mdx_nose_hoover_set(mdx_ctx *ctx, int enable, double temperature, double time_constant, int chain_length,
                         double boltz, double mvv_to_eng);
 int mdx_nose_hoover_energy(mdx_ctx *ctx, double *chain_energy);
+/* Berendsen barostat (FixPressureBerendsen(state, handle, pressure, period,
+ * applyEvery), src/Fixes/FixPressureBerendsen.cu; bulkModulus 10 and
+ * maxDilation 1e-5 there): every `every` steps the molecular pressure sets an
+ * isotropic dilation of the box and of the molecules' centres of mass.
+ * Steps run eagerly while it is on.  mdx_pressure: instantaneous pressure of
+ * the current state with the force terms in `flags` (molecular virial, same
+ * units as nktv_to_press); mdx_barostat_last: the pressure of its last
+ * application. */
+int mdx_barostat_set(mdx_ctx *ctx, int enable, double pressure, double period, int every, double bulk_modulus,
+                     double max_dilation, double nktv_to_press, double mvv_to_eng);
+int mdx_pressure(mdx_ctx *ctx, int flags, double nktv_to_press, double mvv_to_eng, double *pressure);
+int mdx_barostat_last(mdx_ctx *ctx, double *pressure);
 /* Autotuner (Integrator::tune, src/Integrators/Integrator.cu:272-376, grid
  * search of Tunable knobs by timing): times `evals` runs of every list-build
  * kernel/team width, pair team width and E3B team width for the terms in

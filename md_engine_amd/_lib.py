@@ -27,7 +27,8 @@ SYMBOLS = [
     "mdx_profile_enable", "mdx_profile_get", "mdx_nlist_totals", "mdx_option_set", "mdx_rigid_set",
     "mdx_thermostat_set", "mdx_temperature", "mdx_tune", "mdx_run_batch", "mdx_surrogate_set",
     "mdx_surrogate_get", "mdx_surrogate_record", "mdx_surrogate_fit", "mdx_surrogate_eval",
-    "mdx_e3b_molecule_energies", "mdx_nose_hoover_set", "mdx_nose_hoover_energy",
+    "mdx_e3b_molecule_energies", "mdx_nose_hoover_set", "mdx_nose_hoover_energy", "mdx_barostat_set",
+    "mdx_pressure", "mdx_barostat_last", "mdx_box_get",
 ]
 
 # kernel ids of mdx_profile_get
@@ -107,6 +108,10 @@ def load():
     L.mdx_e3b_molecule_energies.argtypes = [_vp, _fp]
     L.mdx_nose_hoover_set.argtypes = [_vp, c, d, d, c, d, d]
     L.mdx_nose_hoover_energy.argtypes = [_vp, C.POINTER(d)]
+    L.mdx_barostat_set.argtypes = [_vp, c, d, d, c, d, d, d, d]
+    L.mdx_pressure.argtypes = [_vp, c, d, d, C.POINTER(d)]
+    L.mdx_barostat_last.argtypes = [_vp, C.POINTER(d)]
+    L.mdx_box_get.argtypes = [_vp, _dp, _dp]
     for name in SYMBOLS:
         fn = getattr(L, name)
         if name not in ("mdx_last_error", "mdx_version"):
@@ -175,6 +180,12 @@ class Engine:
     def set_box(self, lo, L):
         self._chk(self._L.mdx_box_set(self._h, np.asarray(lo, np.float64), np.asarray(L, np.float64)),
                   "mdx_box_set")
+
+    def get_box(self):
+        lo = np.zeros(3)
+        L = np.zeros(3)
+        self._chk(self._L.mdx_box_get(self._h, lo, L), "mdx_box_get")
+        return lo, L
 
     # neighbor list ------------------------------------------------------------
     def nlist_config(self, r_list, skin):
@@ -301,6 +312,23 @@ class Engine:
         e = C.c_double(0.0)
         self._chk(self._L.mdx_nose_hoover_energy(self._h, C.byref(e)), "mdx_nose_hoover_energy")
         return e.value
+
+    def barostat(self, enable, pressure=1.0, period=1000.0, every=10, bulk_modulus=10.0, max_dilation=1e-5,
+                 nktv_to_press=68568.415, mvv_to_eng=1.0):
+        self._chk(self._L.mdx_barostat_set(self._h, int(bool(enable)), float(pressure), float(period), int(every),
+                                           float(bulk_modulus), float(max_dilation), float(nktv_to_press),
+                                           float(mvv_to_eng)), "mdx_barostat_set")
+
+    def pressure(self, flags, nktv_to_press=68568.415, mvv_to_eng=1.0):
+        p = C.c_double(0.0)
+        self._chk(self._L.mdx_pressure(self._h, int(flags), float(nktv_to_press), float(mvv_to_eng), C.byref(p)),
+                  "mdx_pressure")
+        return p.value
+
+    def barostat_last(self):
+        p = C.c_double(0.0)
+        self._chk(self._L.mdx_barostat_last(self._h, C.byref(p)), "mdx_barostat_last")
+        return p.value
 
     # -- learned E3B three-body surrogate -----------------------------------
     SUR_NPARAM = 7459

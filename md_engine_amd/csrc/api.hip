@@ -106,6 +106,50 @@ double degrees_of_freedom(const Ctx &c) {
 
 constexpr int kForceTerms = MDX_PAIR | MDX_PPPM | MDX_E3B | MDX_SURROGATE;
 
+// Instantaneous pressure (nktv_to_press units) from the virial of the last
+// compute with MDX_VIRIAL: P = (sum M V^2 + W - corr) / 3V, W the atomic virial
+// trace, corr the molecular correction (rigid molecules move as centres of
+// mass; free sites count as their own molecules).  Synchronises.
+double pressure_now(Ctx &c) {
+    fill_int(c, (int *)(c.acc + ACC_MOL), 4, 0);
+    molecular_virial(c, ACC_MOL);
+    unsigned long long h[ACC_N];
+    MDX_HIP(hipMemcpyAsync(h, c.acc, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+    MDX_HIP(hipStreamSynchronize(c.stream));
+    auto v = [&](int k) { return (double)(long long)h[k] * kFxInv; };
+    const double W = v(ACC_VIR) + v(ACC_VIR + 1) + v(ACC_VIR + 2);
+    const double vol = c.geo.L[0] * c.geo.L[1] * c.geo.L[2];
+    return (v(ACC_MOL + 1) * c.baro.mvv + W - v(ACC_MOL)) / (3.0 * vol) * c.baro.nktv2p;
+}
+
+// FixPressureBerendsen::stepFinal (src/Fixes/FixPressureBerendsen.cu): mu =
+// (1 - dt/period (P0 - P)/B)^(1/3) clamped to 1 -+ maxDilation, applied to
+// the box about its centre and to the centres of mass; the cell grid is set
+// up again when its cell count would change, P3M refreshes alpha and G.
+void barostat_apply(Ctx &c) {
+    BarostatState &B = c.baro;
+    const double P = pressure_now(c);
+    B.lastP = P;
+    double mu = std::cbrt(1.0 - c.dt / B.period * (B.P0 - P) / B.bulk);
+    mu = std::min(std::max(mu, 1.0 - B.maxDil), 1.0 + B.maxDil);
+    dilate(c, mu);
+    double lo[3], len[3];
+    bool regrid = false;
+    for (int d = 0; d < 3; ++d) {
+        const double ctr = c.geo.lo[d] + 0.5 * c.geo.L[d];
+        len[d] = c.geo.L[d] * mu;
+        lo[d] = ctr - 0.5 * len[d];
+        const double edge = c.rlist / c.reach * (1.0 + 1e-9);
+        regrid = regrid || std::max(1, (int)std::floor(len[d] / edge)) != c.nc[d];
+        if (c.geo.periodic[d]) MDX_REQUIRE(c.rlist < 0.5 * len[d], 1, "box shrank below twice the list cutoff");
+    }
+    set_geometry(c, lo, len);
+    if (regrid) {
+        c.nlistConfigured = false;
+        c.listValid = false;
+    }
+}
+
 void compute(Ctx &c, int flags) {
     const bool energy = flags & MDX_ENERGY, virial = flags & MDX_VIRIAL;
     refresh_xq(c);
@@ -149,6 +193,7 @@ void snapshot_save(Ctx &c) {
     MDX_HIP(hipMemcpyAsync(c.snap.idToIdx, c.idToIdx, sizeof(int) * c.n, hipMemcpyDeviceToDevice, c.stream));
     c.snap.turn = c.turn;
     c.snap.rebuilds = c.nRebuilds;
+    c.snap.geo = c.geo;
 }
 
 void snapshot_restore(Ctx &c) {
@@ -162,6 +207,12 @@ void snapshot_restore(Ctx &c) {
     c.turn = c.snap.turn;
     c.nRebuilds = c.snap.rebuilds;
     c.listValid = false;
+    bool moved = false;
+    for (int d = 0; d < 3; ++d) moved = moved || c.snap.geo.L[d] != c.geo.L[d] || c.snap.geo.lo[d] != c.geo.lo[d];
+    if (moved) {  // a barostat changed the box during the replayed chunk
+        c.geo = c.snap.geo;
+        c.nlistConfigured = false;
+    }
 }
 
 // Reads the dMax slot written one step earlier (waits only for that step's
@@ -267,6 +318,7 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags) {
         Ctx &c = *cp;
         require_system(c);
         MDX_REQUIRE(c.rlist > 0, MDX_ERR_STATE, "mdx_nlist_config not called");
+        MDX_REQUIRE(!c.baro.on, MDX_ERR_STATE, "captured steps do not run a barostat (use mdx_run)");
         MDX_REQUIRE(c.device == cs[0]->device, MDX_ERR_ARG, "batched contexts must share one device");
         MDX_HIP(hipSetDevice(c.device));
         prepare_terms(c);
@@ -530,6 +582,16 @@ int mdx_box_set(mdx_ctx *ctx, const double lo[3], const double len[3]) {
         set_geometry(c, lo, len);
         c.listValid = false;
         c.nlistConfigured = false;
+    });
+}
+
+int mdx_box_get(mdx_ctx *ctx, double lo[3], double len[3]) {
+    return guarded(ctx, [&](Ctx &c) {
+        MDX_REQUIRE(lo && len, MDX_ERR_ARG, "box output required");
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = c.geo.lo[d];
+            len[d] = c.geo.L[d];
+        }
     });
 }
 
@@ -911,7 +973,7 @@ int mdx_run(mdx_ctx *ctx, int nsteps, int flags) {
         require_system(c);
         MDX_REQUIRE(c.rlist > 0, MDX_ERR_STATE, "mdx_nlist_config not called");
         MDX_REQUIRE(nsteps >= 0, MDX_ERR_ARG, "nsteps must be >= 0");
-        if (c.graphSteps) {
+        if (c.graphSteps && !c.baro.on) {  // the barostat needs the host every few steps
             run_captured({&c}, nsteps, flags);
             return;
         }
@@ -941,13 +1003,23 @@ int mdx_run(mdx_ctx *ctx, int nsteps, int flags) {
             check_and_copy(c, 1);
             for (int s = 0; s < len; ++s) {
                 const int par = s & 1;
+                const bool baro = c.baro.on && (c.turn + 1) % c.baro.every == 0;
                 integrate_first_half(c);
                 if (lagged_rebuild_decision(c, 1 - par)) nlist_rebuild_async(c);
                 check_and_copy(c, par);
-                compute(c, fflags);
+                compute(c, fflags | (baro ? MDX_VIRIAL : 0));
                 integrate_second_half(c);
                 ++c.turn;
                 if (c.thermoOn && c.turn % c.thermoEvery == 0) thermostat_apply(c);
+                if (baro) {
+                    barostat_apply(c);
+                    if (!c.listValid) {
+                        ensure_list(c);
+                        compute(c, fflags);
+                        fill_int(c, (int *)c.dMax, 4, 0);
+                        check_and_copy(c, par);
+                    }
+                }
             }
             MDX_HIP(hipStreamSynchronize(c.stream));
             lagged_rebuild_decision(c, (len - 1) & 1);  // count a violation in the last step
@@ -1344,6 +1416,50 @@ int mdx_nose_hoover_energy(mdx_ctx *ctx, double *chain_energy) {
             e += (k == 0 ? H.ndf : 1.0) * kT * st[3 * M + k];
         }
         *chain_energy = H.fresh ? 0.0 : e;
+    });
+}
+
+int mdx_barostat_set(mdx_ctx *ctx, int enable, double pressure, double period, int every, double bulk_modulus,
+                     double max_dilation, double nktv_to_press, double mvv_to_eng) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        MDX_REQUIRE(!enable || (period > 0 && every >= 1 && bulk_modulus > 0 && max_dilation > 0 &&
+                                max_dilation < 0.1 && nktv_to_press > 0 && mvv_to_eng > 0),
+                    MDX_ERR_ARG, "barostat needs period > 0, every >= 1, bulk modulus > 0, 0 < max dilation < 0.1");
+        BarostatState &B = c.baro;
+        B.on = enable != 0;
+        B.P0 = pressure;
+        B.period = period;
+        B.every = every;
+        B.bulk = bulk_modulus;
+        B.maxDil = max_dilation;
+        B.nktv2p = nktv_to_press;
+        B.mvv = mvv_to_eng;
+    });
+}
+
+int mdx_pressure(mdx_ctx *ctx, int flags, double nktv_to_press, double mvv_to_eng, double *pressure) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        MDX_REQUIRE(pressure, MDX_ERR_ARG, "no output");
+        prepare_terms(c);
+        ensure_list(c);
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        if (!capacity_ok(c)) nlist_rebuild(c, true);
+        compute(c, (flags & kForceTerms) | MDX_VIRIAL);
+        const double k = c.baro.nktv2p, m = c.baro.mvv;
+        c.baro.nktv2p = nktv_to_press;
+        c.baro.mvv = mvv_to_eng;
+        *pressure = pressure_now(c);
+        c.baro.nktv2p = k;
+        c.baro.mvv = m;
+    });
+}
+
+int mdx_barostat_last(mdx_ctx *ctx, double *pressure) {
+    return guarded(ctx, [&](Ctx &c) {
+        MDX_REQUIRE(pressure, MDX_ERR_ARG, "no output");
+        *pressure = c.baro.lastP;
     });
 }
 

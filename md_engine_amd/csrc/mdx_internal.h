@@ -36,6 +36,7 @@ enum AccSlot {
     ACC_E3B2 = 2,     // E3B two-body energy
     ACC_E3B3 = 3,     // E3B three-body energy
     ACC_VIR = 4,      // 6 slots: xx yy zz xy xz yz
+    ACC_MOL = 10,     // 2 slots: molecular virial correction, sum M V^2 of centres of mass
     ACC_N = 16
 };
 
@@ -92,6 +93,14 @@ struct NoseHooverState {
     int chain = 10;
     double T = 0.0, freq = 0.0, boltz = 0.0, mvv = 0.0, ndf = 0.0;
     double *dev = nullptr;
+};
+
+// Berendsen barostat (FixPressureBerendsen)
+struct BarostatState {
+    bool on = false;
+    double P0 = 1.0, period = 1000.0, bulk = 10.0, maxDil = 1e-5, nktv2p = 68568.415, mvv = 1.0;
+    int every = 10;
+    double lastP = 0.0;
 };
 
 // learned E3B three-body surrogate (surrogate.hip)
@@ -209,6 +218,7 @@ struct Ctx {
     E3BState e3b;
     SurrogateState sur;
     NoseHooverState nh;
+    BarostatState baro;
     RigidState rigid;
     bool mdxDisableTiledSpread = false;   // test knob: force the plain global-atomic spread
 
@@ -231,6 +241,7 @@ struct Ctx {
         float *q = nullptr;
         int *type = nullptr, *id = nullptr, *idToIdx = nullptr;
         long long turn = 0, rebuilds = 0;
+        Geometry geo;
     } snap;
 
     // profiling: event pairs recorded on the context stream around kernels
@@ -350,6 +361,8 @@ void fill_float(Ctx &c, float *p, int count, float v);
 
 // rigid.hip
 void rigid_map(Ctx &c);
+void molecular_virial(Ctx &c, int slot);  // acc[slot] = sum F.(r - R_com), acc[slot+1] = sum M V^2
+void dilate(Ctx &c, double mu);           // isotropic scaling about the box centre (molecules rigid)
 void rigid_first_half(Ctx &c);
 void rigid_second_half(Ctx &c, bool kick);
 

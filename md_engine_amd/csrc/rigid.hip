@@ -235,7 +235,110 @@ RigidArgs make_args(const Ctx &c) {
     return r;
 }
 
+// Molecular virial correction and centre-of-mass kinetic term of each rigid
+// molecule: corr = sum_s F_s . (r_s - R) over its four sites (R the centre of
+// mass, hydrogens and M taken in O's image), mv2 = M |V|^2.  The pressure of
+// rigid molecules is (M|V|^2 summed + W_atomic - corr) / 3V: the atomic virial
+// minus sum_i F_i . (r_i - R_mol(i)) is the virial of the centre-of-mass
+// separations, with no constraint forces needed.
+__global__ void __launch_bounds__(256) k_mol_virial(RigidArgs r, const double4 *__restrict__ pos,
+                                                    const double4 *__restrict__ vel,
+                                                    const unsigned long long *__restrict__ f,
+                                                    unsigned long long *__restrict__ acc, int slot) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    double corr = 0.0, mv2 = 0.0;
+    if (m < r.nmol) {
+        const int4 a = r.atoms[m];
+        const double mT = r.mO + 2.0 * r.mH;
+        const d3 o = xyz(pos[a.x]);
+        const d3 h1 = o + mi(xyz(pos[a.y]) - o, r.g);
+        const d3 h2 = o + mi(xyz(pos[a.z]) - o, r.g);
+        const d3 R = (1.0 / mT) * (r.mO * o + r.mH * (h1 + h2));
+        corr = dot(force_of(f, a.x), o - R) + dot(force_of(f, a.y), h1 - R) + dot(force_of(f, a.z), h2 - R);
+        if (a.w >= 0) corr += dot(force_of(f, a.w), o + mi(xyz(pos[a.w]) - o, r.g) - R);
+        const d3 V = (1.0 / mT) * (r.mO * xyz(vel[a.x]) + r.mH * (xyz(vel[a.y]) + xyz(vel[a.z])));
+        mv2 = mT * dot(V, V);
+    }
+    corr = team_sum_d<64>(corr);
+    mv2 = team_sum_d<64>(mv2);
+    if ((threadIdx.x & 63) == 0) {
+        fx_add(&acc[slot], corr);
+        fx_add(&acc[slot + 1], mv2);
+    }
+}
+
+// sum m v^2 of massive sites outside rigid molecules -> acc[slot + 1]
+__global__ void __launch_bounds__(256) k_free_mv2(int n, const double4 *__restrict__ vel,
+                                                  const int *__restrict__ member,
+                                                  unsigned long long *__restrict__ acc, int slot) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    if (i < n && !(member && member[i])) {
+        const double4 v = vel[i];
+        if (v.w > 0.0) s = (v.x * v.x + v.y * v.y + v.z * v.z) / v.w;
+    }
+    s = team_sum_d<64>(s);
+    if ((threadIdx.x & 63) == 0 && s != 0.0) fx_add(&acc[slot + 1], s);
+}
+
+// isotropic dilation about the box centre c: rigid molecules move with their
+// centre of mass (geometry kept), other sites are scaled (Mod::scaleSystem,
+// src/Mod.cu:150-240)
+__global__ void __launch_bounds__(256) k_mol_scale(RigidArgs r, double4 *__restrict__ pos, double mu, d3 c) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= r.nmol) return;
+    const int4 a = r.atoms[m];
+    const double mT = r.mO + 2.0 * r.mH;
+    const d3 o = xyz(pos[a.x]);
+    const d3 h1 = o + mi(xyz(pos[a.y]) - o, r.g);
+    const d3 h2 = o + mi(xyz(pos[a.z]) - o, r.g);
+    const d3 R = (1.0 / mT) * (r.mO * o + r.mH * (h1 + h2));
+    const d3 shift = (c + mu * (R - c)) - R;
+    const int site[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        if (site[s] < 0) continue;
+        double4 p = pos[site[s]];
+        p.x += shift.x;
+        p.y += shift.y;
+        p.z += shift.z;
+        pos[site[s]] = p;
+    }
+}
+
+__global__ void k_free_scale(int n, double4 *__restrict__ pos, const int *__restrict__ member, double mu, d3 c) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || (member && member[i])) return;
+    double4 p = pos[i];
+    p.x = c.x + mu * (p.x - c.x);
+    p.y = c.y + mu * (p.y - c.y);
+    p.z = c.z + mu * (p.z - c.z);
+    pos[i] = p;
+}
+
 }  // namespace
+
+void molecular_virial(Ctx &c, int slot) {
+    const int *member = c.rigid.on ? c.rigid.member : nullptr;
+    if (c.rigid.on) {
+        RigidArgs r = make_args(c);
+        hipLaunchKernelGGL(k_mol_virial, dim3(cdiv(r.nmol, 256)), dim3(256), 0, c.stream, r, c.pos[c.cur],
+                           c.vel[c.cur], c.force, c.acc, slot);
+    }
+    hipLaunchKernelGGL(k_free_mv2, dim3(cdiv(c.n, 256)), dim3(256), 0, c.stream, c.n, c.vel[c.cur], member, c.acc,
+                       slot);
+}
+
+void dilate(Ctx &c, double mu) {
+    d3 ctr = {c.geo.lo[0] + 0.5 * c.geo.L[0], c.geo.lo[1] + 0.5 * c.geo.L[1], c.geo.lo[2] + 0.5 * c.geo.L[2]};
+    const int *member = c.rigid.on ? c.rigid.member : nullptr;
+    if (c.rigid.on) {
+        RigidArgs r = make_args(c);
+        hipLaunchKernelGGL(k_mol_scale, dim3(cdiv(r.nmol, 256)), dim3(256), 0, c.stream, r, c.pos[c.cur], mu, ctr);
+    }
+    hipLaunchKernelGGL(k_free_scale, dim3(cdiv(c.n, 256)), dim3(256), 0, c.stream, c.n, c.pos[c.cur], member, mu,
+                       ctr);
+}
 
 void rigid_map(Ctx &c) {
     if (!c.rigid.on) return;

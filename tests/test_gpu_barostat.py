@@ -1,0 +1,101 @@
+"""Molecular pressure and the Berendsen barostat on the GPU.
+
+The virial pressure of rigid molecules (atomic virial minus sum F.(r - R_com))
+must equal -dU/dV under centre-of-mass dilation, which is checked by finite
+differences of the engine's own energy at fixed Ewald splitting; a barostat
+run must keep the constraints and an oracle-exact neighbor list."""
+import sys
+
+import numpy as np
+import pytest
+
+from md_engine_amd import systems
+
+from test_gpu_water_md import G, min_image_diff, start_velocities, water_md_engine
+
+pytestmark = pytest.mark.gpu
+NKTV2P = 68568.415
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def com_dilate(box, x, mu):
+    """molecules translated with their centres of mass scaled about the box centre"""
+    L, lo = box["L"], box["lo"]
+    c = lo + 0.5 * L
+    out = x.copy()
+    m = np.array([G["mO"], G["mH"], G["mH"]])
+    for o, h1, h2, ms in box["mol"]:
+        rel = min_image_diff(x[[h1, h2, ms]], x[o], L)
+        R = (m[0] * x[o] + m[1] * (x[o] + rel[0]) + m[2] * (x[o] + rel[1])) / m.sum()
+        shift = c + mu * (R - c) - R
+        out[[o, h1, h2, ms]] += shift
+    return out, lo + 0.5 * L * (1 - mu), L * mu
+
+
+@pytest.mark.parametrize("terms", ["pair_e3b", "all"])
+def test_pressure_is_volume_derivative(orc, gpu_engine_factory, terms):
+    say(f"virial pressure vs -dU/dV ({terms})")
+    from md_engine_amd import E3B, ENERGY, PAIR, PPPM
+
+    box = systems.tip4p2005_box(216, seed=33, min_sep=1.8)
+    e, _ = water_md_engine(gpu_engine_factory, box, np.zeros_like(box["x"]), rc=7.0, skin=1.0, sz=(32, 32, 32),
+                           order=5, e3b=True, dt=1.0)
+    alpha, _ = e.pppm_info()
+    e.pppm_set((32, 32, 32), 5, 7.0, systems.REAL_QQR_TO_ENG, alpha=alpha)  # fixed splitting
+    flags = PAIR | E3B | (PPPM if terms == "all" else 0)
+    p_vir = e.pressure(flags, NKTV2P, systems.REAL_MVV_TO_ENG)
+    x0 = box["x"]
+    us = []
+    h = 2e-5
+    for mu in (1 + h, 1 - h):
+        x, lo, L = com_dilate(box, x0, mu)
+        e.set_box(lo, L)
+        e.set_positions(x)
+        e.compute(flags | ENERGY)
+        en, _ = e.energies()
+        us.append(en.sum())
+    V0 = np.prod(box["L"])
+    dV = V0 * ((1 + h) ** 3 - (1 - h) ** 3)
+    p_fd = -(us[0] - us[1]) / dV * NKTV2P
+    say(f"P virial {p_vir:.2f} atm, -dU/dV {p_fd:.2f} atm")
+    tol = 2.0 if terms == "pair_e3b" else 0.02 * abs(p_fd) + 20.0
+    assert p_vir == pytest.approx(p_fd, abs=tol)
+
+
+def test_barostat_run(orc, gpu_engine_factory):
+    say("Berendsen barostat 400 steps")
+    from md_engine_amd import ENERGY  # noqa: F401
+
+    box = systems.tip4p2005_box(216, seed=34, min_sep=1.8)
+    v = start_velocities(orc, box, seed=8)
+    e, flags = water_md_engine(gpu_engine_factory, box, v, rc=7.0, skin=1.0, sz=(32, 32, 32), order=5,
+                               e3b=True, dt=1.0)
+    kb, mvv = systems.REAL_BOLTZ, systems.REAL_MVV_TO_ENG
+    e.nose_hoover(True, 298.0, 100.0, 10, kb, mvv)
+    e.barostat(True, 1.0, 200.0, 10, 10.0, 1e-4, NKTV2P, mvv)
+    e.option("graph", 1)  # falls back to eager steps while the barostat is on
+    L0 = box["L"].copy()
+    e.run(400, flags)
+    p = e.barostat_last()
+    x, vv = e.get_system()
+    lo, L = e.get_box()
+    say(f"last pressure {p:.1f} atm, box {L0[0]:.4f} -> {L[0]:.4f}")
+    assert np.isfinite(p)
+    assert L[0] != L0[0] and abs(L[0] / L0[0] - 1) <= 40 * 1e-4 + 1e-12
+    assert np.allclose(L, L[0]) and np.allclose(lo + 0.5 * L, box["lo"] + 0.5 * L0)
+    for o, h1, h2, m in box["mol"]:
+        d = min_image_diff(x[[h1, h2]], x[o], L)
+        assert np.allclose(np.linalg.norm(d, axis=1), G["dOH"], atol=1e-8)
+        for i, j in ((o, h1), (o, h2), (h1, h2)):
+            r = min_image_diff(x[i], x[j], L)
+            assert abs(np.dot(vv[i] - vv[j], r)) < 1e-9
+    from test_gpu_parity import assert_same_lists, oracle_rows_by_id
+    e.nlist_update(force=True)
+    gc, grows = e.nlist_get()
+    excl = orc.exclusions_bfs(len(x), systems.water_bonds(box["mol"]))
+    oc, orows, _ = oracle_rows_by_id(orc, x, lo, L, 8.0, excl=excl, brute=False)
+    assert_same_lists(gc, grows, oc, orows)
