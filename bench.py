@@ -288,7 +288,6 @@ def main():
     ap.add_argument("--pair-team", type=int, default=0)
     ap.add_argument("--build-team", type=int, default=0)
     ap.add_argument("--e3b-team", type=int, default=0)
-    ap.add_argument("--build-mode", type=int, default=0, help="0 default (per atom), 1 per-cell LDS, 2 per-atom")
     ap.add_argument("--tune", action="store_true", help="run the autotuner before timing")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of known size (counter calibration)")
@@ -309,7 +308,6 @@ def main():
     w = make_workload(args.config)
     e, flags = setup_engine(w)
     e.teams_set(args.build_team, args.pair_team, args.e3b_team)
-    e.option("build_mode", args.build_mode)
     n = len(w["box"]["x"])
     R = args.rebuild_every
     surrogate = args.three_body == "surrogate"

@@ -104,6 +104,13 @@ double degrees_of_freedom(const Ctx &c) {
     return ndf > 0 ? ndf : 1.0;
 }
 
+// E3B oxygens without charge are found in the uncharged part of the rows
+void update_ox_flag(Ctx &c) {
+    bool u = true;
+    for (int o : c.e3b.oxIds) u = u && c.qHost[o] == 0.0f;
+    c.e3b.oxUncharged = u;
+}
+
 constexpr int kForceTerms = MDX_PAIR | MDX_PPPM | MDX_E3B | MDX_SURROGATE;
 
 // Instantaneous pressure (nktv_to_press units) from the virial of the last
@@ -496,6 +503,9 @@ int mdx_system_set(mdx_ctx *ctx, int n, const double *x, const double *v, const 
         std::vector<float> hq(n);
         std::vector<int> ht(n), hid(n);
         c.invMassHost.assign(n, 1.0);
+        c.qHost.assign(n, 0.0f);
+        c.typeHost.assign(n, 0);
+        c.pairFlagsDirty = true;
         c.hasCharges = false;
         c.maxType = 0;
         for (int i = 0; i < n; ++i) {
@@ -504,8 +514,10 @@ int mdx_system_set(mdx_ctx *ctx, int n, const double *x, const double *v, const 
             c.invMassHost[i] = im;
             hv[i] = v ? make_double4(v[3 * i], v[3 * i + 1], v[3 * i + 2], im) : make_double4(0, 0, 0, im);
             hq[i] = q ? (float)q[i] : 0.0f;
+            c.qHost[i] = hq[i];
             if (hq[i] != 0.0f) c.hasCharges = true;
             ht[i] = type ? type[i] : 0;
+            c.typeHost[i] = ht[i];
             MDX_REQUIRE(ht[i] >= 0 && ht[i] < kMaxTypes, MDX_ERR_ARG, "atom type out of range [0, 16)");
             c.maxType = std::max(c.maxType, ht[i]);
             hid[i] = i;
@@ -522,6 +534,7 @@ int mdx_system_set(mdx_ctx *ctx, int n, const double *x, const double *v, const 
         c.listValid = false;
         c.nlistConfigured = false;
         c.nRebuilds = 0;
+        update_ox_flag(c);
         if (c.rigid.on) {
             rigid_map(c);
             c.rigid.fresh = true;
@@ -602,8 +615,10 @@ int mdx_nlist_config(mdx_ctx *ctx, double r_list, double skin) {
         if (r_list != c.rlist && c.nl) {
             dfree(c, c.nl);
             dfree(c, c.nlCount);
+            dfree(c, c.nlCountA);
             c.nl = nullptr;
             c.nlCount = nullptr;
+            c.nlCountA = nullptr;
             c.cap = 0;
         }
         c.rlist = r_list;
@@ -704,9 +719,10 @@ int mdx_nlist_get(mdx_ctx *ctx, int32_t *counts, uint32_t *tagged_ids, int row_c
         require_system(c);
         MDX_REQUIRE(c.listValid, MDX_ERR_STATE, "no neighbor list built");
         const int n = c.n;
-        std::vector<int> cnt(n), ids(n);
+        std::vector<int> cnt(n), cntA(n), ids(n);
         std::vector<uint32_t> nl((size_t)n * c.cap);
         MDX_HIP(hipMemcpyAsync(cnt.data(), c.nlCount, sizeof(int) * n, hipMemcpyDeviceToHost, c.stream));
+        MDX_HIP(hipMemcpyAsync(cntA.data(), c.nlCountA, sizeof(int) * n, hipMemcpyDeviceToHost, c.stream));
         MDX_HIP(hipMemcpyAsync(ids.data(), c.id[c.cur], sizeof(int) * n, hipMemcpyDeviceToHost, c.stream));
         MDX_HIP(hipMemcpyAsync(nl.data(), c.nl, sizeof(uint32_t) * nl.size(), hipMemcpyDeviceToHost, c.stream));
         MDX_HIP(hipStreamSynchronize(c.stream));
@@ -715,10 +731,13 @@ int mdx_nlist_get(mdx_ctx *ctx, int32_t *counts, uint32_t *tagged_ids, int row_c
         for (int k = 0; k < n; ++k) {
             const int myid = ids[k];
             const int cn = std::min(cnt[k], c.cap);
+            const int na = std::min(cntA[k], cn), nb = cn - na;
             mx = std::max(mx, cnt[k]);
             row.resize(cn);
+            // charged neighbors lead the row, uncharged ones end it
             for (int s = 0; s < cn; ++s) {
-                uint32_t e = nl[(size_t)k * c.cap + s];
+                const size_t slot = (size_t)k * c.cap + (s < na ? s : c.cap - nb + (s - na));
+                const uint32_t e = nl[slot];
                 row[s] = (uint32_t)ids[e & 0x3FFFFFFFu] | (e & 0xC0000000u);
             }
             std::sort(row.begin(), row.end(),
@@ -758,6 +777,8 @@ int mdx_lj_set(mdx_ctx *ctx, int ntypes, const double *rc2, const double *eps24,
         MDX_HIP(hipStreamSynchronize(c.stream));
         c.ntypes = ntypes;
         c.ljOn = true;
+        c.ljEps.assign(eps24, eps24 + T2);
+        c.pairFlagsDirty = true;
     });
 }
 
@@ -864,6 +885,9 @@ int mdx_e3b_set(mdx_ctx *ctx, int nmol, const int32_t *mol_ids, const double par
         upload(c, E.molIds, ids.data(), nmol);
         fill_int(c, E.atomMolO, c.n, -1);
         MDX_HIP(hipStreamSynchronize(c.stream));
+        E.oxIds.resize(nmol);
+        for (int m = 0; m < nmol; ++m) E.oxIds[m] = ids[m].x;
+        update_ox_flag(c);
         c.listValid = false;
     });
 }
@@ -1118,7 +1142,7 @@ int mdx_tune(mdx_ctx *ctx, int flags, int evals, int32_t *chosen, double *avg_ms
         // list build: (mode, team) candidates; rows regrow if a candidate overflows
         int bestMode = 2, bestTeam = c.buildTeam;
         double bestBuild = 1e30;
-        const int modes[5][2] = {{2, 8}, {2, 16}, {2, 32}, {2, 64}, {1, 0}};
+        const int modes[4][2] = {{2, 8}, {2, 16}, {2, 32}, {2, 64}};
         for (auto &mt : modes) {
             c.buildMode = mt[0];
             if (mt[1]) c.buildTeam = mt[1];
@@ -1355,7 +1379,7 @@ int mdx_option_set(mdx_ctx *ctx, const char *name, int value) {
             c.graphSteps = value;
         }
         else if (k == "build_mode") {
-            MDX_REQUIRE(value >= 0 && value <= 2, MDX_ERR_ARG, "build_mode: 0 default (per atom), 1 per cell, 2 per atom");
+            MDX_REQUIRE(value == 0 || value == 2, MDX_ERR_ARG, "build_mode: 0 or 2 (per-atom teams)");
             c.buildMode = value;
         }
         else throw Error{MDX_ERR_ARG, "unknown option " + k};

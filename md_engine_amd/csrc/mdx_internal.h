@@ -84,6 +84,8 @@ struct E3BState {
     int *nlCount = nullptr;
     float4 *tot = nullptr;    // per molecule totals (x accepted, y H1 donated, z H2 donated)
     int team = 8;
+    bool oxUncharged = false;  // every E3B oxygen site has q = 0 (its neighbors need no charged scan)
+    std::vector<int> oxIds;    // by molecule: oxygen atom id
 };
 
 // Nose-Hoover chain thermostat state, kept on the device so a captured step
@@ -159,7 +161,12 @@ struct Ctx {
     unsigned long long *force = nullptr;   // 3 per atom
     unsigned long long *acc = nullptr;     // ACC_N
     double4 *posLast = nullptr;
-    std::vector<double> invMassHost;       // by id
+    std::vector<double> invMassHost;
+    std::vector<float> qHost;             // charges by id, as last set
+    std::vector<int> typeHost;            // types by id
+    std::vector<double> ljEps;            // 24 eps per type pair (host copy)
+    bool pairFlagsDirty = true;
+    int *pairTypeFlags = nullptr;         // per type: bit 0 LJ with a charged type, bit 1 with an uncharged one       // by id
     bool hasCharges = false;
     int maxType = 0;
 
@@ -179,6 +186,7 @@ struct Ctx {
     int2 *colXY = nullptr;        // by column rank: cell (x, y)
     int cap = 0;
     uint32_t *nl = nullptr;
+    int *nlCountA = nullptr;      // leading (charged-neighbor) part of each row; the rest fills from the row end
     int *nlCount = nullptr;
     // [0] rebuild flag, [1] max count, [2] e3b max count; captured steps:
     // [3] rebuild gate, [4] rebuilds, [5] thermostat gate, [6] turn

@@ -145,15 +145,17 @@ __global__ void k_place(const int *__restrict__ gate, int n, const int *__restri
 }
 
 // Make the order inside each cell independent of atomic slot order: one
-// workgroup per cell ranks its entries by atom id in LDS (ids are unique, so
-// rank = number of smaller ids) and writes them in rank order.
-__global__ void __launch_bounds__(64) k_cellsort(const int *__restrict__ gate, int ncell, const int *__restrict__ start,
-                                                 const int *__restrict__ permIn,
-                                                 int *__restrict__ permOut,
-                                                 const int *__restrict__ id) {
+// workgroup per cell ranks its entries by (class, atom id) in LDS (keys are
+// unique, so rank = number of smaller keys) and writes them in rank order.
+// Class 0 = charged, 1 = uncharged, so atoms that need the same pair work sit
+// together and teams of one wave mostly share their code path.
+__global__ void __launch_bounds__(64) k_cellsort(const int *__restrict__ gate, int ncell,
+                                                 const int *__restrict__ start, const int *__restrict__ permIn,
+                                                 int *__restrict__ permOut, const int *__restrict__ id,
+                                                 const float *__restrict__ q) {
     if (gate && *gate == 0) return;
     constexpr int CAP = 1024;
-    __shared__ int sid[CAP];
+    __shared__ long long skey[CAP];
     __shared__ int sperm[CAP];
     const int c = blockIdx.x;
     const int a = start[c], b = start[c + 1], m = b - a;
@@ -161,21 +163,23 @@ __global__ void __launch_bounds__(64) k_cellsort(const int *__restrict__ gate, i
         for (int t = threadIdx.x; t < m; t += 64) {
             const int p = permIn[a + t];
             sperm[t] = p;
-            sid[t] = id[p];
+            skey[t] = ((long long)(q[p] == 0.0f) << 32) | (long long)id[p];
         }
         __syncthreads();
         for (int t = threadIdx.x; t < m; t += 64) {
-            const int key = sid[t];
+            const long long key = skey[t];
             int r = 0;
-            for (int j = 0; j < m; ++j) r += sid[j] < key;
+            for (int j = 0; j < m; ++j) r += skey[j] < key;
             permOut[a + r] = sperm[t];
         }
     } else if (threadIdx.x == 0) {
         // very crowded cell: serial insertion sort
         for (int s = a; s < b; ++s) permOut[s] = permIn[s];
         for (int s = a + 1; s < b; ++s) {
-            int v = permOut[s], key = id[v], t = s - 1;
-            while (t >= a && id[permOut[t]] > key) {
+            const int v = permOut[s];
+            const long long key = ((long long)(q[v] == 0.0f) << 32) | (long long)id[v];
+            int t = s - 1;
+            while (t >= a && (((long long)(q[permOut[t]] == 0.0f) << 32) | (long long)id[permOut[t]]) > key) {
                 permOut[t + 1] = permOut[t];
                 --t;
             }
@@ -418,7 +422,7 @@ __global__ void __launch_bounds__(256) k_build(const int *__restrict__ gate,
     int n, const double4 *__restrict__ pos, const float4 *__restrict__ xq, Geometry g, CellGrid cg,
     Prefilter pf, Cull cu, const int *__restrict__ colRank, const int *__restrict__ cellStart,
     double rlist2, const uint32_t *__restrict__ exclIdx, int maxExcl, int cap,
-    uint32_t *__restrict__ nl, int *__restrict__ nlCount, int *__restrict__ flags) {
+    uint32_t *__restrict__ nl, int *__restrict__ nlCount, int *__restrict__ nlCountA, int *__restrict__ flags) {
     if (gate && *gate == 0) return;
     constexpr int U = 64 / T;  // chunks of T candidates per load block
     __shared__ int4 sRuns[256 / T][kMaxCols];
@@ -476,7 +480,7 @@ __global__ void __launch_bounds__(256) k_build(const int *__restrict__ gate,
         }
     }
     __syncthreads();
-    int cnt = 0;
+    int cnt = 0, cntA = 0, cntB = 0;
     if (live) {
         const unsigned long long teamMask =
             (T == 64) ? ~0ull : (((1ull << T) - 1ull) << (lane & ~(T - 1)));
@@ -537,16 +541,30 @@ __global__ void __launch_bounds__(256) k_build(const int *__restrict__ gate,
                             if ((e & 0x3FFFFFFFu) == uj) tag = e >> 30;
                         }
                 }
-                const unsigned long long m = __ballot(hit);
+                // charged neighbours fill the row from the front, uncharged
+                // ones from the back (a pair kernel reads only what it needs)
+                const bool charged = cur.w != 0.0f;
+                const unsigned long long mA = __ballot(hit && charged), mB = __ballot(hit && !charged);
                 if (hit) {
-                    const int k = cnt + __popcll(m & pre);
-                    if (k < cap) row[k] = (uint32_t)j | (tag << 30);
+                    const uint32_t e = (uint32_t)j | (tag << 30);
+                    if (charged) {
+                        const int k = cntA + __popcll(mA & pre);
+                        if (k < cap) row[k] = e;
+                    } else {
+                        const int k = cap - 1 - (cntB + __popcll(mB & pre));
+                        if (k >= 0) row[k] = e;
+                    }
                 }
-                cnt += __popcll(m & teamMask);
+                cntA += __popcll(mA & teamMask);
+                cntB += __popcll(mB & teamMask);
                 }
             }
         }
-        if (tl == 0) nlCount[i] = cnt;
+        cnt = cntA + cntB;
+        if (tl == 0) {
+            nlCount[i] = cnt;
+            nlCountA[i] = cntA;
+        }
     }
     // one atomic per wave: same-address atomics serialise in L2
     const int wmax = wave_max(cnt);
@@ -573,195 +591,16 @@ __global__ void k_excl_idx(const int *__restrict__ gate, int n, const int *__res
     }
 }
 
-// LDS-staged build: one workgroup per cell.  Every atom of a cell shares the
-// stencil, so the (2 reach + 1)^2 columns x (2 reach + 1) z-slots of
-// candidates are staged in LDS once per cell, in (column, unwrapped z-slot)
-// order, each as its FP32 position with the candidate index in .w.  A wave then
-// takes one home atom at a time: lane l culls column l against the atom (as
-// in k_build) and turns the surviving z-slots into one LDS range; the wave
-// walks the ranges 64 candidates at a time, all operands wave-uniform except
-// the candidate.  A neighbourhood larger than the stage is processed in
-// windows of kStage candidates, a crowded home cell in batches of 256 atoms.
-// Membership uses the FP32 pre-test and the canonical FP64 distance in the
-// band, as k_build, so the rows hold the same sets (in another order).
-constexpr int kStage = 3072;
-constexpr int kSlots = 5;  // 2 reach + 1
-
-__global__ void __launch_bounds__(256) k_build_cell(const int *__restrict__ gate, 
-    const double4 *__restrict__ pos, const float4 *__restrict__ xq, Geometry g, CellGrid cg,
-    Prefilter pf, Cull cu, const int2 *__restrict__ colXY, const int *__restrict__ colRank,
-    const int *__restrict__ cellStart, double rlist2, const uint32_t *__restrict__ exclIdx, int maxExcl,
-    int cap, uint32_t *__restrict__ nl, int *__restrict__ nlCount, int *__restrict__ flags) {
-    if (gate && *gate == 0) return;
-    __shared__ float4 sXq[kStage];
-    __shared__ int sZoff[kMaxCols][kSlots + 1];  // virtual offset of each z-slot, and the column end
-    __shared__ int sGz[kMaxCols][kSlots];        // first candidate index of each z-slot
-    __shared__ int sCnt[256];
-    const int c = blockIdx.x;
-    const int ncz = cg.nc[2];
-    const int cellA0 = cellStart[c], cellA1 = cellStart[c + 1];
-    if (cellA1 == cellA0) return;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const unsigned long long below = (1ull << lane) - 1ull;
-    const int2 cxy = colXY[c / ncz];
-    const int cz = c % ncz;
-    const int sx = axis_slots(cg.nc[0], cg.reach), sy = axis_slots(cg.nc[1], cg.reach);
-    const int sz = axis_slots(ncz, cg.reach);
-    const int ncol = sx * sy;
-
-    // slot table: counts and first indices (thread = one (column, z-slot))
-    for (int t = threadIdx.x; t < kMaxCols * kSlots; t += blockDim.x) {
-        const int col = t / kSlots, s = t - col * kSlots;
-        int cnt = 0, first = 0;
-        if (col < ncol && s < sz) {
-            const int a = col / sy, b = col - a * sy;
-            const int xx = axis_cell(a, cxy.x, cg.nc[0], cg.reach, g.periodic[0]);
-            const int yy = axis_cell(b, cxy.y, cg.nc[1], cg.reach, g.periodic[1]);
-            const int zz = axis_cell(s, cz, ncz, cg.reach, g.periodic[2]);
-            if (xx >= 0 && yy >= 0 && zz >= 0) {
-                const int cell = colRank[xx * cg.nc[1] + yy] * ncz + zz;
-                first = cellStart[cell];
-                cnt = cellStart[cell + 1] - first;
-            }
-        }
-        sGz[col][s] = first;
-        sZoff[col][s + 1] = cnt;  // counts for now, scanned below
-    }
-    __syncthreads();
-    // exclusive scan of the kMaxCols * kSlots counts by wave 0 (two per lane)
-    if (w == 0) {
-        const int t0 = 2 * lane, t1 = 2 * lane + 1;
-        const int n0 = t0 < kMaxCols * kSlots ? sZoff[t0 / kSlots][t0 % kSlots + 1] : 0;
-        const int n1 = t1 < kMaxCols * kSlots ? sZoff[t1 / kSlots][t1 % kSlots + 1] : 0;
-        int incl = n0 + n1;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int v = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += v;
-        }
-        const int excl = incl - n0 - n1;
-        // every slot gets its start; a column's end is the next column's start
-        if (t0 < kMaxCols * kSlots) sZoff[t0 / kSlots][t0 % kSlots] = excl;
-        if (t1 < kMaxCols * kSlots) sZoff[t1 / kSlots][t1 % kSlots] = excl + n0;
-        if (lane == 63) sZoff[kMaxCols - 1][kSlots] = incl;
-    }
-    __syncthreads();
-    if (threadIdx.x < kMaxCols - 1) sZoff[threadIdx.x][kSlots] = sZoff[threadIdx.x + 1][0];
-    __syncthreads();
-    const int total = sZoff[kMaxCols - 1][kSlots];
-
-    int wmax = 0;
-    for (int a0 = cellA0; a0 < cellA1; a0 += 256) {
-        const int nA = min(cellA1 - a0, 256);
-        for (int k = threadIdx.x; k < nA; k += blockDim.x) sCnt[k] = 0;
-        for (int w0 = 0; w0 < total; w0 += kStage) {
-            const int w1 = min(total, w0 + kStage);
-            __syncthreads();  // previous window fully consumed
-            // stage [w0, w1): wave w fills columns w, w + 4, ...
-            for (int col = w; col < ncol; col += 4) {
-                const int v0 = max(sZoff[col][0], w0), v1 = min(sZoff[col][kSlots], w1);
-                for (int v = v0 + lane; v < v1; v += 64) {
-                    int s = 0;
-#pragma unroll
-                    for (int q = 1; q < kSlots; ++q) s += v >= sZoff[col][q] ? 1 : 0;
-                    const int j = sGz[col][s] + (v - sZoff[col][s]);
-                    float4 p = xq[j];
-                    p.w = __int_as_float(j);
-                    sXq[v - w0] = p;
-                }
-            }
-            __syncthreads();
-            for (int k = w; k < nA; k += 4) {
-                const int i = a0 + k;
-                const float4 xi = xq[i];
-                const double4 pi = pos[i];
-                const uint32_t *exRow = exclIdx + (size_t)i * maxExcl;
-                const uint32_t ex0 = maxExcl > 0 ? exRow[0] : ~0u, ex1 = maxExcl > 1 ? exRow[1] : ~0u;
-                const uint32_t ex2 = maxExcl > 2 ? exRow[2] : ~0u, ex3 = maxExcl > 3 ? exRow[3] : ~0u;
-                const bool anyEx = ex0 != ~0u;
-                uint32_t *row = nl + (size_t)i * cap;
-                int cnt = sCnt[k];
-                // lane l: the LDS range of column l that survives culling
-                int lo = 0, hi = 0;
-                if (lane < ncol) {
-                    const int a = lane / sy, b = lane - a * sy;
-                    const float gx = cu.on[0] ? axis_gap(cxy.x - cg.reach + a, cu.w[0], xi.x) : 0.0f;
-                    const float gy = cu.on[1] ? axis_gap(cxy.y - cg.reach + b, cu.w[1], xi.y) : 0.0f;
-                    const float gxy2 = gx * gx + gy * gy;
-                    if (gxy2 <= cu.r2) {
-                        int slo = 0, shi = sz - 1;
-                        if (cu.on[2]) {
-                            const float h = sqrtf(cu.r2 - gxy2);
-                            slo = max((int)floorf((xi.z - h) / cu.w[2]) - (cz - cg.reach), 0);
-                            shi = min((int)floorf((xi.z + h) / cu.w[2]) - (cz - cg.reach), sz - 1);
-                        }
-                        if (slo <= shi) {
-                            lo = max(sZoff[lane][slo], w0) - w0;
-                            hi = min(sZoff[lane][shi + 1], w1) - w0;
-                        }
-                    }
-                }
-                for (int col = 0; col < ncol; ++col) {
-                    const int t0 = __builtin_amdgcn_readlane(lo, col);
-                    const int t1 = __builtin_amdgcn_readlane(hi, col);
-                    for (int tb = t0; tb < t1; tb += 64) {
-                        const int t = tb + lane;
-                        bool hit = false;
-                        int j = -1;
-                        if (t < t1) {
-                            const float4 pj = sXq[t];
-                            j = __float_as_int(pj.w);
-                            if (j != i) {
-                                const float r2f = dist2f(xi, pj, pf);
-                                hit = r2f < pf.lo2;
-                                if (!hit && r2f <= pf.hi2) hit = dist2(pi, pos[j], g) < rlist2;
-                            }
-                        }
-                        uint32_t tag = 0;
-                        if (anyEx) {
-                            const uint32_t uj = (uint32_t)j;
-                            tag = (ex0 & 0x3FFFFFFFu) == uj ? ex0 >> 30 : tag;
-                            tag = (ex1 & 0x3FFFFFFFu) == uj ? ex1 >> 30 : tag;
-                            tag = (ex2 & 0x3FFFFFFFu) == uj ? ex2 >> 30 : tag;
-                            tag = (ex3 & 0x3FFFFFFFu) == uj ? ex3 >> 30 : tag;
-                            if (maxExcl > 4 && hit)
-                                for (int q = 4; q < maxExcl; ++q) {
-                                    const uint32_t e = exRow[q];
-                                    if ((e & 0x3FFFFFFFu) == uj) tag = e >> 30;
-                                }
-                        }
-                        const unsigned long long m = __ballot(hit);
-                        if (hit) {
-                            const int s = cnt + __popcll(m & below);
-                            if (s < cap) row[s] = (uint32_t)j | (tag << 30);
-                        }
-                        cnt += __popcll(m);
-                    }
-                }
-                if (lane == 0) sCnt[k] = cnt;
-            }
-        }
-        __syncthreads();
-        for (int k = threadIdx.x; k < nA; k += blockDim.x) {
-            nlCount[a0 + k] = sCnt[k];
-            wmax = max(wmax, sCnt[k]);
-        }
-    }
-    // one atomic per wave: same-address atomics serialise in L2
-    wmax = wave_max(wmax);
-    if (lane == 0) atomicMax(&flags[1], wmax);
-}
-
 // E3B molecule list from the atom list: for each water, the oxygens among its
 // oxygen's neighbors within (r2b + skin) at build time.  Replaces the private
 // molecule GridGPU of FixE3B (FixE3B.cu:656-684) that the reference rebuilds
 // every step; with the skin criterion the per-step r2b set is a subset.
 template <int T>
-__global__ void __launch_bounds__(256) k_e3b_list(const int *__restrict__ gate, 
-    int nmol, const int4 *__restrict__ molAtoms, const int *__restrict__ atomMolO,
+__global__ void __launch_bounds__(256) k_e3b_list(
+    const int *__restrict__ gate, int nmol, const int4 *__restrict__ molAtoms, const int *__restrict__ atomMolO,
     const double4 *__restrict__ pos, Geometry g, double rcut2, int cap, const uint32_t *__restrict__ nl,
-    const int *__restrict__ nlCount, int mcap, int *__restrict__ mnl, int *__restrict__ mcount,
-    int *__restrict__ flags) {
+    const int *__restrict__ nlCount, const int *__restrict__ nlCountA, int unchargedOnly, int mcap,
+    int *__restrict__ mnl, int *__restrict__ mcount, int *__restrict__ flags) {
     if (gate && *gate == 0) return;
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int m = gtid / T;
@@ -774,13 +613,18 @@ __global__ void __launch_bounds__(256) k_e3b_list(const int *__restrict__ gate,
         const unsigned long long below = (1ull << lane) - 1ull;
         const int oi = molAtoms[m].x;
         const double4 pi = pos[oi];
-        const int cnt_i = min(nlCount[oi], cap);
+        const int total = min(nlCount[oi], cap);
+        const int nA = min(nlCountA[oi], total), nB = total - nA;
         const uint32_t *row = nl + (size_t)oi * cap;
-        for (int base = 0; base < cnt_i; base += T) {
-            const int k = base + tl;
+        // entries: charged segment [0, nA), uncharged segment [cap - nB, cap);
+        // E3B oxygens of a charge-free O site need only the second
+        const int first = unchargedOnly ? nA : 0;
+        for (int base = first; base < total; base += T) {
+            const int v = base + tl;
             bool hit = false;
             int mj = -1;
-            if (k < cnt_i) {
+            if (v < total) {
+                const int k = v < nA ? v : cap - nB + (v - nA);
                 const int j = (int)(row[k] & 0x3FFFFFFFu);
                 mj = atomMolO[j];
                 if (mj >= 0 && mj != m) hit = dist2(pi, pos[j], g) < rcut2;
@@ -892,7 +736,7 @@ void launch_build(Ctx &c, const CellGrid &cg, int cur) {
     auto kern = shift ? k_build<T, true> : k_build<T, false>;
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), 0, c.stream, rebuild_gate(c), c.n, c.pos[cur], c.xq, c.geo, cg,
                        make_prefilter(c), make_cull(c), c.colRank, c.cellStart, c.rlist * c.rlist,
-                       c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.dFlags);
+                       c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.nlCountA, c.dFlags);
 }
 
 template <int T>
@@ -902,7 +746,8 @@ void launch_e3b_list(Ctx &c, int cur) {
     double rc = (double)c.e3b.r2b + c.skin;
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_list<T>), dim3(blocks), dim3(threads), 0, c.stream, rebuild_gate(c),
                        c.e3b.nmol, c.e3b.molAtoms, c.e3b.atomMolO, c.pos[cur], c.geo, rc * rc, c.cap,
-                       c.nl, c.nlCount, c.e3b.cap, c.e3b.nl, c.e3b.nlCount, c.dFlags);
+                       c.nl, c.nlCount, c.nlCountA, c.e3b.oxUncharged ? 1 : 0, c.e3b.cap, c.e3b.nl,
+                       c.e3b.nlCount, c.dFlags);
 }
 
 }  // namespace
@@ -964,6 +809,7 @@ void nlist_setup(Ctx &c) {
         c.cap = ((int)(expect * 1.3) + 32 + 15) & ~15;
         c.nl = dalloc<uint32_t>(c, (size_t)c.n * c.cap);
         c.nlCount = dalloc<int>(c, c.n);
+        c.nlCountA = dalloc<int>(c, c.n);
     }
     c.nlistConfigured = true;
 }
@@ -994,7 +840,7 @@ static void run_build(Ctx &c, bool sync) {
     hipLaunchKernelGGL(k_place, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, rebuild_gate(c), c.n, c.cellOf,
                        c.cellFill, c.perm);
     hipLaunchKernelGGL(k_cellsort, dim3(c.ncell), dim3(64), 0, c.stream, rebuild_gate(c), c.ncell, c.cellStart, c.perm,
-                       c.perm2, c.id[cur]);
+                       c.perm2, c.id[cur], c.q[cur]);
     hipLaunchKernelGGL(k_permute, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, rebuild_gate(c), c.n, c.perm2,
                        c.pos[cur], c.vel[cur], c.q[cur], c.type[cur], c.id[cur], c.pos[nxt],
                        c.vel[nxt], c.q[nxt], c.type[nxt], c.id[nxt], c.idToIdx, c.posLast, c.xq,
@@ -1032,19 +878,11 @@ static void run_build(Ctx &c, bool sync) {
         if (!c.capture) fill_int(c, c.dFlags + 1, 2, 0);
         {
             ProfScope ps(c, PK_BUILD);
-            if (c.buildMode == 1) {
-                MDX_REQUIRE(c.reach == 2, 1, "the per-cell build kernel assumes a 5-cell stencil");
-                hipLaunchKernelGGL(k_build_cell, dim3(c.ncell), dim3(256), 0, c.stream, rebuild_gate(c), c.pos[c.cur],
-                                   c.xq, c.geo, cg, make_prefilter(c), make_cull(c), c.colXY, c.colRank,
-                                   c.cellStart, c.rlist * c.rlist, c.exclIdx, c.exclOff ? c.maxExcl : 0,
-                                   c.cap, c.nl, c.nlCount, c.dFlags);
-            } else {
-                switch (c.buildTeam) {
-                    case 8: launch_build<8>(c, cg, c.cur); break;
-                    case 16: launch_build<16>(c, cg, c.cur); break;
-                    case 32: launch_build<32>(c, cg, c.cur); break;
-                    default: launch_build<64>(c, cg, c.cur); break;
-                }
+            switch (c.buildTeam) {
+                case 8: launch_build<8>(c, cg, c.cur); break;
+                case 16: launch_build<16>(c, cg, c.cur); break;
+                case 32: launch_build<32>(c, cg, c.cur); break;
+                default: launch_build<64>(c, cg, c.cur); break;
             }
         }
         if (c.e3b.on) {

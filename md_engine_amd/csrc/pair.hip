@@ -9,7 +9,12 @@
 // Design (gfx950): a team of P lanes (P = 4..32) owns one atom and strides
 // through its row, so each row load is a P-wide coalesced segment and the
 // neighbor positions (float4 x - lo, q) are gathered from cell-ordered memory
-// that stays L2-resident.  Arithmetic is FP32; the team's partial sums are
+// that stays L2-resident.  Rows hold charged neighbors first and uncharged
+// ones at the end, so an atom reads only the part that can interact with it:
+// Coulomb needs q_i != 0 and charged neighbors, LJ needs a nonzero epsilon
+// for the type pair (per-type flags say which part has such partners).  For
+// TIP4P water an H or M site reads its charged part with Coulomb only and an
+// O site its uncharged part with LJ only.  Arithmetic is FP32; the team's partial sums are
 // combined with butterfly shuffles and added once per atom into the 64-bit
 // fixed-point force buffer.  Energy and virial (0.5-weighted, full list) are
 // reduced in FP64 per block and added with one fixed-point atomic per block.
@@ -26,6 +31,8 @@ struct PairArgs {
     int cap;
     const uint32_t *nl;
     const int *nlCount;
+    const int *nlCountA;
+    const int *typeFlags;
     const float *params;
     int ntypes;
     float L[3], invL[3];
@@ -38,10 +45,12 @@ struct PairArgs {
 template <int P, bool LJ, bool COUL, bool EV>
 __global__ void __launch_bounds__(256) k_pair(PairArgs a) {
     __shared__ float sp[3 * kMaxTypes * kMaxTypes];
+    __shared__ int sfl[kMaxTypes];
     __shared__ double wred[4][7];
     const int T2 = a.ntypes * a.ntypes;
     if (LJ) {
         for (int t = threadIdx.x; t < 3 * T2; t += blockDim.x) sp[t] = a.params[t];
+        for (int t = threadIdx.x; t < a.ntypes; t += blockDim.x) sfl[t] = a.typeFlags[t];
         __syncthreads();
     }
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
@@ -52,69 +61,77 @@ __global__ void __launch_bounds__(256) k_pair(PairArgs a) {
     if (i < a.n) {
         const float4 pi = a.xq[i];
         const int ti = (LJ && a.ntypes > 1) ? a.type[i] : 0;
+        const int fl = LJ ? sfl[ti] : 0;
         const int cnt = min(a.nlCount[i], a.cap);
+        const int nA = min(a.nlCountA[i], cnt), nB = cnt - nA;
         const uint32_t *row = a.nl + (size_t)i * a.cap;
         const float twoAlphaRtPi = 2.0f * 0.5641895835477563f * a.alpha;
         const float alpha2 = a.alpha * a.alpha;
-        for (int k = tl; k < cnt; k += P) {
-            const uint32_t raw = row[k];
-            const int j = (int)(raw & 0x3FFFFFFFu);
-            const uint32_t tag = raw >> 30;
-            const float4 pj = a.xq[j];
-            float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            dx -= a.L[0] * rintf(dx * a.invL[0]);
-            dy -= a.L[1] * rintf(dy * a.invL[1]);
-            dz -= a.L[2] * rintf(dz * a.invL[2]);
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            const float m = tag == 0 ? 1.0f : a.spec[tag - 1];
-            float fs = 0.f;
-            bool any = false;
-            if (LJ) {
-                const int tp = a.ntypes > 1 ? ti * a.ntypes + a.type[j] : 0;
-                if (r2 < sp[tp]) {
-                    any = true;
-                    if (m != 0.0f) {
-                        const float eps24 = sp[T2 + tp], sig6 = sp[2 * T2 + tp];
-                        const float r2inv = 1.0f / r2;
-                        const float r6inv = r2inv * r2inv * r2inv;
-                        const float p1 = eps24 * 2.0f * sig6 * sig6;
-                        const float p2 = eps24 * sig6;
-                        fs += r6inv * r2inv * (p1 * r6inv - p2) * m;
-                        if (EV) {
-                            const float rc2 = sp[tp];
-                            const float s6r6 = sig6 * r6inv;
-                            const float rc6 = rc2 * rc2 * rc2;
-                            const float s6rc6 = sig6 / rc6;
-                            e += 0.5f * (eps24 * (1.0f / 6.0f)) *
-                                 (s6r6 * (s6r6 - 1.0f) - s6rc6 * (s6rc6 - 1.0f)) * m;
+        // part 0: charged neighbors [0, nA); part 1: uncharged [cap - nB, cap)
+        for (int part = 0; part < 2; ++part) {
+            const bool lj = LJ && (fl & (1 << part));
+            const bool coul = COUL && part == 0 && pi.w != 0.0f;
+            if (!lj && !coul) continue;
+            const int k0 = part == 0 ? 0 : a.cap - nB, k1 = part == 0 ? nA : a.cap;
+            for (int k = k0 + tl; k < k1; k += P) {
+                const uint32_t raw = row[k];
+                const int j = (int)(raw & 0x3FFFFFFFu);
+                const uint32_t tag = raw >> 30;
+                const float4 pj = a.xq[j];
+                float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                dx -= a.L[0] * rintf(dx * a.invL[0]);
+                dy -= a.L[1] * rintf(dy * a.invL[1]);
+                dz -= a.L[2] * rintf(dz * a.invL[2]);
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                const float m = tag == 0 ? 1.0f : a.spec[tag - 1];
+                float fs = 0.f;
+                bool any = false;
+                if (lj) {
+                    const int tp = a.ntypes > 1 ? ti * a.ntypes + a.type[j] : 0;
+                    if (r2 < sp[tp]) {
+                        any = true;
+                        if (m != 0.0f) {
+                            const float eps24 = sp[T2 + tp], sig6 = sp[2 * T2 + tp];
+                            const float r2inv = 1.0f / r2;
+                            const float r6inv = r2inv * r2inv * r2inv;
+                            const float p1 = eps24 * 2.0f * sig6 * sig6;
+                            const float p2 = eps24 * sig6;
+                            fs += r6inv * r2inv * (p1 * r6inv - p2) * m;
+                            if (EV) {
+                                const float rc2 = sp[tp];
+                                const float s6r6 = sig6 * r6inv;
+                                const float rc6 = rc2 * rc2 * rc2;
+                                const float s6rc6 = sig6 / rc6;
+                                e += 0.5f * (eps24 * (1.0f / 6.0f)) *
+                                     (s6r6 * (s6r6 - 1.0f) - s6rc6 * (s6rc6 - 1.0f)) * m;
+                            }
                         }
                     }
                 }
-            }
-            if (COUL && r2 < a.rq2) {
-                any = true;
-                const float ls = fmaxf(r2, 1e-10f);
-                const float rinv = rsqrtf(ls);
-                const float len = ls * rinv;
-                const float qq = a.qqr * pi.w * pj.w;
-                const float ar = a.alpha * len;
-                // erfc(ar) - (1 - m), written as -erf(ar) for a full exclusion
-                const float ec = (m == 0.0f) ? -erff(ar) : (erfcf(ar) - (1.0f - m));
-                const float s = qq * (ec * rinv + twoAlphaRtPi * __expf(-alpha2 * ls));
-                fs += s * rinv * rinv;
-                if (EV) e += 0.5f * qq * rinv * ec;
-            }
-            if (any) {
-                fx += dx * fs;
-                fy += dy * fs;
-                fz += dz * fs;
-                if (EV) {
-                    v0 += dx * fs * dx;
-                    v1 += dy * fs * dy;
-                    v2 += dz * fs * dz;
-                    v3 += dx * fs * dy;
-                    v4 += dx * fs * dz;
-                    v5 += dy * fs * dz;
+                if (coul && r2 < a.rq2) {
+                    any = true;
+                    const float ls = fmaxf(r2, 1e-10f);
+                    const float rinv = rsqrtf(ls);
+                    const float len = ls * rinv;
+                    const float qq = a.qqr * pi.w * pj.w;
+                    // erfc(ar) - (1 - m); for a full exclusion (m = 0) this is -erf(ar)
+                    const float ec = erfcf(a.alpha * len) - (1.0f - m);
+                    const float s = qq * (ec * rinv + twoAlphaRtPi * __expf(-alpha2 * ls));
+                    fs += s * rinv * rinv;
+                    if (EV) e += 0.5f * qq * rinv * ec;
+                }
+                if (any) {
+                    fx += dx * fs;
+                    fy += dy * fs;
+                    fz += dz * fs;
+                    if (EV) {
+                        v0 += dx * fs * dx;
+                        v1 += dy * fs * dy;
+                        v2 += dz * fs * dz;
+                        v3 += dx * fs * dy;
+                        v4 += dx * fs * dz;
+                        v5 += dy * fs * dz;
+                    }
                 }
             }
         }
@@ -163,9 +180,30 @@ void launch2(const PairArgs &a, bool lj, bool coul, bool ev, hipStream_t s) {
 
 }  // namespace
 
+// Per-type flags of the LJ table against the charge classes present: bit 0 =
+// nonzero epsilon with some type that has charged atoms, bit 1 = with some type
+// that has uncharged atoms.  Recomputed when the system or the table changes.
+static void pair_type_flags(Ctx &c) {
+    if (!c.pairTypeFlags) c.pairTypeFlags = dalloc<int>(c, kMaxTypes);
+    std::vector<int> charged(kMaxTypes, 0), neutral(kMaxTypes, 0), fl(kMaxTypes, 0);
+    for (size_t i = 0; i < c.typeHost.size(); ++i) (c.qHost[i] != 0.0f ? charged : neutral)[c.typeHost[i]] = 1;
+    const int T = c.ntypes;
+    for (int a = 0; a < T && c.ljOn; ++a)
+        for (int b = 0; b < T; ++b) {
+            if (c.ljEps[a * T + b] == 0.0) continue;
+            if (charged[b]) fl[a] |= 1;
+            if (neutral[b]) fl[a] |= 2;
+        }
+    MDX_REQUIRE(!c.capture, 3, "pair type flags must be set before a step is captured");
+    MDX_HIP(hipMemcpyAsync(c.pairTypeFlags, fl.data(), sizeof(int) * kMaxTypes, hipMemcpyHostToDevice, c.stream));
+    MDX_HIP(hipStreamSynchronize(c.stream));
+    c.pairFlagsDirty = false;
+}
+
 void pair_forces(Ctx &c, bool energy, bool virial) {
     const bool coul = c.ewaldReal && c.hasCharges;
     if (!c.ljOn && !coul) return;
+    if (c.pairFlagsDirty) pair_type_flags(c);
     PairArgs a;
     a.n = c.n;
     a.xq = c.xq;
@@ -173,6 +211,8 @@ void pair_forces(Ctx &c, bool energy, bool virial) {
     a.cap = c.cap;
     a.nl = c.nl;
     a.nlCount = c.nlCount;
+    a.nlCountA = c.nlCountA;
+    a.typeFlags = c.pairTypeFlags;
     a.params = c.ljParams;
     a.ntypes = c.ntypes;
     for (int d = 0; d < 3; ++d) {

@@ -4,7 +4,8 @@ Mirrors the objects the reference exports for this path (python/DASH.cpp:34-131)
 so water and LJ scripts keep their shape: ``State`` with ``units``,
 ``atomParams``, ``bounds``, ``rCut``/``padding``/``dt``, ``addAtom`` and
 ``createMolecule``; ``FixLJCut``, ``FixChargeEwald``, ``FixE3B``,
-``FixRigid``, ``FixNVTRescale``; ``IntegratorVerlet.run``;
+``FixRigid``, ``FixNVTRescale``, ``FixNoseHoover``, ``FixPressureBerendsen``;
+``IntegratorVerlet.run`` (``state.graph = True`` replays captured steps);
 ``InitializeAtoms.initTemp``; ``EvaluatorLJ``; ``BoundsGPU.minImage``;
 ``state.grid.getNeighborList()``.  Everything runs through
 ``md_engine_amd._lib.Engine`` (the C ABI of libmdx.so); the objects here only
@@ -320,6 +321,19 @@ class State:
         nvt = self._fix(FixNVTRescale)
         if nvt is not None:
             eng.thermostat(True, nvt.temp, nvt.applyEvery, 0.0, self.units.boltz, self.units.mvv_to_eng)
+        nh = self._fix(FixNoseHoover)
+        if nh is not None:
+            if nh.temp is None:
+                raise ValueError("FixNoseHoover: call setTemperature first")
+            eng.nose_hoover(True, nh.temp, nh.timeConstant, nh.chainLength, self.units.boltz,
+                            self.units.mvv_to_eng)
+        ber = self._fix(FixPressureBerendsen)
+        if ber is not None:
+            nktv2p = 68568.415 if self.units.unitType == "REAL" else 1.0
+            eng.barostat(True, ber.pressure, ber.period, ber.applyEvery, ber.bulkModulus, ber.maxDilation,
+                         nktv2p, self.units.mvv_to_eng)
+        if getattr(self, "graph", False):
+            eng.option("graph", 1)
         eng.integrator_set(self.dt, self.units.ftm_to_v)
         if self.nThreadPerAtom:
             eng.teams_set(build=0, pair=int(self.nThreadPerAtom))
@@ -332,6 +346,9 @@ class State:
         for a in self.atoms:
             a.pos = Vector(*x[a.id])
             a.vel = Vector(*v[a.id])
+        lo, L = self._eng.get_box()
+        self.bounds.lo = Vector(*lo)
+        self.bounds.hi = Vector(*(lo + L))
 
 
 class Fix:
@@ -483,6 +500,43 @@ class FixNVTRescale(Fix):
         super().__init__(state, handle, groupHandle)
         self.temp = float(temp)
         self.applyEvery = int(applyEvery)
+
+
+class FixNoseHoover(Fix):
+    """FixNoseHoover(state, handle, groupHandle) with setTemperature(T,
+    timeConstant): Nose-Hoover chain of length 10 (FixNoseHoover.cu:120-1098)."""
+
+    def __init__(self, state, handle, groupHandle="all"):
+        super().__init__(state, handle, groupHandle)
+        self.temp = None
+        self.timeConstant = None
+        self.chainLength = 10
+
+    def setTemperature(self, temperature, timeConstant):
+        self.temp = float(temperature)
+        self.timeConstant = float(timeConstant)
+        self.state._eng = None
+
+    def setPressure(self, *args, **kw):
+        raise NotImplementedError("use FixPressureBerendsen for pressure control")
+
+
+class FixPressureBerendsen(Fix):
+    """FixPressureBerendsen(state, handle, pressure, period, applyEvery):
+    isotropic Berendsen dilation of the box and the molecules' centres of mass
+    (bulkModulus 10 and maxDilation 1e-5 by default, as the reference)."""
+
+    def __init__(self, state, handle, pressure, period, applyEvery=10):
+        super().__init__(state, handle)
+        self.pressure = float(pressure)
+        self.period = float(period)
+        self.applyEvery = int(applyEvery)
+        self.bulkModulus = 10.0
+        self.maxDilation = 1e-5
+
+    def setParameters(self, maxDilation=-1):
+        if maxDilation > 0:
+            self.maxDilation = float(maxDilation)
 
 
 class IntegratorVerlet:

@@ -62,7 +62,10 @@ def test_pressure_is_volume_derivative(orc, gpu_engine_factory, terms):
     dV = V0 * ((1 + h) ** 3 - (1 - h) ** 3)
     p_fd = -(us[0] - us[1]) / dV * NKTV2P
     say(f"P virial {p_vir:.2f} atm, -dU/dV {p_fd:.2f} atm")
-    tol = 2.0 if terms == "pair_e3b" else 0.02 * abs(p_fd) + 20.0
+    # FP32 energies and pairs crossing the cutoffs between the two volumes
+    # limit the finite difference to ~1e-3; a wrong or missing virial term is
+    # off by tens of percent
+    tol = 0.005 * abs(p_fd) + 2.0 if terms == "pair_e3b" else 0.03 * abs(p_fd) + 20.0
     assert p_vir == pytest.approx(p_fd, abs=tol)
 
 

@@ -64,12 +64,12 @@ def assert_same_lists(gc, grows, oc, orows):
 # ---------------------------------------------------------------------------
 # neighbor lists
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("ncell,mode", [(6, 0), (10, 0), (10, 1)])
-def test_nlist_lj_bitexact(orc, gpu_engine_factory, ncell, mode):
-    say(f"nlist LJ fcc ncell={ncell} build mode {mode}")
+@pytest.mark.parametrize("ncell,team", [(6, 8), (10, 8), (10, 64)])
+def test_nlist_lj_bitexact(orc, gpu_engine_factory, ncell, team):
+    say(f"nlist LJ fcc ncell={ncell} build team {team}")
     x, L, lo = perturbed_fcc(ncell)
     e = lj_engine(gpu_engine_factory, x, L, lo)
-    e.option("build_mode", mode)
+    e.teams_set(build=team)
     assert e.nlist_update(force=True)
     gc, grows = e.nlist_get()
     oc, orows, _ = oracle_rows_by_id(orc, x, lo, L, 2.8)
@@ -86,16 +86,16 @@ def test_nlist_lj_full_size_bitexact(orc, gpu_engine_factory):
     assert_same_lists(gc, grows, oc, orows)
 
 
-@pytest.mark.parametrize("mode", [1, 2])
-def test_nlist_small_noncubic_box(orc, gpu_engine_factory, mode):
+@pytest.mark.parametrize("team", [8, 32])
+def test_nlist_small_noncubic_box(orc, gpu_engine_factory, team):
     """Axes with fewer than 5 cells scan every cell once (no duplicates)."""
-    say(f"nlist small non-cubic box, build mode {mode}")
+    say(f"nlist small non-cubic box, build team {team}")
     rng = np.random.default_rng(3)
     L = np.array([6.5, 12.0, 9.0])
     n = int(0.8 * L.prod())
     x = rng.uniform(-3.0, 15.0, size=(n, 3))  # outside the box on purpose: wrapped by the build
     e = lj_engine(gpu_engine_factory, x, L, np.zeros(3))
-    e.option("build_mode", mode)
+    e.teams_set(build=team)
     e.nlist_update(force=True)
     st = e.nlist_stats()
     assert min(st["cells"]) < 5
@@ -104,17 +104,17 @@ def test_nlist_small_noncubic_box(orc, gpu_engine_factory, mode):
     assert_same_lists(gc, grows, oc, orows)
 
 
-@pytest.mark.parametrize("mode", [1, 2])
-def test_nlist_wide_neighbourhood(orc, gpu_engine_factory, mode):
+@pytest.mark.parametrize("team", [8, 64])
+def test_nlist_wide_neighbourhood(orc, gpu_engine_factory, team):
     """r_list = 7 sigma at liquid density: a 5x5x5-cell neighbourhood holds
     ~4500 candidates, more than one LDS stage of the per-cell build."""
-    say(f"nlist wide neighbourhood (r_list 7), build mode {mode}")
+    say(f"nlist wide neighbourhood (r_list 7), build team {team}")
     rng = np.random.default_rng(21)
     L = np.array([17.6, 17.9, 18.3])
     n = int(0.85 * L.prod())
     x = rng.uniform(0.0, 1.0, size=(n, 3)) * L
     e = lj_engine(gpu_engine_factory, x, L, np.zeros(3), rc=6.7, skin=0.3)
-    e.option("build_mode", mode)
+    e.teams_set(build=team)
     e.nlist_update(force=True)
     gc, grows = e.nlist_get()
     oc, orows, _ = oracle_rows_by_id(orc, x, np.zeros(3), L, 7.0, brute=False)
@@ -122,14 +122,14 @@ def test_nlist_wide_neighbourhood(orc, gpu_engine_factory, mode):
     assert_same_lists(gc, grows, oc, orows)
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2])
-def test_nlist_water_tags_bitexact(orc, gpu_engine_factory, mode):
-    say(f"nlist water 512 molecules with exclusion tags, build mode {mode}")
+@pytest.mark.parametrize("team", [8, 16, 64])
+def test_nlist_water_tags_bitexact(orc, gpu_engine_factory, team):
+    say(f"nlist water 512 molecules with exclusion tags, build team {team}")
     box = systems.tip4p2005_box(512, seed=5)
     e = gpu_engine_factory()
     e.set_system(box["x"], box["lo"], box["L"], q=box["q"], types=box["type"])
     e.exclusions_from_bonds(systems.water_bonds(box["mol"]))
-    e.option("build_mode", mode)
+    e.teams_set(build=team)
     e.nlist_config(10.0, 1.0)
     e.nlist_update(force=True)
     gc, grows = e.nlist_get()
@@ -140,18 +140,18 @@ def test_nlist_water_tags_bitexact(orc, gpu_engine_factory, mode):
     assert set(np.unique(tags)) == {0, 1, 2}
 
 
-@pytest.mark.parametrize("mode", [1, 2])
-def test_nlist_capacity_growth(orc, gpu_engine_factory, mode):
+@pytest.mark.parametrize("team", [8, 64])
+def test_nlist_capacity_growth(orc, gpu_engine_factory, team):
     """A dense cluster overflows the initial row capacity and crowds one cell
-    past 256 atoms: rows grow, batches cover the cell, no loss (both builds)."""
-    say(f"nlist capacity growth, build mode {mode}")
+    past 256 atoms: rows grow with no loss (narrow and wide teams)."""
+    say(f"nlist capacity growth, build team {team}")
     rng = np.random.default_rng(9)
     L = np.full(3, 20.0)
     x = rng.uniform(0, 20, size=(3000, 3))
     x[:400] = rng.uniform(8, 10, size=(400, 3))
     x[400:700] = rng.uniform(12.1, 12.9, size=(300, 3))
     e = lj_engine(gpu_engine_factory, x, L, np.zeros(3))
-    e.option("build_mode", mode)
+    e.teams_set(build=team)
     e.nlist_update(force=True)
     gc, grows = e.nlist_get()
     oc, orows, _ = oracle_rows_by_id(orc, x, np.zeros(3), L, 2.8)
@@ -356,7 +356,7 @@ def test_tune_keeps_lists_and_forces(orc, gpu_engine_factory):
     e = water_engine(gpu_engine_factory, box, e3b=True)
     e.compute(PAIR | PPPM | E3B)
     choice = e.tune(PAIR | PPPM | E3B, evals=3)
-    assert choice["build_mode"] in (1, 2) and choice["build_team"] in (8, 16, 32, 64)
+    assert choice["build_mode"] == 2 and choice["build_team"] in (8, 16, 32, 64)
     assert choice["pair_team"] in (4, 8, 16, 32) and choice["e3b_team"] in (4, 8, 16)
     assert choice["build_ms"] > 0 and choice["pair_ms"] > 0 and choice["e3b_ms"] > 0
     gc, grows = e.nlist_get()
