@@ -288,6 +288,9 @@ def main():
     ap.add_argument("--pair-team", type=int, default=0)
     ap.add_argument("--build-team", type=int, default=0)
     ap.add_argument("--e3b-team", type=int, default=0)
+    ap.add_argument("--reach-xy", type=int, default=0, help="cell stencil reach in x/y (0: library default)")
+    ap.add_argument("--reach-z", type=int, default=0, help="cell stencil reach in z (0: library default)")
+    ap.add_argument("--gather-team", type=int, default=0, help="lanes per charged atom in the P3M gather")
     ap.add_argument("--tune", action="store_true", help="run the autotuner before timing")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of known size (counter calibration)")
@@ -308,6 +311,12 @@ def main():
     w = make_workload(args.config)
     e, flags = setup_engine(w)
     e.teams_set(args.build_team, args.pair_team, args.e3b_team)
+    if args.reach_xy:
+        e.option("cell_reach_xy", args.reach_xy)
+    if args.reach_z:
+        e.option("cell_reach_z", args.reach_z)
+    if args.gather_team:
+        e.option("gather_team", args.gather_team)
     n = len(w["box"]["x"])
     R = args.rebuild_every
     surrogate = args.three_body == "surrogate"

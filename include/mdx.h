@@ -205,14 +205,18 @@ int mdx_barostat_last(mdx_ctx *ctx, double *pressure);
  * search of Tunable knobs by timing): times `evals` runs of every list-build
  * kernel/team width, pair team width and E3B team width for the terms in
  * `flags`, keeps the fastest, and recomputes forces.  chosen[4] (may be NULL)
- * = build mode (1 per-cell, 2 per-atom), build team, pair team, E3B team;
+ * = build mode (2, the per-atom team build), build team, pair team, E3B team;
  * avg_ms[3] (may be NULL) = the winners' average build, pair, E3B times. */
 int mdx_tune(mdx_ctx *ctx, int flags, int evals, int32_t *chosen, double *avg_ms);
 int mdx_sync(mdx_ctx *ctx);
 /* Implementation switches (no counterpart in the reference): "tiled_spread"
  * 1 (default) accumulates the charge mesh in LDS tiles, 0 adds every weight
- * to the global mesh directly (bit-identical meshes); "build_mode" 0 (auto),
- * 1 LDS-staged build per cell, 2 per-atom teams (identical lists). */
+ * to the global mesh directly (bit-identical meshes); "graph" 1 (default)
+ * runs mdx_run as captured 8-step graphs, 0 eagerly; "build_mode" 0 or 2
+ * (both the per-atom team build); "cell_reach_xy" 1..2 and "cell_reach_z"
+ * 1..16: cell edges >= rlist / reach per axis (identical lists, finer cells
+ * test fewer candidates); "gather_team" 8 or 16 lanes per charged atom in
+ * the P3M field gather. */
 int mdx_option_set(mdx_ctx *ctx, const char *name, int value);
 
 /* Force-path steps at fixed positions: nsteps evaluations of the terms in

@@ -146,7 +146,7 @@ void barostat_apply(Ctx &c) {
         const double ctr = c.geo.lo[d] + 0.5 * c.geo.L[d];
         len[d] = c.geo.L[d] * mu;
         lo[d] = ctr - 0.5 * len[d];
-        const double edge = c.rlist / c.reach * (1.0 + 1e-9);
+        const double edge = c.rlist / c.reach[d] * (1.0 + 1e-9);
         regrid = regrid || std::max(1, (int)std::floor(len[d] / edge)) != c.nc[d];
         if (c.geo.periodic[d]) MDX_REQUIRE(c.rlist < 0.5 * len[d], 1, "box shrank below twice the list cutoff");
     }
@@ -159,8 +159,7 @@ void barostat_apply(Ctx &c) {
 
 void compute(Ctx &c, int flags) {
     const bool energy = flags & MDX_ENERGY, virial = flags & MDX_VIRIAL;
-    refresh_xq(c);
-    zero_forces(c, true);
+    prepare_forces(c);
     if ((flags & MDX_PAIR)) pair_forces(c, energy, virial);
     if ((flags & MDX_PPPM) && c.pppm.on) pppm_forces(c, energy, virial);
     if ((flags & MDX_SURROGATE) && c.e3b.on) {
@@ -376,10 +375,13 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags) {
             bool regrow = false;
             for (Ctx *cp : cs) {
                 Ctx &c = *cp;
-                int h[4];
-                MDX_HIP(hipMemcpy(h, c.dFlags + 1, sizeof(int) * 4, hipMemcpyDeviceToHost));
+                int h[7];
+                MDX_HIP(hipMemcpy(h, c.dFlags + 1, sizeof(int) * 7, hipMemcpyDeviceToHost));
                 c.pinned[0] = h[0];
                 c.pinned[1] = h[1];
+                c.pinned[6] = h[6];
+                MDX_REQUIRE(h[6] == 0, 1,
+                            "an atom's stencil holds over 2^20 candidates: lower the list cutoff or the cell reach");
                 if (h[0] > c.cap || (c.e3b.on && h[1] > c.e3b.cap)) {
                     snapshot_restore(c);
                     grow_if_needed(c);
@@ -438,6 +440,7 @@ int mdx_create(int device, mdx_ctx **out) {
         c.dMax = dalloc<unsigned>(c, 4);
         c.thermo = dalloc<unsigned long long>(c, 2);
         MDX_HIP(hipHostMalloc((void **)&c.pinned, sizeof(int) * 8, hipHostMallocDefault));
+        std::memset(c.pinned, 0, sizeof(int) * 8);
         MDX_HIP(hipEventCreateWithFlags(&c.capEvent, hipEventDisableTiming));
         for (int k = 0; k < 2; ++k) MDX_HIP(hipEventCreateWithFlags(&c.stepEvent[k], hipEventDisableTiming));
         MDX_HIP(hipMemsetAsync(c.dMax, 0, sizeof(unsigned) * 4, c.stream));
@@ -507,6 +510,7 @@ int mdx_system_set(mdx_ctx *ctx, int n, const double *x, const double *v, const 
         c.typeHost.assign(n, 0);
         c.pairFlagsDirty = true;
         c.hasCharges = false;
+        c.nCharged = 0;
         c.maxType = 0;
         for (int i = 0; i < n; ++i) {
             hp[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], 0.0);
@@ -515,7 +519,10 @@ int mdx_system_set(mdx_ctx *ctx, int n, const double *x, const double *v, const 
             hv[i] = v ? make_double4(v[3 * i], v[3 * i + 1], v[3 * i + 2], im) : make_double4(0, 0, 0, im);
             hq[i] = q ? (float)q[i] : 0.0f;
             c.qHost[i] = hq[i];
-            if (hq[i] != 0.0f) c.hasCharges = true;
+            if (hq[i] != 0.0f) {
+                c.hasCharges = true;
+                ++c.nCharged;
+            }
             ht[i] = type ? type[i] : 0;
             c.typeHost[i] = ht[i];
             MDX_REQUIRE(ht[i] >= 0 && ht[i] < kMaxTypes, MDX_ERR_ARG, "atom type out of range [0, 16)");
@@ -719,10 +726,9 @@ int mdx_nlist_get(mdx_ctx *ctx, int32_t *counts, uint32_t *tagged_ids, int row_c
         require_system(c);
         MDX_REQUIRE(c.listValid, MDX_ERR_STATE, "no neighbor list built");
         const int n = c.n;
-        std::vector<int> cnt(n), cntA(n), ids(n);
+        std::vector<int> cnt(n), ids(n);
         std::vector<uint32_t> nl((size_t)n * c.cap);
         MDX_HIP(hipMemcpyAsync(cnt.data(), c.nlCount, sizeof(int) * n, hipMemcpyDeviceToHost, c.stream));
-        MDX_HIP(hipMemcpyAsync(cntA.data(), c.nlCountA, sizeof(int) * n, hipMemcpyDeviceToHost, c.stream));
         MDX_HIP(hipMemcpyAsync(ids.data(), c.id[c.cur], sizeof(int) * n, hipMemcpyDeviceToHost, c.stream));
         MDX_HIP(hipMemcpyAsync(nl.data(), c.nl, sizeof(uint32_t) * nl.size(), hipMemcpyDeviceToHost, c.stream));
         MDX_HIP(hipStreamSynchronize(c.stream));
@@ -731,13 +737,10 @@ int mdx_nlist_get(mdx_ctx *ctx, int32_t *counts, uint32_t *tagged_ids, int row_c
         for (int k = 0; k < n; ++k) {
             const int myid = ids[k];
             const int cn = std::min(cnt[k], c.cap);
-            const int na = std::min(cntA[k], cn), nb = cn - na;
             mx = std::max(mx, cnt[k]);
             row.resize(cn);
-            // charged neighbors lead the row, uncharged ones end it
             for (int s = 0; s < cn; ++s) {
-                const size_t slot = (size_t)k * c.cap + (s < na ? s : c.cap - nb + (s - na));
-                const uint32_t e = nl[slot];
+                const uint32_t e = nl[(size_t)k * c.cap + s];
                 row[s] = (uint32_t)ids[e & 0x3FFFFFFFu] | (e & 0xC0000000u);
             }
             std::sort(row.begin(), row.end(),
@@ -1099,8 +1102,8 @@ int mdx_force_steps(mdx_ctx *ctx, int nsteps, int flags, int rebuild_every) {
 int mdx_teams_set(mdx_ctx *ctx, int build_team, int pair_team, int e3b_team) {
     return guarded(ctx, [&](Ctx &c) {
         auto ok = [](int v, int lo, int hi) { return v == 0 || (v >= lo && v <= hi && (v & (v - 1)) == 0); };
-        MDX_REQUIRE(ok(build_team, 8, 64) && ok(pair_team, 4, 32) && ok(e3b_team, 4, 16), MDX_ERR_ARG,
-                    "team sizes: build 8..64, pair 4..32, e3b 4..16, powers of two");
+        MDX_REQUIRE(ok(build_team, 8, 64) && ok(pair_team, 4, 32) && ok(e3b_team, 4, 32), MDX_ERR_ARG,
+                    "team sizes: build 8..64, pair 4..32, e3b 4..32, powers of two");
         if (build_team) {
             c.buildTeam = build_team;
             c.buildTeamUser = true;
@@ -1178,7 +1181,7 @@ int mdx_tune(mdx_ctx *ctx, int flags, int evals, int32_t *chosen, double *avg_ms
         int bestE3b = c.e3b.team;
         double bestE3bT = 1e30;
         if ((flags & MDX_E3B) && c.e3b.on) {
-            for (int t : {4, 8, 16}) {
+            for (int t : {4, 8, 16, 32}) {
                 c.e3b.team = t;
                 const double ms = timed([&] { e3b_forces(c, false, false); });
                 if (ms < bestE3bT) {
@@ -1381,6 +1384,23 @@ int mdx_option_set(mdx_ctx *ctx, const char *name, int value) {
         else if (k == "build_mode") {
             MDX_REQUIRE(value == 0 || value == 2, MDX_ERR_ARG, "build_mode: 0 or 2 (per-atom teams)");
             c.buildMode = value;
+        }
+        else if (k == "gather_team") {
+            MDX_REQUIRE(value == 8 || value == 16, MDX_ERR_ARG, "gather_team: 8 or 16 lanes per charged atom");
+            c.pppm.gatherTeam = value;
+        }
+        else if (k == "cell_reach_xy" || k == "cell_reach_z") {
+            // cell edge >= rlist / reach: finer cells cut the candidate volume,
+            // the xy reach also widens the build kernel's column table
+            const bool xy = k == "cell_reach_xy";
+            MDX_REQUIRE(value >= 1 && value <= (xy ? 2 : 16), MDX_ERR_ARG,
+                        "cell_reach_xy: 1..2, cell_reach_z: 1..16");
+            if (xy)
+                c.reach[0] = c.reach[1] = value;
+            else
+                c.reach[2] = value;
+            c.nlistConfigured = false;
+            c.listValid = false;
         }
         else throw Error{MDX_ERR_ARG, "unknown option " + k};
     });

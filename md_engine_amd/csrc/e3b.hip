@@ -310,7 +310,8 @@ void e3b_forces(Ctx &c, bool energy, bool virial, bool three_body, float *emol) 
     switch (E.team) {
         case 4: launch<4>(c, a, ev); break;
         case 8: launch<8>(c, a, ev); break;
-        default: launch<16>(c, a, ev); break;
+        case 16: launch<16>(c, a, ev); break;
+        default: launch<32>(c, a, ev); break;
     }
 }
 

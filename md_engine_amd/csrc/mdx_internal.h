@@ -49,6 +49,7 @@ struct Geometry {
 
 struct PppmState {
     bool on = false;
+    int gatherTeam = 8;           // lanes per charged atom in the field gather
     int sz[3] = {64, 64, 64};
     int order = 5;
     double rcut = 9.0;
@@ -83,7 +84,7 @@ struct E3BState {
     int *nl = nullptr;        // nmol * cap molecule indices
     int *nlCount = nullptr;
     float4 *tot = nullptr;    // per molecule totals (x accepted, y H1 donated, z H2 donated)
-    int team = 8;
+    int team = 32;            // lanes per molecule (E3B passes and molecule list)
     bool oxUncharged = false;  // every E3B oxygen site has q = 0 (its neighbors need no charged scan)
     std::vector<int> oxIds;    // by molecule: oxygen atom id
 };
@@ -168,17 +169,20 @@ struct Ctx {
     bool pairFlagsDirty = true;
     int *pairTypeFlags = nullptr;         // per type: bit 0 LJ with a charged type, bit 1 with an uncharged one       // by id
     bool hasCharges = false;
+    int nCharged = 0;             // atoms with q != 0: indices [0, nCharged) once a list is built
     int maxType = 0;
 
     // neighbor list
     double rlist = 0.0, skin = 0.0;
     bool nlistConfigured = false;
     int nc[3] = {0, 0, 0};
-    int reach = 2;
+    // cell stencil half-width per axis (cell edge >= rlist / reach); thinner z
+    // cells cut the z-runs closer to the sphere
+    int reach[3] = {2, 2, 4};
     int ncell = 0;
     int *colRank = nullptr;
     int *cellCount = nullptr;
-    int *cellStart = nullptr;
+    int *cellStart = nullptr;     // 2 ncell + 1: cells of the charged atoms, then of the charge-free ones
     int *cellFill = nullptr;
     int *cellOf = nullptr;
     int *perm = nullptr;
@@ -186,15 +190,16 @@ struct Ctx {
     int2 *colXY = nullptr;        // by column rank: cell (x, y)
     int cap = 0;
     uint32_t *nl = nullptr;
-    int *nlCountA = nullptr;      // leading (charged-neighbor) part of each row; the rest fills from the row end
+    int *nlCountA = nullptr;      // leading (charged-neighbor) part of each row; charge-free partners follow
     int *nlCount = nullptr;
     // [0] rebuild flag, [1] max count, [2] e3b max count; captured steps:
-    // [3] rebuild gate, [4] rebuilds, [5] thermostat gate, [6] turn
+    // [3] rebuild gate, [4] rebuilds, [5] thermostat gate, [6] turn;
+    // [7] set when an atom's candidate stream overflows the build table
     int *dFlags = nullptr;
     bool capture = false;         // a step is being recorded into a graph
     int graphSteps = 0;           // mdx_run: 1 = captured steps (option "graph")
     unsigned *dMax = nullptr;     // 2 slots x (max |x - x_last|^2, max predicted^2) as float bits
-    int *pinned = nullptr;        // host-pinned: [0..1] last build's max counts, [2..5] dMax copies
+    int *pinned = nullptr;        // host-pinned: [0..1] last build's max counts, [2..5] dMax copies, [6] stream overflow
     hipEvent_t capEvent = nullptr;
     bool capPending = false;
     long long listViolations = 0; // steps whose list was older than the skin allows
@@ -323,6 +328,13 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
+// orders LDS traffic between the lanes of one wave (no cross-wave barrier)
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int T>
 __device__ __forceinline__ double team_sum_d(double v) {
 #pragma unroll
@@ -340,6 +352,8 @@ void nlist_setup(Ctx &c);
 void nlist_check(Ctx &c);             // sets dFlags[0] when a rebuild is due
 void nlist_rebuild(Ctx &c, bool forced);  // rebuild if dFlags[0] or forced (host reads flag)
 void refresh_xq(Ctx &c);
+// refresh_xq + zero_forces(c, true) in one launch
+void prepare_forces(Ctx &c);
 void fill_int(Ctx &c, int *p, int count, int v, const int *gate = nullptr);  // kernel fill (no memset nodes)
 void nlist_rebuild_async(Ctx &c);     // rebuild now; capacity checked later (capacity_ok)
 bool capacity_ok(Ctx &c);             // false: rows grew, results since the last check are invalid

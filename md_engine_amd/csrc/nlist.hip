@@ -33,7 +33,7 @@ namespace {
 struct CellGrid {
     int nc[3];
     double cellInv[3];  // nc / L
-    int reach;
+    int reach[3];       // stencil half-width in cells per axis
 };
 
 __device__ __forceinline__ double mi_d(double d, double L, double invL, int periodic) {
@@ -68,9 +68,14 @@ __global__ void k_check(int n, const double4 *__restrict__ pos, const double4 *_
 }
 
 // Wrap into the box (periodicWrap) and bin into cells (countNumInGridCells).
-__global__ void k_bin(const int *__restrict__ gate, int n, double4 *__restrict__ pos, Geometry g, CellGrid cg,
-                      const int *__restrict__ colRank, int *__restrict__ cellOf,
-                      int *__restrict__ cellCount) {
+// Charged and charge-free atoms go to two copies of the grid (slots
+// [0, ncell) and [ncell, 2 ncell)), so that after the permutation the charged
+// atoms come first, each class sorted by cell: a neighbor row then lists its
+// charged partners first, and the charge spread and gather see only charged
+// atoms.
+__global__ void k_bin(const int *__restrict__ gate, int n, double4 *__restrict__ pos, const float *__restrict__ q,
+                      Geometry g, CellGrid cg, int ncell, const int *__restrict__ colRank,
+                      int *__restrict__ cellOf, int *__restrict__ cellCount) {
     if (gate && *gate == 0) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -87,7 +92,7 @@ __global__ void k_bin(const int *__restrict__ gate, int n, double4 *__restrict__
     int cx = cell_coord(p.x, g.lo[0], cg.cellInv[0], cg.nc[0]);
     int cy = cell_coord(p.y, g.lo[1], cg.cellInv[1], cg.nc[1]);
     int cz = cell_coord(p.z, g.lo[2], cg.cellInv[2], cg.nc[2]);
-    int rank = colRank[cx * cg.nc[1] + cy] * cg.nc[2] + cz;
+    int rank = colRank[cx * cg.nc[1] + cy] * cg.nc[2] + cz + (q[i] == 0.0f ? ncell : 0);
     cellOf[i] = rank;
     atomicAdd(&cellCount[rank], 1);
 }
@@ -145,17 +150,14 @@ __global__ void k_place(const int *__restrict__ gate, int n, const int *__restri
 }
 
 // Make the order inside each cell independent of atomic slot order: one
-// workgroup per cell ranks its entries by (class, atom id) in LDS (keys are
-// unique, so rank = number of smaller keys) and writes them in rank order.
-// Class 0 = charged, 1 = uncharged, so atoms that need the same pair work sit
-// together and teams of one wave mostly share their code path.
+// workgroup per cell slot ranks its entries by atom id in LDS (ids are unique,
+// so rank = number of smaller keys) and writes them in rank order.
 __global__ void __launch_bounds__(64) k_cellsort(const int *__restrict__ gate, int ncell,
                                                  const int *__restrict__ start, const int *__restrict__ permIn,
-                                                 int *__restrict__ permOut, const int *__restrict__ id,
-                                                 const float *__restrict__ q) {
+                                                 int *__restrict__ permOut, const int *__restrict__ id) {
     if (gate && *gate == 0) return;
     constexpr int CAP = 1024;
-    __shared__ long long skey[CAP];
+    __shared__ int skey[CAP];
     __shared__ int sperm[CAP];
     const int c = blockIdx.x;
     const int a = start[c], b = start[c + 1], m = b - a;
@@ -163,11 +165,11 @@ __global__ void __launch_bounds__(64) k_cellsort(const int *__restrict__ gate, i
         for (int t = threadIdx.x; t < m; t += 64) {
             const int p = permIn[a + t];
             sperm[t] = p;
-            skey[t] = ((long long)(q[p] == 0.0f) << 32) | (long long)id[p];
+            skey[t] = id[p];
         }
         __syncthreads();
         for (int t = threadIdx.x; t < m; t += 64) {
-            const long long key = skey[t];
+            const int key = skey[t];
             int r = 0;
             for (int j = 0; j < m; ++j) r += skey[j] < key;
             permOut[a + r] = sperm[t];
@@ -177,9 +179,9 @@ __global__ void __launch_bounds__(64) k_cellsort(const int *__restrict__ gate, i
         for (int s = a; s < b; ++s) permOut[s] = permIn[s];
         for (int s = a + 1; s < b; ++s) {
             const int v = permOut[s];
-            const long long key = ((long long)(q[v] == 0.0f) << 32) | (long long)id[v];
+            const int key = id[v];
             int t = s - 1;
-            while (t >= a && (((long long)(q[permOut[t]] == 0.0f) << 32) | (long long)id[permOut[t]]) > key) {
+            while (t >= a && id[permOut[t]] > key) {
                 permOut[t + 1] = permOut[t];
                 --t;
             }
@@ -220,6 +222,21 @@ __global__ void k_xq(int n, const double4 *__restrict__ pos, const float *__rest
     if (k >= n) return;
     double4 p = pos[k];
     xq[k] = make_float4((float)(p.x - g.lo[0]), (float)(p.y - g.lo[1]), (float)(p.z - g.lo[2]), q[k]);
+}
+
+// k_xq fused with the clearing of the fixed-point force rows and of the
+// energy / virial accumulators that the force kernels add into.
+__global__ void k_xq_zero(int n, const double4 *__restrict__ pos, const float *__restrict__ q,
+                          float4 *__restrict__ xq, Geometry g, unsigned long long *__restrict__ force, int nacc,
+                          unsigned long long *__restrict__ acc) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nacc) acc[k] = 0ull;
+    if (k >= n) return;
+    double4 p = pos[k];
+    xq[k] = make_float4((float)(p.x - g.lo[0]), (float)(p.y - g.lo[1]), (float)(p.z - g.lo[2]), q[k]);
+    force[3 * (size_t)k + 0] = 0ull;
+    force[3 * (size_t)k + 1] = 0ull;
+    force[3 * (size_t)k + 2] = 0ull;
 }
 
 __global__ void k_copy_back(const int *__restrict__ gate, int n, const double4 *__restrict__ pos1,
@@ -396,10 +413,15 @@ __device__ __forceinline__ float axis_gap(int u, float w, float x) {
 // One team of T lanes per atom walks the culled stencil.  The team first
 // lays out its table of candidate runs in LDS, one stencil column per lane
 // (column bases and cell starts are then fetched in parallel, not as a chain
-// of dependent loads per column), then tests the runs T candidates at a time
-// with the next chunk's positions already in flight.  Hits are compacted with
-// a ballot restricted to the team, so row order is (stencil column, z-run,
-// index) - deterministic.
+// of dependent loads per column): the runs of the charged atoms' grid, then
+// those of the charge-free atoms' grid, each run padded to whole chunks of T
+// so that one stream of chunks covers them all and a chunk never straddles two
+// runs (the run of a chunk is uniform over the team; padding lanes test
+// nothing).  The team then walks the stream U chunks at a time with all U
+// loads in flight.  Hits are compacted with a ballot restricted to the team,
+// so row order is (charge class, stencil column, z-run, index) -
+// deterministic - and the row splits at nlCountA into charged and charge-free
+// partners.
 //
 // SHIFT: every axis is open or holds >= 2 reach + 1 cells, so each candidate
 // of a run sits in one known image of the unwrapped stencil (the run's image
@@ -408,9 +430,11 @@ __device__ __forceinline__ float axis_gap(int u, float w, float x) {
 //
 // Exclusion partners arrive as current indices (idx | tag << 30, rows of
 // maxExcl padded with ~0u); the first four sit in registers and are compared
-// only within runs that contain one of them, further ones are read for hit
-// lanes only.
-constexpr int kMaxCols = 25;  // (2 reach + 1)^2 columns for reach 2
+// only within runs flagged as holding one of them or the atom itself, further
+// ones are read for hit lanes only.
+constexpr int kMaxCols = 25;  // (2 reach + 1)^2 columns for an xy reach of up to 2
+constexpr int kEndBits = 20;  // run ends in a team's candidate stream
+constexpr uint32_t kEndMask = (1u << kEndBits) - 1u;
 
 __device__ __forceinline__ bool in_run(uint32_t ex, int s0, int s1) {
     const int j = (int)(ex & 0x3FFFFFFFu);
@@ -420,153 +444,220 @@ __device__ __forceinline__ bool in_run(uint32_t ex, int s0, int s1) {
 template <int T, bool SHIFT>
 __global__ void __launch_bounds__(256) k_build(const int *__restrict__ gate, 
     int n, const double4 *__restrict__ pos, const float4 *__restrict__ xq, Geometry g, CellGrid cg,
-    Prefilter pf, Cull cu, const int *__restrict__ colRank, const int *__restrict__ cellStart,
+    Prefilter pf, Cull cu, const int *__restrict__ colRank, const int *__restrict__ cellStart, int ncell,
     double rlist2, const uint32_t *__restrict__ exclIdx, int maxExcl, int cap,
     uint32_t *__restrict__ nl, int *__restrict__ nlCount, int *__restrict__ nlCountA, int *__restrict__ flags) {
     if (gate && *gate == 0) return;
-    constexpr int U = 64 / T;  // chunks of T candidates per load block
-    __shared__ int4 sRuns[256 / T][kMaxCols];
-    __shared__ int sImg[256 / T][kMaxCols];  // (kx+1) | (ky+1) << 2 | (kz0+1) << 4 | (kz1+1) << 6
+    constexpr int U = T >= 32 ? 1 : 32 / T;  // chunks of T candidates per load block
+    constexpr int R = 4 * kMaxCols;           // runs per table: 2 classes x 2 z-runs per column
+    __shared__ int sOff[256 / T][R];          // atom index = stream position + sOff
+    __shared__ uint32_t sEnd[256 / T][R];     // end of the run's candidates | code << kEndBits
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = gtid / T;
     const int team = threadIdx.x / T;
     const int tl = threadIdx.x & (T - 1);
     const int lane = threadIdx.x & 63;
     const bool live = i < n;
-    const int sx = axis_slots(cg.nc[0], cg.reach), sy = axis_slots(cg.nc[1], cg.reach);
+    const int rx = cg.reach[0], ry = cg.reach[1], rz = cg.reach[2];
+    const int sx = axis_slots(cg.nc[0], rx), sy = axis_slots(cg.nc[1], ry);
     const int ncol = sx * sy;
     const int ncz = cg.nc[2];
+    // team-uniform stream lengths: runs are padded to whole chunks of T, so a
+    // chunk never straddles two runs; [0, totalA) walks the charged atoms
+    int total = 0, totalA = 0;
     if (live) {
         const double4 pi = pos[i];
         const float4 xi = xq[i];
         const int cx = cell_coord(pi.x, g.lo[0], cg.cellInv[0], cg.nc[0]);
         const int cy = cell_coord(pi.y, g.lo[1], cg.cellInv[1], cg.nc[1]);
         const int cz = cell_coord(pi.z, g.lo[2], cg.cellInv[2], cg.nc[2]);
-        for (int col = tl; col < ncol; col += T) {
-            const int a = col / sy, b = col - a * sy;
-            int4 run = make_int4(0, 0, 0, 0);
-            int img = 0;
-            const int xx = axis_cell(a, cx, cg.nc[0], cg.reach, g.periodic[0]);
-            const int yy = axis_cell(b, cy, cg.nc[1], cg.reach, g.periodic[1]);
-            const float gx = cu.on[0] ? axis_gap(cx - cg.reach + a, cu.w[0], xi.x) : 0.0f;
-            const float gy = cu.on[1] ? axis_gap(cy - cg.reach + b, cu.w[1], xi.y) : 0.0f;
-            const float gxy2 = gx * gx + gy * gy;
-            if (xx >= 0 && yy >= 0 && gxy2 <= cu.r2) {
-                ZRuns zr;
-                if (cu.on[2]) {
-                    const float h = sqrtf(cu.r2 - gxy2);
-                    const int ulo = max((int)floorf((xi.z - h) / cu.w[2]), cz - cg.reach);
-                    const int uhi = min((int)floorf((xi.z + h) / cu.w[2]), cz + cg.reach);
-                    zr = z_runs_cut(ulo, uhi, ncz);
-                } else {
-                    zr = z_runs(cz, ncz, cg.reach, g.periodic[2]);
+        const uint32_t *exRow = exclIdx + (size_t)i * maxExcl;
+        const uint32_t ex0 = maxExcl > 0 ? exRow[0] : ~0u, ex1 = maxExcl > 1 ? exRow[1] : ~0u;
+        const uint32_t ex2 = maxExcl > 2 ? exRow[2] : ~0u, ex3 = maxExcl > 3 ? exRow[3] : ~0u;
+        int nrun = 0;
+#pragma unroll 1
+        for (int cls = 0; cls < 2; ++cls) {
+            const int *cs = cellStart + cls * ncell;
+            for (int c0 = 0; c0 < ncol; c0 += T) {
+                const int col = c0 + tl;
+                int st0 = 0, len0 = 0, st1 = 0, len1 = 0;
+                uint32_t code0 = 0, code1 = 0;
+                if (col < ncol) {
+                    const int a = col / sy, b = col - a * sy;
+                    const int xx = axis_cell(a, cx, cg.nc[0], rx, g.periodic[0]);
+                    const int yy = axis_cell(b, cy, cg.nc[1], ry, g.periodic[1]);
+                    const float gx = cu.on[0] ? axis_gap(cx - rx + a, cu.w[0], xi.x) : 0.0f;
+                    const float gy = cu.on[1] ? axis_gap(cy - ry + b, cu.w[1], xi.y) : 0.0f;
+                    const float gxy2 = gx * gx + gy * gy;
+                    if (xx >= 0 && yy >= 0 && gxy2 <= cu.r2) {
+                        ZRuns zr;
+                        if (cu.on[2]) {
+                            const float h = sqrtf(cu.r2 - gxy2);
+                            const int ulo = max((int)floorf((xi.z - h) / cu.w[2]), cz - rz);
+                            const int uhi = min((int)floorf((xi.z + h) / cu.w[2]), cz + rz);
+                            zr = z_runs_cut(ulo, uhi, ncz);
+                        } else {
+                            zr = z_runs(cz, ncz, rz, g.periodic[2]);
+                        }
+                        const int colBase = colRank[xx * cg.nc[1] + yy] * ncz;
+                        if (zr.n > 0) {
+                            st0 = cs[colBase + zr.lo0];
+                            len0 = cs[colBase + zr.hi0 + 1] - st0;
+                        }
+                        if (zr.n > 1) {
+                            st1 = cs[colBase + zr.lo1];
+                            len1 = cs[colBase + zr.hi1 + 1] - st1;
+                        }
+                        const int ux = cx - rx + a, uy = cy - ry + b;
+                        const uint32_t kx = ux < 0 ? 0u : (ux >= cg.nc[0] ? 2u : 1u);
+                        const uint32_t ky = uy < 0 ? 0u : (uy >= cg.nc[1] ? 2u : 1u);
+                        // flag: the run holds the atom itself or an exclusion partner
+                        const uint32_t f0 =
+                            (maxExcl > 4 || (i >= st0 && i < st0 + len0) || in_run(ex0, st0, st0 + len0) ||
+                             in_run(ex1, st0, st0 + len0) || in_run(ex2, st0, st0 + len0) ||
+                             in_run(ex3, st0, st0 + len0)) ? 1u : 0u;
+                        const uint32_t f1 =
+                            (maxExcl > 4 || (i >= st1 && i < st1 + len1) || in_run(ex0, st1, st1 + len1) ||
+                             in_run(ex1, st1, st1 + len1) || in_run(ex2, st1, st1 + len1) ||
+                             in_run(ex3, st1, st1 + len1)) ? 1u : 0u;
+                        code0 = kx | (ky << 2) | ((uint32_t)(zr.k0 + 1) << 4) | (f0 << 6);
+                        code1 = kx | (ky << 2) | ((uint32_t)(zr.k1 + 1) << 4) | (f1 << 6);
+                    }
                 }
-                const int colBase = colRank[xx * cg.nc[1] + yy] * ncz;
-                if (zr.n > 0) {
-                    run.x = cellStart[colBase + zr.lo0];
-                    run.y = cellStart[colBase + zr.hi0 + 1];
+                // team prefix sums place the non-empty runs in column order
+                const int nr = (len0 > 0 ? 1 : 0) + (len1 > 0 ? 1 : 0);
+                const int pl0 = (len0 + T - 1) & ~(T - 1), pl1 = (len1 + T - 1) & ~(T - 1);
+                const int ln = pl0 + pl1;
+                int snr = nr, sln = ln;
+#pragma unroll
+                for (int o = 1; o < T; o <<= 1) {
+                    const int y0 = __shfl_up(snr, o, T), y1 = __shfl_up(sln, o, T);
+                    if (tl >= o) {
+                        snr += y0;
+                        sln += y1;
+                    }
                 }
-                if (zr.n > 1) {
-                    run.z = cellStart[colBase + zr.lo1];
-                    run.w = cellStart[colBase + zr.hi1 + 1];
+                int e = nrun + snr - nr, v = total + sln - ln;
+                if (len0 > 0) {
+                    sOff[team][e] = st0 - v;
+                    sEnd[team][e] = (uint32_t)(v + len0) | (code0 << kEndBits);
+                    v += pl0;
+                    ++e;
                 }
-                const int ux = cx - cg.reach + a, uy = cy - cg.reach + b;
-                const int kx = ux < 0 ? -1 : (ux >= cg.nc[0] ? 1 : 0);
-                const int ky = uy < 0 ? -1 : (uy >= cg.nc[1] ? 1 : 0);
-                img = (kx + 1) | ((ky + 1) << 2) | ((zr.k0 + 1) << 4) | ((zr.k1 + 1) << 6);
+                if (len1 > 0) {
+                    sOff[team][e] = st1 - v;
+                    sEnd[team][e] = (uint32_t)(v + len1) | (code1 << kEndBits);
+                }
+                nrun += __shfl(snr, T - 1, T);
+                total += __shfl(sln, T - 1, T);
             }
-            sRuns[team][col] = run;
-            sImg[team][col] = img;
+            if (cls == 0) totalA = total;
         }
+        // stream positions are kEndBits wide
+        if (tl == 0 && total >= (1 << kEndBits)) atomicOr(&flags[7], 1);
     }
-    __syncthreads();
-    int cnt = 0, cntA = 0, cntB = 0;
-    if (live) {
+    // a team lies inside one wave and reads only its own table
+    wave_sync();
+    int cnt = 0;
+    if (live && total < (1 << kEndBits)) {
         const unsigned long long teamMask =
             (T == 64) ? ~0ull : (((1ull << T) - 1ull) << (lane & ~(T - 1)));
-        const unsigned long long pre = ((1ull << lane) - 1ull) & teamMask;
+        const uint32_t tmLo = (uint32_t)teamMask, tmHi = (uint32_t)(teamMask >> 32);
         const double4 pi = pos[i];
         const float4 xi = xq[i];
         const uint32_t *exRow = exclIdx + (size_t)i * maxExcl;
         const uint32_t ex0 = maxExcl > 0 ? exRow[0] : ~0u, ex1 = maxExcl > 1 ? exRow[1] : ~0u;
         const uint32_t ex2 = maxExcl > 2 ? exRow[2] : ~0u, ex3 = maxExcl > 3 ? exRow[3] : ~0u;
+        const uint32_t e0 = ex0 & 0x3FFFFFFFu, e1 = ex1 & 0x3FFFFFFFu, e2 = ex2 & 0x3FFFFFFFu, e3 = ex3 & 0x3FFFFFFFu;
         uint32_t *row = nl + (size_t)i * cap;
-        for (int r = 0; r < 2 * ncol; ++r) {
-            const int4 run = sRuns[team][r >> 1];
-            const int s0 = (r & 1) ? run.z : run.x;
-            const int s1 = (r & 1) ? run.w : run.y;
-            if (s0 >= s1) continue;
-            float4 xs = xi;  // the atom moved into the run's image frame
-            if (SHIFT) {
-                const int img = sImg[team][r >> 1];
-                const int kz = (r & 1) ? ((img >> 6) & 3) - 1 : ((img >> 4) & 3) - 1;
-                xs.x = xi.x - (float)((img & 3) - 1) * pf.L[0];
-                xs.y = xi.y - (float)(((img >> 2) & 3) - 1) * pf.L[1];
-                xs.z = xi.z - (float)kz * pf.L[2];
-            }
-            const bool runEx = maxExcl > 4 || in_run(ex0, s0, s1) || in_run(ex1, s0, s1) ||
-                               in_run(ex2, s0, s1) || in_run(ex3, s0, s1);
-            // blocks of U chunks: all U loads are issued (indices clamped into
-            // the run, so unconditionally) before the first use, one wait per block
-            for (int b0 = s0; b0 < s1; b0 += U * T) {
-                float4 blk[U];
+        int cntA = 0;
+        // the team's current run (team-uniform): offset, end of its real
+        // candidates, flag bit, and the atom moved into the run's image frame;
+        // the next entry is read ahead
+        int r = -1, off = 0, vend = 0;
+        uint32_t fl = 0, img = 0x15u;  // image code of the interior, (0, 0, 0) + 1 per axis
+        float xsx = xi.x, xsy = xi.y, xsz = xi.z;
+        int nOff = sOff[team][0];
+        uint32_t nEnd = sEnd[team][0];
+        for (int v0 = 0; v0 < total; v0 += U * T) {
+            uint32_t jv[U];  // atom index, bit 31: flagged run
+            float px[U], py[U], pz[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) blk[u] = xq[min(b0 + u * T + tl, s1 - 1)];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                const int j = b0 + u * T + tl;
-                const float4 cur = blk[u];
-                bool hit = false;
-                if (j < s1 && j != i) {
-                    float r2f;
-                    if (SHIFT) {
-                        const float dx = cur.x - xs.x, dy = cur.y - xs.y, dz = cur.z - xs.z;
-                        r2f = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    } else {
-                        r2f = dist2f(xi, cur, pf);
+            for (int u = 0; u < U; ++u) {
+                const int p = v0 + u * T;
+                if (p < total && p >= vend) {
+                    ++r;
+                    off = nOff;
+                    vend = (int)(nEnd & kEndMask);
+                    const uint32_t code = nEnd >> kEndBits;
+                    fl = (code >> 6) << 31;
+                    // most runs share the image of the previous one
+                    if (SHIFT && (code & 0x3Fu) != img) {
+                        img = code & 0x3Fu;
+                        xsx = xi.x - (float)((int)(code & 3u) - 1) * pf.L[0];
+                        xsy = xi.y - (float)((int)((code >> 2) & 3u) - 1) * pf.L[1];
+                        xsz = xi.z - (float)((int)((code >> 4) & 3u) - 1) * pf.L[2];
                     }
-                    hit = r2f < pf.lo2;
-                    if (!hit && r2f <= pf.hi2) hit = dist2(pi, pos[j], g) < rlist2;
+                    const int rn = min(r + 1, R - 1);
+                    nOff = sOff[team][rn];
+                    nEnd = sEnd[team][rn];
                 }
+                const int v = p + tl;
+                // padding and positions past the stream: the atom itself in a
+                // flagged slot (never a hit)
+                jv[u] = (p < total && v < vend) ? ((uint32_t)(v + off) | fl) : ((uint32_t)i | 0x80000000u);
+                px[u] = xsx;
+                py[u] = xsy;
+                pz[u] = xsz;
+            }
+            float4 blk[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) blk[u] = xq[jv[u] & 0x7FFFFFFFu];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (v0 + u * T == totalA) cntA = cnt;  // first charge-free chunk
+                const int j = (int)(jv[u] & 0x7FFFFFFFu);
+                const float4 cur = blk[u];
+                float r2f;
+                if (SHIFT) {
+                    const float dx = cur.x - px[u], dy = cur.y - py[u], dz = cur.z - pz[u];
+                    r2f = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                } else {
+                    r2f = dist2f(xi, cur, pf);
+                }
+                bool hit = r2f < pf.lo2;
+                if (!hit && r2f <= pf.hi2) hit = dist2(pi, pos[j], g) < rlist2;
                 uint32_t tag = 0;
-                if (runEx) {
+                if ((int)jv[u] < 0) {
+                    hit = hit && j != i;
                     const uint32_t uj = (uint32_t)j;
-                    tag = (ex0 & 0x3FFFFFFFu) == uj ? ex0 >> 30 : tag;
-                    tag = (ex1 & 0x3FFFFFFFu) == uj ? ex1 >> 30 : tag;
-                    tag = (ex2 & 0x3FFFFFFFu) == uj ? ex2 >> 30 : tag;
-                    tag = (ex3 & 0x3FFFFFFFu) == uj ? ex3 >> 30 : tag;
+                    tag = e0 == uj ? ex0 >> 30 : tag;
+                    tag = e1 == uj ? ex1 >> 30 : tag;
+                    tag = e2 == uj ? ex2 >> 30 : tag;
+                    tag = e3 == uj ? ex3 >> 30 : tag;
                     if (maxExcl > 4 && hit)
                         for (int q = 4; q < maxExcl; ++q) {
                             const uint32_t e = exRow[q];
                             if ((e & 0x3FFFFFFFu) == uj) tag = e >> 30;
                         }
                 }
-                // charged neighbours fill the row from the front, uncharged
-                // ones from the back (a pair kernel reads only what it needs)
-                const bool charged = cur.w != 0.0f;
-                const unsigned long long mA = __ballot(hit && charged), mB = __ballot(hit && !charged);
+                const unsigned long long m = __ballot(hit);
+                const uint32_t mLo = (uint32_t)m & tmLo, mHi = (uint32_t)(m >> 32) & tmHi;
                 if (hit) {
-                    const uint32_t e = (uint32_t)j | (tag << 30);
-                    if (charged) {
-                        const int k = cntA + __popcll(mA & pre);
-                        if (k < cap) row[k] = e;
-                    } else {
-                        const int k = cap - 1 - (cntB + __popcll(mB & pre));
-                        if (k >= 0) row[k] = e;
-                    }
+                    // team hits in lower lanes, counted by mbcnt on the masked ballot
+                    const int k = (int)__builtin_amdgcn_mbcnt_hi(mHi, __builtin_amdgcn_mbcnt_lo(mLo, (uint32_t)cnt));
+                    if (k < cap) row[k] = (uint32_t)j | (tag << 30);
                 }
-                cntA += __popcll(mA & teamMask);
-                cntB += __popcll(mB & teamMask);
-                }
+                cnt += __popc(mLo) + __popc(mHi);
             }
         }
-        cnt = cntA + cntB;
+        if (totalA >= total) cntA = cnt;  // no charge-free candidates
         if (tl == 0) {
             nlCount[i] = cnt;
             nlCountA[i] = cntA;
         }
     }
-    // one atomic per wave: same-address atomics serialise in L2
+    // one atomic per wave (same-address atomics serialise in L2)
     const int wmax = wave_max(cnt);
     if (lane == 0) atomicMax(&flags[1], wmax);
 }
@@ -614,18 +705,17 @@ __global__ void __launch_bounds__(256) k_e3b_list(
         const int oi = molAtoms[m].x;
         const double4 pi = pos[oi];
         const int total = min(nlCount[oi], cap);
-        const int nA = min(nlCountA[oi], total), nB = total - nA;
+        const int nA = min(nlCountA[oi], total);
         const uint32_t *row = nl + (size_t)oi * cap;
-        // entries: charged segment [0, nA), uncharged segment [cap - nB, cap);
-        // E3B oxygens of a charge-free O site need only the second
+        // row = [charged partners | charge-free partners]; E3B oxygens of a
+        // charge-free O site need only the second part
         const int first = unchargedOnly ? nA : 0;
         for (int base = first; base < total; base += T) {
             const int v = base + tl;
             bool hit = false;
             int mj = -1;
             if (v < total) {
-                const int k = v < nA ? v : cap - nB + (v - nA);
-                const int j = (int)(row[k] & 0x3FFFFFFFu);
+                const int j = (int)(row[v] & 0x3FFFFFFFu);
                 mj = atomMolO[j];
                 if (mj >= 0 && mj != m) hit = dist2(pi, pos[j], g) < rcut2;
             }
@@ -670,8 +760,8 @@ uint32_t morton2(uint32_t x, uint32_t y) {
 
 CellGrid make_grid(const Ctx &c) {
     CellGrid cg;
-    cg.reach = c.reach;
     for (int a = 0; a < 3; ++a) {
+        cg.reach[a] = c.reach[a];
         cg.nc[a] = c.nc[a];
         cg.cellInv[a] = (double)c.nc[a] / c.geo.L[a];
     }
@@ -713,7 +803,7 @@ Cull make_cull(const Ctx &c) {
     double lmax = 0.0;
     for (int a = 0; a < 3; ++a) {
         u.w[a] = (float)(c.geo.L[a] / c.nc[a]);
-        u.on[a] = c.geo.periodic[a] && c.nc[a] >= 2 * c.reach + 1;
+        u.on[a] = c.geo.periodic[a] && c.nc[a] >= 2 * c.reach[a] + 1;
         lmax = std::max(lmax, c.geo.L[a]);
     }
     const double r = c.rlist + 1e-5 * (lmax + c.rlist);
@@ -725,17 +815,17 @@ Cull make_cull(const Ctx &c) {
 
 template <int T>
 void launch_build(Ctx &c, const CellGrid &cg, int cur) {
-    MDX_REQUIRE(std::min(c.nc[0], 2 * c.reach + 1) * std::min(c.nc[1], 2 * c.reach + 1) <= kMaxCols, 1,
+    MDX_REQUIRE(std::min(c.nc[0], 2 * c.reach[0] + 1) * std::min(c.nc[1], 2 * c.reach[1] + 1) <= kMaxCols, 1,
                 "stencil wider than the build kernel's column table");
     int threads = 256;
     int blocks = cdiv((long long)c.n * T, threads);
     // image shifts need every periodic axis to hold the full stencil
     bool shift = true;
     for (int a = 0; a < 3; ++a)
-        if (c.geo.periodic[a] && c.nc[a] < 2 * c.reach + 1) shift = false;
+        if (c.geo.periodic[a] && c.nc[a] < 2 * c.reach[a] + 1) shift = false;
     auto kern = shift ? k_build<T, true> : k_build<T, false>;
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), 0, c.stream, rebuild_gate(c), c.n, c.pos[cur], c.xq, c.geo, cg,
-                       make_prefilter(c), make_cull(c), c.colRank, c.cellStart, c.rlist * c.rlist,
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), 0, c.stream, rebuild_gate(c), c.n, c.pos[cur], c.xq, c.geo,
+                       cg, make_prefilter(c), make_cull(c), c.colRank, c.cellStart, c.ncell, c.rlist * c.rlist,
                        c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.nlCountA, c.dFlags);
 }
 
@@ -759,7 +849,7 @@ void nlist_setup(Ctx &c) {
         if (c.geo.periodic[a])
             MDX_REQUIRE(c.rlist < 0.5 * c.geo.L[a], 1,
                         "neighbor cutoff (rCut + padding) must be below half the box length");
-        double edge = c.rlist / c.reach * (1.0 + 1e-9);
+        double edge = c.rlist / c.reach[a] * (1.0 + 1e-9);
         int nca = (int)std::floor(c.geo.L[a] / edge);
         c.nc[a] = std::max(1, nca);
     }
@@ -775,9 +865,10 @@ void nlist_setup(Ctx &c) {
         c.ncell = ncell;
         c.colRank = dalloc<int>(c, (size_t)c.nc[0] * c.nc[1]);
         c.colXY = dalloc<int2>(c, (size_t)c.nc[0] * c.nc[1]);
-        c.cellCount = dalloc<int>(c, ncell);
-        c.cellStart = dalloc<int>(c, ncell + 1);
-        c.cellFill = dalloc<int>(c, ncell);
+        // two copies of the grid: charged atoms, then charge-free ones
+        c.cellCount = dalloc<int>(c, 2 * (size_t)ncell);
+        c.cellStart = dalloc<int>(c, 2 * (size_t)ncell + 1);
+        c.cellFill = dalloc<int>(c, 2 * (size_t)ncell);
     }
     // column ranks along a 2-D Morton curve
     std::vector<std::pair<uint32_t, int>> keys;
@@ -820,6 +911,12 @@ void refresh_xq(Ctx &c) {
                        c.q[c.cur], c.xq, c.geo);
 }
 
+void prepare_forces(Ctx &c) {
+    const int threads = 256;
+    hipLaunchKernelGGL(k_xq_zero, dim3(cdiv(std::max(c.n, (int)ACC_N), threads)), dim3(threads), 0, c.stream, c.n,
+                       c.pos[c.cur], c.q[c.cur], c.xq, c.geo, c.force, (int)ACC_N, c.acc);
+}
+
 void nlist_check(Ctx &c) {
     int threads = 256;
     double thr = 0.5 * c.skin;
@@ -831,16 +928,17 @@ static void run_build(Ctx &c, bool sync) {
     CellGrid cg = make_grid(c);
     int threads = 256;
     int cur = c.cur, nxt = 1 - c.cur;
-    hipLaunchKernelGGL(k_fill_int, dim3(cdiv(c.ncell, threads)), dim3(threads), 0, c.stream, rebuild_gate(c), c.ncell,
+    const int nslot = 2 * c.ncell;
+    hipLaunchKernelGGL(k_fill_int, dim3(cdiv(nslot, threads)), dim3(threads), 0, c.stream, rebuild_gate(c), nslot,
                        c.cellCount, 0);
     hipLaunchKernelGGL(k_bin, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, rebuild_gate(c), c.n, c.pos[cur],
-                       c.geo, cg, c.colRank, c.cellOf, c.cellCount);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c.stream, rebuild_gate(c), c.ncell, c.cellCount, c.cellStart,
+                       c.q[cur], c.geo, cg, c.ncell, c.colRank, c.cellOf, c.cellCount);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c.stream, rebuild_gate(c), nslot, c.cellCount, c.cellStart,
                        c.cellFill);
     hipLaunchKernelGGL(k_place, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, rebuild_gate(c), c.n, c.cellOf,
                        c.cellFill, c.perm);
-    hipLaunchKernelGGL(k_cellsort, dim3(c.ncell), dim3(64), 0, c.stream, rebuild_gate(c), c.ncell, c.cellStart, c.perm,
-                       c.perm2, c.id[cur], c.q[cur]);
+    hipLaunchKernelGGL(k_cellsort, dim3(nslot), dim3(64), 0, c.stream, rebuild_gate(c), nslot, c.cellStart, c.perm,
+                       c.perm2, c.id[cur]);
     hipLaunchKernelGGL(k_permute, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, rebuild_gate(c), c.n, c.perm2,
                        c.pos[cur], c.vel[cur], c.q[cur], c.type[cur], c.id[cur], c.pos[nxt],
                        c.vel[nxt], c.q[nxt], c.type[nxt], c.id[nxt], c.idToIdx, c.posLast, c.xq,
@@ -896,6 +994,7 @@ static void run_build(Ctx &c, bool sync) {
         // largest row counts -> pinned host memory; checked now (sync) or at
         // the next synchronisation point (capacity_ok)
         MDX_HIP(hipMemcpyAsync(c.pinned, c.dFlags + 1, sizeof(int) * 2, hipMemcpyDeviceToHost, c.stream));
+        MDX_HIP(hipMemcpyAsync(c.pinned + 6, c.dFlags + 7, sizeof(int), hipMemcpyDeviceToHost, c.stream));
         if (!sync) {
             MDX_HIP(hipEventRecord(c.capEvent, c.stream));
             c.capPending = true;
@@ -913,6 +1012,8 @@ static void run_build(Ctx &c, bool sync) {
 // returns true when something grew (the caller rebuilds).  The reference
 // reallocates its list at 1.5x (GridGPU.cu:1026-1030).
 bool grow_if_needed(Ctx &c) {
+    MDX_REQUIRE(c.pinned[6] == 0, 1,
+                "an atom's stencil holds over 2^20 candidates: lower the list cutoff or the cell reach");
     bool grew = false;
     if (c.pinned[0] > c.cap) {
         dfree(c, c.nl);
@@ -966,28 +1067,42 @@ namespace {
 // into one slot of dMax as float bits (non-negative floats order like their
 // bit patterns); the other slot is cleared for the next step.  The host reads
 // the slot one step later.
-__global__ void k_check_max(int n, const double4 *__restrict__ pos, const double4 *__restrict__ last,
-                            const double4 *__restrict__ vel, Geometry g, double dt2,
-                            unsigned *__restrict__ slot, unsigned *__restrict__ other) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
+// Grid-stride over the sites with a workgroup reduction: one atomic pair per
+// workgroup (same-address atomics serialise in L2).
+__global__ void __launch_bounds__(256) k_check_max(int n, const double4 *__restrict__ pos,
+                                                   const double4 *__restrict__ last,
+                                                   const double4 *__restrict__ vel, Geometry g, double dt2,
+                                                   unsigned *__restrict__ slot, unsigned *__restrict__ other) {
+    __shared__ float red[2][4];
+    const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t0 == 0) {
         other[0] = 0u;
         other[1] = 0u;
     }
     float d2 = 0.f, p2 = 0.f;
-    if (i < n) {
+    for (int i = t0; i < n; i += gridDim.x * blockDim.x) {
         const double dd = dist2(pos[i], last[i], g);
         const double4 v = vel[i];
         const double p = sqrt(dd) + dt2 * sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
-        d2 = (float)dd;
-        p2 = (float)(p * p);
+        d2 = fmaxf(d2, (float)dd);
+        p2 = fmaxf(p2, (float)(p * p));
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         d2 = fmaxf(d2, __shfl_xor(d2, o, 64));
         p2 = fmaxf(p2, __shfl_xor(p2, o, 64));
     }
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
+        red[0][w] = d2;
+        red[1][w] = p2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+            d2 = fmaxf(d2, red[0][k]);
+            p2 = fmaxf(p2, red[1][k]);
+        }
         atomicMax(&slot[0], __float_as_uint(d2));
         atomicMax(&slot[1], __float_as_uint(p2));
     }
@@ -1002,8 +1117,9 @@ void decide_step(Ctx &c) {
 }
 
 void check_max(Ctx &c, int parity) {
-    int threads = 256;
-    hipLaunchKernelGGL(k_check_max, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n,
+    const int threads = 256;
+    const int blocks = std::min(cdiv(c.n, threads * 4), 128);
+    hipLaunchKernelGGL(k_check_max, dim3(blocks), dim3(threads), 0, c.stream, c.n,
                        c.pos[c.cur], c.posLast, c.vel[c.cur], c.geo, 2.0 * c.dt, c.dMax + 2 * parity,
                        c.dMax + 2 * (1 - parity));
 }

@@ -63,16 +63,16 @@ __global__ void __launch_bounds__(256) k_pair(PairArgs a) {
         const int ti = (LJ && a.ntypes > 1) ? a.type[i] : 0;
         const int fl = LJ ? sfl[ti] : 0;
         const int cnt = min(a.nlCount[i], a.cap);
-        const int nA = min(a.nlCountA[i], cnt), nB = cnt - nA;
+        const int nA = min(a.nlCountA[i], cnt);
         const uint32_t *row = a.nl + (size_t)i * a.cap;
         const float twoAlphaRtPi = 2.0f * 0.5641895835477563f * a.alpha;
         const float alpha2 = a.alpha * a.alpha;
-        // part 0: charged neighbors [0, nA); part 1: uncharged [cap - nB, cap)
+        // part 0: charged neighbors [0, nA); part 1: charge-free ones [nA, cnt)
         for (int part = 0; part < 2; ++part) {
             const bool lj = LJ && (fl & (1 << part));
             const bool coul = COUL && part == 0 && pi.w != 0.0f;
             if (!lj && !coul) continue;
-            const int k0 = part == 0 ? 0 : a.cap - nB, k1 = part == 0 ? nA : a.cap;
+            const int k0 = part == 0 ? 0 : nA, k1 = part == 0 ? nA : cnt;
             for (int k = k0 + tl; k < k1; k += P) {
                 const uint32_t raw = row[k];
                 const int j = (int)(raw & 0x3FFFFFFFu);

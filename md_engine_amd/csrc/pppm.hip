@@ -327,13 +327,30 @@ __global__ void __launch_bounds__(256) k_kspace_ev(MeshGeo m, int szh, double al
     }
 }
 
+// weight of stencil index k (0 <= k < ORDER) without a dynamically indexed
+// local array (which would live in scratch memory)
 template <int ORDER>
+__device__ __forceinline__ float pick(const float (&w)[ORDER], int k) {
+    float r = w[0];
+#pragma unroll
+    for (int t = 1; t < ORDER; ++t) r = k == t ? w[t] : r;
+    return r;
+}
+
+// Field interpolation (Ewald_long_range_forces_order_*_cu): a team of T
+// lanes per charged atom splits the ORDER^2 stencil rows (the field's three
+// components are consecutive real meshes, as the batched inverse FFT writes
+// them), and the team sums by butterfly shuffles before one fixed-point add
+// per component.
+template <int ORDER, int T>
 __global__ void __launch_bounds__(256) k_gather(int n, const double4 *__restrict__ pos,
                                                 const float *__restrict__ q, MeshGeo m, float scale,
                                                 const float *__restrict__ E,
                                                 unsigned long long *__restrict__ force) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = gt / T;
+    const int tl = threadIdx.x & (T - 1);
+    if (i >= n) return;  // whole teams leave together
     const float qi = q[i];
     if (qi == 0.0f) return;
     const double4 p = pos[i];
@@ -349,29 +366,38 @@ __global__ void __launch_bounds__(256) k_gather(int n, const double4 *__restrict
     constexpr int H = (ORDER - 1) / 2;
     const size_t ng = (size_t)m.sz[0] * m.sz[1] * m.sz[2];
     float ex = 0.f, ey = 0.f, ez = 0.f;
-#pragma unroll
-    for (int ix = 0; ix < ORDER; ++ix) {
+    for (int r = tl; r < ORDER * ORDER; r += T) {
+        const int ix = r / ORDER, iy = r - ix * ORDER;
         const int px = wrapi(g[0] + ix - H, m.sz[0]);
+        const int py = wrapi(g[1] + iy - H, m.sz[1]);
+        const float wxy = pick<ORDER>(wx, ix) * pick<ORDER>(wy, iy);
+        const size_t rowb = ((size_t)px * m.sz[1] + py) * m.sz[2];
+        float e0[ORDER], e1[ORDER], e2[ORDER];
 #pragma unroll
-        for (int iy = 0; iy < ORDER; ++iy) {
-            const int py = wrapi(g[1] + iy - H, m.sz[1]);
-            const float wxy = wx[ix] * wy[iy];
-            const size_t rowb = ((size_t)px * m.sz[1] + py) * m.sz[2];
+        for (int iz = 0; iz < ORDER; ++iz) {
+            const size_t gidx = rowb + wrapi(g[2] + iz - H, m.sz[2]);
+            e0[iz] = E[gidx];
+            e1[iz] = E[ng + gidx];
+            e2[iz] = E[2 * ng + gidx];
+        }
 #pragma unroll
-            for (int iz = 0; iz < ORDER; ++iz) {
-                const size_t gidx = rowb + wrapi(g[2] + iz - H, m.sz[2]);
-                const float w = wxy * wz[iz];
-                ex += w * E[gidx];
-                ey += w * E[ng + gidx];
-                ez += w * E[2 * ng + gidx];
-            }
+        for (int iz = 0; iz < ORDER; ++iz) {
+            const float w = wxy * wz[iz];
+            ex += w * e0[iz];
+            ey += w * e1[iz];
+            ez += w * e2[iz];
         }
     }
-    // F = Qunit q sum W (-E / V)  (Ewald_long_range_forces_order_5_cu)
-    const float s = scale * qi;
-    fx_add(&force[3 * i + 0], s * ex);
-    fx_add(&force[3 * i + 1], s * ey);
-    fx_add(&force[3 * i + 2], s * ez);
+    ex = team_sum<T>(ex);
+    ey = team_sum<T>(ey);
+    ez = team_sum<T>(ez);
+    if (tl == 0) {
+        // F = Qunit q sum W (-E / V)  (Ewald_long_range_forces_order_5_cu)
+        const float s = scale * qi;
+        fx_add(&force[3 * i + 0], s * ex);
+        fx_add(&force[3 * i + 1], s * ey);
+        fx_add(&force[3 * i + 2], s * ez);
+    }
 }
 
 MeshGeo make_mesh(const Ctx &c) {
@@ -578,13 +604,22 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
     }
     const double V = c.geo.L[0] * c.geo.L[1] * c.geo.L[2];
     const float scale = (float)(-std::sqrt(P.qqr) / V);
+    // a built list puts the charged atoms first: only they take the gather
+    const int na = c.listValid ? c.nCharged : c.n;
     ProfScope ps(c, PK_GATHER);
+    if (na == 0) return;
+    const dim3 grid(cdiv((long long)na * P.gatherTeam, threads));
+#define MDX_GATHER(O, T)                                                                                   \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gather<O, T>), grid, dim3(threads), 0, c.stream, na, c.pos[cur], \
+                       c.q[cur], m, scale, P.eReal, c.force)
+    const bool wide = P.gatherTeam == 16;
     switch (P.order) {
-        case 1: hipLaunchKernelGGL(k_gather<1>, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.q[cur], m, scale, P.eReal, c.force); break;
-        case 3: hipLaunchKernelGGL(k_gather<3>, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.q[cur], m, scale, P.eReal, c.force); break;
-        case 5: hipLaunchKernelGGL(k_gather<5>, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.q[cur], m, scale, P.eReal, c.force); break;
-        default: hipLaunchKernelGGL(k_gather<7>, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.q[cur], m, scale, P.eReal, c.force); break;
+        case 1: if (wide) MDX_GATHER(1, 16); else MDX_GATHER(1, 8); break;
+        case 3: if (wide) MDX_GATHER(3, 16); else MDX_GATHER(3, 8); break;
+        case 5: if (wide) MDX_GATHER(5, 16); else MDX_GATHER(5, 8); break;
+        default: if (wide) MDX_GATHER(7, 16); else MDX_GATHER(7, 8); break;
     }
+#undef MDX_GATHER
 }
 
 }  // namespace mdx

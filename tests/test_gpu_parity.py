@@ -357,7 +357,7 @@ def test_tune_keeps_lists_and_forces(orc, gpu_engine_factory):
     e.compute(PAIR | PPPM | E3B)
     choice = e.tune(PAIR | PPPM | E3B, evals=3)
     assert choice["build_mode"] == 2 and choice["build_team"] in (8, 16, 32, 64)
-    assert choice["pair_team"] in (4, 8, 16, 32) and choice["e3b_team"] in (4, 8, 16)
+    assert choice["pair_team"] in (4, 8, 16, 32) and choice["e3b_team"] in (4, 8, 16, 32)
     assert choice["build_ms"] > 0 and choice["pair_ms"] > 0 and choice["e3b_ms"] > 0
     gc, grows = e.nlist_get()
     excl = orc.exclusions_bfs(len(box["x"]), systems.water_bonds(box["mol"]))
