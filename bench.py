@@ -364,21 +364,24 @@ def main():
         timing_pass = "eager pass of %d steps right after the timed graph run" % nprof
         rebuilds0 += e.nlist_stats()["rebuilds"] - st0["rebuilds"]
     pairs, mols = e.nlist_totals()
+    read_entries, read_atoms = e.pair_entries_read()
     st = e.nlist_stats()
     rmse = three_body_rmse(e) if surrogate else None
 
     ms_step = wall / args.steps * 1e3
     value = n * args.steps / wall
-    # dominant kernel: pair (LJ + Ewald real space).  Algorithmic bytes per
-    # launch: per atom xq 16 B + type 4 B + row count 4 B + force 24 B, plus
-    # 4 B per neighbor-list entry.
+    # dominant per-step kernel: pair (LJ + Ewald real space).  Algorithmic
+    # bytes per launch: per atom xq 16 B + type 4 B + two row counts 8 B, per
+    # atom with work a fixed-point force row 24 B, plus 4 B per list entry the
+    # kernel reads (charged partners for charged atoms, charge-free partners for
+    # atoms with LJ to them; the other part of a row is never touched).
     cnt, tot_ms = prof["pair"]
     avg_s = tot_ms / max(cnt, 1) * 1e-3
-    bytes_launch = n * (16 + 4 + 4 + 24) + 4 * pairs
+    bytes_launch = n * (16 + 4 + 8) + 24 * read_atoms + 4 * read_entries
     achieved = bytes_launch / avg_s / 1e9
     kernel_ms = {k: (v[1] / max(v[0], 1)) for k, v in prof.items() if v[0]}
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "c3_pair_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "pair_traffic_tip4p_e3b_8192.json")
     if args.config == "tip4p-e3b" and os.path.exists(tpath):
         with open(tpath) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
@@ -409,7 +412,8 @@ def main():
             "list_rebuilds_timed": e.nlist_stats()["rebuilds"] - rebuilds0,
             "list_violations": e.nlist_stats()["violations"],
             "rc": w["rc"], "skin": w["skin"], "pppm_mesh": w["sz"], "pppm_order": w["order"],
-            "e3b": w["e3b"], "sites": n, "pair_list_entries": pairs, "e3b_list_entries": mols,
+            "e3b": w["e3b"], "sites": n, "pair_list_entries": pairs, "pair_entries_read": read_entries,
+            "e3b_list_entries": mols,
             "list_rebuilds_in_run": st["rebuilds"],
             "ns_per_day_at_dt": (w["dt_fs"] * 1e-6 * args.steps / wall * 86400.0) if w["dt_fs"] else None,
             "kernel_avg_ms": kernel_ms,
@@ -423,7 +427,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/c3_force_path_tip4p_e3b_8192.md"
+            "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/force_path_tip4p_e3b_8192.md"
             if traffic else None,
             "bytes_per_launch": bytes_launch,
             "avg_launch_ms": avg_s * 1e3,

@@ -234,6 +234,11 @@ int mdx_profile_enable(mdx_ctx *ctx, int enable);
 int mdx_profile_get(mdx_ctx *ctx, int kernel, int *count, double *total_ms);
 /* Entries in the current pair list and E3B molecule list (sum of counts). */
 int mdx_nlist_totals(mdx_ctx *ctx, int64_t *pair_entries, int64_t *mol_entries);
+/* Work of the pair kernel on the current list: row entries it reads (charged
+ * partners of charged atoms and of atoms with LJ to charged types, charge-free
+ * partners of atoms with LJ to charge-free types) and atoms that read any.
+ * No counterpart in the reference; the roofline's algorithmic bytes. */
+int mdx_pair_entries_read(mdx_ctx *ctx, int64_t *entries, int64_t *atoms);
 
 #ifdef __cplusplus
 }

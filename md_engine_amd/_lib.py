@@ -24,7 +24,7 @@ SYMBOLS = [
     "mdx_nlist_stats", "mdx_lj_set", "mdx_special_set", "mdx_ewald_real_set", "mdx_pppm_set",
     "mdx_pppm_info", "mdx_e3b_set", "mdx_forces_compute", "mdx_forces_get", "mdx_energy_get",
     "mdx_integrator_set", "mdx_run", "mdx_teams_set", "mdx_sync", "mdx_force_steps",
-    "mdx_profile_enable", "mdx_profile_get", "mdx_nlist_totals", "mdx_option_set", "mdx_rigid_set",
+    "mdx_profile_enable", "mdx_profile_get", "mdx_nlist_totals", "mdx_pair_entries_read", "mdx_option_set", "mdx_rigid_set",
     "mdx_thermostat_set", "mdx_temperature", "mdx_tune", "mdx_run_batch", "mdx_surrogate_set",
     "mdx_surrogate_get", "mdx_surrogate_record", "mdx_surrogate_fit", "mdx_surrogate_eval",
     "mdx_e3b_molecule_energies", "mdx_nose_hoover_set", "mdx_nose_hoover_energy", "mdx_barostat_set",
@@ -97,6 +97,7 @@ def load():
     L.mdx_profile_enable.argtypes = [_vp, c]
     L.mdx_profile_get.argtypes = [_vp, c, C.POINTER(c), C.POINTER(d)]
     L.mdx_nlist_totals.argtypes = [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.mdx_pair_entries_read.argtypes = [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.mdx_tune.argtypes = [_vp, c, c, _ip, _dp]
     L.mdx_run_batch.argtypes = [C.POINTER(_vp), c, c, c]
     _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
@@ -395,6 +396,12 @@ class Engine:
         a = C.c_int64(0)
         b = C.c_int64(0)
         self._chk(self._L.mdx_nlist_totals(self._h, C.byref(a), C.byref(b)), "mdx_nlist_totals")
+        return a.value, b.value
+
+    def pair_entries_read(self):
+        """(row entries the pair kernel reads, atoms that read any) on the current list."""
+        a, b = C.c_int64(0), C.c_int64(0)
+        self._chk(self._L.mdx_pair_entries_read(self._h, C.byref(a), C.byref(b)), "mdx_pair_entries_read")
         return a.value, b.value
 
 

@@ -1354,6 +1354,33 @@ int mdx_profile_get(mdx_ctx *ctx, int kernel, int *count, double *total_ms) {
     });
 }
 
+int mdx_pair_entries_read(mdx_ctx *ctx, int64_t *entries, int64_t *atoms) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        MDX_REQUIRE(c.listValid, MDX_ERR_STATE, "no neighbor list built");
+        const int n = c.n;
+        std::vector<int> cnt(n), cntA(n), type(n);
+        std::vector<float> q(n);
+        MDX_HIP(hipMemcpyAsync(cnt.data(), c.nlCount, sizeof(int) * n, hipMemcpyDeviceToHost, c.stream));
+        MDX_HIP(hipMemcpyAsync(cntA.data(), c.nlCountA, sizeof(int) * n, hipMemcpyDeviceToHost, c.stream));
+        MDX_HIP(hipMemcpyAsync(type.data(), c.type[c.cur], sizeof(int) * n, hipMemcpyDeviceToHost, c.stream));
+        MDX_HIP(hipMemcpyAsync(q.data(), c.q[c.cur], sizeof(float) * n, hipMemcpyDeviceToHost, c.stream));
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        const std::vector<int> fl = pair_type_flag_table(c);
+        const bool coul = c.ewaldReal && c.hasCharges;
+        int64_t e = 0, a = 0;
+        for (int i = 0; i < n; ++i) {
+            const int tot = std::min(cnt[i], c.cap), na = std::min(cntA[i], tot);
+            const int f = c.ljOn ? fl[type[i]] : 0;
+            const bool p0 = (f & 1) || (coul && q[i] != 0.0f), p1 = (f & 2) != 0;
+            e += (p0 ? na : 0) + (p1 ? tot - na : 0);
+            a += (p0 || p1) ? 1 : 0;
+        }
+        if (entries) *entries = e;
+        if (atoms) *atoms = a;
+    });
+}
+
 int mdx_nlist_totals(mdx_ctx *ctx, int64_t *pair_entries, int64_t *mol_entries) {
     return guarded(ctx, [&](Ctx &c) {
         require_system(c);

@@ -84,7 +84,8 @@ struct E3BState {
     int *nl = nullptr;        // nmol * cap molecule indices
     int *nlCount = nullptr;
     float4 *tot = nullptr;    // per molecule totals (x accepted, y H1 donated, z H2 donated)
-    int team = 32;            // lanes per molecule (E3B passes and molecule list)
+    int team = 32;            // lanes per molecule in the E3B passes
+    int listTeam = 8;         // lanes per molecule building the molecule list
     bool oxUncharged = false;  // every E3B oxygen site has q = 0 (its neighbors need no charged scan)
     std::vector<int> oxIds;    // by molecule: oxygen atom id
 };
@@ -363,6 +364,9 @@ void decide_step(Ctx &c);             // captured step: device rebuild / thermos
 
 // pair.hip
 void pair_forces(Ctx &c, bool energy, bool virial);
+// per-type bits of the pair kernel: 1 = LJ with charged atoms, 2 = LJ with
+// charge-free atoms (host copy of what the kernel reads)
+std::vector<int> pair_type_flag_table(const Ctx &c);
 
 // pppm.hip
 void pppm_prepare(Ctx &c);            // alpha + Green function for the current box

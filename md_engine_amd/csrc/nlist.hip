@@ -98,46 +98,43 @@ __global__ void k_bin(const int *__restrict__ gate, int n, double4 *__restrict__
 }
 
 // Exclusive scan of the cell counts by one workgroup (replaces the host
-// cumulativeSum round trip, helpers.h:15-24).  Also seeds the fill cursors.
+// cumulativeSum round trip, helpers.h:15-24).  Each thread sums a contiguous
+// segment, the segment totals are scanned across the workgroup, and each
+// thread then writes its segment's prefixes.  Also seeds the fill cursors.
 __global__ void __launch_bounds__(1024) k_scan(const int *__restrict__ gate, int ncell, const int *__restrict__ count,
                                                int *__restrict__ start, int *__restrict__ fill) {
     if (gate && *gate == 0) return;
     __shared__ int wsum[16];
-    __shared__ int carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int base = 0; base < ncell; base += 1024) {
-        int idx = base + threadIdx.x;
-        int v = idx < ncell ? count[idx] : 0;
-        int x = v;
+    const int seg = (ncell + 1023) / 1024;
+    const int b0 = threadIdx.x * seg, b1 = min(b0 + seg, ncell);
+    int local = 0;
+    for (int k = b0; k < b1; ++k) local += count[k];
+    int x = local;
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            int y = __shfl_up(x, o, 64);
-            if (lane >= o) x += y;
-        }
-        if (lane == 63) wsum[wid] = x;
-        __syncthreads();
-        if (wid == 0) {
-            int s = lane < 16 ? wsum[lane] : 0;
-#pragma unroll
-            for (int o = 1; o < 16; o <<= 1) {
-                int y = __shfl_up(s, o, 64);
-                if (lane >= o) s += y;
-            }
-            if (lane < 16) wsum[lane] = s;
-        }
-        __syncthreads();
-        int excl = carry + (wid > 0 ? wsum[wid - 1] : 0) + x - v;
-        if (idx < ncell) {
-            start[idx] = excl;
-            fill[idx] = excl;
-        }
-        __syncthreads();
-        if (threadIdx.x == 1023) carry = excl + v;
-        __syncthreads();
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
     }
-    if (threadIdx.x == 0) start[ncell] = carry;
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    if (wid == 0) {
+        int t = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            const int y = __shfl_up(t, o, 64);
+            if (lane >= o) t += y;
+        }
+        if (lane < 16) wsum[lane] = t;
+    }
+    __syncthreads();
+    int run = (wid > 0 ? wsum[wid - 1] : 0) + x - local;
+    for (int k = b0; k < b1; ++k) {
+        start[k] = run;
+        fill[k] = run;
+        run += count[k];
+    }
+    if (threadIdx.x == 1023) start[ncell] = wsum[15];
 }
 
 __global__ void k_place(const int *__restrict__ gate, int n, const int *__restrict__ cellOf, int *__restrict__ fill,
@@ -984,7 +981,7 @@ static void run_build(Ctx &c, bool sync) {
             }
         }
         if (c.e3b.on) {
-            switch (c.e3b.team) {
+            switch (c.e3b.listTeam) {
                 case 8: launch_e3b_list<8>(c, c.cur); break;
                 case 16: launch_e3b_list<16>(c, c.cur); break;
                 default: launch_e3b_list<32>(c, c.cur); break;

@@ -183,8 +183,7 @@ void launch2(const PairArgs &a, bool lj, bool coul, bool ev, hipStream_t s) {
 // Per-type flags of the LJ table against the charge classes present: bit 0 =
 // nonzero epsilon with some type that has charged atoms, bit 1 = with some type
 // that has uncharged atoms.  Recomputed when the system or the table changes.
-static void pair_type_flags(Ctx &c) {
-    if (!c.pairTypeFlags) c.pairTypeFlags = dalloc<int>(c, kMaxTypes);
+std::vector<int> pair_type_flag_table(const Ctx &c) {
     std::vector<int> charged(kMaxTypes, 0), neutral(kMaxTypes, 0), fl(kMaxTypes, 0);
     for (size_t i = 0; i < c.typeHost.size(); ++i) (c.qHost[i] != 0.0f ? charged : neutral)[c.typeHost[i]] = 1;
     const int T = c.ntypes;
@@ -194,6 +193,12 @@ static void pair_type_flags(Ctx &c) {
             if (charged[b]) fl[a] |= 1;
             if (neutral[b]) fl[a] |= 2;
         }
+    return fl;
+}
+
+static void pair_type_flags(Ctx &c) {
+    if (!c.pairTypeFlags) c.pairTypeFlags = dalloc<int>(c, kMaxTypes);
+    const std::vector<int> fl = pair_type_flag_table(c);
     MDX_REQUIRE(!c.capture, 3, "pair type flags must be set before a step is captured");
     MDX_HIP(hipMemcpyAsync(c.pairTypeFlags, fl.data(), sizeof(int) * kMaxTypes, hipMemcpyHostToDevice, c.stream));
     MDX_HIP(hipStreamSynchronize(c.stream));
