@@ -291,6 +291,8 @@ def main():
     ap.add_argument("--reach-xy", type=int, default=0, help="cell stencil reach in x/y (0: library default)")
     ap.add_argument("--reach-z", type=int, default=0, help="cell stencil reach in z (0: library default)")
     ap.add_argument("--gather-team", type=int, default=0, help="lanes per charged atom in the P3M gather")
+    ap.add_argument("--barostat", type=int, default=0,
+                    help="md mode, water: 1 atm Berendsen barostat applied every N steps (0: off)")
     ap.add_argument("--tune", action="store_true", help="run the autotuner before timing")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of known size (counter calibration)")
@@ -329,6 +331,11 @@ def main():
         setup_md(e, w)
         if args.graph:
             e.option("graph", 1)
+        if args.barostat:
+            from md_engine_amd import systems as _sy
+
+            # FixPressureBerendsen(state, handle, 1 atm, period 1 ps, applyEvery)
+            e.barostat(True, 1.0, 1000.0, args.barostat, mvv_to_eng=_sy.REAL_MVV_TO_ENG)
         step = lambda k: e.run(k, flags)  # noqa: E731
         if args.equil:
             step(args.equil)

@@ -118,12 +118,11 @@ constexpr int kForceTerms = MDX_PAIR | MDX_PPPM | MDX_E3B | MDX_SURROGATE;
 // trace, corr the molecular correction (rigid molecules move as centres of
 // mass; free sites count as their own molecules).  Synchronises.
 double pressure_now(Ctx &c) {
-    fill_int(c, (int *)(c.acc + ACC_MOL), 4, 0);
+    acc_clear(c, ACC_MOL, 2);
     molecular_virial(c, ACC_MOL);
-    unsigned long long h[ACC_N];
-    MDX_HIP(hipMemcpyAsync(h, c.acc, sizeof(h), hipMemcpyDeviceToHost, c.stream));
-    MDX_HIP(hipStreamSynchronize(c.stream));
-    auto v = [&](int k) { return (double)(long long)h[k] * kFxInv; };
+    double h[ACC_N];
+    acc_read(c, h);
+    auto v = [&](int k) { return h[k]; };
     const double W = v(ACC_VIR) + v(ACC_VIR + 1) + v(ACC_VIR + 2);
     const double vol = c.geo.L[0] * c.geo.L[1] * c.geo.L[2];
     return (v(ACC_MOL + 1) * c.baro.mvv + W - v(ACC_MOL)) / (3.0 * vol) * c.baro.nktv2p;
@@ -435,7 +434,7 @@ int mdx_create(int device, mdx_ctx **out) {
         return MDX_ERR_HIP;
     }
     int rc = guarded(ctx, [](Ctx &c) {
-        c.acc = dalloc<unsigned long long>(c, ACC_N);
+        c.acc = dalloc<unsigned long long>(c, kAccTotal);
         c.dFlags = dalloc<int>(c, 8);
         c.dMax = dalloc<unsigned>(c, 4);
         c.thermo = dalloc<unsigned long long>(c, 2);
@@ -445,7 +444,7 @@ int mdx_create(int device, mdx_ctx **out) {
         for (int k = 0; k < 2; ++k) MDX_HIP(hipEventCreateWithFlags(&c.stepEvent[k], hipEventDisableTiming));
         MDX_HIP(hipMemsetAsync(c.dMax, 0, sizeof(unsigned) * 4, c.stream));
         MDX_HIP(hipMemsetAsync(c.dFlags, 0, sizeof(int) * 8, c.stream));
-        MDX_HIP(hipMemsetAsync(c.acc, 0, sizeof(unsigned long long) * ACC_N, c.stream));
+        MDX_HIP(hipMemsetAsync(c.acc, 0, sizeof(unsigned long long) * kAccTotal, c.stream));
     });
     if (rc != MDX_OK) {
         mdx_destroy(ctx);
@@ -971,10 +970,9 @@ int mdx_forces_get(mdx_ctx *ctx, double *f) {
 
 int mdx_energy_get(mdx_ctx *ctx, double e[4], double virial[6]) {
     return guarded(ctx, [&](Ctx &c) {
-        unsigned long long h[ACC_N];
-        MDX_HIP(hipMemcpyAsync(h, c.acc, sizeof(h), hipMemcpyDeviceToHost, c.stream));
-        MDX_HIP(hipStreamSynchronize(c.stream));
-        auto d = [&](int s) { return (double)(long long)h[s] * kFxInv; };
+        double h[ACC_N];
+        acc_read(c, h);
+        auto d = [&](int s) { return h[s]; };
         if (e) {
             e[0] = d(ACC_PAIR);
             e[1] = d(ACC_KSPACE);

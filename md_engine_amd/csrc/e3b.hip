@@ -81,7 +81,7 @@ template <int P>
 __device__ __forceinline__ void block_acc(double v, int slot, unsigned long long *acc) {
     // one wave64 reduction, then lane 0 adds (few hundred atomics per launch)
     double x = team_sum_d<64>(v);
-    if ((threadIdx.x & 63) == 0 && x != 0.0) fx_add(&acc[slot], x);
+    if ((threadIdx.x & 63) == 0 && x != 0.0) fx_add(&acc_rep(acc)[slot], x);
 }
 
 template <int P, bool EV>

@@ -10,6 +10,12 @@ namespace mdx {
 
 namespace {
 
+// clears slots [slot, slot + count) of every accumulator replica
+__global__ void k_acc_clear(unsigned long long *__restrict__ acc, int slot, int count) {
+    const int r = threadIdx.x / ACC_N, k = threadIdx.x % ACC_N;
+    if (r < kAccRep && k >= slot && k < slot + count) acc[r * ACC_N + k] = 0ull;
+}
+
 __global__ void k_zero_u64(const int *__restrict__ gate, int n, unsigned long long *__restrict__ p) {
     if (gate && *gate == 0) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -207,12 +213,27 @@ void thermostat_apply(Ctx &c) {
                        c.tconv, c.thermoT, c.thermoTau > 0 ? 1 : 0, coupling);
 }
 
+void acc_read(Ctx &c, double out[ACC_N]) {
+    unsigned long long h[kAccTotal];
+    MDX_HIP(hipMemcpyAsync(h, c.acc, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+    MDX_HIP(hipStreamSynchronize(c.stream));
+    for (int s = 0; s < ACC_N; ++s) {
+        unsigned long long t = 0;
+        for (int r = 0; r < kAccRep; ++r) t += h[r * ACC_N + s];
+        out[s] = (double)(long long)t * kFxInv;
+    }
+}
+
+void acc_clear(Ctx &c, int slot, int count) {
+    hipLaunchKernelGGL(k_acc_clear, dim3(1), dim3(kAccRep * ACC_N), 0, c.stream, c.acc, slot, count);
+}
+
 void zero_forces(Ctx &c, bool clear_acc) {
     const int threads = 256;
     hipLaunchKernelGGL(k_zero_u64, dim3(cdiv(3LL * c.n, threads)), dim3(threads), 0, c.stream, nullptr, 3 * c.n,
                        c.force);
     if (clear_acc)
-        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, nullptr, (int)ACC_N, c.acc);
+        hipLaunchKernelGGL(k_zero_u64, dim3(cdiv(kAccTotal, 256)), dim3(256), 0, c.stream, nullptr, kAccTotal, c.acc);
 }
 
 void integrate_first_half(Ctx &c) {

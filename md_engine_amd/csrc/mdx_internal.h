@@ -39,6 +39,13 @@ enum AccSlot {
     ACC_MOL = 10,     // 2 slots: molecular virial correction, sum M V^2 of centres of mass
     ACC_N = 16
 };
+// The accumulator block is replicated kAccRep times; a workgroup adds into
+// replica (blockIdx mod kAccRep) and readers sum the replicas (integer sums
+// commute, so the total is exact and order-free).  Same-address device
+// atomics serialise across the XCDs; spreading them keeps the energy/virial
+// kernels from queueing on one address.
+constexpr int kAccRep = 32;
+constexpr int kAccTotal = ACC_N * kAccRep;
 
 struct Geometry {
     double lo[3];
@@ -161,7 +168,7 @@ struct Ctx {
     int *idToIdx = nullptr;
     float4 *xq = nullptr;
     unsigned long long *force = nullptr;   // 3 per atom
-    unsigned long long *acc = nullptr;     // ACC_N
+    unsigned long long *acc = nullptr;     // kAccTotal: kAccRep replicas of ACC_N slots
     double4 *posLast = nullptr;
     std::vector<double> invMassHost;
     std::vector<float> qHost;             // charges by id, as last set
@@ -197,6 +204,8 @@ struct Ctx {
     // [3] rebuild gate, [4] rebuilds, [5] thermostat gate, [6] turn;
     // [7] set when an atom's candidate stream overflows the build table
     int *dFlags = nullptr;
+    int *waveMax = nullptr;       // per-wave longest rows: [0, n + 4) pair list, then the molecule list
+    int waveMaxN = 0, waveMaxA = 0, waveMaxB = 0;
     bool capture = false;         // a step is being recorded into a graph
     int graphSteps = 0;           // mdx_run: 1 = captured steps (option "graph")
     unsigned *dMax = nullptr;     // 2 slots x (max |x - x_last|^2, max predicted^2) as float bits
@@ -204,9 +213,9 @@ struct Ctx {
     hipEvent_t capEvent = nullptr;
     bool capPending = false;
     long long listViolations = 0; // steps whose list was older than the skin allows
-    int buildTeam = 32;
-    bool buildTeamUser = false;   // set by mdx_teams_set, else chosen from the cell population
-    int buildMode = 0;            // 1 LDS-staged per-cell build; 0 (default) and 2 per-atom teams
+    int buildTeam = 64;           // lanes per atom in the list build (64: one wave per atom)
+    bool buildTeamUser = false;   // set by mdx_teams_set
+    int buildMode = 0;            // 0 (default) and 2 per-atom teams
     long long nRebuilds = 0;
     bool listValid = false;
 
@@ -314,6 +323,11 @@ __device__ __forceinline__ void fx_add(unsigned long long *p, double v) {
     atomicAdd(p, (unsigned long long)to_fx(v));
 }
 
+// this workgroup's replica of the accumulator block
+__device__ __forceinline__ unsigned long long *acc_rep(unsigned long long *acc) {
+    return acc + (blockIdx.x & (kAccRep - 1)) * ACC_N;
+}
+
 // sub-wave team sum by butterfly shuffles (T divides 64)
 template <int T>
 __device__ __forceinline__ float team_sum(float v) {
@@ -395,6 +409,10 @@ void rigid_second_half(Ctx &c, bool kick);
 // integrate.hip
 void integrate_first_half(Ctx &c);
 void integrate_second_half(Ctx &c);
+// accumulator slots summed over the replicas (synchronises the stream)
+void acc_read(Ctx &c, double out[ACC_N]);
+// clear slots [slot, slot + count) in every replica
+void acc_clear(Ctx &c, int slot, int count);
 void zero_forces(Ctx &c, bool clear_acc);
 void nose_hoover_start(Ctx &c);       // chain masses and forces from the current kinetic energy
 void thermo_measure(Ctx &c);          // sum m v^2 into thermo[0]

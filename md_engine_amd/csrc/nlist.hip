@@ -443,7 +443,8 @@ __global__ void __launch_bounds__(256) k_build(const int *__restrict__ gate,
     int n, const double4 *__restrict__ pos, const float4 *__restrict__ xq, Geometry g, CellGrid cg,
     Prefilter pf, Cull cu, const int *__restrict__ colRank, const int *__restrict__ cellStart, int ncell,
     double rlist2, const uint32_t *__restrict__ exclIdx, int maxExcl, int cap,
-    uint32_t *__restrict__ nl, int *__restrict__ nlCount, int *__restrict__ nlCountA, int *__restrict__ flags) {
+    uint32_t *__restrict__ nl, int *__restrict__ nlCount, int *__restrict__ nlCountA, int *__restrict__ flags,
+    int *__restrict__ waveMax) {
     if (gate && *gate == 0) return;
     constexpr int U = T >= 32 ? 1 : 32 / T;  // chunks of T candidates per load block
     constexpr int R = 4 * kMaxCols;           // runs per table: 2 classes x 2 z-runs per column
@@ -654,9 +655,231 @@ __global__ void __launch_bounds__(256) k_build(const int *__restrict__ gate,
             nlCountA[i] = cntA;
         }
     }
-    // one atomic per wave (same-address atomics serialise in L2)
+    // the longest row of the wave goes to the wave's own slot (k_row_max
+    // reduces them): same-address device atomics serialise across the XCDs
     const int wmax = wave_max(cnt);
-    if (lane == 0) atomicMax(&flags[1], wmax);
+    if (lane == 0) waveMax[gtid >> 6] = wmax;
+}
+
+// One wave per atom (the team width 64 of the tuner).  The 64 lanes lay out
+// the table in one step: lanes 0..31 take the columns of the charged atoms'
+// grid, lanes 32..63 those of the charge-free atoms' grid, and a wave-wide
+// prefix sum places the runs, unpadded, in one stream (class, column, z-run
+// order).  Each table entry carries the run's end in the stream, the offset
+// from stream position to atom index, its flag and the atom moved into the
+// run's image frame (uniform over the wave, so computed once per run).  The
+// wave then walks the stream 64 candidates per chunk and U chunks per block:
+// a lane finds its run from the run of the chunk start (runs are rarely
+// shorter than the few lanes they straddle), so the bookkeeping is a couple
+// of LDS reads per lane and the compaction is one mbcnt on the wave ballot.
+struct RunEntry {
+    int vend;      // end of the run in the stream
+    int off;       // atom index = stream position + off
+    uint32_t fl;   // bit 31: the run holds the atom itself or an exclusion partner
+    float xs[3];   // the atom in the run's image frame
+};
+
+template <bool SHIFT>
+__global__ void __launch_bounds__(256) k_build_wave(const int *__restrict__ gate,
+    int n, const double4 *__restrict__ pos, const float4 *__restrict__ xq, Geometry g, CellGrid cg,
+    Prefilter pf, Cull cu, const int *__restrict__ colRank, const int *__restrict__ cellStart, int ncell,
+    double rlist2, const uint32_t *__restrict__ exclIdx, int maxExcl, int cap,
+    uint32_t *__restrict__ nl, int *__restrict__ nlCount, int *__restrict__ nlCountA, int *__restrict__ flags,
+    int *__restrict__ waveMax) {
+    if (gate && *gate == 0) return;
+    constexpr int U = 4;             // chunks of 64 candidates per load block
+    constexpr int R = 4 * kMaxCols;  // 2 classes x 2 z-runs per column
+    __shared__ RunEntry sRun[4][R + 1];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (int)(threadIdx.x >> 6));
+    if (i >= n) {
+        if (lane == 0) waveMax[i] = 0;
+        return;
+    }
+    const int rx = cg.reach[0], ry = cg.reach[1], rz = cg.reach[2];
+    const int sx = axis_slots(cg.nc[0], rx), sy = axis_slots(cg.nc[1], ry);
+    const int ncol = sx * sy;
+    const int ncz = cg.nc[2];
+    const double4 pi = pos[i];
+    const float4 xi = xq[i];
+    const int cx = cell_coord(pi.x, g.lo[0], cg.cellInv[0], cg.nc[0]);
+    const int cy = cell_coord(pi.y, g.lo[1], cg.cellInv[1], cg.nc[1]);
+    const int cz = cell_coord(pi.z, g.lo[2], cg.cellInv[2], cg.nc[2]);
+    const uint32_t *exRow = exclIdx + (size_t)i * maxExcl;
+    const uint32_t ex0 = maxExcl > 0 ? exRow[0] : ~0u, ex1 = maxExcl > 1 ? exRow[1] : ~0u;
+    const uint32_t ex2 = maxExcl > 2 ? exRow[2] : ~0u, ex3 = maxExcl > 3 ? exRow[3] : ~0u;
+    // ---- table: one (class, column) per lane
+    const int cls = lane >> 5, col = lane & 31;
+    const int *cs = cellStart + cls * ncell;
+    int st0 = 0, len0 = 0, st1 = 0, len1 = 0;
+    uint32_t f0 = 0, f1 = 0;
+    float s0x = xi.x, s0y = xi.y, s0z = xi.z, s1x = xi.x, s1y = xi.y, s1z = xi.z;
+    if (col < ncol) {
+        const int a = col / sy, b = col - a * sy;
+        const int xx = axis_cell(a, cx, cg.nc[0], rx, g.periodic[0]);
+        const int yy = axis_cell(b, cy, cg.nc[1], ry, g.periodic[1]);
+        const float gx = cu.on[0] ? axis_gap(cx - rx + a, cu.w[0], xi.x) : 0.0f;
+        const float gy = cu.on[1] ? axis_gap(cy - ry + b, cu.w[1], xi.y) : 0.0f;
+        const float gxy2 = gx * gx + gy * gy;
+        if (xx >= 0 && yy >= 0 && gxy2 <= cu.r2) {
+            ZRuns zr;
+            if (cu.on[2]) {
+                const float h = sqrtf(cu.r2 - gxy2);
+                const int ulo = max((int)floorf((xi.z - h) / cu.w[2]), cz - rz);
+                const int uhi = min((int)floorf((xi.z + h) / cu.w[2]), cz + rz);
+                zr = z_runs_cut(ulo, uhi, ncz);
+            } else {
+                zr = z_runs(cz, ncz, rz, g.periodic[2]);
+            }
+            const int colBase = colRank[xx * cg.nc[1] + yy] * ncz;
+            if (zr.n > 0) {
+                st0 = cs[colBase + zr.lo0];
+                len0 = cs[colBase + zr.hi0 + 1] - st0;
+            }
+            if (zr.n > 1) {
+                st1 = cs[colBase + zr.lo1];
+                len1 = cs[colBase + zr.hi1 + 1] - st1;
+            }
+            f0 = (maxExcl > 4 || (i >= st0 && i < st0 + len0) || in_run(ex0, st0, st0 + len0) ||
+                  in_run(ex1, st0, st0 + len0) || in_run(ex2, st0, st0 + len0) || in_run(ex3, st0, st0 + len0))
+                     ? 0x80000000u : 0u;
+            f1 = (maxExcl > 4 || (i >= st1 && i < st1 + len1) || in_run(ex0, st1, st1 + len1) ||
+                  in_run(ex1, st1, st1 + len1) || in_run(ex2, st1, st1 + len1) || in_run(ex3, st1, st1 + len1))
+                     ? 0x80000000u : 0u;
+            if (SHIFT) {
+                const int ux = cx - rx + a, uy = cy - ry + b;
+                const float kx = ux < 0 ? -1.0f : (ux >= cg.nc[0] ? 1.0f : 0.0f);
+                const float ky = uy < 0 ? -1.0f : (uy >= cg.nc[1] ? 1.0f : 0.0f);
+                s0x = s1x = xi.x - kx * pf.L[0];
+                s0y = s1y = xi.y - ky * pf.L[1];
+                s0z = xi.z - (float)zr.k0 * pf.L[2];
+                s1z = xi.z - (float)zr.k1 * pf.L[2];
+            }
+        }
+    }
+    const int nr = (len0 > 0 ? 1 : 0) + (len1 > 0 ? 1 : 0);
+    const int ln = len0 + len1;
+    int snr = nr, sln = ln;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y0 = __shfl_up(snr, o, 64), y1 = __shfl_up(sln, o, 64);
+        if (lane >= o) {
+            snr += y0;
+            sln += y1;
+        }
+    }
+    {
+        int e = snr - nr, v = sln - ln;
+        if (len0 > 0) {
+            RunEntry &q = sRun[w][e];
+            q.vend = v + len0;
+            q.off = st0 - v;
+            q.fl = f0;
+            q.xs[0] = s0x;
+            q.xs[1] = s0y;
+            q.xs[2] = s0z;
+            v += len0;
+            ++e;
+        }
+        if (len1 > 0) {
+            RunEntry &q = sRun[w][e];
+            q.vend = v + len1;
+            q.off = st1 - v;
+            q.fl = f1;
+            q.xs[0] = s1x;
+            q.xs[1] = s1y;
+            q.xs[2] = s1z;
+        }
+    }
+    const int nrun = __shfl(snr, 63, 64);
+    const int total = __shfl(sln, 63, 64);
+    const int totalA = __shfl(sln, 31, 64);
+    // a sentinel past the last run stops every search
+    if (lane == 0) {
+        RunEntry &q = sRun[w][nrun];
+        q.vend = 0x7FFFFFFF;
+        q.off = 0;
+        q.fl = 0x80000000u;
+        q.xs[0] = xi.x;
+        q.xs[1] = xi.y;
+        q.xs[2] = xi.z;
+    }
+    wave_sync();
+    // ---- walk
+    const uint32_t e0 = ex0 & 0x3FFFFFFFu, e1 = ex1 & 0x3FFFFFFFu, e2 = ex2 & 0x3FFFFFFFu, e3 = ex3 & 0x3FFFFFFFu;
+    uint32_t *row = nl + (size_t)i * cap;
+    int cnt = 0, cntA = 0;
+    int ra = 0;  // run of the current chunk start (uniform)
+    for (int p0 = 0; p0 < total; p0 += 64 * U) {
+        uint32_t jv[U];
+        float px[U], py[U], pz[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int p = p0 + 64 * u;
+            // chunk start's run (uniform search)
+            while (__builtin_amdgcn_readfirstlane(sRun[w][ra].vend) <= p) ++ra;
+            const int v = p + lane;
+            int r = ra;
+            while (v >= sRun[w][r].vend) ++r;
+            const RunEntry q = sRun[w][r];
+            // past the stream: the atom itself in a flagged slot (never a hit)
+            jv[u] = v < total ? ((uint32_t)(v + q.off) | q.fl) : ((uint32_t)i | 0x80000000u);
+            px[u] = q.xs[0];
+            py[u] = q.xs[1];
+            pz[u] = q.xs[2];
+        }
+        float4 blk[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) blk[u] = xq[jv[u] & 0x7FFFFFFFu];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int p = p0 + 64 * u;
+            const int j = (int)(jv[u] & 0x7FFFFFFFu);
+            const float4 cur = blk[u];
+            float r2f;
+            if (SHIFT) {
+                const float dx = cur.x - px[u], dy = cur.y - py[u], dz = cur.z - pz[u];
+                r2f = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            } else {
+                r2f = dist2f(xi, cur, pf);
+            }
+            bool hit = r2f < pf.lo2;
+            if (!hit && r2f <= pf.hi2) hit = dist2(pi, pos[j], g) < rlist2;
+            uint32_t tag = 0;
+            if ((int)jv[u] < 0) {
+                hit = hit && j != i;
+                const uint32_t uj = (uint32_t)j;
+                tag = e0 == uj ? ex0 >> 30 : tag;
+                tag = e1 == uj ? ex1 >> 30 : tag;
+                tag = e2 == uj ? ex2 >> 30 : tag;
+                tag = e3 == uj ? ex3 >> 30 : tag;
+                if (maxExcl > 4 && hit)
+                    for (int qq = 4; qq < maxExcl; ++qq) {
+                        const uint32_t e = exRow[qq];
+                        if ((e & 0x3FFFFFFFu) == uj) tag = e >> 30;
+                    }
+            }
+            const unsigned long long m = __ballot(hit);
+            if (hit) {
+                const int k = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m, (uint32_t)cnt));
+                if (k < cap) row[k] = (uint32_t)j | (tag << 30);
+            }
+            // the charged/charge-free split falls inside this chunk
+            if (p <= totalA && totalA < p + 64) {
+                const int below = totalA - p;
+                const unsigned long long lo = below >= 64 ? ~0ull : ((1ull << below) - 1ull);
+                cntA = cnt + __popcll(m & lo);
+            }
+            cnt += __popcll(m);
+        }
+    }
+    if (totalA >= total) cntA = cnt;
+    if (lane == 0) {
+        nlCount[i] = cnt;
+        nlCountA[i] = cntA;
+        waveMax[i] = cnt;
+    }
 }
 
 // Exclusion partners of each atom as current indices (idx | tag << 30), one
@@ -688,7 +911,7 @@ __global__ void __launch_bounds__(256) k_e3b_list(
     const int *__restrict__ gate, int nmol, const int4 *__restrict__ molAtoms, const int *__restrict__ atomMolO,
     const double4 *__restrict__ pos, Geometry g, double rcut2, int cap, const uint32_t *__restrict__ nl,
     const int *__restrict__ nlCount, const int *__restrict__ nlCountA, int unchargedOnly, int mcap,
-    int *__restrict__ mnl, int *__restrict__ mcount, int *__restrict__ flags) {
+    int *__restrict__ mnl, int *__restrict__ mcount, int *__restrict__ waveMax) {
     if (gate && *gate == 0) return;
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int m = gtid / T;
@@ -726,7 +949,7 @@ __global__ void __launch_bounds__(256) k_e3b_list(
         if (tl == 0) mcount[m] = cnt;
     }
     const int wmax = wave_max(cnt);
-    if (lane == 0) atomicMax(&flags[2], wmax);
+    if (lane == 0) waveMax[gtid >> 6] = wmax;
 }
 
 __global__ void k_mol_atoms(const int *__restrict__ gate, int nmol, const int4 *__restrict__ molIds, const int *__restrict__ idToIdx,
@@ -744,6 +967,33 @@ __global__ void k_fill_int(const int *__restrict__ gate, int n, int *__restrict_
     if (gate && *gate == 0) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// Longest pair row and molecule row of the build from the per-wave slots,
+// folded into the running maxima flags[1], flags[2] (one workgroup).
+__global__ void __launch_bounds__(1024) k_row_max(const int *__restrict__ gate, int na, const int *__restrict__ a,
+                                                  int nb, const int *__restrict__ b, int *__restrict__ flags) {
+    if (gate && *gate == 0) return;
+    __shared__ int red[2][16];
+    int ma = 0, mb = 0;
+    for (int k = threadIdx.x; k < na; k += 1024) ma = max(ma, a[k]);
+    for (int k = threadIdx.x; k < nb; k += 1024) mb = max(mb, b[k]);
+    ma = wave_max(ma);
+    mb = wave_max(mb);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[0][w] = ma;
+        red[1][w] = mb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 16; ++k) {
+            ma = max(ma, red[0][k]);
+            mb = max(mb, red[1][k]);
+        }
+        flags[1] = max(flags[1], ma);
+        flags[2] = max(flags[2], mb);
+    }
 }
 
 uint32_t morton2(uint32_t x, uint32_t y) {
@@ -810,6 +1060,24 @@ Cull make_cull(const Ctx &c) {
     return u;
 }
 
+// slots of the pair-list waves in c.waveMax; the molecule-list slots follow
+size_t wave_slots_a(const Ctx &c) { return (size_t)c.n + 4; }
+
+void launch_build_wave(Ctx &c, const CellGrid &cg, int cur) {
+    MDX_REQUIRE(std::min(c.nc[0], 2 * c.reach[0] + 1) * std::min(c.nc[1], 2 * c.reach[1] + 1) <= kMaxCols, 1,
+                "stencil wider than the build kernel's column table");
+    const int blocks = cdiv(c.n, 4);
+    bool shift = true;
+    for (int a = 0; a < 3; ++a)
+        if (c.geo.periodic[a] && c.nc[a] < 2 * c.reach[a] + 1) shift = false;
+    auto kern = shift ? k_build_wave<true> : k_build_wave<false>;
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, c.stream, rebuild_gate(c), c.n, c.pos[cur], c.xq, c.geo, cg,
+                       make_prefilter(c), make_cull(c), c.colRank, c.cellStart, c.ncell, c.rlist * c.rlist,
+                       c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.nlCountA, c.dFlags,
+                       c.waveMax);
+    c.waveMaxA = blocks * 4;
+}
+
 template <int T>
 void launch_build(Ctx &c, const CellGrid &cg, int cur) {
     MDX_REQUIRE(std::min(c.nc[0], 2 * c.reach[0] + 1) * std::min(c.nc[1], 2 * c.reach[1] + 1) <= kMaxCols, 1,
@@ -823,7 +1091,9 @@ void launch_build(Ctx &c, const CellGrid &cg, int cur) {
     auto kern = shift ? k_build<T, true> : k_build<T, false>;
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), 0, c.stream, rebuild_gate(c), c.n, c.pos[cur], c.xq, c.geo,
                        cg, make_prefilter(c), make_cull(c), c.colRank, c.cellStart, c.ncell, c.rlist * c.rlist,
-                       c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.nlCountA, c.dFlags);
+                       c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.nlCountA, c.dFlags,
+                       c.waveMax);
+    c.waveMaxA = blocks * (threads / 64);
 }
 
 template <int T>
@@ -834,7 +1104,8 @@ void launch_e3b_list(Ctx &c, int cur) {
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_list<T>), dim3(blocks), dim3(threads), 0, c.stream, rebuild_gate(c),
                        c.e3b.nmol, c.e3b.molAtoms, c.e3b.atomMolO, c.pos[cur], c.geo, rc * rc, c.cap,
                        c.nl, c.nlCount, c.nlCountA, c.e3b.oxUncharged ? 1 : 0, c.e3b.cap, c.e3b.nl,
-                       c.e3b.nlCount, c.dFlags);
+                       c.e3b.nlCount, c.waveMax + wave_slots_a(c));
+    c.waveMaxB = blocks * (threads / 64);
 }
 
 }  // namespace
@@ -888,10 +1159,10 @@ void nlist_setup(Ctx &c) {
         c.perm = dalloc<int>(c, c.n);
         c.perm2 = dalloc<int>(c, c.n);
     }
-    // team width of the per-atom build: 8 lanes measured fastest for dense
-    // water (runs of ~60 culled candidates) and LJ alike; mdx_tune revisits it
+    // one wave per atom (k_build_wave) measured fastest for dense water and
+    // LJ alike; mdx_tune revisits the width
+    if (!c.buildTeamUser) c.buildTeam = 64;
     const double dens = c.n / (c.geo.L[0] * c.geo.L[1] * c.geo.L[2]);
-    if (!c.buildTeamUser) c.buildTeam = 8;
     if (c.cap == 0) {
         double expect = dens * 4.18879020479 * c.rlist * c.rlist * c.rlist;
         c.cap = ((int)(expect * 1.3) + 32 + 15) & ~15;
@@ -910,8 +1181,8 @@ void refresh_xq(Ctx &c) {
 
 void prepare_forces(Ctx &c) {
     const int threads = 256;
-    hipLaunchKernelGGL(k_xq_zero, dim3(cdiv(std::max(c.n, (int)ACC_N), threads)), dim3(threads), 0, c.stream, c.n,
-                       c.pos[c.cur], c.q[c.cur], c.xq, c.geo, c.force, (int)ACC_N, c.acc);
+    hipLaunchKernelGGL(k_xq_zero, dim3(cdiv(std::max(c.n, kAccTotal), threads)), dim3(threads), 0, c.stream, c.n,
+                       c.pos[c.cur], c.q[c.cur], c.xq, c.geo, c.force, kAccTotal, c.acc);
 }
 
 void nlist_check(Ctx &c) {
@@ -922,6 +1193,15 @@ void nlist_check(Ctx &c) {
 }
 
 static void run_build(Ctx &c, bool sync) {
+    // per-wave row maxima: n + 4 slots cover the build at up to 64 lanes per
+    // atom (workgroups of 4 waves), the rest the molecule list likewise
+    const size_t needSlots = wave_slots_a(c) + (size_t)(c.e3b.on ? c.e3b.nmol : 0) + 4;
+    if ((size_t)c.waveMaxN < needSlots) {
+        MDX_REQUIRE(!c.capture, 3, "row-maximum slots must exist before a step is captured");
+        if (c.waveMax) dfree(c, c.waveMax);
+        c.waveMax = dalloc<int>(c, needSlots);
+        c.waveMaxN = (int)needSlots;
+    }
     CellGrid cg = make_grid(c);
     int threads = 256;
     int cur = c.cur, nxt = 1 - c.cur;
@@ -977,9 +1257,10 @@ static void run_build(Ctx &c, bool sync) {
                 case 8: launch_build<8>(c, cg, c.cur); break;
                 case 16: launch_build<16>(c, cg, c.cur); break;
                 case 32: launch_build<32>(c, cg, c.cur); break;
-                default: launch_build<64>(c, cg, c.cur); break;
+                default: launch_build_wave(c, cg, c.cur); break;
             }
         }
+        c.waveMaxB = 0;
         if (c.e3b.on) {
             switch (c.e3b.listTeam) {
                 case 8: launch_e3b_list<8>(c, c.cur); break;
@@ -987,6 +1268,8 @@ static void run_build(Ctx &c, bool sync) {
                 default: launch_e3b_list<32>(c, c.cur); break;
             }
         }
+        hipLaunchKernelGGL(k_row_max, dim3(1), dim3(1024), 0, c.stream, rebuild_gate(c), c.waveMaxA, c.waveMax,
+                           c.waveMaxB, c.waveMax + wave_slots_a(c), c.dFlags);
         if (c.capture) return;  // rows checked by the host after the chunk
         // largest row counts -> pinned host memory; checked now (sync) or at
         // the next synchronisation point (capacity_ok)
@@ -1115,7 +1398,7 @@ void decide_step(Ctx &c) {
 
 void check_max(Ctx &c, int parity) {
     const int threads = 256;
-    const int blocks = std::min(cdiv(c.n, threads * 4), 128);
+    const int blocks = std::min(cdiv(c.n, threads * 8), 16);
     hipLaunchKernelGGL(k_check_max, dim3(blocks), dim3(threads), 0, c.stream, c.n,
                        c.pos[c.cur], c.posLast, c.vel[c.cur], c.geo, 2.0 * c.dt, c.dMax + 2 * parity,
                        c.dMax + 2 * (1 - parity));

@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(256) k_pair(PairArgs a) {
         if (threadIdx.x < 7) {
             double x = 0.0;
             for (int ww = 0; ww < (int)(blockDim.x >> 6); ++ww) x += wred[ww][threadIdx.x];
-            fx_add(&a.acc[threadIdx.x == 0 ? ACC_PAIR : ACC_VIR + threadIdx.x - 1], x);
+            fx_add(&acc_rep(a.acc)[threadIdx.x == 0 ? ACC_PAIR : ACC_VIR + threadIdx.x - 1], x);
         }
     }
 }

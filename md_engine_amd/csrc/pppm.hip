@@ -323,7 +323,7 @@ __global__ void __launch_bounds__(256) k_kspace_ev(MeshGeo m, int szh, double al
     if (threadIdx.x < 7) {
         double x = 0.0;
         for (int ww = 0; ww < (int)(blockDim.x >> 6); ++ww) x += wred[ww][threadIdx.x];
-        fx_add(&acc[threadIdx.x == 0 ? ACC_KSPACE : ACC_VIR + threadIdx.x - 1], x);
+        fx_add(&acc_rep(acc)[threadIdx.x == 0 ? ACC_KSPACE : ACC_VIR + threadIdx.x - 1], x);
     }
 }
 

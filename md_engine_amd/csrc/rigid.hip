@@ -262,8 +262,8 @@ __global__ void __launch_bounds__(256) k_mol_virial(RigidArgs r, const double4 *
     corr = team_sum_d<64>(corr);
     mv2 = team_sum_d<64>(mv2);
     if ((threadIdx.x & 63) == 0) {
-        fx_add(&acc[slot], corr);
-        fx_add(&acc[slot + 1], mv2);
+        fx_add(&acc_rep(acc)[slot], corr);
+        fx_add(&acc_rep(acc)[slot + 1], mv2);
     }
 }
 
@@ -278,7 +278,7 @@ __global__ void __launch_bounds__(256) k_free_mv2(int n, const double4 *__restri
         if (v.w > 0.0) s = (v.x * v.x + v.y * v.y + v.z * v.z) / v.w;
     }
     s = team_sum_d<64>(s);
-    if ((threadIdx.x & 63) == 0 && s != 0.0) fx_add(&acc[slot + 1], s);
+    if ((threadIdx.x & 63) == 0 && s != 0.0) fx_add(&acc_rep(acc)[slot + 1], s);
 }
 
 // isotropic dilation about the box centre c: rigid molecules move with their

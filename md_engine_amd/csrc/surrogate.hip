@@ -346,7 +346,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_sur_mlp(int nmol, const float *
         o[NB + k] = gs + gp * d2;
         o[2 * NB + k] = gs + gp * d1;
     }
-    if (energy && l == 0 && esum != 0.0) fx_add(&acc[ACC_E3B3], esum);
+    if (energy && l == 0 && esum != 0.0) fx_add(&acc_rep(acc)[ACC_E3B3], esum);
 }
 
 // ---------------------------------------------------------------------------

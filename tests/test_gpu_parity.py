@@ -86,7 +86,7 @@ def test_nlist_lj_full_size_bitexact(orc, gpu_engine_factory):
     assert_same_lists(gc, grows, oc, orows)
 
 
-@pytest.mark.parametrize("team", [8, 32])
+@pytest.mark.parametrize("team", [8, 32, 64])
 def test_nlist_small_noncubic_box(orc, gpu_engine_factory, team):
     """Axes with fewer than 5 cells scan every cell once (no duplicates)."""
     say(f"nlist small non-cubic box, build team {team}")
