@@ -81,6 +81,7 @@ void set_geometry(Ctx &c, const double lo[3], const double len[3]) {
         c.geo.L[a] = len[a];
         c.geo.invL[a] = 1.0 / len[a];
     }
+    if (c.n > 0) box_upload(c);
 }
 
 void require_system(const Ctx &c) { MDX_REQUIRE(c.n > 0, MDX_ERR_STATE, "no system set"); }
@@ -119,7 +120,7 @@ constexpr int kForceTerms = MDX_PAIR | MDX_PPPM | MDX_E3B | MDX_SURROGATE;
 // mass; free sites count as their own molecules).  Synchronises.
 double pressure_now(Ctx &c) {
     acc_clear(c, ACC_MOL, 2);
-    molecular_virial(c, ACC_MOL);
+    molecular_virial(c, ACC_MOL, nullptr);
     double h[ACC_N];
     acc_read(c, h);
     auto v = [&](int k) { return h[k]; };
@@ -132,28 +133,30 @@ double pressure_now(Ctx &c) {
 // (1 - dt/period (P0 - P)/B)^(1/3) clamped to 1 -+ maxDilation, applied to
 // the box about its centre and to the centres of mass; the cell grid is set
 // up again when its cell count would change, P3M refreshes alpha and G.
+bool regrid_if_needed(Ctx &c);
+
 void barostat_apply(Ctx &c) {
-    BarostatState &B = c.baro;
-    const double P = pressure_now(c);
-    B.lastP = P;
-    double mu = std::cbrt(1.0 - c.dt / B.period * (B.P0 - P) / B.bulk);
-    mu = std::min(std::max(mu, 1.0 - B.maxDil), 1.0 + B.maxDil);
-    dilate(c, mu);
-    double lo[3], len[3];
-    bool regrid = false;
+    acc_clear(c, ACC_MOL, 2);  // the captured step relies on the per-step clear instead
+    baro_step(c, nullptr);
+    box_pull(c);
+    regrid_if_needed(c);
+}
+
+// After the box changed on the device: set the grid up again when a cell edge
+// fell below r_list / reach (the list would miss pairs) or the box grew by a
+// whole cell (finer cells are cheaper).
+bool regrid_if_needed(Ctx &c) {
+    bool regrid = c.boxHost.regrid != 0;
     for (int d = 0; d < 3; ++d) {
-        const double ctr = c.geo.lo[d] + 0.5 * c.geo.L[d];
-        len[d] = c.geo.L[d] * mu;
-        lo[d] = ctr - 0.5 * len[d];
+        if (c.geo.periodic[d]) MDX_REQUIRE(c.rlist < 0.5 * c.geo.L[d], 1, "box shrank below twice the list cutoff");
         const double edge = c.rlist / c.reach[d] * (1.0 + 1e-9);
-        regrid = regrid || std::max(1, (int)std::floor(len[d] / edge)) != c.nc[d];
-        if (c.geo.periodic[d]) MDX_REQUIRE(c.rlist < 0.5 * len[d], 1, "box shrank below twice the list cutoff");
+        regrid = regrid || std::max(1, (int)std::floor(c.geo.L[d] * grid_headroom(c) / edge)) > c.nc[d];
     }
-    set_geometry(c, lo, len);
     if (regrid) {
         c.nlistConfigured = false;
         c.listValid = false;
     }
+    return regrid;
 }
 
 void compute(Ctx &c, int flags) {
@@ -168,7 +171,7 @@ void compute(Ctx &c, int flags) {
     } else if ((flags & MDX_E3B) && c.e3b.on) {
         e3b_forces(c, energy, virial);
     }
-    c.forceFlags = flags;
+    c.forceFlags = flags & kForceTerms;
 }
 
 void prepare_terms(Ctx &c) {
@@ -199,6 +202,10 @@ void snapshot_save(Ctx &c) {
     c.snap.turn = c.turn;
     c.snap.rebuilds = c.nRebuilds;
     c.snap.geo = c.geo;
+    if (c.box) {
+        if (!c.snap.box) c.snap.box = dalloc<BoxDev>(c, 1);
+        MDX_HIP(hipMemcpyAsync(c.snap.box, c.box, sizeof(BoxDev), hipMemcpyDeviceToDevice, c.stream));
+    }
 }
 
 void snapshot_restore(Ctx &c) {
@@ -214,6 +221,14 @@ void snapshot_restore(Ctx &c) {
     c.listValid = false;
     bool moved = false;
     for (int d = 0; d < 3; ++d) moved = moved || c.snap.geo.L[d] != c.geo.L[d] || c.snap.geo.lo[d] != c.geo.lo[d];
+    if (c.baro.on && c.snap.box) {
+        // the barostat moved the device box inside the chunk (the host mirror
+        // may not have seen it): the box, alpha and G go back too
+        MDX_HIP(hipMemcpyAsync(c.box, c.snap.box, sizeof(BoxDev), hipMemcpyDeviceToDevice, c.stream));
+        box_pull(c);
+        if (c.pppm.on) pppm_green_device(c, nullptr, c.stream);
+        moved = true;
+    }
     if (moved) {  // a barostat changed the box during the replayed chunk
         c.geo = c.snap.geo;
         c.nlistConfigured = false;
@@ -261,9 +276,11 @@ void captured_step(Ctx &c, int fflags) {
     integrate_first_half(c);
     decide_step(c);
     nlist_rebuild_async(c);  // every kernel gated by dFlags[3]
-    compute(c, fflags);
+    // a barostat needs the virial; its steps are gated by dFlags[8]
+    compute(c, fflags | (c.baro.on ? MDX_VIRIAL : 0));
     integrate_second_half(c);
     if (c.thermoOn) thermostat_apply(c);  // gated by dFlags[5]
+    if (c.baro.on) baro_step(c, baro_gate(c));
 }
 
 hipGraphExec_t capture_steps(const std::vector<Ctx *> &cs, int steps, int fflags) {
@@ -323,7 +340,6 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags) {
         Ctx &c = *cp;
         require_system(c);
         MDX_REQUIRE(c.rlist > 0, MDX_ERR_STATE, "mdx_nlist_config not called");
-        MDX_REQUIRE(!c.baro.on, MDX_ERR_STATE, "captured steps do not run a barostat (use mdx_run)");
         MDX_REQUIRE(c.device == cs[0]->device, MDX_ERR_ARG, "batched contexts must share one device");
         MDX_HIP(hipSetDevice(c.device));
         prepare_terms(c);
@@ -362,7 +378,10 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags) {
                 // chunk state: displacement slots, running row maxima, counters, turn
                 fill_int(c, (int *)c.dMax, 4, 0);
                 fill_int(c, c.dFlags + 1, 5, 0);
-                fill_int(c, c.dFlags + 6, 1, c.thermoOn ? c.turn % c.thermoEvery : 0);
+                // device turn counter: the thermostat and barostat gates take it
+                // modulo their intervals
+                const long long period = (long long)(c.thermoOn ? c.thermoEvery : 1) * (c.baro.on ? c.baro.every : 1);
+                fill_int(c, c.dFlags + 6, 1, (int)(c.turn % period));
                 if (cp != cs[0]) stream_after(cs[0]->stream, c.stream);
             }
             if (!big) big = capture_steps(cs, kGraphSteps, fflags);
@@ -371,7 +390,7 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags) {
             for (int k = 0; k < nbig; ++k) MDX_HIP(hipGraphLaunch(big, cs[0]->stream));
             for (int k = 0; k < nrest; ++k) MDX_HIP(hipGraphLaunch(one, cs[0]->stream));
             MDX_HIP(hipStreamSynchronize(cs[0]->stream));
-            bool regrow = false;
+            bool regrow = false, regrid = false;
             for (Ctx *cp : cs) {
                 Ctx &c = *cp;
                 int h[7];
@@ -386,11 +405,26 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags) {
                     grow_if_needed(c);
                     c.forceFlags = -1;
                     regrow = true;
-                } else {
-                    c.nRebuilds += h[3];
-                    c.turn += len;
+                    continue;
                 }
+                if (c.baro.on) {
+                    box_pull(c);
+                    if (c.boxHost.regrid) {
+                        // a cell edge fell below r_list / reach inside the chunk:
+                        // replay it on a grid set up for the starting box
+                        snapshot_restore(c);
+                        c.nlistConfigured = false;
+                        c.forceFlags = -1;
+                        regrow = true;
+                        continue;
+                    }
+                }
+                c.nRebuilds += h[3];
+                c.turn += len;
+                // the box grew by a cell: the next chunk takes a finer grid
+                if (c.baro.on && regrid_if_needed(c)) regrid = true;
             }
+            if (regrid && !regrow) drop();
             if (regrow) {
                 // rows were reallocated: every replica replays the chunk
                 for (Ctx *cp : cs) {
@@ -435,7 +469,7 @@ int mdx_create(int device, mdx_ctx **out) {
     }
     int rc = guarded(ctx, [](Ctx &c) {
         c.acc = dalloc<unsigned long long>(c, kAccTotal);
-        c.dFlags = dalloc<int>(c, 8);
+        c.dFlags = dalloc<int>(c, 12);
         c.dMax = dalloc<unsigned>(c, 4);
         c.thermo = dalloc<unsigned long long>(c, 2);
         MDX_HIP(hipHostMalloc((void **)&c.pinned, sizeof(int) * 8, hipHostMallocDefault));
@@ -443,7 +477,7 @@ int mdx_create(int device, mdx_ctx **out) {
         MDX_HIP(hipEventCreateWithFlags(&c.capEvent, hipEventDisableTiming));
         for (int k = 0; k < 2; ++k) MDX_HIP(hipEventCreateWithFlags(&c.stepEvent[k], hipEventDisableTiming));
         MDX_HIP(hipMemsetAsync(c.dMax, 0, sizeof(unsigned) * 4, c.stream));
-        MDX_HIP(hipMemsetAsync(c.dFlags, 0, sizeof(int) * 8, c.stream));
+        MDX_HIP(hipMemsetAsync(c.dFlags, 0, sizeof(int) * 12, c.stream));
         MDX_HIP(hipMemsetAsync(c.acc, 0, sizeof(unsigned long long) * kAccTotal, c.stream));
     });
     if (rc != MDX_OK) {
@@ -551,6 +585,7 @@ int mdx_system_set(mdx_ctx *ctx, int n, const double *x, const double *v, const 
             c.pppm.totalQ2 = c.pppm.qqr * s;
             c.pppm.alpha = 0.0;
         }
+        box_upload(c);  // periodic flags and the atom count are set now
     });
 }
 
@@ -798,6 +833,7 @@ int mdx_ewald_real_set(mdx_ctx *ctx, int enable, double alpha, double qqr, doubl
         c.alpha = (float)alpha;
         c.qqr = (float)qqr;
         c.rq2 = (float)(r_cut * r_cut);
+        if (c.n > 0) box_upload(c);
     });
 }
 
@@ -998,7 +1034,7 @@ int mdx_run(mdx_ctx *ctx, int nsteps, int flags) {
         require_system(c);
         MDX_REQUIRE(c.rlist > 0, MDX_ERR_STATE, "mdx_nlist_config not called");
         MDX_REQUIRE(nsteps >= 0, MDX_ERR_ARG, "nsteps must be >= 0");
-        if (c.graphSteps && !c.baro.on) {  // the barostat needs the host every few steps
+        if (c.graphSteps) {
             run_captured({&c}, nsteps, flags);
             return;
         }

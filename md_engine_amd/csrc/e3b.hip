@@ -29,7 +29,8 @@ struct E3BArgs {
     int cap;
     const int *nl;
     const int *nlCount;
-    float L[3], invL[3];
+    float L[3], invL[3];  // taken from bx in the kernels
+    const BoxDev *bx;
     float E2, Ea, Eb, Ec, k2, k3, rs, rf, r2b2, rmax;
     float4 *tot;
     unsigned long long *force;
@@ -77,15 +78,35 @@ __device__ __forceinline__ float hb_only(const float3 &dr, const E3BArgs &p) {
     return u * u * (p.rf + 2.0f * r - 3.0f * p.rs) / ((p.rf - p.rs) * (p.rf - p.rs) * (p.rf - p.rs)) * es;
 }
 
-template <int P>
-__device__ __forceinline__ void block_acc(double v, int slot, unsigned long long *acc) {
-    // one wave64 reduction, then lane 0 adds (few hundred atomics per launch)
-    double x = team_sum_d<64>(v);
-    if ((threadIdx.x & 63) == 0 && x != 0.0) fx_add(&acc_rep(acc)[slot], x);
+// energy (into eslot) and the six virial components: wave sums, then one
+// atomic per workgroup and value into the workgroup's accumulator replica.
+// Every thread of the workgroup calls it.
+__device__ __forceinline__ void block_acc7(const double (&v)[7], int eslot, unsigned long long *acc) {
+    __shared__ double red[4][7];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int s = 0; s < 7; ++s) {
+        const double x = team_sum_d<64>(v[s]);
+        if (lane == 0) red[w][s] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 7) {
+        double x = 0.0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) x += red[k][threadIdx.x];
+        if (x != 0.0) fx_add(&acc_rep(acc)[threadIdx.x == 0 ? eslot : ACC_VIR + threadIdx.x - 1], x);
+    }
+}
+
+__device__ __forceinline__ void box_into(E3BArgs &p) {
+    for (int d = 0; d < 3; ++d) {
+        p.L[d] = p.bx->Lf[d];
+        p.invL[d] = p.bx->invLf[d];
+    }
 }
 
 template <int P, bool EV>
 __global__ void __launch_bounds__(256) k_e3b_pass1(E3BArgs p) {
+    box_into(p);
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = gt / P;
     const int tl = threadIdx.x & (P - 1);
@@ -140,18 +161,14 @@ __global__ void __launch_bounds__(256) k_e3b_pass1(E3BArgs p) {
         p.tot[i] = make_float4(tx, ty, tz, 0.f);
     }
     if (EV) {
-        block_acc<P>(e2, ACC_E3B2, p.acc);
-        block_acc<P>(0.5 * w0, ACC_VIR + 0, p.acc);
-        block_acc<P>(0.5 * w1, ACC_VIR + 1, p.acc);
-        block_acc<P>(0.5 * w2, ACC_VIR + 2, p.acc);
-        block_acc<P>(0.5 * w3, ACC_VIR + 3, p.acc);
-        block_acc<P>(0.5 * w4, ACC_VIR + 4, p.acc);
-        block_acc<P>(0.5 * w5, ACC_VIR + 5, p.acc);
+        const double v[7] = {e2, 0.5 * w0, 0.5 * w1, 0.5 * w2, 0.5 * w3, 0.5 * w4, 0.5 * w5};
+        block_acc7(v, ACC_E3B2, p.acc);
     }
 }
 
 template <int P, bool EV>
 __global__ void __launch_bounds__(256) k_e3b_pass2(E3BArgs p) {
+    box_into(p);
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = gt / P;
     const int tl = threadIdx.x & (P - 1);
@@ -245,13 +262,8 @@ __global__ void __launch_bounds__(256) k_e3b_pass2(E3BArgs p) {
                  0.5 * (double)p.Ec * ((double)ti.x * ti.x - ec);
             if (p.emol) p.emol[i] = (float)e3;
         }
-        block_acc<P>(e3, ACC_E3B3, p.acc);
-        block_acc<P>(w0, ACC_VIR + 0, p.acc);
-        block_acc<P>(w1, ACC_VIR + 1, p.acc);
-        block_acc<P>(w2, ACC_VIR + 2, p.acc);
-        block_acc<P>(w3, ACC_VIR + 3, p.acc);
-        block_acc<P>(w4, ACC_VIR + 4, p.acc);
-        block_acc<P>(w5, ACC_VIR + 5, p.acc);
+        const double v[7] = {e3, w0, w1, w2, w3, w4, w5};
+        block_acc7(v, ACC_E3B3, p.acc);
     }
 }
 
@@ -288,9 +300,10 @@ void e3b_forces(Ctx &c, bool energy, bool virial, bool three_body, float *emol) 
     a.nl = E.nl;
     a.nlCount = E.nlCount;
     for (int d = 0; d < 3; ++d) {
-        a.L[d] = (float)c.geo.L[d];
-        a.invL[d] = c.geo.periodic[d] ? (float)(1.0 / c.geo.L[d]) : 0.0f;
+        a.L[d] = 0.0f;
+        a.invL[d] = 0.0f;
     }
+    a.bx = c.box;
     a.E2 = E.E2;
     a.Ea = E.Ea;
     a.Eb = E.Eb;

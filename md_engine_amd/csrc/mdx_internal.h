@@ -54,6 +54,185 @@ struct Geometry {
     int periodic[3];
 };
 
+// ---- P3M splitting parameter (find_optimal_parameters, FixChargeEwald.cu:
+// 174-217 and 935-967): secant solve of DeltaF_k(alpha) = DeltaF_real(alpha)
+// in FP64, with the reference's start 4.79853 / rcut, tolerance 1e-5 and
+// 100-iteration limit.  Host and device run the same code (the device one
+// after a barostat changed the box inside a captured step).
+__host__ __device__ inline double pppm_amp(int order, int m) {
+    constexpr double t[7][7] = {
+        {2.0 / 3.0, 0, 0, 0, 0, 0, 0},
+        {1.0 / 50.0, 5.0 / 294.0, 0, 0, 0, 0, 0},
+        {1.0 / 588.0, 7.0 / 1440.0, 21.0 / 3872.0, 0, 0, 0, 0},
+        {1.0 / 4320.0, 3.0 / 1936.0, 7601.0 / 2271360.0, 143.0 / 28800.0, 0, 0, 0},
+        {1.0 / 23232.0, 7601.0 / 12628160.0, 143.0 / 69120.0, 517231.0 / 106536960.0,
+         106640677.0 / 11737571328.0, 0, 0},
+        {691.0 / 68140800.0, 13.0 / 57600.0, 47021.0 / 35512320.0, 9694607.0 / 2095994880.0,
+         733191589.0 / 59609088000.0, 326190917.0 / 11700633600.0, 0},
+        {1.0 / 345600.0, 3617.0 / 35512320.0, 745739.0 / 838397952.0, 56399353.0 / 12773376000.0,
+         25091609.0 / 1560084480.0, 1755948832039.0 / 36229939200000.0, 48887769399.0 / 37838389248.0}};
+    return t[order - 1][m];
+}
+
+__host__ __device__ inline double deltaf_k(double a, int natoms, double q2, const double *L, const int *sz,
+                                           int order) {
+    const double h[3] = {L[0] / sz[0], L[1] / sz[1], L[2] / sz[2]};
+    double sum[3] = {0, 0, 0};
+    for (int mm = 0; mm < order; ++mm) {
+        const double amp = pppm_amp(order, mm);
+        for (int d = 0; d < 3; ++d) sum[d] += amp * pow(h[d] * a, (double)(2 * mm));
+    }
+    double acc = 0.0;
+    for (int d = 0; d < 3; ++d)
+        acc += 1.0 / (L[d] * L[d]) * pow(a * h[d], (double)order) *
+               sqrt(a * L[d] / natoms * sqrt(2.0 * M_PI) * sum[d]);
+    return q2 / 3.0 * acc;
+}
+
+__host__ __device__ inline double deltaf_real(double a, int natoms, double q2, const double *L, double rcut) {
+    return 2 * q2 / sqrt(natoms * rcut * L[0] * L[1] * L[2]) * exp(-a * a * rcut * rcut);
+}
+
+__host__ __device__ inline double pppm_solve_alpha(int natoms, double q2, const double *L, const int *sz,
+                                                   int order, double rcut, double *err) {
+    double xa = 0.0, xb = 4.79853 / rcut;
+    double ya = deltaf_k(xa, natoms, q2, L, sz, order) - deltaf_real(xa, natoms, q2, L, rcut);
+    double yb = deltaf_k(xb, natoms, q2, L, sz, order) - deltaf_real(xb, natoms, q2, L, rcut);
+    const double tol = 1e-5;
+    int it = 0;
+    while ((fabs(yb) / deltaf_real(xb, natoms, q2, L, rcut) > tol) && it < 100) {
+        const double kinv = (xb - xa) / (yb - ya);
+        ya = yb;
+        xa = xb;
+        xb = xa - ya * kinv;
+        yb = deltaf_k(xb, natoms, q2, L, sz, order) - deltaf_real(xb, natoms, q2, L, rcut);
+        ++it;
+    }
+    if (err) *err = deltaf_k(xb, natoms, q2, L, sz, order) + deltaf_real(xb, natoms, q2, L, rcut);
+    return xb;
+}
+
+// alias range of the Green function's image sums (calc_Green_function,
+// FixChargeEwald.cu:314-434)
+__host__ __device__ inline int green_lim(double alpha, double hmean) {
+    return (int)(alpha * hmean / 3.14159 * (sqrt(-log(10E-7)))) + 1;
+}
+
+// ---- box-dependent launch data, kept in device memory ----------------------
+// Every kernel reads what depends on the box from one BoxDev in device
+// memory, so a captured graph stays valid while a barostat changes the box on
+// the device.  box_derive computes all of it from (lo, L) and the parameters
+// fixed while a cell grid lives, with the same expressions on host and device.
+struct CellGrid {
+    int nc[3];
+    double cellInv[3];  // nc / L
+    int reach[3];       // stencil half-width in cells per axis
+};
+
+// FP32 pre-test on xq (x - lo in FP32): candidates with r^2 clearly inside or
+// outside r_list^2 (by more than `band`, a bound on the FP32 rounding of the
+// positions and of the distance) are decided in FP32; only the few inside the
+// band take the canonical FP64 test, so membership is unchanged.
+struct Prefilter {
+    float L[3], invL[3];
+    float lo2, hi2;  // rlist^2 -+ band
+};
+
+// Per-atom culling of the stencil: column (a, b) is skipped when its xy
+// distance from the atom exceeds r_list, and its z-run is cut to the cells
+// within sqrt(r_list^2 - d_xy^2) of the atom.  Distances are measured to the
+// unwrapped cell (cx - reach + a) * w: with nc >= 2 reach + 1 each wrapped
+// cell appears once in the stencil and every neighbour's minimum image lies in
+// its unwrapped cell.  The margin in r2 covers the FP32 rounding of the
+// coordinates and of the cell bounds.  Axes that are open or hold fewer than
+// 2 reach + 1 cells are not culled.
+struct Cull {
+    float w[3];
+    float r2;
+    int on[3];
+};
+
+struct MeshGeo {
+    int sz[3];
+    double lo[3], L[3], invL[3], h[3];
+    int periodic[3];
+};
+
+struct BoxParams {  // fixed while a cell grid and a mesh live
+    int periodic[3];
+    int nc[3], reach[3];
+    int meshSz[3];
+    double rlist, skin, qqr;
+};
+
+struct BoxDev {
+    Geometry geo;
+    CellGrid cg;
+    Prefilter pf;
+    Cull cu;
+    MeshGeo mesh;
+    float Lf[3], invLf[3];  // FP32 minimum image (invLf = 0 on open axes)
+    double vol;
+    double alpha;           // Ewald splitting parameter (FP64) and its FP32 copy
+    float alphaF;
+    float gatherScale;      // -sqrt(qqr) / V, the P3M field-to-force factor
+    double tileCellW[3], tileMargin;
+    int greenLim;           // alias range of the Green function sums
+    double mu;              // last barostat dilation factor
+    double pressure;        // last barostat pressure
+    int regrid;             // a cell edge fell below rlist / reach (grid must be rebuilt)
+};
+
+__host__ __device__ inline void box_derive(BoxDev &b, const BoxParams &p, const double lo[3], const double L[3]) {
+    double lmax = 0.0;
+    bool open = false;
+    for (int a = 0; a < 3; ++a) {
+        b.geo.lo[a] = lo[a];
+        b.geo.L[a] = L[a];
+        b.geo.invL[a] = 1.0 / L[a];
+        b.geo.periodic[a] = p.periodic[a];
+        lmax = lmax > L[a] ? lmax : L[a];
+        open = open || !p.periodic[a];
+    }
+    for (int a = 0; a < 3; ++a) {
+        b.cg.nc[a] = p.nc[a];
+        b.cg.reach[a] = p.reach[a];
+        b.cg.cellInv[a] = p.nc[a] > 0 ? (double)p.nc[a] / L[a] : 0.0;
+        b.Lf[a] = (float)L[a];
+        b.invLf[a] = p.periodic[a] ? (float)(1.0 / L[a]) : 0.0f;
+        b.pf.L[a] = b.Lf[a];
+        b.pf.invL[a] = b.invLf[a];
+        b.cu.w[a] = p.nc[a] > 0 ? (float)(L[a] / p.nc[a]) : 0.0f;
+        b.cu.on[a] = p.periodic[a] && p.nc[a] >= 2 * p.reach[a] + 1;
+        b.mesh.sz[a] = p.meshSz[a];
+        b.mesh.lo[a] = lo[a];
+        b.mesh.L[a] = L[a];
+        b.mesh.invL[a] = b.geo.invL[a];
+        b.mesh.h[a] = p.meshSz[a] > 0 ? L[a] / p.meshSz[a] : 0.0;
+        b.mesh.periodic[a] = p.periodic[a];
+        b.tileCellW[a] = p.nc[a] > 0 ? L[a] / p.nc[a] : 0.0;
+    }
+    // band of the FP32 pre-test: 8 ulp of the largest coordinate per axis on
+    // the distance and 16 ulp on r^2 (2^-23 = FP32 epsilon)
+    const double r2 = p.rlist * p.rlist;
+    const double eps = 1.1920928955078125e-07;
+    const double band = 8.0 * 2.0 * p.rlist * 1.01 * lmax * eps * 1.7320508075688772 + 16.0 * r2 * eps;
+    b.pf.lo2 = (float)(r2 - band);
+    b.pf.hi2 = (float)(r2 + band);
+    const double rc = p.rlist + 1e-5 * (lmax + p.rlist);
+    b.cu.r2 = (float)(rc * rc);
+    if (open) {
+        // an open axis leaves positions unwrapped (no bound on |x - lo|): every
+        // candidate takes the FP64 test, and no axis is culled
+        b.pf.lo2 = -1.0f;
+        b.pf.hi2 = __builtin_inff();
+        b.cu.on[0] = b.cu.on[1] = b.cu.on[2] = 0;
+    }
+    b.vol = L[0] * L[1] * L[2];
+    b.gatherScale = (float)(-sqrt(p.qqr) / b.vol);
+    b.tileMargin = 0.5 * p.skin + 1e-6 * L[0];
+}
+
 struct PppmState {
     bool on = false;
     int gatherTeam = 8;           // lanes per charged atom in the field gather
@@ -169,6 +348,8 @@ struct Ctx {
     float4 *xq = nullptr;
     unsigned long long *force = nullptr;   // 3 per atom
     unsigned long long *acc = nullptr;     // kAccTotal: kAccRep replicas of ACC_N slots
+    BoxDev *box = nullptr;                 // device copy of the box-dependent launch data
+    BoxDev boxHost;                        // host mirror (last upload or read-back)
     double4 *posLast = nullptr;
     std::vector<double> invMassHost;
     std::vector<float> qHost;             // charges by id, as last set
@@ -202,7 +383,8 @@ struct Ctx {
     int *nlCount = nullptr;
     // [0] rebuild flag, [1] max count, [2] e3b max count; captured steps:
     // [3] rebuild gate, [4] rebuilds, [5] thermostat gate, [6] turn;
-    // [7] set when an atom's candidate stream overflows the build table
+    // [7] set when an atom's candidate stream overflows the build table;
+    // [8] barostat gate
     int *dFlags = nullptr;
     int *waveMax = nullptr;       // per-wave longest rows: [0, n + 4) pair list, then the molecule list
     int waveMaxN = 0, waveMaxA = 0, waveMaxB = 0;
@@ -265,6 +447,7 @@ struct Ctx {
         int *type = nullptr, *id = nullptr, *idToIdx = nullptr;
         long long turn = 0, rebuilds = 0;
         Geometry geo;
+        BoxDev *box = nullptr;
     } snap;
 
     // profiling: event pairs recorded on the context stream around kernels
@@ -360,6 +543,10 @@ __device__ __forceinline__ double team_sum_d(double v) {
 // device words that gate kernels inside a captured step (nullptr: always run)
 inline const int *rebuild_gate(const Ctx &c) { return c.capture ? c.dFlags + 3 : nullptr; }
 inline const int *thermo_gate(const Ctx &c) { return c.capture ? c.dFlags + 5 : nullptr; }
+inline const int *baro_gate(const Ctx &c) { return c.capture ? c.dFlags + 8 : nullptr; }
+// cells are sized for a box 0.1 % smaller while a barostat runs, so that the
+// grid outlives about a hundred maximal dilations
+inline double grid_headroom(const Ctx &c) { return c.baro.on ? 0.999 : 1.0; }
 
 // per-module entry points ------------------------------------------------------
 // nlist.hip
@@ -386,8 +573,7 @@ std::vector<int> pair_type_flag_table(const Ctx &c);
 void pppm_prepare(Ctx &c);            // alpha + Green function for the current box
 void pppm_forces(Ctx &c, bool energy, bool virial);
 void pppm_release(Ctx &c);
-double pppm_solve_alpha(int natoms, double totalQ2, const double *L, const int *sz, int order,
-                        double rcut, double *err);
+
 
 // e3b.hip
 void e3b_forces(Ctx &c, bool energy, bool virial, bool three_body = true, float *emol = nullptr);
@@ -401,8 +587,18 @@ void fill_float(Ctx &c, float *p, int count, float v);
 
 // rigid.hip
 void rigid_map(Ctx &c);
-void molecular_virial(Ctx &c, int slot);  // acc[slot] = sum F.(r - R_com), acc[slot+1] = sum M V^2
-void dilate(Ctx &c, double mu);           // isotropic scaling about the box centre (molecules rigid)
+// acc[slot] = sum F.(r - R_com), acc[slot+1] = sum M V^2 (gate: nullptr = always)
+void molecular_virial(Ctx &c, int slot, const int *gate);
+// isotropic scaling by the device box's mu about its centre (molecules rigid)
+void dilate(Ctx &c, const int *gate);
+// device box: upload the host state, read it back, and the barostat step on
+// the device (pressure, dilation, new box, alpha and Green function)
+BoxParams box_params(const Ctx &c);
+void box_upload(Ctx &c);
+void box_pull(Ctx &c);
+void baro_step(Ctx &c, const int *gate);
+// Green function from the device box on stream s (gate: nullptr = always)
+void pppm_green_device(Ctx &c, const int *gate, hipStream_t s);
 void rigid_first_half(Ctx &c);
 void rigid_second_half(Ctx &c, bool kick);
 

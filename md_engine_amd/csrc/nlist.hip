@@ -30,11 +30,6 @@ namespace mdx {
 
 namespace {
 
-struct CellGrid {
-    int nc[3];
-    double cellInv[3];  // nc / L
-    int reach[3];       // stencil half-width in cells per axis
-};
 
 __device__ __forceinline__ double mi_d(double d, double L, double invL, int periodic) {
     if (periodic) {
@@ -60,7 +55,8 @@ __device__ __forceinline__ int cell_coord(double x, double lo, double cellInv, i
 // Max displacement since the last build (setBuildFlag, GridGPU.cu:755-774):
 // flag when any |minImage(x - x_last)|^2 > (skin/2)^2.
 __global__ void k_check(int n, const double4 *__restrict__ pos, const double4 *__restrict__ last,
-                        Geometry g, double thr2, int *__restrict__ flags) {
+                        const BoxDev *__restrict__ bx, double thr2, int *__restrict__ flags) {
+    const Geometry g = bx->geo;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool move = false;
     if (i < n) move = dist2(pos[i], last[i], g) > thr2;
@@ -74,9 +70,11 @@ __global__ void k_check(int n, const double4 *__restrict__ pos, const double4 *_
 // charged partners first, and the charge spread and gather see only charged
 // atoms.
 __global__ void k_bin(const int *__restrict__ gate, int n, double4 *__restrict__ pos, const float *__restrict__ q,
-                      Geometry g, CellGrid cg, int ncell, const int *__restrict__ colRank,
+                      const BoxDev *__restrict__ bx, int ncell, const int *__restrict__ colRank,
                       int *__restrict__ cellOf, int *__restrict__ cellCount) {
     if (gate && *gate == 0) return;
+    const Geometry g = bx->geo;
+    const CellGrid cg = bx->cg;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double4 p = pos[i];
@@ -195,8 +193,9 @@ __global__ void k_permute(const int *__restrict__ gate, int n, const int *__rest
                           double4 *__restrict__ pos1, double4 *__restrict__ vel1,
                           float *__restrict__ q1, int *__restrict__ t1, int *__restrict__ id1,
                           int *__restrict__ idToIdx, double4 *__restrict__ last,
-                          float4 *__restrict__ xq, Geometry g) {
+                          float4 *__restrict__ xq, const BoxDev *__restrict__ bx) {
     if (gate && *gate == 0) return;
+    const Geometry g = bx->geo;
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     int o = perm[k];
@@ -214,7 +213,8 @@ __global__ void k_permute(const int *__restrict__ gate, int n, const int *__rest
 }
 
 __global__ void k_xq(int n, const double4 *__restrict__ pos, const float *__restrict__ q,
-                     float4 *__restrict__ xq, Geometry g) {
+                     float4 *__restrict__ xq, const BoxDev *__restrict__ bx) {
+    const Geometry g = bx->geo;
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     double4 p = pos[k];
@@ -224,8 +224,9 @@ __global__ void k_xq(int n, const double4 *__restrict__ pos, const float *__rest
 // k_xq fused with the clearing of the fixed-point force rows and of the
 // energy / virial accumulators that the force kernels add into.
 __global__ void k_xq_zero(int n, const double4 *__restrict__ pos, const float *__restrict__ q,
-                          float4 *__restrict__ xq, Geometry g, unsigned long long *__restrict__ force, int nacc,
-                          unsigned long long *__restrict__ acc) {
+                          float4 *__restrict__ xq, const BoxDev *__restrict__ bx,
+                          unsigned long long *__restrict__ force, int nacc, unsigned long long *__restrict__ acc) {
+    const Geometry g = bx->geo;
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < nacc) acc[k] = 0ull;
     if (k >= n) return;
@@ -256,7 +257,8 @@ __global__ void k_copy_back(const int *__restrict__ gate, int n, const double4 *
 // reference's setBuildFlag test, GridGPU.cu:755-774), count it, and set the
 // thermostat gate for the steps where the turn reaches a multiple of its
 // interval.  Clears the displacement slot for the next step.
-__global__ void k_decide(unsigned *__restrict__ dmax, int *__restrict__ flags, float half2, int every) {
+__global__ void k_decide(unsigned *__restrict__ dmax, int *__restrict__ flags, float half2, int every,
+                         int baroEvery) {
     const float d2 = __uint_as_float(dmax[0]);
     const int build = d2 > half2 ? 1 : 0;
     flags[3] = build;
@@ -264,6 +266,7 @@ __global__ void k_decide(unsigned *__restrict__ dmax, int *__restrict__ flags, f
     const int turn = flags[6] + 1;
     flags[6] = turn;
     flags[5] = (every > 0 && turn % every == 0) ? 1 : 0;
+    flags[8] = (baroEvery > 0 && turn % baroEvery == 0) ? 1 : 0;
     dmax[0] = 0u;
     dmax[1] = 0u;
 }
@@ -326,15 +329,6 @@ __device__ __forceinline__ ZRuns z_runs(int cz, int ncz, int reach, int periodic
     return z;
 }
 
-// FP32 pre-test on xq (x - lo in FP32): candidates with r^2 clearly inside or
-// outside r_list^2 (by more than `band`, a bound on the FP32 rounding of the
-// positions and of the distance) are decided in FP32; only the few inside the
-// band take the canonical FP64 test, so membership is unchanged.
-struct Prefilter {
-    float L[3], invL[3];
-    float lo2, hi2;   // rlist^2 -+ band
-};
-
 __device__ __forceinline__ float dist2f(const float4 &a, const float4 &b, const Prefilter &p) {
     float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
     dx -= p.L[0] * rintf(dx * p.invL[0]);
@@ -342,20 +336,6 @@ __device__ __forceinline__ float dist2f(const float4 &a, const float4 &b, const 
     dz -= p.L[2] * rintf(dz * p.invL[2]);
     return dx * dx + dy * dy + dz * dz;
 }
-
-// Per-atom culling of the stencil: column (a, b) is skipped when its xy
-// distance from the atom exceeds r_list, and its z-run is cut to the cells
-// within sqrt(r_list^2 - d_xy^2) of the atom.  Distances are measured to the
-// unwrapped cell (cx - reach + a) * w: with nc >= 2 reach + 1 each wrapped
-// cell appears once in the stencil and every neighbour's minimum image lies in
-// its unwrapped cell.  The margin in r2 covers the FP32 rounding of the
-// coordinates and of the cell bounds.  Axes that are open or hold fewer than
-// 2 reach + 1 cells are not culled.
-struct Cull {
-    float w[3];
-    float r2;
-    int on[3];
-};
 
 // z-cells [ulo, uhi] (unwrapped, within one period of [0, nc)) as one or two
 // contiguous runs of a periodic column
@@ -440,12 +420,16 @@ __device__ __forceinline__ bool in_run(uint32_t ex, int s0, int s1) {
 
 template <int T, bool SHIFT>
 __global__ void __launch_bounds__(256) k_build(const int *__restrict__ gate, 
-    int n, const double4 *__restrict__ pos, const float4 *__restrict__ xq, Geometry g, CellGrid cg,
-    Prefilter pf, Cull cu, const int *__restrict__ colRank, const int *__restrict__ cellStart, int ncell,
+    int n, const double4 *__restrict__ pos, const float4 *__restrict__ xq, const BoxDev *__restrict__ bx,
+    const int *__restrict__ colRank, const int *__restrict__ cellStart, int ncell,
     double rlist2, const uint32_t *__restrict__ exclIdx, int maxExcl, int cap,
     uint32_t *__restrict__ nl, int *__restrict__ nlCount, int *__restrict__ nlCountA, int *__restrict__ flags,
     int *__restrict__ waveMax) {
     if (gate && *gate == 0) return;
+    const Geometry g = bx->geo;
+    const CellGrid cg = bx->cg;
+    const Prefilter pf = bx->pf;
+    const Cull cu = bx->cu;
     constexpr int U = T >= 32 ? 1 : 32 / T;  // chunks of T candidates per load block
     constexpr int R = 4 * kMaxCols;           // runs per table: 2 classes x 2 z-runs per column
     __shared__ int sOff[256 / T][R];          // atom index = stream position + sOff
@@ -681,12 +665,16 @@ struct RunEntry {
 
 template <bool SHIFT>
 __global__ void __launch_bounds__(256) k_build_wave(const int *__restrict__ gate,
-    int n, const double4 *__restrict__ pos, const float4 *__restrict__ xq, Geometry g, CellGrid cg,
-    Prefilter pf, Cull cu, const int *__restrict__ colRank, const int *__restrict__ cellStart, int ncell,
+    int n, const double4 *__restrict__ pos, const float4 *__restrict__ xq, const BoxDev *__restrict__ bx,
+    const int *__restrict__ colRank, const int *__restrict__ cellStart, int ncell,
     double rlist2, const uint32_t *__restrict__ exclIdx, int maxExcl, int cap,
     uint32_t *__restrict__ nl, int *__restrict__ nlCount, int *__restrict__ nlCountA, int *__restrict__ flags,
     int *__restrict__ waveMax) {
     if (gate && *gate == 0) return;
+    const Geometry g = bx->geo;
+    const CellGrid cg = bx->cg;
+    const Prefilter pf = bx->pf;
+    const Cull cu = bx->cu;
     constexpr int U = 4;             // chunks of 64 candidates per load block
     constexpr int R = 4 * kMaxCols;  // 2 classes x 2 z-runs per column
     __shared__ RunEntry sRun[4][R + 1];
@@ -909,10 +897,12 @@ __global__ void k_excl_idx(const int *__restrict__ gate, int n, const int *__res
 template <int T>
 __global__ void __launch_bounds__(256) k_e3b_list(
     const int *__restrict__ gate, int nmol, const int4 *__restrict__ molAtoms, const int *__restrict__ atomMolO,
-    const double4 *__restrict__ pos, Geometry g, double rcut2, int cap, const uint32_t *__restrict__ nl,
+    const double4 *__restrict__ pos, const BoxDev *__restrict__ bx, double rcut2, int cap,
+    const uint32_t *__restrict__ nl,
     const int *__restrict__ nlCount, const int *__restrict__ nlCountA, int unchargedOnly, int mcap,
     int *__restrict__ mnl, int *__restrict__ mcount, int *__restrict__ waveMax) {
     if (gate && *gate == 0) return;
+    const Geometry g = bx->geo;
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int m = gtid / T;
     const int tl = threadIdx.x & (T - 1);
@@ -1005,65 +995,19 @@ uint32_t morton2(uint32_t x, uint32_t y) {
     return (uint32_t)r;
 }
 
-CellGrid make_grid(const Ctx &c) {
-    CellGrid cg;
-    for (int a = 0; a < 3; ++a) {
-        cg.reach[a] = c.reach[a];
-        cg.nc[a] = c.nc[a];
-        cg.cellInv[a] = (double)c.nc[a] / c.geo.L[a];
-    }
-    return cg;
-}
-
-// Band that covers the FP32 error of r^2 for wrapped positions (x - lo in
-// [0, L]) and r <= r_list.  Per component the FP32 difference is off by at
-// most ~5 L 2^-24 (two rounded inputs, the subtraction, the rounded L and the
-// image shift); r^2 is then off by at most 2 sqrt(3) r_list of that plus a few
-// ulp of r^2 for the squares and the sum.  The band takes 16 L 2^-24 and
-// 32 r^2 2^-24, about three times both.
-Prefilter make_prefilter(const Ctx &c) {
-    Prefilter p;
-    double lmax = 0.0;
-    for (int a = 0; a < 3; ++a) {
-        p.L[a] = (float)c.geo.L[a];
-        p.invL[a] = c.geo.periodic[a] ? (float)(1.0 / c.geo.L[a]) : 0.0f;
-        lmax = std::max(lmax, c.geo.L[a]);
-    }
-    const double r2 = c.rlist * c.rlist;
-    const double eps = std::ldexp(1.0, -23);
-    const double band = 8.0 * 2.0 * c.rlist * 1.01 * lmax * eps * std::sqrt(3.0) + 16.0 * r2 * eps;
-    p.lo2 = (float)(r2 - band);
-    p.hi2 = (float)(r2 + band);
-    // an open axis leaves positions unwrapped (no bound on |x - lo|): every
-    // candidate then takes the FP64 test
-    if (!c.geo.periodic[0] || !c.geo.periodic[1] || !c.geo.periodic[2]) {
-        p.lo2 = -1.0f;
-        p.hi2 = INFINITY;
-    }
-    return p;
-}
-
-// Culling margin: far above the FP32 rounding of coordinates and cell bounds
-// (~L 2^-24), far below a cell.
-Cull make_cull(const Ctx &c) {
-    Cull u;
-    double lmax = 0.0;
-    for (int a = 0; a < 3; ++a) {
-        u.w[a] = (float)(c.geo.L[a] / c.nc[a]);
-        u.on[a] = c.geo.periodic[a] && c.nc[a] >= 2 * c.reach[a] + 1;
-        lmax = std::max(lmax, c.geo.L[a]);
-    }
-    const double r = c.rlist + 1e-5 * (lmax + c.rlist);
-    u.r2 = (float)(r * r);
-    // open axes leave atoms outside the box in the edge cells: no culling
-    if (!c.geo.periodic[0] || !c.geo.periodic[1] || !c.geo.periodic[2]) u.on[0] = u.on[1] = u.on[2] = 0;
-    return u;
-}
+// The FP32 pre-test band (box_derive) covers the FP32 error of r^2 for
+// wrapped positions (x - lo in [0, L]) and r <= r_list.  Per component the
+// FP32 difference is off by at most ~5 L 2^-24 (two rounded inputs, the
+// subtraction, the rounded L and the image shift); r^2 is then off by at most
+// 2 sqrt(3) r_list of that plus a few ulp of r^2 for the squares and the sum.
+// The band takes 16 L 2^-24 and 32 r^2 2^-24, about three times both.  The
+// culling margin 1e-5 (L + r_list) is far above the FP32 rounding of
+// coordinates and cell bounds (~L 2^-24) and far below a cell.
 
 // slots of the pair-list waves in c.waveMax; the molecule-list slots follow
 size_t wave_slots_a(const Ctx &c) { return (size_t)c.n + 4; }
 
-void launch_build_wave(Ctx &c, const CellGrid &cg, int cur) {
+void launch_build_wave(Ctx &c, int cur) {
     MDX_REQUIRE(std::min(c.nc[0], 2 * c.reach[0] + 1) * std::min(c.nc[1], 2 * c.reach[1] + 1) <= kMaxCols, 1,
                 "stencil wider than the build kernel's column table");
     const int blocks = cdiv(c.n, 4);
@@ -1071,15 +1015,15 @@ void launch_build_wave(Ctx &c, const CellGrid &cg, int cur) {
     for (int a = 0; a < 3; ++a)
         if (c.geo.periodic[a] && c.nc[a] < 2 * c.reach[a] + 1) shift = false;
     auto kern = shift ? k_build_wave<true> : k_build_wave<false>;
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, c.stream, rebuild_gate(c), c.n, c.pos[cur], c.xq, c.geo, cg,
-                       make_prefilter(c), make_cull(c), c.colRank, c.cellStart, c.ncell, c.rlist * c.rlist,
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, c.stream, rebuild_gate(c), c.n, c.pos[cur], c.xq, c.box,
+                       c.colRank, c.cellStart, c.ncell, c.rlist * c.rlist,
                        c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.nlCountA, c.dFlags,
                        c.waveMax);
     c.waveMaxA = blocks * 4;
 }
 
 template <int T>
-void launch_build(Ctx &c, const CellGrid &cg, int cur) {
+void launch_build(Ctx &c, int cur) {
     MDX_REQUIRE(std::min(c.nc[0], 2 * c.reach[0] + 1) * std::min(c.nc[1], 2 * c.reach[1] + 1) <= kMaxCols, 1,
                 "stencil wider than the build kernel's column table");
     int threads = 256;
@@ -1089,8 +1033,8 @@ void launch_build(Ctx &c, const CellGrid &cg, int cur) {
     for (int a = 0; a < 3; ++a)
         if (c.geo.periodic[a] && c.nc[a] < 2 * c.reach[a] + 1) shift = false;
     auto kern = shift ? k_build<T, true> : k_build<T, false>;
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), 0, c.stream, rebuild_gate(c), c.n, c.pos[cur], c.xq, c.geo,
-                       cg, make_prefilter(c), make_cull(c), c.colRank, c.cellStart, c.ncell, c.rlist * c.rlist,
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), 0, c.stream, rebuild_gate(c), c.n, c.pos[cur], c.xq, c.box,
+                       c.colRank, c.cellStart, c.ncell, c.rlist * c.rlist,
                        c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.nlCountA, c.dFlags,
                        c.waveMax);
     c.waveMaxA = blocks * (threads / 64);
@@ -1102,7 +1046,7 @@ void launch_e3b_list(Ctx &c, int cur) {
     int blocks = cdiv((long long)c.e3b.nmol * T, threads);
     double rc = (double)c.e3b.r2b + c.skin;
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_list<T>), dim3(blocks), dim3(threads), 0, c.stream, rebuild_gate(c),
-                       c.e3b.nmol, c.e3b.molAtoms, c.e3b.atomMolO, c.pos[cur], c.geo, rc * rc, c.cap,
+                       c.e3b.nmol, c.e3b.molAtoms, c.e3b.atomMolO, c.pos[cur], c.box, rc * rc, c.cap,
                        c.nl, c.nlCount, c.nlCountA, c.e3b.oxUncharged ? 1 : 0, c.e3b.cap, c.e3b.nl,
                        c.e3b.nlCount, c.waveMax + wave_slots_a(c));
     c.waveMaxB = blocks * (threads / 64);
@@ -1118,7 +1062,7 @@ void nlist_setup(Ctx &c) {
             MDX_REQUIRE(c.rlist < 0.5 * c.geo.L[a], 1,
                         "neighbor cutoff (rCut + padding) must be below half the box length");
         double edge = c.rlist / c.reach[a] * (1.0 + 1e-9);
-        int nca = (int)std::floor(c.geo.L[a] / edge);
+        int nca = (int)std::floor(c.geo.L[a] * grid_headroom(c) / edge);
         c.nc[a] = std::max(1, nca);
     }
     int ncell = c.nc[0] * c.nc[1] * c.nc[2];
@@ -1171,25 +1115,26 @@ void nlist_setup(Ctx &c) {
         c.nlCountA = dalloc<int>(c, c.n);
     }
     c.nlistConfigured = true;
+    box_upload(c);  // cell counts changed
 }
 
 void refresh_xq(Ctx &c) {
     int threads = 256;
     hipLaunchKernelGGL(k_xq, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[c.cur],
-                       c.q[c.cur], c.xq, c.geo);
+                       c.q[c.cur], c.xq, c.box);
 }
 
 void prepare_forces(Ctx &c) {
     const int threads = 256;
     hipLaunchKernelGGL(k_xq_zero, dim3(cdiv(std::max(c.n, kAccTotal), threads)), dim3(threads), 0, c.stream, c.n,
-                       c.pos[c.cur], c.q[c.cur], c.xq, c.geo, c.force, kAccTotal, c.acc);
+                       c.pos[c.cur], c.q[c.cur], c.xq, c.box, c.force, kAccTotal, c.acc);
 }
 
 void nlist_check(Ctx &c) {
     int threads = 256;
     double thr = 0.5 * c.skin;
     hipLaunchKernelGGL(k_check, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n,
-                       c.pos[c.cur], c.posLast, c.geo, thr * thr, c.dFlags);
+                       c.pos[c.cur], c.posLast, c.box, thr * thr, c.dFlags);
 }
 
 static void run_build(Ctx &c, bool sync) {
@@ -1202,14 +1147,13 @@ static void run_build(Ctx &c, bool sync) {
         c.waveMax = dalloc<int>(c, needSlots);
         c.waveMaxN = (int)needSlots;
     }
-    CellGrid cg = make_grid(c);
     int threads = 256;
     int cur = c.cur, nxt = 1 - c.cur;
     const int nslot = 2 * c.ncell;
     hipLaunchKernelGGL(k_fill_int, dim3(cdiv(nslot, threads)), dim3(threads), 0, c.stream, rebuild_gate(c), nslot,
                        c.cellCount, 0);
     hipLaunchKernelGGL(k_bin, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, rebuild_gate(c), c.n, c.pos[cur],
-                       c.q[cur], c.geo, cg, c.ncell, c.colRank, c.cellOf, c.cellCount);
+                       c.q[cur], c.box, c.ncell, c.colRank, c.cellOf, c.cellCount);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c.stream, rebuild_gate(c), nslot, c.cellCount, c.cellStart,
                        c.cellFill);
     hipLaunchKernelGGL(k_place, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, rebuild_gate(c), c.n, c.cellOf,
@@ -1219,7 +1163,7 @@ static void run_build(Ctx &c, bool sync) {
     hipLaunchKernelGGL(k_permute, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, rebuild_gate(c), c.n, c.perm2,
                        c.pos[cur], c.vel[cur], c.q[cur], c.type[cur], c.id[cur], c.pos[nxt],
                        c.vel[nxt], c.q[nxt], c.type[nxt], c.id[nxt], c.idToIdx, c.posLast, c.xq,
-                       c.geo);
+                       c.box);
     if (c.capture) {
         // a captured step keeps fixed buffers: the sorted records come back
         hipLaunchKernelGGL(k_copy_back, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, rebuild_gate(c),
@@ -1254,10 +1198,10 @@ static void run_build(Ctx &c, bool sync) {
         {
             ProfScope ps(c, PK_BUILD);
             switch (c.buildTeam) {
-                case 8: launch_build<8>(c, cg, c.cur); break;
-                case 16: launch_build<16>(c, cg, c.cur); break;
-                case 32: launch_build<32>(c, cg, c.cur); break;
-                default: launch_build_wave(c, cg, c.cur); break;
+                case 8: launch_build<8>(c, c.cur); break;
+                case 16: launch_build<16>(c, c.cur); break;
+                case 32: launch_build<32>(c, c.cur); break;
+                default: launch_build_wave(c, c.cur); break;
             }
         }
         c.waveMaxB = 0;
@@ -1351,8 +1295,10 @@ namespace {
 // workgroup (same-address atomics serialise in L2).
 __global__ void __launch_bounds__(256) k_check_max(int n, const double4 *__restrict__ pos,
                                                    const double4 *__restrict__ last,
-                                                   const double4 *__restrict__ vel, Geometry g, double dt2,
-                                                   unsigned *__restrict__ slot, unsigned *__restrict__ other) {
+                                                   const double4 *__restrict__ vel, const BoxDev *__restrict__ bx,
+                                                   double dt2, unsigned *__restrict__ slot,
+                                                   unsigned *__restrict__ other) {
+    const Geometry g = bx->geo;
     __shared__ float red[2][4];
     const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
     if (t0 == 0) {
@@ -1393,14 +1339,14 @@ void decide_step(Ctx &c) {
     check_max(c, 0);
     const float half = (float)(0.5 * c.skin);
     hipLaunchKernelGGL(k_decide, dim3(1), dim3(1), 0, c.stream, c.dMax, c.dFlags, half * half,
-                       c.thermoOn ? c.thermoEvery : 0);
+                       c.thermoOn ? c.thermoEvery : 0, c.baro.on ? c.baro.every : 0);
 }
 
 void check_max(Ctx &c, int parity) {
     const int threads = 256;
     const int blocks = std::min(cdiv(c.n, threads * 8), 16);
     hipLaunchKernelGGL(k_check_max, dim3(blocks), dim3(threads), 0, c.stream, c.n,
-                       c.pos[c.cur], c.posLast, c.vel[c.cur], c.geo, 2.0 * c.dt, c.dMax + 2 * parity,
+                       c.pos[c.cur], c.posLast, c.vel[c.cur], c.box, 2.0 * c.dt, c.dMax + 2 * parity,
                        c.dMax + 2 * (1 - parity));
 }
 

@@ -35,15 +35,20 @@ struct PairArgs {
     const int *typeFlags;
     const float *params;
     int ntypes;
-    float L[3], invL[3];
+    const BoxDev *bx;             // box lengths and alpha
     float spec[3];
-    float alpha, qqr, rq2;
+    float qqr, rq2;
     unsigned long long *force;
     unsigned long long *acc;
 };
 
 template <int P, bool LJ, bool COUL, bool EV>
 __global__ void __launch_bounds__(256) k_pair(PairArgs a) {
+    // box values into locals (writing them into the argument struct would move
+    // it to scratch memory: a.spec is indexed by the pair tag)
+    const float bL0 = a.bx->Lf[0], bL1 = a.bx->Lf[1], bL2 = a.bx->Lf[2];
+    const float bI0 = a.bx->invLf[0], bI1 = a.bx->invLf[1], bI2 = a.bx->invLf[2];
+    const float alphaB = a.bx->alphaF;
     __shared__ float sp[3 * kMaxTypes * kMaxTypes];
     __shared__ int sfl[kMaxTypes];
     __shared__ double wred[4][7];
@@ -65,8 +70,8 @@ __global__ void __launch_bounds__(256) k_pair(PairArgs a) {
         const int cnt = min(a.nlCount[i], a.cap);
         const int nA = min(a.nlCountA[i], cnt);
         const uint32_t *row = a.nl + (size_t)i * a.cap;
-        const float twoAlphaRtPi = 2.0f * 0.5641895835477563f * a.alpha;
-        const float alpha2 = a.alpha * a.alpha;
+        const float twoAlphaRtPi = 2.0f * 0.5641895835477563f * alphaB;
+        const float alpha2 = alphaB * alphaB;
         // part 0: charged neighbors [0, nA); part 1: charge-free ones [nA, cnt)
         for (int part = 0; part < 2; ++part) {
             const bool lj = LJ && (fl & (1 << part));
@@ -79,9 +84,9 @@ __global__ void __launch_bounds__(256) k_pair(PairArgs a) {
                 const uint32_t tag = raw >> 30;
                 const float4 pj = a.xq[j];
                 float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                dx -= a.L[0] * rintf(dx * a.invL[0]);
-                dy -= a.L[1] * rintf(dy * a.invL[1]);
-                dz -= a.L[2] * rintf(dz * a.invL[2]);
+                dx -= bL0 * rintf(dx * bI0);
+                dy -= bL1 * rintf(dy * bI1);
+                dz -= bL2 * rintf(dz * bI2);
                 const float r2 = dx * dx + dy * dy + dz * dz;
                 const float m = tag == 0 ? 1.0f : a.spec[tag - 1];
                 float fs = 0.f;
@@ -115,7 +120,7 @@ __global__ void __launch_bounds__(256) k_pair(PairArgs a) {
                     const float len = ls * rinv;
                     const float qq = a.qqr * pi.w * pj.w;
                     // erfc(ar) - (1 - m); for a full exclusion (m = 0) this is -erf(ar)
-                    const float ec = erfcf(a.alpha * len) - (1.0f - m);
+                    const float ec = erfcf(alphaB * len) - (1.0f - m);
                     const float s = qq * (ec * rinv + twoAlphaRtPi * __expf(-alpha2 * ls));
                     fs += s * rinv * rinv;
                     if (EV) e += 0.5f * qq * rinv * ec;
@@ -221,11 +226,9 @@ void pair_forces(Ctx &c, bool energy, bool virial) {
     a.params = c.ljParams;
     a.ntypes = c.ntypes;
     for (int d = 0; d < 3; ++d) {
-        a.L[d] = (float)c.geo.L[d];
-        a.invL[d] = c.geo.periodic[d] ? (float)(1.0 / c.geo.L[d]) : 0.0f;
         a.spec[d] = c.special[d];
     }
-    a.alpha = c.alpha;
+    a.bx = c.box;
     a.qqr = c.qqr;
     a.rq2 = c.rq2;
     a.force = c.force;

@@ -55,11 +55,6 @@ __device__ __forceinline__ void weights(float x, float *w) {
     }
 }
 
-struct MeshGeo {
-    int sz[3];
-    double lo[3], L[3], invL[3], h[3];
-    int periodic[3];
-};
 
 // Nearest mesh point and offset, FP64 like the oracle: the coordinate is
 // wrapped into [lo, lo + L) first, then ngp = int((pos + h/2) / h) and
@@ -79,8 +74,9 @@ __device__ __forceinline__ int wrapi(int p, int n) {
 
 template <int ORDER>
 __global__ void __launch_bounds__(256) k_spread(int n, const double4 *__restrict__ pos,
-                                                const float *__restrict__ q, MeshGeo m, float qunit,
-                                                unsigned long long *__restrict__ mesh) {
+                                                const float *__restrict__ q, const BoxDev *__restrict__ bx,
+                                                float qunit, unsigned long long *__restrict__ mesh) {
+    const MeshGeo m = bx->mesh;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float qi = q[i];
@@ -124,11 +120,9 @@ __global__ void __launch_bounds__(256) k_spread(int n, const double4 *__restrict
 // added to the global mesh with one atomic per point whose addresses are
 // consecutive across a wave.  Rows that leave the box (never, unless an atom
 // drifted further than the margin) go straight to global memory.
-struct TileGeo {
+struct TileGeo {  // cell widths and margin come from the device box
     int ncz, zs, nzc;
     int ext[3];
-    double cellw[3];
-    double margin;
 };
 
 __device__ __forceinline__ int tile_origin(double lo_pos, double h) {
@@ -138,11 +132,12 @@ __device__ __forceinline__ int tile_origin(double lo_pos, double h) {
 
 template <int ORDER>
 __global__ void __launch_bounds__(256) k_spread_tiled(const double4 *__restrict__ pos,
-                                                      const float *__restrict__ q, MeshGeo m,
+                                                      const float *__restrict__ q, const BoxDev *__restrict__ bx,
                                                       float qunit, const int *__restrict__ cellStart,
                                                       const int2 *__restrict__ colXY, TileGeo tg,
                                                       unsigned long long *__restrict__ mesh) {
     extern __shared__ unsigned long long tile[];
+    const MeshGeo m = bx->mesh;
     constexpr int H = (ORDER - 1) / 2;
     const int col = blockIdx.x / tg.nzc;
     const int zc = blockIdx.x % tg.nzc;
@@ -151,9 +146,10 @@ __global__ void __launch_bounds__(256) k_spread_tiled(const double4 *__restrict_
     const int a0 = cellStart[col * tg.ncz + z0];
     const int a1 = cellStart[col * tg.ncz + z1];
     const int2 cxy = colXY[col];
-    const int ox = tile_origin(cxy.x * tg.cellw[0] - tg.margin, m.h[0]) - H;
-    const int oy = tile_origin(cxy.y * tg.cellw[1] - tg.margin, m.h[1]) - H;
-    const int oz = tile_origin(z0 * tg.cellw[2] - tg.margin, m.h[2]) - H;
+    const double margin = bx->tileMargin;
+    const int ox = tile_origin(cxy.x * bx->tileCellW[0] - margin, m.h[0]) - H;
+    const int oy = tile_origin(cxy.y * bx->tileCellW[1] - margin, m.h[1]) - H;
+    const int oz = tile_origin(z0 * bx->tileCellW[2] - margin, m.h[2]) - H;
     const int ex = tg.ext[0], ey = tg.ext[1], ez = tg.ext[2];
     const int vol = ex * ey * ez;
     for (int k = threadIdx.x; k < vol; k += blockDim.x) tile[k] = 0ull;
@@ -237,19 +233,48 @@ __device__ __forceinline__ double sinc_d(double x) {
     return sin(x) / x;
 }
 
-// Optimal influence function on the half spectrum, FP64 arithmetic.
-__global__ void k_green(MeshGeo m, int szh, double alpha, int order, int lim, float *__restrict__ G) {
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    int total = m.sz[0] * m.sz[1] * szh;
-    if (idx >= total) return;
-    int iz = idx % szh, iy = (idx / szh) % m.sz[1], ix = idx / (szh * m.sz[1]);
-    double k[3] = {kcomp(ix, m.sz[0], m.L[0]), kcomp(iy, m.sz[1], m.L[1]), kcomp(iz, m.sz[2], m.L[2])};
-    double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
-    if (k2 == 0.0) {
-        G[idx] = 0.0f;
-        return;
-    }
-    double f4a2 = 0.25 / alpha / alpha;
+// Green function at one mesh point with the alias sums unrolled for a fixed
+// image range: the Gaussian exp(-|k'|^2 / 4 alpha^2) and the assignment
+// factor W(k')^2p both factor over the three axes, so each axis's terms are
+// computed once per image index (3 (2 LIM + 1) exp and sinc instead of
+// (2 LIM + 1)^3).
+template <int LIM>
+__device__ __forceinline__ double green_point(const double *k, const MeshGeo &m, double f4a2, int order) {
+    constexpr int NI = 2 * LIM + 1;
+    double kp[3][NI], ex[3][NI], w[3][NI];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+            const double q = k[d] + 2.0 * kPi * (i - LIM) / m.h[d];
+            kp[d][i] = q;
+            ex[d][i] = exp(-q * q * f4a2);
+            const double W = sinc_d(q * m.h[d] * 0.5);
+            double W2 = 1.0;
+            for (int o = 0; o < 2 * order; ++o) W2 *= W;
+            w[d][i] = W2;
+        }
+    double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int a = 0; a < NI; ++a)
+#pragma unroll
+        for (int b = 0; b < NI; ++b) {
+            const double wab = w[0][a] * w[1][b], eab = ex[0][a] * ex[1][b];
+            const double kab = kp[0][a] * kp[0][a] + kp[1][b] * kp[1][b];
+            const double dab = k[0] * kp[0][a] + k[1] * kp[1][b];
+#pragma unroll
+            for (int cc = 0; cc < NI; ++cc) {
+                const double W2 = wab * w[2][cc];
+                const double kl = kab + kp[2][cc] * kp[2][cc];
+                s1 += 4.0 * kPi * eab * ex[2][cc] * (dab + k[2] * kp[2][cc]) / kl * W2;
+                s2 += W2;
+            }
+        }
+    return s1 / (s2 * s2);
+}
+
+// optimal influence function (calc_Green_function, FixChargeEwald.cu:314-434)
+__device__ double green_point_any(const double *k, const MeshGeo &m, double f4a2, int order, int lim) {
     double s1 = 0.0, s2 = 0.0;
     for (int mx = -lim; mx <= lim; ++mx)
         for (int my = -lim; my <= lim; ++my)
@@ -264,12 +289,41 @@ __global__ void k_green(MeshGeo m, int szh, double alpha, int order, int lim, fl
                 s1 += 4.0 * kPi * exp(-kl * f4a2) * (k[0] * kp0 + k[1] * kp1 + k[2] * kp2) / kl * W2;
                 s2 += W2;
             }
-    G[idx] = (float)(s1 / (s2 * s2) / k2);
+    return s1 / (s2 * s2);
+}
+
+// Optimal influence function on the half spectrum, FP64 arithmetic.
+__global__ void k_green(const int *__restrict__ gate, const BoxDev *__restrict__ bx, int szh, int order,
+                        float *__restrict__ G) {
+    if (gate && *gate == 0) return;
+    const MeshGeo m = bx->mesh;
+    const double alpha = bx->alpha;
+    const int lim = bx->greenLim;
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    int total = m.sz[0] * m.sz[1] * szh;
+    if (idx >= total) return;
+    int iz = idx % szh, iy = (idx / szh) % m.sz[1], ix = idx / (szh * m.sz[1]);
+    double k[3] = {kcomp(ix, m.sz[0], m.L[0]), kcomp(iy, m.sz[1], m.L[1]), kcomp(iz, m.sz[2], m.L[2])};
+    double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+    if (k2 == 0.0) {
+        G[idx] = 0.0f;
+        return;
+    }
+    const double f4a2 = 0.25 / alpha / alpha;
+    double g;
+    switch (lim) {
+        case 0: g = green_point<0>(k, m, f4a2, order); break;
+        case 1: g = green_point<1>(k, m, f4a2, order); break;
+        case 2: g = green_point<2>(k, m, f4a2, order); break;
+        default: g = green_point_any(k, m, f4a2, order, lim); break;
+    }
+    G[idx] = (float)(g / k2);
 }
 
 // E_hat_d = i k_d G rho_hat (E_field_cu), k_d := 0 on the d-th Nyquist plane.
-__global__ void k_efield(MeshGeo m, int szh, const float *__restrict__ G,
+__global__ void k_efield(const BoxDev *__restrict__ bx, int szh, const float *__restrict__ G,
                          const float2 *__restrict__ rho, float2 *__restrict__ E) {
+    const MeshGeo m = bx->mesh;
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     int total = m.sz[0] * m.sz[1] * szh;
     if (idx >= total) return;
@@ -287,10 +341,12 @@ __global__ void k_efield(MeshGeo m, int szh, const float *__restrict__ G,
 
 // Energy 0.5 sum |rho_hat|^2 G / V over the full spectrum and the k-space
 // virial (virials_cu): half-spectrum entries with 0 < iz < sz/2 count twice.
-__global__ void __launch_bounds__(256) k_kspace_ev(MeshGeo m, int szh, double alpha, double invV,
+__global__ void __launch_bounds__(256) k_kspace_ev(const BoxDev *__restrict__ bx, int szh,
                                                    const float *__restrict__ G,
                                                    const float2 *__restrict__ rho,
                                                    unsigned long long *__restrict__ acc) {
+    const MeshGeo m = bx->mesh;
+    const double alpha = bx->alpha, invV = 1.0 / bx->vol;
     __shared__ double wred[4][7];
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     int total = m.sz[0] * m.sz[1] * szh;
@@ -344,9 +400,11 @@ __device__ __forceinline__ float pick(const float (&w)[ORDER], int k) {
 // per component.
 template <int ORDER, int T>
 __global__ void __launch_bounds__(256) k_gather(int n, const double4 *__restrict__ pos,
-                                                const float *__restrict__ q, MeshGeo m, float scale,
+                                                const float *__restrict__ q, const BoxDev *__restrict__ bx,
                                                 const float *__restrict__ E,
                                                 unsigned long long *__restrict__ force) {
+    const MeshGeo m = bx->mesh;
+    const float scale = bx->gatherScale;
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = gt / T;
     const int tl = threadIdx.x & (T - 1);
@@ -400,48 +458,6 @@ __global__ void __launch_bounds__(256) k_gather(int n, const double4 *__restrict
     }
 }
 
-MeshGeo make_mesh(const Ctx &c) {
-    MeshGeo m;
-    for (int a = 0; a < 3; ++a) {
-        m.sz[a] = c.pppm.sz[a];
-        m.lo[a] = c.geo.lo[a];
-        m.L[a] = c.geo.L[a];
-        m.invL[a] = c.geo.invL[a];
-        m.h[a] = c.geo.L[a] / c.pppm.sz[a];
-        m.periodic[a] = c.geo.periodic[a];
-    }
-    return m;
-}
-
-const double kAmpTable[7][7] = {
-    {2.0 / 3.0, 0, 0, 0, 0, 0, 0},
-    {1.0 / 50.0, 5.0 / 294.0, 0, 0, 0, 0, 0},
-    {1.0 / 588.0, 7.0 / 1440.0, 21.0 / 3872.0, 0, 0, 0, 0},
-    {1.0 / 4320.0, 3.0 / 1936.0, 7601.0 / 2271360.0, 143.0 / 28800.0, 0, 0, 0},
-    {1.0 / 23232.0, 7601.0 / 12628160.0, 143.0 / 69120.0, 517231.0 / 106536960.0,
-     106640677.0 / 11737571328.0, 0, 0},
-    {691.0 / 68140800.0, 13.0 / 57600.0, 47021.0 / 35512320.0, 9694607.0 / 2095994880.0,
-     733191589.0 / 59609088000.0, 326190917.0 / 11700633600.0, 0},
-    {1.0 / 345600.0, 3617.0 / 35512320.0, 745739.0 / 838397952.0, 56399353.0 / 12773376000.0,
-     25091609.0 / 1560084480.0, 1755948832039.0 / 36229939200000.0, 48887769399.0 / 37838389248.0}};
-
-double deltaf_k(double a, int natoms, double q2, const double *L, const int *sz, int order) {
-    double h[3] = {L[0] / sz[0], L[1] / sz[1], L[2] / sz[2]};
-    double sum[3] = {0, 0, 0};
-    for (int mm = 0; mm < order; ++mm) {
-        double amp = kAmpTable[order - 1][mm];
-        for (int d = 0; d < 3; ++d) sum[d] += amp * std::pow(h[d] * a, 2 * mm);
-    }
-    double acc = 0.0;
-    for (int d = 0; d < 3; ++d)
-        acc += 1.0 / (L[d] * L[d]) * std::pow(a * h[d], order) *
-               std::sqrt(a * L[d] / natoms * std::sqrt(2.0 * M_PI) * sum[d]);
-    return q2 / 3.0 * acc;
-}
-
-double deltaf_real(double a, int natoms, double q2, const double *L, double rcut) {
-    return 2 * q2 / std::sqrt(natoms * rcut * L[0] * L[1] * L[2]) * std::exp(-a * a * rcut * rcut);
-}
 
 // Tile shape for the LDS spread: z-chunks of zs cells so that the tile stays
 // within 48 KB of LDS.  Returns false (plain spread) when no cell list is
@@ -452,11 +468,13 @@ bool tiled_geometry(const Ctx &c, TileGeo &tg) {
         if (!c.geo.periodic[a]) return false;
     const int order = c.pppm.order;
     tg.ncz = c.nc[2];
-    tg.margin = 0.5 * c.skin + 1e-6 * c.geo.L[0];
-    for (int a = 0; a < 3; ++a) tg.cellw[a] = c.geo.L[a] / c.nc[a];
+    // the kernel takes cell widths and margin from the device box (box_derive);
+    // the extents keep two mesh points of slack for a box that changes by a
+    // barostat while the grid lives
+    const double margin = 0.5 * c.skin + 1e-6 * c.geo.L[0];
     auto extent = [&](int a, int ncells) {
         double h = c.geo.L[a] / c.pppm.sz[a];
-        return (int)std::floor((ncells * tg.cellw[a] + 2.0 * tg.margin) / h) + order + 2;
+        return (int)std::floor((ncells * (c.geo.L[a] / c.nc[a]) + 2.0 * margin) / h) + order + 2;
     };
     tg.ext[0] = extent(0, 1);
     tg.ext[1] = extent(1, 1);
@@ -476,27 +494,6 @@ bool tiled_geometry(const Ctx &c, TileGeo &tg) {
 }
 
 }  // namespace
-
-// Secant solve of DeltaF_k(alpha) = DeltaF_real(alpha)
-// (find_optimal_parameters, FixChargeEwald.cu:935-967), FP64 on the host.
-double pppm_solve_alpha(int natoms, double q2, const double *L, const int *sz, int order, double rcut,
-                        double *err) {
-    double xa = 0.0, xb = 4.79853 / rcut;
-    double ya = deltaf_k(xa, natoms, q2, L, sz, order) - deltaf_real(xa, natoms, q2, L, rcut);
-    double yb = deltaf_k(xb, natoms, q2, L, sz, order) - deltaf_real(xb, natoms, q2, L, rcut);
-    double tol = 1e-5;
-    int it = 0;
-    while ((std::fabs(yb) / deltaf_real(xb, natoms, q2, L, rcut) > tol) && it < 100) {
-        double kinv = (xb - xa) / (yb - ya);
-        ya = yb;
-        xa = xb;
-        xb = xa - ya * kinv;
-        yb = deltaf_k(xb, natoms, q2, L, sz, order) - deltaf_real(xb, natoms, q2, L, rcut);
-        ++it;
-    }
-    if (err) *err = deltaf_k(xb, natoms, q2, L, sz, order) + deltaf_real(xb, natoms, q2, L, rcut);
-    return xb;
-}
 
 void pppm_release(Ctx &c) {
     if (c.pppm.plansMade) {
@@ -542,12 +539,16 @@ void pppm_prepare(Ctx &c) {
         P.alpha = pppm_solve_alpha(c.n, P.totalQ2, c.geo.L, sz, P.order, P.rcut, &P.err);
     }
     c.alpha = (float)P.alpha;
-    MeshGeo m = make_mesh(c);
-    double hm = std::pow(m.h[0] * m.h[1] * m.h[2], 1.0 / 3.0);
-    int lim = (int)(P.alpha * hm / 3.14159 * (std::sqrt(-std::log(10E-7)))) + 1;
-    hipLaunchKernelGGL(k_green, dim3(cdiv(nh, 256)), dim3(256), 0, c.stream, m, szh, P.alpha, P.order,
-                       lim, P.green);
+    box_upload(c);  // the Green function reads alpha and the mesh from the device box
+    pppm_green_device(c, nullptr, c.stream);
     for (int a = 0; a < 3; ++a) P.boxAtGreen[a] = c.geo.L[a];
+}
+
+void pppm_green_device(Ctx &c, const int *gate, hipStream_t s) {
+    const PppmState &P = c.pppm;
+    const int szh = P.sz[2] / 2 + 1;
+    const size_t nh = (size_t)P.sz[0] * P.sz[1] * szh;
+    hipLaunchKernelGGL(k_green, dim3(cdiv(nh, 256)), dim3(256), 0, s, gate, c.box, szh, P.order, P.green);
 }
 
 void pppm_forces(Ctx &c, bool energy, bool virial) {
@@ -557,7 +558,6 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
     const int szh = sz[2] / 2 + 1;
     const int ng = sz[0] * sz[1] * sz[2];
     const int nh = sz[0] * sz[1] * szh;
-    MeshGeo m = make_mesh(c);
     const float qunit = (float)std::sqrt(P.qqr);
     const int threads = 256;
     const int cur = c.cur;
@@ -568,17 +568,17 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
         const int nblk = c.nc[0] * c.nc[1] * tg.nzc;
         const size_t shm = sizeof(unsigned long long) * tg.ext[0] * tg.ext[1] * tg.ext[2];
         switch (P.order) {
-            case 1: hipLaunchKernelGGL(k_spread_tiled<1>, dim3(nblk), dim3(threads), shm, c.stream, c.pos[cur], c.q[cur], m, qunit, c.cellStart, c.colXY, tg, (unsigned long long *)P.mesh); break;
-            case 3: hipLaunchKernelGGL(k_spread_tiled<3>, dim3(nblk), dim3(threads), shm, c.stream, c.pos[cur], c.q[cur], m, qunit, c.cellStart, c.colXY, tg, (unsigned long long *)P.mesh); break;
-            case 5: hipLaunchKernelGGL(k_spread_tiled<5>, dim3(nblk), dim3(threads), shm, c.stream, c.pos[cur], c.q[cur], m, qunit, c.cellStart, c.colXY, tg, (unsigned long long *)P.mesh); break;
-            default: hipLaunchKernelGGL(k_spread_tiled<7>, dim3(nblk), dim3(threads), shm, c.stream, c.pos[cur], c.q[cur], m, qunit, c.cellStart, c.colXY, tg, (unsigned long long *)P.mesh); break;
+            case 1: hipLaunchKernelGGL(k_spread_tiled<1>, dim3(nblk), dim3(threads), shm, c.stream, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, (unsigned long long *)P.mesh); break;
+            case 3: hipLaunchKernelGGL(k_spread_tiled<3>, dim3(nblk), dim3(threads), shm, c.stream, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, (unsigned long long *)P.mesh); break;
+            case 5: hipLaunchKernelGGL(k_spread_tiled<5>, dim3(nblk), dim3(threads), shm, c.stream, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, (unsigned long long *)P.mesh); break;
+            default: hipLaunchKernelGGL(k_spread_tiled<7>, dim3(nblk), dim3(threads), shm, c.stream, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, (unsigned long long *)P.mesh); break;
         }
     } else
     switch (P.order) {
-        case 1: hipLaunchKernelGGL(k_spread<1>, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.q[cur], m, qunit, (unsigned long long *)P.mesh); break;
-        case 3: hipLaunchKernelGGL(k_spread<3>, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.q[cur], m, qunit, (unsigned long long *)P.mesh); break;
-        case 5: hipLaunchKernelGGL(k_spread<5>, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.q[cur], m, qunit, (unsigned long long *)P.mesh); break;
-        default: hipLaunchKernelGGL(k_spread<7>, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.q[cur], m, qunit, (unsigned long long *)P.mesh); break;
+        case 1: hipLaunchKernelGGL(k_spread<1>, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.q[cur], c.box, qunit, (unsigned long long *)P.mesh); break;
+        case 3: hipLaunchKernelGGL(k_spread<3>, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.q[cur], c.box, qunit, (unsigned long long *)P.mesh); break;
+        case 5: hipLaunchKernelGGL(k_spread<5>, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.q[cur], c.box, qunit, (unsigned long long *)P.mesh); break;
+        default: hipLaunchKernelGGL(k_spread<7>, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.q[cur], c.box, qunit, (unsigned long long *)P.mesh); break;
     }
     hipLaunchKernelGGL(k_mesh_to_real, dim3(cdiv(ng, threads)), dim3(threads), 0, c.stream, ng, P.mesh, P.real);
     }
@@ -588,13 +588,12 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
                 "hipfftExecR2C failed");
     }
     if (energy || virial) {
-        double V = c.geo.L[0] * c.geo.L[1] * c.geo.L[2];
-        hipLaunchKernelGGL(k_kspace_ev, dim3(cdiv(nh, threads)), dim3(threads), 0, c.stream, m, szh,
-                           P.alpha, 1.0 / V, P.green, P.rhoHat, c.acc);
+        hipLaunchKernelGGL(k_kspace_ev, dim3(cdiv(nh, threads)), dim3(threads), 0, c.stream, c.box, szh,
+                           P.green, P.rhoHat, c.acc);
     }
     {
         ProfScope ps(c, PK_EFIELD);
-        hipLaunchKernelGGL(k_efield, dim3(cdiv(nh, threads)), dim3(threads), 0, c.stream, m, szh, P.green,
+        hipLaunchKernelGGL(k_efield, dim3(cdiv(nh, threads)), dim3(threads), 0, c.stream, c.box, szh, P.green,
                            P.rhoHat, P.eHat);
     }
     {
@@ -602,8 +601,6 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
         MDX_REQUIRE(hipfftExecC2R(P.planC2R, (hipfftComplex *)P.eHat, P.eReal) == HIPFFT_SUCCESS, 2,
                     "hipfftExecC2R failed");
     }
-    const double V = c.geo.L[0] * c.geo.L[1] * c.geo.L[2];
-    const float scale = (float)(-std::sqrt(P.qqr) / V);
     // a built list puts the charged atoms first: only they take the gather
     const int na = c.listValid ? c.nCharged : c.n;
     ProfScope ps(c, PK_GATHER);
@@ -611,7 +608,7 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
     const dim3 grid(cdiv((long long)na * P.gatherTeam, threads));
 #define MDX_GATHER(O, T)                                                                                   \
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gather<O, T>), grid, dim3(threads), 0, c.stream, na, c.pos[cur], \
-                       c.q[cur], m, scale, P.eReal, c.force)
+                       c.q[cur], c.box, P.eReal, c.force)
     const bool wide = P.gatherTeam == 16;
     switch (P.order) {
         case 1: if (wide) MDX_GATHER(1, 16); else MDX_GATHER(1, 8); break;

@@ -47,7 +47,8 @@ struct RigidArgs {
     const int4 *atoms;
     double mO, mH, dOH, dHH, gamma;
     double dt, hdtf;
-    Geometry g;
+    Geometry g;          // taken from bx in the kernels
+    const BoxDev *bx;
 };
 
 // SETTLE: old positions relative to the old oxygen (A0 = 0, B0, C0) and
@@ -140,6 +141,7 @@ __device__ __forceinline__ void dist_forces(const unsigned long long *f, int4 a,
 
 __global__ void k_rigid_vv1(RigidArgs r, double4 *__restrict__ pos, double4 *__restrict__ vel,
                             const unsigned long long *__restrict__ f) {
+    r.g = r.bx->geo;
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= r.nmol) return;
     const int4 a = r.atoms[m];
@@ -178,6 +180,7 @@ __global__ void k_rigid_vv1(RigidArgs r, double4 *__restrict__ pos, double4 *__r
 // kick = false: only the velocity constraint and the M placement (start of a run)
 __global__ void k_rigid_vv2(RigidArgs r, double4 *__restrict__ pos, double4 *__restrict__ vel,
                             const unsigned long long *__restrict__ f, int kick) {
+    r.g = r.bx->geo;
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= r.nmol) return;
     const int4 a = r.atoms[m];
@@ -232,6 +235,7 @@ RigidArgs make_args(const Ctx &c) {
     r.dt = c.dt;
     r.hdtf = 0.5 * c.dt * c.ftmToV;
     r.g = c.geo;
+    r.bx = c.box;
     return r;
 }
 
@@ -241,14 +245,23 @@ RigidArgs make_args(const Ctx &c) {
 // rigid molecules is (M|V|^2 summed + W_atomic - corr) / 3V: the atomic virial
 // minus sum_i F_i . (r_i - R_mol(i)) is the virial of the centre-of-mass
 // separations, with no constraint forces needed.
-__global__ void __launch_bounds__(256) k_mol_virial(RigidArgs r, const double4 *__restrict__ pos,
+// One thread per rigid molecule (m < nmol) and per site outside rigid
+// molecules (i < n, member[i] == 0): the molecule's correction and
+// centre-of-mass M|V|^2, or the site's m v^2, summed per workgroup into
+// acc[slot] (corr) and acc[slot + 1] (M|V|^2 + m v^2).
+__global__ void __launch_bounds__(256) k_mol_virial(RigidArgs r, int n, const int *__restrict__ member,
+                                                    const double4 *__restrict__ pos,
                                                     const double4 *__restrict__ vel,
                                                     const unsigned long long *__restrict__ f,
-                                                    unsigned long long *__restrict__ acc, int slot) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+                                                    unsigned long long *__restrict__ acc, int slot,
+                                                    const int *__restrict__ gate) {
+    if (gate && *gate == 0) return;
+    __shared__ double red[2][4];
+    r.g = r.bx->geo;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double corr = 0.0, mv2 = 0.0;
-    if (m < r.nmol) {
-        const int4 a = r.atoms[m];
+    if (t < r.nmol) {
+        const int4 a = r.atoms[t];
         const double mT = r.mO + 2.0 * r.mH;
         const d3 o = xyz(pos[a.x]);
         const d3 h1 = o + mi(xyz(pos[a.y]) - o, r.g);
@@ -259,85 +272,98 @@ __global__ void __launch_bounds__(256) k_mol_virial(RigidArgs r, const double4 *
         const d3 V = (1.0 / mT) * (r.mO * xyz(vel[a.x]) + r.mH * (xyz(vel[a.y]) + xyz(vel[a.z])));
         mv2 = mT * dot(V, V);
     }
+    if (t < n && !(member && member[t])) {
+        const double4 v = vel[t];
+        if (v.w > 0.0) mv2 += (v.x * v.x + v.y * v.y + v.z * v.z) / v.w;
+    }
     corr = team_sum_d<64>(corr);
     mv2 = team_sum_d<64>(mv2);
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        fx_add(&acc_rep(acc)[slot], corr);
-        fx_add(&acc_rep(acc)[slot + 1], mv2);
+        red[0][w] = corr;
+        red[1][w] = mv2;
     }
-}
-
-// sum m v^2 of massive sites outside rigid molecules -> acc[slot + 1]
-__global__ void __launch_bounds__(256) k_free_mv2(int n, const double4 *__restrict__ vel,
-                                                  const int *__restrict__ member,
-                                                  unsigned long long *__restrict__ acc, int slot) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double s = 0.0;
-    if (i < n && !(member && member[i])) {
-        const double4 v = vel[i];
-        if (v.w > 0.0) s = (v.x * v.x + v.y * v.y + v.z * v.z) / v.w;
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        double x = 0.0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) x += red[threadIdx.x][k];
+        if (x != 0.0) fx_add(&acc_rep(acc)[slot + threadIdx.x], x);
     }
-    s = team_sum_d<64>(s);
-    if ((threadIdx.x & 63) == 0 && s != 0.0) fx_add(&acc_rep(acc)[slot + 1], s);
 }
 
 // isotropic dilation about the box centre c: rigid molecules move with their
 // centre of mass (geometry kept), other sites are scaled (Mod::scaleSystem,
 // src/Mod.cu:150-240)
-__global__ void __launch_bounds__(256) k_mol_scale(RigidArgs r, double4 *__restrict__ pos, double mu, d3 c) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= r.nmol) return;
-    const int4 a = r.atoms[m];
-    const double mT = r.mO + 2.0 * r.mH;
-    const d3 o = xyz(pos[a.x]);
-    const d3 h1 = o + mi(xyz(pos[a.y]) - o, r.g);
-    const d3 h2 = o + mi(xyz(pos[a.z]) - o, r.g);
-    const d3 R = (1.0 / mT) * (r.mO * o + r.mH * (h1 + h2));
-    const d3 shift = (c + mu * (R - c)) - R;
-    const int site[4] = {a.x, a.y, a.z, a.w};
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        if (site[s] < 0) continue;
-        double4 p = pos[site[s]];
-        p.x += shift.x;
-        p.y += shift.y;
-        p.z += shift.z;
-        pos[site[s]] = p;
-    }
+// mu and the centre c come from the device box (the barostat kernel set them)
+__device__ __forceinline__ d3 box_centre(const BoxDev *bx) {
+    return {bx->geo.lo[0] + 0.5 * bx->geo.L[0], bx->geo.lo[1] + 0.5 * bx->geo.L[1],
+            bx->geo.lo[2] + 0.5 * bx->geo.L[2]};
 }
 
-__global__ void k_free_scale(int n, double4 *__restrict__ pos, const int *__restrict__ member, double mu, d3 c) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || (member && member[i])) return;
-    double4 p = pos[i];
-    p.x = c.x + mu * (p.x - c.x);
-    p.y = c.y + mu * (p.y - c.y);
-    p.z = c.z + mu * (p.z - c.z);
-    pos[i] = p;
+// one thread per rigid molecule (t < nmol) and per free site (t < n); the two
+// parts write disjoint sites.  k_baro already holds the new box; the
+// intramolecular minimum images (about 1 A) are the same in either box.
+__global__ void __launch_bounds__(256) k_dilate(RigidArgs r, int n, const int *__restrict__ member,
+                                                double4 *__restrict__ pos, const int *__restrict__ gate) {
+    if (gate && *gate == 0) return;
+    r.g = r.bx->geo;
+    const double mu = r.bx->mu;
+    const d3 c = box_centre(r.bx);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < r.nmol) {
+        const int4 a = r.atoms[t];
+        const double mT = r.mO + 2.0 * r.mH;
+        const d3 o = xyz(pos[a.x]);
+        const d3 h1 = o + mi(xyz(pos[a.y]) - o, r.g);
+        const d3 h2 = o + mi(xyz(pos[a.z]) - o, r.g);
+        const d3 R = (1.0 / mT) * (r.mO * o + r.mH * (h1 + h2));
+        const d3 shift = (c + mu * (R - c)) - R;
+        const int site[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            if (site[s] < 0) continue;
+            double4 p = pos[site[s]];
+            p.x += shift.x;
+            p.y += shift.y;
+            p.z += shift.z;
+            pos[site[s]] = p;
+        }
+    }
+    if (t < n && !(member && member[t])) {
+        double4 p = pos[t];
+        p.x = c.x + mu * (p.x - c.x);
+        p.y = c.y + mu * (p.y - c.y);
+        p.z = c.z + mu * (p.z - c.z);
+        pos[t] = p;
+    }
 }
 
 }  // namespace
 
-void molecular_virial(Ctx &c, int slot) {
-    const int *member = c.rigid.on ? c.rigid.member : nullptr;
+void molecular_virial(Ctx &c, int slot, const int *gate) {
+    RigidArgs r = make_args(c);
+    const int *member = nullptr;
     if (c.rigid.on) {
-        RigidArgs r = make_args(c);
-        hipLaunchKernelGGL(k_mol_virial, dim3(cdiv(r.nmol, 256)), dim3(256), 0, c.stream, r, c.pos[c.cur],
-                           c.vel[c.cur], c.force, c.acc, slot);
+        member = c.rigid.member;
+    } else {
+        r.nmol = 0;
+        r.atoms = nullptr;
     }
-    hipLaunchKernelGGL(k_free_mv2, dim3(cdiv(c.n, 256)), dim3(256), 0, c.stream, c.n, c.vel[c.cur], member, c.acc,
-                       slot);
+    hipLaunchKernelGGL(k_mol_virial, dim3(cdiv(std::max(r.nmol, c.n), 256)), dim3(256), 0, c.stream, r, c.n, member,
+                       c.pos[c.cur], c.vel[c.cur], c.force, c.acc, slot, gate);
 }
 
-void dilate(Ctx &c, double mu) {
-    d3 ctr = {c.geo.lo[0] + 0.5 * c.geo.L[0], c.geo.lo[1] + 0.5 * c.geo.L[1], c.geo.lo[2] + 0.5 * c.geo.L[2]};
-    const int *member = c.rigid.on ? c.rigid.member : nullptr;
+void dilate(Ctx &c, const int *gate) {
+    RigidArgs r = make_args(c);
+    const int *member = nullptr;
     if (c.rigid.on) {
-        RigidArgs r = make_args(c);
-        hipLaunchKernelGGL(k_mol_scale, dim3(cdiv(r.nmol, 256)), dim3(256), 0, c.stream, r, c.pos[c.cur], mu, ctr);
+        member = c.rigid.member;
+    } else {
+        r.nmol = 0;
+        r.atoms = nullptr;
     }
-    hipLaunchKernelGGL(k_free_scale, dim3(cdiv(c.n, 256)), dim3(256), 0, c.stream, c.n, c.pos[c.cur], member, mu,
-                       ctr);
+    hipLaunchKernelGGL(k_dilate, dim3(cdiv(std::max(r.nmol, c.n), 256)), dim3(256), 0, c.stream, r, c.n, member,
+                       c.pos[c.cur], gate);
 }
 
 void rigid_map(Ctx &c) {

@@ -63,6 +63,15 @@ struct Geo3 {
     float L[3], invL[3];
 };
 
+__device__ __forceinline__ Geo3 geo3(const BoxDev *bx) {
+    Geo3 g;
+    for (int d = 0; d < 3; ++d) {
+        g.L[d] = bx->Lf[d];
+        g.invL[d] = bx->invLf[d];
+    }
+    return g;
+}
+
 __device__ __forceinline__ float3 mi(float4 a, float4 b, const Geo3 &g) {
     float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
     dx -= g.L[0] * rintf(dx * g.invL[0]);
@@ -80,9 +89,10 @@ constexpr float kSkipOO2 = (RF_C + 1.0f) * (RF_C + 1.0f);
 constexpr int kFeatLanes = 16;
 
 __global__ void __launch_bounds__(256) k_sur_feat(int nmol, const int4 *__restrict__ molAtoms,
-                                                  const float4 *__restrict__ xq, Geo3 g, int cap,
-                                                  const int *__restrict__ nl, const int *__restrict__ nlCount,
-                                                  float *__restrict__ feat) {
+                                                  const float4 *__restrict__ xq, const BoxDev *__restrict__ bx,
+                                                  int cap, const int *__restrict__ nl,
+                                                  const int *__restrict__ nlCount, float *__restrict__ feat) {
+    const Geo3 g = geo3(bx);
     __shared__ float part[256 / kFeatLanes][kFeatLanes][NF + 1];
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = gt / kFeatLanes, tl = threadIdx.x % kFeatLanes, team = threadIdx.x / kFeatLanes;
@@ -354,10 +364,11 @@ __global__ void __launch_bounds__(64 * kWaves) k_sur_mlp(int nmol, const float *
 // lane holds the molecule's 48 channel weights, gathers the neighbour's, and
 // the lanes' partial forces are summed by shuffles
 __global__ void __launch_bounds__(256) k_sur_force(int nmol, const int4 *__restrict__ molAtoms,
-                                                   const float4 *__restrict__ xq, Geo3 g, int cap,
-                                                   const int *__restrict__ nl, const int *__restrict__ nlCount,
-                                                   const float *__restrict__ wgt,
+                                                   const float4 *__restrict__ xq, const BoxDev *__restrict__ bx,
+                                                   int cap, const int *__restrict__ nl,
+                                                   const int *__restrict__ nlCount, const float *__restrict__ wgt,
                                                    unsigned long long *__restrict__ force) {
+    const Geo3 g = geo3(bx);
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = gt / kFeatLanes, tl = threadIdx.x % kFeatLanes;
     float3 FO = make_float3(0.f, 0.f, 0.f), F1 = FO, F2 = FO;
@@ -623,14 +634,6 @@ __global__ void __launch_bounds__(256) k_sur_norm(int nsamp, const float *__rest
     }
 }
 
-Geo3 geo3(const Ctx &c) {
-    Geo3 g;
-    for (int d = 0; d < 3; ++d) {
-        g.L[d] = (float)c.geo.L[d];
-        g.invL[d] = c.geo.periodic[d] ? (float)(1.0 / c.geo.L[d]) : 0.0f;
-    }
-    return g;
-}
 
 }  // namespace
 
@@ -639,7 +642,7 @@ static_assert(OEH + 1 == kSurNParam, "packed surrogate layout");
 void surrogate_features(Ctx &c, float *raw) {
     E3BState &E = c.e3b;
     hipLaunchKernelGGL(k_sur_feat, dim3(cdiv((long long)E.nmol * kFeatLanes, 256)), dim3(256), 0, c.stream, E.nmol,
-                       E.molAtoms, c.xq, geo3(c), E.cap, E.nl, E.nlCount, raw);
+                       E.molAtoms, c.xq, c.box, E.cap, E.nl, E.nlCount, raw);
 }
 
 void surrogate_forces(Ctx &c, bool energy) {
@@ -649,7 +652,7 @@ void surrogate_forces(Ctx &c, bool energy) {
     hipLaunchKernelGGL(k_sur_mlp, dim3(cdiv(E.nmol, 16 * kWaves)), dim3(64 * kWaves), 0, c.stream, E.nmol, S.params, S.feat,
                        S.emol, S.wgt, c.acc, energy ? 1 : 0);
     hipLaunchKernelGGL(k_sur_force, dim3(cdiv((long long)E.nmol * kFeatLanes, 256)), dim3(256), 0, c.stream, E.nmol,
-                       E.molAtoms, c.xq, geo3(c), E.cap, E.nl, E.nlCount, S.wgt, c.force);
+                       E.molAtoms, c.xq, c.box, E.cap, E.nl, E.nlCount, S.wgt, c.force);
 }
 
 void surrogate_normalize(Ctx &c) {

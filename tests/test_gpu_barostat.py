@@ -80,7 +80,7 @@ def test_barostat_run(orc, gpu_engine_factory):
     kb, mvv = systems.REAL_BOLTZ, systems.REAL_MVV_TO_ENG
     e.nose_hoover(True, 298.0, 100.0, 10, kb, mvv)
     e.barostat(True, 1.0, 200.0, 10, 10.0, 1e-4, NKTV2P, mvv)
-    e.option("graph", 1)  # falls back to eager steps while the barostat is on
+    e.option("graph", 1)  # the barostat runs on the device inside captured steps
     L0 = box["L"].copy()
     e.run(400, flags)
     p = e.barostat_last()
@@ -102,3 +102,33 @@ def test_barostat_run(orc, gpu_engine_factory):
     excl = orc.exclusions_bfs(len(x), systems.water_bonds(box["mol"]))
     oc, orows, _ = oracle_rows_by_id(orc, x, lo, L, 8.0, excl=excl, brute=False)
     assert_same_lists(gc, grows, oc, orows)
+
+
+def test_barostat_graph_matches_eager(orc, gpu_engine_factory):
+    """The barostat runs on the device inside captured steps: box, pressure,
+    positions and the re-solved Ewald splitting agree with eager steps, and
+    alpha equals the oracle's secant solve for the final box."""
+    say("Berendsen barostat every step: captured graph vs eager steps")
+    box = systems.tip4p2005_box(216, seed=35, min_sep=1.8)
+    v = start_velocities(orc, box, seed=9)
+    mvv = systems.REAL_MVV_TO_ENG
+    out = []
+    for graph in (0, 1):
+        e, flags = water_md_engine(gpu_engine_factory, box, v, rc=7.0, skin=1.0, sz=(32, 32, 32), order=5,
+                                   e3b=True, dt=1.0)
+        e.barostat(True, 1.0, 200.0, 1, 10.0, 1e-4, NKTV2P, mvv)
+        e.option("graph", graph)
+        e.run(40, flags)
+        x, vv = e.get_system()
+        lo, L = e.get_box()
+        out.append((x, L, e.barostat_last(), e.pppm_info()[0]))
+    (xa, La, pa, aa), (xb, Lb, pb, ab) = out
+    say(f"box {La[0]:.9f} / {Lb[0]:.9f}, P {pa:.3f} / {pb:.3f} atm, alpha {aa:.12f} / {ab:.12f}")
+    assert La[0] != box["L"][0]
+    assert np.allclose(La, Lb, rtol=1e-10, atol=0)
+    assert pb == pytest.approx(pa, rel=1e-4, abs=0.05)
+    assert np.abs(min_image_diff(xa, xb, La)).max() < 1e-6
+    q2 = systems.REAL_QQR_TO_ENG * (box["q"] ** 2).sum()
+    a_orc, _ = orc.pppm_alpha(len(box["x"]), q2, Lb, np.array([32, 32, 32]), 5, 7.0)
+    assert ab == pytest.approx(a_orc, rel=1e-12)
+    assert aa == pytest.approx(a_orc, rel=1e-12)
