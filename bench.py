@@ -187,9 +187,12 @@ def copy_probe(mib=256, reps=5):
                       "launches": reps}))
 
 
-def train_surrogate(e, w, flags, frames=20, every=100, epochs=60, batch=8192):
+def train_surrogate(e, w, flags, frames=20, every=100, batch=1024,
+                    schedule=((100, 1e-3), (40, 3e-4), (40, 1e-4), (20, 3e-5))):
     """Record `frames` frames of the thermostatted run (every `every` steps)
-    and fit the surrogate on the GPU; returns fit statistics."""
+    and fit the surrogate on the GPU (Adam, mini-batches of `batch`
+    molecules, `schedule` = (epochs, learning rate) stages); returns fit
+    statistics."""
     from md_engine_amd import SURROGATE  # noqa: F401
     from md_engine_amd._lib import Engine
 
@@ -201,11 +204,12 @@ def train_surrogate(e, w, flags, frames=20, every=100, epochs=60, batch=8192):
     rng = np.random.default_rng(7)
     p = _init_surrogate(rng)
     e.surrogate_set(p)
-    l0 = e.surrogate_fit(1, batch=batch, lr=1e-3, seed=1, normalize=True)
-    l1 = e.surrogate_fit(epochs, batch=batch, lr=1e-3, seed=2)
-    l2 = e.surrogate_fit(epochs // 2, batch=batch, lr=2e-4, seed=3)
-    return dict(frames=frames, samples=frames * nm, epochs=epochs + epochs // 2 + 1, loss_first=l0,
-                loss_final=l2, train_s=time.perf_counter() - t0, model_params=Engine.SUR_NPARAM)
+    l0 = e.surrogate_fit(1, batch=8192, lr=1e-3, seed=1, normalize=True)
+    loss = l0
+    for k, (ep, lr) in enumerate(schedule):
+        loss = e.surrogate_fit(ep, batch=batch, lr=lr, seed=2 + k)
+    return dict(frames=frames, samples=frames * nm, epochs=1 + sum(ep for ep, _ in schedule), batch=batch,
+                loss_first=l0, loss_final=loss, train_s=time.perf_counter() - t0, model_params=Engine.SUR_NPARAM)
 
 
 def _init_surrogate(rng):

@@ -161,16 +161,20 @@ int mdx_teams_set(mdx_ctx *ctx, int build_team, int pair_team, int e3b_team);
 /* Learned E3B three-body surrogate.  No counterpart in the reference (its
  * FixE3B, src/Fixes/FixE3B.cu, only has the analytic three-body term); a
  * model replaces that term under MDX_SURROGATE while the two-body E3B term
- * stays analytic.  Model: per-molecule descriptors (16 radial Gaussians on
- * the H..O distances: accepted, H1-donated, H2-donated channels), MLP
- * 48 -> 64 -> 64 -> 1 with SiLU on BF16 MFMA tiles, forces from its
+ * stays analytic.  Model: per-molecule descriptors (14 radial functions
+ * exp(-kappa r) fc(r) on the H..O distances in three channels: accepted,
+ * H1-donated, H2-donated; three pair products of the E3B bond function over
+ * each neighbour: both own H to one O, both H of one neighbour to the own O,
+ * mutual donation), network input of 48 features (the channels, d1 + d2,
+ * d1 * d2, the pair products and three products of the E3B-basis channels),
+ * MLP 48 -> 64 -> 64 -> 1 with SiLU on BF16 MFMA tiles, forces from its
  * hand-written backward pass.  params: kSurNParam = 7459 floats packed as
  * W1[48][64] b1[64] W2[64][64] b2[64] w3[64] b3 mu[48] sd[48] escale eshift.
- * record: appends the current frame (features + the E3B three-body energy
+ * record: appends the current frame (raw channels + the E3B three-body energy
  * owned by each molecule, the training target) to the device sample set;
  * fit: Adam on the per-molecule energy MSE (normalize: set mu, sd, escale,
  * eshift from the samples first); eval: per-molecule energies (nmol) and raw
- * descriptor channels (nmol x 48), either may be NULL. */
+ * descriptor channels (nmol x 45), either may be NULL. */
 int mdx_surrogate_set(mdx_ctx *ctx, const float *params, int nparams);
 int mdx_surrogate_get(mdx_ctx *ctx, float *params, int nparams);
 int mdx_surrogate_record(mdx_ctx *ctx, int reset, int capacity);

@@ -333,6 +333,7 @@ class Engine:
 
     # -- learned E3B three-body surrogate -----------------------------------
     SUR_NPARAM = 7459
+    SUR_NRAW = 45  # raw descriptor channels per molecule
 
     def surrogate_set(self, params):
         p = np.ascontiguousarray(params, np.float32)
@@ -355,7 +356,7 @@ class Engine:
 
     def surrogate_eval(self, nmol):
         e = np.zeros(nmol, np.float32)
-        f = np.zeros((nmol, 48), np.float32)
+        f = np.zeros((nmol, self.SUR_NRAW), np.float32)
         self._chk(self._L.mdx_surrogate_eval(self._h, e.ctypes.data, f.ctypes.data), "mdx_surrogate_eval")
         return e, f
 

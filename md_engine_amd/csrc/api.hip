@@ -167,7 +167,7 @@ void compute(Ctx &c, int flags) {
     if ((flags & MDX_SURROGATE) && c.e3b.on) {
         MDX_REQUIRE(c.sur.on, MDX_ERR_STATE, "MDX_SURROGATE needs a model (mdx_surrogate_set)");
         e3b_forces(c, energy, virial, false);  // two-body term stays analytic
-        surrogate_forces(c, energy);
+        surrogate_forces(c, energy, virial);
     } else if ((flags & MDX_E3B) && c.e3b.on) {
         e3b_forces(c, energy, virial);
     }
@@ -1255,9 +1255,9 @@ static void surrogate_alloc(Ctx &c) {
             dfree(c, S.wgt);
         }
         S.nmolAlloc = c.e3b.nmol;
-        S.feat = dalloc<float>(c, (size_t)S.nmolAlloc * 3 * kSurNB);
+        S.feat = dalloc<float>(c, (size_t)S.nmolAlloc * kSurNR);
         S.emol = dalloc<float>(c, S.nmolAlloc);
-        S.wgt = dalloc<float>(c, (size_t)S.nmolAlloc * 3 * kSurNB);
+        S.wgt = dalloc<float>(c, (size_t)S.nmolAlloc * kSurNR);
     }
 }
 
@@ -1296,14 +1296,14 @@ int mdx_surrogate_record(mdx_ctx *ctx, int reset, int capacity) {
                 dfree(c, S.trainT);
             }
             S.capSamp = capacity;
-            S.trainX = dalloc<float>(c, (size_t)capacity * 3 * kSurNB);
+            S.trainX = dalloc<float>(c, (size_t)capacity * kSurNR);
             S.trainT = dalloc<float>(c, capacity);
         }
         MDX_REQUIRE(S.nsamp + c.e3b.nmol <= S.capSamp, MDX_ERR_ARG, "sample capacity exhausted");
         prepare_terms(c);
         ensure_list(c);
         refresh_xq(c);
-        surrogate_features(c, S.trainX + (size_t)S.nsamp * 3 * kSurNB);
+        surrogate_features(c, S.trainX + (size_t)S.nsamp * kSurNR);
         zero_forces(c, true);
         e3b_forces(c, false, false, true, S.trainT + S.nsamp);
         MDX_HIP(hipStreamSynchronize(c.stream));
@@ -1334,11 +1334,11 @@ int mdx_surrogate_eval(mdx_ctx *ctx, float *emol, float *feat) {
         ensure_list(c);
         refresh_xq(c);
         zero_forces(c, true);
-        surrogate_forces(c, false);
+        surrogate_forces(c, false, false);
         const size_t m = c.e3b.nmol;
         if (emol) MDX_HIP(hipMemcpyAsync(emol, c.sur.emol, sizeof(float) * m, hipMemcpyDeviceToHost, c.stream));
         if (feat)
-            MDX_HIP(hipMemcpyAsync(feat, c.sur.feat, sizeof(float) * m * 3 * kSurNB, hipMemcpyDeviceToHost,
+            MDX_HIP(hipMemcpyAsync(feat, c.sur.feat, sizeof(float) * m * kSurNR, hipMemcpyDeviceToHost,
                                    c.stream));
         MDX_HIP(hipStreamSynchronize(c.stream));
         c.forceFlags = -1;

@@ -294,15 +294,17 @@ struct BarostatState {
 };
 
 // learned E3B three-body surrogate (surrogate.hip)
-constexpr int kSurNB = 16, kSurNH = 64;
-constexpr int kSurNParam = 3 * kSurNB * kSurNH + kSurNH + kSurNH * kSurNH + kSurNH + kSurNH + 1 + 6 * kSurNB + 2;
+// surrogate sizes: radial basis functions, raw channels per molecule (acc,
+// d1, d2 per basis + three pair products), network inputs, hidden width
+constexpr int kSurNB = 14, kSurNR = 3 * kSurNB + 3, kSurNF = 3 * kSurNB + 6, kSurNH = 64;
+constexpr int kSurNParam = kSurNF * kSurNH + kSurNH + kSurNH * kSurNH + kSurNH + kSurNH + 1 + 2 * kSurNF + 2;
 
 struct SurrogateState {
     bool on = false;
     float *params = nullptr;  // packed model (kSurNParam)
-    float *feat = nullptr;    // per molecule raw channels acc, d1, d2 (48)
+    float *feat = nullptr;    // per molecule raw channels (kSurNR)
     float *emol = nullptr;    // per molecule energy
-    float *wgt = nullptr;     // per molecule dE / d(raw channel) (48)
+    float *wgt = nullptr;     // per molecule dE / d(raw channel) (kSurNR)
     int nmolAlloc = 0;
     float *trainX = nullptr;  // recorded samples: raw channels
     float *trainT = nullptr;  // recorded samples: E3B three-body energy of the molecule
@@ -580,7 +582,7 @@ void e3b_forces(Ctx &c, bool energy, bool virial, bool three_body = true, float 
 
 // surrogate.hip
 void surrogate_features(Ctx &c, float *raw);
-void surrogate_forces(Ctx &c, bool energy);
+void surrogate_forces(Ctx &c, bool energy, bool virial);
 void surrogate_normalize(Ctx &c);
 double surrogate_fit(Ctx &c, int epochs, int batch, double lr, unsigned seed);
 void fill_float(Ctx &c, float *p, int count, float v);

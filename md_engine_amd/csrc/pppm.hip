@@ -233,46 +233,6 @@ __device__ __forceinline__ double sinc_d(double x) {
     return sin(x) / x;
 }
 
-// Green function at one mesh point with the alias sums unrolled for a fixed
-// image range: the Gaussian exp(-|k'|^2 / 4 alpha^2) and the assignment
-// factor W(k')^2p both factor over the three axes, so each axis's terms are
-// computed once per image index (3 (2 LIM + 1) exp and sinc instead of
-// (2 LIM + 1)^3).
-template <int LIM>
-__device__ __forceinline__ double green_point(const double *k, const MeshGeo &m, double f4a2, int order) {
-    constexpr int NI = 2 * LIM + 1;
-    double kp[3][NI], ex[3][NI], w[3][NI];
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-#pragma unroll
-        for (int i = 0; i < NI; ++i) {
-            const double q = k[d] + 2.0 * kPi * (i - LIM) / m.h[d];
-            kp[d][i] = q;
-            ex[d][i] = exp(-q * q * f4a2);
-            const double W = sinc_d(q * m.h[d] * 0.5);
-            double W2 = 1.0;
-            for (int o = 0; o < 2 * order; ++o) W2 *= W;
-            w[d][i] = W2;
-        }
-    double s1 = 0.0, s2 = 0.0;
-#pragma unroll
-    for (int a = 0; a < NI; ++a)
-#pragma unroll
-        for (int b = 0; b < NI; ++b) {
-            const double wab = w[0][a] * w[1][b], eab = ex[0][a] * ex[1][b];
-            const double kab = kp[0][a] * kp[0][a] + kp[1][b] * kp[1][b];
-            const double dab = k[0] * kp[0][a] + k[1] * kp[1][b];
-#pragma unroll
-            for (int cc = 0; cc < NI; ++cc) {
-                const double W2 = wab * w[2][cc];
-                const double kl = kab + kp[2][cc] * kp[2][cc];
-                s1 += 4.0 * kPi * eab * ex[2][cc] * (dab + k[2] * kp[2][cc]) / kl * W2;
-                s2 += W2;
-            }
-        }
-    return s1 / (s2 * s2);
-}
-
 // optimal influence function (calc_Green_function, FixChargeEwald.cu:314-434)
 __device__ double green_point_any(const double *k, const MeshGeo &m, double f4a2, int order, int lim) {
     double s1 = 0.0, s2 = 0.0;
@@ -292,9 +252,10 @@ __device__ double green_point_any(const double *k, const MeshGeo &m, double f4a2
     return s1 / (s2 * s2);
 }
 
-// Optimal influence function on the half spectrum, FP64 arithmetic.
-__global__ void k_green(const int *__restrict__ gate, const BoxDev *__restrict__ bx, int szh, int order,
-                        float *__restrict__ G) {
+// Optimal influence function on the half spectrum, FP64 arithmetic, one
+// thread per point with the alias sums written out (any image range).
+__global__ void k_green_direct(const int *__restrict__ gate, const BoxDev *__restrict__ bx, int szh, int order,
+                               float *__restrict__ G) {
     if (gate && *gate == 0) return;
     const MeshGeo m = bx->mesh;
     const double alpha = bx->alpha;
@@ -309,15 +270,114 @@ __global__ void k_green(const int *__restrict__ gate, const BoxDev *__restrict__
         G[idx] = 0.0f;
         return;
     }
-    const double f4a2 = 0.25 / alpha / alpha;
-    double g;
-    switch (lim) {
-        case 0: g = green_point<0>(k, m, f4a2, order); break;
-        case 1: g = green_point<1>(k, m, f4a2, order); break;
-        case 2: g = green_point<2>(k, m, f4a2, order); break;
-        default: g = green_point_any(k, m, f4a2, order, lim); break;
+    G[idx] = (float)(green_point_any(k, m, 0.25 / alpha / alpha, order, lim) / k2);
+}
+
+// 1/x from the hardware estimate and two Newton steps (close to the
+// correctly rounded quotient; the Green function is stored in FP32)
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
+// One point of the tabulated Green function: the 3 (2 LIM + 1) table entries
+// of its indices are loaded first, then the (2 LIM + 1)^3 images are summed
+// in registers in the order of k_green_direct.
+template <int LIM>
+__device__ __forceinline__ float green_tab_point(const double *tkp, const double *tex, const double *tw, int off1,
+                                                 int off2, const MeshGeo &m, int ix, int iy, int iz) {
+    constexpr int NI = 2 * LIM + 1;
+    double kx[NI], ey[NI], wy[NI], ky[NI], ex[NI], wx[NI], kz[NI], ez[NI], wz[NI];
+#pragma unroll
+    for (int j = 0; j < NI; ++j) {
+        const int a = j * m.sz[0] + ix, b = off1 + j * m.sz[1] + iy, c = off2 + j * m.sz[2] + iz;
+        kx[j] = tkp[a];
+        ex[j] = tex[a];
+        wx[j] = tw[a];
+        ky[j] = tkp[b];
+        ey[j] = tex[b];
+        wy[j] = tw[b];
+        kz[j] = tkp[c];
+        ez[j] = tex[c];
+        wz[j] = tw[c];
     }
-    G[idx] = (float)(g / k2);
+    // image 0 of each axis is k itself
+    const double k0 = kx[LIM], k1 = ky[LIM], k2v = kz[LIM];
+    const double k2 = k0 * k0 + k1 * k1 + k2v * k2v;
+    if (k2 == 0.0) return 0.0f;
+    double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int a = 0; a < NI; ++a)
+#pragma unroll
+        for (int b = 0; b < NI; ++b) {
+            const double wab = wx[a] * wy[b], eab = ex[a] * ey[b];
+            const double kab = kx[a] * kx[a] + ky[b] * ky[b], dab = k0 * kx[a] + k1 * ky[b];
+#pragma unroll
+            for (int cc = 0; cc < NI; ++cc) {
+                const double W2 = wab * wz[cc];
+                const double kl = kab + kz[cc] * kz[cc];
+                s1 += eab * ez[cc] * (dab + k2v * kz[cc]) * rcp_nr(kl) * W2;
+                s2 += W2;
+            }
+        }
+    return (float)(4.0 * kPi * s1 / (s2 * s2) / k2);
+}
+
+// Same function with the per-axis factors tabulated: the Gaussian
+// exp(-|k'|^2 / 4 alpha^2) and the assignment factor W(k')^2p factor over the
+// axes, so for every axis d, mesh index i and image j the workgroup first
+// computes k'_d = k_d + 2 pi j / h_d, exp(-k'_d^2 / 4 alpha^2) and
+// sinc(k'_d h_d / 2)^2p once into LDS (3 (2 lim + 1) (sz_x + sz_y + sz_z)
+// entries); each point then sums its (2 lim + 1)^3 images from products of
+// table entries.  lim > kGreenTabLim takes the direct sums.
+constexpr int kGreenTabLim = 2;
+__global__ void __launch_bounds__(256) k_green(const int *__restrict__ gate, const BoxDev *__restrict__ bx,
+                                               int szh, int order, float *__restrict__ G) {
+    extern __shared__ double gtab[];
+    if (gate && *gate == 0) return;
+    const MeshGeo m = bx->mesh;
+    const double alpha = bx->alpha;
+    const int lim = bx->greenLim;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int total = m.sz[0] * m.sz[1] * szh;
+    const double f4a2 = 0.25 / alpha / alpha;
+    if (lim > kGreenTabLim) {  // uniform over the grid
+        if (idx >= total) return;
+        const int iz = idx % szh, iy = (idx / szh) % m.sz[1], ix = idx / (szh * m.sz[1]);
+        const double k[3] = {kcomp(ix, m.sz[0], m.L[0]), kcomp(iy, m.sz[1], m.L[1]), kcomp(iz, m.sz[2], m.L[2])};
+        const double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+        G[idx] = k2 == 0.0 ? 0.0f : (float)(green_point_any(k, m, f4a2, order, lim) / k2);
+        return;
+    }
+    const int NI = 2 * lim + 1;
+    const int off1 = NI * m.sz[0], off2 = NI * (m.sz[0] + m.sz[1]);
+    const int ne = NI * (m.sz[0] + m.sz[1] + m.sz[2]);
+    double *tkp = gtab, *tex = gtab + ne, *tw = gtab + 2 * ne;
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+        const bool z = e >= off2, y = !z && e >= off1;
+        const int n = z ? m.sz[2] : (y ? m.sz[1] : m.sz[0]);
+        const double Ld = z ? m.L[2] : (y ? m.L[1] : m.L[0]);
+        const double hd = z ? m.h[2] : (y ? m.h[1] : m.h[0]);
+        const int r = e - (z ? off2 : (y ? off1 : 0)), j = r / n, i = r - j * n;
+        const double q = kcomp(i, n, Ld) + 2.0 * kPi * (j - lim) / hd;
+        const double W = sinc_d(q * hd * 0.5);
+        double W2 = 1.0;
+        for (int o = 0; o < 2 * order; ++o) W2 *= W;
+        tkp[e] = q;
+        tex[e] = exp(-q * q * f4a2);
+        tw[e] = W2;
+    }
+    __syncthreads();
+    if (idx >= total) return;
+    const int iz = idx % szh, iy = (idx / szh) % m.sz[1], ix = idx / (szh * m.sz[1]);
+    float g;
+    switch (lim) {
+        case 0: g = green_tab_point<0>(tkp, tex, tw, off1, off2, m, ix, iy, iz); break;
+        case 1: g = green_tab_point<1>(tkp, tex, tw, off1, off2, m, ix, iy, iz); break;
+        default: g = green_tab_point<2>(tkp, tex, tw, off1, off2, m, ix, iy, iz); break;
+    }
+    G[idx] = g;
 }
 
 // E_hat_d = i k_d G rho_hat (E_field_cu), k_d := 0 on the d-th Nyquist plane.
@@ -548,7 +608,14 @@ void pppm_green_device(Ctx &c, const int *gate, hipStream_t s) {
     const PppmState &P = c.pppm;
     const int szh = P.sz[2] / 2 + 1;
     const size_t nh = (size_t)P.sz[0] * P.sz[1] * szh;
-    hipLaunchKernelGGL(k_green, dim3(cdiv(nh, 256)), dim3(256), 0, s, gate, c.box, szh, P.order, P.green);
+    // table of 3 (2 kGreenTabLim + 1) (sz_x + sz_y + sz_z) doubles; meshes too
+    // large for it take the direct kernel
+    const size_t tab = sizeof(double) * 3 * (2 * kGreenTabLim + 1) * (P.sz[0] + P.sz[1] + P.sz[2]);
+    if (tab <= 64 * 1024)
+        hipLaunchKernelGGL(k_green, dim3(cdiv(nh, 256)), dim3(256), tab, s, gate, c.box, szh, P.order, P.green);
+    else
+        hipLaunchKernelGGL(k_green_direct, dim3(cdiv(nh, 256)), dim3(256), 0, s, gate, c.box, szh, P.order,
+                           P.green);
 }
 
 void pppm_forces(Ctx &c, bool energy, bool virial) {

@@ -4,12 +4,17 @@
 // BF16 MFMA tiles (FP32 accumulation) with its hand-written backward pass for
 // forces, and on-device training (per-molecule energy regression, Adam).
 //
-// Descriptors of molecule i for 16 radial functions
-// g_k(r) = exp(-kappa_k r) fc(r)  (fc: the E3B switching polynomial RS..RF):
+// Raw channels of molecule i for 14 radial functions
+// g_k(r) = exp(-kappa_k r) fc(r), kappa_k = k3 (0.7 + 0.1 k)  (fc: the E3B
+// switching polynomial RS..RF; basis KE = 3 is the E3B bond function f,
+// basis K2 = 13 has kappa = 2 k3):
 //   acc_k = sum_{j != i, H of j} g_k(|H - O_i|),
-//   d1_k  = sum_{j != i} g_k(|H1_i - O_j|), d2_k likewise for H2_i;
-// network input F = [acc, d1 + d2, d1 * d2], standardised by (mu, sd).
-// Forces: dE/dF per molecule (MLP backward) -> per-channel pair weights ->
+//   d1_k  = sum_{j != i} g_k(|H1_i - O_j|), d2_k likewise for H2_i,
+//   pAA = sum_j f(H1_i - O_j) f(H2_i - O_j), pBB = sum_j f(H1_j - O_i) f(H2_j - O_i),
+//   pAB = sum_j D_ij D_ji  (D_ij = f(H1_i - O_j) + f(H2_i - O_j));
+// network input F = [acc, d1 + d2, d1 * d2, pAA, pBB, pAB, acc_KE s_KE,
+// acc_KE^2, s_KE^2] (s = d1 + d2), standardised by (mu, sd).
+// Forces: dE/dF per molecule (MLP backward) -> per-raw-channel weights ->
 // every owner recomputes its own pairs (deterministic, no cross-molecule
 // atomics beyond the fixed-point force buffer).
 #include "mdx_internal.h"
@@ -18,10 +23,13 @@ namespace mdx {
 
 namespace {
 
-constexpr int NB = kSurNB, NF = 3 * kSurNB, NH = kSurNH;
-// radial basis g_k(r) = exp(-kappa_k r) fc(r), kappa_k = k3 (0.5 + 0.1 k): basis 5
-// is exactly the E3B hydrogen-bond function (k3 = 1.907 / A, switching 5.0-5.2 A)
-constexpr float RS_C = 5.0f, RF_C = 5.2f, KAPPA0 = 0.5f * 1.907f, DKAPPA = 0.1f * 1.907f;
+constexpr int NB = kSurNB, NR = kSurNR, NF = kSurNF, NH = kSurNH;
+constexpr int KE = 3;  // basis with kappa = k3: the E3B hydrogen-bond function
+// raw channel offsets: acc, d1, d2 (NB each), then pAA, pBB, pAB
+constexpr int RD1 = NB, RD2 = 2 * NB, RAA = 3 * NB, RBB = 3 * NB + 1, RAB = 3 * NB + 2;
+// radial basis g_k(r) = exp(-kappa_k r) fc(r), kappa_k = k3 (0.7 + 0.1 k)
+// (k3 = 1.907 / A, switching 5.0-5.2 A)
+constexpr float RS_C = 5.0f, RF_C = 5.2f, KAPPA0 = 0.7f * 1.907f, DKAPPA = 0.1f * 1.907f;
 
 // parameter offsets in the packed model (oracle/surrogate.py layout)
 constexpr int OW1 = 0, OB1 = OW1 + NF * NH, OW2 = OB1 + NH, OB2 = OW2 + NH * NH, OW3 = OB2 + NH,
@@ -82,9 +90,9 @@ __device__ __forceinline__ float3 mi(float4 a, float4 b, const Geo3 &g) {
 
 // ---------------------------------------------------------------------------
 // descriptors: 16 lanes per molecule split its neighbours; each lane sums all
-// 48 channels over its neighbours in registers, the 16 partial rows go
-// through LDS and lane t adds up channel column t (acc, d1, d2).  A
-// neighbour whose O-O distance exceeds RF_C + r_OH cannot reach the cutoff.
+// 45 raw channels over its neighbours in registers, the 16 partial rows go
+// through LDS and lanes add up the channel columns.  A neighbour whose O-O
+// distance exceeds RF_C + r_OH cannot reach the cutoff.
 constexpr float kSkipOO2 = (RF_C + 1.0f) * (RF_C + 1.0f);
 constexpr int kFeatLanes = 16;
 
@@ -93,10 +101,11 @@ __global__ void __launch_bounds__(256) k_sur_feat(int nmol, const int4 *__restri
                                                   int cap, const int *__restrict__ nl,
                                                   const int *__restrict__ nlCount, float *__restrict__ feat) {
     const Geo3 g = geo3(bx);
-    __shared__ float part[256 / kFeatLanes][kFeatLanes][NF + 1];
+    __shared__ float part[256 / kFeatLanes][kFeatLanes][NR + 1];
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = gt / kFeatLanes, tl = threadIdx.x % kFeatLanes, team = threadIdx.x / kFeatLanes;
     float acc[NB], d1[NB], d2[NB];
+    float pAA = 0.f, pBB = 0.f, pAB = 0.f;
 #pragma unroll
     for (int k = 0; k < NB; ++k) acc[k] = d1[k] = d2[k] = 0.f;
     if (i < nmol) {
@@ -110,9 +119,12 @@ __global__ void __launch_bounds__(256) k_sur_feat(int nmol, const int4 *__restri
             const float3 oo = mi(Oj, Oi, g);
             if (oo.x * oo.x + oo.y * oo.y + oo.z * oo.z > kSkipOO2) continue;
             const float4 H1j = xq[aj.y], H2j = xq[aj.z];
+            // bonds H1_j, H2_j -> O_i (acceptor) and H1_i, H2_i -> O_j (donor)
             const float3 r4[4] = {mi(H1j, Oi, g), mi(H2j, Oi, g), mi(H1i, Oj, g), mi(H2i, Oj, g)};
+            float fb[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
+                fb[q] = 0.f;
                 const float r = sqrtf(r4[q].x * r4[q].x + r4[q].y * r4[q].y + r4[q].z * r4[q].z);
                 float sw, ds;
                 fcut(r, sw, ds);
@@ -125,31 +137,46 @@ __global__ void __launch_bounds__(256) k_sur_feat(int nmol, const int4 *__restri
                     if (q < 2) acc[k] += v;
                     else if (q == 2) d1[k] += v;
                     else d2[k] += v;
+                    if (k == KE) fb[q] = v;
                     v *= step;
                 }
             }
+            pAA += fb[2] * fb[3];
+            pBB += fb[0] * fb[1];
+            pAB += (fb[2] + fb[3]) * (fb[0] + fb[1]);
         }
     }
     float *pr = part[team][tl];
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
         pr[k] = acc[k];
-        pr[NB + k] = d1[k];
-        pr[2 * NB + k] = d2[k];
+        pr[RD1 + k] = d1[k];
+        pr[RD2 + k] = d2[k];
     }
+    pr[RAA] = pAA;
+    pr[RBB] = pBB;
+    pr[RAB] = pAB;
     __syncthreads();
     if (i < nmol) {
-        float s0 = 0.f, s1 = 0.f, s2 = 0.f;
-        for (int l = 0; l < kFeatLanes; ++l) {
-            s0 += part[team][l][tl];
-            s1 += part[team][l][NB + tl];
-            s2 += part[team][l][2 * NB + tl];
+        float *f = feat + (size_t)i * NR;
+        for (int col = tl; col < NR; col += kFeatLanes) {
+            float sum = 0.f;
+            for (int l = 0; l < kFeatLanes; ++l) sum += part[team][l][col];
+            f[col] = sum;
         }
-        float *f = feat + (size_t)i * NF;
-        f[tl] = s0;
-        f[NB + tl] = s1;
-        f[2 * NB + tl] = s2;
     }
+}
+
+// network input column `col` of one molecule from its raw channels
+__device__ __forceinline__ float feature_of(const float *__restrict__ f, int col) {
+    if (col < NB) return f[col];
+    if (col < 2 * NB) return f[RD1 + col - NB] + f[RD2 + col - NB];
+    if (col < 3 * NB) return f[RD1 + col - 2 * NB] * f[RD2 + col - 2 * NB];
+    if (col < 3 * NB + 3) return f[col];  // pAA, pBB, pAB
+    const float x = f[KE], sk = f[RD1 + KE] + f[RD2 + KE];
+    if (col == 3 * NB + 3) return x * sk;
+    if (col == 3 * NB + 4) return x * x;
+    return sk * sk;
 }
 
 // ---------------------------------------------------------------------------
@@ -193,25 +220,14 @@ struct WaveBuf {
     float T[16 * NH];  // dZ2, then dH1 / dZ1
 };
 
-// Standardised input rows of 16 samples from raw channels (acc, d1, d2).
+// Standardised input rows of 16 samples from their raw channels.
 __device__ __forceinline__ void load_x(WaveBuf &w, const float *__restrict__ raw, const int *rows, int nrow,
                                        const float *mu, const float *sd) {
     const int l = threadIdx.x & 63;
-    for (int e = l; e < 16 * NB; e += 64) {
-        const int s = e / NB, k = e % NB;
-        float a = 0.f, d1 = 0.f, d2 = 0.f;
-        if (s < nrow) {
-            const float *f = raw + (size_t)rows[s] * NF;
-            a = f[k];
-            d1 = f[NB + k];
-            d2 = f[2 * NB + k];
-        }
-        const float F[3] = {a, d1 + d2, d1 * d2};
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const int col = c * NB + k;
-            w.X[s * NF + col] = (F[c] - mu[col]) / sd[col];
-        }
+    for (int e = l; e < 16 * NF; e += 64) {
+        const int s = e / NF, col = e % NF;
+        const float v = s < nrow ? feature_of(raw + (size_t)rows[s] * NR, col) : 0.f;
+        w.X[s * NF + col] = (v - mu[col]) / sd[col];
     }
 }
 
@@ -343,48 +359,71 @@ __global__ void __launch_bounds__(64 * kWaves) k_sur_mlp(int nmol, const float *
 #pragma unroll
     for (int t = 0; t < 3; ++t) st16(w.X, NF, t * 16, dx[t]);  // X now holds dE/dx_hat
     wave_sync();
+    // dE/d(raw channel): d(d1 d2)/dd1 = d2, and the KE products
+    // x s, x^2, s^2 feed acc_KE, d1_KE and d2_KE
     for (int e = l; e < 16 * NB; e += 64) {
         const int s = e / NB, k = e % NB;
         if (s >= nrow) continue;
-        const float *f = raw + (size_t)(m0 + s) * NF;
-        const float d1 = f[NB + k], d2 = f[2 * NB + k];
-        const float ga = w.X[s * NF + k] / ssd[k];
-        const float gs = w.X[s * NF + NB + k] / ssd[NB + k];
-        const float gp = w.X[s * NF + 2 * NB + k] / ssd[2 * NB + k];
-        float *o = wgt + (size_t)(m0 + s) * NF;
+        const float *f = raw + (size_t)(m0 + s) * NR;
+        const float *gx = w.X + s * NF;
+        const float d1 = f[RD1 + k], d2 = f[RD2 + k];
+        float ga = gx[k] / ssd[k];
+        float gs = gx[NB + k] / ssd[NB + k];
+        const float gp = gx[2 * NB + k] / ssd[2 * NB + k];
+        if (k == KE) {
+            const float x = f[KE], sk = f[RD1 + KE] + f[RD2 + KE];
+            const float gxs = gx[3 * NB + 3] / ssd[3 * NB + 3], gxx = gx[3 * NB + 4] / ssd[3 * NB + 4],
+                        gss = gx[3 * NB + 5] / ssd[3 * NB + 5];
+            ga += gxs * sk + 2.0f * gxx * x;
+            gs += gxs * x + 2.0f * gss * sk;
+        }
+        float *o = wgt + (size_t)(m0 + s) * NR;
         o[k] = ga;
-        o[NB + k] = gs + gp * d2;
-        o[2 * NB + k] = gs + gp * d1;
+        o[RD1 + k] = gs + gp * d2;
+        o[RD2 + k] = gs + gp * d1;
+    }
+    for (int e = l; e < 16 * 3; e += 64) {
+        const int s = e / 3, c = e % 3;
+        if (s >= nrow) continue;
+        wgt[(size_t)(m0 + s) * NR + RAA + c] = w.X[s * NF + 3 * NB + c] / ssd[3 * NB + c];
     }
     if (energy && l == 0 && esum != 0.0) fx_add(&acc_rep(acc)[ACC_E3B3], esum);
 }
 
 // ---------------------------------------------------------------------------
 // forces: 16 lanes per molecule split its neighbours (owner-computes); each
-// lane holds the molecule's 48 channel weights, gathers the neighbour's, and
-// the lanes' partial forces are summed by shuffles
+// lane holds the molecule's 45 raw-channel weights, gathers the neighbour's,
+// and the lanes' partial forces are summed by shuffles.  A bond H -> O of the
+// pair (i, j) with value a = f feeds: the donor channel of H's molecule and
+// the acceptor channel of O's molecule (radial weights), the product of the
+// two bonds of H's molecule to that O (pAA of H's molecule, pBB of O's), and
+// the mutual product D_ij D_ji (pAB of both).  VIR: the virial of every
+// bond, -(dE/dr / r) d (x) d, half from each of its two owners.
+template <bool VIR>
 __global__ void __launch_bounds__(256) k_sur_force(int nmol, const int4 *__restrict__ molAtoms,
                                                    const float4 *__restrict__ xq, const BoxDev *__restrict__ bx,
                                                    int cap, const int *__restrict__ nl,
                                                    const int *__restrict__ nlCount, const float *__restrict__ wgt,
-                                                   unsigned long long *__restrict__ force) {
+                                                   unsigned long long *__restrict__ force,
+                                                   unsigned long long *__restrict__ acc) {
     const Geo3 g = geo3(bx);
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = gt / kFeatLanes, tl = threadIdx.x % kFeatLanes;
     float3 FO = make_float3(0.f, 0.f, 0.f), F1 = FO, F2 = FO;
+    float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
     int4 ai = make_int4(0, 0, 0, 0);
     if (i < nmol) {
         ai = molAtoms[i];
         const float4 Oi = xq[ai.x], H1i = xq[ai.y], H2i = xq[ai.z];
+        const float *wi = wgt + (size_t)i * NR;
         float wa_i[NB], w1_i[NB], w2_i[NB];
-        const float4 *wi = (const float4 *)(wgt + (size_t)i * NF);
 #pragma unroll
-        for (int v = 0; v < NB / 4; ++v) {
-            const float4 a4 = wi[v], b4 = wi[NB / 4 + v], c4 = wi[2 * NB / 4 + v];
-            wa_i[4 * v] = a4.x; wa_i[4 * v + 1] = a4.y; wa_i[4 * v + 2] = a4.z; wa_i[4 * v + 3] = a4.w;
-            w1_i[4 * v] = b4.x; w1_i[4 * v + 1] = b4.y; w1_i[4 * v + 2] = b4.z; w1_i[4 * v + 3] = b4.w;
-            w2_i[4 * v] = c4.x; w2_i[4 * v + 1] = c4.y; w2_i[4 * v + 2] = c4.z; w2_i[4 * v + 3] = c4.w;
+        for (int k = 0; k < NB; ++k) {
+            wa_i[k] = wi[k];
+            w1_i[k] = wi[RD1 + k];
+            w2_i[k] = wi[RD2 + k];
         }
+        const float wAA_i = wi[RAA], wBB_i = wi[RBB], wAB_i = wi[RAB];
         const int cnt = min(nlCount[i], cap);
         const int *row = nl + (size_t)i * cap;
         for (int t = tl; t < cnt; t += kFeatLanes) {
@@ -394,29 +433,50 @@ __global__ void __launch_bounds__(256) k_sur_force(int nmol, const int4 *__restr
             const float3 oo = mi(Oj, Oi, g);
             if (oo.x * oo.x + oo.y * oo.y + oo.z * oo.z > kSkipOO2) continue;
             const float4 H1j = xq[aj.y], H2j = xq[aj.z];
-            const float *wj = wgt + (size_t)mj * NF;
-            // an H of i feeds i's donor channel and j's acceptor channel;
-            // an H of j feeds j's donor channel and i's acceptor channel
+            const float *wj = wgt + (size_t)mj * NR;
+            // q = 0, 1: H1_i, H2_i -> O_j (i's donor channels, j's acceptor);
+            // q = 2, 3: H1_j, H2_j -> O_i (j's donor channels, i's acceptor)
             const float3 d[4] = {mi(H1i, Oj, g), mi(H2i, Oj, g), mi(H1j, Oi, g), mi(H2j, Oi, g)};
+            float r[4], sw[4], ds[4], fb[4], dfb[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const float r = sqrtf(d[q].x * d[q].x + d[q].y * d[q].y + d[q].z * d[q].z);
-                float s, ds;
-                fcut(r, s, ds);
-                if (s <= 0.f && ds == 0.f) continue;
+                r[q] = sqrtf(d[q].x * d[q].x + d[q].y * d[q].y + d[q].z * d[q].z);
+                fcut(r[q], sw[q], ds[q]);
+                const float ek = __expf(-(KAPPA0 + DKAPPA * KE) * r[q]);
+                fb[q] = ek * sw[q];
+                dfb[q] = ek * (ds[q] - (KAPPA0 + DKAPPA * KE) * sw[q]);
+            }
+            const float wAA_j = wj[RAA], wBB_j = wj[RBB], wAB = wAB_i + wj[RAB];
+            const float Dij = fb[0] + fb[1], Dji = fb[2] + fb[3];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (sw[q] <= 0.f && ds[q] == 0.f) continue;
                 // d/dr [exp(-kappa r) fc] = exp(-kappa r) (fc' - kappa fc)
                 float dEdr = 0.f;
-                const float step = __expf(-DKAPPA * r);
-                float ek = __expf(-KAPPA0 * r);
+                const float step = __expf(-DKAPPA * r[q]);
+                float ek = __expf(-KAPPA0 * r[q]);
 #pragma unroll
                 for (int k = 0; k < NB; ++k) {
                     const float w = q == 0 ? w1_i[k] + wj[k]
                                   : q == 1 ? w2_i[k] + wj[k]
-                                  : q == 2 ? wj[NB + k] + wa_i[k] : wj[2 * NB + k] + wa_i[k];
-                    dEdr += w * ek * (ds - kappa_k(k) * s);
+                                  : q == 2 ? wj[RD1 + k] + wa_i[k] : wj[RD2 + k] + wa_i[k];
+                    dEdr += w * ek * (ds[q] - kappa_k(k) * sw[q]);
                     ek *= step;
                 }
-                const float c = dEdr / r;
+                // pair products: the other bond of the same hydrogen pair, and D of the reverse direction
+                const float other = fb[q ^ 1];
+                const float wpair = q < 2 ? wAA_i + wBB_j : wAA_j + wBB_i;
+                dEdr += (wpair * other + wAB * (q < 2 ? Dji : Dij)) * dfb[q];
+                const float c = dEdr / r[q];
+                if (VIR) {
+                    const float h = -0.5f * c;
+                    v0 += h * d[q].x * d[q].x;
+                    v1 += h * d[q].y * d[q].y;
+                    v2 += h * d[q].z * d[q].z;
+                    v3 += h * d[q].x * d[q].y;
+                    v4 += h * d[q].x * d[q].z;
+                    v5 += h * d[q].y * d[q].z;
+                }
                 // F_H = -c d, F_O = +c d  (d = r_H - r_O)
                 if (q == 0) {
                     F1.x -= c * d[q].x;
@@ -453,6 +513,22 @@ __global__ void __launch_bounds__(256) k_sur_force(int nmol, const int4 *__restr
         fx_add(&force[3 * ai.z + 0], F2.x);
         fx_add(&force[3 * ai.z + 1], F2.y);
         fx_add(&force[3 * ai.z + 2], F2.z);
+    }
+    if (VIR) {
+        // wave sums, then one atomic per workgroup and component
+        __shared__ double red[4][6];
+        const double vv[6] = {v0, v1, v2, v3, v4, v5};
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const double x = team_sum_d<64>(vv[k]);
+            if (lane == 0) red[w][k] = x;
+        }
+        __syncthreads();
+        if (threadIdx.x < 6) {
+            const double x = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+            if (x != 0.0) fx_add(&acc_rep(acc)[ACC_VIR + threadIdx.x], x);
+        }
     }
 }
 
@@ -602,10 +678,7 @@ __global__ void __launch_bounds__(256) k_sur_norm(int nsamp, const float *__rest
         if (col == NF) {
             v = target[s];
         } else {
-            const float *f = raw + (size_t)s * NF;
-            const int c = col / NB, k = col % NB;
-            const float d1 = f[NB + k], d2 = f[2 * NB + k];
-            v = c == 0 ? f[k] : (c == 1 ? d1 + d2 : d1 * d2);
+            v = feature_of(raw + (size_t)s * NR, col);
         }
         a += v;
         b += (double)v * v;
@@ -645,14 +718,19 @@ void surrogate_features(Ctx &c, float *raw) {
                        E.molAtoms, c.xq, c.box, E.cap, E.nl, E.nlCount, raw);
 }
 
-void surrogate_forces(Ctx &c, bool energy) {
+void surrogate_forces(Ctx &c, bool energy, bool virial) {
     SurrogateState &S = c.sur;
     E3BState &E = c.e3b;
     surrogate_features(c, S.feat);
     hipLaunchKernelGGL(k_sur_mlp, dim3(cdiv(E.nmol, 16 * kWaves)), dim3(64 * kWaves), 0, c.stream, E.nmol, S.params, S.feat,
                        S.emol, S.wgt, c.acc, energy ? 1 : 0);
-    hipLaunchKernelGGL(k_sur_force, dim3(cdiv((long long)E.nmol * kFeatLanes, 256)), dim3(256), 0, c.stream, E.nmol,
-                       E.molAtoms, c.xq, c.box, E.cap, E.nl, E.nlCount, S.wgt, c.force);
+    const dim3 grid(cdiv((long long)E.nmol * kFeatLanes, 256));
+    if (virial)
+        hipLaunchKernelGGL(k_sur_force<true>, grid, dim3(256), 0, c.stream, E.nmol, E.molAtoms, c.xq, c.box, E.cap,
+                           E.nl, E.nlCount, S.wgt, c.force, c.acc);
+    else
+        hipLaunchKernelGGL(k_sur_force<false>, grid, dim3(256), 0, c.stream, E.nmol, E.molAtoms, c.xq, c.box, E.cap,
+                           E.nl, E.nlCount, S.wgt, c.force, c.acc);
 }
 
 void surrogate_normalize(Ctx &c) {

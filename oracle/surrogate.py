@@ -7,16 +7,26 @@ The surrogate has no counterpart in the reference: it is a learned
 replacement for the E3B three-body energy of FixE3B (src/Fixes/FixE3B.cu,
 src/Evaluators/ThreeBodyE3B.h).  Its definition, shared with the HIP kernels:
 
-* descriptors per molecule i (48): for 16 radial functions
-  g_k(r) = exp(-KAPPA_k r) fc(r), KAPPA_k = k3 (0.5 + 0.1 k) (basis 5 is the
-  E3B hydrogen-bond function itself), fc the E3B switching polynomial between
-  RS_C = 5.0 and RF_C = 5.2,
+* raw channels per molecule i (45): for 14 radial functions
+  g_k(r) = exp(-KAPPA_k r) fc(r), KAPPA_k = k3 (0.7 + 0.1 k) (basis KE = 3 is
+  the E3B hydrogen-bond function f itself, basis K2 = 13 has 2 k3, i.e. f^2
+  up to the switch), fc the E3B switching polynomial between RS_C = 5.0 and
+  RF_C = 5.2,
     acc_k = sum_{j != i} sum_{H of j} g_k(|H - O_i|)   (hydrogens donated to O_i)
     d1_k  = sum_{j != i} g_k(|H1_i - O_j|),  d2_k likewise for H2_i,
-  features F = [acc, d1 + d2, d1 * d2] (symmetric in H1 <-> H2);
+  and three pair-product channels with f = g_KE over the same neighbours j,
+    pAA = sum_j f(H1_i - O_j) f(H2_i - O_j)      (both my H towards one O_j)
+    pBB = sum_j f(H1_j - O_i) f(H2_j - O_i)      (both H of j towards my O)
+    pAB = sum_j D_ij D_ji,  D_ij = f(H1_i - O_j) + f(H2_i - O_j);
+* network input F (48) = [acc, d1 + d2, d1 * d2, pAA, pBB, pAB,
+  acc_KE (d1 + d2)_KE, acc_KE^2, (d1 + d2)_KE^2] (symmetric in H1 <-> H2);
 * x = (F - mu) / sd; h1 = silu(x W1 + b1); h2 = silu(h1 W2 + b2);
   e = h2 . w3 + b3; E_i = escale e + eshift; E = sum_i E_i;
-* forces = -dE/dpositions through the descriptors (chain rule below).
+* forces = -dE/dpositions through the raw channels (chain rule below).
+
+The pair-product channels and the products of the KE channels are the
+three-body correlations the E3B energy is built from (its per-molecule
+energy is linear in F), so the network can reach the E3B forces closely.
 
 Pinned by: finite differences of ``energy`` against ``forces`` (FP64);
 ``descriptors`` against a per-pair loop (``descriptors_loop``); the per-molecule
@@ -27,10 +37,13 @@ from __future__ import annotations
 
 import numpy as np
 
-NB = 16
-NF = 3 * NB
+NB = 14
+KE = 3    # basis with kappa = k3 (the E3B function)
+K2 = 13   # basis with kappa = 2 k3
+NR = 3 * NB + 3
+NF = 3 * NB + 6
 NH = 64
-KAPPA = 1.907 * (0.5 + 0.1 * np.arange(NB))
+KAPPA = 1.907 * (0.7 + 0.1 * np.arange(NB))
 RS_C, RF_C = 5.0, 5.2
 NPARAM = NF * NH + NH + NH * NH + NH + NH + 1 + NF + NF + 2
 
@@ -102,43 +115,82 @@ def _mi(d, L):
     return d - L * np.rint(d / L)
 
 
+def _pair_channels(mol, x, L, pr):
+    """g (n, NB, 4) of the four H-O distances of each directed pair (i, j):
+    H1_i - O_j, H2_i - O_j, H1_j - O_i, H2_j - O_i; and the distance vectors"""
+    i, j = pr[:, 0], pr[:, 1]
+    dr = [_mi(x[mol[i, 1]] - x[mol[j, 0]], L), _mi(x[mol[i, 2]] - x[mol[j, 0]], L),
+          _mi(x[mol[j, 1]] - x[mol[i, 0]], L), _mi(x[mol[j, 2]] - x[mol[i, 0]], L)]
+    return dr
+
+
 def descriptors(mol, x, L):
-    """raw per-molecule channels acc, d1, d2 (M, NB each)"""
+    """raw per-molecule channels (M, NR): acc, d1, d2 (NB each), pAA, pBB, pAB"""
     M = len(mol)
     pr = _pairs(mol, x, L)
-    i, j = pr[:, 0], pr[:, 1]
-    acc = np.zeros((M, NB))
-    d1 = np.zeros((M, NB))
-    d2 = np.zeros((M, NB))
-    for h, dh in ((1, d1), (2, d2)):
-        dr = _mi(x[mol[i, h]] - x[mol[j, 0]], L)      # H_h(i) -> O(j)
-        g, _ = basis(np.linalg.norm(dr, axis=1))
-        np.add.at(dh, i, g)
-        np.add.at(acc, j, g)
-    return acc, d1, d2
+    i = pr[:, 0]
+    dr = _pair_channels(mol, x, L, pr)
+    g = [basis(np.linalg.norm(d, axis=1))[0] for d in dr]
+    raw = np.zeros((M, NR))
+    np.add.at(raw[:, NB:2 * NB], i, g[0])
+    np.add.at(raw[:, 2 * NB:3 * NB], i, g[1])
+    np.add.at(raw[:, :NB], i, g[2] + g[3])
+    a1, a2, b1, b2 = (gq[:, KE] for gq in g)
+    np.add.at(raw[:, 3 * NB], i, a1 * a2)
+    np.add.at(raw[:, 3 * NB + 1], i, b1 * b2)
+    np.add.at(raw[:, 3 * NB + 2], i, (a1 + a2) * (b1 + b2))
+    return raw
 
 
 def descriptors_loop(mol, x, L):
     """the same channels by a plain double loop (small systems)"""
     M = len(mol)
-    acc = np.zeros((M, NB))
-    d1 = np.zeros((M, NB))
-    d2 = np.zeros((M, NB))
+    raw = np.zeros((M, NR))
+
+    def gk(a, b):
+        r = np.linalg.norm(_mi(a - b, L))
+        return basis(np.array([r]))[0][0] if r < RF_C else np.zeros(NB)
+
     for i in range(M):
         for j in range(M):
             if i == j:
                 continue
-            for h, dh in ((1, d1), (2, d2)):
-                r = np.linalg.norm(_mi(x[mol[i, h]] - x[mol[j, 0]], L))
-                if r < RF_C:
-                    g, _ = basis(np.array([r]))
-                    dh[i] += g[0]
-                    acc[j] += g[0]
-    return acc, d1, d2
+            if np.linalg.norm(_mi(x[mol[i, 0]] - x[mol[j, 0]], L)) >= 6.2:
+                continue
+            a1, a2 = gk(x[mol[i, 1]], x[mol[j, 0]]), gk(x[mol[i, 2]], x[mol[j, 0]])
+            b1, b2 = gk(x[mol[j, 1]], x[mol[i, 0]]), gk(x[mol[j, 2]], x[mol[i, 0]])
+            raw[i, NB:2 * NB] += a1
+            raw[i, 2 * NB:3 * NB] += a2
+            raw[i, :NB] += b1 + b2
+            raw[i, 3 * NB] += a1[KE] * a2[KE]
+            raw[i, 3 * NB + 1] += b1[KE] * b2[KE]
+            raw[i, 3 * NB + 2] += (a1[KE] + a2[KE]) * (b1[KE] + b2[KE])
+    return raw
 
 
-def features(acc, d1, d2):
-    return np.concatenate([acc, d1 + d2, d1 * d2], axis=1)
+def features(raw):
+    acc, d1, d2 = raw[:, :NB], raw[:, NB:2 * NB], raw[:, 2 * NB:3 * NB]
+    s = d1 + d2
+    x5, s5 = acc[:, KE], s[:, KE]
+    return np.concatenate([acc, s, d1 * d2, raw[:, 3 * NB:], (x5 * s5)[:, None], (x5 * x5)[:, None],
+                           (s5 * s5)[:, None]], axis=1)
+
+
+def raw_weights(dF, raw):
+    """dE/d(raw channel) (M, NR) from dE/dF (M, NF)"""
+    acc, d1, d2 = raw[:, :NB], raw[:, NB:2 * NB], raw[:, 2 * NB:3 * NB]
+    x5, s5 = acc[:, KE], d1[:, KE] + d2[:, KE]
+    w = np.zeros_like(raw)
+    w[:, :NB] = dF[:, :NB]
+    w[:, NB:2 * NB] = dF[:, NB:2 * NB] + dF[:, 2 * NB:3 * NB] * d2
+    w[:, 2 * NB:3 * NB] = dF[:, NB:2 * NB] + dF[:, 2 * NB:3 * NB] * d1
+    w[:, 3 * NB:] = dF[:, 3 * NB:3 * NB + 3]
+    gx = dF[:, 3 * NB + 3] * s5 + 2.0 * dF[:, 3 * NB + 4] * x5
+    gs = dF[:, 3 * NB + 3] * x5 + 2.0 * dF[:, 3 * NB + 5] * s5
+    w[:, KE] += gx
+    w[:, NB + KE] += gs
+    w[:, 2 * NB + KE] += gs
+    return w
 
 
 def silu(z):
@@ -167,29 +219,44 @@ def mlp(p, F):
 
 
 def energy(p, mol, x, L):
-    acc, d1, d2 = descriptors(mol, x, L)
-    E, _ = mlp(p, features(acc, d1, d2))
+    E, _ = mlp(p, features(descriptors(mol, x, L)))
     return E.sum()
 
 
-def forces(p, mol, x, L):
-    """-dE/dx (n, 3) for the site positions x (M sites unaffected)."""
-    acc, d1, d2 = descriptors(mol, x, L)
-    E, dF = mlp(p, features(acc, d1, d2))
-    wa = dF[:, :NB]
-    w1 = dF[:, NB:2 * NB] + dF[:, 2 * NB:] * d2
-    w2 = dF[:, NB:2 * NB] + dF[:, 2 * NB:] * d1
+def forces(p, mol, x, L, virial=False):
+    """-dE/dx (n, 3) for the site positions x (M sites unaffected); with
+    virial=True also the virial sum over H-O bonds of d (x) F_H (d = r_H - r_O),
+    as (xx, yy, zz, xy, xz, yz).
+
+    Every H-O bond H_h(i) - O_j belongs to one directed pair (i, j); its
+    value a_h = f feeds i's donor channel h, j's acceptor channel, the
+    products a1 a2 (pAA of i, pBB of j) and D_ij D_ji (pAB of i and of j)."""
+    raw = descriptors(mol, x, L)
+    E, dF = mlp(p, features(raw))
+    w = raw_weights(dF, raw)
+    wacc, wd = w[:, :NB], (w[:, NB:2 * NB], w[:, 2 * NB:3 * NB])
+    wAA, wBB, wAB = w[:, 3 * NB], w[:, 3 * NB + 1], w[:, 3 * NB + 2]
     f = np.zeros_like(x)
     pr = _pairs(mol, x, L)
     i, j = pr[:, 0], pr[:, 1]
-    for h, wh in ((1, w1), (2, w2)):
-        dr = _mi(x[mol[i, h]] - x[mol[j, 0]], L)
-        r = np.linalg.norm(dr, axis=1)
-        _, dg = basis(r)
-        c = ((wh[i] + wa[j]) * dg).sum(1)             # dE/dr of the pair H_h(i) - O(j)
-        fv = -(c / np.where(r > 0, r, 1.0))[:, None] * dr
-        np.add.at(f, mol[i, h], fv)
+    dr = _pair_channels(mol, x, L, pr)
+    r = [np.linalg.norm(d, axis=1) for d in dr]
+    gg = [basis(rq) for rq in r]
+    a = [gg[q][0][:, KE] for q in range(4)]
+    W = np.zeros(6)
+    for h in (0, 1):
+        g, dg = gg[h]
+        c = ((wd[h][i] + wacc[j]) * dg).sum(1)
+        c += (wAA[i] + wBB[j]) * a[1 - h] * dg[:, KE]
+        c += (wAB[i] + wAB[j]) * (a[2] + a[3]) * dg[:, KE]
+        fv = -(c / np.where(r[h] > 0, r[h], 1.0))[:, None] * dr[h]
+        np.add.at(f, mol[i, h + 1], fv)
         np.add.at(f, mol[j, 0], -fv)
+        d = dr[h]
+        W += [(d[:, 0] * fv[:, 0]).sum(), (d[:, 1] * fv[:, 1]).sum(), (d[:, 2] * fv[:, 2]).sum(),
+              (d[:, 0] * fv[:, 1]).sum(), (d[:, 0] * fv[:, 2]).sum(), (d[:, 1] * fv[:, 2]).sum()]
+    if virial:
+        return f, E, W
     return f, E
 
 

@@ -42,6 +42,21 @@ def e2_forces(box, params):
     return f
 
 
+def e2_virial(box, params):
+    """two-body E3B virial sum d (x) F over O-O pairs (xx, yy, zz, xy, xz, yz)"""
+    from scipy.spatial import cKDTree
+
+    mol, x, L = box["mol"], box["x"], box["L"]
+    O = x[mol[:, 0]]
+    pairs = cKDTree(O % L, boxsize=L).query_pairs(params.r2b, output_type="ndarray")
+    d = O[pairs[:, 0]] - O[pairs[:, 1]]
+    d -= L * np.rint(d / L)
+    r = np.linalg.norm(d, axis=1)
+    F = d * (params.k2 * params.E2 * np.exp(-params.k2 * r) / r)[:, None]
+    return np.array([(d[:, 0] * F[:, 0]).sum(), (d[:, 1] * F[:, 1]).sum(), (d[:, 2] * F[:, 2]).sum(),
+                     (d[:, 0] * F[:, 1]).sum(), (d[:, 0] * F[:, 2]).sum(), (d[:, 1] * F[:, 2]).sum()])
+
+
 def random_model(seed, feats):
     rng = np.random.default_rng(seed)
     m = S.unpack(S.init_params(seed))
@@ -70,14 +85,14 @@ def test_surrogate_forward_backward_vs_oracle(orc, gpu_engine_factory):
     from md_engine_amd import ENERGY, SURROGATE
 
     box = systems.tip4p2005_box(216, seed=9, min_sep=1.8)
-    acc, d1, d2 = S.descriptors(box["mol"], box["x"], box["L"])
-    p = random_model(1, S.features(acc, d1, d2))
+    raw = S.descriptors(box["mol"], box["x"], box["L"])
+    p = random_model(1, S.features(raw))
     e, _ = water_md_engine(gpu_engine_factory, box, np.zeros_like(box["x"]), rc=7.0, skin=1.0,
                            sz=(32, 32, 32), order=5, e3b=True, dt=1.0)
     e.surrogate_set(p)
     eg, fg = e.surrogate_eval(len(box["mol"]))
-    raw = np.concatenate([acc, d1, d2], axis=1)
-    assert np.allclose(fg, raw, rtol=1e-4, atol=1e-5 * np.abs(raw).max())
+    for c0, c1 in ((0, 3 * S.NB), (3 * S.NB, S.NR)):  # radial channels, pair products
+        assert np.allclose(fg[:, c0:c1], raw[:, c0:c1], rtol=1e-4, atol=1e-5 * np.abs(raw[:, c0:c1]).max())
     fo, eo = S.forces(p.astype(np.float64), box["mol"], box["x"], box["L"])
     # BF16 operands, FP32 accumulation: a few 1e-3 of the energy spread
     assert np.abs(eg - eo).max() < 2e-2 * (np.abs(eo - eo.mean()).max() + 1e-3)
@@ -86,6 +101,15 @@ def test_surrogate_forward_backward_vs_oracle(orc, gpu_engine_factory):
     assert rel_rmse(f, fo) < 3e-2
     en, _ = e.energies()
     assert en[3] == pytest.approx(eo.sum(), rel=2e-2)
+    # virial of the surrogate term (FP32 bond sums against the FP64 oracle)
+    from md_engine_amd import VIRIAL
+
+    e.compute(SURROGATE | VIRIAL)
+    _, vir = e.energies()
+    _, _, wo = S.forces(p.astype(np.float64), box["mol"], box["x"], box["L"], virial=True)
+    wg = vir - e2_virial(box, orc.e3b_params())
+    assert np.abs(wo[:3]).max() > 1e-3
+    assert np.allclose(wg, wo, rtol=0, atol=3e-2 * np.abs(wo).max())
 
 
 def test_surrogate_training_gain(orc, gpu_engine_factory):
@@ -105,9 +129,10 @@ def test_surrogate_training_gain(orc, gpu_engine_factory):
         e.run(50, flags)
     e.surrogate_set(S.init_params(3).astype(np.float32))
     l0 = e.surrogate_fit(1, batch=512, lr=1e-3, seed=1, normalize=True)
-    l1 = e.surrogate_fit(400, batch=512, lr=1e-3, seed=2)
-    say(f"loss {l0:.4f} -> {l1:.4f}")
-    assert l1 < 0.5 * l0
+    l1 = e.surrogate_fit(300, batch=512, lr=1e-3, seed=2)
+    l1 = e.surrogate_fit(100, batch=512, lr=1e-4, seed=3)
+    say(f"loss {l0:.4f} -> {l1:.6f}")
+    assert l1 < 0.01 * l0
     e.run(100, flags)
     e.compute(E3B)
     f_exact = e.forces()
@@ -120,4 +145,7 @@ def test_surrogate_training_gain(orc, gpu_engine_factory):
     err_sur = np.sqrt(((f_sur - f_exact) ** 2).sum(1).mean())
     err_drop = np.sqrt((three ** 2).sum(1).mean())
     say(f"three-body force rms: dropped {err_drop:.4f}, surrogate error {err_sur:.4f} kcal/mol/A")
-    assert err_sur < 0.8 * err_drop
+    # the network inputs hold the correlations the E3B energy is built from:
+    # a fit on eight frames reproduces the three-body forces of a later frame
+    # to a few percent
+    assert err_sur < 0.1 * err_drop

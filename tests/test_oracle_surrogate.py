@@ -18,9 +18,29 @@ def test_descriptors_match_loop(small_box):
     b = small_box
     a1 = S.descriptors(b["mol"], b["x"], b["L"])
     a2 = S.descriptors_loop(b["mol"], b["x"], b["L"])
-    for u, v in zip(a1, a2):
-        assert np.allclose(u, v, rtol=1e-12, atol=1e-14)
-    assert a1[0].sum() > 0 and a1[1].sum() > 0
+    assert np.allclose(a1, a2, rtol=1e-12, atol=1e-14)
+    # every channel family is populated
+    for c0, c1 in ((0, S.NB), (S.NB, 2 * S.NB), (2 * S.NB, 3 * S.NB), (3 * S.NB, S.NR)):
+        assert a1[:, c0:c1].sum() > 0
+
+
+def test_e3b_energy_is_linear_in_the_features(orc, small_box):
+    """The per-molecule E3B three-body energy (pinned to the C oracle below)
+    is Ea (p_KE - pAA) + Eb (acc_KE s_KE - pAB) + 0.5 Ec (acc_KE^2 - acc_K2 - 2 pBB)
+    in the network inputs, up to the switch entering acc_K2 once instead of
+    twice (f^2 = exp(-2 k3 r) s^2)."""
+    b = small_box
+    P = orc.e3b_params()
+    F = S.features(S.descriptors(b["mol"], b["x"], b["L"]))
+    NB, KE, K2 = S.NB, S.KE, S.K2
+    lin = (P.Ea * (F[:, 2 * NB + KE] - F[:, 3 * NB]) + P.Eb * (F[:, 3 * NB + 3] - F[:, 3 * NB + 2])
+           + 0.5 * P.Ec * (F[:, 3 * NB + 4] - F[:, K2] - 2 * F[:, 3 * NB + 1]))
+    em = S.e3b_molecule_energies(b["mol"], b["x"], b["L"])
+    assert np.abs(em).max() > 1e-3
+    # each bond in the switching shell (5.0-5.2 A) leaves at most
+    # 0.5 Ec exp(-2 k3 rs) max(s - s^2) = 3e-7 kcal/mol
+    assert np.allclose(lin, em, rtol=0, atol=1e-5 * np.abs(em).max())
+    assert not np.allclose(lin, 0.0)
 
 
 def test_forces_are_minus_gradient(small_box):
@@ -48,6 +68,22 @@ def test_forces_are_minus_gradient(small_box):
     assert np.all(f[b["mol"][:, 3]] == 0)
 
 
+def test_virial_trace_is_minus_scaling_derivative(small_box):
+    """E depends on minimum-image H-O vectors only, so scaling positions and
+    box by lambda scales every bond: trace W = -dE/dlambda at lambda = 1."""
+    b = small_box
+    p = S.init_params(5)
+    m = S.unpack(p)
+    m["escale"] = 0.7
+    p = S.pack(m)
+    _, _, W = S.forces(p, b["mol"], b["x"], b["L"], virial=True)
+    h = 1e-6
+    ep = S.energy(p, b["mol"], b["x"] * (1 + h), b["L"] * (1 + h))
+    em = S.energy(p, b["mol"], b["x"] * (1 - h), b["L"] * (1 - h))
+    assert abs(W[:3].sum()) > 1e-6
+    assert W[:3].sum() == pytest.approx(-(ep - em) / (2 * h), rel=1e-6)
+
+
 def test_energy_translation_invariant(small_box):
     b = small_box
     p = S.init_params(2)
@@ -64,12 +100,13 @@ def test_molecule_energies_sum_to_total(orc):
     assert em.sum() == pytest.approx(e3, rel=1e-10)
 
 
-def test_basis_five_is_the_e3b_bond_function(orc):
+def test_basis_ke_is_the_e3b_bond_function(orc):
     P = orc.e3b_params()
     r = np.linspace(0.8, 5.3, 200)
     g, dg = S.basis(r)
     s = np.where(r <= P.rs, 1.0, np.where(r > P.rf, 0.0, (P.rf - r) ** 2 * (P.rf + 2 * r - 3 * P.rs) / (P.rf - P.rs) ** 3))
-    assert np.allclose(g[:, 5], np.exp(-P.k3 * r) * s, rtol=1e-12, atol=0)
+    assert np.allclose(g[:, S.KE], np.exp(-P.k3 * r) * s, rtol=1e-12, atol=0)
+    assert np.allclose(g[:, S.K2], np.exp(-2 * P.k3 * r) * s, rtol=1e-12, atol=0)
     h = 1e-6
     gp, _ = S.basis(r + h)
     gm, _ = S.basis(r - h)
