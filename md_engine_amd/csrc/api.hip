@@ -1267,6 +1267,7 @@ int mdx_surrogate_set(mdx_ctx *ctx, const float *params, int nparams) {
         MDX_REQUIRE(params && nparams == kSurNParam, MDX_ERR_ARG, "surrogate parameter count mismatch");
         surrogate_alloc(c);
         upload(c, c.sur.params, params, kSurNParam);
+        surrogate_pack(c);
         MDX_HIP(hipStreamSynchronize(c.stream));
         c.sur.on = true;
         c.forceFlags = -1;

@@ -305,6 +305,7 @@ struct SurrogateState {
     float *feat = nullptr;    // per molecule raw channels (kSurNR)
     float *emol = nullptr;    // per molecule energy
     float *wgt = nullptr;     // per molecule dE / d(raw channel) (kSurNR)
+    unsigned short *frag = nullptr;  // BF16 MFMA operand fragments of W1, W2 (both orientations)
     int nmolAlloc = 0;
     float *trainX = nullptr;  // recorded samples: raw channels
     float *trainT = nullptr;  // recorded samples: E3B three-body energy of the molecule
@@ -584,6 +585,8 @@ void e3b_forces(Ctx &c, bool energy, bool virial, bool three_body = true, float 
 void surrogate_features(Ctx &c, float *raw);
 void surrogate_forces(Ctx &c, bool energy, bool virial);
 void surrogate_normalize(Ctx &c);
+// BF16 weight fragments of the inference kernel from the FP32 parameters
+void surrogate_pack(Ctx &c);
 double surrogate_fit(Ctx &c, int epochs, int batch, double lr, unsigned seed);
 void fill_float(Ctx &c, float *p, int count, float v);
 

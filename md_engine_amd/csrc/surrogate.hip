@@ -47,6 +47,10 @@ __device__ __forceinline__ short to_bf16(float f) {
     return (short)(u >> 16);
 }
 
+__device__ __forceinline__ float bf16_round(float f) {
+    return __uint_as_float(((unsigned)(unsigned short)to_bf16(f)) << 16);
+}
+
 __device__ __forceinline__ void fcut(float r, float &s, float &ds) {
     const float inv = 1.0f / ((RF_C - RS_C) * (RF_C - RS_C) * (RF_C - RS_C));
     if (r <= RS_C) {
@@ -300,94 +304,199 @@ __device__ __forceinline__ void backward_hidden(WaveBuf &w, const float *gscale,
     wave_sync();
 }
 
-constexpr int kWaves = 2;  // waves (16-molecule tiles) per workgroup of the MLP kernels
-
-struct SurModel {
-    const float *p;  // packed parameters (device)
-};
+constexpr int kWaves = 2;  // waves (16-molecule tiles) per workgroup of the training kernel
 
 // ---------------------------------------------------------------------------
-// inference: energies and dE/d(raw channel) per molecule, 4 waves x 16 molecules
-__global__ void __launch_bounds__(64 * kWaves) k_sur_mlp(int nmol, const float *__restrict__ P,
-                                                 const float *__restrict__ raw, float *__restrict__ emol,
-                                                 float *__restrict__ wgt, unsigned long long *__restrict__ acc,
-                                                 int energy) {
-    __shared__ float sW1[NF * NH], sW2[NH * NH], sb1[NH], sb2[NH], sw3[NH], smu[NF], ssd[NF];
-    __shared__ WaveBuf wb[kWaves];
-    __shared__ float sg[kWaves][16], se[kWaves][16];
-    __shared__ int srow[kWaves][16];
-    for (int e = threadIdx.x; e < NF * NH; e += blockDim.x) sW1[e] = P[OW1 + e];
-    for (int e = threadIdx.x; e < NH * NH; e += blockDim.x) sW2[e] = P[OW2 + e];
-    for (int e = threadIdx.x; e < NH; e += blockDim.x) {
-        sb1[e] = P[OB1 + e];
-        sb2[e] = P[OB2 + e];
-        sw3[e] = P[OW3 + e];
+// inference with the activations kept in registers.  Every product is taken
+// transposed (hidden x molecules): the C tile of one layer, rows 4 (l / 16) + j
+// and column l % 16 of lane l, is exactly the B operand layout of the next
+// layer's k-block, so a 16-molecule tile runs forward and backward through
+// both layers without touching LDS.  The weights come as BF16 A-operand
+// fragments packed once per model (k_sur_pack): lane l of fragment (t, kb)
+// holds 4 consecutive k of row l % 16.
+//   Z1^T = W1^T X^T, Z2^T = W2^T H1^T (forward), dH1^T = W2 dZ2^T, dX^T = W1 dZ1^T
+constexpr int FR_A1 = 0;                    // W1^T: 4 hidden tiles x 3 feature blocks
+constexpr int FR_A2 = FR_A1 + 12;           // W2^T: 4 x 4
+constexpr int FR_A3 = FR_A2 + 16;           // W2:   4 x 4
+constexpr int FR_A4 = FR_A3 + 16;           // W1:   3 feature tiles x 4 hidden blocks
+constexpr int FR_N = FR_A4 + 12;
+static_assert(FR_N * 64 * 4 == 56 * 256, "fragment table size");
+
+// one thread per (fragment, lane): 4 BF16 values
+__global__ void k_sur_pack(const float *__restrict__ P, unsigned short *__restrict__ frag) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= FR_N * 64) return;
+    const int fr = e / 64, l = e % 64, r = l & 15, q = l >> 4;
+    const float *W1 = P + OW1, *W2 = P + OW2;
+    for (int j = 0; j < 4; ++j) {
+        float v;
+        if (fr < FR_A2) {  // A = W1^T: row hidden 16 t + r, k feature 16 kb + 4 q + j
+            const int t = (fr - FR_A1) / 3, kb = (fr - FR_A1) % 3;
+            v = W1[(16 * kb + 4 * q + j) * NH + 16 * t + r];
+        } else if (fr < FR_A3) {  // A = W2^T: row hidden2 16 t + r, k hidden1 16 kb + 4 q + j
+            const int t = (fr - FR_A2) / 4, kb = (fr - FR_A2) % 4;
+            v = W2[(16 * kb + 4 * q + j) * NH + 16 * t + r];
+        } else if (fr < FR_A4) {  // A = W2: row hidden1 16 t + r, k hidden2 16 kb + 4 q + j
+            const int t = (fr - FR_A3) / 4, kb = (fr - FR_A3) % 4;
+            v = W2[(16 * t + r) * NH + 16 * kb + 4 * q + j];
+        } else {  // A = W1: row feature 16 t + r, k hidden 16 kb + 4 q + j
+            const int t = (fr - FR_A4) / 4, kb = (fr - FR_A4) % 4;
+            v = W1[(16 * t + r) * NH + 16 * kb + 4 * q + j];
+        }
+        frag[(size_t)e * 4 + j] = (unsigned short)to_bf16(v);
     }
-    for (int e = threadIdx.x; e < NF; e += blockDim.x) {
-        smu[e] = P[OMU + e];
-        ssd[e] = P[OSD + e];
-    }
-    __syncthreads();
-    const float b3 = P[OB3], es = P[OES], eh = P[OEH];
-    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
-    WaveBuf &w = wb[wv];
-    const int m0 = (blockIdx.x * kWaves + wv) * 16;
+}
+
+__device__ __forceinline__ bf16x4 ld_frag(const unsigned short *__restrict__ frag, int fr, int l) {
+    const uint2 u = *(const uint2 *)(frag + ((size_t)fr * 64 + l) * 4);
+    bf16x4 a;
+    a[0] = (short)(u.x & 0xFFFFu);
+    a[1] = (short)(u.x >> 16);
+    a[2] = (short)(u.y & 0xFFFFu);
+    a[3] = (short)(u.y >> 16);
+    return a;
+}
+
+__device__ __forceinline__ bf16x4 pack4(f32x4 v) {
+    bf16x4 b;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[j] = to_bf16(v[j]);
+    return b;
+}
+
+constexpr int kMlpWaves = 4;
+
+__global__ void __launch_bounds__(64 * kMlpWaves) k_sur_infer(int nmol, const float *__restrict__ P,
+                                                              const unsigned short *__restrict__ frag,
+                                                              const float *__restrict__ raw, float *__restrict__ emol,
+                                                              float *__restrict__ wgt,
+                                                              unsigned long long *__restrict__ acc, int energy) {
+    __shared__ float sdx[kMlpWaves][16][NF + 1];
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+    const int m0 = (blockIdx.x * kMlpWaves + wv) * 16;
     if (m0 >= nmol) return;  // whole wave; no block barrier follows
     const int nrow = min(16, nmol - m0);
-    if (l < 16) srow[wv][l] = m0 + min(l, nrow - 1);
-    wave_sync();
-    load_x(w, raw, srow[wv], nrow, smu, ssd);
-    wave_sync();
-    forward(w, sW1, sb1, sW2, sb2);
-    out_rows(w, sw3, b3, se[wv]);
-    wave_sync();
-    double esum = 0.0;
-    if (l < 16) sg[wv][l] = l < nrow ? es : 0.f;  // dE_i/de = escale
-    if (l == 0)
-        for (int s = 0; s < nrow; ++s) {
-            const float e = es * se[wv][s] + eh;
-            emol[m0 + s] = e;
-            esum += (double)e;
+    const int m = m0 + min(r, nrow - 1);
+    const float *fm = raw + (size_t)m * NR;
+    // X^T k-blocks: lane holds features 16 kb + 4 q + j of molecule r
+    bf16x4 xb[3];
+#pragma unroll
+    for (int kb = 0; kb < 3; ++kb) {
+        f32x4 v;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int col = 16 * kb + 4 * q + j;
+            v[j] = (feature_of(fm, col) - P[OMU + col]) / P[OSD + col];
         }
-    wave_sync();
-    backward_hidden(w, sg[wv], sw3, sW2);
-    // dX = dZ1 W1^T (16 x 48), then to raw channels
-    f32x4 dx[3];
+        xb[kb] = pack4(v);
+    }
+    // layer 1
+    f32x4 z1[4];
+    bf16x4 h1[4];
 #pragma unroll
-    for (int t = 0; t < 3; ++t) dx[t] = mm16(w.Z1, NH, 1, sW1 + t * 16 * NH, 1, NH, NH, zero4());
-    wave_sync();
+    for (int t = 0; t < 4; ++t) {
+        f32x4 c = zero4();
 #pragma unroll
-    for (int t = 0; t < 3; ++t) st16(w.X, NF, t * 16, dx[t]);  // X now holds dE/dx_hat
+        for (int kb = 0; kb < 3; ++kb)
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld_frag(frag, FR_A1 + 3 * t + kb, l), xb[kb], c, 0, 0, 0);
+        f32x4 h;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            c[j] += P[OB1 + 16 * t + 4 * q + j];
+            h[j] = silu(c[j]);
+        }
+        z1[t] = c;
+        h1[t] = pack4(h);
+    }
+    // layer 2 and the output row
+    f32x4 z2[4];
+    float part = 0.f;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        f32x4 c = zero4();
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb)
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld_frag(frag, FR_A2 + 4 * t + kb, l), h1[kb], c, 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            c[j] += P[OB2 + 16 * t + 4 * q + j];
+            // the output layer as in the training kernel: BF16 H2 and w3
+            part += bf16_round(silu(c[j])) * bf16_round(P[OW3 + 16 * t + 4 * q + j]);
+        }
+        z2[t] = c;
+    }
+    part += __shfl_xor(part, 16);
+    part += __shfl_xor(part, 32);
+    const float es = P[OES];
+    const float e = es * (part + P[OB3]) + P[OEH];
+    double esum = 0.0;
+    if (q == 0 && r < nrow) {
+        emol[m0 + r] = e;
+        esum = (double)e;
+    }
+    // backward: dZ2^T = es w3 silu'(Z2^T)
+    bf16x4 dz2[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        f32x4 d;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) d[j] = es * P[OW3 + 16 * t + 4 * q + j] * dsilu(z2[t][j]);
+        dz2[t] = pack4(d);
+    }
+    // dZ1^T = (W2 dZ2^T) silu'(Z1^T)
+    bf16x4 dz1[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        f32x4 c = zero4();
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb)
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld_frag(frag, FR_A3 + 4 * t + kb, l), dz2[kb], c, 0, 0, 0);
+        f32x4 d;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) d[j] = c[j] * dsilu(z1[t][j]);
+        dz1[t] = pack4(d);
+    }
+    // dX^T = W1 dZ1^T, then dE/dF = dX / sd into LDS (row = molecule)
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        f32x4 c = zero4();
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb)
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld_frag(frag, FR_A4 + 4 * t + kb, l), dz1[kb], c, 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int col = 16 * t + 4 * q + j;
+            sdx[wv][r][col] = c[j] / P[OSD + col];
+        }
+    }
     wave_sync();
-    // dE/d(raw channel): d(d1 d2)/dd1 = d2, and the KE products
-    // x s, x^2, s^2 feed acc_KE, d1_KE and d2_KE
-    for (int e = l; e < 16 * NB; e += 64) {
-        const int s = e / NB, k = e % NB;
+    // dE/d(raw channel) per molecule
+    for (int e2 = l; e2 < 16 * NB; e2 += 64) {
+        const int s = e2 / NB, k = e2 % NB;
         if (s >= nrow) continue;
         const float *f = raw + (size_t)(m0 + s) * NR;
-        const float *gx = w.X + s * NF;
+        const float *gx = sdx[wv][s];
         const float d1 = f[RD1 + k], d2 = f[RD2 + k];
-        float ga = gx[k] / ssd[k];
-        float gs = gx[NB + k] / ssd[NB + k];
-        const float gp = gx[2 * NB + k] / ssd[2 * NB + k];
+        float ga = gx[k], gs = gx[NB + k];
+        const float gp = gx[2 * NB + k];
         if (k == KE) {
             const float x = f[KE], sk = f[RD1 + KE] + f[RD2 + KE];
-            const float gxs = gx[3 * NB + 3] / ssd[3 * NB + 3], gxx = gx[3 * NB + 4] / ssd[3 * NB + 4],
-                        gss = gx[3 * NB + 5] / ssd[3 * NB + 5];
-            ga += gxs * sk + 2.0f * gxx * x;
-            gs += gxs * x + 2.0f * gss * sk;
+            ga += gx[3 * NB + 3] * sk + 2.0f * gx[3 * NB + 4] * x;
+            gs += gx[3 * NB + 3] * x + 2.0f * gx[3 * NB + 5] * sk;
         }
         float *o = wgt + (size_t)(m0 + s) * NR;
         o[k] = ga;
         o[RD1 + k] = gs + gp * d2;
         o[RD2 + k] = gs + gp * d1;
     }
-    for (int e = l; e < 16 * 3; e += 64) {
-        const int s = e / 3, c = e % 3;
-        if (s >= nrow) continue;
-        wgt[(size_t)(m0 + s) * NR + RAA + c] = w.X[s * NF + 3 * NB + c] / ssd[3 * NB + c];
+    if (l < 48) {
+        const int s = l / 3, c3 = l % 3;
+        if (s < nrow) wgt[(size_t)(m0 + s) * NR + RAA + c3] = sdx[wv][s][3 * NB + c3];
     }
-    if (energy && l == 0 && esum != 0.0) fx_add(&acc_rep(acc)[ACC_E3B3], esum);
+    if (energy) {
+        esum = team_sum_d<64>(esum);
+        if (l == 0 && esum != 0.0) fx_add(&acc_rep(acc)[ACC_E3B3], esum);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -722,8 +831,8 @@ void surrogate_forces(Ctx &c, bool energy, bool virial) {
     SurrogateState &S = c.sur;
     E3BState &E = c.e3b;
     surrogate_features(c, S.feat);
-    hipLaunchKernelGGL(k_sur_mlp, dim3(cdiv(E.nmol, 16 * kWaves)), dim3(64 * kWaves), 0, c.stream, E.nmol, S.params, S.feat,
-                       S.emol, S.wgt, c.acc, energy ? 1 : 0);
+    hipLaunchKernelGGL(k_sur_infer, dim3(cdiv(E.nmol, 16 * kMlpWaves)), dim3(64 * kMlpWaves), 0, c.stream, E.nmol,
+                       S.params, S.frag, S.feat, S.emol, S.wgt, c.acc, energy ? 1 : 0);
     const dim3 grid(cdiv((long long)E.nmol * kFeatLanes, 256));
     if (virial)
         hipLaunchKernelGGL(k_sur_force<true>, grid, dim3(256), 0, c.stream, E.nmol, E.molAtoms, c.xq, c.box, E.cap,
@@ -731,6 +840,12 @@ void surrogate_forces(Ctx &c, bool energy, bool virial) {
     else
         hipLaunchKernelGGL(k_sur_force<false>, grid, dim3(256), 0, c.stream, E.nmol, E.molAtoms, c.xq, c.box, E.cap,
                            E.nl, E.nlCount, S.wgt, c.force, c.acc);
+}
+
+void surrogate_pack(Ctx &c) {
+    SurrogateState &S = c.sur;
+    if (!S.frag) S.frag = dalloc<unsigned short>(c, (size_t)FR_N * 64 * 4);
+    hipLaunchKernelGGL(k_sur_pack, dim3(cdiv(FR_N * 64, 256)), dim3(256), 0, c.stream, S.params, S.frag);
 }
 
 void surrogate_normalize(Ctx &c) {
@@ -782,6 +897,7 @@ double surrogate_fit(Ctx &c, int epochs, int batch, double lr, unsigned seed) {
         }
         if (nb) last = esum / nb;
     }
+    surrogate_pack(c);
     return last;
 }
 
