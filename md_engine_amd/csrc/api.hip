@@ -1257,7 +1257,7 @@ static void surrogate_alloc(Ctx &c) {
         S.nmolAlloc = c.e3b.nmol;
         S.feat = dalloc<float>(c, (size_t)S.nmolAlloc * kSurNR);
         S.emol = dalloc<float>(c, S.nmolAlloc);
-        S.wgt = dalloc<float>(c, (size_t)S.nmolAlloc * kSurNR);
+        S.wgt = dalloc<float>(c, (size_t)S.nmolAlloc * kSurNW);
     }
 }
 

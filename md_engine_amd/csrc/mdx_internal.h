@@ -297,6 +297,7 @@ struct BarostatState {
 // surrogate sizes: radial basis functions, raw channels per molecule (acc,
 // d1, d2 per basis + three pair products), network inputs, hidden width
 constexpr int kSurNB = 14, kSurNR = 3 * kSurNB + 3, kSurNF = 3 * kSurNB + 6, kSurNH = 64;
+constexpr int kSurNW = 48;  // row stride of the per-molecule weights (16-byte rows)
 constexpr int kSurNParam = kSurNF * kSurNH + kSurNH + kSurNH * kSurNH + kSurNH + kSurNH + 1 + 2 * kSurNF + 2;
 
 struct SurrogateState {
@@ -304,7 +305,7 @@ struct SurrogateState {
     float *params = nullptr;  // packed model (kSurNParam)
     float *feat = nullptr;    // per molecule raw channels (kSurNR)
     float *emol = nullptr;    // per molecule energy
-    float *wgt = nullptr;     // per molecule dE / d(raw channel) (kSurNR)
+    float *wgt = nullptr;     // per molecule dE / d(raw channel) (kSurNR, row stride kSurNW)
     unsigned short *frag = nullptr;  // BF16 MFMA operand fragments of W1, W2 (both orientations)
     int nmolAlloc = 0;
     float *trainX = nullptr;  // recorded samples: raw channels

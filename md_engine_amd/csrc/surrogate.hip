@@ -23,7 +23,8 @@ namespace mdx {
 
 namespace {
 
-constexpr int NB = kSurNB, NR = kSurNR, NF = kSurNF, NH = kSurNH;
+constexpr int NB = kSurNB, NR = kSurNR, NF = kSurNF, NH = kSurNH, NW = kSurNW;
+static_assert(NW >= NR && NW % 4 == 0, "weight rows hold the raw channels in float4 units");
 constexpr int KE = 3;  // basis with kappa = k3: the E3B hydrogen-bond function
 // raw channel offsets: acc, d1, d2 (NB each), then pAA, pBB, pAB
 constexpr int RD1 = NB, RD2 = 2 * NB, RAA = 3 * NB, RBB = 3 * NB + 1, RAB = 3 * NB + 2;
@@ -65,9 +66,12 @@ __device__ __forceinline__ void fcut(float r, float &s, float &ds) {
     }
 }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// logistic by the hardware reciprocal (about 1 ulp; the network runs on BF16
+// operands)
+__device__ __forceinline__ float sigm(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
 __device__ __forceinline__ float dsilu(float z) {
-    const float s = 1.0f / (1.0f + __expf(-z));
+    const float s = sigm(z);
     return s * (1.0f + z * (1.0f - s));
 }
 
@@ -93,12 +97,50 @@ __device__ __forceinline__ float3 mi(float4 a, float4 b, const Geo3 &g) {
 }
 
 // ---------------------------------------------------------------------------
-// descriptors: 16 lanes per molecule split its neighbours; each lane sums all
-// 45 raw channels over its neighbours in registers, the 16 partial rows go
-// through LDS and lanes add up the channel columns.  A neighbour whose O-O
-// distance exceeds RF_C + r_OH cannot reach the cutoff.
+// descriptors: 16 lanes per molecule split its neighbours, two neighbours
+// per lane and round with all their loads in flight together; each lane sums
+// all 45 raw channels in registers, the 16 partial rows go through LDS and
+// lanes add up the channel columns.  A neighbour whose O-O distance exceeds
+// RF_C + r_OH cannot reach the cutoff.
 constexpr float kSkipOO2 = (RF_C + 1.0f) * (RF_C + 1.0f);
 constexpr int kFeatLanes = 16;
+
+struct FeatAcc {
+    float acc[NB], d1[NB], d2[NB];
+    float pAA, pBB, pAB;
+};
+
+// channels of one neighbour j of molecule i
+__device__ __forceinline__ void feat_pair(FeatAcc &A, float4 Oi, float4 H1i, float4 H2i, float4 Oj, float4 H1j,
+                                          float4 H2j, const Geo3 &g) {
+    const float3 oo = mi(Oj, Oi, g);
+    if (oo.x * oo.x + oo.y * oo.y + oo.z * oo.z > kSkipOO2) return;
+    // bonds H1_j, H2_j -> O_i (acceptor) and H1_i, H2_i -> O_j (donor)
+    const float3 r4[4] = {mi(H1j, Oi, g), mi(H2j, Oi, g), mi(H1i, Oj, g), mi(H2i, Oj, g)};
+    float fb[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        fb[q] = 0.f;
+        const float r = sqrtf(r4[q].x * r4[q].x + r4[q].y * r4[q].y + r4[q].z * r4[q].z);
+        float sw, ds;
+        fcut(r, sw, ds);
+        if (sw <= 0.f) continue;
+        // exp(-kappa_k r) by recurrence: two exponentials per distance
+        const float step = __expf(-DKAPPA * r);
+        float v = __expf(-KAPPA0 * r) * sw;
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            if (q < 2) A.acc[k] += v;
+            else if (q == 2) A.d1[k] += v;
+            else A.d2[k] += v;
+            if (k == KE) fb[q] = v;
+            v *= step;
+        }
+    }
+    A.pAA += fb[2] * fb[3];
+    A.pBB += fb[0] * fb[1];
+    A.pAB += (fb[2] + fb[3]) * (fb[0] + fb[1]);
+}
 
 __global__ void __launch_bounds__(256) k_sur_feat(int nmol, const int4 *__restrict__ molAtoms,
                                                   const float4 *__restrict__ xq, const BoxDev *__restrict__ bx,
@@ -108,58 +150,35 @@ __global__ void __launch_bounds__(256) k_sur_feat(int nmol, const int4 *__restri
     __shared__ float part[256 / kFeatLanes][kFeatLanes][NR + 1];
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = gt / kFeatLanes, tl = threadIdx.x % kFeatLanes, team = threadIdx.x / kFeatLanes;
-    float acc[NB], d1[NB], d2[NB];
-    float pAA = 0.f, pBB = 0.f, pAB = 0.f;
+    FeatAcc A;
 #pragma unroll
-    for (int k = 0; k < NB; ++k) acc[k] = d1[k] = d2[k] = 0.f;
+    for (int k = 0; k < NB; ++k) A.acc[k] = A.d1[k] = A.d2[k] = 0.f;
+    A.pAA = A.pBB = A.pAB = 0.f;
     if (i < nmol) {
         const int4 ai = molAtoms[i];
         const float4 Oi = xq[ai.x], H1i = xq[ai.y], H2i = xq[ai.z];
         const int cnt = min(nlCount[i], cap);
         const int *row = nl + (size_t)i * cap;
-        for (int t = tl; t < cnt; t += kFeatLanes) {
-            const int4 aj = molAtoms[row[t]];
-            const float4 Oj = xq[aj.x];
-            const float3 oo = mi(Oj, Oi, g);
-            if (oo.x * oo.x + oo.y * oo.y + oo.z * oo.z > kSkipOO2) continue;
-            const float4 H1j = xq[aj.y], H2j = xq[aj.z];
-            // bonds H1_j, H2_j -> O_i (acceptor) and H1_i, H2_i -> O_j (donor)
-            const float3 r4[4] = {mi(H1j, Oi, g), mi(H2j, Oi, g), mi(H1i, Oj, g), mi(H2i, Oj, g)};
-            float fb[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                fb[q] = 0.f;
-                const float r = sqrtf(r4[q].x * r4[q].x + r4[q].y * r4[q].y + r4[q].z * r4[q].z);
-                float sw, ds;
-                fcut(r, sw, ds);
-                if (sw <= 0.f) continue;
-                // exp(-kappa_k r) by recurrence: two exponentials per distance
-                const float step = __expf(-DKAPPA * r);
-                float v = __expf(-KAPPA0 * r) * sw;
-#pragma unroll
-                for (int k = 0; k < NB; ++k) {
-                    if (q < 2) acc[k] += v;
-                    else if (q == 2) d1[k] += v;
-                    else d2[k] += v;
-                    if (k == KE) fb[q] = v;
-                    v *= step;
-                }
-            }
-            pAA += fb[2] * fb[3];
-            pBB += fb[0] * fb[1];
-            pAB += (fb[2] + fb[3]) * (fb[0] + fb[1]);
+        for (int t = tl; t < cnt; t += 2 * kFeatLanes) {
+            const bool two = t + kFeatLanes < cnt;
+            const int mA = row[t], mB = two ? row[t + kFeatLanes] : mA;
+            const int4 aA = molAtoms[mA], aB = molAtoms[mB];
+            const float4 OA = xq[aA.x], H1A = xq[aA.y], H2A = xq[aA.z];
+            const float4 OB = xq[aB.x], H1B = xq[aB.y], H2B = xq[aB.z];
+            feat_pair(A, Oi, H1i, H2i, OA, H1A, H2A, g);
+            if (two) feat_pair(A, Oi, H1i, H2i, OB, H1B, H2B, g);
         }
     }
     float *pr = part[team][tl];
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        pr[k] = acc[k];
-        pr[RD1 + k] = d1[k];
-        pr[RD2 + k] = d2[k];
+        pr[k] = A.acc[k];
+        pr[RD1 + k] = A.d1[k];
+        pr[RD2 + k] = A.d2[k];
     }
-    pr[RAA] = pAA;
-    pr[RBB] = pBB;
-    pr[RAB] = pAB;
+    pr[RAA] = A.pAA;
+    pr[RBB] = A.pBB;
+    pr[RAB] = A.pAB;
     __syncthreads();
     if (i < nmol) {
         float *f = feat + (size_t)i * NR;
@@ -347,8 +366,8 @@ __global__ void k_sur_pack(const float *__restrict__ P, unsigned short *__restri
     }
 }
 
-__device__ __forceinline__ bf16x4 ld_frag(const unsigned short *__restrict__ frag, int fr, int l) {
-    const uint2 u = *(const uint2 *)(frag + ((size_t)fr * 64 + l) * 4);
+__device__ __forceinline__ bf16x4 ld_frag(const uint2 *sfr, int fr, int l) {
+    const uint2 u = sfr[fr * 64 + l];
     bf16x4 a;
     a[0] = (short)(u.x & 0xFFFFu);
     a[1] = (short)(u.x >> 16);
@@ -364,20 +383,57 @@ __device__ __forceinline__ bf16x4 pack4(f32x4 v) {
     return b;
 }
 
-constexpr int kMlpWaves = 4;
+constexpr int kMlpWaves = 2;
 
 __global__ void __launch_bounds__(64 * kMlpWaves) k_sur_infer(int nmol, const float *__restrict__ P,
                                                               const unsigned short *__restrict__ frag,
                                                               const float *__restrict__ raw, float *__restrict__ emol,
                                                               float *__restrict__ wgt,
                                                               unsigned long long *__restrict__ acc, int energy) {
+    // the weight fragments and vectors are staged in LDS once per workgroup
+    __shared__ uint2 sfr[FR_N * 64];
+    __shared__ float sb1[NH], sb2[NH], sw3[NH], sw3r[NH], smu[NF], sisd[NF];
+    __shared__ float sraw[kMlpWaves][16][NR + 1];
     __shared__ float sdx[kMlpWaves][16][NF + 1];
+    static_assert(FR_N * 64 % (64 * kMlpWaves) == 0, "fragment staging");
+    {
+        // all loads first, then the LDS stores
+        uint2 tmp[FR_N * 64 / (64 * kMlpWaves)];
+#pragma unroll
+        for (int u = 0; u < FR_N * 64 / (64 * kMlpWaves); ++u)
+            tmp[u] = ((const uint2 *)frag)[threadIdx.x + u * 64 * kMlpWaves];
+#pragma unroll
+        for (int u = 0; u < FR_N * 64 / (64 * kMlpWaves); ++u) sfr[threadIdx.x + u * 64 * kMlpWaves] = tmp[u];
+    }
+    for (int e = threadIdx.x; e < NH; e += blockDim.x) {
+        sb1[e] = P[OB1 + e];
+        sb2[e] = P[OB2 + e];
+        sw3[e] = P[OW3 + e];
+        sw3r[e] = bf16_round(P[OW3 + e]);
+    }
+    for (int e = threadIdx.x; e < NF; e += blockDim.x) {
+        smu[e] = P[OMU + e];
+        sisd[e] = 1.0f / P[OSD + e];
+    }
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, q = l >> 4;
     const int m0 = (blockIdx.x * kMlpWaves + wv) * 16;
-    if (m0 >= nmol) return;  // whole wave; no block barrier follows
-    const int nrow = min(16, nmol - m0);
-    const int m = m0 + min(r, nrow - 1);
-    const float *fm = raw + (size_t)m * NR;
+    const int nrow = max(0, min(16, nmol - m0));
+    {
+        float tmp[(16 * NR + 63) / 64];
+#pragma unroll
+        for (int u = 0; u < (16 * NR + 63) / 64; ++u) {
+            const int e = l + 64 * u, s2 = e / NR;
+            tmp[u] = e < 16 * NR && s2 < nrow ? raw[(size_t)m0 * NR + e] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < (16 * NR + 63) / 64; ++u) {
+            const int e = l + 64 * u;
+            if (e < 16 * NR) sraw[wv][e / NR][e % NR] = tmp[u];
+        }
+    }
+    __syncthreads();
+    if (nrow == 0) return;  // whole wave; no barrier follows
+    const float *fm = sraw[wv][min(r, nrow - 1)];
     // X^T k-blocks: lane holds features 16 kb + 4 q + j of molecule r
     bf16x4 xb[3];
 #pragma unroll
@@ -386,7 +442,7 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_sur_infer(int nmol, const fl
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int col = 16 * kb + 4 * q + j;
-            v[j] = (feature_of(fm, col) - P[OMU + col]) / P[OSD + col];
+            v[j] = (feature_of(fm, col) - smu[col]) * sisd[col];
         }
         xb[kb] = pack4(v);
     }
@@ -398,11 +454,11 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_sur_infer(int nmol, const fl
         f32x4 c = zero4();
 #pragma unroll
         for (int kb = 0; kb < 3; ++kb)
-            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld_frag(frag, FR_A1 + 3 * t + kb, l), xb[kb], c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld_frag(sfr, FR_A1 + 3 * t + kb, l), xb[kb], c, 0, 0, 0);
         f32x4 h;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            c[j] += P[OB1 + 16 * t + 4 * q + j];
+            c[j] += sb1[16 * t + 4 * q + j];
             h[j] = silu(c[j]);
         }
         z1[t] = c;
@@ -416,12 +472,12 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_sur_infer(int nmol, const fl
         f32x4 c = zero4();
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb)
-            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld_frag(frag, FR_A2 + 4 * t + kb, l), h1[kb], c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld_frag(sfr, FR_A2 + 4 * t + kb, l), h1[kb], c, 0, 0, 0);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            c[j] += P[OB2 + 16 * t + 4 * q + j];
+            c[j] += sb2[16 * t + 4 * q + j];
             // the output layer as in the training kernel: BF16 H2 and w3
-            part += bf16_round(silu(c[j])) * bf16_round(P[OW3 + 16 * t + 4 * q + j]);
+            part += bf16_round(silu(c[j])) * sw3r[16 * t + 4 * q + j];
         }
         z2[t] = c;
     }
@@ -440,7 +496,7 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_sur_infer(int nmol, const fl
     for (int t = 0; t < 4; ++t) {
         f32x4 d;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) d[j] = es * P[OW3 + 16 * t + 4 * q + j] * dsilu(z2[t][j]);
+        for (int j = 0; j < 4; ++j) d[j] = es * sw3[16 * t + 4 * q + j] * dsilu(z2[t][j]);
         dz2[t] = pack4(d);
     }
     // dZ1^T = (W2 dZ2^T) silu'(Z1^T)
@@ -450,7 +506,7 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_sur_infer(int nmol, const fl
         f32x4 c = zero4();
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb)
-            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld_frag(frag, FR_A3 + 4 * t + kb, l), dz2[kb], c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld_frag(sfr, FR_A3 + 4 * t + kb, l), dz2[kb], c, 0, 0, 0);
         f32x4 d;
 #pragma unroll
         for (int j = 0; j < 4; ++j) d[j] = c[j] * dsilu(z1[t][j]);
@@ -462,11 +518,11 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_sur_infer(int nmol, const fl
         f32x4 c = zero4();
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb)
-            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld_frag(frag, FR_A4 + 4 * t + kb, l), dz1[kb], c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld_frag(sfr, FR_A4 + 4 * t + kb, l), dz1[kb], c, 0, 0, 0);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int col = 16 * t + 4 * q + j;
-            sdx[wv][r][col] = c[j] / P[OSD + col];
+            sdx[wv][r][col] = c[j] * sisd[col];
         }
     }
     wave_sync();
@@ -474,7 +530,7 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_sur_infer(int nmol, const fl
     for (int e2 = l; e2 < 16 * NB; e2 += 64) {
         const int s = e2 / NB, k = e2 % NB;
         if (s >= nrow) continue;
-        const float *f = raw + (size_t)(m0 + s) * NR;
+        const float *f = sraw[wv][s];
         const float *gx = sdx[wv][s];
         const float d1 = f[RD1 + k], d2 = f[RD2 + k];
         float ga = gx[k], gs = gx[NB + k];
@@ -484,14 +540,14 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_sur_infer(int nmol, const fl
             ga += gx[3 * NB + 3] * sk + 2.0f * gx[3 * NB + 4] * x;
             gs += gx[3 * NB + 3] * x + 2.0f * gx[3 * NB + 5] * sk;
         }
-        float *o = wgt + (size_t)(m0 + s) * NR;
+        float *o = wgt + (size_t)(m0 + s) * NW;
         o[k] = ga;
         o[RD1 + k] = gs + gp * d2;
         o[RD2 + k] = gs + gp * d1;
     }
     if (l < 48) {
         const int s = l / 3, c3 = l % 3;
-        if (s < nrow) wgt[(size_t)(m0 + s) * NR + RAA + c3] = sdx[wv][s][3 * NB + c3];
+        if (s < nrow) wgt[(size_t)(m0 + s) * NW + RAA + c3] = sdx[wv][s][3 * NB + c3];
     }
     if (energy) {
         esum = team_sum_d<64>(esum);
@@ -508,6 +564,74 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_sur_infer(int nmol, const fl
 // two bonds of H's molecule to that O (pAA of H's molecule, pBB of O's), and
 // the mutual product D_ij D_ji (pAB of both).  VIR: the virial of every
 // bond, -(dE/dr / r) d (x) d, half from each of its two owners.
+// forces on molecule i's own atoms from its bonds with one neighbour j
+template <bool VIR>
+__device__ __forceinline__ void force_pair(float4 Oi, float4 H1i, float4 H2i, float4 Oj, float4 H1j, float4 H2j,
+                                           const float (&wi)[NW], const float (&wj)[NW], const Geo3 &g,
+                                           float3 &FO, float3 &F1, float3 &F2, float &v0, float &v1, float &v2,
+                                           float &v3, float &v4, float &v5) {
+    const float3 oo = mi(Oj, Oi, g);
+    if (oo.x * oo.x + oo.y * oo.y + oo.z * oo.z > kSkipOO2) return;
+    // q = 0, 1: H1_i, H2_i -> O_j (i's donor channels, j's acceptor);
+    // q = 2, 3: H1_j, H2_j -> O_i (j's donor channels, i's acceptor)
+    const float3 d[4] = {mi(H1i, Oj, g), mi(H2i, Oj, g), mi(H1j, Oi, g), mi(H2j, Oi, g)};
+    float r[4], sw[4], ds[4], fb[4], dfb[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        r[q] = sqrtf(d[q].x * d[q].x + d[q].y * d[q].y + d[q].z * d[q].z);
+        fcut(r[q], sw[q], ds[q]);
+        const float ek = __expf(-(KAPPA0 + DKAPPA * KE) * r[q]);
+        fb[q] = ek * sw[q];
+        dfb[q] = ek * (ds[q] - (KAPPA0 + DKAPPA * KE) * sw[q]);
+    }
+    const float wAB = wi[RAB] + wj[RAB];
+    const float Dij = fb[0] + fb[1], Dji = fb[2] + fb[3];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (sw[q] <= 0.f && ds[q] == 0.f) continue;
+        // d/dr [exp(-kappa r) fc] = exp(-kappa r) (fc' - kappa fc)
+        float dEdr = 0.f;
+        const float step = __expf(-DKAPPA * r[q]);
+        float ek = __expf(-KAPPA0 * r[q]);
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            const float w = q == 0 ? wi[RD1 + k] + wj[k]
+                          : q == 1 ? wi[RD2 + k] + wj[k]
+                          : q == 2 ? wj[RD1 + k] + wi[k] : wj[RD2 + k] + wi[k];
+            dEdr += w * ek * (ds[q] - kappa_k(k) * sw[q]);
+            ek *= step;
+        }
+        // pair products: the other bond of the same hydrogen pair, and D of the reverse direction
+        const float other = fb[q ^ 1];
+        const float wpair = q < 2 ? wi[RAA] + wj[RBB] : wj[RAA] + wi[RBB];
+        dEdr += (wpair * other + wAB * (q < 2 ? Dji : Dij)) * dfb[q];
+        const float c = dEdr / r[q];
+        if (VIR) {
+            const float h = -0.5f * c;
+            v0 += h * d[q].x * d[q].x;
+            v1 += h * d[q].y * d[q].y;
+            v2 += h * d[q].z * d[q].z;
+            v3 += h * d[q].x * d[q].y;
+            v4 += h * d[q].x * d[q].z;
+            v5 += h * d[q].y * d[q].z;
+        }
+        // F_H = -c d, F_O = +c d  (d = r_H - r_O)
+        if (q == 0) {
+            F1.x -= c * d[q].x;
+            F1.y -= c * d[q].y;
+            F1.z -= c * d[q].z;
+        } else if (q == 1) {
+            F2.x -= c * d[q].x;
+            F2.y -= c * d[q].y;
+            F2.z -= c * d[q].z;
+        } else {
+            FO.x += c * d[q].x;
+            FO.y += c * d[q].y;
+            FO.z += c * d[q].z;
+        }
+    }
+}
+
 template <bool VIR>
 __global__ void __launch_bounds__(256) k_sur_force(int nmol, const int4 *__restrict__ molAtoms,
                                                    const float4 *__restrict__ xq, const BoxDev *__restrict__ bx,
@@ -524,83 +648,46 @@ __global__ void __launch_bounds__(256) k_sur_force(int nmol, const int4 *__restr
     if (i < nmol) {
         ai = molAtoms[i];
         const float4 Oi = xq[ai.x], H1i = xq[ai.y], H2i = xq[ai.z];
-        const float *wi = wgt + (size_t)i * NR;
-        float wa_i[NB], w1_i[NB], w2_i[NB];
+        float wi[NW];
+        {
+            const float4 *w4 = (const float4 *)(wgt + (size_t)i * NW);
 #pragma unroll
-        for (int k = 0; k < NB; ++k) {
-            wa_i[k] = wi[k];
-            w1_i[k] = wi[RD1 + k];
-            w2_i[k] = wi[RD2 + k];
+            for (int v = 0; v < NW / 4; ++v) {
+                const float4 t4 = w4[v];
+                wi[4 * v] = t4.x;
+                wi[4 * v + 1] = t4.y;
+                wi[4 * v + 2] = t4.z;
+                wi[4 * v + 3] = t4.w;
+            }
         }
-        const float wAA_i = wi[RAA], wBB_i = wi[RBB], wAB_i = wi[RAB];
         const int cnt = min(nlCount[i], cap);
         const int *row = nl + (size_t)i * cap;
-        for (int t = tl; t < cnt; t += kFeatLanes) {
-            const int mj = row[t];
-            const int4 aj = molAtoms[mj];
-            const float4 Oj = xq[aj.x];
-            const float3 oo = mi(Oj, Oi, g);
-            if (oo.x * oo.x + oo.y * oo.y + oo.z * oo.z > kSkipOO2) continue;
-            const float4 H1j = xq[aj.y], H2j = xq[aj.z];
-            const float *wj = wgt + (size_t)mj * NR;
-            // q = 0, 1: H1_i, H2_i -> O_j (i's donor channels, j's acceptor);
-            // q = 2, 3: H1_j, H2_j -> O_i (j's donor channels, i's acceptor)
-            const float3 d[4] = {mi(H1i, Oj, g), mi(H2i, Oj, g), mi(H1j, Oi, g), mi(H2j, Oi, g)};
-            float r[4], sw[4], ds[4], fb[4], dfb[4];
+        for (int t = tl; t < cnt; t += 2 * kFeatLanes) {
+            const bool two = t + kFeatLanes < cnt;
+            const int mA = row[t], mB = two ? row[t + kFeatLanes] : mA;
+            // both neighbours' weights and atoms are loaded at once
+            float wA[NW], wB[NW];
+            {
+                const float4 *a4 = (const float4 *)(wgt + (size_t)mA * NW);
+                const float4 *b4 = (const float4 *)(wgt + (size_t)mB * NW);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                r[q] = sqrtf(d[q].x * d[q].x + d[q].y * d[q].y + d[q].z * d[q].z);
-                fcut(r[q], sw[q], ds[q]);
-                const float ek = __expf(-(KAPPA0 + DKAPPA * KE) * r[q]);
-                fb[q] = ek * sw[q];
-                dfb[q] = ek * (ds[q] - (KAPPA0 + DKAPPA * KE) * sw[q]);
-            }
-            const float wAA_j = wj[RAA], wBB_j = wj[RBB], wAB = wAB_i + wj[RAB];
-            const float Dij = fb[0] + fb[1], Dji = fb[2] + fb[3];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (sw[q] <= 0.f && ds[q] == 0.f) continue;
-                // d/dr [exp(-kappa r) fc] = exp(-kappa r) (fc' - kappa fc)
-                float dEdr = 0.f;
-                const float step = __expf(-DKAPPA * r[q]);
-                float ek = __expf(-KAPPA0 * r[q]);
-#pragma unroll
-                for (int k = 0; k < NB; ++k) {
-                    const float w = q == 0 ? w1_i[k] + wj[k]
-                                  : q == 1 ? w2_i[k] + wj[k]
-                                  : q == 2 ? wj[RD1 + k] + wa_i[k] : wj[RD2 + k] + wa_i[k];
-                    dEdr += w * ek * (ds[q] - kappa_k(k) * sw[q]);
-                    ek *= step;
-                }
-                // pair products: the other bond of the same hydrogen pair, and D of the reverse direction
-                const float other = fb[q ^ 1];
-                const float wpair = q < 2 ? wAA_i + wBB_j : wAA_j + wBB_i;
-                dEdr += (wpair * other + wAB * (q < 2 ? Dji : Dij)) * dfb[q];
-                const float c = dEdr / r[q];
-                if (VIR) {
-                    const float h = -0.5f * c;
-                    v0 += h * d[q].x * d[q].x;
-                    v1 += h * d[q].y * d[q].y;
-                    v2 += h * d[q].z * d[q].z;
-                    v3 += h * d[q].x * d[q].y;
-                    v4 += h * d[q].x * d[q].z;
-                    v5 += h * d[q].y * d[q].z;
-                }
-                // F_H = -c d, F_O = +c d  (d = r_H - r_O)
-                if (q == 0) {
-                    F1.x -= c * d[q].x;
-                    F1.y -= c * d[q].y;
-                    F1.z -= c * d[q].z;
-                } else if (q == 1) {
-                    F2.x -= c * d[q].x;
-                    F2.y -= c * d[q].y;
-                    F2.z -= c * d[q].z;
-                } else {
-                    FO.x += c * d[q].x;
-                    FO.y += c * d[q].y;
-                    FO.z += c * d[q].z;
+                for (int v = 0; v < NW / 4; ++v) {
+                    const float4 ta = a4[v], tb = b4[v];
+                    wA[4 * v] = ta.x;
+                    wA[4 * v + 1] = ta.y;
+                    wA[4 * v + 2] = ta.z;
+                    wA[4 * v + 3] = ta.w;
+                    wB[4 * v] = tb.x;
+                    wB[4 * v + 1] = tb.y;
+                    wB[4 * v + 2] = tb.z;
+                    wB[4 * v + 3] = tb.w;
                 }
             }
+            const int4 aA = molAtoms[mA], aB = molAtoms[mB];
+            const float4 OA = xq[aA.x], H1A = xq[aA.y], H2A = xq[aA.z];
+            const float4 OB = xq[aB.x], H1B = xq[aB.y], H2B = xq[aB.z];
+            force_pair<VIR>(Oi, H1i, H2i, OA, H1A, H2A, wi, wA, g, FO, F1, F2, v0, v1, v2, v3, v4, v5);
+            if (two) force_pair<VIR>(Oi, H1i, H2i, OB, H1B, H2B, wi, wB, g, FO, F1, F2, v0, v1, v2, v3, v4, v5);
         }
     }
     FO.x = team_sum<kFeatLanes>(FO.x);
