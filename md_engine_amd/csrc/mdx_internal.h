@@ -250,6 +250,7 @@ struct PppmState {
     float2 *rhoHat = nullptr;        // sx*sy*(sz/2+1)
     float2 *eHat = nullptr;          // 3 * sx*sy*(sz/2+1)
     float *eReal = nullptr;          // 3 * sx*sy*sz
+    float4 *eField4 = nullptr;       // the field per mesh point (Ex, Ey, Ez, 0)
     float *green = nullptr;          // sx*sy*(sz/2+1)
     hipfftHandle planR2C = 0;
     hipfftHandle planC2R = 0;

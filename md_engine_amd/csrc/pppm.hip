@@ -458,10 +458,18 @@ __device__ __forceinline__ float pick(const float (&w)[ORDER], int k) {
 // components are consecutive real meshes, as the batched inverse FFT writes
 // them), and the team sums by butterfly shuffles before one fixed-point add
 // per component.
+// the three real field components interleaved per mesh point, so the gather
+// takes one 16-byte load per stencil point
+__global__ void k_field_pack(int ng, const float *__restrict__ E, float4 *__restrict__ E4) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    E4[g] = make_float4(E[g], E[ng + g], E[2 * (size_t)ng + g], 0.0f);
+}
+
 template <int ORDER, int T>
 __global__ void __launch_bounds__(256) k_gather(int n, const double4 *__restrict__ pos,
                                                 const float *__restrict__ q, const BoxDev *__restrict__ bx,
-                                                const float *__restrict__ E,
+                                                const float4 *__restrict__ E4,
                                                 unsigned long long *__restrict__ force) {
     const MeshGeo m = bx->mesh;
     const float scale = bx->gatherScale;
@@ -482,7 +490,6 @@ __global__ void __launch_bounds__(256) k_gather(int n, const double4 *__restrict
     weights<ORDER>(d[1], wy);
     weights<ORDER>(d[2], wz);
     constexpr int H = (ORDER - 1) / 2;
-    const size_t ng = (size_t)m.sz[0] * m.sz[1] * m.sz[2];
     float ex = 0.f, ey = 0.f, ez = 0.f;
     for (int r = tl; r < ORDER * ORDER; r += T) {
         const int ix = r / ORDER, iy = r - ix * ORDER;
@@ -490,20 +497,15 @@ __global__ void __launch_bounds__(256) k_gather(int n, const double4 *__restrict
         const int py = wrapi(g[1] + iy - H, m.sz[1]);
         const float wxy = pick<ORDER>(wx, ix) * pick<ORDER>(wy, iy);
         const size_t rowb = ((size_t)px * m.sz[1] + py) * m.sz[2];
-        float e0[ORDER], e1[ORDER], e2[ORDER];
+        float4 e[ORDER];
 #pragma unroll
-        for (int iz = 0; iz < ORDER; ++iz) {
-            const size_t gidx = rowb + wrapi(g[2] + iz - H, m.sz[2]);
-            e0[iz] = E[gidx];
-            e1[iz] = E[ng + gidx];
-            e2[iz] = E[2 * ng + gidx];
-        }
+        for (int iz = 0; iz < ORDER; ++iz) e[iz] = E4[rowb + wrapi(g[2] + iz - H, m.sz[2])];
 #pragma unroll
         for (int iz = 0; iz < ORDER; ++iz) {
             const float w = wxy * wz[iz];
-            ex += w * e0[iz];
-            ey += w * e1[iz];
-            ez += w * e2[iz];
+            ex += w * e[iz].x;
+            ey += w * e[iz].y;
+            ez += w * e[iz].z;
         }
     }
     ex = team_sum<T>(ex);
@@ -577,6 +579,7 @@ void pppm_prepare(Ctx &c) {
         P.rhoHat = dalloc<float2>(c, nh);
         P.eHat = dalloc<float2>(c, 3 * nh);
         P.eReal = dalloc<float>(c, 3 * ng);
+        P.eField4 = dalloc<float4>(c, ng);
         P.green = dalloc<float>(c, nh);
         MDX_HIP(hipMemsetAsync(P.mesh, 0, sizeof(long long) * ng, c.stream));
         MDX_REQUIRE(hipfftPlan3d(&P.planR2C, sz[0], sz[1], sz[2], HIPFFT_R2C) == HIPFFT_SUCCESS, 2,
@@ -672,10 +675,11 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
     const int na = c.listValid ? c.nCharged : c.n;
     ProfScope ps(c, PK_GATHER);
     if (na == 0) return;
+    hipLaunchKernelGGL(k_field_pack, dim3(cdiv(ng, threads)), dim3(threads), 0, c.stream, ng, P.eReal, P.eField4);
     const dim3 grid(cdiv((long long)na * P.gatherTeam, threads));
 #define MDX_GATHER(O, T)                                                                                   \
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gather<O, T>), grid, dim3(threads), 0, c.stream, na, c.pos[cur], \
-                       c.q[cur], c.box, P.eReal, c.force)
+                       c.q[cur], c.box, P.eField4, c.force)
     const bool wide = P.gatherTeam == 16;
     switch (P.order) {
         case 1: if (wide) MDX_GATHER(1, 16); else MDX_GATHER(1, 8); break;
