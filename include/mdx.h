@@ -34,6 +34,7 @@ typedef struct mdx_ctx mdx_ctx;
 #define MDX_PPPM 2      /* P3M mesh part (FixChargeEwald) */
 #define MDX_E3B 4       /* E3B two- and three-body (FixE3B) */
 #define MDX_ENERGY 8    /* also accumulate energies */
+#define MDX_BONDED 64   /* intramolecular terms of flexible water (mdx_flexible_water_set) */
 #define MDX_VIRIAL 16   /* also accumulate the virial tensor */
 #define MDX_SURROGATE 32 /* E3B with the learned three-body term (mdx_surrogate_set) */
 
@@ -140,6 +141,19 @@ int mdx_forces_get(mdx_ctx *ctx, double *f);
  * e[1] P3M reciprocal incl. self term, e[2] E3B two-body, e[3] E3B three-body;
  * virial[6] xx yy zz xy xz yz (sum F (x) dr). */
 int mdx_energy_get(mdx_ctx *ctx, double e[4], double virial[6]);
+/* Flexible q-TIP4P/F water, replacing the rigid model: FixBondQuartic
+ * (src/Fixes/FixBondQuartic.cu; U = k2 dr^2 + k3 dr^3 + k4 dr^4 per O-H
+ * bond, src/Evaluators/BondEvaluatorQuartic.h:9-36), FixAngleHarmonic
+ * (src/Fixes/FixAngleHarmonic.cu; U = ktheta/2 (theta - theta0)^2 per H-O-H,
+ * src/Evaluators/AngleEvaluatorHarmonic.h:12-78) and the massless M site of
+ * FixTIP4PFlexible (src/Fixes/FixTIP4PFlexible.cu:92-300: placed at
+ * gamma r_O + (1 - gamma)/2 (r_H1 + r_H2) before every force evaluation, its
+ * force spread back onto O and H).  mol_ids: nmol x (O, H1, H2, M) ids (M -1:
+ * three-site).  The intramolecular forces enter compute under MDX_BONDED;
+ * their energy is read with mdx_bonded_energy_get. */
+int mdx_flexible_water_set(mdx_ctx *ctx, int nmol, const int32_t *mol_ids, double k2, double k3, double k4,
+                           double r0, double ktheta, double theta0, double gamma);
+int mdx_bonded_energy_get(mdx_ctx *ctx, double *e);
 
 /* Velocity Verlet (IntegratorVerlet::run, src/Integrators/IntegratorVerlet.cu:649-770):
  * dt and the force-to-acceleration unit factor (Units::ftm_to_v). */

@@ -14,7 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmdx.so")
 
-PAIR, PPPM, E3B, ENERGY, VIRIAL, SURROGATE = 1, 2, 4, 8, 16, 32
+PAIR, PPPM, E3B, ENERGY, VIRIAL, SURROGATE, BONDED = 1, 2, 4, 8, 16, 32, 64
 
 # every symbol declared in include/mdx.h
 SYMBOLS = [
@@ -28,7 +28,7 @@ SYMBOLS = [
     "mdx_thermostat_set", "mdx_temperature", "mdx_tune", "mdx_run_batch", "mdx_surrogate_set",
     "mdx_surrogate_get", "mdx_surrogate_record", "mdx_surrogate_fit", "mdx_surrogate_eval",
     "mdx_e3b_molecule_energies", "mdx_nose_hoover_set", "mdx_nose_hoover_energy", "mdx_barostat_set",
-    "mdx_pressure", "mdx_barostat_last", "mdx_box_get",
+    "mdx_pressure", "mdx_barostat_last", "mdx_box_get", "mdx_flexible_water_set", "mdx_bonded_energy_get",
 ]
 
 # kernel ids of mdx_profile_get
@@ -78,6 +78,8 @@ def load():
     L.mdx_nlist_stats.argtypes = [_vp, C.POINTER(C.c_int64), C.POINTER(c), _ip, C.POINTER(C.c_int64)]
     L.mdx_option_set.argtypes = [_vp, C.c_char_p, c]
     L.mdx_rigid_set.argtypes = [_vp, c, _ip, d, d, d, d, d]
+    L.mdx_flexible_water_set.argtypes = [_vp, c, _ip, d, d, d, d, d, d, d]
+    L.mdx_bonded_energy_get.argtypes = [_vp, C.POINTER(d)]
     L.mdx_thermostat_set.argtypes = [_vp, c, d, c, d, d, d]
     L.mdx_temperature.argtypes = [_vp, d, d, C.POINTER(d), C.POINTER(d)]
     L.mdx_lj_set.argtypes = [_vp, c, _dp, _dp, _dp]
@@ -267,6 +269,17 @@ class Engine:
         m = np.ascontiguousarray(mol_ids, np.int32).reshape(-1)
         self._chk(self._L.mdx_rigid_set(self._h, len(m) // 4, m, float(m_O), float(m_H), float(d_OH),
                                         float(d_HH), float(gamma)), "mdx_rigid_set")
+
+    def flexible_water_set(self, mol_ids, k2, k3, k4, r0, ktheta, theta0, gamma):
+        m = np.ascontiguousarray(mol_ids, np.int32).reshape(-1)
+        self._chk(self._L.mdx_flexible_water_set(self._h, len(m) // 4, m, float(k2), float(k3), float(k4),
+                                                 float(r0), float(ktheta), float(theta0), float(gamma)),
+                  "mdx_flexible_water_set")
+
+    def bonded_energy(self):
+        e = C.c_double(0.0)
+        self._chk(self._L.mdx_bonded_energy_get(self._h, C.byref(e)), "mdx_bonded_energy_get")
+        return e.value
 
     def thermostat(self, enable, temperature=0.0, every=1, tau=0.0, boltz=1.0, mvv_to_eng=1.0):
         self._chk(self._L.mdx_thermostat_set(self._h, int(enable), float(temperature), int(every),

@@ -112,7 +112,7 @@ void update_ox_flag(Ctx &c) {
     c.e3b.oxUncharged = u;
 }
 
-constexpr int kForceTerms = MDX_PAIR | MDX_PPPM | MDX_E3B | MDX_SURROGATE;
+constexpr int kForceTerms = MDX_PAIR | MDX_PPPM | MDX_E3B | MDX_SURROGATE | MDX_BONDED;
 
 // Instantaneous pressure (nktv_to_press units) from the virial of the last
 // compute with MDX_VIRIAL: P = (sum M V^2 + W - corr) / 3V, W the atomic virial
@@ -161,6 +161,7 @@ bool regrid_if_needed(Ctx &c) {
 
 void compute(Ctx &c, int flags) {
     const bool energy = flags & MDX_ENERGY, virial = flags & MDX_VIRIAL;
+    if (c.flex.on) flex_place_m(c);  // the M sites follow O and H before any force term
     prepare_forces(c);
     if ((flags & MDX_PAIR)) pair_forces(c, energy, virial);
     if ((flags & MDX_PPPM) && c.pppm.on) pppm_forces(c, energy, virial);
@@ -170,6 +171,10 @@ void compute(Ctx &c, int flags) {
         surrogate_forces(c, energy, virial);
     } else if ((flags & MDX_E3B) && c.e3b.on) {
         e3b_forces(c, energy, virial);
+    }
+    if (c.flex.on) {
+        if (flags & MDX_BONDED) flex_forces(c, energy, virial);
+        flex_spread_m(c);  // every term's M force goes onto O and H before the kick
     }
     c.forceFlags = flags & kForceTerms;
 }
@@ -579,6 +584,7 @@ int mdx_system_set(mdx_ctx *ctx, int n, const double *x, const double *v, const 
             rigid_map(c);
             c.rigid.fresh = true;
         }
+        flex_map(c);
         if (c.pppm.on) {
             double s = 0.0;
             for (int i = 0; i < n; ++i) s += (double)hq[i] * hq[i];
@@ -976,6 +982,61 @@ int mdx_rigid_set(mdx_ctx *ctx, int nmol, const int32_t *mol_ids, double m_O, do
         upload(c, R.ids, ids.data(), nmol);
         rigid_map(c);
         MDX_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int mdx_flexible_water_set(mdx_ctx *ctx, int nmol, const int32_t *mol_ids, double k2, double k3, double k4,
+                           double r0, double ktheta, double theta0, double gamma) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        MDX_REQUIRE(nmol > 0 && mol_ids, MDX_ERR_ARG, "molecules required");
+        MDX_REQUIRE(!c.rigid.on, MDX_ERR_STATE, "flexible and rigid water cannot be combined");
+        MDX_REQUIRE(r0 > 0 && theta0 > 0 && theta0 < M_PI && gamma > 0 && gamma <= 1, MDX_ERR_ARG,
+                    "invalid flexible water parameters");
+        FlexState &F = c.flex;
+        if (F.ids) {
+            dfree(c, F.ids);
+            dfree(c, F.atoms);
+        }
+        std::vector<int4> ids(nmol);
+        std::vector<int> seen(c.n, 0);
+        for (int m = 0; m < nmol; ++m) {
+            const int *p = mol_ids + 4 * m;
+            for (int k = 0; k < 3; ++k) {
+                MDX_REQUIRE(p[k] >= 0 && p[k] < c.n, MDX_ERR_ARG, "molecule id out of range");
+                MDX_REQUIRE(!seen[p[k]]++, MDX_ERR_ARG, "atom listed in two molecules");
+            }
+            MDX_REQUIRE(p[3] < c.n, MDX_ERR_ARG, "M-site id out of range");
+            if (p[3] >= 0) {
+                MDX_REQUIRE(!seen[p[3]]++, MDX_ERR_ARG, "atom listed in two molecules");
+                MDX_REQUIRE(c.invMassHost[p[3]] == 0.0, MDX_ERR_ARG, "M site must be massless");
+            }
+            ids[m] = make_int4(p[0], p[1], p[2], p[3]);
+        }
+        F.on = true;
+        F.nmol = nmol;
+        F.k2 = k2;
+        F.k3 = k3;
+        F.k4 = k4;
+        F.r0 = r0;
+        F.ktheta = ktheta;
+        F.theta0 = theta0;
+        F.gamma = gamma;
+        F.ids = dalloc<int4>(c, nmol);
+        F.atoms = dalloc<int4>(c, nmol);
+        upload(c, F.ids, ids.data(), nmol);
+        flex_map(c);
+        MDX_HIP(hipStreamSynchronize(c.stream));
+        c.forceFlags = -1;
+    });
+}
+
+int mdx_bonded_energy_get(mdx_ctx *ctx, double *e) {
+    return guarded(ctx, [&](Ctx &c) {
+        MDX_REQUIRE(e, MDX_ERR_ARG, "output required");
+        double h[ACC_N];
+        acc_read(c, h);
+        *e = h[ACC_BOND];
     });
 }
 

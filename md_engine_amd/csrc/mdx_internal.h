@@ -37,6 +37,7 @@ enum AccSlot {
     ACC_E3B3 = 3,     // E3B three-body energy
     ACC_VIR = 4,      // 6 slots: xx yy zz xy xz yz
     ACC_MOL = 10,     // 2 slots: molecular virial correction, sum M V^2 of centres of mass
+    ACC_BOND = 12,    // intramolecular (quartic bond + harmonic angle) energy
     ACC_N = 16
 };
 // The accumulator block is replicated kAccRep times; a workgroup adds into
@@ -257,6 +258,16 @@ struct PppmState {
     bool plansMade = false;
 };
 
+// Flexible water: quartic O-H bonds, harmonic H-O-H angle and a massless M
+// site at gamma r_O + (1 - gamma)/2 (r_H1 + r_H2) (q-TIP4P/F)
+struct FlexState {
+    bool on = false;
+    int nmol = 0;
+    int4 *ids = nullptr;    // by molecule: O, H1, H2, M ids
+    int4 *atoms = nullptr;  // by molecule: current indices
+    double k2 = 0, k3 = 0, k4 = 0, r0 = 0, ktheta = 0, theta0 = 0, gamma = 0;
+};
+
 struct E3BState {
     bool on = false;
     int nmol = 0;
@@ -426,6 +437,7 @@ struct Ctx {
 
     PppmState pppm;
     E3BState e3b;
+    FlexState flex;
     SurrogateState sur;
     NoseHooverState nh;
     BarostatState baro;
@@ -607,6 +619,12 @@ void baro_step(Ctx &c, const int *gate);
 // Green function from the device box on stream s (gate: nullptr = always)
 void pppm_green_device(Ctx &c, const int *gate, hipStream_t s);
 void rigid_first_half(Ctx &c);
+// flex.hip: molecule indices after a re-sort, M placement, intramolecular
+// forces and the M force spread onto O, H1, H2
+void flex_map(Ctx &c);
+void flex_place_m(Ctx &c);
+void flex_forces(Ctx &c, bool energy, bool virial);
+void flex_spread_m(Ctx &c);
 void rigid_second_half(Ctx &c, bool kick);
 
 // integrate.hip

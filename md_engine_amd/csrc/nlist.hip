@@ -1184,6 +1184,7 @@ static void run_build(Ctx &c, bool sync) {
                            c.id[c.cur], c.exclOff, c.exclIds, c.idToIdx, c.maxExcl, c.exclIdx);
     }
     rigid_map(c);
+    flex_map(c);
     if (c.e3b.on) {
         fill_int(c, c.e3b.atomMolO, c.n, -1, rebuild_gate(c));  // indices were permuted: drop the old map
         hipLaunchKernelGGL(k_mol_atoms, dim3(cdiv(c.e3b.nmol, threads)), dim3(threads), 0, c.stream, rebuild_gate(c),
