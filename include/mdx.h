@@ -168,6 +168,19 @@ int mdx_run(mdx_ctx *ctx, int nsteps, int flags);
  * whose integrator runs one system per process; results per replica equal a
  * single-context captured run bit for bit.  Errors are reported on ctxs[0]. */
 int mdx_run_batch(mdx_ctx *const *ctxs, int n, int nsteps, int flags);
+/* Path-integral MD: P bead contexts (the same atoms, box, time step and
+ * force terms; flexible molecules, no barostat) advanced as one ring polymer
+ * per atom, like IntegratorVerlet with nPerRingPoly = P
+ * (src/Integrators/IntegratorVerlet.cu:63-304, 342-593): every step each
+ * bead is kicked by half a step of its own forces, the free ring polymer of
+ * every atom is propagated exactly for dt in normal modes (frequencies
+ * w_k = 2 omegaP sin(k pi / P); the reference takes omegaP = kB T / hbar at
+ * its bead temperature T = P T_physical, IntegratorVerlet.cu:794), every bead
+ * evaluates its forces and takes the second half kick and its thermostat.
+ * Each bead keeps its own neighbor list and P3M (the reference builds the
+ * list on centroids and contracts P3M to the centroid).  P: a power of two
+ * from 2 to 64.  The beads run on bead 0's stream inside captured graphs. */
+int mdx_ring_run(mdx_ctx *const *beads, int P, int nsteps, int flags, double omegaP);
 /* Launch geometry knobs (Tunable nThreadPerAtom, src/Tunable.h:3-24):
  * lanes per atom in the list build and the pair kernel, lanes per molecule in
  * E3B.  0 keeps the current value.  Powers of two only. */

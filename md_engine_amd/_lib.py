@@ -28,7 +28,7 @@ SYMBOLS = [
     "mdx_thermostat_set", "mdx_temperature", "mdx_tune", "mdx_run_batch", "mdx_surrogate_set",
     "mdx_surrogate_get", "mdx_surrogate_record", "mdx_surrogate_fit", "mdx_surrogate_eval",
     "mdx_e3b_molecule_energies", "mdx_nose_hoover_set", "mdx_nose_hoover_energy", "mdx_barostat_set",
-    "mdx_pressure", "mdx_barostat_last", "mdx_box_get", "mdx_flexible_water_set", "mdx_bonded_energy_get",
+    "mdx_pressure", "mdx_barostat_last", "mdx_box_get", "mdx_flexible_water_set", "mdx_bonded_energy_get", "mdx_ring_run",
 ]
 
 # kernel ids of mdx_profile_get
@@ -80,6 +80,7 @@ def load():
     L.mdx_rigid_set.argtypes = [_vp, c, _ip, d, d, d, d, d]
     L.mdx_flexible_water_set.argtypes = [_vp, c, _ip, d, d, d, d, d, d, d]
     L.mdx_bonded_energy_get.argtypes = [_vp, C.POINTER(d)]
+    L.mdx_ring_run.argtypes = [C.POINTER(_vp), c, c, c, d]
     L.mdx_thermostat_set.argtypes = [_vp, c, d, c, d, d, d]
     L.mdx_temperature.argtypes = [_vp, d, d, C.POINTER(d), C.POINTER(d)]
     L.mdx_lj_set.argtypes = [_vp, c, _dp, _dp, _dp]
@@ -431,3 +432,23 @@ def run_batch(engines, nsteps, flags):
     if rc != 0:
         msg = L.mdx_last_error(engines[0]._h)
         raise MdxError(f"mdx_run_batch failed ({rc}): {msg.decode() if msg else ''}")
+
+
+def ring_omega(temperature_beads, boltz, hbar):
+    """Ring-polymer frequency omegaP = kB T / hbar at the bead temperature
+    (the reference's convention: T = P x physical temperature,
+    IntegratorVerlet.cu:794)."""
+    return boltz * temperature_beads / hbar
+
+
+def run_ring(beads, nsteps, flags, omega_p):
+    """Advance P bead engines as ring polymers (mdx_ring_run)."""
+    beads = list(beads)
+    if not beads:
+        return
+    L = load()
+    arr = (_vp * len(beads))(*[e._h for e in beads])
+    rc = L.mdx_ring_run(arr, len(beads), int(nsteps), int(flags), float(omega_p))
+    if rc != 0:
+        msg = L.mdx_last_error(beads[0]._h)
+        raise MdxError(f"mdx_ring_run failed ({rc}): {msg.decode() if msg else ''}")

@@ -20,7 +20,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmdx.so")
 
 SOURCES = ["api.hip", "nlist.hip", "pair.hip", "pppm.hip", "e3b.hip", "integrate.hip", "rigid.hip",
-           "surrogate.hip", "box.hip", "flex.hip"]
+           "surrogate.hip", "box.hip", "flex.hip", "ring.hip"]
 NO_CONTRACT = {"nlist.hip"}
 
 ARCH = os.environ.get("MDX_OFFLOAD_ARCH", "gfx950")

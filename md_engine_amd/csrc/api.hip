@@ -288,7 +288,23 @@ void captured_step(Ctx &c, int fflags) {
     if (c.baro.on) baro_step(c, baro_gate(c));
 }
 
-hipGraphExec_t capture_steps(const std::vector<Ctx *> &cs, int steps, int fflags) {
+// One step of a ring polymer whose beads share one stream: every bead's half
+// kick, the free ring-polymer propagation across beads, then every bead's
+// rebuild gate, forces, second half kick and thermostat.
+void captured_ring_step(const std::vector<Ctx *> &cs, const RingSpec &ring, int fflags) {
+    for (Ctx *c : cs) integrate_kick_first(*c);
+    ring_drift(ring, cs[0]->box, cs[0]->stream);
+    for (Ctx *cp : cs) {
+        Ctx &c = *cp;
+        decide_step(c);
+        nlist_rebuild_async(c);
+        compute(c, fflags);
+        integrate_second_half(c);
+        if (c.thermoOn) thermostat_apply(c);
+    }
+}
+
+hipGraphExec_t capture_steps(const std::vector<Ctx *> &cs, int steps, int fflags, const RingSpec *ring = nullptr) {
     Ctx &c0 = *cs[0];
     std::vector<hipEvent_t> evs(cs.size() * 2, nullptr);
     for (auto &e : evs) MDX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -297,6 +313,19 @@ hipGraphExec_t capture_steps(const std::vector<Ctx *> &cs, int steps, int fflags
     MDX_HIP(hipStreamBeginCapture(c0.stream, hipStreamCaptureModeThreadLocal));
     try {
         MDX_HIP(hipEventRecord(evs[0], c0.stream));
+        if (ring) {
+            // the beads run on c0's stream, interleaved step by step
+            for (size_t k = 0; k < cs.size(); ++k) {
+                prof[k] = cs[k]->prof;
+                cs[k]->prof = false;
+                cs[k]->capture = true;
+            }
+            for (int s = 0; s < steps; ++s) captured_ring_step(cs, *ring, fflags);
+            for (size_t k = 0; k < cs.size(); ++k) {
+                cs[k]->capture = false;
+                cs[k]->prof = prof[k];
+            }
+        } else
         for (size_t k = 0; k < cs.size(); ++k) {
             Ctx &c = *cs[k];
             if (k) MDX_HIP(hipStreamWaitEvent(c.stream, evs[0], 0));
@@ -339,7 +368,19 @@ void stream_after(hipStream_t to, hipStream_t from) {
     MDX_HIP(hipEventDestroy(e));
 }
 
-void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags) {
+// the ring's per-bead pointer table for the current buffers
+void ring_table(const std::vector<Ctx *> &cs, const RingSpec &ring) {
+    std::vector<void *> t(3 * cs.size());
+    for (size_t k = 0; k < cs.size(); ++k) {
+        t[3 * k] = cs[k]->pos[cs[k]->cur];
+        t[3 * k + 1] = cs[k]->vel[cs[k]->cur];
+        t[3 * k + 2] = cs[k]->idToIdx;
+    }
+    MDX_HIP(hipMemcpyAsync(ring.tab, t.data(), sizeof(void *) * t.size(), hipMemcpyHostToDevice, cs[0]->stream));
+    MDX_HIP(hipStreamSynchronize(cs[0]->stream));
+}
+
+void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags, const RingSpec *ring = nullptr) {
     const int fflags = flags & kForceTerms;
     for (Ctx *cp : cs) {
         Ctx &c = *cp;
@@ -359,6 +400,7 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags) {
         }
     }
     hipGraphExec_t big = nullptr, one = nullptr;
+    std::vector<int> ringCur;
     auto drop = [&] {
         if (big) hipGraphExecDestroy(big);
         if (one) hipGraphExecDestroy(one);
@@ -389,9 +431,18 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags) {
                 fill_int(c, c.dFlags + 6, 1, (int)(c.turn % period));
                 if (cp != cs[0]) stream_after(cs[0]->stream, c.stream);
             }
-            if (!big) big = capture_steps(cs, kGraphSteps, fflags);
+            if (ring) {
+                // a bead's buffers may have flipped in the set-up above: a new
+                // table and new graphs
+                std::vector<int> curs;
+                for (Ctx *cp : cs) curs.push_back(cp->cur);
+                if (curs != ringCur) drop();
+                ringCur = curs;
+                ring_table(cs, *ring);
+            }
+            if (!big) big = capture_steps(cs, kGraphSteps, fflags, ring);
             const int nbig = len / kGraphSteps, nrest = len % kGraphSteps;
-            if (nrest && !one) one = capture_steps(cs, 1, fflags);
+            if (nrest && !one) one = capture_steps(cs, 1, fflags, ring);
             for (int k = 0; k < nbig; ++k) MDX_HIP(hipGraphLaunch(big, cs[0]->stream));
             for (int k = 0; k < nrest; ++k) MDX_HIP(hipGraphLaunch(one, cs[0]->stream));
             MDX_HIP(hipStreamSynchronize(cs[0]->stream));
@@ -1169,6 +1220,89 @@ int mdx_run_batch(mdx_ctx *const *ctxs, int n, int nsteps, int flags) {
             cs.push_back(&ctxs[k]->c);
         }
         run_captured(cs, nsteps, flags);
+    });
+}
+
+int mdx_ring_run(mdx_ctx *const *beads, int P, int nsteps, int flags, double omegaP) {
+    if (!beads || P <= 0) return MDX_ERR_ARG;
+    for (int k = 0; k < P; ++k)
+        if (!beads[k]) return MDX_ERR_ARG;
+    return guarded(beads[0], [&](Ctx &c0) {
+        MDX_REQUIRE(nsteps >= 0, MDX_ERR_ARG, "nsteps must be >= 0");
+        MDX_REQUIRE(P >= 2 && P <= 64 && (P & (P - 1)) == 0, MDX_ERR_ARG, "beads: a power of two from 2 to 64");
+        MDX_REQUIRE(omegaP > 0, MDX_ERR_ARG, "ring frequency must be positive");
+        std::vector<Ctx *> cs;
+        for (int k = 0; k < P; ++k) {
+            Ctx &c = beads[k]->c;
+            for (Ctx *o : cs) MDX_REQUIRE(o != &c, MDX_ERR_ARG, "a bead appears twice");
+            require_system(c);
+            MDX_REQUIRE(c.rlist > 0, MDX_ERR_STATE, "mdx_nlist_config not called");
+            MDX_REQUIRE(c.n == c0.n && c.device == c0.device && c.dt == c0.dt, MDX_ERR_ARG,
+                        "beads must hold the same atoms, device and time step");
+            MDX_REQUIRE(!c.rigid.on && !c.baro.on, MDX_ERR_STATE,
+                        "ring polymers take flexible molecules at constant volume");
+            for (int d = 0; d < 3; ++d)
+                MDX_REQUIRE(c.geo.L[d] == c0.geo.L[d] && c.geo.lo[d] == c0.geo.lo[d], MDX_ERR_ARG,
+                            "beads must share the box");
+            cs.push_back(&c);
+        }
+        // normal-mode matrix and per-mode rotation (nve_xPIMD_cu), in FP64 on the host
+        std::vector<double> C((size_t)P * P), cw(3 * (size_t)P);
+        const int h = P / 2;
+        for (int nb = 0; nb < P; ++nb) {
+            const int n = nb + 1;
+            for (int k = 0; k < P; ++k) {
+                double v;
+                if (k == 0) v = std::sqrt(1.0 / P);
+                else if (k < h) v = std::sqrt(2.0 / P) * std::cos(2.0 * M_PI * k * n / P);
+                else if (k == h) v = std::sqrt(1.0 / P) * ((n & 1) ? -1.0 : 1.0);
+                else v = std::sqrt(2.0 / P) * std::sin(2.0 * M_PI * k * n / P);
+                C[(size_t)nb * P + k] = v;
+            }
+        }
+        const double dt = c0.dt;
+        for (int k = 0; k < P; ++k) {
+            if (k == 0) {
+                cw[0] = 1.0;
+                cw[1] = dt;
+                cw[2] = 0.0;
+            } else {
+                const double w = 2.0 * omegaP * std::sin(k * M_PI / P);
+                cw[3 * k] = std::cos(w * dt);
+                cw[3 * k + 1] = std::sin(w * dt) / w;
+                cw[3 * k + 2] = w * std::sin(w * dt);
+            }
+        }
+        RingSpec ring;
+        ring.P = P;
+        ring.natoms = c0.n;
+        ring.C = dalloc<double>(c0, C.size());
+        ring.cw = dalloc<double>(c0, cw.size());
+        ring.tab = (void **)dalloc<char>(c0, sizeof(void *) * 3 * P);
+        upload(c0, ring.C, C.data(), C.size());
+        upload(c0, ring.cw, cw.data(), cw.size());
+        // every bead runs on bead 0's stream for the ring's step-by-step coupling
+        std::vector<hipStream_t> own;
+        for (Ctx *c : cs) {
+            MDX_HIP(hipStreamSynchronize(c->stream));
+            own.push_back(c->stream);
+            c->stream = c0.stream;
+        }
+        auto restore = [&] {
+            for (size_t k = 0; k < cs.size(); ++k) cs[k]->stream = own[k];
+            dfree(c0, ring.C);
+            dfree(c0, ring.cw);
+            dfree(c0, (char *)ring.tab);
+        };
+        try {
+            run_captured(cs, nsteps, flags, &ring);
+        } catch (...) {
+            hipStreamSynchronize(c0.stream);
+            restore();
+            throw;
+        }
+        MDX_HIP(hipStreamSynchronize(c0.stream));
+        restore();
     });
 }
 

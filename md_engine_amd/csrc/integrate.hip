@@ -44,6 +44,19 @@ __global__ void k_vv1(int n, double4 *__restrict__ pos, double4 *__restrict__ ve
     pos[i] = x;
 }
 
+// half kick only (the drift of a ring polymer is its normal-mode propagation)
+__global__ void k_kick(int n, double4 *__restrict__ vel, const unsigned long long *__restrict__ f, double hdtf) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 v = vel[i];
+    if (v.w == 0.0) return;
+    const double s = hdtf * v.w;
+    v.x += s * fx_to_d(f[3 * i + 0]);
+    v.y += s * fx_to_d(f[3 * i + 1]);
+    v.z += s * fx_to_d(f[3 * i + 2]);
+    vel[i] = v;
+}
+
 __global__ void k_vv2(int n, double4 *__restrict__ vel, const unsigned long long *__restrict__ f,
                       double hdtf, const int *__restrict__ member) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -244,6 +257,12 @@ void integrate_first_half(Ctx &c) {
         hipLaunchKernelGGL(k_vv1, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n,
                            c.pos[c.cur], c.vel[c.cur], c.force, 0.5 * c.dt * c.ftmToV, c.dt, member);
     if (c.rigid.on) rigid_first_half(c);
+}
+
+void integrate_kick_first(Ctx &c) {
+    if (c.nh.on) nh_half(c, false);
+    hipLaunchKernelGGL(k_kick, dim3(cdiv(c.n, 256)), dim3(256), 0, c.stream, c.n, c.vel[c.cur], c.force,
+                       0.5 * c.dt * c.ftmToV);
 }
 
 void integrate_second_half(Ctx &c) {

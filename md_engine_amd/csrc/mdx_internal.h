@@ -254,6 +254,7 @@ struct PppmState {
     float4 *eField4 = nullptr;       // the field per mesh point (Ex, Ey, Ez, 0)
     float *green = nullptr;          // sx*sy*(sz/2+1)
     hipfftHandle planR2C = 0;
+    hipStream_t planStream = nullptr;  // stream the plans are bound to
     hipfftHandle planC2R = 0;
     bool plansMade = false;
 };
@@ -619,6 +620,17 @@ void baro_step(Ctx &c, const int *gate);
 // Green function from the device box on stream s (gate: nullptr = always)
 void pppm_green_device(Ctx &c, const int *gate, hipStream_t s);
 void rigid_first_half(Ctx &c);
+// ring.hip: ring-polymer beads, one context per bead, coupled by the
+// normal-mode propagation of the free ring polymer
+struct RingSpec {
+    int P = 0;            // beads
+    int natoms = 0;       // atoms per bead
+    double *C = nullptr;  // P x P orthonormal normal-mode matrix, C[n * P + k]
+    double *cw = nullptr; // per mode k: cos(w_k dt), sin(w_k dt) / w_k, w_k sin(w_k dt)
+    void **tab = nullptr; // per bead: pos, vel, idToIdx (device pointers)
+};
+void ring_drift(const RingSpec &r, const BoxDev *box, hipStream_t s);
+
 // flex.hip: molecule indices after a re-sort, M placement, intramolecular
 // forces and the M force spread onto O, H1, H2
 void flex_map(Ctx &c);
@@ -629,6 +641,8 @@ void rigid_second_half(Ctx &c, bool kick);
 
 // integrate.hip
 void integrate_first_half(Ctx &c);
+// thermostat half step and half kick without the drift (ring polymers)
+void integrate_kick_first(Ctx &c);
 void integrate_second_half(Ctx &c);
 // accumulator slots summed over the replicas (synchronises the stream)
 void acc_read(Ctx &c, double out[ACC_N]);

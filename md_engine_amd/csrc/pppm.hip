@@ -590,6 +590,7 @@ void pppm_prepare(Ctx &c) {
                     2, "hipfftPlanMany C2R failed");
         hipfftSetStream(P.planR2C, c.stream);
         hipfftSetStream(P.planC2R, c.stream);
+        P.planStream = c.stream;
         P.plansMade = true;
     }
     bool same = true;
@@ -651,6 +652,14 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
         default: hipLaunchKernelGGL(k_spread<7>, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.q[cur], c.box, qunit, (unsigned long long *)P.mesh); break;
     }
     hipLaunchKernelGGL(k_mesh_to_real, dim3(cdiv(ng, threads)), dim3(threads), 0, c.stream, ng, P.mesh, P.real);
+    }
+    if (P.planStream != c.stream) {
+        // the context's work moved to another stream (a ring polymer's beads
+        // share one): the transforms follow it
+        MDX_REQUIRE(hipfftSetStream(P.planR2C, c.stream) == HIPFFT_SUCCESS &&
+                        hipfftSetStream(P.planC2R, c.stream) == HIPFFT_SUCCESS,
+                    2, "hipfftSetStream failed");
+        P.planStream = c.stream;
     }
     {
     ProfScope ps(c, PK_FFT);

@@ -1,0 +1,117 @@
+// Ring-polymer (PIMD) drift: the free ring polymer of every atom propagated
+// exactly in normal modes over one time step, as nve_xPIMD_cu / preForcePIMD_cu
+// do (src/Integrators/IntegratorVerlet.cu:63-304, 342-593): positions and
+// velocities of the P beads go to normal-mode coordinates with the orthonormal
+// matrix C (k = 0: 1/sqrt P; 1 <= k < P/2: sqrt(2/P) cos(2 pi k n / P);
+// k = P/2: (-1)^n / sqrt P; P/2 < k < P: sqrt(2/P) sin(2 pi k n / P), n the
+// bead number from 1), mode 0 drifts, mode k rotates at
+// w_k = 2 w_P sin(k pi / P), and the beads come back with C^T.  Beads are
+// first put in the periodic image of bead 0, as the reference does.
+//
+// Here each bead is a whole system in its own context (its own neighbor
+// list, P3M and cell order), so atom `id` of bead n sits at idToIdx_n[id].
+// One team of P lanes per atom: lane n loads bead n, computes mode n, and
+// after an LDS exchange bead n again.  Massless sites (M) are left alone;
+// the force evaluation places them.
+#include "mdx_internal.h"
+
+namespace mdx {
+
+namespace {
+
+constexpr int kRingThreads = 256;
+
+__global__ void __launch_bounds__(kRingThreads) k_ring(int natoms, int P, void *const *__restrict__ tab,
+                                                       const double *__restrict__ C,
+                                                       const double *__restrict__ cw,
+                                                       const BoxDev *__restrict__ bx) {
+    extern __shared__ double sh[];  // [teams][P][6]: x, v of every bead
+    const int teams = kRingThreads / P;
+    const int team = threadIdx.x / P, n = threadIdx.x % P;
+    const int id = blockIdx.x * teams + team;
+    double *me = sh + (size_t)team * P * 6;
+    const Geometry g = bx->geo;
+    double4 *pos = (double4 *)tab[3 * n];
+    double4 *vel = (double4 *)tab[3 * n + 1];
+    const int *idToIdx = (const int *)tab[3 * n + 2];
+    const bool live = id < natoms;
+    int idx = 0;
+    double4 x = make_double4(0, 0, 0, 0), v = make_double4(0, 0, 0, 0);
+    if (live) {
+        idx = idToIdx[id];
+        x = pos[idx];
+        v = vel[idx];
+    }
+    const bool moves = live && v.w != 0.0;  // the same for every bead of the atom
+    me[n * 6 + 0] = x.x;
+    me[n * 6 + 1] = x.y;
+    me[n * 6 + 2] = x.z;
+    __syncthreads();
+    // bead n in bead 0's image
+    double xn[3] = {x.x, x.y, x.z};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double x0 = me[d];
+        double dd = xn[d] - x0;
+        if (g.periodic[d]) dd -= g.L[d] * rint(dd * g.invL[d]);
+        xn[d] = x0 + dd;
+    }
+    __syncthreads();
+    me[n * 6 + 0] = xn[0];
+    me[n * 6 + 1] = xn[1];
+    me[n * 6 + 2] = xn[2];
+    me[n * 6 + 3] = v.x;
+    me[n * 6 + 4] = v.y;
+    me[n * 6 + 5] = v.z;
+    __syncthreads();
+    // mode k = n
+    const int k = n;
+    double xt[3] = {0, 0, 0}, vt[3] = {0, 0, 0};
+    for (int m = 0; m < P; ++m) {
+        const double c = C[m * P + k];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            xt[d] += c * me[m * 6 + d];
+            vt[d] += c * me[m * 6 + 3 + d];
+        }
+    }
+    const double cs = cw[3 * k], sw = cw[3 * k + 1], ws = cw[3 * k + 2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double x1 = xt[d] * cs + vt[d] * sw;
+        const double v1 = vt[d] * cs - xt[d] * ws;
+        xt[d] = x1;
+        vt[d] = v1;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        me[k * 6 + d] = xt[d];
+        me[k * 6 + 3 + d] = vt[d];
+    }
+    __syncthreads();
+    double xb[3] = {0, 0, 0}, vb[3] = {0, 0, 0};
+    for (int kk = 0; kk < P; ++kk) {
+        const double c = C[n * P + kk];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            xb[d] += c * me[kk * 6 + d];
+            vb[d] += c * me[kk * 6 + 3 + d];
+        }
+    }
+    if (moves) {
+        pos[idx] = make_double4(xb[0], xb[1], xb[2], x.w);
+        vel[idx] = make_double4(vb[0], vb[1], vb[2], v.w);
+    }
+}
+
+}  // namespace
+
+void ring_drift(const RingSpec &r, const BoxDev *box, hipStream_t s) {
+    const int teams = kRingThreads / r.P;
+    const size_t shm = sizeof(double) * 6 * kRingThreads;
+    hipLaunchKernelGGL(k_ring, dim3(cdiv(r.natoms, teams)), dim3(kRingThreads), shm, s, r.natoms, r.P, r.tab, r.C,
+                       r.cw, box);
+}
+
+}  // namespace mdx

@@ -20,6 +20,7 @@ import numpy as np
 
 # units (src/Units.cpp:5-32)
 REAL_BOLTZ = 0.0019872067
+REAL_HBAR = 15.178728314015  # kcal/mol fs (Units.cpp:22)
 REAL_MVV_TO_ENG = 48.88821291 * 48.88821291
 REAL_QQR_TO_ENG = 332.06371
 REAL_FTM_TO_V = 1.0 / (48.88821291 * 48.88821291)

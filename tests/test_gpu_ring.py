@@ -1,0 +1,124 @@
+"""Ring-polymer (PIMD) steps on the GPU (mdx_ring_run, md_engine_amd/csrc/
+ring.hip): the free ring polymer against the exact normal-mode propagation of
+oracle/ring.py, and the ring Hamiltonian of flexible q-TIP4P/F water beads
+conserved to second order in dt."""
+import sys
+
+import numpy as np
+import pytest
+
+from md_engine_amd import systems
+from md_engine_amd._lib import ring_omega, run_ring
+from oracle import flexwater as FW
+from oracle import ring as R
+
+from test_gpu_flex import flex_engine
+from test_gpu_water_md import start_velocities
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def bead_systems(box, P, seed, spread=0.05):
+    """P copies of the box, the massive sites displaced a little per bead"""
+    rng = np.random.default_rng(seed)
+    mol = box["mol"]
+    out = []
+    for _ in range(P):
+        x = box["x"].copy()
+        for h in (0, 1, 2):
+            x[mol[:, h]] += rng.normal(0, spread, (len(mol), 3))
+        out.append(FW.place_m(mol, x, box["L"], FW.QTIP4PF["gamma"]))
+    return out
+
+
+def ring_engines(make, box, xs, vs, dt, rc=5.5, skin=0.5, sz=(32, 32, 32)):
+    es = []
+    for x, v in zip(xs, vs):
+        e, flags = flex_engine(make, dict(box, x=x), v, rc=rc, skin=skin, sz=sz, order=5, dt=dt)
+        es.append(e)
+    return es, flags
+
+
+def test_free_ring_matches_normal_modes(orc, gpu_engine_factory):
+    say("free ring polymer, 4 beads")
+    P = 4
+    box = systems.tip4p2005_box(64, seed=31, min_sep=1.8)
+    xs = bead_systems(box, P, 1)
+    rng = np.random.default_rng(2)
+    inv = np.where(box["mass"] > 0, 1.0, 0.0)[:, None]
+    vs = [rng.normal(0, 0.005, box["x"].shape) * inv for _ in range(P)]
+    es, _ = ring_engines(gpu_engine_factory, box, xs, vs, dt=0.5)
+    for e in es[1:]:
+        e.compute(0)  # lists and P3M plans made on the beads' own streams first
+    wP = ring_omega(P * 298.0, systems.REAL_BOLTZ, systems.REAL_HBAR)
+    nsteps = 12
+    run_ring(es, nsteps, 0, wP)  # no force terms: only the ring springs act
+    # oracle: the same beads put in bead 0's image, then exact steps
+    L = box["L"]
+    x = np.array(xs)
+    x = x[0] + (x - x[0]) - L * np.rint((x - x[0]) / L)
+    v = np.array(vs)
+    for _ in range(nsteps):
+        x, v = R.free_step(x, v, wP, 0.5)
+        x = x[0] + (x - x[0]) - L * np.rint((x - x[0]) / L)
+    mol = box["mol"]
+    massive = np.concatenate([mol[:, 0], mol[:, 1], mol[:, 2]])
+    for k, e in enumerate(es):
+        xg, vg = e.get_system()
+        d = xg[massive] - x[k][massive]
+        d -= L * np.rint(d / L)
+        assert np.abs(d).max() < 1e-9, f"bead {k}"
+        assert np.abs(vg[massive] - v[k][massive]).max() < 1e-11, f"bead {k}"
+        e.close()
+
+
+def test_ring_hamiltonian_second_order(orc, gpu_engine_factory):
+    """H = sum over beads of (KE + V) + the ring springs; its deviation over the
+    same 40 fs falls about fourfold when dt halves."""
+    say("flexible water ring, 4 beads: energy error vs dt")
+    from md_engine_amd import ENERGY
+
+    P = 4
+    box = systems.tip4p2005_box(64, seed=32, min_sep=1.8)
+    xs = bead_systems(box, P, 3, spread=0.02)
+    vs = [start_velocities(orc, dict(box, x=x), seed=10 + k) for k, x in enumerate(xs)]
+    wP = ring_omega(P * 298.0, systems.REAL_BOLTZ, systems.REAL_HBAR)
+    mvv = systems.REAL_MVV_TO_ENG
+    L = box["L"]
+    errs = []
+    for dt, nsteps in ((0.2, 200), (0.1, 400)):
+        es, flags = ring_engines(gpu_engine_factory, box, xs, vs, dt=dt)
+
+        def total():
+            h, xb = 0.0, []
+            ke = 0.0
+            for e in es:
+                e.compute(flags | ENERGY)
+                en, _ = e.energies()
+                x, v = e.get_system()
+                k = 0.5 * mvv * (box["mass"][:, None] * v ** 2).sum()
+                h += en.sum() + e.bonded_energy() + k
+                ke += k
+                xb.append(x)
+            xb = np.array(xb)
+            xb = xb[0] + (xb - xb[0]) - L * np.rint((xb - xb[0]) / L)
+            total.ke = ke
+            return h + R.spring_energy(xb, box["mass"], wP, mvv)
+
+        h0 = total()
+        ke0 = total.ke
+        dev = 0.0
+        for _ in range(10):
+            run_ring(es, nsteps // 10, flags, wP)
+            dev = max(dev, abs(total() - h0))
+        errs.append(dev)
+        for e in es:
+            e.close()
+    say(f"max |H - H0|: dt 0.2 fs {errs[0]:.5f}, dt 0.1 fs {errs[1]:.5f} kcal/mol")
+    assert 0 < errs[0] < 0.01 * ke0
+    assert errs[1] < 0.4 * errs[0]
