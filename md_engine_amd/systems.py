@@ -177,3 +177,51 @@ def water_bonds(mol):
     for o, h1, h2, m in np.asarray(mol):
         b += [(o, h1), (o, h2), (h1, h2), (o, m)]
     return b
+
+
+# q-TIP4P/F flexible water (mdx_flexible_water_set): the quartic O-H bond and
+# harmonic angle constants of the reference's example (examples/PIMD_TIP4P.py:
+# FixBondQuartic k2, k3, k4, r0; FixAngleHarmonic k, theta0) and the M-site
+# weight FixTIP4PFlexible uses for q-TIP4P/F (FixTIP4PFlexible.cu:325-330).
+# Charges, LJ and masses equal TIP4P/2005's.
+QTIP4PF = dict(k2=607.19, k3=-1388.65, k4=1852.58, r0=0.9419, ktheta=87.85, theta0=107.4 / 180.0 * np.pi,
+               gamma=0.73612)
+
+
+def ring_modes(P):
+    """orthonormal normal-mode matrix C[n, k] of a P-bead ring polymer
+    (nve_xPIMD_cu, IntegratorVerlet.cu:63-304; beads n = 1..P)"""
+    C = np.zeros((P, P))
+    h = P // 2
+    for nb in range(P):
+        n = nb + 1
+        for k in range(P):
+            if k == 0:
+                C[nb, k] = np.sqrt(1.0 / P)
+            elif k < h:
+                C[nb, k] = np.sqrt(2.0 / P) * np.cos(2 * np.pi * k * n / P)
+            elif k == h:
+                C[nb, k] = np.sqrt(1.0 / P) * (-1.0 if n % 2 else 1.0)
+            else:
+                C[nb, k] = np.sqrt(2.0 / P) * np.sin(2 * np.pi * k * n / P)
+    return C
+
+
+def ring_beads(x, mass, P, temp_beads, seed, boltz=REAL_BOLTZ, hbar=REAL_HBAR, mvv_to_eng=REAL_MVV_TO_ENG):
+    """P bead copies of positions x with the free ring polymer's normal modes
+    sampled at the bead temperature and the centroid at x, as
+    State::preparePIMD does (src/State.cpp:1066-1130): mode 0 = sqrt(P) x,
+    mode k ~ N(0, sqrt(kB T / m) / w_k), w_k = 2 (kB T / hbar) sin(k pi / P).
+    Massless sites stay on the centroid."""
+    rng = np.random.default_rng(seed)
+    n = len(x)
+    wp = boltz * temp_beads / hbar
+    xt = np.zeros((P, n, 3))
+    xt[0] = np.sqrt(P) * x
+    live = mass > 0
+    for k in range(1, P):
+        wk = 2.0 * wp * np.sin(k * np.pi / P)
+        sig = np.zeros(n)
+        sig[live] = np.sqrt(boltz * temp_beads / (mass[live] * mvv_to_eng)) / wk
+        xt[k] = rng.normal(size=(n, 3)) * sig[:, None]
+    return list(np.einsum("nk,kad->nad", ring_modes(P), xt))
