@@ -105,3 +105,49 @@ def tip4p_init_box(path, idealize=True):
     if "v" in fr:
         out["v"] = fr["v"]
     return out
+
+
+def _g(v):
+    """a double as std::ostream prints it by default (6 significant digits)"""
+    return "%g" % v
+
+
+def write_xyz(path, x, atomic_nums, lo, hi, v=None, f=None, append=True):
+    """Append one frame in the layout of WriteConfig's xyz writer
+    (writeXYZFile, src/WriteConfig.cpp:206-254): the atom count, then
+    ``bounds lo (x, y, z) hi (x, y, z)`` and one line per atom with its atomic
+    number (or type) and position, then velocity and force when given; the
+    frame ends with a newline.  x, v, f: (n, 3) by atom id."""
+    x = np.asarray(x, np.float64)
+    n = len(x)
+    vec = lambda a: "(" + ", ".join(_g(c) for c in a) + ")"  # noqa: E731  Vector::asStr
+    out = [str(n) + "\n" + "bounds lo " + vec(lo) + " hi " + vec(hi)]
+    for i in range(n):
+        row = "\n" + str(int(atomic_nums[i])) + " " + " ".join(_g(c) for c in x[i])
+        if v is not None:
+            row += " " + " ".join(_g(c) for c in v[i])
+        if f is not None:
+            row += " " + " ".join(_g(c) for c in f[i])
+        out.append(row)
+    out.append("\n")
+    with open(path, "a" if append else "w") as fh:
+        fh.write("".join(out))
+
+
+def read_xyz(path):
+    """frames written by write_xyz: list of (atomic numbers, (n, k) values, lo, hi)"""
+    frames = []
+    with open(path) as fh:
+        lines = fh.read().split("\n")
+    i = 0
+    while i < len(lines) and lines[i].strip():
+        n = int(lines[i])
+        b = re.findall(r"\(([^)]*)\)", lines[i + 1])
+        lo = np.array([float(t) for t in b[0].split(",")])
+        hi = np.array([float(t) for t in b[1].split(",")])
+        rows = [ln.split() for ln in lines[i + 2:i + 2 + n]]
+        nums = np.array([int(r[0]) for r in rows])
+        vals = np.array([[float(t) for t in r[1:]] for r in rows])
+        frames.append((nums, vals, lo, hi))
+        i += 2 + n
+    return frames
