@@ -527,6 +527,7 @@ int mdx_create(int device, mdx_ctx **out) {
         c.acc = dalloc<unsigned long long>(c, kAccTotal);
         c.dFlags = dalloc<int>(c, 12);
         c.dMax = dalloc<unsigned>(c, 4);
+        c.checkPart = dalloc<float>(c, 2 * kCheckBlocks);
         c.thermo = dalloc<unsigned long long>(c, 2);
         MDX_HIP(hipHostMalloc((void **)&c.pinned, sizeof(int) * 8, hipHostMallocDefault));
         std::memset(c.pinned, 0, sizeof(int) * 8);

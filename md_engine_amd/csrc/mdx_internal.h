@@ -408,6 +408,7 @@ struct Ctx {
     bool capture = false;         // a step is being recorded into a graph
     int graphSteps = 0;           // mdx_run: 1 = captured steps (option "graph")
     unsigned *dMax = nullptr;     // 2 slots x (max |x - x_last|^2, max predicted^2) as float bits
+    float *checkPart = nullptr;   // per-workgroup maxima of the displacement check (2 x kCheckBlocks)
     int *pinned = nullptr;        // host-pinned: [0..1] last build's max counts, [2..5] dMax copies, [6] stream overflow
     hipEvent_t capEvent = nullptr;
     bool capPending = false;
@@ -578,6 +579,7 @@ void fill_int(Ctx &c, int *p, int count, int v, const int *gate = nullptr);  // 
 void nlist_rebuild_async(Ctx &c);     // rebuild now; capacity checked later (capacity_ok)
 bool capacity_ok(Ctx &c);             // false: rows grew, results since the last check are invalid
 bool grow_if_needed(Ctx &c);
+constexpr int kCheckBlocks = 512;     // workgroups of the displacement check
 void check_max(Ctx &c, int parity);   // max displacement / speed into dMax slot `parity`
 void decide_step(Ctx &c);             // captured step: device rebuild / thermostat gates
 

@@ -252,14 +252,31 @@ __global__ void k_copy_back(const int *__restrict__ gate, int n, const double4 *
     id0[k] = id1[k];
 }
 
+// one wave: the maxima of the nb workgroups
+__device__ __forceinline__ void part_max(int nb, const float *__restrict__ part, float &d2, float &p2) {
+    d2 = 0.f;
+    p2 = 0.f;
+    for (int b = threadIdx.x; b < nb; b += 64) {
+        d2 = fmaxf(d2, part[2 * b]);
+        p2 = fmaxf(p2, part[2 * b + 1]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        d2 = fmaxf(d2, __shfl_xor(d2, o, 64));
+        p2 = fmaxf(p2, __shfl_xor(p2, o, 64));
+    }
+}
+
 // Device-side step decisions of a captured step (one thread): rebuild the
 // list iff the largest displacement since the last build exceeds skin/2 (the
 // reference's setBuildFlag test, GridGPU.cu:755-774), count it, and set the
 // thermostat gate for the steps where the turn reaches a multiple of its
 // interval.  Clears the displacement slot for the next step.
-__global__ void k_decide(unsigned *__restrict__ dmax, int *__restrict__ flags, float half2, int every,
-                         int baroEvery) {
-    const float d2 = __uint_as_float(dmax[0]);
+__global__ void k_decide(int nb, const float *__restrict__ part, unsigned *__restrict__ dmax,
+                         int *__restrict__ flags, float half2, int every, int baroEvery) {
+    float d2, p2;
+    part_max(nb, part, d2, p2);
+    if (threadIdx.x != 0) return;
     const int build = d2 > half2 ? 1 : 0;
     flags[3] = build;
     flags[4] += build;
@@ -1294,18 +1311,15 @@ namespace {
 // the slot one step later.
 // Grid-stride over the sites with a workgroup reduction: one atomic pair per
 // workgroup (same-address atomics serialise in L2).
-__global__ void __launch_bounds__(256) k_check_max(int n, const double4 *__restrict__ pos,
-                                                   const double4 *__restrict__ last,
-                                                   const double4 *__restrict__ vel, const BoxDev *__restrict__ bx,
-                                                   double dt2, unsigned *__restrict__ slot,
-                                                   unsigned *__restrict__ other) {
+// Per-workgroup maxima into part[2 b], part[2 b + 1] (float bits); a
+// one-wave kernel reduces them (no same-address atomics).
+__global__ void __launch_bounds__(256) k_check_part(int n, const double4 *__restrict__ pos,
+                                                    const double4 *__restrict__ last,
+                                                    const double4 *__restrict__ vel, const BoxDev *__restrict__ bx,
+                                                    double dt2, float *__restrict__ part) {
     const Geometry g = bx->geo;
     __shared__ float red[2][4];
     const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t0 == 0) {
-        other[0] = 0u;
-        other[1] = 0u;
-    }
     float d2 = 0.f, p2 = 0.f;
     for (int i = t0; i < n; i += gridDim.x * blockDim.x) {
         const double dd = dist2(pos[i], last[i], g);
@@ -1330,25 +1344,43 @@ __global__ void __launch_bounds__(256) k_check_max(int n, const double4 *__restr
             d2 = fmaxf(d2, red[0][k]);
             p2 = fmaxf(p2, red[1][k]);
         }
-        atomicMax(&slot[0], __float_as_uint(d2));
-        atomicMax(&slot[1], __float_as_uint(p2));
+        part[2 * blockIdx.x] = d2;
+        part[2 * blockIdx.x + 1] = p2;
+    }
+}
+
+// slot = max(slot, this check) as float bits; the other slot cleared for the next step
+__global__ void k_check_reduce(int nb, const float *__restrict__ part, unsigned *__restrict__ slot,
+                               unsigned *__restrict__ other) {
+    float d2, p2;
+    part_max(nb, part, d2, p2);
+    if (threadIdx.x == 0) {
+        slot[0] = max(slot[0], __float_as_uint(d2));
+        slot[1] = max(slot[1], __float_as_uint(p2));
+        other[0] = 0u;
+        other[1] = 0u;
     }
 }
 }  // namespace
 
+static int check_blocks(const Ctx &c) { return std::max(1, std::min(cdiv(c.n, 256), kCheckBlocks)); }
+
+static void check_part(Ctx &c) {
+    hipLaunchKernelGGL(k_check_part, dim3(check_blocks(c)), dim3(256), 0, c.stream, c.n, c.pos[c.cur], c.posLast,
+                       c.vel[c.cur], c.box, 2.0 * c.dt, c.checkPart);
+}
+
 void decide_step(Ctx &c) {
-    check_max(c, 0);
+    check_part(c);
     const float half = (float)(0.5 * c.skin);
-    hipLaunchKernelGGL(k_decide, dim3(1), dim3(1), 0, c.stream, c.dMax, c.dFlags, half * half,
-                       c.thermoOn ? c.thermoEvery : 0, c.baro.on ? c.baro.every : 0);
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, c.stream, check_blocks(c), c.checkPart, c.dMax, c.dFlags,
+                       half * half, c.thermoOn ? c.thermoEvery : 0, c.baro.on ? c.baro.every : 0);
 }
 
 void check_max(Ctx &c, int parity) {
-    const int threads = 256;
-    const int blocks = std::min(cdiv(c.n, threads * 8), 16);
-    hipLaunchKernelGGL(k_check_max, dim3(blocks), dim3(threads), 0, c.stream, c.n,
-                       c.pos[c.cur], c.posLast, c.vel[c.cur], c.box, 2.0 * c.dt, c.dMax + 2 * parity,
-                       c.dMax + 2 * (1 - parity));
+    check_part(c);
+    hipLaunchKernelGGL(k_check_reduce, dim3(1), dim3(64), 0, c.stream, check_blocks(c), c.checkPart,
+                       c.dMax + 2 * parity, c.dMax + 2 * (1 - parity));
 }
 
 }  // namespace mdx
