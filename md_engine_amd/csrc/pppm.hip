@@ -273,22 +273,15 @@ __global__ void k_green_direct(const int *__restrict__ gate, const BoxDev *__res
     G[idx] = (float)(green_point_any(k, m, 0.25 / alpha / alpha, order, lim) / k2);
 }
 
-// 1/x from the hardware estimate and two Newton steps (close to the
-// correctly rounded quotient; the Green function is stored in FP32)
-__device__ __forceinline__ double rcp_nr(double x) {
-    double r = __builtin_amdgcn_rcp(x);
-    r = fma(r, fma(-x, r, 1.0), r);
-    return fma(r, fma(-x, r, 1.0), r);
-}
-
 // One point of the tabulated Green function: the 3 (2 LIM + 1) table entries
 // of its indices are loaded first, then the (2 LIM + 1)^3 images are summed
-// in registers in the order of k_green_direct.
+// in registers in the order of k_green_direct.  FP32, like the reference's
+// Green_function_cu (the result is stored in FP32).
 template <int LIM>
-__device__ __forceinline__ float green_tab_point(const double *tkp, const double *tex, const double *tw, int off1,
+__device__ __forceinline__ float green_tab_point(const float *tkp, const float *tex, const float *tw, int off1,
                                                  int off2, const MeshGeo &m, int ix, int iy, int iz) {
     constexpr int NI = 2 * LIM + 1;
-    double kx[NI], ey[NI], wy[NI], ky[NI], ex[NI], wx[NI], kz[NI], ez[NI], wz[NI];
+    float kx[NI], ey[NI], wy[NI], ky[NI], ex[NI], wx[NI], kz[NI], ez[NI], wz[NI];
 #pragma unroll
     for (int j = 0; j < NI; ++j) {
         const int a = j * m.sz[0] + ix, b = off1 + j * m.sz[1] + iy, c = off2 + j * m.sz[2] + iz;
@@ -303,25 +296,34 @@ __device__ __forceinline__ float green_tab_point(const double *tkp, const double
         wz[j] = tw[c];
     }
     // image 0 of each axis is k itself
-    const double k0 = kx[LIM], k1 = ky[LIM], k2v = kz[LIM];
-    const double k2 = k0 * k0 + k1 * k1 + k2v * k2v;
-    if (k2 == 0.0) return 0.0f;
-    double s1 = 0.0, s2 = 0.0;
+    const float k0 = kx[LIM], k1 = ky[LIM], k2v = kz[LIM];
+    const float k2 = k0 * k0 + k1 * k1 + k2v * k2v;
+    if (k2 == 0.0f) return 0.0f;
+    float s1 = 0.0f, s2 = 0.0f;
 #pragma unroll
     for (int a = 0; a < NI; ++a)
 #pragma unroll
         for (int b = 0; b < NI; ++b) {
-            const double wab = wx[a] * wy[b], eab = ex[a] * ey[b];
-            const double kab = kx[a] * kx[a] + ky[b] * ky[b], dab = k0 * kx[a] + k1 * ky[b];
+            const float wab = wx[a] * wy[b], eab = ex[a] * ey[b];
+            const float kab = kx[a] * kx[a] + ky[b] * ky[b], dab = k0 * kx[a] + k1 * ky[b];
 #pragma unroll
             for (int cc = 0; cc < NI; ++cc) {
-                const double W2 = wab * wz[cc];
-                const double kl = kab + kz[cc] * kz[cc];
-                s1 += eab * ez[cc] * (dab + k2v * kz[cc]) * rcp_nr(kl) * W2;
+                const float W2 = wab * wz[cc];
+                const float kl = kab + kz[cc] * kz[cc];
+                s1 += eab * ez[cc] * (dab + k2v * kz[cc]) / kl * W2;
                 s2 += W2;
             }
         }
-    return (float)(4.0 * kPi * s1 / (s2 * s2) / k2);
+    return 4.0f * (float)kPi * s1 / (s2 * s2) / k2;
+}
+
+// sinc with the reference's small-argument series (FixChargeEwald.cu:302-312), FP32
+__device__ __forceinline__ float sinc_f(float x) {
+    if (x < 0.1f && x > -0.1f) {
+        const float x2 = x * x;
+        return 1.0f - x2 * 0.16666666667f + x2 * x2 * 0.008333333333333333f - x2 * x2 * x2 * 0.00019841269841269841f;
+    }
+    return sinf(x) / x;
 }
 
 // Same function with the per-axis factors tabulated: the Gaussian
@@ -330,11 +332,11 @@ __device__ __forceinline__ float green_tab_point(const double *tkp, const double
 // computes k'_d = k_d + 2 pi j / h_d, exp(-k'_d^2 / 4 alpha^2) and
 // sinc(k'_d h_d / 2)^2p once into LDS (3 (2 lim + 1) (sz_x + sz_y + sz_z)
 // entries); each point then sums its (2 lim + 1)^3 images from products of
-// table entries.  lim > kGreenTabLim takes the direct sums.
+// table entries.  lim > kGreenTabLim takes the direct FP64 sums.
 constexpr int kGreenTabLim = 2;
 __global__ void __launch_bounds__(256) k_green(const int *__restrict__ gate, const BoxDev *__restrict__ bx,
                                                int szh, int order, float *__restrict__ G) {
-    extern __shared__ double gtab[];
+    extern __shared__ float gtab[];
     if (gate && *gate == 0) return;
     const MeshGeo m = bx->mesh;
     const double alpha = bx->alpha;
@@ -353,19 +355,20 @@ __global__ void __launch_bounds__(256) k_green(const int *__restrict__ gate, con
     const int NI = 2 * lim + 1;
     const int off1 = NI * m.sz[0], off2 = NI * (m.sz[0] + m.sz[1]);
     const int ne = NI * (m.sz[0] + m.sz[1] + m.sz[2]);
-    double *tkp = gtab, *tex = gtab + ne, *tw = gtab + 2 * ne;
+    float *tkp = gtab, *tex = gtab + ne, *tw = gtab + 2 * ne;
+    const float f4a2f = (float)f4a2;
     for (int e = threadIdx.x; e < ne; e += blockDim.x) {
         const bool z = e >= off2, y = !z && e >= off1;
         const int n = z ? m.sz[2] : (y ? m.sz[1] : m.sz[0]);
         const double Ld = z ? m.L[2] : (y ? m.L[1] : m.L[0]);
         const double hd = z ? m.h[2] : (y ? m.h[1] : m.h[0]);
         const int r = e - (z ? off2 : (y ? off1 : 0)), j = r / n, i = r - j * n;
-        const double q = kcomp(i, n, Ld) + 2.0 * kPi * (j - lim) / hd;
-        const double W = sinc_d(q * hd * 0.5);
-        double W2 = 1.0;
+        const float q = (float)(kcomp(i, n, Ld) + 2.0 * kPi * (j - lim) / hd);
+        const float W = sinc_f(q * (float)hd * 0.5f);
+        float W2 = 1.0f;
         for (int o = 0; o < 2 * order; ++o) W2 *= W;
         tkp[e] = q;
-        tex[e] = exp(-q * q * f4a2);
+        tex[e] = expf(-q * q * f4a2f);
         tw[e] = W2;
     }
     __syncthreads();
@@ -612,9 +615,9 @@ void pppm_green_device(Ctx &c, const int *gate, hipStream_t s) {
     const PppmState &P = c.pppm;
     const int szh = P.sz[2] / 2 + 1;
     const size_t nh = (size_t)P.sz[0] * P.sz[1] * szh;
-    // table of 3 (2 kGreenTabLim + 1) (sz_x + sz_y + sz_z) doubles; meshes too
+    // table of 3 (2 kGreenTabLim + 1) (sz_x + sz_y + sz_z) floats; meshes too
     // large for it take the direct kernel
-    const size_t tab = sizeof(double) * 3 * (2 * kGreenTabLim + 1) * (P.sz[0] + P.sz[1] + P.sz[2]);
+    const size_t tab = sizeof(float) * 3 * (2 * kGreenTabLim + 1) * (P.sz[0] + P.sz[1] + P.sz[2]);
     if (tab <= 64 * 1024)
         hipLaunchKernelGGL(k_green, dim3(cdiv(nh, 256)), dim3(256), tab, s, gate, c.box, szh, P.order, P.green);
     else
