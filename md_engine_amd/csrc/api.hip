@@ -124,7 +124,7 @@ double pressure_now(Ctx &c) {
     double h[ACC_N];
     acc_read(c, h);
     auto v = [&](int k) { return h[k]; };
-    const double W = v(ACC_VIR) + v(ACC_VIR + 1) + v(ACC_VIR + 2);
+    const double W = v(ACC_VIR) + v(ACC_VIR + 1) + v(ACC_VIR + 2) + v(ACC_VTRACE);
     const double vol = c.geo.L[0] * c.geo.L[1] * c.geo.L[2];
     return (v(ACC_MOL + 1) * c.baro.mvv + W - v(ACC_MOL)) / (3.0 * vol) * c.baro.nktv2p;
 }
@@ -160,17 +160,21 @@ bool regrid_if_needed(Ctx &c) {
 }
 
 void compute(Ctx &c, int flags) {
-    const bool energy = flags & MDX_ENERGY, virial = flags & MDX_VIRIAL;
+    const bool energy = flags & MDX_ENERGY;
+    // a trace-only request still gets the full tensor from the terms that
+    // have no trace mode (P3M, flexible water, surrogate)
+    const bool virial = (flags & MDX_VIRIAL) || (flags & kFlagVirialTrace);
+    c.vtrace = (flags & kFlagVirialTrace) && !(flags & MDX_VIRIAL) && !energy;
     if (c.flex.on) flex_place_m(c);  // the M sites follow O and H before any force term
     prepare_forces(c);
-    if ((flags & MDX_PAIR)) pair_forces(c, energy, virial);
+    if ((flags & MDX_PAIR)) pair_forces(c, energy, c.vtrace ? false : virial);
     if ((flags & MDX_PPPM) && c.pppm.on) pppm_forces(c, energy, virial);
     if ((flags & MDX_SURROGATE) && c.e3b.on) {
         MDX_REQUIRE(c.sur.on, MDX_ERR_STATE, "MDX_SURROGATE needs a model (mdx_surrogate_set)");
         e3b_forces(c, energy, virial, false);  // two-body term stays analytic
         surrogate_forces(c, energy, virial);
     } else if ((flags & MDX_E3B) && c.e3b.on) {
-        e3b_forces(c, energy, virial);
+        e3b_forces(c, energy, c.vtrace ? false : virial);
     }
     if (c.flex.on) {
         if (flags & MDX_BONDED) flex_forces(c, energy, virial);
@@ -282,7 +286,7 @@ void captured_step(Ctx &c, int fflags) {
     decide_step(c);
     nlist_rebuild_async(c);  // every kernel gated by dFlags[3]
     // a barostat needs the virial; its steps are gated by dFlags[8]
-    compute(c, fflags | (c.baro.on ? MDX_VIRIAL : 0));
+    compute(c, fflags | (c.baro.on ? kFlagVirialTrace : 0));
     integrate_second_half(c);
     if (c.thermoOn) thermostat_apply(c);  // gated by dFlags[5]
     if (c.baro.on) baro_step(c, baro_gate(c));
@@ -1181,7 +1185,7 @@ int mdx_run(mdx_ctx *ctx, int nsteps, int flags) {
                 integrate_first_half(c);
                 if (lagged_rebuild_decision(c, 1 - par)) nlist_rebuild_async(c);
                 check_and_copy(c, par);
-                compute(c, fflags | (baro ? MDX_VIRIAL : 0));
+                compute(c, fflags | (baro ? kFlagVirialTrace : 0));
                 integrate_second_half(c);
                 ++c.turn;
                 if (c.thermoOn && c.turn % c.thermoEvery == 0) thermostat_apply(c);

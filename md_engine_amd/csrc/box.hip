@@ -97,16 +97,17 @@ __global__ void __launch_bounds__(64) k_baro(const int *__restrict__ gate, BoxDe
                                              BaroArgs ba, const unsigned long long *__restrict__ acc) {
     if (gate && *gate == 0) return;
     const int lane = threadIdx.x;
-    constexpr int kSlots[5] = {ACC_VIR, ACC_VIR + 1, ACC_VIR + 2, ACC_MOL, ACC_MOL + 1};
-    long long t[5];
+    constexpr int kSlots[6] = {ACC_VIR, ACC_VIR + 1, ACC_VIR + 2, ACC_MOL, ACC_MOL + 1, ACC_VTRACE};
+    long long t[6];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < 6; ++k) {
         t[k] = lane < kAccRep ? (long long)acc[lane * ACC_N + kSlots[k]] : 0ll;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) t[k] += __shfl_xor(t[k], o);
     }
     if (lane != 0) return;
-    const double W = (double)t[0] * kFxInv + (double)t[1] * kFxInv + (double)t[2] * kFxInv;
+    // trace of the virial: the tensor's diagonal plus the trace-only terms
+    const double W = (double)t[0] * kFxInv + (double)t[1] * kFxInv + (double)t[2] * kFxInv + (double)t[5] * kFxInv;
     const double corr = (double)t[3] * kFxInv, mv2 = (double)t[4] * kFxInv;
     const double P = (mv2 * ba.mvv + W - corr) / (3.0 * bx->vol) * ba.nktv2p;
     double mu = cbrt(1.0 - ba.dt / ba.period * (ba.P0 - P) / ba.bulk);

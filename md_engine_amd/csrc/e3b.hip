@@ -97,6 +97,20 @@ __device__ __forceinline__ void block_acc7(const double (&v)[7], int eslot, unsi
     }
 }
 
+// one value: wave sums, one atomic per workgroup
+__device__ __forceinline__ void block_acc1(double v, int slot, unsigned long long *acc) {
+    __shared__ double red1[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const double x = team_sum_d<64>(v);
+    if (lane == 0) red1[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += red1[k];
+        if (t != 0.0) fx_add(&acc_rep(acc)[slot], t);
+    }
+}
+
 __device__ __forceinline__ void box_into(E3BArgs &p) {
     for (int d = 0; d < 3; ++d) {
         p.L[d] = p.bx->Lf[d];
@@ -104,8 +118,10 @@ __device__ __forceinline__ void box_into(E3BArgs &p) {
     }
 }
 
-template <int P, bool EV>
+// EVM: 0 forces only; 1 energy and virial tensor; 2 the virial's trace only
+template <int P, int EVM>
 __global__ void __launch_bounds__(256) k_e3b_pass1(E3BArgs p) {
+    constexpr bool EV = EVM == 1, VT = EVM == 2;
     box_into(p);
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = gt / P;
@@ -130,6 +146,7 @@ __global__ void __launch_bounds__(256) k_e3b_pass1(E3BArgs p) {
                 fx += d.x * s;
                 fy += d.y * s;
                 fz += d.z * s;
+                if (VT) w0 += s * r2;
                 if (EV) {
                     e2 += 0.5f * p.E2 * ex;
                     w0 += d.x * s * d.x;
@@ -164,10 +181,12 @@ __global__ void __launch_bounds__(256) k_e3b_pass1(E3BArgs p) {
         const double v[7] = {e2, 0.5 * w0, 0.5 * w1, 0.5 * w2, 0.5 * w3, 0.5 * w4, 0.5 * w5};
         block_acc7(v, ACC_E3B2, p.acc);
     }
+    if (VT) block_acc1(0.5 * w0, ACC_VTRACE, p.acc);
 }
 
-template <int P, bool EV>
+template <int P, int EVM>
 __global__ void __launch_bounds__(256) k_e3b_pass2(E3BArgs p) {
+    constexpr bool EV = EVM == 1, VT = EVM == 2;
     box_into(p);
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = gt / P;
@@ -215,6 +234,11 @@ __global__ void __launch_bounds__(256) k_e3b_pass2(E3BArgs p) {
             FO.x -= cB1 * gB1.x + cB2 * gB2.x;
             FO.y -= cB1 * gB1.y + cB2 * gB2.y;
             FO.z -= cB1 * gB1.z + cB2 * gB2.z;
+            if (VT)
+                w0 += 0.5f * (cA1 * (gA1.x * dA1.x + gA1.y * dA1.y + gA1.z * dA1.z) +
+                              cA2 * (gA2.x * dA2.x + gA2.y * dA2.y + gA2.z * dA2.z) +
+                              cB1 * (gB1.x * dB1.x + gB1.y * dB1.y + gB1.z * dB1.z) +
+                              cB2 * (gB2.x * dB2.x + gB2.y * dB2.y + gB2.z * dB2.z));
             if (EV) {
                 ea += fA1 * fA2;
                 eb += Dij * Dji;
@@ -265,26 +289,33 @@ __global__ void __launch_bounds__(256) k_e3b_pass2(E3BArgs p) {
         const double v[7] = {e3, w0, w1, w2, w3, w4, w5};
         block_acc7(v, ACC_E3B3, p.acc);
     }
+    if (VT) {
+        block_acc1(w0, ACC_VTRACE, p.acc);
+    }
 }
 
 template <int P>
-void launch(Ctx &c, const E3BArgs &a, bool ev) {
+void launch(Ctx &c, const E3BArgs &a, int evm) {
     const int threads = 256;
     const int blocks = cdiv((long long)a.nmol * P, threads);
     hipStream_t s = c.stream;
     {
         ProfScope ps(c, PK_E3B1);
-        if (ev)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass1<P, true>), dim3(blocks), dim3(threads), 0, s, a);
+        if (evm == 1)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass1<P, 1>), dim3(blocks), dim3(threads), 0, s, a);
+        else if (evm == 2)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass1<P, 2>), dim3(blocks), dim3(threads), 0, s, a);
         else
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass1<P, false>), dim3(blocks), dim3(threads), 0, s, a);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass1<P, 0>), dim3(blocks), dim3(threads), 0, s, a);
     }
     if (!a.three) return;
     ProfScope ps(c, PK_E3B2);
-    if (ev)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass2<P, true>), dim3(blocks), dim3(threads), 0, s, a);
+    if (evm == 1)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass2<P, 1>), dim3(blocks), dim3(threads), 0, s, a);
+    else if (evm == 2)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass2<P, 2>), dim3(blocks), dim3(threads), 0, s, a);
     else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass2<P, false>), dim3(blocks), dim3(threads), 0, s, a);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass2<P, 0>), dim3(blocks), dim3(threads), 0, s, a);
 }
 
 }  // namespace
@@ -319,12 +350,12 @@ void e3b_forces(Ctx &c, bool energy, bool virial, bool three_body, float *emol) 
     a.acc = c.acc;
     a.three = three_body ? 1 : 0;
     a.emol = emol;
-    const bool ev = energy || virial || emol;
+    const int evm = (energy || virial || emol) ? 1 : (c.vtrace ? 2 : 0);
     switch (E.team) {
-        case 4: launch<4>(c, a, ev); break;
-        case 8: launch<8>(c, a, ev); break;
-        case 16: launch<16>(c, a, ev); break;
-        default: launch<32>(c, a, ev); break;
+        case 4: launch<4>(c, a, evm); break;
+        case 8: launch<8>(c, a, evm); break;
+        case 16: launch<16>(c, a, evm); break;
+        default: launch<32>(c, a, evm); break;
     }
 }
 

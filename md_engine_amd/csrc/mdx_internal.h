@@ -38,6 +38,7 @@ enum AccSlot {
     ACC_VIR = 4,      // 6 slots: xx yy zz xy xz yz
     ACC_MOL = 10,     // 2 slots: molecular virial correction, sum M V^2 of centres of mass
     ACC_BOND = 12,    // intramolecular (quartic bond + harmonic angle) energy
+    ACC_VTRACE = 13,  // virial trace of terms evaluated in trace-only mode (barostat steps)
     ACC_N = 16
 };
 // The accumulator block is replicated kAccRep times; a workgroup adds into
@@ -347,8 +348,13 @@ struct RigidState {
     double mO = 0, mH = 0, dOH = 0, dHH = 0, gamma = 0;
 };
 
+// compute flag (internal): the virial's trace is enough (a barostat step);
+// the pair and E3B kernels then add r . F into ACC_VTRACE only
+constexpr int kFlagVirialTrace = 1 << 12;
+
 struct Ctx {
     int device = 0;
+    bool vtrace = false;  // set by compute for the terms it launches
     hipStream_t stream = nullptr;
     std::string err;
 

@@ -42,8 +42,11 @@ struct PairArgs {
     unsigned long long *acc;
 };
 
-template <int P, bool LJ, bool COUL, bool EV>
+// EVM: 0 forces only; 1 energy and virial tensor; 2 the virial's trace only
+// (sum r F, what a barostat needs), into ACC_VTRACE
+template <int P, bool LJ, bool COUL, int EVM>
 __global__ void __launch_bounds__(256) k_pair(PairArgs a) {
+    constexpr bool EV = EVM == 1, VT = EVM == 2;
     // box values into locals (writing them into the argument struct would move
     // it to scratch memory: a.spec is indexed by the pair tag)
     const float bL0 = a.bx->Lf[0], bL1 = a.bx->Lf[1], bL2 = a.bx->Lf[2];
@@ -129,6 +132,7 @@ __global__ void __launch_bounds__(256) k_pair(PairArgs a) {
                     fx += dx * fs;
                     fy += dy * fs;
                     fz += dz * fs;
+                    if (VT) v0 += fs * r2;
                     if (EV) {
                         v0 += dx * fs * dx;
                         v1 += dy * fs * dy;
@@ -149,6 +153,17 @@ __global__ void __launch_bounds__(256) k_pair(PairArgs a) {
         fx_add(&a.force[3 * i + 1], fy);
         fx_add(&a.force[3 * i + 2], fz);
     }
+    if (VT) {
+        double x = team_sum_d<64>(0.5 * (double)v0);
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        if (lane == 0) wred[w][0] = x;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            x = 0.0;
+            for (int ww = 0; ww < (int)(blockDim.x >> 6); ++ww) x += wred[ww][0];
+            fx_add(&acc_rep(a.acc)[ACC_VTRACE], x);
+        }
+    }
     if (EV) {
         double vals[7] = {e, 0.5 * v0, 0.5 * v1, 0.5 * v2, 0.5 * v3, 0.5 * v4, 0.5 * v5};
         const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -167,20 +182,22 @@ __global__ void __launch_bounds__(256) k_pair(PairArgs a) {
 }
 
 template <int P, bool LJ, bool COUL>
-void launch3(const PairArgs &a, bool ev, hipStream_t s) {
+void launch3(const PairArgs &a, int evm, hipStream_t s) {
     const int threads = 256;
     const int blocks = cdiv((long long)a.n * P, threads);
-    if (ev)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair<P, LJ, COUL, true>), dim3(blocks), dim3(threads), 0, s, a);
+    if (evm == 1)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair<P, LJ, COUL, 1>), dim3(blocks), dim3(threads), 0, s, a);
+    else if (evm == 2)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair<P, LJ, COUL, 2>), dim3(blocks), dim3(threads), 0, s, a);
     else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair<P, LJ, COUL, false>), dim3(blocks), dim3(threads), 0, s, a);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair<P, LJ, COUL, 0>), dim3(blocks), dim3(threads), 0, s, a);
 }
 
 template <int P>
-void launch2(const PairArgs &a, bool lj, bool coul, bool ev, hipStream_t s) {
-    if (lj && coul) launch3<P, true, true>(a, ev, s);
-    else if (lj) launch3<P, true, false>(a, ev, s);
-    else launch3<P, false, true>(a, ev, s);
+void launch2(const PairArgs &a, bool lj, bool coul, int evm, hipStream_t s) {
+    if (lj && coul) launch3<P, true, true>(a, evm, s);
+    else if (lj) launch3<P, true, false>(a, evm, s);
+    else launch3<P, false, true>(a, evm, s);
 }
 
 }  // namespace
@@ -233,13 +250,13 @@ void pair_forces(Ctx &c, bool energy, bool virial) {
     a.rq2 = c.rq2;
     a.force = c.force;
     a.acc = c.acc;
-    const bool ev = energy || virial;
+    const int evm = (energy || virial) ? 1 : (c.vtrace ? 2 : 0);
     ProfScope ps(c, PK_PAIR);
     switch (c.pairTeam) {
-        case 4: launch2<4>(a, c.ljOn, coul, ev, c.stream); break;
-        case 8: launch2<8>(a, c.ljOn, coul, ev, c.stream); break;
-        case 16: launch2<16>(a, c.ljOn, coul, ev, c.stream); break;
-        default: launch2<32>(a, c.ljOn, coul, ev, c.stream); break;
+        case 4: launch2<4>(a, c.ljOn, coul, evm, c.stream); break;
+        case 8: launch2<8>(a, c.ljOn, coul, evm, c.stream); break;
+        case 16: launch2<16>(a, c.ljOn, coul, evm, c.stream); break;
+        default: launch2<32>(a, c.ljOn, coul, evm, c.stream); break;
     }
 }
 
