@@ -287,9 +287,13 @@ void captured_step(Ctx &c, int fflags) {
     nlist_rebuild_async(c);  // every kernel gated by dFlags[3]
     // a barostat needs the virial; its steps are gated by dFlags[8]
     compute(c, fflags | (c.baro.on ? kFlagVirialTrace : 0));
+    // all-water systems without a Nose-Hoover chain: the rigid second half
+    // sums the molecular virial terms (no separate pass over the molecules)
+    c.fuseMolVirial = c.baro.on && c.rigid.on && c.rigid.nsites == c.n && !c.nh.on;
     integrate_second_half(c);
     if (c.thermoOn) thermostat_apply(c);  // gated by dFlags[5]
     if (c.baro.on) baro_step(c, baro_gate(c));
+    c.fuseMolVirial = false;
 }
 
 // One step of a ring polymer whose beads share one stream: every bead's half

@@ -28,6 +28,12 @@ struct BaroArgs {
     double q2, rcut;
     // cell edges below these lengths invalidate the grid
     double minL[3];
+    // the centre-of-mass M V^2 was summed before a velocity rescale of the same
+    // step (tgate open): scale it by that rescale's lambda^2 (k_vscale's formula)
+    int rescaled, tmode;
+    const int *tgate;
+    const unsigned long long *thermo;
+    double tconv, T0, coupling;
 };
 
 // The secant solve of pppm_solve_alpha for a fixed interpolation order, with
@@ -108,7 +114,15 @@ __global__ void __launch_bounds__(64) k_baro(const int *__restrict__ gate, BoxDe
     if (lane != 0) return;
     // trace of the virial: the tensor's diagonal plus the trace-only terms
     const double W = (double)t[0] * kFxInv + (double)t[1] * kFxInv + (double)t[2] * kFxInv + (double)t[5] * kFxInv;
-    const double corr = (double)t[3] * kFxInv, mv2 = (double)t[4] * kFxInv;
+    const double corr = (double)t[3] * kFxInv;
+    double mv2 = (double)t[4] * kFxInv;
+    if (ba.rescaled && (!ba.tgate || *ba.tgate)) {
+        const double T = (double)(long long)ba.thermo[0] * kFxInv * ba.tconv;
+        if (T > 0.0) {
+            const double l2 = ba.tmode == 0 ? ba.T0 / T : 1.0 + ba.coupling * (ba.T0 / T - 1.0);
+            mv2 *= fmax(l2, 0.0);
+        }
+    }
     const double P = (mv2 * ba.mvv + W - corr) / (3.0 * bx->vol) * ba.nktv2p;
     double mu = cbrt(1.0 - ba.dt / ba.period * (ba.P0 - P) / ba.bulk);
     mu = fmin(fmax(mu, 1.0 - ba.maxDil), 1.0 + ba.maxDil);
@@ -188,11 +202,12 @@ void box_pull(Ctx &c) {
 }
 
 // Kernels of one barostat application (gate: nullptr = always).  The forces
-// of the step must have been computed with the virial.
+// of the step must have been computed with the virial.  With c.fuseMolVirial
+// the rigid second half already summed the molecular terms into ACC_MOL.
 void baro_step(Ctx &c, const int *gate) {
     const BarostatState &B = c.baro;
     // ACC_MOL starts at zero: every force evaluation clears the accumulators
-    molecular_virial(c, ACC_MOL, gate);
+    if (!c.fuseMolVirial) molecular_virial(c, ACC_MOL, gate);
     BaroArgs ba;
     ba.P0 = B.P0;
     ba.period = B.period;
@@ -207,6 +222,13 @@ void baro_step(Ctx &c, const int *gate) {
     ba.q2 = c.pppm.totalQ2;
     ba.rcut = c.pppm.rcut;
     for (int a = 0; a < 3; ++a) ba.minL[a] = c.nc[a] * (c.rlist / c.reach[a] * (1.0 + 1e-9));
+    ba.rescaled = c.fuseMolVirial && c.thermoOn ? 1 : 0;
+    ba.tmode = c.thermoTau > 0 ? 1 : 0;
+    ba.tgate = thermo_gate(c);
+    ba.thermo = c.thermo;
+    ba.tconv = c.tconv;
+    ba.T0 = c.thermoT;
+    ba.coupling = c.thermoTau > 0 ? c.thermoEvery * c.dt / c.thermoTau : 1.0;
     hipLaunchKernelGGL(k_baro, dim3(1), dim3(64), 0, c.stream, gate, c.box, box_params(c), ba, c.acc);
     dilate(c, gate);
     if (c.pppm.on) pppm_green_device(c, gate, c.stream);

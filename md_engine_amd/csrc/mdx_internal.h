@@ -355,6 +355,7 @@ constexpr int kFlagVirialTrace = 1 << 12;
 struct Ctx {
     int device = 0;
     bool vtrace = false;  // set by compute for the terms it launches
+    bool fuseMolVirial = false;  // captured barostat steps: rigid second half sums ACC_MOL
     hipStream_t stream = nullptr;
     std::string err;
 

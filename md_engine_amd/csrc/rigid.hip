@@ -178,11 +178,19 @@ __global__ void k_rigid_vv1(RigidArgs r, double4 *__restrict__ pos, double4 *__r
 }
 
 // kick = false: only the velocity constraint and the M placement (start of a run)
-__global__ void k_rigid_vv2(RigidArgs r, double4 *__restrict__ pos, double4 *__restrict__ vel,
-                            const unsigned long long *__restrict__ f, int kick) {
+// VIR: also the molecular virial correction and centre-of-mass M|V|^2 of the
+// final velocities (as k_mol_virial) into acc[ACC_MOL], on the steps the gate
+// allows: the barostat's pressure without a pass of its own.
+template <bool VIR>
+__global__ void __launch_bounds__(128) k_rigid_vv2(RigidArgs r, double4 *__restrict__ pos, double4 *__restrict__ vel,
+                                                   const unsigned long long *__restrict__ f, int kick,
+                                                   unsigned long long *__restrict__ acc,
+                                                   const int *__restrict__ vgate) {
     r.g = r.bx->geo;
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= r.nmol) return;
+    const bool vir = VIR && (!vgate || *vgate);  // uniform over the grid
+    double corr = 0.0, mv2 = 0.0;
+    if (m < r.nmol) {
     const int4 a = r.atoms[m];
     const double4 wO = vel[a.x], wH1 = vel[a.y], wH2 = vel[a.z];
     d3 vO = xyz(wO), vH1 = xyz(wH1), vH2 = xyz(wH2);
@@ -204,6 +212,31 @@ __global__ void k_rigid_vv2(RigidArgs r, double4 *__restrict__ pos, double4 *__r
         const double4 pM = pos[a.w];
         pos[a.w] = make_double4(nM.x, nM.y, nM.z, pM.w);
         vel[a.w] = make_double4(0.0, 0.0, 0.0, vel[a.w].w);
+    }
+    if (vir) {
+        const double mT = r.mO + 2.0 * r.mH;
+        const d3 h1 = xO + rH1, h2 = xO + rH2;
+        const d3 R = (1.0 / mT) * (r.mO * xO + r.mH * (h1 + h2));
+        corr = dot(force_of(f, a.x), xO - R) + dot(force_of(f, a.y), h1 - R) + dot(force_of(f, a.z), h2 - R);
+        if (a.w >= 0) corr += dot(force_of(f, a.w), xO + mi(xyz(pos[a.w]) - xO, r.g) - R);
+        const d3 V = (1.0 / mT) * (r.mO * vO + r.mH * (vH1 + vH2));
+        mv2 = mT * dot(V, V);
+    }
+    }
+    if (vir) {
+        __shared__ double red[2][2];
+        corr = team_sum_d<64>(corr);
+        mv2 = team_sum_d<64>(mv2);
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) {
+            red[0][w] = corr;
+            red[1][w] = mv2;
+        }
+        __syncthreads();
+        if (threadIdx.x < 2) {
+            const double x = red[threadIdx.x][0] + red[threadIdx.x][1];
+            if (x != 0.0) fx_add(&acc_rep(acc)[ACC_MOL + threadIdx.x], x);
+        }
     }
 }
 
@@ -381,8 +414,12 @@ void rigid_first_half(Ctx &c) {
 
 void rigid_second_half(Ctx &c, bool kick) {
     RigidArgs r = make_args(c);
-    hipLaunchKernelGGL(k_rigid_vv2, dim3(cdiv(r.nmol, 128)), dim3(128), 0, c.stream, r, c.pos[c.cur],
-                       c.vel[c.cur], c.force, kick ? 1 : 0);
+    if (kick && c.fuseMolVirial)
+        hipLaunchKernelGGL(k_rigid_vv2<true>, dim3(cdiv(r.nmol, 128)), dim3(128), 0, c.stream, r, c.pos[c.cur],
+                           c.vel[c.cur], c.force, 1, c.acc, baro_gate(c));
+    else
+        hipLaunchKernelGGL(k_rigid_vv2<false>, dim3(cdiv(r.nmol, 128)), dim3(128), 0, c.stream, r, c.pos[c.cur],
+                           c.vel[c.cur], c.force, kick ? 1 : 0, c.acc, nullptr);
 }
 
 }  // namespace mdx

@@ -104,11 +104,15 @@ def test_barostat_run(orc, gpu_engine_factory):
     assert_same_lists(gc, grows, oc, orows)
 
 
-def test_barostat_graph_matches_eager(orc, gpu_engine_factory):
+@pytest.mark.parametrize("thermo", [None, (350.0, 3, 0.0), (330.0, 1, 40.0)],
+                         ids=["nve", "rescale-every-3", "berendsen-every-1"])
+def test_barostat_graph_matches_eager(orc, gpu_engine_factory, thermo):
     """The barostat runs on the device inside captured steps: box, pressure,
     positions and the re-solved Ewald splitting agree with eager steps, and
-    alpha equals the oracle's secant solve for the final box."""
-    say("Berendsen barostat every step: captured graph vs eager steps")
+    alpha equals the oracle's secant solve for the final box.  Captured steps
+    sum the centre-of-mass kinetic term before a velocity rescale of the same
+    step and scale it by the rescale's lambda^2; the thermostat cases pin that."""
+    say(f"Berendsen barostat every step, thermostat {thermo}: captured graph vs eager steps")
     box = systems.tip4p2005_box(216, seed=35, min_sep=1.8)
     v = start_velocities(orc, box, seed=9)
     mvv = systems.REAL_MVV_TO_ENG
@@ -117,6 +121,8 @@ def test_barostat_graph_matches_eager(orc, gpu_engine_factory):
         e, flags = water_md_engine(gpu_engine_factory, box, v, rc=7.0, skin=1.0, sz=(32, 32, 32), order=5,
                                    e3b=True, dt=1.0)
         e.barostat(True, 1.0, 200.0, 1, 10.0, 1e-4, NKTV2P, mvv)
+        if thermo:
+            e.thermostat(True, thermo[0], thermo[1], thermo[2], systems.REAL_BOLTZ, mvv)
         e.option("graph", graph)
         e.run(40, flags)
         x, vv = e.get_system()
