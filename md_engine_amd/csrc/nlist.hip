@@ -99,7 +99,9 @@ __global__ void k_bin(const int *__restrict__ gate, int n, double4 *__restrict__
 // cumulativeSum round trip, helpers.h:15-24).  Each thread sums a contiguous
 // segment, the segment totals are scanned across the workgroup, and each
 // thread then writes its segment's prefixes.  Also seeds the fill cursors.
-__global__ void __launch_bounds__(1024) k_scan(const int *__restrict__ gate, int ncell, const int *__restrict__ count,
+// The counts are cleared as they are consumed, ready for the next binning
+// (no separate clearing launch in every captured step).
+__global__ void __launch_bounds__(1024) k_scan(const int *__restrict__ gate, int ncell, int *__restrict__ count,
                                                int *__restrict__ start, int *__restrict__ fill) {
     if (gate && *gate == 0) return;
     __shared__ int wsum[16];
@@ -131,6 +133,7 @@ __global__ void __launch_bounds__(1024) k_scan(const int *__restrict__ gate, int
         start[k] = run;
         fill[k] = run;
         run += count[k];
+        count[k] = 0;
     }
     if (threadIdx.x == 1023) start[ncell] = wsum[15];
 }
@@ -237,11 +240,28 @@ __global__ void k_xq_zero(int n, const double4 *__restrict__ pos, const float *_
     force[3 * (size_t)k + 2] = 0ull;
 }
 
+// exclusion row of atom i (id myid) in current indices, tags kept
+__device__ __forceinline__ void excl_row(int i, int myid, const int *__restrict__ exclOff,
+                                         const uint32_t *__restrict__ exclIds, const int *__restrict__ idToIdx,
+                                         int maxExcl, uint32_t *__restrict__ exclIdx) {
+    const int e0 = exclOff[myid], e1 = exclOff[myid + 1];
+    for (int q = 0; q < maxExcl; ++q) {
+        uint32_t v = ~0u;
+        if (q < e1 - e0) {
+            const uint32_t e = exclIds[e0 + q];
+            v = (uint32_t)idToIdx[e & 0x3FFFFFFFu] | (e & 0xC0000000u);
+        }
+        exclIdx[(size_t)i * maxExcl + q] = v;
+    }
+}
+
 __global__ void k_copy_back(const int *__restrict__ gate, int n, const double4 *__restrict__ pos1,
                             const double4 *__restrict__ vel1, const float *__restrict__ q1,
                             const int *__restrict__ t1, const int *__restrict__ id1, double4 *__restrict__ pos0,
                             double4 *__restrict__ vel0, float *__restrict__ q0, int *__restrict__ t0,
-                            int *__restrict__ id0) {
+                            int *__restrict__ id0, const int *__restrict__ exclOff,
+                            const uint32_t *__restrict__ exclIds, const int *__restrict__ idToIdx, int maxExcl,
+                            uint32_t *__restrict__ exclIdx, int *__restrict__ atomMolO) {
     if (gate && *gate == 0) return;
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -249,7 +269,12 @@ __global__ void k_copy_back(const int *__restrict__ gate, int n, const double4 *
     vel0[k] = vel1[k];
     q0[k] = q1[k];
     t0[k] = t1[k];
-    id0[k] = id1[k];
+    const int myid = id1[k];
+    id0[k] = myid;
+    // the exclusion rows (k_excl_idx) and the cleared oxygen map of the new
+    // order in the same pass: one launch fewer in every captured step
+    if (maxExcl > 0) excl_row(k, myid, exclOff, exclIds, idToIdx, maxExcl, exclIdx);
+    if (atomMolO) atomMolO[k] = -1;
 }
 
 // one wave: the maxima of the nb workgroups
@@ -895,16 +920,7 @@ __global__ void k_excl_idx(const int *__restrict__ gate, int n, const int *__res
     if (gate && *gate == 0) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int myid = id[i];
-    const int e0 = exclOff[myid], e1 = exclOff[myid + 1];
-    for (int q = 0; q < maxExcl; ++q) {
-        uint32_t v = ~0u;
-        if (q < e1 - e0) {
-            const uint32_t e = exclIds[e0 + q];
-            v = (uint32_t)idToIdx[e & 0x3FFFFFFFu] | (e & 0xC0000000u);
-        }
-        exclIdx[(size_t)i * maxExcl + q] = v;
-    }
+    excl_row(i, id[i], exclOff, exclIds, idToIdx, maxExcl, exclIdx);
 }
 
 // E3B molecule list from the atom list: for each water, the oxygens among its
@@ -1084,6 +1100,7 @@ void nlist_setup(Ctx &c) {
     }
     int ncell = c.nc[0] * c.nc[1] * c.nc[2];
     if (ncell != c.ncell) {
+        MDX_REQUIRE(!c.capture, 3, "the cell grid cannot change while a step is captured");
         if (c.ncell) {
             dfree(c, c.colXY);
             dfree(c, c.colRank);
@@ -1096,6 +1113,8 @@ void nlist_setup(Ctx &c) {
         c.colXY = dalloc<int2>(c, (size_t)c.nc[0] * c.nc[1]);
         // two copies of the grid: charged atoms, then charge-free ones
         c.cellCount = dalloc<int>(c, 2 * (size_t)ncell);
+        // k_scan leaves the counts at zero for the next binning
+        MDX_HIP(hipMemsetAsync(c.cellCount, 0, sizeof(int) * 2 * (size_t)ncell, c.stream));
         c.cellStart = dalloc<int>(c, 2 * (size_t)ncell + 1);
         c.cellFill = dalloc<int>(c, 2 * (size_t)ncell);
     }
@@ -1167,8 +1186,6 @@ static void run_build(Ctx &c, bool sync) {
     int threads = 256;
     int cur = c.cur, nxt = 1 - c.cur;
     const int nslot = 2 * c.ncell;
-    hipLaunchKernelGGL(k_fill_int, dim3(cdiv(nslot, threads)), dim3(threads), 0, c.stream, rebuild_gate(c), nslot,
-                       c.cellCount, 0);
     hipLaunchKernelGGL(k_bin, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, rebuild_gate(c), c.n, c.pos[cur],
                        c.q[cur], c.box, c.ncell, c.colRank, c.cellOf, c.cellCount);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c.stream, rebuild_gate(c), nslot, c.cellCount, c.cellStart,
@@ -1181,15 +1198,8 @@ static void run_build(Ctx &c, bool sync) {
                        c.pos[cur], c.vel[cur], c.q[cur], c.type[cur], c.id[cur], c.pos[nxt],
                        c.vel[nxt], c.q[nxt], c.type[nxt], c.id[nxt], c.idToIdx, c.posLast, c.xq,
                        c.box);
-    if (c.capture) {
-        // a captured step keeps fixed buffers: the sorted records come back
-        hipLaunchKernelGGL(k_copy_back, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, rebuild_gate(c),
-                           c.n, c.pos[nxt], c.vel[nxt], c.q[nxt], c.type[nxt], c.id[nxt], c.pos[cur],
-                           c.vel[cur], c.q[cur], c.type[cur], c.id[cur]);
-    } else {
-        c.cur = nxt;
-    }
-    if (c.exclOff && c.maxExcl > 0) {
+    const bool excl = c.exclOff && c.maxExcl > 0;
+    if (excl) {
         MDX_REQUIRE(!c.capture || c.exclIdxN == (size_t)c.n * c.maxExcl, 3,
                     "exclusion index rows must exist before a step is captured");
         if (c.exclIdxN != (size_t)c.n * c.maxExcl) {
@@ -1197,13 +1207,25 @@ static void run_build(Ctx &c, bool sync) {
             c.exclIdxN = (size_t)c.n * c.maxExcl;
             c.exclIdx = dalloc<uint32_t>(c, c.exclIdxN);
         }
-        hipLaunchKernelGGL(k_excl_idx, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, rebuild_gate(c), c.n,
-                           c.id[c.cur], c.exclOff, c.exclIds, c.idToIdx, c.maxExcl, c.exclIdx);
+    }
+    if (c.capture) {
+        // a captured step keeps fixed buffers: the sorted records come back,
+        // with the exclusion rows and the cleared E3B oxygen map
+        hipLaunchKernelGGL(k_copy_back, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, rebuild_gate(c),
+                           c.n, c.pos[nxt], c.vel[nxt], c.q[nxt], c.type[nxt], c.id[nxt], c.pos[cur],
+                           c.vel[cur], c.q[cur], c.type[cur], c.id[cur], c.exclOff, c.exclIds, c.idToIdx,
+                           excl ? c.maxExcl : 0, c.exclIdx, c.e3b.on ? c.e3b.atomMolO : nullptr);
+    } else {
+        c.cur = nxt;
+        if (excl)
+            hipLaunchKernelGGL(k_excl_idx, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, nullptr, c.n,
+                               c.id[c.cur], c.exclOff, c.exclIds, c.idToIdx, c.maxExcl, c.exclIdx);
     }
     rigid_map(c);
     flex_map(c);
     if (c.e3b.on) {
-        fill_int(c, c.e3b.atomMolO, c.n, -1, rebuild_gate(c));  // indices were permuted: drop the old map
+        // indices were permuted: drop the old map (captured steps did so in k_copy_back)
+        if (!c.capture) fill_int(c, c.e3b.atomMolO, c.n, -1, nullptr);
         hipLaunchKernelGGL(k_mol_atoms, dim3(cdiv(c.e3b.nmol, threads)), dim3(threads), 0, c.stream, rebuild_gate(c),
                            c.e3b.nmol, c.e3b.molIds, c.idToIdx, c.e3b.molAtoms, c.e3b.atomMolO);
         MDX_REQUIRE(c.rlist >= (double)c.e3b.r2b + c.skin - 1e-9, 1,
