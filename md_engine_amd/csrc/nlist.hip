@@ -261,7 +261,8 @@ __global__ void k_copy_back(const int *__restrict__ gate, int n, const double4 *
                             double4 *__restrict__ vel0, float *__restrict__ q0, int *__restrict__ t0,
                             int *__restrict__ id0, const int *__restrict__ exclOff,
                             const uint32_t *__restrict__ exclIds, const int *__restrict__ idToIdx, int maxExcl,
-                            uint32_t *__restrict__ exclIdx, int *__restrict__ atomMolO) {
+                            uint32_t *__restrict__ exclIdx, int *__restrict__ atomMolO,
+                            int *__restrict__ member) {
     if (gate && *gate == 0) return;
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -275,6 +276,7 @@ __global__ void k_copy_back(const int *__restrict__ gate, int n, const double4 *
     // order in the same pass: one launch fewer in every captured step
     if (maxExcl > 0) excl_row(k, myid, exclOff, exclIds, idToIdx, maxExcl, exclIdx);
     if (atomMolO) atomMolO[k] = -1;
+    if (member) member[k] = 0;
 }
 
 // one wave: the maxima of the nb workgroups
@@ -975,10 +977,23 @@ __global__ void __launch_bounds__(256) k_e3b_list(
     if (lane == 0) waveMax[gtid >> 6] = wmax;
 }
 
+// Captured steps also map the rigid molecules here (k_rigid_map's work, rigid
+// molecules nmolR > 0, member cleared by k_copy_back): one launch for both.
 __global__ void k_mol_atoms(const int *__restrict__ gate, int nmol, const int4 *__restrict__ molIds, const int *__restrict__ idToIdx,
-                            int4 *__restrict__ molAtoms, int *__restrict__ atomMolO) {
+                            int4 *__restrict__ molAtoms, int *__restrict__ atomMolO, int nmolR,
+                            const int4 *__restrict__ rigidIds, int4 *__restrict__ rigidAtoms,
+                            int *__restrict__ member) {
     if (gate && *gate == 0) return;
     int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < nmolR) {
+        const int4 i = rigidIds[m];
+        const int4 a = make_int4(idToIdx[i.x], idToIdx[i.y], idToIdx[i.z], i.w >= 0 ? idToIdx[i.w] : -1);
+        rigidAtoms[m] = a;
+        member[a.x] = 1;
+        member[a.y] = 1;
+        member[a.z] = 1;
+        if (a.w >= 0) member[a.w] = 1;
+    }
     if (m >= nmol) return;
     int4 ids = molIds[m];
     int4 a = make_int4(idToIdx[ids.x], idToIdx[ids.y], idToIdx[ids.z], idToIdx[ids.w]);
@@ -1214,7 +1229,8 @@ static void run_build(Ctx &c, bool sync) {
         hipLaunchKernelGGL(k_copy_back, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, rebuild_gate(c),
                            c.n, c.pos[nxt], c.vel[nxt], c.q[nxt], c.type[nxt], c.id[nxt], c.pos[cur],
                            c.vel[cur], c.q[cur], c.type[cur], c.id[cur], c.exclOff, c.exclIds, c.idToIdx,
-                           excl ? c.maxExcl : 0, c.exclIdx, c.e3b.on ? c.e3b.atomMolO : nullptr);
+                           excl ? c.maxExcl : 0, c.exclIdx, c.e3b.on ? c.e3b.atomMolO : nullptr,
+                           c.rigid.on ? c.rigid.member : nullptr);
     } else {
         c.cur = nxt;
         if (excl)
@@ -1226,8 +1242,10 @@ static void run_build(Ctx &c, bool sync) {
     if (c.e3b.on) {
         // indices were permuted: drop the old map (captured steps did so in k_copy_back)
         if (!c.capture) fill_int(c, c.e3b.atomMolO, c.n, -1, nullptr);
-        hipLaunchKernelGGL(k_mol_atoms, dim3(cdiv(c.e3b.nmol, threads)), dim3(threads), 0, c.stream, rebuild_gate(c),
-                           c.e3b.nmol, c.e3b.molIds, c.idToIdx, c.e3b.molAtoms, c.e3b.atomMolO);
+        const int nmolR = c.capture && c.rigid.on ? c.rigid.nmol : 0;  // see rigid_map
+        hipLaunchKernelGGL(k_mol_atoms, dim3(cdiv(std::max(c.e3b.nmol, nmolR), threads)), dim3(threads), 0, c.stream,
+                           rebuild_gate(c), c.e3b.nmol, c.e3b.molIds, c.idToIdx, c.e3b.molAtoms, c.e3b.atomMolO,
+                           nmolR, c.rigid.ids, c.rigid.atoms, c.rigid.member);
         MDX_REQUIRE(c.rlist >= (double)c.e3b.r2b + c.skin - 1e-9, 1,
                     "E3B needs rCut + padding >= 6.2 A + padding (the molecule list is taken from the "
                     "pair list)");

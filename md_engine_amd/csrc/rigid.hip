@@ -401,7 +401,10 @@ void dilate(Ctx &c, const int *gate) {
 
 void rigid_map(Ctx &c) {
     if (!c.rigid.on) return;
-    fill_int(c, c.rigid.member, c.n, 0, rebuild_gate(c));
+    // captured list rebuilds clear member in k_copy_back and, with E3B on,
+    // map the molecules in the E3B map kernel (fewer gated launches per step)
+    if (c.capture && c.e3b.on) return;
+    if (!c.capture) fill_int(c, c.rigid.member, c.n, 0, nullptr);
     hipLaunchKernelGGL(k_rigid_map, dim3(cdiv(c.rigid.nmol, 128)), dim3(128), 0, c.stream, rebuild_gate(c), c.rigid.nmol,
                        c.rigid.ids, c.idToIdx, c.rigid.atoms, c.rigid.member);
 }
