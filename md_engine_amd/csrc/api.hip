@@ -282,18 +282,22 @@ constexpr int kGraphSteps = 8;
 constexpr int kGraphChunk = 64;
 
 void captured_step(Ctx &c, int fflags) {
+    // all-water systems without a Nose-Hoover chain: the rigid second half
+    // sums the molecular virial terms and the thermostat's m v^2 (no separate
+    // passes over the molecules; k_decide clears the m v^2 sum)
+    const bool allRigid = c.rigid.on && c.rigid.nsites == c.n && !c.nh.on;
+    c.fuseMolVirial = c.baro.on && allRigid;
+    c.fuseThermo = c.thermoOn && allRigid;
     integrate_first_half(c);
     decide_step(c);
     nlist_rebuild_async(c);  // every kernel gated by dFlags[3]
     // a barostat needs the virial; its steps are gated by dFlags[8]
     compute(c, fflags | (c.baro.on ? kFlagVirialTrace : 0));
-    // all-water systems without a Nose-Hoover chain: the rigid second half
-    // sums the molecular virial terms (no separate pass over the molecules)
-    c.fuseMolVirial = c.baro.on && c.rigid.on && c.rigid.nsites == c.n && !c.nh.on;
     integrate_second_half(c);
     if (c.thermoOn) thermostat_apply(c);  // gated by dFlags[5]
     if (c.baro.on) baro_step(c, baro_gate(c));
     c.fuseMolVirial = false;
+    c.fuseThermo = false;
 }
 
 // One step of a ring polymer whose beads share one stream: every bead's half

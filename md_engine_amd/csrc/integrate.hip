@@ -220,7 +220,7 @@ void fill_float(Ctx &c, float *p, int count, float v) {
 void thermo_measure(Ctx &c) { measure_mv2(c, thermo_gate(c)); }
 
 void thermostat_apply(Ctx &c) {
-    thermo_measure(c);
+    if (!c.fuseThermo) thermo_measure(c);  // else summed by the rigid second half
     const double coupling = c.thermoTau > 0 ? c.thermoEvery * c.dt / c.thermoTau : 1.0;
     hipLaunchKernelGGL(k_vscale, dim3(cdiv(c.n, 256)), dim3(256), 0, c.stream, thermo_gate(c), c.n, c.vel[c.cur], c.thermo,
                        c.tconv, c.thermoT, c.thermoTau > 0 ? 1 : 0, coupling);

@@ -356,6 +356,7 @@ struct Ctx {
     int device = 0;
     bool vtrace = false;  // set by compute for the terms it launches
     bool fuseMolVirial = false;  // captured barostat steps: rigid second half sums ACC_MOL
+    bool fuseThermo = false;     // captured thermostat steps: rigid second half sums m v^2
     hipStream_t stream = nullptr;
     std::string err;
 

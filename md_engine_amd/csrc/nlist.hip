@@ -300,7 +300,8 @@ __device__ __forceinline__ void part_max(int nb, const float *__restrict__ part,
 // thermostat gate for the steps where the turn reaches a multiple of its
 // interval.  Clears the displacement slot for the next step.
 __global__ void k_decide(int nb, const float *__restrict__ part, unsigned *__restrict__ dmax,
-                         int *__restrict__ flags, float half2, int every, int baroEvery) {
+                         int *__restrict__ flags, float half2, int every, int baroEvery,
+                         unsigned long long *__restrict__ thermo) {
     float d2, p2;
     part_max(nb, part, d2, p2);
     if (threadIdx.x != 0) return;
@@ -310,6 +311,11 @@ __global__ void k_decide(int nb, const float *__restrict__ part, unsigned *__res
     const int turn = flags[6] + 1;
     flags[6] = turn;
     flags[5] = (every > 0 && turn % every == 0) ? 1 : 0;
+    // thermo: the rigid second half of this step sums m v^2 into it
+    if (thermo && flags[5]) {
+        thermo[0] = 0ull;
+        thermo[1] = 0ull;
+    }
     flags[8] = (baroEvery > 0 && turn % baroEvery == 0) ? 1 : 0;
     dmax[0] = 0u;
     dmax[1] = 0u;
@@ -1414,7 +1420,8 @@ void decide_step(Ctx &c) {
     check_part(c);
     const float half = (float)(0.5 * c.skin);
     hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, c.stream, check_blocks(c), c.checkPart, c.dMax, c.dFlags,
-                       half * half, c.thermoOn ? c.thermoEvery : 0, c.baro.on ? c.baro.every : 0);
+                       half * half, c.thermoOn ? c.thermoEvery : 0, c.baro.on ? c.baro.every : 0,
+                       c.fuseThermo ? c.thermo : nullptr);
 }
 
 void check_max(Ctx &c, int parity) {

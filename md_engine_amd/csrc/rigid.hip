@@ -178,18 +178,24 @@ __global__ void k_rigid_vv1(RigidArgs r, double4 *__restrict__ pos, double4 *__r
 }
 
 // kick = false: only the velocity constraint and the M placement (start of a run)
-// VIR: also the molecular virial correction and centre-of-mass M|V|^2 of the
-// final velocities (as k_mol_virial) into acc[ACC_MOL], on the steps the gate
-// allows: the barostat's pressure without a pass of its own.
-template <bool VIR>
+// SUMS: on the steps their gates allow, also (doVir) the molecular virial
+// correction and centre-of-mass M|V|^2 of the final velocities (as
+// k_mol_virial) into acc[ACC_MOL], the barostat's pressure without a pass of
+// its own, and (doTh) the sum of m v^2 over the sites (as k_mv2) into
+// thermo[0] for the velocity rescale.
+template <bool SUMS>
 __global__ void __launch_bounds__(128) k_rigid_vv2(RigidArgs r, double4 *__restrict__ pos, double4 *__restrict__ vel,
                                                    const unsigned long long *__restrict__ f, int kick,
                                                    unsigned long long *__restrict__ acc,
-                                                   const int *__restrict__ vgate) {
+                                                   const int *__restrict__ vgate, int doVir,
+                                                   unsigned long long *__restrict__ thermo,
+                                                   const int *__restrict__ tgate, int doTh) {
     r.g = r.bx->geo;
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool vir = VIR && (!vgate || *vgate);  // uniform over the grid
-    double corr = 0.0, mv2 = 0.0;
+    // uniform over the grid
+    const bool vir = SUMS && doVir && (!vgate || *vgate);
+    const bool th = SUMS && doTh && (!tgate || *tgate);
+    double corr = 0.0, mv2 = 0.0, ke = 0.0;
     if (m < r.nmol) {
     const int4 a = r.atoms[m];
     const double4 wO = vel[a.x], wH1 = vel[a.y], wH2 = vel[a.z];
@@ -222,21 +228,33 @@ __global__ void __launch_bounds__(128) k_rigid_vv2(RigidArgs r, double4 *__restr
         const d3 V = (1.0 / mT) * (r.mO * vO + r.mH * (vH1 + vH2));
         mv2 = mT * dot(V, V);
     }
+    if (th) {
+        if (wO.w > 0.0) ke += dot(vO, vO) / wO.w;
+        if (wH1.w > 0.0) ke += dot(vH1, vH1) / wH1.w;
+        if (wH2.w > 0.0) ke += dot(vH2, vH2) / wH2.w;
+        if (a.w >= 0) {
+            const double4 v = vel[a.w];
+            if (v.w > 0.0) ke += (v.x * v.x + v.y * v.y + v.z * v.z) / v.w;
+        }
     }
-    if (vir) {
-        __shared__ double red[2][2];
+    }
+    if (vir || th) {
+        __shared__ double red[3][2];
         corr = team_sum_d<64>(corr);
         mv2 = team_sum_d<64>(mv2);
+        ke = team_sum_d<64>(ke);
         const int w = threadIdx.x >> 6;
         if ((threadIdx.x & 63) == 0) {
             red[0][w] = corr;
             red[1][w] = mv2;
+            red[2][w] = ke;
         }
         __syncthreads();
-        if (threadIdx.x < 2) {
+        if (vir && threadIdx.x < 2) {
             const double x = red[threadIdx.x][0] + red[threadIdx.x][1];
             if (x != 0.0) fx_add(&acc_rep(acc)[ACC_MOL + threadIdx.x], x);
         }
+        if (th && threadIdx.x == 2) fx_add(&thermo[0], red[2][0] + red[2][1]);
     }
 }
 
@@ -417,12 +435,13 @@ void rigid_first_half(Ctx &c) {
 
 void rigid_second_half(Ctx &c, bool kick) {
     RigidArgs r = make_args(c);
-    if (kick && c.fuseMolVirial)
+    if (kick && (c.fuseMolVirial || c.fuseThermo))
         hipLaunchKernelGGL(k_rigid_vv2<true>, dim3(cdiv(r.nmol, 128)), dim3(128), 0, c.stream, r, c.pos[c.cur],
-                           c.vel[c.cur], c.force, 1, c.acc, baro_gate(c));
+                           c.vel[c.cur], c.force, 1, c.acc, baro_gate(c), c.fuseMolVirial ? 1 : 0, c.thermo,
+                           thermo_gate(c), c.fuseThermo ? 1 : 0);
     else
         hipLaunchKernelGGL(k_rigid_vv2<false>, dim3(cdiv(r.nmol, 128)), dim3(128), 0, c.stream, r, c.pos[c.cur],
-                           c.vel[c.cur], c.force, kick ? 1 : 0, c.acc, nullptr);
+                           c.vel[c.cur], c.force, kick ? 1 : 0, c.acc, nullptr, 0, nullptr, nullptr, 0);
 }
 
 }  // namespace mdx
