@@ -288,6 +288,7 @@ void captured_step(Ctx &c, int fflags) {
     const bool allRigid = c.rigid.on && c.rigid.nsites == c.n && !c.nh.on;
     c.fuseMolVirial = c.baro.on && allRigid;
     c.fuseThermo = c.thermoOn && allRigid;
+    c.fuseCheck = c.rigid.on && c.rigid.nsites == c.n;
     integrate_first_half(c);
     decide_step(c);
     nlist_rebuild_async(c);  // every kernel gated by dFlags[3]
@@ -298,6 +299,7 @@ void captured_step(Ctx &c, int fflags) {
     if (c.baro.on) baro_step(c, baro_gate(c));
     c.fuseMolVirial = false;
     c.fuseThermo = false;
+    c.fuseCheck = false;
 }
 
 // One step of a ring polymer whose beads share one stream: every bead's half

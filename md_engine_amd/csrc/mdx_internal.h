@@ -357,6 +357,7 @@ struct Ctx {
     bool vtrace = false;  // set by compute for the terms it launches
     bool fuseMolVirial = false;  // captured barostat steps: rigid second half sums ACC_MOL
     bool fuseThermo = false;     // captured thermostat steps: rigid second half sums m v^2
+    bool fuseCheck = false;      // captured steps: rigid first half gives the skin check's maxima
     hipStream_t stream = nullptr;
     std::string err;
 
@@ -648,6 +649,7 @@ void flex_place_m(Ctx &c);
 void flex_forces(Ctx &c, bool energy, bool virial);
 void flex_spread_m(Ctx &c);
 void rigid_second_half(Ctx &c, bool kick);
+int fused_check_blocks(const Ctx &c);  // workgroups of k_rigid_vv1 when it can give the skin check, else 0
 
 // integrate.hip
 void integrate_first_half(Ctx &c);

@@ -1417,9 +1417,12 @@ static void check_part(Ctx &c) {
 }
 
 void decide_step(Ctx &c) {
-    check_part(c);
+    // all-rigid captured steps: the rigid first half wrote the partial maxima
+    const int fused = c.fuseCheck ? fused_check_blocks(c) : 0;
+    if (!fused) check_part(c);
     const float half = (float)(0.5 * c.skin);
-    hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, c.stream, check_blocks(c), c.checkPart, c.dMax, c.dFlags,
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, c.stream, fused ? fused : check_blocks(c), c.checkPart,
+                       c.dMax, c.dFlags,
                        half * half, c.thermoOn ? c.thermoEvery : 0, c.baro.on ? c.baro.every : 0,
                        c.fuseThermo ? c.thermo : nullptr);
 }

@@ -139,11 +139,31 @@ __device__ __forceinline__ void dist_forces(const unsigned long long *f, int4 a,
     }
 }
 
-__global__ void k_rigid_vv1(RigidArgs r, double4 *__restrict__ pos, double4 *__restrict__ vel,
-                            const unsigned long long *__restrict__ f) {
+// squared minimum-image displacement, the same expressions as the list's
+// skin check (dist2 of the neighbor-list code), so the maxima agree bit for bit
+__device__ __forceinline__ double disp2(const double4 &a, const double4 &b, const Geometry &g) {
+    double d[3] = {b.x - a.x, b.y - a.y, b.z - a.z};
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        if (g.periodic[k]) {
+            double img = rint(d[k] * g.invL[k]);
+            d[k] = d[k] - img * g.L[k];
+        }
+    return d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+}
+
+// part != nullptr: also the skin check's per-workgroup maxima (k_check_part)
+// of the sites' displacement since the last build and of the displacement
+// plus 2 dt |v|, from the positions and velocities just written; valid when
+// every site belongs to a rigid molecule.
+__global__ void __launch_bounds__(128) k_rigid_vv1(RigidArgs r, double4 *__restrict__ pos, double4 *__restrict__ vel,
+                                                   const unsigned long long *__restrict__ f,
+                                                   const double4 *__restrict__ last, double dt2,
+                                                   float *__restrict__ part) {
     r.g = r.bx->geo;
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= r.nmol) return;
+    float d2 = 0.f, p2 = 0.f;
+    if (m < r.nmol) {
     const int4 a = r.atoms[m];
     d3 fO, fH1, fH2;
     dist_forces(f, a, r.gamma, fO, fH1, fH2);
@@ -169,11 +189,51 @@ __global__ void k_rigid_vv1(RigidArgs r, double4 *__restrict__ pos, double4 *__r
     vel[a.x] = make_double4(vO.x, vO.y, vO.z, wO.w);
     vel[a.y] = make_double4(vH1.x, vH1.y, vH1.z, wH1.w);
     vel[a.z] = make_double4(vH2.x, vH2.y, vH2.z, wH2.w);
+    double4 sp[4], sv[4];
+    sp[0] = make_double4(nO.x, nO.y, nO.z, pO.w);
+    sp[1] = make_double4(nH1.x, nH1.y, nH1.z, pH1.w);
+    sp[2] = make_double4(nH2.x, nH2.y, nH2.z, pH2.w);
+    sv[0] = make_double4(vO.x, vO.y, vO.z, wO.w);
+    sv[1] = make_double4(vH1.x, vH1.y, vH1.z, wH1.w);
+    sv[2] = make_double4(vH2.x, vH2.y, vH2.z, wH2.w);
     if (a.w >= 0) {
         const d3 nM = nO + (0.5 * (1.0 - r.gamma)) * ((B3 - A3) + (C3 - A3));
         const double4 pM = pos[a.w];
-        pos[a.w] = make_double4(nM.x, nM.y, nM.z, pM.w);
-        vel[a.w] = make_double4(0.0, 0.0, 0.0, vel[a.w].w);
+        sp[3] = make_double4(nM.x, nM.y, nM.z, pM.w);
+        sv[3] = make_double4(0.0, 0.0, 0.0, vel[a.w].w);
+        pos[a.w] = sp[3];
+        vel[a.w] = sv[3];
+    }
+    if (part) {
+        const int site[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            if (site[s] < 0) continue;
+            const double dd = disp2(sp[s], last[site[s]], r.g);
+            const double4 v = sv[s];
+            const double p = sqrt(dd) + dt2 * sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+            d2 = fmaxf(d2, (float)dd);
+            p2 = fmaxf(p2, (float)(p * p));
+        }
+    }
+    }
+    if (part) {
+        __shared__ float red[2][2];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            d2 = fmaxf(d2, __shfl_xor(d2, o, 64));
+            p2 = fmaxf(p2, __shfl_xor(p2, o, 64));
+        }
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) {
+            red[0][w] = d2;
+            red[1][w] = p2;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            part[2 * blockIdx.x] = fmaxf(red[0][0], red[0][1]);
+            part[2 * blockIdx.x + 1] = fmaxf(red[1][0], red[1][1]);
+        }
     }
 }
 
@@ -427,10 +487,17 @@ void rigid_map(Ctx &c) {
                        c.rigid.ids, c.idToIdx, c.rigid.atoms, c.rigid.member);
 }
 
+int fused_check_blocks(const Ctx &c) {
+    const int nb = cdiv(c.rigid.nmol, 128);
+    return c.rigid.on && c.rigid.nsites == c.n && nb <= kCheckBlocks ? nb : 0;
+}
+
 void rigid_first_half(Ctx &c) {
     RigidArgs r = make_args(c);
+    // captured steps of all-rigid systems: the skin check's partial maxima here
+    const bool chk = c.fuseCheck && fused_check_blocks(c) > 0;
     hipLaunchKernelGGL(k_rigid_vv1, dim3(cdiv(r.nmol, 128)), dim3(128), 0, c.stream, r, c.pos[c.cur],
-                       c.vel[c.cur], c.force);
+                       c.vel[c.cur], c.force, c.posLast, 2.0 * c.dt, chk ? c.checkPart : nullptr);
 }
 
 void rigid_second_half(Ctx &c, bool kick) {
