@@ -230,8 +230,9 @@ void baro_step(Ctx &c, const int *gate) {
     ba.T0 = c.thermoT;
     ba.coupling = c.thermoTau > 0 ? c.thermoEvery * c.dt / c.thermoTau : 1.0;
     hipLaunchKernelGGL(k_baro, dim3(1), dim3(64), 0, c.stream, gate, c.box, box_params(c), ba, c.acc);
-    dilate(c, gate);
-    if (c.pppm.on) pppm_green_device(c, gate, c.stream);
+    // the dilation rides on the Green function's launch where it can
+    const DilateArgs dl = dilate_args(c);
+    if (!(c.pppm.on && pppm_green_device(c, gate, c.stream, &dl))) dilate(c, gate);
 }
 
 }  // namespace mdx

@@ -622,6 +622,60 @@ void rigid_map(Ctx &c);
 void molecular_virial(Ctx &c, int slot, const int *gate);
 // isotropic scaling by the device box's mu about its centre (molecules rigid)
 void dilate(Ctx &c, const int *gate);
+
+// Isotropic dilation about the box centre by bx->mu (the barostat kernel set
+// the new box and mu): thread t < nmol moves rigid molecule t with its centre
+// of mass (geometry kept), thread t < n scales site t unless it belongs to a
+// rigid molecule (Mod::scaleSystem, src/Mod.cu:150-240).  The intramolecular
+// minimum images (about 1 A) are the same in the old and the new box.
+struct DilateArgs {
+    int nmol, n;
+    const int4 *atoms;
+    const int *member;
+    double mO, mH;
+    double4 *pos;
+};
+DilateArgs dilate_args(const Ctx &c);
+__device__ inline double dil_mi(double d, const Geometry &g, int k) {
+    return g.periodic[k] ? d - g.L[k] * rint(d * g.invL[k]) : d;
+}
+__device__ inline void dilate_thread(int t, const DilateArgs &d, const BoxDev *bx) {
+    const Geometry &g = bx->geo;
+    const double mu = bx->mu;
+    double c[3];
+    for (int k = 0; k < 3; ++k) c[k] = g.lo[k] + 0.5 * g.L[k];
+    if (t < d.nmol) {
+        const int4 a = d.atoms[t];
+        const double mT = d.mO + 2.0 * d.mH;
+        const double4 o = d.pos[a.x], p1 = d.pos[a.y], p2 = d.pos[a.z];
+        const double oc[3] = {o.x, o.y, o.z};
+        const double h1[3] = {o.x + dil_mi(p1.x - o.x, g, 0), o.y + dil_mi(p1.y - o.y, g, 1),
+                              o.z + dil_mi(p1.z - o.z, g, 2)};
+        const double h2[3] = {o.x + dil_mi(p2.x - o.x, g, 0), o.y + dil_mi(p2.y - o.y, g, 1),
+                              o.z + dil_mi(p2.z - o.z, g, 2)};
+        double sh[3];
+        for (int k = 0; k < 3; ++k) {
+            const double R = (1.0 / mT) * (d.mO * oc[k] + d.mH * (h1[k] + h2[k]));
+            sh[k] = (c[k] + mu * (R - c[k])) - R;
+        }
+        const int site[4] = {a.x, a.y, a.z, a.w};
+        for (int s = 0; s < 4; ++s) {
+            if (site[s] < 0) continue;
+            double4 p = d.pos[site[s]];
+            p.x += sh[0];
+            p.y += sh[1];
+            p.z += sh[2];
+            d.pos[site[s]] = p;
+        }
+    }
+    if (t < d.n && !(d.member && d.member[t])) {
+        double4 p = d.pos[t];
+        p.x = c[0] + mu * (p.x - c[0]);
+        p.y = c[1] + mu * (p.y - c[1]);
+        p.z = c[2] + mu * (p.z - c[2]);
+        d.pos[t] = p;
+    }
+}
 // device box: upload the host state, read it back, and the barostat step on
 // the device (pressure, dilation, new box, alpha and Green function)
 BoxParams box_params(const Ctx &c);
@@ -629,7 +683,9 @@ void box_upload(Ctx &c);
 void box_pull(Ctx &c);
 void baro_step(Ctx &c, const int *gate);
 // Green function from the device box on stream s (gate: nullptr = always)
-void pppm_green_device(Ctx &c, const int *gate, hipStream_t s);
+// dl != nullptr: the same launch also dilates (extra workgroups after the
+// Green function's); returns whether it did
+bool pppm_green_device(Ctx &c, const int *gate, hipStream_t s, const DilateArgs *dl = nullptr);
 void rigid_first_half(Ctx &c);
 // ring.hip: ring-polymer beads, one context per bead, coupled by the
 // normal-mode propagation of the free ring polymer

@@ -334,10 +334,17 @@ __device__ __forceinline__ float sinc_f(float x) {
 // entries); each point then sums its (2 lim + 1)^3 images from products of
 // table entries.  lim > kGreenTabLim takes the direct FP64 sums.
 constexpr int kGreenTabLim = 2;
+// Workgroups from gblocks on dilate the system instead (dilate_thread): the
+// barostat's two box-change passes in one launch; both only read the box.
 __global__ void __launch_bounds__(256) k_green(const int *__restrict__ gate, const BoxDev *__restrict__ bx,
-                                               int szh, int order, float *__restrict__ G) {
+                                               int szh, int order, float *__restrict__ G, int gblocks,
+                                               DilateArgs dl) {
     extern __shared__ float gtab[];
     if (gate && *gate == 0) return;
+    if ((int)blockIdx.x >= gblocks) {  // uniform over the workgroup
+        dilate_thread(((int)blockIdx.x - gblocks) * blockDim.x + threadIdx.x, dl, bx);
+        return;
+    }
     const MeshGeo m = bx->mesh;
     const double alpha = bx->alpha;
     const int lim = bx->greenLim;
@@ -611,18 +618,23 @@ void pppm_prepare(Ctx &c) {
     for (int a = 0; a < 3; ++a) P.boxAtGreen[a] = c.geo.L[a];
 }
 
-void pppm_green_device(Ctx &c, const int *gate, hipStream_t s) {
+bool pppm_green_device(Ctx &c, const int *gate, hipStream_t s, const DilateArgs *dl) {
     const PppmState &P = c.pppm;
     const int szh = P.sz[2] / 2 + 1;
     const size_t nh = (size_t)P.sz[0] * P.sz[1] * szh;
     // table of 3 (2 kGreenTabLim + 1) (sz_x + sz_y + sz_z) floats; meshes too
     // large for it take the direct kernel
     const size_t tab = sizeof(float) * 3 * (2 * kGreenTabLim + 1) * (P.sz[0] + P.sz[1] + P.sz[2]);
-    if (tab <= 64 * 1024)
-        hipLaunchKernelGGL(k_green, dim3(cdiv(nh, 256)), dim3(256), tab, s, gate, c.box, szh, P.order, P.green);
-    else
-        hipLaunchKernelGGL(k_green_direct, dim3(cdiv(nh, 256)), dim3(256), 0, s, gate, c.box, szh, P.order,
-                           P.green);
+    const int gb = cdiv(nh, 256);
+    if (tab <= 64 * 1024) {
+        DilateArgs none{};
+        const int db = dl ? cdiv(std::max(dl->nmol, dl->n), 256) : 0;
+        hipLaunchKernelGGL(k_green, dim3(gb + db), dim3(256), tab, s, gate, c.box, szh, P.order, P.green, gb,
+                           dl ? *dl : none);
+        return dl != nullptr;
+    }
+    hipLaunchKernelGGL(k_green_direct, dim3(gb), dim3(256), 0, s, gate, c.box, szh, P.order, P.green);
+    return false;
 }
 
 void pppm_forces(Ctx &c, bool energy, bool virial) {

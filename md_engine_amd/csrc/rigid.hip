@@ -402,51 +402,10 @@ __global__ void __launch_bounds__(256) k_mol_virial(RigidArgs r, int n, const in
     }
 }
 
-// isotropic dilation about the box centre c: rigid molecules move with their
-// centre of mass (geometry kept), other sites are scaled (Mod::scaleSystem,
-// src/Mod.cu:150-240)
-// mu and the centre c come from the device box (the barostat kernel set them)
-__device__ __forceinline__ d3 box_centre(const BoxDev *bx) {
-    return {bx->geo.lo[0] + 0.5 * bx->geo.L[0], bx->geo.lo[1] + 0.5 * bx->geo.L[1],
-            bx->geo.lo[2] + 0.5 * bx->geo.L[2]};
-}
-
-// one thread per rigid molecule (t < nmol) and per free site (t < n); the two
-// parts write disjoint sites.  k_baro already holds the new box; the
-// intramolecular minimum images (about 1 A) are the same in either box.
-__global__ void __launch_bounds__(256) k_dilate(RigidArgs r, int n, const int *__restrict__ member,
-                                                double4 *__restrict__ pos, const int *__restrict__ gate) {
+__global__ void __launch_bounds__(256) k_dilate(DilateArgs d, const BoxDev *__restrict__ bx,
+                                                const int *__restrict__ gate) {
     if (gate && *gate == 0) return;
-    r.g = r.bx->geo;
-    const double mu = r.bx->mu;
-    const d3 c = box_centre(r.bx);
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < r.nmol) {
-        const int4 a = r.atoms[t];
-        const double mT = r.mO + 2.0 * r.mH;
-        const d3 o = xyz(pos[a.x]);
-        const d3 h1 = o + mi(xyz(pos[a.y]) - o, r.g);
-        const d3 h2 = o + mi(xyz(pos[a.z]) - o, r.g);
-        const d3 R = (1.0 / mT) * (r.mO * o + r.mH * (h1 + h2));
-        const d3 shift = (c + mu * (R - c)) - R;
-        const int site[4] = {a.x, a.y, a.z, a.w};
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            if (site[s] < 0) continue;
-            double4 p = pos[site[s]];
-            p.x += shift.x;
-            p.y += shift.y;
-            p.z += shift.z;
-            pos[site[s]] = p;
-        }
-    }
-    if (t < n && !(member && member[t])) {
-        double4 p = pos[t];
-        p.x = c.x + mu * (p.x - c.x);
-        p.y = c.y + mu * (p.y - c.y);
-        p.z = c.z + mu * (p.z - c.z);
-        pos[t] = p;
-    }
+    dilate_thread(blockIdx.x * blockDim.x + threadIdx.x, d, bx);
 }
 
 }  // namespace
@@ -464,17 +423,21 @@ void molecular_virial(Ctx &c, int slot, const int *gate) {
                        c.pos[c.cur], c.vel[c.cur], c.force, c.acc, slot, gate);
 }
 
+DilateArgs dilate_args(const Ctx &c) {
+    DilateArgs d;
+    d.nmol = c.rigid.on ? c.rigid.nmol : 0;
+    d.n = c.n;
+    d.atoms = c.rigid.on ? c.rigid.atoms : nullptr;
+    d.member = c.rigid.on ? c.rigid.member : nullptr;
+    d.mO = c.rigid.mO;
+    d.mH = c.rigid.mH;
+    d.pos = c.pos[c.cur];
+    return d;
+}
+
 void dilate(Ctx &c, const int *gate) {
-    RigidArgs r = make_args(c);
-    const int *member = nullptr;
-    if (c.rigid.on) {
-        member = c.rigid.member;
-    } else {
-        r.nmol = 0;
-        r.atoms = nullptr;
-    }
-    hipLaunchKernelGGL(k_dilate, dim3(cdiv(std::max(r.nmol, c.n), 256)), dim3(256), 0, c.stream, r, c.n, member,
-                       c.pos[c.cur], gate);
+    const DilateArgs d = dilate_args(c);
+    hipLaunchKernelGGL(k_dilate, dim3(cdiv(std::max(d.nmol, d.n), 256)), dim3(256), 0, c.stream, d, c.box, gate);
 }
 
 void rigid_map(Ctx &c) {
