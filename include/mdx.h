@@ -135,6 +135,18 @@ int mdx_temperature(mdx_ctx *ctx, double boltz, double mvv_to_eng, double *tempe
 /* Zero and accumulate the requested force terms at the current positions
  * (IntegratorUtil::force, src/Integrators/IntegratorUtil.cpp:15-30). */
 int mdx_forces_compute(mdx_ctx *ctx, int flags);
+/* Same as mdx_forces_compute, without allocating or freeing device memory:
+ * no counterpart in the reference; for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops).  The P3M work arrays (charge
+ * mesh, spectra, field) come from a library-wide device scratch that holds
+ * meshes of up to 64^3 points (about 14 MiB per GPU, kept until the process
+ * ends); larger meshes are rejected with MDX_ERR_ARG.  The context must be
+ * set up by one mdx_forces_compute call first (list rows, P3M plans); a call
+ * that would need new memory (first build, rows that overflow, a regridded
+ * cell list) fails with MDX_ERR_STATE instead.  Results equal
+ * mdx_forces_compute's.  Every call on a GPU shares its scratch: calls that
+ * overlap on one GPU (contexts on different host threads) race. */
+int mdx_forces_compute_noalloc(mdx_ctx *ctx, int flags);
 /* Forces by id, n x 3 FP64 (converted from the 64-bit fixed-point buffer). */
 int mdx_forces_get(mdx_ctx *ctx, double *f);
 /* Energies of the last compute with MDX_ENERGY: e[0] LJ + real space,
@@ -207,6 +219,13 @@ int mdx_surrogate_get(mdx_ctx *ctx, float *params, int nparams);
 int mdx_surrogate_record(mdx_ctx *ctx, int reset, int capacity);
 int mdx_surrogate_fit(mdx_ctx *ctx, int epochs, int batch, double lr, int seed, int normalize, double *loss);
 int mdx_surrogate_eval(mdx_ctx *ctx, float *emol, float *feat);
+/* Gradient of the training loss (mean squared error of the standardised
+ * per-molecule energy) over one mini-batch of recorded samples, rows
+ * (bstart + s) * mul + add mod nsamp for s < bsize, as mdx_surrogate_fit
+ * computes it before its Adam update: grad[0 .. 7425) for W1 b1 W2 b2 w3 b3
+ * in the packed layout, grad[7425] the batch loss (n >= 7426).  For checking
+ * the hand-written backward pass against an FP64 restatement. */
+int mdx_surrogate_grad(mdx_ctx *ctx, int bstart, int bsize, uint32_t mul, uint32_t add, float *grad, int n);
 /* E3B three-body energy owned by each molecule (donor of type A, centre of
  * type B, acceptor of type C; the per-molecule shares of
  * compute_E3B_energy_threebody, src/Evaluators/ThreeBodyE3B.h:553-918). */
@@ -220,6 +239,11 @@ int mdx_e3b_molecule_energies(mdx_ctx *ctx, float *emol);
 int mdx_nose_hoover_set(mdx_ctx *ctx, int enable, double temperature, double time_constant, int chain_length,
                         double boltz, double mvv_to_eng);
 int mdx_nose_hoover_energy(mdx_ctx *ctx, double *chain_energy);
+/* Raw chain state (n >= 4 chain_length + 2 doubles): chain velocities,
+ * forces, masses, positions, the current sum m v^2 (energy units) and the last
+ * velocity scale factor; the thermVel / thermForce / thermMass vectors and
+ * ke_current of FixNoseHoover (src/Fixes/FixNoseHoover.cu:795-866). */
+int mdx_nose_hoover_state(mdx_ctx *ctx, double *state, int n);
 /* Berendsen barostat (FixPressureBerendsen(state, handle, pressure, period,
  * applyEvery), src/Fixes/FixPressureBerendsen.cu; bulkModulus 10 and
  * maxDilation 1e-5 there): every `every` steps the molecular pressure sets an
@@ -247,7 +271,8 @@ int mdx_sync(mdx_ctx *ctx);
  * (both the per-atom team build); "cell_reach_xy" 1..2 and "cell_reach_z"
  * 1..16: cell edges >= rlist / reach per axis (identical lists, finer cells
  * test fewer candidates); "gather_team" 8 or 16 lanes per charged atom in
- * the P3M field gather. */
+ * the P3M field gather; "row_cap" >= 16 sets the neighbor-row capacity after
+ * the first build (rows still grow on overflow). */
 int mdx_option_set(mdx_ctx *ctx, const char *name, int value);
 
 /* Force-path steps at fixed positions: nsteps evaluations of the terms in

@@ -16,6 +16,9 @@ LIB_PATH = os.path.join(HERE, "libmdx.so")
 
 PAIR, PPPM, E3B, ENERGY, VIRIAL, SURROGATE, BONDED = 1, 2, 4, 8, 16, 32, 64
 
+# trained surrogate parameters (W1 b1 W2 b2 w3 b3 of the 48-64-64-1 network)
+NTRAIN = 48 * 64 + 64 + 64 * 64 + 64 + 64 + 1
+
 # every symbol declared in include/mdx.h
 SYMBOLS = [
     "mdx_version", "mdx_create", "mdx_destroy", "mdx_last_error", "mdx_system_set",
@@ -23,11 +26,11 @@ SYMBOLS = [
     "mdx_exclusions_set", "mdx_exclusions_from_bonds", "mdx_nlist_update", "mdx_nlist_get",
     "mdx_nlist_stats", "mdx_lj_set", "mdx_special_set", "mdx_ewald_real_set", "mdx_pppm_set",
     "mdx_pppm_info", "mdx_e3b_set", "mdx_forces_compute", "mdx_forces_get", "mdx_energy_get",
-    "mdx_integrator_set", "mdx_run", "mdx_teams_set", "mdx_sync", "mdx_force_steps",
+    "mdx_integrator_set", "mdx_run", "mdx_forces_compute_noalloc", "mdx_teams_set", "mdx_sync", "mdx_force_steps",
     "mdx_profile_enable", "mdx_profile_get", "mdx_nlist_totals", "mdx_pair_entries_read", "mdx_option_set", "mdx_rigid_set",
     "mdx_thermostat_set", "mdx_temperature", "mdx_tune", "mdx_run_batch", "mdx_surrogate_set",
-    "mdx_surrogate_get", "mdx_surrogate_record", "mdx_surrogate_fit", "mdx_surrogate_eval",
-    "mdx_e3b_molecule_energies", "mdx_nose_hoover_set", "mdx_nose_hoover_energy", "mdx_barostat_set",
+    "mdx_surrogate_get", "mdx_surrogate_record", "mdx_surrogate_fit", "mdx_surrogate_eval", "mdx_surrogate_grad",
+    "mdx_e3b_molecule_energies", "mdx_nose_hoover_set", "mdx_nose_hoover_energy", "mdx_nose_hoover_state", "mdx_barostat_set",
     "mdx_pressure", "mdx_barostat_last", "mdx_box_get", "mdx_flexible_water_set", "mdx_bonded_energy_get", "mdx_ring_run",
 ]
 
@@ -90,6 +93,7 @@ def load():
     L.mdx_pppm_info.argtypes = [_vp, C.POINTER(d), C.POINTER(d)]
     L.mdx_e3b_set.argtypes = [_vp, c, _ip, _dp, d, d]
     L.mdx_forces_compute.argtypes = [_vp, c]
+    L.mdx_forces_compute_noalloc.argtypes = [_vp, c]
     L.mdx_forces_get.argtypes = [_vp, _dp]
     L.mdx_energy_get.argtypes = [_vp, _dp, _dp]
     L.mdx_integrator_set.argtypes = [_vp, d, d]
@@ -109,9 +113,11 @@ def load():
     L.mdx_surrogate_record.argtypes = [_vp, c, c]
     L.mdx_surrogate_fit.argtypes = [_vp, c, c, d, c, c, C.POINTER(d)]
     L.mdx_surrogate_eval.argtypes = [_vp, _vp, _vp]
+    L.mdx_surrogate_grad.argtypes = [_vp, c, c, C.c_uint32, C.c_uint32, _vp, c]
     L.mdx_e3b_molecule_energies.argtypes = [_vp, _fp]
     L.mdx_nose_hoover_set.argtypes = [_vp, c, d, d, c, d, d]
     L.mdx_nose_hoover_energy.argtypes = [_vp, C.POINTER(d)]
+    L.mdx_nose_hoover_state.argtypes = [_vp, _dp, c]
     L.mdx_barostat_set.argtypes = [_vp, c, d, d, c, d, d, d, d]
     L.mdx_pressure.argtypes = [_vp, c, d, d, C.POINTER(d)]
     L.mdx_barostat_last.argtypes = [_vp, C.POINTER(d)]
@@ -297,6 +303,11 @@ class Engine:
     def compute(self, flags):
         self._chk(self._L.mdx_forces_compute(self._h, int(flags)), "mdx_forces_compute")
 
+    def compute_noalloc(self, flags):
+        """mdx_forces_compute without device allocation (P3M work arrays in
+        the library's scratch; needs one compute() first)."""
+        self._chk(self._L.mdx_forces_compute_noalloc(self._h, int(flags)), "mdx_forces_compute_noalloc")
+
     def forces(self):
         f = np.zeros(self.n * 3)
         self._chk(self._L.mdx_forces_get(self._h, f), "mdx_forces_get")
@@ -327,6 +338,13 @@ class Engine:
         e = C.c_double(0.0)
         self._chk(self._L.mdx_nose_hoover_energy(self._h, C.byref(e)), "mdx_nose_hoover_energy")
         return e.value
+
+    def nose_hoover_state(self, chain_length):
+        """Chain block: velocities, forces, masses, positions (chain_length
+        each), sum m v^2, last scale factor."""
+        st = np.zeros(4 * chain_length + 2)
+        self._chk(self._L.mdx_nose_hoover_state(self._h, st, st.size), "mdx_nose_hoover_state")
+        return st
 
     def barostat(self, enable, pressure=1.0, period=1000.0, every=10, bulk_modulus=10.0, max_dilation=1e-5,
                  nktv_to_press=68568.415, mvv_to_eng=1.0):
@@ -367,6 +385,14 @@ class Engine:
         self._chk(self._L.mdx_surrogate_fit(self._h, int(epochs), int(batch), float(lr), int(seed),
                                             int(bool(normalize)), C.byref(loss)), "mdx_surrogate_fit")
         return loss.value
+
+    def surrogate_grad(self, bstart, bsize, mul=1, add=0):
+        """Summed gradient of the trained parameters over one mini-batch of
+        recorded samples and the batch loss (last entry)."""
+        g = np.zeros(NTRAIN + 1, np.float32)
+        self._chk(self._L.mdx_surrogate_grad(self._h, int(bstart), int(bsize), int(mul), int(add), g.ctypes.data,
+                                             g.size), "mdx_surrogate_grad")
+        return g
 
     def surrogate_eval(self, nmol):
         e = np.zeros(nmol, np.float32)

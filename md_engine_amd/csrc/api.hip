@@ -1,6 +1,7 @@
 // C ABI of libmdx: argument checking, host<->device marshalling by atom id and
 // the step loop.  See include/mdx.h for the contract of each entry point.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -17,6 +18,7 @@ namespace mdx {
 
 void dfree(Ctx &c, void *p) {
     if (!p) return;
+    MDX_REQUIRE(!c.noAlloc, 3, "this call would free device memory");
     auto it = std::find(c.allocs.begin(), c.allocs.end(), p);
     if (it != c.allocs.end()) {
         hipFree(p);
@@ -215,6 +217,20 @@ void snapshot_save(Ctx &c) {
         if (!c.snap.box) c.snap.box = dalloc<BoxDev>(c, 1);
         MDX_HIP(hipMemcpyAsync(c.snap.box, c.box, sizeof(BoxDev), hipMemcpyDeviceToDevice, c.stream));
     }
+    // the thermostat state advances with the particles: a replayed chunk must
+    // start from the chain and the kinetic-energy sums of its first step
+    if (c.nh.on && c.nh.dev) {
+        const int len = 4 * c.nh.chain + 2;
+        if (c.snap.nhLen != len) {
+            if (c.snap.nh) dfree(c, c.snap.nh);
+            c.snap.nh = dalloc<double>(c, len);
+            c.snap.nhLen = len;
+        }
+        MDX_HIP(hipMemcpyAsync(c.snap.nh, c.nh.dev, sizeof(double) * len, hipMemcpyDeviceToDevice, c.stream));
+    }
+    if (!c.snap.thermo) c.snap.thermo = dalloc<unsigned long long>(c, 2);
+    MDX_HIP(hipMemcpyAsync(c.snap.thermo, c.thermo, sizeof(unsigned long long) * 2, hipMemcpyDeviceToDevice,
+                           c.stream));
 }
 
 void snapshot_restore(Ctx &c) {
@@ -225,6 +241,12 @@ void snapshot_restore(Ctx &c) {
     MDX_HIP(hipMemcpyAsync(c.type[k], c.snap.type, sizeof(int) * c.n, hipMemcpyDeviceToDevice, c.stream));
     MDX_HIP(hipMemcpyAsync(c.id[k], c.snap.id, sizeof(int) * c.n, hipMemcpyDeviceToDevice, c.stream));
     MDX_HIP(hipMemcpyAsync(c.idToIdx, c.snap.idToIdx, sizeof(int) * c.n, hipMemcpyDeviceToDevice, c.stream));
+    if (c.nh.on && c.nh.dev && c.snap.nh && c.snap.nhLen == 4 * c.nh.chain + 2)
+        MDX_HIP(hipMemcpyAsync(c.nh.dev, c.snap.nh, sizeof(double) * c.snap.nhLen, hipMemcpyDeviceToDevice,
+                               c.stream));
+    if (c.snap.thermo)
+        MDX_HIP(hipMemcpyAsync(c.thermo, c.snap.thermo, sizeof(unsigned long long) * 2, hipMemcpyDeviceToDevice,
+                               c.stream));
     c.turn = c.snap.turn;
     c.nRebuilds = c.snap.rebuilds;
     c.listValid = false;
@@ -460,52 +482,51 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags, const Rin
             for (int k = 0; k < nbig; ++k) MDX_HIP(hipGraphLaunch(big, cs[0]->stream));
             for (int k = 0; k < nrest; ++k) MDX_HIP(hipGraphLaunch(one, cs[0]->stream));
             MDX_HIP(hipStreamSynchronize(cs[0]->stream));
-            bool regrow = false, regrid = false;
-            for (Ctx *cp : cs) {
-                Ctx &c = *cp;
-                int h[7];
+            // Every replica advanced by len steps in the graph.  First read
+            // what each one found (row overflow, a grid its box outgrew); only
+            // then either replay the chunk for all of them or accept it for
+            // all of them, so no replica is left one chunk ahead.
+            std::vector<std::array<int, 7>> hs(cs.size());
+            std::vector<char> overflow(cs.size(), 0), badGrid(cs.size(), 0);
+            bool regrow = false;
+            for (size_t k = 0; k < cs.size(); ++k) {
+                Ctx &c = *cs[k];
+                int *h = hs[k].data();
                 MDX_HIP(hipMemcpy(h, c.dFlags + 1, sizeof(int) * 7, hipMemcpyDeviceToHost));
                 c.pinned[0] = h[0];
                 c.pinned[1] = h[1];
                 c.pinned[6] = h[6];
                 MDX_REQUIRE(h[6] == 0, 1,
                             "an atom's stencil holds over 2^20 candidates: lower the list cutoff or the cell reach");
-                if (h[0] > c.cap || (c.e3b.on && h[1] > c.e3b.cap)) {
-                    snapshot_restore(c);
-                    grow_if_needed(c);
-                    c.forceFlags = -1;
-                    regrow = true;
-                    continue;
-                }
+                overflow[k] = h[0] > c.cap || (c.e3b.on && h[1] > c.e3b.cap);
                 if (c.baro.on) {
                     box_pull(c);
-                    if (c.boxHost.regrid) {
-                        // a cell edge fell below r_list / reach inside the chunk:
-                        // replay it on a grid set up for the starting box
-                        snapshot_restore(c);
-                        c.nlistConfigured = false;
-                        c.forceFlags = -1;
-                        regrow = true;
-                        continue;
-                    }
+                    // a cell edge fell below r_list / reach inside the chunk:
+                    // replay it on a grid set up for the starting box
+                    badGrid[k] = c.boxHost.regrid != 0;
                 }
-                c.nRebuilds += h[3];
-                c.turn += len;
-                // the box grew by a cell: the next chunk takes a finer grid
-                if (c.baro.on && regrid_if_needed(c)) regrid = true;
+                regrow = regrow || overflow[k] || badGrid[k];
             }
-            if (regrid && !regrow) drop();
             if (regrow) {
-                // rows were reallocated: every replica replays the chunk
-                for (Ctx *cp : cs) {
-                    if (cp->listValid) {
-                        snapshot_restore(*cp);
-                        cp->forceFlags = -1;
-                    }
+                for (size_t k = 0; k < cs.size(); ++k) {
+                    Ctx &c = *cs[k];
+                    snapshot_restore(c);
+                    if (overflow[k]) grow_if_needed(c);
+                    if (badGrid[k]) c.nlistConfigured = false;
+                    c.forceFlags = -1;
                 }
                 drop();
                 continue;
             }
+            bool regrid = false;
+            for (size_t k = 0; k < cs.size(); ++k) {
+                Ctx &c = *cs[k];
+                c.nRebuilds += hs[k][3];
+                c.turn += len;
+                // the box grew by a cell: the next chunk takes a finer grid
+                if (c.baro.on && regrid_if_needed(c)) regrid = true;
+            }
+            if (regrid) drop();
             done += len;
         }
     } catch (...) {
@@ -538,6 +559,7 @@ int mdx_create(int device, mdx_ctx **out) {
         return MDX_ERR_HIP;
     }
     int rc = guarded(ctx, [](Ctx &c) {
+        pppm_scratch_init(c.device);  // device address of the no-allocation entry point's scratch
         c.acc = dalloc<unsigned long long>(c, kAccTotal);
         c.dFlags = dalloc<int>(c, 12);
         c.dMax = dalloc<unsigned>(c, 4);
@@ -1117,6 +1139,34 @@ int mdx_forces_compute(mdx_ctx *ctx, int flags) {
     });
 }
 
+int mdx_forces_compute_noalloc(mdx_ctx *ctx, int flags) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        MDX_REQUIRE(c.rlist > 0, MDX_ERR_STATE, "mdx_nlist_config not called");
+        MDX_REQUIRE(c.nlistConfigured && c.nl && c.waveMax, MDX_ERR_STATE,
+                    "no neighbor list yet: call mdx_forces_compute once to set the context up");
+        const bool mesh = (flags & MDX_PPPM) && c.pppm.on;
+        MDX_REQUIRE(!mesh || c.pppm.plansMade, MDX_ERR_STATE,
+                    "no P3M plans yet: call mdx_forces_compute once to set the context up");
+        struct NoAlloc {
+            Ctx &c;
+            explicit NoAlloc(Ctx &c_) : c(c_) { c.noAlloc = true; }
+            ~NoAlloc() { c.noAlloc = false; }
+        } guard(c);
+        prepare_terms(c);
+        if (mesh) {
+            PppmScratchScope scr(c);
+            ensure_list(c);
+            compute(c, flags);
+            MDX_HIP(hipStreamSynchronize(c.stream));
+        } else {
+            ensure_list(c);
+            compute(c, flags);
+            MDX_HIP(hipStreamSynchronize(c.stream));
+        }
+    });
+}
+
 int mdx_forces_get(mdx_ctx *ctx, double *f) {
     return guarded(ctx, [&](Ctx &c) {
         require_system(c);
@@ -1537,6 +1587,27 @@ int mdx_surrogate_fit(mdx_ctx *ctx, int epochs, int batch, double lr, int seed, 
     });
 }
 
+int mdx_surrogate_grad(mdx_ctx *ctx, int bstart, int bsize, uint32_t mul, uint32_t add, float *grad, int n) {
+    return guarded(ctx, [&](Ctx &c) {
+        SurrogateState &S = c.sur;
+        const int nt = surrogate_trained_count();
+        MDX_REQUIRE(S.params && S.nsamp > 0, MDX_ERR_STATE, "no model or no recorded samples");
+        MDX_REQUIRE(grad && n >= nt + 1, MDX_ERR_ARG, "grad needs the trained parameter count + 1 floats");
+        MDX_REQUIRE(bstart >= 0 && bsize >= 1 && bsize <= S.nsamp && mul >= 1, MDX_ERR_ARG,
+                    "0 <= bstart, 1 <= bsize <= recorded samples, mul >= 1");
+        float *d = dalloc<float>(c, nt + 1);
+        try {
+            surrogate_grad(c, bstart, bsize, mul, add, d);
+            MDX_HIP(hipMemcpyAsync(grad, d, sizeof(float) * (nt + 1), hipMemcpyDeviceToHost, c.stream));
+            MDX_HIP(hipStreamSynchronize(c.stream));
+        } catch (...) {
+            dfree(c, d);
+            throw;
+        }
+        dfree(c, d);
+    });
+}
+
 int mdx_surrogate_eval(mdx_ctx *ctx, float *emol, float *feat) {
     return guarded(ctx, [&](Ctx &c) {
         require_system(c);
@@ -1674,6 +1745,16 @@ int mdx_option_set(mdx_ctx *ctx, const char *name, int value) {
             c.nlistConfigured = false;
             c.listValid = false;
         }
+        else if (k == "row_cap") {
+            // neighbor-row capacity now (rows still grow on overflow, the
+            // step loop replays a chunk whose rebuild overflowed)
+            MDX_REQUIRE(value >= 16 && c.nlCount, MDX_ERR_ARG, "row_cap: >= 16, after the first list build");
+            MDX_HIP(hipStreamSynchronize(c.stream));
+            dfree(c, c.nl);
+            c.cap = value;
+            c.nl = dalloc<uint32_t>(c, (size_t)c.n * c.cap);
+            c.listValid = false;
+        }
         else throw Error{MDX_ERR_ARG, "unknown option " + k};
     });
 }
@@ -1732,6 +1813,16 @@ int mdx_nose_hoover_energy(mdx_ctx *ctx, double *chain_energy) {
             e += (k == 0 ? H.ndf : 1.0) * kT * st[3 * M + k];
         }
         *chain_energy = H.fresh ? 0.0 : e;
+    });
+}
+
+int mdx_nose_hoover_state(mdx_ctx *ctx, double *state, int n) {
+    return guarded(ctx, [&](Ctx &c) {
+        NoseHooverState &H = c.nh;
+        MDX_REQUIRE(H.on && H.dev && state, MDX_ERR_STATE, "Nose-Hoover is off");
+        MDX_REQUIRE(n >= 4 * H.chain + 2, MDX_ERR_ARG, "state needs 4 chain_length + 2 doubles");
+        MDX_HIP(hipMemcpyAsync(state, H.dev, sizeof(double) * (4 * H.chain + 2), hipMemcpyDeviceToHost, c.stream));
+        MDX_HIP(hipStreamSynchronize(c.stream));
     });
 }
 

@@ -354,6 +354,7 @@ constexpr int kFlagVirialTrace = 1 << 12;
 
 struct Ctx {
     int device = 0;
+    bool noAlloc = false;  // inside mdx_forces_compute_noalloc: any device allocation is an error
     bool vtrace = false;  // set by compute for the terms it launches
     bool fuseMolVirial = false;  // captured barostat steps: rigid second half sums ACC_MOL
     bool fuseThermo = false;     // captured thermostat steps: rigid second half sums m v^2
@@ -476,6 +477,9 @@ struct Ctx {
         long long turn = 0, rebuilds = 0;
         Geometry geo;
         BoxDev *box = nullptr;
+        double *nh = nullptr;                 // Nose-Hoover chain state (4 chain + 2 doubles)
+        int nhLen = 0;
+        unsigned long long *thermo = nullptr; // thermostat sums (2 words)
     } snap;
 
     // profiling: event pairs recorded on the context stream around kernels
@@ -516,6 +520,9 @@ struct Error {
 
 template <class T>
 T *dalloc(Ctx &c, size_t count) {
+    MDX_REQUIRE(!c.noAlloc, 3,
+                "this call would allocate device memory (first list build, grown rows, new buffers): "
+                "call mdx_forces_compute once to set the context up");
     void *p = nullptr;
     if (count == 0) count = 1;
     MDX_HIP(hipMalloc(&p, count * sizeof(T)));
@@ -602,6 +609,22 @@ std::vector<int> pair_type_flag_table(const Ctx &c);
 void pppm_prepare(Ctx &c);            // alpha + Green function for the current box
 void pppm_forces(Ctx &c, bool energy, bool virial);
 void pppm_release(Ctx &c);
+// File-scope device scratch for the P3M mesh work arrays of
+// mdx_forces_compute_noalloc: its address on `device` (looked up once per
+// device), the largest mesh it holds, and a scope that points the context's
+// mesh arrays at it (cleared charge mesh) for the duration of one call.
+constexpr int kScratchMeshMax = 64 * 64 * 64;
+void pppm_scratch_init(int device);
+struct PppmScratchScope {
+    Ctx &c;
+    long long *mesh;
+    float *real;
+    float2 *rhoHat, *eHat;
+    float *eReal;
+    float4 *eField4;
+    explicit PppmScratchScope(Ctx &c_);
+    ~PppmScratchScope();
+};
 
 
 // e3b.hip
@@ -614,6 +637,11 @@ void surrogate_normalize(Ctx &c);
 // BF16 weight fragments of the inference kernel from the FP32 parameters
 void surrogate_pack(Ctx &c);
 double surrogate_fit(Ctx &c, int epochs, int batch, double lr, unsigned seed);
+// summed gradient of the trained parameters over one mini-batch of recorded
+// samples (rows (bstart + s) mul + add mod nsamp), loss after them, into the
+// device array out (surrogate_trained_count() + 1 floats)
+void surrogate_grad(Ctx &c, int bstart, int bsize, unsigned mul, unsigned add, float *out);
+int surrogate_trained_count();
 void fill_float(Ctx &c, float *p, int count, float v);
 
 // rigid.hip

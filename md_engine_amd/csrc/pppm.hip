@@ -565,7 +565,84 @@ bool tiled_geometry(const Ctx &c, TileGeo &tg) {
     return true;
 }
 
+// clears n 64-bit words (the charge mesh of the scratch)
+__global__ void k_scr_zero(long long n, unsigned long long *__restrict__ p) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// Scratch of mdx_forces_compute_noalloc: the P3M work arrays for meshes of up
+// to kScratchMeshMax points (charge mesh int64, real mesh, half spectrum, the
+// three field spectra, the three real field components, the packed field),
+// about 14 MiB per GPU, held until the process ends.  Host code never uses
+// the symbol itself (a host-side stand-in): it takes the device address from
+// hipGetSymbolAddress, once per device, with that device current.  Calls that
+// overlap on one GPU share this memory and race.
+namespace {
+constexpr size_t kScrAlign = 256;
+constexpr size_t kScrNh = kScratchMeshMax / 2 + kScratchMeshMax / 32;  // sx sy (sz / 2 + 1) for sz >= 32
+constexpr size_t scr_round(size_t b) { return (b + kScrAlign - 1) / kScrAlign * kScrAlign; }
+constexpr size_t kScrMesh = 0;
+constexpr size_t kScrReal = kScrMesh + scr_round(sizeof(long long) * kScratchMeshMax);
+constexpr size_t kScrRho = kScrReal + scr_round(sizeof(float) * kScratchMeshMax);
+constexpr size_t kScrEHat = kScrRho + scr_round(sizeof(float2) * kScrNh);
+constexpr size_t kScrEReal = kScrEHat + scr_round(sizeof(float2) * 3 * kScrNh);
+constexpr size_t kScrField = kScrEReal + scr_round(sizeof(float) * 3 * kScratchMeshMax);
+constexpr size_t kScrBytes = kScrField + scr_round(sizeof(float4) * kScratchMeshMax);
+constexpr int kScrMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScrBytes];
+
+static unsigned char *g_scratch[kScrMaxDevices] = {};
+
+void pppm_scratch_init(int device) {
+    MDX_REQUIRE(device >= 0 && device < kScrMaxDevices, 1, "device index beyond the scratch table");
+    if (g_scratch[device]) return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    MDX_REQUIRE(e == hipSuccess && p, 2, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[device] = (unsigned char *)p;
+}
+
+PppmScratchScope::PppmScratchScope(Ctx &c_) : c(c_) {
+    PppmState &P = c.pppm;
+    mesh = P.mesh;
+    real = P.real;
+    rhoHat = P.rhoHat;
+    eHat = P.eHat;
+    eReal = P.eReal;
+    eField4 = P.eField4;
+    const long long ng = (long long)P.sz[0] * P.sz[1] * P.sz[2];
+    const long long nh = (long long)P.sz[0] * P.sz[1] * (P.sz[2] / 2 + 1);
+    MDX_REQUIRE(ng <= kScratchMeshMax && nh <= (long long)kScrNh, 1,
+                "mesh larger than the scratch of mdx_forces_compute_noalloc (64^3 points)");
+    unsigned char *base = g_scratch[c.device];
+    MDX_REQUIRE(base, 3, "scratch address not looked up on this device");
+    P.mesh = (long long *)(base + kScrMesh);
+    P.real = (float *)(base + kScrReal);
+    P.rhoHat = (float2 *)(base + kScrRho);
+    P.eHat = (float2 *)(base + kScrEHat);
+    P.eReal = (float *)(base + kScrEReal);
+    P.eField4 = (float4 *)(base + kScrField);
+    // the spread adds into the charge mesh, which the last call on this GPU
+    // left in any state: clear it with a kernel (no memset node)
+    hipLaunchKernelGGL(k_scr_zero, dim3(cdiv(ng, 256)), dim3(256), 0, c.stream, ng,
+                       (unsigned long long *)P.mesh);
+}
+
+PppmScratchScope::~PppmScratchScope() {
+    PppmState &P = c.pppm;
+    P.mesh = mesh;
+    P.real = real;
+    P.rhoHat = rhoHat;
+    P.eHat = eHat;
+    P.eReal = eReal;
+    P.eField4 = eField4;
+}
 
 void pppm_release(Ctx &c) {
     if (c.pppm.plansMade) {

@@ -862,6 +862,15 @@ __global__ void k_sur_adam(const float *__restrict__ part, int nwaves, float *__
     P[e] -= lr * (a * c1) / (sqrtf(v * c2) + 1e-8f);
 }
 
+// the summed gradient (and loss) of one mini-batch, without an update
+__global__ void k_sur_gsum(const float *__restrict__ part, int nwaves, float *__restrict__ out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e > NTRAIN) return;
+    float gsum = 0.f;
+    for (int w = 0; w < nwaves; ++w) gsum += part[(size_t)w * kGradStride + e];
+    out[e] = gsum;
+}
+
 // per-feature mean and standard deviation of the network inputs over the
 // recorded samples (one block per input column, fixed reduction order)
 __global__ void __launch_bounds__(256) k_sur_norm(int nsamp, const float *__restrict__ raw,
@@ -940,7 +949,7 @@ void surrogate_normalize(Ctx &c) {
     hipLaunchKernelGGL(k_sur_norm, dim3(NF + 1), dim3(256), 0, c.stream, S.nsamp, S.trainX, S.trainT, S.params);
 }
 
-double surrogate_fit(Ctx &c, int epochs, int batch, double lr, unsigned seed) {
+static void surrogate_train_alloc(Ctx &c) {
     SurrogateState &S = c.sur;
     if (!S.part) {
         S.part = dalloc<float>(c, (size_t)kTrainWaves * kGradStride);
@@ -950,6 +959,21 @@ double surrogate_fit(Ctx &c, int epochs, int batch, double lr, unsigned seed) {
         fill_float(c, S.m1, NTRAIN, 0.f);
         fill_float(c, S.m2, NTRAIN, 0.f);
     }
+}
+
+int surrogate_trained_count() { return NTRAIN; }
+
+void surrogate_grad(Ctx &c, int bstart, int bsize, unsigned mul, unsigned add, float *out) {
+    SurrogateState &S = c.sur;
+    surrogate_train_alloc(c);
+    hipLaunchKernelGGL(k_sur_grad, dim3(kTrainWaves / kWaves), dim3(64 * kWaves), 0, c.stream, S.nsamp, bstart, bsize,
+                       mul, add, S.params, S.trainX, S.trainT, S.part);
+    hipLaunchKernelGGL(k_sur_gsum, dim3(cdiv(NTRAIN + 1, 256)), dim3(256), 0, c.stream, S.part, kTrainWaves, out);
+}
+
+double surrogate_fit(Ctx &c, int epochs, int batch, double lr, unsigned seed) {
+    SurrogateState &S = c.sur;
+    surrogate_train_alloc(c);
     const int n = S.nsamp;
     batch = std::min(batch, n);
     double last = 0.0;

@@ -23,10 +23,21 @@ How each function is pinned independently of this restatement:
   cut-and-shift closed form at r = rc.
 * ``pair`` (Ewald real space): scipy's erfc closed form, finite differences,
   the exclusion limit -erf(alpha r)/r.
+* ``deltaf_k`` / ``deltaf_real`` (the RMS error estimates behind
+  ``pppm_alpha``): the measured RMS force error of P3M against the converged
+  reciprocal Ewald sum (orders 1, 3, 5, 7, h alpha 0.25-0.7), and of the cut
+  real-space sum against the converged one, on random charges
+  (tests/test_oracle_pppm_error.py); doubling or halving any entry of the
+  coefficient table moves the estimate out of the tolerance band.  The
+  reference's last order-7 entry is 10x the published value (see that test);
+  the oracle keeps the reference's entry so that alpha matches it.
 * ``pppm_alpha``: the two RMS error estimates are equal at the returned alpha.
 * ``pppm_green`` / ``pppm``: converged classical Ewald (``ewald_direct``) on
-  small charge-neutral systems; sum of the spread mesh = Qunit * sum q;
-  partition of unity of the weights; the FFT against numpy.fft.
+  small charge-neutral systems; at h alpha 0.3-0.7 the optimal influence
+  function gives a smaller error than the plain deconvolved kernel
+  4 pi exp(-k^2/4 alpha^2) / (k^2 U^2(k)) and the undeconvolved one, and its
+  measured error matches Deserno-Holm's estimate; sum of the spread mesh =
+  Qunit * sum q; partition of unity of the weights; the FFT against numpy.fft.
 * ``e3b``: brute-force bond-pair enumeration (``e3b_brute_energy``) for the
   energy, finite differences for the forces, Newton's third law.
 * ``lj_verlet``: energy conservation over many steps.
@@ -84,6 +95,8 @@ def _declare(L):
     L.orc_deltaf_k.restype = C.c_double
     L.orc_deltaf_real.argtypes = [C.c_double, C.c_int, C.c_double, _dp, C.c_double]
     L.orc_deltaf_real.restype = C.c_double
+    L.orc_pppm_amp.argtypes = [C.c_int, C.c_int]
+    L.orc_pppm_amp.restype = C.c_double
     L.orc_pppm_alpha.argtypes = [C.c_int, C.c_double, _dp, _ip, C.c_int, C.c_double, C.c_void_p]
     L.orc_pppm_alpha.restype = C.c_double
     L.orc_pppm_green.argtypes = [_dp, _ip, C.c_double, C.c_int, _dp]
@@ -280,6 +293,11 @@ def pppm_alpha(natoms, total_q2, L, sz, order, rcut):
 
 def deltaf_k(a, natoms, total_q2, L, sz, order):
     return lib().orc_deltaf_k(float(a), int(natoms), float(total_q2), _d(L), _i(sz), int(order))
+
+
+def pppm_amp(order, m):
+    """Entry m of the order's row of the RMS-error coefficient table."""
+    return lib().orc_pppm_amp(int(order), int(m))
 
 
 def deltaf_real(a, natoms, total_q2, L, rcut):

@@ -330,6 +330,10 @@ static const double orc_amp_table[7][7] = {
      25091609.0 / 1560084480.0, 1755948832039.0 / 36229939200000.0,
      48887769399.0 / 37838389248.0}};
 
+/* One entry of the RMS-error coefficient table (FixChargeEwald.cu:854-862),
+ * for tests that perturb the table. */
+double orc_pppm_amp(int order, int m) { return orc_amp_table[order - 1][m]; }
+
 /* DeltaF_k (FixChargeEwald.cu:880-895). */
 double orc_deltaf_k(double a, int natoms, double total_q2, const double *L,
                     const int *sz, int order) {

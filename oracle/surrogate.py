@@ -28,7 +28,14 @@ The pair-product channels and the products of the KE channels are the
 three-body correlations the E3B energy is built from (its per-molecule
 energy is linear in F), so the network can reach the E3B forces closely.
 
+Training loss (the GPU's ``k_sur_grad``): mean over a mini-batch of the
+squared difference between the network output e and the standardised
+target (t - eshift) / escale; ``loss_grad`` gives it with its gradient in the
+trained parameters (W1 b1 W2 b2 w3 b3, packed order).
+
 Pinned by: finite differences of ``energy`` against ``forces`` (FP64);
+finite differences of ``loss_grad``'s loss against its gradient along random
+directions;
 ``descriptors`` against a per-pair loop (``descriptors_loop``); the per-molecule
 three-body energies ``e3b_molecule_energies`` sum to the C oracle's E3B
 three-body energy, which is itself pinned by brute-force enumeration.
@@ -216,6 +223,36 @@ def mlp(p, F):
     dz1 = (dz2 @ m["W2"].T) * dsilu(z1)
     dF = m["escale"] * (dz1 @ m["W1"].T) / m["sd"]
     return E, dF
+
+
+NTRAIN = NF * NH + NH + NH * NH + NH + NH + 1
+
+
+def loss_grad(p, raw, target):
+    """Energy-regression loss of one mini-batch (raw channels (B, NR), E3B
+    per-molecule three-body energies (B,)) and its gradient (NTRAIN,) in the
+    packed order W1, b1, W2, b2, w3, b3."""
+    m = unpack(p)
+    B = len(raw)
+    xh = (features(raw) - m["mu"]) / m["sd"]
+    z1 = xh @ m["W1"] + m["b1"]
+    h1 = silu(z1)
+    z2 = h1 @ m["W2"] + m["b2"]
+    h2 = silu(z2)
+    e = h2 @ m["w3"] + m["b3"]
+    t = (np.asarray(target, np.float64) - m["eshift"]) / m["escale"]
+    diff = e - t
+    loss = float((diff * diff).sum() / B)
+    g = 2.0 * diff / B                      # dL/de
+    gw3 = h2.T @ g
+    gb3 = g.sum()
+    dz2 = (g[:, None] * m["w3"][None, :]) * dsilu(z2)
+    gW2 = h1.T @ dz2
+    gb2 = dz2.sum(0)
+    dz1 = (dz2 @ m["W2"].T) * dsilu(z1)
+    gW1 = xh.T @ dz1
+    gb1 = dz1.sum(0)
+    return loss, np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gw3, [gb3]])
 
 
 def energy(p, mol, x, L):

@@ -149,3 +149,32 @@ def test_surrogate_training_gain(orc, gpu_engine_factory):
     # a fit on eight frames reproduces the three-body forces of a later frame
     # to a few percent
     assert err_sur < 0.1 * err_drop
+
+
+def test_training_gradient_vs_oracle(orc, gpu_engine_factory):
+    """The hand-written backward pass of training (k_sur_grad: BF16 MFMA
+    tiles, FP32 accumulation) against the FP64 loss gradient
+    (oracle/surrogate.py loss_grad) on the same recorded samples."""
+    say("surrogate training gradient vs FP64")
+    box = systems.tip4p2005_box(216, seed=13, min_sep=1.8)
+    e, _ = water_md_engine(gpu_engine_factory, box, np.zeros_like(box["x"]), rc=7.0, skin=1.0,
+                           sz=(32, 32, 32), order=5, e3b=True, dt=1.0)
+    nm = len(box["mol"])
+    raw = S.descriptors(box["mol"], box["x"], box["L"])
+    p = random_model(2, S.features(raw))
+    e.surrogate_set(p)
+    e.surrogate_record(reset=True, capacity=nm)
+    target = e.e3b_molecule_energies(nm).astype(np.float64)
+    _, feat = e.surrogate_eval(nm)
+    for bstart, bsize, mul, add in ((0, nm, 1, 0), (16, 96, 7, 3)):
+        g = e.surrogate_grad(bstart, bsize, mul, add)
+        rows = ((bstart + np.arange(bsize)) * mul + add) % nm
+        lo, go = S.loss_grad(p.astype(np.float64), feat[rows].astype(np.float64), target[rows])
+        rel = np.linalg.norm(g[:-1] - go) / np.linalg.norm(go)
+        say(f"batch {bstart}+{bsize}: loss {g[-1]:.5f} vs {lo:.5f}, gradient rel err {rel:.2e}")
+        assert g[-1] == pytest.approx(lo, rel=2e-2)
+        assert rel < 2e-2
+        # every block of the gradient, not just the largest
+        blocks = np.cumsum([0, S.NF * S.NH, S.NH, S.NH * S.NH, S.NH, S.NH, 1])
+        for a, b in zip(blocks[:-1], blocks[1:]):
+            assert np.linalg.norm(g[a:b] - go[a:b]) <= 3e-2 * np.linalg.norm(go[a:b]) + 1e-6, (a, b)

@@ -162,3 +162,40 @@ def test_nose_hoover_chain(orc, gpu_engine_factory, graph):
     say(f"T mean {np.mean(temps):.1f} K, extended-energy span {np.ptp(hs + [h0]):.3f} (KE {ke0:.1f})")
     assert np.mean(temps) == pytest.approx(298.0, rel=0.05)
     assert np.ptp(hs + [h0]) < 0.01 * ke0
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+def test_nose_hoover_chain_matches_oracle(orc, gpu_engine_factory, graph):
+    """50 steps under a 10-link chain at 298 K from a 350 K start: particle
+    velocities, positions and every chain variable against the FP64
+    restatement of FixNoseHoover::thermostatIntegrate (oracle/thermostat.py)."""
+    say(f"Nose-Hoover chain vs oracle, graph={graph}")
+    from oracle.thermostat import water_md_nh
+
+    box = systems.tip4p2005_box(216, seed=25, min_sep=1.8)
+    v = start_velocities(orc, box, seed=8) * np.sqrt(350.0 / 298.0)
+    kw = dict(rc=7.0, skin=1.0, sz=(32, 32, 32), order=5)
+    e, flags = water_md_engine(gpu_engine_factory, box, v, e3b=False, dt=1.0, **kw)
+    kb, mvv = systems.REAL_BOLTZ, systems.REAL_MVV_TO_ENG
+    e.nose_hoover(True, 298.0, 20.0, 10, kb, mvv)
+    if graph:
+        e.option("graph", 1)
+    e.run(50, flags)
+    xg, vg = e.get_system()
+    st = e.nose_hoover_state(10)
+    xo, vo, ch = water_md_nh(box, v, 50, 298.0, 20.0, chain=10, dt=1.0, e3b_on=False, **kw)
+    so = ch.state()
+    e.close()
+    dx = np.abs(min_image_diff(xg, xo, box["L"])).max()
+    heavy = box["mass"] > 0
+    rel_v = np.abs(vg[heavy] - vo[heavy]).max() / np.abs(vo[heavy]).max()
+    say(f"dx {dx:.2e} A, rel v {rel_v:.2e}, chain v {st[:10]} vs {so[:10]}")
+    assert dx < 1e-5, dx
+    assert rel_v < 1e-4, rel_v
+    # chain velocities, forces, masses and positions, sum m v^2, last scale
+    for name, a, b in (("v", st[0:10], so[0:10]), ("G", st[10:20], so[10:20]), ("Q", st[20:30], so[20:30]),
+                       ("xi", st[30:40], so[30:40])):
+        err = np.abs(a - b).max() / np.abs(b).max()
+        assert err < 1e-4, (name, err)
+    assert st[40] == pytest.approx(so[40], rel=1e-5)
+    assert st[41] == pytest.approx(so[41], rel=1e-9)

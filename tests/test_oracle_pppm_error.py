@@ -1,0 +1,163 @@
+"""Pins of the oracle's P3M error estimates and optimal influence function by
+measurement, not by their own formulas.
+
+On random charges in a periodic box the RMS force error of P3M against the
+converged reciprocal Ewald sum is what Deserno and Holm's estimate DeltaF_k
+predicts (J. Chem. Phys. 109, 7694 (1998), eq. 31 with their table of
+coefficients; FixChargeEwald.cu:854-895), and the RMS error of the cut
+real-space sum is Kolafa and Perram's DeltaF_real (FixChargeEwald.cu:897-900).
+The estimate assumes the optimal influence function, so the same measurement
+pins ``pppm_green`` where aliasing matters (h alpha 0.3-0.7), and the
+optimal function must beat the plain kernels there.
+"""
+import numpy as np
+import pytest
+
+BOX = 32.0
+NQ = 500
+
+
+@pytest.fixture(scope="module")
+def charges():
+    rng = np.random.default_rng(7)
+    x = rng.uniform(0, BOX, (NQ, 3))
+    q = np.where(np.arange(NQ) % 2 == 0, 1.0, -1.0)
+    return x, q
+
+
+@pytest.fixture(scope="module")
+def recip(orc, charges):
+    """Converged reciprocal-space Ewald forces per alpha (no real-space pairs:
+    the real-space cutoff is below every distance)."""
+    x, q = charges
+    L = np.full(3, BOX)
+    out = {}
+    for a in (0.25, 0.3, 0.35):
+        kmax = int(np.ceil(12 * a * BOX / (2 * np.pi)))
+        out[a] = orc.ewald_direct(x, q, L, a, 1.0, rcut_real=1e-3, kmax=kmax)[1]
+    return out
+
+
+def _measured(orc, charges, recip, a, mesh, order, G=None):
+    x, q = charges
+    L = np.full(3, BOX)
+    sz = np.array([mesh] * 3)
+    if G is None:
+        G = orc.pppm_green(L, sz, a, order)
+    f = orc.pppm(x, q, np.zeros(3), L, sz, order, 1.0, a, (q * q).sum(), G)["f"]
+    return float(np.sqrt(((f - recip[a]) ** 2).sum(1).mean()))
+
+
+# (order, mesh, alpha): h = 32 / mesh A, so h alpha runs from 0.25 to 0.7
+# (0.125 to 0.35 for the nearest-grid-point order 1, whose estimate is coarse
+# at h alpha 0.5 and above)
+CASES = [(o, m, a) for o in (1, 3, 5) for m in ((32, 64) if o == 1 else (16, 32)) for a in (0.25, 0.3, 0.35)]
+BAND = {1: 0.09, 3: 0.05, 5: 0.06}
+
+
+@pytest.fixture(scope="module")
+def measured(orc, charges, recip):
+    return {c: _measured(orc, charges, recip, c[2], c[1], c[0]) for c in CASES}
+
+
+def test_deltaf_k_predicts_measured_error(orc, measured):
+    L = np.full(3, BOX)
+    for (order, mesh, a), err in measured.items():
+        est = orc.deltaf_k(a, NQ, float(NQ), L, np.array([mesh] * 3), order)
+        assert abs(err / est - 1) < BAND[order], (order, mesh, a, err, est)
+
+
+def _deltaf_k_from(amp, a, natoms, q2, L, sz, order):
+    """Deserno-Holm's estimate from an explicit coefficient row (used only to
+    show what a wrong coefficient would do; checked equal to the oracle's
+    with the oracle's own row)."""
+    h = L / sz
+    acc = 0.0
+    for d in range(3):
+        s = sum(amp[m] * (h[d] * a) ** (2 * m) for m in range(order))
+        acc += 1.0 / L[d] ** 2 * (a * h[d]) ** order * np.sqrt(a * L[d] / natoms * np.sqrt(2 * np.pi) * s)
+    return q2 / 3.0 * acc
+
+
+def test_a_wrong_coefficient_leaves_the_band(orc, measured):
+    L = np.full(3, BOX)
+    for order in (1, 3, 5):
+        row = [orc.pppm_amp(order, m) for m in range(order)]
+        for (o, mesh, a), err in measured.items():
+            if o != order:
+                continue
+            mine = _deltaf_k_from(row, a, NQ, float(NQ), L, np.array([mesh] * 3), order)
+            assert mine == pytest.approx(orc.deltaf_k(a, NQ, float(NQ), L, np.array([mesh] * 3), order), rel=1e-12)
+        for m in range(order):
+            for factor in (2.0, 0.5):
+                bad = list(row)
+                bad[m] *= factor
+                worst = max(abs(err / _deltaf_k_from(bad, a, NQ, float(NQ), L, np.array([mesh] * 3), order) - 1)
+                            for (o, mesh, a), err in measured.items() if o == order)
+                assert worst > BAND[order], (order, m, factor, worst)
+
+
+def test_order7_table_entry(orc, charges, recip):
+    """The reference's last order-7 coefficient, 48887769399/37838389248, is
+    ten times the published 4887769399/37838389248.  The oracle keeps the
+    reference's entry (alpha must match the reference's); measurement sides
+    with the published value where that term matters (h alpha 0.5, 0.6),
+    and both agree where it does not (h alpha 0.25)."""
+    L = np.full(3, BOX)
+    row = [orc.pppm_amp(7, m) for m in range(7)]
+    assert row[6] == 48887769399.0 / 37838389248.0
+    published = row[:6] + [4887769399.0 / 37838389248.0]
+    for mesh, a in ((16, 0.25), (16, 0.3)):
+        err = _measured(orc, charges, recip, a, mesh, 7)
+        sz = np.array([mesh] * 3)
+        assert abs(err / _deltaf_k_from(published, a, NQ, float(NQ), L, sz, 7) - 1) < 0.05
+        assert err / orc.deltaf_k(a, NQ, float(NQ), L, sz, 7) < 0.75
+    err = _measured(orc, charges, recip, 0.25, 32, 7)
+    assert abs(err / orc.deltaf_k(0.25, NQ, float(NQ), L, np.array([32] * 3), 7) - 1) < 0.05
+
+
+@pytest.mark.parametrize("a,rc", [(0.25, 9.0), (0.3, 8.0), (0.35, 7.0), (0.3, 10.0)])
+def test_deltaf_real_predicts_measured_error(orc, charges, a, rc):
+    x, q = charges
+    L = np.full(3, BOX)
+    _, f_cut = orc.ewald_direct(x, q, L, a, 1.0, rcut_real=rc, kmax=0)
+    _, f_all = orc.ewald_direct(x, q, L, a, 1.0, rcut_real=7.0 / a, kmax=0)
+    err = float(np.sqrt(((f_cut - f_all) ** 2).sum(1).mean()))
+    est = orc.deltaf_real(a, NQ, float(NQ), L, rc)
+    assert abs(err / est - 1) < 0.07, (err, est)
+
+
+def _plain_kernels(L, sz, a, order):
+    """Textbook alternatives to the optimal influence function: the Ewald
+    kernel 4 pi exp(-k^2 / 4 a^2) / k^2 divided by the squared assignment
+    function U^2(k) = prod sinc(k_i h_i / 2)^(2 order) (one deconvolution for
+    the spread and one for the gather), and not divided at all."""
+    ks = [2 * np.pi * np.fft.fftfreq(n, d=Li / n) for n, Li in zip(sz, L)]
+    kx, ky, kz = np.meshgrid(*ks, indexing="ij")
+    k2 = kx * kx + ky * ky + kz * kz
+    h = L / sz
+    U2 = (np.sinc(kx * h[0] / (2 * np.pi)) * np.sinc(ky * h[1] / (2 * np.pi)) *
+          np.sinc(kz * h[2] / (2 * np.pi))) ** (2 * order)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        g = np.where(k2 > 0, 4 * np.pi * np.exp(-k2 / (4 * a * a)) / k2, 0.0)
+    return g / U2, g
+
+
+@pytest.mark.parametrize("order", [1, 3, 5, 7])
+@pytest.mark.parametrize("a", [0.3, 0.35])
+def test_optimal_green_beats_plain_kernels(orc, charges, recip, order, a):
+    """At h = 2 A (h alpha 0.6-0.7) the alias sums of the optimal influence
+    function lower the force error by 2-10 % against the deconvolved kernel
+    (the optimal function with its alias sums dropped) and by far more against
+    the raw Ewald kernel.  At h = 1 A the two differ by under 0.1 %: there the
+    alias sums barely touch the forces, and the error estimate above pins G."""
+    L = np.full(3, BOX)
+    for mesh, gain in ((16, 1.015), (32, 1.0)):
+        sz = np.array([mesh] * 3)
+        deconv, raw = _plain_kernels(L, sz, a, order)
+        e_opt = _measured(orc, charges, recip, a, mesh, order)
+        e_dec = _measured(orc, charges, recip, a, mesh, order, G=deconv)
+        e_raw = _measured(orc, charges, recip, a, mesh, order, G=raw)
+        assert e_opt * gain < e_dec, (mesh, e_opt, e_dec)
+        if order > 1:
+            assert e_opt < 0.2 * e_raw, (mesh, e_opt, e_raw)

@@ -112,3 +112,41 @@ def test_basis_ke_is_the_e3b_bond_function(orc):
     gm, _ = S.basis(r - h)
     inner = (r > 0.9) & (np.abs(r - P.rs) > 1e-3) & (np.abs(r - P.rf) > 1e-3)
     assert np.allclose(dg[inner], ((gp - gm) / (2 * h))[inner], rtol=1e-6, atol=1e-12)
+
+
+def test_loss_gradient_matches_finite_differences(small_box):
+    """The training-loss gradient (the restatement of the GPU's backward pass
+    through both hidden layers) against central differences of the loss along
+    random directions and along single parameters of every block."""
+    b = small_box
+    raw = S.descriptors(b["mol"], b["x"], b["L"])
+    F = S.features(raw)
+    rng = np.random.default_rng(4)
+    m = S.unpack(S.init_params(5))
+    m["mu"], m["sd"] = F.mean(0), np.maximum(F.std(0), 1e-3)
+    m["escale"], m["eshift"] = 0.07, 0.02
+    m["b1"] = rng.normal(0, 0.2, S.NH)
+    m["b2"] = rng.normal(0, 0.2, S.NH)
+    p = S.pack(m)
+    target = rng.normal(0.0, 0.05, len(raw))
+    loss, g = S.loss_grad(p, raw, target)
+    assert g.shape == (S.NTRAIN,)
+
+    def L(q):
+        return S.loss_grad(q, raw, target)[0]
+
+    h = 1e-6
+    for k in range(4):
+        d = np.zeros_like(p)
+        d[: S.NTRAIN] = rng.normal(size=S.NTRAIN)
+        fd = (L(p + h * d) - L(p - h * d)) / (2 * h)
+        assert fd == pytest.approx(g @ d[: S.NTRAIN], rel=1e-6, abs=1e-10)
+    # one parameter from each block: W1, b1, W2, b2, w3, b3
+    offs = [0, S.NF * S.NH, S.NF * S.NH + S.NH, S.NF * S.NH + S.NH + S.NH * S.NH,
+            S.NF * S.NH + 2 * S.NH + S.NH * S.NH, S.NTRAIN - 1]
+    for o in offs:
+        k = o + (7 if o < S.NTRAIN - 1 else 0)
+        d = np.zeros_like(p)
+        d[k] = 1.0
+        fd = (L(p + h * d) - L(p - h * d)) / (2 * h)
+        assert fd == pytest.approx(g[k], rel=1e-5, abs=1e-9), k
