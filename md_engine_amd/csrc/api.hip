@@ -2,6 +2,7 @@
 // the step loop.  See include/mdx.h for the contract of each entry point.
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -19,6 +20,7 @@ namespace mdx {
 void dfree(Ctx &c, void *p) {
     if (!p) return;
     MDX_REQUIRE(!c.noAlloc, 3, "this call would free device memory");
+    ++c.layoutEpoch;
     auto it = std::find(c.allocs.begin(), c.allocs.end(), p);
     if (it != c.allocs.end()) {
         hipFree(p);
@@ -60,9 +62,29 @@ static void prof_clear(Ctx &c) {
 
 namespace {
 
+// Runs f with errors turned into return codes.  Every call through it may
+// change the set-up, so cached step graphs of the context are retired;
+// read-only entry points use guarded_ro.
+template <class F>
+int guarded_ro(mdx_ctx *ctx, F &&f) {
+    if (!ctx) return MDX_ERR_ARG;
+    try {
+        f(ctx->c);
+        ctx->c.err.clear();
+        return MDX_OK;
+    } catch (const Error &e) {
+        ctx->c.err = e.msg;
+        return e.code;
+    } catch (const std::exception &e) {
+        ctx->c.err = e.what();
+        return MDX_ERR_INTERNAL;
+    }
+}
+
 template <class F>
 int guarded(mdx_ctx *ctx, F &&f) {
     if (!ctx) return MDX_ERR_ARG;
+    ++ctx->c.apiEpoch;
     try {
         f(ctx->c);
         ctx->c.err.clear();
@@ -435,12 +457,35 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags, const Rin
             nose_hoover_start(c);
         }
     }
+    // Without a ring, the step graphs stay with the first context after the
+    // call: the next call replays them as long as nothing they bake in (the
+    // contexts, their set-up, buffers, cell grid, buffer parity, the force
+    // terms) changed.  A ring's bead table is made per call.
+    Ctx::GraphCache *cache = ring ? nullptr : &cs[0]->graphs;
     hipGraphExec_t big = nullptr, one = nullptr;
+    if (cache) {
+        big = cache->big;
+        one = cache->one;
+    }
     std::vector<int> ringCur;
     auto drop = [&] {
         if (big) hipGraphExecDestroy(big);
         if (one) hipGraphExecDestroy(one);
         big = one = nullptr;
+        if (cache) {
+            cache->big = cache->one = nullptr;
+            cache->key.clear();
+        }
+    };
+    auto key_now = [&] {
+        std::vector<uint64_t> k{(uint64_t)fflags, (uint64_t)cs.size()};
+        for (Ctx *cp : cs) {
+            k.push_back(cp->serial);
+            k.push_back(cp->apiEpoch);
+            k.push_back(cp->layoutEpoch);
+            k.push_back((uint64_t)cp->cur);
+        }
+        return k;
     };
     try {
         int done = 0;
@@ -476,9 +521,20 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags, const Rin
                 ringCur = curs;
                 ring_table(cs, *ring);
             }
+            if (cache) {
+                const std::vector<uint64_t> k = key_now();
+                if (k != cache->key) {
+                    drop();
+                    cache->key = k;
+                }
+            }
             if (!big) big = capture_steps(cs, kGraphSteps, fflags, ring);
             const int nbig = len / kGraphSteps, nrest = len % kGraphSteps;
             if (nrest && !one) one = capture_steps(cs, 1, fflags, ring);
+            if (cache) {
+                cache->big = big;
+                cache->one = one;
+            }
             for (int k = 0; k < nbig; ++k) MDX_HIP(hipGraphLaunch(big, cs[0]->stream));
             for (int k = 0; k < nrest; ++k) MDX_HIP(hipGraphLaunch(one, cs[0]->stream));
             MDX_HIP(hipStreamSynchronize(cs[0]->stream));
@@ -533,7 +589,7 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags, const Rin
         drop();
         throw;
     }
-    drop();
+    if (!cache) drop();
     for (Ctx *cp : cs) MDX_HIP(hipStreamSynchronize(cp->stream));
 }
 
@@ -552,8 +608,10 @@ int mdx_create(int device, mdx_ctx **out) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return MDX_ERR_HIP;
     if (device < 0 || device >= ndev) return MDX_ERR_ARG;
+    static std::atomic<uint64_t> serials{0};
     mdx_ctx *ctx = new mdx_ctx();
     ctx->c.device = device;
+    ctx->c.serial = ++serials;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->c.stream) != hipSuccess) {
         delete ctx;
         return MDX_ERR_HIP;
@@ -586,6 +644,8 @@ int mdx_destroy(mdx_ctx *ctx) {
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     if (c.stream) hipStreamSynchronize(c.stream);
+    if (c.graphs.big) hipGraphExecDestroy(c.graphs.big);
+    if (c.graphs.one) hipGraphExecDestroy(c.graphs.one);
     pppm_release(c);
     prof_clear(c);
     if (c.pinned) hipHostFree(c.pinned);
@@ -699,7 +759,7 @@ int mdx_positions_set(mdx_ctx *ctx, const double *x) {
 }
 
 int mdx_system_get(mdx_ctx *ctx, double *x, double *v) {
-    return guarded(ctx, [&](Ctx &c) {
+    return guarded_ro(ctx, [&](Ctx &c) {
         require_system(c);
         std::vector<int> idx(c.n);
         std::vector<double4> hp(c.n), hv(c.n);
@@ -734,7 +794,7 @@ int mdx_box_set(mdx_ctx *ctx, const double lo[3], const double len[3]) {
 }
 
 int mdx_box_get(mdx_ctx *ctx, double lo[3], double len[3]) {
-    return guarded(ctx, [&](Ctx &c) {
+    return guarded_ro(ctx, [&](Ctx &c) {
         MDX_REQUIRE(lo && len, MDX_ERR_ARG, "box output required");
         for (int d = 0; d < 3; ++d) {
             lo[d] = c.geo.lo[d];
@@ -850,7 +910,7 @@ int mdx_nlist_update(mdx_ctx *ctx, int force, int *rebuilt) {
 }
 
 int mdx_nlist_get(mdx_ctx *ctx, int32_t *counts, uint32_t *tagged_ids, int row_cap, int *max_count) {
-    return guarded(ctx, [&](Ctx &c) {
+    return guarded_ro(ctx, [&](Ctx &c) {
         require_system(c);
         MDX_REQUIRE(c.listValid, MDX_ERR_STATE, "no neighbor list built");
         const int n = c.n;
@@ -883,7 +943,7 @@ int mdx_nlist_get(mdx_ctx *ctx, int32_t *counts, uint32_t *tagged_ids, int row_c
 
 int mdx_nlist_stats(mdx_ctx *ctx, int64_t *rebuilds, int *row_cap, int32_t cells[3],
                     int64_t *violations) {
-    return guarded(ctx, [&](Ctx &c) {
+    return guarded_ro(ctx, [&](Ctx &c) {
         if (rebuilds) *rebuilds = c.nRebuilds;
         if (violations) *violations = c.listViolations;
         if (row_cap) *row_cap = c.cap;
@@ -962,7 +1022,7 @@ int mdx_pppm_set(mdx_ctx *ctx, int nx, int ny, int nz, int order, double r_cut, 
 }
 
 int mdx_pppm_info(mdx_ctx *ctx, double *alpha, double *rms_error) {
-    return guarded(ctx, [&](Ctx &c) {
+    return guarded_ro(ctx, [&](Ctx &c) {
         MDX_REQUIRE(c.pppm.on, MDX_ERR_STATE, "P3M not configured");
         require_system(c);
         pppm_prepare(c);
@@ -1120,7 +1180,7 @@ int mdx_flexible_water_set(mdx_ctx *ctx, int nmol, const int32_t *mol_ids, doubl
 }
 
 int mdx_bonded_energy_get(mdx_ctx *ctx, double *e) {
-    return guarded(ctx, [&](Ctx &c) {
+    return guarded_ro(ctx, [&](Ctx &c) {
         MDX_REQUIRE(e, MDX_ERR_ARG, "output required");
         double h[ACC_N];
         acc_read(c, h);
@@ -1168,7 +1228,7 @@ int mdx_forces_compute_noalloc(mdx_ctx *ctx, int flags) {
 }
 
 int mdx_forces_get(mdx_ctx *ctx, double *f) {
-    return guarded(ctx, [&](Ctx &c) {
+    return guarded_ro(ctx, [&](Ctx &c) {
         require_system(c);
         MDX_REQUIRE(f, MDX_ERR_ARG, "output required");
         std::vector<unsigned long long> hf(3 * (size_t)c.n);
@@ -1182,7 +1242,7 @@ int mdx_forces_get(mdx_ctx *ctx, double *f) {
 }
 
 int mdx_energy_get(mdx_ctx *ctx, double e[4], double virial[6]) {
-    return guarded(ctx, [&](Ctx &c) {
+    return guarded_ro(ctx, [&](Ctx &c) {
         double h[ACC_N];
         acc_read(c, h);
         auto d = [&](int s) { return h[s]; };
@@ -1207,7 +1267,7 @@ int mdx_integrator_set(mdx_ctx *ctx, double dt, double ftm_to_v) {
 }
 
 int mdx_run(mdx_ctx *ctx, int nsteps, int flags) {
-    return guarded(ctx, [&](Ctx &c) {
+    return guarded_ro(ctx, [&](Ctx &c) {
         require_system(c);
         MDX_REQUIRE(c.rlist > 0, MDX_ERR_STATE, "mdx_nlist_config not called");
         MDX_REQUIRE(nsteps >= 0, MDX_ERR_ARG, "nsteps must be >= 0");
@@ -1277,7 +1337,7 @@ int mdx_run_batch(mdx_ctx *const *ctxs, int n, int nsteps, int flags) {
     if (!ctxs || n <= 0) return MDX_ERR_ARG;
     for (int k = 0; k < n; ++k)
         if (!ctxs[k]) return MDX_ERR_ARG;
-    return guarded(ctxs[0], [&](Ctx &) {
+    return guarded_ro(ctxs[0], [&](Ctx &) {
         MDX_REQUIRE(nsteps >= 0, MDX_ERR_ARG, "nsteps must be >= 0");
         std::vector<Ctx *> cs;
         for (int k = 0; k < n; ++k) {
@@ -1535,7 +1595,7 @@ int mdx_surrogate_set(mdx_ctx *ctx, const float *params, int nparams) {
 }
 
 int mdx_surrogate_get(mdx_ctx *ctx, float *params, int nparams) {
-    return guarded(ctx, [&](Ctx &c) {
+    return guarded_ro(ctx, [&](Ctx &c) {
         MDX_REQUIRE(c.sur.params, MDX_ERR_STATE, "no surrogate model");
         MDX_REQUIRE(params && nparams == kSurNParam, MDX_ERR_ARG, "surrogate parameter count mismatch");
         MDX_HIP(hipMemcpyAsync(params, c.sur.params, sizeof(float) * kSurNParam, hipMemcpyDeviceToHost, c.stream));
@@ -1645,7 +1705,7 @@ int mdx_e3b_molecule_energies(mdx_ctx *ctx, float *emol) {
 }
 
 int mdx_profile_enable(mdx_ctx *ctx, int enable) {
-    return guarded(ctx, [&](Ctx &c) {
+    return guarded_ro(ctx, [&](Ctx &c) {
         MDX_HIP(hipStreamSynchronize(c.stream));
         prof_clear(c);
         c.prof = enable != 0;
@@ -1653,7 +1713,7 @@ int mdx_profile_enable(mdx_ctx *ctx, int enable) {
 }
 
 int mdx_profile_get(mdx_ctx *ctx, int kernel, int *count, double *total_ms) {
-    return guarded(ctx, [&](Ctx &c) {
+    return guarded_ro(ctx, [&](Ctx &c) {
         MDX_REQUIRE(kernel >= 0 && kernel < PK_N, MDX_ERR_ARG, "unknown kernel id");
         MDX_HIP(hipStreamSynchronize(c.stream));
         int n = 0;
@@ -1671,7 +1731,7 @@ int mdx_profile_get(mdx_ctx *ctx, int kernel, int *count, double *total_ms) {
 }
 
 int mdx_pair_entries_read(mdx_ctx *ctx, int64_t *entries, int64_t *atoms) {
-    return guarded(ctx, [&](Ctx &c) {
+    return guarded_ro(ctx, [&](Ctx &c) {
         require_system(c);
         MDX_REQUIRE(c.listValid, MDX_ERR_STATE, "no neighbor list built");
         const int n = c.n;
@@ -1698,7 +1758,7 @@ int mdx_pair_entries_read(mdx_ctx *ctx, int64_t *entries, int64_t *atoms) {
 }
 
 int mdx_nlist_totals(mdx_ctx *ctx, int64_t *pair_entries, int64_t *mol_entries) {
-    return guarded(ctx, [&](Ctx &c) {
+    return guarded_ro(ctx, [&](Ctx &c) {
         require_system(c);
         MDX_REQUIRE(c.listValid, MDX_ERR_STATE, "no neighbor list built");
         std::vector<int> cnt(c.n);
@@ -1799,7 +1859,7 @@ int mdx_nose_hoover_set(mdx_ctx *ctx, int enable, double temperature, double tim
 }
 
 int mdx_nose_hoover_energy(mdx_ctx *ctx, double *chain_energy) {
-    return guarded(ctx, [&](Ctx &c) {
+    return guarded_ro(ctx, [&](Ctx &c) {
         NoseHooverState &H = c.nh;
         MDX_REQUIRE(H.on && H.dev && chain_energy, MDX_ERR_STATE, "Nose-Hoover is off");
         std::vector<double> st(4 * H.chain + 2);
@@ -1817,7 +1877,7 @@ int mdx_nose_hoover_energy(mdx_ctx *ctx, double *chain_energy) {
 }
 
 int mdx_nose_hoover_state(mdx_ctx *ctx, double *state, int n) {
-    return guarded(ctx, [&](Ctx &c) {
+    return guarded_ro(ctx, [&](Ctx &c) {
         NoseHooverState &H = c.nh;
         MDX_REQUIRE(H.on && H.dev && state, MDX_ERR_STATE, "Nose-Hoover is off");
         MDX_REQUIRE(n >= 4 * H.chain + 2, MDX_ERR_ARG, "state needs 4 chain_length + 2 doubles");
@@ -1864,7 +1924,7 @@ int mdx_pressure(mdx_ctx *ctx, int flags, double nktv_to_press, double mvv_to_en
 }
 
 int mdx_barostat_last(mdx_ctx *ctx, double *pressure) {
-    return guarded(ctx, [&](Ctx &c) {
+    return guarded_ro(ctx, [&](Ctx &c) {
         MDX_REQUIRE(pressure, MDX_ERR_ARG, "no output");
         *pressure = c.baro.lastP;
     });
@@ -1872,7 +1932,7 @@ int mdx_barostat_last(mdx_ctx *ctx, double *pressure) {
 
 int mdx_temperature(mdx_ctx *ctx, double boltz, double mvv_to_eng, double *temperature,
                     double *kinetic) {
-    return guarded(ctx, [&](Ctx &c) {
+    return guarded_ro(ctx, [&](Ctx &c) {
         require_system(c);
         thermo_measure(c);
         unsigned long long h = 0;
@@ -1885,7 +1945,7 @@ int mdx_temperature(mdx_ctx *ctx, double boltz, double mvv_to_eng, double *tempe
 }
 
 int mdx_sync(mdx_ctx *ctx) {
-    return guarded(ctx, [&](Ctx &c) { MDX_HIP(hipStreamSynchronize(c.stream)); });
+    return guarded_ro(ctx, [&](Ctx &c) { MDX_HIP(hipStreamSynchronize(c.stream)); });
 }
 
 }  // extern "C"

@@ -355,6 +355,17 @@ constexpr int kFlagVirialTrace = 1 << 12;
 struct Ctx {
     int device = 0;
     bool noAlloc = false;  // inside mdx_forces_compute_noalloc: any device allocation is an error
+    // Captured step graphs are kept across mdx_run calls while nothing they
+    // bake in changed: serial identifies the context for its lifetime,
+    // apiEpoch counts calls that may change the set-up, layoutEpoch counts
+    // device (re)allocations and cell-grid set-ups (kernel arguments).
+    uint64_t serial = 0;
+    uint64_t apiEpoch = 0;
+    uint64_t layoutEpoch = 0;
+    struct GraphCache {
+        hipGraphExec_t big = nullptr, one = nullptr;
+        std::vector<uint64_t> key;
+    } graphs;
     bool vtrace = false;  // set by compute for the terms it launches
     bool fuseMolVirial = false;  // captured barostat steps: rigid second half sums ACC_MOL
     bool fuseThermo = false;     // captured thermostat steps: rigid second half sums m v^2
@@ -525,6 +536,7 @@ T *dalloc(Ctx &c, size_t count) {
                 "call mdx_forces_compute once to set the context up");
     void *p = nullptr;
     if (count == 0) count = 1;
+    ++c.layoutEpoch;
     MDX_HIP(hipMalloc(&p, count * sizeof(T)));
     c.allocs.push_back(p);
     return static_cast<T *>(p);

@@ -1172,6 +1172,7 @@ void nlist_setup(Ctx &c) {
         c.nlCountA = dalloc<int>(c, c.n);
     }
     c.nlistConfigured = true;
+    ++c.layoutEpoch;  // cell counts and tables are kernel arguments of captured steps
     box_upload(c);  // cell counts changed
 }
 

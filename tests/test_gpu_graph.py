@@ -111,3 +111,41 @@ def test_batch_equals_single(orc, gpu_engine_factory):
         assert np.array_equal(xb, xs)
         assert np.array_equal(vb, vs)
         assert eb.nlist_stats()["rebuilds"] == es.nlist_stats()["rebuilds"]
+
+
+def test_cached_graphs_across_calls(orc, gpu_engine_factory):
+    """Step graphs are kept between mdx_run calls: 96 steps as one call, as
+    twelve calls of 8 and as calls of odd lengths (the 1-step graph too) give
+    the same trajectory bit for bit, also when the set-up changes between
+    calls (the thermostat is switched on: the graphs are captured again),
+    with the barostat moving the box."""
+    say("cached step graphs across calls")
+    box = systems.tip4p2005_box(343, seed=33, min_sep=1.8)
+    v = start_velocities(orc, box, seed=7)
+    out = []
+    for pattern in ([48, 48], [8] * 6 + [48], [5, 13, 30, 3, 17, 28]):
+        e, flags = water_md_engine(gpu_engine_factory, box, v, rc=8.0, skin=1.0, sz=(32, 32, 32), order=5,
+                                   e3b=True, dt=2.0)
+        e.barostat(True, 1.0, 500.0, 1, mvv_to_eng=systems.REAL_MVV_TO_ENG)
+        e.option("graph", 1)
+        done = 0
+        for k in pattern:
+            if done < 48 <= done + k:
+                # split the call at step 48 so the set-up change lands there
+                a = 48 - done
+                if a:
+                    e.run(a, flags)
+                e.thermostat(True, 298.0, 5, 0.0, systems.REAL_BOLTZ, systems.REAL_MVV_TO_ENG)
+                if k - a:
+                    e.run(k - a, flags)
+            else:
+                e.run(k, flags)
+            done += k
+        assert done == 96
+        x, vv = e.get_system()
+        out.append((x, vv, e.get_box()[1]))
+        e.close()
+    for x, vv, L in out[1:]:
+        assert np.array_equal(x, out[0][0])
+        assert np.array_equal(vv, out[0][1])
+        assert np.array_equal(L, out[0][2])
