@@ -108,6 +108,8 @@ def _declare(L):
     L.orc_ewald_direct.restype = C.c_double
     L.orc_e3b.argtypes = [C.c_int, _ip, _dp, _dp, _ip, C.POINTER(E3BParams), _dp, C.c_void_p,
                           C.c_void_p]
+    L.orc_e3b_allpairs.argtypes = L.orc_e3b.argtypes
+    L.orc_set_threads.argtypes = [C.c_int]
     L.orc_e3b_brute_energy.argtypes = [C.c_int, _ip, _dp, _dp, _ip, C.POINTER(E3BParams)]
     L.orc_e3b_brute_energy.restype = C.c_double
     L.orc_lj_verlet.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp, _dp, C.c_double, C.c_double,
@@ -380,6 +382,24 @@ def e3b(mol, x, L, params=None, periodic=(1, 1, 1)):
     e3 = np.zeros(1)
     lib().orc_e3b(len(mol), mol.ravel(), x, _d(L), _i(periodic), C.byref(p), f, _ptr(e2), _ptr(e3))
     return f, float(e2[0]), float(e3[0])
+
+
+def e3b_allpairs(mol, x, L, params=None, periodic=(1, 1, 1)):
+    """``e3b`` with the molecule pairs found by an all-pairs scan instead of
+    the oxygen cell grid (same sums in the same order)."""
+    p = params or e3b_params()
+    x = _d(x).reshape(-1, 3)
+    mol = _i(mol).reshape(-1, 4)
+    f = np.zeros((len(x), 3))
+    e2 = np.zeros(1)
+    e3 = np.zeros(1)
+    lib().orc_e3b_allpairs(len(mol), mol.ravel(), x, _d(L), _i(periodic), C.byref(p), f, _ptr(e2), _ptr(e3))
+    return f, float(e2[0]), float(e3[0])
+
+
+def set_threads(n):
+    """OpenMP threads for the following oracle calls."""
+    lib().orc_set_threads(int(n))
 
 
 def e3b_brute_energy(mol, x, L, params=None, periodic=(1, 1, 1)):

@@ -769,10 +769,113 @@ static void orc_e3b_dr(const double *x, int a, int b, const double *L,
         dr[d] = orc_mi(x[3 * a + d] - x[3 * b + d], L[d], invL[d], per[d]);
 }
 
+/* Molecule list of E3B: for each molecule the molecules whose oxygen lies
+ * within r2b of its own, ascending (FixE3B's grid over the oxygen positions,
+ * FixE3B.cu:67-83 and 656-680, with cutoff rf + padding).  A cell grid of
+ * edge >= r2b over the oxygens when every axis is periodic and holds at
+ * least 3 cells; otherwise every pair is scanned.  Rows are sorted so that
+ * the sums over a row run in the same order as an all-pairs scan.  Returns
+ * the CSR offsets (nmol + 1) and fills *nbr (caller frees both). */
+static int orc_cmp_int(const void *a, const void *b) {
+    int x = *(const int *)a, y = *(const int *)b;
+    return (x > y) - (x < y);
+}
+
+static int *orc_e3b_list(int nmol, const int *mol, const double *x, const double *L,
+                         const double *invL, const int *per, double r2b, int **nbr) {
+    const double r2 = r2b * r2b;
+    int nc[3];
+    int cells = 1;
+    for (int d = 0; d < 3; ++d) {
+        nc[d] = (int)floor(L[d] / r2b);
+        cells = cells && per[d] && nc[d] >= 3;
+    }
+    int cap = 64;
+    int *cnt = (int *)calloc((size_t)nmol, sizeof(int));
+    int *tmp = NULL;
+    int *head = NULL, *next = NULL, *cellOf = NULL;
+    if (cells) {
+        const int ncell = nc[0] * nc[1] * nc[2];
+        head = (int *)malloc(sizeof(int) * (size_t)ncell);
+        next = (int *)malloc(sizeof(int) * (size_t)nmol);
+        cellOf = (int *)malloc(sizeof(int) * (size_t)nmol);
+        for (int c = 0; c < ncell; ++c) head[c] = -1;
+        for (int i = nmol - 1; i >= 0; --i) {
+            int cc[3];
+            for (int d = 0; d < 3; ++d) {
+                double u = x[3 * mol[4 * i] + d] * invL[d];
+                u -= floor(u);
+                int k = (int)(u * nc[d]);
+                cc[d] = k >= nc[d] ? nc[d] - 1 : k;
+            }
+            const int c = (cc[0] * nc[1] + cc[1]) * nc[2] + cc[2];
+            cellOf[i] = c;
+            next[i] = head[c];
+            head[c] = i;
+        }
+    }
+    for (;;) {
+        int overflow = 0;
+        tmp = (int *)malloc(sizeof(int) * (size_t)nmol * cap);
+#pragma omp parallel for schedule(dynamic, 32) reduction(| : overflow)
+        for (int i = 0; i < nmol; ++i) {
+            int *row = tmp + (size_t)i * cap;
+            int k = 0;
+            if (cells) {
+                const int c = cellOf[i];
+                const int cx = c / (nc[1] * nc[2]), cy = (c / nc[2]) % nc[1], cz = c % nc[2];
+                for (int ax = -1; ax <= 1; ++ax)
+                    for (int ay = -1; ay <= 1; ++ay)
+                        for (int az = -1; az <= 1; ++az) {
+                            const int nx = (cx + ax + nc[0]) % nc[0], ny = (cy + ay + nc[1]) % nc[1],
+                                      nz = (cz + az + nc[2]) % nc[2];
+                            for (int j = head[(nx * nc[1] + ny) * nc[2] + nz]; j >= 0; j = next[j]) {
+                                if (j == i) continue;
+                                double dr[3];
+                                orc_e3b_dr(x, mol[4 * i], mol[4 * j], L, invL, per, dr);
+                                if (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2] < r2) {
+                                    if (k < cap) row[k] = j;
+                                    ++k;
+                                }
+                            }
+                        }
+            } else {
+                for (int j = 0; j < nmol; ++j) {
+                    if (j == i) continue;
+                    double dr[3];
+                    orc_e3b_dr(x, mol[4 * i], mol[4 * j], L, invL, per, dr);
+                    if (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2] < r2) {
+                        if (k < cap) row[k] = j;
+                        ++k;
+                    }
+                }
+            }
+            cnt[i] = k;
+            if (k > cap) overflow = 1;
+            else qsort(row, (size_t)k, sizeof(int), orc_cmp_int);
+        }
+        if (!overflow) break;
+        free(tmp);
+        cap *= 4;
+    }
+    int *off = (int *)malloc(sizeof(int) * ((size_t)nmol + 1));
+    off[0] = 0;
+    for (int i = 0; i < nmol; ++i) off[i + 1] = off[i] + cnt[i];
+    *nbr = (int *)malloc(sizeof(int) * ((size_t)off[nmol] + 1));
+    for (int i = 0; i < nmol; ++i) memcpy(*nbr + off[i], tmp + (size_t)i * cap, sizeof(int) * (size_t)cnt[i]);
+    free(tmp);
+    free(cnt);
+    free(head);
+    free(next);
+    free(cellOf);
+    return off;
+}
+
 /* E3B forces and energies (FixE3B::compute, FixE3B.cu:85-347, kernels
  * compute_E3B_force_twobody / _threebody, ThreeBodyE3B.h:18-549, force assembly
  * threeBodyPairForce, EvaluatorE3B.h:145-249).  mol holds (O, H1, H2, M) atom
- * indices per molecule.  Pairs are found by an O(M^2) scan.
+ * indices per molecule.  Pairs come from the molecule list (orc_e3b_list);
+ * orc_e3b_allpairs runs the same sums over an O(M^2) scan.
  *
  * Two-body: for every ordered molecule pair with |r_OO| < r2b (the molecule
  * list membership, rebuilt every step in the reference), F_O += dr k2 E2
@@ -786,9 +889,9 @@ static void orc_e3b_dr(const double *x, int a, int b, const double *L,
  * Forces: each f(H_d^h -> O_a) carries the coefficient c = dE/df
  * (the fscale values of threeBodyPairForce) and adds c g to H and -c g to O.
  * f (natoms x 3) is ADDED to.  e2, e3 receive the energies (may be NULL). */
-void orc_e3b(int nmol, const int *mol, const double *x, const double *L,
-             const int *per, const orc_e3b_params *p, double *f, double *e2,
-             double *e3) {
+static void orc_e3b_core(int nmol, const int *mol, const double *x, const double *L,
+                         const int *per, const orc_e3b_params *p, double *f, double *e2,
+                         double *e3, const int *off, const int *nbr) {
     double invL[3] = {1.0 / L[0], 1.0 / L[1], 1.0 / L[2]};
     double *tot = (double *)calloc((size_t)nmol * 3, sizeof(double));
     double r2b2 = p->r2b * p->r2b;
@@ -799,7 +902,9 @@ void orc_e3b(int nmol, const int *mol, const double *x, const double *L,
         int Oi = mol[4 * i], H1i = mol[4 * i + 1], H2i = mol[4 * i + 2];
         double fO[3] = {0, 0, 0};
         double t[3] = {0, 0, 0};
-        for (int j = 0; j < nmol; ++j) {
+        const int jn = off ? off[i + 1] - off[i] : nmol;
+        for (int jj = 0; jj < jn; ++jj) {
+            const int j = off ? nbr[off[i] + jj] : jj;
             if (j == i) continue;
             int Oj = mol[4 * j], H1j = mol[4 * j + 1], H2j = mol[4 * j + 2];
             double dr[3];
@@ -837,7 +942,9 @@ void orc_e3b(int nmol, const int *mol, const double *x, const double *L,
         double xi = tot[3 * i], yi = tot[3 * i + 1], zi = tot[3 * i + 2];
         double FO[3] = {0, 0, 0}, FH1[3] = {0, 0, 0}, FH2[3] = {0, 0, 0};
         double sum_f1f2 = 0.0, sum_DD = 0.0, sum_D2in = 0.0;
-        for (int j = 0; j < nmol; ++j) {
+        const int jn = off ? off[i + 1] - off[i] : nmol;
+        for (int jj = 0; jj < jn; ++jj) {
+            const int j = off ? nbr[off[i] + jj] : jj;
             if (j == i) continue;
             int Oj = mol[4 * j], H1j = mol[4 * j + 1], H2j = mol[4 * j + 2];
             double dr[3];
@@ -882,6 +989,23 @@ void orc_e3b(int nmol, const int *mol, const double *x, const double *L,
     free(tot);
     if (e2) *e2 = etwo;
     if (e3) *e3 = ea + eb + ec;
+}
+
+void orc_e3b(int nmol, const int *mol, const double *x, const double *L,
+             const int *per, const orc_e3b_params *p, double *f, double *e2,
+             double *e3) {
+    double invL[3] = {1.0 / L[0], 1.0 / L[1], 1.0 / L[2]};
+    int *nbr = NULL;
+    int *off = orc_e3b_list(nmol, mol, x, L, invL, per, p->r2b > p->rmax ? p->r2b : p->rmax, &nbr);
+    orc_e3b_core(nmol, mol, x, L, per, p, f, e2, e3, off, nbr);
+    free(off);
+    free(nbr);
+}
+
+void orc_e3b_allpairs(int nmol, const int *mol, const double *x, const double *L,
+                      const int *per, const orc_e3b_params *p, double *f, double *e2,
+                      double *e3) {
+    orc_e3b_core(nmol, mol, x, L, per, p, f, e2, e3, NULL, NULL);
 }
 
 /* Brute-force E3B three-body energy straight from the bond-pair definitions
@@ -1017,6 +1141,11 @@ int orc_lj_verlet(int n, double *x, double *v, double mass, const double *lo,
     free(types);
     free(qs);
     return rebuilds;
+}
+
+/* OpenMP threads of the following calls (the CPU baseline's 1-thread leg). */
+void orc_set_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
 }
 
 int orc_num_threads(void) {

@@ -350,3 +350,17 @@ def test_lj_verlet_energy_conservation(orc):
     # shifted-energy LJ is not force-shifted: small jumps as pairs cross rc
     drift = (et.max() - et.min()) / len(x)
     assert drift < 2e-3
+
+
+@pytest.mark.parametrize("nmol,seed", [(64, 1), (216, 2), (512, 3)])
+def test_e3b_cell_list_equals_all_pairs(orc, nmol, seed):
+    """The oxygen cell grid finds every molecule pair the all-pairs scan
+    finds: forces bit-identical (same pairs summed in the same order), the
+    energies up to the order of the OpenMP reduction."""
+    from md_engine_amd import systems
+
+    b = systems.tip4p2005_box(nmol, seed=seed, min_sep=1.6)
+    f1, e21, e31 = orc.e3b(b["mol"], b["x"], b["L"])
+    f2, e22, e32 = orc.e3b_allpairs(b["mol"], b["x"], b["L"])
+    assert np.array_equal(f1, f2)
+    assert e21 == pytest.approx(e22, rel=1e-13) and e31 == pytest.approx(e32, rel=1e-12)
