@@ -1,30 +1,38 @@
 #!/usr/bin/env python3
 """Benchmark of the per-step force path on one MI355X.
 
-Default workload: TIP4P/2005 + E3B3 water, 8,192 molecules (32,768 sites
-counting M), cubic box at 0.997 g/cm^3 (L = 62.64 A), LJ O-O + Ewald real
-space at rc = 9 A with a 1 A skin, P3M 64^3 order 5, E3B3 (5.0-5.2 A switch,
-two-body O-O correction), rigid molecules (FP64 SETTLE / RATTLE), dt = 2 fs,
-298 K velocity rescaling every 10 steps, 1 bar Berendsen barostat applied
-every step (--barostat N: every N steps, 0: off).  One step = one MD step:
-kick-drift and constraints, skin check and list rebuild when due, LJ + Ewald
-real space, P3M spread/FFT/field/FFT/gather, E3B, kick and constraints, then
-the barostat: molecular virial pressure, centre-of-mass dilation, alpha
-re-solve and Green function for the new box.  The lattice start is
-equilibrated (untimed) first; inputs are resident in HBM before timing.
+Headline workload (the metric's configuration): TIP4P/2005 + E3B3 water,
+8,192 molecules (32,768 sites counting M), cubic box at 0.997 g/cm^3
+(L = 62.64 A), LJ O-O + Ewald real space at rc = 9 A with a 1 A skin, P3M
+64^3 order 5, E3B3 (5.0-5.2 A switch, two-body O-O correction), rigid
+molecules (FP64 SETTLE / RATTLE), dt = 2 fs, 298 K velocity rescaling every
+10 steps, 1 bar Berendsen barostat applied every step.  One step = one MD
+step: kick-drift and constraints, skin check and list rebuild when due, LJ +
+Ewald real space, P3M spread/FFT/field/FFT/gather, E3B, kick and
+constraints, then the barostat (molecular virial pressure, centre-of-mass
+dilation, alpha re-solve and Green function for the new box).  Steps run as
+captured 8-step hipGraphs replayed from a cache (captured during warm-up).
+The lattice start is equilibrated (untimed); inputs are resident in HBM
+before timing.
 
-Modes: md (default; --graph 1 replays captured 8-step hipGraphs with device-
-side rebuild decisions), force (force path at fixed positions, list rebuilt
+The one JSON line holds both halves of the metric, "plain E3B vs MLP
+three-body at equal force RMSE":
+  value            atom-steps/s of the E3B step (sites x steps / s);
+  config.surrogate the same step with the learned three-body term (trained
+                   on the GPU from frames of this run, untimed): atom-steps/s,
+                   ns/day, three-body force RMSE against E3B (kJ/mol/nm) and
+                   the E3B judge's checks every 100 steps;
+  config.parity    force RMSE of the final timed frame against the FP64
+                   oracle, NVE energy drift per ns, list-staleness violations;
+  config.c1 / c2   LJ 32,000 atoms NVE and 4,096-molecule TIP4P/2005 NVT;
+  roofline         the pair kernel's algorithmic bytes / its mean duration
+                   (HIP events on the library stream, eager pass);
+  cpu_baseline     the FP64 oracle of the same step on the host cores, one
+                   thread and all cores, on a bounded sample.
+
+Modes: md (default), force (force path at fixed positions, list rebuilt
 every --rebuild-every steps), batch (48 uneven TIP4P/2005 boxes of 512-4096
-molecules advanced by one captured graph).  --three-body surrogate swaps the
-E3B three-body term for the learned surrogate: it is trained on the GPU from
-frames of the equilibration (untimed), the timed run uses it, and the line
-reports its three-body force RMSE against E3B in kJ/mol/nm.
-
-Prints one JSON line: atom-steps/s (sites x steps / s), ms per step, the
-roofline of the dominant kernel (HIP events on the library stream during the
-timed region) and an FP64 CPU baseline of the same formulas (the oracle,
-OpenMP on the host cores) on a bounded sample.
+molecules in one captured graph, plus the 32-bead q-TIP4P/F ring).
 """
 from __future__ import annotations
 
@@ -42,6 +50,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 E3B3_KJ = (453000.0, 150.0, -1005.0, 1880.0)
 KJ_TO_KCAL = 0.23900573614
+KCAL_A_TO_KJ_NM = 41.84
 BAR_TO_ATM = 1.0 / 1.01325
 
 
@@ -66,7 +75,7 @@ def make_workload(name):
         box = dict(x=x, L=L, lo=lo, q=np.zeros(len(x)), type=np.zeros(len(x), np.int32),
                    mass=np.ones(len(x)), mol=None)
         return dict(box=box, rc=2.5, skin=0.3, sz=None, order=None, e3b=False, dt_fs=None,
-                    desc="LJ fluid FCC 32000 atoms, rho*=0.8442")
+                    desc="LJ fluid FCC 32000 atoms, rho*=0.8442, T*=1.2")
     raise SystemExit(f"unknown config {name}")
 
 
@@ -95,9 +104,9 @@ def setup_engine(w):
     return e, flags
 
 
-def setup_md(e, w):
-    """Rigid TIP4P/2005 MD: Maxwell-Boltzmann velocities at 298 K (seeded),
-    SETTLE geometry, dt = 2 fs."""
+def setup_md(e, w, thermostat="rescale"):
+    """Velocities from the seeded Maxwell-Boltzmann recipe; rigid TIP4P/2005
+    (SETTLE geometry, dt 2 fs) with a 298 K thermostat, or LJ NVE at dt 0.005."""
     from md_engine_amd import systems
 
     box = w["box"]
@@ -114,77 +123,141 @@ def setup_md(e, w):
     e.set_system(box["x"], box["lo"], box["L"], v=v, inv_mass=inv, q=box["q"], types=box["type"])
     e.rigid_set(box["mol"], p["m_O"], p["m_H"], p["r_OH"], p["r_HH"], p["gamma"])
     e.integrator_set(w["dt_fs"], systems.REAL_FTM_TO_V)
-    # 298 K velocity rescaling every 10 steps (FixNVTRescale) during
-    # equilibration and the timed run
-    e.thermostat(True, 298.0, 10, 0.0, systems.REAL_BOLTZ, systems.REAL_MVV_TO_ENG)
+    if thermostat == "rescale":
+        # 298 K velocity rescaling every 10 steps (FixNVTRescale)
+        e.thermostat(True, 298.0, 10, 0.0, systems.REAL_BOLTZ, systems.REAL_MVV_TO_ENG)
+    elif thermostat == "nose-hoover":
+        e.nose_hoover(True, 298.0, 100.0, 10, systems.REAL_BOLTZ, systems.REAL_MVV_TO_ENG)
 
 
-def cpu_baseline(w, rebuild_every, budget_s=12.0):
-    """FP64 oracle of the same step on the host cores (bounded sample)."""
+def timed(e, flags, steps, warmup):
+    """warm-up (captures the step graphs), then exactly `steps` steps between
+    two stream synchronisations; returns the wall time."""
+    e.run(warmup, flags)
+    e.sync()
+    t0 = time.perf_counter()
+    e.run(steps, flags)
+    e.sync()
+    return time.perf_counter() - t0
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(w, x, v, L, rebuild_every, gpu_forces, one_thread_s=12.0, all_core_s=12.0):
+    """FP64 oracle of the same MD step on the host cores, from the GPU run's
+    final frame: kick-drift + SETTLE, LJ + Ewald real space over a cell/Verlet
+    list, P3M, E3B over an oxygen cell grid, kick + RATTLE, with one list
+    build per `rebuild_every` steps (the GPU run's measured rebuild interval).
+    One thread and all threads, each on a bounded number of steps.  The
+    oracle's forces at the frame also give the GPU forces' RMSE against FP64."""
     import oracle
     from md_engine_amd import systems
 
-    box = w["box"]
-    n = len(box["x"])
-    L, lo, rc, skin = box["L"], box["lo"], w["rc"], w["skin"]
-    x = oracle.wrap(box["x"], lo, L)
-    t0 = time.perf_counter()
-    counts, tab = oracle.nlist_cell(x, lo, L, rc + skin)
-    t_build = time.perf_counter() - t0
-    rows = [tab[i, : counts[i]] for i in range(n)]
-    if box["mol"] is not None:
-        excl = oracle.exclusions_bfs(n, systems.water_bonds(box["mol"]))
-        rows = oracle.tag_rows(rows, np.arange(n), excl)
-        eps24, sig6 = systems.water_lj_tables()
-        rc2 = np.full((3, 3), rc * rc)
-        qqr = systems.REAL_QQR_TO_ENG
-        q2 = qqr * (box["q"] ** 2).sum()
-        alpha, _ = oracle.pppm_alpha(n, q2, L, np.array(w["sz"]), w["order"], rc)
-        G = oracle.pppm_green(L, np.array(w["sz"]), alpha, w["order"])
-    else:
-        rows = [r.astype(np.uint32) for r in rows]
-        eps24, sig6, rc2 = np.array([[24.0]]), np.array([[1.0]]), np.array([[rc * rc]])
-    rows = oracle.rows_to_table(counts, rows)
+    box = dict(w["box"], L=np.asarray(L, np.float64))
+    n = len(x)
+    lo, rc, skin = box["lo"], w["rc"], w["skin"]
+    qqr = systems.REAL_QQR_TO_ENG
+    q2 = qqr * (box["q"] ** 2).sum()
+    sz = np.array(w["sz"])
+    eps24, sig6 = systems.water_lj_tables()
+    rc2 = np.full((3, 3), rc * rc)
+    excl = oracle.exclusions_bfs(n, systems.water_bonds(box["mol"]))
+    alpha, _ = oracle.pppm_alpha(n, q2, box["L"], sz, w["order"], rc)
+    G = oracle.pppm_green(box["L"], sz, alpha, w["order"])
+    ftm = systems.REAL_FTM_TO_V
+    nthreads = oracle.num_threads()
 
-    def step():
-        if box["mol"] is not None:
-            oracle.pair(x, box["q"], box["type"], L, counts, rows, rc2, eps24, sig6, with_charge=True,
-                        alpha=alpha, qqr=qqr, rq=rc)
-            oracle.pppm(x, box["q"], lo, L, np.array(w["sz"]), w["order"], qqr, alpha, q2, G)
-            if w["e3b"]:
-                oracle.e3b(box["mol"], x, L)
-        else:
-            oracle.pair(x, box["q"], box["type"], L, counts, rows, rc2, eps24, sig6)
+    def build(xx):
+        xw = oracle.wrap(xx, lo, box["L"])
+        counts, tab = oracle.nlist_cell(xw, lo, box["L"], rc + skin)
+        rows = oracle.tag_rows([tab[i, : counts[i]] for i in range(n)], np.arange(n), excl)
+        return counts, oracle.rows_to_table(counts, rows)
 
-    step()  # warm
-    steps = 0
-    t0 = time.perf_counter()
-    while True:
-        step()
-        steps += 1
-        if time.perf_counter() - t0 > budget_s or steps >= 50:
-            break
-    t_step = (time.perf_counter() - t0) / steps
-    per_step = t_step + (t_build / rebuild_every if rebuild_every else 0.0)
-    return dict(value=n / per_step, unit="atom-steps/s", cores=oracle.num_threads(), kind="port",
-                sample=f"{steps} force-path steps of the same system + 1 list build amortised over "
-                       f"{rebuild_every} steps; FP64 oracle (C, OpenMP); {t_step * 1e3:.1f} ms/step, "
-                       f"build {t_build * 1e3:.1f} ms")
+    def forces(xx, lst):
+        xw = oracle.wrap(xx, lo, box["L"])
+        fp, _, _ = oracle.pair(xw, box["q"], box["type"], box["L"], lst[0], lst[1], rc2, eps24, sig6,
+                               with_charge=True, alpha=alpha, qqr=qqr, rq=rc)
+        fk = oracle.pppm(xx, box["q"], lo, box["L"], sz, w["order"], qqr, alpha, q2, G)["f"]
+        f = fp + fk
+        if w["e3b"]:
+            f = f + oracle.e3b(box["mol"], xx, box["L"])[0]
+        return f
+
+    # parity of the final GPU frame (the oracle as the checker)
+    lst = build(x)
+    f_ref = forces(x, lst)
+    rmse = float(np.sqrt(((gpu_forces - f_ref) ** 2).sum() / (f_ref ** 2).sum()))
+
+    def leg(threads, budget):
+        oracle.set_threads(threads)
+        xx, vv = x.copy(), v.copy()
+        t0 = time.perf_counter()
+        lst = build(xx)
+        t_build = time.perf_counter() - t0
+        f = forces(xx, lst)
+        steps = 0
+        t0 = time.perf_counter()
+        while True:
+            oracle.water_step_half(1, box["mol"], xx, vv, f, w["dt_fs"], ftm, box["L"])
+            f = forces(xx, lst)
+            oracle.water_step_half(2, box["mol"], xx, vv, f, w["dt_fs"], ftm, box["L"])
+            steps += 1
+            if time.perf_counter() - t0 > budget or steps >= 50:
+                break
+        t_step = (time.perf_counter() - t0) / steps
+        per_step = t_step + t_build / max(rebuild_every, 1)
+        return dict(atom_steps_per_s=n / per_step, ms_per_step=per_step * 1e3, steps=steps,
+                    build_ms=t_build * 1e3, threads=threads,
+                    ns_per_day=w["dt_fs"] * 1e-6 / per_step * 86400.0)
+
+    one = leg(1, one_thread_s)
+    allc = leg(nthreads, all_core_s)
+    oracle.set_threads(nthreads)
+    base = dict(value=allc["atom_steps_per_s"], unit="atom-steps/s", cores=nthreads, kind="port",
+                sample=("FP64 oracle (C, OpenMP) of the full MD step of the same 32,768-site system from the GPU "
+                        "run's final frame: kick-drift + SETTLE, LJ + Ewald real over the Verlet list, P3M 64^3, "
+                        "E3B over an oxygen cell grid, kick + RATTLE, one list build per %d steps; %d steps on "
+                        "%d threads (%.0f ms/step), %d steps on 1 thread (%.0f ms/step)"
+                        % (rebuild_every, allc["steps"], nthreads, allc["ms_per_step"], one["steps"],
+                           one["ms_per_step"])),
+                one_thread=one, all_cores=allc, cpu_model=cpu_model(), nproc=os.cpu_count())
+    return base, rmse
 
 
-def copy_probe(mib=256, reps=5):
-    """Plain device-to-device copy of a known size, for scaling the FETCH_SIZE /
-    WRITE_SIZE counters to bytes (profiles/ records the factor)."""
-    import torch
+def nve_drift(e, flags, steps=2000, every=200):
+    """Energy drift of plain NVE from the current state (thermostat and
+    barostat off): slope of the total energy over the segment, per ns and
+    per molecule (kcal/mol), and in kT."""
+    from md_engine_amd import ENERGY, systems
 
-    n = mib * 1024 * 1024 // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(reps):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_bytes_read_per_launch": n * 4, "bytes_written_per_launch": n * 4,
-                      "launches": reps}))
+    kb, mvv = systems.REAL_BOLTZ, systems.REAL_MVV_TO_ENG
+    e.thermostat(False)
+    e.barostat(False)
+
+    def total():
+        e.compute(flags | ENERGY)
+        en, _ = e.energies()
+        _, ke = e.temperature(kb, mvv)
+        return float(en.sum() + ke)
+
+    t, h = [0.0], [total()]
+    for k in range(steps // every):
+        e.run(every, flags)
+        t.append((k + 1) * every)
+        h.append(total())
+    t = np.asarray(t)
+    h = np.asarray(h)
+    slope = np.polyfit(t, h, 1)[0]  # kcal/mol per step
+    return slope, h
 
 
 def train_surrogate(e, w, flags, frames=20, every=100, batch=1024,
@@ -193,7 +266,6 @@ def train_surrogate(e, w, flags, frames=20, every=100, batch=1024,
     and fit the surrogate on the GPU (Adam, mini-batches of `batch`
     molecules, `schedule` = (epochs, learning rate) stages); returns fit
     statistics."""
-    from md_engine_amd import SURROGATE  # noqa: F401
     from md_engine_amd._lib import Engine
 
     nm = len(w["box"]["mol"])
@@ -202,8 +274,7 @@ def train_surrogate(e, w, flags, frames=20, every=100, batch=1024,
         e.run(every, flags)
         e.surrogate_record(reset=(k == 0), capacity=frames * nm)
     rng = np.random.default_rng(7)
-    p = _init_surrogate(rng)
-    e.surrogate_set(p)
+    e.surrogate_set(_init_surrogate(rng))
     l0 = e.surrogate_fit(1, batch=8192, lr=1e-3, seed=1, normalize=True)
     loss = l0
     for k, (ep, lr) in enumerate(schedule):
@@ -229,7 +300,26 @@ def three_body_rmse(e):
     e.compute(SURROGATE)
     fs = e.forces()
     rms = float(np.sqrt(((fs - fe) ** 2).mean()))
-    return rms, rms * 41.84
+    return rms, rms * KCAL_A_TO_KJ_NM
+
+
+def sub_result(config, steps, warmup, equil, thermostat):
+    """atom-steps/s of a secondary configuration (C1, C2)."""
+    w = make_workload(config)
+    e, flags = setup_engine(w)
+    setup_md(e, w, thermostat=thermostat)
+    e.option("graph", 1)
+    if equil:
+        e.run(equil, flags)
+    wall = timed(e, flags, steps, warmup)
+    n = len(w["box"]["x"])
+    dt = w["dt_fs"] if w["dt_fs"] else None
+    out = dict(workload=w["desc"], atom_steps_per_s=n * steps / wall, ms_per_step=wall / steps * 1e3,
+               steps=steps, list_violations=e.nlist_stats()["violations"],
+               ns_per_day=(dt * 1e-6 * steps / wall * 86400.0) if dt else None,
+               tau_per_day=(0.005 * steps / wall * 86400.0) if not dt else None)
+    e.close()
+    return out
 
 
 def batch_workload(seed=20261202, nbox=48):
@@ -267,7 +357,7 @@ def pimd_ring(nbeads=32, nmol=1024, seed=20261203):
 
 
 def run_batch_mode(args):
-    from md_engine_amd import E3B, PAIR, PPPM, run_batch, systems
+    from md_engine_amd import E3B, PAIR, PPPM, run_batch
 
     boxes = batch_workload()
     engines = []
@@ -288,7 +378,6 @@ def run_batch_mode(args):
     for e in engines:
         e.sync()
     wall = time.perf_counter() - t0
-    # the 32-bead PIMD ring, its own captured graphs
     from md_engine_amd._lib import run_ring
 
     beads, rflags, wp, nsite = pimd_ring()
@@ -323,21 +412,46 @@ def run_batch_mode(args):
     print(json.dumps(out))
 
 
+def pair_roofline(e, n, prof):
+    """Algorithmic bytes of one pair launch over its mean duration: per atom
+    xq 16 B + type 4 B + two row counts 8 B, per atom with work a fixed-point
+    force row 24 B, plus 4 B per list entry the kernel reads (charged partners
+    for charged atoms, charge-free partners for atoms with LJ to them; the
+    other part of a row is never touched)."""
+    read_entries, read_atoms = e.pair_entries_read()
+    cnt, tot_ms = prof["pair"]
+    avg_s = tot_ms / max(cnt, 1) * 1e-3
+    bytes_launch = n * (16 + 4 + 8) + 24 * read_atoms + 4 * read_entries
+    achieved = bytes_launch / avg_s / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "pair_traffic_tip4p_e3b_8192.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+    return {
+        "kernel": "k_pair (LJ + Ewald real space)", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+        "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/force_path_tip4p_e3b_8192.md" if traffic else None,
+        "bytes_per_launch": bytes_launch, "avg_launch_ms": avg_s * 1e3, "launches": cnt,
+    }, read_entries
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--config", default="tip4p-e3b", choices=["tip4p-e3b", "tip4p", "lj"])
-    ap.add_argument("--rebuild-every", type=int, default=10,
-                    help="force mode: forced list rebuild interval")
+    ap.add_argument("--rebuild-every", type=int, default=10, help="force mode: forced list rebuild interval")
     ap.add_argument("--mode", default="md", choices=["md", "force", "batch"],
-                    help="md: rigid-water MD steps; force: force path at fixed positions; batch: 48 replicas")
+                    help="md: MD steps; force: force path at fixed positions; batch: 48 replicas + PIMD ring")
     ap.add_argument("--graph", type=int, default=1, help="md mode: 1 = captured hipGraph steps")
-    ap.add_argument("--three-body", default="e3b", choices=["e3b", "surrogate"])
     ap.add_argument("--equil", type=int, default=3000,
                     help="md mode: untimed thermostatted steps before warm-up (lattice start)")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the FP64 parity frame")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the learned three-body leg")
+    ap.add_argument("--no-sub", action="store_true", help="skip the C1 / C2 sub-results")
+    ap.add_argument("--drift-steps", type=int, default=2500, help="NVE segment for the energy drift (0: skip)")
     ap.add_argument("--pair-team", type=int, default=0)
     ap.add_argument("--build-team", type=int, default=0)
     ap.add_argument("--e3b-team", type=int, default=0)
@@ -354,14 +468,14 @@ def main():
         copy_probe()
         return
     if args.gpus != 1:
-        print(f"bench.py runs on exactly one GPU (asked for --gpus {args.gpus}); this build has no "
-              "multi-GPU path", file=sys.stderr)
+        print(f"bench.py runs on exactly one GPU (asked for --gpus {args.gpus}): every GPU box here has one "
+              "GPU, and this build's multi-GPU sharding is a design only", file=sys.stderr)
         sys.exit(2)
     if args.mode == "batch":
         run_batch_mode(args)
         return
 
-    from md_engine_amd import E3B, SURROGATE
+    from md_engine_amd import E3B, SURROGATE, systems
 
     w = make_workload(args.config)
     e, flags = setup_engine(w)
@@ -373,138 +487,147 @@ def main():
     if args.gather_team:
         e.option("gather_team", args.gather_team)
     n = len(w["box"]["x"])
-    R = args.rebuild_every
-    surrogate = args.three_body == "surrogate"
-    if surrogate and not (flags & E3B):
-        raise SystemExit("--three-body surrogate needs the tip4p-e3b config")
-
     md = args.mode == "md"
-    train = None
+    water = w["box"]["mol"] is not None
+    headline = md and args.config == "tip4p-e3b"
     if md:
         setup_md(e, w)
-        if args.graph:
-            e.option("graph", 1)
-        if args.config != "tip4p-e3b":
+        e.option("graph", args.graph)
+        if not headline:
             args.barostat = 0  # the pressure-coupled workload is the E3B water box
         if args.barostat:
-            from md_engine_amd import systems as _sy
-
             # FixPressureBerendsen(state, handle, 1 bar, period 1 ps, applyEvery)
-            e.barostat(True, BAR_TO_ATM, 1000.0, args.barostat, mvv_to_eng=_sy.REAL_MVV_TO_ENG)
-        step = lambda k: e.run(k, flags)  # noqa: E731
+            e.barostat(True, BAR_TO_ATM, 1000.0, args.barostat, mvv_to_eng=systems.REAL_MVV_TO_ENG)
         if args.equil:
-            step(args.equil)
-        if surrogate:
-            train = train_surrogate(e, w, flags)
-            flags = (flags & ~E3B) | SURROGATE
+            e.run(args.equil, flags)
+        step = lambda k: e.run(k, flags)  # noqa: E731
     else:
-        step = lambda k: e.force_steps(k, flags, R)  # noqa: E731
+        step = lambda k: e.force_steps(k, flags, args.rebuild_every)  # noqa: E731
     tuned = e.tune(flags, evals=10) if args.tune else None
+
+    # ---- timed region: the E3B step ------------------------------------
     step(args.warmup)
     e.sync()
-    rebuilds0 = e.nlist_stats()["rebuilds"]
-    e.profile(True)
+    st0 = e.nlist_stats()
     t0 = time.perf_counter()
     step(args.steps)
     e.sync()
-    t1 = time.perf_counter()
-    wall = t1 - t0
-    prof = {k: e.profile_get(k) for k in ("pair", "build", "spread", "gather", "fft", "field", "e3b1", "e3b2")}
-    e.profile(False)
-    timing_pass = "timed region"
+    wall = time.perf_counter() - t0
+    st1 = e.nlist_stats()
+    rebuilds_timed = st1["rebuilds"] - st0["rebuilds"]
+    x_final, v_final = e.get_system()
+    L_final = np.asarray(e.get_box()[1])
+    if headline and not args.no_cpu:
+        e.compute(flags)
+        f_final = e.forces()
+
+    # ---- per-kernel times: an eager pass of the same kernels -------------
+    # (kernels inside a captured graph carry no events; the eager pass runs
+    # right after the timed region from the same state)
+    kernels = ("pair", "build", "spread", "gather", "fft", "field", "e3b1", "e3b2")
+    nprof = min(args.steps, 200)
     if md and args.graph:
-        # kernels inside a captured graph carry no events: time the same
-        # kernels over an eager pass of the same state right after
-        st0 = e.nlist_stats()
         e.option("graph", 0)
-        e.profile(True)
-        nprof = min(args.steps, 200)
-        step(nprof)
-        e.sync()
-        prof = {k: e.profile_get(k) for k in prof}
-        e.profile(False)
-        timing_pass = "eager pass of %d steps right after the timed graph run" % nprof
-        rebuilds0 += e.nlist_stats()["rebuilds"] - st0["rebuilds"]
-    pairs, mols = e.nlist_totals()
-    read_entries, read_atoms = e.pair_entries_read()
-    st = e.nlist_stats()
-    rmse = three_body_rmse(e) if surrogate else None
-
-    ms_step = wall / args.steps * 1e3
-    value = n * args.steps / wall
-    # dominant per-step kernel: pair (LJ + Ewald real space).  Algorithmic
-    # bytes per launch: per atom xq 16 B + type 4 B + two row counts 8 B, per
-    # atom with work a fixed-point force row 24 B, plus 4 B per list entry the
-    # kernel reads (charged partners for charged atoms, charge-free partners for
-    # atoms with LJ to them; the other part of a row is never touched).
-    cnt, tot_ms = prof["pair"]
-    avg_s = tot_ms / max(cnt, 1) * 1e-3
-    bytes_launch = n * (16 + 4 + 8) + 24 * read_atoms + 4 * read_entries
-    achieved = bytes_launch / avg_s / 1e9
+    e.profile(True)
+    step(nprof)
+    e.sync()
+    prof = {k: e.profile_get(k) for k in kernels}
+    e.profile(False)
+    if md and args.graph:
+        e.option("graph", 1)
     kernel_ms = {k: (v[1] / max(v[0], 1)) for k, v in prof.items() if v[0]}
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "pair_traffic_tip4p_e3b_8192.json")
-    if args.config == "tip4p-e3b" and os.path.exists(tpath):
-        with open(tpath) as fh:
-            traffic = json.load(fh).get("bytes_per_launch")
+    roof, read_entries = pair_roofline(e, n, prof)
+    pairs, mols = e.nlist_totals()
 
-    out = {
-        "metric": "atom-steps/s",
-        "value": value,
-        "unit": "atom-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 pair/3-body arithmetic, f64 positions, i64 fixed-point accumulation",
-        "data": "synthetic (seeded lattice, random orientations)",
-        "config": {
-            "workload": w["desc"],
-            "step": ("MD step: velocity Verlet + FP64 SETTLE/RATTLE + M site, skin-driven list rebuilds, "
-                     "LJ+Ewald real, P3M, %s%s%s" % ("E3B two-body + learned three-body" if surrogate else "E3B",
-                                                    (", 1 bar Berendsen barostat every %d steps" % args.barostat)
-                                                    if args.barostat else ", NVT (no barostat)",
-                                                    ", captured hipGraph (8 steps)" if args.graph else "")
-                     if md else "force path: list rebuild every %d steps, LJ+Ewald real, P3M, E3B" % R),
-            "three_body": args.three_body,
-            "surrogate_training": train,
-            "three_body_force_rmse_vs_e3b": ({"kcal_mol_A": rmse[0], "kJ_mol_nm": rmse[1]} if rmse else None),
-            "tuned": tuned,
-            "list_rebuilds_timed": e.nlist_stats()["rebuilds"] - rebuilds0,
-            "list_violations": e.nlist_stats()["violations"],
-            "rc": w["rc"], "skin": w["skin"], "pppm_mesh": w["sz"], "pppm_order": w["order"],
-            "e3b": w["e3b"], "sites": n, "pair_list_entries": pairs, "pair_entries_read": read_entries,
-            "e3b_list_entries": mols,
-            "list_rebuilds_in_run": st["rebuilds"],
-            "ns_per_day_at_dt": (w["dt_fs"] * 1e-6 * args.steps / wall * 86400.0) if w["dt_fs"] else None,
-            "kernel_avg_ms": kernel_ms,
-            "kernel_timing": timing_pass,
-        },
-        "roofline": {
-            "kernel": "k_pair (LJ + Ewald real space)",
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/force_path_tip4p_e3b_8192.md"
-            if traffic else None,
-            "bytes_per_launch": bytes_launch,
-            "avg_launch_ms": avg_s * 1e3,
-            "launches": cnt,
-        },
+    value = n * args.steps / wall
+    dt = w["dt_fs"]
+    config = {
+        "workload": w["desc"],
+        "step": ("MD step: velocity Verlet + FP64 SETTLE/RATTLE + M site, skin-driven list rebuilds, LJ+Ewald "
+                 "real, P3M, %s%s%s" % ("E3B" if w["e3b"] else "no E3B",
+                                        (", 1 bar Berendsen barostat every %d steps" % args.barostat)
+                                        if args.barostat else "",
+                                        ", captured hipGraphs (8 steps) replayed from the cache" if args.graph else "")
+                 if md else "force path: list rebuild every %d steps, LJ+Ewald real, P3M, E3B" % args.rebuild_every),
+        "ns_per_day": (dt * 1e-6 * args.steps / wall * 86400.0) if (md and dt) else None,
+        "tuned": tuned,
+        "list_rebuilds_timed": rebuilds_timed,
+        "rc": w["rc"], "skin": w["skin"], "pppm_mesh": w["sz"], "pppm_order": w["order"], "e3b": w["e3b"],
+        "sites": n, "pair_list_entries": pairs, "pair_entries_read": read_entries, "e3b_list_entries": mols,
+        "kernel_avg_ms": kernel_ms,
+        "kernel_timing": "HIP events on the library stream, eager pass of %d steps right after the timed run" % nprof,
     }
-    if not args.no_cpu:
-        # in md mode the CPU sample amortises one build over the GPU run's
-        # measured rebuild interval
-        rb = out["config"]["list_rebuilds_timed"]
-        out["cpu_baseline"] = cpu_baseline(w, max(1, round(args.steps / rb)) if md and rb else R)
+    out = {
+        "metric": "atom-steps/s", "value": value, "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 pair/3-body arithmetic, f64 positions and integration, i64 fixed-point accumulation",
+        "data": "synthetic (seeded lattice, random orientations)", "config": config, "roofline": roof,
+    }
+
+    if headline:
+        parity = {"list_violations": e.nlist_stats()["violations"]}
+        # ---- the learned three-body leg (trained untimed on this run) -----
+        if not args.no_surrogate:
+            e.option("graph", 1)
+            train = train_surrogate(e, w, flags)
+            sflags = (flags & ~E3B) | SURROGATE
+            e.surrogate_judge(100, 0.5 / KCAL_A_TO_KJ_NM)
+            wall_s = timed(e, sflags, args.steps, args.warmup)
+            judge = e.surrogate_judge_stats()
+            rms = three_body_rmse(e)
+            config["surrogate"] = {
+                "atom_steps_per_s": n * args.steps / wall_s, "ms_per_step": wall_s / args.steps * 1e3,
+                "ns_per_day": dt * 1e-6 * args.steps / wall_s * 86400.0,
+                "speed_vs_e3b": (n * args.steps / wall_s) / value,
+                "three_body_force_rmse_vs_e3b": {"kcal_mol_A": rms[0], "kJ_mol_nm": rms[1], "bar_kJ_mol_nm": 0.5},
+                "e3b_judge": judge, "training": train,
+                "model": "48 symmetry-function inputs -> 64 -> 64 -> 1 SiLU, BF16 MFMA, FP32 accumulation",
+            }
+        # ---- NVE energy drift from the same system (untimed) --------------
+        if args.drift_steps:
+            e.option("graph", 1)
+            slope, h = nve_drift(e, flags, steps=args.drift_steps)
+            nmol = len(w["box"]["mol"])
+            steps_per_ns = 1e6 / dt
+            kT = systems.REAL_BOLTZ * 298.0
+            parity["nve_drift"] = {
+                "kcal_mol_per_ns": slope * steps_per_ns,
+                "kcal_mol_per_ns_per_molecule": slope * steps_per_ns / nmol,
+                "kT_per_ns_per_molecule": slope * steps_per_ns / nmol / kT,
+                "segment_ps": args.drift_steps * dt * 1e-3, "energy_span_kcal_mol": float(np.ptp(h)),
+            }
+        config["parity"] = parity
+    if not args.no_sub and headline:
+        config["c1"] = sub_result("lj", max(args.steps, 100), args.warmup, 0, None)
+        config["c2"] = sub_result("tip4p", args.steps, args.warmup, min(args.equil, 1000), "nose-hoover")
+    if headline and not args.no_cpu:
+        rb = rebuilds_timed
+        base, rmse = cpu_baseline(w, x_final, v_final, L_final, max(1, round(args.steps / rb)) if rb else 10,
+                                  f_final)
+        out["cpu_baseline"] = base
+        config.setdefault("parity", {})["force_rmse_vs_fp64"] = rmse
     e.close()
     print(json.dumps(out))
+
+
+def copy_probe(mib=1024, reps=5):
+    """Plain device-to-device copy of a known size, for scaling the FETCH_SIZE /
+    WRITE_SIZE counters to bytes: 1 GiB (four times the 256 MB Infinity
+    Cache) from a source written by a separate fill and read back before the
+    copies, so neither the cache nor first-touch effects enter the factor."""
+    import torch
+
+    n = mib * 1024 * 1024 // 4
+    a = torch.empty(n, dtype=torch.float32, device="cuda")
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read_per_launch": n * 4, "bytes_written_per_launch": n * 4,
+                      "launches": reps}))
 
 
 if __name__ == "__main__":

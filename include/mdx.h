@@ -219,6 +219,16 @@ int mdx_surrogate_get(mdx_ctx *ctx, float *params, int nparams);
 int mdx_surrogate_record(mdx_ctx *ctx, int reset, int capacity);
 int mdx_surrogate_fit(mdx_ctx *ctx, int epochs, int batch, double lr, int seed, int normalize, double *loss);
 int mdx_surrogate_eval(mdx_ctx *ctx, float *emol, float *feat);
+/* E3B as the judge of the learned term: during mdx_run with MDX_SURROGATE,
+ * every `every` steps (0: never) the analytic E3B three-body forces and the
+ * surrogate's are both evaluated on the current frame; when their RMSE over
+ * atoms and components exceeds max_rmse (force units) the rest of the run,
+ * and later runs, fall back to E3B (MDX_E3B) until the judge is set again.
+ * Steps run in chunks that end on the judge points.  stats: checks so far,
+ * the last and the worst RMSE, whether the run fell back.  No counterpart in
+ * the reference (its FixE3B has only the analytic term). */
+int mdx_surrogate_judge_set(mdx_ctx *ctx, int every, double max_rmse);
+int mdx_surrogate_judge_stats(mdx_ctx *ctx, int64_t *checks, double *last_rmse, double *worst_rmse, int *fell_back);
 /* Gradient of the training loss (mean squared error of the standardised
  * per-molecule energy) over one mini-batch of recorded samples, rows
  * (bstart + s) * mul + add mod nsamp for s < bsize, as mdx_surrogate_fit

@@ -30,6 +30,7 @@ SYMBOLS = [
     "mdx_profile_enable", "mdx_profile_get", "mdx_nlist_totals", "mdx_pair_entries_read", "mdx_option_set", "mdx_rigid_set",
     "mdx_thermostat_set", "mdx_temperature", "mdx_tune", "mdx_run_batch", "mdx_surrogate_set",
     "mdx_surrogate_get", "mdx_surrogate_record", "mdx_surrogate_fit", "mdx_surrogate_eval", "mdx_surrogate_grad",
+    "mdx_surrogate_judge_set", "mdx_surrogate_judge_stats",
     "mdx_e3b_molecule_energies", "mdx_nose_hoover_set", "mdx_nose_hoover_energy", "mdx_nose_hoover_state", "mdx_barostat_set",
     "mdx_pressure", "mdx_barostat_last", "mdx_box_get", "mdx_flexible_water_set", "mdx_bonded_energy_get", "mdx_ring_run",
 ]
@@ -114,6 +115,8 @@ def load():
     L.mdx_surrogate_fit.argtypes = [_vp, c, c, d, c, c, C.POINTER(d)]
     L.mdx_surrogate_eval.argtypes = [_vp, _vp, _vp]
     L.mdx_surrogate_grad.argtypes = [_vp, c, c, C.c_uint32, C.c_uint32, _vp, c]
+    L.mdx_surrogate_judge_set.argtypes = [_vp, c, d]
+    L.mdx_surrogate_judge_stats.argtypes = [_vp, C.POINTER(C.c_int64), C.POINTER(d), C.POINTER(d), C.POINTER(c)]
     L.mdx_e3b_molecule_energies.argtypes = [_vp, _fp]
     L.mdx_nose_hoover_set.argtypes = [_vp, c, d, d, c, d, d]
     L.mdx_nose_hoover_energy.argtypes = [_vp, C.POINTER(d)]
@@ -393,6 +396,20 @@ class Engine:
         self._chk(self._L.mdx_surrogate_grad(self._h, int(bstart), int(bsize), int(mul), int(add), g.ctypes.data,
                                              g.size), "mdx_surrogate_grad")
         return g
+
+    def surrogate_judge(self, every, max_rmse):
+        """E3B judges the learned term every `every` steps of a run; above
+        max_rmse (kcal/mol/A) the run falls back to E3B."""
+        self._chk(self._L.mdx_surrogate_judge_set(self._h, int(every), float(max_rmse)), "mdx_surrogate_judge_set")
+
+    def surrogate_judge_stats(self):
+        n = C.c_int64(0)
+        last = C.c_double(0)
+        worst = C.c_double(0)
+        fb = C.c_int(0)
+        self._chk(self._L.mdx_surrogate_judge_stats(self._h, C.byref(n), C.byref(last), C.byref(worst), C.byref(fb)),
+                  "mdx_surrogate_judge_stats")
+        return dict(checks=n.value, last_rmse=last.value, worst_rmse=worst.value, fell_back=bool(fb.value))
 
     def surrogate_eval(self, nmol):
         e = np.zeros(nmol, np.float32)

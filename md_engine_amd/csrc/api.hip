@@ -438,8 +438,36 @@ void ring_table(const std::vector<Ctx *> &cs, const RingSpec &ring) {
     MDX_HIP(hipStreamSynchronize(cs[0]->stream));
 }
 
+// The E3B judge of a run with the learned three-body term (one context):
+// force terms after a fallback, steps to the next judge point, and the check
+// at a judge point.
+bool judging(const Ctx &c, int fflags) {
+    return (fflags & MDX_SURROGATE) && c.sur.judgeEvery > 0 && c.e3b.on && c.sur.on;
+}
+
+int judged_flags(const Ctx &c, int fflags) {
+    return (c.sur.fellBack && (fflags & MDX_SURROGATE)) ? ((fflags & ~MDX_SURROGATE) | MDX_E3B) : fflags;
+}
+
+int steps_to_judge(const Ctx &c) {
+    const long long e = c.sur.judgeEvery;
+    return (int)(e - c.turn % e);
+}
+
+void judge_if_due(Ctx &c, int fflags) {
+    if (!judging(c, fflags) || c.sur.fellBack || c.turn % c.sur.judgeEvery != 0) return;
+    const double r = surrogate_judge(c);
+    SurrogateState &S = c.sur;
+    ++S.judgeChecks;
+    S.judgeLast = r;
+    S.judgeWorst = std::max(S.judgeWorst, r);
+    if (!(r <= S.judgeMax)) S.fellBack = true;
+}
+
 void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags, const RingSpec *ring = nullptr) {
-    const int fflags = flags & kForceTerms;
+    const int flags0 = flags & kForceTerms;
+    int fflags = judged_flags(*cs[0], flags0);
+    const bool judge = cs.size() == 1 && !ring && judging(*cs[0], flags0);
     for (Ctx *cp : cs) {
         Ctx &c = *cp;
         require_system(c);
@@ -490,7 +518,8 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags, const Rin
     try {
         int done = 0;
         while (done < nsteps) {
-            const int len = std::min(kGraphChunk, nsteps - done);
+            int len = std::min(kGraphChunk, nsteps - done);
+            if (judge && !cs[0]->sur.fellBack) len = std::min(len, steps_to_judge(*cs[0]));
             for (Ctx *cp : cs) {
                 Ctx &c = *cp;
                 snapshot_save(c);
@@ -584,6 +613,10 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags, const Rin
             }
             if (regrid) drop();
             done += len;
+            if (judge) {
+                judge_if_due(*cs[0], flags0);
+                fflags = judged_flags(*cs[0], flags0);  // a fallback recaptures with E3B
+            }
         }
     } catch (...) {
         drop();
@@ -1275,7 +1308,9 @@ int mdx_run(mdx_ctx *ctx, int nsteps, int flags) {
             run_captured({&c}, nsteps, flags);
             return;
         }
-        const int fflags = flags & kForceTerms;
+        const int flags0 = flags & kForceTerms;
+        int fflags = judged_flags(c, flags0);
+        const bool judge = judging(c, flags0);
         prepare_terms(c);
         if (c.rigid.on && c.rigid.fresh) {
             // constrain the starting velocities and place the M sites
@@ -1290,7 +1325,8 @@ int mdx_run(mdx_ctx *ctx, int nsteps, int flags) {
         constexpr int kChunk = 64;
         int done = 0;
         while (done < nsteps) {
-            const int len = std::min(kChunk, nsteps - done);
+            int len = std::min(kChunk, nsteps - done);
+            if (judge && !c.sur.fellBack) len = std::min(len, steps_to_judge(c));
             snapshot_save(c);
             if (!c.listValid || c.forceFlags != fflags) {
                 ensure_list(c);
@@ -1328,6 +1364,10 @@ int mdx_run(mdx_ctx *ctx, int nsteps, int flags) {
                 continue;
             }
             done += len;
+            if (judge) {
+                judge_if_due(c, flags0);
+                fflags = judged_flags(c, flags0);
+            }
         }
         MDX_HIP(hipStreamSynchronize(c.stream));
     });
@@ -1665,6 +1705,39 @@ int mdx_surrogate_grad(mdx_ctx *ctx, int bstart, int bsize, uint32_t mul, uint32
             throw;
         }
         dfree(c, d);
+    });
+}
+
+int mdx_surrogate_judge_set(mdx_ctx *ctx, int every, double max_rmse) {
+    return guarded(ctx, [&](Ctx &c) {
+        require_system(c);
+        MDX_REQUIRE(every >= 0 && (every == 0 || max_rmse > 0), MDX_ERR_ARG, "judge: every >= 0, max_rmse > 0");
+        SurrogateState &S = c.sur;
+        S.judgeEvery = every;
+        S.judgeMax = max_rmse;
+        S.judgeChecks = 0;
+        S.judgeLast = S.judgeWorst = 0.0;
+        S.fellBack = false;
+        if (every > 0 && S.judgeN != c.n) {
+            if (S.fSave) {
+                dfree(c, S.fSave);
+                dfree(c, S.fE3B);
+            }
+            S.fSave = dalloc<unsigned long long>(c, 3 * (size_t)c.n);
+            S.fE3B = dalloc<unsigned long long>(c, 3 * (size_t)c.n);
+            if (!S.judgeSum) S.judgeSum = dalloc<double>(c, 1);
+            S.judgeN = c.n;
+        }
+    });
+}
+
+int mdx_surrogate_judge_stats(mdx_ctx *ctx, int64_t *checks, double *last_rmse, double *worst_rmse, int *fell_back) {
+    return guarded_ro(ctx, [&](Ctx &c) {
+        const SurrogateState &S = c.sur;
+        if (checks) *checks = S.judgeChecks;
+        if (last_rmse) *last_rmse = S.judgeLast;
+        if (worst_rmse) *worst_rmse = S.judgeWorst;
+        if (fell_back) *fell_back = S.fellBack ? 1 : 0;
     });
 }
 

@@ -327,6 +327,17 @@ struct SurrogateState {
     int nsamp = 0, capSamp = 0;
     float *part = nullptr, *m1 = nullptr, *m2 = nullptr, *loss = nullptr;
     long step = 0;
+    // E3B as judge: every judgeEvery steps of a run with MDX_SURROGATE the
+    // three-body forces of both forms are compared on the current frame; an
+    // RMSE above judgeMax makes the rest of the run (and later runs) use E3B
+    int judgeEvery = 0;
+    double judgeMax = 0.0;
+    long long judgeChecks = 0;
+    double judgeLast = 0.0, judgeWorst = 0.0;
+    bool fellBack = false;
+    unsigned long long *fSave = nullptr, *fE3B = nullptr;  // 3 n each
+    double *judgeSum = nullptr;                             // device scalar
+    int judgeN = 0;                                         // atoms the buffers hold
 };
 
 // kernels timed with HIP events when profiling is on (mdx_profile_enable)
@@ -654,6 +665,10 @@ double surrogate_fit(Ctx &c, int epochs, int batch, double lr, unsigned seed);
 // device array out (surrogate_trained_count() + 1 floats)
 void surrogate_grad(Ctx &c, int bstart, int bsize, unsigned mul, unsigned add, float *out);
 int surrogate_trained_count();
+// RMSE over atoms and components between the E3B and the surrogate
+// three-body forces at the current positions; the context's force buffer
+// is left as it was (synchronises)
+double surrogate_judge(Ctx &c);
 void fill_float(Ctx &c, float *p, int count, float v);
 
 // rigid.hip

@@ -871,6 +871,26 @@ __global__ void k_sur_gsum(const float *__restrict__ part, int nwaves, float *__
     out[e] = gsum;
 }
 
+// sum over 3n force components of (a - b)^2 (fixed point to FP64), one
+// workgroup: the lanes' partial sums are combined in a fixed order
+__global__ void __launch_bounds__(256) k_force_sqdiff(int m, const unsigned long long *__restrict__ a,
+                                                      const unsigned long long *__restrict__ b,
+                                                      double *__restrict__ out) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+        const double d = (double)(long long)(a[i] - b[i]) * kFxInv;
+        s += d * d;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
 // per-feature mean and standard deviation of the network inputs over the
 // recorded samples (one block per input column, fixed reduction order)
 __global__ void __launch_bounds__(256) k_sur_norm(int nsamp, const float *__restrict__ raw,
@@ -962,6 +982,27 @@ static void surrogate_train_alloc(Ctx &c) {
 }
 
 int surrogate_trained_count() { return NTRAIN; }
+
+double surrogate_judge(Ctx &c) {
+    SurrogateState &S = c.sur;
+    MDX_REQUIRE(S.fSave && S.judgeN == c.n, 3, "judge buffers not set up");
+    const size_t bytes = sizeof(unsigned long long) * 3 * (size_t)c.n;
+    MDX_HIP(hipMemcpyAsync(S.fSave, c.force, bytes, hipMemcpyDeviceToDevice, c.stream));
+    // both forms share the analytic two-body term: the difference of the
+    // totals is the difference of the three-body forces
+    prepare_forces(c);
+    e3b_forces(c, false, false, true);
+    MDX_HIP(hipMemcpyAsync(S.fE3B, c.force, bytes, hipMemcpyDeviceToDevice, c.stream));
+    prepare_forces(c);
+    e3b_forces(c, false, false, false);
+    surrogate_forces(c, false, false);
+    hipLaunchKernelGGL(k_force_sqdiff, dim3(1), dim3(256), 0, c.stream, 3 * c.n, S.fE3B, c.force, S.judgeSum);
+    MDX_HIP(hipMemcpyAsync(c.force, S.fSave, bytes, hipMemcpyDeviceToDevice, c.stream));
+    double h = 0.0;
+    MDX_HIP(hipMemcpyAsync(&h, S.judgeSum, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    MDX_HIP(hipStreamSynchronize(c.stream));
+    return std::sqrt(h / (3.0 * c.n));
+}
 
 void surrogate_grad(Ctx &c, int bstart, int bsize, unsigned mul, unsigned add, float *out) {
     SurrogateState &S = c.sur;

@@ -178,3 +178,54 @@ def test_training_gradient_vs_oracle(orc, gpu_engine_factory):
         blocks = np.cumsum([0, S.NF * S.NH, S.NH, S.NH * S.NH, S.NH, S.NH, 1])
         for a, b in zip(blocks[:-1], blocks[1:]):
             assert np.linalg.norm(g[a:b] - go[a:b]) <= 3e-2 * np.linalg.norm(go[a:b]) + 1e-6, (a, b)
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+def test_e3b_judge_and_fallback(orc, gpu_engine_factory, graph):
+    """E3B judges the learned term every 100 steps: the checks are counted,
+    the last RMSE is the three-body force RMSE of the frame it saw, and a
+    threshold it cannot meet makes the run continue with E3B exactly as a run
+    that switched to E3B at that step."""
+    say(f"E3B judge and fallback, graph={graph}")
+    from md_engine_amd import E3B, SURROGATE
+
+    box = systems.tip4p2005_box(343, seed=14, min_sep=1.8)
+    v = start_velocities(orc, box, seed=3)
+    raw = S.descriptors(box["mol"], box["x"], box["L"])
+    p = random_model(4, S.features(raw))
+
+    def engine():
+        e, flags = water_md_engine(gpu_engine_factory, box, v, rc=8.0, skin=1.0, sz=(32, 32, 32), order=5,
+                                   e3b=True, dt=1.0)
+        e.surrogate_set(p)
+        e.option("graph", graph)
+        return e, (flags & ~E3B) | SURROGATE
+
+    # generous bar: three checks, no fallback, the last one at step 300
+    e, sflags = engine()
+    e.surrogate_judge(100, 1e9)
+    e.run(300, sflags)
+    st = e.surrogate_judge_stats()
+    e.compute(E3B)
+    fe = e.forces()
+    e.compute(SURROGATE)
+    fs = e.forces()
+    rms = float(np.sqrt(((fs - fe) ** 2).mean()))
+    say(f"judge {st}, host RMSE {rms:.6f}")
+    assert st["checks"] == 3 and not st["fell_back"]
+    assert st["last_rmse"] == pytest.approx(rms, rel=1e-9)
+    e.close()
+    # a bar no model meets: the run falls back at step 100
+    a, sflags = engine()
+    a.surrogate_judge(100, 1e-12)
+    a.run(150, sflags)
+    st = a.surrogate_judge_stats()
+    assert st["checks"] == 1 and st["fell_back"]
+    b, _ = engine()
+    b.run(100, sflags)
+    b.run(50, (sflags & ~SURROGATE) | E3B)
+    xa, va = a.get_system()
+    xb, vb = b.get_system()
+    assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+    a.close()
+    b.close()
