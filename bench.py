@@ -576,12 +576,24 @@ def main():
             wall_s = timed(e, sflags, args.steps, args.warmup)
             judge = e.surrogate_judge_stats()
             rms = three_body_rmse(e)
+            # kernel times of the three-body forms, eager passes from the same state
+            e.option("graph", 0)
+            kt = {}
+            for name, fl in (("e3b", flags), ("surrogate", sflags)):
+                e.profile(True)
+                e.run(min(args.steps, 100), fl)
+                e.sync()
+                kt[name] = {k: v[1] / max(v[0], 1) for k, v in
+                            ((k, e.profile_get(k)) for k in ("e3b1", "e3b2", "sur_feat", "sur_mlp", "sur_force"))
+                            if v[0]}
+                e.profile(False)
+            e.option("graph", 1)
             config["surrogate"] = {
                 "atom_steps_per_s": n * args.steps / wall_s, "ms_per_step": wall_s / args.steps * 1e3,
                 "ns_per_day": dt * 1e-6 * args.steps / wall_s * 86400.0,
                 "speed_vs_e3b": (n * args.steps / wall_s) / value,
                 "three_body_force_rmse_vs_e3b": {"kcal_mol_A": rms[0], "kJ_mol_nm": rms[1], "bar_kJ_mol_nm": 0.5},
-                "e3b_judge": judge, "training": train,
+                "e3b_judge": judge, "training": train, "three_body_kernel_avg_ms": kt,
                 "model": "48 symmetry-function inputs -> 64 -> 64 -> 1 SiLU, BF16 MFMA, FP32 accumulation",
             }
         # ---- NVE energy drift from the same system (untimed) --------------

@@ -295,7 +295,8 @@ int mdx_force_steps(mdx_ctx *ctx, int nsteps, int flags, int rebuild_every);
  * reference, whose only timer is the wall clock of IntegratorVerlet.cu:687,
  * 759-767).  Enabling clears earlier records.  Kernel ids: 0 list build,
  * 1 pair, 2 mesh spread, 3 mesh gather, 4 E3B pass 1, 5 E3B pass 2,
- * 6 FFTs, 7 field. */
+ * 6 FFTs, 7 field, 8 surrogate descriptors, 9 surrogate network, 10
+ * surrogate forces. */
 int mdx_profile_enable(mdx_ctx *ctx, int enable);
 int mdx_profile_get(mdx_ctx *ctx, int kernel, int *count, double *total_ms);
 /* Entries in the current pair list and E3B molecule list (sum of counts). */

@@ -37,7 +37,7 @@ SYMBOLS = [
 
 # kernel ids of mdx_profile_get
 PROF_KERNELS = {"build": 0, "pair": 1, "spread": 2, "gather": 3, "e3b1": 4, "e3b2": 5, "fft": 6,
-                "field": 7}
+                "field": 7, "sur_feat": 8, "sur_mlp": 9, "sur_force": 10}
 
 
 class MdxError(RuntimeError):

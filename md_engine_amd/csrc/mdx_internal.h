@@ -341,7 +341,10 @@ struct SurrogateState {
 };
 
 // kernels timed with HIP events when profiling is on (mdx_profile_enable)
-enum ProfKernel { PK_BUILD = 0, PK_PAIR, PK_SPREAD, PK_GATHER, PK_E3B1, PK_E3B2, PK_FFT, PK_EFIELD, PK_N };
+enum ProfKernel {
+    PK_BUILD = 0, PK_PAIR, PK_SPREAD, PK_GATHER, PK_E3B1, PK_E3B2, PK_FFT, PK_EFIELD,
+    PK_SUR_FEAT, PK_SUR_MLP, PK_SUR_FORCE, PK_N
+};
 
 struct ProfRec {
     int kernel;

@@ -946,9 +946,16 @@ void surrogate_features(Ctx &c, float *raw) {
 void surrogate_forces(Ctx &c, bool energy, bool virial) {
     SurrogateState &S = c.sur;
     E3BState &E = c.e3b;
-    surrogate_features(c, S.feat);
-    hipLaunchKernelGGL(k_sur_infer, dim3(cdiv(E.nmol, 16 * kMlpWaves)), dim3(64 * kMlpWaves), 0, c.stream, E.nmol,
-                       S.params, S.frag, S.feat, S.emol, S.wgt, c.acc, energy ? 1 : 0);
+    {
+        ProfScope ps(c, PK_SUR_FEAT);
+        surrogate_features(c, S.feat);
+    }
+    {
+        ProfScope ps(c, PK_SUR_MLP);
+        hipLaunchKernelGGL(k_sur_infer, dim3(cdiv(E.nmol, 16 * kMlpWaves)), dim3(64 * kMlpWaves), 0, c.stream,
+                           E.nmol, S.params, S.frag, S.feat, S.emol, S.wgt, c.acc, energy ? 1 : 0);
+    }
+    ProfScope ps(c, PK_SUR_FORCE);
     const dim3 grid(cdiv((long long)E.nmol * kFeatLanes, 256));
     if (virial)
         hipLaunchKernelGGL(k_sur_force<true>, grid, dim3(256), 0, c.stream, E.nmol, E.molAtoms, c.xq, c.box, E.cap,
