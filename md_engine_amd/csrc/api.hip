@@ -195,8 +195,7 @@ void compute(Ctx &c, int flags) {
     if ((flags & MDX_PPPM) && c.pppm.on) pppm_forces(c, energy, virial);
     if ((flags & MDX_SURROGATE) && c.e3b.on) {
         MDX_REQUIRE(c.sur.on, MDX_ERR_STATE, "MDX_SURROGATE needs a model (mdx_surrogate_set)");
-        e3b_forces(c, energy, virial, false);  // two-body term stays analytic
-        surrogate_forces(c, energy, virial);
+        surrogate_forces(c, energy, virial);  // the two-body term stays analytic, in the same launch
     } else if ((flags & MDX_E3B) && c.e3b.on) {
         e3b_forces(c, energy, c.vtrace ? false : virial);
     }
@@ -1749,7 +1748,7 @@ int mdx_surrogate_eval(mdx_ctx *ctx, float *emol, float *feat) {
         ensure_list(c);
         refresh_xq(c);
         zero_forces(c, true);
-        surrogate_forces(c, false, false);
+        surrogate_forces(c, false, false, feat != nullptr);
         const size_t m = c.e3b.nmol;
         if (emol) MDX_HIP(hipMemcpyAsync(emol, c.sur.emol, sizeof(float) * m, hipMemcpyDeviceToHost, c.stream));
         if (feat)

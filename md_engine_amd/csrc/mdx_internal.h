@@ -658,7 +658,10 @@ void e3b_forces(Ctx &c, bool energy, bool virial, bool three_body = true, float 
 
 // surrogate.hip
 void surrogate_features(Ctx &c, float *raw);
-void surrogate_forces(Ctx &c, bool energy, bool virial);
+// the learned three-body term with the analytic E3B two-body term (one
+// fused descriptor + network launch and the force pass); with_feat also
+// writes the raw channels
+void surrogate_forces(Ctx &c, bool energy, bool virial, bool with_feat = false);
 void surrogate_normalize(Ctx &c);
 // BF16 weight fragments of the inference kernel from the FP32 parameters
 void surrogate_pack(Ctx &c);

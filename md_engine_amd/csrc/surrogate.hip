@@ -556,6 +556,257 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_sur_infer(int nmol, const fl
 }
 
 // ---------------------------------------------------------------------------
+// Descriptors, the analytic E3B two-body term and the network in one launch.
+// A workgroup of 256 lanes takes 16 molecules, 16 lanes each: every lane
+// walks a share of its molecule's list, sums the 45 raw channels and the
+// two-body force on the oxygen (F = d k2 E2 exp(-k2 r) / r for r_OO < r2b,
+// the same term as the E3B pass, EvaluatorE3B.h:67-87); the lanes' rows are
+// reduced through LDS, and the first wave runs the 16 molecules through the
+// network as one MFMA column tile (forward, then the backward pass to
+// dE / d(raw channel)), with the BF16 weight fragments read straight from the
+// packed table.  Writes the per-molecule energies and channel weights for the
+// force pass (and the raw channels when `feat` is given).
+struct TwoBody {
+    float E2, k2, r2b2;
+};
+
+template <bool EV, bool VIR>
+__global__ void __launch_bounds__(256) k_sur_fused(int nmol, const int4 *__restrict__ molAtoms,
+                                                   const float4 *__restrict__ xq, const BoxDev *__restrict__ bx,
+                                                   int cap, const int *__restrict__ nl, const int *__restrict__ nlCount,
+                                                   TwoBody tb, const float *__restrict__ P,
+                                                   const unsigned short *__restrict__ frag,
+                                                   float *__restrict__ feat, float *__restrict__ emol,
+                                                   float *__restrict__ wgt, unsigned long long *__restrict__ force,
+                                                   unsigned long long *__restrict__ acc) {
+    const Geo3 g = geo3(bx);
+    __shared__ float part[256 / kFeatLanes][kFeatLanes][NR + 1];
+    __shared__ float sdx[16][NF + 1];
+    __shared__ double red[4][7];
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = gt / kFeatLanes, tl = threadIdx.x % kFeatLanes, team = threadIdx.x / kFeatLanes;
+    const int m0 = blockIdx.x * (256 / kFeatLanes);
+    FeatAcc A;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) A.acc[k] = A.d1[k] = A.d2[k] = 0.f;
+    A.pAA = A.pBB = A.pAB = 0.f;
+    float fx = 0.f, fy = 0.f, fz = 0.f, e2 = 0.f;
+    float w0 = 0.f, w1 = 0.f, w2 = 0.f, w3 = 0.f, w4 = 0.f, w5 = 0.f;
+    int4 ai = make_int4(0, 0, 0, 0);
+    if (i < nmol) {
+        ai = molAtoms[i];
+        const float4 Oi = xq[ai.x], H1i = xq[ai.y], H2i = xq[ai.z];
+        const int cnt = min(nlCount[i], cap);
+        const int *row = nl + (size_t)i * cap;
+        for (int t = tl; t < cnt; t += 2 * kFeatLanes) {
+            const bool two = t + kFeatLanes < cnt;
+            const int mA = row[t], mB = two ? row[t + kFeatLanes] : mA;
+            const int4 aA = molAtoms[mA], aB = molAtoms[mB];
+            const float4 OA = xq[aA.x], H1A = xq[aA.y], H2A = xq[aA.z];
+            const float4 OB = xq[aB.x], H1B = xq[aB.y], H2B = xq[aB.z];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                if (u == 1 && !two) break;
+                const float4 Oj = u ? OB : OA;
+                const float3 oo = mi(Oj, Oi, g);
+                const float r2 = oo.x * oo.x + oo.y * oo.y + oo.z * oo.z;
+                if (r2 < tb.r2b2) {
+                    const float r = sqrtf(r2);
+                    const float ex = __expf(-tb.k2 * r);
+                    const float sc = tb.k2 * tb.E2 * ex / r;
+                    // d = r_Oi - r_Oj = -oo: a repulsion along d
+                    fx -= oo.x * sc;
+                    fy -= oo.y * sc;
+                    fz -= oo.z * sc;
+                    if (EV) e2 += 0.5f * tb.E2 * ex;
+                    if (VIR) {
+                        w0 += oo.x * sc * oo.x;
+                        w1 += oo.y * sc * oo.y;
+                        w2 += oo.z * sc * oo.z;
+                        w3 += oo.x * sc * oo.y;
+                        w4 += oo.x * sc * oo.z;
+                        w5 += oo.y * sc * oo.z;
+                    }
+                }
+                if (u == 0) feat_pair(A, Oi, H1i, H2i, OA, H1A, H2A, g);
+                else feat_pair(A, Oi, H1i, H2i, OB, H1B, H2B, g);
+            }
+        }
+    }
+    fx = team_sum<kFeatLanes>(fx);
+    fy = team_sum<kFeatLanes>(fy);
+    fz = team_sum<kFeatLanes>(fz);
+    if (tl == 0 && i < nmol) {
+        fx_add(&force[3 * ai.x + 0], fx);
+        fx_add(&force[3 * ai.x + 1], fy);
+        fx_add(&force[3 * ai.x + 2], fz);
+    }
+    float *pr = part[team][tl];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        pr[k] = A.acc[k];
+        pr[RD1 + k] = A.d1[k];
+        pr[RD2 + k] = A.d2[k];
+    }
+    pr[RAA] = A.pAA;
+    pr[RBB] = A.pBB;
+    pr[RAB] = A.pAB;
+    if (EV || VIR) {
+        // two-body energy and virial: wave sums, one atomic per workgroup and value
+        const double v[7] = {e2, 0.5 * w0, 0.5 * w1, 0.5 * w2, 0.5 * w3, 0.5 * w4, 0.5 * w5};
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+        for (int s2 = 0; s2 < 7; ++s2) {
+            const double x = team_sum_d<64>(v[s2]);
+            if (lane == 0) red[w][s2] = x;
+        }
+    }
+    __syncthreads();
+    if (EV || VIR) {
+        if (threadIdx.x < 7) {
+            double x = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+            if (x != 0.0) fx_add(&acc_rep(acc)[threadIdx.x == 0 ? ACC_E3B2 : ACC_VIR + threadIdx.x - 1], x);
+        }
+    }
+    // the 16 molecules' channel sums: row 0 of each team's slab
+    for (int e = threadIdx.x; e < 16 * NR; e += 256) {
+        const int s = e / NR, col = e % NR;
+        float sum = 0.f;
+        for (int l = 1; l < kFeatLanes; ++l) sum += part[s][l][col];
+        // lane 0's own partial stays in place; add the others after reading all
+        part[s][0][col] += sum;
+        if (feat && m0 + s < nmol) feat[(size_t)(m0 + s) * NR + col] = part[s][0][col];
+    }
+    __syncthreads();
+    if (threadIdx.x >= 64) return;  // the network runs in the first wave; no barrier follows
+    const int l = threadIdx.x, r = l & 15, q = l >> 4;
+    const int nrow = max(0, min(16, nmol - m0));
+    if (nrow == 0) return;
+    const float *fm = part[min(r, nrow - 1)][0];
+    const uint2 *fr2 = (const uint2 *)frag;
+    auto ldf = [&](int f) {
+        const uint2 u = fr2[f * 64 + l];
+        bf16x4 a;
+        a[0] = (short)(u.x & 0xFFFFu);
+        a[1] = (short)(u.x >> 16);
+        a[2] = (short)(u.y & 0xFFFFu);
+        a[3] = (short)(u.y >> 16);
+        return a;
+    };
+    bf16x4 xb[3];
+#pragma unroll
+    for (int kb = 0; kb < 3; ++kb) {
+        f32x4 v;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int col = 16 * kb + 4 * q + j;
+            v[j] = (feature_of(fm, col) - P[OMU + col]) / P[OSD + col];
+        }
+        xb[kb] = pack4(v);
+    }
+    f32x4 z1[4];
+    bf16x4 h1[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        f32x4 c = zero4();
+#pragma unroll
+        for (int kb = 0; kb < 3; ++kb)
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ldf(FR_A1 + 3 * t + kb), xb[kb], c, 0, 0, 0);
+        f32x4 h;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            c[j] += P[OB1 + 16 * t + 4 * q + j];
+            h[j] = silu(c[j]);
+        }
+        z1[t] = c;
+        h1[t] = pack4(h);
+    }
+    f32x4 z2[4];
+    float outp = 0.f;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        f32x4 c = zero4();
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb)
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ldf(FR_A2 + 4 * t + kb), h1[kb], c, 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            c[j] += P[OB2 + 16 * t + 4 * q + j];
+            outp += bf16_round(silu(c[j])) * bf16_round(P[OW3 + 16 * t + 4 * q + j]);
+        }
+        z2[t] = c;
+    }
+    outp += __shfl_xor(outp, 16);
+    outp += __shfl_xor(outp, 32);
+    const float es = P[OES];
+    const float eo = es * (outp + P[OB3]) + P[OEH];
+    double esum = 0.0;
+    if (q == 0 && r < nrow) {
+        emol[m0 + r] = eo;
+        esum = (double)eo;
+    }
+    bf16x4 dz2[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        f32x4 d;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) d[j] = es * P[OW3 + 16 * t + 4 * q + j] * dsilu(z2[t][j]);
+        dz2[t] = pack4(d);
+    }
+    bf16x4 dz1[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        f32x4 c = zero4();
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb)
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ldf(FR_A3 + 4 * t + kb), dz2[kb], c, 0, 0, 0);
+        f32x4 d;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) d[j] = c[j] * dsilu(z1[t][j]);
+        dz1[t] = pack4(d);
+    }
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        f32x4 c = zero4();
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb)
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ldf(FR_A4 + 4 * t + kb), dz1[kb], c, 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int col = 16 * t + 4 * q + j;
+            sdx[r][col] = c[j] / P[OSD + col];
+        }
+    }
+    wave_sync();
+    for (int e = l; e < 16 * NB; e += 64) {
+        const int s = e / NB, k = e % NB;
+        if (s >= nrow) continue;
+        const float *f = part[s][0];
+        const float *gx = sdx[s];
+        const float d1 = f[RD1 + k], d2 = f[RD2 + k];
+        float ga = gx[k], gs = gx[NB + k];
+        const float gp = gx[2 * NB + k];
+        if (k == KE) {
+            const float x = f[KE], sk = f[RD1 + KE] + f[RD2 + KE];
+            ga += gx[3 * NB + 3] * sk + 2.0f * gx[3 * NB + 4] * x;
+            gs += gx[3 * NB + 3] * x + 2.0f * gx[3 * NB + 5] * sk;
+        }
+        float *o = wgt + (size_t)(m0 + s) * NW;
+        o[k] = ga;
+        o[RD1 + k] = gs + gp * d2;
+        o[RD2 + k] = gs + gp * d1;
+    }
+    if (l < 48) {
+        const int s = l / 3, c3 = l % 3;
+        if (s < nrow) wgt[(size_t)(m0 + s) * NW + RAA + c3] = sdx[s][3 * NB + c3];
+    }
+    if (EV) {
+        esum = team_sum_d<64>(esum);
+        if (l == 0 && esum != 0.0) fx_add(&acc_rep(acc)[ACC_E3B3], esum);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // forces: 16 lanes per molecule split its neighbours (owner-computes); each
 // lane holds the molecule's 45 raw-channel weights, gathers the neighbour's,
 // and the lanes' partial forces are summed by shuffles.  A bond H -> O of the
@@ -632,6 +883,8 @@ __device__ __forceinline__ void force_pair(float4 Oi, float4 H1i, float4 H2i, fl
     }
 }
 
+constexpr int kForceLanes = 32;  // lanes per molecule in the force pass
+
 template <bool VIR>
 __global__ void __launch_bounds__(256) k_sur_force(int nmol, const int4 *__restrict__ molAtoms,
                                                    const float4 *__restrict__ xq, const BoxDev *__restrict__ bx,
@@ -639,66 +892,61 @@ __global__ void __launch_bounds__(256) k_sur_force(int nmol, const int4 *__restr
                                                    const int *__restrict__ nlCount, const float *__restrict__ wgt,
                                                    unsigned long long *__restrict__ force,
                                                    unsigned long long *__restrict__ acc) {
+    // the molecule's own channel weights are shared by its lanes through LDS;
+    // a neighbour's are fetched only once its O-O distance passed the test
+    constexpr int M = 256 / kForceLanes;
+    __shared__ float4 swi[M][NW / 4];
     const Geo3 g = geo3(bx);
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = gt / kFeatLanes, tl = threadIdx.x % kFeatLanes;
+    const int i = gt / kForceLanes, tl = threadIdx.x % kForceLanes, team = threadIdx.x / kForceLanes;
+    if (tl < NW / 4) swi[team][tl] = i < nmol ? ((const float4 *)(wgt + (size_t)i * NW))[tl] : make_float4(0, 0, 0, 0);
+    __syncthreads();
+    float wi[NW];
+#pragma unroll
+    for (int v = 0; v < NW / 4; ++v) {
+        const float4 t4 = swi[team][v];
+        wi[4 * v] = t4.x;
+        wi[4 * v + 1] = t4.y;
+        wi[4 * v + 2] = t4.z;
+        wi[4 * v + 3] = t4.w;
+    }
     float3 FO = make_float3(0.f, 0.f, 0.f), F1 = FO, F2 = FO;
     float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
     int4 ai = make_int4(0, 0, 0, 0);
     if (i < nmol) {
         ai = molAtoms[i];
         const float4 Oi = xq[ai.x], H1i = xq[ai.y], H2i = xq[ai.z];
-        float wi[NW];
-        {
-            const float4 *w4 = (const float4 *)(wgt + (size_t)i * NW);
+        const int cnt = min(nlCount[i], cap);
+        const int *row = nl + (size_t)i * cap;
+        for (int t = tl; t < cnt; t += kForceLanes) {
+            const int mj = row[t];
+            const int4 aj = molAtoms[mj];
+            const float4 Oj = xq[aj.x];
+            const float3 oo = mi(Oj, Oi, g);
+            if (oo.x * oo.x + oo.y * oo.y + oo.z * oo.z > kSkipOO2) continue;
+            const float4 H1j = xq[aj.y], H2j = xq[aj.z];
+            float wj[NW];
+            const float4 *w4 = (const float4 *)(wgt + (size_t)mj * NW);
 #pragma unroll
             for (int v = 0; v < NW / 4; ++v) {
                 const float4 t4 = w4[v];
-                wi[4 * v] = t4.x;
-                wi[4 * v + 1] = t4.y;
-                wi[4 * v + 2] = t4.z;
-                wi[4 * v + 3] = t4.w;
+                wj[4 * v] = t4.x;
+                wj[4 * v + 1] = t4.y;
+                wj[4 * v + 2] = t4.z;
+                wj[4 * v + 3] = t4.w;
             }
-        }
-        const int cnt = min(nlCount[i], cap);
-        const int *row = nl + (size_t)i * cap;
-        for (int t = tl; t < cnt; t += 2 * kFeatLanes) {
-            const bool two = t + kFeatLanes < cnt;
-            const int mA = row[t], mB = two ? row[t + kFeatLanes] : mA;
-            // both neighbours' weights and atoms are loaded at once
-            float wA[NW], wB[NW];
-            {
-                const float4 *a4 = (const float4 *)(wgt + (size_t)mA * NW);
-                const float4 *b4 = (const float4 *)(wgt + (size_t)mB * NW);
-#pragma unroll
-                for (int v = 0; v < NW / 4; ++v) {
-                    const float4 ta = a4[v], tb = b4[v];
-                    wA[4 * v] = ta.x;
-                    wA[4 * v + 1] = ta.y;
-                    wA[4 * v + 2] = ta.z;
-                    wA[4 * v + 3] = ta.w;
-                    wB[4 * v] = tb.x;
-                    wB[4 * v + 1] = tb.y;
-                    wB[4 * v + 2] = tb.z;
-                    wB[4 * v + 3] = tb.w;
-                }
-            }
-            const int4 aA = molAtoms[mA], aB = molAtoms[mB];
-            const float4 OA = xq[aA.x], H1A = xq[aA.y], H2A = xq[aA.z];
-            const float4 OB = xq[aB.x], H1B = xq[aB.y], H2B = xq[aB.z];
-            force_pair<VIR>(Oi, H1i, H2i, OA, H1A, H2A, wi, wA, g, FO, F1, F2, v0, v1, v2, v3, v4, v5);
-            if (two) force_pair<VIR>(Oi, H1i, H2i, OB, H1B, H2B, wi, wB, g, FO, F1, F2, v0, v1, v2, v3, v4, v5);
+            force_pair<VIR>(Oi, H1i, H2i, Oj, H1j, H2j, wi, wj, g, FO, F1, F2, v0, v1, v2, v3, v4, v5);
         }
     }
-    FO.x = team_sum<kFeatLanes>(FO.x);
-    FO.y = team_sum<kFeatLanes>(FO.y);
-    FO.z = team_sum<kFeatLanes>(FO.z);
-    F1.x = team_sum<kFeatLanes>(F1.x);
-    F1.y = team_sum<kFeatLanes>(F1.y);
-    F1.z = team_sum<kFeatLanes>(F1.z);
-    F2.x = team_sum<kFeatLanes>(F2.x);
-    F2.y = team_sum<kFeatLanes>(F2.y);
-    F2.z = team_sum<kFeatLanes>(F2.z);
+    FO.x = team_sum<kForceLanes>(FO.x);
+    FO.y = team_sum<kForceLanes>(FO.y);
+    FO.z = team_sum<kForceLanes>(FO.z);
+    F1.x = team_sum<kForceLanes>(F1.x);
+    F1.y = team_sum<kForceLanes>(F1.y);
+    F1.z = team_sum<kForceLanes>(F1.z);
+    F2.x = team_sum<kForceLanes>(F2.x);
+    F2.y = team_sum<kForceLanes>(F2.y);
+    F2.z = team_sum<kForceLanes>(F2.z);
     if (tl == 0 && i < nmol) {
         fx_add(&force[3 * ai.x + 0], FO.x);
         fx_add(&force[3 * ai.x + 1], FO.y);
@@ -943,20 +1191,25 @@ void surrogate_features(Ctx &c, float *raw) {
                        E.molAtoms, c.xq, c.box, E.cap, E.nl, E.nlCount, raw);
 }
 
-void surrogate_forces(Ctx &c, bool energy, bool virial) {
+void surrogate_forces(Ctx &c, bool energy, bool virial, bool with_feat) {
     SurrogateState &S = c.sur;
     E3BState &E = c.e3b;
     {
         ProfScope ps(c, PK_SUR_FEAT);
-        surrogate_features(c, S.feat);
-    }
-    {
-        ProfScope ps(c, PK_SUR_MLP);
-        hipLaunchKernelGGL(k_sur_infer, dim3(cdiv(E.nmol, 16 * kMlpWaves)), dim3(64 * kMlpWaves), 0, c.stream,
-                           E.nmol, S.params, S.frag, S.feat, S.emol, S.wgt, c.acc, energy ? 1 : 0);
+        const TwoBody tb{E.E2, E.k2, E.r2b * E.r2b};
+        const dim3 g1(cdiv((long long)E.nmol * kFeatLanes, 256));
+        float *feat = with_feat ? S.feat : nullptr;
+#define MDX_FUSED(EV, VI)                                                                                     \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sur_fused<EV, VI>), g1, dim3(256), 0, c.stream, E.nmol, E.molAtoms, c.xq, \
+                       c.box, E.cap, E.nl, E.nlCount, tb, S.params, S.frag, feat, S.emol, S.wgt, c.force, c.acc)
+        if (energy && virial) MDX_FUSED(true, true);
+        else if (energy) MDX_FUSED(true, false);
+        else if (virial) MDX_FUSED(false, true);
+        else MDX_FUSED(false, false);
+#undef MDX_FUSED
     }
     ProfScope ps(c, PK_SUR_FORCE);
-    const dim3 grid(cdiv((long long)E.nmol * kFeatLanes, 256));
+    const dim3 grid(cdiv((long long)E.nmol * kForceLanes, 256));
     if (virial)
         hipLaunchKernelGGL(k_sur_force<true>, grid, dim3(256), 0, c.stream, E.nmol, E.molAtoms, c.xq, c.box, E.cap,
                            E.nl, E.nlCount, S.wgt, c.force, c.acc);
@@ -1001,7 +1254,6 @@ double surrogate_judge(Ctx &c) {
     e3b_forces(c, false, false, true);
     MDX_HIP(hipMemcpyAsync(S.fE3B, c.force, bytes, hipMemcpyDeviceToDevice, c.stream));
     prepare_forces(c);
-    e3b_forces(c, false, false, false);
     surrogate_forces(c, false, false);
     hipLaunchKernelGGL(k_force_sqdiff, dim3(1), dim3(256), 0, c.stream, 3 * c.n, S.fE3B, c.force, S.judgeSum);
     MDX_HIP(hipMemcpyAsync(c.force, S.fSave, bytes, hipMemcpyDeviceToDevice, c.stream));
