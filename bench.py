@@ -152,7 +152,7 @@ def cpu_model():
     return None
 
 
-def cpu_baseline(w, x, v, L, rebuild_every, gpu_forces, one_thread_s=12.0, all_core_s=12.0):
+def cpu_baseline(w, x, v, lo, L, rebuild_every, gpu_forces, one_thread_s=12.0, all_core_s=12.0):
     """FP64 oracle of the same MD step on the host cores, from the GPU run's
     final frame: kick-drift + SETTLE, LJ + Ewald real space over a cell/Verlet
     list, P3M, E3B over an oxygen cell grid, kick + RATTLE, with one list
@@ -162,11 +162,12 @@ def cpu_baseline(w, x, v, L, rebuild_every, gpu_forces, one_thread_s=12.0, all_c
     import oracle
     from md_engine_amd import systems
 
-    box = dict(w["box"], L=np.asarray(L, np.float64))
+    # the barostat moved the box (it dilates about the centre: lo moves too)
+    box = dict(w["box"], lo=np.asarray(lo, np.float64), L=np.asarray(L, np.float64))
     n = len(x)
     lo, rc, skin = box["lo"], w["rc"], w["skin"]
     qqr = systems.REAL_QQR_TO_ENG
-    q2 = qqr * (box["q"] ** 2).sum()
+    q2 = oracle.total_q2(box["q"], qqr)
     sz = np.array(w["sz"])
     eps24, sig6 = systems.water_lj_tables()
     rc2 = np.full((3, 3), rc * rc)
@@ -516,7 +517,7 @@ def main():
     st1 = e.nlist_stats()
     rebuilds_timed = st1["rebuilds"] - st0["rebuilds"]
     x_final, v_final = e.get_system()
-    L_final = np.asarray(e.get_box()[1])
+    lo_final, L_final = (np.asarray(a) for a in e.get_box())
     if headline and not args.no_cpu:
         e.compute(flags)
         f_final = e.forces()
@@ -615,8 +616,8 @@ def main():
         config["c2"] = sub_result("tip4p", args.steps, args.warmup, min(args.equil, 1000), "nose-hoover")
     if headline and not args.no_cpu:
         rb = rebuilds_timed
-        base, rmse = cpu_baseline(w, x_final, v_final, L_final, max(1, round(args.steps / rb)) if rb else 10,
-                                  f_final)
+        base, rmse = cpu_baseline(w, x_final, v_final, lo_final, L_final,
+                                  max(1, round(args.steps / rb)) if rb else 10, f_final)
         out["cpu_baseline"] = base
         config.setdefault("parity", {})["force_rmse_vs_fp64"] = rmse
     e.close()

@@ -1043,11 +1043,10 @@ int mdx_pppm_set(mdx_ctx *ctx, int nx, int ny, int nz, int order, double r_cut, 
         c.pppm.qqr = qqr;
         c.pppm.alphaUser = alpha;
         if (c.n > 0) {
-            std::vector<float> hq(c.n);
-            MDX_HIP(hipMemcpyAsync(hq.data(), c.q[c.cur], sizeof(float) * c.n, hipMemcpyDeviceToHost, c.stream));
-            MDX_HIP(hipStreamSynchronize(c.stream));
+            // qqr sum q^2 over the stored (FP32) charges in FP64, in id order
+            // (setTotalQ2, FixChargeEwald.cu:903-915)
             double s = 0.0;
-            for (int i = 0; i < c.n; ++i) s += (double)hq[i] * hq[i];
+            for (int i = 0; i < c.n; ++i) s += (double)c.qHost[i] * c.qHost[i];
             c.pppm.totalQ2 = qqr * s;
         }
     });

@@ -285,6 +285,15 @@ def wp(order, i, d):
     return lib().orc_wp(int(order), int(i), float(d))
 
 
+def total_q2(q, qqr):
+    """qqr * sum q^2 (setTotalQ2, FixChargeEwald.cu:903-915) over the charges
+    as the device stores them (FP32), summed in FP64 in atom-id order: the
+    same value the library computes (the reference sums the FP32 squares on
+    the GPU in FP32, in no fixed order)."""
+    q32 = np.asarray(q, np.float64).astype(np.float32).astype(np.float64)
+    return float(qqr * np.cumsum(q32 * q32)[-1]) if len(q32) else 0.0
+
+
 def pppm_alpha(natoms, total_q2, L, sz, order, rcut):
     """Secant solve for the Ewald splitting parameter (FixChargeEwald.cu:853-967)."""
     err = C.c_double(0.0)
@@ -490,7 +499,7 @@ def water_forces(box, x, rc, skin, sz, order, e3b_on, alpha=None):
     n = len(x)
     L, lo = box["L"], box["lo"]
     qqr = systems.REAL_QQR_TO_ENG
-    q2 = qqr * (box["q"] ** 2).sum()
+    q2 = total_q2(box["q"], qqr)
     if alpha is None:
         alpha, _ = pppm_alpha(n, q2, L, np.array(sz), order, rc)
     xw = wrap(x, lo, L)

@@ -105,7 +105,7 @@ def water_state_parity(orc, e, box, flags, e3b, rc=9.0, skin=1.0, sz=(64, 64, 64
     entries = assert_same_lists(e, oc, orows)
     n = len(x)
     qqr = systems.REAL_QQR_TO_ENG
-    q2 = qqr * (box["q"] ** 2).sum()
+    q2 = orc.total_q2(box["q"], qqr)
     alpha, _ = orc.pppm_alpha(n, q2, st["L"], np.array(sz), 5, rc)
     a_gpu, _ = e.pppm_info()
     assert a_gpu == alpha

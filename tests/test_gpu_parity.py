@@ -224,7 +224,7 @@ def e3b_param_vector():
 
 def oracle_water(orc, box, rc=9.0, skin=1.0, sz=(32, 32, 32), order=5):
     n = len(box["x"])
-    q2 = systems.REAL_QQR_TO_ENG * (box["q"] ** 2).sum()
+    q2 = orc.total_q2(box["q"], systems.REAL_QQR_TO_ENG)
     alpha, err = orc.pppm_alpha(n, q2, box["L"], np.array(sz), order, rc)
     excl = orc.exclusions_bfs(n, systems.water_bonds(box["mol"]))
     oc, orows, xw = oracle_rows_by_id(orc, box["x"], box["lo"], box["L"], rc + skin, excl=excl,
@@ -243,7 +243,7 @@ def test_water_alpha_bit_identical(orc, gpu_engine_factory):
     box = systems.tip4p2005_box(512, seed=5)
     e = water_engine(gpu_engine_factory, box)
     a, _ = e.pppm_info()
-    q2 = systems.REAL_QQR_TO_ENG * (box["q"] ** 2).sum()
+    q2 = orc.total_q2(box["q"], systems.REAL_QQR_TO_ENG)
     ao, _ = orc.pppm_alpha(len(box["x"]), q2, box["L"], np.array([32, 32, 32]), 5, 9.0)
     assert a == ao
 

@@ -48,18 +48,19 @@ def water(make, nmol, seed, nh):
 
 
 def tight_rows(e, flags):
-    """Rows just long enough for the starting list: a later rebuild in the run
-    overflows them and its chunk is replayed."""
+    """Rows exactly as long as the starting list's longest: a later rebuild
+    in which any row grew overflows them and its chunk is replayed."""
     e.compute(flags)
     counts, _ = e.nlist_get()
-    e.option("row_cap", max(16, int(counts.max()) + 1))
-    return int(counts.max()) + 1
+    cap = max(16, int(counts.max()))
+    e.option("row_cap", cap)
+    return cap
 
 
 @pytest.mark.parametrize("graph", [0, 1])
 def test_overflow_replay_with_nose_hoover_is_exact(gpu_engine_factory, graph):
     say(f"overflow replay with Nose-Hoover, graph={graph}")
-    steps = 192
+    steps = 384
     ref, flags = water(gpu_engine_factory, 343, 31, nh=True)
     tst, _ = water(gpu_engine_factory, 343, 31, nh=True)
     for e in (ref, tst):
