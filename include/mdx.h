@@ -103,6 +103,23 @@ int mdx_pppm_set(mdx_ctx *ctx, int nx, int ny, int nz, int order, double r_cut, 
                  double alpha);
 /* Current alpha and the RMS error estimate DeltaF_k + DeltaF_real. */
 int mdx_pppm_info(mdx_ctx *ctx, double *alpha, double *rms_error);
+/* Error mode (FixChargeEwald::setError + setGridToErrorTolerance,
+ * FixChargeEwald.cu:1024-1096): the mesh starts from the one set before
+ * (else 64^3) and the axis with the fewest points per length is doubled until
+ * DeltaF_k + DeltaF_real at the solved alpha is at most `error`; the search is
+ * repeated whenever the box changed (at the start of a run or force call).
+ * Meshes beyond 1024 points per axis are refused. */
+int mdx_pppm_set_error(mdx_ctx *ctx, double error, double r_cut, int order, double qqr);
+/* Long-range interval (FixChargeEwald::setLongRangeInterval,
+ * FixChargeEwald.cu:1251, 1402-1451, 1614-1618): the mesh part runs on every
+ * interval-th step of a run, counted from its first, and stores its forces by
+ * atom id; the steps between add the stored forces (applyStoredForces,
+ * FixChargeEwald.cu:804-814).  0 keeps the current value.  Energy and virial
+ * requests always run the mesh.  Runs with an interval above 1 take eager
+ * steps (captured steps evaluate the mesh every step). */
+int mdx_pppm_interval_set(mdx_ctx *ctx, int interval);
+/* Mesh in use (after an error-mode search) and the long-range interval. */
+int mdx_pppm_mesh_get(mdx_ctx *ctx, int32_t sz[3], int *interval);
 
 /* E3B water (FixE3B::addMolecule / createEvaluator, src/Fixes/FixE3B.cu:515-593,
  * 753-792).  mol_ids: 4 ids per molecule ordered O, H1, H2, M.  params:

@@ -25,7 +25,7 @@ SYMBOLS = [
     "mdx_positions_set", "mdx_system_get", "mdx_box_set", "mdx_nlist_config",
     "mdx_exclusions_set", "mdx_exclusions_from_bonds", "mdx_nlist_update", "mdx_nlist_get",
     "mdx_nlist_stats", "mdx_lj_set", "mdx_special_set", "mdx_ewald_real_set", "mdx_pppm_set",
-    "mdx_pppm_info", "mdx_e3b_set", "mdx_forces_compute", "mdx_forces_get", "mdx_energy_get",
+    "mdx_pppm_info", "mdx_pppm_set_error", "mdx_pppm_interval_set", "mdx_pppm_mesh_get", "mdx_e3b_set", "mdx_forces_compute", "mdx_forces_get", "mdx_energy_get",
     "mdx_integrator_set", "mdx_run", "mdx_forces_compute_noalloc", "mdx_teams_set", "mdx_sync", "mdx_force_steps",
     "mdx_profile_enable", "mdx_profile_get", "mdx_nlist_totals", "mdx_pair_entries_read", "mdx_option_set", "mdx_rigid_set",
     "mdx_thermostat_set", "mdx_temperature", "mdx_tune", "mdx_run_batch", "mdx_surrogate_set",
@@ -92,6 +92,9 @@ def load():
     L.mdx_ewald_real_set.argtypes = [_vp, c, d, d, d]
     L.mdx_pppm_set.argtypes = [_vp, c, c, c, c, d, d, d]
     L.mdx_pppm_info.argtypes = [_vp, C.POINTER(d), C.POINTER(d)]
+    L.mdx_pppm_set_error.argtypes = [_vp, d, d, c, d]
+    L.mdx_pppm_interval_set.argtypes = [_vp, c]
+    L.mdx_pppm_mesh_get.argtypes = [_vp, _ip, C.POINTER(c)]
     L.mdx_e3b_set.argtypes = [_vp, c, _ip, _dp, d, d]
     L.mdx_forces_compute.argtypes = [_vp, c]
     L.mdx_forces_compute_noalloc.argtypes = [_vp, c]
@@ -268,6 +271,21 @@ class Engine:
         e = C.c_double(0)
         self._chk(self._L.mdx_pppm_info(self._h, C.byref(a), C.byref(e)), "mdx_pppm_info")
         return a.value, e.value
+
+    def pppm_set_error(self, error, r_cut, order, qqr):
+        """FixChargeEwald.setError: the mesh is picked for this RMS force error."""
+        self._chk(self._L.mdx_pppm_set_error(self._h, float(error), float(r_cut), int(order), float(qqr)),
+                  "mdx_pppm_set_error")
+
+    def pppm_interval(self, interval):
+        """FixChargeEwald.setLongRangeInterval."""
+        self._chk(self._L.mdx_pppm_interval_set(self._h, int(interval)), "mdx_pppm_interval_set")
+
+    def pppm_mesh(self):
+        sz = np.zeros(3, np.int32)
+        iv = C.c_int(0)
+        self._chk(self._L.mdx_pppm_mesh_get(self._h, sz, C.byref(iv)), "mdx_pppm_mesh_get")
+        return tuple(int(v) for v in sz), iv.value
 
     def e3b_set(self, mol_ids, params, r_two_body=6.2, r_max=6.1572):
         m = np.ascontiguousarray(mol_ids, np.int32).reshape(-1)

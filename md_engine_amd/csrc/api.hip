@@ -192,7 +192,7 @@ void compute(Ctx &c, int flags) {
     if (c.flex.on) flex_place_m(c);  // the M sites follow O and H before any force term
     prepare_forces(c);
     if ((flags & MDX_PAIR)) pair_forces(c, energy, c.vtrace ? false : virial);
-    if ((flags & MDX_PPPM) && c.pppm.on) pppm_forces(c, energy, virial);
+    if ((flags & MDX_PPPM) && c.pppm.on) pppm_step(c, energy, virial);
     if ((flags & MDX_SURROGATE) && c.e3b.on) {
         MDX_REQUIRE(c.sur.on, MDX_ERR_STATE, "MDX_SURROGATE needs a model (mdx_surrogate_set)");
         surrogate_forces(c, energy, virial);  // the two-body term stays analytic, in the same launch
@@ -252,6 +252,12 @@ void snapshot_save(Ctx &c) {
     if (!c.snap.thermo) c.snap.thermo = dalloc<unsigned long long>(c, 2);
     MDX_HIP(hipMemcpyAsync(c.snap.thermo, c.thermo, sizeof(unsigned long long) * 2, hipMemcpyDeviceToDevice,
                            c.stream));
+    if (c.pppm.on && c.pppm.interval > 1 && c.pppm.stored) {
+        if (!c.snap.pppmStored) c.snap.pppmStored = dalloc<long long>(c, 3 * (size_t)c.n);
+        MDX_HIP(hipMemcpyAsync(c.snap.pppmStored, c.pppm.stored, sizeof(long long) * 3 * c.n,
+                               hipMemcpyDeviceToDevice, c.stream));
+        c.snap.pppmStoredValid = c.pppm.storedValid;
+    }
 }
 
 void snapshot_restore(Ctx &c) {
@@ -268,6 +274,11 @@ void snapshot_restore(Ctx &c) {
     if (c.snap.thermo)
         MDX_HIP(hipMemcpyAsync(c.thermo, c.snap.thermo, sizeof(unsigned long long) * 2, hipMemcpyDeviceToDevice,
                                c.stream));
+    if (c.pppm.on && c.pppm.interval > 1 && c.pppm.stored && c.snap.pppmStored) {
+        MDX_HIP(hipMemcpyAsync(c.pppm.stored, c.snap.pppmStored, sizeof(long long) * 3 * c.n,
+                               hipMemcpyDeviceToDevice, c.stream));
+        c.pppm.storedValid = c.snap.pppmStoredValid;
+    }
     c.turn = c.snap.turn;
     c.nRebuilds = c.snap.rebuilds;
     c.listValid = false;
@@ -472,6 +483,8 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags, const Rin
         require_system(c);
         MDX_REQUIRE(c.rlist > 0, MDX_ERR_STATE, "mdx_nlist_config not called");
         MDX_REQUIRE(c.device == cs[0]->device, MDX_ERR_ARG, "batched contexts must share one device");
+        MDX_REQUIRE(!c.pppm.on || c.pppm.interval <= 1, MDX_ERR_STATE,
+                    "captured steps evaluate the mesh every step: long-range interval must be 1");
         MDX_HIP(hipSetDevice(c.device));
         prepare_terms(c);
         if (c.rigid.on && c.rigid.fresh) {
@@ -1032,9 +1045,11 @@ int mdx_pppm_set(mdx_ctx *ctx, int nx, int ny, int nz, int order, double r_cut, 
         MDX_REQUIRE(order == 1 || order == 3 || order == 5 || order == 7, MDX_ERR_ARG,
                     "interpolation order must be 1, 3, 5 or 7");
         MDX_REQUIRE(r_cut > 0, MDX_ERR_ARG, "r_cut must be positive");
+        const int interval = c.pppm.on ? c.pppm.interval : 1;
         pppm_release(c);
         c.pppm = PppmState();
         c.pppm.on = true;
+        c.pppm.interval = interval;
         c.pppm.sz[0] = nx;
         c.pppm.sz[1] = ny;
         c.pppm.sz[2] = nz;
@@ -1049,6 +1064,53 @@ int mdx_pppm_set(mdx_ctx *ctx, int nx, int ny, int nz, int order, double r_cut, 
             for (int i = 0; i < c.n; ++i) s += (double)c.qHost[i] * c.qHost[i];
             c.pppm.totalQ2 = qqr * s;
         }
+    });
+}
+
+int mdx_pppm_set_error(mdx_ctx *ctx, double error, double r_cut, int order, double qqr) {
+    return guarded(ctx, [&](Ctx &c) {
+        MDX_REQUIRE(error > 0 && r_cut > 0, MDX_ERR_ARG, "error and r_cut must be positive");
+        MDX_REQUIRE(order == 1 || order == 3 || order == 5 || order == 7, MDX_ERR_ARG,
+                    "interpolation order must be 1, 3, 5 or 7");
+        // the search starts from the mesh set before, else FixChargeEwald's 64^3
+        int sz[3] = {64, 64, 64};
+        if (c.pppm.on)
+            for (int a = 0; a < 3; ++a) sz[a] = c.pppm.sz[a];
+        const int interval = c.pppm.on ? c.pppm.interval : 1;
+        pppm_release(c);
+        c.pppm = PppmState();
+        c.pppm.on = true;
+        for (int a = 0; a < 3; ++a) c.pppm.sz[a] = sz[a];
+        c.pppm.order = order;
+        c.pppm.rcut = r_cut;
+        c.pppm.qqr = qqr;
+        c.pppm.errorTarget = error;
+        c.pppm.interval = interval;
+        if (c.n > 0) {
+            double s = 0.0;
+            for (int i = 0; i < c.n; ++i) s += (double)c.qHost[i] * c.qHost[i];
+            c.pppm.totalQ2 = qqr * s;
+        }
+    });
+}
+
+int mdx_pppm_interval_set(mdx_ctx *ctx, int interval) {
+    return guarded(ctx, [&](Ctx &c) {
+        MDX_REQUIRE(c.pppm.on, MDX_ERR_STATE, "P3M not configured");
+        MDX_REQUIRE(interval >= 0, MDX_ERR_ARG, "interval must be >= 0");
+        if (interval) c.pppm.interval = interval;  // 0 keeps it, as the reference does
+        c.pppm.storedValid = false;
+    });
+}
+
+int mdx_pppm_mesh_get(mdx_ctx *ctx, int32_t sz[3], int *interval) {
+    return guarded_ro(ctx, [&](Ctx &c) {
+        MDX_REQUIRE(c.pppm.on, MDX_ERR_STATE, "P3M not configured");
+        require_system(c);
+        pppm_prepare(c);
+        if (sz)
+            for (int a = 0; a < 3; ++a) sz[a] = c.pppm.sz[a];
+        if (interval) *interval = c.pppm.interval;
     });
 }
 
@@ -1302,7 +1364,10 @@ int mdx_run(mdx_ctx *ctx, int nsteps, int flags) {
         require_system(c);
         MDX_REQUIRE(c.rlist > 0, MDX_ERR_STATE, "mdx_nlist_config not called");
         MDX_REQUIRE(nsteps >= 0, MDX_ERR_ARG, "nsteps must be >= 0");
-        if (c.graphSteps) {
+        c.pppm.turnInit = c.turn;  // the long-range interval counts from a run's first step
+        // a long-range interval > 1 runs the steps eagerly (the host picks the
+        // mesh steps); captured steps evaluate the mesh every step
+        if (c.graphSteps && c.pppm.interval <= 1) {
             run_captured({&c}, nsteps, flags);
             return;
         }
@@ -1320,6 +1385,11 @@ int mdx_run(mdx_ctx *ctx, int nsteps, int flags) {
             c.nh.ndf = degrees_of_freedom(c);
             nose_hoover_start(c);
         }
+        struct InRun {
+            Ctx &c;
+            explicit InRun(Ctx &c_) : c(c_) { c.inRun = true; }
+            ~InRun() { c.inRun = false; }
+        } inRun(c);
         constexpr int kChunk = 64;
         int done = 0;
         while (done < nsteps) {

@@ -258,6 +258,16 @@ struct PppmState {
     hipStream_t planStream = nullptr;  // stream the plans are bound to
     hipfftHandle planC2R = 0;
     bool plansMade = false;
+    // FixChargeEwald::setError: the mesh is the smallest the doubling search
+    // of setGridToErrorTolerance finds for this RMS force error (0: off)
+    double errorTarget = 0.0;
+    // FixChargeEwald::setLongRangeInterval: the mesh part runs every
+    // `interval` steps counted from the run's first; the steps between add the
+    // forces it stored by atom id (fixed point, 3 per id)
+    int interval = 1;
+    long long turnInit = 0;
+    long long *stored = nullptr;
+    bool storedValid = false;
 };
 
 // Flexible water: quartic O-H bonds, harmonic H-O-H angle and a massless M
@@ -369,6 +379,7 @@ constexpr int kFlagVirialTrace = 1 << 12;
 struct Ctx {
     int device = 0;
     bool noAlloc = false;  // inside mdx_forces_compute_noalloc: any device allocation is an error
+    bool inRun = false;    // inside mdx_run's step loop (the long-range interval applies)
     // Captured step graphs are kept across mdx_run calls while nothing they
     // bake in changed: serial identifies the context for its lifetime,
     // apiEpoch counts calls that may change the set-up, layoutEpoch counts
@@ -505,6 +516,8 @@ struct Ctx {
         double *nh = nullptr;                 // Nose-Hoover chain state (4 chain + 2 doubles)
         int nhLen = 0;
         unsigned long long *thermo = nullptr; // thermostat sums (2 words)
+        long long *pppmStored = nullptr;      // stored mesh forces (long-range interval > 1)
+        bool pppmStoredValid = false;
     } snap;
 
     // profiling: event pairs recorded on the context stream around kernels
@@ -634,6 +647,12 @@ std::vector<int> pair_type_flag_table(const Ctx &c);
 // pppm.hip
 void pppm_prepare(Ctx &c);            // alpha + Green function for the current box
 void pppm_forces(Ctx &c, bool energy, bool virial);
+// the mesh part of one step: pppm_forces, or the stored forces between the
+// steps of a long-range interval (energy and virial requests always run the mesh)
+void pppm_step(Ctx &c, bool energy, bool virial);
+// setGridToErrorTolerance: mesh for the error target at the current box
+// (grows the mesh, reallocating when it changed)
+void pppm_mesh_for_error(Ctx &c);
 void pppm_release(Ctx &c);
 // File-scope device scratch for the P3M mesh work arrays of
 // mdx_forces_compute_noalloc: its address on `device` (looked up once per

@@ -480,7 +480,8 @@ template <int ORDER, int T>
 __global__ void __launch_bounds__(256) k_gather(int n, const double4 *__restrict__ pos,
                                                 const float *__restrict__ q, const BoxDev *__restrict__ bx,
                                                 const float4 *__restrict__ E4,
-                                                unsigned long long *__restrict__ force) {
+                                                unsigned long long *__restrict__ force,
+                                                const int *__restrict__ ids, long long *__restrict__ stored) {
     const MeshGeo m = bx->mesh;
     const float scale = bx->gatherScale;
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
@@ -524,10 +525,31 @@ __global__ void __launch_bounds__(256) k_gather(int n, const double4 *__restrict
     if (tl == 0) {
         // F = Qunit q sum W (-E / V)  (Ewald_long_range_forces_order_5_cu)
         const float s = scale * qi;
-        fx_add(&force[3 * i + 0], s * ex);
-        fx_add(&force[3 * i + 1], s * ey);
-        fx_add(&force[3 * i + 2], s * ez);
+        const long long f0 = to_fx(s * ex), f1 = to_fx(s * ey), f2 = to_fx(s * ez);
+        atomicAdd(&force[3 * i + 0], (unsigned long long)f0);
+        atomicAdd(&force[3 * i + 1], (unsigned long long)f1);
+        atomicAdd(&force[3 * i + 2], (unsigned long long)f2);
+        if (stored) {  // the same fixed-point increments, by atom id, for the steps between
+            const int id = ids[i];
+            stored[3 * id + 0] = f0;
+            stored[3 * id + 1] = f1;
+            stored[3 * id + 2] = f2;
+        }
     }
+}
+
+// the stored mesh forces by id added to the current order's force rows
+// (applyStoredForces, FixChargeEwald.cu:804-814)
+__global__ void k_apply_stored(int n, const int *__restrict__ ids, const long long *__restrict__ stored,
+                               unsigned long long *__restrict__ force) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int id = ids[i];
+    const long long f0 = stored[3 * id], f1 = stored[3 * id + 1], f2 = stored[3 * id + 2];
+    if ((f0 | f1 | f2) == 0) return;
+    atomicAdd(&force[3 * i + 0], (unsigned long long)f0);
+    atomicAdd(&force[3 * i + 1], (unsigned long long)f1);
+    atomicAdd(&force[3 * i + 2], (unsigned long long)f2);
 }
 
 
@@ -656,6 +678,11 @@ void pppm_release(Ctx &c) {
 // FixChargeEwald.cu:1206-1229): recomputed whenever the box changed.
 void pppm_prepare(Ctx &c) {
     PppmState &P = c.pppm;
+    if (P.errorTarget > 0) {
+        bool same = true;
+        for (int a = 0; a < 3; ++a) same = same && (P.boxAtGreen[a] == c.geo.L[a]);
+        if (!same || !P.plansMade) pppm_mesh_for_error(c);
+    }
     const int *sz = P.sz;
     const int szh = sz[2] / 2 + 1;
     const size_t ng = (size_t)sz[0] * sz[1] * sz[2];
@@ -780,7 +807,7 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
     const dim3 grid(cdiv((long long)na * P.gatherTeam, threads));
 #define MDX_GATHER(O, T)                                                                                   \
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gather<O, T>), grid, dim3(threads), 0, c.stream, na, c.pos[cur], \
-                       c.q[cur], c.box, P.eField4, c.force)
+                       c.q[cur], c.box, P.eField4, c.force, c.id[cur], P.interval > 1 ? P.stored : nullptr)
     const bool wide = P.gatherTeam == 16;
     switch (P.order) {
         case 1: if (wide) MDX_GATHER(1, 16); else MDX_GATHER(1, 8); break;
@@ -789,6 +816,69 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
         default: if (wide) MDX_GATHER(7, 16); else MDX_GATHER(7, 8); break;
     }
 #undef MDX_GATHER
+}
+
+void pppm_step(Ctx &c, bool energy, bool virial) {
+    PppmState &P = c.pppm;
+    if (P.interval > 1 && c.inRun) {
+        if (!P.stored) {
+            P.stored = dalloc<long long>(c, 3 * (size_t)c.n);
+            MDX_HIP(hipMemsetAsync(P.stored, 0, sizeof(long long) * 3 * c.n, c.stream));
+            P.storedValid = false;
+        }
+        const bool due = (c.turn - P.turnInit) % P.interval == 0;
+        if (!due && P.storedValid && !energy && !virial && !c.vtrace) {
+            hipLaunchKernelGGL(k_apply_stored, dim3(cdiv(c.n, 256)), dim3(256), 0, c.stream, c.n, c.id[c.cur],
+                               P.stored, c.force);
+            return;
+        }
+        pppm_forces(c, energy, virial);
+        P.storedValid = c.listValid;  // a gather without a valid list visits every atom; stored is complete too
+        return;
+    }
+    pppm_forces(c, energy, virial);
+}
+
+void pppm_mesh_for_error(Ctx &c) {
+    PppmState &P = c.pppm;
+    int sz[3] = {P.sz[0], P.sz[1], P.sz[2]};
+    double err = 0.0;
+    pppm_solve_alpha(c.n, P.totalQ2, c.geo.L, sz, P.order, P.rcut, &err);
+    int tries = 0;
+    while (tries < 100 && (err > P.errorTarget || err != err || err < 0)) {
+        // double the axis with the fewest mesh points per length
+        int k = 0;
+        for (int a = 1; a < 3; ++a)
+            if (sz[a] / c.geo.L[a] < sz[k] / c.geo.L[k]) k = a;
+        MDX_REQUIRE(sz[k] < 1024, 1, "P3M error target needs a mesh beyond 1024 points per axis");
+        sz[k] *= 2;
+        pppm_solve_alpha(c.n, P.totalQ2, c.geo.L, sz, P.order, P.rcut, &err);
+        ++tries;
+    }
+    if (sz[0] == P.sz[0] && sz[1] == P.sz[1] && sz[2] == P.sz[2]) return;
+    // a new mesh: new work arrays, plans, Green function and device box
+    if (P.plansMade) {
+        pppm_release(c);
+        dfree(c, P.mesh);
+        dfree(c, P.real);
+        dfree(c, P.rhoHat);
+        dfree(c, P.eHat);
+        dfree(c, P.eReal);
+        dfree(c, P.eField4);
+        dfree(c, P.green);
+        P.mesh = nullptr;
+        P.real = nullptr;
+        P.rhoHat = P.eHat = nullptr;
+        P.eReal = nullptr;
+        P.eField4 = nullptr;
+        P.green = nullptr;
+    }
+    for (int a = 0; a < 3; ++a) {
+        P.sz[a] = sz[a];
+        P.boxAtGreen[a] = 0.0;
+    }
+    P.alpha = 0.0;
+    P.storedValid = false;
 }
 
 }  // namespace mdx

@@ -315,6 +315,26 @@ def deltaf_real(a, natoms, total_q2, L, rcut):
     return lib().orc_deltaf_real(float(a), int(natoms), float(total_q2), _d(L), float(rcut))
 
 
+def pppm_mesh_for_error(natoms, total_q2, L, sz, order, rcut, target):
+    """setGridToErrorTolerance (FixChargeEwald.cu:1024-1058): from mesh sz,
+    double the axis with the fewest points per length (the first such axis
+    on ties) until the estimate DeltaF_k + DeltaF_real at the solved alpha
+    is at most `target` (at most 100 doublings).  Returns (mesh, error)."""
+    sz = [int(v) for v in sz]
+    L = np.asarray(L, np.float64)
+    _, err = pppm_alpha(natoms, total_q2, L, np.array(sz), order, rcut)
+    tries = 0
+    while tries < 100 and (err > target or err != err or err < 0):
+        k = 0
+        for a in (1, 2):
+            if sz[a] / L[a] < sz[k] / L[k]:
+                k = a
+        sz[k] *= 2
+        _, err = pppm_alpha(natoms, total_q2, L, np.array(sz), order, rcut)
+        tries += 1
+    return tuple(sz), err
+
+
 def pppm_green(L, sz, alpha, order):
     """Optimal influence function on the full mesh (FixChargeEwald.cu:314-366, 1097-1103)."""
     G = np.zeros(int(np.prod(sz)))

@@ -286,7 +286,7 @@ def test_tiled_spread_bit_identical(gpu_engine_factory):
     assert np.array_equal(f_tiled, f_direct)
 
 
-@pytest.mark.parametrize("order", [3, 7])
+@pytest.mark.parametrize("order", [1, 3, 7])
 def test_water_pppm_orders(orc, gpu_engine_factory, order):
     say(f"water P3M order {order}")
     from md_engine_amd._lib import PPPM
@@ -296,6 +296,25 @@ def test_water_pppm_orders(orc, gpu_engine_factory, order):
     e = water_engine(gpu_engine_factory, box, rc=7.0, sz=(32, 32, 64), order=order)
     e.compute(PPPM)
     assert rel_rmse(e.forces(), ref["fk"]) <= FORCE_RMSE
+
+
+def test_water_e3b2(orc, gpu_engine_factory):
+    """The E3B2 parameter set (FixE3B.cu:557-579) through the same kernels."""
+    say("water E3B2 512 molecules")
+    from md_engine_amd._lib import E3B, ENERGY
+
+    box = systems.tip4p2005_box(512, seed=7)
+    e = water_engine(gpu_engine_factory, box)
+    p = orc.e3b_params("E3B2")
+    e.e3b_set(box["mol"], [p.E2, p.Ea, p.Eb, p.Ec, p.k2, p.k3, p.rs, p.rf])
+    e.compute(E3B | ENERGY)
+    f = e.forces()
+    en, _ = e.energies()
+    fo, e2, e3 = orc.e3b(box["mol"], box["x"], box["L"], params=p)
+    assert abs(e3) > 1.0
+    assert rel_rmse(f, fo) <= FORCE_RMSE
+    assert en[2] == pytest.approx(e2, rel=ENERGY_REL)
+    assert en[3] == pytest.approx(e3, rel=ENERGY_REL)
 
 
 def test_water_e3b(orc, gpu_engine_factory):

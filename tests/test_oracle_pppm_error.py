@@ -161,3 +161,24 @@ def test_optimal_green_beats_plain_kernels(orc, charges, recip, order, a):
         assert e_opt * gain < e_dec, (mesh, e_opt, e_dec)
         if order > 1:
             assert e_opt < 0.2 * e_raw, (mesh, e_opt, e_raw)
+
+
+def test_mesh_search_for_an_error_target(orc):
+    """The doubling search stops at the first mesh on its path that meets the
+    target: that mesh meets it, the one before it does not, and the axis
+    doubled is always the one with the fewest points per length."""
+    L = np.array([40.0, 50.0, 62.0])
+    n, q2 = 30000, 332.06371 * 15000 * 0.4
+    for target in (1e-2, 3e-3, 1e-3):
+        sz, err = orc.pppm_mesh_for_error(n, q2, L, (32, 32, 32), 5, 9.0, target)
+        assert err <= target
+        path = [(32, 32, 32)]
+        while path[-1] != sz:
+            cur = list(path[-1])
+            k = int(np.argmin(np.array(cur) / L))
+            cur[k] *= 2
+            path.append(tuple(cur))
+            assert len(path) < 30
+        if len(path) > 1:
+            _, e_prev = orc.pppm_alpha(n, q2, L, np.array(path[-2]), 5, 9.0)
+            assert e_prev > target
