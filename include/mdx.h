@@ -298,7 +298,8 @@ int mdx_sync(mdx_ctx *ctx);
  * (both the per-atom team build); "cell_reach_xy" 1..2 and "cell_reach_z"
  * 1..16: cell edges >= rlist / reach per axis (identical lists, finer cells
  * test fewer candidates); "gather_team" 8 or 16 lanes per charged atom in
- * the P3M field gather; "row_cap" >= 16 sets the neighbor-row capacity after
+ * the P3M field gather; "sur_force_lanes" 8, 16, 32 or 64 lanes per molecule
+ * in the learned term's force pass; "row_cap" >= 16 sets the neighbor-row capacity after
  * the first build (rows still grow on overflow). */
 int mdx_option_set(mdx_ctx *ctx, const char *name, int value);
 

@@ -1946,6 +1946,11 @@ int mdx_option_set(mdx_ctx *ctx, const char *name, int value) {
             c.nlistConfigured = false;
             c.listValid = false;
         }
+        else if (k == "sur_force_lanes") {
+            MDX_REQUIRE(value == 8 || value == 16 || value == 32 || value == 64, MDX_ERR_ARG,
+                        "sur_force_lanes: 8, 16, 32 or 64 lanes per molecule");
+            c.sur.forceLanes = value;
+        }
         else if (k == "row_cap") {
             // neighbor-row capacity now (rows still grow on overflow, the
             // step loop replays a chunk whose rebuild overflowed)

@@ -337,6 +337,7 @@ struct SurrogateState {
     int nsamp = 0, capSamp = 0;
     float *part = nullptr, *m1 = nullptr, *m2 = nullptr, *loss = nullptr;
     long step = 0;
+    int forceLanes = 16;      // lanes per molecule in the force pass (8, 16, 32, 64)
     // E3B as judge: every judgeEvery steps of a run with MDX_SURROGATE the
     // three-body forces of both forms are compared on the current frame; an
     // RMSE above judgeMax makes the rest of the run (and later runs) use E3B

@@ -883,9 +883,7 @@ __device__ __forceinline__ void force_pair(float4 Oi, float4 H1i, float4 H2i, fl
     }
 }
 
-constexpr int kForceLanes = 32;  // lanes per molecule in the force pass
-
-template <bool VIR>
+template <bool VIR, int kForceLanes>
 __global__ void __launch_bounds__(256) k_sur_force(int nmol, const int4 *__restrict__ molAtoms,
                                                    const float4 *__restrict__ xq, const BoxDev *__restrict__ bx,
                                                    int cap, const int *__restrict__ nl,
@@ -899,7 +897,8 @@ __global__ void __launch_bounds__(256) k_sur_force(int nmol, const int4 *__restr
     const Geo3 g = geo3(bx);
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = gt / kForceLanes, tl = threadIdx.x % kForceLanes, team = threadIdx.x / kForceLanes;
-    if (tl < NW / 4) swi[team][tl] = i < nmol ? ((const float4 *)(wgt + (size_t)i * NW))[tl] : make_float4(0, 0, 0, 0);
+    for (int v = tl; v < NW / 4; v += kForceLanes)
+        swi[team][v] = i < nmol ? ((const float4 *)(wgt + (size_t)i * NW))[v] : make_float4(0, 0, 0, 0);
     __syncthreads();
     float wi[NW];
 #pragma unroll
@@ -1209,13 +1208,18 @@ void surrogate_forces(Ctx &c, bool energy, bool virial, bool with_feat) {
 #undef MDX_FUSED
     }
     ProfScope ps(c, PK_SUR_FORCE);
-    const dim3 grid(cdiv((long long)E.nmol * kForceLanes, 256));
-    if (virial)
-        hipLaunchKernelGGL(k_sur_force<true>, grid, dim3(256), 0, c.stream, E.nmol, E.molAtoms, c.xq, c.box, E.cap,
-                           E.nl, E.nlCount, S.wgt, c.force, c.acc);
-    else
-        hipLaunchKernelGGL(k_sur_force<false>, grid, dim3(256), 0, c.stream, E.nmol, E.molAtoms, c.xq, c.box, E.cap,
-                           E.nl, E.nlCount, S.wgt, c.force, c.acc);
+    const int lanes = S.forceLanes;
+    const dim3 grid(cdiv((long long)E.nmol * lanes, 256));
+#define MDX_SFORCE(V, T)                                                                                        \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sur_force<V, T>), grid, dim3(256), 0, c.stream, E.nmol, E.molAtoms, c.xq, \
+                       c.box, E.cap, E.nl, E.nlCount, S.wgt, c.force, c.acc)
+    switch (lanes) {
+        case 8: if (virial) MDX_SFORCE(true, 8); else MDX_SFORCE(false, 8); break;
+        case 16: if (virial) MDX_SFORCE(true, 16); else MDX_SFORCE(false, 16); break;
+        case 64: if (virial) MDX_SFORCE(true, 64); else MDX_SFORCE(false, 64); break;
+        default: if (virial) MDX_SFORCE(true, 32); else MDX_SFORCE(false, 32); break;
+    }
+#undef MDX_SFORCE
 }
 
 void surrogate_pack(Ctx &c) {
