@@ -18,6 +18,7 @@
 // combined with butterfly shuffles and added once per atom into the 64-bit
 // fixed-point force buffer.  Energy and virial (0.5-weighted, full list) are
 // reduced in FP64 per block and added with one fixed-point atomic per block.
+#include "erfc_fit.h"
 #include "mdx_internal.h"
 
 namespace mdx {
@@ -122,9 +123,11 @@ __global__ void __launch_bounds__(256) k_pair(PairArgs a) {
                     const float rinv = rsqrtf(ls);
                     const float len = ls * rinv;
                     const float qq = a.qqr * pi.w * pj.w;
-                    // erfc(ar) - (1 - m); for a full exclusion (m = 0) this is -erf(ar)
-                    const float ec = erfcf(alphaB * len) - (1.0f - m);
-                    const float s = qq * (ec * rinv + twoAlphaRtPi * __expf(-alpha2 * ls));
+                    // erfc(ar) - (1 - m); for a full exclusion (m = 0) this is -erf(ar).
+                    // erfc shares the Gaussian's exponential (erfc_fit.h)
+                    const float gauss = __expf(-alpha2 * ls);
+                    const float ec = erfc_from_gauss(alphaB * len, gauss) - (1.0f - m);
+                    const float s = qq * (ec * rinv + twoAlphaRtPi * gauss);
                     fs += s * rinv * rinv;
                     if (EV) e += 0.5f * qq * rinv * ec;
                 }
