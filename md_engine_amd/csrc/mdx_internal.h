@@ -594,6 +594,20 @@ __device__ __forceinline__ float team_sum(float v) {
     return v;
 }
 
+// XCD-aware block order.  Workgroups are dispatched round-robin over the
+// chip's 8 XCDs (workgroup b to XCD b mod 8), each with its own L2; kernels
+// whose blocks walk spatially sorted data take logical block
+// (b mod 8) * per + b / 8 so that each XCD works through one contiguous
+// eighth (its L2 then holds that region's share of a shared table, not all
+// of it).  The grid is rounded up to a multiple of 8; blocks whose logical
+// index is past the end return.
+constexpr int kXcds = 8;
+inline int xcd_grid(long long nlogical) { return cdiv(nlogical, kXcds) * kXcds; }
+__device__ __forceinline__ int xcd_block(int nlogical) {
+    const int per = (nlogical + kXcds - 1) / kXcds;
+    return ((int)blockIdx.x % kXcds) * per + (int)blockIdx.x / kXcds;
+}
+
 // maximum over the 64 lanes of a wave (every lane gets it)
 __device__ __forceinline__ int wave_max(int v) {
 #pragma unroll

@@ -134,13 +134,15 @@ template <int ORDER>
 __global__ void __launch_bounds__(256) k_spread_tiled(const double4 *__restrict__ pos,
                                                       const float *__restrict__ q, const BoxDev *__restrict__ bx,
                                                       float qunit, const int *__restrict__ cellStart,
-                                                      const int2 *__restrict__ colXY, TileGeo tg,
+                                                      const int2 *__restrict__ colXY, TileGeo tg, int nblk,
                                                       unsigned long long *__restrict__ mesh) {
     extern __shared__ unsigned long long tile[];
+    const int lb = xcd_block(nblk);
+    if (lb >= nblk) return;  // uniform over the workgroup
     const MeshGeo m = bx->mesh;
     constexpr int H = (ORDER - 1) / 2;
-    const int col = blockIdx.x / tg.nzc;
-    const int zc = blockIdx.x % tg.nzc;
+    const int col = lb / tg.nzc;
+    const int zc = lb % tg.nzc;
     const int z0 = zc * tg.zs;
     const int z1 = min(z0 + tg.zs, tg.ncz);
     const int a0 = cellStart[col * tg.ncz + z0];
@@ -482,9 +484,12 @@ __global__ void __launch_bounds__(256) k_gather(int n, const double4 *__restrict
                                                 const float4 *__restrict__ E4,
                                                 unsigned long long *__restrict__ force,
                                                 const int *__restrict__ ids, long long *__restrict__ stored) {
+    const int nlog = (int)(((long long)n * T + blockDim.x - 1) / blockDim.x);
+    const int lb = xcd_block(nlog);
+    if (lb >= nlog) return;  // uniform over the workgroup
     const MeshGeo m = bx->mesh;
     const float scale = bx->gatherScale;
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gt = lb * blockDim.x + threadIdx.x;
     const int i = gt / T;
     const int tl = threadIdx.x & (T - 1);
     if (i >= n) return;  // whole teams leave together
@@ -758,10 +763,10 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
         const int nblk = c.nc[0] * c.nc[1] * tg.nzc;
         const size_t shm = sizeof(unsigned long long) * tg.ext[0] * tg.ext[1] * tg.ext[2];
         switch (P.order) {
-            case 1: hipLaunchKernelGGL(k_spread_tiled<1>, dim3(nblk), dim3(threads), shm, c.stream, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, (unsigned long long *)P.mesh); break;
-            case 3: hipLaunchKernelGGL(k_spread_tiled<3>, dim3(nblk), dim3(threads), shm, c.stream, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, (unsigned long long *)P.mesh); break;
-            case 5: hipLaunchKernelGGL(k_spread_tiled<5>, dim3(nblk), dim3(threads), shm, c.stream, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, (unsigned long long *)P.mesh); break;
-            default: hipLaunchKernelGGL(k_spread_tiled<7>, dim3(nblk), dim3(threads), shm, c.stream, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, (unsigned long long *)P.mesh); break;
+            case 1: hipLaunchKernelGGL(k_spread_tiled<1>, dim3(xcd_grid(nblk)), dim3(threads), shm, c.stream, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh); break;
+            case 3: hipLaunchKernelGGL(k_spread_tiled<3>, dim3(xcd_grid(nblk)), dim3(threads), shm, c.stream, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh); break;
+            case 5: hipLaunchKernelGGL(k_spread_tiled<5>, dim3(xcd_grid(nblk)), dim3(threads), shm, c.stream, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh); break;
+            default: hipLaunchKernelGGL(k_spread_tiled<7>, dim3(xcd_grid(nblk)), dim3(threads), shm, c.stream, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh); break;
         }
     } else
     switch (P.order) {
@@ -804,7 +809,7 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
     ProfScope ps(c, PK_GATHER);
     if (na == 0) return;
     hipLaunchKernelGGL(k_field_pack, dim3(cdiv(ng, threads)), dim3(threads), 0, c.stream, ng, P.eReal, P.eField4);
-    const dim3 grid(cdiv((long long)na * P.gatherTeam, threads));
+    const dim3 grid(xcd_grid(cdiv((long long)na * P.gatherTeam, threads)));
 #define MDX_GATHER(O, T)                                                                                   \
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gather<O, T>), grid, dim3(threads), 0, c.stream, na, c.pos[cur], \
                        c.q[cur], c.box, P.eField4, c.force, c.id[cur], P.interval > 1 ? P.stored : nullptr)
