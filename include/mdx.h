@@ -37,6 +37,10 @@ typedef struct mdx_ctx mdx_ctx;
 #define MDX_BONDED 64   /* intramolecular terms of flexible water (mdx_flexible_water_set) */
 #define MDX_VIRIAL 16   /* also accumulate the virial tensor */
 #define MDX_SURROGATE 32 /* E3B with the learned three-body term (mdx_surrogate_set) */
+#define MDX_VIRIAL_TRACE 4096 /* only the virial's trace where a term has a trace-only form (pair, E3B):
+                                 what a barostat reads (FixPressureBerendsen's scalar pressure);
+                                 read it with mdx_virial_trace_get */
+#define MDX_PPPM_PER_BEAD 8192 /* ring polymers: every bead its own P3M instead of the centroid contraction */
 
 int mdx_version(void);
 
@@ -170,6 +174,11 @@ int mdx_forces_get(mdx_ctx *ctx, double *f);
  * e[1] P3M reciprocal incl. self term, e[2] E3B two-body, e[3] E3B three-body;
  * virial[6] xx yy zz xy xz yz (sum F (x) dr). */
 int mdx_energy_get(mdx_ctx *ctx, double e[4], double virial[6]);
+/* Trace of the virial of the last compute with MDX_VIRIAL or MDX_VIRIAL_TRACE
+ * (xx + yy + zz of the tensor plus the trace-only terms), the scalar the
+ * pressure computes from (the scalar virial sum of DataComputerPressure::computeScalar_GPU,
+ * src/DataStorageUser/DataComputerPressure.cu:26-91). */
+int mdx_virial_trace_get(mdx_ctx *ctx, double *trace);
 /* Flexible q-TIP4P/F water, replacing the rigid model: FixBondQuartic
  * (src/Fixes/FixBondQuartic.cu; U = k2 dr^2 + k3 dr^3 + k4 dr^4 per O-H
  * bond, src/Evaluators/BondEvaluatorQuartic.h:9-36), FixAngleHarmonic
@@ -206,10 +215,23 @@ int mdx_run_batch(mdx_ctx *const *ctxs, int n, int nsteps, int flags);
  * w_k = 2 omegaP sin(k pi / P); the reference takes omegaP = kB T / hbar at
  * its bead temperature T = P T_physical, IntegratorVerlet.cu:794), every bead
  * evaluates its forces and takes the second half kick and its thermostat.
- * Each bead keeps its own neighbor list and P3M (the reference builds the
- * list on centroids and contracts P3M to the centroid).  P: a power of two
- * from 2 to 64.  The beads run on bead 0's stream inside captured graphs. */
+ * The mesh part is contracted to the ring's centroid as the reference does
+ * (src/Fixes/FixChargeEwald.cu:1255-1266, 1395-1405): charges spread from
+ * each atom's centroid (computeCentroid, :22-50) on bead 0's mesh, the field
+ * interpolated there, the whole force on every bead; alpha is solved over
+ * all P x n ring atoms with one bead's Q^2 (:866-900).  MDX_PPPM_PER_BEAD in
+ * flags gives every bead its own uncontracted P3M instead.  Each bead keeps
+ * its own neighbor list (the reference builds one on centroids; both hold
+ * every pair of a bead within the cutoff while beads stay within the skin of
+ * their centroid).  P: a power of two from 2 to 64.  The beads run on bead
+ * 0's stream inside captured graphs. */
 int mdx_ring_run(mdx_ctx *const *beads, int P, int nsteps, int flags, double omegaP);
+/* Forces of every bead of a ring at the current positions, as one step of
+ * mdx_ring_run evaluates them (contracted mesh part unless
+ * MDX_PPPM_PER_BEAD); read them per bead with mdx_forces_get.  The ring
+ * force evaluation of IntegratorUtil::force with nPerRingPoly = P
+ * (src/Integrators/IntegratorUtil.cpp:15-30). */
+int mdx_ring_forces_compute(mdx_ctx *const *beads, int P, int flags);
 /* Launch geometry knobs (Tunable nThreadPerAtom, src/Tunable.h:3-24):
  * lanes per atom in the list build and the pair kernel, lanes per molecule in
  * E3B.  0 keeps the current value.  Powers of two only. */

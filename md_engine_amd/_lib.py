@@ -15,6 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmdx.so")
 
 PAIR, PPPM, E3B, ENERGY, VIRIAL, SURROGATE, BONDED = 1, 2, 4, 8, 16, 32, 64
+VIRIAL_TRACE = 4096  # only the virial's trace (the barostat's form); read with Engine.virial_trace()
+PPPM_PER_BEAD = 8192  # ring polymers: per-bead P3M instead of the centroid contraction
 
 # trained surrogate parameters (W1 b1 W2 b2 w3 b3 of the 48-64-64-1 network)
 NTRAIN = 48 * 64 + 64 + 64 * 64 + 64 + 64 + 1
@@ -25,7 +27,7 @@ SYMBOLS = [
     "mdx_positions_set", "mdx_system_get", "mdx_box_set", "mdx_nlist_config",
     "mdx_exclusions_set", "mdx_exclusions_from_bonds", "mdx_nlist_update", "mdx_nlist_get",
     "mdx_nlist_stats", "mdx_lj_set", "mdx_special_set", "mdx_ewald_real_set", "mdx_pppm_set",
-    "mdx_pppm_info", "mdx_pppm_set_error", "mdx_pppm_interval_set", "mdx_pppm_mesh_get", "mdx_e3b_set", "mdx_forces_compute", "mdx_forces_get", "mdx_energy_get",
+    "mdx_pppm_info", "mdx_pppm_set_error", "mdx_pppm_interval_set", "mdx_pppm_mesh_get", "mdx_e3b_set", "mdx_forces_compute", "mdx_forces_get", "mdx_energy_get", "mdx_virial_trace_get",
     "mdx_integrator_set", "mdx_run", "mdx_forces_compute_noalloc", "mdx_teams_set", "mdx_sync", "mdx_force_steps",
     "mdx_profile_enable", "mdx_profile_get", "mdx_nlist_totals", "mdx_pair_entries_read", "mdx_option_set", "mdx_rigid_set",
     "mdx_thermostat_set", "mdx_temperature", "mdx_tune", "mdx_run_batch", "mdx_surrogate_set",
@@ -33,6 +35,7 @@ SYMBOLS = [
     "mdx_surrogate_judge_set", "mdx_surrogate_judge_stats",
     "mdx_e3b_molecule_energies", "mdx_nose_hoover_set", "mdx_nose_hoover_energy", "mdx_nose_hoover_state", "mdx_barostat_set",
     "mdx_pressure", "mdx_barostat_last", "mdx_box_get", "mdx_flexible_water_set", "mdx_bonded_energy_get", "mdx_ring_run",
+    "mdx_ring_forces_compute",
 ]
 
 # kernel ids of mdx_profile_get
@@ -85,6 +88,7 @@ def load():
     L.mdx_flexible_water_set.argtypes = [_vp, c, _ip, d, d, d, d, d, d, d]
     L.mdx_bonded_energy_get.argtypes = [_vp, C.POINTER(d)]
     L.mdx_ring_run.argtypes = [C.POINTER(_vp), c, c, c, d]
+    L.mdx_ring_forces_compute.argtypes = [C.POINTER(_vp), c, c]
     L.mdx_thermostat_set.argtypes = [_vp, c, d, c, d, d, d]
     L.mdx_temperature.argtypes = [_vp, d, d, C.POINTER(d), C.POINTER(d)]
     L.mdx_lj_set.argtypes = [_vp, c, _dp, _dp, _dp]
@@ -100,6 +104,7 @@ def load():
     L.mdx_forces_compute_noalloc.argtypes = [_vp, c]
     L.mdx_forces_get.argtypes = [_vp, _dp]
     L.mdx_energy_get.argtypes = [_vp, _dp, _dp]
+    L.mdx_virial_trace_get.argtypes = [_vp, C.POINTER(d)]
     L.mdx_integrator_set.argtypes = [_vp, d, d]
     L.mdx_run.argtypes = [_vp, c, c]
     L.mdx_teams_set.argtypes = [_vp, c, c, c]
@@ -340,6 +345,12 @@ class Engine:
         self._chk(self._L.mdx_energy_get(self._h, e, v), "mdx_energy_get")
         return e, v
 
+    def virial_trace(self):
+        """xx + yy + zz of the last compute's virial (MDX_VIRIAL or VIRIAL_TRACE)."""
+        w = C.c_double()
+        self._chk(self._L.mdx_virial_trace_get(self._h, C.byref(w)), "mdx_virial_trace_get")
+        return w.value
+
     # integration ------------------------------------------------------------------
     def integrator_set(self, dt, ftm_to_v):
         self._chk(self._L.mdx_integrator_set(self._h, float(dt), float(ftm_to_v)), "mdx_integrator_set")
@@ -513,3 +524,15 @@ def run_ring(beads, nsteps, flags, omega_p):
     if rc != 0:
         msg = L.mdx_last_error(beads[0]._h)
         raise MdxError(f"mdx_ring_run failed ({rc}): {msg.decode() if msg else ''}")
+
+
+def ring_forces(beads, flags):
+    """Forces of every bead of a ring at the current positions
+    (mdx_ring_forces_compute); read each bead's with Engine.forces()."""
+    beads = list(beads)
+    L = load()
+    arr = (_vp * len(beads))(*[e._h for e in beads])
+    rc = L.mdx_ring_forces_compute(arr, len(beads), int(flags))
+    if rc != 0:
+        msg = L.mdx_last_error(beads[0]._h)
+        raise MdxError(f"mdx_ring_forces_compute failed ({rc}): {msg.decode() if msg else ''}")

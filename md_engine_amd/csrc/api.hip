@@ -136,6 +136,7 @@ void update_ox_flag(Ctx &c) {
     c.e3b.oxUncharged = u;
 }
 
+static_assert(kFlagVirialTrace == MDX_VIRIAL_TRACE, "trace flag of the C ABI");
 constexpr int kForceTerms = MDX_PAIR | MDX_PPPM | MDX_E3B | MDX_SURROGATE | MDX_BONDED;
 
 // Instantaneous pressure (nktv_to_press units) from the virial of the last
@@ -201,7 +202,7 @@ void compute(Ctx &c, int flags) {
     }
     if (c.flex.on) {
         if (flags & MDX_BONDED) flex_forces(c, energy, virial);
-        flex_spread_m(c);  // every term's M force goes onto O and H before the kick
+        if (!c.deferMSpread) flex_spread_m(c);  // every term's M force goes onto O and H before the kick
     }
     c.forceFlags = flags & kForceTerms;
 }
@@ -359,9 +360,42 @@ void captured_step(Ctx &c, int fflags) {
 // One step of a ring polymer whose beads share one stream: every bead's half
 // kick, the free ring-polymer propagation across beads, then every bead's
 // rebuild gate, forces, second half kick and thermostat.
+// Ring-polymer forces of every bead: per-bead terms, then (contraction) the
+// mesh part once from the centroids onto every bead, then the M-site forces
+// of flexible water onto O and H.
+bool ring_contracts(const RingSpec &ring, int fflags) { return ring.contract && (fflags & MDX_PPPM); }
+
+void ring_forces(const std::vector<Ctx *> &cs, const RingSpec &ring, int fflags) {
+    const bool contract = ring_contracts(ring, fflags);
+    for (Ctx *cp : cs) {
+        Ctx &c = *cp;
+        c.deferMSpread = contract;
+        compute(c, contract ? (fflags & ~MDX_PPPM) : fflags);
+        c.deferMSpread = false;
+    }
+    if (contract) {
+        ring_pppm(*cs[0], ring);
+        for (Ctx *cp : cs)
+            if (cp->flex.on) flex_spread_m(*cp);
+    }
+    for (Ctx *cp : cs) cp->forceFlags = fflags & kForceTerms;
+}
+
 void captured_ring_step(const std::vector<Ctx *> &cs, const RingSpec &ring, int fflags) {
     for (Ctx *c : cs) integrate_kick_first(*c);
     ring_drift(ring, cs[0]->box, cs[0]->stream);
+    if (ring_contracts(ring, fflags)) {
+        for (Ctx *cp : cs) {
+            decide_step(*cp);
+            nlist_rebuild_async(*cp);
+        }
+        ring_forces(cs, ring, fflags);
+        for (Ctx *cp : cs) {
+            integrate_second_half(*cp);
+            if (cp->thermoOn) thermostat_apply(*cp);
+        }
+        return;
+    }
     for (Ctx *cp : cs) {
         Ctx &c = *cp;
         decide_step(c);
@@ -438,11 +472,12 @@ void stream_after(hipStream_t to, hipStream_t from) {
 
 // the ring's per-bead pointer table for the current buffers
 void ring_table(const std::vector<Ctx *> &cs, const RingSpec &ring) {
-    std::vector<void *> t(3 * cs.size());
+    std::vector<void *> t(kRingTab * cs.size());
     for (size_t k = 0; k < cs.size(); ++k) {
-        t[3 * k] = cs[k]->pos[cs[k]->cur];
-        t[3 * k + 1] = cs[k]->vel[cs[k]->cur];
-        t[3 * k + 2] = cs[k]->idToIdx;
+        t[kRingTab * k] = cs[k]->pos[cs[k]->cur];
+        t[kRingTab * k + 1] = cs[k]->vel[cs[k]->cur];
+        t[kRingTab * k + 2] = cs[k]->idToIdx;
+        t[kRingTab * k + 3] = cs[k]->force;
     }
     MDX_HIP(hipMemcpyAsync(ring.tab, t.data(), sizeof(void *) * t.size(), hipMemcpyHostToDevice, cs[0]->stream));
     MDX_HIP(hipStreamSynchronize(cs[0]->stream));
@@ -532,17 +567,22 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags, const Rin
         while (done < nsteps) {
             int len = std::min(kGraphChunk, nsteps - done);
             if (judge && !cs[0]->sur.fellBack) len = std::min(len, steps_to_judge(*cs[0]));
+            // a contracted ring evaluates every bead's forces together, below
+            const bool ringForces = ring && ring_contracts(*ring, fflags);
+            bool ringStale = false;
             for (Ctx *cp : cs) {
                 Ctx &c = *cp;
                 snapshot_save(c);
                 if (!c.listValid || c.forceFlags != fflags) {
                     ensure_list(c);
-                    compute(c, fflags);
+                    if (ringForces) ringStale = true;
+                    else compute(c, fflags);
                 }
                 MDX_HIP(hipStreamSynchronize(c.stream));
                 if (!capacity_ok(c)) {
                     nlist_rebuild(c, true);
-                    compute(c, fflags);
+                    if (ringForces) ringStale = true;
+                    else compute(c, fflags);
                 }
                 // chunk state: displacement slots, running row maxima, counters, turn
                 fill_int(c, (int *)c.dMax, 4, 0);
@@ -561,6 +601,10 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags, const Rin
                 if (curs != ringCur) drop();
                 ringCur = curs;
                 ring_table(cs, *ring);
+                if (ringStale) {
+                    ring_forces(cs, *ring, fflags);
+                    MDX_HIP(hipStreamSynchronize(cs[0]->stream));
+                }
             }
             if (cache) {
                 const std::vector<uint64_t> k = key_now();
@@ -636,6 +680,122 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags, const Rin
     }
     if (!cache) drop();
     for (Ctx *cp : cs) MDX_HIP(hipStreamSynchronize(cp->stream));
+}
+
+}  // namespace
+}  // namespace mdx
+
+namespace mdx {
+namespace {
+
+// Beads of one ring (validated), their ring description on the first bead's
+// device and stream, and the work `f` between set-up and tear-down.  Every
+// bead runs on the first bead's stream for the step-by-step coupling.
+// Contraction (unless MDX_PPPM_PER_BEAD): every bead takes alpha from the
+// secant solve over all P x n ring atoms with one bead's Q^2, as the
+// reference's single State of P n atoms does (FixChargeEwald.cu:866-900).
+template <class F>
+void with_ring(mdx_ctx *const *beads, int P, int flags, double omegaP, F &&f) {
+    Ctx &c0 = beads[0]->c;
+    MDX_REQUIRE(P >= 2 && P <= 64 && (P & (P - 1)) == 0, MDX_ERR_ARG, "beads: a power of two from 2 to 64");
+    std::vector<Ctx *> cs;
+    for (int k = 0; k < P; ++k) {
+        Ctx &c = beads[k]->c;
+        for (Ctx *o : cs) MDX_REQUIRE(o != &c, MDX_ERR_ARG, "a bead appears twice");
+        require_system(c);
+        MDX_REQUIRE(c.rlist > 0, MDX_ERR_STATE, "mdx_nlist_config not called");
+        MDX_REQUIRE(c.n == c0.n && c.device == c0.device && c.dt == c0.dt, MDX_ERR_ARG,
+                    "beads must hold the same atoms, device and time step");
+        MDX_REQUIRE(!c.rigid.on && !c.baro.on, MDX_ERR_STATE,
+                    "ring polymers take flexible molecules at constant volume");
+        for (int d = 0; d < 3; ++d)
+            MDX_REQUIRE(c.geo.L[d] == c0.geo.L[d] && c.geo.lo[d] == c0.geo.lo[d], MDX_ERR_ARG,
+                        "beads must share the box");
+        cs.push_back(&c);
+    }
+    RingSpec ring;
+    ring.P = P;
+    ring.natoms = c0.n;
+    ring.contract = (flags & MDX_PPPM) && !(flags & MDX_PPPM_PER_BEAD) && c0.pppm.on;
+    if (ring.contract) {
+        for (Ctx *c : cs) {
+            MDX_REQUIRE(c->pppm.on && c->pppm.alphaUser == c0.pppm.alphaUser && c->pppm.rcut == c0.pppm.rcut,
+                        MDX_ERR_ARG, "contracted beads must share the P3M set-up");
+            MDX_REQUIRE(c->pppm.interval <= 1, MDX_ERR_STATE, "ring contraction evaluates the mesh every step");
+            MDX_REQUIRE(c->qHost == c0.qHost, MDX_ERR_ARG, "beads must carry the same charges");
+        }
+    }
+    const int want = ring.contract ? P * c0.n : 0;
+    for (Ctx *c : cs)
+        if (c->pppm.on && c->pppm.alphaAtoms != want) {
+            c->pppm.alphaAtoms = want;
+            c->pppm.alpha = 0.0;  // solved again at the next preparation
+        }
+    if (omegaP > 0) {
+        // normal-mode matrix and per-mode rotation (nve_xPIMD_cu), in FP64 on the host
+        std::vector<double> C((size_t)P * P), cw(3 * (size_t)P);
+        const int h = P / 2;
+        for (int nb = 0; nb < P; ++nb) {
+            const int n = nb + 1;
+            for (int k = 0; k < P; ++k) {
+                double v;
+                if (k == 0) v = std::sqrt(1.0 / P);
+                else if (k < h) v = std::sqrt(2.0 / P) * std::cos(2.0 * M_PI * k * n / P);
+                else if (k == h) v = std::sqrt(1.0 / P) * ((n & 1) ? -1.0 : 1.0);
+                else v = std::sqrt(2.0 / P) * std::sin(2.0 * M_PI * k * n / P);
+                C[(size_t)nb * P + k] = v;
+            }
+        }
+        const double dt = c0.dt;
+        for (int k = 0; k < P; ++k) {
+            if (k == 0) {
+                cw[0] = 1.0;
+                cw[1] = dt;
+                cw[2] = 0.0;
+            } else {
+                const double w = 2.0 * omegaP * std::sin(k * M_PI / P);
+                cw[3 * k] = std::cos(w * dt);
+                cw[3 * k + 1] = std::sin(w * dt) / w;
+                cw[3 * k + 2] = w * std::sin(w * dt);
+            }
+        }
+        ring.C = dalloc<double>(c0, C.size());
+        ring.cw = dalloc<double>(c0, cw.size());
+        upload(c0, ring.C, C.data(), C.size());
+        upload(c0, ring.cw, cw.data(), cw.size());
+    }
+    ring.tab = (void **)dalloc<char>(c0, sizeof(void *) * kRingTab * P);
+    if (ring.contract) {
+        ring.cen = dalloc<double4>(c0, c0.n);
+        ring.q = dalloc<float>(c0, c0.n);
+        ring.cf = dalloc<long long>(c0, 3 * (size_t)c0.n);
+        upload(c0, ring.q, c0.qHost.data(), c0.n);
+        MDX_HIP(hipMemsetAsync(ring.cf, 0, sizeof(long long) * 3 * c0.n, c0.stream));
+    }
+    std::vector<hipStream_t> own;
+    for (Ctx *c : cs) {
+        MDX_HIP(hipStreamSynchronize(c->stream));
+        own.push_back(c->stream);
+        c->stream = c0.stream;
+    }
+    auto restore = [&] {
+        hipStreamSynchronize(c0.stream);
+        for (size_t k = 0; k < cs.size(); ++k) cs[k]->stream = own[k];
+        dfree(c0, ring.C);
+        dfree(c0, ring.cw);
+        dfree(c0, (char *)ring.tab);
+        dfree(c0, ring.cen);
+        dfree(c0, ring.q);
+        dfree(c0, ring.cf);
+    };
+    try {
+        f(cs, ring);
+    } catch (...) {
+        restore();
+        throw;
+    }
+    MDX_HIP(hipStreamSynchronize(c0.stream));
+    restore();
 }
 
 }  // namespace
@@ -1351,6 +1511,15 @@ int mdx_energy_get(mdx_ctx *ctx, double e[4], double virial[6]) {
     });
 }
 
+int mdx_virial_trace_get(mdx_ctx *ctx, double *trace) {
+    return guarded_ro(ctx, [&](Ctx &c) {
+        MDX_REQUIRE(trace, MDX_ERR_ARG, "output required");
+        double h[ACC_N];
+        acc_read(c, h);
+        *trace = h[ACC_VIR] + h[ACC_VIR + 1] + h[ACC_VIR + 2] + h[ACC_VTRACE];
+    });
+}
+
 int mdx_integrator_set(mdx_ctx *ctx, double dt, double ftm_to_v) {
     return guarded(ctx, [&](Ctx &c) {
         MDX_REQUIRE(dt > 0, MDX_ERR_ARG, "dt must be positive");
@@ -1460,82 +1629,29 @@ int mdx_ring_run(mdx_ctx *const *beads, int P, int nsteps, int flags, double ome
     if (!beads || P <= 0) return MDX_ERR_ARG;
     for (int k = 0; k < P; ++k)
         if (!beads[k]) return MDX_ERR_ARG;
-    return guarded(beads[0], [&](Ctx &c0) {
+    return guarded(beads[0], [&](Ctx &) {
         MDX_REQUIRE(nsteps >= 0, MDX_ERR_ARG, "nsteps must be >= 0");
-        MDX_REQUIRE(P >= 2 && P <= 64 && (P & (P - 1)) == 0, MDX_ERR_ARG, "beads: a power of two from 2 to 64");
         MDX_REQUIRE(omegaP > 0, MDX_ERR_ARG, "ring frequency must be positive");
-        std::vector<Ctx *> cs;
-        for (int k = 0; k < P; ++k) {
-            Ctx &c = beads[k]->c;
-            for (Ctx *o : cs) MDX_REQUIRE(o != &c, MDX_ERR_ARG, "a bead appears twice");
-            require_system(c);
-            MDX_REQUIRE(c.rlist > 0, MDX_ERR_STATE, "mdx_nlist_config not called");
-            MDX_REQUIRE(c.n == c0.n && c.device == c0.device && c.dt == c0.dt, MDX_ERR_ARG,
-                        "beads must hold the same atoms, device and time step");
-            MDX_REQUIRE(!c.rigid.on && !c.baro.on, MDX_ERR_STATE,
-                        "ring polymers take flexible molecules at constant volume");
-            for (int d = 0; d < 3; ++d)
-                MDX_REQUIRE(c.geo.L[d] == c0.geo.L[d] && c.geo.lo[d] == c0.geo.lo[d], MDX_ERR_ARG,
-                            "beads must share the box");
-            cs.push_back(&c);
-        }
-        // normal-mode matrix and per-mode rotation (nve_xPIMD_cu), in FP64 on the host
-        std::vector<double> C((size_t)P * P), cw(3 * (size_t)P);
-        const int h = P / 2;
-        for (int nb = 0; nb < P; ++nb) {
-            const int n = nb + 1;
-            for (int k = 0; k < P; ++k) {
-                double v;
-                if (k == 0) v = std::sqrt(1.0 / P);
-                else if (k < h) v = std::sqrt(2.0 / P) * std::cos(2.0 * M_PI * k * n / P);
-                else if (k == h) v = std::sqrt(1.0 / P) * ((n & 1) ? -1.0 : 1.0);
-                else v = std::sqrt(2.0 / P) * std::sin(2.0 * M_PI * k * n / P);
-                C[(size_t)nb * P + k] = v;
+        with_ring(beads, P, flags, omegaP, [&](std::vector<Ctx *> &cs, RingSpec &ring) {
+            run_captured(cs, nsteps, flags & ~MDX_PPPM_PER_BEAD, &ring);
+        });
+    });
+}
+
+int mdx_ring_forces_compute(mdx_ctx *const *beads, int P, int flags) {
+    if (!beads || P <= 0) return MDX_ERR_ARG;
+    for (int k = 0; k < P; ++k)
+        if (!beads[k]) return MDX_ERR_ARG;
+    return guarded(beads[0], [&](Ctx &) {
+        with_ring(beads, P, flags, 0.0, [&](std::vector<Ctx *> &cs, RingSpec &ring) {
+            const int fflags = flags & ~MDX_PPPM_PER_BEAD;
+            for (Ctx *c : cs) {
+                prepare_terms(*c);
+                ensure_list(*c);
             }
-        }
-        const double dt = c0.dt;
-        for (int k = 0; k < P; ++k) {
-            if (k == 0) {
-                cw[0] = 1.0;
-                cw[1] = dt;
-                cw[2] = 0.0;
-            } else {
-                const double w = 2.0 * omegaP * std::sin(k * M_PI / P);
-                cw[3 * k] = std::cos(w * dt);
-                cw[3 * k + 1] = std::sin(w * dt) / w;
-                cw[3 * k + 2] = w * std::sin(w * dt);
-            }
-        }
-        RingSpec ring;
-        ring.P = P;
-        ring.natoms = c0.n;
-        ring.C = dalloc<double>(c0, C.size());
-        ring.cw = dalloc<double>(c0, cw.size());
-        ring.tab = (void **)dalloc<char>(c0, sizeof(void *) * 3 * P);
-        upload(c0, ring.C, C.data(), C.size());
-        upload(c0, ring.cw, cw.data(), cw.size());
-        // every bead runs on bead 0's stream for the ring's step-by-step coupling
-        std::vector<hipStream_t> own;
-        for (Ctx *c : cs) {
-            MDX_HIP(hipStreamSynchronize(c->stream));
-            own.push_back(c->stream);
-            c->stream = c0.stream;
-        }
-        auto restore = [&] {
-            for (size_t k = 0; k < cs.size(); ++k) cs[k]->stream = own[k];
-            dfree(c0, ring.C);
-            dfree(c0, ring.cw);
-            dfree(c0, (char *)ring.tab);
-        };
-        try {
-            run_captured(cs, nsteps, flags, &ring);
-        } catch (...) {
-            hipStreamSynchronize(c0.stream);
-            restore();
-            throw;
-        }
-        MDX_HIP(hipStreamSynchronize(c0.stream));
-        restore();
+            ring_table(cs, ring);
+            ring_forces(cs, ring, fflags);
+        });
     });
 }
 

@@ -37,6 +37,7 @@ struct E3BArgs {
     unsigned long long *acc;
     int three;    // 0: two-body term only
     float *emol;  // optional: three-body energy owned by each molecule
+    const int *vgate;  // EVM 2 inside a captured step: the barostat gate (0: no trace this step)
 };
 
 __device__ __forceinline__ float3 mi3(float4 a, float4 b, const E3BArgs &p) {
@@ -181,7 +182,7 @@ __global__ void __launch_bounds__(256) k_e3b_pass1(E3BArgs p) {
         const double v[7] = {e2, 0.5 * w0, 0.5 * w1, 0.5 * w2, 0.5 * w3, 0.5 * w4, 0.5 * w5};
         block_acc7(v, ACC_E3B2, p.acc);
     }
-    if (VT) block_acc1(0.5 * w0, ACC_VTRACE, p.acc);
+    if (VT && (!p.vgate || *p.vgate)) block_acc1(0.5 * w0, ACC_VTRACE, p.acc);
 }
 
 template <int P, int EVM>
@@ -289,9 +290,7 @@ __global__ void __launch_bounds__(256) k_e3b_pass2(E3BArgs p) {
         const double v[7] = {e3, w0, w1, w2, w3, w4, w5};
         block_acc7(v, ACC_E3B3, p.acc);
     }
-    if (VT) {
-        block_acc1(w0, ACC_VTRACE, p.acc);
-    }
+    if (VT && (!p.vgate || *p.vgate)) block_acc1(w0, ACC_VTRACE, p.acc);
 }
 
 template <int P>
@@ -350,6 +349,7 @@ void e3b_forces(Ctx &c, bool energy, bool virial, bool three_body, float *emol) 
     a.acc = c.acc;
     a.three = three_body ? 1 : 0;
     a.emol = emol;
+    a.vgate = baro_gate(c);
     const int evm = (energy || virial || emol) ? 1 : (c.vtrace ? 2 : 0);
     switch (E.team) {
         case 4: launch<4>(c, a, evm); break;

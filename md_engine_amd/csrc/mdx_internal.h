@@ -244,6 +244,7 @@ struct PppmState {
     double qqr = 332.06371;
     double alpha = 0.0;
     double alphaUser = -1.0;   // > 0: fixed alpha, else secant solve
+    int alphaAtoms = 0;        // > 0: atom count of the secant solve (a ring's beads x atoms), else n
     double totalQ2 = 0.0;
     double err = 0.0;
     double boxAtGreen[3] = {0, 0, 0};
@@ -375,7 +376,7 @@ struct RigidState {
 
 // compute flag (internal): the virial's trace is enough (a barostat step);
 // the pair and E3B kernels then add r . F into ACC_VTRACE only
-constexpr int kFlagVirialTrace = 1 << 12;
+constexpr int kFlagVirialTrace = 1 << 12;  // MDX_VIRIAL_TRACE of the C ABI
 
 struct Ctx {
     int device = 0;
@@ -396,6 +397,7 @@ struct Ctx {
     bool fuseMolVirial = false;  // captured barostat steps: rigid second half sums ACC_MOL
     bool fuseThermo = false;     // captured thermostat steps: rigid second half sums m v^2
     bool fuseCheck = false;      // captured steps: rigid first half gives the skin check's maxima
+    bool deferMSpread = false;   // compute leaves the M-site forces for a later flex_spread_m (ring contraction)
     hipStream_t stream = nullptr;
     std::string err;
 
@@ -784,14 +786,25 @@ bool pppm_green_device(Ctx &c, const int *gate, hipStream_t s, const DilateArgs 
 void rigid_first_half(Ctx &c);
 // ring.hip: ring-polymer beads, one context per bead, coupled by the
 // normal-mode propagation of the free ring polymer
+constexpr int kRingTab = 4;  // pointers per bead in RingSpec::tab
 struct RingSpec {
     int P = 0;            // beads
     int natoms = 0;       // atoms per bead
     double *C = nullptr;  // P x P orthonormal normal-mode matrix, C[n * P + k]
     double *cw = nullptr; // per mode k: cos(w_k dt), sin(w_k dt) / w_k, w_k sin(w_k dt)
-    void **tab = nullptr; // per bead: pos, vel, idToIdx (device pointers)
+    void **tab = nullptr; // per bead: pos, vel, idToIdx, force (device pointers)
+    // ring-polymer contraction of the mesh part (false: every bead its own P3M)
+    bool contract = false;
+    double4 *cen = nullptr;   // centroids by atom id
+    float *q = nullptr;       // charges by atom id
+    long long *cf = nullptr;  // centroid forces by atom id, fixed point (kept zero between uses)
 };
 void ring_drift(const RingSpec &r, const BoxDev *box, hipStream_t s);
+void ring_centroids(const RingSpec &r, const BoxDev *box, hipStream_t s);
+void ring_apply(const RingSpec &r, hipStream_t s);
+// pppm.hip: the mesh part of every bead from the ring's centroids, on the
+// first bead's mesh (centroid force into r.cf, then onto every bead)
+void ring_pppm(Ctx &c0, const RingSpec &r);
 
 // flex.hip: molecule indices after a re-sort, M placement, intramolecular
 // forces and the M force spread onto O, H1, H2

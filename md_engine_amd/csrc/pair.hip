@@ -41,10 +41,12 @@ struct PairArgs {
     float qqr, rq2;
     unsigned long long *force;
     unsigned long long *acc;
+    const int *vgate;  // EVM 2 inside a captured step: the barostat gate (0: no trace this step)
 };
 
 // EVM: 0 forces only; 1 energy and virial tensor; 2 the virial's trace only
-// (sum r F, what a barostat needs), into ACC_VTRACE
+// (sum r F, what a barostat needs), into ACC_VTRACE, skipped on steps whose
+// barostat gate is 0
 template <int P, bool LJ, bool COUL, int EVM>
 __global__ void __launch_bounds__(256) k_pair(PairArgs a) {
     constexpr bool EV = EVM == 1, VT = EVM == 2;
@@ -156,7 +158,7 @@ __global__ void __launch_bounds__(256) k_pair(PairArgs a) {
         fx_add(&a.force[3 * i + 1], fy);
         fx_add(&a.force[3 * i + 2], fz);
     }
-    if (VT) {
+    if (VT && (!a.vgate || *a.vgate)) {  // one value for the whole workgroup
         double x = team_sum_d<64>(0.5 * (double)v0);
         const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
         if (lane == 0) wred[w][0] = x;
@@ -253,6 +255,7 @@ void pair_forces(Ctx &c, bool energy, bool virial) {
     a.rq2 = c.rq2;
     a.force = c.force;
     a.acc = c.acc;
+    a.vgate = baro_gate(c);
     const int evm = (energy || virial) ? 1 : (c.vtrace ? 2 : 0);
     ProfScope ps(c, PK_PAIR);
     switch (c.pairTeam) {

@@ -413,10 +413,13 @@ __global__ void k_efield(const BoxDev *__restrict__ bx, int szh, const float *__
 
 // Energy 0.5 sum |rho_hat|^2 G / V over the full spectrum and the k-space
 // virial (virials_cu): half-spectrum entries with 0 < iz < sz/2 count twice.
-__global__ void __launch_bounds__(256) k_kspace_ev(const BoxDev *__restrict__ bx, int szh,
-                                                   const float *__restrict__ G,
+// gate (virial for a barostat inside a captured step): nothing on steps
+// whose barostat gate is 0.
+__global__ void __launch_bounds__(256) k_kspace_ev(const int *__restrict__ gate, const BoxDev *__restrict__ bx,
+                                                   int szh, const float *__restrict__ G,
                                                    const float2 *__restrict__ rho,
                                                    unsigned long long *__restrict__ acc) {
+    if (gate && *gate == 0) return;  // the whole grid returns together
     const MeshGeo m = bx->mesh;
     const double alpha = bx->alpha, invV = 1.0 / bx->vol;
     __shared__ double wred[4][7];
@@ -719,7 +722,8 @@ void pppm_prepare(Ctx &c) {
         P.alpha = P.alphaUser;
         P.err = 0.0;
     } else {
-        P.alpha = pppm_solve_alpha(c.n, P.totalQ2, c.geo.L, sz, P.order, P.rcut, &P.err);
+        P.alpha = pppm_solve_alpha(P.alphaAtoms > 0 ? P.alphaAtoms : c.n, P.totalQ2, c.geo.L, sz, P.order, P.rcut,
+                                   &P.err);
     }
     c.alpha = (float)P.alpha;
     box_upload(c);  // the Green function reads alpha and the mesh from the device box
@@ -791,7 +795,9 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
                 "hipfftExecR2C failed");
     }
     if (energy || virial) {
-        hipLaunchKernelGGL(k_kspace_ev, dim3(cdiv(nh, threads)), dim3(threads), 0, c.stream, c.box, szh,
+        // a trace-only request (a barostat's) is skipped on steps it does not serve
+        const int *gate = c.vtrace ? baro_gate(c) : nullptr;
+        hipLaunchKernelGGL(k_kspace_ev, dim3(cdiv(nh, threads)), dim3(threads), 0, c.stream, gate, c.box, szh,
                            P.green, P.rhoHat, c.acc);
     }
     {
@@ -821,6 +827,64 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
         default: if (wide) MDX_GATHER(7, 16); else MDX_GATHER(7, 8); break;
     }
 #undef MDX_GATHER
+}
+
+// Mesh part of a ring polymer on the first bead's mesh: charges spread from
+// the centroids (by id), field gathered at the centroids into r.cf, then the
+// centroid force onto every bead (FixChargeEwald.cu:1255-1266, 1395-1405).
+// Same kernels as a single system's chain; alpha and G are the first bead's.
+void ring_pppm(Ctx &c0, const RingSpec &r) {
+    PppmState &P = c0.pppm;
+    MDX_REQUIRE(P.on && P.plansMade, 3, "ring contraction needs P3M set up on the first bead");
+    const int *sz = P.sz;
+    const int szh = sz[2] / 2 + 1;
+    const int ng = sz[0] * sz[1] * sz[2];
+    const int nh = sz[0] * sz[1] * szh;
+    const float qunit = (float)std::sqrt(P.qqr);
+    const int threads = 256;
+    hipStream_t s = c0.stream;
+    ring_centroids(r, c0.box, s);
+    {
+        ProfScope ps(c0, PK_SPREAD);
+        switch (P.order) {
+            case 1: hipLaunchKernelGGL(k_spread<1>, dim3(cdiv(r.natoms, threads)), dim3(threads), 0, s, r.natoms, r.cen, r.q, c0.box, qunit, (unsigned long long *)P.mesh); break;
+            case 3: hipLaunchKernelGGL(k_spread<3>, dim3(cdiv(r.natoms, threads)), dim3(threads), 0, s, r.natoms, r.cen, r.q, c0.box, qunit, (unsigned long long *)P.mesh); break;
+            case 5: hipLaunchKernelGGL(k_spread<5>, dim3(cdiv(r.natoms, threads)), dim3(threads), 0, s, r.natoms, r.cen, r.q, c0.box, qunit, (unsigned long long *)P.mesh); break;
+            default: hipLaunchKernelGGL(k_spread<7>, dim3(cdiv(r.natoms, threads)), dim3(threads), 0, s, r.natoms, r.cen, r.q, c0.box, qunit, (unsigned long long *)P.mesh); break;
+        }
+        hipLaunchKernelGGL(k_mesh_to_real, dim3(cdiv(ng, threads)), dim3(threads), 0, s, ng, P.mesh, P.real);
+    }
+    if (P.planStream != s) {
+        MDX_REQUIRE(hipfftSetStream(P.planR2C, s) == HIPFFT_SUCCESS && hipfftSetStream(P.planC2R, s) == HIPFFT_SUCCESS,
+                    2, "hipfftSetStream failed");
+        P.planStream = s;
+    }
+    {
+        ProfScope ps(c0, PK_FFT);
+        MDX_REQUIRE(hipfftExecR2C(P.planR2C, P.real, (hipfftComplex *)P.rhoHat) == HIPFFT_SUCCESS, 2,
+                    "hipfftExecR2C failed");
+    }
+    {
+        ProfScope ps(c0, PK_EFIELD);
+        hipLaunchKernelGGL(k_efield, dim3(cdiv(nh, threads)), dim3(threads), 0, s, c0.box, szh, P.green, P.rhoHat,
+                           P.eHat);
+    }
+    {
+        ProfScope ps(c0, PK_FFT);
+        MDX_REQUIRE(hipfftExecC2R(P.planC2R, (hipfftComplex *)P.eHat, P.eReal) == HIPFFT_SUCCESS, 2,
+                    "hipfftExecC2R failed");
+    }
+    ProfScope ps(c0, PK_GATHER);
+    hipLaunchKernelGGL(k_field_pack, dim3(cdiv(ng, threads)), dim3(threads), 0, s, ng, P.eReal, P.eField4);
+    const dim3 grid(xcd_grid(cdiv((long long)r.natoms * 8, threads)));
+    unsigned long long *cf = (unsigned long long *)r.cf;
+    switch (P.order) {
+        case 1: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gather<1, 8>), grid, dim3(threads), 0, s, r.natoms, r.cen, r.q, c0.box, P.eField4, cf, nullptr, nullptr); break;
+        case 3: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gather<3, 8>), grid, dim3(threads), 0, s, r.natoms, r.cen, r.q, c0.box, P.eField4, cf, nullptr, nullptr); break;
+        case 5: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gather<5, 8>), grid, dim3(threads), 0, s, r.natoms, r.cen, r.q, c0.box, P.eField4, cf, nullptr, nullptr); break;
+        default: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gather<7, 8>), grid, dim3(threads), 0, s, r.natoms, r.cen, r.q, c0.box, P.eField4, cf, nullptr, nullptr); break;
+    }
+    ring_apply(r, s);
 }
 
 void pppm_step(Ctx &c, bool energy, bool virial) {

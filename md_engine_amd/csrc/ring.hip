@@ -9,10 +9,18 @@
 // first put in the periodic image of bead 0, as the reference does.
 //
 // Here each bead is a whole system in its own context (its own neighbor
-// list, P3M and cell order), so atom `id` of bead n sits at idToIdx_n[id].
+// list and cell order), so atom `id` of bead n sits at idToIdx_n[id].
 // One team of P lanes per atom: lane n loads bead n, computes mode n, and
 // after an LDS exchange bead n again.  Massless sites (M) are left alone;
 // the force evaluation places them.
+//
+// Ring-polymer contraction of the mesh part (FixChargeEwald.cu:1255-1266,
+// 1395-1405): the charges are spread from the centroid of each atom's ring
+// (computeCentroid, FixChargeEwald.cu:22-50: bead 0 plus the mean of the
+// other beads' minimum-image offsets from it, wrapped into the box), the
+// field is interpolated at the centroid, and the whole force q E goes onto
+// every bead of the atom.  k_ring_centroid and k_ring_apply are the two ends
+// of that; the mesh work in between is the single-system P3M chain.
 #include "mdx_internal.h"
 
 namespace mdx {
@@ -31,9 +39,9 @@ __global__ void __launch_bounds__(kRingThreads) k_ring(int natoms, int P, void *
     const int id = blockIdx.x * teams + team;
     double *me = sh + (size_t)team * P * 6;
     const Geometry g = bx->geo;
-    double4 *pos = (double4 *)tab[3 * n];
-    double4 *vel = (double4 *)tab[3 * n + 1];
-    const int *idToIdx = (const int *)tab[3 * n + 2];
+    double4 *pos = (double4 *)tab[kRingTab * n];
+    double4 *vel = (double4 *)tab[kRingTab * n + 1];
+    const int *idToIdx = (const int *)tab[kRingTab * n + 2];
     const bool live = id < natoms;
     int idx = 0;
     double4 x = make_double4(0, 0, 0, 0), v = make_double4(0, 0, 0, 0);
@@ -105,7 +113,61 @@ __global__ void __launch_bounds__(kRingThreads) k_ring(int natoms, int P, void *
     }
 }
 
+// centroid of every atom's ring, by atom id (FP64; the reference's is in its
+// working precision)
+__global__ void __launch_bounds__(256) k_ring_centroid(int natoms, int P, void *const *__restrict__ tab,
+                                                       const BoxDev *__restrict__ bx, double4 *__restrict__ cen) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= natoms) return;
+    const Geometry g = bx->geo;
+    const double4 x0 = ((const double4 *)tab[0])[((const int *)tab[2])[id]];
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int n = 1; n < P; ++n) {
+        const double4 xn = ((const double4 *)tab[kRingTab * n])[((const int *)tab[kRingTab * n + 2])[id]];
+        double dd[3] = {xn.x - x0.x, xn.y - x0.y, xn.z - x0.z};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (g.periodic[d]) dd[d] -= g.L[d] * rint(dd[d] * g.invL[d]);
+            s[d] += dd[d];
+        }
+    }
+    double c[3] = {x0.x + s[0] / P, x0.y + s[1] / P, x0.z + s[2] / P};
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        if (g.periodic[d]) c[d] -= g.L[d] * floor((c[d] - g.lo[d]) * g.invL[d]);
+    cen[id] = make_double4(c[0], c[1], c[2], 0.0);
+}
+
+// the centroid force (fixed point, by id) onto every bead of the atom; the
+// by-id buffer is cleared for the next evaluation
+__global__ void __launch_bounds__(256) k_ring_apply(int natoms, int P, void *const *__restrict__ tab,
+                                                    unsigned long long *__restrict__ cf) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= natoms) return;
+    const unsigned long long f0 = cf[3 * id], f1 = cf[3 * id + 1], f2 = cf[3 * id + 2];
+    if ((f0 | f1 | f2) == 0ull) return;
+    cf[3 * id] = 0ull;
+    cf[3 * id + 1] = 0ull;
+    cf[3 * id + 2] = 0ull;
+    for (int n = 0; n < P; ++n) {
+        unsigned long long *f = (unsigned long long *)tab[kRingTab * n + 3];
+        const int i = ((const int *)tab[kRingTab * n + 2])[id];
+        atomicAdd(&f[3 * i + 0], f0);
+        atomicAdd(&f[3 * i + 1], f1);
+        atomicAdd(&f[3 * i + 2], f2);
+    }
+}
+
 }  // namespace
+
+void ring_centroids(const RingSpec &r, const BoxDev *box, hipStream_t s) {
+    hipLaunchKernelGGL(k_ring_centroid, dim3(cdiv(r.natoms, 256)), dim3(256), 0, s, r.natoms, r.P, r.tab, box, r.cen);
+}
+
+void ring_apply(const RingSpec &r, hipStream_t s) {
+    hipLaunchKernelGGL(k_ring_apply, dim3(cdiv(r.natoms, 256)), dim3(256), 0, s, r.natoms, r.P, r.tab,
+                       (unsigned long long *)r.cf);
+}
 
 void ring_drift(const RingSpec &r, const BoxDev *box, hipStream_t s) {
     const int teams = kRingThreads / r.P;

@@ -182,6 +182,37 @@ class Grid:
         counts, rows = eng.nlist_get()
         return [sorted(int(x) for x in (r & 0x3FFFFFFF)) for r in rows]
 
+    def verifyNeighborlists(self, neighCut):
+        """GridGPU::verifyNeighborlists (GridGPU.cu:1109-1200): every atom's
+        neighbor ids against an all-pairs host search (minimum image,
+        r^2 < neighCut^2) at the positions the list was built from; the first
+        atom that differs is printed.  The list holds every pair within its
+        own cutoff (r_cut + padding), so neighCut is that cutoff.  Returns
+        whether all rows agree (the reference prints and returns true)."""
+        eng = self.state._engine(rebuild=True)
+        eng.nlist_update(force=True)
+        counts, rows = eng.nlist_get()
+        x, _ = eng.get_system()
+        lo, L = (np.asarray(a, np.float64) for a in eng.get_box())
+        per = np.array([bool(p) for p in self.state.bounds.periodic[:3]])
+        cut2 = float(neighCut) ** 2
+        n = len(x)
+        invL = 1.0 / L
+        for i in range(n):
+            d = x - x[i]
+            d = np.where(per, d - L * np.rint(d * invL), d)
+            r2 = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]
+            r2[i] = np.inf
+            cpu = np.nonzero(r2 < cut2)[0]
+            gpu = np.sort((rows[i] & 0x3FFFFFFF).astype(np.int64))
+            if not np.array_equal(cpu, gpu):
+                print(f"problem at id {i}")
+                print(f"cpu {len(cpu)} gpu {len(gpu)}")
+                print("cpu neighbor ids\n" + " ".join(map(str, cpu)))
+                print("gpu neighbor ids\n" + " ".join(map(str, gpu)))
+                return False
+        return True
+
 
 class State:
     """Holds the system and the active fixes; the engine is built from them at

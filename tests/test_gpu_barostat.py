@@ -104,37 +104,70 @@ def test_barostat_run(orc, gpu_engine_factory):
     assert_same_lists(gc, grows, oc, orows)
 
 
-@pytest.mark.parametrize("thermo", [None, (350.0, 3, 0.0), (330.0, 1, 40.0)],
-                         ids=["nve", "rescale-every-3", "berendsen-every-1"])
-def test_barostat_graph_matches_eager(orc, gpu_engine_factory, thermo):
+@pytest.mark.parametrize("thermo,every,mesh", [(None, 1, True), ((350.0, 3, 0.0), 1, True),
+                                               ((330.0, 1, 40.0), 1, True), (None, 3, True), (None, 1, False)],
+                         ids=["nve", "rescale-every-3", "berendsen-every-1", "barostat-every-3", "no-mesh"])
+def test_barostat_graph_matches_eager(orc, gpu_engine_factory, thermo, every, mesh):
     """The barostat runs on the device inside captured steps: box, pressure,
     positions and the re-solved Ewald splitting agree with eager steps, and
     alpha equals the oracle's secant solve for the final box.  Captured steps
     sum the centre-of-mass kinetic term before a velocity rescale of the same
-    step and scale it by the rescale's lambda^2; the thermostat cases pin that."""
-    say(f"Berendsen barostat every step, thermostat {thermo}: captured graph vs eager steps")
+    step and scale it by the rescale's lambda^2; the thermostat cases pin that.
+    Every 3 steps: the virial-trace reductions and the k-space virial are
+    skipped on the steps in between (device gate).  Without a mesh the
+    dilation runs as its own kernel instead of riding on the Green function's."""
+    say(f"Berendsen barostat every {every} step(s), thermostat {thermo}, mesh {mesh}: captured graph vs eager")
     box = systems.tip4p2005_box(216, seed=35, min_sep=1.8)
     v = start_velocities(orc, box, seed=9)
     mvv = systems.REAL_MVV_TO_ENG
+    sz = (32, 32, 32) if mesh else None
     out = []
     for graph in (0, 1):
-        e, flags = water_md_engine(gpu_engine_factory, box, v, rc=7.0, skin=1.0, sz=(32, 32, 32), order=5,
+        e, flags = water_md_engine(gpu_engine_factory, box, v, rc=7.0, skin=1.0, sz=sz, order=5,
                                    e3b=True, dt=1.0)
-        e.barostat(True, 1.0, 200.0, 1, 10.0, 1e-4, NKTV2P, mvv)
+        e.barostat(True, 1.0, 200.0, every, 10.0, 1e-4, NKTV2P, mvv)
         if thermo:
             e.thermostat(True, thermo[0], thermo[1], thermo[2], systems.REAL_BOLTZ, mvv)
         e.option("graph", graph)
         e.run(40, flags)
         x, vv = e.get_system()
         lo, L = e.get_box()
-        out.append((x, L, e.barostat_last(), e.pppm_info()[0]))
+        out.append((x, L, e.barostat_last(), e.pppm_info()[0] if mesh else None))
+        e.close()
     (xa, La, pa, aa), (xb, Lb, pb, ab) = out
-    say(f"box {La[0]:.9f} / {Lb[0]:.9f}, P {pa:.3f} / {pb:.3f} atm, alpha {aa:.12f} / {ab:.12f}")
+    say(f"box {La[0]:.9f} / {Lb[0]:.9f}, P {pa:.3f} / {pb:.3f} atm, alpha {aa} / {ab}")
     assert La[0] != box["L"][0]
     assert np.allclose(La, Lb, rtol=1e-10, atol=0)
     assert pb == pytest.approx(pa, rel=1e-4, abs=0.05)
     assert np.abs(min_image_diff(xa, xb, La)).max() < 1e-6
-    q2 = orc.total_q2(box["q"], systems.REAL_QQR_TO_ENG)
-    a_orc, _ = orc.pppm_alpha(len(box["x"]), q2, Lb, np.array([32, 32, 32]), 5, 7.0)
-    assert ab == pytest.approx(a_orc, rel=1e-12)
-    assert aa == pytest.approx(a_orc, rel=1e-12)
+    if mesh:
+        q2 = orc.total_q2(box["q"], systems.REAL_QQR_TO_ENG)
+        a_orc, _ = orc.pppm_alpha(len(box["x"]), q2, Lb, np.array([32, 32, 32]), 5, 7.0)
+        assert ab == pytest.approx(a_orc, rel=1e-12)
+        assert aa == pytest.approx(a_orc, rel=1e-12)
+
+
+@pytest.mark.parametrize("terms", ["pair", "pppm", "e3b", "all"])
+def test_virial_trace_matches_full_tensor(gpu_engine_factory, terms):
+    """The barostat reads the virial through the trace-only kernels (one r.F
+    sum per pair or bond into its own slot); their trace must equal the trace
+    of the full tensor of the same configuration, term by term."""
+    say(f"trace-only virial vs trace of the full tensor ({terms})")
+    from md_engine_amd import E3B, PAIR, PPPM, VIRIAL, VIRIAL_TRACE
+
+    box = systems.tip4p2005_box(343, seed=36, min_sep=1.8)
+    e, _ = water_md_engine(gpu_engine_factory, box, np.zeros_like(box["x"]), rc=7.0, skin=1.0, sz=(32, 32, 32),
+                           order=5, e3b=True, dt=1.0)
+    flags = {"pair": PAIR, "pppm": PPPM, "e3b": E3B, "all": PAIR | PPPM | E3B}[terms]
+    e.compute(flags | VIRIAL)
+    _, vir = e.energies()
+    w_full = e.virial_trace()
+    assert w_full == pytest.approx(vir[:3].sum(), rel=1e-12, abs=1e-9)
+    e.compute(flags | VIRIAL_TRACE)
+    w_tr = e.virial_trace()
+    scale = np.abs(vir[:3]).sum()
+    say(f"trace {w_full:.6f} full tensor, {w_tr:.6f} trace-only (scale {scale:.3f})")
+    assert scale > 0
+    # FP32 pair arithmetic summed in another order: ~1e-6 of the diagonal's size
+    assert abs(w_tr - w_full) <= 1e-5 * scale
+    e.close()

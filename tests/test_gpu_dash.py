@@ -103,3 +103,7 @@ def test_water_script_matches_engine(orc, gpu_engine_factory):
     assert np.array_equal(vs, ve)
     nl = st.grid.getNeighborList()
     assert len(nl) == len(st.atoms) and all(i not in row for i, row in enumerate(nl))
+    # GridGPU::verifyNeighborlists: the list against an all-pairs host search
+    # at its own cutoff (r_cut + padding); a shorter cutoff must disagree
+    assert st.grid.verifyNeighborlists(st.rCut + st.padding)
+    assert not st.grid.verifyNeighborlists(st.rCut)

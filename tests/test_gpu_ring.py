@@ -1,14 +1,16 @@
 """Ring-polymer (PIMD) steps on the GPU (mdx_ring_run, md_engine_amd/csrc/
 ring.hip): the free ring polymer against the exact normal-mode propagation of
-oracle/ring.py, and the ring Hamiltonian of flexible q-TIP4P/F water beads
-conserved to second order in dt."""
+oracle/ring.py, every bead's forces against the FP64 oracle with the mesh
+part contracted to the centroids (the reference's scheme) and per bead, and
+the ring Hamiltonian of flexible q-TIP4P/F water beads conserved to second
+order in dt in both schemes."""
 import sys
 
 import numpy as np
 import pytest
 
 from md_engine_amd import systems
-from md_engine_amd._lib import ring_omega, run_ring
+from md_engine_amd._lib import PPPM_PER_BEAD, ring_forces, ring_omega, run_ring
 from oracle import flexwater as FW
 from oracle import ring as R
 
@@ -77,11 +79,59 @@ def test_free_ring_matches_normal_modes(orc, gpu_engine_factory):
         e.close()
 
 
-def test_ring_hamiltonian_second_order(orc, gpu_engine_factory):
+@pytest.mark.parametrize("mode", ["contracted", "per-bead"])
+def test_ring_forces_vs_oracle(orc, gpu_engine_factory, mode):
+    """Every bead's force: LJ + Ewald real space of the bead's own
+    configuration, the q-TIP4P/F intramolecular terms, and the mesh part
+    either from the ring's centroids (whole force on every bead) or per bead,
+    M-site forces spread onto O and H; relative RMSE <= 1e-4 per bead."""
+    say(f"ring forces, 4 beads of 64 molecules, mesh part {mode}")
+    from test_gpu_parity import oracle_rows_by_id
+
+    P, rc, skin, sz = 4, 5.5, 0.5, (32, 32, 32)
+    box = systems.tip4p2005_box(64, seed=37, min_sep=1.8)
+    xs = bead_systems(box, P, 4, spread=0.08)
+    vs = [np.zeros_like(x) for x in xs]
+    es, flags = ring_engines(gpu_engine_factory, box, xs, vs, dt=0.5, rc=rc, skin=skin, sz=sz)
+    ring_forces(es, flags | (PPPM_PER_BEAD if mode == "per-bead" else 0))
+    L, lo, mol = box["L"], box["lo"], box["mol"]
+    n = len(box["x"])
+    qqr = systems.REAL_QQR_TO_ENG
+    q2 = orc.total_q2(box["q"], qqr)
+    xa = np.array(xs)
+    if mode == "contracted":
+        alpha, fmesh = R.contracted_pppm(xa, box["q"], lo, L, sz, 5, qqr, rc)
+    excl = orc.exclusions_bfs(n, systems.water_bonds(mol))
+    eps24, sig6 = systems.water_lj_tables()
+    gamma = FW.QTIP4PF["gamma"]
+    for k, e in enumerate(es):
+        x = xa[k]
+        if mode == "per-bead":
+            alpha, _ = orc.pppm_alpha(n, q2, L, np.array(sz), 5, rc)
+            G = orc.pppm_green(L, np.array(sz), alpha, 5)
+            fmesh = orc.pppm(x, box["q"], lo, L, np.array(sz), 5, qqr, alpha, q2, G)["f"]
+        assert e.pppm_info()[0] == alpha
+        oc, orows, xw = oracle_rows_by_id(orc, x, lo, L, rc + skin, excl=excl, brute=False)
+        fp, _, _ = orc.pair(xw, box["q"], box["type"], L, oc, orows, np.full((3, 3), rc * rc), eps24, sig6,
+                            with_charge=True, alpha=alpha, qqr=qqr, rq=rc)
+        fb, _, _ = FW.bonded(mol, x, L)
+        ref = FW.spread_m(mol, fp + fmesh, gamma) + fb
+        f = e.forces()
+        rel = float(np.sqrt(((f - ref) ** 2).sum() / (ref ** 2).sum()))
+        say(f"bead {k}: relative force RMSE {rel:.2e}")
+        assert rel < 1e-4, f"bead {k}"
+    for e in es:
+        e.close()
+
+
+@pytest.mark.parametrize("mode", ["contracted", "per-bead"])
+def test_ring_hamiltonian_second_order(orc, gpu_engine_factory, mode):
     """H = sum over beads of (KE + V) + the ring springs; its deviation over the
-    same 40 fs falls about fourfold when dt halves."""
-    say("flexible water ring, 4 beads: energy error vs dt")
-    from md_engine_amd import ENERGY
+    same 40 fs falls about fourfold when dt halves.  With the contraction the
+    mesh part of H is P times the P3M energy at the centroids (evaluated by a
+    separate single-system context at the ring's alpha)."""
+    say(f"flexible water ring, 4 beads: energy error vs dt, mesh part {mode}")
+    from md_engine_amd import ENERGY, PPPM
 
     P = 4
     box = systems.tip4p2005_box(64, seed=32, min_sep=1.8)
@@ -91,14 +141,23 @@ def test_ring_hamiltonian_second_order(orc, gpu_engine_factory):
     mvv = systems.REAL_MVV_TO_ENG
     L = box["L"]
     errs = []
+    contracted = mode == "contracted"
     for dt, nsteps in ((0.2, 200), (0.1, 400)):
         es, flags = ring_engines(gpu_engine_factory, box, xs, vs, dt=dt)
+        run_flags = flags | (0 if contracted else PPPM_PER_BEAD)
+        ring_forces(es, run_flags)  # alpha of the scheme set on every bead
+        cen = None
+        if contracted:
+            # the mesh term at the centroids, on a context of its own
+            cen, _ = flex_engine(gpu_engine_factory, box, np.zeros_like(box["x"]), rc=5.5, skin=0.5,
+                                 sz=(32, 32, 32), order=5, dt=dt)
+            cen.pppm_set((32, 32, 32), 5, 5.5, systems.REAL_QQR_TO_ENG, alpha=es[0].pppm_info()[0])
 
         def total():
             h, xb = 0.0, []
             ke = 0.0
             for e in es:
-                e.compute(flags | ENERGY)
+                e.compute((flags & ~PPPM if contracted else flags) | ENERGY)
                 en, _ = e.energies()
                 x, v = e.get_system()
                 k = 0.5 * mvv * (box["mass"][:, None] * v ** 2).sum()
@@ -106,6 +165,10 @@ def test_ring_hamiltonian_second_order(orc, gpu_engine_factory):
                 ke += k
                 xb.append(x)
             xb = np.array(xb)
+            if contracted:
+                cen.set_positions(R.centroids(xb, box["lo"], L))
+                cen.compute(PPPM | ENERGY)
+                h += P * cen.energies()[0][1]
             xb = xb[0] + (xb - xb[0]) - L * np.rint((xb - xb[0]) / L)
             total.ke = ke
             return h + R.spring_energy(xb, box["mass"], wP, mvv)
@@ -114,10 +177,10 @@ def test_ring_hamiltonian_second_order(orc, gpu_engine_factory):
         ke0 = total.ke
         dev = 0.0
         for _ in range(10):
-            run_ring(es, nsteps // 10, flags, wP)
+            run_ring(es, nsteps // 10, run_flags, wP)
             dev = max(dev, abs(total() - h0))
         errs.append(dev)
-        for e in es:
+        for e in es + ([cen] if cen else []):
             e.close()
     say(f"max |H - H0|: dt 0.2 fs {errs[0]:.5f}, dt 0.1 fs {errs[1]:.5f} kcal/mol")
     assert 0 < errs[0] < 0.01 * ke0

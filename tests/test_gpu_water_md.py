@@ -37,9 +37,14 @@ def water_md_engine(make, box, v, rc, skin, sz, order, e3b, dt):
     e.nlist_config(rc + skin, skin)
     eps24, sig6 = systems.water_lj_tables()
     e.lj_set(np.full((3, 3), rc * rc), eps24, sig6)
-    e.pppm_set(sz, order, rc, systems.REAL_QQR_TO_ENG)
-    e.ewald_real_set(True, -1.0, systems.REAL_QQR_TO_ENG, rc)
-    flags = PAIR | PPPM
+    flags = PAIR
+    if sz is not None:
+        e.pppm_set(sz, order, rc, systems.REAL_QQR_TO_ENG)
+        e.ewald_real_set(True, -1.0, systems.REAL_QQR_TO_ENG, rc)
+        flags |= PPPM
+    else:
+        # no mesh: Ewald real space alone at a fixed splitting
+        e.ewald_real_set(True, 0.3, systems.REAL_QQR_TO_ENG, rc)
     if e3b:
         k = 0.23900573614
         e.e3b_set(box["mol"], [453000 * k, 150 * k, -1005 * k, 1880 * k, 4.872, 1.907, 5.0, 5.2])
