@@ -322,7 +322,10 @@ int mdx_sync(mdx_ctx *ctx);
  * test fewer candidates); "gather_team" 8 or 16 lanes per charged atom in
  * the P3M field gather; "sur_force_lanes" 8, 16, 32 or 64 lanes per molecule
  * in the learned term's force pass; "row_cap" >= 16 sets the neighbor-row capacity after
- * the first build (rows still grow on overflow). */
+ * the first build (rows still grow on overflow); "replay_chunk" k >= 1 makes the
+ * k-th 64-step chunk of every following mdx_run / mdx_run_batch call replay once
+ * from its snapshot as if its rows had overflowed (0: off), for testing the
+ * replay path. */
 int mdx_option_set(mdx_ctx *ctx, const char *name, int value);
 
 /* Force-path steps at fixed positions: nsteps evaluations of the terms in

@@ -235,6 +235,18 @@ void snapshot_save(Ctx &c) {
     c.snap.turn = c.turn;
     c.snap.rebuilds = c.nRebuilds;
     c.snap.geo = c.geo;
+    // A list carried into the chunk was built earlier, from other positions:
+    // its rows (their order sets the FP32 summation order of the pair sums)
+    // and the skin check's reference positions are the build's.  Keep the
+    // positions and box it was built from, so a replay starts from the same list.
+    c.snap.listValid = c.listValid && c.nlistConfigured && c.posLast && c.box && c.boxLast;
+    if (c.snap.listValid) {
+        if (!c.snap.posLast) c.snap.posLast = dalloc<double4>(c, c.n);
+        if (!c.snap.boxLast) c.snap.boxLast = dalloc<BoxDev>(c, 1);
+        MDX_HIP(hipMemcpyAsync(c.snap.posLast, c.posLast, sizeof(double4) * c.n, hipMemcpyDeviceToDevice,
+                               c.stream));
+        MDX_HIP(hipMemcpyAsync(c.snap.boxLast, c.boxLast, sizeof(BoxDev), hipMemcpyDeviceToDevice, c.stream));
+    }
     if (c.box) {
         if (!c.snap.box) c.snap.box = dalloc<BoxDev>(c, 1);
         MDX_HIP(hipMemcpyAsync(c.snap.box, c.box, sizeof(BoxDev), hipMemcpyDeviceToDevice, c.stream));
@@ -281,8 +293,24 @@ void snapshot_restore(Ctx &c) {
         c.pppm.storedValid = c.snap.pppmStoredValid;
     }
     c.turn = c.snap.turn;
-    c.nRebuilds = c.snap.rebuilds;
     c.listValid = false;
+    const bool relist = c.snap.listValid && c.nlistConfigured;
+    if (relist) {
+        // Build the chunk's starting list again from the positions and box of
+        // its build: the records are already in that build's order, so the
+        // permutation is the identity and the rows come out as they were.
+        // Then the chunk-start positions and box go back in.
+        MDX_HIP(hipMemcpyAsync(c.pos[c.cur], c.snap.posLast, sizeof(double4) * c.n, hipMemcpyDeviceToDevice,
+                               c.stream));
+        MDX_HIP(hipMemcpyAsync(c.box, c.snap.boxLast, sizeof(BoxDev), hipMemcpyDeviceToDevice, c.stream));
+        nlist_rebuild(c, true);
+        MDX_HIP(hipMemcpyAsync(c.pos[c.cur], c.snap.pos, sizeof(double4) * c.n, hipMemcpyDeviceToDevice,
+                               c.stream));
+        MDX_HIP(hipMemcpyAsync(c.box, c.snap.box, sizeof(BoxDev), hipMemcpyDeviceToDevice, c.stream));
+        box_pull(c);
+        refresh_xq(c);
+    }
+    c.nRebuilds = c.snap.rebuilds;
     bool moved = false;
     for (int d = 0; d < 3; ++d) moved = moved || c.snap.geo.L[d] != c.geo.L[d] || c.snap.geo.lo[d] != c.geo.lo[d];
     if (c.baro.on && c.snap.box) {
@@ -295,7 +323,9 @@ void snapshot_restore(Ctx &c) {
     }
     if (moved) {  // a barostat changed the box during the replayed chunk
         c.geo = c.snap.geo;
-        c.nlistConfigured = false;
+        // the grid that built the starting list was valid for the starting
+        // box; without a carried list the grid is set up again
+        if (!relist) c.nlistConfigured = false;
     }
 }
 
@@ -563,7 +593,8 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags, const Rin
         return k;
     };
     try {
-        int done = 0;
+        int done = 0, chunkNo = 0;
+        bool forcedDone = false;
         while (done < nsteps) {
             int len = std::min(kGraphChunk, nsteps - done);
             if (judge && !cs[0]->sur.fellBack) len = std::min(len, steps_to_judge(*cs[0]));
@@ -629,7 +660,7 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags, const Rin
             // all of them, so no replica is left one chunk ahead.
             std::vector<std::array<int, 7>> hs(cs.size());
             std::vector<char> overflow(cs.size(), 0), badGrid(cs.size(), 0);
-            bool regrow = false;
+            bool regrow = false, forcedNow = false;
             for (size_t k = 0; k < cs.size(); ++k) {
                 Ctx &c = *cs[k];
                 int *h = hs[k].data();
@@ -647,18 +678,23 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags, const Rin
                     badGrid[k] = c.boxHost.regrid != 0;
                 }
                 regrow = regrow || overflow[k] || badGrid[k];
+                forcedNow = forcedNow || (!forcedDone && c.replayChunk == chunkNo + 1);
             }
+            forcedDone = forcedDone || forcedNow;
+            regrow = regrow || forcedNow;
             if (regrow) {
                 for (size_t k = 0; k < cs.size(); ++k) {
                     Ctx &c = *cs[k];
-                    snapshot_restore(c);
+                    // rows grow first: the restore builds the starting list into them
                     if (overflow[k]) grow_if_needed(c);
+                    snapshot_restore(c);
                     if (badGrid[k]) c.nlistConfigured = false;
                     c.forceFlags = -1;
                 }
                 drop();
                 continue;
             }
+            ++chunkNo;
             bool regrid = false;
             for (size_t k = 0; k < cs.size(); ++k) {
                 Ctx &c = *cs[k];
@@ -1560,7 +1596,8 @@ int mdx_run(mdx_ctx *ctx, int nsteps, int flags) {
             ~InRun() { c.inRun = false; }
         } inRun(c);
         constexpr int kChunk = 64;
-        int done = 0;
+        int done = 0, chunkNo = 0;
+        bool forcedDone = false;
         while (done < nsteps) {
             int len = std::min(kChunk, nsteps - done);
             if (judge && !c.sur.fellBack) len = std::min(len, steps_to_judge(c));
@@ -1594,12 +1631,15 @@ int mdx_run(mdx_ctx *ctx, int nsteps, int flags) {
             }
             MDX_HIP(hipStreamSynchronize(c.stream));
             lagged_rebuild_decision(c, (len - 1) & 1);  // count a violation in the last step
-            if (!capacity_ok(c)) {
+            const bool forced = !forcedDone && c.replayChunk == chunkNo + 1;
+            forcedDone = forcedDone || forced;
+            if (!capacity_ok(c) || forced) {
                 // a build overflowed its rows: redo the chunk with the grown rows
                 snapshot_restore(c);
                 c.forceFlags = -1;
                 continue;
             }
+            ++chunkNo;
             done += len;
             if (judge) {
                 judge_if_due(c, flags0);
@@ -2066,6 +2106,10 @@ int mdx_option_set(mdx_ctx *ctx, const char *name, int value) {
             MDX_REQUIRE(value == 8 || value == 16 || value == 32 || value == 64, MDX_ERR_ARG,
                         "sur_force_lanes: 8, 16, 32 or 64 lanes per molecule");
             c.sur.forceLanes = value;
+        }
+        else if (k == "replay_chunk") {
+            MDX_REQUIRE(value >= 0, MDX_ERR_ARG, "replay_chunk: chunk number >= 1, 0 for off");
+            c.replayChunk = value;
         }
         else if (k == "row_cap") {
             // neighbor-row capacity now (rows still grow on overflow, the

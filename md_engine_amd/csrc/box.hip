@@ -165,6 +165,7 @@ BoxParams box_params(const Ctx &c) {
 void box_upload(Ctx &c) {
     MDX_REQUIRE(!c.capture, 3, "the box cannot change while a step is captured");
     if (!c.box) c.box = dalloc<BoxDev>(c, 1);
+    if (!c.boxLast) c.boxLast = dalloc<BoxDev>(c, 1);
     BoxDev &b = c.boxHost;
     std::memset(&b, 0, sizeof(BoxDev));
     box_derive(b, box_params(c), c.geo.lo, c.geo.L);

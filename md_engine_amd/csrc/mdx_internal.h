@@ -415,6 +415,8 @@ struct Ctx {
     unsigned long long *force = nullptr;   // 3 per atom
     unsigned long long *acc = nullptr;     // kAccTotal: kAccRep replicas of ACC_N slots
     BoxDev *box = nullptr;                 // device copy of the box-dependent launch data
+    BoxDev *boxLast = nullptr;             // the box of the last list build (copied by k_scan)
+    int replayChunk = 0;                   // option replay_chunk: chunk of each run replayed once
     BoxDev boxHost;                        // host mirror (last upload or read-back)
     double4 *posLast = nullptr;
     std::vector<double> invMassHost;
@@ -516,6 +518,11 @@ struct Ctx {
         long long turn = 0, rebuilds = 0;
         Geometry geo;
         BoxDev *box = nullptr;
+        // the list the chunk started with, when one carried into it: the
+        // positions and box of its build (a replay rebuilds it from them)
+        bool listValid = false;
+        double4 *posLast = nullptr;
+        BoxDev *boxLast = nullptr;
         double *nh = nullptr;                 // Nose-Hoover chain state (4 chain + 2 doubles)
         int nhLen = 0;
         unsigned long long *thermo = nullptr; // thermostat sums (2 words)

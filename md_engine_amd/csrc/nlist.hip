@@ -102,8 +102,15 @@ __global__ void k_bin(const int *__restrict__ gate, int n, double4 *__restrict__
 // The counts are cleared as they are consumed, ready for the next binning
 // (no separate clearing launch in every captured step).
 __global__ void __launch_bounds__(1024) k_scan(const int *__restrict__ gate, int ncell, int *__restrict__ count,
-                                               int *__restrict__ start, int *__restrict__ fill) {
+                                               int *__restrict__ start, int *__restrict__ fill,
+                                               const BoxDev *__restrict__ box, BoxDev *__restrict__ boxLast) {
     if (gate && *gate == 0) return;
+    // the box this build uses, kept for rebuilding the same list later
+    constexpr int kBoxWords = (int)(sizeof(BoxDev) / sizeof(int));
+    static_assert(sizeof(BoxDev) % sizeof(int) == 0, "BoxDev copied as words");
+    if (boxLast)
+        for (int k = threadIdx.x; k < kBoxWords; k += blockDim.x)
+            reinterpret_cast<int *>(boxLast)[k] = reinterpret_cast<const int *>(box)[k];
     __shared__ int wsum[16];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int seg = (ncell + 1023) / 1024;
@@ -1211,7 +1218,7 @@ static void run_build(Ctx &c, bool sync) {
     hipLaunchKernelGGL(k_bin, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, rebuild_gate(c), c.n, c.pos[cur],
                        c.q[cur], c.box, c.ncell, c.colRank, c.cellOf, c.cellCount);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c.stream, rebuild_gate(c), nslot, c.cellCount, c.cellStart,
-                       c.cellFill);
+                       c.cellFill, c.box, c.boxLast);
     hipLaunchKernelGGL(k_place, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, rebuild_gate(c), c.n, c.cellOf,
                        c.cellFill, c.perm);
     hipLaunchKernelGGL(k_cellsort, dim3(nslot), dim3(64), 0, c.stream, rebuild_gate(c), nslot, c.cellStart, c.perm,

@@ -77,7 +77,8 @@ def test_long_range_interval_replays_on_the_host(gpu_engine_factory):
     """Two steps with interval 2: the first evaluates the mesh at its
     positions and stores the forces, the second adds those stored forces to
     its own short-range ones.  The host replays both Verlet steps with
-    forces the engine reports for each part and lands on the same state."""
+    forces the engine reports for each part and lands on the same state, to
+    the FP32 rounding of the forces."""
     say("long-range interval 2 vs host replay")
     from md_engine_amd import PAIR, PPPM
 
@@ -110,10 +111,17 @@ def test_long_range_interval_replays_on_the_host(gpu_engine_factory):
     d -= L * np.rint(d / L)
     say(f"  max |dx| {np.abs(d).max():.2e}, max |dv| {np.abs(va - v2).max():.2e}, "
         f"stored vs fresh mesh force difference {np.abs(f2 - f2_fresh).max():.2e}")
-    assert np.abs(d).max() < 1e-12
-    assert np.allclose(va, v2, rtol=0, atol=1e-12 * np.abs(v2).max())
+    # The host's FP64 positions differ from the device's in the last bits
+    # (fused multiply-adds), which can flip the FP32 rounding of a pair
+    # input: the replay agrees to FP32 force rounding, ~1e-10 here.  Using a
+    # fresh mesh in the second step instead of the stored one would move the
+    # velocities by k |f2 - f2_fresh| (~1e-6 here): the bound sits a thousand
+    # times below that.
+    dv_fresh = (k * np.abs(f2 - f2_fresh)).max()
+    assert np.abs(va - v2).max() < 1e-3 * dv_fresh
+    assert np.abs(d).max() < 1e-3 * dt * dv_fresh + 1e-12
     # the stored forces are really older than the step: they differ from a fresh mesh
-    assert np.abs(f2 - f2_fresh).max() > 1e3 * 1e-12 * np.abs(f2).max()
+    assert np.abs(f2 - f2_fresh).max() > 1e-6 * np.abs(f2).max()
     a.close()
     b.close()
 

@@ -57,38 +57,57 @@ def tight_rows(e, flags):
     return cap
 
 
-@pytest.mark.parametrize("graph", [0, 1])
-def test_overflow_replay_with_nose_hoover_is_exact(gpu_engine_factory, graph):
-    say(f"overflow replay with Nose-Hoover, graph={graph}")
+@pytest.mark.parametrize("graph,trigger", [(0, "forced"), (1, "forced"), (1, "rows")])
+def test_overflow_replay_with_nose_hoover_is_exact(gpu_engine_factory, graph, trigger):
+    """A replayed chunk that starts with a list carried in from an earlier
+    build: the replay builds that list again from the positions and box of
+    its build, so the rows (and the FP32 summation order they set) are the
+    same, and the Nose-Hoover chain and thermostat sums come back with the
+    particles.  "forced" replays the third chunk (turns 128-191) through the
+    replay_chunk option; "rows" makes the rows exactly as long as the
+    starting list's longest, so a later rebuild overflows them."""
+    say(f"overflow replay with Nose-Hoover, graph={graph}, trigger={trigger}")
     steps = 384
     ref, flags = water(gpu_engine_factory, 343, 31, nh=True)
     tst, _ = water(gpu_engine_factory, 343, 31, nh=True)
     for e in (ref, tst):
         e.option("graph", graph)
-    cap0 = tight_rows(tst, flags)
+    if trigger == "rows":
+        # both engines take the same set-up path and differ only in row length
+        cap0 = tight_rows(tst, flags)
+        ref.compute(flags)
+        ref.option("row_cap", 2 * cap0)
+    else:
+        tst.option("replay_chunk", 3)
     ref.run(steps, flags)
     tst.run(steps, flags)
-    cap1 = tst.nlist_stats()["row_cap"]
-    say(f"row capacity {cap0} -> {cap1}")
-    assert cap1 > cap0, "no rebuild overflowed the tight rows: the replay was not exercised"
+    if trigger == "rows":
+        cap1 = tst.nlist_stats()["row_cap"]
+        say(f"row capacity {cap0} -> {cap1}")
+        assert cap1 > cap0, "no rebuild overflowed the tight rows: the replay was not exercised"
     xr, vr = ref.get_system()
     xt, vt = tst.get_system()
     sr, stt = ref.nose_hoover_state(10), tst.nose_hoover_state(10)
     er, et = ref.nose_hoover_energy(), tst.nose_hoover_energy()
+    nr, nt = ref.nlist_stats()["rebuilds"], tst.nlist_stats()["rebuilds"]
     ref.close()
     tst.close()
+    say(f"  max |dx| {np.abs(xr - xt).max():.2e}, rebuilds {nr} / {nt}")
+    assert nr == nt
     assert np.array_equal(xr, xt)
     assert np.array_equal(vr, vt)
     assert np.array_equal(sr, stt)
     assert er == et
 
 
-def test_batch_replay_keeps_replicas_in_step(gpu_engine_factory):
-    """Three replicas in one batch, one with tight rows: its overflow replays
-    the chunk for every replica, and each still equals its own single run."""
+@pytest.mark.parametrize("trigger", ["rows", "forced"])
+def test_batch_replay_keeps_replicas_in_step(gpu_engine_factory, trigger):
+    """Three replicas in one batch, one with tight rows (or made to replay
+    its second chunk): its replay replays the chunk for every replica, and
+    each still equals its own single run."""
     from md_engine_amd._lib import run_batch
 
-    say("batch replay")
+    say(f"batch replay ({trigger})")
     steps = 128
     seeds = (41, 42, 43)
     singles = []
@@ -102,11 +121,15 @@ def test_batch_replay_keeps_replicas_in_step(gpu_engine_factory):
     for s in seeds:
         e, flags = water(gpu_engine_factory, 216, s, nh=False)
         batch.append(e)
-    cap0 = tight_rows(batch[1], flags)
+    if trigger == "rows":
+        cap0 = tight_rows(batch[1], flags)
+    else:
+        batch[1].option("replay_chunk", 2)
     run_batch(batch, steps, flags)
-    cap1 = batch[1].nlist_stats()["row_cap"]
-    say(f"replica 1 row capacity {cap0} -> {cap1}")
-    assert cap1 > cap0
+    if trigger == "rows":
+        cap1 = batch[1].nlist_stats()["row_cap"]
+        say(f"replica 1 row capacity {cap0} -> {cap1}")
+        assert cap1 > cap0
     for e, (xs, vs) in zip(batch, singles):
         xb, vb = e.get_system()
         assert np.array_equal(xb, xs)
