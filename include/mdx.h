@@ -202,10 +202,16 @@ int mdx_integrator_set(mdx_ctx *ctx, double dt, double ftm_to_v);
 int mdx_run(mdx_ctx *ctx, int nsteps, int flags);
 /* Batched replicas: advances every context (same device, each with its own
  * system) by nsteps with one captured graph per 8 steps that holds all of
- * them, each replica on its own stream.  No counterpart in the reference,
- * whose integrator runs one system per process; results per replica equal a
- * single-context captured run bit for bit.  Errors are reported on ctxs[0]. */
+ * them.  When the replicas run the same kernels (same force terms and set-up,
+ * any sizes) every kernel of a step is ONE launch over all replicas, each
+ * workgroup reading its replica's arguments from a table (option
+ * "batch_merge" 0 on ctxs[0]: one stream per replica instead).  No
+ * counterpart in the reference, whose integrator runs one system per
+ * process; results per replica equal a single-context captured run bit for
+ * bit.  Errors are reported on ctxs[0]. */
 int mdx_run_batch(mdx_ctx *const *ctxs, int n, int nsteps, int flags);
+/* 1 if the last batch led by ctx ran as one launch per kernel, else 0. */
+int mdx_batch_merged(mdx_ctx *ctx, int *merged);
 /* Path-integral MD: P bead contexts (the same atoms, box, time step and
  * force terms; flexible molecules, no barostat) advanced as one ring polymer
  * per atom, like IntegratorVerlet with nPerRingPoly = P

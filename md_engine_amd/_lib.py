@@ -30,7 +30,7 @@ SYMBOLS = [
     "mdx_pppm_info", "mdx_pppm_set_error", "mdx_pppm_interval_set", "mdx_pppm_mesh_get", "mdx_e3b_set", "mdx_forces_compute", "mdx_forces_get", "mdx_energy_get", "mdx_virial_trace_get",
     "mdx_integrator_set", "mdx_run", "mdx_forces_compute_noalloc", "mdx_teams_set", "mdx_sync", "mdx_force_steps",
     "mdx_profile_enable", "mdx_profile_get", "mdx_nlist_totals", "mdx_pair_entries_read", "mdx_option_set", "mdx_rigid_set",
-    "mdx_thermostat_set", "mdx_temperature", "mdx_tune", "mdx_run_batch", "mdx_surrogate_set",
+    "mdx_thermostat_set", "mdx_temperature", "mdx_tune", "mdx_run_batch", "mdx_batch_merged", "mdx_surrogate_set",
     "mdx_surrogate_get", "mdx_surrogate_record", "mdx_surrogate_fit", "mdx_surrogate_eval", "mdx_surrogate_grad",
     "mdx_surrogate_judge_set", "mdx_surrogate_judge_stats",
     "mdx_e3b_molecule_energies", "mdx_nose_hoover_set", "mdx_nose_hoover_energy", "mdx_nose_hoover_state", "mdx_barostat_set",
@@ -116,6 +116,7 @@ def load():
     L.mdx_pair_entries_read.argtypes = [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.mdx_tune.argtypes = [_vp, c, c, _ip, _dp]
     L.mdx_run_batch.argtypes = [C.POINTER(_vp), c, c, c]
+    L.mdx_batch_merged.argtypes = [_vp, C.POINTER(c)]
     _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
     L.mdx_surrogate_set.argtypes = [_vp, _fp, c]
     L.mdx_surrogate_get.argtypes = [_vp, _fp, c]
@@ -249,6 +250,12 @@ class Engine:
         self._chk(self._L.mdx_nlist_stats(self._h, C.byref(nb), C.byref(cap), cells, C.byref(viol)),
                   "mdx_nlist_stats")
         return dict(rebuilds=nb.value, row_cap=cap.value, cells=cells.tolist(), violations=viol.value)
+
+    def batch_merged(self):
+        """Whether the last batch led by this engine ran as one launch per kernel."""
+        m = C.c_int(0)
+        self._chk(self._L.mdx_batch_merged(self._h, C.byref(m)), "mdx_batch_merged")
+        return bool(m.value)
 
     def option(self, name, value):
         self._chk(self._L.mdx_option_set(self._h, name.encode(), int(value)), "mdx_option_set")

@@ -318,7 +318,7 @@ void snapshot_restore(Ctx &c) {
         // may not have seen it): the box, alpha and G go back too
         MDX_HIP(hipMemcpyAsync(c.box, c.snap.box, sizeof(BoxDev), hipMemcpyDeviceToDevice, c.stream));
         box_pull(c);
-        if (c.pppm.on) pppm_green_device(c, nullptr, c.stream);
+        if (c.pppm.on) pppm_green_device(c, nullptr);
         moved = true;
     }
     if (moved) {  // a barostat changed the box during the replayed chunk
@@ -387,56 +387,398 @@ void captured_step(Ctx &c, int fflags) {
     c.fuseCheck = false;
 }
 
-// One step of a ring polymer whose beads share one stream: every bead's half
-// kick, the free ring-polymer propagation across beads, then every bead's
-// rebuild gate, forces, second half kick and thermostat.
+// How the operations of a captured step are issued.  each(): every context
+// runs the same operations (one phase); one(): a single context runs an
+// operation that serves them all (a ring-level launch).  Issued directly,
+// the launches go to the contexts' streams as they are made; recorded, a
+// phase becomes one launch per kernel across the contexts.
+struct StepOps {
+    virtual ~StepOps() = default;
+    virtual void each(const std::vector<Ctx *> &cs, const std::function<void(Ctx &)> &f) = 0;
+    virtual void one(Ctx &c, const std::function<void()> &f) = 0;
+};
+
+struct DirectOps : StepOps {
+    void each(const std::vector<Ctx *> &cs, const std::function<void(Ctx &)> &f) override {
+        for (Ctx *c : cs) f(*c);
+    }
+    void one(Ctx &, const std::function<void()> &f) override { f(); }
+};
+
 // Ring-polymer forces of every bead: per-bead terms, then (contraction) the
 // mesh part once from the centroids onto every bead, then the M-site forces
 // of flexible water onto O and H.
 bool ring_contracts(const RingSpec &ring, int fflags) { return ring.contract && (fflags & MDX_PPPM); }
 
-void ring_forces(const std::vector<Ctx *> &cs, const RingSpec &ring, int fflags) {
+void ring_forces(const std::vector<Ctx *> &cs, const RingSpec &ring, int fflags, StepOps &ops) {
     const bool contract = ring_contracts(ring, fflags);
-    for (Ctx *cp : cs) {
-        Ctx &c = *cp;
+    ops.each(cs, [&](Ctx &c) {
         c.deferMSpread = contract;
         compute(c, contract ? (fflags & ~MDX_PPPM) : fflags);
         c.deferMSpread = false;
-    }
+    });
     if (contract) {
-        ring_pppm(*cs[0], ring);
-        for (Ctx *cp : cs)
-            if (cp->flex.on) flex_spread_m(*cp);
+        ops.one(*cs[0], [&] { ring_pppm(*cs[0], ring); });
+        ops.each(cs, [&](Ctx &c) {
+            if (c.flex.on) flex_spread_m(c);
+        });
     }
     for (Ctx *cp : cs) cp->forceFlags = fflags & kForceTerms;
 }
 
-void captured_ring_step(const std::vector<Ctx *> &cs, const RingSpec &ring, int fflags) {
-    for (Ctx *c : cs) integrate_kick_first(*c);
-    ring_drift(ring, cs[0]->box, cs[0]->stream);
+void ring_forces(const std::vector<Ctx *> &cs, const RingSpec &ring, int fflags) {
+    DirectOps direct;
+    ring_forces(cs, ring, fflags, direct);
+}
+
+// One step of a ring polymer whose beads share one stream: every bead's half
+// kick, the free ring-polymer propagation across beads, then every bead's
+// rebuild gate, forces, second half kick and thermostat.
+void captured_ring_step(const std::vector<Ctx *> &cs, const RingSpec &ring, int fflags, StepOps &ops) {
+    ops.each(cs, [](Ctx &c) { integrate_kick_first(c); });
+    ops.one(*cs[0], [&] { ring_drift(ring, *cs[0]); });
     if (ring_contracts(ring, fflags)) {
-        for (Ctx *cp : cs) {
-            decide_step(*cp);
-            nlist_rebuild_async(*cp);
-        }
-        ring_forces(cs, ring, fflags);
-        for (Ctx *cp : cs) {
-            integrate_second_half(*cp);
-            if (cp->thermoOn) thermostat_apply(*cp);
-        }
+        ops.each(cs, [](Ctx &c) {
+            decide_step(c);
+            nlist_rebuild_async(c);
+        });
+        ring_forces(cs, ring, fflags, ops);
+        ops.each(cs, [](Ctx &c) {
+            integrate_second_half(c);
+            if (c.thermoOn) thermostat_apply(c);  // gated by dFlags[5]
+        });
         return;
     }
-    for (Ctx *cp : cs) {
-        Ctx &c = *cp;
+    ops.each(cs, [&](Ctx &c) {
         decide_step(c);
         nlist_rebuild_async(c);
         compute(c, fflags);
         integrate_second_half(c);
         if (c.thermoOn) thermostat_apply(c);
+    });
+}
+
+// Replicas of a merged batch whose P3M meshes have the same dimensions share
+// one batched transform per direction: their work arrays (real mesh, half
+// spectrum, field spectra, real field) are laid out back to back in one
+// allocation while the batch is recorded and captured, so one plan of batch
+// m (forward) and 3 m (inverse) covers all of them.  The replicas' own arrays
+// come back afterwards; the graph keeps the shared ones.
+struct FftGroup {
+    std::vector<size_t> members;  // replica indices
+    hipfftHandle r2c = 0, c2r = 0;
+};
+
+struct SavedMesh {
+    float *real;
+    float2 *rhoHat, *eHat;
+    float *eReal;
+};
+
+std::vector<FftGroup> group_ffts(const std::vector<Ctx *> &cs, BatchKeep &keep, std::vector<SavedMesh> &saved,
+                                 std::vector<int> &groupOf) {
+    const size_t R = cs.size();
+    std::vector<FftGroup> groups;
+    groupOf.assign(R, -1);
+    saved.resize(R);
+    for (size_t k = 0; k < R; ++k) {
+        const PppmState &P = cs[k]->pppm;
+        saved[k] = {P.real, P.rhoHat, P.eHat, P.eReal};
+        if (!P.on || !P.plansMade || groupOf[k] >= 0) continue;
+        FftGroup g;
+        for (size_t j = k; j < R; ++j) {
+            const PppmState &Q = cs[j]->pppm;
+            if (groupOf[j] < 0 && Q.on && Q.plansMade && Q.sz[0] == P.sz[0] && Q.sz[1] == P.sz[1] &&
+                Q.sz[2] == P.sz[2]) {
+                groupOf[j] = (int)groups.size();
+                g.members.push_back(j);
+            }
+        }
+        groups.push_back(g);
+    }
+    for (FftGroup &g : groups) {
+        const int *sz = cs[g.members[0]]->pppm.sz;
+        const int m = (int)g.members.size();
+        const size_t ng = (size_t)sz[0] * sz[1] * sz[2], nh = (size_t)sz[0] * sz[1] * (sz[2] / 2 + 1);
+        void *real = nullptr, *rho = nullptr, *eh = nullptr, *er = nullptr;
+        MDX_HIP(hipMalloc(&real, sizeof(float) * ng * m));
+        keep.mem.push_back(real);
+        MDX_HIP(hipMalloc(&rho, sizeof(float2) * nh * m));
+        keep.mem.push_back(rho);
+        MDX_HIP(hipMalloc(&eh, sizeof(float2) * 3 * nh * m));
+        keep.mem.push_back(eh);
+        MDX_HIP(hipMalloc(&er, sizeof(float) * 3 * ng * m));
+        keep.mem.push_back(er);
+        for (int t = 0; t < m; ++t) {
+            PppmState &P = cs[g.members[t]]->pppm;
+            P.real = static_cast<float *>(real) + ng * t;
+            P.rhoHat = static_cast<float2 *>(rho) + nh * t;
+            P.eHat = static_cast<float2 *>(eh) + 3 * nh * t;
+            P.eReal = static_cast<float *>(er) + 3 * ng * t;
+        }
+        int dims[3] = {sz[0], sz[1], sz[2]};
+        MDX_REQUIRE(hipfftPlanMany(&g.r2c, 3, dims, nullptr, 1, (int)ng, nullptr, 1, (int)nh, HIPFFT_R2C, m) ==
+                        HIPFFT_SUCCESS,
+                    2, "hipfftPlanMany R2C (batch) failed");
+        keep.plans.push_back(g.r2c);
+        MDX_REQUIRE(hipfftPlanMany(&g.c2r, 3, dims, nullptr, 1, (int)nh, nullptr, 1, (int)ng, HIPFFT_C2R, 3 * m) ==
+                        HIPFFT_SUCCESS,
+                    2, "hipfftPlanMany C2R (batch) failed");
+        keep.plans.push_back(g.c2r);
+        MDX_REQUIRE(hipfftSetStream(g.r2c, cs[0]->stream) == HIPFFT_SUCCESS &&
+                        hipfftSetStream(g.c2r, cs[0]->stream) == HIPFFT_SUCCESS,
+                    2, "hipfftSetStream failed");
+    }
+    return groups;
+}
+
+void restore_meshes(const std::vector<Ctx *> &cs, const std::vector<SavedMesh> &saved) {
+    for (size_t k = 0; k < cs.size() && k < saved.size(); ++k) {
+        PppmState &P = cs[k]->pppm;
+        P.real = saved[k].real;
+        P.rhoHat = saved[k].rhoHat;
+        P.eHat = saved[k].eHat;
+        P.eReal = saved[k].eReal;
     }
 }
 
-hipGraphExec_t capture_steps(const std::vector<Ctx *> &cs, int steps, int fflags, const RingSpec *ring = nullptr) {
+// Records the operations of captured steps instead of issuing them.  While a
+// context records, a stream that is being captured into a throw-away graph
+// stands in for its stream, so a launch that bypassed the launch layer is
+// caught (it lands in that graph) instead of running.
+struct RecordOps : StepOps {
+    struct Item {
+        bool phase = false;                        // each() (else one())
+        bool merged = false;                       // every context recorded the same kernels
+        std::vector<std::vector<LaunchRec>> recs;  // per context (one() items: one)
+    };
+    std::vector<Item> items;
+    hipStream_t dummy = nullptr;
+    bool stray = false;
+
+    RecordOps() { MDX_HIP(hipStreamCreateWithFlags(&dummy, hipStreamNonBlocking)); }
+    ~RecordOps() { hipStreamDestroy(dummy); }
+
+    void record(Ctx &c, std::vector<LaunchRec> &out, const std::function<void()> &f) {
+        const hipStream_t own = c.stream;
+        const bool prof = c.prof, cap = c.capture;
+        hipGraph_t g = nullptr;
+        MDX_HIP(hipStreamBeginCapture(dummy, hipStreamCaptureModeThreadLocal));
+        c.stream = dummy;
+        c.prof = false;
+        c.capture = true;
+        c.rec = &out;
+        auto undo = [&] {
+            c.rec = nullptr;
+            c.capture = cap;
+            c.prof = prof;
+            c.stream = own;
+        };
+        try {
+            f();
+        } catch (...) {
+            undo();
+            hipStreamEndCapture(dummy, &g);
+            if (g) hipGraphDestroy(g);
+            throw;
+        }
+        undo();
+        MDX_HIP(hipStreamEndCapture(dummy, &g));
+        size_t nodes = 0;
+        if (g) {
+            MDX_HIP(hipGraphGetNodes(g, nullptr, &nodes));
+            MDX_HIP(hipGraphDestroy(g));
+        }
+        stray = stray || nodes > 0;
+    }
+
+    void each(const std::vector<Ctx *> &cs, const std::function<void(Ctx &)> &f) override {
+        Item it;
+        it.phase = true;
+        it.recs.resize(cs.size());
+        for (size_t k = 0; k < cs.size(); ++k) record(*cs[k], it.recs[k], [&] { f(*cs[k]); });
+        it.merged = true;
+        for (size_t k = 1; k < cs.size() && it.merged; ++k) {
+            it.merged = it.recs[k].size() == it.recs[0].size();
+            for (size_t i = 0; it.merged && i < it.recs[0].size(); ++i)
+                it.merged = it.recs[k][i].kid == it.recs[0][i].kid &&
+                            (it.recs[k][i].kid != nullptr || it.recs[k][i].host) &&
+                            it.recs[k][i].fft == it.recs[0][i].fft &&
+                            it.recs[k][i].args.size() == it.recs[0][i].args.size();
+        }
+        items.push_back(std::move(it));
+    }
+
+    void one(Ctx &c, const std::function<void()> &f) override {
+        Item it;
+        it.recs.resize(1);
+        record(c, it.recs[0], f);
+        items.push_back(std::move(it));
+    }
+};
+
+// Captured steps of several contexts with one launch per kernel and phase.
+// The steps are recorded first (RecordOps); each kernel of a merged phase
+// becomes one launch over the concatenated grids of its contexts: an
+// argument table (one packed argument set per context) and a block map
+// ((context, block, blocks) per workgroup) in device memory, kept with the
+// graph; single operations and unmerged phases become launches of a
+// one-entry table, in recorded order.  P3M transforms of merged phases run
+// once per group of equal meshes (group_ffts).  Returns nullptr when a launch
+// bypassed the layer, or when a batch's replicas recorded different kernels
+// (the caller then captures them as separate streams).
+hipGraphExec_t capture_merged(const std::vector<Ctx *> &cs, int steps, int fflags, BatchKeep &keep,
+                              const RingSpec *ring) {
+    Ctx &c0 = *cs[0];
+    const size_t R = cs.size();
+    std::vector<SavedMesh> saved;
+    std::vector<int> groupOf;
+    std::vector<FftGroup> groups;
+    try {
+        groups = group_ffts(cs, keep, saved, groupOf);
+    } catch (...) {
+        restore_meshes(cs, saved);
+        throw;
+    }
+    struct Restore {
+        const std::vector<Ctx *> &cs;
+        const std::vector<SavedMesh> &saved;
+        ~Restore() { restore_meshes(cs, saved); }
+    } restore{cs, saved};
+    RecordOps rec;
+    if (ring) {
+        for (int s = 0; s < steps; ++s) captured_ring_step(cs, *ring, fflags, rec);
+    } else {
+        rec.each(cs, [&](Ctx &c) {
+            for (int s = 0; s < steps; ++s) captured_step(c, fflags);
+        });
+    }
+    bool allMerged = true;
+    for (const auto &it : rec.items)
+        if (it.phase) allMerged = allMerged && it.merged;
+    c0.batchMerged = !rec.stray && allMerged ? 1 : 0;
+    if (rec.stray || (!ring && !allMerged)) return nullptr;
+
+    // every launch: its argument table and block map in one device block
+    struct Emit {
+        const LaunchRec *op;                   // a kernel (args, launcher) or a host operation
+        std::vector<const LaunchRec *> parts;  // the contexts' records it covers (kernels)
+        std::vector<const LaunchRec *> hosts;  // host operations to run, in order
+        int fftKind = 0;                       // grouped transform of a merged phase
+        size_t tab = 0, map = 0;
+        int total = 0;
+        size_t shm = 0;
+    };
+    std::vector<Emit> emits;
+    for (const auto &it : rec.items) {
+        if (it.merged && it.recs.size() == R && it.phase) {
+            for (size_t i = 0; i < it.recs[0].size(); ++i) {
+                Emit e;
+                e.op = &it.recs[0][i];
+                if (e.op->kid) {
+                    for (size_t k = 0; k < R; ++k) e.parts.push_back(&it.recs[k][i]);
+                } else if (e.op->fft) {
+                    e.fftKind = e.op->fft;
+                    for (size_t k = 0; k < R; ++k) e.hosts.push_back(&it.recs[k][i]);
+                } else {
+                    for (size_t k = 0; k < R; ++k) e.hosts.push_back(&it.recs[k][i]);
+                }
+                emits.push_back(std::move(e));
+            }
+        } else {
+            for (const auto &r : it.recs)
+                for (const LaunchRec &op : r) {
+                    Emit e;
+                    e.op = &op;
+                    if (op.kid) e.parts.push_back(&op);
+                    else e.hosts.push_back(&op);
+                    emits.push_back(std::move(e));
+                }
+        }
+    }
+    auto align_to = [](size_t v, size_t a) { return (v + a - 1) / a * a; };
+    size_t bytes = 0;
+    for (Emit &e : emits) {
+        if (e.parts.empty()) continue;
+        const size_t stride = align_to(e.op->args.size(), e.op->align);
+        bytes = align_to(bytes, 256);
+        e.tab = bytes;
+        bytes += stride * e.parts.size();
+        for (const LaunchRec *p : e.parts) {
+            e.total += std::max(p->blocks, 0);
+            e.shm = std::max(e.shm, p->shm);
+        }
+        bytes = align_to(bytes, 256);
+        e.map = bytes;
+        bytes += sizeof(int4) * (size_t)std::max(e.total, 1);
+    }
+    std::vector<unsigned char> host(std::max<size_t>(bytes, 1), 0);
+    for (const Emit &e : emits) {
+        if (e.parts.empty()) continue;
+        const size_t stride = align_to(e.op->args.size(), e.op->align);
+        int4 *map = reinterpret_cast<int4 *>(host.data() + e.map);
+        int b = 0;
+        for (size_t k = 0; k < e.parts.size(); ++k) {
+            const LaunchRec &p = *e.parts[k];
+            std::memcpy(host.data() + e.tab + stride * k, p.args.data(), p.args.size());
+            for (int j = 0; j < p.blocks; ++j) map[b++] = make_int4((int)k, j, p.blocks, 0);
+        }
+    }
+    void *dev = nullptr;
+    MDX_HIP(hipMalloc(&dev, host.size()));
+    keep.mem.push_back(dev);
+    MDX_HIP(hipMemcpy(dev, host.data(), host.size(), hipMemcpyHostToDevice));
+    unsigned char *d = static_cast<unsigned char *>(dev);
+    hipGraph_t graph = nullptr;
+    MDX_HIP(hipStreamBeginCapture(c0.stream, hipStreamCaptureModeThreadLocal));
+    try {
+        for (const Emit &e : emits) {
+            if (!e.parts.empty()) {
+                if (e.total == 0) continue;
+                e.op->many(c0.stream, d + e.tab, reinterpret_cast<const int4 *>(d + e.map), e.total, e.shm);
+                MDX_HIP(hipGetLastError());
+            } else if (e.fftKind) {
+                // one transform per group of equal meshes, in context order
+                std::vector<char> done(groups.size(), 0);
+                for (size_t k = 0; k < e.hosts.size(); ++k) {
+                    const int g = groupOf[k];
+                    if (g < 0) {
+                        e.hosts[k]->host(c0.stream);
+                        continue;
+                    }
+                    if (done[g]) continue;
+                    done[g] = 1;
+                    const PppmState &P = cs[groups[g].members[0]]->pppm;
+                    if (e.fftKind == 1)
+                        MDX_REQUIRE(hipfftExecR2C(groups[g].r2c, P.real, (hipfftComplex *)P.rhoHat) == HIPFFT_SUCCESS,
+                                    2, "hipfftExecR2C (batch) failed");
+                    else
+                        MDX_REQUIRE(hipfftExecC2R(groups[g].c2r, (hipfftComplex *)P.eHat, P.eReal) == HIPFFT_SUCCESS,
+                                    2, "hipfftExecC2R (batch) failed");
+                }
+            } else {
+                for (const LaunchRec *h : e.hosts) h->host(c0.stream);
+            }
+        }
+    } catch (...) {
+        hipStreamEndCapture(c0.stream, &graph);
+        if (graph) hipGraphDestroy(graph);
+        throw;
+    }
+    MDX_HIP(hipStreamEndCapture(c0.stream, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    MDX_HIP(ie);
+    return exec;
+}
+
+hipGraphExec_t capture_steps(const std::vector<Ctx *> &cs, int steps, int fflags, const RingSpec *ring = nullptr,
+                             BatchKeep *keep = nullptr) {
+    if (cs.size() > 1 && keep && cs[0]->batchMerge) {
+        hipGraphExec_t merged = capture_merged(cs, steps, fflags, *keep, ring);
+        if (merged) return merged;
+    }
     Ctx &c0 = *cs[0];
     std::vector<hipEvent_t> evs(cs.size() * 2, nullptr);
     for (auto &e : evs) MDX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -452,7 +794,8 @@ hipGraphExec_t capture_steps(const std::vector<Ctx *> &cs, int steps, int fflags
                 cs[k]->prof = false;
                 cs[k]->capture = true;
             }
-            for (int s = 0; s < steps; ++s) captured_ring_step(cs, *ring, fflags);
+            DirectOps direct;
+            for (int s = 0; s < steps; ++s) captured_ring_step(cs, *ring, fflags, direct);
             for (size_t k = 0; k < cs.size(); ++k) {
                 cs[k]->capture = false;
                 cs[k]->prof = prof[k];
@@ -568,6 +911,8 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags, const Rin
     // terms) changed.  A ring's bead table is made per call.
     Ctx::GraphCache *cache = ring ? nullptr : &cs[0]->graphs;
     hipGraphExec_t big = nullptr, one = nullptr;
+    BatchKeep localKeep;
+    BatchKeep &keep = cache ? cache->keep : localKeep;
     if (cache) {
         big = cache->big;
         one = cache->one;
@@ -577,6 +922,11 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags, const Rin
         if (big) hipGraphExecDestroy(big);
         if (one) hipGraphExecDestroy(one);
         big = one = nullptr;
+        if (!keep.empty()) {
+            // what merged launches use goes with their graphs
+            hipStreamSynchronize(cs[0]->stream);
+            keep.release();
+        }
         if (cache) {
             cache->big = cache->one = nullptr;
             cache->key.clear();
@@ -644,9 +994,9 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags, const Rin
                     cache->key = k;
                 }
             }
-            if (!big) big = capture_steps(cs, kGraphSteps, fflags, ring);
+            if (!big) big = capture_steps(cs, kGraphSteps, fflags, ring, &keep);
             const int nbig = len / kGraphSteps, nrest = len % kGraphSteps;
-            if (nrest && !one) one = capture_steps(cs, 1, fflags, ring);
+            if (nrest && !one) one = capture_steps(cs, 1, fflags, ring, &keep);
             if (cache) {
                 cache->big = big;
                 cache->one = one;
@@ -887,6 +1237,7 @@ int mdx_destroy(mdx_ctx *ctx) {
     if (c.stream) hipStreamSynchronize(c.stream);
     if (c.graphs.big) hipGraphExecDestroy(c.graphs.big);
     if (c.graphs.one) hipGraphExecDestroy(c.graphs.one);
+    c.graphs.keep.release();
     pppm_release(c);
     prof_clear(c);
     if (c.pinned) hipHostFree(c.pinned);
@@ -1665,6 +2016,12 @@ int mdx_run_batch(mdx_ctx *const *ctxs, int n, int nsteps, int flags) {
     });
 }
 
+int mdx_batch_merged(mdx_ctx *ctx, int *merged) {
+    return guarded_ro(ctx, [&](Ctx &c) {
+        if (merged) *merged = c.batchMerged;
+    });
+}
+
 int mdx_ring_run(mdx_ctx *const *beads, int P, int nsteps, int flags, double omegaP) {
     if (!beads || P <= 0) return MDX_ERR_ARG;
     for (int k = 0; k < P; ++k)
@@ -2106,6 +2463,18 @@ int mdx_option_set(mdx_ctx *ctx, const char *name, int value) {
             MDX_REQUIRE(value == 8 || value == 16 || value == 32 || value == 64, MDX_ERR_ARG,
                         "sur_force_lanes: 8, 16, 32 or 64 lanes per molecule");
             c.sur.forceLanes = value;
+        }
+        else if (k == "batch_merge") {
+            // batches led by this context: one launch per kernel across the
+            // replicas (1, default) or one stream per replica (0)
+            c.batchMerge = value != 0;
+            if (c.graphs.big || c.graphs.one) {
+                MDX_HIP(hipStreamSynchronize(c.stream));
+                if (c.graphs.big) hipGraphExecDestroy(c.graphs.big);
+                if (c.graphs.one) hipGraphExecDestroy(c.graphs.one);
+                c.graphs.keep.release();
+                c.graphs = Ctx::GraphCache{};
+            }
         }
         else if (k == "replay_chunk") {
             MDX_REQUIRE(value >= 0, MDX_ERR_ARG, "replay_chunk: chunk number >= 1, 0 for off");

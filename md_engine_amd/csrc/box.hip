@@ -99,7 +99,7 @@ __device__ double alpha_solve_any(int order, int natoms, double q2, const double
 // One wave: the accumulator replicas are summed across lanes (integer sums,
 // exact in any order), then lane 0 computes the pressure, the dilation factor,
 // the new box and, for P3M, alpha and the Green function's image range.
-__global__ void __launch_bounds__(64) k_baro(const int *__restrict__ gate, BoxDev *__restrict__ bx, BoxParams bp,
+__device__ __forceinline__ void baro_body(int blk, int nblk, const int *__restrict__ gate, BoxDev *__restrict__ bx, BoxParams bp,
                                              BaroArgs ba, const unsigned long long *__restrict__ acc) {
     if (gate && *gate == 0) return;
     const int lane = threadIdx.x;
@@ -230,10 +230,10 @@ void baro_step(Ctx &c, const int *gate) {
     ba.tconv = c.tconv;
     ba.T0 = c.thermoT;
     ba.coupling = c.thermoTau > 0 ? c.thermoEvery * c.dt / c.thermoTau : 1.0;
-    hipLaunchKernelGGL(k_baro, dim3(1), dim3(64), 0, c.stream, gate, c.box, box_params(c), ba, c.acc);
+    launch<baro_body, 64>(c, 1, 0, gate, c.box, box_params(c), ba, c.acc);
     // the dilation rides on the Green function's launch where it can
     const DilateArgs dl = dilate_args(c);
-    if (!(c.pppm.on && pppm_green_device(c, gate, c.stream, &dl))) dilate(c, gate);
+    if (!(c.pppm.on && pppm_green_device(c, gate, &dl))) dilate(c, gate);
 }
 
 }  // namespace mdx

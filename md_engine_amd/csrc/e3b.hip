@@ -121,10 +121,10 @@ __device__ __forceinline__ void box_into(E3BArgs &p) {
 
 // EVM: 0 forces only; 1 energy and virial tensor; 2 the virial's trace only
 template <int P, int EVM>
-__global__ void __launch_bounds__(256) k_e3b_pass1(E3BArgs p) {
+__device__ __forceinline__ void e3b_pass1_body(int blk, int nblk, E3BArgs p) {
     constexpr bool EV = EVM == 1, VT = EVM == 2;
     box_into(p);
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gt = blk * blockDim.x + threadIdx.x;
     const int i = gt / P;
     const int tl = threadIdx.x & (P - 1);
     float fx = 0.f, fy = 0.f, fz = 0.f, tx = 0.f, ty = 0.f, tz = 0.f, e2 = 0.f;
@@ -186,10 +186,10 @@ __global__ void __launch_bounds__(256) k_e3b_pass1(E3BArgs p) {
 }
 
 template <int P, int EVM>
-__global__ void __launch_bounds__(256) k_e3b_pass2(E3BArgs p) {
+__device__ __forceinline__ void e3b_pass2_body(int blk, int nblk, E3BArgs p) {
     constexpr bool EV = EVM == 1, VT = EVM == 2;
     box_into(p);
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gt = blk * blockDim.x + threadIdx.x;
     const int i = gt / P;
     const int tl = threadIdx.x & (P - 1);
     float3 FO = make_float3(0.f, 0.f, 0.f), FH1 = FO, FH2 = FO;
@@ -294,27 +294,26 @@ __global__ void __launch_bounds__(256) k_e3b_pass2(E3BArgs p) {
 }
 
 template <int P>
-void launch(Ctx &c, const E3BArgs &a, int evm) {
+void launch_e3b(Ctx &c, const E3BArgs &a, int evm) {
     const int threads = 256;
     const int blocks = cdiv((long long)a.nmol * P, threads);
-    hipStream_t s = c.stream;
     {
         ProfScope ps(c, PK_E3B1);
         if (evm == 1)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass1<P, 1>), dim3(blocks), dim3(threads), 0, s, a);
+            launch<e3b_pass1_body<P, 1>, 256>(c, blocks, 0, a);
         else if (evm == 2)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass1<P, 2>), dim3(blocks), dim3(threads), 0, s, a);
+            launch<e3b_pass1_body<P, 2>, 256>(c, blocks, 0, a);
         else
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass1<P, 0>), dim3(blocks), dim3(threads), 0, s, a);
+            launch<e3b_pass1_body<P, 0>, 256>(c, blocks, 0, a);
     }
     if (!a.three) return;
     ProfScope ps(c, PK_E3B2);
     if (evm == 1)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass2<P, 1>), dim3(blocks), dim3(threads), 0, s, a);
+        launch<e3b_pass2_body<P, 1>, 256>(c, blocks, 0, a);
     else if (evm == 2)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass2<P, 2>), dim3(blocks), dim3(threads), 0, s, a);
+        launch<e3b_pass2_body<P, 2>, 256>(c, blocks, 0, a);
     else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_pass2<P, 0>), dim3(blocks), dim3(threads), 0, s, a);
+        launch<e3b_pass2_body<P, 0>, 256>(c, blocks, 0, a);
 }
 
 }  // namespace
@@ -352,10 +351,10 @@ void e3b_forces(Ctx &c, bool energy, bool virial, bool three_body, float *emol) 
     a.vgate = baro_gate(c);
     const int evm = (energy || virial || emol) ? 1 : (c.vtrace ? 2 : 0);
     switch (E.team) {
-        case 4: launch<4>(c, a, evm); break;
-        case 8: launch<8>(c, a, evm); break;
-        case 16: launch<16>(c, a, evm); break;
-        default: launch<32>(c, a, evm); break;
+        case 4: launch_e3b<4>(c, a, evm); break;
+        case 8: launch_e3b<8>(c, a, evm); break;
+        case 16: launch_e3b<16>(c, a, evm); break;
+        default: launch_e3b<32>(c, a, evm); break;
     }
 }
 

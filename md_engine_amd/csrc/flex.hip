@@ -38,18 +38,18 @@ __device__ __forceinline__ void fadd(unsigned long long *f, int i, v3 v) {
     fx_add(&f[3 * i + 2], v.z);
 }
 
-__global__ void k_flex_map(const int *__restrict__ gate, int nmol, const int4 *__restrict__ ids,
+__device__ __forceinline__ void flex_map_body(int blk, int nblk, const int *__restrict__ gate, int nmol, const int4 *__restrict__ ids,
                            const int *__restrict__ idToIdx, int4 *__restrict__ atoms) {
     if (gate && *gate == 0) return;
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blk * blockDim.x + threadIdx.x;
     if (m >= nmol) return;
     const int4 d = ids[m];
     atoms[m] = make_int4(idToIdx[d.x], idToIdx[d.y], idToIdx[d.z], d.w >= 0 ? idToIdx[d.w] : -1);
 }
 
 // M = O + (1 - gamma)/2 ((H1 - O) + (H2 - O)), hydrogens in O's image
-__global__ void k_flex_place_m(FlexArgs a, double4 *__restrict__ pos) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void flex_place_m_body(int blk, int nblk, FlexArgs a, double4 *__restrict__ pos) {
+    const int m = blk * blockDim.x + threadIdx.x;
     if (m >= a.nmol) return;
     const int4 t = a.atoms[m];
     if (t.w < 0) return;
@@ -68,10 +68,10 @@ __global__ void k_flex_place_m(FlexArgs a, double4 *__restrict__ pos) {
 // k/2 (theta - theta0)^2 for H1-O-H2.  Energy into ACC_BOND, virial sum of
 // r_H - r_O (x) F_H over the three atoms (the terms are translation invariant).
 template <bool EV>
-__global__ void __launch_bounds__(256) k_flex_forces(FlexArgs a, const double4 *__restrict__ pos,
+__device__ __forceinline__ void flex_forces_body(int blk, int nblk, FlexArgs a, const double4 *__restrict__ pos,
                                                      unsigned long long *__restrict__ force,
                                                      unsigned long long *__restrict__ acc) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blk * blockDim.x + threadIdx.x;
     double e = 0.0, w[6] = {0, 0, 0, 0, 0, 0};
     if (m < a.nmol) {
         const int4 t = a.atoms[m];
@@ -131,8 +131,8 @@ __global__ void __launch_bounds__(256) k_flex_forces(FlexArgs a, const double4 *
 // F_O += gamma F_M, F_H1 = F_H2 += (1 - gamma)/2 F_M, F_M = 0.  Integer
 // arithmetic: the H shares are rounded, the O share takes the remainder, so
 // the total force is conserved exactly.
-__global__ void k_flex_spread_m(FlexArgs a, unsigned long long *__restrict__ force) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void flex_spread_m_body(int blk, int nblk, FlexArgs a, unsigned long long *__restrict__ force) {
+    const int m = blk * blockDim.x + threadIdx.x;
     if (m >= a.nmol) return;
     const int4 t = a.atoms[m];
     if (t.w < 0) return;
@@ -169,27 +169,23 @@ FlexArgs make_args(const Ctx &c) {
 
 void flex_map(Ctx &c) {
     if (!c.flex.on) return;
-    hipLaunchKernelGGL(k_flex_map, dim3(cdiv(c.flex.nmol, 256)), dim3(256), 0, c.stream, rebuild_gate(c), c.flex.nmol,
-                       c.flex.ids, c.idToIdx, c.flex.atoms);
+    launch<flex_map_body, 256>(c, cdiv(c.flex.nmol, 256), 0, rebuild_gate(c), c.flex.nmol, c.flex.ids, c.idToIdx, c.flex.atoms);
 }
 
 void flex_place_m(Ctx &c) {
-    hipLaunchKernelGGL(k_flex_place_m, dim3(cdiv(c.flex.nmol, 256)), dim3(256), 0, c.stream, make_args(c),
-                       c.pos[c.cur]);
+    launch<flex_place_m_body, 256>(c, cdiv(c.flex.nmol, 256), 0, make_args(c), c.pos[c.cur]);
 }
 
 void flex_forces(Ctx &c, bool energy, bool virial) {
     const dim3 grid(cdiv(c.flex.nmol, 256));
     if (energy || virial)
-        hipLaunchKernelGGL(k_flex_forces<true>, grid, dim3(256), 0, c.stream, make_args(c), c.pos[c.cur], c.force,
-                           c.acc);
+        launch<flex_forces_body<true>, 256>(c, grid.x, 0, make_args(c), c.pos[c.cur], c.force, c.acc);
     else
-        hipLaunchKernelGGL(k_flex_forces<false>, grid, dim3(256), 0, c.stream, make_args(c), c.pos[c.cur], c.force,
-                           c.acc);
+        launch<flex_forces_body<false>, 256>(c, grid.x, 0, make_args(c), c.pos[c.cur], c.force, c.acc);
 }
 
 void flex_spread_m(Ctx &c) {
-    hipLaunchKernelGGL(k_flex_spread_m, dim3(cdiv(c.flex.nmol, 256)), dim3(256), 0, c.stream, make_args(c), c.force);
+    launch<flex_spread_m_body, 256>(c, cdiv(c.flex.nmol, 256), 0, make_args(c), c.force);
 }
 
 }  // namespace mdx

@@ -11,23 +11,23 @@ namespace mdx {
 namespace {
 
 // clears slots [slot, slot + count) of every accumulator replica
-__global__ void k_acc_clear(unsigned long long *__restrict__ acc, int slot, int count) {
+__device__ __forceinline__ void acc_clear_body(int blk, int nblk, unsigned long long *__restrict__ acc, int slot, int count) {
     const int r = threadIdx.x / ACC_N, k = threadIdx.x % ACC_N;
     if (r < kAccRep && k >= slot && k < slot + count) acc[r * ACC_N + k] = 0ull;
 }
 
-__global__ void k_zero_u64(const int *__restrict__ gate, int n, unsigned long long *__restrict__ p) {
+__device__ __forceinline__ void zero_u64_body(int blk, int nblk, const int *__restrict__ gate, int n, unsigned long long *__restrict__ p) {
     if (gate && *gate == 0) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blk * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;
 }
 
 __device__ __forceinline__ double fx_to_d(unsigned long long v) { return (double)(long long)v * kFxInv; }
 
-__global__ void k_vv1(int n, double4 *__restrict__ pos, double4 *__restrict__ vel,
+__device__ __forceinline__ void vv1_body(int blk, int nblk, int n, double4 *__restrict__ pos, double4 *__restrict__ vel,
                       const unsigned long long *__restrict__ f, double hdtf, double dt,
                       const int *__restrict__ member) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blk * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (member && member[i]) return;  // moved by its rigid molecule
     double4 v = vel[i];
@@ -45,8 +45,8 @@ __global__ void k_vv1(int n, double4 *__restrict__ pos, double4 *__restrict__ ve
 }
 
 // half kick only (the drift of a ring polymer is its normal-mode propagation)
-__global__ void k_kick(int n, double4 *__restrict__ vel, const unsigned long long *__restrict__ f, double hdtf) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void kick_body(int blk, int nblk, int n, double4 *__restrict__ vel, const unsigned long long *__restrict__ f, double hdtf) {
+    int i = blk * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double4 v = vel[i];
     if (v.w == 0.0) return;
@@ -57,9 +57,9 @@ __global__ void k_kick(int n, double4 *__restrict__ vel, const unsigned long lon
     vel[i] = v;
 }
 
-__global__ void k_vv2(int n, double4 *__restrict__ vel, const unsigned long long *__restrict__ f,
+__device__ __forceinline__ void vv2_body(int blk, int nblk, int n, double4 *__restrict__ vel, const unsigned long long *__restrict__ f,
                       double hdtf, const int *__restrict__ member) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blk * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (member && member[i]) return;
     double4 v = vel[i];
@@ -73,11 +73,11 @@ __global__ void k_vv2(int n, double4 *__restrict__ vel, const unsigned long long
 
 // Sum of m v^2 over massive sites (FP64 per block, one fixed-point atomic per
 // block) -> thermo[0]; the temperature follows from the degrees of freedom.
-__global__ void __launch_bounds__(256) k_mv2(const int *__restrict__ gate, int n, const double4 *__restrict__ vel,
+__device__ __forceinline__ void mv2_body(int blk, int nblk, const int *__restrict__ gate, int n, const double4 *__restrict__ vel,
                                              unsigned long long *__restrict__ thermo) {
     if (gate && *gate == 0) return;
     __shared__ double wsum[4];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blk * blockDim.x + threadIdx.x;
     double s = 0.0;
     if (i < n) {
         double4 v = vel[i];
@@ -96,10 +96,10 @@ __global__ void __launch_bounds__(256) k_mv2(const int *__restrict__ gate, int n
 // Velocity rescaling toward T0: mode 0 sets T to T0 (FixNVTRescale,
 // src/Fixes/FixNVTRescale.cu), mode 1 is Berendsen weak coupling
 // lambda^2 = 1 + (interval dt / tau)(T0/T - 1).
-__global__ void k_vscale(const int *__restrict__ gate, int n, double4 *__restrict__ vel, const unsigned long long *__restrict__ thermo,
+__device__ __forceinline__ void vscale_body(int blk, int nblk, const int *__restrict__ gate, int n, double4 *__restrict__ vel, const unsigned long long *__restrict__ thermo,
                          double tconv, double T0, int mode, double coupling) {
     if (gate && *gate == 0) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blk * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double T = (double)(long long)thermo[0] * kFxInv * tconv;
     if (!(T > 0.0)) return;
@@ -113,8 +113,8 @@ __global__ void k_vscale(const int *__restrict__ gate, int n, double4 *__restric
     vel[i] = v;
 }
 
-__global__ void k_fill_float(int n, float *__restrict__ p, float v) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void fill_float_body(int blk, int nblk, int n, float *__restrict__ p, float v) {
+    int i = blk * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
 
@@ -125,7 +125,7 @@ __global__ void k_fill_float(int n, float *__restrict__ p, float v) {
 // energy (sum m v^2, which equipartition pairs with ndf kT) from thermo[0]
 // first (the stepFinal half).  Chain positions are integrated
 // as well so that the conserved extended energy can be reported.
-__global__ void k_nh_half(const int *__restrict__ gate, const unsigned long long *__restrict__ thermo,
+__device__ __forceinline__ void nh_half_body(int blk, int nblk, const int *__restrict__ gate, const unsigned long long *__restrict__ thermo,
                           double *__restrict__ st, int M, double ndf, double kT, double dt, double mvv,
                           int measure) {
     if (gate && *gate == 0) return;
@@ -163,7 +163,7 @@ __global__ void k_nh_half(const int *__restrict__ gate, const unsigned long long
 
 // chain start (FixNoseHoover::prepareFinal, FixNoseHoover.cu:240-300 and
 // updateThermalMasses, 1034-1042): Q_0 = ndf kT / w^2, Q_k = kT / w^2
-__global__ void k_nh_start(const unsigned long long *__restrict__ thermo, double *__restrict__ st, int M,
+__device__ __forceinline__ void nh_start_body(int blk, int nblk, const unsigned long long *__restrict__ thermo, double *__restrict__ st, int M,
                            double ndf, double kT, double freq, double mvv) {
     double *vel = st, *frc = st + M, *mass = st + 2 * M, *pos = st + 3 * M;
     const double ke = (double)(long long)thermo[0] * kFxInv * mvv;  // 2 KE
@@ -178,8 +178,8 @@ __global__ void k_nh_start(const unsigned long long *__restrict__ thermo, double
     for (int k = 1; k < M; ++k) frc[k] = (mass[k - 1] * vel[k - 1] * vel[k - 1] - kT) / mass[k];
 }
 
-__global__ void k_vscale_dev(int n, double4 *__restrict__ vel, const double *__restrict__ scale) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void vscale_dev_body(int blk, int nblk, int n, double4 *__restrict__ vel, const double *__restrict__ scale) {
+    int i = blk * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double s = *scale;
     double4 v = vel[i];
@@ -192,29 +192,26 @@ __global__ void k_vscale_dev(int n, double4 *__restrict__ vel, const double *__r
 }  // namespace
 
 static void measure_mv2(Ctx &c, const int *gate) {
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, gate, 2, c.thermo);
-    hipLaunchKernelGGL(k_mv2, dim3(cdiv(c.n, 256)), dim3(256), 0, c.stream, gate, c.n, c.vel[c.cur], c.thermo);
+    launch<zero_u64_body, 64>(c, 1, 0, gate, 2, c.thermo);
+    launch<mv2_body, 256>(c, cdiv(c.n, 256), 0, gate, c.n, c.vel[c.cur], c.thermo);
 }
 
 static void nh_half(Ctx &c, bool final_half) {
     NoseHooverState &H = c.nh;
     if (final_half) measure_mv2(c, nullptr);  // every step, also inside a captured step
-    hipLaunchKernelGGL(k_nh_half, dim3(1), dim3(1), 0, c.stream, nullptr, c.thermo, H.dev, H.chain, H.ndf,
-                       H.boltz * H.T, c.dt, H.mvv, final_half ? 1 : 0);
-    hipLaunchKernelGGL(k_vscale_dev, dim3(cdiv(c.n, 256)), dim3(256), 0, c.stream, c.n, c.vel[c.cur],
-                       H.dev + 4 * H.chain + 1);
+    launch<nh_half_body, 1>(c, 1, 0, nullptr, c.thermo, H.dev, H.chain, H.ndf, H.boltz * H.T, c.dt, H.mvv, final_half ? 1 : 0);
+    launch<vscale_dev_body, 256>(c, cdiv(c.n, 256), 0, c.n, c.vel[c.cur], H.dev + 4 * H.chain + 1);
 }
 
 void nose_hoover_start(Ctx &c) {
     NoseHooverState &H = c.nh;
     measure_mv2(c, nullptr);
-    hipLaunchKernelGGL(k_nh_start, dim3(1), dim3(1), 0, c.stream, c.thermo, H.dev, H.chain, H.ndf,
-                       H.boltz * H.T, H.freq, H.mvv);
+    launch<nh_start_body, 1>(c, 1, 0, c.thermo, H.dev, H.chain, H.ndf, H.boltz * H.T, H.freq, H.mvv);
     H.fresh = false;
 }
 
 void fill_float(Ctx &c, float *p, int count, float v) {
-    hipLaunchKernelGGL(k_fill_float, dim3(cdiv(count, 256)), dim3(256), 0, c.stream, count, p, v);
+    launch<fill_float_body, 256>(c, cdiv(count, 256), 0, count, p, v);
 }
 
 void thermo_measure(Ctx &c) { measure_mv2(c, thermo_gate(c)); }
@@ -222,8 +219,7 @@ void thermo_measure(Ctx &c) { measure_mv2(c, thermo_gate(c)); }
 void thermostat_apply(Ctx &c) {
     if (!c.fuseThermo) thermo_measure(c);  // else summed by the rigid second half
     const double coupling = c.thermoTau > 0 ? c.thermoEvery * c.dt / c.thermoTau : 1.0;
-    hipLaunchKernelGGL(k_vscale, dim3(cdiv(c.n, 256)), dim3(256), 0, c.stream, thermo_gate(c), c.n, c.vel[c.cur], c.thermo,
-                       c.tconv, c.thermoT, c.thermoTau > 0 ? 1 : 0, coupling);
+    launch<vscale_body, 256>(c, cdiv(c.n, 256), 0, thermo_gate(c), c.n, c.vel[c.cur], c.thermo, c.tconv, c.thermoT, c.thermoTau > 0 ? 1 : 0, coupling);
 }
 
 void acc_read(Ctx &c, double out[ACC_N]) {
@@ -238,15 +234,14 @@ void acc_read(Ctx &c, double out[ACC_N]) {
 }
 
 void acc_clear(Ctx &c, int slot, int count) {
-    hipLaunchKernelGGL(k_acc_clear, dim3(1), dim3(kAccRep * ACC_N), 0, c.stream, c.acc, slot, count);
+    launch<acc_clear_body, kAccRep * ACC_N>(c, 1, 0, c.acc, slot, count);
 }
 
 void zero_forces(Ctx &c, bool clear_acc) {
     const int threads = 256;
-    hipLaunchKernelGGL(k_zero_u64, dim3(cdiv(3LL * c.n, threads)), dim3(threads), 0, c.stream, nullptr, 3 * c.n,
-                       c.force);
+    launch<zero_u64_body, 256>(c, cdiv(3LL * c.n, threads), 0, nullptr, 3 * c.n, c.force);
     if (clear_acc)
-        hipLaunchKernelGGL(k_zero_u64, dim3(cdiv(kAccTotal, 256)), dim3(256), 0, c.stream, nullptr, kAccTotal, c.acc);
+        launch<zero_u64_body, 256>(c, cdiv(kAccTotal, 256), 0, nullptr, kAccTotal, c.acc);
 }
 
 void integrate_first_half(Ctx &c) {
@@ -254,23 +249,20 @@ void integrate_first_half(Ctx &c) {
     if (c.nh.on) nh_half(c, false);
     const int *member = c.rigid.on ? c.rigid.member : nullptr;
     if (!c.rigid.on || c.rigid.nsites < c.n)
-        hipLaunchKernelGGL(k_vv1, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n,
-                           c.pos[c.cur], c.vel[c.cur], c.force, 0.5 * c.dt * c.ftmToV, c.dt, member);
+        launch<vv1_body, 256>(c, cdiv(c.n, threads), 0, c.n, c.pos[c.cur], c.vel[c.cur], c.force, 0.5 * c.dt * c.ftmToV, c.dt, member);
     if (c.rigid.on) rigid_first_half(c);
 }
 
 void integrate_kick_first(Ctx &c) {
     if (c.nh.on) nh_half(c, false);
-    hipLaunchKernelGGL(k_kick, dim3(cdiv(c.n, 256)), dim3(256), 0, c.stream, c.n, c.vel[c.cur], c.force,
-                       0.5 * c.dt * c.ftmToV);
+    launch<kick_body, 256>(c, cdiv(c.n, 256), 0, c.n, c.vel[c.cur], c.force, 0.5 * c.dt * c.ftmToV);
 }
 
 void integrate_second_half(Ctx &c) {
     const int threads = 256;
     const int *member = c.rigid.on ? c.rigid.member : nullptr;
     if (!c.rigid.on || c.rigid.nsites < c.n)
-        hipLaunchKernelGGL(k_vv2, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n,
-                           c.vel[c.cur], c.force, 0.5 * c.dt * c.ftmToV, member);
+        launch<vv2_body, 256>(c, cdiv(c.n, threads), 0, c.n, c.vel[c.cur], c.force, 0.5 * c.dt * c.ftmToV, member);
     if (c.rigid.on) rigid_second_half(c, true);
     if (c.nh.on) nh_half(c, true);
 }

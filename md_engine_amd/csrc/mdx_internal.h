@@ -21,6 +21,8 @@
 #include <numeric>
 #include <vector>
 
+#include "mdx_launch.h"
+
 namespace mdx {
 
 constexpr double kFxScale = 4294967296.0;         // 2^32, forces / energies / virials
@@ -378,6 +380,19 @@ struct RigidState {
 // the pair and E3B kernels then add r . F into ACC_VTRACE only
 constexpr int kFlagVirialTrace = 1 << 12;  // MDX_VIRIAL_TRACE of the C ABI
 
+// Device memory and FFT plans that a merged batch graph uses, released with it.
+struct BatchKeep {
+    std::vector<void *> mem;
+    std::vector<hipfftHandle> plans;
+    bool empty() const { return mem.empty() && plans.empty(); }
+    void release() {
+        for (void *p : mem) hipFree(p);
+        for (hipfftHandle h : plans) hipfftDestroy(h);
+        mem.clear();
+        plans.clear();
+    }
+};
+
 struct Ctx {
     int device = 0;
     bool noAlloc = false;  // inside mdx_forces_compute_noalloc: any device allocation is an error
@@ -392,7 +407,10 @@ struct Ctx {
     struct GraphCache {
         hipGraphExec_t big = nullptr, one = nullptr;
         std::vector<uint64_t> key;
+        BatchKeep keep;  // what merged batch launches use: tables, block maps, grouped FFTs
     } graphs;
+    bool batchMerge = true;      // option batch_merge: one launch per kernel across a batch
+    int batchMerged = 0;         // the last batch capture merged its replicas' launches (1) or not (0)
     bool vtrace = false;  // set by compute for the terms it launches
     bool fuseMolVirial = false;  // captured barostat steps: rigid second half sums ACC_MOL
     bool fuseThermo = false;     // captured thermostat steps: rigid second half sums m v^2
@@ -417,6 +435,7 @@ struct Ctx {
     BoxDev *box = nullptr;                 // device copy of the box-dependent launch data
     BoxDev *boxLast = nullptr;             // the box of the last list build (copied by k_scan)
     int replayChunk = 0;                   // option replay_chunk: chunk of each run replayed once
+    std::vector<LaunchRec> *rec = nullptr; // set while a batch records this context's step launches
     BoxDev boxHost;                        // host mirror (last upload or read-back)
     double4 *posLast = nullptr;
     std::vector<double> invMassHost;
@@ -548,6 +567,70 @@ struct ProfScope {
     ~ProfScope();
 };
 
+// Launches a per-step kernel body for this context, or records it when a
+// batch is being put together (see mdx_launch.h).  The arguments are
+// converted to the body's parameter types first, so every context of a batch
+// records the same kernel whatever the argument expressions' types.
+template <auto BODY, int NT, class... P>
+void launch_typed(Ctx &c, int blocks, size_t shm, P... a) {
+    if (c.rec) {
+        LaunchRec r;
+        r.kid = reinterpret_cast<const void *>(&k_many<BODY, NT, P...>);
+        r.blocks = blocks;
+        r.shm = shm;
+        ArgPack<P...> p;
+        pack_fill(p, a...);
+        r.args.resize(sizeof(p));
+        std::memcpy(r.args.data(), &p, sizeof(p));
+        r.align = alignof(ArgPack<P...>);
+        r.many = &launch_many<BODY, NT, P...>;
+        c.rec->push_back(std::move(r));
+        return;
+    }
+    if (blocks > 0) hipLaunchKernelGGL((k_one<BODY, NT, P...>), dim3(blocks), dim3(NT), shm, c.stream, a...);
+}
+
+template <class F>
+struct BodyParams;
+template <class... P>
+struct BodyParams<void (*)(int, int, P...)> {
+    template <auto BODY, int NT, class... A>
+    static void launch(Ctx &c, int blocks, size_t shm, A... a) {
+        launch_typed<BODY, NT, P...>(c, blocks, shm, static_cast<P>(a)...);
+    }
+    template <auto BODY, int NT, class... A>
+    static void launch_on(hipStream_t s, int blocks, size_t shm, A... a) {
+        if (blocks > 0) hipLaunchKernelGGL((k_one<BODY, NT, P...>), dim3(blocks), dim3(NT), shm, s, static_cast<P>(a)...);
+    }
+};
+
+template <auto BODY, int NT, class... A>
+void launch(Ctx &c, int blocks, size_t shm, A... a) {
+    BodyParams<decltype(BODY)>::template launch<BODY, NT>(c, blocks, shm, a...);
+}
+
+// A per-step kernel body on a given stream, never recorded (launches that
+// serve several contexts at once, such as the ring polymer's).
+template <auto BODY, int NT, class... A>
+void launch_on(hipStream_t s, int blocks, size_t shm, A... a) {
+    BodyParams<decltype(BODY)>::template launch_on<BODY, NT>(s, blocks, shm, a...);
+}
+
+// A stream operation that is not a kernel of the launch layer (an FFT):
+// run now, or recorded in order with the kernels of a batch.  fft (1 forward
+// R2C of the charge mesh, 2 inverse C2R of the three field components) lets a
+// merged batch run the transforms of replicas with equal meshes as one.
+inline void stream_op(Ctx &c, std::function<void(hipStream_t)> f, int fft = 0) {
+    if (c.rec) {
+        LaunchRec r;
+        r.host = std::move(f);
+        r.fft = fft;
+        c.rec->push_back(std::move(r));
+        return;
+    }
+    f(c.stream);
+}
+
 // error helpers -------------------------------------------------------------
 struct Error {
     int code;
@@ -612,9 +695,9 @@ __device__ __forceinline__ float team_sum(float v) {
 // index is past the end return.
 constexpr int kXcds = 8;
 inline int xcd_grid(long long nlogical) { return cdiv(nlogical, kXcds) * kXcds; }
-__device__ __forceinline__ int xcd_block(int nlogical) {
+__device__ __forceinline__ int xcd_block(int blk, int nlogical) {
     const int per = (nlogical + kXcds - 1) / kXcds;
-    return ((int)blockIdx.x % kXcds) * per + (int)blockIdx.x / kXcds;
+    return (blk % kXcds) * per + blk / kXcds;
 }
 
 // maximum over the 64 lanes of a wave (every lane gets it)
@@ -789,7 +872,7 @@ void baro_step(Ctx &c, const int *gate);
 // Green function from the device box on stream s (gate: nullptr = always)
 // dl != nullptr: the same launch also dilates (extra workgroups after the
 // Green function's); returns whether it did
-bool pppm_green_device(Ctx &c, const int *gate, hipStream_t s, const DilateArgs *dl = nullptr);
+bool pppm_green_device(Ctx &c, const int *gate, const DilateArgs *dl = nullptr);
 void rigid_first_half(Ctx &c);
 // ring.hip: ring-polymer beads, one context per bead, coupled by the
 // normal-mode propagation of the free ring polymer
@@ -806,9 +889,10 @@ struct RingSpec {
     float *q = nullptr;       // charges by atom id
     long long *cf = nullptr;  // centroid forces by atom id, fixed point (kept zero between uses)
 };
-void ring_drift(const RingSpec &r, const BoxDev *box, hipStream_t s);
-void ring_centroids(const RingSpec &r, const BoxDev *box, hipStream_t s);
-void ring_apply(const RingSpec &r, hipStream_t s);
+// ring-level launches, issued (or recorded) on the first bead's context
+void ring_drift(const RingSpec &r, Ctx &c0);
+void ring_centroids(const RingSpec &r, Ctx &c0);
+void ring_apply(const RingSpec &r, Ctx &c0);
 // pppm.hip: the mesh part of every bead from the ring's centroids, on the
 // first bead's mesh (centroid force into r.cf, then onto every bead)
 void ring_pppm(Ctx &c0, const RingSpec &r);

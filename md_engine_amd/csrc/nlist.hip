@@ -54,10 +54,10 @@ __device__ __forceinline__ int cell_coord(double x, double lo, double cellInv, i
 
 // Max displacement since the last build (setBuildFlag, GridGPU.cu:755-774):
 // flag when any |minImage(x - x_last)|^2 > (skin/2)^2.
-__global__ void k_check(int n, const double4 *__restrict__ pos, const double4 *__restrict__ last,
+__device__ __forceinline__ void check_body(int blk, int nblk, int n, const double4 *__restrict__ pos, const double4 *__restrict__ last,
                         const BoxDev *__restrict__ bx, double thr2, int *__restrict__ flags) {
     const Geometry g = bx->geo;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blk * blockDim.x + threadIdx.x;
     bool move = false;
     if (i < n) move = dist2(pos[i], last[i], g) > thr2;
     if (__any(move) && (threadIdx.x & 63) == 0) flags[0] = 1;
@@ -69,13 +69,13 @@ __global__ void k_check(int n, const double4 *__restrict__ pos, const double4 *_
 // atoms come first, each class sorted by cell: a neighbor row then lists its
 // charged partners first, and the charge spread and gather see only charged
 // atoms.
-__global__ void k_bin(const int *__restrict__ gate, int n, double4 *__restrict__ pos, const float *__restrict__ q,
+__device__ __forceinline__ void bin_body(int blk, int nblk, const int *__restrict__ gate, int n, double4 *__restrict__ pos, const float *__restrict__ q,
                       const BoxDev *__restrict__ bx, int ncell, const int *__restrict__ colRank,
                       int *__restrict__ cellOf, int *__restrict__ cellCount) {
     if (gate && *gate == 0) return;
     const Geometry g = bx->geo;
     const CellGrid cg = bx->cg;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blk * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double4 p = pos[i];
     double *pc = &p.x;
@@ -101,7 +101,7 @@ __global__ void k_bin(const int *__restrict__ gate, int n, double4 *__restrict__
 // thread then writes its segment's prefixes.  Also seeds the fill cursors.
 // The counts are cleared as they are consumed, ready for the next binning
 // (no separate clearing launch in every captured step).
-__global__ void __launch_bounds__(1024) k_scan(const int *__restrict__ gate, int ncell, int *__restrict__ count,
+__device__ __forceinline__ void scan_body(int blk, int nblk, const int *__restrict__ gate, int ncell, int *__restrict__ count,
                                                int *__restrict__ start, int *__restrict__ fill,
                                                const BoxDev *__restrict__ box, BoxDev *__restrict__ boxLast) {
     if (gate && *gate == 0) return;
@@ -115,8 +115,17 @@ __global__ void __launch_bounds__(1024) k_scan(const int *__restrict__ gate, int
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int seg = (ncell + 1023) / 1024;
     const int b0 = threadIdx.x * seg, b1 = min(b0 + seg, ncell);
+    // a thread's segment is read eight counts at a time with the loads in
+    // flight together (one dependent load per count made this kernel ~17 us)
+    constexpr int U = 8;
+    int v[U];
     int local = 0;
-    for (int k = b0; k < b1; ++k) local += count[k];
+    for (int k0 = b0; k0 < b1; k0 += U) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = k0 + u < b1 ? count[k0 + u] : 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) local += v[u];
+    }
     int x = local;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -136,19 +145,28 @@ __global__ void __launch_bounds__(1024) k_scan(const int *__restrict__ gate, int
     }
     __syncthreads();
     int run = (wid > 0 ? wsum[wid - 1] : 0) + x - local;
-    for (int k = b0; k < b1; ++k) {
-        start[k] = run;
-        fill[k] = run;
-        run += count[k];
-        count[k] = 0;
+    for (int k0 = b0; k0 < b1; k0 += U) {
+        if (seg > U) {  // uniform; otherwise v still holds this segment
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = k0 + u < b1 ? count[k0 + u] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (k0 + u < b1) {
+                start[k0 + u] = run;
+                fill[k0 + u] = run;
+                count[k0 + u] = 0;
+                run += v[u];
+            }
+        }
     }
     if (threadIdx.x == 1023) start[ncell] = wsum[15];
 }
 
-__global__ void k_place(const int *__restrict__ gate, int n, const int *__restrict__ cellOf, int *__restrict__ fill,
+__device__ __forceinline__ void place_body(int blk, int nblk, const int *__restrict__ gate, int n, const int *__restrict__ cellOf, int *__restrict__ fill,
                         int *__restrict__ perm) {
     if (gate && *gate == 0) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blk * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int slot = atomicAdd(&fill[cellOf[i]], 1);
     perm[slot] = i;
@@ -157,14 +175,14 @@ __global__ void k_place(const int *__restrict__ gate, int n, const int *__restri
 // Make the order inside each cell independent of atomic slot order: one
 // workgroup per cell slot ranks its entries by atom id in LDS (ids are unique,
 // so rank = number of smaller keys) and writes them in rank order.
-__global__ void __launch_bounds__(64) k_cellsort(const int *__restrict__ gate, int ncell,
+__device__ __forceinline__ void cellsort_body(int blk, int nblk, const int *__restrict__ gate, int ncell,
                                                  const int *__restrict__ start, const int *__restrict__ permIn,
                                                  int *__restrict__ permOut, const int *__restrict__ id) {
     if (gate && *gate == 0) return;
     constexpr int CAP = 1024;
     __shared__ int skey[CAP];
     __shared__ int sperm[CAP];
-    const int c = blockIdx.x;
+    const int c = blk;
     const int a = start[c], b = start[c + 1], m = b - a;
     if (m <= CAP) {
         for (int t = threadIdx.x; t < m; t += 64) {
@@ -197,7 +215,7 @@ __global__ void __launch_bounds__(64) k_cellsort(const int *__restrict__ gate, i
 
 // Gather the per-atom records into cell order (sortPerAtomArrays,
 // GridGPU.cu:253-293) and refresh idToIdx, the last-build positions and xq.
-__global__ void k_permute(const int *__restrict__ gate, int n, const int *__restrict__ perm, const double4 *__restrict__ pos0,
+__device__ __forceinline__ void permute_body(int blk, int nblk, const int *__restrict__ gate, int n, const int *__restrict__ perm, const double4 *__restrict__ pos0,
                           const double4 *__restrict__ vel0, const float *__restrict__ q0,
                           const int *__restrict__ t0, const int *__restrict__ id0,
                           double4 *__restrict__ pos1, double4 *__restrict__ vel1,
@@ -206,7 +224,7 @@ __global__ void k_permute(const int *__restrict__ gate, int n, const int *__rest
                           float4 *__restrict__ xq, const BoxDev *__restrict__ bx) {
     if (gate && *gate == 0) return;
     const Geometry g = bx->geo;
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = blk * blockDim.x + threadIdx.x;
     if (k >= n) return;
     int o = perm[k];
     double4 p = pos0[o];
@@ -222,10 +240,10 @@ __global__ void k_permute(const int *__restrict__ gate, int n, const int *__rest
     xq[k] = make_float4((float)(p.x - g.lo[0]), (float)(p.y - g.lo[1]), (float)(p.z - g.lo[2]), qq);
 }
 
-__global__ void k_xq(int n, const double4 *__restrict__ pos, const float *__restrict__ q,
+__device__ __forceinline__ void xq_body(int blk, int nblk, int n, const double4 *__restrict__ pos, const float *__restrict__ q,
                      float4 *__restrict__ xq, const BoxDev *__restrict__ bx) {
     const Geometry g = bx->geo;
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = blk * blockDim.x + threadIdx.x;
     if (k >= n) return;
     double4 p = pos[k];
     xq[k] = make_float4((float)(p.x - g.lo[0]), (float)(p.y - g.lo[1]), (float)(p.z - g.lo[2]), q[k]);
@@ -233,11 +251,11 @@ __global__ void k_xq(int n, const double4 *__restrict__ pos, const float *__rest
 
 // k_xq fused with the clearing of the fixed-point force rows and of the
 // energy / virial accumulators that the force kernels add into.
-__global__ void k_xq_zero(int n, const double4 *__restrict__ pos, const float *__restrict__ q,
+__device__ __forceinline__ void xq_zero_body(int blk, int nblk, int n, const double4 *__restrict__ pos, const float *__restrict__ q,
                           float4 *__restrict__ xq, const BoxDev *__restrict__ bx,
                           unsigned long long *__restrict__ force, int nacc, unsigned long long *__restrict__ acc) {
     const Geometry g = bx->geo;
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = blk * blockDim.x + threadIdx.x;
     if (k < nacc) acc[k] = 0ull;
     if (k >= n) return;
     double4 p = pos[k];
@@ -262,7 +280,7 @@ __device__ __forceinline__ void excl_row(int i, int myid, const int *__restrict_
     }
 }
 
-__global__ void k_copy_back(const int *__restrict__ gate, int n, const double4 *__restrict__ pos1,
+__device__ __forceinline__ void copy_back_body(int blk, int nblk, const int *__restrict__ gate, int n, const double4 *__restrict__ pos1,
                             const double4 *__restrict__ vel1, const float *__restrict__ q1,
                             const int *__restrict__ t1, const int *__restrict__ id1, double4 *__restrict__ pos0,
                             double4 *__restrict__ vel0, float *__restrict__ q0, int *__restrict__ t0,
@@ -271,7 +289,7 @@ __global__ void k_copy_back(const int *__restrict__ gate, int n, const double4 *
                             uint32_t *__restrict__ exclIdx, int *__restrict__ atomMolO,
                             int *__restrict__ member) {
     if (gate && *gate == 0) return;
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = blk * blockDim.x + threadIdx.x;
     if (k >= n) return;
     pos0[k] = pos1[k];
     vel0[k] = vel1[k];
@@ -306,7 +324,7 @@ __device__ __forceinline__ void part_max(int nb, const float *__restrict__ part,
 // reference's setBuildFlag test, GridGPU.cu:755-774), count it, and set the
 // thermostat gate for the steps where the turn reaches a multiple of its
 // interval.  Clears the displacement slot for the next step.
-__global__ void k_decide(int nb, const float *__restrict__ part, unsigned *__restrict__ dmax,
+__device__ __forceinline__ void decide_body(int blk, int nblk, int nb, const float *__restrict__ part, unsigned *__restrict__ dmax,
                          int *__restrict__ flags, float half2, int every, int baroEvery,
                          unsigned long long *__restrict__ thermo) {
     float d2, p2;
@@ -476,7 +494,7 @@ __device__ __forceinline__ bool in_run(uint32_t ex, int s0, int s1) {
 }
 
 template <int T, bool SHIFT>
-__global__ void __launch_bounds__(256) k_build(const int *__restrict__ gate, 
+__device__ __forceinline__ void build_body(int blk, int nblk, const int *__restrict__ gate, 
     int n, const double4 *__restrict__ pos, const float4 *__restrict__ xq, const BoxDev *__restrict__ bx,
     const int *__restrict__ colRank, const int *__restrict__ cellStart, int ncell,
     double rlist2, const uint32_t *__restrict__ exclIdx, int maxExcl, int cap,
@@ -491,7 +509,7 @@ __global__ void __launch_bounds__(256) k_build(const int *__restrict__ gate,
     constexpr int R = 4 * kMaxCols;           // runs per table: 2 classes x 2 z-runs per column
     __shared__ int sOff[256 / T][R];          // atom index = stream position + sOff
     __shared__ uint32_t sEnd[256 / T][R];     // end of the run's candidates | code << kEndBits
-    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gtid = blk * blockDim.x + threadIdx.x;
     const int i = gtid / T;
     const int team = threadIdx.x / T;
     const int tl = threadIdx.x & (T - 1);
@@ -721,7 +739,7 @@ struct RunEntry {
 };
 
 template <bool SHIFT>
-__global__ void __launch_bounds__(256) k_build_wave(const int *__restrict__ gate,
+__device__ __forceinline__ void build_wave_body(int blk, int nblk, const int *__restrict__ gate,
     int n, const double4 *__restrict__ pos, const float4 *__restrict__ xq, const BoxDev *__restrict__ bx,
     const int *__restrict__ colRank, const int *__restrict__ cellStart, int ncell,
     double rlist2, const uint32_t *__restrict__ exclIdx, int maxExcl, int cap,
@@ -736,7 +754,7 @@ __global__ void __launch_bounds__(256) k_build_wave(const int *__restrict__ gate
     constexpr int R = 4 * kMaxCols;  // 2 classes x 2 z-runs per column
     __shared__ RunEntry sRun[4][R + 1];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (int)(threadIdx.x >> 6));
+    const int i = __builtin_amdgcn_readfirstlane(blk * 4 + (int)(threadIdx.x >> 6));
     if (i >= n) {
         if (lane == 0) waveMax[i] = 0;
         return;
@@ -929,11 +947,11 @@ __global__ void __launch_bounds__(256) k_build_wave(const int *__restrict__ gate
 
 // Exclusion partners of each atom as current indices (idx | tag << 30), one
 // row of maxExcl per atom padded with ~0u, refreshed after each permutation.
-__global__ void k_excl_idx(const int *__restrict__ gate, int n, const int *__restrict__ id, const int *__restrict__ exclOff,
+__device__ __forceinline__ void excl_idx_body(int blk, int nblk, const int *__restrict__ gate, int n, const int *__restrict__ id, const int *__restrict__ exclOff,
                            const uint32_t *__restrict__ exclIds, const int *__restrict__ idToIdx,
                            int maxExcl, uint32_t *__restrict__ exclIdx) {
     if (gate && *gate == 0) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blk * blockDim.x + threadIdx.x;
     if (i >= n) return;
     excl_row(i, id[i], exclOff, exclIds, idToIdx, maxExcl, exclIdx);
 }
@@ -943,7 +961,7 @@ __global__ void k_excl_idx(const int *__restrict__ gate, int n, const int *__res
 // molecule GridGPU of FixE3B (FixE3B.cu:656-684) that the reference rebuilds
 // every step; with the skin criterion the per-step r2b set is a subset.
 template <int T>
-__global__ void __launch_bounds__(256) k_e3b_list(
+__device__ __forceinline__ void e3b_list_body(int blk, int nblk, 
     const int *__restrict__ gate, int nmol, const int4 *__restrict__ molAtoms, const int *__restrict__ atomMolO,
     const double4 *__restrict__ pos, const BoxDev *__restrict__ bx, double rcut2, int cap,
     const uint32_t *__restrict__ nl,
@@ -951,7 +969,7 @@ __global__ void __launch_bounds__(256) k_e3b_list(
     int *__restrict__ mnl, int *__restrict__ mcount, int *__restrict__ waveMax) {
     if (gate && *gate == 0) return;
     const Geometry g = bx->geo;
-    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gtid = blk * blockDim.x + threadIdx.x;
     const int m = gtid / T;
     const int tl = threadIdx.x & (T - 1);
     const int lane = threadIdx.x & 63;
@@ -992,12 +1010,12 @@ __global__ void __launch_bounds__(256) k_e3b_list(
 
 // Captured steps also map the rigid molecules here (k_rigid_map's work, rigid
 // molecules nmolR > 0, member cleared by k_copy_back): one launch for both.
-__global__ void k_mol_atoms(const int *__restrict__ gate, int nmol, const int4 *__restrict__ molIds, const int *__restrict__ idToIdx,
+__device__ __forceinline__ void mol_atoms_body(int blk, int nblk, const int *__restrict__ gate, int nmol, const int4 *__restrict__ molIds, const int *__restrict__ idToIdx,
                             int4 *__restrict__ molAtoms, int *__restrict__ atomMolO, int nmolR,
                             const int4 *__restrict__ rigidIds, int4 *__restrict__ rigidAtoms,
                             int *__restrict__ member) {
     if (gate && *gate == 0) return;
-    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    int m = blk * blockDim.x + threadIdx.x;
     if (m < nmolR) {
         const int4 i = rigidIds[m];
         const int4 a = make_int4(idToIdx[i.x], idToIdx[i.y], idToIdx[i.z], i.w >= 0 ? idToIdx[i.w] : -1);
@@ -1014,21 +1032,35 @@ __global__ void k_mol_atoms(const int *__restrict__ gate, int nmol, const int4 *
     atomMolO[a.x] = m;
 }
 
-__global__ void k_fill_int(const int *__restrict__ gate, int n, int *__restrict__ p, int v) {
+__device__ __forceinline__ void fill_int_body(int blk, int nblk, const int *__restrict__ gate, int n, int *__restrict__ p, int v) {
     if (gate && *gate == 0) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blk * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
 
 // Longest pair row and molecule row of the build from the per-wave slots,
 // folded into the running maxima flags[1], flags[2] (one workgroup).
-__global__ void __launch_bounds__(1024) k_row_max(const int *__restrict__ gate, int na, const int *__restrict__ a,
+__device__ __forceinline__ void row_max_body(int blk, int nblk, const int *__restrict__ gate, int na, const int *__restrict__ a,
                                                   int nb, const int *__restrict__ b, int *__restrict__ flags) {
     if (gate && *gate == 0) return;
     __shared__ int red[2][16];
     int ma = 0, mb = 0;
-    for (int k = threadIdx.x; k < na; k += 1024) ma = max(ma, a[k]);
-    for (int k = threadIdx.x; k < nb; k += 1024) mb = max(mb, b[k]);
+    // sixteen loads in flight per thread and pass
+    constexpr int U = 16;
+    for (int k0 = threadIdx.x; k0 < na; k0 += U * 1024) {
+        int v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = k0 + u * 1024 < na ? a[k0 + u * 1024] : 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) ma = max(ma, v[u]);
+    }
+    for (int k0 = threadIdx.x; k0 < nb; k0 += U * 1024) {
+        int v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = k0 + u * 1024 < nb ? b[k0 + u * 1024] : 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) mb = max(mb, v[u]);
+    }
     ma = wave_max(ma);
     mb = wave_max(mb);
     const int w = threadIdx.x >> 6;
@@ -1075,11 +1107,10 @@ void launch_build_wave(Ctx &c, int cur) {
     bool shift = true;
     for (int a = 0; a < 3; ++a)
         if (c.geo.periodic[a] && c.nc[a] < 2 * c.reach[a] + 1) shift = false;
-    auto kern = shift ? k_build_wave<true> : k_build_wave<false>;
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, c.stream, rebuild_gate(c), c.n, c.pos[cur], c.xq, c.box,
-                       c.colRank, c.cellStart, c.ncell, c.rlist * c.rlist,
-                       c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.nlCountA, c.dFlags,
-                       c.waveMax);
+    if (shift)
+        launch<build_wave_body<true>, 256>(c, blocks, 0, rebuild_gate(c), c.n, c.pos[cur], c.xq, c.box, c.colRank, c.cellStart, c.ncell, c.rlist * c.rlist, c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.nlCountA, c.dFlags, c.waveMax);
+    else
+        launch<build_wave_body<false>, 256>(c, blocks, 0, rebuild_gate(c), c.n, c.pos[cur], c.xq, c.box, c.colRank, c.cellStart, c.ncell, c.rlist * c.rlist, c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.nlCountA, c.dFlags, c.waveMax);
     c.waveMaxA = blocks * 4;
 }
 
@@ -1093,11 +1124,10 @@ void launch_build(Ctx &c, int cur) {
     bool shift = true;
     for (int a = 0; a < 3; ++a)
         if (c.geo.periodic[a] && c.nc[a] < 2 * c.reach[a] + 1) shift = false;
-    auto kern = shift ? k_build<T, true> : k_build<T, false>;
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), 0, c.stream, rebuild_gate(c), c.n, c.pos[cur], c.xq, c.box,
-                       c.colRank, c.cellStart, c.ncell, c.rlist * c.rlist,
-                       c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.nlCountA, c.dFlags,
-                       c.waveMax);
+    if (shift)
+        launch<build_body<T, true>, 256>(c, blocks, 0, rebuild_gate(c), c.n, c.pos[cur], c.xq, c.box, c.colRank, c.cellStart, c.ncell, c.rlist * c.rlist, c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.nlCountA, c.dFlags, c.waveMax);
+    else
+        launch<build_body<T, false>, 256>(c, blocks, 0, rebuild_gate(c), c.n, c.pos[cur], c.xq, c.box, c.colRank, c.cellStart, c.ncell, c.rlist * c.rlist, c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.nlCountA, c.dFlags, c.waveMax);
     c.waveMaxA = blocks * (threads / 64);
 }
 
@@ -1106,10 +1136,7 @@ void launch_e3b_list(Ctx &c, int cur) {
     int threads = 256;
     int blocks = cdiv((long long)c.e3b.nmol * T, threads);
     double rc = (double)c.e3b.r2b + c.skin;
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_e3b_list<T>), dim3(blocks), dim3(threads), 0, c.stream, rebuild_gate(c),
-                       c.e3b.nmol, c.e3b.molAtoms, c.e3b.atomMolO, c.pos[cur], c.box, rc * rc, c.cap,
-                       c.nl, c.nlCount, c.nlCountA, c.e3b.oxUncharged ? 1 : 0, c.e3b.cap, c.e3b.nl,
-                       c.e3b.nlCount, c.waveMax + wave_slots_a(c));
+    launch<e3b_list_body<T>, 256>(c, blocks, 0, rebuild_gate(c), c.e3b.nmol, c.e3b.molAtoms, c.e3b.atomMolO, c.pos[cur], c.box, rc * rc, c.cap, c.nl, c.nlCount, c.nlCountA, c.e3b.oxUncharged ? 1 : 0, c.e3b.cap, c.e3b.nl, c.e3b.nlCount, c.waveMax + wave_slots_a(c));
     c.waveMaxB = blocks * (threads / 64);
 }
 
@@ -1185,21 +1212,18 @@ void nlist_setup(Ctx &c) {
 
 void refresh_xq(Ctx &c) {
     int threads = 256;
-    hipLaunchKernelGGL(k_xq, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[c.cur],
-                       c.q[c.cur], c.xq, c.box);
+    launch<xq_body, 256>(c, cdiv(c.n, threads), 0, c.n, c.pos[c.cur], c.q[c.cur], c.xq, c.box);
 }
 
 void prepare_forces(Ctx &c) {
     const int threads = 256;
-    hipLaunchKernelGGL(k_xq_zero, dim3(cdiv(std::max(c.n, kAccTotal), threads)), dim3(threads), 0, c.stream, c.n,
-                       c.pos[c.cur], c.q[c.cur], c.xq, c.box, c.force, kAccTotal, c.acc);
+    launch<xq_zero_body, 256>(c, cdiv(std::max(c.n, kAccTotal), threads), 0, c.n, c.pos[c.cur], c.q[c.cur], c.xq, c.box, c.force, kAccTotal, c.acc);
 }
 
 void nlist_check(Ctx &c) {
     int threads = 256;
     double thr = 0.5 * c.skin;
-    hipLaunchKernelGGL(k_check, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n,
-                       c.pos[c.cur], c.posLast, c.box, thr * thr, c.dFlags);
+    launch<check_body, 256>(c, cdiv(c.n, threads), 0, c.n, c.pos[c.cur], c.posLast, c.box, thr * thr, c.dFlags);
 }
 
 static void run_build(Ctx &c, bool sync) {
@@ -1215,18 +1239,11 @@ static void run_build(Ctx &c, bool sync) {
     int threads = 256;
     int cur = c.cur, nxt = 1 - c.cur;
     const int nslot = 2 * c.ncell;
-    hipLaunchKernelGGL(k_bin, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, rebuild_gate(c), c.n, c.pos[cur],
-                       c.q[cur], c.box, c.ncell, c.colRank, c.cellOf, c.cellCount);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c.stream, rebuild_gate(c), nslot, c.cellCount, c.cellStart,
-                       c.cellFill, c.box, c.boxLast);
-    hipLaunchKernelGGL(k_place, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, rebuild_gate(c), c.n, c.cellOf,
-                       c.cellFill, c.perm);
-    hipLaunchKernelGGL(k_cellsort, dim3(nslot), dim3(64), 0, c.stream, rebuild_gate(c), nslot, c.cellStart, c.perm,
-                       c.perm2, c.id[cur]);
-    hipLaunchKernelGGL(k_permute, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, rebuild_gate(c), c.n, c.perm2,
-                       c.pos[cur], c.vel[cur], c.q[cur], c.type[cur], c.id[cur], c.pos[nxt],
-                       c.vel[nxt], c.q[nxt], c.type[nxt], c.id[nxt], c.idToIdx, c.posLast, c.xq,
-                       c.box);
+    launch<bin_body, 256>(c, cdiv(c.n, threads), 0, rebuild_gate(c), c.n, c.pos[cur], c.q[cur], c.box, c.ncell, c.colRank, c.cellOf, c.cellCount);
+    launch<scan_body, 1024>(c, 1, 0, rebuild_gate(c), nslot, c.cellCount, c.cellStart, c.cellFill, c.box, c.boxLast);
+    launch<place_body, 256>(c, cdiv(c.n, threads), 0, rebuild_gate(c), c.n, c.cellOf, c.cellFill, c.perm);
+    launch<cellsort_body, 64>(c, nslot, 0, rebuild_gate(c), nslot, c.cellStart, c.perm, c.perm2, c.id[cur]);
+    launch<permute_body, 256>(c, cdiv(c.n, threads), 0, rebuild_gate(c), c.n, c.perm2, c.pos[cur], c.vel[cur], c.q[cur], c.type[cur], c.id[cur], c.pos[nxt], c.vel[nxt], c.q[nxt], c.type[nxt], c.id[nxt], c.idToIdx, c.posLast, c.xq, c.box);
     const bool excl = c.exclOff && c.maxExcl > 0;
     if (excl) {
         MDX_REQUIRE(!c.capture || c.exclIdxN == (size_t)c.n * c.maxExcl, 3,
@@ -1240,16 +1257,11 @@ static void run_build(Ctx &c, bool sync) {
     if (c.capture) {
         // a captured step keeps fixed buffers: the sorted records come back,
         // with the exclusion rows and the cleared E3B oxygen map
-        hipLaunchKernelGGL(k_copy_back, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, rebuild_gate(c),
-                           c.n, c.pos[nxt], c.vel[nxt], c.q[nxt], c.type[nxt], c.id[nxt], c.pos[cur],
-                           c.vel[cur], c.q[cur], c.type[cur], c.id[cur], c.exclOff, c.exclIds, c.idToIdx,
-                           excl ? c.maxExcl : 0, c.exclIdx, c.e3b.on ? c.e3b.atomMolO : nullptr,
-                           c.rigid.on ? c.rigid.member : nullptr);
+        launch<copy_back_body, 256>(c, cdiv(c.n, threads), 0, rebuild_gate(c), c.n, c.pos[nxt], c.vel[nxt], c.q[nxt], c.type[nxt], c.id[nxt], c.pos[cur], c.vel[cur], c.q[cur], c.type[cur], c.id[cur], c.exclOff, c.exclIds, c.idToIdx, excl ? c.maxExcl : 0, c.exclIdx, c.e3b.on ? c.e3b.atomMolO : nullptr, c.rigid.on ? c.rigid.member : nullptr);
     } else {
         c.cur = nxt;
         if (excl)
-            hipLaunchKernelGGL(k_excl_idx, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, nullptr, c.n,
-                               c.id[c.cur], c.exclOff, c.exclIds, c.idToIdx, c.maxExcl, c.exclIdx);
+            launch<excl_idx_body, 256>(c, cdiv(c.n, threads), 0, nullptr, c.n, c.id[c.cur], c.exclOff, c.exclIds, c.idToIdx, c.maxExcl, c.exclIdx);
     }
     rigid_map(c);
     flex_map(c);
@@ -1257,9 +1269,7 @@ static void run_build(Ctx &c, bool sync) {
         // indices were permuted: drop the old map (captured steps did so in k_copy_back)
         if (!c.capture) fill_int(c, c.e3b.atomMolO, c.n, -1, nullptr);
         const int nmolR = c.capture && c.rigid.on ? c.rigid.nmol : 0;  // see rigid_map
-        hipLaunchKernelGGL(k_mol_atoms, dim3(cdiv(std::max(c.e3b.nmol, nmolR), threads)), dim3(threads), 0, c.stream,
-                           rebuild_gate(c), c.e3b.nmol, c.e3b.molIds, c.idToIdx, c.e3b.molAtoms, c.e3b.atomMolO,
-                           nmolR, c.rigid.ids, c.rigid.atoms, c.rigid.member);
+        launch<mol_atoms_body, 256>(c, cdiv(std::max(c.e3b.nmol, nmolR), threads), 0, rebuild_gate(c), c.e3b.nmol, c.e3b.molIds, c.idToIdx, c.e3b.molAtoms, c.e3b.atomMolO, nmolR, c.rigid.ids, c.rigid.atoms, c.rigid.member);
         MDX_REQUIRE(c.rlist >= (double)c.e3b.r2b + c.skin - 1e-9, 1,
                     "E3B needs rCut + padding >= 6.2 A + padding (the molecule list is taken from the "
                     "pair list)");
@@ -1284,8 +1294,7 @@ static void run_build(Ctx &c, bool sync) {
                 default: launch_e3b_list<32>(c, c.cur); break;
             }
         }
-        hipLaunchKernelGGL(k_row_max, dim3(1), dim3(1024), 0, c.stream, rebuild_gate(c), c.waveMaxA, c.waveMax,
-                           c.waveMaxB, c.waveMax + wave_slots_a(c), c.dFlags);
+        launch<row_max_body, 1024>(c, 1, 0, rebuild_gate(c), c.waveMaxA, c.waveMax, c.waveMaxB, c.waveMax + wave_slots_a(c), c.dFlags);
         if (c.capture) return;  // rows checked by the host after the chunk
         // largest row counts -> pinned host memory; checked now (sync) or at
         // the next synchronisation point (capacity_ok)
@@ -1353,7 +1362,7 @@ void nlist_rebuild_async(Ctx &c) {
 }
 
 void fill_int(Ctx &c, int *p, int count, int v, const int *gate) {
-    hipLaunchKernelGGL(k_fill_int, dim3(cdiv(count, 256)), dim3(256), 0, c.stream, gate, count, p, v);
+    launch<fill_int_body, 256>(c, cdiv(count, 256), 0, gate, count, p, v);
 }
 
 namespace {
@@ -1367,15 +1376,15 @@ namespace {
 // workgroup (same-address atomics serialise in L2).
 // Per-workgroup maxima into part[2 b], part[2 b + 1] (float bits); a
 // one-wave kernel reduces them (no same-address atomics).
-__global__ void __launch_bounds__(256) k_check_part(int n, const double4 *__restrict__ pos,
+__device__ __forceinline__ void check_part_body(int blk, int nblk, int n, const double4 *__restrict__ pos,
                                                     const double4 *__restrict__ last,
                                                     const double4 *__restrict__ vel, const BoxDev *__restrict__ bx,
                                                     double dt2, float *__restrict__ part) {
     const Geometry g = bx->geo;
     __shared__ float red[2][4];
-    const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t0 = blk * blockDim.x + threadIdx.x;
     float d2 = 0.f, p2 = 0.f;
-    for (int i = t0; i < n; i += gridDim.x * blockDim.x) {
+    for (int i = t0; i < n; i += nblk * blockDim.x) {
         const double dd = dist2(pos[i], last[i], g);
         const double4 v = vel[i];
         const double p = sqrt(dd) + dt2 * sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
@@ -1398,13 +1407,13 @@ __global__ void __launch_bounds__(256) k_check_part(int n, const double4 *__rest
             d2 = fmaxf(d2, red[0][k]);
             p2 = fmaxf(p2, red[1][k]);
         }
-        part[2 * blockIdx.x] = d2;
-        part[2 * blockIdx.x + 1] = p2;
+        part[2 * blk] = d2;
+        part[2 * blk + 1] = p2;
     }
 }
 
 // slot = max(slot, this check) as float bits; the other slot cleared for the next step
-__global__ void k_check_reduce(int nb, const float *__restrict__ part, unsigned *__restrict__ slot,
+__device__ __forceinline__ void check_reduce_body(int blk, int nblk, int nb, const float *__restrict__ part, unsigned *__restrict__ slot,
                                unsigned *__restrict__ other) {
     float d2, p2;
     part_max(nb, part, d2, p2);
@@ -1420,8 +1429,7 @@ __global__ void k_check_reduce(int nb, const float *__restrict__ part, unsigned 
 static int check_blocks(const Ctx &c) { return std::max(1, std::min(cdiv(c.n, 256), kCheckBlocks)); }
 
 static void check_part(Ctx &c) {
-    hipLaunchKernelGGL(k_check_part, dim3(check_blocks(c)), dim3(256), 0, c.stream, c.n, c.pos[c.cur], c.posLast,
-                       c.vel[c.cur], c.box, 2.0 * c.dt, c.checkPart);
+    launch<check_part_body, 256>(c, check_blocks(c), 0, c.n, c.pos[c.cur], c.posLast, c.vel[c.cur], c.box, 2.0 * c.dt, c.checkPart);
 }
 
 void decide_step(Ctx &c) {
@@ -1429,16 +1437,12 @@ void decide_step(Ctx &c) {
     const int fused = c.fuseCheck ? fused_check_blocks(c) : 0;
     if (!fused) check_part(c);
     const float half = (float)(0.5 * c.skin);
-    hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, c.stream, fused ? fused : check_blocks(c), c.checkPart,
-                       c.dMax, c.dFlags,
-                       half * half, c.thermoOn ? c.thermoEvery : 0, c.baro.on ? c.baro.every : 0,
-                       c.fuseThermo ? c.thermo : nullptr);
+    launch<decide_body, 64>(c, 1, 0, fused ? fused : check_blocks(c), c.checkPart, c.dMax, c.dFlags, half * half, c.thermoOn ? c.thermoEvery : 0, c.baro.on ? c.baro.every : 0, c.fuseThermo ? c.thermo : nullptr);
 }
 
 void check_max(Ctx &c, int parity) {
     check_part(c);
-    hipLaunchKernelGGL(k_check_reduce, dim3(1), dim3(64), 0, c.stream, check_blocks(c), c.checkPart,
-                       c.dMax + 2 * parity, c.dMax + 2 * (1 - parity));
+    launch<check_reduce_body, 64>(c, 1, 0, check_blocks(c), c.checkPart, c.dMax + 2 * parity, c.dMax + 2 * (1 - parity));
 }
 
 }  // namespace mdx

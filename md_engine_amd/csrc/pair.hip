@@ -48,11 +48,11 @@ struct PairArgs {
 // (sum r F, what a barostat needs), into ACC_VTRACE, skipped on steps whose
 // barostat gate is 0
 template <int P, bool LJ, bool COUL, int EVM>
-__global__ void __launch_bounds__(256) k_pair(PairArgs a) {
+__device__ __forceinline__ void pair_body(int blk, int, PairArgs a) {
     constexpr bool EV = EVM == 1, VT = EVM == 2;
-    // box values into locals (writing them into the argument struct would move
-    // it to scratch memory: a.spec is indexed by the pair tag)
+    // box values into locals
     const float bL0 = a.bx->Lf[0], bL1 = a.bx->Lf[1], bL2 = a.bx->Lf[2];
+    const float sp1 = a.spec[0], sp2 = a.spec[1], sp3 = a.spec[2];
     const float bI0 = a.bx->invLf[0], bI1 = a.bx->invLf[1], bI2 = a.bx->invLf[2];
     const float alphaB = a.bx->alphaF;
     __shared__ float sp[3 * kMaxTypes * kMaxTypes];
@@ -64,7 +64,7 @@ __global__ void __launch_bounds__(256) k_pair(PairArgs a) {
         for (int t = threadIdx.x; t < a.ntypes; t += blockDim.x) sfl[t] = a.typeFlags[t];
         __syncthreads();
     }
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gt = blk * blockDim.x + threadIdx.x;
     const int i = gt / P;
     const int tl = threadIdx.x & (P - 1);
     float fx = 0.f, fy = 0.f, fz = 0.f;
@@ -94,7 +94,9 @@ __global__ void __launch_bounds__(256) k_pair(PairArgs a) {
                 dy -= bL1 * rintf(dy * bI1);
                 dz -= bL2 * rintf(dz * bI2);
                 const float r2 = dx * dx + dy * dy + dz * dz;
-                const float m = tag == 0 ? 1.0f : a.spec[tag - 1];
+                // special-pair multiplier by selection (a dynamic index would
+                // put the argument pack in scratch memory)
+                const float m = tag == 0 ? 1.0f : (tag == 1 ? sp1 : (tag == 2 ? sp2 : sp3));
                 float fs = 0.f;
                 bool any = false;
                 if (lj) {
@@ -187,22 +189,18 @@ __global__ void __launch_bounds__(256) k_pair(PairArgs a) {
 }
 
 template <int P, bool LJ, bool COUL>
-void launch3(const PairArgs &a, int evm, hipStream_t s) {
-    const int threads = 256;
-    const int blocks = cdiv((long long)a.n * P, threads);
-    if (evm == 1)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair<P, LJ, COUL, 1>), dim3(blocks), dim3(threads), 0, s, a);
-    else if (evm == 2)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair<P, LJ, COUL, 2>), dim3(blocks), dim3(threads), 0, s, a);
-    else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair<P, LJ, COUL, 0>), dim3(blocks), dim3(threads), 0, s, a);
+void launch3(Ctx &c, const PairArgs &a, int evm) {
+    const int blocks = cdiv((long long)a.n * P, 256);
+    if (evm == 1) launch<pair_body<P, LJ, COUL, 1>, 256>(c, blocks, 0, a);
+    else if (evm == 2) launch<pair_body<P, LJ, COUL, 2>, 256>(c, blocks, 0, a);
+    else launch<pair_body<P, LJ, COUL, 0>, 256>(c, blocks, 0, a);
 }
 
 template <int P>
-void launch2(const PairArgs &a, bool lj, bool coul, int evm, hipStream_t s) {
-    if (lj && coul) launch3<P, true, true>(a, evm, s);
-    else if (lj) launch3<P, true, false>(a, evm, s);
-    else launch3<P, false, true>(a, evm, s);
+void launch2(Ctx &c, const PairArgs &a, bool lj, bool coul, int evm) {
+    if (lj && coul) launch3<P, true, true>(c, a, evm);
+    else if (lj) launch3<P, true, false>(c, a, evm);
+    else launch3<P, false, true>(c, a, evm);
 }
 
 }  // namespace
@@ -259,10 +257,10 @@ void pair_forces(Ctx &c, bool energy, bool virial) {
     const int evm = (energy || virial) ? 1 : (c.vtrace ? 2 : 0);
     ProfScope ps(c, PK_PAIR);
     switch (c.pairTeam) {
-        case 4: launch2<4>(a, c.ljOn, coul, evm, c.stream); break;
-        case 8: launch2<8>(a, c.ljOn, coul, evm, c.stream); break;
-        case 16: launch2<16>(a, c.ljOn, coul, evm, c.stream); break;
-        default: launch2<32>(a, c.ljOn, coul, evm, c.stream); break;
+        case 4: launch2<4>(c, a, c.ljOn, coul, evm); break;
+        case 8: launch2<8>(c, a, c.ljOn, coul, evm); break;
+        case 16: launch2<16>(c, a, c.ljOn, coul, evm); break;
+        default: launch2<32>(c, a, c.ljOn, coul, evm); break;
     }
 }
 

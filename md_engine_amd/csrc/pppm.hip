@@ -73,11 +73,11 @@ __device__ __forceinline__ int wrapi(int p, int n) {
 }
 
 template <int ORDER>
-__global__ void __launch_bounds__(256) k_spread(int n, const double4 *__restrict__ pos,
+__device__ __forceinline__ void spread_body(int blk, int nblk, int n, const double4 *__restrict__ pos,
                                                 const float *__restrict__ q, const BoxDev *__restrict__ bx,
                                                 float qunit, unsigned long long *__restrict__ mesh) {
     const MeshGeo m = bx->mesh;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blk * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float qi = q[i];
     if (qi == 0.0f) return;
@@ -131,14 +131,14 @@ __device__ __forceinline__ int tile_origin(double lo_pos, double h) {
 }
 
 template <int ORDER>
-__global__ void __launch_bounds__(256) k_spread_tiled(const double4 *__restrict__ pos,
+__device__ __forceinline__ void spread_tiled_body(int blk, int nblk, const double4 *__restrict__ pos,
                                                       const float *__restrict__ q, const BoxDev *__restrict__ bx,
                                                       float qunit, const int *__restrict__ cellStart,
-                                                      const int2 *__restrict__ colXY, TileGeo tg, int nblk,
+                                                      const int2 *__restrict__ colXY, TileGeo tg, int nlog,
                                                       unsigned long long *__restrict__ mesh) {
     extern __shared__ unsigned long long tile[];
-    const int lb = xcd_block(nblk);
-    if (lb >= nblk) return;  // uniform over the workgroup
+    const int lb = xcd_block(blk, nlog);
+    if (lb >= nlog) return;  // uniform over the workgroup
     const MeshGeo m = bx->mesh;
     constexpr int H = (ORDER - 1) / 2;
     const int col = lb / tg.nzc;
@@ -213,8 +213,8 @@ __global__ void __launch_bounds__(256) k_spread_tiled(const double4 *__restrict_
     }
 }
 
-__global__ void k_mesh_to_real(int ng, long long *__restrict__ mesh, float *__restrict__ out) {
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void mesh_to_real_body(int blk, int nblk, int ng, long long *__restrict__ mesh, float *__restrict__ out) {
+    int g = blk * blockDim.x + threadIdx.x;
     if (g >= ng) return;
     out[g] = (float)((double)mesh[g] * kMeshInv);
     mesh[g] = 0;
@@ -256,13 +256,13 @@ __device__ double green_point_any(const double *k, const MeshGeo &m, double f4a2
 
 // Optimal influence function on the half spectrum, FP64 arithmetic, one
 // thread per point with the alias sums written out (any image range).
-__global__ void k_green_direct(const int *__restrict__ gate, const BoxDev *__restrict__ bx, int szh, int order,
+__device__ __forceinline__ void green_direct_body(int blk, int nblk, const int *__restrict__ gate, const BoxDev *__restrict__ bx, int szh, int order,
                                float *__restrict__ G) {
     if (gate && *gate == 0) return;
     const MeshGeo m = bx->mesh;
     const double alpha = bx->alpha;
     const int lim = bx->greenLim;
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    int idx = blk * blockDim.x + threadIdx.x;
     int total = m.sz[0] * m.sz[1] * szh;
     if (idx >= total) return;
     int iz = idx % szh, iy = (idx / szh) % m.sz[1], ix = idx / (szh * m.sz[1]);
@@ -279,14 +279,20 @@ __global__ void k_green_direct(const int *__restrict__ gate, const BoxDev *__res
 // of its indices are loaded first, then the (2 LIM + 1)^3 images are summed
 // in registers in the order of k_green_direct.  FP32, like the reference's
 // Green_function_cu (the result is stored in FP32).
+struct GreenWin {
+    int s[3], n[3];  // per axis: first mesh index of the window and its length
+    int off1, off2;  // table offsets of the y and z windows
+};
+
 template <int LIM>
-__device__ __forceinline__ float green_tab_point(const float *tkp, const float *tex, const float *tw, int off1,
-                                                 int off2, const MeshGeo &m, int ix, int iy, int iz) {
+__device__ __forceinline__ float green_tab_point(const float *tkp, const float *tex, const float *tw,
+                                                 const GreenWin &w, int ix, int iy, int iz) {
     constexpr int NI = 2 * LIM + 1;
     float kx[NI], ey[NI], wy[NI], ky[NI], ex[NI], wx[NI], kz[NI], ez[NI], wz[NI];
 #pragma unroll
     for (int j = 0; j < NI; ++j) {
-        const int a = j * m.sz[0] + ix, b = off1 + j * m.sz[1] + iy, c = off2 + j * m.sz[2] + iz;
+        const int a = j * w.n[0] + ix - w.s[0], b = w.off1 + j * w.n[1] + iy - w.s[1],
+                  c = w.off2 + j * w.n[2] + iz - w.s[2];
         kx[j] = tkp[a];
         ex[j] = tex[a];
         wx[j] = tw[a];
@@ -332,25 +338,27 @@ __device__ __forceinline__ float sinc_f(float x) {
 // exp(-|k'|^2 / 4 alpha^2) and the assignment factor W(k')^2p factor over the
 // axes, so for every axis d, mesh index i and image j the workgroup first
 // computes k'_d = k_d + 2 pi j / h_d, exp(-k'_d^2 / 4 alpha^2) and
-// sinc(k'_d h_d / 2)^2p once into LDS (3 (2 lim + 1) (sz_x + sz_y + sz_z)
-// entries); each point then sums its (2 lim + 1)^3 images from products of
-// table entries.  lim > kGreenTabLim takes the direct FP64 sums.
+// sinc(k'_d h_d / 2)^2p once into LDS for the index windows its points use
+// (one or two x planes, the y rows they touch, the z half-spectrum columns:
+// at most 3 (2 lim + 1) (sz_x + sz_y + sz_z) entries); each point then sums
+// its (2 lim + 1)^3 images from products of table entries.  lim >
+// kGreenTabLim takes the direct FP64 sums.
 constexpr int kGreenTabLim = 2;
 // Workgroups from gblocks on dilate the system instead (dilate_thread): the
 // barostat's two box-change passes in one launch; both only read the box.
-__global__ void __launch_bounds__(256) k_green(const int *__restrict__ gate, const BoxDev *__restrict__ bx,
+__device__ __forceinline__ void green_body(int blk, int nblk, const int *__restrict__ gate, const BoxDev *__restrict__ bx,
                                                int szh, int order, float *__restrict__ G, int gblocks,
                                                DilateArgs dl) {
     extern __shared__ float gtab[];
     if (gate && *gate == 0) return;
-    if ((int)blockIdx.x >= gblocks) {  // uniform over the workgroup
-        dilate_thread(((int)blockIdx.x - gblocks) * blockDim.x + threadIdx.x, dl, bx);
+    if ((int)blk >= gblocks) {  // uniform over the workgroup
+        dilate_thread(((int)blk - gblocks) * blockDim.x + threadIdx.x, dl, bx);
         return;
     }
     const MeshGeo m = bx->mesh;
     const double alpha = bx->alpha;
     const int lim = bx->greenLim;
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int idx = blk * blockDim.x + threadIdx.x;
     const int total = m.sz[0] * m.sz[1] * szh;
     const double f4a2 = 0.25 / alpha / alpha;
     if (lim > kGreenTabLim) {  // uniform over the grid
@@ -362,16 +370,37 @@ __global__ void __launch_bounds__(256) k_green(const int *__restrict__ gate, con
         return;
     }
     const int NI = 2 * lim + 1;
-    const int off1 = NI * m.sz[0], off2 = NI * (m.sz[0] + m.sz[1]);
-    const int ne = NI * (m.sz[0] + m.sz[1] + m.sz[2]);
+    // index windows of this workgroup's points (iz fastest, then iy, then ix)
+    GreenWin w;
+    {
+        const int first = blk * blockDim.x, last = min(first + (int)blockDim.x, total) - 1;
+        const int fz = first % szh, fy = (first / szh) % m.sz[1], fx = first / (szh * m.sz[1]);
+        const int lz = last % szh, ly = (last / szh) % m.sz[1], lx = last / (szh * m.sz[1]);
+        w.s[0] = fx;
+        w.n[0] = lx - fx + 1;
+        const bool onePlane = lx == fx;
+        w.s[1] = onePlane ? fy : 0;
+        w.n[1] = onePlane ? ly - fy + 1 : m.sz[1];
+        const bool oneRow = onePlane && ly == fy;
+        w.s[2] = oneRow ? fz : 0;
+        w.n[2] = oneRow ? lz - fz + 1 : szh;
+        w.off1 = NI * w.n[0];
+        w.off2 = NI * (w.n[0] + w.n[1]);
+    }
+    const int ne = NI * (m.sz[0] + m.sz[1] + m.sz[2]);  // the table's allocation (its upper bound)
+    const int nu = NI * (w.n[0] + w.n[1] + w.n[2]);      // entries this workgroup fills
     float *tkp = gtab, *tex = gtab + ne, *tw = gtab + 2 * ne;
     const float f4a2f = (float)f4a2;
-    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
-        const bool z = e >= off2, y = !z && e >= off1;
+    for (int e = threadIdx.x; e < nu; e += blockDim.x) {
+        // per-axis values by selection, not by a dynamic index (which would
+        // put the local structs in scratch memory)
+        const bool z = e >= w.off2, y = !z && e >= w.off1;
         const int n = z ? m.sz[2] : (y ? m.sz[1] : m.sz[0]);
+        const int nw = z ? w.n[2] : (y ? w.n[1] : w.n[0]);
+        const int ws = z ? w.s[2] : (y ? w.s[1] : w.s[0]);
         const double Ld = z ? m.L[2] : (y ? m.L[1] : m.L[0]);
         const double hd = z ? m.h[2] : (y ? m.h[1] : m.h[0]);
-        const int r = e - (z ? off2 : (y ? off1 : 0)), j = r / n, i = r - j * n;
+        const int r = e - (z ? w.off2 : (y ? w.off1 : 0)), j = r / nw, i = ws + (r - j * nw);
         const float q = (float)(kcomp(i, n, Ld) + 2.0 * kPi * (j - lim) / hd);
         const float W = sinc_f(q * (float)hd * 0.5f);
         float W2 = 1.0f;
@@ -385,18 +414,18 @@ __global__ void __launch_bounds__(256) k_green(const int *__restrict__ gate, con
     const int iz = idx % szh, iy = (idx / szh) % m.sz[1], ix = idx / (szh * m.sz[1]);
     float g;
     switch (lim) {
-        case 0: g = green_tab_point<0>(tkp, tex, tw, off1, off2, m, ix, iy, iz); break;
-        case 1: g = green_tab_point<1>(tkp, tex, tw, off1, off2, m, ix, iy, iz); break;
-        default: g = green_tab_point<2>(tkp, tex, tw, off1, off2, m, ix, iy, iz); break;
+        case 0: g = green_tab_point<0>(tkp, tex, tw, w, ix, iy, iz); break;
+        case 1: g = green_tab_point<1>(tkp, tex, tw, w, ix, iy, iz); break;
+        default: g = green_tab_point<2>(tkp, tex, tw, w, ix, iy, iz); break;
     }
     G[idx] = g;
 }
 
 // E_hat_d = i k_d G rho_hat (E_field_cu), k_d := 0 on the d-th Nyquist plane.
-__global__ void k_efield(const BoxDev *__restrict__ bx, int szh, const float *__restrict__ G,
+__device__ __forceinline__ void efield_body(int blk, int nblk, const BoxDev *__restrict__ bx, int szh, const float *__restrict__ G,
                          const float2 *__restrict__ rho, float2 *__restrict__ E) {
     const MeshGeo m = bx->mesh;
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    int idx = blk * blockDim.x + threadIdx.x;
     int total = m.sz[0] * m.sz[1] * szh;
     if (idx >= total) return;
     int iz = idx % szh, iy = (idx / szh) % m.sz[1], ix = idx / (szh * m.sz[1]);
@@ -415,7 +444,7 @@ __global__ void k_efield(const BoxDev *__restrict__ bx, int szh, const float *__
 // virial (virials_cu): half-spectrum entries with 0 < iz < sz/2 count twice.
 // gate (virial for a barostat inside a captured step): nothing on steps
 // whose barostat gate is 0.
-__global__ void __launch_bounds__(256) k_kspace_ev(const int *__restrict__ gate, const BoxDev *__restrict__ bx,
+__device__ __forceinline__ void kspace_ev_body(int blk, int nblk, const int *__restrict__ gate, const BoxDev *__restrict__ bx,
                                                    int szh, const float *__restrict__ G,
                                                    const float2 *__restrict__ rho,
                                                    unsigned long long *__restrict__ acc) {
@@ -423,7 +452,7 @@ __global__ void __launch_bounds__(256) k_kspace_ev(const int *__restrict__ gate,
     const MeshGeo m = bx->mesh;
     const double alpha = bx->alpha, invV = 1.0 / bx->vol;
     __shared__ double wred[4][7];
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    int idx = blk * blockDim.x + threadIdx.x;
     int total = m.sz[0] * m.sz[1] * szh;
     double vals[7] = {0, 0, 0, 0, 0, 0, 0};
     if (idx < total) {
@@ -475,20 +504,20 @@ __device__ __forceinline__ float pick(const float (&w)[ORDER], int k) {
 // per component.
 // the three real field components interleaved per mesh point, so the gather
 // takes one 16-byte load per stencil point
-__global__ void k_field_pack(int ng, const float *__restrict__ E, float4 *__restrict__ E4) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void field_pack_body(int blk, int nblk, int ng, const float *__restrict__ E, float4 *__restrict__ E4) {
+    const int g = blk * blockDim.x + threadIdx.x;
     if (g >= ng) return;
     E4[g] = make_float4(E[g], E[ng + g], E[2 * (size_t)ng + g], 0.0f);
 }
 
 template <int ORDER, int T>
-__global__ void __launch_bounds__(256) k_gather(int n, const double4 *__restrict__ pos,
+__device__ __forceinline__ void gather_body(int blk, int nblk, int n, const double4 *__restrict__ pos,
                                                 const float *__restrict__ q, const BoxDev *__restrict__ bx,
                                                 const float4 *__restrict__ E4,
                                                 unsigned long long *__restrict__ force,
                                                 const int *__restrict__ ids, long long *__restrict__ stored) {
     const int nlog = (int)(((long long)n * T + blockDim.x - 1) / blockDim.x);
-    const int lb = xcd_block(nlog);
+    const int lb = xcd_block(blk, nlog);
     if (lb >= nlog) return;  // uniform over the workgroup
     const MeshGeo m = bx->mesh;
     const float scale = bx->gatherScale;
@@ -548,9 +577,9 @@ __global__ void __launch_bounds__(256) k_gather(int n, const double4 *__restrict
 
 // the stored mesh forces by id added to the current order's force rows
 // (applyStoredForces, FixChargeEwald.cu:804-814)
-__global__ void k_apply_stored(int n, const int *__restrict__ ids, const long long *__restrict__ stored,
+__device__ __forceinline__ void apply_stored_body(int blk, int nblk, int n, const int *__restrict__ ids, const long long *__restrict__ stored,
                                unsigned long long *__restrict__ force) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blk * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int id = ids[i];
     const long long f0 = stored[3 * id], f1 = stored[3 * id + 1], f2 = stored[3 * id + 2];
@@ -596,8 +625,8 @@ bool tiled_geometry(const Ctx &c, TileGeo &tg) {
 }
 
 // clears n 64-bit words (the charge mesh of the scratch)
-__global__ void k_scr_zero(long long n, unsigned long long *__restrict__ p) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void scr_zero_body(int blk, int nblk, long long n, unsigned long long *__restrict__ p) {
+    const long long i = (long long)blk * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;
 }
 
@@ -660,8 +689,7 @@ PppmScratchScope::PppmScratchScope(Ctx &c_) : c(c_) {
     P.eField4 = (float4 *)(base + kScrField);
     // the spread adds into the charge mesh, which the last call on this GPU
     // left in any state: clear it with a kernel (no memset node)
-    hipLaunchKernelGGL(k_scr_zero, dim3(cdiv(ng, 256)), dim3(256), 0, c.stream, ng,
-                       (unsigned long long *)P.mesh);
+    launch<scr_zero_body, 256>(c, cdiv(ng, 256), 0, ng, (unsigned long long *)P.mesh);
 }
 
 PppmScratchScope::~PppmScratchScope() {
@@ -727,11 +755,11 @@ void pppm_prepare(Ctx &c) {
     }
     c.alpha = (float)P.alpha;
     box_upload(c);  // the Green function reads alpha and the mesh from the device box
-    pppm_green_device(c, nullptr, c.stream);
+    pppm_green_device(c, nullptr);
     for (int a = 0; a < 3; ++a) P.boxAtGreen[a] = c.geo.L[a];
 }
 
-bool pppm_green_device(Ctx &c, const int *gate, hipStream_t s, const DilateArgs *dl) {
+bool pppm_green_device(Ctx &c, const int *gate, const DilateArgs *dl) {
     const PppmState &P = c.pppm;
     const int szh = P.sz[2] / 2 + 1;
     const size_t nh = (size_t)P.sz[0] * P.sz[1] * szh;
@@ -742,12 +770,20 @@ bool pppm_green_device(Ctx &c, const int *gate, hipStream_t s, const DilateArgs 
     if (tab <= 64 * 1024) {
         DilateArgs none{};
         const int db = dl ? cdiv(std::max(dl->nmol, dl->n), 256) : 0;
-        hipLaunchKernelGGL(k_green, dim3(gb + db), dim3(256), tab, s, gate, c.box, szh, P.order, P.green, gb,
-                           dl ? *dl : none);
+        launch<green_body, 256>(c, gb + db, tab, gate, c.box, szh, P.order, P.green, gb, dl ? *dl : none);
         return dl != nullptr;
     }
-    hipLaunchKernelGGL(k_green_direct, dim3(gb), dim3(256), 0, s, gate, c.box, szh, P.order, P.green);
+    launch<green_direct_body, 256>(c, gb, 0, gate, c.box, szh, P.order, P.green);
     return false;
+}
+
+// The transforms run on the stream their context's work is on (a ring
+// polymer's beads share one, a batch records its replicas onto one).
+static void pppm_plans_on(PppmState &P, hipStream_t s) {
+    if (P.planStream == s) return;
+    MDX_REQUIRE(hipfftSetStream(P.planR2C, s) == HIPFFT_SUCCESS && hipfftSetStream(P.planC2R, s) == HIPFFT_SUCCESS, 2,
+                "hipfftSetStream failed");
+    P.planStream = s;
 }
 
 void pppm_forces(Ctx &c, bool energy, bool virial) {
@@ -767,58 +803,54 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
         const int nblk = c.nc[0] * c.nc[1] * tg.nzc;
         const size_t shm = sizeof(unsigned long long) * tg.ext[0] * tg.ext[1] * tg.ext[2];
         switch (P.order) {
-            case 1: hipLaunchKernelGGL(k_spread_tiled<1>, dim3(xcd_grid(nblk)), dim3(threads), shm, c.stream, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh); break;
-            case 3: hipLaunchKernelGGL(k_spread_tiled<3>, dim3(xcd_grid(nblk)), dim3(threads), shm, c.stream, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh); break;
-            case 5: hipLaunchKernelGGL(k_spread_tiled<5>, dim3(xcd_grid(nblk)), dim3(threads), shm, c.stream, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh); break;
-            default: hipLaunchKernelGGL(k_spread_tiled<7>, dim3(xcd_grid(nblk)), dim3(threads), shm, c.stream, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh); break;
+            case 1: launch<spread_tiled_body<1>, 256>(c, xcd_grid(nblk), shm, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh); break;
+            case 3: launch<spread_tiled_body<3>, 256>(c, xcd_grid(nblk), shm, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh); break;
+            case 5: launch<spread_tiled_body<5>, 256>(c, xcd_grid(nblk), shm, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh); break;
+            default: launch<spread_tiled_body<7>, 256>(c, xcd_grid(nblk), shm, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh); break;
         }
     } else
     switch (P.order) {
-        case 1: hipLaunchKernelGGL(k_spread<1>, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.q[cur], c.box, qunit, (unsigned long long *)P.mesh); break;
-        case 3: hipLaunchKernelGGL(k_spread<3>, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.q[cur], c.box, qunit, (unsigned long long *)P.mesh); break;
-        case 5: hipLaunchKernelGGL(k_spread<5>, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.q[cur], c.box, qunit, (unsigned long long *)P.mesh); break;
-        default: hipLaunchKernelGGL(k_spread<7>, dim3(cdiv(c.n, threads)), dim3(threads), 0, c.stream, c.n, c.pos[cur], c.q[cur], c.box, qunit, (unsigned long long *)P.mesh); break;
+        case 1: launch<spread_body<1>, 256>(c, cdiv(c.n, threads), 0, c.n, c.pos[cur], c.q[cur], c.box, qunit, (unsigned long long *)P.mesh); break;
+        case 3: launch<spread_body<3>, 256>(c, cdiv(c.n, threads), 0, c.n, c.pos[cur], c.q[cur], c.box, qunit, (unsigned long long *)P.mesh); break;
+        case 5: launch<spread_body<5>, 256>(c, cdiv(c.n, threads), 0, c.n, c.pos[cur], c.q[cur], c.box, qunit, (unsigned long long *)P.mesh); break;
+        default: launch<spread_body<7>, 256>(c, cdiv(c.n, threads), 0, c.n, c.pos[cur], c.q[cur], c.box, qunit, (unsigned long long *)P.mesh); break;
     }
-    hipLaunchKernelGGL(k_mesh_to_real, dim3(cdiv(ng, threads)), dim3(threads), 0, c.stream, ng, P.mesh, P.real);
-    }
-    if (P.planStream != c.stream) {
-        // the context's work moved to another stream (a ring polymer's beads
-        // share one): the transforms follow it
-        MDX_REQUIRE(hipfftSetStream(P.planR2C, c.stream) == HIPFFT_SUCCESS &&
-                        hipfftSetStream(P.planC2R, c.stream) == HIPFFT_SUCCESS,
-                    2, "hipfftSetStream failed");
-        P.planStream = c.stream;
+    launch<mesh_to_real_body, 256>(c, cdiv(ng, threads), 0, ng, P.mesh, P.real);
     }
     {
     ProfScope ps(c, PK_FFT);
-    MDX_REQUIRE(hipfftExecR2C(P.planR2C, P.real, (hipfftComplex *)P.rhoHat) == HIPFFT_SUCCESS, 2,
-                "hipfftExecR2C failed");
+    stream_op(c, [&P](hipStream_t s) {
+        pppm_plans_on(P, s);
+        MDX_REQUIRE(hipfftExecR2C(P.planR2C, P.real, (hipfftComplex *)P.rhoHat) == HIPFFT_SUCCESS, 2,
+                    "hipfftExecR2C failed");
+    }, 1);
     }
     if (energy || virial) {
         // a trace-only request (a barostat's) is skipped on steps it does not serve
         const int *gate = c.vtrace ? baro_gate(c) : nullptr;
-        hipLaunchKernelGGL(k_kspace_ev, dim3(cdiv(nh, threads)), dim3(threads), 0, c.stream, gate, c.box, szh,
-                           P.green, P.rhoHat, c.acc);
+        launch<kspace_ev_body, 256>(c, cdiv(nh, threads), 0, gate, c.box, szh, P.green, P.rhoHat, c.acc);
     }
     {
         ProfScope ps(c, PK_EFIELD);
-        hipLaunchKernelGGL(k_efield, dim3(cdiv(nh, threads)), dim3(threads), 0, c.stream, c.box, szh, P.green,
-                           P.rhoHat, P.eHat);
+        launch<efield_body, 256>(c, cdiv(nh, threads), 0, c.box, szh, P.green, P.rhoHat, P.eHat);
     }
     {
         ProfScope ps(c, PK_FFT);
-        MDX_REQUIRE(hipfftExecC2R(P.planC2R, (hipfftComplex *)P.eHat, P.eReal) == HIPFFT_SUCCESS, 2,
-                    "hipfftExecC2R failed");
+        stream_op(c, [&P](hipStream_t s) {
+            pppm_plans_on(P, s);
+            MDX_REQUIRE(hipfftExecC2R(P.planC2R, (hipfftComplex *)P.eHat, P.eReal) == HIPFFT_SUCCESS, 2,
+                        "hipfftExecC2R failed");
+        }, 2);
     }
     // a built list puts the charged atoms first: only they take the gather
     const int na = c.listValid ? c.nCharged : c.n;
     ProfScope ps(c, PK_GATHER);
     if (na == 0) return;
-    hipLaunchKernelGGL(k_field_pack, dim3(cdiv(ng, threads)), dim3(threads), 0, c.stream, ng, P.eReal, P.eField4);
-    const dim3 grid(xcd_grid(cdiv((long long)na * P.gatherTeam, threads)));
-#define MDX_GATHER(O, T)                                                                                   \
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gather<O, T>), grid, dim3(threads), 0, c.stream, na, c.pos[cur], \
-                       c.q[cur], c.box, P.eField4, c.force, c.id[cur], P.interval > 1 ? P.stored : nullptr)
+    launch<field_pack_body, 256>(c, cdiv(ng, threads), 0, ng, P.eReal, P.eField4);
+    const int gridx = xcd_grid(cdiv((long long)na * P.gatherTeam, threads));
+    long long *stored = P.interval > 1 ? P.stored : nullptr;
+#define MDX_GATHER(O, T) \
+    launch<gather_body<O, T>, 256>(c, gridx, 0, na, c.pos[cur], c.q[cur], c.box, P.eField4, c.force, c.id[cur], stored)
     const bool wide = P.gatherTeam == 16;
     switch (P.order) {
         case 1: if (wide) MDX_GATHER(1, 16); else MDX_GATHER(1, 8); break;
@@ -842,49 +874,48 @@ void ring_pppm(Ctx &c0, const RingSpec &r) {
     const int nh = sz[0] * sz[1] * szh;
     const float qunit = (float)std::sqrt(P.qqr);
     const int threads = 256;
-    hipStream_t s = c0.stream;
-    ring_centroids(r, c0.box, s);
+    ring_centroids(r, c0);
     {
         ProfScope ps(c0, PK_SPREAD);
         switch (P.order) {
-            case 1: hipLaunchKernelGGL(k_spread<1>, dim3(cdiv(r.natoms, threads)), dim3(threads), 0, s, r.natoms, r.cen, r.q, c0.box, qunit, (unsigned long long *)P.mesh); break;
-            case 3: hipLaunchKernelGGL(k_spread<3>, dim3(cdiv(r.natoms, threads)), dim3(threads), 0, s, r.natoms, r.cen, r.q, c0.box, qunit, (unsigned long long *)P.mesh); break;
-            case 5: hipLaunchKernelGGL(k_spread<5>, dim3(cdiv(r.natoms, threads)), dim3(threads), 0, s, r.natoms, r.cen, r.q, c0.box, qunit, (unsigned long long *)P.mesh); break;
-            default: hipLaunchKernelGGL(k_spread<7>, dim3(cdiv(r.natoms, threads)), dim3(threads), 0, s, r.natoms, r.cen, r.q, c0.box, qunit, (unsigned long long *)P.mesh); break;
+            case 1: launch<spread_body<1>, 256>(c0, cdiv(r.natoms, threads), 0, r.natoms, r.cen, r.q, c0.box, qunit, (unsigned long long *)P.mesh); break;
+            case 3: launch<spread_body<3>, 256>(c0, cdiv(r.natoms, threads), 0, r.natoms, r.cen, r.q, c0.box, qunit, (unsigned long long *)P.mesh); break;
+            case 5: launch<spread_body<5>, 256>(c0, cdiv(r.natoms, threads), 0, r.natoms, r.cen, r.q, c0.box, qunit, (unsigned long long *)P.mesh); break;
+            default: launch<spread_body<7>, 256>(c0, cdiv(r.natoms, threads), 0, r.natoms, r.cen, r.q, c0.box, qunit, (unsigned long long *)P.mesh); break;
         }
-        hipLaunchKernelGGL(k_mesh_to_real, dim3(cdiv(ng, threads)), dim3(threads), 0, s, ng, P.mesh, P.real);
-    }
-    if (P.planStream != s) {
-        MDX_REQUIRE(hipfftSetStream(P.planR2C, s) == HIPFFT_SUCCESS && hipfftSetStream(P.planC2R, s) == HIPFFT_SUCCESS,
-                    2, "hipfftSetStream failed");
-        P.planStream = s;
+        launch<mesh_to_real_body, 256>(c0, cdiv(ng, threads), 0, ng, P.mesh, P.real);
     }
     {
         ProfScope ps(c0, PK_FFT);
-        MDX_REQUIRE(hipfftExecR2C(P.planR2C, P.real, (hipfftComplex *)P.rhoHat) == HIPFFT_SUCCESS, 2,
-                    "hipfftExecR2C failed");
+        stream_op(c0, [&P](hipStream_t s) {
+            pppm_plans_on(P, s);
+            MDX_REQUIRE(hipfftExecR2C(P.planR2C, P.real, (hipfftComplex *)P.rhoHat) == HIPFFT_SUCCESS, 2,
+                        "hipfftExecR2C failed");
+        });
     }
     {
         ProfScope ps(c0, PK_EFIELD);
-        hipLaunchKernelGGL(k_efield, dim3(cdiv(nh, threads)), dim3(threads), 0, s, c0.box, szh, P.green, P.rhoHat,
-                           P.eHat);
+        launch<efield_body, 256>(c0, cdiv(nh, threads), 0, c0.box, szh, P.green, P.rhoHat, P.eHat);
     }
     {
         ProfScope ps(c0, PK_FFT);
-        MDX_REQUIRE(hipfftExecC2R(P.planC2R, (hipfftComplex *)P.eHat, P.eReal) == HIPFFT_SUCCESS, 2,
-                    "hipfftExecC2R failed");
+        stream_op(c0, [&P](hipStream_t s) {
+            pppm_plans_on(P, s);
+            MDX_REQUIRE(hipfftExecC2R(P.planC2R, (hipfftComplex *)P.eHat, P.eReal) == HIPFFT_SUCCESS, 2,
+                        "hipfftExecC2R failed");
+        });
     }
     ProfScope ps(c0, PK_GATHER);
-    hipLaunchKernelGGL(k_field_pack, dim3(cdiv(ng, threads)), dim3(threads), 0, s, ng, P.eReal, P.eField4);
+    launch<field_pack_body, 256>(c0, cdiv(ng, threads), 0, ng, P.eReal, P.eField4);
     const dim3 grid(xcd_grid(cdiv((long long)r.natoms * 8, threads)));
     unsigned long long *cf = (unsigned long long *)r.cf;
     switch (P.order) {
-        case 1: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gather<1, 8>), grid, dim3(threads), 0, s, r.natoms, r.cen, r.q, c0.box, P.eField4, cf, nullptr, nullptr); break;
-        case 3: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gather<3, 8>), grid, dim3(threads), 0, s, r.natoms, r.cen, r.q, c0.box, P.eField4, cf, nullptr, nullptr); break;
-        case 5: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gather<5, 8>), grid, dim3(threads), 0, s, r.natoms, r.cen, r.q, c0.box, P.eField4, cf, nullptr, nullptr); break;
-        default: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gather<7, 8>), grid, dim3(threads), 0, s, r.natoms, r.cen, r.q, c0.box, P.eField4, cf, nullptr, nullptr); break;
+        case 1: launch<gather_body<1, 8>, 256>(c0, grid.x, 0, r.natoms, r.cen, r.q, c0.box, P.eField4, cf, nullptr, nullptr); break;
+        case 3: launch<gather_body<3, 8>, 256>(c0, grid.x, 0, r.natoms, r.cen, r.q, c0.box, P.eField4, cf, nullptr, nullptr); break;
+        case 5: launch<gather_body<5, 8>, 256>(c0, grid.x, 0, r.natoms, r.cen, r.q, c0.box, P.eField4, cf, nullptr, nullptr); break;
+        default: launch<gather_body<7, 8>, 256>(c0, grid.x, 0, r.natoms, r.cen, r.q, c0.box, P.eField4, cf, nullptr, nullptr); break;
     }
-    ring_apply(r, s);
+    ring_apply(r, c0);
 }
 
 void pppm_step(Ctx &c, bool energy, bool virial) {
@@ -897,8 +928,7 @@ void pppm_step(Ctx &c, bool energy, bool virial) {
         }
         const bool due = (c.turn - P.turnInit) % P.interval == 0;
         if (!due && P.storedValid && !energy && !virial && !c.vtrace) {
-            hipLaunchKernelGGL(k_apply_stored, dim3(cdiv(c.n, 256)), dim3(256), 0, c.stream, c.n, c.id[c.cur],
-                               P.stored, c.force);
+            launch<apply_stored_body, 256>(c, cdiv(c.n, 256), 0, c.n, c.id[c.cur], P.stored, c.force);
             return;
         }
         pppm_forces(c, energy, virial);

@@ -156,12 +156,12 @@ __device__ __forceinline__ double disp2(const double4 &a, const double4 &b, cons
 // of the sites' displacement since the last build and of the displacement
 // plus 2 dt |v|, from the positions and velocities just written; valid when
 // every site belongs to a rigid molecule.
-__global__ void __launch_bounds__(128) k_rigid_vv1(RigidArgs r, double4 *__restrict__ pos, double4 *__restrict__ vel,
+__device__ __forceinline__ void rigid_vv1_body(int blk, int nblk, RigidArgs r, double4 *__restrict__ pos, double4 *__restrict__ vel,
                                                    const unsigned long long *__restrict__ f,
                                                    const double4 *__restrict__ last, double dt2,
                                                    float *__restrict__ part) {
     r.g = r.bx->geo;
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blk * blockDim.x + threadIdx.x;
     float d2 = 0.f, p2 = 0.f;
     if (m < r.nmol) {
     const int4 a = r.atoms[m];
@@ -231,8 +231,8 @@ __global__ void __launch_bounds__(128) k_rigid_vv1(RigidArgs r, double4 *__restr
         }
         __syncthreads();
         if (threadIdx.x == 0) {
-            part[2 * blockIdx.x] = fmaxf(red[0][0], red[0][1]);
-            part[2 * blockIdx.x + 1] = fmaxf(red[1][0], red[1][1]);
+            part[2 * blk] = fmaxf(red[0][0], red[0][1]);
+            part[2 * blk + 1] = fmaxf(red[1][0], red[1][1]);
         }
     }
 }
@@ -244,14 +244,14 @@ __global__ void __launch_bounds__(128) k_rigid_vv1(RigidArgs r, double4 *__restr
 // its own, and (doTh) the sum of m v^2 over the sites (as k_mv2) into
 // thermo[0] for the velocity rescale.
 template <bool SUMS>
-__global__ void __launch_bounds__(128) k_rigid_vv2(RigidArgs r, double4 *__restrict__ pos, double4 *__restrict__ vel,
+__device__ __forceinline__ void rigid_vv2_body(int blk, int nblk, RigidArgs r, double4 *__restrict__ pos, double4 *__restrict__ vel,
                                                    const unsigned long long *__restrict__ f, int kick,
                                                    unsigned long long *__restrict__ acc,
                                                    const int *__restrict__ vgate, int doVir,
                                                    unsigned long long *__restrict__ thermo,
                                                    const int *__restrict__ tgate, int doTh) {
     r.g = r.bx->geo;
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blk * blockDim.x + threadIdx.x;
     // uniform over the grid
     const bool vir = SUMS && doVir && (!vgate || *vgate);
     const bool th = SUMS && doTh && (!tgate || *tgate);
@@ -320,10 +320,10 @@ __global__ void __launch_bounds__(128) k_rigid_vv2(RigidArgs r, double4 *__restr
 
 // molecule -> current site indices, and the per-site membership flag that
 // keeps the plain integrator off these sites
-__global__ void k_rigid_map(const int *__restrict__ gate, int nmol, const int4 *__restrict__ ids, const int *__restrict__ idToIdx,
+__device__ __forceinline__ void rigid_map_body(int blk, int nblk, const int *__restrict__ gate, int nmol, const int4 *__restrict__ ids, const int *__restrict__ idToIdx,
                             int4 *__restrict__ atoms, int *__restrict__ member) {
     if (gate && *gate == 0) return;
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blk * blockDim.x + threadIdx.x;
     if (m >= nmol) return;
     const int4 i = ids[m];
     const int4 a = make_int4(idToIdx[i.x], idToIdx[i.y], idToIdx[i.z], i.w >= 0 ? idToIdx[i.w] : -1);
@@ -360,7 +360,7 @@ RigidArgs make_args(const Ctx &c) {
 // molecules (i < n, member[i] == 0): the molecule's correction and
 // centre-of-mass M|V|^2, or the site's m v^2, summed per workgroup into
 // acc[slot] (corr) and acc[slot + 1] (M|V|^2 + m v^2).
-__global__ void __launch_bounds__(256) k_mol_virial(RigidArgs r, int n, const int *__restrict__ member,
+__device__ __forceinline__ void mol_virial_body(int blk, int nblk, RigidArgs r, int n, const int *__restrict__ member,
                                                     const double4 *__restrict__ pos,
                                                     const double4 *__restrict__ vel,
                                                     const unsigned long long *__restrict__ f,
@@ -369,7 +369,7 @@ __global__ void __launch_bounds__(256) k_mol_virial(RigidArgs r, int n, const in
     if (gate && *gate == 0) return;
     __shared__ double red[2][4];
     r.g = r.bx->geo;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blk * blockDim.x + threadIdx.x;
     double corr = 0.0, mv2 = 0.0;
     if (t < r.nmol) {
         const int4 a = r.atoms[t];
@@ -402,10 +402,10 @@ __global__ void __launch_bounds__(256) k_mol_virial(RigidArgs r, int n, const in
     }
 }
 
-__global__ void __launch_bounds__(256) k_dilate(DilateArgs d, const BoxDev *__restrict__ bx,
+__device__ __forceinline__ void dilate_body(int blk, int nblk, DilateArgs d, const BoxDev *__restrict__ bx,
                                                 const int *__restrict__ gate) {
     if (gate && *gate == 0) return;
-    dilate_thread(blockIdx.x * blockDim.x + threadIdx.x, d, bx);
+    dilate_thread(blk * blockDim.x + threadIdx.x, d, bx);
 }
 
 }  // namespace
@@ -419,8 +419,7 @@ void molecular_virial(Ctx &c, int slot, const int *gate) {
         r.nmol = 0;
         r.atoms = nullptr;
     }
-    hipLaunchKernelGGL(k_mol_virial, dim3(cdiv(std::max(r.nmol, c.n), 256)), dim3(256), 0, c.stream, r, c.n, member,
-                       c.pos[c.cur], c.vel[c.cur], c.force, c.acc, slot, gate);
+    launch<mol_virial_body, 256>(c, cdiv(std::max(r.nmol, c.n), 256), 0, r, c.n, member, c.pos[c.cur], c.vel[c.cur], c.force, c.acc, slot, gate);
 }
 
 DilateArgs dilate_args(const Ctx &c) {
@@ -437,7 +436,7 @@ DilateArgs dilate_args(const Ctx &c) {
 
 void dilate(Ctx &c, const int *gate) {
     const DilateArgs d = dilate_args(c);
-    hipLaunchKernelGGL(k_dilate, dim3(cdiv(std::max(d.nmol, d.n), 256)), dim3(256), 0, c.stream, d, c.box, gate);
+    launch<dilate_body, 256>(c, cdiv(std::max(d.nmol, d.n), 256), 0, d, c.box, gate);
 }
 
 void rigid_map(Ctx &c) {
@@ -446,8 +445,7 @@ void rigid_map(Ctx &c) {
     // map the molecules in the E3B map kernel (fewer gated launches per step)
     if (c.capture && c.e3b.on) return;
     if (!c.capture) fill_int(c, c.rigid.member, c.n, 0, nullptr);
-    hipLaunchKernelGGL(k_rigid_map, dim3(cdiv(c.rigid.nmol, 128)), dim3(128), 0, c.stream, rebuild_gate(c), c.rigid.nmol,
-                       c.rigid.ids, c.idToIdx, c.rigid.atoms, c.rigid.member);
+    launch<rigid_map_body, 128>(c, cdiv(c.rigid.nmol, 128), 0, rebuild_gate(c), c.rigid.nmol, c.rigid.ids, c.idToIdx, c.rigid.atoms, c.rigid.member);
 }
 
 int fused_check_blocks(const Ctx &c) {
@@ -459,19 +457,15 @@ void rigid_first_half(Ctx &c) {
     RigidArgs r = make_args(c);
     // captured steps of all-rigid systems: the skin check's partial maxima here
     const bool chk = c.fuseCheck && fused_check_blocks(c) > 0;
-    hipLaunchKernelGGL(k_rigid_vv1, dim3(cdiv(r.nmol, 128)), dim3(128), 0, c.stream, r, c.pos[c.cur],
-                       c.vel[c.cur], c.force, c.posLast, 2.0 * c.dt, chk ? c.checkPart : nullptr);
+    launch<rigid_vv1_body, 128>(c, cdiv(r.nmol, 128), 0, r, c.pos[c.cur], c.vel[c.cur], c.force, c.posLast, 2.0 * c.dt, chk ? c.checkPart : nullptr);
 }
 
 void rigid_second_half(Ctx &c, bool kick) {
     RigidArgs r = make_args(c);
     if (kick && (c.fuseMolVirial || c.fuseThermo))
-        hipLaunchKernelGGL(k_rigid_vv2<true>, dim3(cdiv(r.nmol, 128)), dim3(128), 0, c.stream, r, c.pos[c.cur],
-                           c.vel[c.cur], c.force, 1, c.acc, baro_gate(c), c.fuseMolVirial ? 1 : 0, c.thermo,
-                           thermo_gate(c), c.fuseThermo ? 1 : 0);
+        launch<rigid_vv2_body<true>, 128>(c, cdiv(r.nmol, 128), 0, r, c.pos[c.cur], c.vel[c.cur], c.force, 1, c.acc, baro_gate(c), c.fuseMolVirial ? 1 : 0, c.thermo, thermo_gate(c), c.fuseThermo ? 1 : 0);
     else
-        hipLaunchKernelGGL(k_rigid_vv2<false>, dim3(cdiv(r.nmol, 128)), dim3(128), 0, c.stream, r, c.pos[c.cur],
-                           c.vel[c.cur], c.force, kick ? 1 : 0, c.acc, nullptr, 0, nullptr, nullptr, 0);
+        launch<rigid_vv2_body<false>, 128>(c, cdiv(r.nmol, 128), 0, r, c.pos[c.cur], c.vel[c.cur], c.force, kick ? 1 : 0, c.acc, nullptr, 0, nullptr, nullptr, 0);
 }
 
 }  // namespace mdx

@@ -29,14 +29,14 @@ namespace {
 
 constexpr int kRingThreads = 256;
 
-__global__ void __launch_bounds__(kRingThreads) k_ring(int natoms, int P, void *const *__restrict__ tab,
+__device__ __forceinline__ void ring_body(int blk, int nblk, int natoms, int P, void *const *__restrict__ tab,
                                                        const double *__restrict__ C,
                                                        const double *__restrict__ cw,
                                                        const BoxDev *__restrict__ bx) {
     extern __shared__ double sh[];  // [teams][P][6]: x, v of every bead
     const int teams = kRingThreads / P;
     const int team = threadIdx.x / P, n = threadIdx.x % P;
-    const int id = blockIdx.x * teams + team;
+    const int id = blk * teams + team;
     double *me = sh + (size_t)team * P * 6;
     const Geometry g = bx->geo;
     double4 *pos = (double4 *)tab[kRingTab * n];
@@ -115,9 +115,9 @@ __global__ void __launch_bounds__(kRingThreads) k_ring(int natoms, int P, void *
 
 // centroid of every atom's ring, by atom id (FP64; the reference's is in its
 // working precision)
-__global__ void __launch_bounds__(256) k_ring_centroid(int natoms, int P, void *const *__restrict__ tab,
+__device__ __forceinline__ void ring_centroid_body(int blk, int nblk, int natoms, int P, void *const *__restrict__ tab,
                                                        const BoxDev *__restrict__ bx, double4 *__restrict__ cen) {
-    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    const int id = blk * blockDim.x + threadIdx.x;
     if (id >= natoms) return;
     const Geometry g = bx->geo;
     const double4 x0 = ((const double4 *)tab[0])[((const int *)tab[2])[id]];
@@ -140,9 +140,9 @@ __global__ void __launch_bounds__(256) k_ring_centroid(int natoms, int P, void *
 
 // the centroid force (fixed point, by id) onto every bead of the atom; the
 // by-id buffer is cleared for the next evaluation
-__global__ void __launch_bounds__(256) k_ring_apply(int natoms, int P, void *const *__restrict__ tab,
+__device__ __forceinline__ void ring_apply_body(int blk, int nblk, int natoms, int P, void *const *__restrict__ tab,
                                                     unsigned long long *__restrict__ cf) {
-    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    const int id = blk * blockDim.x + threadIdx.x;
     if (id >= natoms) return;
     const unsigned long long f0 = cf[3 * id], f1 = cf[3 * id + 1], f2 = cf[3 * id + 2];
     if ((f0 | f1 | f2) == 0ull) return;
@@ -160,20 +160,18 @@ __global__ void __launch_bounds__(256) k_ring_apply(int natoms, int P, void *con
 
 }  // namespace
 
-void ring_centroids(const RingSpec &r, const BoxDev *box, hipStream_t s) {
-    hipLaunchKernelGGL(k_ring_centroid, dim3(cdiv(r.natoms, 256)), dim3(256), 0, s, r.natoms, r.P, r.tab, box, r.cen);
+void ring_centroids(const RingSpec &r, Ctx &c0) {
+    launch<ring_centroid_body, 256>(c0, cdiv(r.natoms, 256), 0, r.natoms, r.P, r.tab, c0.box, r.cen);
 }
 
-void ring_apply(const RingSpec &r, hipStream_t s) {
-    hipLaunchKernelGGL(k_ring_apply, dim3(cdiv(r.natoms, 256)), dim3(256), 0, s, r.natoms, r.P, r.tab,
-                       (unsigned long long *)r.cf);
+void ring_apply(const RingSpec &r, Ctx &c0) {
+    launch<ring_apply_body, 256>(c0, cdiv(r.natoms, 256), 0, r.natoms, r.P, r.tab, (unsigned long long *)r.cf);
 }
 
-void ring_drift(const RingSpec &r, const BoxDev *box, hipStream_t s) {
+void ring_drift(const RingSpec &r, Ctx &c0) {
     const int teams = kRingThreads / r.P;
     const size_t shm = sizeof(double) * 6 * kRingThreads;
-    hipLaunchKernelGGL(k_ring, dim3(cdiv(r.natoms, teams)), dim3(kRingThreads), shm, s, r.natoms, r.P, r.tab, r.C,
-                       r.cw, box);
+    launch<ring_body, kRingThreads>(c0, cdiv(r.natoms, teams), shm, r.natoms, r.P, r.tab, r.C, r.cw, c0.box);
 }
 
 }  // namespace mdx

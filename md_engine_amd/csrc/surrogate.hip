@@ -571,7 +571,7 @@ struct TwoBody {
 };
 
 template <bool EV, bool VIR>
-__global__ void __launch_bounds__(256) k_sur_fused(int nmol, const int4 *__restrict__ molAtoms,
+__device__ __forceinline__ void sur_fused_body(int blk, int nblk, int nmol, const int4 *__restrict__ molAtoms,
                                                    const float4 *__restrict__ xq, const BoxDev *__restrict__ bx,
                                                    int cap, const int *__restrict__ nl, const int *__restrict__ nlCount,
                                                    TwoBody tb, const float *__restrict__ P,
@@ -583,9 +583,9 @@ __global__ void __launch_bounds__(256) k_sur_fused(int nmol, const int4 *__restr
     __shared__ float part[256 / kFeatLanes][kFeatLanes][NR + 1];
     __shared__ float sdx[16][NF + 1];
     __shared__ double red[4][7];
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gt = blk * blockDim.x + threadIdx.x;
     const int i = gt / kFeatLanes, tl = threadIdx.x % kFeatLanes, team = threadIdx.x / kFeatLanes;
-    const int m0 = blockIdx.x * (256 / kFeatLanes);
+    const int m0 = blk * (256 / kFeatLanes);
     FeatAcc A;
 #pragma unroll
     for (int k = 0; k < NB; ++k) A.acc[k] = A.d1[k] = A.d2[k] = 0.f;
@@ -884,7 +884,7 @@ __device__ __forceinline__ void force_pair(float4 Oi, float4 H1i, float4 H2i, fl
 }
 
 template <bool VIR, int kForceLanes>
-__global__ void __launch_bounds__(256) k_sur_force(int nmol, const int4 *__restrict__ molAtoms,
+__device__ __forceinline__ void sur_force_body(int blk, int nblk, int nmol, const int4 *__restrict__ molAtoms,
                                                    const float4 *__restrict__ xq, const BoxDev *__restrict__ bx,
                                                    int cap, const int *__restrict__ nl,
                                                    const int *__restrict__ nlCount, const float *__restrict__ wgt,
@@ -895,7 +895,7 @@ __global__ void __launch_bounds__(256) k_sur_force(int nmol, const int4 *__restr
     constexpr int M = 256 / kForceLanes;
     __shared__ float4 swi[M][NW / 4];
     const Geo3 g = geo3(bx);
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gt = blk * blockDim.x + threadIdx.x;
     const int i = gt / kForceLanes, tl = threadIdx.x % kForceLanes, team = threadIdx.x / kForceLanes;
     for (int v = tl; v < NW / 4; v += kForceLanes)
         swi[team][v] = i < nmol ? ((const float4 *)(wgt + (size_t)i * NW))[v] : make_float4(0, 0, 0, 0);
@@ -1198,9 +1198,9 @@ void surrogate_forces(Ctx &c, bool energy, bool virial, bool with_feat) {
         const TwoBody tb{E.E2, E.k2, E.r2b * E.r2b};
         const dim3 g1(cdiv((long long)E.nmol * kFeatLanes, 256));
         float *feat = with_feat ? S.feat : nullptr;
-#define MDX_FUSED(EV, VI)                                                                                     \
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sur_fused<EV, VI>), g1, dim3(256), 0, c.stream, E.nmol, E.molAtoms, c.xq, \
-                       c.box, E.cap, E.nl, E.nlCount, tb, S.params, S.frag, feat, S.emol, S.wgt, c.force, c.acc)
+#define MDX_FUSED(EV, VI)                                                                                \
+    launch<sur_fused_body<EV, VI>, 256>(c, g1.x, 0, E.nmol, E.molAtoms, c.xq, c.box, E.cap, E.nl, E.nlCount, tb, \
+                                        S.params, S.frag, feat, S.emol, S.wgt, c.force, c.acc)
         if (energy && virial) MDX_FUSED(true, true);
         else if (energy) MDX_FUSED(true, false);
         else if (virial) MDX_FUSED(false, true);
@@ -1210,9 +1210,9 @@ void surrogate_forces(Ctx &c, bool energy, bool virial, bool with_feat) {
     ProfScope ps(c, PK_SUR_FORCE);
     const int lanes = S.forceLanes;
     const dim3 grid(cdiv((long long)E.nmol * lanes, 256));
-#define MDX_SFORCE(V, T)                                                                                        \
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sur_force<V, T>), grid, dim3(256), 0, c.stream, E.nmol, E.molAtoms, c.xq, \
-                       c.box, E.cap, E.nl, E.nlCount, S.wgt, c.force, c.acc)
+#define MDX_SFORCE(V, T)                                                                          \
+    launch<sur_force_body<V, T>, 256>(c, grid.x, 0, E.nmol, E.molAtoms, c.xq, c.box, E.cap, E.nl, E.nlCount, \
+                                      S.wgt, c.force, c.acc)
     switch (lanes) {
         case 8: if (virial) MDX_SFORCE(true, 8); else MDX_SFORCE(false, 8); break;
         case 16: if (virial) MDX_SFORCE(true, 16); else MDX_SFORCE(false, 16); break;

@@ -288,6 +288,7 @@ def test_c5_batch_48_boxes_equal_single_runs(orc, gpu_engine_factory):
         s.close()
     say(f"  sizes {int(sizes.min())}-{int(sizes.max())} molecules, {4 * int(sizes.sum())} sites")
     run_batch(batch, steps, flags)
+    assert batch[0].batch_merged(), "the 48 replicas did not run as one launch per kernel"
     for k, (e, (xs, vs)) in enumerate(zip(batch, singles)):
         xb, vb = e.get_system()
         assert np.array_equal(xb, xs), k

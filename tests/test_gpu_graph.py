@@ -85,12 +85,17 @@ def test_graph_thermostat_turns(orc, gpu_engine_factory):
     assert T == pytest.approx(350.0, rel=1e-6)
 
 
-def test_batch_equals_single(orc, gpu_engine_factory):
-    """Three replicas of different sizes in one graph == each alone."""
-    say("batched replicas vs single runs")
+@pytest.mark.parametrize("merge", [1, 0])
+def test_batch_equals_single(orc, gpu_engine_factory, merge):
+    """Three replicas of different sizes in one graph == each alone, with one
+    launch per kernel across the replicas (merge 1) and with one stream per
+    replica (merge 0)."""
+    say(f"batched replicas vs single runs (merge {merge})")
     from md_engine_amd import run_batch
 
-    specs = [(216, 41), (125, 42), (343, 43)]
+    # sizes whose replicas run the same kernels (replicas too small for the
+    # tiled charge spread take the per-atom spread and are not merged)
+    specs = [(512, 41), (729, 42), (1000, 43)]
     boxes = [systems.tip4p2005_box(m, seed=s, min_sep=1.8) for m, s in specs]
     vels = [start_velocities(orc, b, seed=s) for b, (_, s) in zip(boxes, specs)]
 
@@ -102,7 +107,9 @@ def test_batch_equals_single(orc, gpu_engine_factory):
 
     batch = [make(b, v) for b, v in zip(boxes, vels)]
     flags = batch[0][1]
+    batch[0][0].option("batch_merge", merge)
     run_batch([e for e, _ in batch], 70, flags)
+    assert batch[0][0].batch_merged() == bool(merge)
     for (eb, _), b, v in zip(batch, boxes, vels):
         es, fl = make(b, v)
         es.run(70, fl)

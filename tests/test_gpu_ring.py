@@ -185,3 +185,29 @@ def test_ring_hamiltonian_second_order(orc, gpu_engine_factory, mode):
     say(f"max |H - H0|: dt 0.2 fs {errs[0]:.5f}, dt 0.1 fs {errs[1]:.5f} kcal/mol")
     assert 0 < errs[0] < 0.01 * ke0
     assert errs[1] < 0.4 * errs[0]
+
+
+@pytest.mark.parametrize("mode", ["contracted", "per-bead"])
+def test_ring_merged_launches_equal_separate(orc, gpu_engine_factory, mode):
+    """The ring's per-bead kernels as one launch per kernel over all beads
+    (the default) give the trajectory of the beads launched one by one, bit
+    for bit, with the mesh part contracted to the centroids or per bead."""
+    say(f"ring: merged bead launches vs separate, mesh part {mode}")
+    P = 4
+    box = systems.tip4p2005_box(64, seed=33, min_sep=1.8)
+    xs = bead_systems(box, P, 5, spread=0.02)
+    vs = [start_velocities(orc, dict(box, x=x), seed=20 + k) for k, x in enumerate(xs)]
+    wP = ring_omega(P * 298.0, systems.REAL_BOLTZ, systems.REAL_HBAR)
+    out = []
+    for merge in (1, 0):
+        es, flags = ring_engines(gpu_engine_factory, box, xs, vs, dt=0.5)
+        run_flags = flags | (0 if mode == "contracted" else PPPM_PER_BEAD)
+        es[0].option("batch_merge", merge)
+        run_ring(es, 40, run_flags, wP)
+        assert es[0].batch_merged() == bool(merge)
+        out.append([e.get_system() for e in es])
+        for e in es:
+            e.close()
+    for (xa, va), (xb, vb) in zip(out[0], out[1]):
+        assert np.array_equal(xa, xb)
+        assert np.array_equal(va, vb)
