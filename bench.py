@@ -285,7 +285,9 @@ def train_surrogate(e, w, flags, frames=20, every=100, batch=1024,
 
 
 def _init_surrogate(rng):
-    nf, nh = 48, 64
+    from md_engine_amd._lib import SUR_NF, SUR_NH
+
+    nf, nh = SUR_NF, SUR_NH
     parts = [rng.normal(0, 1 / np.sqrt(nf), nf * nh), np.zeros(nh), rng.normal(0, 1 / np.sqrt(nh), nh * nh),
              np.zeros(nh), rng.normal(0, 1 / np.sqrt(nh), nh), [0.0], np.zeros(nf), np.ones(nf), [1.0, 0.0]]
     return np.concatenate([np.asarray(a, np.float64) for a in parts]).astype(np.float32)
@@ -371,6 +373,10 @@ def run_batch_mode(args):
         engines.append(e)
     n = sum(len(b["x"]) for b in boxes)
     run_batch(engines, args.equil, flags)
+    # team widths and list-build kernel chosen by timing whole merged batch steps
+    from md_engine_amd import tune_batch
+
+    tuned, tuned_ms = tune_batch(engines, flags, steps=16)
     run_batch(engines, args.warmup, flags)
     for e in engines:
         e.sync()
@@ -400,12 +406,16 @@ def run_batch_mode(args):
         "data": "synthetic (seeded lattices, random orientations)",
         "config": {"workload": "48 TIP4P/2005+E3B3 boxes, %d-%d molecules (seeded uniform), %d sites in all"
                                % (min(len(b["mol"]) for b in boxes), max(len(b["mol"]) for b in boxes), n),
-                   "step": "one captured hipGraph per 8 MD steps holding every replica; the ring in its own "
-                           "captured graphs (value: both sets of sites over both wall times)",
+                   "step": "one captured hipGraph per 8 MD steps holding every replica, one launch per kernel "
+                           "across the replicas; the ring likewise in its own captured graphs, one launch per "
+                           "kernel across the beads (value: both sets of sites over both wall times)",
                    "boxes_atom_steps_per_s": n * args.steps / wall,
+                   "merged_launches": bool(engines[0].batch_merged()),
+                   "tuned": dict(tuned, ms_per_step=tuned_ms),
                    "pimd_ring": {"beads": len(beads), "molecules": nsite // 4, "bead_sites": nring,
                                  "model": "flexible q-TIP4P/F, 298 K (bead temperature 32 x 298 K), dt 0.5 fs",
                                  "ms_per_step": wall_r / args.steps * 1e3,
+                                 "merged_launches": bool(beads[0].batch_merged()),
                                  "bead_site_steps_per_s": nring * args.steps / wall_r}},
     }
     for e in engines + beads:
@@ -570,6 +580,8 @@ def main():
         parity = {"list_violations": e.nlist_stats()["violations"]}
         # ---- the learned three-body leg (trained untimed on this run) -----
         if not args.no_surrogate:
+            from md_engine_amd._lib import SUR_NB, SUR_NF
+
             e.option("graph", 1)
             train = train_surrogate(e, w, flags)
             sflags = (flags & ~E3B) | SURROGATE
@@ -595,7 +607,8 @@ def main():
                 "speed_vs_e3b": (n * args.steps / wall_s) / value,
                 "three_body_force_rmse_vs_e3b": {"kcal_mol_A": rms[0], "kJ_mol_nm": rms[1], "bar_kJ_mol_nm": 0.5},
                 "e3b_judge": judge, "training": train, "three_body_kernel_avg_ms": kt,
-                "model": "48 symmetry-function inputs -> 64 -> 64 -> 1 SiLU, BF16 MFMA, FP32 accumulation",
+                "model": "%d symmetry-function inputs (%d radial channels) -> 64 -> 64 -> 1 SiLU, BF16 MFMA, FP32 accumulation"
+                         % (SUR_NF, SUR_NB),
             }
         # ---- NVE energy drift from the same system (untimed) --------------
         if args.drift_steps:

@@ -318,6 +318,17 @@ int mdx_barostat_last(mdx_ctx *ctx, double *pressure);
  * = build mode (2, the per-atom team build), build team, pair team, E3B team;
  * avg_ms[3] (may be NULL) = the winners' average build, pair, E3B times. */
 int mdx_tune(mdx_ctx *ctx, int flags, int evals, int32_t *chosen, double *avg_ms);
+/* Autotuner of a batch: like the reference's tuning during a run
+ * (Integrator.cu:272-376, real steps timed between candidates), times
+ * `steps` captured steps of the whole batch (mdx_run_batch, merged launches)
+ * for every candidate of the pair team width, the E3B team width and the
+ * list-build kernel (wave per atom, or teams of 8 or 16 lanes), one knob
+ * after the other, and keeps the fastest for every replica, so the batch
+ * still runs one launch per kernel whatever its size classes.  The timed
+ * steps advance the replicas (2 x steps per candidate).  chosen[3] (may be
+ * NULL) = build team (64: wave per atom), pair team, E3B team; ms_per_step
+ * (may be NULL) = the winner's time per batch step. */
+int mdx_tune_batch(mdx_ctx *const *ctxs, int n, int flags, int steps, int32_t *chosen, double *ms_per_step);
 int mdx_sync(mdx_ctx *ctx);
 /* Implementation switches (no counterpart in the reference): "tiled_spread"
  * 1 (default) accumulates the charge mesh in LDS tiles, 0 adds every weight

@@ -18,8 +18,14 @@ PAIR, PPPM, E3B, ENERGY, VIRIAL, SURROGATE, BONDED = 1, 2, 4, 8, 16, 32, 64
 VIRIAL_TRACE = 4096  # only the virial's trace (the barostat's form); read with Engine.virial_trace()
 PPPM_PER_BEAD = 8192  # ring polymers: per-bead P3M instead of the centroid contraction
 
-# trained surrogate parameters (W1 b1 W2 b2 w3 b3 of the 48-64-64-1 network)
-NTRAIN = 48 * 64 + 64 + 64 * 64 + 64 + 64 + 1
+# learned three-body model: SUR_NB radial channels, SUR_NF network inputs,
+# SUR_NF-64-64-1 network; trained parameters W1 b1 W2 b2 w3 b3, then the fixed
+# input mean and scale and the energy scale and shift
+SUR_NB = 5
+SUR_NF = 3 * SUR_NB + 6
+SUR_NH = 64
+NTRAIN = SUR_NF * SUR_NH + SUR_NH + SUR_NH * SUR_NH + SUR_NH + SUR_NH + 1
+SUR_NPARAM = NTRAIN + 2 * SUR_NF + 2
 
 # every symbol declared in include/mdx.h
 SYMBOLS = [
@@ -30,7 +36,7 @@ SYMBOLS = [
     "mdx_pppm_info", "mdx_pppm_set_error", "mdx_pppm_interval_set", "mdx_pppm_mesh_get", "mdx_e3b_set", "mdx_forces_compute", "mdx_forces_get", "mdx_energy_get", "mdx_virial_trace_get",
     "mdx_integrator_set", "mdx_run", "mdx_forces_compute_noalloc", "mdx_teams_set", "mdx_sync", "mdx_force_steps",
     "mdx_profile_enable", "mdx_profile_get", "mdx_nlist_totals", "mdx_pair_entries_read", "mdx_option_set", "mdx_rigid_set",
-    "mdx_thermostat_set", "mdx_temperature", "mdx_tune", "mdx_run_batch", "mdx_batch_merged", "mdx_surrogate_set",
+    "mdx_thermostat_set", "mdx_temperature", "mdx_tune", "mdx_run_batch", "mdx_batch_merged", "mdx_tune_batch", "mdx_surrogate_set",
     "mdx_surrogate_get", "mdx_surrogate_record", "mdx_surrogate_fit", "mdx_surrogate_eval", "mdx_surrogate_grad",
     "mdx_surrogate_judge_set", "mdx_surrogate_judge_stats",
     "mdx_e3b_molecule_energies", "mdx_nose_hoover_set", "mdx_nose_hoover_energy", "mdx_nose_hoover_state", "mdx_barostat_set",
@@ -117,6 +123,7 @@ def load():
     L.mdx_tune.argtypes = [_vp, c, c, _ip, _dp]
     L.mdx_run_batch.argtypes = [C.POINTER(_vp), c, c, c]
     L.mdx_batch_merged.argtypes = [_vp, C.POINTER(c)]
+    L.mdx_tune_batch.argtypes = [C.POINTER(_vp), c, c, c, _ip, C.POINTER(C.c_double)]
     _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
     L.mdx_surrogate_set.argtypes = [_vp, _fp, c]
     L.mdx_surrogate_get.argtypes = [_vp, _fp, c]
@@ -403,8 +410,8 @@ class Engine:
         return p.value
 
     # -- learned E3B three-body surrogate -----------------------------------
-    SUR_NPARAM = 7459
-    SUR_NRAW = 45  # raw descriptor channels per molecule
+    SUR_NPARAM = SUR_NPARAM
+    SUR_NRAW = 3 * SUR_NB + 3  # raw descriptor channels per molecule
 
     def surrogate_set(self, params):
         p = np.ascontiguousarray(params, np.float32)
@@ -497,6 +504,22 @@ class Engine:
         a, b = C.c_int64(0), C.c_int64(0)
         self._chk(self._L.mdx_pair_entries_read(self._h, C.byref(a), C.byref(b)), "mdx_pair_entries_read")
         return a.value, b.value
+
+
+def tune_batch(engines, flags, steps=16):
+    """Time whole batch steps for every candidate team width and list-build
+    kernel (mdx_tune_batch) and keep the fastest on every replica.  Returns
+    the choice (build team, pair team, E3B team) and the ms per batch step."""
+    engines = list(engines)
+    L = load()
+    arr = (_vp * len(engines))(*[e._h for e in engines])
+    ch = np.zeros(3, np.int32)
+    ms = C.c_double(0.0)
+    rc = L.mdx_tune_batch(arr, len(engines), int(flags), int(steps), ch, C.byref(ms))
+    if rc != 0:
+        msg = L.mdx_last_error(engines[0]._h)
+        raise MdxError(f"mdx_tune_batch failed ({rc}): {msg.decode() if msg else ''}")
+    return dict(build_team=int(ch[0]), pair_team=int(ch[1]), e3b_team=int(ch[2])), ms.value
 
 
 def run_batch(engines, nsteps, flags):

@@ -3,12 +3,14 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <map>
 #include <set>
 
 #include "../../include/mdx.h"
+
 #include "mdx_internal.h"
 
 struct mdx_ctx {
@@ -2013,6 +2015,61 @@ int mdx_run_batch(mdx_ctx *const *ctxs, int n, int nsteps, int flags) {
             cs.push_back(&ctxs[k]->c);
         }
         run_captured(cs, nsteps, flags);
+    });
+}
+
+int mdx_tune_batch(mdx_ctx *const *ctxs, int n, int flags, int steps, int32_t *chosen, double *ms_per_step) {
+    if (!ctxs || n <= 0) return MDX_ERR_ARG;
+    for (int k = 0; k < n; ++k)
+        if (!ctxs[k]) return MDX_ERR_ARG;
+    return guarded_ro(ctxs[0], [&](Ctx &) {
+        MDX_REQUIRE(steps > 0 && steps <= 10000, MDX_ERR_ARG, "steps must be in 1..10000");
+        std::vector<Ctx *> cs;
+        for (int k = 0; k < n; ++k) {
+            for (Ctx *o : cs) MDX_REQUIRE(o != &ctxs[k]->c, MDX_ERR_ARG, "a context appears twice in the batch");
+            cs.push_back(&ctxs[k]->c);
+        }
+        // one value of a knob on every replica; the step graphs are captured again
+        auto apply = [&](int build, int pair, int e3b) {
+            for (Ctx *c : cs) {
+                c->buildTeam = build;
+                c->buildTeamUser = true;
+                c->pairTeam = pair;
+                c->e3b.team = e3b;
+                ++c->apiEpoch;
+            }
+        };
+        auto timed = [&](int build, int pair, int e3b) {
+            apply(build, pair, e3b);
+            run_captured(cs, steps, flags);  // captures the graphs, warms the caches
+            const auto t0 = std::chrono::steady_clock::now();
+            run_captured(cs, steps, flags);
+            const auto t1 = std::chrono::steady_clock::now();
+            return std::chrono::duration<double, std::milli>(t1 - t0).count() / steps;
+        };
+        int build = cs[0]->buildTeam, pair = cs[0]->pairTeam, e3b = cs[0]->e3b.team;
+        double best = timed(build, pair, e3b);
+        auto knob = [&](int &v, std::initializer_list<int> cands, auto make) {
+            for (int cand : cands) {
+                if (cand == v) continue;
+                const double t = make(cand);
+                if (t < best) {
+                    best = t;
+                    v = cand;
+                }
+            }
+        };
+        if (flags & MDX_PAIR) knob(pair, {8, 16, 32}, [&](int t) { return timed(build, t, e3b); });
+        if ((flags & (MDX_E3B | MDX_SURROGATE)) && cs[0]->e3b.on)
+            knob(e3b, {16, 32}, [&](int t) { return timed(build, pair, t); });
+        knob(build, {64, 8, 16}, [&](int t) { return timed(t, pair, e3b); });
+        apply(build, pair, e3b);
+        if (chosen) {
+            chosen[0] = build;
+            chosen[1] = pair;
+            chosen[2] = e3b;
+        }
+        if (ms_per_step) *ms_per_step = best;
     });
 }
 

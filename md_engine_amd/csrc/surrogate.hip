@@ -24,12 +24,18 @@ namespace mdx {
 namespace {
 
 constexpr int NB = kSurNB, NR = kSurNR, NF = kSurNF, NH = kSurNH, NW = kSurNW;
+// network inputs padded with zero columns to KB whole k-blocks of 16 (the
+// padding has zero weights and zero inputs, so it changes no result)
+constexpr int NFP = kSurNFP, KB = NFP / 16;
 static_assert(NW >= NR && NW % 4 == 0, "weight rows hold the raw channels in float4 units");
-constexpr int KE = 3;  // basis with kappa = k3: the E3B hydrogen-bond function
+constexpr int KE = 3;       // basis with kappa = k3: the E3B hydrogen-bond function
+constexpr int K2 = NB - 1;  // basis with kappa = 2 k3: f^2 up to the switch
+static_assert(K2 == KE + 1, "the last channel squares the E3B one");
 // raw channel offsets: acc, d1, d2 (NB each), then pAA, pBB, pAB
 constexpr int RD1 = NB, RD2 = 2 * NB, RAA = 3 * NB, RBB = 3 * NB + 1, RAB = 3 * NB + 2;
-// radial basis g_k(r) = exp(-kappa_k r) fc(r), kappa_k = k3 (0.7 + 0.1 k)
-// (k3 = 1.907 / A, switching 5.0-5.2 A)
+// radial basis g_k(r) = exp(-kappa_k r) fc(r), kappa_k = k3 (0.7 + 0.1 k) for
+// k <= KE and kappa = 2 k3 for the last channel (k3 = 1.907 / A, switching
+// 5.0-5.2 A)
 constexpr float RS_C = 5.0f, RF_C = 5.2f, KAPPA0 = 0.7f * 1.907f, DKAPPA = 0.1f * 1.907f;
 
 // parameter offsets in the packed model (oracle/surrogate.py layout)
@@ -40,7 +46,22 @@ constexpr int NTRAIN = OMU;  // W1 b1 W2 b2 w3 b3 are trained; mu sd escale eshi
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ float kappa_k(int k) { return KAPPA0 + DKAPPA * (float)k; }
+__device__ __forceinline__ float kappa_k(int k) {
+    return k < K2 ? KAPPA0 + DKAPPA * (float)k : 2.0f * (KAPPA0 + DKAPPA * (float)KE);
+}
+
+// exp(-kappa_k r) of every channel: one exponential per factor, the rest by
+// recurrence and the last channel as the square of the E3B one
+__device__ __forceinline__ void basis_exp(float r, float (&e)[NB]) {
+    const float step = __expf(-DKAPPA * r);
+    float v = __expf(-KAPPA0 * r);
+#pragma unroll
+    for (int k = 0; k < K2; ++k) {
+        e[k] = v;
+        v *= step;
+    }
+    e[K2] = e[KE] * e[KE];
+}
 
 __device__ __forceinline__ short to_bf16(float f) {
     unsigned u = __float_as_uint(f);
@@ -125,16 +146,15 @@ __device__ __forceinline__ void feat_pair(FeatAcc &A, float4 Oi, float4 H1i, flo
         float sw, ds;
         fcut(r, sw, ds);
         if (sw <= 0.f) continue;
-        // exp(-kappa_k r) by recurrence: two exponentials per distance
-        const float step = __expf(-DKAPPA * r);
-        float v = __expf(-KAPPA0 * r) * sw;
+        float e[NB];
+        basis_exp(r, e);
 #pragma unroll
         for (int k = 0; k < NB; ++k) {
+            const float v = e[k] * sw;
             if (q < 2) A.acc[k] += v;
             else if (q == 2) A.d1[k] += v;
             else A.d2[k] += v;
             if (k == KE) fb[q] = v;
-            v *= step;
         }
     }
     A.pAA += fb[2] * fb[3];
@@ -192,6 +212,7 @@ __global__ void __launch_bounds__(256) k_sur_feat(int nmol, const int4 *__restri
 
 // network input column `col` of one molecule from its raw channels
 __device__ __forceinline__ float feature_of(const float *__restrict__ f, int col) {
+    if (col >= NF) return 0.f;  // padding
     if (col < NB) return f[col];
     if (col < 2 * NB) return f[RD1 + col - NB] + f[RD2 + col - NB];
     if (col < 3 * NB) return f[RD1 + col - 2 * NB] * f[RD2 + col - 2 * NB];
@@ -236,7 +257,7 @@ __device__ __forceinline__ f32x4 zero4() {
 
 // per-wave LDS scratch of the fused MLP (16 molecules)
 struct WaveBuf {
-    float X[16 * NF];
+    float X[16 * NFP];
     float Z1[16 * NH];
     float H1[16 * NH];
     float Z2[16 * NH];
@@ -247,10 +268,10 @@ struct WaveBuf {
 __device__ __forceinline__ void load_x(WaveBuf &w, const float *__restrict__ raw, const int *rows, int nrow,
                                        const float *mu, const float *sd) {
     const int l = threadIdx.x & 63;
-    for (int e = l; e < 16 * NF; e += 64) {
-        const int s = e / NF, col = e % NF;
+    for (int e = l; e < 16 * NFP; e += 64) {
+        const int s = e / NFP, col = e % NFP;
         const float v = s < nrow ? feature_of(raw + (size_t)rows[s] * NR, col) : 0.f;
-        w.X[s * NF + col] = (v - mu[col]) / sd[col];
+        w.X[s * NFP + col] = col < NF ? (v - mu[col]) / sd[col] : 0.f;
     }
 }
 
@@ -259,7 +280,7 @@ __device__ __forceinline__ void forward(WaveBuf &w, const float *W1, const float
                                         const float *b2) {
     const int l = threadIdx.x & 63;
 #pragma unroll
-    for (int n0 = 0; n0 < NH; n0 += 16) st16(w.Z1, NH, n0, mm16(w.X, NF, 1, W1 + n0, NH, 1, NF, zero4()));
+    for (int n0 = 0; n0 < NH; n0 += 16) st16(w.Z1, NH, n0, mm16(w.X, NFP, 1, W1 + n0, NH, 1, NFP, zero4()));
     wave_sync();
     for (int e = l; e < 16 * NH; e += 64) {
         const float z = w.Z1[e] + b1[e % NH];
@@ -334,12 +355,11 @@ constexpr int kWaves = 2;  // waves (16-molecule tiles) per workgroup of the tra
 // fragments packed once per model (k_sur_pack): lane l of fragment (t, kb)
 // holds 4 consecutive k of row l % 16.
 //   Z1^T = W1^T X^T, Z2^T = W2^T H1^T (forward), dH1^T = W2 dZ2^T, dX^T = W1 dZ1^T
-constexpr int FR_A1 = 0;                    // W1^T: 4 hidden tiles x 3 feature blocks
-constexpr int FR_A2 = FR_A1 + 12;           // W2^T: 4 x 4
+constexpr int FR_A1 = 0;                    // W1^T: 4 hidden tiles x KB feature blocks
+constexpr int FR_A2 = FR_A1 + 4 * KB;       // W2^T: 4 x 4
 constexpr int FR_A3 = FR_A2 + 16;           // W2:   4 x 4
-constexpr int FR_A4 = FR_A3 + 16;           // W1:   3 feature tiles x 4 hidden blocks
-constexpr int FR_N = FR_A4 + 12;
-static_assert(FR_N * 64 * 4 == 56 * 256, "fragment table size");
+constexpr int FR_A4 = FR_A3 + 16;           // W1:   KB feature tiles x 4 hidden blocks
+constexpr int FR_N = FR_A4 + 4 * KB;
 
 // one thread per (fragment, lane): 4 BF16 values
 __global__ void k_sur_pack(const float *__restrict__ P, unsigned short *__restrict__ frag) {
@@ -350,8 +370,8 @@ __global__ void k_sur_pack(const float *__restrict__ P, unsigned short *__restri
     for (int j = 0; j < 4; ++j) {
         float v;
         if (fr < FR_A2) {  // A = W1^T: row hidden 16 t + r, k feature 16 kb + 4 q + j
-            const int t = (fr - FR_A1) / 3, kb = (fr - FR_A1) % 3;
-            v = W1[(16 * kb + 4 * q + j) * NH + 16 * t + r];
+            const int t = (fr - FR_A1) / KB, kb = (fr - FR_A1) % KB, f = 16 * kb + 4 * q + j;
+            v = f < NF ? W1[f * NH + 16 * t + r] : 0.f;
         } else if (fr < FR_A3) {  // A = W2^T: row hidden2 16 t + r, k hidden1 16 kb + 4 q + j
             const int t = (fr - FR_A2) / 4, kb = (fr - FR_A2) % 4;
             v = W2[(16 * kb + 4 * q + j) * NH + 16 * t + r];
@@ -359,8 +379,8 @@ __global__ void k_sur_pack(const float *__restrict__ P, unsigned short *__restri
             const int t = (fr - FR_A3) / 4, kb = (fr - FR_A3) % 4;
             v = W2[(16 * t + r) * NH + 16 * kb + 4 * q + j];
         } else {  // A = W1: row feature 16 t + r, k hidden 16 kb + 4 q + j
-            const int t = (fr - FR_A4) / 4, kb = (fr - FR_A4) % 4;
-            v = W1[(16 * t + r) * NH + 16 * kb + 4 * q + j];
+            const int t = (fr - FR_A4) / 4, kb = (fr - FR_A4) % 4, f = 16 * t + r;
+            v = f < NF ? W1[f * NH + 16 * kb + 4 * q + j] : 0.f;
         }
         frag[(size_t)e * 4 + j] = (unsigned short)to_bf16(v);
     }
@@ -392,9 +412,9 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_sur_infer(int nmol, const fl
                                                               unsigned long long *__restrict__ acc, int energy) {
     // the weight fragments and vectors are staged in LDS once per workgroup
     __shared__ uint2 sfr[FR_N * 64];
-    __shared__ float sb1[NH], sb2[NH], sw3[NH], sw3r[NH], smu[NF], sisd[NF];
+    __shared__ float sb1[NH], sb2[NH], sw3[NH], sw3r[NH], smu[NFP], sisd[NFP];
     __shared__ float sraw[kMlpWaves][16][NR + 1];
-    __shared__ float sdx[kMlpWaves][16][NF + 1];
+    __shared__ float sdx[kMlpWaves][16][NFP + 1];
     static_assert(FR_N * 64 % (64 * kMlpWaves) == 0, "fragment staging");
     {
         // all loads first, then the LDS stores
@@ -411,9 +431,9 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_sur_infer(int nmol, const fl
         sw3[e] = P[OW3 + e];
         sw3r[e] = bf16_round(P[OW3 + e]);
     }
-    for (int e = threadIdx.x; e < NF; e += blockDim.x) {
-        smu[e] = P[OMU + e];
-        sisd[e] = 1.0f / P[OSD + e];
+    for (int e = threadIdx.x; e < NFP; e += blockDim.x) {
+        smu[e] = e < NF ? P[OMU + e] : 0.f;
+        sisd[e] = e < NF ? 1.0f / P[OSD + e] : 0.f;
     }
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, q = l >> 4;
     const int m0 = (blockIdx.x * kMlpWaves + wv) * 16;
@@ -435,9 +455,9 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_sur_infer(int nmol, const fl
     if (nrow == 0) return;  // whole wave; no barrier follows
     const float *fm = sraw[wv][min(r, nrow - 1)];
     // X^T k-blocks: lane holds features 16 kb + 4 q + j of molecule r
-    bf16x4 xb[3];
+    bf16x4 xb[KB];
 #pragma unroll
-    for (int kb = 0; kb < 3; ++kb) {
+    for (int kb = 0; kb < KB; ++kb) {
         f32x4 v;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -453,8 +473,8 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_sur_infer(int nmol, const fl
     for (int t = 0; t < 4; ++t) {
         f32x4 c = zero4();
 #pragma unroll
-        for (int kb = 0; kb < 3; ++kb)
-            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld_frag(sfr, FR_A1 + 3 * t + kb, l), xb[kb], c, 0, 0, 0);
+        for (int kb = 0; kb < KB; ++kb)
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld_frag(sfr, FR_A1 + KB * t + kb, l), xb[kb], c, 0, 0, 0);
         f32x4 h;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -514,7 +534,7 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_sur_infer(int nmol, const fl
     }
     // dX^T = W1 dZ1^T, then dE/dF = dX / sd into LDS (row = molecule)
 #pragma unroll
-    for (int t = 0; t < 3; ++t) {
+    for (int t = 0; t < KB; ++t) {
         f32x4 c = zero4();
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb)
@@ -581,7 +601,7 @@ __device__ __forceinline__ void sur_fused_body(int blk, int nblk, int nmol, cons
                                                    unsigned long long *__restrict__ acc) {
     const Geo3 g = geo3(bx);
     __shared__ float part[256 / kFeatLanes][kFeatLanes][NR + 1];
-    __shared__ float sdx[16][NF + 1];
+    __shared__ float sdx[16][NFP + 1];
     __shared__ double red[4][7];
     const int gt = blk * blockDim.x + threadIdx.x;
     const int i = gt / kFeatLanes, tl = threadIdx.x % kFeatLanes, team = threadIdx.x / kFeatLanes;
@@ -693,14 +713,14 @@ __device__ __forceinline__ void sur_fused_body(int blk, int nblk, int nmol, cons
         a[3] = (short)(u.y >> 16);
         return a;
     };
-    bf16x4 xb[3];
+    bf16x4 xb[KB];
 #pragma unroll
-    for (int kb = 0; kb < 3; ++kb) {
+    for (int kb = 0; kb < KB; ++kb) {
         f32x4 v;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int col = 16 * kb + 4 * q + j;
-            v[j] = (feature_of(fm, col) - P[OMU + col]) / P[OSD + col];
+            v[j] = col < NF ? (feature_of(fm, col) - P[OMU + col]) / P[OSD + col] : 0.f;
         }
         xb[kb] = pack4(v);
     }
@@ -710,8 +730,8 @@ __device__ __forceinline__ void sur_fused_body(int blk, int nblk, int nmol, cons
     for (int t = 0; t < 4; ++t) {
         f32x4 c = zero4();
 #pragma unroll
-        for (int kb = 0; kb < 3; ++kb)
-            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ldf(FR_A1 + 3 * t + kb), xb[kb], c, 0, 0, 0);
+        for (int kb = 0; kb < KB; ++kb)
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ldf(FR_A1 + KB * t + kb), xb[kb], c, 0, 0, 0);
         f32x4 h;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -766,7 +786,7 @@ __device__ __forceinline__ void sur_fused_body(int blk, int nblk, int nmol, cons
         dz1[t] = pack4(d);
     }
 #pragma unroll
-    for (int t = 0; t < 3; ++t) {
+    for (int t = 0; t < KB; ++t) {
         f32x4 c = zero4();
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb)
@@ -774,7 +794,7 @@ __device__ __forceinline__ void sur_fused_body(int blk, int nblk, int nmol, cons
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int col = 16 * t + 4 * q + j;
-            sdx[r][col] = c[j] / P[OSD + col];
+            sdx[r][col] = col < NF ? c[j] / P[OSD + col] : 0.f;
         }
     }
     wave_sync();
@@ -826,14 +846,15 @@ __device__ __forceinline__ void force_pair(float4 Oi, float4 H1i, float4 H2i, fl
     // q = 0, 1: H1_i, H2_i -> O_j (i's donor channels, j's acceptor);
     // q = 2, 3: H1_j, H2_j -> O_i (j's donor channels, i's acceptor)
     const float3 d[4] = {mi(H1i, Oj, g), mi(H2i, Oj, g), mi(H1j, Oi, g), mi(H2j, Oi, g)};
-    float r[4], sw[4], ds[4], fb[4], dfb[4];
+    float r[4], sw[4], ds[4], fb[4], dfb[4], eb[4][NB];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         r[q] = sqrtf(d[q].x * d[q].x + d[q].y * d[q].y + d[q].z * d[q].z);
         fcut(r[q], sw[q], ds[q]);
-        const float ek = __expf(-(KAPPA0 + DKAPPA * KE) * r[q]);
+        basis_exp(r[q], eb[q]);
+        const float ek = eb[q][KE];
         fb[q] = ek * sw[q];
-        dfb[q] = ek * (ds[q] - (KAPPA0 + DKAPPA * KE) * sw[q]);
+        dfb[q] = ek * (ds[q] - kappa_k(KE) * sw[q]);
     }
     const float wAB = wi[RAB] + wj[RAB];
     const float Dij = fb[0] + fb[1], Dji = fb[2] + fb[3];
@@ -842,15 +863,12 @@ __device__ __forceinline__ void force_pair(float4 Oi, float4 H1i, float4 H2i, fl
         if (sw[q] <= 0.f && ds[q] == 0.f) continue;
         // d/dr [exp(-kappa r) fc] = exp(-kappa r) (fc' - kappa fc)
         float dEdr = 0.f;
-        const float step = __expf(-DKAPPA * r[q]);
-        float ek = __expf(-KAPPA0 * r[q]);
 #pragma unroll
         for (int k = 0; k < NB; ++k) {
             const float w = q == 0 ? wi[RD1 + k] + wj[k]
                           : q == 1 ? wi[RD2 + k] + wj[k]
                           : q == 2 ? wj[RD1 + k] + wi[k] : wj[RD2 + k] + wi[k];
-            dEdr += w * ek * (ds[q] - kappa_k(k) * sw[q]);
-            ek *= step;
+            dEdr += w * eb[q][k] * (ds[q] - kappa_k(k) * sw[q]);
         }
         // pair products: the other bond of the same hydrogen pair, and D of the reverse direction
         const float other = fb[q ^ 1];
@@ -984,29 +1002,29 @@ constexpr int kGradStride = NTRAIN + 1;  // trained parameters + loss
 __global__ void __launch_bounds__(64 * kWaves) k_sur_grad(int nsamp, int bstart, int bsize, unsigned mulA, unsigned addB,
                                                   const float *__restrict__ P, const float *__restrict__ raw,
                                                   const float *__restrict__ target, float *__restrict__ part) {
-    __shared__ float sW1[NF * NH], sW2[NH * NH], sb1[NH], sb2[NH], sw3[NH], smu[NF], ssd[NF];
+    __shared__ float sW1[NFP * NH], sW2[NH * NH], sb1[NH], sb2[NH], sw3[NH], smu[NFP], ssd[NFP];
     __shared__ WaveBuf wb[kWaves];
     __shared__ float sg[kWaves][16], se[kWaves][16];
     __shared__ int srow[kWaves][16];
-    for (int e = threadIdx.x; e < NF * NH; e += blockDim.x) sW1[e] = P[OW1 + e];
+    for (int e = threadIdx.x; e < NFP * NH; e += blockDim.x) sW1[e] = e < NF * NH ? P[OW1 + e] : 0.f;
     for (int e = threadIdx.x; e < NH * NH; e += blockDim.x) sW2[e] = P[OW2 + e];
     for (int e = threadIdx.x; e < NH; e += blockDim.x) {
         sb1[e] = P[OB1 + e];
         sb2[e] = P[OB2 + e];
         sw3[e] = P[OW3 + e];
     }
-    for (int e = threadIdx.x; e < NF; e += blockDim.x) {
-        smu[e] = P[OMU + e];
-        ssd[e] = P[OSD + e];
+    for (int e = threadIdx.x; e < NFP; e += blockDim.x) {
+        smu[e] = e < NF ? P[OMU + e] : 0.f;
+        ssd[e] = e < NF ? P[OSD + e] : 1.f;
     }
     __syncthreads();
     const float b3 = P[OB3], es = P[OES], eh = P[OEH];
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, q = l >> 4;
     const int wave = blockIdx.x * kWaves + wv;
     WaveBuf &w = wb[wv];
-    f32x4 gW1[3][4], gW2[4][4];
+    f32x4 gW1[KB][4], gW2[4][4];
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
+    for (int a = 0; a < KB; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) gW1[a][b] = zero4();
 #pragma unroll
@@ -1062,18 +1080,19 @@ __global__ void __launch_bounds__(64 * kWaves) k_sur_grad(int nsamp, int bstart,
             for (int b = 0; b < 4; ++b) gW2[a][b] = mm16(w.H1 + a * 16, 1, NH, w.T + b * 16, NH, 1, 16, gW2[a][b]);
         // dW1 += X^T dZ1
 #pragma unroll
-        for (int a = 0; a < 3; ++a)
+        for (int a = 0; a < KB; ++a)
 #pragma unroll
-            for (int b = 0; b < 4; ++b) gW1[a][b] = mm16(w.X + a * 16, 1, NF, w.Z1 + b * 16, NH, 1, 16, gW1[a][b]);
+            for (int b = 0; b < 4; ++b) gW1[a][b] = mm16(w.X + a * 16, 1, NFP, w.Z1 + b * 16, NH, 1, 16, gW1[a][b]);
         wave_sync();
     }
     float *o = part + (size_t)wave * kGradStride;
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
+    for (int a = 0; a < KB; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) o[OW1 + (a * 16 + 4 * q + j) * NH + b * 16 + r] = gW1[a][b][j];
+            for (int j = 0; j < 4; ++j)
+                if (a * 16 + 4 * q + j < NF) o[OW1 + (a * 16 + 4 * q + j) * NH + b * 16 + r] = gW1[a][b][j];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll

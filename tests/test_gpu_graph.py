@@ -156,3 +156,34 @@ def test_cached_graphs_across_calls(orc, gpu_engine_factory):
         assert np.array_equal(x, out[0][0])
         assert np.array_equal(vv, out[0][1])
         assert np.array_equal(L, out[0][2])
+
+
+def test_batch_tuner_keeps_launches_merged(orc, gpu_engine_factory):
+    """The batch autotuner times whole merged steps per candidate and leaves
+    one choice on every replica: the batch still runs one launch per kernel
+    afterwards, and the replicas keep their constraints."""
+    say("batch autotuner")
+    from md_engine_amd import run_batch, tune_batch
+
+    specs = [(512, 51), (729, 52), (1000, 53)]
+    boxes = [systems.tip4p2005_box(m, seed=s, min_sep=1.8) for m, s in specs]
+    vels = [start_velocities(orc, b, seed=s) for b, (_, s) in zip(boxes, specs)]
+    batch = []
+    for b, v in zip(boxes, vels):
+        e, flags = water_md_engine(gpu_engine_factory, b, v, rc=6.5, skin=0.9, sz=(32, 32, 32), order=5,
+                                   e3b=True, dt=2.0)
+        e.option("graph", 1)
+        batch.append(e)
+    choice, ms = tune_batch(batch, flags, steps=8)
+    say(f"  chosen {choice}, {ms:.3f} ms per batch step")
+    assert choice["build_team"] in (64, 8, 16) and choice["pair_team"] in (8, 16, 32)
+    assert choice["e3b_team"] in (16, 32) and ms > 0
+    run_batch(batch, 24, flags)
+    assert batch[0].batch_merged()
+    for e, b in zip(batch, boxes):
+        x, _ = e.get_system()
+        o, h1 = x[b["mol"][:, 0]], x[b["mol"][:, 1]]
+        d = h1 - o
+        d -= b["L"] * np.rint(d / b["L"])
+        assert np.allclose(np.linalg.norm(d, axis=1), G["dOH"], atol=1e-8)
+        e.close()
