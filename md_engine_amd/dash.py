@@ -226,6 +226,10 @@ class State:
         self.bounds = Bounds()
         self.rCut = -1.0
         self.padding = 0.5
+        # The reference wraps atoms and tests the skin criterion every
+        # periodicInterval turns (IntegratorVerlet.cu:696); the engine tests it
+        # on the device every step, so a list is never older than the
+        # reference's.  Kept so scripts that set it run unchanged.
         self.periodicInterval = 50
         self.shoutEvery = 0
         self.dt = 0.005
