@@ -137,7 +137,9 @@ __device__ __forceinline__ void e3b_pass1_body(int blk, int nblk, E3BArgs p) {
         const int *row = p.nl + (size_t)i * p.cap;
         for (int k = tl; k < cnt; k += P) {
             const int4 aj = p.molAtoms[row[k]];
-            const float4 Oj = p.xq[aj.x];
+            // the hydrogens are loaded with the oxygen (one load level less
+            // than loading them after the distance test)
+            const float4 Oj = p.xq[aj.x], H1j = p.xq[aj.y], H2j = p.xq[aj.z];
             const float3 d = mi3(Oi, Oj, p);
             const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
             const float r = sqrtf(r2);
@@ -159,7 +161,6 @@ __device__ __forceinline__ void e3b_pass1_body(int blk, int nblk, E3BArgs p) {
                 }
             }
             if (p.three && r <= p.rmax) {
-                const float4 H1j = p.xq[aj.y], H2j = p.xq[aj.z];
                 tx += hb_only(mi3(H1j, Oi, p), p) + hb_only(mi3(H2j, Oi, p), p);
                 ty += hb_only(mi3(H1i, Oj, p), p);
                 tz += hb_only(mi3(H2i, Oj, p), p);
@@ -206,12 +207,12 @@ __device__ __forceinline__ void e3b_pass2_body(int blk, int nblk, E3BArgs p) {
         for (int k = tl; k < cnt; k += P) {
             const int mj = row[k];
             const int4 aj = p.molAtoms[mj];
-            const float4 Oj = p.xq[aj.x];
+            // every record of the neighbour is loaded at once (one load level)
+            const float4 Oj = p.xq[aj.x], H1j = p.xq[aj.y], H2j = p.xq[aj.z];
+            const float4 tj = p.tot[mj];
             const float3 d = mi3(Oi, Oj, p);
             const float r = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
             if (!(r <= p.rmax)) continue;
-            const float4 H1j = p.xq[aj.y], H2j = p.xq[aj.z];
-            const float4 tj = p.tot[mj];
             float3 gA1, gA2, gB1, gB2;
             const float3 dA1 = mi3(H1i, Oj, p), dA2 = mi3(H2i, Oj, p);
             const float3 dB1 = mi3(H1j, Oi, p), dB2 = mi3(H2j, Oi, p);

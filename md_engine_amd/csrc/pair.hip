@@ -78,77 +78,97 @@ __device__ __forceinline__ void pair_body(int blk, int, PairArgs a) {
         const uint32_t *row = a.nl + (size_t)i * a.cap;
         const float twoAlphaRtPi = 2.0f * 0.5641895835477563f * alphaB;
         const float alpha2 = alphaB * alphaB;
-        // part 0: charged neighbors [0, nA); part 1: charge-free ones [nA, cnt)
+        // one row entry: tag, neighbor record, neighbor type
+        auto term = [&](uint32_t tag, const float4 &pj, int tj, bool lj, bool coul) {
+            float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            dx -= bL0 * rintf(dx * bI0);
+            dy -= bL1 * rintf(dy * bI1);
+            dz -= bL2 * rintf(dz * bI2);
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            // special-pair multiplier by selection (a dynamic index would
+            // put the argument pack in scratch memory)
+            const float m = tag == 0 ? 1.0f : (tag == 1 ? sp1 : (tag == 2 ? sp2 : sp3));
+            float fs = 0.f;
+            bool any = false;
+            if (lj) {
+                const int tp = a.ntypes > 1 ? ti * a.ntypes + tj : 0;
+                if (r2 < sp[tp]) {
+                    any = true;
+                    if (m != 0.0f) {
+                        const float eps24 = sp[T2 + tp], sig6 = sp[2 * T2 + tp];
+                        const float r2inv = 1.0f / r2;
+                        const float r6inv = r2inv * r2inv * r2inv;
+                        const float p1 = eps24 * 2.0f * sig6 * sig6;
+                        const float p2 = eps24 * sig6;
+                        fs += r6inv * r2inv * (p1 * r6inv - p2) * m;
+                        if (EV) {
+                            const float rc2 = sp[tp];
+                            const float s6r6 = sig6 * r6inv;
+                            const float rc6 = rc2 * rc2 * rc2;
+                            const float s6rc6 = sig6 / rc6;
+                            e += 0.5f * (eps24 * (1.0f / 6.0f)) * (s6r6 * (s6r6 - 1.0f) - s6rc6 * (s6rc6 - 1.0f)) * m;
+                        }
+                    }
+                }
+            }
+            if (coul && r2 < a.rq2) {
+                any = true;
+                const float ls = fmaxf(r2, 1e-10f);
+                const float rinv = rsqrtf(ls);
+                const float len = ls * rinv;
+                const float qq = a.qqr * pi.w * pj.w;
+                // erfc(ar) - (1 - m); for a full exclusion (m = 0) this is -erf(ar).
+                // erfc shares the Gaussian's exponential (erfc_fit.h)
+                const float gauss = __expf(-alpha2 * ls);
+                const float ec = erfc_from_gauss(alphaB * len, gauss) - (1.0f - m);
+                const float s = qq * (ec * rinv + twoAlphaRtPi * gauss);
+                fs += s * rinv * rinv;
+                if (EV) e += 0.5f * qq * rinv * ec;
+            }
+            if (any) {
+                fx += dx * fs;
+                fy += dy * fs;
+                fz += dz * fs;
+                if (VT) v0 += fs * r2;
+                if (EV) {
+                    v0 += dx * fs * dx;
+                    v1 += dy * fs * dy;
+                    v2 += dz * fs * dz;
+                    v3 += dx * fs * dy;
+                    v4 += dx * fs * dz;
+                    v5 += dy * fs * dz;
+                }
+            }
+        };
+        // part 0: charged neighbors [0, nA); part 1: charge-free ones [nA, cnt).
+        // A lane takes entries tl, tl + P, tl + 2P, ... in that order, U of them
+        // per pass with all their loads in flight before any is used (one
+        // dependent row-then-record load chain per entry left the lanes waiting).
+        constexpr int U = 4;
         for (int part = 0; part < 2; ++part) {
             const bool lj = LJ && (fl & (1 << part));
             const bool coul = COUL && part == 0 && pi.w != 0.0f;
             if (!lj && !coul) continue;
             const int k0 = part == 0 ? 0 : nA, k1 = part == 0 ? nA : cnt;
-            for (int k = k0 + tl; k < k1; k += P) {
-                const uint32_t raw = row[k];
-                const int j = (int)(raw & 0x3FFFFFFFu);
-                const uint32_t tag = raw >> 30;
-                const float4 pj = a.xq[j];
-                float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                dx -= bL0 * rintf(dx * bI0);
-                dy -= bL1 * rintf(dy * bI1);
-                dz -= bL2 * rintf(dz * bI2);
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                // special-pair multiplier by selection (a dynamic index would
-                // put the argument pack in scratch memory)
-                const float m = tag == 0 ? 1.0f : (tag == 1 ? sp1 : (tag == 2 ? sp2 : sp3));
-                float fs = 0.f;
-                bool any = false;
-                if (lj) {
-                    const int tp = a.ntypes > 1 ? ti * a.ntypes + a.type[j] : 0;
-                    if (r2 < sp[tp]) {
-                        any = true;
-                        if (m != 0.0f) {
-                            const float eps24 = sp[T2 + tp], sig6 = sp[2 * T2 + tp];
-                            const float r2inv = 1.0f / r2;
-                            const float r6inv = r2inv * r2inv * r2inv;
-                            const float p1 = eps24 * 2.0f * sig6 * sig6;
-                            const float p2 = eps24 * sig6;
-                            fs += r6inv * r2inv * (p1 * r6inv - p2) * m;
-                            if (EV) {
-                                const float rc2 = sp[tp];
-                                const float s6r6 = sig6 * r6inv;
-                                const float rc6 = rc2 * rc2 * rc2;
-                                const float s6rc6 = sig6 / rc6;
-                                e += 0.5f * (eps24 * (1.0f / 6.0f)) *
-                                     (s6r6 * (s6r6 - 1.0f) - s6rc6 * (s6rc6 - 1.0f)) * m;
-                            }
-                        }
-                    }
+            for (int kb = k0 + tl; kb < k1; kb += U * P) {
+                uint32_t raw[U];
+                bool ok[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    ok[u] = kb + u * P < k1;
+                    raw[u] = ok[u] ? row[kb + u * P] : 0u;
                 }
-                if (coul && r2 < a.rq2) {
-                    any = true;
-                    const float ls = fmaxf(r2, 1e-10f);
-                    const float rinv = rsqrtf(ls);
-                    const float len = ls * rinv;
-                    const float qq = a.qqr * pi.w * pj.w;
-                    // erfc(ar) - (1 - m); for a full exclusion (m = 0) this is -erf(ar).
-                    // erfc shares the Gaussian's exponential (erfc_fit.h)
-                    const float gauss = __expf(-alpha2 * ls);
-                    const float ec = erfc_from_gauss(alphaB * len, gauss) - (1.0f - m);
-                    const float s = qq * (ec * rinv + twoAlphaRtPi * gauss);
-                    fs += s * rinv * rinv;
-                    if (EV) e += 0.5f * qq * rinv * ec;
+                float4 pj[U];
+                int tj[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int j = ok[u] ? (int)(raw[u] & 0x3FFFFFFFu) : i;
+                    pj[u] = a.xq[j];
+                    tj[u] = (lj && a.ntypes > 1) ? a.type[j] : 0;
                 }
-                if (any) {
-                    fx += dx * fs;
-                    fy += dy * fs;
-                    fz += dz * fs;
-                    if (VT) v0 += fs * r2;
-                    if (EV) {
-                        v0 += dx * fs * dx;
-                        v1 += dy * fs * dy;
-                        v2 += dz * fs * dz;
-                        v3 += dx * fs * dy;
-                        v4 += dx * fs * dz;
-                        v5 += dy * fs * dz;
-                    }
-                }
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (ok[u]) term(raw[u] >> 30, pj[u], tj[u], lj, coul);
             }
         }
     }

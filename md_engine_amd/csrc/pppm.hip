@@ -539,21 +539,36 @@ __device__ __forceinline__ void gather_body(int blk, int nblk, int n, const doub
     weights<ORDER>(d[2], wz);
     constexpr int H = (ORDER - 1) / 2;
     float ex = 0.f, ey = 0.f, ez = 0.f;
-    for (int r = tl; r < ORDER * ORDER; r += T) {
+    // a lane's stencil rows r = tl, tl + T, ...: every row's field points are
+    // loaded before any is used (the rows one after another left each lane
+    // waiting on one row's loads at a time); the sums run in row order
+    constexpr int NRW = (ORDER * ORDER + T - 1) / T;
+    int zi[ORDER];
+#pragma unroll
+    for (int iz = 0; iz < ORDER; ++iz) zi[iz] = wrapi(g[2] + iz - H, m.sz[2]);
+    float4 e[NRW][ORDER];
+#pragma unroll
+    for (int rr = 0; rr < NRW; ++rr) {
+        const int r = min(tl + rr * T, ORDER * ORDER - 1);
         const int ix = r / ORDER, iy = r - ix * ORDER;
         const int px = wrapi(g[0] + ix - H, m.sz[0]);
         const int py = wrapi(g[1] + iy - H, m.sz[1]);
-        const float wxy = pick<ORDER>(wx, ix) * pick<ORDER>(wy, iy);
         const size_t rowb = ((size_t)px * m.sz[1] + py) * m.sz[2];
-        float4 e[ORDER];
 #pragma unroll
-        for (int iz = 0; iz < ORDER; ++iz) e[iz] = E4[rowb + wrapi(g[2] + iz - H, m.sz[2])];
+        for (int iz = 0; iz < ORDER; ++iz) e[rr][iz] = E4[rowb + zi[iz]];
+    }
+#pragma unroll
+    for (int rr = 0; rr < NRW; ++rr) {
+        const int r = tl + rr * T;
+        if (r >= ORDER * ORDER) break;
+        const int ix = r / ORDER, iy = r - ix * ORDER;
+        const float wxy = pick<ORDER>(wx, ix) * pick<ORDER>(wy, iy);
 #pragma unroll
         for (int iz = 0; iz < ORDER; ++iz) {
             const float w = wxy * wz[iz];
-            ex += w * e[iz].x;
-            ey += w * e[iz].y;
-            ez += w * e[iz].z;
+            ex += w * e[rr][iz].x;
+            ey += w * e[rr][iz].y;
+            ez += w * e[rr][iz].z;
         }
     }
     ex = team_sum<T>(ex);

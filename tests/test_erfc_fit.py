@@ -47,3 +47,4 @@ def test_fp32_relative_and_absolute_error():
     assert rel.max() < 5e-7, rel.max()
     x = np.linspace(0.0, 10.0, 50001).astype(np.float32).astype(np.float64)
     assert np.abs(evaluate(x, p, c, np.float32) - erfc(x)).max() < 5e-7
+
