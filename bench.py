@@ -434,15 +434,19 @@ def pair_roofline(e, n, prof):
     avg_s = tot_ms / max(cnt, 1) * 1e-3
     bytes_launch = n * (16 + 4 + 8) + 24 * read_atoms + 4 * read_entries
     achieved = bytes_launch / avg_s / 1e9
-    traffic = None
+    traffic = valu_busy = None
     tpath = os.path.join(ROOT, "profiles", "pair_traffic_tip4p_e3b_8192.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
-            traffic = json.load(fh).get("bytes_per_launch")
+            rec = json.load(fh)
+        traffic, valu_busy = rec.get("bytes_per_launch"), rec.get("valu_busy")
     return {
         "kernel": "k_pair (LJ + Ewald real space)", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
         "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/force_path_tip4p_e3b_8192.md" if traffic else None,
+        # fraction of SIMD cycles issuing vector ALU work (SQ_ACTIVE_INST_VALU):
+        # the kernel is limited by instruction issue, not by HBM
+        "valu_busy": valu_busy,
         "bytes_per_launch": bytes_launch, "avg_launch_ms": avg_s * 1e3, "launches": cnt,
     }, read_entries
 
