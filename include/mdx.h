@@ -107,6 +107,12 @@ int mdx_pppm_set(mdx_ctx *ctx, int nx, int ny, int nz, int order, double r_cut, 
                  double alpha);
 /* Current alpha and the RMS error estimate DeltaF_k + DeltaF_real. */
 int mdx_pppm_info(mdx_ctx *ctx, double *alpha, double *rms_error);
+/* Kernels the last mesh evaluation ran (no counterpart in the reference;
+ * for tests and benchmarks): spread 2 owner-computes cell blocks, 1 LDS tiles
+ * with global atomics, 0 global atomics; gather 2 LDS field tiles, 1 row
+ * teams from the per-site mesh records, 0 row teams from the positions; -1
+ * before any evaluation. */
+int mdx_pppm_paths(mdx_ctx *ctx, int *spread, int *gather);
 /* Error mode (FixChargeEwald::setError + setGridToErrorTolerance,
  * FixChargeEwald.cu:1024-1096): the mesh starts from the one set before
  * (else 64^3) and the axis with the fewest points per length is doubled until
@@ -333,7 +339,13 @@ int mdx_tune_batch(mdx_ctx *const *ctxs, int n, int flags, int steps, int32_t *c
 int mdx_sync(mdx_ctx *ctx);
 /* Implementation switches (no counterpart in the reference): "tiled_spread"
  * 1 (default) accumulates the charge mesh in LDS tiles, 0 adds every weight
- * to the global mesh directly (bit-identical meshes); "graph" 1 (default)
+ * to the global mesh directly (bit-identical meshes); "p3m_spread" 1 (default)
+ * LDS tiles with one global atomic per tile point, 2 owner-computes cell
+ * blocks writing the real mesh directly (no global atomics), 0 global
+ * atomics only (bit-identical meshes); "p3m_gather" 1 (default) row teams
+ * from the per-site mesh records of the position refresh, 2 the field box of
+ * each cell-column tile staged in LDS, 0 row teams computing the mesh point
+ * from the positions (identical forces); "graph" 1 (default)
  * runs mdx_run as captured 8-step graphs, 0 eagerly; "build_mode" 0 or 2
  * (both the per-atom team build); "cell_reach_xy" 1..2 and "cell_reach_z"
  * 1..16: cell edges >= rlist / reach per axis (identical lists, finer cells

@@ -33,7 +33,7 @@ SYMBOLS = [
     "mdx_positions_set", "mdx_system_get", "mdx_box_set", "mdx_nlist_config",
     "mdx_exclusions_set", "mdx_exclusions_from_bonds", "mdx_nlist_update", "mdx_nlist_get",
     "mdx_nlist_stats", "mdx_lj_set", "mdx_special_set", "mdx_ewald_real_set", "mdx_pppm_set",
-    "mdx_pppm_info", "mdx_pppm_set_error", "mdx_pppm_interval_set", "mdx_pppm_mesh_get", "mdx_e3b_set", "mdx_forces_compute", "mdx_forces_get", "mdx_energy_get", "mdx_virial_trace_get",
+    "mdx_pppm_info", "mdx_pppm_paths", "mdx_pppm_set_error", "mdx_pppm_interval_set", "mdx_pppm_mesh_get", "mdx_e3b_set", "mdx_forces_compute", "mdx_forces_get", "mdx_energy_get", "mdx_virial_trace_get",
     "mdx_integrator_set", "mdx_run", "mdx_forces_compute_noalloc", "mdx_teams_set", "mdx_sync", "mdx_force_steps",
     "mdx_profile_enable", "mdx_profile_get", "mdx_nlist_totals", "mdx_pair_entries_read", "mdx_option_set", "mdx_rigid_set",
     "mdx_thermostat_set", "mdx_temperature", "mdx_tune", "mdx_run_batch", "mdx_batch_merged", "mdx_tune_batch", "mdx_surrogate_set",
@@ -102,6 +102,7 @@ def load():
     L.mdx_ewald_real_set.argtypes = [_vp, c, d, d, d]
     L.mdx_pppm_set.argtypes = [_vp, c, c, c, c, d, d, d]
     L.mdx_pppm_info.argtypes = [_vp, C.POINTER(d), C.POINTER(d)]
+    L.mdx_pppm_paths.argtypes = [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.mdx_pppm_set_error.argtypes = [_vp, d, d, c, d]
     L.mdx_pppm_interval_set.argtypes = [_vp, c]
     L.mdx_pppm_mesh_get.argtypes = [_vp, _ip, C.POINTER(c)]
@@ -290,6 +291,13 @@ class Engine:
         e = C.c_double(0)
         self._chk(self._L.mdx_pppm_info(self._h, C.byref(a), C.byref(e)), "mdx_pppm_info")
         return a.value, e.value
+
+    def pppm_paths(self):
+        """Kernels of the last mesh evaluation: (spread, gather), see mdx_pppm_paths."""
+        a = C.c_int(-1)
+        b = C.c_int(-1)
+        self._chk(self._L.mdx_pppm_paths(self._h, C.byref(a), C.byref(b)), "mdx_pppm_paths")
+        return a.value, b.value
 
     def pppm_set_error(self, error, r_cut, order, qqr):
         """FixChargeEwald.setError: the mesh is picked for this RMS force error."""

@@ -1020,8 +1020,7 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags, const Rin
                 c.pinned[0] = h[0];
                 c.pinned[1] = h[1];
                 c.pinned[6] = h[6];
-                MDX_REQUIRE(h[6] == 0, 1,
-                            "an atom's stencil holds over 2^20 candidates: lower the list cutoff or the cell reach");
+                flag7_check(h[6]);
                 overflow[k] = h[0] > c.cap || (c.e3b.on && h[1] > c.e3b.cap);
                 if (c.baro.on) {
                     box_pull(c);
@@ -1279,6 +1278,7 @@ int mdx_system_set(mdx_ctx *ctx, int n, const double *x, const double *v, const 
             }
             c.idToIdx = dalloc<int>(c, n);
             c.xq = dalloc<float4>(c, n);
+            c.meshRec = dalloc<float4>(c, n);
             c.force = dalloc<unsigned long long>(c, 3 * (size_t)n);
             c.posLast = dalloc<double4>(c, n);
         }
@@ -1670,6 +1670,13 @@ int mdx_pppm_info(mdx_ctx *ctx, double *alpha, double *rms_error) {
         pppm_prepare(c);
         if (alpha) *alpha = c.pppm.alpha;
         if (rms_error) *rms_error = c.pppm.err;
+    });
+}
+
+int mdx_pppm_paths(mdx_ctx *ctx, int *spread, int *gather) {
+    return guarded_ro(ctx, [&](Ctx &c) {
+        if (spread) *spread = c.pppm.lastSpread;
+        if (gather) *gather = c.pppm.lastGather;
     });
 }
 
@@ -2498,6 +2505,16 @@ int mdx_option_set(mdx_ctx *ctx, const char *name, int value) {
         else if (k == "build_mode") {
             MDX_REQUIRE(value == 0 || value == 2, MDX_ERR_ARG, "build_mode: 0 or 2 (per-atom teams)");
             c.buildMode = value;
+        }
+        else if (k == "p3m_spread") {
+            MDX_REQUIRE(value >= 0 && value <= 2, MDX_ERR_ARG,
+                        "p3m_spread: 2 owner-computes tiles, 1 LDS tiles with global atomics, 0 global atomics");
+            c.pppm.spreadMode = value;
+        }
+        else if (k == "p3m_gather") {
+            MDX_REQUIRE(value >= 0 && value <= 2, MDX_ERR_ARG,
+                        "p3m_gather: 2 LDS tiles of the field, 1 row teams from the mesh records, 0 row teams from the positions");
+            c.pppm.gatherMode = value;
         }
         else if (k == "gather_team") {
             MDX_REQUIRE(value == 8 || value == 16, MDX_ERR_ARG, "gather_team: 8 or 16 lanes per charged atom");

@@ -162,6 +162,35 @@ struct MeshGeo {
     int periodic[3];
 };
 
+// Nearest P3M mesh point and offset, FP64 like the oracle: the coordinate is
+// wrapped into [lo, lo + L) first, then ngp = int((pos + h/2) / h) and
+// d = pos / h - ngp (FixChargeEwald.cu:178-190).  ngp lies in [0, sz].
+__host__ __device__ __forceinline__ void mesh_point(double x, int a, const MeshGeo &m, int &ngp, float &d) {
+#pragma clang fp contract(off)
+    double img = floor((x - m.lo[a]) * m.invL[a]);
+    double xw = x - img * m.L[a];
+    double p = xw - m.lo[a];
+    ngp = (int)((p + 0.5 * m.h[a]) / m.h[a]);
+    d = (float)(p / m.h[a] - ngp);
+}
+
+// Per charged site, written with the force pass's position refresh: the three
+// offsets d and the nearest mesh point wrapped into [0, sz) per axis, packed
+// 10 bits an axis in w (meshes up to 1024 points an axis).
+__device__ __forceinline__ float4 mesh_record(const double4 &p, const MeshGeo &m) {
+    int g[3];
+    float d[3];
+    mesh_point(p.x, 0, m, g[0], d[0]);
+    mesh_point(p.y, 1, m, g[1], d[1]);
+    mesh_point(p.z, 2, m, g[2], d[2]);
+    unsigned packed = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) packed |= (unsigned)(g[a] >= m.sz[a] ? g[a] - m.sz[a] : g[a]) << (10 * a);
+    return make_float4(d[0], d[1], d[2], __uint_as_float(packed));
+}
+
+__device__ __forceinline__ int rec_point(const float4 &r, int a) { return (int)((__float_as_uint(r.w) >> (10 * a)) & 1023u); }
+
 struct BoxParams {  // fixed while a cell grid and a mesh live
     int periodic[3];
     int nc[3], reach[3];
@@ -240,6 +269,9 @@ __host__ __device__ inline void box_derive(BoxDev &b, const BoxParams &p, const 
 struct PppmState {
     bool on = false;
     int gatherTeam = 8;           // lanes per charged atom in the field gather
+    int spreadMode = 1;           // 2 owner-computes cell blocks, 1 LDS tiles with global atomics, 0 global atomics
+    int gatherMode = 1;           // 2 LDS tiles of the field, 1 row teams from the mesh records, 0 from the positions
+    int lastSpread = -1, lastGather = -1;  // paths the last evaluation took (mdx_pppm_paths)
     int sz[3] = {64, 64, 64};
     int order = 5;
     double rcut = 9.0;
@@ -431,6 +463,7 @@ struct Ctx {
     int cur = 0;
     int *idToIdx = nullptr;
     float4 *xq = nullptr;
+    float4 *meshRec = nullptr;             // per charged site: mesh offsets and point (mesh_record)
     unsigned long long *force = nullptr;   // 3 per atom
     unsigned long long *acc = nullptr;     // kAccTotal: kAccRep replicas of ACC_N slots
     BoxDev *box = nullptr;                 // device copy of the box-dependent launch data
@@ -471,7 +504,8 @@ struct Ctx {
     int *nlCount = nullptr;
     // [0] rebuild flag, [1] max count, [2] e3b max count; captured steps:
     // [3] rebuild gate, [4] rebuilds, [5] thermostat gate, [6] turn;
-    // [7] set when an atom's candidate stream overflows the build table;
+    // [7] bit 0: an atom's candidate stream overflowed the build table, bit 1:
+    // a charged site's P3M stencil left the cells next to its own;
     // [8] barostat gate
     int *dFlags = nullptr;
     int *waveMax = nullptr;       // per-wave longest rows: [0, n + 4) pair list, then the molecule list
@@ -734,6 +768,7 @@ inline double grid_headroom(const Ctx &c) { return c.baro.on ? 0.999 : 1.0; }
 // nlist.hip
 void nlist_setup(Ctx &c);
 void nlist_check(Ctx &c);             // sets dFlags[0] when a rebuild is due
+void flag7_check(int v);              // raises the error dFlags[7] holds, if any
 void nlist_rebuild(Ctx &c, bool forced);  // rebuild if dFlags[0] or forced (host reads flag)
 void refresh_xq(Ctx &c);
 // refresh_xq + zero_forces(c, true) in one launch

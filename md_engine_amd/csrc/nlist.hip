@@ -250,19 +250,24 @@ __device__ __forceinline__ void xq_body(int blk, int nblk, int n, const double4 
 }
 
 // k_xq fused with the clearing of the fixed-point force rows and of the
-// energy / virial accumulators that the force kernels add into.
+// energy / virial accumulators that the force kernels add into; with P3M on
+// (rec set) also each charged site's mesh point and offsets for the spread
+// and the gather of the same force evaluation.
 __device__ __forceinline__ void xq_zero_body(int blk, int nblk, int n, const double4 *__restrict__ pos, const float *__restrict__ q,
                           float4 *__restrict__ xq, const BoxDev *__restrict__ bx,
-                          unsigned long long *__restrict__ force, int nacc, unsigned long long *__restrict__ acc) {
+                          unsigned long long *__restrict__ force, int nacc, unsigned long long *__restrict__ acc,
+                          float4 *__restrict__ rec) {
     const Geometry g = bx->geo;
     int k = blk * blockDim.x + threadIdx.x;
     if (k < nacc) acc[k] = 0ull;
     if (k >= n) return;
     double4 p = pos[k];
-    xq[k] = make_float4((float)(p.x - g.lo[0]), (float)(p.y - g.lo[1]), (float)(p.z - g.lo[2]), q[k]);
+    const float qk = q[k];
+    xq[k] = make_float4((float)(p.x - g.lo[0]), (float)(p.y - g.lo[1]), (float)(p.z - g.lo[2]), qk);
     force[3 * (size_t)k + 0] = 0ull;
     force[3 * (size_t)k + 1] = 0ull;
     force[3 * (size_t)k + 2] = 0ull;
+    if (rec && qk != 0.0f) rec[k] = mesh_record(p, bx->mesh);
 }
 
 // exclusion row of atom i (id myid) in current indices, tags kept
@@ -1217,7 +1222,8 @@ void refresh_xq(Ctx &c) {
 
 void prepare_forces(Ctx &c) {
     const int threads = 256;
-    launch<xq_zero_body, 256>(c, cdiv(std::max(c.n, kAccTotal), threads), 0, c.n, c.pos[c.cur], c.q[c.cur], c.xq, c.box, c.force, kAccTotal, c.acc);
+    launch<xq_zero_body, 256>(c, cdiv(std::max(c.n, kAccTotal), threads), 0, c.n, c.pos[c.cur], c.q[c.cur], c.xq, c.box, c.force, kAccTotal, c.acc,
+                              c.pppm.on && c.hasCharges ? c.meshRec : nullptr);
 }
 
 void nlist_check(Ctx &c) {
@@ -1313,12 +1319,18 @@ static void run_build(Ctx &c, bool sync) {
     c.listValid = true;
 }
 
+void flag7_check(int v) {
+    MDX_REQUIRE(!(v & 1), 1, "an atom's stencil holds over 2^20 candidates: lower the list cutoff or the cell reach");
+    MDX_REQUIRE(!(v & 2), 4,
+                "a charged site's P3M stencil left the cells next to its own (a site moved beyond skin/2 "
+                "between list builds)");
+}
+
 // Grows the pair / molecule rows after an overflow seen in pinned[0..1];
 // returns true when something grew (the caller rebuilds).  The reference
 // reallocates its list at 1.5x (GridGPU.cu:1026-1030).
 bool grow_if_needed(Ctx &c) {
-    MDX_REQUIRE(c.pinned[6] == 0, 1,
-                "an atom's stencil holds over 2^20 candidates: lower the list cutoff or the cell reach");
+    flag7_check(c.pinned[6]);
     bool grew = false;
     if (c.pinned[0] > c.cap) {
         dfree(c, c.nl);

@@ -26,45 +26,73 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 
-// W_p for orders 1/3/5/7 (FixChargeEwald.cu:111-115, 163-172, 220-234)
+// W_p for orders 1/3/5/7 (FixChargeEwald.cu:111-115, 163-172, 220-234) as
+// polynomials in the offset x, coefficients in ascending powers, evaluated by
+// Horner's rule with explicit fused multiply-adds: the stencil index may be a
+// compile-time constant (weights) or a run-time one (weight_at, coefficients
+// copied to LDS), and both give the same bits.
 template <int ORDER>
-__device__ __forceinline__ void weights(float x, float *w) {
-    const float x2 = x * x;
-    if (ORDER == 1) {
-        w[0] = 1.0f;
-    } else if (ORDER == 3) {
-        w[0] = 0.125f - 0.5f * x + 0.5f * x2;
-        w[1] = 0.75f - x2;
-        w[2] = 0.125f + 0.5f * x + 0.5f * x2;
-    } else if (ORDER == 5) {
-        const float x3 = x2 * x, x4 = x2 * x2;
-        w[0] = (1.0f / 384.0f) * (1.0f - 8.0f * x + 24.0f * x2 - 32.0f * x3 + 16.0f * x4);
-        w[1] = (1.0f / 96.0f) * (19.0f - 44.0f * x + 24.0f * x2 + 16.0f * x3 - 16.0f * x4);
-        w[2] = (1.0f / 192.0f) * (115.0f - 120.0f * x2 + 48.0f * x4);
-        w[3] = (1.0f / 96.0f) * (19.0f + 44.0f * x + 24.0f * x2 - 16.0f * x3 - 16.0f * x4);
-        w[4] = (1.0f / 384.0f) * (1.0f + 8.0f * x + 24.0f * x2 + 32.0f * x3 + 16.0f * x4);
-    } else {
-        const float x3 = x2 * x, x4 = x2 * x2, x5 = x3 * x2, x6 = x3 * x3;
-        w[0] = (1.0f / 46080.0f) * (1.0f - 12.0f * x + 60.0f * x2 - 160.0f * x3 + 240.0f * x4 - 192.0f * x5 + 64.0f * x6);
-        w[1] = (1.0f / 23040.0f) * (361.0f - 1416.0f * x + 2220.0f * x2 - 1600.0f * x3 + 240.0f * x4 + 384.0f * x5 - 192.0f * x6);
-        w[2] = (1.0f / 46080.0f) * (10543.0f - 17340.0f * x + 4740.0f * x2 + 6880.0f * x3 - 4080.0f * x4 - 960.0f * x5 + 960.0f * x6);
-        w[3] = (1.0f / 11520.0f) * (5887.0f - 4620.0f * x2 + 1680.0f * x4 - 320.0f * x6);
-        w[4] = (1.0f / 46080.0f) * (10543.0f + 17340.0f * x + 4740.0f * x2 - 6880.0f * x3 - 4080.0f * x4 + 960.0f * x5 + 960.0f * x6);
-        w[5] = (1.0f / 23040.0f) * (361.0f + 1416.0f * x + 2220.0f * x2 + 1600.0f * x3 + 240.0f * x4 - 384.0f * x5 - 192.0f * x6);
-        w[6] = (1.0f / 46080.0f) * (1.0f + 12.0f * x + 60.0f * x2 + 160.0f * x3 + 240.0f * x4 + 192.0f * x5 + 64.0f * x6);
-    }
+struct WCoef;
+template <>
+struct WCoef<1> {
+    static constexpr float c[1][1] = {{1.0f}};
+};
+template <>
+struct WCoef<3> {
+    static constexpr float c[3][3] = {{0.125f, -0.5f, 0.5f}, {0.75f, 0.0f, -1.0f}, {0.125f, 0.5f, 0.5f}};
+};
+#define MDX_W(d, a, b, c_, e, f) {(float)(a / d), (float)(b / d), (float)(c_ / d), (float)(e / d), (float)(f / d)}
+template <>
+struct WCoef<5> {
+    static constexpr float c[5][5] = {MDX_W(384.0, 1.0, -8.0, 24.0, -32.0, 16.0),
+                                      MDX_W(96.0, 19.0, -44.0, 24.0, 16.0, -16.0),
+                                      MDX_W(192.0, 115.0, 0.0, -120.0, 0.0, 48.0),
+                                      MDX_W(96.0, 19.0, 44.0, 24.0, -16.0, -16.0),
+                                      MDX_W(384.0, 1.0, 8.0, 24.0, 32.0, 16.0)};
+};
+#undef MDX_W
+#define MDX_W(d, a, b, c_, e, f, g, h) \
+    {(float)(a / d), (float)(b / d), (float)(c_ / d), (float)(e / d), (float)(f / d), (float)(g / d), (float)(h / d)}
+template <>
+struct WCoef<7> {
+    static constexpr float c[7][7] = {MDX_W(46080.0, 1.0, -12.0, 60.0, -160.0, 240.0, -192.0, 64.0),
+                                      MDX_W(23040.0, 361.0, -1416.0, 2220.0, -1600.0, 240.0, 384.0, -192.0),
+                                      MDX_W(46080.0, 10543.0, -17340.0, 4740.0, 6880.0, -4080.0, -960.0, 960.0),
+                                      MDX_W(11520.0, 5887.0, 0.0, -4620.0, 0.0, 1680.0, 0.0, -320.0),
+                                      MDX_W(46080.0, 10543.0, 17340.0, 4740.0, -6880.0, -4080.0, 960.0, 960.0),
+                                      MDX_W(23040.0, 361.0, 1416.0, 2220.0, 1600.0, 240.0, -384.0, -192.0),
+                                      MDX_W(46080.0, 1.0, 12.0, 60.0, 160.0, 240.0, 192.0, 64.0)};
+};
+#undef MDX_W
+
+template <int ORDER>
+__device__ __forceinline__ float horner(const float *c, float x) {
+    float w = c[ORDER - 1];
+#pragma unroll
+    for (int m = ORDER - 2; m >= 0; --m) w = __builtin_fmaf(w, x, c[m]);
+    return w;
 }
 
+template <int ORDER>
+__device__ __forceinline__ void weights(float x, float *w) {
+#pragma unroll
+    for (int j = 0; j < ORDER; ++j) w[j] = horner<ORDER>(WCoef<ORDER>::c[j], x);
+}
 
-// Nearest mesh point and offset, FP64 like the oracle: the coordinate is
-// wrapped into [lo, lo + L) first, then ngp = int((pos + h/2) / h) and
-// d = pos / h - ngp (FixChargeEwald.cu:178-190).
-__device__ __forceinline__ void mesh_point(double x, int a, const MeshGeo &m, int &ngp, float &d) {
-    double img = floor((x - m.lo[a]) * m.invL[a]);
-    double xw = x - img * m.L[a];
-    double p = xw - m.lo[a];
-    ngp = (int)((p + 0.5 * m.h[a]) / m.h[a]);
-    d = (float)(p / m.h[a] - ngp);
+// weight of run-time stencil index j from the coefficient table in LDS
+template <int ORDER>
+__device__ __forceinline__ float weight_at(const float *lcoef, int j, float x) {
+    return horner<ORDER>(lcoef + j * ORDER, x);
+}
+
+// weight of stencil index k (0 <= k < ORDER) without a dynamically indexed
+// local array (which would live in scratch memory)
+template <int ORDER>
+__device__ __forceinline__ float pick(const float (&w)[ORDER], int k) {
+    float r = w[0];
+#pragma unroll
+    for (int t = 1; t < ORDER; ++t) r = k == t ? w[t] : r;
+    return r;
 }
 
 __device__ __forceinline__ int wrapi(int p, int n) {
@@ -135,7 +163,8 @@ __device__ __forceinline__ void spread_tiled_body(int blk, int nblk, const doubl
                                                       const float *__restrict__ q, const BoxDev *__restrict__ bx,
                                                       float qunit, const int *__restrict__ cellStart,
                                                       const int2 *__restrict__ colXY, TileGeo tg, int nlog,
-                                                      unsigned long long *__restrict__ mesh) {
+                                                      unsigned long long *__restrict__ mesh,
+                                                      const float4 *__restrict__ rec) {
     extern __shared__ unsigned long long tile[];
     const int lb = xcd_block(blk, nlog);
     if (lb >= nlog) return;  // uniform over the workgroup
@@ -161,13 +190,24 @@ __device__ __forceinline__ void spread_tiled_body(int blk, int nblk, const doubl
         const int i = a0 + it / ORDER;
         const int ix = it % ORDER;
         const float qi = q[i];
-        if (qi == 0.0f) continue;
-        const double4 p = pos[i];
         int g[3];
         float d[3];
-        mesh_point(p.x, 0, m, g[0], d[0]);
-        mesh_point(p.y, 1, m, g[1], d[1]);
-        mesh_point(p.z, 2, m, g[2], d[2]);
+        if (rec) {  // the position refresh's mesh point and offsets, loaded with the charge
+            const float4 rc = rec[i];
+            if (qi == 0.0f) continue;
+            g[0] = rec_point(rc, 0);
+            g[1] = rec_point(rc, 1);
+            g[2] = rec_point(rc, 2);
+            d[0] = rc.x;
+            d[1] = rc.y;
+            d[2] = rc.z;
+        } else {
+            if (qi == 0.0f) continue;
+            const double4 p = pos[i];
+            mesh_point(p.x, 0, m, g[0], d[0]);
+            mesh_point(p.y, 1, m, g[1], d[1]);
+            mesh_point(p.z, 2, m, g[2], d[2]);
+        }
         float wx[ORDER], wy[ORDER], wz[ORDER];
         weights<ORDER>(d[0], wx);
         weights<ORDER>(d[1], wy);
@@ -210,6 +250,205 @@ __device__ __forceinline__ void spread_tiled_body(int blk, int nblk, const doubl
         const int kz = k % ez, ky = (k / ez) % ey, kx = k / (ez * ey);
         const int gx = wrapi(ox + kx, m.sz[0]), gy = wrapi(oy + ky, m.sz[1]), gz = wrapi(oz + kz, m.sz[2]);
         atomicAdd(&mesh[((size_t)gx * m.sz[1] + gy) * m.sz[2] + gz], v);
+    }
+}
+
+// v in (-2n, 2n) to [0, n) without an integer division
+__device__ __forceinline__ int wrap_near(int v, int n) {
+    v += v < 0 ? n : 0;
+    v += v < 0 ? n : 0;
+    return v >= n ? v - n : v;
+}
+
+// First mesh point of cell k along an axis of sz points and nc cells,
+// ceil(k sz / nc), for k >= -nc: point p belongs to cell floor(p nc / sz), so
+// the cells partition the mesh by integer arithmetic alone, whatever the box.
+__host__ __device__ __forceinline__ int own_start(int k, int sz, int nc) {
+    return k >= 0 ? (k * sz + nc - 1) / nc : -((-k * sz) / nc);
+}
+
+// Owner-computes spread.  A workgroup owns the mesh points of zs cells of one
+// cell column and sums into LDS every contribution to them: from the charged
+// sites of its own cells and of the cells within reach (one column either
+// side in x and y, rz cells in z), each site's mesh point and offsets taken
+// from the record the position refresh wrote.  A site stays within skin/2 of
+// the cell it was sorted into (the list's own condition for a rebuild), so
+// its stencil lies in the cells within reach: the host checks that from the
+// geometry and every workgroup checks it for its own sites (bit 1 of
+// flags[7] otherwise).  The workgroup then writes its points to the real mesh
+// in FP32: no global atomics and no pass converting and clearing a
+// fixed-point mesh.  The sums are the same 64-bit integers as the other
+// spreads', so the real mesh is the same bit for bit.
+//
+// Pass 1 (each wave takes whole candidate runs, a lane a site): the sites
+// whose stencil meets the owned block are listed with one item per x plane
+// that meets it; pass 2 (a thread an item): the plane's 25 (order 5) points,
+// those outside the block masked.  Candidates are taken kOwnCap at a time in
+// their run order, so crowded cells only add rounds.
+struct OwnGeo {
+    int nc[3];      // cells per axis
+    int sz[3];      // mesh points per axis
+    int zs, nzc;    // z cells per workgroup, workgroups per column
+    int rz;         // z reach in cells (x and y reach one column)
+    int whole;      // zs + 2 rz >= nc[2]: the z range is the whole column
+};
+
+constexpr int kOwnRanges = 18;  // 9 columns x at most 2 z runs
+constexpr int kOwnCap = 1024;   // candidates per round
+constexpr int kOwnZMax = 64;    // z cells per workgroup at most
+
+template <int ORDER, int NT>
+__device__ __forceinline__ void spread_own_body(int blk, int nblk, const float4 *__restrict__ rec,
+                                                    const float *__restrict__ q, const int *__restrict__ cellStart,
+                                                    const int *__restrict__ colRank, const int2 *__restrict__ colXY,
+                                                    OwnGeo og, float qunit, int nlog, float *__restrict__ real,
+                                                    int *__restrict__ flags) {
+    extern __shared__ unsigned long long own[];
+    __shared__ int rS[kOwnRanges], rP[kOwnRanges + 1];  // run start, candidate prefix
+    __shared__ float4 eRec[kOwnCap];
+    __shared__ float eQ[kOwnCap];
+    __shared__ unsigned short items[kOwnCap * ORDER];  // listed site << 3 | x plane
+    __shared__ float lcoef[ORDER * ORDER];
+    __shared__ int nE, nI;
+    const int lb = xcd_block(blk, nlog);
+    if (lb >= nlog) return;  // uniform over the workgroup
+    constexpr int H = (ORDER - 1) / 2;
+    constexpr int NW = NT / 64;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int col = lb / og.nzc, zc = lb % og.nzc;
+    const int ncz = og.nc[2];
+    const int z0 = zc * og.zs, z1 = min(z0 + og.zs, ncz);
+    const int2 cxy = colXY[col];
+    const int X0 = own_start(cxy.x, og.sz[0], og.nc[0]), nx = own_start(cxy.x + 1, og.sz[0], og.nc[0]) - X0;
+    const int Y0 = own_start(cxy.y, og.sz[1], og.nc[1]), ny = own_start(cxy.y + 1, og.sz[1], og.nc[1]) - Y0;
+    const int Z0 = own_start(z0, og.sz[2], ncz), nz = own_start(z1, og.sz[2], ncz) - Z0;
+    const int vol = nx * ny * nz;
+    for (int k = threadIdx.x; k < vol; k += NT) own[k] = 0ull;
+    if (threadIdx.x < ORDER * ORDER) lcoef[threadIdx.x] = WCoef<ORDER>::c[threadIdx.x / ORDER][threadIdx.x % ORDER];
+    __shared__ int rN[kOwnRanges], ownZ[kOwnZMax + 1];
+    if (threadIdx.x < 9) {
+        // candidate runs: 9 columns, each a z range that wraps at one end at most
+        const int t = threadIdx.x;
+        int cx = cxy.x + t / 3 - 1, cy = cxy.y + t % 3 - 1;
+        cx += cx < 0 ? og.nc[0] : (cx >= og.nc[0] ? -og.nc[0] : 0);
+        cy += cy < 0 ? og.nc[1] : (cy >= og.nc[1] ? -og.nc[1] : 0);
+        const int base = colRank[cx * og.nc[1] + cy] * ncz;
+        int s0, e0, s1 = 0, e1 = 0;
+        if (og.whole) {
+            s0 = 0;
+            e0 = ncz;
+        } else {
+            const int zlo = z0 - og.rz, zhi = z1 + og.rz;
+            if (zlo < 0) {
+                s0 = zlo + ncz; e0 = ncz; s1 = 0; e1 = zhi;
+            } else if (zhi > ncz) {
+                s0 = zlo; e0 = ncz; s1 = 0; e1 = zhi - ncz;
+            } else {
+                s0 = zlo; e0 = zhi;
+            }
+        }
+        const int a0 = cellStart[base + s0], b0 = cellStart[base + e0];
+        const int a1 = cellStart[base + s1], b1 = cellStart[base + e1];
+        rS[2 * t] = a0;
+        rN[2 * t] = b0 - a0;
+        rS[2 * t + 1] = a1;
+        rN[2 * t + 1] = b1 - a1;
+    } else if (threadIdx.x - 9 <= z1 - z0) {
+        // the own column's cell starts, for the own sites' cells
+        ownZ[threadIdx.x - 9] = cellStart[col * ncz + z0 + threadIdx.x - 9];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int pre = 0;
+        for (int r = 0; r < kOwnRanges; ++r) {
+            rP[r] = pre;
+            pre += rN[r];
+        }
+        rP[kOwnRanges] = pre;
+    }
+    __syncthreads();
+    const int total = rP[kOwnRanges];
+    // this workgroup's own sites and the x / y span their stencils must stay in
+    const int ownA = ownZ[0], ownB = ownZ[z1 - z0];
+    const int sxl = own_start(cxy.x - 1, og.sz[0], og.nc[0]), sxn = own_start(cxy.x + 2, og.sz[0], og.nc[0]) - sxl;
+    const int syl = own_start(cxy.y - 1, og.sz[1], og.nc[1]), syn = own_start(cxy.y + 2, og.sz[1], og.nc[1]) - syl;
+    for (int w0 = 0; w0 < total; w0 += kOwnCap) {
+        if (threadIdx.x == 0) {
+            nE = 0;
+            nI = 0;
+        }
+        __syncthreads();
+        // pass 1: candidates f in [w0, w0 + kOwnCap) of the flattened runs
+        for (int r = wv; r < kOwnRanges; r += NW) {
+            const int f0 = max(rP[r], w0), f1 = min(rP[r + 1], w0 + kOwnCap);
+            for (int f = f0 + lane; f < f1; f += 64) {
+                const int i = rS[r] + f - rP[r];
+                const float4 rc = rec[i];
+                const float qi = q[i];
+                const int gx = rec_point(rc, 0), gy = rec_point(rc, 1), gz = rec_point(rc, 2);
+                unsigned xm = 0;
+                bool ym = false, zm = false;
+#pragma unroll
+                for (int k = 0; k < ORDER; ++k) {
+                    xm |= (wrap_near(gx - H + k - X0, og.sz[0]) < nx ? 1u : 0u) << k;
+                    ym = ym || wrap_near(gy - H + k - Y0, og.sz[1]) < ny;
+                    zm = zm || wrap_near(gz - H + k - Z0, og.sz[2]) < nz;
+                }
+                if (xm && ym && zm && qi != 0.0f) {
+                    const int e = atomicAdd(&nE, 1);
+                    eRec[e] = rc;
+                    eQ[e] = qi;
+                    int it = atomicAdd(&nI, __popc(xm));
+#pragma unroll
+                    for (int k = 0; k < ORDER; ++k)
+                        if (xm >> k & 1u) items[it++] = (unsigned short)(e << 3 | k);
+                }
+                if (i >= ownA && i < ownB) {
+                    // the site's own cell in z, then its stencil against the cells within reach
+                    int zo = z0;
+                    for (int z = 1; z < z1 - z0; ++z) zo += ownZ[z] <= i ? 1 : 0;
+                    const int szl = own_start(zo - og.rz, og.sz[2], ncz);
+                    const int szn = own_start(zo + 1 + og.rz, og.sz[2], ncz) - szl;
+                    const bool inx = wrap_near(gx - H - sxl, og.sz[0]) + ORDER <= sxn;
+                    const bool iny = wrap_near(gy - H - syl, og.sz[1]) + ORDER <= syn;
+                    const bool inz = og.whole || wrap_near(gz - H - szl, og.sz[2]) + ORDER <= szn;
+                    if (!(inx && iny && inz)) atomicOr(&flags[7], 2);
+                }
+            }
+        }
+        __syncthreads();
+        // pass 2: one listed x plane per thread
+        const int ni = nI;
+        for (int t = threadIdx.x; t < ni; t += NT) {
+            const int e = items[t] >> 3, ix = items[t] & 7;
+            const float4 rc = eRec[e];
+            const float a = qunit * eQ[e] * weight_at<ORDER>(lcoef, ix, rc.x);
+            const int lx = wrap_near(rec_point(rc, 0) - H + ix - X0, og.sz[0]);
+            const int yb = rec_point(rc, 1) - H - Y0, zb = rec_point(rc, 2) - H - Z0;
+            float wy[ORDER], wz[ORDER];
+            weights<ORDER>(rc.y, wy);
+            weights<ORDER>(rc.z, wz);
+#pragma unroll
+            for (int iy = 0; iy < ORDER; ++iy) {
+                const int ly = wrap_near(yb + iy, og.sz[1]);
+                const float b = a * wy[iy];
+                const int rb = (lx * ny + ly) * nz;
+#pragma unroll
+                for (int iz = 0; iz < ORDER; ++iz) {
+                    const int lz = wrap_near(zb + iz, og.sz[2]);
+                    if (ly < ny && lz < nz) {
+                        const long long v = __double2ll_rn((double)(b * wz[iz]) * kMeshScale);
+                        atomicAdd(&own[rb + lz], (unsigned long long)v);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (int k = threadIdx.x; k < vol; k += NT) {
+        const int kz = k % nz, ky = (k / nz) % ny, kx = k / (nz * ny);
+        real[((size_t)(X0 + kx) * og.sz[1] + (Y0 + ky)) * og.sz[2] + (Z0 + kz)] =
+            (float)((double)(long long)own[k] * kMeshInv);
     }
 }
 
@@ -421,58 +660,29 @@ __device__ __forceinline__ void green_body(int blk, int nblk, const int *__restr
     G[idx] = g;
 }
 
-// E_hat_d = i k_d G rho_hat (E_field_cu), k_d := 0 on the d-th Nyquist plane.
-__device__ __forceinline__ void efield_body(int blk, int nblk, const BoxDev *__restrict__ bx, int szh, const float *__restrict__ G,
-                         const float2 *__restrict__ rho, float2 *__restrict__ E) {
-    const MeshGeo m = bx->mesh;
-    int idx = blk * blockDim.x + threadIdx.x;
-    int total = m.sz[0] * m.sz[1] * szh;
-    if (idx >= total) return;
-    int iz = idx % szh, iy = (idx / szh) % m.sz[1], ix = idx / (szh * m.sz[1]);
-    float kx = (2 * ix == m.sz[0]) ? 0.0f : (float)kcomp(ix, m.sz[0], m.L[0]);
-    float ky = (2 * iy == m.sz[1]) ? 0.0f : (float)kcomp(iy, m.sz[1], m.L[1]);
-    float kz = (2 * iz == m.sz[2]) ? 0.0f : (float)kcomp(iz, m.sz[2], m.L[2]);
-    float g = G[idx];
-    float2 r = rho[idx];
-    float gr = g * r.x, gi = g * r.y;
-    E[idx] = make_float2(-kx * gi, kx * gr);
-    E[total + idx] = make_float2(-ky * gi, ky * gr);
-    E[2 * total + idx] = make_float2(-kz * gi, kz * gr);
-}
-
 // Energy 0.5 sum |rho_hat|^2 G / V over the full spectrum and the k-space
 // virial (virials_cu): half-spectrum entries with 0 < iz < sz/2 count twice.
-// gate (virial for a barostat inside a captured step): nothing on steps
-// whose barostat gate is 0.
-__device__ __forceinline__ void kspace_ev_body(int blk, int nblk, const int *__restrict__ gate, const BoxDev *__restrict__ bx,
-                                                   int szh, const float *__restrict__ G,
-                                                   const float2 *__restrict__ rho,
-                                                   unsigned long long *__restrict__ acc) {
-    if (gate && *gate == 0) return;  // the whole grid returns together
-    const MeshGeo m = bx->mesh;
-    const double alpha = bx->alpha, invV = 1.0 / bx->vol;
-    __shared__ double wred[4][7];
-    int idx = blk * blockDim.x + threadIdx.x;
-    int total = m.sz[0] * m.sz[1] * szh;
-    double vals[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (idx < total) {
-        int iz = idx % szh, iy = (idx / szh) % m.sz[1], ix = idx / (szh * m.sz[1]);
-        double w = (iz == 0 || 2 * iz == m.sz[2]) ? 1.0 : 2.0;
-        double k[3] = {kcomp(ix, m.sz[0], m.L[0]), kcomp(iy, m.sz[1], m.L[1]), kcomp(iz, m.sz[2], m.L[2])};
-        double kl = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
-        float2 r = rho[idx];
-        double E = w * ((double)r.x * r.x + (double)r.y * r.y) * G[idx];
-        vals[0] = 0.5 * E * invV;
-        if (kl != 0.0) {
-            double diff = -2.0 * (1.0 / kl + 0.25 / (alpha * alpha));
-            vals[1] = 0.5 * invV * (1.0 + diff * k[0] * k[0]) * E;
-            vals[2] = 0.5 * invV * (1.0 + diff * k[1] * k[1]) * E;
-            vals[3] = 0.5 * invV * (1.0 + diff * k[2] * k[2]) * E;
-            vals[4] = 0.5 * invV * diff * k[0] * k[1] * E;
-            vals[5] = 0.5 * invV * diff * k[0] * k[2] * E;
-            vals[6] = 0.5 * invV * diff * k[1] * k[2] * E;
-        }
+__device__ __forceinline__ void kspace_ev_point(const MeshGeo &m, double alpha, double invV, int ix, int iy, int iz,
+                                                float g, float2 r, double vals[7]) {
+    double w = (iz == 0 || 2 * iz == m.sz[2]) ? 1.0 : 2.0;
+    double k[3] = {kcomp(ix, m.sz[0], m.L[0]), kcomp(iy, m.sz[1], m.L[1]), kcomp(iz, m.sz[2], m.L[2])};
+    double kl = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+    double E = w * ((double)r.x * r.x + (double)r.y * r.y) * g;
+    vals[0] = 0.5 * E * invV;
+    if (kl != 0.0) {
+        double diff = -2.0 * (1.0 / kl + 0.25 / (alpha * alpha));
+        vals[1] = 0.5 * invV * (1.0 + diff * k[0] * k[0]) * E;
+        vals[2] = 0.5 * invV * (1.0 + diff * k[1] * k[1]) * E;
+        vals[3] = 0.5 * invV * (1.0 + diff * k[2] * k[2]) * E;
+        vals[4] = 0.5 * invV * diff * k[0] * k[1] * E;
+        vals[5] = 0.5 * invV * diff * k[0] * k[2] * E;
+        vals[6] = 0.5 * invV * diff * k[1] * k[2] * E;
     }
+}
+
+// workgroup sums of the seven values, one fixed-point add per value
+__device__ __forceinline__ void kspace_ev_reduce(const double vals[7], unsigned long long *__restrict__ acc) {
+    __shared__ double wred[4][7];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int s = 0; s < 7; ++s) {
@@ -487,14 +697,32 @@ __device__ __forceinline__ void kspace_ev_body(int blk, int nblk, const int *__r
     }
 }
 
-// weight of stencil index k (0 <= k < ORDER) without a dynamically indexed
-// local array (which would live in scratch memory)
-template <int ORDER>
-__device__ __forceinline__ float pick(const float (&w)[ORDER], int k) {
-    float r = w[0];
-#pragma unroll
-    for (int t = 1; t < ORDER; ++t) r = k == t ? w[t] : r;
-    return r;
+// E_hat_d = i k_d G rho_hat (E_field_cu), k_d := 0 on the d-th Nyquist plane.
+// With ev set (and the gate, if any, open) the same pass also sums the
+// k-space energy and virial (kspace_ev_point, one read of G and rho_hat for
+// both); ev and the gate are uniform over the grid.
+__device__ __forceinline__ void efield_body(int blk, int nblk, const BoxDev *__restrict__ bx, int szh, const float *__restrict__ G,
+                         const float2 *__restrict__ rho, float2 *__restrict__ E, int ev, const int *__restrict__ gate,
+                         unsigned long long *__restrict__ acc) {
+    const MeshGeo m = bx->mesh;
+    int idx = blk * blockDim.x + threadIdx.x;
+    int total = m.sz[0] * m.sz[1] * szh;
+    const bool sums = ev && (!gate || *gate != 0);
+    double vals[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (idx < total) {
+        int iz = idx % szh, iy = (idx / szh) % m.sz[1], ix = idx / (szh * m.sz[1]);
+        float kx = (2 * ix == m.sz[0]) ? 0.0f : (float)kcomp(ix, m.sz[0], m.L[0]);
+        float ky = (2 * iy == m.sz[1]) ? 0.0f : (float)kcomp(iy, m.sz[1], m.L[1]);
+        float kz = (2 * iz == m.sz[2]) ? 0.0f : (float)kcomp(iz, m.sz[2], m.L[2]);
+        float g = G[idx];
+        float2 r = rho[idx];
+        float gr = g * r.x, gi = g * r.y;
+        E[idx] = make_float2(-kx * gi, kx * gr);
+        E[total + idx] = make_float2(-ky * gi, ky * gr);
+        E[2 * total + idx] = make_float2(-kz * gi, kz * gr);
+        if (sums) kspace_ev_point(m, bx->alpha, 1.0 / bx->vol, ix, iy, iz, g, r, vals);
+    }
+    if (sums) kspace_ev_reduce(vals, acc);
 }
 
 // Field interpolation (Ewald_long_range_forces_order_*_cu): a team of T
@@ -515,7 +743,8 @@ __device__ __forceinline__ void gather_body(int blk, int nblk, int n, const doub
                                                 const float *__restrict__ q, const BoxDev *__restrict__ bx,
                                                 const float4 *__restrict__ E4,
                                                 unsigned long long *__restrict__ force,
-                                                const int *__restrict__ ids, long long *__restrict__ stored) {
+                                                const int *__restrict__ ids, long long *__restrict__ stored,
+                                                const float4 *__restrict__ rec) {
     const int nlog = (int)(((long long)n * T + blockDim.x - 1) / blockDim.x);
     const int lb = xcd_block(blk, nlog);
     if (lb >= nlog) return;  // uniform over the workgroup
@@ -526,13 +755,24 @@ __device__ __forceinline__ void gather_body(int blk, int nblk, int n, const doub
     const int tl = threadIdx.x & (T - 1);
     if (i >= n) return;  // whole teams leave together
     const float qi = q[i];
-    if (qi == 0.0f) return;
-    const double4 p = pos[i];
     int g[3];
     float d[3];
-    mesh_point(p.x, 0, m, g[0], d[0]);
-    mesh_point(p.y, 1, m, g[1], d[1]);
-    mesh_point(p.z, 2, m, g[2], d[2]);
+    if (rec) {  // the position refresh's mesh point and offsets, loaded with the charge
+        const float4 rc = rec[i];
+        if (qi == 0.0f) return;
+        g[0] = rec_point(rc, 0);
+        g[1] = rec_point(rc, 1);
+        g[2] = rec_point(rc, 2);
+        d[0] = rc.x;
+        d[1] = rc.y;
+        d[2] = rc.z;
+    } else {
+        if (qi == 0.0f) return;
+        const double4 p = pos[i];
+        mesh_point(p.x, 0, m, g[0], d[0]);
+        mesh_point(p.y, 1, m, g[1], d[1]);
+        mesh_point(p.z, 2, m, g[2], d[2]);
+    }
     float wx[ORDER], wy[ORDER], wz[ORDER];
     weights<ORDER>(d[0], wx);
     weights<ORDER>(d[1], wy);
@@ -590,6 +830,138 @@ __device__ __forceinline__ void gather_body(int blk, int nblk, int n, const doub
     }
 }
 
+// LDS-tiled field interpolation.  A workgroup takes the charged sites of one
+// tile of the LDS spread (a cell column's chunk of zs cells): it loads the
+// box of mesh points their stencils can reach - every component from the
+// planar output of the inverse transform, z rows read contiguously - into
+// LDS once, then a team of T lanes per site sums the stencil rows from LDS
+// (rows r = lane, lane + T, ..., as the row-team gather).  A site whose
+// stencil leaves the box (it drifted further than the margin) reads the rows
+// from global memory instead.  The site's charge and record are loaded before
+// the box, so both round trips overlap.
+template <int ORDER, int T, int NT>
+__device__ __forceinline__ void gather_tile_body(int blk, int nblk, const float4 *__restrict__ rec,
+                                                     const float *__restrict__ q, const BoxDev *__restrict__ bx,
+                                                     const int *__restrict__ cellStart, const int2 *__restrict__ colXY,
+                                                     TileGeo tg, int nlog, const float *__restrict__ E,
+                                                     unsigned long long *__restrict__ force,
+                                                     const int *__restrict__ ids, long long *__restrict__ stored) {
+    extern __shared__ float fb[];  // 3 planes of the box
+    __shared__ float lcoef[ORDER * ORDER];
+    const int lb = xcd_block(blk, nlog);
+    if (lb >= nlog) return;  // uniform over the workgroup
+    constexpr int H = (ORDER - 1) / 2;
+    constexpr int NW = NT / 64;
+    constexpr int SPR = NT / T;  // sites per round
+    const MeshGeo m = bx->mesh;
+    const int sx = m.sz[0], sy = m.sz[1], sz = m.sz[2];
+    const int col = lb / tg.nzc, zc = lb % tg.nzc;
+    const int z0 = zc * tg.zs, z1 = min(z0 + tg.zs, tg.ncz);
+    const int a0 = cellStart[col * tg.ncz + z0], a1 = cellStart[col * tg.ncz + z1];
+    const int tl = threadIdx.x & (T - 1);
+    // first round's site: charge and record in flight with the box loads
+    int s = threadIdx.x / T;
+    float qi = 0.0f;
+    float4 rc = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (a0 + s < a1) {
+        qi = q[a0 + s];
+        rc = rec[a0 + s];
+    }
+    const int2 cxy = colXY[col];
+    const double margin = bx->tileMargin;
+    const int ox = tile_origin(cxy.x * bx->tileCellW[0] - margin, m.h[0]) - H;
+    const int oy = tile_origin(cxy.y * bx->tileCellW[1] - margin, m.h[1]) - H;
+    const int oz = tile_origin(z0 * bx->tileCellW[2] - margin, m.h[2]) - H;
+    const int ex = tg.ext[0], ey = tg.ext[1], ez = tg.ext[2];
+    const int vol = ex * ey * ez;
+    const size_t ng = (size_t)sx * sy * sz;
+    if (threadIdx.x < ORDER * ORDER) lcoef[threadIdx.x] = WCoef<ORDER>::c[threadIdx.x / ORDER][threadIdx.x % ORDER];
+    {
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        for (int row = wv; row < ex * ey; row += NW) {
+            const int kx = row / ey, ky = row - kx * ey;
+            const size_t rb = ((size_t)wrap_near(ox + kx, sx) * sy + wrap_near(oy + ky, sy)) * sz;
+            for (int kz = lane; kz < ez; kz += 64) {
+                const size_t g = rb + wrap_near(oz + kz, sz);
+                const int k = row * ez + kz;
+                fb[k] = E[g];
+                fb[vol + k] = E[ng + g];
+                fb[2 * vol + k] = E[2 * ng + g];
+            }
+        }
+    }
+    __syncthreads();
+    const float scale = bx->gatherScale;
+    constexpr int NRW = (ORDER * ORDER + T - 1) / T;
+    for (;;) {
+        const int i = a0 + s;
+        if (i < a1 && qi != 0.0f) {
+            const int gx = rec_point(rc, 0), gy = rec_point(rc, 1), gz = rec_point(rc, 2);
+            const int rx = wrap_near(gx - H - ox, sx), ry = wrap_near(gy - H - oy, sy), rz = wrap_near(gz - H - oz, sz);
+            const bool inside = rx + ORDER <= ex && ry + ORDER <= ey && rz + ORDER <= ez;
+            float wz[ORDER];
+            weights<ORDER>(rc.z, wz);
+            float fx = 0.f, fy = 0.f, fz = 0.f;
+#pragma unroll
+            for (int rr = 0; rr < NRW; ++rr) {
+                const int r = tl + rr * T;
+                if (r < ORDER * ORDER) {
+                    const int ix = r / ORDER, iy = r - ix * ORDER;
+                    const float wxy = weight_at<ORDER>(lcoef, ix, rc.x) * weight_at<ORDER>(lcoef, iy, rc.y);
+                    float e0[ORDER], e1[ORDER], e2[ORDER];
+                    if (inside) {
+                        const int b = ((rx + ix) * ey + (ry + iy)) * ez + rz;
+#pragma unroll
+                        for (int iz = 0; iz < ORDER; ++iz) {
+                            e0[iz] = fb[b + iz];
+                            e1[iz] = fb[vol + b + iz];
+                            e2[iz] = fb[2 * vol + b + iz];
+                        }
+                    } else {
+                        const size_t b = ((size_t)wrap_near(gx - H + ix, sx) * sy + wrap_near(gy - H + iy, sy)) * sz;
+#pragma unroll
+                        for (int iz = 0; iz < ORDER; ++iz) {
+                            const size_t g = b + wrap_near(gz - H + iz, sz);
+                            e0[iz] = E[g];
+                            e1[iz] = E[ng + g];
+                            e2[iz] = E[2 * ng + g];
+                        }
+                    }
+#pragma unroll
+                    for (int iz = 0; iz < ORDER; ++iz) {
+                        const float w = wxy * wz[iz];
+                        fx += w * e0[iz];
+                        fy += w * e1[iz];
+                        fz += w * e2[iz];
+                    }
+                }
+            }
+            fx = team_sum<T>(fx);
+            fy = team_sum<T>(fy);
+            fz = team_sum<T>(fz);
+            if (tl == 0) {
+                const float sc = scale * qi;
+                const long long f0 = to_fx(sc * fx), f1 = to_fx(sc * fy), f2 = to_fx(sc * fz);
+                atomicAdd(&force[3 * i + 0], (unsigned long long)f0);
+                atomicAdd(&force[3 * i + 1], (unsigned long long)f1);
+                atomicAdd(&force[3 * i + 2], (unsigned long long)f2);
+                if (stored) {
+                    const int id = ids[i];
+                    stored[3 * id + 0] = f0;
+                    stored[3 * id + 1] = f1;
+                    stored[3 * id + 2] = f2;
+                }
+            }
+        }
+        s += SPR;
+        if (a0 + s - (int)(threadIdx.x / T) >= a1) break;  // uniform: the round's first site is past the end
+        if (a0 + s < a1) {
+            qi = q[a0 + s];
+            rc = rec[a0 + s];
+        }
+    }
+}
+
 // the stored mesh forces by id added to the current order's force rows
 // (applyStoredForces, FixChargeEwald.cu:804-814)
 __device__ __forceinline__ void apply_stored_body(int blk, int nblk, int n, const int *__restrict__ ids, const long long *__restrict__ stored,
@@ -608,7 +980,7 @@ __device__ __forceinline__ void apply_stored_body(int blk, int nblk, int n, cons
 // Tile shape for the LDS spread: z-chunks of zs cells so that the tile stays
 // within 48 KB of LDS.  Returns false (plain spread) when no cell list is
 // valid, the box is not periodic, or even one cell does not fit.
-bool tiled_geometry(const Ctx &c, TileGeo &tg) {
+bool tiled_geometry(const Ctx &c, TileGeo &tg, int budget = 48 * 1024 / 8) {
     if (!c.listValid || c.ncell == 0 || !c.colXY || c.mdxDisableTiledSpread) return false;
     for (int a = 0; a < 3; ++a)
         if (!c.geo.periodic[a]) return false;
@@ -625,7 +997,6 @@ bool tiled_geometry(const Ctx &c, TileGeo &tg) {
     tg.ext[0] = extent(0, 1);
     tg.ext[1] = extent(1, 1);
     if (tg.ext[0] >= c.pppm.sz[0] || tg.ext[1] >= c.pppm.sz[1]) return false;
-    const int budget = 48 * 1024 / 8;
     int zs = 0;
     for (int z = 1; z <= tg.ncz; ++z) {
         int ez = extent(2, z);
@@ -636,6 +1007,59 @@ bool tiled_geometry(const Ctx &c, TileGeo &tg) {
     tg.zs = zs;
     tg.nzc = (tg.ncz + zs - 1) / zs;
     tg.ext[2] = extent(2, zs);
+    return true;
+}
+
+// Geometry of the owner-computes spread, or false (another spread) when no
+// cell list is valid, an axis is open, a mesh axis is beyond the records'
+// 10 bits, x or y has fewer than 3 cells, or a site within the margin of its
+// cell could reach past the cells next to it in x or y or past 8 cells in z.
+// The margin is skin/2 with a quarter's slack (a barostat changes h and the
+// margin's share of it while the grid lives).
+bool own_geometry(const Ctx &c, OwnGeo &og, size_t &shm) {
+    if (!c.listValid || c.ncell == 0 || !c.colXY || !c.meshRec || !c.hasCharges || c.mdxDisableTiledSpread)
+        return false;
+    const int order = c.pppm.order, H = (order - 1) / 2;
+    for (int a = 0; a < 3; ++a) {
+        if (!c.geo.periodic[a] || c.pppm.sz[a] > 1024 || c.pppm.sz[a] < order + 1) return false;
+        og.nc[a] = c.nc[a];
+        og.sz[a] = c.pppm.sz[a];
+    }
+    if (og.nc[0] < 3 || og.nc[1] < 3) return false;
+    // reach: every cell's sites keep their stencils within [cell - R, cell + R]
+    auto reaches = [&](int a, int R) {
+        const int nc = og.nc[a], sz = og.sz[a];
+        const double h = c.geo.L[a] / sz, r = (double)sz / nc;
+        const double mu = (0.5 * c.skin + 1e-6 * c.geo.L[0]) / h * 1.25 + 0.01;
+        for (int k = 0; k < nc; ++k) {
+            const int glo = (int)std::floor(k * r - mu + 0.5), ghi = (int)std::floor((k + 1) * r + mu + 0.5);
+            if (glo - H < own_start(k - R, sz, nc) || ghi + H >= own_start(k + 1 + R, sz, nc)) return false;
+        }
+        return true;
+    };
+    if (!reaches(0, 1) || !reaches(1, 1)) return false;
+    og.rz = 0;
+    for (int R = 1; R <= 8 && R <= og.nc[2] && !og.rz; ++R)
+        if (reaches(2, R)) og.rz = R;
+    if (!og.rz) return false;
+    // z cells per workgroup: about 512 workgroups, the owned points within 48 KB of LDS
+    auto most = [&](int a, int span) {
+        int m = 0;
+        for (int k = 0; k < og.nc[a]; ++k) {
+            const int e = std::min(k + span, og.nc[a]);
+            m = std::max(m, own_start(e, og.sz[a], og.nc[a]) - own_start(k, og.sz[a], og.nc[a]));
+        }
+        return m;
+    };
+    const int mx = most(0, 1), my = most(1, 1);
+    const int ncol = og.nc[0] * og.nc[1];
+    int zs = std::max(1, std::min({og.nc[2], ncol * og.nc[2] / 512, kOwnZMax}));
+    while (zs > 1 && (size_t)mx * my * most(2, zs) * 8 > 48 * 1024) --zs;
+    if ((size_t)mx * my * most(2, zs) * 8 > 48 * 1024) return false;
+    og.zs = zs;
+    og.nzc = (og.nc[2] + zs - 1) / zs;
+    og.whole = zs + 2 * og.rz >= og.nc[2] ? 1 : 0;
+    shm = (size_t)mx * my * most(2, zs) * 8;
     return true;
 }
 
@@ -814,14 +1238,29 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
     {
     ProfScope ps(c, PK_SPREAD);
     TileGeo tg;
-    if (tiled_geometry(c, tg)) {
+    OwnGeo og;
+    size_t oshm = 0;
+    if (P.spreadMode == 2 && own_geometry(c, og, oshm)) {
+        P.lastSpread = 2;
+        const int nlog = c.nc[0] * c.nc[1] * og.nzc;
+        switch (P.order) {
+            case 1: launch<spread_own_body<1, 256>, 256>(c, xcd_grid(nlog), oshm, c.meshRec, c.q[cur], c.cellStart, c.colRank, c.colXY, og, qunit, nlog, P.real, c.dFlags); break;
+            case 3: launch<spread_own_body<3, 256>, 256>(c, xcd_grid(nlog), oshm, c.meshRec, c.q[cur], c.cellStart, c.colRank, c.colXY, og, qunit, nlog, P.real, c.dFlags); break;
+            case 5: launch<spread_own_body<5, 256>, 256>(c, xcd_grid(nlog), oshm, c.meshRec, c.q[cur], c.cellStart, c.colRank, c.colXY, og, qunit, nlog, P.real, c.dFlags); break;
+            default: launch<spread_own_body<7, 256>, 256>(c, xcd_grid(nlog), oshm, c.meshRec, c.q[cur], c.cellStart, c.colRank, c.colXY, og, qunit, nlog, P.real, c.dFlags); break;
+        }
+    } else {
+    const bool tiled = P.spreadMode >= 1 && tiled_geometry(c, tg);
+    P.lastSpread = tiled ? 1 : 0;
+    const float4 *rec = c.meshRec && c.hasCharges ? c.meshRec : nullptr;
+    if (tiled) {
         const int nblk = c.nc[0] * c.nc[1] * tg.nzc;
         const size_t shm = sizeof(unsigned long long) * tg.ext[0] * tg.ext[1] * tg.ext[2];
         switch (P.order) {
-            case 1: launch<spread_tiled_body<1>, 256>(c, xcd_grid(nblk), shm, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh); break;
-            case 3: launch<spread_tiled_body<3>, 256>(c, xcd_grid(nblk), shm, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh); break;
-            case 5: launch<spread_tiled_body<5>, 256>(c, xcd_grid(nblk), shm, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh); break;
-            default: launch<spread_tiled_body<7>, 256>(c, xcd_grid(nblk), shm, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh); break;
+            case 1: launch<spread_tiled_body<1>, 256>(c, xcd_grid(nblk), shm, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh, rec); break;
+            case 3: launch<spread_tiled_body<3>, 256>(c, xcd_grid(nblk), shm, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh, rec); break;
+            case 5: launch<spread_tiled_body<5>, 256>(c, xcd_grid(nblk), shm, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh, rec); break;
+            default: launch<spread_tiled_body<7>, 256>(c, xcd_grid(nblk), shm, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh, rec); break;
         }
     } else
     switch (P.order) {
@@ -832,6 +1271,7 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
     }
     launch<mesh_to_real_body, 256>(c, cdiv(ng, threads), 0, ng, P.mesh, P.real);
     }
+    }
     {
     ProfScope ps(c, PK_FFT);
     stream_op(c, [&P](hipStream_t s) {
@@ -840,14 +1280,13 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
                     "hipfftExecR2C failed");
     }, 1);
     }
-    if (energy || virial) {
-        // a trace-only request (a barostat's) is skipped on steps it does not serve
-        const int *gate = c.vtrace ? baro_gate(c) : nullptr;
-        launch<kspace_ev_body, 256>(c, cdiv(nh, threads), 0, gate, c.box, szh, P.green, P.rhoHat, c.acc);
-    }
     {
+        // energy and virial in the field pass; a trace-only request (a
+        // barostat's) is skipped on steps it does not serve
         ProfScope ps(c, PK_EFIELD);
-        launch<efield_body, 256>(c, cdiv(nh, threads), 0, c.box, szh, P.green, P.rhoHat, P.eHat);
+        const int *gate = c.vtrace ? baro_gate(c) : nullptr;
+        launch<efield_body, 256>(c, cdiv(nh, threads), 0, c.box, szh, P.green, P.rhoHat, P.eHat,
+                                 (energy || virial) ? 1 : 0, gate, c.acc);
     }
     {
         ProfScope ps(c, PK_FFT);
@@ -861,11 +1300,31 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
     const int na = c.listValid ? c.nCharged : c.n;
     ProfScope ps(c, PK_GATHER);
     if (na == 0) return;
-    launch<field_pack_body, 256>(c, cdiv(ng, threads), 0, ng, P.eReal, P.eField4);
-    const int gridx = xcd_grid(cdiv((long long)na * P.gatherTeam, threads));
     long long *stored = P.interval > 1 ? P.stored : nullptr;
+    TileGeo gg;
+    if (P.gatherMode == 2 && c.meshRec && c.hasCharges && tiled_geometry(c, gg, 48 * 1024 / 12)) {
+        P.lastGather = 2;
+        const int nlog = c.nc[0] * c.nc[1] * gg.nzc;
+        const size_t shm = sizeof(float) * 3 * gg.ext[0] * gg.ext[1] * gg.ext[2];
+#define MDX_GTILE(O) \
+    launch<gather_tile_body<O, 8, 256>, 256>(c, xcd_grid(nlog), shm, c.meshRec, c.q[cur], c.box, c.cellStart, c.colXY, gg, nlog, P.eReal, c.force, c.id[cur], stored)
+        switch (P.order) {
+            case 1: MDX_GTILE(1); break;
+            case 3: MDX_GTILE(3); break;
+            case 5: MDX_GTILE(5); break;
+            default: MDX_GTILE(7); break;
+        }
+#undef MDX_GTILE
+        return;
+    }
+    launch<field_pack_body, 256>(c, cdiv(ng, threads), 0, ng, P.eReal, P.eField4);
+    // 1 (and 2 where the tiles do not fit): the mesh points and offsets from
+    // the position refresh's records; 0: computed here from the positions
+    const float4 *grec = P.gatherMode >= 1 && c.meshRec && c.hasCharges ? c.meshRec : nullptr;
+    P.lastGather = grec ? 1 : 0;
+    const int gridx = xcd_grid(cdiv((long long)na * P.gatherTeam, threads));
 #define MDX_GATHER(O, T) \
-    launch<gather_body<O, T>, 256>(c, gridx, 0, na, c.pos[cur], c.q[cur], c.box, P.eField4, c.force, c.id[cur], stored)
+    launch<gather_body<O, T>, 256>(c, gridx, 0, na, c.pos[cur], c.q[cur], c.box, P.eField4, c.force, c.id[cur], stored, grec)
     const bool wide = P.gatherTeam == 16;
     switch (P.order) {
         case 1: if (wide) MDX_GATHER(1, 16); else MDX_GATHER(1, 8); break;
@@ -910,7 +1369,7 @@ void ring_pppm(Ctx &c0, const RingSpec &r) {
     }
     {
         ProfScope ps(c0, PK_EFIELD);
-        launch<efield_body, 256>(c0, cdiv(nh, threads), 0, c0.box, szh, P.green, P.rhoHat, P.eHat);
+        launch<efield_body, 256>(c0, cdiv(nh, threads), 0, c0.box, szh, P.green, P.rhoHat, P.eHat, 0, nullptr, c0.acc);
     }
     {
         ProfScope ps(c0, PK_FFT);
@@ -925,10 +1384,10 @@ void ring_pppm(Ctx &c0, const RingSpec &r) {
     const dim3 grid(xcd_grid(cdiv((long long)r.natoms * 8, threads)));
     unsigned long long *cf = (unsigned long long *)r.cf;
     switch (P.order) {
-        case 1: launch<gather_body<1, 8>, 256>(c0, grid.x, 0, r.natoms, r.cen, r.q, c0.box, P.eField4, cf, nullptr, nullptr); break;
-        case 3: launch<gather_body<3, 8>, 256>(c0, grid.x, 0, r.natoms, r.cen, r.q, c0.box, P.eField4, cf, nullptr, nullptr); break;
-        case 5: launch<gather_body<5, 8>, 256>(c0, grid.x, 0, r.natoms, r.cen, r.q, c0.box, P.eField4, cf, nullptr, nullptr); break;
-        default: launch<gather_body<7, 8>, 256>(c0, grid.x, 0, r.natoms, r.cen, r.q, c0.box, P.eField4, cf, nullptr, nullptr); break;
+        case 1: launch<gather_body<1, 8>, 256>(c0, grid.x, 0, r.natoms, r.cen, r.q, c0.box, P.eField4, cf, nullptr, nullptr, nullptr); break;
+        case 3: launch<gather_body<3, 8>, 256>(c0, grid.x, 0, r.natoms, r.cen, r.q, c0.box, P.eField4, cf, nullptr, nullptr, nullptr); break;
+        case 5: launch<gather_body<5, 8>, 256>(c0, grid.x, 0, r.natoms, r.cen, r.q, c0.box, P.eField4, cf, nullptr, nullptr, nullptr); break;
+        default: launch<gather_body<7, 8>, 256>(c0, grid.x, 0, r.natoms, r.cen, r.q, c0.box, P.eField4, cf, nullptr, nullptr, nullptr); break;
     }
     ring_apply(r, c0);
 }

@@ -271,19 +271,46 @@ def test_water_pair_and_pppm(orc, gpu_engine_factory):
     assert rel_rmse(f, ref["fp"] + ref["fk"]) <= FORCE_RMSE
 
 
-def test_tiled_spread_bit_identical(gpu_engine_factory):
-    """LDS-tiled and direct global-atomic spreads give the same bits."""
-    say("tiled vs direct spread")
+@pytest.mark.parametrize("sz", [(32, 32, 32), (32, 32, 64)])
+def test_spread_paths_bit_identical(gpu_engine_factory, sz):
+    """Owner-computes cell blocks, LDS tiles with global atomics and direct
+    global atomics sum the same 64-bit integers: the same force bits."""
+    say(f"spread paths, mesh {sz}")
     from md_engine_amd._lib import PPPM
 
     box = systems.tip4p2005_box(512, seed=12)
-    e = water_engine(gpu_engine_factory, box)
-    e.compute(PPPM)
-    f_tiled = e.forces()
-    e.option("tiled_spread", 0)
-    e.compute(PPPM)
-    f_direct = e.forces()
-    assert np.array_equal(f_tiled, f_direct)
+    e = water_engine(gpu_engine_factory, box, sz=sz)
+    out = {}
+    for mode in (2, 1, 0):
+        e.option("p3m_spread", mode)
+        e.compute(PPPM)
+        assert e.pppm_paths()[0] == mode
+        out[mode] = e.forces()
+    assert np.array_equal(out[2], out[1])
+    assert np.array_equal(out[2], out[0])
+
+
+@pytest.mark.parametrize("order", [1, 3, 5, 7])
+def test_gather_paths(orc, gpu_engine_factory, order):
+    """The gathers (LDS tiles of the field, row teams from the per-site mesh
+    records, row teams from the positions) take the same weights, field
+    points and summation order: the same bits, and the oracle's forces."""
+    say(f"gather paths, order {order}")
+    from md_engine_amd._lib import PPPM
+
+    box = systems.tip4p2005_box(512, seed=14)
+    ref = oracle_water(orc, box, order=order)
+    e = water_engine(gpu_engine_factory, box, order=order)
+    out = {}
+    for mode in (2, 1, 0):
+        e.option("p3m_gather", mode)
+        e.compute(PPPM)
+        # order 7's field box does not fit the LDS here: the records' row teams
+        assert e.pppm_paths() == (1, 1 if (mode == 2 and order == 7) else mode)
+        out[mode] = e.forces()
+    assert np.array_equal(out[2], out[1])
+    assert np.array_equal(out[1], out[0])
+    assert rel_rmse(out[1], ref["fk"]) <= FORCE_RMSE
 
 
 @pytest.mark.parametrize("order", [1, 3, 7])
