@@ -251,21 +251,22 @@ int mdx_teams_set(mdx_ctx *ctx, int build_team, int pair_team, int e3b_team);
 /* Learned E3B three-body surrogate.  No counterpart in the reference (its
  * FixE3B, src/Fixes/FixE3B.cu, only has the analytic three-body term); a
  * model replaces that term under MDX_SURROGATE while the two-body E3B term
- * stays analytic.  Model: per-molecule descriptors (5 radial functions
- * exp(-kappa r) fc(r), kappa = k3 (0.7, 0.8, 0.9, 1.0, 2.0), on the H..O
- * distances in three channels: accepted,
- * H1-donated, H2-donated; three pair products of the E3B bond function over
- * each neighbour: both own H to one O, both H of one neighbour to the own O,
- * mutual donation), network input of 21 features (the channels, d1 + d2,
- * d1 * d2, the pair products and three products of the E3B-basis channels),
- * MLP 21 -> 64 -> 64 -> 1 with SiLU on BF16 MFMA tiles, forces from its
- * hand-written backward pass.  params: kSurNParam = 5677 floats packed as
- * W1[21][64] b1[64] W2[64][64] b2[64] w3[64] b3 mu[21] sd[21] escale eshift.
+ * stays analytic.  Model: per-molecule descriptors (2 radial functions
+ * exp(-kappa r) fc(r), kappa = k3 and 2 k3 - the E3B bond function f and
+ * f^2 up to the switch - on the H..O distances in three channels: accepted,
+ * H1-donated, H2-donated; three pair products of f over each neighbour:
+ * both own H to one O, both H of one neighbour to the own O, mutual
+ * donation), network input of 12 features (the channels, d1 + d2, d1 * d2,
+ * the pair products and three products of the f channels; the E3B
+ * three-body energy of a molecule is linear in them), MLP 12 -> 64 -> 64 -> 1
+ * with SiLU on BF16 MFMA tiles, forces from its hand-written backward pass.
+ * params: kSurNParam = 5083 floats packed as W1[12][64] b1[64] W2[64][64]
+ * b2[64] w3[64] b3 mu[12] sd[12] escale eshift.
  * record: appends the current frame (raw channels + the E3B three-body energy
  * owned by each molecule, the training target) to the device sample set;
  * fit: Adam on the per-molecule energy MSE (normalize: set mu, sd, escale,
  * eshift from the samples first); eval: per-molecule energies (nmol) and raw
- * descriptor channels (nmol x 18), either may be NULL. */
+ * descriptor channels (nmol x 9), either may be NULL. */
 int mdx_surrogate_set(mdx_ctx *ctx, const float *params, int nparams);
 int mdx_surrogate_get(mdx_ctx *ctx, float *params, int nparams);
 int mdx_surrogate_record(mdx_ctx *ctx, int reset, int capacity);
@@ -284,8 +285,8 @@ int mdx_surrogate_judge_stats(mdx_ctx *ctx, int64_t *checks, double *last_rmse, 
 /* Gradient of the training loss (mean squared error of the standardised
  * per-molecule energy) over one mini-batch of recorded samples, rows
  * (bstart + s) * mul + add mod nsamp for s < bsize, as mdx_surrogate_fit
- * computes it before its Adam update: grad[0 .. 5633) for W1 b1 W2 b2 w3 b3
- * in the packed layout, grad[5633] the batch loss (n >= 5634).  For checking
+ * computes it before its Adam update: grad[0 .. 5057) for W1 b1 W2 b2 w3 b3
+ * in the packed layout, grad[5057] the batch loss (n >= 5058).  For checking
  * the hand-written backward pass against an FP64 restatement. */
 int mdx_surrogate_grad(mdx_ctx *ctx, int bstart, int bsize, uint32_t mul, uint32_t add, float *grad, int n);
 /* E3B three-body energy owned by each molecule (donor of type A, centre of

@@ -21,7 +21,7 @@ PPPM_PER_BEAD = 8192  # ring polymers: per-bead P3M instead of the centroid cont
 # learned three-body model: SUR_NB radial channels, SUR_NF network inputs,
 # SUR_NF-64-64-1 network; trained parameters W1 b1 W2 b2 w3 b3, then the fixed
 # input mean and scale and the energy scale and shift
-SUR_NB = 5
+SUR_NB = 2
 SUR_NF = 3 * SUR_NB + 6
 SUR_NH = 64
 NTRAIN = SUR_NF * SUR_NH + SUR_NH + SUR_NH * SUR_NH + SUR_NH + SUR_NH + 1

@@ -355,9 +355,9 @@ struct BarostatState {
 // learned E3B three-body surrogate (surrogate.hip)
 // surrogate sizes: radial basis functions, raw channels per molecule (acc,
 // d1, d2 per basis + three pair products), network inputs, hidden width
-constexpr int kSurNB = 5, kSurNR = 3 * kSurNB + 3, kSurNF = 3 * kSurNB + 6, kSurNH = 64;
+constexpr int kSurNB = 2, kSurNR = 3 * kSurNB + 3, kSurNF = 3 * kSurNB + 6, kSurNH = 64;
 constexpr int kSurNFP = (kSurNF + 15) / 16 * 16;  // network inputs padded to whole MFMA k-blocks
-constexpr int kSurNW = 20;  // row stride of the per-molecule weights (16-byte rows)
+constexpr int kSurNW = (kSurNR + 3) / 4 * 4;  // row stride of the per-molecule weights (16-byte rows)
 constexpr int kSurNParam = kSurNF * kSurNH + kSurNH + kSurNH * kSurNH + kSurNH + kSurNH + 1 + 2 * kSurNF + 2;
 
 struct SurrogateState {

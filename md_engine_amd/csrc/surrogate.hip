@@ -1,13 +1,14 @@
 // Learned E3B three-body surrogate (no counterpart in the reference, which
 // only has the analytic E3B of src/Evaluators/ThreeBodyE3B.h): per-molecule
-// descriptors from the E3B molecule list, a 48 -> 64 -> 64 -> 1 SiLU MLP on
+// descriptors from the E3B molecule list, an NF -> 64 -> 64 -> 1 SiLU MLP on
 // BF16 MFMA tiles (FP32 accumulation) with its hand-written backward pass for
 // forces, and on-device training (per-molecule energy regression, Adam).
 //
-// Raw channels of molecule i for 14 radial functions
-// g_k(r) = exp(-kappa_k r) fc(r), kappa_k = k3 (0.7 + 0.1 k)  (fc: the E3B
-// switching polynomial RS..RF; basis KE = 3 is the E3B bond function f,
-// basis K2 = 13 has kappa = 2 k3):
+// Raw channels of molecule i for NB radial functions
+// g_k(r) = exp(-kappa_k r) fc(r) (fc: the E3B switching polynomial RS..RF),
+// kappa_k = k3 (1 - 0.1 (KE - k)) for k <= KE = NB - 2, so basis KE is the
+// E3B bond function f, and kappa = 2 k3 for basis K2 = NB - 1 (f^2 up to the
+// switch):
 //   acc_k = sum_{j != i, H of j} g_k(|H - O_i|),
 //   d1_k  = sum_{j != i} g_k(|H1_i - O_j|), d2_k likewise for H2_i,
 //   pAA = sum_j f(H1_i - O_j) f(H2_i - O_j), pBB = sum_j f(H1_j - O_i) f(H2_j - O_i),
@@ -28,15 +29,15 @@ constexpr int NB = kSurNB, NR = kSurNR, NF = kSurNF, NH = kSurNH, NW = kSurNW;
 // padding has zero weights and zero inputs, so it changes no result)
 constexpr int NFP = kSurNFP, KB = NFP / 16;
 static_assert(NW >= NR && NW % 4 == 0, "weight rows hold the raw channels in float4 units");
-constexpr int KE = 3;       // basis with kappa = k3: the E3B hydrogen-bond function
+constexpr int KE = NB - 2;  // basis with kappa = k3: the E3B hydrogen-bond function
 constexpr int K2 = NB - 1;  // basis with kappa = 2 k3: f^2 up to the switch
 static_assert(K2 == KE + 1, "the last channel squares the E3B one");
 // raw channel offsets: acc, d1, d2 (NB each), then pAA, pBB, pAB
 constexpr int RD1 = NB, RD2 = 2 * NB, RAA = 3 * NB, RBB = 3 * NB + 1, RAB = 3 * NB + 2;
-// radial basis g_k(r) = exp(-kappa_k r) fc(r), kappa_k = k3 (0.7 + 0.1 k) for
-// k <= KE and kappa = 2 k3 for the last channel (k3 = 1.907 / A, switching
-// 5.0-5.2 A)
-constexpr float RS_C = 5.0f, RF_C = 5.2f, KAPPA0 = 0.7f * 1.907f, DKAPPA = 0.1f * 1.907f;
+// radial basis g_k(r) = exp(-kappa_k r) fc(r), kappa_k = k3 (1 - 0.1 (KE - k))
+// for k <= KE and kappa = 2 k3 for the last channel (k3 = 1.907 / A,
+// switching 5.0-5.2 A)
+constexpr float RS_C = 5.0f, RF_C = 5.2f, KAPPA0 = (1.0f - 0.1f * KE) * 1.907f, DKAPPA = 0.1f * 1.907f;
 
 // parameter offsets in the packed model (oracle/surrogate.py layout)
 constexpr int OW1 = 0, OB1 = OW1 + NF * NH, OW2 = OB1 + NH, OB2 = OW2 + NH * NH, OW3 = OB2 + NH,

@@ -7,18 +7,18 @@ The surrogate has no counterpart in the reference: it is a learned
 replacement for the E3B three-body energy of FixE3B (src/Fixes/FixE3B.cu,
 src/Evaluators/ThreeBodyE3B.h).  Its definition, shared with the HIP kernels:
 
-* raw channels per molecule i (18): for 5 radial functions
-  g_k(r) = exp(-KAPPA_k r) fc(r), KAPPA_k = k3 (0.7 + 0.1 k) for k <= 3 (basis
-  KE = 3 is the E3B hydrogen-bond function f itself) and KAPPA_4 = 2 k3
-  (basis K2 = 4, f^2 up to the switch), fc the E3B switching polynomial
-  between RS_C = 5.0 and RF_C = 5.2,
+* raw channels per molecule i (3 NB + 3): for NB radial functions
+  g_k(r) = exp(-KAPPA_k r) fc(r), KAPPA_k = k3 (1 - 0.1 (KE - k)) for
+  k <= KE = NB - 2 (basis KE is the E3B hydrogen-bond function f itself) and
+  KAPPA = 2 k3 for basis K2 = NB - 1 (f^2 up to the switch), fc the E3B
+  switching polynomial between RS_C = 5.0 and RF_C = 5.2,
     acc_k = sum_{j != i} sum_{H of j} g_k(|H - O_i|)   (hydrogens donated to O_i)
     d1_k  = sum_{j != i} g_k(|H1_i - O_j|),  d2_k likewise for H2_i,
   and three pair-product channels with f = g_KE over the same neighbours j,
     pAA = sum_j f(H1_i - O_j) f(H2_i - O_j)      (both my H towards one O_j)
     pBB = sum_j f(H1_j - O_i) f(H2_j - O_i)      (both H of j towards my O)
     pAB = sum_j D_ij D_ji,  D_ij = f(H1_i - O_j) + f(H2_i - O_j);
-* network input F (21) = [acc, d1 + d2, d1 * d2, pAA, pBB, pAB,
+* network input F (3 NB + 6) = [acc, d1 + d2, d1 * d2, pAA, pBB, pAB,
   acc_KE (d1 + d2)_KE, acc_KE^2, (d1 + d2)_KE^2] (symmetric in H1 <-> H2);
 * x = (F - mu) / sd; h1 = silu(x W1 + b1); h2 = silu(h1 W2 + b2);
   e = h2 . w3 + b3; E_i = escale e + eshift; E = sum_i E_i;
@@ -44,13 +44,13 @@ from __future__ import annotations
 
 import numpy as np
 
-NB = 5
-KE = 3    # basis with kappa = k3 (the E3B function)
-K2 = 4    # basis with kappa = 2 k3
+NB = 2
+KE = NB - 2    # basis with kappa = k3 (the E3B function)
+K2 = NB - 1    # basis with kappa = 2 k3
 NR = 3 * NB + 3
 NF = 3 * NB + 6
 NH = 64
-KAPPA = 1.907 * np.array([0.7, 0.8, 0.9, 1.0, 2.0])
+KAPPA = 1.907 * np.array([1.0 - 0.1 * (KE - k) for k in range(K2)] + [2.0])
 RS_C, RF_C = 5.0, 5.2
 NPARAM = NF * NH + NH + NH * NH + NH + NH + 1 + NF + NF + 2
 
