@@ -194,14 +194,34 @@ void compute(Ctx &c, int flags) {
     c.vtrace = (flags & kFlagVirialTrace) && !(flags & MDX_VIRIAL) && !energy;
     if (c.flex.on) flex_place_m(c);  // the M sites follow O and H before any force term
     prepare_forces(c);
+    const bool mesh = (flags & MDX_PPPM) && c.pppm.on;
+    // In a captured step the P3M chain (mostly latency-bound kernels) runs
+    // on the side stream beside the pair and E3B kernels: all three start
+    // from the refreshed positions and add into the fixed-point force and
+    // accumulator buffers, whose integer sums do not depend on the order.
+    const bool fork = mesh && c.capture && !c.rec && c.side && c.meshStream;
+    if (fork) {
+        MDX_HIP(hipEventRecord(c.evFork, c.stream));
+        MDX_HIP(hipStreamWaitEvent(c.side, c.evFork, 0));
+        std::swap(c.stream, c.side);
+        try {
+            pppm_step(c, energy, virial);
+        } catch (...) {
+            std::swap(c.stream, c.side);
+            throw;
+        }
+        std::swap(c.stream, c.side);
+        MDX_HIP(hipEventRecord(c.evJoin, c.side));
+    }
     if ((flags & MDX_PAIR)) pair_forces(c, energy, c.vtrace ? false : virial);
-    if ((flags & MDX_PPPM) && c.pppm.on) pppm_step(c, energy, virial);
+    if (mesh && !fork) pppm_step(c, energy, virial);
     if ((flags & MDX_SURROGATE) && c.e3b.on) {
         MDX_REQUIRE(c.sur.on, MDX_ERR_STATE, "MDX_SURROGATE needs a model (mdx_surrogate_set)");
         surrogate_forces(c, energy, virial);  // the two-body term stays analytic, in the same launch
     } else if ((flags & MDX_E3B) && c.e3b.on) {
         e3b_forces(c, energy, c.vtrace ? false : virial);
     }
+    if (fork) MDX_HIP(hipStreamWaitEvent(c.stream, c.evJoin, 0));
     if (c.flex.on) {
         if (flags & MDX_BONDED) flex_forces(c, energy, virial);
         if (!c.deferMSpread) flex_spread_m(c);  // every term's M force goes onto O and H before the kick
@@ -782,6 +802,12 @@ hipGraphExec_t capture_steps(const std::vector<Ctx *> &cs, int steps, int fflags
         if (merged) return merged;
     }
     Ctx &c0 = *cs[0];
+    for (Ctx *c : cs)
+        if (c->pppm.on && c->meshStream && !c->side) {  // default priority, like the main stream
+            MDX_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+            MDX_HIP(hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming));
+            MDX_HIP(hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming));
+        }
     std::vector<hipEvent_t> evs(cs.size() * 2, nullptr);
     for (auto &e : evs) MDX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     std::vector<bool> prof(cs.size());
@@ -1249,6 +1275,9 @@ int mdx_destroy(mdx_ctx *ctx) {
     c.eventPool.clear();
     for (void *p : c.allocs) hipFree(p);
     c.allocs.clear();
+    if (c.evFork) hipEventDestroy(c.evFork);
+    if (c.evJoin) hipEventDestroy(c.evJoin);
+    if (c.side) hipStreamDestroy(c.side);
     if (c.stream) hipStreamDestroy(c.stream);
     delete ctx;
     return MDX_OK;
@@ -2505,6 +2534,17 @@ int mdx_option_set(mdx_ctx *ctx, const char *name, int value) {
         else if (k == "build_mode") {
             MDX_REQUIRE(value == 0 || value == 2, MDX_ERR_ARG, "build_mode: 0 or 2 (per-atom teams)");
             c.buildMode = value;
+        }
+        else if (k == "mesh_stream") {
+            // captured steps: 1 (default) the P3M chain on a second stream beside the pair and E3B kernels
+            c.meshStream = value != 0;
+            if (c.graphs.big || c.graphs.one) {
+                MDX_HIP(hipStreamSynchronize(c.stream));
+                if (c.graphs.big) hipGraphExecDestroy(c.graphs.big);
+                if (c.graphs.one) hipGraphExecDestroy(c.graphs.one);
+                c.graphs.keep.release();
+                c.graphs = Ctx::GraphCache{};
+            }
         }
         else if (k == "p3m_spread") {
             MDX_REQUIRE(value >= 0 && value <= 2, MDX_ERR_ARG,

@@ -450,6 +450,11 @@ struct Ctx {
     bool fuseCheck = false;      // captured steps: rigid first half gives the skin check's maxima
     bool deferMSpread = false;   // compute leaves the M-site forces for a later flex_spread_m (ring contraction)
     hipStream_t stream = nullptr;
+    // second stream of captured steps: the P3M chain runs beside the pair and
+    // E3B kernels (fork after the position refresh, join before the kick)
+    hipStream_t side = nullptr;
+    hipEvent_t evFork = nullptr, evJoin = nullptr;
+    bool meshStream = true;       // option "mesh_stream"
     std::string err;
 
     // system

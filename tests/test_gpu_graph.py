@@ -187,3 +187,30 @@ def test_batch_tuner_keeps_launches_merged(orc, gpu_engine_factory):
         d -= b["L"] * np.rint(d / b["L"])
         assert np.allclose(np.linalg.norm(d, axis=1), G["dOH"], atol=1e-8)
         e.close()
+
+
+@pytest.mark.parametrize("barostat", [False, True])
+def test_mesh_stream_matches_one_stream(orc, gpu_engine_factory, barostat):
+    """The P3M chain on the side stream of a captured step (beside the pair
+    and E3B kernels) gives the one-stream trajectory bit for bit: every term
+    adds into fixed-point buffers."""
+    say(f"mesh stream vs one stream, barostat {barostat}")
+    box = systems.tip4p2005_box(343, seed=35, min_sep=1.8)
+    v = start_velocities(orc, box, seed=9)
+    out = []
+    for side in (1, 0):
+        e, flags = water_md_engine(gpu_engine_factory, box, v, rc=8.0, skin=1.0, sz=(32, 32, 32), order=5,
+                                   e3b=True, dt=2.0)
+        if barostat:
+            e.barostat(True, 1.0, 500.0, 1, mvv_to_eng=systems.REAL_MVV_TO_ENG)
+        e.option("graph", 1)
+        e.option("mesh_stream", side)
+        e.run(60, flags)
+        x, vv = e.get_system()
+        out.append((x, vv, e.get_box()[1], e.nlist_stats()["rebuilds"]))
+        e.close()
+    for x, vv, L, nb in out[1:]:
+        assert np.array_equal(x, out[0][0])
+        assert np.array_equal(vv, out[0][1])
+        assert np.array_equal(L, out[0][2])
+        assert nb == out[0][3]
