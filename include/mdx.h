@@ -343,10 +343,12 @@ int mdx_sync(mdx_ctx *ctx);
  * to the global mesh directly (bit-identical meshes); "p3m_spread" 1 (default)
  * LDS tiles with one global atomic per tile point, 2 owner-computes cell
  * blocks writing the real mesh directly (no global atomics), 0 global
- * atomics only (bit-identical meshes); "p3m_gather" 1 (default) row teams
- * from the per-site mesh records of the position refresh, 2 the field box of
- * each cell-column tile staged in LDS, 0 row teams computing the mesh point
- * from the positions (identical forces); "graph" 1 (default)
+ * atomics only (bit-identical meshes); "p3m_gather" 2 (default) the field
+ * box of each cell-column tile staged in LDS, 1 row teams from the per-site
+ * mesh records of the position refresh, 0 row teams computing the mesh point
+ * from the positions (identical forces); "mesh_stream" 1 (default) runs the
+ * P3M chain of a captured step on a second stream beside the pair and E3B
+ * kernels, 0 on one stream (identical trajectories); "graph" 1 (default)
  * runs mdx_run as captured 8-step graphs, 0 eagerly; "build_mode" 0 or 2
  * (both the per-atom team build); "cell_reach_xy" 1..2 and "cell_reach_z"
  * 1..16: cell edges >= rlist / reach per axis (identical lists, finer cells

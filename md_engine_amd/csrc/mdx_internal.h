@@ -270,7 +270,7 @@ struct PppmState {
     bool on = false;
     int gatherTeam = 8;           // lanes per charged atom in the field gather
     int spreadMode = 1;           // 2 owner-computes cell blocks, 1 LDS tiles with global atomics, 0 global atomics
-    int gatherMode = 1;           // 2 LDS tiles of the field, 1 row teams from the mesh records, 0 from the positions
+    int gatherMode = 2;           // 2 LDS tiles of the field, 1 row teams from the mesh records, 0 from the positions
     int lastSpread = -1, lastGather = -1;  // paths the last evaluation took (mdx_pppm_paths)
     int sz[3] = {64, 64, 64};
     int order = 5;

@@ -832,9 +832,10 @@ __device__ __forceinline__ void gather_body(int blk, int nblk, int n, const doub
 
 // LDS-tiled field interpolation.  A workgroup takes the charged sites of one
 // tile of the LDS spread (a cell column's chunk of zs cells): it loads the
-// box of mesh points their stencils can reach - every component from the
-// planar output of the inverse transform, z rows read contiguously - into
-// LDS once, then a team of T lanes per site sums the stencil rows from LDS
+// box of mesh points their stencils can reach - the field interleaved per
+// mesh point, one 16-byte load per point with every lane of the workgroup
+// busy, split into three LDS planes - once, then a team of T lanes per site
+// sums the stencil rows from LDS
 // (rows r = lane, lane + T, ..., as the row-team gather).  A site whose
 // stencil leaves the box (it drifted further than the margin) reads the rows
 // from global memory instead.  The site's charge and record are loaded before
@@ -844,6 +845,7 @@ __device__ __forceinline__ void gather_tile_body(int blk, int nblk, const float4
                                                      const float *__restrict__ q, const BoxDev *__restrict__ bx,
                                                      const int *__restrict__ cellStart, const int2 *__restrict__ colXY,
                                                      TileGeo tg, int nlog, const float *__restrict__ E,
+                                                     const float4 *__restrict__ E4,
                                                      unsigned long long *__restrict__ force,
                                                      const int *__restrict__ ids, long long *__restrict__ stored) {
     extern __shared__ float fb[];  // 3 planes of the box
@@ -851,7 +853,6 @@ __device__ __forceinline__ void gather_tile_body(int blk, int nblk, const float4
     const int lb = xcd_block(blk, nlog);
     if (lb >= nlog) return;  // uniform over the workgroup
     constexpr int H = (ORDER - 1) / 2;
-    constexpr int NW = NT / 64;
     constexpr int SPR = NT / T;  // sites per round
     const MeshGeo m = bx->mesh;
     const int sx = m.sz[0], sy = m.sz[1], sz = m.sz[2];
@@ -876,19 +877,13 @@ __device__ __forceinline__ void gather_tile_body(int blk, int nblk, const float4
     const int vol = ex * ey * ez;
     const size_t ng = (size_t)sx * sy * sz;
     if (threadIdx.x < ORDER * ORDER) lcoef[threadIdx.x] = WCoef<ORDER>::c[threadIdx.x / ORDER][threadIdx.x % ORDER];
-    {
-        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-        for (int row = wv; row < ex * ey; row += NW) {
-            const int kx = row / ey, ky = row - kx * ey;
-            const size_t rb = ((size_t)wrap_near(ox + kx, sx) * sy + wrap_near(oy + ky, sy)) * sz;
-            for (int kz = lane; kz < ez; kz += 64) {
-                const size_t g = rb + wrap_near(oz + kz, sz);
-                const int k = row * ez + kz;
-                fb[k] = E[g];
-                fb[vol + k] = E[ng + g];
-                fb[2 * vol + k] = E[2 * ng + g];
-            }
-        }
+    for (int k = threadIdx.x; k < vol; k += NT) {
+        const int row = k / ez, kz = k - row * ez;
+        const int kx = row / ey, ky = row - kx * ey;
+        const float4 v = E4[((size_t)wrap_near(ox + kx, sx) * sy + wrap_near(oy + ky, sy)) * sz + wrap_near(oz + kz, sz)];
+        fb[k] = v.x;
+        fb[vol + k] = v.y;
+        fb[2 * vol + k] = v.z;
     }
     __syncthreads();
     const float scale = bx->gatherScale;
@@ -1304,10 +1299,11 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
     TileGeo gg;
     if (P.gatherMode == 2 && c.meshRec && c.hasCharges && tiled_geometry(c, gg, 48 * 1024 / 12)) {
         P.lastGather = 2;
+        launch<field_pack_body, 256>(c, cdiv(ng, threads), 0, ng, P.eReal, P.eField4);
         const int nlog = c.nc[0] * c.nc[1] * gg.nzc;
         const size_t shm = sizeof(float) * 3 * gg.ext[0] * gg.ext[1] * gg.ext[2];
 #define MDX_GTILE(O) \
-    launch<gather_tile_body<O, 8, 256>, 256>(c, xcd_grid(nlog), shm, c.meshRec, c.q[cur], c.box, c.cellStart, c.colXY, gg, nlog, P.eReal, c.force, c.id[cur], stored)
+    launch<gather_tile_body<O, 8, 256>, 256>(c, xcd_grid(nlog), shm, c.meshRec, c.q[cur], c.box, c.cellStart, c.colXY, gg, nlog, P.eReal, P.eField4, c.force, c.id[cur], stored)
         switch (P.order) {
             case 1: MDX_GTILE(1); break;
             case 3: MDX_GTILE(3); break;
