@@ -344,7 +344,10 @@ int mdx_sync(mdx_ctx *ctx);
  * LDS tiles with one global atomic per tile point, 2 owner-computes cell
  * blocks writing the real mesh directly (no global atomics), 0 global
  * atomics only (bit-identical meshes); "p3m_gather" 2 (default) the field
- * box of each cell-column tile staged in LDS, 1 row teams from the per-site
+ * box of each cell-column tile staged in LDS ("gather_tile_kb" 8..64 KB of
+ * LDS a tile, default 48; "gather_tile_team" 4, 8 (default) or 16 lanes a
+ * site; "gather_tile_planar" 1 (default) loads the three planar field
+ * components, 0 interleaves them first), 1 row teams from the per-site
  * mesh records of the position refresh, 0 row teams computing the mesh point
  * from the positions (identical forces); "mesh_stream" 1 (default) runs the
  * P3M chain of a captured step on a second stream beside the pair and E3B

@@ -2556,6 +2556,17 @@ int mdx_option_set(mdx_ctx *ctx, const char *name, int value) {
                         "p3m_gather: 2 LDS tiles of the field, 1 row teams from the mesh records, 0 row teams from the positions");
             c.pppm.gatherMode = value;
         }
+        else if (k == "gather_tile_kb") {
+            MDX_REQUIRE(value >= 8 && value <= 64, MDX_ERR_ARG, "gather_tile_kb: 8..64 KB of LDS per tile");
+            c.pppm.tileKB = value;
+        }
+        else if (k == "gather_tile_team") {
+            MDX_REQUIRE(value == 4 || value == 8 || value == 16, MDX_ERR_ARG, "gather_tile_team: 4, 8 or 16");
+            c.pppm.tileTeam = value;
+        }
+        else if (k == "gather_tile_planar") {
+            c.pppm.tilePlanar = value != 0;
+        }
         else if (k == "gather_team") {
             MDX_REQUIRE(value == 8 || value == 16, MDX_ERR_ARG, "gather_team: 8 or 16 lanes per charged atom");
             c.pppm.gatherTeam = value;

@@ -271,6 +271,9 @@ struct PppmState {
     int gatherTeam = 8;           // lanes per charged atom in the field gather
     int spreadMode = 1;           // 2 owner-computes cell blocks, 1 LDS tiles with global atomics, 0 global atomics
     int gatherMode = 2;           // 2 LDS tiles of the field, 1 row teams from the mesh records, 0 from the positions
+    int tileKB = 48;              // LDS budget of a gather tile's field box (KB)
+    int tileTeam = 8;             // lanes per charged site in the tile gather
+    int tilePlanar = 1;           // 1: the tile gather loads the three planar components (no interleaving pass)
     int lastSpread = -1, lastGather = -1;  // paths the last evaluation took (mdx_pppm_paths)
     int sz[3] = {64, 64, 64};
     int order = 5;

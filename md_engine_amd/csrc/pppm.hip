@@ -832,10 +832,11 @@ __device__ __forceinline__ void gather_body(int blk, int nblk, int n, const doub
 
 // LDS-tiled field interpolation.  A workgroup takes the charged sites of one
 // tile of the LDS spread (a cell column's chunk of zs cells): it loads the
-// box of mesh points their stencils can reach - the field interleaved per
-// mesh point, one 16-byte load per point with every lane of the workgroup
-// busy, split into three LDS planes - once, then a team of T lanes per site
-// sums the stencil rows from LDS
+// box of mesh points their stencils can reach - the three components from
+// the planar output of the inverse transform (or the field interleaved per
+// mesh point, one 16-byte load a point), every lane of the workgroup busy,
+// into three LDS planes - once, then a team of T lanes per site sums the
+// stencil rows from LDS
 // (rows r = lane, lane + T, ..., as the row-team gather).  A site whose
 // stencil leaves the box (it drifted further than the margin) reads the rows
 // from global memory instead.  The site's charge and record are loaded before
@@ -880,10 +881,17 @@ __device__ __forceinline__ void gather_tile_body(int blk, int nblk, const float4
     for (int k = threadIdx.x; k < vol; k += NT) {
         const int row = k / ez, kz = k - row * ez;
         const int kx = row / ey, ky = row - kx * ey;
-        const float4 v = E4[((size_t)wrap_near(ox + kx, sx) * sy + wrap_near(oy + ky, sy)) * sz + wrap_near(oz + kz, sz)];
-        fb[k] = v.x;
-        fb[vol + k] = v.y;
-        fb[2 * vol + k] = v.z;
+        const size_t g = ((size_t)wrap_near(ox + kx, sx) * sy + wrap_near(oy + ky, sy)) * sz + wrap_near(oz + kz, sz);
+        if (E4) {
+            const float4 v = E4[g];
+            fb[k] = v.x;
+            fb[vol + k] = v.y;
+            fb[2 * vol + k] = v.z;
+        } else {
+            fb[k] = E[g];
+            fb[vol + k] = E[ng + g];
+            fb[2 * vol + k] = E[2 * ng + g];
+        }
     }
     __syncthreads();
     const float scale = bx->gatherScale;
@@ -1297,19 +1305,26 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
     if (na == 0) return;
     long long *stored = P.interval > 1 ? P.stored : nullptr;
     TileGeo gg;
-    if (P.gatherMode == 2 && c.meshRec && c.hasCharges && tiled_geometry(c, gg, 48 * 1024 / 12)) {
+    if (P.gatherMode == 2 && c.meshRec && c.hasCharges && tiled_geometry(c, gg, P.tileKB * 1024 / 12)) {
         P.lastGather = 2;
-        launch<field_pack_body, 256>(c, cdiv(ng, threads), 0, ng, P.eReal, P.eField4);
+        const float4 *e4 = nullptr;
+        if (!P.tilePlanar) {
+            launch<field_pack_body, 256>(c, cdiv(ng, threads), 0, ng, P.eReal, P.eField4);
+            e4 = P.eField4;
+        }
         const int nlog = c.nc[0] * c.nc[1] * gg.nzc;
         const size_t shm = sizeof(float) * 3 * gg.ext[0] * gg.ext[1] * gg.ext[2];
-#define MDX_GTILE(O) \
-    launch<gather_tile_body<O, 8, 256>, 256>(c, xcd_grid(nlog), shm, c.meshRec, c.q[cur], c.box, c.cellStart, c.colXY, gg, nlog, P.eReal, P.eField4, c.force, c.id[cur], stored)
+#define MDX_GTILE(O, T) \
+    launch<gather_tile_body<O, T, 256>, 256>(c, xcd_grid(nlog), shm, c.meshRec, c.q[cur], c.box, c.cellStart, c.colXY, gg, nlog, P.eReal, e4, c.force, c.id[cur], stored)
+#define MDX_GTILE_T(O) \
+    if (P.tileTeam == 4) MDX_GTILE(O, 4); else if (P.tileTeam == 16) MDX_GTILE(O, 16); else MDX_GTILE(O, 8)
         switch (P.order) {
-            case 1: MDX_GTILE(1); break;
-            case 3: MDX_GTILE(3); break;
-            case 5: MDX_GTILE(5); break;
-            default: MDX_GTILE(7); break;
+            case 1: MDX_GTILE_T(1); break;
+            case 3: MDX_GTILE_T(3); break;
+            case 5: MDX_GTILE_T(5); break;
+            default: MDX_GTILE_T(7); break;
         }
+#undef MDX_GTILE_T
 #undef MDX_GTILE
         return;
     }

@@ -311,6 +311,12 @@ def test_gather_paths(orc, gpu_engine_factory, order):
     assert np.array_equal(out[2], out[1])
     assert np.array_equal(out[1], out[0])
     assert rel_rmse(out[1], ref["fk"]) <= FORCE_RMSE
+    # the tile gather from the interleaved field, on smaller tiles
+    e.option("p3m_gather", 2)
+    e.option("gather_tile_planar", 0)
+    e.option("gather_tile_kb", 24)
+    e.compute(PPPM)
+    assert np.array_equal(e.forces(), out[0])
 
 
 @pytest.mark.parametrize("order", [1, 3, 7])
