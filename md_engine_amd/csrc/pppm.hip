@@ -95,6 +95,13 @@ __device__ __forceinline__ float pick(const float (&w)[ORDER], int k) {
     return r;
 }
 
+// v in (-2n, 2n) to [0, n) without an integer division
+__device__ __forceinline__ int wrap_near(int v, int n) {
+    v += v < 0 ? n : 0;
+    v += v < 0 ? n : 0;
+    return v >= n ? v - n : v;
+}
+
 __device__ __forceinline__ int wrapi(int p, int n) {
     p %= n;
     return p < 0 ? p + n : p;
@@ -244,20 +251,20 @@ __device__ __forceinline__ void spread_tiled_body(int blk, int nblk, const doubl
         }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < vol; k += blockDim.x) {
-        const unsigned long long v = tile[k];
+    // flush: a wave takes whole z rows of the box (as many as fit in its 64
+    // lanes), so the row's coordinates are computed once per row and the
+    // atomics of a row go to consecutive addresses
+    const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    const int per = 64 / ez;  // rows per wave pass (ez <= 64)
+    const int sub = lane / ez, kz = lane - sub * ez;
+    const int gz = wrap_near(oz + kz, m.sz[2]);
+    for (int row = (threadIdx.x >> 6) * per + sub; sub < per && row < ex * ey; row += nw * per) {
+        const int kx = row / ey, ky = row - kx * ey;
+        const unsigned long long v = tile[row * ez + kz];
         if (v == 0ull) continue;
-        const int kz = k % ez, ky = (k / ez) % ey, kx = k / (ez * ey);
-        const int gx = wrapi(ox + kx, m.sz[0]), gy = wrapi(oy + ky, m.sz[1]), gz = wrapi(oz + kz, m.sz[2]);
+        const int gx = wrap_near(ox + kx, m.sz[0]), gy = wrap_near(oy + ky, m.sz[1]);
         atomicAdd(&mesh[((size_t)gx * m.sz[1] + gy) * m.sz[2] + gz], v);
     }
-}
-
-// v in (-2n, 2n) to [0, n) without an integer division
-__device__ __forceinline__ int wrap_near(int v, int n) {
-    v += v < 0 ? n : 0;
-    v += v < 0 ? n : 0;
-    return v >= n ? v - n : v;
 }
 
 // First mesh point of cell k along an axis of sz points and nc cells,
@@ -449,6 +456,118 @@ __device__ __forceinline__ void spread_own_body(int blk, int nblk, const float4 
         const int kz = k % nz, ky = (k / nz) % ny, kx = k / (nz * ny);
         real[((size_t)(X0 + kx) * og.sz[1] + (Y0 + ky)) * og.sz[2] + (Z0 + kz)] =
             (float)((double)(long long)own[k] * kMeshInv);
+    }
+}
+
+// LDS-tiled spread from the mesh records.  The tile's sites are taken SL at
+// a time: a thread per site loads its record and charge and puts its x-plane
+// amplitudes qunit q w_x, its y and z weights and its base in the box into
+// LDS; then every thread takes stencil rows (site, x plane, y row) and adds
+// the row's points into the LDS box.  A site whose stencil leaves the box
+// (it drifted further than the margin) adds its points to the global mesh
+// directly.  The values are the other spreads' 64-bit integers.
+template <int ORDER, int NT>
+__device__ __forceinline__ void spread_rows_body(int blk, int nblk, const float *__restrict__ q,
+                                                     const BoxDev *__restrict__ bx, float qunit,
+                                                     const int *__restrict__ cellStart,
+                                                     const int2 *__restrict__ colXY, TileGeo tg, int nlog,
+                                                     unsigned long long *__restrict__ mesh,
+                                                     const float4 *__restrict__ rec) {
+    constexpr int SL = 128;  // sites per pass
+    constexpr int H = (ORDER - 1) / 2;
+    constexpr int RPS = ORDER * ORDER;  // stencil rows per site
+    extern __shared__ unsigned long long tile[];
+    __shared__ float sA[SL][ORDER], sY[SL][ORDER], sZ[SL][ORDER];
+    __shared__ int sB[SL];
+    const int lb = xcd_block(blk, nlog);
+    if (lb >= nlog) return;  // uniform over the workgroup
+    const MeshGeo m = bx->mesh;
+    const int col = lb / tg.nzc;
+    const int zc = lb % tg.nzc;
+    const int z0 = zc * tg.zs;
+    const int z1 = min(z0 + tg.zs, tg.ncz);
+    const int a0 = cellStart[col * tg.ncz + z0];
+    const int a1 = cellStart[col * tg.ncz + z1];
+    const int2 cxy = colXY[col];
+    const double margin = bx->tileMargin;
+    const int ox = tile_origin(cxy.x * bx->tileCellW[0] - margin, m.h[0]) - H;
+    const int oy = tile_origin(cxy.y * bx->tileCellW[1] - margin, m.h[1]) - H;
+    const int oz = tile_origin(z0 * bx->tileCellW[2] - margin, m.h[2]) - H;
+    const int ex = tg.ext[0], ey = tg.ext[1], ez = tg.ext[2];
+    const int vol = ex * ey * ez;
+    for (int k = threadIdx.x; k < vol; k += NT) tile[k] = 0ull;
+    for (int base = a0; base < a1; base += SL) {
+        const int ns = min(SL, a1 - base);
+        if (threadIdx.x < ns) {
+            const int i = base + threadIdx.x;
+            const float4 rc = rec[i];
+            const float qi = q[i];
+            int b = -1;
+            if (qi != 0.0f) {
+                const int gx = rec_point(rc, 0), gy = rec_point(rc, 1), gz = rec_point(rc, 2);
+                float wx[ORDER], wy[ORDER], wz[ORDER];
+                weights<ORDER>(rc.x, wx);
+                weights<ORDER>(rc.y, wy);
+                weights<ORDER>(rc.z, wz);
+                const float qq = qunit * qi;
+                const int rx = wrap_near(gx - H - ox, m.sz[0]), ry = wrap_near(gy - H - oy, m.sz[1]),
+                          rz = wrap_near(gz - H - oz, m.sz[2]);
+                if (rx + ORDER <= ex && ry + ORDER <= ey && rz + ORDER <= ez) {
+                    b = (rx * ey + ry) * ez + rz;
+#pragma unroll
+                    for (int k = 0; k < ORDER; ++k) {
+                        sA[threadIdx.x][k] = qq * wx[k];
+                        sY[threadIdx.x][k] = wy[k];
+                        sZ[threadIdx.x][k] = wz[k];
+                    }
+                } else {
+#pragma unroll
+                    for (int ix = 0; ix < ORDER; ++ix) {
+                        const int px = wrap_near(gx + ix - H, m.sz[0]);
+                        const float a = qq * wx[ix];
+#pragma unroll
+                        for (int iy = 0; iy < ORDER; ++iy) {
+                            const int py = wrap_near(gy + iy - H, m.sz[1]);
+                            const float bb = a * wy[iy];
+                            const size_t rowb = ((size_t)px * m.sz[1] + py) * m.sz[2];
+#pragma unroll
+                            for (int iz = 0; iz < ORDER; ++iz) {
+                                const long long v = __double2ll_rn((double)(bb * wz[iz]) * kMeshScale);
+                                atomicAdd(&mesh[rowb + wrap_near(gz + iz - H, m.sz[2])], (unsigned long long)v);
+                            }
+                        }
+                    }
+                }
+            }
+            sB[threadIdx.x] = b;
+        }
+        __syncthreads();
+        for (int r = threadIdx.x; r < ns * RPS; r += NT) {
+            const int st = r / RPS, rw = r - st * RPS;
+            const int ix = rw / ORDER, iy = rw - ix * ORDER;
+            const int bb = sB[st];
+            if (bb < 0) continue;
+            const float b = sA[st][ix] * sY[st][iy];
+            const int off = bb + (ix * ey + iy) * ez;
+#pragma unroll
+            for (int iz = 0; iz < ORDER; ++iz) {
+                const long long v = __double2ll_rn((double)(b * sZ[st][iz]) * kMeshScale);
+                atomicAdd(&tile[off + iz], (unsigned long long)v);
+            }
+        }
+        __syncthreads();
+    }
+    // flush by z rows per wave, as the tiled spread
+    const int lane = threadIdx.x & 63, nw = NT >> 6;
+    const int per = 64 / ez;
+    const int sub = lane / ez, kz = lane - sub * ez;
+    const int gz = wrap_near(oz + kz, m.sz[2]);
+    for (int row = (threadIdx.x >> 6) * per + sub; sub < per && row < ex * ey; row += nw * per) {
+        const int kx = row / ey, ky = row - kx * ey;
+        const unsigned long long v = tile[row * ez + kz];
+        if (v == 0ull) continue;
+        const int gx = wrap_near(ox + kx, m.sz[0]), gy = wrap_near(oy + ky, m.sz[1]);
+        atomicAdd(&mesh[((size_t)gx * m.sz[1] + gy) * m.sz[2] + gz], v);
     }
 }
 
@@ -1008,6 +1127,10 @@ bool tiled_geometry(const Ctx &c, TileGeo &tg, int budget = 48 * 1024 / 8) {
     }
     if (zs == 0) return false;
     tg.zs = zs;
+    // the spread flushes the box by z rows per wave: at most 64 points a row
+    while (tg.zs > 1 && extent(2, tg.zs) > 64) --tg.zs;
+    if (extent(2, tg.zs) > 64) return false;
+    zs = tg.zs;
     tg.nzc = (tg.ncz + zs - 1) / zs;
     tg.ext[2] = extent(2, zs);
     return true;
@@ -1256,7 +1379,16 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
     const bool tiled = P.spreadMode >= 1 && tiled_geometry(c, tg);
     P.lastSpread = tiled ? 1 : 0;
     const float4 *rec = c.meshRec && c.hasCharges ? c.meshRec : nullptr;
-    if (tiled) {
+    if (tiled && rec) {
+        const int nblk = c.nc[0] * c.nc[1] * tg.nzc;
+        const size_t shm = sizeof(unsigned long long) * tg.ext[0] * tg.ext[1] * tg.ext[2];
+        switch (P.order) {
+            case 1: launch<spread_rows_body<1, 256>, 256>(c, xcd_grid(nblk), shm, c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh, rec); break;
+            case 3: launch<spread_rows_body<3, 256>, 256>(c, xcd_grid(nblk), shm, c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh, rec); break;
+            case 5: launch<spread_rows_body<5, 256>, 256>(c, xcd_grid(nblk), shm, c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh, rec); break;
+            default: launch<spread_rows_body<7, 256>, 256>(c, xcd_grid(nblk), shm, c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh, rec); break;
+        }
+    } else if (tiled) {
         const int nblk = c.nc[0] * c.nc[1] * tg.nzc;
         const size_t shm = sizeof(unsigned long long) * tg.ext[0] * tg.ext[1] * tg.ext[2];
         switch (P.order) {
