@@ -519,6 +519,7 @@ struct Ctx {
     int *waveMax = nullptr;       // per-wave longest rows: [0, n + 4) pair list, then the molecule list
     int waveMaxN = 0, waveMaxA = 0, waveMaxB = 0;
     bool capture = false;         // a step is being recorded into a graph
+    bool rowMaxPending = false;   // a captured build's row maxima go with the next position refresh
     int graphSteps = 0;           // mdx_run: 1 = captured steps (option "graph")
     unsigned *dMax = nullptr;     // 2 slots x (max |x - x_last|^2, max predicted^2) as float bits
     float *checkPart = nullptr;   // per-workgroup maxima of the displacement check (2 x kCheckBlocks)

@@ -249,6 +249,58 @@ __device__ __forceinline__ void xq_body(int blk, int nblk, int n, const double4 
     xq[k] = make_float4((float)(p.x - g.lo[0]), (float)(p.y - g.lo[1]), (float)(p.z - g.lo[2]), q[k]);
 }
 
+// Longest pair row and molecule row of a build from the per-wave slots a
+// (na) and b (nb), folded into the running maxima flags[1], flags[2]: one
+// workgroup of any size (at most 16 waves).
+__device__ __forceinline__ void row_max_wg(int na, const int *__restrict__ a, int nb, const int *__restrict__ b,
+                                           int *__restrict__ flags) {
+    __shared__ int red[2][16];
+    const int nt = blockDim.x;
+    int ma = 0, mb = 0;
+    // sixteen loads in flight per thread and pass
+    constexpr int U = 16;
+    for (int k0 = threadIdx.x; k0 < na; k0 += U * nt) {
+        int v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = k0 + u * nt < na ? a[k0 + u * nt] : 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) ma = max(ma, v[u]);
+    }
+    for (int k0 = threadIdx.x; k0 < nb; k0 += U * nt) {
+        int v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = k0 + u * nt < nb ? b[k0 + u * nt] : 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) mb = max(mb, v[u]);
+    }
+    ma = wave_max(ma);
+    mb = wave_max(mb);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[0][w] = ma;
+        red[1][w] = mb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < nt / 64; ++k) {
+            ma = max(ma, red[0][k]);
+            mb = max(mb, red[1][k]);
+        }
+        flags[1] = max(flags[1], ma);
+        flags[2] = max(flags[2], mb);
+    }
+}
+
+// the row maxima of a captured step's build, taken by one extra workgroup of
+// the position refresh that follows the build (one launch fewer per step)
+struct RowMaxArgs {
+    int on;
+    const int *gate;
+    int na, nb;
+    const int *a, *b;
+    int *flags;
+};
+
 // k_xq fused with the clearing of the fixed-point force rows and of the
 // energy / virial accumulators that the force kernels add into; with P3M on
 // (rec set) also each charged site's mesh point and offsets for the spread
@@ -256,7 +308,11 @@ __device__ __forceinline__ void xq_body(int blk, int nblk, int n, const double4 
 __device__ __forceinline__ void xq_zero_body(int blk, int nblk, int n, const double4 *__restrict__ pos, const float *__restrict__ q,
                           float4 *__restrict__ xq, const BoxDev *__restrict__ bx,
                           unsigned long long *__restrict__ force, int nacc, unsigned long long *__restrict__ acc,
-                          float4 *__restrict__ rec) {
+                          float4 *__restrict__ rec, RowMaxArgs rm) {
+    if (rm.on && blk == nblk - 1) {  // uniform over the workgroup
+        if (!rm.gate || *rm.gate != 0) row_max_wg(rm.na, rm.a, rm.nb, rm.b, rm.flags);
+        return;
+    }
     const Geometry g = bx->geo;
     int k = blk * blockDim.x + threadIdx.x;
     if (k < nacc) acc[k] = 0ull;
@@ -1037,6 +1093,40 @@ __device__ __forceinline__ void mol_atoms_body(int blk, int nblk, const int *__r
     atomMolO[a.x] = m;
 }
 
+// The list build with the molecule map of a rebuild (mol_atoms_body) as
+// extra workgroups of the same launch: the map only needs the new indices,
+// which the permutation wrote before, and the E3B list that reads it comes
+// after (one launch fewer per step).
+struct MolMapArgs {
+    int blocks;
+    int nmol;
+    const int4 *molIds;
+    const int *idToIdx;
+    int4 *molAtoms;
+    int *atomMolO;
+    int nmolR;
+    const int4 *rigidIds;
+    int4 *rigidAtoms;
+    int *member;
+};
+
+template <bool SHIFT>
+__device__ __forceinline__ void build_map_body(int blk, int nblk, const int *__restrict__ gate,
+    int n, const double4 *__restrict__ pos, const float4 *__restrict__ xq, const BoxDev *__restrict__ bx,
+    const int *__restrict__ colRank, const int *__restrict__ cellStart, int ncell,
+    double rlist2, const uint32_t *__restrict__ exclIdx, int maxExcl, int cap,
+    uint32_t *__restrict__ nl, int *__restrict__ nlCount, int *__restrict__ nlCountA, int *__restrict__ flags,
+    int *__restrict__ waveMax, MolMapArgs mm) {
+    const int nb = nblk - mm.blocks;
+    if (blk >= nb) {  // uniform over the workgroup
+        mol_atoms_body(blk - nb, mm.blocks, gate, mm.nmol, mm.molIds, mm.idToIdx, mm.molAtoms, mm.atomMolO, mm.nmolR,
+                       mm.rigidIds, mm.rigidAtoms, mm.member);
+        return;
+    }
+    build_wave_body<SHIFT>(blk, nb, gate, n, pos, xq, bx, colRank, cellStart, ncell, rlist2, exclIdx, maxExcl, cap, nl,
+                           nlCount, nlCountA, flags, waveMax);
+}
+
 __device__ __forceinline__ void fill_int_body(int blk, int nblk, const int *__restrict__ gate, int n, int *__restrict__ p, int v) {
     if (gate && *gate == 0) return;
     int i = blk * blockDim.x + threadIdx.x;
@@ -1048,40 +1138,7 @@ __device__ __forceinline__ void fill_int_body(int blk, int nblk, const int *__re
 __device__ __forceinline__ void row_max_body(int blk, int nblk, const int *__restrict__ gate, int na, const int *__restrict__ a,
                                                   int nb, const int *__restrict__ b, int *__restrict__ flags) {
     if (gate && *gate == 0) return;
-    __shared__ int red[2][16];
-    int ma = 0, mb = 0;
-    // sixteen loads in flight per thread and pass
-    constexpr int U = 16;
-    for (int k0 = threadIdx.x; k0 < na; k0 += U * 1024) {
-        int v[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = k0 + u * 1024 < na ? a[k0 + u * 1024] : 0;
-#pragma unroll
-        for (int u = 0; u < U; ++u) ma = max(ma, v[u]);
-    }
-    for (int k0 = threadIdx.x; k0 < nb; k0 += U * 1024) {
-        int v[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = k0 + u * 1024 < nb ? b[k0 + u * 1024] : 0;
-#pragma unroll
-        for (int u = 0; u < U; ++u) mb = max(mb, v[u]);
-    }
-    ma = wave_max(ma);
-    mb = wave_max(mb);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        red[0][w] = ma;
-        red[1][w] = mb;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 1; k < 16; ++k) {
-            ma = max(ma, red[0][k]);
-            mb = max(mb, red[1][k]);
-        }
-        flags[1] = max(flags[1], ma);
-        flags[2] = max(flags[2], mb);
-    }
+    row_max_wg(na, a, nb, b, flags);
 }
 
 uint32_t morton2(uint32_t x, uint32_t y) {
@@ -1105,7 +1162,8 @@ uint32_t morton2(uint32_t x, uint32_t y) {
 // slots of the pair-list waves in c.waveMax; the molecule-list slots follow
 size_t wave_slots_a(const Ctx &c) { return (size_t)c.n + 4; }
 
-void launch_build_wave(Ctx &c, int cur) {
+// mm.blocks > 0: the molecule map of the rebuild runs in the same launch
+void launch_build_wave(Ctx &c, int cur, const MolMapArgs &mm) {
     MDX_REQUIRE(std::min(c.nc[0], 2 * c.reach[0] + 1) * std::min(c.nc[1], 2 * c.reach[1] + 1) <= kMaxCols, 1,
                 "stencil wider than the build kernel's column table");
     const int blocks = cdiv(c.n, 4);
@@ -1113,9 +1171,9 @@ void launch_build_wave(Ctx &c, int cur) {
     for (int a = 0; a < 3; ++a)
         if (c.geo.periodic[a] && c.nc[a] < 2 * c.reach[a] + 1) shift = false;
     if (shift)
-        launch<build_wave_body<true>, 256>(c, blocks, 0, rebuild_gate(c), c.n, c.pos[cur], c.xq, c.box, c.colRank, c.cellStart, c.ncell, c.rlist * c.rlist, c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.nlCountA, c.dFlags, c.waveMax);
+        launch<build_map_body<true>, 256>(c, blocks + mm.blocks, 0, rebuild_gate(c), c.n, c.pos[cur], c.xq, c.box, c.colRank, c.cellStart, c.ncell, c.rlist * c.rlist, c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.nlCountA, c.dFlags, c.waveMax, mm);
     else
-        launch<build_wave_body<false>, 256>(c, blocks, 0, rebuild_gate(c), c.n, c.pos[cur], c.xq, c.box, c.colRank, c.cellStart, c.ncell, c.rlist * c.rlist, c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.nlCountA, c.dFlags, c.waveMax);
+        launch<build_map_body<false>, 256>(c, blocks + mm.blocks, 0, rebuild_gate(c), c.n, c.pos[cur], c.xq, c.box, c.colRank, c.cellStart, c.ncell, c.rlist * c.rlist, c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.nlCountA, c.dFlags, c.waveMax, mm);
     c.waveMaxA = blocks * 4;
 }
 
@@ -1222,8 +1280,19 @@ void refresh_xq(Ctx &c) {
 
 void prepare_forces(Ctx &c) {
     const int threads = 256;
-    launch<xq_zero_body, 256>(c, cdiv(std::max(c.n, kAccTotal), threads), 0, c.n, c.pos[c.cur], c.q[c.cur], c.xq, c.box, c.force, kAccTotal, c.acc,
-                              c.pppm.on && c.hasCharges ? c.meshRec : nullptr);
+    RowMaxArgs rm{};
+    if (c.rowMaxPending) {
+        rm.on = 1;
+        rm.gate = rebuild_gate(c);
+        rm.na = c.waveMaxA;
+        rm.a = c.waveMax;
+        rm.nb = c.waveMaxB;
+        rm.b = c.waveMax + wave_slots_a(c);
+        rm.flags = c.dFlags;
+        c.rowMaxPending = false;
+    }
+    launch<xq_zero_body, 256>(c, cdiv(std::max(c.n, kAccTotal), threads) + rm.on, 0, c.n, c.pos[c.cur], c.q[c.cur], c.xq, c.box, c.force, kAccTotal, c.acc,
+                              c.pppm.on && c.hasCharges ? c.meshRec : nullptr, rm);
 }
 
 void nlist_check(Ctx &c) {
@@ -1245,6 +1314,8 @@ static void run_build(Ctx &c, bool sync) {
     int threads = 256;
     int cur = c.cur, nxt = 1 - c.cur;
     const int nslot = 2 * c.ncell;
+    MolMapArgs mm{};
+    bool mapFused = false;
     launch<bin_body, 256>(c, cdiv(c.n, threads), 0, rebuild_gate(c), c.n, c.pos[cur], c.q[cur], c.box, c.ncell, c.colRank, c.cellOf, c.cellCount);
     launch<scan_body, 1024>(c, 1, 0, rebuild_gate(c), nslot, c.cellCount, c.cellStart, c.cellFill, c.box, c.boxLast);
     launch<place_body, 256>(c, cdiv(c.n, threads), 0, rebuild_gate(c), c.n, c.cellOf, c.cellFill, c.perm);
@@ -1275,7 +1346,22 @@ static void run_build(Ctx &c, bool sync) {
         // indices were permuted: drop the old map (captured steps did so in k_copy_back)
         if (!c.capture) fill_int(c, c.e3b.atomMolO, c.n, -1, nullptr);
         const int nmolR = c.capture && c.rigid.on ? c.rigid.nmol : 0;  // see rigid_map
-        launch<mol_atoms_body, 256>(c, cdiv(std::max(c.e3b.nmol, nmolR), threads), 0, rebuild_gate(c), c.e3b.nmol, c.e3b.molIds, c.idToIdx, c.e3b.molAtoms, c.e3b.atomMolO, nmolR, c.rigid.ids, c.rigid.atoms, c.rigid.member);
+        mm.blocks = cdiv(std::max(c.e3b.nmol, nmolR), threads);
+        mm.nmol = c.e3b.nmol;
+        mm.molIds = c.e3b.molIds;
+        mm.idToIdx = c.idToIdx;
+        mm.molAtoms = c.e3b.molAtoms;
+        mm.atomMolO = c.e3b.atomMolO;
+        mm.nmolR = nmolR;
+        mm.rigidIds = c.rigid.ids;
+        mm.rigidAtoms = c.rigid.atoms;
+        mm.member = c.rigid.member;
+        // the wave build takes the map as extra workgroups; the team builds launch it first
+        if (c.buildTeam != 8 && c.buildTeam != 16 && c.buildTeam != 32) {
+            mapFused = true;
+        } else {
+            launch<mol_atoms_body, 256>(c, mm.blocks, 0, rebuild_gate(c), mm.nmol, mm.molIds, mm.idToIdx, mm.molAtoms, mm.atomMolO, mm.nmolR, mm.rigidIds, mm.rigidAtoms, mm.member);
+        }
         MDX_REQUIRE(c.rlist >= (double)c.e3b.r2b + c.skin - 1e-9, 1,
                     "E3B needs rCut + padding >= 6.2 A + padding (the molecule list is taken from the "
                     "pair list)");
@@ -1289,7 +1375,7 @@ static void run_build(Ctx &c, bool sync) {
                 case 8: launch_build<8>(c, c.cur); break;
                 case 16: launch_build<16>(c, c.cur); break;
                 case 32: launch_build<32>(c, c.cur); break;
-                default: launch_build_wave(c, c.cur); break;
+                default: launch_build_wave(c, c.cur, mapFused ? mm : MolMapArgs{}); break;
             }
         }
         c.waveMaxB = 0;
@@ -1300,8 +1386,13 @@ static void run_build(Ctx &c, bool sync) {
                 default: launch_e3b_list<32>(c, c.cur); break;
             }
         }
+        if (c.capture) {
+            // the row maxima ride on the position refresh that follows; the
+            // host checks the rows after the chunk
+            c.rowMaxPending = true;
+            return;
+        }
         launch<row_max_body, 1024>(c, 1, 0, rebuild_gate(c), c.waveMaxA, c.waveMax, c.waveMaxB, c.waveMax + wave_slots_a(c), c.dFlags);
-        if (c.capture) return;  // rows checked by the host after the chunk
         // largest row counts -> pinned host memory; checked now (sync) or at
         // the next synchronisation point (capacity_ok)
         MDX_HIP(hipMemcpyAsync(c.pinned, c.dFlags + 1, sizeof(int) * 2, hipMemcpyDeviceToHost, c.stream));
