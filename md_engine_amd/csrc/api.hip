@@ -402,8 +402,10 @@ void captured_step(Ctx &c, int fflags) {
     // a barostat needs the virial; its steps are gated by dFlags[8]
     compute(c, fflags | (c.baro.on ? kFlagVirialTrace : 0));
     integrate_second_half(c);
-    if (c.thermoOn) thermostat_apply(c);  // gated by dFlags[5]
-    if (c.baro.on) baro_step(c, baro_gate(c));
+    // with the barostat on, the rescale (gated by dFlags[5]) rides on its launch
+    const bool rescaleWithBaro = c.thermoOn && c.fuseThermo && c.baro.on && c.fuseStep;
+    if (c.thermoOn && !rescaleWithBaro) thermostat_apply(c);  // gated by dFlags[5]
+    if (c.baro.on) baro_step(c, baro_gate(c), rescaleWithBaro);
     c.fuseMolVirial = false;
     c.fuseThermo = false;
     c.fuseCheck = false;
@@ -2534,6 +2536,19 @@ int mdx_option_set(mdx_ctx *ctx, const char *name, int value) {
         else if (k == "build_mode") {
             MDX_REQUIRE(value == 0 || value == 2, MDX_ERR_ARG, "build_mode: 0 or 2 (per-atom teams)");
             c.buildMode = value;
+        }
+        else if (k == "fuse_step") {
+            // captured steps: 1 (default) the step decisions, the rebuild's
+            // molecule map and row maxima and the thermostat's rescale ride on
+            // other launches; 0 each in a launch of its own (same results)
+            c.fuseStep = value != 0;
+            if (c.graphs.big || c.graphs.one) {
+                MDX_HIP(hipStreamSynchronize(c.stream));
+                if (c.graphs.big) hipGraphExecDestroy(c.graphs.big);
+                if (c.graphs.one) hipGraphExecDestroy(c.graphs.one);
+                c.graphs.keep.release();
+                c.graphs = Ctx::GraphCache{};
+            }
         }
         else if (k == "mesh_stream") {
             // captured steps: 1 (default) the P3M chain on a second stream beside the pair and E3B kernels

@@ -205,7 +205,7 @@ void box_pull(Ctx &c) {
 // Kernels of one barostat application (gate: nullptr = always).  The forces
 // of the step must have been computed with the virial.  With c.fuseMolVirial
 // the rigid second half already summed the molecular terms into ACC_MOL.
-void baro_step(Ctx &c, const int *gate) {
+void baro_step(Ctx &c, const int *gate, bool rescale) {
     const BarostatState &B = c.baro;
     // ACC_MOL starts at zero: every force evaluation clears the accumulators
     if (!c.fuseMolVirial) molecular_virial(c, ACC_MOL, gate);
@@ -231,9 +231,26 @@ void baro_step(Ctx &c, const int *gate) {
     ba.T0 = c.thermoT;
     ba.coupling = c.thermoTau > 0 ? c.thermoEvery * c.dt / c.thermoTau : 1.0;
     launch<baro_body, 64>(c, 1, 0, gate, c.box, box_params(c), ba, c.acc);
-    // the dilation rides on the Green function's launch where it can
-    const DilateArgs dl = dilate_args(c);
-    if (!(c.pppm.on && pppm_green_device(c, gate, &dl))) dilate(c, gate);
+    // The dilation rides on the Green function's launch where it can, and
+    // so does the thermostat's velocity rescale of the same step (rescale:
+    // its m v^2 was summed by the second half kick; the pressure above
+    // already scaled M V^2 by the rescale's lambda^2, and the dilation reads
+    // positions only, so the velocities end as with the rescale first).
+    DilateArgs dl = dilate_args(c);
+    if (rescale) {
+        dl.vblocks = cdiv(c.n, 256);
+        dl.tgate = thermo_gate(c);
+        dl.vel = c.vel[c.cur];
+        dl.thermo = c.thermo;
+        dl.tconv = c.tconv;
+        dl.T0 = c.thermoT;
+        dl.tmode = c.thermoTau > 0 ? 1 : 0;
+        dl.coupling = c.thermoTau > 0 ? c.thermoEvery * c.dt / c.thermoTau : 1.0;
+    }
+    if (!(c.pppm.on && pppm_green_device(c, gate, &dl))) {
+        dilate(c, gate);
+        if (rescale) thermostat_apply(c);
+    }
 }
 
 }  // namespace mdx

@@ -99,18 +99,7 @@ __device__ __forceinline__ void mv2_body(int blk, int nblk, const int *__restric
 __device__ __forceinline__ void vscale_body(int blk, int nblk, const int *__restrict__ gate, int n, double4 *__restrict__ vel, const unsigned long long *__restrict__ thermo,
                          double tconv, double T0, int mode, double coupling) {
     if (gate && *gate == 0) return;
-    int i = blk * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double T = (double)(long long)thermo[0] * kFxInv * tconv;
-    if (!(T > 0.0)) return;
-    double l2 = mode == 0 ? T0 / T : 1.0 + coupling * (T0 / T - 1.0);
-    if (l2 < 0.0) l2 = 0.0;
-    const double lam = sqrt(l2);
-    double4 v = vel[i];
-    v.x *= lam;
-    v.y *= lam;
-    v.z *= lam;
-    vel[i] = v;
+    vscale_thread(blk * blockDim.x + threadIdx.x, n, vel, thermo, tconv, T0, mode, coupling);
 }
 
 __device__ __forceinline__ void fill_float_body(int blk, int nblk, int n, float *__restrict__ p, float v) {

@@ -514,12 +514,14 @@ struct Ctx {
     // [3] rebuild gate, [4] rebuilds, [5] thermostat gate, [6] turn;
     // [7] bit 0: an atom's candidate stream overflowed the build table, bit 1:
     // a charged site's P3M stencil left the cells next to its own;
-    // [8] barostat gate
+    // [8] barostat gate; [10] last-workgroup ticket of the first half kick
     int *dFlags = nullptr;
     int *waveMax = nullptr;       // per-wave longest rows: [0, n + 4) pair list, then the molecule list
     int waveMaxN = 0, waveMaxA = 0, waveMaxB = 0;
     bool capture = false;         // a step is being recorded into a graph
     bool rowMaxPending = false;   // a captured build's row maxima go with the next position refresh
+    bool decided = false;         // the first half kick of this captured step took the step's decisions
+    bool fuseStep = true;         // option "fuse_step": captured steps fold small launches into others
     int graphSteps = 0;           // mdx_run: 1 = captured steps (option "graph")
     unsigned *dMax = nullptr;     // 2 slots x (max |x - x_last|^2, max predicted^2) as float bits
     float *checkPart = nullptr;   // per-workgroup maxima of the displacement check (2 x kCheckBlocks)
@@ -744,6 +746,76 @@ __device__ __forceinline__ int xcd_block(int blk, int nlogical) {
     return (blk % kXcds) * per + blk / kXcds;
 }
 
+// Device-side step decisions of a captured step: rebuild the list iff the
+// largest displacement since the last build exceeds skin/2 (the reference's
+// setBuildFlag test, GridGPU.cu:755-774), count it, and set the thermostat
+// and barostat gates for the steps where the turn reaches a multiple of their
+// intervals; clears the displacement slots for the next step.  The maxima
+// come from nb per-workgroup partials; the first wave of the calling
+// workgroup reduces them and its thread 0 writes.
+struct DecideArgs {
+    int *ticket;                 // last-workgroup counter when fused into another launch
+    const float *part;
+    unsigned *dmax;
+    int *flags;
+    float half2;
+    int every, baroEvery;
+    unsigned long long *thermo;  // the rigid second half of this step sums m v^2 into it
+};
+
+__device__ __forceinline__ void part_max(int nb, const float *__restrict__ part, float &d2, float &p2) {
+    d2 = 0.f;
+    p2 = 0.f;
+    for (int b = threadIdx.x; b < nb; b += 64) {
+        d2 = fmaxf(d2, part[2 * b]);
+        p2 = fmaxf(p2, part[2 * b + 1]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        d2 = fmaxf(d2, __shfl_xor(d2, o, 64));
+        p2 = fmaxf(p2, __shfl_xor(p2, o, 64));
+    }
+}
+
+__device__ __forceinline__ void decide_wg(int nb, const DecideArgs &d) {
+    if (threadIdx.x >= 64) return;
+    float d2, p2;
+    part_max(nb, d.part, d2, p2);
+    if (threadIdx.x != 0) return;
+    int *flags = d.flags;
+    const int build = d2 > d.half2 ? 1 : 0;
+    flags[3] = build;
+    flags[4] += build;
+    const int turn = flags[6] + 1;
+    flags[6] = turn;
+    flags[5] = (d.every > 0 && turn % d.every == 0) ? 1 : 0;
+    if (d.thermo && flags[5]) {
+        d.thermo[0] = 0ull;
+        d.thermo[1] = 0ull;
+    }
+    flags[8] = (d.baroEvery > 0 && turn % d.baroEvery == 0) ? 1 : 0;
+    d.dmax[0] = 0u;
+    d.dmax[1] = 0u;
+}
+
+// True in the last workgroup of a launch to get here (every thread of every
+// workgroup calls it once).  Device-scope fences order each workgroup's
+// earlier writes before its ticket and the last workgroup's later reads after
+// it, so the last workgroup sees what every workgroup wrote; it returns the
+// ticket to 0 for the next launch.
+__device__ __forceinline__ bool last_block(int *ticket, int nblk) {
+    __shared__ int last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(ticket, 1) == nblk - 1 ? 1 : 0;
+        if (last) *ticket = 0;
+    }
+    __syncthreads();
+    if (last) __threadfence();
+    return last != 0;
+}
+
 // maximum over the 64 lanes of a wave (every lane gets it)
 __device__ __forceinline__ int wave_max(int v) {
 #pragma unroll
@@ -777,6 +849,7 @@ inline double grid_headroom(const Ctx &c) { return c.baro.on ? 0.999 : 1.0; }
 // nlist.hip
 void nlist_setup(Ctx &c);
 void nlist_check(Ctx &c);             // sets dFlags[0] when a rebuild is due
+DecideArgs decide_args(const Ctx &c);  // the step decisions of a captured step
 void flag7_check(int v);              // raises the error dFlags[7] holds, if any
 void nlist_rebuild(Ctx &c, bool forced);  // rebuild if dFlags[0] or forced (host reads flag)
 void refresh_xq(Ctx &c);
@@ -866,11 +939,35 @@ struct DilateArgs {
     const int *member;
     double mO, mH;
     double4 *pos;
+    // the thermostat's velocity rescale of the same step, when it rides on
+    // the same launch (vblocks > 0): its own gate, the summed m v^2, the rule
+    int vblocks;
+    const int *tgate;
+    double4 *vel;
+    const unsigned long long *thermo;
+    double tconv, T0, coupling;
+    int tmode;
 };
 DilateArgs dilate_args(const Ctx &c);
 __device__ inline double dil_mi(double d, const Geometry &g, int k) {
     return g.periodic[k] ? d - g.L[k] * rint(d * g.invL[k]) : d;
 }
+// velocity rescale of site i (FixNVTRescale / Berendsen weak coupling), as vscale_body
+__device__ inline void vscale_thread(int i, int n, double4 *vel, const unsigned long long *thermo, double tconv,
+                                     double T0, int mode, double coupling) {
+    if (i >= n) return;
+    const double T = (double)(long long)thermo[0] * kFxInv * tconv;
+    if (!(T > 0.0)) return;
+    double l2 = mode == 0 ? T0 / T : 1.0 + coupling * (T0 / T - 1.0);
+    if (l2 < 0.0) l2 = 0.0;
+    const double lam = sqrt(l2);
+    double4 v = vel[i];
+    v.x *= lam;
+    v.y *= lam;
+    v.z *= lam;
+    vel[i] = v;
+}
+
 __device__ inline void dilate_thread(int t, const DilateArgs &d, const BoxDev *bx) {
     const Geometry &g = bx->geo;
     const double mu = bx->mu;
@@ -913,7 +1010,9 @@ __device__ inline void dilate_thread(int t, const DilateArgs &d, const BoxDev *b
 BoxParams box_params(const Ctx &c);
 void box_upload(Ctx &c);
 void box_pull(Ctx &c);
-void baro_step(Ctx &c, const int *gate);
+// rescale: the thermostat's velocity rescale of this step goes with the
+// barostat's launches (its m v^2 summed by the second half kick)
+void baro_step(Ctx &c, const int *gate, bool rescale = false);
 // Green function from the device box on stream s (gate: nullptr = always)
 // dl != nullptr: the same launch also dilates (extra workgroups after the
 // Green function's); returns whether it did

@@ -366,19 +366,6 @@ __device__ __forceinline__ void copy_back_body(int blk, int nblk, const int *__r
 }
 
 // one wave: the maxima of the nb workgroups
-__device__ __forceinline__ void part_max(int nb, const float *__restrict__ part, float &d2, float &p2) {
-    d2 = 0.f;
-    p2 = 0.f;
-    for (int b = threadIdx.x; b < nb; b += 64) {
-        d2 = fmaxf(d2, part[2 * b]);
-        p2 = fmaxf(p2, part[2 * b + 1]);
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        d2 = fmaxf(d2, __shfl_xor(d2, o, 64));
-        p2 = fmaxf(p2, __shfl_xor(p2, o, 64));
-    }
-}
 
 // Device-side step decisions of a captured step (one thread): rebuild the
 // list iff the largest displacement since the last build exceeds skin/2 (the
@@ -388,23 +375,15 @@ __device__ __forceinline__ void part_max(int nb, const float *__restrict__ part,
 __device__ __forceinline__ void decide_body(int blk, int nblk, int nb, const float *__restrict__ part, unsigned *__restrict__ dmax,
                          int *__restrict__ flags, float half2, int every, int baroEvery,
                          unsigned long long *__restrict__ thermo) {
-    float d2, p2;
-    part_max(nb, part, d2, p2);
-    if (threadIdx.x != 0) return;
-    const int build = d2 > half2 ? 1 : 0;
-    flags[3] = build;
-    flags[4] += build;
-    const int turn = flags[6] + 1;
-    flags[6] = turn;
-    flags[5] = (every > 0 && turn % every == 0) ? 1 : 0;
-    // thermo: the rigid second half of this step sums m v^2 into it
-    if (thermo && flags[5]) {
-        thermo[0] = 0ull;
-        thermo[1] = 0ull;
-    }
-    flags[8] = (baroEvery > 0 && turn % baroEvery == 0) ? 1 : 0;
-    dmax[0] = 0u;
-    dmax[1] = 0u;
+    DecideArgs d{};
+    d.part = part;
+    d.dmax = dmax;
+    d.flags = flags;
+    d.half2 = half2;
+    d.every = every;
+    d.baroEvery = baroEvery;
+    d.thermo = thermo;
+    decide_wg(nb, d);
 }
 
 // Stencil walk without local arrays (a dynamically indexed local array would
@@ -1357,7 +1336,7 @@ static void run_build(Ctx &c, bool sync) {
         mm.rigidAtoms = c.rigid.atoms;
         mm.member = c.rigid.member;
         // the wave build takes the map as extra workgroups; the team builds launch it first
-        if (c.buildTeam != 8 && c.buildTeam != 16 && c.buildTeam != 32) {
+        if (c.fuseStep && c.buildTeam != 8 && c.buildTeam != 16 && c.buildTeam != 32) {
             mapFused = true;
         } else {
             launch<mol_atoms_body, 256>(c, mm.blocks, 0, rebuild_gate(c), mm.nmol, mm.molIds, mm.idToIdx, mm.molAtoms, mm.atomMolO, mm.nmolR, mm.rigidIds, mm.rigidAtoms, mm.member);
@@ -1386,13 +1365,14 @@ static void run_build(Ctx &c, bool sync) {
                 default: launch_e3b_list<32>(c, c.cur); break;
             }
         }
-        if (c.capture) {
+        if (c.capture && c.fuseStep) {
             // the row maxima ride on the position refresh that follows; the
             // host checks the rows after the chunk
             c.rowMaxPending = true;
             return;
         }
         launch<row_max_body, 1024>(c, 1, 0, rebuild_gate(c), c.waveMaxA, c.waveMax, c.waveMaxB, c.waveMax + wave_slots_a(c), c.dFlags);
+        if (c.capture) return;  // rows checked by the host after the chunk
         // largest row counts -> pinned host memory; checked now (sync) or at
         // the next synchronisation point (capacity_ok)
         MDX_HIP(hipMemcpyAsync(c.pinned, c.dFlags + 1, sizeof(int) * 2, hipMemcpyDeviceToHost, c.stream));
@@ -1535,12 +1515,30 @@ static void check_part(Ctx &c) {
     launch<check_part_body, 256>(c, check_blocks(c), 0, c.n, c.pos[c.cur], c.posLast, c.vel[c.cur], c.box, 2.0 * c.dt, c.checkPart);
 }
 
+DecideArgs decide_args(const Ctx &c) {
+    DecideArgs d{};
+    d.part = c.checkPart;
+    d.dmax = c.dMax;
+    d.flags = c.dFlags;
+    const float half = (float)(0.5 * c.skin);
+    d.half2 = half * half;
+    d.every = c.thermoOn ? c.thermoEvery : 0;
+    d.baroEvery = c.baro.on ? c.baro.every : 0;
+    d.thermo = c.fuseThermo ? c.thermo : nullptr;
+    return d;
+}
+
 void decide_step(Ctx &c) {
     // all-rigid captured steps: the rigid first half wrote the partial maxima
+    // and took the decisions in its last workgroup
+    if (c.decided) {
+        c.decided = false;
+        return;
+    }
     const int fused = c.fuseCheck ? fused_check_blocks(c) : 0;
     if (!fused) check_part(c);
-    const float half = (float)(0.5 * c.skin);
-    launch<decide_body, 64>(c, 1, 0, fused ? fused : check_blocks(c), c.checkPart, c.dMax, c.dFlags, half * half, c.thermoOn ? c.thermoEvery : 0, c.baro.on ? c.baro.every : 0, c.fuseThermo ? c.thermo : nullptr);
+    const DecideArgs d = decide_args(c);
+    launch<decide_body, 64>(c, 1, 0, fused ? fused : check_blocks(c), d.part, d.dmax, d.flags, d.half2, d.every, d.baroEvery, d.thermo);
 }
 
 void check_max(Ctx &c, int parity) {

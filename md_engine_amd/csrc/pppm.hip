@@ -708,6 +708,12 @@ __device__ __forceinline__ void green_body(int blk, int nblk, const int *__restr
                                                int szh, int order, float *__restrict__ G, int gblocks,
                                                DilateArgs dl) {
     extern __shared__ float gtab[];
+    if ((int)blk >= nblk - dl.vblocks) {  // the thermostat's rescale, under its own gate
+        if (!dl.tgate || *dl.tgate != 0)
+            vscale_thread(((int)blk - (nblk - dl.vblocks)) * blockDim.x + threadIdx.x, dl.n, dl.vel, dl.thermo, dl.tconv,
+                          dl.T0, dl.tmode, dl.coupling);
+        return;
+    }
     if (gate && *gate == 0) return;
     if ((int)blk >= gblocks) {  // uniform over the workgroup
         dilate_thread(((int)blk - gblocks) * blockDim.x + threadIdx.x, dl, bx);
@@ -1335,7 +1341,8 @@ bool pppm_green_device(Ctx &c, const int *gate, const DilateArgs *dl) {
     if (tab <= 64 * 1024) {
         DilateArgs none{};
         const int db = dl ? cdiv(std::max(dl->nmol, dl->n), 256) : 0;
-        launch<green_body, 256>(c, gb + db, tab, gate, c.box, szh, P.order, P.green, gb, dl ? *dl : none);
+        const int vb = dl ? dl->vblocks : 0;
+        launch<green_body, 256>(c, gb + db + vb, tab, gate, c.box, szh, P.order, P.green, gb, dl ? *dl : none);
         return dl != nullptr;
     }
     launch<green_direct_body, 256>(c, gb, 0, gate, c.box, szh, P.order, P.green);

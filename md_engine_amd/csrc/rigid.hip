@@ -159,7 +159,7 @@ __device__ __forceinline__ double disp2(const double4 &a, const double4 &b, cons
 __device__ __forceinline__ void rigid_vv1_body(int blk, int nblk, RigidArgs r, double4 *__restrict__ pos, double4 *__restrict__ vel,
                                                    const unsigned long long *__restrict__ f,
                                                    const double4 *__restrict__ last, double dt2,
-                                                   float *__restrict__ part) {
+                                                   float *__restrict__ part, DecideArgs da) {
     r.g = r.bx->geo;
     const int m = blk * blockDim.x + threadIdx.x;
     float d2 = 0.f, p2 = 0.f;
@@ -234,6 +234,8 @@ __device__ __forceinline__ void rigid_vv1_body(int blk, int nblk, RigidArgs r, d
             part[2 * blk] = fmaxf(red[0][0], red[0][1]);
             part[2 * blk + 1] = fmaxf(red[1][0], red[1][1]);
         }
+        // the step's decisions in the last workgroup (one launch fewer)
+        if (da.ticket && last_block(da.ticket, nblk)) decide_wg(nblk, da);
     }
 }
 
@@ -431,6 +433,12 @@ DilateArgs dilate_args(const Ctx &c) {
     d.mO = c.rigid.mO;
     d.mH = c.rigid.mH;
     d.pos = c.pos[c.cur];
+    d.vblocks = 0;
+    d.tgate = nullptr;
+    d.vel = nullptr;
+    d.thermo = nullptr;
+    d.tconv = d.T0 = d.coupling = 0.0;
+    d.tmode = 0;
     return d;
 }
 
@@ -455,9 +463,16 @@ int fused_check_blocks(const Ctx &c) {
 
 void rigid_first_half(Ctx &c) {
     RigidArgs r = make_args(c);
-    // captured steps of all-rigid systems: the skin check's partial maxima here
+    // captured steps of all-rigid systems: the skin check's partial maxima
+    // here, and the step's decisions in the launch's last workgroup
     const bool chk = c.fuseCheck && fused_check_blocks(c) > 0;
-    launch<rigid_vv1_body, 128>(c, cdiv(r.nmol, 128), 0, r, c.pos[c.cur], c.vel[c.cur], c.force, c.posLast, 2.0 * c.dt, chk ? c.checkPart : nullptr);
+    DecideArgs da{};
+    if (chk && c.fuseStep) {
+        da = decide_args(c);
+        da.ticket = c.dFlags + 10;
+        c.decided = true;
+    }
+    launch<rigid_vv1_body, 128>(c, cdiv(r.nmol, 128), 0, r, c.pos[c.cur], c.vel[c.cur], c.force, c.posLast, 2.0 * c.dt, chk ? c.checkPart : nullptr, da);
 }
 
 void rigid_second_half(Ctx &c, bool kick) {

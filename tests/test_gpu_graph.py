@@ -214,3 +214,30 @@ def test_mesh_stream_matches_one_stream(orc, gpu_engine_factory, barostat):
         assert np.array_equal(vv, out[0][1])
         assert np.array_equal(L, out[0][2])
         assert nb == out[0][3]
+
+
+def test_fused_step_launches_match_separate(orc, gpu_engine_factory):
+    """The captured step's folded launches (step decisions in the first half
+    kick, molecule map in the list build, row maxima in the position refresh,
+    the thermostat's rescale with the barostat's Green function) give the
+    trajectory of separate launches bit for bit, rebuilds and box included."""
+    say("fused step launches vs separate")
+    box = systems.tip4p2005_box(343, seed=37, min_sep=1.8)
+    v = start_velocities(orc, box, seed=11)
+    out = []
+    for fuse in (1, 0):
+        e, flags = water_md_engine(gpu_engine_factory, box, v, rc=8.0, skin=1.0, sz=(32, 32, 32), order=5,
+                                   e3b=True, dt=2.0)
+        e.thermostat(True, 298.0, 5, 0.0, systems.REAL_BOLTZ, systems.REAL_MVV_TO_ENG)
+        e.barostat(True, 1.0, 500.0, 1, mvv_to_eng=systems.REAL_MVV_TO_ENG)
+        e.option("graph", 1)
+        e.option("fuse_step", fuse)
+        e.run(70, flags)
+        x, vv = e.get_system()
+        out.append((x, vv, e.get_box()[1], e.nlist_stats()["rebuilds"]))
+        e.close()
+    assert out[0][3] > 3
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][2], out[1][2])
+    assert out[0][3] == out[1][3]
