@@ -193,7 +193,10 @@ void compute(Ctx &c, int flags) {
     const bool virial = (flags & MDX_VIRIAL) || (flags & kFlagVirialTrace);
     c.vtrace = (flags & kFlagVirialTrace) && !(flags & MDX_VIRIAL) && !energy;
     if (c.flex.on) flex_place_m(c);  // the M sites follow O and H before any force term
-    prepare_forces(c);
+    if (c.refreshed)
+        c.refreshed = false;  // the first half kick of this captured step did the refresh
+    else
+        prepare_forces(c);
     const bool mesh = (flags & MDX_PPPM) && c.pppm.on;
     // In a captured step the P3M chain (mostly latency-bound kernels) runs
     // on the side stream beside the pair and E3B kernels: all three start
@@ -214,6 +217,7 @@ void compute(Ctx &c, int flags) {
         MDX_HIP(hipEventRecord(c.evJoin, c.side));
     }
     if ((flags & MDX_PAIR)) pair_forces(c, energy, c.vtrace ? false : virial);
+    row_max_flush(c);  // a captured build's row maxima when no pair launch took them
     if (mesh && !fork) pppm_step(c, energy, virial);
     if ((flags & MDX_SURROGATE) && c.e3b.on) {
         MDX_REQUIRE(c.sur.on, MDX_ERR_STATE, "MDX_SURROGATE needs a model (mdx_surrogate_set)");

@@ -521,6 +521,7 @@ struct Ctx {
     bool capture = false;         // a step is being recorded into a graph
     bool rowMaxPending = false;   // a captured build's row maxima go with the next position refresh
     bool decided = false;         // the first half kick of this captured step took the step's decisions
+    bool refreshed = false;       // ... and refreshed xq, the mesh records, forces and accumulators
     bool fuseStep = true;         // option "fuse_step": captured steps fold small launches into others
     int graphSteps = 0;           // mdx_run: 1 = captured steps (option "graph")
     unsigned *dMax = nullptr;     // 2 slots x (max |x - x_last|^2, max predicted^2) as float bits
@@ -823,6 +824,59 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
+// Longest pair row and molecule row of a build from the per-wave slots a
+// (na) and b (nb), folded into the running maxima flags[1], flags[2]: one
+// workgroup of any size (at most 16 waves).
+__device__ __forceinline__ void row_max_wg(int na, const int *__restrict__ a, int nb, const int *__restrict__ b,
+                                           int *__restrict__ flags) {
+    __shared__ int red[2][16];
+    const int nt = blockDim.x;
+    int ma = 0, mb = 0;
+    // sixteen loads in flight per thread and pass
+    constexpr int U = 16;
+    for (int k0 = threadIdx.x; k0 < na; k0 += U * nt) {
+        int v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = k0 + u * nt < na ? a[k0 + u * nt] : 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) ma = max(ma, v[u]);
+    }
+    for (int k0 = threadIdx.x; k0 < nb; k0 += U * nt) {
+        int v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = k0 + u * nt < nb ? b[k0 + u * nt] : 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) mb = max(mb, v[u]);
+    }
+    ma = wave_max(ma);
+    mb = wave_max(mb);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[0][w] = ma;
+        red[1][w] = mb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < nt / 64; ++k) {
+            ma = max(ma, red[0][k]);
+            mb = max(mb, red[1][k]);
+        }
+        flags[1] = max(flags[1], ma);
+        flags[2] = max(flags[2], mb);
+    }
+}
+
+// the row maxima of a captured step's build, taken by one extra workgroup of
+// a launch that follows the build (one launch fewer per step)
+struct RowMaxArgs {
+    int on;
+    const int *gate;
+    int na, nb;
+    const int *a, *b;
+    int *flags;
+};
+
+
 // orders LDS traffic between the lanes of one wave (no cross-wave barrier)
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -850,6 +904,8 @@ inline double grid_headroom(const Ctx &c) { return c.baro.on ? 0.999 : 1.0; }
 void nlist_setup(Ctx &c);
 void nlist_check(Ctx &c);             // sets dFlags[0] when a rebuild is due
 DecideArgs decide_args(const Ctx &c);  // the step decisions of a captured step
+RowMaxArgs row_max_args(Ctx &c);        // a captured build's pending row maxima (clears the request)
+void row_max_flush(Ctx &c);             // launch them alone if still pending
 void flag7_check(int v);              // raises the error dFlags[7] holds, if any
 void nlist_rebuild(Ctx &c, bool forced);  // rebuild if dFlags[0] or forced (host reads flag)
 void refresh_xq(Ctx &c);

@@ -221,7 +221,7 @@ __device__ __forceinline__ void permute_body(int blk, int nblk, const int *__res
                           double4 *__restrict__ pos1, double4 *__restrict__ vel1,
                           float *__restrict__ q1, int *__restrict__ t1, int *__restrict__ id1,
                           int *__restrict__ idToIdx, double4 *__restrict__ last,
-                          float4 *__restrict__ xq, const BoxDev *__restrict__ bx) {
+                          float4 *__restrict__ xq, const BoxDev *__restrict__ bx, float4 *__restrict__ rec) {
     if (gate && *gate == 0) return;
     const Geometry g = bx->geo;
     int k = blk * blockDim.x + threadIdx.x;
@@ -238,6 +238,9 @@ __device__ __forceinline__ void permute_body(int blk, int nblk, const int *__res
     idToIdx[myid] = k;
     last[k] = p;
     xq[k] = make_float4((float)(p.x - g.lo[0]), (float)(p.y - g.lo[1]), (float)(p.z - g.lo[2]), qq);
+    // the P3M mesh records in the new order (captured steps whose first half
+    // kick refreshed them before the rebuild)
+    if (rec && qq != 0.0f) rec[k] = mesh_record(p, bx->mesh);
 }
 
 __device__ __forceinline__ void xq_body(int blk, int nblk, int n, const double4 *__restrict__ pos, const float *__restrict__ q,
@@ -248,58 +251,6 @@ __device__ __forceinline__ void xq_body(int blk, int nblk, int n, const double4 
     double4 p = pos[k];
     xq[k] = make_float4((float)(p.x - g.lo[0]), (float)(p.y - g.lo[1]), (float)(p.z - g.lo[2]), q[k]);
 }
-
-// Longest pair row and molecule row of a build from the per-wave slots a
-// (na) and b (nb), folded into the running maxima flags[1], flags[2]: one
-// workgroup of any size (at most 16 waves).
-__device__ __forceinline__ void row_max_wg(int na, const int *__restrict__ a, int nb, const int *__restrict__ b,
-                                           int *__restrict__ flags) {
-    __shared__ int red[2][16];
-    const int nt = blockDim.x;
-    int ma = 0, mb = 0;
-    // sixteen loads in flight per thread and pass
-    constexpr int U = 16;
-    for (int k0 = threadIdx.x; k0 < na; k0 += U * nt) {
-        int v[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = k0 + u * nt < na ? a[k0 + u * nt] : 0;
-#pragma unroll
-        for (int u = 0; u < U; ++u) ma = max(ma, v[u]);
-    }
-    for (int k0 = threadIdx.x; k0 < nb; k0 += U * nt) {
-        int v[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = k0 + u * nt < nb ? b[k0 + u * nt] : 0;
-#pragma unroll
-        for (int u = 0; u < U; ++u) mb = max(mb, v[u]);
-    }
-    ma = wave_max(ma);
-    mb = wave_max(mb);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        red[0][w] = ma;
-        red[1][w] = mb;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 1; k < nt / 64; ++k) {
-            ma = max(ma, red[0][k]);
-            mb = max(mb, red[1][k]);
-        }
-        flags[1] = max(flags[1], ma);
-        flags[2] = max(flags[2], mb);
-    }
-}
-
-// the row maxima of a captured step's build, taken by one extra workgroup of
-// the position refresh that follows the build (one launch fewer per step)
-struct RowMaxArgs {
-    int on;
-    const int *gate;
-    int na, nb;
-    const int *a, *b;
-    int *flags;
-};
 
 // k_xq fused with the clearing of the fixed-point force rows and of the
 // energy / virial accumulators that the force kernels add into; with P3M on
@@ -1257,8 +1208,8 @@ void refresh_xq(Ctx &c) {
     launch<xq_body, 256>(c, cdiv(c.n, threads), 0, c.n, c.pos[c.cur], c.q[c.cur], c.xq, c.box);
 }
 
-void prepare_forces(Ctx &c) {
-    const int threads = 256;
+// the pending row maxima of a captured build for the launch that takes them
+RowMaxArgs row_max_args(Ctx &c) {
     RowMaxArgs rm{};
     if (c.rowMaxPending) {
         rm.on = 1;
@@ -1270,6 +1221,17 @@ void prepare_forces(Ctx &c) {
         rm.flags = c.dFlags;
         c.rowMaxPending = false;
     }
+    return rm;
+}
+
+void row_max_flush(Ctx &c) {
+    const RowMaxArgs rm = row_max_args(c);
+    if (rm.on) launch<row_max_body, 1024>(c, 1, 0, rm.gate, rm.na, rm.a, rm.nb, rm.b, rm.flags);
+}
+
+void prepare_forces(Ctx &c) {
+    const int threads = 256;
+    const RowMaxArgs rm = row_max_args(c);
     launch<xq_zero_body, 256>(c, cdiv(std::max(c.n, kAccTotal), threads) + rm.on, 0, c.n, c.pos[c.cur], c.q[c.cur], c.xq, c.box, c.force, kAccTotal, c.acc,
                               c.pppm.on && c.hasCharges ? c.meshRec : nullptr, rm);
 }
@@ -1299,7 +1261,8 @@ static void run_build(Ctx &c, bool sync) {
     launch<scan_body, 1024>(c, 1, 0, rebuild_gate(c), nslot, c.cellCount, c.cellStart, c.cellFill, c.box, c.boxLast);
     launch<place_body, 256>(c, cdiv(c.n, threads), 0, rebuild_gate(c), c.n, c.cellOf, c.cellFill, c.perm);
     launch<cellsort_body, 64>(c, nslot, 0, rebuild_gate(c), nslot, c.cellStart, c.perm, c.perm2, c.id[cur]);
-    launch<permute_body, 256>(c, cdiv(c.n, threads), 0, rebuild_gate(c), c.n, c.perm2, c.pos[cur], c.vel[cur], c.q[cur], c.type[cur], c.id[cur], c.pos[nxt], c.vel[nxt], c.q[nxt], c.type[nxt], c.id[nxt], c.idToIdx, c.posLast, c.xq, c.box);
+    launch<permute_body, 256>(c, cdiv(c.n, threads), 0, rebuild_gate(c), c.n, c.perm2, c.pos[cur], c.vel[cur], c.q[cur], c.type[cur], c.id[cur], c.pos[nxt], c.vel[nxt], c.q[nxt], c.type[nxt], c.id[nxt], c.idToIdx, c.posLast, c.xq, c.box,
+                              c.refreshed && c.pppm.on && c.hasCharges ? c.meshRec : nullptr);
     const bool excl = c.exclOff && c.maxExcl > 0;
     if (excl) {
         MDX_REQUIRE(!c.capture || c.exclIdxN == (size_t)c.n * c.maxExcl, 3,

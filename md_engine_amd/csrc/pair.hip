@@ -42,14 +42,19 @@ struct PairArgs {
     unsigned long long *force;
     unsigned long long *acc;
     const int *vgate;  // EVM 2 inside a captured step: the barostat gate (0: no trace this step)
+    RowMaxArgs rm;     // a captured build's row maxima in one extra workgroup (rm.on)
 };
 
 // EVM: 0 forces only; 1 energy and virial tensor; 2 the virial's trace only
 // (sum r F, what a barostat needs), into ACC_VTRACE, skipped on steps whose
 // barostat gate is 0
 template <int P, bool LJ, bool COUL, int EVM>
-__device__ __forceinline__ void pair_body(int blk, int, PairArgs a) {
+__device__ __forceinline__ void pair_body(int blk, int nblk, PairArgs a) {
     constexpr bool EV = EVM == 1, VT = EVM == 2;
+    if (a.rm.on && blk == nblk - 1) {  // uniform over the workgroup
+        if (!a.rm.gate || *a.rm.gate != 0) row_max_wg(a.rm.na, a.rm.a, a.rm.nb, a.rm.b, a.rm.flags);
+        return;
+    }
     // box values into locals
     const float bL0 = a.bx->Lf[0], bL1 = a.bx->Lf[1], bL2 = a.bx->Lf[2];
     const float sp1 = a.spec[0], sp2 = a.spec[1], sp3 = a.spec[2];
@@ -210,7 +215,7 @@ __device__ __forceinline__ void pair_body(int blk, int, PairArgs a) {
 
 template <int P, bool LJ, bool COUL>
 void launch3(Ctx &c, const PairArgs &a, int evm) {
-    const int blocks = cdiv((long long)a.n * P, 256);
+    const int blocks = cdiv((long long)a.n * P, 256) + a.rm.on;
     if (evm == 1) launch<pair_body<P, LJ, COUL, 1>, 256>(c, blocks, 0, a);
     else if (evm == 2) launch<pair_body<P, LJ, COUL, 2>, 256>(c, blocks, 0, a);
     else launch<pair_body<P, LJ, COUL, 0>, 256>(c, blocks, 0, a);
@@ -274,6 +279,7 @@ void pair_forces(Ctx &c, bool energy, bool virial) {
     a.force = c.force;
     a.acc = c.acc;
     a.vgate = baro_gate(c);
+    a.rm = row_max_args(c);
     const int evm = (energy || virial) ? 1 : (c.vtrace ? 2 : 0);
     ProfScope ps(c, PK_PAIR);
     switch (c.pairTeam) {

@@ -1003,19 +1003,44 @@ __device__ __forceinline__ void gather_tile_body(int blk, int nblk, const float4
     const int vol = ex * ey * ez;
     const size_t ng = (size_t)sx * sy * sz;
     if (threadIdx.x < ORDER * ORDER) lcoef[threadIdx.x] = WCoef<ORDER>::c[threadIdx.x / ORDER][threadIdx.x % ORDER];
-    for (int k = threadIdx.x; k < vol; k += NT) {
-        const int row = k / ez, kz = k - row * ez;
-        const int kx = row / ey, ky = row - kx * ey;
-        const size_t g = ((size_t)wrap_near(ox + kx, sx) * sy + wrap_near(oy + ky, sy)) * sz + wrap_near(oz + kz, sz);
+    // UL points per thread and pass, all their loads issued before any LDS store
+    constexpr int UL = 4;
+    for (int k0 = threadIdx.x; k0 < vol; k0 += UL * NT) {
+        size_t g[UL];
+        bool in[UL];
+#pragma unroll
+        for (int u = 0; u < UL; ++u) {
+            const int k = k0 + u * NT;
+            in[u] = k < vol;
+            const int kk = in[u] ? k : 0;
+            const int row = kk / ez, kz = kk - row * ez;
+            const int kx = row / ey, ky = row - kx * ey;
+            g[u] = ((size_t)wrap_near(ox + kx, sx) * sy + wrap_near(oy + ky, sy)) * sz + wrap_near(oz + kz, sz);
+        }
+        float v0[UL], v1[UL], v2[UL];
         if (E4) {
-            const float4 v = E4[g];
-            fb[k] = v.x;
-            fb[vol + k] = v.y;
-            fb[2 * vol + k] = v.z;
+#pragma unroll
+            for (int u = 0; u < UL; ++u) {
+                const float4 v = E4[g[u]];
+                v0[u] = v.x;
+                v1[u] = v.y;
+                v2[u] = v.z;
+            }
         } else {
-            fb[k] = E[g];
-            fb[vol + k] = E[ng + g];
-            fb[2 * vol + k] = E[2 * ng + g];
+#pragma unroll
+            for (int u = 0; u < UL; ++u) {
+                v0[u] = E[g[u]];
+                v1[u] = E[ng + g[u]];
+                v2[u] = E[2 * ng + g[u]];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UL; ++u) {
+            if (!in[u]) continue;
+            const int k = k0 + u * NT;
+            fb[k] = v0[u];
+            fb[vol + k] = v1[u];
+            fb[2 * vol + k] = v2[u];
         }
     }
     __syncthreads();

@@ -156,13 +156,30 @@ __device__ __forceinline__ double disp2(const double4 &a, const double4 &b, cons
 // of the sites' displacement since the last build and of the displacement
 // plus 2 dt |v|, from the positions and velocities just written; valid when
 // every site belongs to a rigid molecule.
+// With xq set (captured all-rigid steps) the kick also does the force
+// evaluation's position refresh for its sites - FP32 positions, the P3M mesh
+// records of the charged ones, cleared fixed-point force rows (read above)
+// and, in workgroup 0, the cleared energy / virial accumulators - so no
+// separate refresh launch is needed (a rebuild's permutation rewrites xq and
+// the records in the new order).
+struct RefreshArgs {
+    float4 *xq;
+    float4 *rec;
+    const float *q;
+    unsigned long long *force;
+    unsigned long long *acc;
+    int nacc;
+};
+
 __device__ __forceinline__ void rigid_vv1_body(int blk, int nblk, RigidArgs r, double4 *__restrict__ pos, double4 *__restrict__ vel,
                                                    const unsigned long long *__restrict__ f,
                                                    const double4 *__restrict__ last, double dt2,
-                                                   float *__restrict__ part, DecideArgs da) {
+                                                   float *__restrict__ part, DecideArgs da, RefreshArgs ra) {
     r.g = r.bx->geo;
     const int m = blk * blockDim.x + threadIdx.x;
     float d2 = 0.f, p2 = 0.f;
+    if (ra.xq && blk == 0)
+        for (int k = threadIdx.x; k < ra.nacc; k += blockDim.x) ra.acc[k] = 0ull;
     if (m < r.nmol) {
     const int4 a = r.atoms[m];
     d3 fO, fH1, fH2;
@@ -203,6 +220,20 @@ __device__ __forceinline__ void rigid_vv1_body(int blk, int nblk, RigidArgs r, d
         sv[3] = make_double4(0.0, 0.0, 0.0, vel[a.w].w);
         pos[a.w] = sp[3];
         vel[a.w] = sv[3];
+    }
+    if (ra.xq) {
+        const int site[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            if (site[s] < 0) continue;
+            const double4 p = sp[s];
+            const float qs = ra.q[site[s]];
+            ra.xq[site[s]] = make_float4((float)(p.x - r.g.lo[0]), (float)(p.y - r.g.lo[1]), (float)(p.z - r.g.lo[2]), qs);
+            if (ra.rec && qs != 0.0f) ra.rec[site[s]] = mesh_record(p, r.bx->mesh);
+            ra.force[3 * (size_t)site[s] + 0] = 0ull;
+            ra.force[3 * (size_t)site[s] + 1] = 0ull;
+            ra.force[3 * (size_t)site[s] + 2] = 0ull;
+        }
     }
     if (part) {
         const int site[4] = {a.x, a.y, a.z, a.w};
@@ -467,12 +498,20 @@ void rigid_first_half(Ctx &c) {
     // here, and the step's decisions in the launch's last workgroup
     const bool chk = c.fuseCheck && fused_check_blocks(c) > 0;
     DecideArgs da{};
+    RefreshArgs ra{};
     if (chk && c.fuseStep) {
         da = decide_args(c);
         da.ticket = c.dFlags + 10;
         c.decided = true;
+        ra.xq = c.xq;
+        ra.rec = c.pppm.on && c.hasCharges ? c.meshRec : nullptr;
+        ra.q = c.q[c.cur];
+        ra.force = c.force;
+        ra.acc = c.acc;
+        ra.nacc = kAccTotal;
+        c.refreshed = true;
     }
-    launch<rigid_vv1_body, 128>(c, cdiv(r.nmol, 128), 0, r, c.pos[c.cur], c.vel[c.cur], c.force, c.posLast, 2.0 * c.dt, chk ? c.checkPart : nullptr, da);
+    launch<rigid_vv1_body, 128>(c, cdiv(r.nmol, 128), 0, r, c.pos[c.cur], c.vel[c.cur], c.force, c.posLast, 2.0 * c.dt, chk ? c.checkPart : nullptr, da, ra);
 }
 
 void rigid_second_half(Ctx &c, bool kick) {
