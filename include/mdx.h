@@ -41,6 +41,7 @@ typedef struct mdx_ctx mdx_ctx;
                                  what a barostat reads (FixPressureBerendsen's scalar pressure);
                                  read it with mdx_virial_trace_get */
 #define MDX_PPPM_PER_BEAD 8192 /* ring polymers: every bead its own P3M instead of the centroid contraction */
+#define MDX_RING_BEAD_LISTS 16384 /* ring polymers: every bead its own neighbor list instead of one on the centroids */
 
 int mdx_version(void);
 
@@ -232,11 +233,16 @@ int mdx_batch_merged(mdx_ctx *ctx, int *merged);
  * each atom's centroid (computeCentroid, :22-50) on bead 0's mesh, the field
  * interpolated there, the whole force on every bead; alpha is solved over
  * all P x n ring atoms with one bead's Q^2 (:866-900).  MDX_PPPM_PER_BEAD in
- * flags gives every bead its own uncontracted P3M instead.  Each bead keeps
- * its own neighbor list (the reference builds one on centroids; both hold
- * every pair of a bead within the cutoff while beads stay within the skin of
- * their centroid).  P: a power of two from 2 to 64.  The beads run on bead
- * 0's stream inside captured graphs. */
+ * flags gives every bead its own uncontracted P3M instead.  The neighbor
+ * list is built on the centroids as the reference builds it
+ * (src/GridGPU.cu:163-189, 888-895): atoms are binned and paired by the
+ * centroid of their ring, every bead evaluates the same id pairs at its own
+ * positions (bead k with bead k, PairEvaluateIso.h:77-78, 123-124), and all
+ * beads rebuild together when any bead has moved half the skin since the last
+ * build (setBuildFlag over all P n atoms); no E3B then.  MDX_RING_BEAD_LISTS
+ * gives every bead its own list on its own positions instead.  P: a power of
+ * two from 2 to 64.  The beads run on bead 0's stream inside captured
+ * graphs. */
 int mdx_ring_run(mdx_ctx *const *beads, int P, int nsteps, int flags, double omegaP);
 /* Forces of every bead of a ring at the current positions, as one step of
  * mdx_ring_run evaluates them (contracted mesh part unless

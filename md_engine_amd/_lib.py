@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libmdx.so")
 PAIR, PPPM, E3B, ENERGY, VIRIAL, SURROGATE, BONDED = 1, 2, 4, 8, 16, 32, 64
 VIRIAL_TRACE = 4096  # only the virial's trace (the barostat's form); read with Engine.virial_trace()
 PPPM_PER_BEAD = 8192  # ring polymers: per-bead P3M instead of the centroid contraction
+RING_BEAD_LISTS = 16384  # ring polymers: per-bead neighbor lists instead of one on the centroids
 
 # learned three-body model: SUR_NB radial channels, SUR_NF network inputs,
 # SUR_NF-64-64-1 network; trained parameters W1 b1 W2 b2 w3 b3, then the fixed

@@ -114,6 +114,7 @@ void require_system(const Ctx &c) { MDX_REQUIRE(c.n > 0, MDX_ERR_STATE, "no syst
 
 void ensure_list(Ctx &c) {
     if (!c.nlistConfigured) nlist_setup(c);
+    if (c.centroidList && !c.cenById) c.listValid = false;  // a ring's centroid list, the ring gone
     if (!c.listValid) {
         nlist_rebuild(c, true);
     } else {
@@ -465,9 +466,17 @@ void ring_forces(const std::vector<Ctx *> &cs, const RingSpec &ring, int fflags)
 void captured_ring_step(const std::vector<Ctx *> &cs, const RingSpec &ring, int fflags, StepOps &ops) {
     ops.each(cs, [](Ctx &c) { integrate_kick_first(c); });
     ops.one(*cs[0], [&] { ring_drift(ring, *cs[0]); });
+    if (ring.centroidList) {
+        // one decision for the whole ring from every bead's displacement
+        // (GridGPU::setBuildFlag over all P n atoms), then the centroids of
+        // a rebuild step, then every bead's build on them
+        ops.each(cs, [](Ctx &c) { skin_check(c); });
+        ops.each(cs, [](Ctx &c) { skin_decide(c); });
+        ops.one(*cs[0], [&] { ring_centroids(ring, *cs[0], rebuild_gate(*cs[0])); });
+    }
     if (ring_contracts(ring, fflags)) {
-        ops.each(cs, [](Ctx &c) {
-            decide_step(c);
+        ops.each(cs, [&](Ctx &c) {
+            if (!ring.centroidList) decide_step(c);
             nlist_rebuild_async(c);
         });
         ring_forces(cs, ring, fflags, ops);
@@ -478,7 +487,7 @@ void captured_ring_step(const std::vector<Ctx *> &cs, const RingSpec &ring, int 
         return;
     }
     ops.each(cs, [&](Ctx &c) {
-        decide_step(c);
+        if (!ring.centroidList) decide_step(c);
         nlist_rebuild_async(c);
         compute(c, fflags);
         integrate_second_half(c);
@@ -890,6 +899,25 @@ void ring_table(const std::vector<Ctx *> &cs, const RingSpec &ring) {
     MDX_HIP(hipStreamSynchronize(cs[0]->stream));
 }
 
+// Centroid lists outside the captured steps: when any bead lacks a list,
+// every bead's list is built now from the centroids of this moment (one list
+// for the ring, as the reference builds it).  Returns true if it built.
+bool ring_lists(const std::vector<Ctx *> &cs, const RingSpec &ring) {
+    if (!ring.centroidList) return false;
+    bool any = false;
+    for (Ctx *c : cs) any = any || !c->nlistConfigured || !c->listValid || !c->centroidList;
+    if (!any) return false;
+    for (Ctx *c : cs)
+        if (!c->nlistConfigured) nlist_setup(*c);
+    ring_table(cs, ring);
+    ring_centroids(ring, *cs[0]);
+    for (Ctx *c : cs) {
+        nlist_rebuild(*c, true);
+        c->forceFlags = -1;  // forces again on the new lists
+    }
+    return true;
+}
+
 // The E3B judge of a run with the learned three-body term (one context):
 // force terms after a fallback, steps to the next judge point, and the check
 // at a judge point.
@@ -985,6 +1013,7 @@ void run_captured(const std::vector<Ctx *> &cs, int nsteps, int flags, const Rin
             // a contracted ring evaluates every bead's forces together, below
             const bool ringForces = ring && ring_contracts(*ring, fflags);
             bool ringStale = false;
+            if (ring) ring_lists(cs, *ring);  // centroid lists: every bead's at once
             for (Ctx *cp : cs) {
                 Ctx &c = *cp;
                 snapshot_save(c);
@@ -1136,6 +1165,13 @@ void with_ring(mdx_ctx *const *beads, int P, int flags, double omegaP, F &&f) {
     ring.P = P;
     ring.natoms = c0.n;
     ring.contract = (flags & MDX_PPPM) && !(flags & MDX_PPPM_PER_BEAD) && c0.pppm.on;
+    ring.centroidList = !(flags & MDX_RING_BEAD_LISTS);
+    for (Ctx *c : cs) {
+        MDX_REQUIRE(!ring.centroidList || !c->e3b.on, MDX_ERR_STATE,
+                    "a ring's centroid list takes no E3B (use MDX_RING_BEAD_LISTS)");
+        // a list of the other kind is built again
+        if (c->centroidList != ring.centroidList) c->listValid = false;
+    }
     if (ring.contract) {
         for (Ctx *c : cs) {
             MDX_REQUIRE(c->pppm.on && c->pppm.alphaUser == c0.pppm.alphaUser && c->pppm.rcut == c0.pppm.rcut,
@@ -1184,8 +1220,22 @@ void with_ring(mdx_ctx *const *beads, int P, int flags, double omegaP, F &&f) {
         upload(c0, ring.cw, cw.data(), cw.size());
     }
     ring.tab = (void **)dalloc<char>(c0, sizeof(void *) * kRingTab * P);
+    if (ring.contract || ring.centroidList) ring.cen = dalloc<double4>(c0, c0.n);
+    if (ring.centroidList) {
+        ring.part = dalloc<float>(c0, (size_t)P * 2 * kCheckBlocks);
+        MDX_HIP(hipMemsetAsync(ring.part, 0, sizeof(float) * P * 2 * kCheckBlocks, c0.stream));
+        for (int k = 0; k < P; ++k) {
+            Ctx &c = *cs[k];
+            ring.buildPos.push_back(dalloc<double4>(c0, c0.n));
+            ring.ownPart.push_back(c.checkPart);
+            c.checkPart = ring.part + (size_t)k * 2 * kCheckBlocks;
+            c.ringPart = ring.part;
+            c.ringPartNb = P * kCheckBlocks;
+            c.cenById = ring.cen;
+            c.buildPos = ring.buildPos.back();
+        }
+    }
     if (ring.contract) {
-        ring.cen = dalloc<double4>(c0, c0.n);
         ring.q = dalloc<float>(c0, c0.n);
         ring.cf = dalloc<long long>(c0, 3 * (size_t)c0.n);
         upload(c0, ring.q, c0.qHost.data(), c0.n);
@@ -1200,6 +1250,17 @@ void with_ring(mdx_ctx *const *beads, int P, int flags, double omegaP, F &&f) {
     auto restore = [&] {
         hipStreamSynchronize(c0.stream);
         for (size_t k = 0; k < cs.size(); ++k) cs[k]->stream = own[k];
+        for (size_t k = 0; k < ring.ownPart.size(); ++k) {
+            Ctx &c = *cs[k];
+            c.checkPart = ring.ownPart[k];
+            c.ringPart = nullptr;
+            c.ringPartNb = 0;
+            c.cenById = nullptr;
+            c.buildPos = nullptr;
+            c.forceFlags = -1;  // a later run on its own builds its own list first
+            dfree(c0, ring.buildPos[k]);
+        }
+        dfree(c0, ring.part);
         dfree(c0, ring.C);
         dfree(c0, ring.cw);
         dfree(c0, (char *)ring.tab);
@@ -2129,7 +2190,7 @@ int mdx_ring_run(mdx_ctx *const *beads, int P, int nsteps, int flags, double ome
         MDX_REQUIRE(nsteps >= 0, MDX_ERR_ARG, "nsteps must be >= 0");
         MDX_REQUIRE(omegaP > 0, MDX_ERR_ARG, "ring frequency must be positive");
         with_ring(beads, P, flags, omegaP, [&](std::vector<Ctx *> &cs, RingSpec &ring) {
-            run_captured(cs, nsteps, flags & ~MDX_PPPM_PER_BEAD, &ring);
+            run_captured(cs, nsteps, flags & ~(MDX_PPPM_PER_BEAD | MDX_RING_BEAD_LISTS), &ring);
         });
     });
 }
@@ -2140,11 +2201,10 @@ int mdx_ring_forces_compute(mdx_ctx *const *beads, int P, int flags) {
         if (!beads[k]) return MDX_ERR_ARG;
     return guarded(beads[0], [&](Ctx &) {
         with_ring(beads, P, flags, 0.0, [&](std::vector<Ctx *> &cs, RingSpec &ring) {
-            const int fflags = flags & ~MDX_PPPM_PER_BEAD;
-            for (Ctx *c : cs) {
-                prepare_terms(*c);
-                ensure_list(*c);
-            }
+            const int fflags = flags & ~(MDX_PPPM_PER_BEAD | MDX_RING_BEAD_LISTS);
+            for (Ctx *c : cs) prepare_terms(*c);
+            ring_lists(cs, ring);
+            for (Ctx *c : cs) ensure_list(*c);
             ring_table(cs, ring);
             ring_forces(cs, ring, fflags);
         });
@@ -2269,7 +2329,7 @@ int mdx_tune(mdx_ctx *ctx, int flags, int evals, int32_t *chosen, double *avg_ms
         MDX_HIP(hipEventDestroy(e0));
         MDX_HIP(hipEventDestroy(e1));
         // leave forces consistent with the current positions and flags
-        compute(c, c.forceFlags ? c.forceFlags : (flags & kForceTerms));
+        compute(c, c.forceFlags > 0 ? c.forceFlags : (flags & kForceTerms));
         MDX_HIP(hipStreamSynchronize(c.stream));
         if (chosen) {
             chosen[0] = bestMode;

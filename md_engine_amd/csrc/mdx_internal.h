@@ -535,6 +535,17 @@ struct Ctx {
     int buildMode = 0;            // 0 (default) and 2 per-atom teams
     long long nRebuilds = 0;
     bool listValid = false;
+    // Ring polymers with one list on the centroids (GridGPU.cu:163-189,
+    // 888-895): while a ring call runs, the build bins and pairs atoms by the
+    // centroid of their ring (cenById, by atom id) instead of their own
+    // position, in buildPos (the centroids in this bead's order); the skin
+    // check's partial maxima of every bead go to one ring table (ringPart,
+    // ringPartNb workgroups) so every bead takes the same rebuild decision.
+    const double4 *cenById = nullptr;
+    double4 *buildPos = nullptr;
+    const float *ringPart = nullptr;
+    int ringPartNb = 0;
+    bool centroidList = false;    // the current list was built on ring centroids
 
     // exclusions (CSR by id, entries id | tag << 30)
     int *exclOff = nullptr;
@@ -918,6 +929,8 @@ bool grow_if_needed(Ctx &c);
 constexpr int kCheckBlocks = 512;     // workgroups of the displacement check
 void check_max(Ctx &c, int parity);   // max displacement / speed into dMax slot `parity`
 void decide_step(Ctx &c);             // captured step: device rebuild / thermostat gates
+void skin_check(Ctx &c);              // the displacement check's partial maxima only
+void skin_decide(Ctx &c);             // the decisions from them (a ring's table when set)
 
 // pair.hip
 void pair_forces(Ctx &c, bool energy, bool virial);
@@ -1086,12 +1099,18 @@ struct RingSpec {
     // ring-polymer contraction of the mesh part (false: every bead its own P3M)
     bool contract = false;
     double4 *cen = nullptr;   // centroids by atom id
+    // one neighbor list on the centroids, bead k paired with bead k (false:
+    // every bead its own list on its own positions)
+    bool centroidList = false;
+    float *part = nullptr;    // every bead's skin-check partial maxima (P x 2 x kCheckBlocks)
+    std::vector<double4 *> buildPos;  // per bead: centroids in the bead's order
+    std::vector<float *> ownPart;     // per bead: its own check table, restored afterwards
     float *q = nullptr;       // charges by atom id
     long long *cf = nullptr;  // centroid forces by atom id, fixed point (kept zero between uses)
 };
 // ring-level launches, issued (or recorded) on the first bead's context
 void ring_drift(const RingSpec &r, Ctx &c0);
-void ring_centroids(const RingSpec &r, Ctx &c0);
+void ring_centroids(const RingSpec &r, Ctx &c0, const int *gate = nullptr);
 void ring_apply(const RingSpec &r, Ctx &c0);
 // pppm.hip: the mesh part of every bead from the ring's centroids, on the
 // first bead's mesh (centroid force into r.cf, then onto every bead)

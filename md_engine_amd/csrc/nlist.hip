@@ -71,7 +71,8 @@ __device__ __forceinline__ void check_body(int blk, int nblk, int n, const doubl
 // atoms.
 __device__ __forceinline__ void bin_body(int blk, int nblk, const int *__restrict__ gate, int n, double4 *__restrict__ pos, const float *__restrict__ q,
                       const BoxDev *__restrict__ bx, int ncell, const int *__restrict__ colRank,
-                      int *__restrict__ cellOf, int *__restrict__ cellCount) {
+                      int *__restrict__ cellOf, int *__restrict__ cellCount,
+                      const double4 *__restrict__ cen, const int *__restrict__ ids) {
     if (gate && *gate == 0) return;
     const Geometry g = bx->geo;
     const CellGrid cg = bx->cg;
@@ -87,6 +88,9 @@ __device__ __forceinline__ void bin_body(int blk, int nblk, const int *__restric
         }
     }
     pos[i] = p;
+    // ring centroid lists: the atom goes to the cell of its ring's centroid
+    // (already in the box)
+    if (cen) p = cen[ids[i]];
     int cx = cell_coord(p.x, g.lo[0], cg.cellInv[0], cg.nc[0]);
     int cy = cell_coord(p.y, g.lo[1], cg.cellInv[1], cg.nc[1]);
     int cz = cell_coord(p.z, g.lo[2], cg.cellInv[2], cg.nc[2]);
@@ -221,7 +225,8 @@ __device__ __forceinline__ void permute_body(int blk, int nblk, const int *__res
                           double4 *__restrict__ pos1, double4 *__restrict__ vel1,
                           float *__restrict__ q1, int *__restrict__ t1, int *__restrict__ id1,
                           int *__restrict__ idToIdx, double4 *__restrict__ last,
-                          float4 *__restrict__ xq, const BoxDev *__restrict__ bx, float4 *__restrict__ rec) {
+                          float4 *__restrict__ xq, const BoxDev *__restrict__ bx, float4 *__restrict__ rec,
+                          const double4 *__restrict__ cen, double4 *__restrict__ bpos) {
     if (gate && *gate == 0) return;
     const Geometry g = bx->geo;
     int k = blk * blockDim.x + threadIdx.x;
@@ -237,10 +242,16 @@ __device__ __forceinline__ void permute_body(int blk, int nblk, const int *__res
     id1[k] = myid;
     idToIdx[myid] = k;
     last[k] = p;
-    xq[k] = make_float4((float)(p.x - g.lo[0]), (float)(p.y - g.lo[1]), (float)(p.z - g.lo[2]), qq);
     // the P3M mesh records in the new order (captured steps whose first half
     // kick refreshed them before the rebuild)
     if (rec && qq != 0.0f) rec[k] = mesh_record(p, bx->mesh);
+    // ring centroid lists: the build works on the centroids (xq is written
+    // again from the positions before the forces)
+    if (cen) {
+        p = cen[myid];
+        bpos[k] = p;
+    }
+    xq[k] = make_float4((float)(p.x - g.lo[0]), (float)(p.y - g.lo[1]), (float)(p.z - g.lo[2]), qq);
 }
 
 __device__ __forceinline__ void xq_body(int blk, int nblk, int n, const double4 *__restrict__ pos, const float *__restrict__ q,
@@ -1097,13 +1108,14 @@ void launch_build_wave(Ctx &c, int cur, const MolMapArgs &mm) {
     MDX_REQUIRE(std::min(c.nc[0], 2 * c.reach[0] + 1) * std::min(c.nc[1], 2 * c.reach[1] + 1) <= kMaxCols, 1,
                 "stencil wider than the build kernel's column table");
     const int blocks = cdiv(c.n, 4);
+    const double4 *bpos = c.cenById ? c.buildPos : c.pos[cur];  // ring centroids or the positions
     bool shift = true;
     for (int a = 0; a < 3; ++a)
         if (c.geo.periodic[a] && c.nc[a] < 2 * c.reach[a] + 1) shift = false;
     if (shift)
-        launch<build_map_body<true>, 256>(c, blocks + mm.blocks, 0, rebuild_gate(c), c.n, c.pos[cur], c.xq, c.box, c.colRank, c.cellStart, c.ncell, c.rlist * c.rlist, c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.nlCountA, c.dFlags, c.waveMax, mm);
+        launch<build_map_body<true>, 256>(c, blocks + mm.blocks, 0, rebuild_gate(c), c.n, bpos, c.xq, c.box, c.colRank, c.cellStart, c.ncell, c.rlist * c.rlist, c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.nlCountA, c.dFlags, c.waveMax, mm);
     else
-        launch<build_map_body<false>, 256>(c, blocks + mm.blocks, 0, rebuild_gate(c), c.n, c.pos[cur], c.xq, c.box, c.colRank, c.cellStart, c.ncell, c.rlist * c.rlist, c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.nlCountA, c.dFlags, c.waveMax, mm);
+        launch<build_map_body<false>, 256>(c, blocks + mm.blocks, 0, rebuild_gate(c), c.n, bpos, c.xq, c.box, c.colRank, c.cellStart, c.ncell, c.rlist * c.rlist, c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.nlCountA, c.dFlags, c.waveMax, mm);
     c.waveMaxA = blocks * 4;
 }
 
@@ -1113,14 +1125,15 @@ void launch_build(Ctx &c, int cur) {
                 "stencil wider than the build kernel's column table");
     int threads = 256;
     int blocks = cdiv((long long)c.n * T, threads);
+    const double4 *bpos = c.cenById ? c.buildPos : c.pos[cur];  // ring centroids or the positions
     // image shifts need every periodic axis to hold the full stencil
     bool shift = true;
     for (int a = 0; a < 3; ++a)
         if (c.geo.periodic[a] && c.nc[a] < 2 * c.reach[a] + 1) shift = false;
     if (shift)
-        launch<build_body<T, true>, 256>(c, blocks, 0, rebuild_gate(c), c.n, c.pos[cur], c.xq, c.box, c.colRank, c.cellStart, c.ncell, c.rlist * c.rlist, c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.nlCountA, c.dFlags, c.waveMax);
+        launch<build_body<T, true>, 256>(c, blocks, 0, rebuild_gate(c), c.n, bpos, c.xq, c.box, c.colRank, c.cellStart, c.ncell, c.rlist * c.rlist, c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.nlCountA, c.dFlags, c.waveMax);
     else
-        launch<build_body<T, false>, 256>(c, blocks, 0, rebuild_gate(c), c.n, c.pos[cur], c.xq, c.box, c.colRank, c.cellStart, c.ncell, c.rlist * c.rlist, c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.nlCountA, c.dFlags, c.waveMax);
+        launch<build_body<T, false>, 256>(c, blocks, 0, rebuild_gate(c), c.n, bpos, c.xq, c.box, c.colRank, c.cellStart, c.ncell, c.rlist * c.rlist, c.exclIdx, c.exclOff ? c.maxExcl : 0, c.cap, c.nl, c.nlCount, c.nlCountA, c.dFlags, c.waveMax);
     c.waveMaxA = blocks * (threads / 64);
 }
 
@@ -1257,12 +1270,16 @@ static void run_build(Ctx &c, bool sync) {
     const int nslot = 2 * c.ncell;
     MolMapArgs mm{};
     bool mapFused = false;
-    launch<bin_body, 256>(c, cdiv(c.n, threads), 0, rebuild_gate(c), c.n, c.pos[cur], c.q[cur], c.box, c.ncell, c.colRank, c.cellOf, c.cellCount);
+    const double4 *cen = c.cenById;
+    MDX_REQUIRE(!cen || (c.buildPos && !c.e3b.on), 3,
+                "a centroid list needs its build positions and takes no E3B molecule list");
+    c.centroidList = cen != nullptr;
+    launch<bin_body, 256>(c, cdiv(c.n, threads), 0, rebuild_gate(c), c.n, c.pos[cur], c.q[cur], c.box, c.ncell, c.colRank, c.cellOf, c.cellCount, cen, c.id[cur]);
     launch<scan_body, 1024>(c, 1, 0, rebuild_gate(c), nslot, c.cellCount, c.cellStart, c.cellFill, c.box, c.boxLast);
     launch<place_body, 256>(c, cdiv(c.n, threads), 0, rebuild_gate(c), c.n, c.cellOf, c.cellFill, c.perm);
     launch<cellsort_body, 64>(c, nslot, 0, rebuild_gate(c), nslot, c.cellStart, c.perm, c.perm2, c.id[cur]);
     launch<permute_body, 256>(c, cdiv(c.n, threads), 0, rebuild_gate(c), c.n, c.perm2, c.pos[cur], c.vel[cur], c.q[cur], c.type[cur], c.id[cur], c.pos[nxt], c.vel[nxt], c.q[nxt], c.type[nxt], c.id[nxt], c.idToIdx, c.posLast, c.xq, c.box,
-                              c.refreshed && c.pppm.on && c.hasCharges ? c.meshRec : nullptr);
+                              c.refreshed && c.pppm.on && c.hasCharges ? c.meshRec : nullptr, cen, c.buildPos);
     const bool excl = c.exclOff && c.maxExcl > 0;
     if (excl) {
         MDX_REQUIRE(!c.capture || c.exclIdxN == (size_t)c.n * c.maxExcl, 3,
@@ -1489,6 +1506,15 @@ DecideArgs decide_args(const Ctx &c) {
     d.baroEvery = c.baro.on ? c.baro.every : 0;
     d.thermo = c.fuseThermo ? c.thermo : nullptr;
     return d;
+}
+
+void skin_check(Ctx &c) { check_part(c); }
+
+void skin_decide(Ctx &c) {
+    const DecideArgs d = decide_args(c);
+    const float *part = c.ringPart ? c.ringPart : d.part;
+    const int nb = c.ringPart ? c.ringPartNb : check_blocks(c);
+    launch<decide_body, 64>(c, 1, 0, nb, part, d.dmax, d.flags, d.half2, d.every, d.baroEvery, d.thermo);
 }
 
 void decide_step(Ctx &c) {

@@ -113,10 +113,14 @@ __device__ __forceinline__ void ring_body(int blk, int nblk, int natoms, int P, 
     }
 }
 
-// centroid of every atom's ring, by atom id (FP64; the reference's is in its
-// working precision)
-__device__ __forceinline__ void ring_centroid_body(int blk, int nblk, int natoms, int P, void *const *__restrict__ tab,
-                                                       const BoxDev *__restrict__ bx, double4 *__restrict__ cen) {
+// centroid of every atom's ring, by atom id
+// (FP64; the reference's is in its working precision).  No FMA contraction,
+// so the centroids are those of the same operations in plain FP64 arithmetic.
+__device__ __forceinline__ void ring_centroid_body(int blk, int nblk, const int *__restrict__ gate, int natoms, int P,
+                                                   void *const *__restrict__ tab, const BoxDev *__restrict__ bx,
+                                                   double4 *__restrict__ cen) {
+#pragma clang fp contract(off)
+    if (gate && *gate == 0) return;
     const int id = blk * blockDim.x + threadIdx.x;
     if (id >= natoms) return;
     const Geometry g = bx->geo;
@@ -160,8 +164,8 @@ __device__ __forceinline__ void ring_apply_body(int blk, int nblk, int natoms, i
 
 }  // namespace
 
-void ring_centroids(const RingSpec &r, Ctx &c0) {
-    launch<ring_centroid_body, 256>(c0, cdiv(r.natoms, 256), 0, r.natoms, r.P, r.tab, c0.box, r.cen);
+void ring_centroids(const RingSpec &r, Ctx &c0, const int *gate) {
+    launch<ring_centroid_body, 256>(c0, cdiv(r.natoms, 256), 0, gate, r.natoms, r.P, r.tab, c0.box, r.cen);
 }
 
 void ring_apply(const RingSpec &r, Ctx &c0) {

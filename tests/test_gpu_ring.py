@@ -1,7 +1,8 @@
 """Ring-polymer (PIMD) steps on the GPU (mdx_ring_run, md_engine_amd/csrc/
 ring.hip): the free ring polymer against the exact normal-mode propagation of
 oracle/ring.py, every bead's forces against the FP64 oracle with the mesh
-part contracted to the centroids (the reference's scheme) and per bead, and
+part contracted to the centroids (the reference's scheme) and per bead, the
+neighbor list built on the centroids (the reference's) or per bead, and
 the ring Hamiltonian of flexible q-TIP4P/F water beads conserved to second
 order in dt in both schemes."""
 import sys
@@ -10,7 +11,7 @@ import numpy as np
 import pytest
 
 from md_engine_amd import systems
-from md_engine_amd._lib import PPPM_PER_BEAD, ring_forces, ring_omega, run_ring
+from md_engine_amd._lib import PPPM_PER_BEAD, RING_BEAD_LISTS, ring_forces, ring_omega, run_ring
 from oracle import flexwater as FW
 from oracle import ring as R
 
@@ -104,6 +105,8 @@ def test_ring_forces_vs_oracle(orc, gpu_engine_factory, mode):
     excl = orc.exclusions_bfs(n, systems.water_bonds(mol))
     eps24, sig6 = systems.water_lj_tables()
     gamma = FW.QTIP4PF["gamma"]
+    # one list on the ring's centroids, every bead evaluating its id pairs
+    oc, orows, _ = oracle_rows_by_id(orc, R.centroids(xa, lo, L), lo, L, rc + skin, excl=excl, brute=False)
     for k, e in enumerate(es):
         x = xa[k]
         if mode == "per-bead":
@@ -111,7 +114,7 @@ def test_ring_forces_vs_oracle(orc, gpu_engine_factory, mode):
             G = orc.pppm_green(L, np.array(sz), alpha, 5)
             fmesh = orc.pppm(x, box["q"], lo, L, np.array(sz), 5, qqr, alpha, q2, G)["f"]
         assert e.pppm_info()[0] == alpha
-        oc, orows, xw = oracle_rows_by_id(orc, x, lo, L, rc + skin, excl=excl, brute=False)
+        xw = orc.wrap(x, lo, L)
         fp, _, _ = orc.pair(xw, box["q"], box["type"], L, oc, orows, np.full((3, 3), rc * rc), eps24, sig6,
                             with_charge=True, alpha=alpha, qqr=qqr, rq=rc)
         fb, _, _ = FW.bonded(mol, x, L)
@@ -120,6 +123,59 @@ def test_ring_forces_vs_oracle(orc, gpu_engine_factory, mode):
         rel = float(np.sqrt(((f - ref) ** 2).sum() / (ref ** 2).sum()))
         say(f"bead {k}: relative force RMSE {rel:.2e}")
         assert rel < 1e-4, f"bead {k}"
+    for e in es:
+        e.close()
+
+
+def test_ring_centroid_lists(orc, gpu_engine_factory):
+    """The ring's neighbor list is built on the centroids (GridGPU.cu:163-189,
+    888-895): every bead's rows, read back by id, are the oracle's cell list of
+    the centroids bit for bit, and its forces pair bead k with bead k over
+    those rows.  With beads spread far enough that lists of the bead positions
+    differ from the centroid list, MDX_RING_BEAD_LISTS gives every bead the
+    oracle's list of its own positions instead."""
+    say("ring neighbor lists on the centroids, 4 beads of 64 molecules")
+    from test_gpu_parity import assert_same_lists, oracle_rows_by_id
+
+    P, rc, skin, sz = 4, 5.5, 0.5, (32, 32, 32)
+    box = systems.tip4p2005_box(64, seed=41, min_sep=1.8)
+    xs = bead_systems(box, P, 9, spread=0.25)
+    vs = [np.zeros_like(x) for x in xs]
+    es, flags = ring_engines(gpu_engine_factory, box, xs, vs, dt=0.5, rc=rc, skin=skin, sz=sz)
+    L, lo, mol = box["L"], box["lo"], box["mol"]
+    n = len(box["x"])
+    xa = np.array(xs)
+    excl = orc.exclusions_bfs(n, systems.water_bonds(mol))
+    cen = R.centroids(xa, lo, L)
+    oc, orows, _ = oracle_rows_by_id(orc, cen, lo, L, rc + skin, excl=excl, brute=False)
+    bead_rows = [oracle_rows_by_id(orc, xa[k], lo, L, rc + skin, excl=excl, brute=False)[:2] for k in range(P)]
+    differ = sum(not np.array_equal(bc, oc) for bc, _ in bead_rows)
+    say(f"beads whose own list differs from the centroid list: {differ} of {P}")
+    assert differ > 0, "the beads are too close to their centroids to tell the lists apart"
+
+    ring_forces(es, flags)
+    qqr = systems.REAL_QQR_TO_ENG
+    alpha, fmesh = R.contracted_pppm(xa, box["q"], lo, L, sz, 5, qqr, rc)
+    eps24, sig6 = systems.water_lj_tables()
+    gamma = FW.QTIP4PF["gamma"]
+    for k, e in enumerate(es):
+        gc, grows = e.nlist_get()
+        assert_same_lists(gc, grows, oc, orows)
+        xw = orc.wrap(xa[k], lo, L)
+        fp, _, _ = orc.pair(xw, box["q"], box["type"], L, oc, orows, np.full((3, 3), rc * rc), eps24, sig6,
+                            with_charge=True, alpha=alpha, qqr=qqr, rq=rc)
+        fb, _, _ = FW.bonded(mol, xa[k], L)
+        ref = FW.spread_m(mol, fp + fmesh, gamma) + fb
+        f = e.forces()
+        rel = float(np.sqrt(((f - ref) ** 2).sum() / (ref ** 2).sum()))
+        say(f"bead {k}: centroid list exact, relative force RMSE {rel:.2e}")
+        assert rel < 1e-4, f"bead {k}"
+
+    ring_forces(es, flags | RING_BEAD_LISTS)
+    for k, e in enumerate(es):
+        gc, grows = e.nlist_get()
+        assert_same_lists(gc, grows, *bead_rows[k])
+    say("per-bead lists: every bead's own list exact")
     for e in es:
         e.close()
 
@@ -187,11 +243,12 @@ def test_ring_hamiltonian_second_order(orc, gpu_engine_factory, mode):
     assert errs[1] < 0.4 * errs[0]
 
 
-@pytest.mark.parametrize("mode", ["contracted", "per-bead"])
+@pytest.mark.parametrize("mode", ["contracted", "per-bead", "bead-lists"])
 def test_ring_merged_launches_equal_separate(orc, gpu_engine_factory, mode):
     """The ring's per-bead kernels as one launch per kernel over all beads
     (the default) give the trajectory of the beads launched one by one, bit
-    for bit, with the mesh part contracted to the centroids or per bead."""
+    for bit, with the mesh part contracted to the centroids or per bead, and
+    with one list on the centroids or every bead its own."""
     say(f"ring: merged bead launches vs separate, mesh part {mode}")
     P = 4
     box = systems.tip4p2005_box(64, seed=33, min_sep=1.8)
@@ -201,7 +258,7 @@ def test_ring_merged_launches_equal_separate(orc, gpu_engine_factory, mode):
     out = []
     for merge in (1, 0):
         es, flags = ring_engines(gpu_engine_factory, box, xs, vs, dt=0.5)
-        run_flags = flags | (0 if mode == "contracted" else PPPM_PER_BEAD)
+        run_flags = flags | {"contracted": 0, "per-bead": PPPM_PER_BEAD, "bead-lists": RING_BEAD_LISTS}[mode]
         es[0].option("batch_merge", merge)
         run_ring(es, 40, run_flags, wP)
         assert es[0].batch_merged() == bool(merge)
