@@ -111,9 +111,11 @@ int mdx_pppm_info(mdx_ctx *ctx, double *alpha, double *rms_error);
 /* Kernels the last mesh evaluation ran (no counterpart in the reference;
  * for tests and benchmarks): spread 2 owner-computes cell blocks, 1 LDS tiles
  * with global atomics, 0 global atomics; gather 2 LDS field tiles, 1 row
- * teams from the per-site mesh records, 0 row teams from the positions; -1
- * before any evaluation. */
-int mdx_pppm_paths(mdx_ctx *ctx, int *spread, int *gather);
+ * teams from the per-site mesh records, 0 row teams from the positions; fft
+ * 1 the mesh solve in three LDS passes (power-of-two meshes whose planes fit
+ * a workgroup), 0 hipFFT with separate field and conversion kernels; -1
+ * before any evaluation.  Any pointer may be NULL. */
+int mdx_pppm_paths(mdx_ctx *ctx, int *spread, int *gather, int *fft);
 /* Error mode (FixChargeEwald::setError + setGridToErrorTolerance,
  * FixChargeEwald.cu:1024-1096): the mesh starts from the one set before
  * (else 64^3) and the axis with the fewest points per length is doubled until

@@ -103,7 +103,7 @@ def load():
     L.mdx_ewald_real_set.argtypes = [_vp, c, d, d, d]
     L.mdx_pppm_set.argtypes = [_vp, c, c, c, c, d, d, d]
     L.mdx_pppm_info.argtypes = [_vp, C.POINTER(d), C.POINTER(d)]
-    L.mdx_pppm_paths.argtypes = [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.mdx_pppm_paths.argtypes = [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.mdx_pppm_set_error.argtypes = [_vp, d, d, c, d]
     L.mdx_pppm_interval_set.argtypes = [_vp, c]
     L.mdx_pppm_mesh_get.argtypes = [_vp, _ip, C.POINTER(c)]
@@ -294,11 +294,12 @@ class Engine:
         return a.value, e.value
 
     def pppm_paths(self):
-        """Kernels of the last mesh evaluation: (spread, gather), see mdx_pppm_paths."""
+        """Kernels of the last mesh evaluation: (spread, gather, fft), see mdx_pppm_paths."""
         a = C.c_int(-1)
         b = C.c_int(-1)
-        self._chk(self._L.mdx_pppm_paths(self._h, C.byref(a), C.byref(b)), "mdx_pppm_paths")
-        return a.value, b.value
+        f = C.c_int(-1)
+        self._chk(self._L.mdx_pppm_paths(self._h, C.byref(a), C.byref(b), C.byref(f)), "mdx_pppm_paths")
+        return a.value, b.value, f.value
 
     def pppm_set_error(self, error, r_cut, order, qqr):
         """FixChargeEwald.setError: the mesh is picked for this RMS force error."""

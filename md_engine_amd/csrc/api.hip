@@ -521,11 +521,11 @@ std::vector<FftGroup> group_ffts(const std::vector<Ctx *> &cs, BatchKeep &keep, 
     for (size_t k = 0; k < R; ++k) {
         const PppmState &P = cs[k]->pppm;
         saved[k] = {P.real, P.rhoHat, P.eHat, P.eReal};
-        if (!P.on || !P.plansMade || groupOf[k] >= 0) continue;
+        if (!P.on || !P.plansMade || pppm_lds_fft(P) || groupOf[k] >= 0) continue;
         FftGroup g;
         for (size_t j = k; j < R; ++j) {
             const PppmState &Q = cs[j]->pppm;
-            if (groupOf[j] < 0 && Q.on && Q.plansMade && Q.sz[0] == P.sz[0] && Q.sz[1] == P.sz[1] &&
+            if (groupOf[j] < 0 && Q.on && Q.plansMade && !pppm_lds_fft(Q) && Q.sz[0] == P.sz[0] && Q.sz[1] == P.sz[1] &&
                 Q.sz[2] == P.sz[2]) {
                 groupOf[j] = (int)groups.size();
                 g.members.push_back(j);
@@ -1769,10 +1769,11 @@ int mdx_pppm_info(mdx_ctx *ctx, double *alpha, double *rms_error) {
     });
 }
 
-int mdx_pppm_paths(mdx_ctx *ctx, int *spread, int *gather) {
+int mdx_pppm_paths(mdx_ctx *ctx, int *spread, int *gather, int *fft) {
     return guarded_ro(ctx, [&](Ctx &c) {
         if (spread) *spread = c.pppm.lastSpread;
         if (gather) *gather = c.pppm.lastGather;
+        if (fft) *fft = c.pppm.lastFft;
     });
 }
 
@@ -2645,6 +2646,10 @@ int mdx_option_set(mdx_ctx *ctx, const char *name, int value) {
         }
         else if (k == "gather_tile_planar") {
             c.pppm.tilePlanar = value != 0;
+        }
+        else if (k == "p3m_fft") {
+            MDX_REQUIRE(value == 0 || value == 1, MDX_ERR_ARG, "p3m_fft: 1 LDS passes, 0 hipFFT");
+            c.pppm.fftMode = value;
         }
         else if (k == "gather_team") {
             MDX_REQUIRE(value == 8 || value == 16, MDX_ERR_ARG, "gather_team: 8 or 16 lanes per charged atom");

@@ -274,7 +274,8 @@ struct PppmState {
     int tileKB = 48;              // LDS budget of a gather tile's field box (KB)
     int tileTeam = 8;             // lanes per charged site in the tile gather
     int tilePlanar = 1;           // 1: the tile gather loads the three planar components (no interleaving pass)
-    int lastSpread = -1, lastGather = -1;  // paths the last evaluation took (mdx_pppm_paths)
+    int fftMode = 1;              // 1: the mesh solve in three LDS passes where the mesh allows, 0: hipFFT
+    int lastSpread = -1, lastGather = -1, lastFft = -1;  // paths the last evaluation took (mdx_pppm_paths)
     int sz[3] = {64, 64, 64};
     int order = 5;
     double rcut = 9.0;
@@ -1115,6 +1116,8 @@ void ring_apply(const RingSpec &r, Ctx &c0);
 // pppm.hip: the mesh part of every bead from the ring's centroids, on the
 // first bead's mesh (centroid force into r.cf, then onto every bead)
 void ring_pppm(Ctx &c0, const RingSpec &r);
+// the mesh solve of these P3M settings runs as the three LDS passes (no hipFFT)
+bool pppm_lds_fft(const PppmState &P);
 
 // flex.hip: molecule indices after a re-sort, M placement, intramolecular
 // forces and the M force spread onto O, H1, H2

@@ -807,7 +807,7 @@ __device__ __forceinline__ void kspace_ev_point(const MeshGeo &m, double alpha, 
 
 // workgroup sums of the seven values, one fixed-point add per value
 __device__ __forceinline__ void kspace_ev_reduce(const double vals[7], unsigned long long *__restrict__ acc) {
-    __shared__ double wred[4][7];
+    __shared__ double wred[16][7];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int s = 0; s < 7; ++s) {
@@ -861,6 +861,390 @@ __device__ __forceinline__ void field_pack_body(int blk, int nblk, int ng, const
     const int g = blk * blockDim.x + threadIdx.x;
     if (g >= ng) return;
     E4[g] = make_float4(E[g], E[ng + g], E[2 * (size_t)ng + g], 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// The mesh solve in three passes, for power-of-two meshes whose planes fit
+// one workgroup's LDS: the transforms of E_field_cu's pipeline
+// (FixChargeEwald.cu: forward R2C of the charge mesh, E_hat_d =
+// i k_d G rho_hat, inverse C2R of each component), here as
+//   pass 1, one workgroup per x plane: the plane's charges (fixed point, read
+//     and cleared for the next spread; or the owner spread's real mesh), real
+//     FFTs along z two rows at a time (one complex FFT of row a + i row b,
+//     split by Hermitian symmetry), complex FFTs along y -> S1[x][ky][kz];
+//   pass 2, one wave per (ky, group of kz columns, component d): complex FFT
+//     along x gives rho_hat(kx, ky, kz); the component-0 waves also sum the
+//     k-space energy and virial (kspace_ev_point); i k_d G rho_hat goes back
+//     along x -> S2[d][x][ky][kz];
+//   pass 3, one workgroup per (x plane, component): inverse complex FFTs along
+//     y, then the Hermitian inverse along z two rows at a time (row a + i row
+//     b, the imaginary parts of the kz = 0 and nz/2 terms dropped as a real
+//     transform does) -> the planar real field E[d][x][y][z].
+// Unnormalised in both directions, forward exp(-i), inverse exp(+i), as the
+// library transforms of the other path; S1 and S2 live in that path's
+// half-spectrum arrays (rho_hat, E_hat).  Every transform belongs to one
+// wave (its stages need no workgroup barrier): a Stockham autosort FFT
+// (natural order in and out) of radix-8 stages and one radix-4 or radix-2
+// stage, each lane doing whole radix-R butterflies in registers between two
+// LDS buffers (Govindaraju et al., "High performance discrete Fourier
+// transforms on graphics processors", SC 2008).
+constexpr int kFftMax = 128;      // longest axis; twiddles exp(-2 pi i k / 128), k < 128
+constexpr int kFftThreads = 256;  // pass 2: 4 waves, one (ky, columns, component) task each
+constexpr int kFftPlaneThreads = 512;  // passes 1 and 3: 8 waves, one group of lines each
+constexpr int kFftWavePts = 512;  // points a wave transforms at a time (lines x length)
+
+__device__ __forceinline__ int fft_log2(int n) { return __ffs(n) - 1; }
+
+// Workgroup barrier ordering LDS only: the passes read global memory that
+// earlier launches wrote and write what later launches read, so their
+// barriers need not wait for outstanding global loads and stores.
+__device__ __forceinline__ void fft_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Between the stages of one wave's transforms: the wave's LDS accesses are
+// processed in order, so this only keeps the compiler from moving them.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+__device__ __forceinline__ void fft_twiddles(float2 *tw) {
+    for (int k = threadIdx.x; k < kFftMax; k += blockDim.x) {
+        float sn, cs;
+        sincospif((float)k / (kFftMax / 2), &sn, &cs);
+        tw[k] = make_float2(cs, -sn);
+    }
+}
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// multiplication by -i (forward) or +i (inverse)
+__device__ __forceinline__ float2 rot_i(float2 a, bool inv) {
+    return inv ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+template <int R>
+__device__ __forceinline__ void small_dft(float2 (&v)[R], bool inv);
+
+template <>
+__device__ __forceinline__ void small_dft<2>(float2 (&v)[2], bool) {
+    const float2 a = v[0], b = v[1];
+    v[0] = make_float2(a.x + b.x, a.y + b.y);
+    v[1] = make_float2(a.x - b.x, a.y - b.y);
+}
+
+template <>
+__device__ __forceinline__ void small_dft<4>(float2 (&v)[4], bool inv) {
+    const float2 t0 = make_float2(v[0].x + v[2].x, v[0].y + v[2].y);
+    const float2 t1 = make_float2(v[0].x - v[2].x, v[0].y - v[2].y);
+    const float2 t2 = make_float2(v[1].x + v[3].x, v[1].y + v[3].y);
+    const float2 t3 = rot_i(make_float2(v[1].x - v[3].x, v[1].y - v[3].y), inv);
+    v[0] = make_float2(t0.x + t2.x, t0.y + t2.y);
+    v[2] = make_float2(t0.x - t2.x, t0.y - t2.y);
+    v[1] = make_float2(t1.x + t3.x, t1.y + t3.y);
+    v[3] = make_float2(t1.x - t3.x, t1.y - t3.y);
+}
+
+template <>
+__device__ __forceinline__ void small_dft<8>(float2 (&v)[8], bool inv) {
+    float2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+    small_dft<4>(e, inv);
+    small_dft<4>(o, inv);
+    const float h = 0.70710678118654752f;
+    // o_k times exp(-+ 2 pi i k / 8)
+    o[1] = inv ? make_float2(h * (o[1].x - o[1].y), h * (o[1].x + o[1].y))
+               : make_float2(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x));
+    o[2] = rot_i(o[2], inv);
+    o[3] = inv ? make_float2(-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y))
+               : make_float2(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v[k] = make_float2(e[k].x + o[k].x, e[k].y + o[k].y);
+        v[k + 4] = make_float2(e[k].x - o[k].x, e[k].y - o[k].y);
+    }
+}
+
+// One Stockham stage of radix R (sub-transforms of length ns done so far) of
+// the calling wave's nl transforms of n points, element e of line l at
+// l * ls + e * es.  Neighbouring lanes take neighbouring lines.
+template <int R>
+__device__ __forceinline__ void wave_stage(const float2 *src, float2 *dst, int n, int ns, int nl, int ls, int es,
+                                           const float2 *tw, bool inv) {
+    const int nb = n / R;
+    const int step = kFftMax / (ns * R);
+    for (int t = threadIdx.x & 63; t < nl * nb; t += 64) {
+        const int j = t / nl, l = t - j * nl;
+        const int jm = j & (ns - 1);
+        const float2 *sp = src + l * ls;
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = sp[(j + r * nb) * es];
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+            float2 w = tw[(r * jm * step) & (kFftMax - 1)];
+            if (inv) w.y = -w.y;
+            v[r] = cmul(v[r], w);
+        }
+        small_dft<R>(v, inv);
+        float2 *dp = dst + l * ls;
+        const int base = (j - jm) * R + jm;
+#pragma unroll
+        for (int r = 0; r < R; ++r) dp[(base + r * ns) * es] = v[r];
+    }
+}
+
+// The calling wave's nl transforms of n points (a power of two, 2..128) from
+// a, with b the other buffer (the same layout); returns the one holding the
+// result.  The wave's earlier LDS writes to a are visible to it.
+__device__ __forceinline__ float2 *wave_fft(float2 *a, float2 *b, int n, int nl, int ls, int es, const float2 *tw,
+                                            bool inv) {
+    int ns = 1;
+    wave_sync();
+    while (ns < n) {
+        const int rem = n / ns;
+        if (rem >= 8) {
+            wave_stage<8>(a, b, n, ns, nl, ls, es, tw, inv);
+            ns *= 8;
+        } else if (rem == 4) {
+            wave_stage<4>(a, b, n, ns, nl, ls, es, tw, inv);
+            ns *= 4;
+        } else {
+            wave_stage<2>(a, b, n, ns, nl, ls, es, tw, inv);
+            ns *= 2;
+        }
+        float2 *x = a;
+        a = b;
+        b = x;
+        wave_sync();
+    }
+    return a;
+}
+
+// true when an n-point wave_fft ends in its first buffer (even stage count)
+__device__ __forceinline__ bool fft_ends_in_a(int n) {
+    int stages = 0;
+    for (int ns = 1; ns < n;) {
+        const int rem = n / ns;
+        ns *= rem >= 8 ? 8 : rem;
+        ++stages;
+    }
+    return (stages & 1) == 0;
+}
+
+__device__ __forceinline__ void fft_pass1_body(int blk, int nblk, const BoxDev *__restrict__ bx, long long *__restrict__ mesh,
+                                               const float *__restrict__ real, float2 *__restrict__ S1) {
+    extern __shared__ float2 fsh[];
+    const MeshGeo m = bx->mesh;
+    const int ny = m.sz[1], nz = m.sz[2], szh = nz / 2 + 1;
+    const int lz = fft_log2(nz);
+    const int rs = nz + 1;  // padded row pitch of the z transforms
+    const int cap = max((ny / 2) * rs, ny * szh);
+    float2 *tw = fsh;
+    float2 *A = tw + kFftMax, *B = A + cap, *Pl = B + cap;
+    fft_twiddles(tw);
+    const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    // the plane as ny / 2 row pairs (pitch rs): row 2r in .x, row 2r + 1 in .y
+    const size_t base = (size_t)blk * ny * nz;
+    const int total = ny * nz;
+    constexpr int U = 8;
+    for (int t0 = threadIdx.x; t0 < total; t0 += U * kFftPlaneThreads) {
+        float v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int t = t0 + u * kFftPlaneThreads;
+            if (t < total) v[u] = mesh ? (float)((double)mesh[base + t] * kMeshInv) : real[base + t];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int t = t0 + u * kFftPlaneThreads;
+            if (t < total) {
+                if (mesh) mesh[base + t] = 0;
+                const int y = t >> lz, z = t & (nz - 1);
+                ((float *)(A + (y >> 1) * rs + z))[y & 1] = v[u];
+            }
+        }
+    }
+    fft_sync();
+    // z: groups of row pairs per wave; each wave splits its own rows into Pl
+    const int lzw = kFftWavePts / nz;
+    const float2 *Z = A;
+    for (int r0 = wave * lzw; r0 < ny / 2; r0 += nw * lzw) {
+        const int nl = min(lzw, ny / 2 - r0);
+        Z = wave_fft(A + r0 * rs, B + r0 * rs, nz, nl, rs, 1, tw, false) - r0 * rs;
+        for (int t = threadIdx.x & 63; t < nl * szh; t += 64) {
+            const int rr = t / szh, k = t - rr * szh, r = r0 + rr;
+            const float2 P = Z[r * rs + k], Q = Z[r * rs + ((nz - k) & (nz - 1))];
+            Pl[2 * r * szh + k] = make_float2(0.5f * (P.x + Q.x), 0.5f * (P.y - Q.y));
+            Pl[(2 * r + 1) * szh + k] = make_float2(0.5f * (P.y + Q.y), -0.5f * (P.x - Q.x));
+        }
+    }
+    fft_sync();
+    // y: groups of kz columns per wave, written out column group by group
+    const int lyw = kFftWavePts / ny;
+    float2 *out = S1 + (size_t)blk * ny * szh;
+    for (int c0 = wave * lyw; c0 < szh; c0 += nw * lyw) {
+        const int nl = min(lyw, szh - c0);
+        const float2 *Y = wave_fft(Pl + c0, A + c0, ny, nl, 1, szh, tw, false) - c0;
+        for (int t = threadIdx.x & 63; t < ny * nl; t += 64) {
+            const int y = t / nl, k = c0 + t - y * nl;
+            out[y * szh + k] = Y[y * szh + k];
+        }
+    }
+}
+
+__device__ __forceinline__ void fft_pass2_body(int blk, int nblk, const BoxDev *__restrict__ bx, const float *__restrict__ G,
+                                               const float2 *__restrict__ S1, float2 *__restrict__ S2, int ev,
+                                               const int *__restrict__ gate, unsigned long long *__restrict__ acc) {
+    extern __shared__ float2 fsh[];
+    float2 *tw = fsh;
+    fft_twiddles(tw);
+    fft_sync();
+    const MeshGeo m = bx->mesh;
+    const int nx = m.sz[0], ny = m.sz[1], szh = m.sz[2] / 2 + 1;
+    const size_t nh = (size_t)nx * ny * szh;
+    const int lw = kFftWavePts / nx;  // kz columns of a wave (a power of two)
+    const int ngr = (szh + lw - 1) / lw;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int task = blk * (blockDim.x >> 6) + wave;
+    if (task >= ny * ngr * 3) return;
+    const int d = task % 3, ky = (task / 3) / ngr, kz0 = ((task / 3) % ngr) * lw;
+    const int nl = min(lw, szh - kz0);
+    float2 *A = tw + kFftMax + wave * 2 * kFftWavePts, *Bf = A + kFftWavePts;
+    // element (x, column l) at A[l + x * lw]; columns beyond szh are zero
+    constexpr int U = kFftWavePts / 64;
+    float2 v[U];
+    float g[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int t = lane + 64 * u, l = t & (lw - 1), x = t / lw;
+        v[u] = make_float2(0.0f, 0.0f);
+        g[u] = 0.0f;
+        if (l < nl) {
+            const size_t i = ((size_t)x * ny + ky) * szh + kz0 + l;
+            v[u] = S1[i];
+            g[u] = G[i];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) A[lane + 64 * u] = v[u];
+    float2 *R = wave_fft(A, Bf, nx, lw, 1, lw, tw, false);
+    float2 *F = R == A ? Bf : A;
+    const bool sums = d == 0 && ev && (!gate || *gate != 0);
+    double vals[7] = {0, 0, 0, 0, 0, 0, 0};
+    const double invV = 1.0 / bx->vol;
+    const float kyv = (2 * ky == ny) ? 0.0f : (float)kcomp(ky, ny, m.L[1]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int t = lane + 64 * u, l = t & (lw - 1), kx = t / lw, kz = kz0 + l;
+        const float2 r = R[t];
+        if (sums && l < nl) {
+            double w[7] = {0, 0, 0, 0, 0, 0, 0};
+            kspace_ev_point(m, bx->alpha, invV, kx, ky, kz, g[u], r, w);
+#pragma unroll
+            for (int s2 = 0; s2 < 7; ++s2) vals[s2] += w[s2];
+        }
+        float kd;
+        if (d == 0) kd = (2 * kx == nx) ? 0.0f : (float)kcomp(kx, nx, m.L[0]);
+        else if (d == 1) kd = kyv;
+        else kd = (2 * kz == m.sz[2]) ? 0.0f : (float)kcomp(kz, m.sz[2], m.L[2]);
+        const float gr = g[u] * r.x, gi = g[u] * r.y;
+        F[t] = make_float2(-kd * gi, kd * gr);
+    }
+    const float2 *X = wave_fft(F, R, nx, lw, 1, lw, tw, true);
+    float2 *out = S2 + d * nh;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int t = lane + 64 * u, l = t & (lw - 1), x = t / lw;
+        if (l < nl) out[((size_t)x * ny + ky) * szh + kz0 + l] = X[t];
+    }
+    if (sums) {
+#pragma unroll
+        for (int s2 = 0; s2 < 7; ++s2) {
+            const double x = team_sum_d<64>(vals[s2]);
+            if (lane == 0) fx_add(&acc_rep(acc)[s2 == 0 ? ACC_KSPACE : ACC_VIR + s2 - 1], x);
+        }
+    }
+}
+
+__device__ __forceinline__ void fft_pass3_body(int blk, int nblk, const BoxDev *__restrict__ bx,
+                                               const float2 *__restrict__ S2, float *__restrict__ E) {
+    extern __shared__ float2 fsh[];
+    const MeshGeo m = bx->mesh;
+    const int nx = m.sz[0], ny = m.sz[1], nz = m.sz[2], szh = nz / 2 + 1;
+    const size_t nh = (size_t)nx * ny * szh, ng = (size_t)nx * ny * nz;
+    const int d = blk / nx, x = blk - d * nx;
+    const int rs = nz + 1;  // padded row pitch of the z transforms
+    const int cap = max((ny / 2) * rs, ny * szh);
+    float2 *tw = fsh;
+    float2 *A = tw + kFftMax, *B = A + cap, *Q = B + cap;
+    fft_twiddles(tw);
+    fft_sync();
+    const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6, lane = threadIdx.x & 63;
+    // y: each wave loads and transforms its groups of kz columns
+    const float2 *in = S2 + d * nh + (size_t)x * ny * szh;
+    const int lyw = kFftWavePts / ny;
+    const float2 *P = fft_ends_in_a(ny) ? A : B;  // the same for every wave's columns
+    for (int c0 = wave * lyw; c0 < szh; c0 += nw * lyw) {
+        const int nl = min(lyw, szh - c0);
+        constexpr int U = 8;
+        for (int t0 = lane; t0 < ny * nl; t0 += 64 * U) {
+            float2 v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int t = t0 + 64 * u;
+                if (t < ny * nl) {
+                    const int y = t / nl, k = c0 + t - y * nl;
+                    v[u] = in[y * szh + k];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int t = t0 + 64 * u;
+                if (t < ny * nl) {
+                    const int y = t / nl, k = c0 + t - y * nl;
+                    A[y * szh + k] = v[u];
+                }
+            }
+        }
+        wave_fft(A + c0, B + c0, ny, nl, 1, szh, tw, true);
+    }
+    fft_sync();
+    // z: each wave builds and transforms its groups of row pairs
+    float2 *Pw = P == A ? B : A;  // free now (every wave's y transforms are done)
+    const int lzw = kFftWavePts / nz;
+    float *out = E + d * ng + (size_t)x * ny * nz;
+    for (int r0 = wave * lzw; r0 < ny / 2; r0 += nw * lzw) {
+        const int nl = min(lzw, ny / 2 - r0);
+        for (int t = lane; t < nl * nz; t += 64) {
+            const int rr = t / nz, k = t - rr * nz, r = r0 + rr;
+            float2 a, b;
+            if (2 * k <= nz) {
+                a = P[2 * r * szh + k];
+                b = P[(2 * r + 1) * szh + k];
+                if (k == 0 || 2 * k == nz) {
+                    a.y = 0.0f;
+                    b.y = 0.0f;
+                }
+            } else {
+                a = P[2 * r * szh + nz - k];
+                b = P[(2 * r + 1) * szh + nz - k];
+                a.y = -a.y;
+                b.y = -b.y;
+            }
+            Q[r * rs + k] = make_float2(a.x - b.y, a.y + b.x);
+        }
+        const float2 *Z = wave_fft(Q + r0 * rs, Pw + r0 * rs, nz, nl, rs, 1, tw, true) - r0 * rs;
+        for (int t = lane; t < 2 * nl * nz; t += 64) {
+            const int y = 2 * r0 + t / nz, z = t & (nz - 1);
+            out[y * nz + z] = ((const float *)(Z + (y >> 1) * rs + z))[y & 1];
+        }
+    }
 }
 
 template <int ORDER, int T>
@@ -1383,6 +1767,73 @@ static void pppm_plans_on(PppmState &P, hipStream_t s) {
     P.planStream = s;
 }
 
+// LDS bytes of passes 1 and 3 of the mesh solve, 0 where the mesh does not
+// allow it (every axis a power of two from 4 to 128, the planes within
+// 60 KB).  Pass 2 takes kFftMax + 2 * kFftWavePts points per wave.
+static size_t lds_fft_shm(const PppmState &P) {
+    for (int a = 0; a < 3; ++a) {
+        const int n = P.sz[a];
+        if (n < 4 || n > kFftMax || (n & (n - 1))) return 0;
+    }
+    const size_t szh = P.sz[2] / 2 + 1;
+    const size_t cap = std::max((size_t)(P.sz[1] / 2) * (P.sz[2] + 1), (size_t)P.sz[1] * szh);
+    const size_t b = sizeof(float2) * (kFftMax + 3 * cap);
+    return b <= 60 * 1024 ? b : 0;
+}
+
+bool pppm_lds_fft(const PppmState &P) { return P.fftMode == 1 && lds_fft_shm(P) > 0; }
+
+// Charge mesh -> the planar real field P.eReal, with the k-space energy and
+// virial when ev is set (and the gate, if any, open).  fromReal: the charges
+// are in P.real (owner spread) instead of the fixed-point mesh.
+static void mesh_solve(Ctx &c, bool fromReal, int ev, const int *gate) {
+    PppmState &P = c.pppm;
+    const int *sz = P.sz;
+    const int szh = sz[2] / 2 + 1;
+    const int ng = sz[0] * sz[1] * sz[2];
+    const int nh = sz[0] * sz[1] * szh;
+    const int threads = 256;
+    const size_t shm = pppm_lds_fft(P) ? lds_fft_shm(P) : 0;
+    if (shm) {
+        P.lastFft = 1;
+        ProfScope ps(c, PK_FFT);
+        const int waves = kFftThreads / 64;
+        const int lw = kFftWavePts / sz[0], tasks = sz[1] * ((szh + lw - 1) / lw) * 3;
+        const size_t shm2 = sizeof(float2) * (kFftMax + waves * 2 * kFftWavePts);
+        launch<fft_pass1_body, kFftPlaneThreads>(c, sz[0], shm, c.box, fromReal ? nullptr : P.mesh,
+                                                 fromReal ? P.real : nullptr, P.rhoHat);
+        launch<fft_pass2_body, kFftThreads>(c, cdiv(tasks, waves), shm2, c.box, P.green, P.rhoHat, P.eHat, ev, gate,
+                                            c.acc);
+        launch<fft_pass3_body, kFftPlaneThreads>(c, 3 * sz[0], shm, c.box, P.eHat, P.eReal);
+        return;
+    }
+    P.lastFft = 0;
+    if (!fromReal) {
+        ProfScope ps(c, PK_SPREAD);
+        launch<mesh_to_real_body, 256>(c, cdiv(ng, threads), 0, ng, P.mesh, P.real);
+    }
+    {
+        ProfScope ps(c, PK_FFT);
+        stream_op(c, [&P](hipStream_t s) {
+            pppm_plans_on(P, s);
+            MDX_REQUIRE(hipfftExecR2C(P.planR2C, P.real, (hipfftComplex *)P.rhoHat) == HIPFFT_SUCCESS, 2,
+                        "hipfftExecR2C failed");
+        }, 1);
+    }
+    {
+        ProfScope ps(c, PK_EFIELD);
+        launch<efield_body, 256>(c, cdiv(nh, threads), 0, c.box, szh, P.green, P.rhoHat, P.eHat, ev, gate, c.acc);
+    }
+    {
+        ProfScope ps(c, PK_FFT);
+        stream_op(c, [&P](hipStream_t s) {
+            pppm_plans_on(P, s);
+            MDX_REQUIRE(hipfftExecC2R(P.planC2R, (hipfftComplex *)P.eHat, P.eReal) == HIPFFT_SUCCESS, 2,
+                        "hipfftExecC2R failed");
+        }, 2);
+    }
+}
+
 void pppm_forces(Ctx &c, bool energy, bool virial) {
     PppmState &P = c.pppm;
     pppm_prepare(c);
@@ -1436,33 +1887,11 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
         case 5: launch<spread_body<5>, 256>(c, cdiv(c.n, threads), 0, c.n, c.pos[cur], c.q[cur], c.box, qunit, (unsigned long long *)P.mesh); break;
         default: launch<spread_body<7>, 256>(c, cdiv(c.n, threads), 0, c.n, c.pos[cur], c.q[cur], c.box, qunit, (unsigned long long *)P.mesh); break;
     }
-    launch<mesh_to_real_body, 256>(c, cdiv(ng, threads), 0, ng, P.mesh, P.real);
     }
     }
-    {
-    ProfScope ps(c, PK_FFT);
-    stream_op(c, [&P](hipStream_t s) {
-        pppm_plans_on(P, s);
-        MDX_REQUIRE(hipfftExecR2C(P.planR2C, P.real, (hipfftComplex *)P.rhoHat) == HIPFFT_SUCCESS, 2,
-                    "hipfftExecR2C failed");
-    }, 1);
-    }
-    {
-        // energy and virial in the field pass; a trace-only request (a
-        // barostat's) is skipped on steps it does not serve
-        ProfScope ps(c, PK_EFIELD);
-        const int *gate = c.vtrace ? baro_gate(c) : nullptr;
-        launch<efield_body, 256>(c, cdiv(nh, threads), 0, c.box, szh, P.green, P.rhoHat, P.eHat,
-                                 (energy || virial) ? 1 : 0, gate, c.acc);
-    }
-    {
-        ProfScope ps(c, PK_FFT);
-        stream_op(c, [&P](hipStream_t s) {
-            pppm_plans_on(P, s);
-            MDX_REQUIRE(hipfftExecC2R(P.planC2R, (hipfftComplex *)P.eHat, P.eReal) == HIPFFT_SUCCESS, 2,
-                        "hipfftExecC2R failed");
-        }, 2);
-    }
+    // energy and virial in the field pass; a trace-only request (a
+    // barostat's) is skipped on steps it does not serve
+    mesh_solve(c, P.lastSpread == 2, (energy || virial) ? 1 : 0, c.vtrace ? baro_gate(c) : nullptr);
     // a built list puts the charged atoms first: only they take the gather
     const int na = c.listValid ? c.nCharged : c.n;
     ProfScope ps(c, PK_GATHER);
@@ -1532,28 +1961,8 @@ void ring_pppm(Ctx &c0, const RingSpec &r) {
             case 5: launch<spread_body<5>, 256>(c0, cdiv(r.natoms, threads), 0, r.natoms, r.cen, r.q, c0.box, qunit, (unsigned long long *)P.mesh); break;
             default: launch<spread_body<7>, 256>(c0, cdiv(r.natoms, threads), 0, r.natoms, r.cen, r.q, c0.box, qunit, (unsigned long long *)P.mesh); break;
         }
-        launch<mesh_to_real_body, 256>(c0, cdiv(ng, threads), 0, ng, P.mesh, P.real);
     }
-    {
-        ProfScope ps(c0, PK_FFT);
-        stream_op(c0, [&P](hipStream_t s) {
-            pppm_plans_on(P, s);
-            MDX_REQUIRE(hipfftExecR2C(P.planR2C, P.real, (hipfftComplex *)P.rhoHat) == HIPFFT_SUCCESS, 2,
-                        "hipfftExecR2C failed");
-        });
-    }
-    {
-        ProfScope ps(c0, PK_EFIELD);
-        launch<efield_body, 256>(c0, cdiv(nh, threads), 0, c0.box, szh, P.green, P.rhoHat, P.eHat, 0, nullptr, c0.acc);
-    }
-    {
-        ProfScope ps(c0, PK_FFT);
-        stream_op(c0, [&P](hipStream_t s) {
-            pppm_plans_on(P, s);
-            MDX_REQUIRE(hipfftExecC2R(P.planC2R, (hipfftComplex *)P.eHat, P.eReal) == HIPFFT_SUCCESS, 2,
-                        "hipfftExecC2R failed");
-        });
-    }
+    mesh_solve(c0, false, 0, nullptr);
     ProfScope ps(c0, PK_GATHER);
     launch<field_pack_body, 256>(c0, cdiv(ng, threads), 0, ng, P.eReal, P.eField4);
     const dim3 grid(xcd_grid(cdiv((long long)r.natoms * 8, threads)));

@@ -290,6 +290,41 @@ def test_spread_paths_bit_identical(gpu_engine_factory, sz):
     assert np.array_equal(out[2], out[0])
 
 
+@pytest.mark.parametrize("sz", [(32, 32, 32), (64, 32, 32), (128, 32, 32), (64, 64, 64), (32, 64, 128)])
+def test_mesh_solve_paths(orc, gpu_engine_factory, sz):
+    """The mesh solve in three LDS passes (real FFT along z two rows at a time,
+    complex FFTs along y and x, the field and the k-space energy and virial
+    beside rho_hat, the inverse likewise) against the FP64 oracle's P3M and
+    against the hipFFT path; a mesh whose planes do not fit a workgroup's LDS
+    (32 x 64 x 128) takes the hipFFT path."""
+    say(f"mesh solve paths, mesh {sz}")
+    from md_engine_amd._lib import ENERGY, PPPM, VIRIAL
+
+    box = systems.tip4p2005_box(512, seed=15)
+    ref = oracle_water(orc, box, sz=sz)
+    e = water_engine(gpu_engine_factory, box, sz=sz)
+    lds = sz != (32, 64, 128)
+    out = {}
+    for mode in (1, 0):
+        e.option("p3m_fft", mode)
+        e.compute(PPPM | ENERGY | VIRIAL)
+        assert e.pppm_paths()[2] == (mode if lds else 0)
+        f = e.forces()
+        en, vk = e.energies()
+        say(f"fft {mode}: force RMSE {rel_rmse(f, ref['fk']):.2e}, energy {en[1]:.6f} vs {ref['ek']:.6f}")
+        assert rel_rmse(f, ref["fk"]) <= FORCE_RMSE
+        assert en[1] == pytest.approx(ref["ek"], rel=ENERGY_REL)
+        assert np.allclose(vk, ref["vk"], rtol=1e-4, atol=1e-4 * np.abs(ref["vk"]).max())
+        out[mode] = (f, en[1], vk)
+    # both FP32 transforms of the same mesh: agreement to rounding
+    assert rel_rmse(out[1][0], out[0][0]) <= 2e-6
+    assert out[1][1] == pytest.approx(out[0][1], rel=1e-6)
+    # a second evaluation starts from the cleared charge mesh
+    e.option("p3m_fft", 1)
+    e.compute(PPPM)
+    assert np.array_equal(e.forces(), out[1][0])
+
+
 @pytest.mark.parametrize("order", [1, 3, 5, 7])
 def test_gather_paths(orc, gpu_engine_factory, order):
     """The gathers (LDS tiles of the field, row teams from the per-site mesh
@@ -306,7 +341,7 @@ def test_gather_paths(orc, gpu_engine_factory, order):
         e.option("p3m_gather", mode)
         e.compute(PPPM)
         # order 7's field box does not fit the LDS here: the records' row teams
-        assert e.pppm_paths() == (1, 1 if (mode == 2 and order == 7) else mode)
+        assert e.pppm_paths()[:2] == (1, 1 if (mode == 2 and order == 7) else mode)
         out[mode] = e.forces()
     assert np.array_equal(out[2], out[1])
     assert np.array_equal(out[1], out[0])
