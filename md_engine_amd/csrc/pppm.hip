@@ -1138,7 +1138,9 @@ __device__ __forceinline__ void fft_pass2_body(int blk, int nblk, const BoxDev *
     const bool sums = d == 0 && ev && (!gate || *gate != 0);
     double vals[7] = {0, 0, 0, 0, 0, 0, 0};
     const double invV = 1.0 / bx->vol;
-    const float kyv = (2 * ky == ny) ? 0.0f : (float)kcomp(ky, ny, m.L[1]);
+    // k_d of this wave's component: 2 pi m / L_d in FP32 (0 on the Nyquist plane)
+    const int nd = m.sz[d];
+    const float kunit = (float)(2.0 * kPi / m.L[d]);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int t = lane + 64 * u, l = t & (lw - 1), kx = t / lw, kz = kz0 + l;
@@ -1149,10 +1151,9 @@ __device__ __forceinline__ void fft_pass2_body(int blk, int nblk, const BoxDev *
 #pragma unroll
             for (int s2 = 0; s2 < 7; ++s2) vals[s2] += w[s2];
         }
-        float kd;
-        if (d == 0) kd = (2 * kx == nx) ? 0.0f : (float)kcomp(kx, nx, m.L[0]);
-        else if (d == 1) kd = kyv;
-        else kd = (2 * kz == m.sz[2]) ? 0.0f : (float)kcomp(kz, m.sz[2], m.L[2]);
+        const int id = d == 0 ? kx : (d == 1 ? ky : kz);
+        const int mi = 2 * id > nd ? id - nd : id;
+        const float kd = 2 * id == nd ? 0.0f : kunit * (float)mi;
         const float gr = g[u] * r.x, gi = g[u] * r.y;
         F[t] = make_float2(-kd * gi, kd * gr);
     }
