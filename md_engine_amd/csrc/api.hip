@@ -204,8 +204,10 @@ void compute(Ctx &c, int flags) {
     // from the refreshed positions and add into the fixed-point force and
     // accumulator buffers, whose integer sums do not depend on the order.
     const bool fork = mesh && c.capture && !c.rec && c.side && c.meshStream;
+    const bool forked = c.forked;  // recorded by this step's rebuild after its copy-back
+    c.forked = false;
     if (fork) {
-        MDX_HIP(hipEventRecord(c.evFork, c.stream));
+        if (!forked) MDX_HIP(hipEventRecord(c.evFork, c.stream));
         MDX_HIP(hipStreamWaitEvent(c.side, c.evFork, 0));
         std::swap(c.stream, c.side);
         try {

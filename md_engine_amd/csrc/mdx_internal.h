@@ -523,6 +523,7 @@ struct Ctx {
     bool rowMaxPending = false;   // a captured build's row maxima go with the next position refresh
     bool decided = false;         // the first half kick of this captured step took the step's decisions
     bool refreshed = false;       // ... and refreshed xq, the mesh records, forces and accumulators
+    bool forked = false;          // the captured rebuild recorded the mesh stream's fork after its copy-back
     bool fuseStep = true;         // option "fuse_step": captured steps fold small launches into others
     int graphSteps = 0;           // mdx_run: 1 = captured steps (option "graph")
     unsigned *dMax = nullptr;     // 2 slots x (max |x - x_last|^2, max predicted^2) as float bits

@@ -1294,6 +1294,14 @@ static void run_build(Ctx &c, bool sync) {
         // a captured step keeps fixed buffers: the sorted records come back,
         // with the exclusion rows and the cleared E3B oxygen map
         launch<copy_back_body, 256>(c, cdiv(c.n, threads), 0, rebuild_gate(c), c.n, c.pos[nxt], c.vel[nxt], c.q[nxt], c.type[nxt], c.id[nxt], c.pos[cur], c.vel[cur], c.q[cur], c.type[cur], c.id[cur], c.exclOff, c.exclIds, c.idToIdx, excl ? c.maxExcl : 0, c.exclIdx, c.e3b.on ? c.e3b.atomMolO : nullptr, c.rigid.on ? c.rigid.member : nullptr);
+        // The P3M chain needs the sorted sites, their mesh records (the
+        // permutation wrote them, the first half kick having refreshed them)
+        // and the cells, not the list: its stream forks here, so the list
+        // build below runs beside it.
+        if (c.refreshed && c.side && c.meshStream && !c.rec && c.pppm.on) {
+            MDX_HIP(hipEventRecord(c.evFork, c.stream));
+            c.forked = true;
+        }
     } else {
         c.cur = nxt;
         if (excl)
