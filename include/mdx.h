@@ -357,7 +357,11 @@ int mdx_sync(mdx_ctx *ctx);
  * site; "gather_tile_planar" 1 (default) loads the three planar field
  * components, 0 interleaves them first), 1 row teams from the per-site
  * mesh records of the position refresh, 0 row teams computing the mesh point
- * from the positions (identical forces); "mesh_stream" 1 (default) runs the
+ * from the positions (identical forces); "p3m_fft" 1 (default) solves the mesh
+ * in three hand-written passes (FFTs in LDS with the charge conversion, the
+ * field and the k-space energy and virial fused; power-of-two meshes whose
+ * planes fit a workgroup, else hipFFT), 0 hipFFT with separate conversion and
+ * field kernels (forces equal to FP32 rounding); "mesh_stream" 1 (default) runs the
  * P3M chain of a captured step on a second stream beside the pair and E3B
  * kernels, 0 on one stream (identical trajectories); "graph" 1 (default)
  * runs mdx_run as captured 8-step graphs, 0 eagerly; "build_mode" 0 or 2
