@@ -524,6 +524,7 @@ struct Ctx {
     bool decided = false;         // the first half kick of this captured step took the step's decisions
     bool refreshed = false;       // ... and refreshed xq, the mesh records, forces and accumulators
     bool forked = false;          // the captured rebuild recorded the mesh stream's fork after its copy-back
+    bool baroFused = false;       // the rigid second half of this captured step ran the barostat step
     bool fuseStep = true;         // option "fuse_step": captured steps fold small launches into others
     int graphSteps = 0;           // mdx_run: 1 = captured steps (option "graph")
     unsigned *dMax = nullptr;     // 2 slots x (max |x - x_last|^2, max predicted^2) as float bits
@@ -737,6 +738,136 @@ __device__ __forceinline__ void fx_add(unsigned long long *p, double v) {
 __device__ __forceinline__ unsigned long long *acc_rep(unsigned long long *acc) {
     return acc + (blockIdx.x & (kAccRep - 1)) * ACC_N;
 }
+
+// ---- Berendsen barostat step on the device (FixPressureBerendsen::stepFinal,
+// src/Fixes/FixPressureBerendsen.cu:20-51; alpha as handleBoundsChange,
+// src/Fixes/FixChargeEwald.cu:1200-1229), run by one wave: its own launch or
+// the last workgroup of the rigid second half.
+struct BaroArgs {
+    double P0, period, bulk, maxDil, nktv2p, mvv, dt;
+    // P3M alpha re-solve (alphaSolve = 0: alpha stays as uploaded)
+    int alphaSolve, natoms, order;
+    double q2, rcut;
+    // cell edges below these lengths invalidate the grid
+    double minL[3];
+    // the centre-of-mass M V^2 was summed before a velocity rescale of the same
+    // step (tgate open): scale it by that rescale's lambda^2 (k_vscale's formula)
+    int rescaled, tmode;
+    const int *tgate;
+    const unsigned long long *thermo;
+    double tconv, T0, coupling;
+};
+
+// The secant solve of pppm_solve_alpha for a fixed interpolation order, with
+// the box-dependent factors taken out of the iteration and the powers
+// unrolled into multiplications (one device thread runs this dependent
+// chain; pow would cost an exp and a log per term).  Same functions
+// DeltaF_k - DeltaF_real, start 4.79853 / rcut, tolerance 1e-5 and
+// iteration limit as the host solve.
+template <int ORDER>
+__device__ double alpha_solve_dev(int natoms, double q2, const double *L, const int *sz, double rcut) {
+    double h[3], cL[3], sL[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        h[d] = L[d] / sz[d];
+        cL[d] = 1.0 / (L[d] * L[d]);
+        sL[d] = L[d] / natoms * sqrt(2.0 * M_PI);
+    }
+    const double R0 = 2 * q2 / sqrt(natoms * rcut * L[0] * L[1] * L[2]), rc2 = rcut * rcut, q3 = q2 / 3.0;
+    auto dfk = [&](double a) {
+        double acc = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double x = h[d] * a, x2 = x * x;
+            double sum = 0.0, p = 1.0, po = 1.0;
+#pragma unroll
+            for (int mm = 0; mm < ORDER; ++mm) {
+                sum += pppm_amp(ORDER, mm) * p;
+                p *= x2;
+                po *= x;
+            }
+            acc += cL[d] * po * sqrt(a * sL[d] * sum);
+        }
+        return q3 * acc;
+    };
+    auto dfr = [&](double a) { return R0 * exp(-a * a * rc2); };
+    double xa = 0.0, xb = 4.79853 / rcut;
+    double ya = dfk(xa) - dfr(xa);
+    double fr = dfr(xb);
+    double yb = dfk(xb) - fr;
+    for (int it = 0; it < 100 && fabs(yb) / fr > 1e-5; ++it) {
+        const double kinv = (xb - xa) / (yb - ya);
+        ya = yb;
+        xa = xb;
+        xb = xa - ya * kinv;
+        fr = dfr(xb);
+        yb = dfk(xb) - fr;
+    }
+    return xb;
+}
+
+__device__ inline double alpha_solve_any(int order, int natoms, double q2, const double *L, const int *sz, double rcut) {
+    switch (order) {
+        case 1: return alpha_solve_dev<1>(natoms, q2, L, sz, rcut);
+        case 2: return alpha_solve_dev<2>(natoms, q2, L, sz, rcut);
+        case 3: return alpha_solve_dev<3>(natoms, q2, L, sz, rcut);
+        case 4: return alpha_solve_dev<4>(natoms, q2, L, sz, rcut);
+        case 5: return alpha_solve_dev<5>(natoms, q2, L, sz, rcut);
+        case 6: return alpha_solve_dev<6>(natoms, q2, L, sz, rcut);
+        default: return alpha_solve_dev<7>(natoms, q2, L, sz, rcut);
+    }
+}
+
+// One wave: the accumulator replicas are summed across lanes (integer sums,
+// exact in any order), then lane 0 computes the pressure, the dilation factor,
+// the new box and, for P3M, alpha and the Green function's image range.
+__device__ __forceinline__ void baro_wave(const int *__restrict__ gate, BoxDev *__restrict__ bx, const BoxParams &bp,
+                                          const BaroArgs &ba, const unsigned long long *__restrict__ acc) {
+    if (gate && *gate == 0) return;
+    const int lane = threadIdx.x & 63;
+    constexpr int kSlots[6] = {ACC_VIR, ACC_VIR + 1, ACC_VIR + 2, ACC_MOL, ACC_MOL + 1, ACC_VTRACE};
+    long long t[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        t[k] = lane < kAccRep ? (long long)acc[lane * ACC_N + kSlots[k]] : 0ll;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) t[k] += __shfl_xor(t[k], o);
+    }
+    if (lane != 0) return;
+    // trace of the virial: the tensor's diagonal plus the trace-only terms
+    const double W = (double)t[0] * kFxInv + (double)t[1] * kFxInv + (double)t[2] * kFxInv + (double)t[5] * kFxInv;
+    const double corr = (double)t[3] * kFxInv;
+    double mv2 = (double)t[4] * kFxInv;
+    if (ba.rescaled && (!ba.tgate || *ba.tgate)) {
+        const double T = (double)(long long)ba.thermo[0] * kFxInv * ba.tconv;
+        if (T > 0.0) {
+            const double l2 = ba.tmode == 0 ? ba.T0 / T : 1.0 + ba.coupling * (ba.T0 / T - 1.0);
+            mv2 *= fmax(l2, 0.0);
+        }
+    }
+    const double P = (mv2 * ba.mvv + W - corr) / (3.0 * bx->vol) * ba.nktv2p;
+    double mu = cbrt(1.0 - ba.dt / ba.period * (ba.P0 - P) / ba.bulk);
+    mu = fmin(fmax(mu, 1.0 - ba.maxDil), 1.0 + ba.maxDil);
+    bx->pressure = P;
+    bx->mu = mu;
+    double lo[3], L[3];
+    for (int d = 0; d < 3; ++d) {
+        const double ctr = bx->geo.lo[d] + 0.5 * bx->geo.L[d];
+        L[d] = bx->geo.L[d] * mu;
+        lo[d] = ctr - 0.5 * L[d];
+        if (bp.periodic[d] && L[d] < ba.minL[d]) bx->regrid = 1;
+    }
+    // box_derive leaves alpha, alphaF and greenLim as they are
+    box_derive(*bx, bp, lo, L);
+    if (ba.alphaSolve) {
+        const double a = alpha_solve_any(ba.order, ba.natoms, ba.q2, L, bp.meshSz, ba.rcut);
+        bx->alpha = a;
+        bx->alphaF = (float)a;
+        const double hm = cbrt(bx->mesh.h[0] * bx->mesh.h[1] * bx->mesh.h[2]);
+        bx->greenLim = green_lim(a, hm);
+    }
+}
+
 
 // sub-wave team sum by butterfly shuffles (T divides 64)
 template <int T>
@@ -1084,6 +1215,7 @@ void box_pull(Ctx &c);
 // rescale: the thermostat's velocity rescale of this step goes with the
 // barostat's launches (its m v^2 summed by the second half kick)
 void baro_step(Ctx &c, const int *gate, bool rescale = false);
+BaroArgs baro_args(const Ctx &c);  // the barostat step's arguments for this context
 // Green function from the device box on stream s (gate: nullptr = always)
 // dl != nullptr: the same launch also dilates (extra workgroups after the
 // Green function's); returns whether it did

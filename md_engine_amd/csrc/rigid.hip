@@ -276,13 +276,23 @@ __device__ __forceinline__ void rigid_vv1_body(int blk, int nblk, RigidArgs r, d
 // k_mol_virial) into acc[ACC_MOL], the barostat's pressure without a pass of
 // its own, and (doTh) the sum of m v^2 over the sites (as k_mv2) into
 // thermo[0] for the velocity rescale.
+// the barostat step for the last workgroup of the rigid second half
+// (ticket = nullptr: none)
+struct BaroFuse {
+    int *ticket;
+    const int *gate;
+    BoxDev *bx;
+    BoxParams bp;
+    BaroArgs ba;
+};
+
 template <bool SUMS>
 __device__ __forceinline__ void rigid_vv2_body(int blk, int nblk, RigidArgs r, double4 *__restrict__ pos, double4 *__restrict__ vel,
                                                    const unsigned long long *__restrict__ f, int kick,
                                                    unsigned long long *__restrict__ acc,
                                                    const int *__restrict__ vgate, int doVir,
                                                    unsigned long long *__restrict__ thermo,
-                                                   const int *__restrict__ tgate, int doTh) {
+                                                   const int *__restrict__ tgate, int doTh, BaroFuse bf) {
     r.g = r.bx->geo;
     const int m = blk * blockDim.x + threadIdx.x;
     // uniform over the grid
@@ -349,6 +359,8 @@ __device__ __forceinline__ void rigid_vv2_body(int blk, int nblk, RigidArgs r, d
         }
         if (th && threadIdx.x == 2) fx_add(&thermo[0], red[2][0] + red[2][1]);
     }
+    // the last workgroup sees every workgroup's sums: the barostat step
+    if (bf.ticket && last_block(bf.ticket, nblk) && threadIdx.x < 64) baro_wave(bf.gate, bf.bx, bf.bp, bf.ba, acc);
 }
 
 // molecule -> current site indices, and the per-site membership flag that
@@ -516,10 +528,21 @@ void rigid_first_half(Ctx &c) {
 
 void rigid_second_half(Ctx &c, bool kick) {
     RigidArgs r = make_args(c);
+    BaroFuse bf{};
+    // a captured all-rigid barostat step: the pressure, box and alpha in the
+    // last workgroup (the molecular virial and M V^2 are summed here)
+    if (kick && c.fuseMolVirial && c.fuseStep && c.baro.on && c.capture) {
+        bf.ticket = c.dFlags + 9;
+        bf.gate = baro_gate(c);
+        bf.bx = c.box;
+        bf.bp = box_params(c);
+        bf.ba = baro_args(c);
+        c.baroFused = true;
+    }
     if (kick && (c.fuseMolVirial || c.fuseThermo))
-        launch<rigid_vv2_body<true>, 128>(c, cdiv(r.nmol, 128), 0, r, c.pos[c.cur], c.vel[c.cur], c.force, 1, c.acc, baro_gate(c), c.fuseMolVirial ? 1 : 0, c.thermo, thermo_gate(c), c.fuseThermo ? 1 : 0);
+        launch<rigid_vv2_body<true>, 128>(c, cdiv(r.nmol, 128), 0, r, c.pos[c.cur], c.vel[c.cur], c.force, 1, c.acc, baro_gate(c), c.fuseMolVirial ? 1 : 0, c.thermo, thermo_gate(c), c.fuseThermo ? 1 : 0, bf);
     else
-        launch<rigid_vv2_body<false>, 128>(c, cdiv(r.nmol, 128), 0, r, c.pos[c.cur], c.vel[c.cur], c.force, kick ? 1 : 0, c.acc, nullptr, 0, nullptr, nullptr, 0);
+        launch<rigid_vv2_body<false>, 128>(c, cdiv(r.nmol, 128), 0, r, c.pos[c.cur], c.vel[c.cur], c.force, kick ? 1 : 0, c.acc, nullptr, 0, nullptr, nullptr, 0, bf);
 }
 
 }  // namespace mdx
