@@ -873,9 +873,9 @@ __device__ __forceinline__ void field_pack_body(int blk, int nblk, int ng, const
 //     FFTs along z two rows at a time (one complex FFT of row a + i row b,
 //     split by Hermitian symmetry), complex FFTs along y -> S1[x][ky][kz];
 //   pass 2, one wave per (ky, group of kz columns, component d): complex FFT
-//     along x gives rho_hat(kx, ky, kz); the component-0 waves also sum the
-//     k-space energy and virial (kspace_ev_point); i k_d G rho_hat goes back
-//     along x -> S2[d][x][ky][kz];
+//     along x gives rho_hat(kx, ky, kz), and i k_d G rho_hat goes back along
+//     x -> S2[d][x][ky][kz]; with energy or virial requested a fourth wave per
+//     column group sums them (kspace_ev_point) from its own rho_hat;
 //   pass 3, one workgroup per (x plane, component): inverse complex FFTs along
 //     y, then the Hermitian inverse along z two rows at a time (row a + i row
 //     b, the imaginary parts of the kz = 0 and nz/2 terms dropped as a real
@@ -894,6 +894,11 @@ constexpr int kFftPlaneThreads = 512;  // passes 1 and 3: 8 waves, one group of 
 constexpr int kFftWavePts = 512;  // points a wave transforms at a time (lines x length)
 
 __device__ __forceinline__ int fft_log2(int n) { return __ffs(n) - 1; }
+
+// t / n for 0 <= t < 2^16 and 0 < n <= 2^10 from a float reciprocal: the
+// quotient of (t + 0.5) / n lies at least 0.5 / n from an integer, far above
+// the reciprocal's rounding, so truncation gives the exact integer quotient.
+__device__ __forceinline__ int fft_div(int t, float rn) { return (int)(((float)t + 0.5f) * rn); }
 
 // Workgroup barrier ordering LDS only: the passes read global memory that
 // earlier launches wrote and write what later launches read, so their
@@ -978,8 +983,9 @@ __device__ __forceinline__ void wave_stage(const float2 *src, float2 *dst, int n
                                            const float2 *tw, bool inv) {
     const int nb = n / R;
     const int step = kFftMax / (ns * R);
+    const float rnl = 1.0f / (float)nl;
     for (int t = threadIdx.x & 63; t < nl * nb; t += 64) {
-        const int j = t / nl, l = t - j * nl;
+        const int j = fft_div(t, rnl), l = t - j * nl;
         const int jm = j & (ns - 1);
         const float2 *sp = src + l * ls;
         float2 v[R];
@@ -1077,8 +1083,9 @@ __device__ __forceinline__ void fft_pass1_body(int blk, int nblk, const BoxDev *
     for (int r0 = wave * lzw; r0 < ny / 2; r0 += nw * lzw) {
         const int nl = min(lzw, ny / 2 - r0);
         Z = wave_fft(A + r0 * rs, B + r0 * rs, nz, nl, rs, 1, tw, false) - r0 * rs;
+        const float rszh = 1.0f / (float)szh;
         for (int t = threadIdx.x & 63; t < nl * szh; t += 64) {
-            const int rr = t / szh, k = t - rr * szh, r = r0 + rr;
+            const int rr = fft_div(t, rszh), k = t - rr * szh, r = r0 + rr;
             const float2 P = Z[r * rs + k], Q = Z[r * rs + ((nz - k) & (nz - 1))];
             Pl[2 * r * szh + k] = make_float2(0.5f * (P.x + Q.x), 0.5f * (P.y - Q.y));
             Pl[(2 * r + 1) * szh + k] = make_float2(0.5f * (P.y + Q.y), -0.5f * (P.x - Q.x));
@@ -1091,8 +1098,9 @@ __device__ __forceinline__ void fft_pass1_body(int blk, int nblk, const BoxDev *
     for (int c0 = wave * lyw; c0 < szh; c0 += nw * lyw) {
         const int nl = min(lyw, szh - c0);
         const float2 *Y = wave_fft(Pl + c0, A + c0, ny, nl, 1, szh, tw, false) - c0;
+        const float rnl = 1.0f / (float)nl;
         for (int t = threadIdx.x & 63; t < ny * nl; t += 64) {
-            const int y = t / nl, k = c0 + t - y * nl;
+            const int y = fft_div(t, rnl), k = c0 + t - y * nl;
             out[y * szh + k] = Y[y * szh + k];
         }
     }
@@ -1107,13 +1115,19 @@ __device__ __forceinline__ void fft_pass2_body(int blk, int nblk, const BoxDev *
     fft_sync();
     const MeshGeo m = bx->mesh;
     const int nx = m.sz[0], ny = m.sz[1], szh = m.sz[2] / 2 + 1;
-    const size_t nh = (size_t)nx * ny * szh;
+    const int nh = nx * ny * szh;
     const int lw = kFftWavePts / nx;  // kz columns of a wave (a power of two)
+    const int llw = fft_log2(lw);
     const int ngr = (szh + lw - 1) / lw;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int task = blk * (blockDim.x >> 6) + wave;
-    if (task >= ny * ngr * 3) return;
-    const int d = task % 3, ky = (task / 3) / ngr, kz0 = ((task / 3) % ngr) * lw;
+    // task kinds: components 0..2, and with ev a fourth wave per column group
+    // that only sums the k-space energy and virial (off the field waves' path)
+    const int kinds = ev ? 4 : 3;
+    if (task >= ny * ngr * kinds) return;
+    const int d = task % kinds, ky = (task / kinds) / ngr, kz0 = ((task / kinds) % ngr) * lw;
+    const bool sums = d == 3 && (!gate || *gate != 0);
+    if (d == 3 && !sums) return;
     const int nl = min(lw, szh - kz0);
     float2 *A = tw + kFftMax + wave * 2 * kFftWavePts, *Bf = A + kFftWavePts;
     // element (x, column l) at A[l + x * lw]; columns beyond szh are zero
@@ -1122,11 +1136,11 @@ __device__ __forceinline__ void fft_pass2_body(int blk, int nblk, const BoxDev *
     float g[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const int t = lane + 64 * u, l = t & (lw - 1), x = t / lw;
+        const int t = lane + 64 * u, l = t & (lw - 1), x = t >> llw;
         v[u] = make_float2(0.0f, 0.0f);
         g[u] = 0.0f;
         if (l < nl) {
-            const size_t i = ((size_t)x * ny + ky) * szh + kz0 + l;
+            const int i = (x * ny + ky) * szh + kz0 + l;
             v[u] = S1[i];
             g[u] = G[i];
         }
@@ -1135,22 +1149,51 @@ __device__ __forceinline__ void fft_pass2_body(int blk, int nblk, const BoxDev *
     for (int u = 0; u < U; ++u) A[lane + 64 * u] = v[u];
     float2 *R = wave_fft(A, Bf, nx, lw, 1, lw, tw, false);
     float2 *F = R == A ? Bf : A;
-    const bool sums = d == 0 && ev && (!gate || *gate != 0);
-    double vals[7] = {0, 0, 0, 0, 0, 0, 0};
-    const double invV = 1.0 / bx->vol;
+    if (sums) {
+        // kspace_ev_point's sums with the per-axis factors taken out of the
+        // loop (k_d = m_d (2 pi / L_d), 1 / (4 alpha^2)): one division a point
+        double vals[7] = {0, 0, 0, 0, 0, 0, 0};
+        const double invV = 1.0 / bx->vol, hV = 0.5 * invV;
+        const double c4 = 0.25 / (bx->alpha * bx->alpha);
+        const double ku0 = 2.0 * kPi / m.L[0], ku2 = 2.0 * kPi / m.L[2];
+        const double kyd = (double)(2 * ky > ny ? ky - ny : ky) * (2.0 * kPi / m.L[1]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int t = lane + 64 * u, l = t & (lw - 1), kx = t >> llw, kz = kz0 + l;
+            if (l < nl) {
+                const double k0 = (double)(2 * kx > nx ? kx - nx : kx) * ku0;
+                const double k2 = (double)(2 * kz > m.sz[2] ? kz - m.sz[2] : kz) * ku2;
+                const double kl = k0 * k0 + kyd * kyd + k2 * k2;
+                const float2 r = R[t];
+                const double wgt = (kz == 0 || 2 * kz == m.sz[2]) ? 1.0 : 2.0;
+                const double E = wgt * ((double)r.x * r.x + (double)r.y * r.y) * g[u];
+                vals[0] += hV * E;
+                if (kl != 0.0) {
+                    const double diff = -2.0 * (1.0 / kl + c4);
+                    const double hE = hV * E, dE = diff * hE;
+                    vals[1] += hE + dE * k0 * k0;
+                    vals[2] += hE + dE * kyd * kyd;
+                    vals[3] += hE + dE * k2 * k2;
+                    vals[4] += dE * k0 * kyd;
+                    vals[5] += dE * k0 * k2;
+                    vals[6] += dE * kyd * k2;
+                }
+            }
+        }
+#pragma unroll
+        for (int s2 = 0; s2 < 7; ++s2) {
+            const double x = team_sum_d<64>(vals[s2]);
+            if (lane == 0) fx_add(&acc_rep(acc)[s2 == 0 ? ACC_KSPACE : ACC_VIR + s2 - 1], x);
+        }
+        return;
+    }
     // k_d of this wave's component: 2 pi m / L_d in FP32 (0 on the Nyquist plane)
     const int nd = m.sz[d];
     const float kunit = (float)(2.0 * kPi / m.L[d]);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const int t = lane + 64 * u, l = t & (lw - 1), kx = t / lw, kz = kz0 + l;
+        const int t = lane + 64 * u, l = t & (lw - 1), kx = t >> llw, kz = kz0 + l;
         const float2 r = R[t];
-        if (sums && l < nl) {
-            double w[7] = {0, 0, 0, 0, 0, 0, 0};
-            kspace_ev_point(m, bx->alpha, invV, kx, ky, kz, g[u], r, w);
-#pragma unroll
-            for (int s2 = 0; s2 < 7; ++s2) vals[s2] += w[s2];
-        }
         const int id = d == 0 ? kx : (d == 1 ? ky : kz);
         const int mi = 2 * id > nd ? id - nd : id;
         const float kd = 2 * id == nd ? 0.0f : kunit * (float)mi;
@@ -1161,15 +1204,8 @@ __device__ __forceinline__ void fft_pass2_body(int blk, int nblk, const BoxDev *
     float2 *out = S2 + d * nh;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const int t = lane + 64 * u, l = t & (lw - 1), x = t / lw;
-        if (l < nl) out[((size_t)x * ny + ky) * szh + kz0 + l] = X[t];
-    }
-    if (sums) {
-#pragma unroll
-        for (int s2 = 0; s2 < 7; ++s2) {
-            const double x = team_sum_d<64>(vals[s2]);
-            if (lane == 0) fx_add(&acc_rep(acc)[s2 == 0 ? ACC_KSPACE : ACC_VIR + s2 - 1], x);
-        }
+        const int t = lane + 64 * u, l = t & (lw - 1), x = t >> llw;
+        if (l < nl) out[(x * ny + ky) * szh + kz0 + l] = X[t];
     }
 }
 
@@ -1180,6 +1216,7 @@ __device__ __forceinline__ void fft_pass3_body(int blk, int nblk, const BoxDev *
     const int nx = m.sz[0], ny = m.sz[1], nz = m.sz[2], szh = nz / 2 + 1;
     const size_t nh = (size_t)nx * ny * szh, ng = (size_t)nx * ny * nz;
     const int d = blk / nx, x = blk - d * nx;
+    const int lz = fft_log2(nz);
     const int rs = nz + 1;  // padded row pitch of the z transforms
     const int cap = max((ny / 2) * rs, ny * szh);
     float2 *tw = fsh;
@@ -1194,13 +1231,14 @@ __device__ __forceinline__ void fft_pass3_body(int blk, int nblk, const BoxDev *
     for (int c0 = wave * lyw; c0 < szh; c0 += nw * lyw) {
         const int nl = min(lyw, szh - c0);
         constexpr int U = 8;
+        const float rnl = 1.0f / (float)nl;
         for (int t0 = lane; t0 < ny * nl; t0 += 64 * U) {
             float2 v[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int t = t0 + 64 * u;
                 if (t < ny * nl) {
-                    const int y = t / nl, k = c0 + t - y * nl;
+                    const int y = fft_div(t, rnl), k = c0 + t - y * nl;
                     v[u] = in[y * szh + k];
                 }
             }
@@ -1208,7 +1246,7 @@ __device__ __forceinline__ void fft_pass3_body(int blk, int nblk, const BoxDev *
             for (int u = 0; u < U; ++u) {
                 const int t = t0 + 64 * u;
                 if (t < ny * nl) {
-                    const int y = t / nl, k = c0 + t - y * nl;
+                    const int y = fft_div(t, rnl), k = c0 + t - y * nl;
                     A[y * szh + k] = v[u];
                 }
             }
@@ -1223,7 +1261,7 @@ __device__ __forceinline__ void fft_pass3_body(int blk, int nblk, const BoxDev *
     for (int r0 = wave * lzw; r0 < ny / 2; r0 += nw * lzw) {
         const int nl = min(lzw, ny / 2 - r0);
         for (int t = lane; t < nl * nz; t += 64) {
-            const int rr = t / nz, k = t - rr * nz, r = r0 + rr;
+            const int rr = t >> lz, k = t & (nz - 1), r = r0 + rr;
             float2 a, b;
             if (2 * k <= nz) {
                 a = P[2 * r * szh + k];
@@ -1799,7 +1837,7 @@ static void mesh_solve(Ctx &c, bool fromReal, int ev, const int *gate) {
         P.lastFft = 1;
         ProfScope ps(c, PK_FFT);
         const int waves = kFftThreads / 64;
-        const int lw = kFftWavePts / sz[0], tasks = sz[1] * ((szh + lw - 1) / lw) * 3;
+        const int lw = kFftWavePts / sz[0], tasks = sz[1] * ((szh + lw - 1) / lw) * (ev ? 4 : 3);
         const size_t shm2 = sizeof(float2) * (kFftMax + waves * 2 * kFftWavePts);
         launch<fft_pass1_body, kFftPlaneThreads>(c, sz[0], shm, c.box, fromReal ? nullptr : P.mesh,
                                                  fromReal ? P.real : nullptr, P.rhoHat);
