@@ -69,15 +69,12 @@ __device__ __forceinline__ void check_body(int blk, int nblk, int n, const doubl
 // atoms come first, each class sorted by cell: a neighbor row then lists its
 // charged partners first, and the charge spread and gather see only charged
 // atoms.
-__device__ __forceinline__ void bin_body(int blk, int nblk, const int *__restrict__ gate, int n, double4 *__restrict__ pos, const float *__restrict__ q,
-                      const BoxDev *__restrict__ bx, int ncell, const int *__restrict__ colRank,
-                      int *__restrict__ cellOf, int *__restrict__ cellCount,
-                      const double4 *__restrict__ cen, const int *__restrict__ ids) {
-    if (gate && *gate == 0) return;
+__device__ __forceinline__ void bin_atom(int i, double4 *__restrict__ pos, const float *__restrict__ q,
+                                         const BoxDev *__restrict__ bx, int ncell, const int *__restrict__ colRank,
+                                         int *__restrict__ cellOf, int *__restrict__ cellCount,
+                                         const double4 *__restrict__ cen, const int *__restrict__ ids) {
     const Geometry g = bx->geo;
     const CellGrid cg = bx->cg;
-    int i = blk * blockDim.x + threadIdx.x;
-    if (i >= n) return;
     double4 p = pos[i];
     double *pc = &p.x;
 #pragma unroll
@@ -99,16 +96,25 @@ __device__ __forceinline__ void bin_body(int blk, int nblk, const int *__restric
     atomicAdd(&cellCount[rank], 1);
 }
 
+__device__ __forceinline__ void bin_body(int blk, int nblk, const int *__restrict__ gate, int n, double4 *__restrict__ pos, const float *__restrict__ q,
+                      const BoxDev *__restrict__ bx, int ncell, const int *__restrict__ colRank,
+                      int *__restrict__ cellOf, int *__restrict__ cellCount,
+                      const double4 *__restrict__ cen, const int *__restrict__ ids) {
+    if (gate && *gate == 0) return;
+    int i = blk * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    bin_atom(i, pos, q, bx, ncell, colRank, cellOf, cellCount, cen, ids);
+}
+
 // Exclusive scan of the cell counts by one workgroup (replaces the host
 // cumulativeSum round trip, helpers.h:15-24).  Each thread sums a contiguous
 // segment, the segment totals are scanned across the workgroup, and each
 // thread then writes its segment's prefixes.  Also seeds the fill cursors.
 // The counts are cleared as they are consumed, ready for the next binning
 // (no separate clearing launch in every captured step).
-__device__ __forceinline__ void scan_body(int blk, int nblk, const int *__restrict__ gate, int ncell, int *__restrict__ count,
-                                               int *__restrict__ start, int *__restrict__ fill,
-                                               const BoxDev *__restrict__ box, BoxDev *__restrict__ boxLast) {
-    if (gate && *gate == 0) return;
+// one workgroup of 1024 threads
+__device__ __forceinline__ void scan_wg(int ncell, int *__restrict__ count, int *__restrict__ start, int *__restrict__ fill,
+                                        const BoxDev *__restrict__ box, BoxDev *__restrict__ boxLast) {
     // the box this build uses, kept for rebuilding the same list later
     constexpr int kBoxWords = (int)(sizeof(BoxDev) / sizeof(int));
     static_assert(sizeof(BoxDev) % sizeof(int) == 0, "BoxDev copied as words");
@@ -165,6 +171,28 @@ __device__ __forceinline__ void scan_body(int blk, int nblk, const int *__restri
         }
     }
     if (threadIdx.x == 1023) start[ncell] = wsum[15];
+}
+
+__device__ __forceinline__ void scan_body(int blk, int nblk, const int *__restrict__ gate, int ncell, int *__restrict__ count,
+                                          int *__restrict__ start, int *__restrict__ fill,
+                                          const BoxDev *__restrict__ box, BoxDev *__restrict__ boxLast) {
+    if (gate && *gate == 0) return;
+    scan_wg(ncell, count, start, fill, box, boxLast);
+}
+
+// Captured steps: the binning with the scan in its last workgroup (1024
+// threads a workgroup), one launch fewer on every step's path.
+__device__ __forceinline__ void bin_scan_body(int blk, int nblk, const int *__restrict__ gate, int n,
+                                              double4 *__restrict__ pos, const float *__restrict__ q,
+                                              const BoxDev *__restrict__ bx, int ncell, const int *__restrict__ colRank,
+                                              int *__restrict__ cellOf, int *__restrict__ cellCount,
+                                              const double4 *__restrict__ cen, const int *__restrict__ ids, int nslot,
+                                              int *__restrict__ start, int *__restrict__ fill,
+                                              BoxDev *__restrict__ boxLast, int *__restrict__ ticket) {
+    if (gate && *gate == 0) return;
+    const int i = blk * blockDim.x + threadIdx.x;
+    if (i < n) bin_atom(i, pos, q, bx, ncell, colRank, cellOf, cellCount, cen, ids);
+    if (last_block(ticket, nblk)) scan_wg(nslot, cellCount, start, fill, bx, boxLast);
 }
 
 __device__ __forceinline__ void place_body(int blk, int nblk, const int *__restrict__ gate, int n, const int *__restrict__ cellOf, int *__restrict__ fill,
@@ -1274,8 +1302,12 @@ static void run_build(Ctx &c, bool sync) {
     MDX_REQUIRE(!cen || (c.buildPos && !c.e3b.on), 3,
                 "a centroid list needs its build positions and takes no E3B molecule list");
     c.centroidList = cen != nullptr;
-    launch<bin_body, 256>(c, cdiv(c.n, threads), 0, rebuild_gate(c), c.n, c.pos[cur], c.q[cur], c.box, c.ncell, c.colRank, c.cellOf, c.cellCount, cen, c.id[cur]);
-    launch<scan_body, 1024>(c, 1, 0, rebuild_gate(c), nslot, c.cellCount, c.cellStart, c.cellFill, c.box, c.boxLast);
+    if (c.capture && c.fuseStep) {
+        launch<bin_scan_body, 1024>(c, cdiv(c.n, 1024), 0, rebuild_gate(c), c.n, c.pos[cur], c.q[cur], c.box, c.ncell, c.colRank, c.cellOf, c.cellCount, cen, c.id[cur], nslot, c.cellStart, c.cellFill, c.boxLast, c.dFlags + 11);
+    } else {
+        launch<bin_body, 256>(c, cdiv(c.n, threads), 0, rebuild_gate(c), c.n, c.pos[cur], c.q[cur], c.box, c.ncell, c.colRank, c.cellOf, c.cellCount, cen, c.id[cur]);
+        launch<scan_body, 1024>(c, 1, 0, rebuild_gate(c), nslot, c.cellCount, c.cellStart, c.cellFill, c.box, c.boxLast);
+    }
     launch<place_body, 256>(c, cdiv(c.n, threads), 0, rebuild_gate(c), c.n, c.cellOf, c.cellFill, c.perm);
     launch<cellsort_body, 64>(c, nslot, 0, rebuild_gate(c), nslot, c.cellStart, c.perm, c.perm2, c.id[cur]);
     launch<permute_body, 256>(c, cdiv(c.n, threads), 0, rebuild_gate(c), c.n, c.perm2, c.pos[cur], c.vel[cur], c.q[cur], c.type[cur], c.id[cur], c.pos[nxt], c.vel[nxt], c.q[nxt], c.type[nxt], c.id[nxt], c.idToIdx, c.posLast, c.xq, c.box,
