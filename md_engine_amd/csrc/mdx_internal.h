@@ -823,38 +823,51 @@ __device__ inline double alpha_solve_any(int order, int natoms, double q2, const
 // the new box and, for P3M, alpha and the Green function's image range.
 __device__ __forceinline__ void baro_wave(const int *__restrict__ gate, BoxDev *__restrict__ bx, const BoxParams &bp,
                                           const BaroArgs &ba, const unsigned long long *__restrict__ acc) {
-    if (gate && *gate == 0) return;
     const int lane = threadIdx.x & 63;
+    // every input is read up front, the gate among them, so the wave waits
+    // for one memory round trip rather than one per dependent read
     constexpr int kSlots[6] = {ACC_VIR, ACC_VIR + 1, ACC_VIR + 2, ACC_MOL, ACC_MOL + 1, ACC_VTRACE};
+    const int open = gate ? *gate : 1;
     long long t[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        t[k] = lane < kAccRep ? (long long)acc[lane * ACC_N + kSlots[k]] : 0ll;
+    for (int k = 0; k < 6; ++k) t[k] = lane < kAccRep ? (long long)acc[lane * ACC_N + kSlots[k]] : 0ll;
+    const int topen = ba.tgate ? *ba.tgate : 1;
+    const long long th = ba.rescaled ? (long long)ba.thermo[0] : 0ll;
+    const double vol = bx->vol;
+    double glo[3], gL[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        glo[d] = bx->geo.lo[d];
+        gL[d] = bx->geo.L[d];
+    }
+    if (open == 0) return;
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) t[k] += __shfl_xor(t[k], o);
-    }
     if (lane != 0) return;
     // trace of the virial: the tensor's diagonal plus the trace-only terms
     const double W = (double)t[0] * kFxInv + (double)t[1] * kFxInv + (double)t[2] * kFxInv + (double)t[5] * kFxInv;
     const double corr = (double)t[3] * kFxInv;
     double mv2 = (double)t[4] * kFxInv;
-    if (ba.rescaled && (!ba.tgate || *ba.tgate)) {
-        const double T = (double)(long long)ba.thermo[0] * kFxInv * ba.tconv;
+    if (ba.rescaled && topen) {
+        const double T = (double)th * kFxInv * ba.tconv;
         if (T > 0.0) {
             const double l2 = ba.tmode == 0 ? ba.T0 / T : 1.0 + ba.coupling * (ba.T0 / T - 1.0);
             mv2 *= fmax(l2, 0.0);
         }
     }
-    const double P = (mv2 * ba.mvv + W - corr) / (3.0 * bx->vol) * ba.nktv2p;
+    const double P = (mv2 * ba.mvv + W - corr) / (3.0 * vol) * ba.nktv2p;
     double mu = cbrt(1.0 - ba.dt / ba.period * (ba.P0 - P) / ba.bulk);
     mu = fmin(fmax(mu, 1.0 - ba.maxDil), 1.0 + ba.maxDil);
     bx->pressure = P;
     bx->mu = mu;
-    double lo[3], L[3];
+    double lo[3], L[3], h[3];
     for (int d = 0; d < 3; ++d) {
-        const double ctr = bx->geo.lo[d] + 0.5 * bx->geo.L[d];
-        L[d] = bx->geo.L[d] * mu;
+        const double ctr = glo[d] + 0.5 * gL[d];
+        L[d] = gL[d] * mu;
         lo[d] = ctr - 0.5 * L[d];
+        h[d] = bp.meshSz[d] > 0 ? L[d] / bp.meshSz[d] : 0.0;  // box_derive's mesh spacing
         if (bp.periodic[d] && L[d] < ba.minL[d]) bx->regrid = 1;
     }
     // box_derive leaves alpha, alphaF and greenLim as they are
@@ -863,7 +876,7 @@ __device__ __forceinline__ void baro_wave(const int *__restrict__ gate, BoxDev *
         const double a = alpha_solve_any(ba.order, ba.natoms, ba.q2, L, bp.meshSz, ba.rcut);
         bx->alpha = a;
         bx->alphaF = (float)a;
-        const double hm = cbrt(bx->mesh.h[0] * bx->mesh.h[1] * bx->mesh.h[2]);
+        const double hm = cbrt(h[0] * h[1] * h[2]);
         bx->greenLim = green_lim(a, hm);
     }
 }
