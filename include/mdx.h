@@ -357,7 +357,10 @@ int mdx_sync(mdx_ctx *ctx);
  * site; "gather_tile_planar" 1 (default) loads the three planar field
  * components, 0 interleaves them first), 1 row teams from the per-site
  * mesh records of the position refresh, 0 row teams computing the mesh point
- * from the positions (identical forces); "p3m_fft" 1 (default) solves the mesh
+ * from the positions (identical forces); "p3m_tile_cols" 1 (default) or 2
+ * cell columns per tile side in the LDS-tiled spread and gather (2 x 2
+ * columns cut the halo points but halve the workgroups: measured slower
+ * here; identical forces); "p3m_fft" 1 (default) solves the mesh
  * in three hand-written passes (FFTs in LDS with the charge conversion, the
  * field and the k-space energy and virial fused; power-of-two meshes whose
  * planes fit a workgroup, else hipFFT), 0 hipFFT with separate conversion and

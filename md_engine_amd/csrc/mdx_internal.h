@@ -274,6 +274,7 @@ struct PppmState {
     int tileKB = 48;              // LDS budget of a gather tile's field box (KB)
     int tileTeam = 8;             // lanes per charged site in the tile gather
     int tilePlanar = 1;           // 1: the tile gather loads the three planar components (no interleaving pass)
+    int tileCols = 1;             // cell columns per tile side in the tiled spread and gather (1 or 2)
     int fftMode = 1;              // 1: the mesh solve in three LDS passes where the mesh allows, 0: hipFFT
     int lastSpread = -1, lastGather = -1, lastFft = -1;  // paths the last evaluation took (mdx_pppm_paths)
     int sz[3] = {64, 64, 64};

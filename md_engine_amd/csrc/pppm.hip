@@ -158,6 +158,10 @@ __device__ __forceinline__ void spread_body(int blk, int nblk, int n, const doub
 struct TileGeo {  // cell widths and margin come from the device box
     int ncz, zs, nzc;
     int ext[3];
+    // cb x cb cell columns a tile (1 or 2): the columns are in 2-D Morton
+    // order, so with even column counts the four columns of an aligned 2 x 2
+    // block hold ranks 4k..4k+3, rank 4k its lowest corner
+    int cb;
 };
 
 __device__ __forceinline__ int tile_origin(double lo_pos, double h) {
@@ -482,13 +486,12 @@ __device__ __forceinline__ void spread_rows_body(int blk, int nblk, const float 
     const int lb = xcd_block(blk, nlog);
     if (lb >= nlog) return;  // uniform over the workgroup
     const MeshGeo m = bx->mesh;
-    const int col = lb / tg.nzc;
+    const int ncol = tg.cb * tg.cb;
+    const int col0 = (lb / tg.nzc) * ncol;
     const int zc = lb % tg.nzc;
     const int z0 = zc * tg.zs;
     const int z1 = min(z0 + tg.zs, tg.ncz);
-    const int a0 = cellStart[col * tg.ncz + z0];
-    const int a1 = cellStart[col * tg.ncz + z1];
-    const int2 cxy = colXY[col];
+    const int2 cxy = colXY[col0];
     const double margin = bx->tileMargin;
     const int ox = tile_origin(cxy.x * bx->tileCellW[0] - margin, m.h[0]) - H;
     const int oy = tile_origin(cxy.y * bx->tileCellW[1] - margin, m.h[1]) - H;
@@ -496,6 +499,9 @@ __device__ __forceinline__ void spread_rows_body(int blk, int nblk, const float 
     const int ex = tg.ext[0], ey = tg.ext[1], ez = tg.ext[2];
     const int vol = ex * ey * ez;
     for (int k = threadIdx.x; k < vol; k += NT) tile[k] = 0ull;
+    for (int cu = 0; cu < ncol; ++cu) {
+    const int a0 = cellStart[(col0 + cu) * tg.ncz + z0];
+    const int a1 = cellStart[(col0 + cu) * tg.ncz + z1];
     for (int base = a0; base < a1; base += SL) {
         const int ns = min(SL, a1 - base);
         if (threadIdx.x < ns) {
@@ -556,6 +562,7 @@ __device__ __forceinline__ void spread_rows_body(int blk, int nblk, const float 
             }
         }
         __syncthreads();
+    }
     }
     // flush by z rows per wave, as the tiled spread
     const int lane = threadIdx.x & 63, nw = NT >> 6;
@@ -1405,9 +1412,10 @@ __device__ __forceinline__ void gather_tile_body(int blk, int nblk, const float4
     constexpr int SPR = NT / T;  // sites per round
     const MeshGeo m = bx->mesh;
     const int sx = m.sz[0], sy = m.sz[1], sz = m.sz[2];
-    const int col = lb / tg.nzc, zc = lb % tg.nzc;
+    const int ncol = tg.cb * tg.cb;
+    const int col0 = (lb / tg.nzc) * ncol, zc = lb % tg.nzc;
     const int z0 = zc * tg.zs, z1 = min(z0 + tg.zs, tg.ncz);
-    const int a0 = cellStart[col * tg.ncz + z0], a1 = cellStart[col * tg.ncz + z1];
+    int a0 = cellStart[col0 * tg.ncz + z0], a1 = cellStart[col0 * tg.ncz + z1];
     const int tl = threadIdx.x & (T - 1);
     // first round's site: charge and record in flight with the box loads
     int s = threadIdx.x / T;
@@ -1417,7 +1425,7 @@ __device__ __forceinline__ void gather_tile_body(int blk, int nblk, const float4
         qi = q[a0 + s];
         rc = rec[a0 + s];
     }
-    const int2 cxy = colXY[col];
+    const int2 cxy = colXY[col0];
     const double margin = bx->tileMargin;
     const int ox = tile_origin(cxy.x * bx->tileCellW[0] - margin, m.h[0]) - H;
     const int oy = tile_origin(cxy.y * bx->tileCellW[1] - margin, m.h[1]) - H;
@@ -1469,7 +1477,7 @@ __device__ __forceinline__ void gather_tile_body(int blk, int nblk, const float4
     __syncthreads();
     const float scale = bx->gatherScale;
     constexpr int NRW = (ORDER * ORDER + T - 1) / T;
-    for (;;) {
+    for (int cu = 0;;) {
         const int i = a0 + s;
         if (i < a1 && qi != 0.0f) {
             const int gx = rec_point(rc, 0), gy = rec_point(rc, 1), gz = rec_point(rc, 2);
@@ -1530,7 +1538,13 @@ __device__ __forceinline__ void gather_tile_body(int blk, int nblk, const float4
             }
         }
         s += SPR;
-        if (a0 + s - (int)(threadIdx.x / T) >= a1) break;  // uniform: the round's first site is past the end
+        if (a0 + s - (int)(threadIdx.x / T) >= a1) {  // uniform: the round's first site is past the end
+            if (++cu >= ncol) break;  // the tile's next cell column
+            a0 = cellStart[(col0 + cu) * tg.ncz + z0];
+            a1 = cellStart[(col0 + cu) * tg.ncz + z1];
+            s = threadIdx.x / T;
+        }
+        qi = 0.0f;
         if (a0 + s < a1) {
             qi = q[a0 + s];
             rc = rec[a0 + s];
@@ -1556,11 +1570,9 @@ __device__ __forceinline__ void apply_stored_body(int blk, int nblk, int n, cons
 // Tile shape for the LDS spread: z-chunks of zs cells so that the tile stays
 // within 48 KB of LDS.  Returns false (plain spread) when no cell list is
 // valid, the box is not periodic, or even one cell does not fit.
-bool tiled_geometry(const Ctx &c, TileGeo &tg, int budget = 48 * 1024 / 8) {
-    if (!c.listValid || c.ncell == 0 || !c.colXY || c.mdxDisableTiledSpread) return false;
-    for (int a = 0; a < 3; ++a)
-        if (!c.geo.periodic[a]) return false;
+static bool tiled_geometry_cols(const Ctx &c, TileGeo &tg, int budget, int cb) {
     const int order = c.pppm.order;
+    tg.cb = cb;
     tg.ncz = c.nc[2];
     // the kernel takes cell widths and margin from the device box (box_derive);
     // the extents keep two mesh points of slack for a box that changes by a
@@ -1570,8 +1582,8 @@ bool tiled_geometry(const Ctx &c, TileGeo &tg, int budget = 48 * 1024 / 8) {
         double h = c.geo.L[a] / c.pppm.sz[a];
         return (int)std::floor((ncells * (c.geo.L[a] / c.nc[a]) + 2.0 * margin) / h) + order + 2;
     };
-    tg.ext[0] = extent(0, 1);
-    tg.ext[1] = extent(1, 1);
+    tg.ext[0] = extent(0, cb);
+    tg.ext[1] = extent(1, cb);
     if (tg.ext[0] >= c.pppm.sz[0] || tg.ext[1] >= c.pppm.sz[1]) return false;
     int zs = 0;
     for (int z = 1; z <= tg.ncz; ++z) {
@@ -1588,6 +1600,18 @@ bool tiled_geometry(const Ctx &c, TileGeo &tg, int budget = 48 * 1024 / 8) {
     tg.nzc = (tg.ncz + zs - 1) / zs;
     tg.ext[2] = extent(2, zs);
     return true;
+}
+
+// Tiles of 2 x 2 cell columns (option p3m_tile_cols 2, even column counts)
+// where they fit the budget, else of single columns.
+bool tiled_geometry(const Ctx &c, TileGeo &tg, int budget = 48 * 1024 / 8, int maxCols = 2) {
+    if (!c.listValid || c.ncell == 0 || !c.colXY || c.mdxDisableTiledSpread) return false;
+    for (int a = 0; a < 3; ++a)
+        if (!c.geo.periodic[a]) return false;
+    if (c.pppm.tileCols == 2 && maxCols >= 2 && c.nc[0] % 2 == 0 && c.nc[1] % 2 == 0 &&
+        tiled_geometry_cols(c, tg, budget, 2))
+        return true;
+    return tiled_geometry_cols(c, tg, budget, 1);
 }
 
 // Geometry of the owner-computes spread, or false (another spread) when no
@@ -1902,7 +1926,7 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
     P.lastSpread = tiled ? 1 : 0;
     const float4 *rec = c.meshRec && c.hasCharges ? c.meshRec : nullptr;
     if (tiled && rec) {
-        const int nblk = c.nc[0] * c.nc[1] * tg.nzc;
+        const int nblk = c.nc[0] * c.nc[1] / (tg.cb * tg.cb) * tg.nzc;
         const size_t shm = sizeof(unsigned long long) * tg.ext[0] * tg.ext[1] * tg.ext[2];
         switch (P.order) {
             case 1: launch<spread_rows_body<1, 256>, 256>(c, xcd_grid(nblk), shm, c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh, rec); break;
@@ -1910,8 +1934,8 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
             case 5: launch<spread_rows_body<5, 256>, 256>(c, xcd_grid(nblk), shm, c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh, rec); break;
             default: launch<spread_rows_body<7, 256>, 256>(c, xcd_grid(nblk), shm, c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh, rec); break;
         }
-    } else if (tiled) {
-        const int nblk = c.nc[0] * c.nc[1] * tg.nzc;
+    } else if (tiled && tiled_geometry(c, tg, 48 * 1024 / 8, 1)) {  // single-column tiles (spread_tiled_body)
+        const int nblk = c.nc[0] * c.nc[1] / (tg.cb * tg.cb) * tg.nzc;
         const size_t shm = sizeof(unsigned long long) * tg.ext[0] * tg.ext[1] * tg.ext[2];
         switch (P.order) {
             case 1: launch<spread_tiled_body<1>, 256>(c, xcd_grid(nblk), shm, c.pos[cur], c.q[cur], c.box, qunit, c.cellStart, c.colXY, tg, nblk, (unsigned long long *)P.mesh, rec); break;
@@ -1944,7 +1968,7 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
             launch<field_pack_body, 256>(c, cdiv(ng, threads), 0, ng, P.eReal, P.eField4);
             e4 = P.eField4;
         }
-        const int nlog = c.nc[0] * c.nc[1] * gg.nzc;
+        const int nlog = c.nc[0] * c.nc[1] / (gg.cb * gg.cb) * gg.nzc;
         const size_t shm = sizeof(float) * 3 * gg.ext[0] * gg.ext[1] * gg.ext[2];
 #define MDX_GTILE(O, T) \
     launch<gather_tile_body<O, T, 256>, 256>(c, xcd_grid(nlog), shm, c.meshRec, c.q[cur], c.box, c.cellStart, c.colXY, gg, nlog, P.eReal, e4, c.force, c.id[cur], stored)
