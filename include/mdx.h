@@ -357,7 +357,8 @@ int mdx_sync(mdx_ctx *ctx);
  * site; "gather_tile_planar" 1 (default) loads the three planar field
  * components, 0 interleaves them first), 1 row teams from the per-site
  * mesh records of the position refresh, 0 row teams computing the mesh point
- * from the positions (identical forces); "p3m_tile_cols" 1 (default) or 2
+ * from the positions (identical forces); "spread_tile_kb" 8..64 KB of LDS a
+ * spread tile (default 48: 24, 32 and 16 measured slower); "p3m_tile_cols" 1 (default) or 2
  * cell columns per tile side in the LDS-tiled spread and gather (2 x 2
  * columns cut the halo points but halve the workgroups: measured slower
  * here; identical forces); "p3m_fft" 1 (default) solves the mesh

@@ -2649,6 +2649,10 @@ int mdx_option_set(mdx_ctx *ctx, const char *name, int value) {
         else if (k == "gather_tile_planar") {
             c.pppm.tilePlanar = value != 0;
         }
+        else if (k == "spread_tile_kb") {
+            MDX_REQUIRE(value >= 8 && value <= 64, MDX_ERR_ARG, "spread_tile_kb: 8 to 64 KB of LDS a spread tile");
+            c.pppm.spreadKB = value;
+        }
         else if (k == "p3m_tile_cols") {
             MDX_REQUIRE(value == 1 || value == 2, MDX_ERR_ARG, "p3m_tile_cols: 1 or 2 cell columns per tile side");
             c.pppm.tileCols = value;

@@ -272,6 +272,7 @@ struct PppmState {
     int spreadMode = 1;           // 2 owner-computes cell blocks, 1 LDS tiles with global atomics, 0 global atomics
     int gatherMode = 2;           // 2 LDS tiles of the field, 1 row teams from the mesh records, 0 from the positions
     int tileKB = 48;              // LDS budget of a gather tile's field box (KB)
+    int spreadKB = 48;            // LDS budget of a spread tile's charge box (KB)
     int tileTeam = 8;             // lanes per charged site in the tile gather
     int tilePlanar = 1;           // 1: the tile gather loads the three planar components (no interleaving pass)
     int tileCols = 1;             // cell columns per tile side in the tiled spread and gather (1 or 2)

@@ -1922,7 +1922,7 @@ void pppm_forces(Ctx &c, bool energy, bool virial) {
             default: launch<spread_own_body<7, 256>, 256>(c, xcd_grid(nlog), oshm, c.meshRec, c.q[cur], c.cellStart, c.colRank, c.colXY, og, qunit, nlog, P.real, c.dFlags); break;
         }
     } else {
-    const bool tiled = P.spreadMode >= 1 && tiled_geometry(c, tg);
+    const bool tiled = P.spreadMode >= 1 && tiled_geometry(c, tg, P.spreadKB * 1024 / 8);
     P.lastSpread = tiled ? 1 : 0;
     const float4 *rec = c.meshRec && c.hasCharges ? c.meshRec : nullptr;
     if (tiled && rec) {
