@@ -24,6 +24,11 @@ namespace mdx {
 
 namespace {
 
+// t / n for 0 <= t < 2^16 and 0 < n <= 2^10 from a float reciprocal: the
+// quotient of (t + 0.5) / n lies at least 0.5 / n from an integer, far above
+// the reciprocal's rounding, so truncation gives the exact integer quotient.
+__device__ __forceinline__ int fft_div(int t, float rn) { return (int)(((float)t + 0.5f) * rn); }
+
 constexpr double kPi = 3.14159265358979323846;
 
 // W_p for orders 1/3/5/7 (FixChargeEwald.cu:111-115, 163-172, 220-234) as
@@ -567,10 +572,11 @@ __device__ __forceinline__ void spread_rows_body(int blk, int nblk, const float 
     // flush by z rows per wave, as the tiled spread
     const int lane = threadIdx.x & 63, nw = NT >> 6;
     const int per = 64 / ez;
-    const int sub = lane / ez, kz = lane - sub * ez;
+    const int sub = fft_div(lane, 1.0f / (float)ez), kz = lane - sub * ez;
     const int gz = wrap_near(oz + kz, m.sz[2]);
+    const float rey = 1.0f / (float)ey;
     for (int row = (threadIdx.x >> 6) * per + sub; sub < per && row < ex * ey; row += nw * per) {
-        const int kx = row / ey, ky = row - kx * ey;
+        const int kx = fft_div(row, rey), ky = row - kx * ey;
         const unsigned long long v = tile[row * ez + kz];
         if (v == 0ull) continue;
         const int gx = wrap_near(ox + kx, m.sz[0]), gy = wrap_near(oy + ky, m.sz[1]);
@@ -902,10 +908,6 @@ constexpr int kFftWavePts = 512;  // points a wave transforms at a time (lines x
 
 __device__ __forceinline__ int fft_log2(int n) { return __ffs(n) - 1; }
 
-// t / n for 0 <= t < 2^16 and 0 < n <= 2^10 from a float reciprocal: the
-// quotient of (t + 0.5) / n lies at least 0.5 / n from an integer, far above
-// the reciprocal's rounding, so truncation gives the exact integer quotient.
-__device__ __forceinline__ int fft_div(int t, float rn) { return (int)(((float)t + 0.5f) * rn); }
 
 // Workgroup barrier ordering LDS only: the passes read global memory that
 // earlier launches wrote and write what later launches read, so their
@@ -1436,6 +1438,7 @@ __device__ __forceinline__ void gather_tile_body(int blk, int nblk, const float4
     if (threadIdx.x < ORDER * ORDER) lcoef[threadIdx.x] = WCoef<ORDER>::c[threadIdx.x / ORDER][threadIdx.x % ORDER];
     // UL points per thread and pass, all their loads issued before any LDS store
     constexpr int UL = 4;
+    const float rez = 1.0f / (float)ez, rey = 1.0f / (float)ey;
     for (int k0 = threadIdx.x; k0 < vol; k0 += UL * NT) {
         size_t g[UL];
         bool in[UL];
@@ -1444,8 +1447,8 @@ __device__ __forceinline__ void gather_tile_body(int blk, int nblk, const float4
             const int k = k0 + u * NT;
             in[u] = k < vol;
             const int kk = in[u] ? k : 0;
-            const int row = kk / ez, kz = kk - row * ez;
-            const int kx = row / ey, ky = row - kx * ey;
+            const int row = fft_div(kk, rez), kz = kk - row * ez;
+            const int kx = fft_div(row, rey), ky = row - kx * ey;
             g[u] = ((size_t)wrap_near(ox + kx, sx) * sy + wrap_near(oy + ky, sy)) * sz + wrap_near(oz + kz, sz);
         }
         float v0[UL], v1[UL], v2[UL];
