@@ -217,10 +217,13 @@ def test_mesh_stream_matches_one_stream(orc, gpu_engine_factory, barostat):
 
 
 def test_fused_step_launches_match_separate(orc, gpu_engine_factory):
-    """The captured step's folded launches (step decisions in the first half
-    kick, molecule map in the list build, row maxima in the position refresh,
-    the thermostat's rescale with the barostat's Green function) give the
-    trajectory of separate launches bit for bit, rebuilds and box included."""
+    """The captured step's folded launches (step decisions and the position
+    refresh in the first half kick, the cell scan in the binning, molecule
+    map in the list build, row maxima in the pair launch, the mesh stream's
+    fork after the rebuild's copy-back, the barostat step in the second half
+    kick, the thermostat's rescale with the barostat's Green function) give
+    the trajectory of separate launches bit for bit, rebuilds and box
+    included."""
     say("fused step launches vs separate")
     box = systems.tip4p2005_box(343, seed=37, min_sep=1.8)
     v = start_velocities(orc, box, seed=11)
