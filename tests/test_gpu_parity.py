@@ -288,6 +288,17 @@ def test_spread_paths_bit_identical(gpu_engine_factory, sz):
         out[mode] = e.forces()
     assert np.array_equal(out[2], out[1])
     assert np.array_equal(out[2], out[0])
+    # tiles of 2 x 2 cell columns (spread and gather) and a smaller spread
+    # tile (where it does not fit the mesh's z extent, the plain spread runs)
+    e.option("p3m_spread", 1)
+    for cols, kb in ((2, 48), (1, 24)):
+        e.option("p3m_tile_cols", cols)
+        e.option("spread_tile_kb", kb)
+        e.compute(PPPM)
+        assert e.pppm_paths()[0] in ((1,) if kb == 48 else (1, 0))
+        assert np.array_equal(e.forces(), out[1]), f"tile columns {cols}, spread tile {kb} KB"
+    e.option("p3m_tile_cols", 1)
+    e.option("spread_tile_kb", 48)
 
 
 @pytest.mark.parametrize("sz", [(32, 32, 32), (64, 32, 32), (128, 32, 32), (64, 64, 64), (32, 64, 128)])
